@@ -1,0 +1,232 @@
+/*
+ * rhg.h -- C ABI of the MI355X host-guest energy engine (librhg.so).
+ *
+ * Drop-in boundary for RASPA2's host-guest energy path: framework van der
+ * Waals and real-space Ewald energies at trial positions, the Fourier-space
+ * Ewald change of an inserted molecule, Widom/CBMC insertion, the Widom
+ * estimators, and energy grids.  Plain pointers and sizes only; every entry
+ * point returns an int status (RHG_OK or a negative RHG_ERR_*) and leaves a
+ * message for rhg_last_error().
+ *
+ * Unit conventions follow RASPA2: positions in Angstrom, charges in e, LJ
+ * epsilon in K on input, energies in RASPA2's internal unit
+ * (amu*A^2/ps^2 = 1.6605402e-23 J; src/constants.c:147).
+ *
+ * Threading: calls on one context are serialised by the caller; distinct
+ * contexts are independent.  Each context owns one HIP stream.
+ */
+#ifndef RHG_H
+#define RHG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RHG_OK 0
+#define RHG_ERR_INVALID (-1)
+#define RHG_ERR_HIP (-2)
+#define RHG_ERR_OOM (-3)
+#define RHG_ERR_NOT_ADMITTED (-4)
+#define RHG_ERR_UNSUPPORTED (-5)
+#define RHG_ERR_FAULT_INJECTED (-6)
+
+/* BoundaryCondition values (src/potentials.h:232-271) */
+#define RHG_BC_RECTANGULAR 1
+#define RHG_BC_TRICLINIC 2
+
+/* ChargeMethod values used on this path (src/potentials.c:12430-12478) */
+#define RHG_CHARGE_NONE 0
+#define RHG_CHARGE_EWALD 1
+
+/* arithmetic of the pair kernels */
+#define RHG_PREC_FP64 0  /* plain: FP64 throughout, RASPA2's operation order */
+#define RHG_PREC_MIXED 1 /* FP32 pair terms, FP64 differences and accumulation */
+
+#define RHG_MAX_SITES 8
+#define RHG_NR_BLOCKS 5 /* src/statistics.h:40 */
+
+typedef struct rhg_ctx rhg_ctx;
+typedef struct rhg_grid rhg_grid;
+
+/* Framework atoms, one pseudo-atom per atom (Lorentz-Berthelot mixing on
+ * the fly).  Replaces Framework[s].Atoms[f][i] (src/framework.h:180,
+ * src/molecule.h:227-263) and the framework rows of PseudoAtoms[]. */
+typedef struct {
+  int32_t n_atoms;
+  const double *x, *y, *z; /* Cartesian positions [A] */
+  const double *charge;    /* [e]; NULL = uncharged framework */
+  const double *eps_k;     /* LJ epsilon/k_B [K] */
+  const double *sigma;     /* LJ sigma [A] */
+  const double *mass;      /* [g/mol]; NULL = 12.0 each */
+} rhg_framework;
+
+/* Simulation box: RASPA2's Box matrix (fields ax ay az bx by bz cx cy cz,
+ * the columns are the cell vectors; src/framework.c:3049-3066). */
+typedef struct {
+  double box[9];
+  int32_t boundary; /* RHG_BC_* */
+} rhg_cell;
+
+typedef struct {
+  double cutoff_vdw;      /* CutOffVDW [A] */
+  int32_t shift_vdw;      /* 1: LJ shifted to zero at the cutoff */
+  int32_t charge_method;  /* RHG_CHARGE_* */
+  double cutoff_charge;   /* CutOffChargeCharge [A] */
+  double ewald_precision; /* EwaldPrecision (automatic alpha and kmax) */
+  double temperature;     /* ExternalTemperature [K] */
+} rhg_forcefield;
+
+/* A rigid adsorbate (one rigid group holding all atoms). */
+typedef struct {
+  int32_t n_sites;
+  int32_t type[RHG_MAX_SITES];        /* pseudo-atom ids from rhg_add_pseudo_atom */
+  double body[RHG_MAX_SITES][3];      /* Components[c].Positions [A] */
+  int32_t start;                      /* StartingBead */
+  int32_t k_fb;                       /* NumberOfTrialPositionsForTheFirstBeadWidom */
+  int32_t k_or;                       /* NumberOfTrialPositionsWidom */
+} rhg_molecule;
+
+/* Widom block accumulators (WidomRosenbluthFactorCount/Accumulated,
+ * WidomEnergyDifferenceAccumulated, WidomIdealGasAccumulated;
+ * src/mc_moves.c:7862-7933). */
+typedef struct {
+  double count[RHG_NR_BLOCKS];
+  double sum_w[RHG_NR_BLOCKS];
+  double sum_wdu[RHG_NR_BLOCKS];
+  double ideal_gas[RHG_NR_BLOCKS];
+} rhg_widom_blocks;
+
+/* per-insertion record layout (rec arrays are n x RHG_REC_N doubles) */
+#define RHG_REC_W 0        /* RosenbluthNew incl. Fourier factor; 0 on overlap */
+#define RHG_REC_DU 1       /* DeltaU of WidomAdsorbateMove */
+#define RHG_REC_UVDW 2     /* UHostVDWNew */
+#define RHG_REC_UREAL 3    /* UHostChargeChargeNew (real space) */
+#define RHG_REC_UFHOST 4   /* UHostAdsorbateChargeChargeFourierDelta */
+#define RHG_REC_UFADS 5    /* UAdsorbateAdsorbateChargeChargeFourierDelta */
+#define RHG_REC_OVERLAP 6  /* 1: rejected (OVERLAP) */
+#define RHG_REC_SELFB 7
+#define RHG_REC_SELOR 8
+#define RHG_REC_FBMASK 9   /* overlapping first-bead trials, bit i */
+#define RHG_REC_ORMASK 10  /* overlapping orientation trials, bit i */
+#define RHG_REC_W1 11
+#define RHG_REC_W2 12
+#define RHG_REC_STAGE 13   /* 0 fb all overlap,1 W1<min,2 or all overlap,3 W<min,4 accepted,5 blocked */
+#define RHG_REC_N 14
+
+/* ------------------------------------------------------------ context */
+/* Replaces ReadInput's framework/force-field/Ewald set-up for one system:
+ * ReadFrameworkDefinition, mixing (src/potentials.c:3164-3171),
+ * ComputePotentialShifts (src/potentials.c:5263), InitializeEwald
+ * (src/ewald.c:200), SetupKVectors (src/ewald.c:532), EwaldEnergyIon
+ * (src/ewald.c:615) and PrecomputeTotalEwaldContributions
+ * (src/ewald.c:1289).  device = HIP device ordinal. */
+int rhg_create(const rhg_framework *fw, const rhg_cell *cell, const rhg_forcefield *ff, int device,
+               rhg_ctx **out);
+void rhg_destroy(rhg_ctx *ctx);
+const char *rhg_last_error(const rhg_ctx *ctx);
+const char *rhg_version(void);
+/* Adds a probe pseudo-atom (a PseudoAtoms[] row, src/potentials.c:268-274). */
+int rhg_add_pseudo_atom(rhg_ctx *ctx, double eps_k, double sigma, double charge, int32_t *type_out);
+int rhg_set_precision(rhg_ctx *ctx, int32_t precision);
+/* HBM bytes held by the context (framework tables, k tables) */
+int rhg_footprint(const rhg_ctx *ctx, size_t *bytes);
+/* out: alpha, kmax_x, kmax_y, kmax_z, nk, uion, beta, volume, framework mass, density [kg/m^3] */
+int rhg_info(const rhg_ctx *ctx, double out[10]);
+
+/* ------------------------------------------------- energy at a position */
+/* REAL CalculateFrameworkVDWEnergyAtPosition(POINT,int,REAL)
+ * (src/framework_energy.h:43, src/framework_energy.c:60) */
+int rhg_framework_vdw_energy_at_position(rhg_ctx *ctx, const double pos[3], int32_t type,
+                                         double scaling, double *u_vdw);
+/* void CalculateFrameworkChargeEnergyAtPosition(POINT,int,REAL*,REAL*,REAL)
+ * (src/framework_energy.h:46, src/framework_energy.c:176); ucbd is 0
+ * (no bond dipoles on this path). */
+int rhg_framework_charge_energy_at_position(rhg_ctx *ctx, const double pos[3], int32_t type,
+                                            double scaling, double *ucc, double *ucbd);
+/* Batched form of both: n positions (xyz interleaved) of one pseudo-atom
+ * type.  overlap[i] = (u_vdw >= 1e7 || u_cc >= 1e7), EnergyOverlapCriteria
+ * of src/cbmc.c:699-712.  Any output may be NULL. */
+int rhg_framework_energy_at_positions(rhg_ctx *ctx, int64_t n, const double *xyz, int32_t type,
+                                      double *u_vdw, double *u_cc, uint8_t *overlap);
+
+/* ---------------------------------------------------------------- Ewald */
+/* Ewald tables in RASPA2's k order (kx 0..kmax, ky -kmax..kmax,
+ * kz -kmax..kmax): KFactor and StoreTotalChargeFramework
+ * (src/ewald.c:532-600, 1289-1731).  Arrays hold nk values. */
+int rhg_ewald_tables(rhg_ctx *ctx, double *kfactor, double *sf_re, double *sf_im);
+/* int CalculateEwaldFourierAdsorbate(int New,int Old,int mol,int store)
+ * (src/ewald.h:86, src/ewald.c:6859) for n_mol independent new molecules
+ * at zero loading: d_host = UHostAdsorbateChargeChargeFourierDelta,
+ * d_ads = UAdsorbateAdsorbateChargeChargeFourierDelta.  xyz: n_mol x
+ * n_sites x 3. */
+int rhg_ewald_fourier_adsorbate(rhg_ctx *ctx, int64_t n_mol, int32_t n_sites, const int32_t *types,
+                                const double *xyz, double *d_host, double *d_ads);
+
+/* ---------------------------------------------------------------- Widom */
+/* WidomAdsorbateMove (src/mc_moves.c:7844) for insertions [first, first+n)
+ * of a job of n_total insertions and steps_per_cycle MC steps per cycle
+ * (the block of insertion i is that of cycle i/steps_per_cycle,
+ * src/statistics.c:904,1126).  Trial positions come from the per-insertion
+ * counter stream of `seed` in RASPA2's per-move draw order (first-bead z,y,x
+ * src/cbmc.c:652-667; Marsaglia sphere and Shoemake rotation
+ * src/utils.c:688,835).  Blocks are ADDED to *blocks.  rec (nullable)
+ * receives n x RHG_REC_N doubles. */
+int rhg_widom_run(rhg_ctx *ctx, const rhg_molecule *mol, uint64_t seed, int64_t n_total,
+                  int64_t first, int64_t n, int32_t steps_per_cycle, rhg_widom_blocks *blocks,
+                  double *rec);
+/* Same move with host-supplied trials (the caller's own RNG, e.g. RASPA2's
+ * MT19937-64): fb n x k_fb x 3 first-bead positions, cord n x k_or x
+ * n_sites x 3 rotated offsets from the starting bead, u_sel n x 2 selection
+ * uniforms.  No blocks; rec is required. */
+int rhg_widom_run_trials(rhg_ctx *ctx, const rhg_molecule *mol, int64_t n, const double *fb,
+                         const double *cord, const double *u_sel, double *rec);
+/* GetAverageHenryCoefficient / block prints (src/statistics.c:2539,
+ * 4747-4800): out[0..4] K_H per block [mol/kg/Pa], out[5] average, out[6]
+ * 2 sigma, out[7..11] <U_gh>_1-<U_h>_0 per block [K], out[12] average,
+ * out[13] 2 sigma, out[14] overall K_H, out[15] overall <W>. */
+int rhg_widom_estimators(const rhg_widom_blocks *blocks, double temperature, double density_kg_m3,
+                         double ideal_gas_weight, double out[16]);
+
+/* ----------------------------------------------------------------- grids */
+/* MakeGrid (src/grids.c:431-583) for one probe type over a unit-cell box.
+ * fp64 = 0 stores RASPA2's FP32 node layout [i][j][k][8]; fp64 = 1 keeps the
+ * same 8 channels in FP64. */
+int rhg_make_grid(rhg_ctx *ctx, int32_t type, const double unit_box[9], double spacing,
+                  int32_t fp64, rhg_grid **out);
+/* n[3], size[3], shift[3], delta[3] as doubles, then bytes per node */
+int rhg_grid_info(const rhg_grid *g, double out[13]);
+int rhg_grid_download(const rhg_grid *g, void *host, size_t bytes);
+/* REAL InterpolateVDWGrid(int,VECTOR) (src/grids.h:70, src/grids.c:979) */
+int rhg_interpolate_vdw_grid(rhg_grid *g, int64_t n, const double *xyz, double *u);
+void rhg_grid_free(rhg_grid *g);
+/* void CalculateDerivativesAtPositionVDW(VECTOR,int,REAL*,VECTOR*,REAL_MATRIX3x3*,REAL*)
+ * (src/framework_hessian.h:43, src/framework_hessian.c:63): out n x 14
+ * (U, dU[3], H[9] in ax..cz order, third). */
+int rhg_derivatives_at_positions_vdw(rhg_ctx *ctx, int32_t type, int64_t n, const double *xyz,
+                                     double *out);
+
+/* ------------------------------------------------------------- profiling */
+/* HIP-event timing of the kernels launched on the context's stream.
+ * kernel names: "site_energy", "fourier", "widom_fb", "widom_or", "grid_build",
+ * "grid_interp", "sf".  Returns launch count and summed milliseconds. */
+int rhg_profile_enable(rhg_ctx *ctx, int32_t on);
+int rhg_profile_get(rhg_ctx *ctx, const char *kernel, int64_t *launches, double *total_ms);
+int rhg_profile_reset(rhg_ctx *ctx);
+int rhg_synchronize(rhg_ctx *ctx);
+
+/* ------------------------------------------------------ workload utility */
+/* No counterpart in the reference (its frameworks come from CIF files):
+ * synthetic framework positions for benchmark and test workloads, by random
+ * sequential addition of uniform positions with a minimum-image minimum
+ * separation d_min [A].  Returns RHG_ERR_INVALID if fewer than n atoms could
+ * be placed within max_attempts (n_placed says how many). */
+int rhg_synthetic_positions(uint64_t seed, const double box[9], int32_t n, double d_min,
+                            int64_t max_attempts, double *xyz, int32_t *n_placed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

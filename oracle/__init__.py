@@ -1,0 +1,333 @@
+"""ctypes binding of the oracle (CPU FP64 restatement of RASPA2's host-guest path).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, by __graft_entry__.smoke() and by
+bench.py's cpu_baseline leg, where it is the checker.  The product package
+(raspa2_amd) never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+
+BC_RECTANGULAR = 1
+BC_TRICLINIC = 2
+CHARGE_NONE = 0
+CHARGE_EWALD = 1
+MAX_SITES = 8
+
+REC = dict(W=0, DU=1, UVDW=2, UREAL=3, UFHOST=4, UFADS=5, OVERLAP=6, SELFB=7, SELOR=8,
+           FBMASK=9, ORMASK=10, W1=11, W2=12, STAGE=13)
+REC_N = 14
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = ctypes.CDLL(_LIB)
+        _declare(_lib)
+    return _lib
+
+
+_D = ctypes.c_double
+_I = ctypes.c_int
+_I64 = ctypes.c_int64
+_U64 = ctypes.c_uint64
+_P = ctypes.c_void_p
+
+
+class Molecule(ctypes.Structure):
+    _fields_ = [("n_sites", _I), ("type", _I * MAX_SITES), ("body", (_D * 3) * MAX_SITES),
+                ("start", _I), ("k_fb", _I), ("k_or", _I)]
+
+
+class Trials(ctypes.Structure):
+    _fields_ = [("fb", (_D * 3) * 64), ("cord", ((_D * 3) * MAX_SITES) * 64),
+                ("u_sel_fb", _D), ("u_sel_or", _D)]
+
+
+def _declare(L):
+    def f(name, res, *args):
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = list(args)
+    f("orc_units", None, _P)
+    f("orc_mt64_sequence", None, _U64, _I64, _P)
+    f("orc_mt64_uniform_sequence", None, _U64, _I64, _P)
+    f("orc_stream_key", _U64, _U64, _U64)
+    f("orc_stream_uniform", _D, _U64, _U64)
+    f("orc_stream_uniforms", None, _U64, _U64, _I64, _P)
+    f("orc_cell_make", None, _D, _D, _D, _D, _D, _D, _P, _P, _P)
+    f("orc_invert3x3", None, _P, _P, _P)
+    f("orc_cell_properties", None, _P, _P)
+    f("orc_min_image", None, _P, _P, _I, _P)
+    f("orc_system_create", _P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _D, _I, _I, _D, _D, _D)
+    f("orc_system_destroy", None, _P)
+    f("orc_add_pseudo_atom", _I, _P, _D, _D, _D)
+    f("orc_pair_params", None, _P, _I, _I, _P)
+    f("orc_system_info", None, _P, _P)
+    f("orc_vdw_energy_at", _D, _P, _I, _P, _D)
+    f("orc_charge_energy_at", _D, _P, _I, _P, _D)
+    f("orc_energy_at_many", None, _P, _I, _I64, _P, _P, _P)
+    f("orc_ewald_nk", _I, _P)
+    f("orc_ewald_tables", None, _P, _P, _P, _P, _P, _P)
+    f("orc_ewald_fourier_self_energy_framework", _D, _P)
+    f("orc_ewald_real_energy_framework", _D, _P)
+    f("orc_ewald_self_energy_framework", _D, _P)
+    f("orc_fourier_adsorbate", None, _P, _I, _P, _P, _P)
+    f("orc_widom_insertion_trials", None, _P, _P, _P, _P)
+    f("orc_widom_run_stream", None, _P, _P, _U64, _I64, _I64, _P)
+    f("orc_stream_trials", None, _P, _P, _U64, _I64, _P, _P, _P)
+    f("orc_widom_run_mt", _I64, _P, _P, _U64, _I64, _I, _P)
+    f("orc_block_of_cycles", None, _I64, _P)
+    f("orc_widom_estimators", None, _P, _D, _D, _D, _P)
+    f("orc_framework_density", _D, _D, _D)
+    f("orc_grid_geometry", None, _P, _D, _P)
+    f("orc_derivatives_vdw_at", None, _P, _I, _P, _P)
+    f("orc_grid_node", None, _P, _I, _P, _I, _I, _I, _P)
+    f("orc_lm_coeff", None, _P)
+    f("orc_interpolate_grid", _D, _P, _P, _I, _P, _P, _P)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def units():
+    out = np.zeros(6)
+    lib().orc_units(_ptr(out))
+    return dict(energy=out[0], coulomb=out[1], energy_to_kelvin=out[2], kelvin_to_energy=out[3],
+                k_b=out[4], coulomb_kelvin=out[5])
+
+
+def mt64(seed, n):
+    out = np.zeros(n, dtype=np.uint64)
+    lib().orc_mt64_sequence(seed, n, _ptr(out))
+    return out
+
+
+def mt64_uniform(seed, n):
+    out = np.zeros(n)
+    lib().orc_mt64_uniform_sequence(seed, n, _ptr(out))
+    return out
+
+
+def stream_uniforms(seed, insertion, n):
+    out = np.zeros(n)
+    lib().orc_stream_uniforms(seed, insertion, n, _ptr(out))
+    return out
+
+
+def cell_make(a, b, c, alpha=90.0, beta=90.0, gamma=90.0):
+    box, inv, props = np.zeros(9), np.zeros(9), np.zeros(7)
+    lib().orc_cell_make(a, b, c, alpha, beta, gamma, _ptr(box), _ptr(inv), _ptr(props))
+    return box, inv, props
+
+
+def invert3x3(m):
+    m = _f64(m)
+    out = np.zeros(9)
+    det = np.zeros(1)
+    lib().orc_invert3x3(_ptr(m), _ptr(out), _ptr(det))
+    return out, det[0]
+
+
+def min_image(box, inv, bc, dr):
+    box, inv = _f64(box), _f64(inv)
+    d = _f64(dr).copy()
+    lib().orc_min_image(_ptr(box), _ptr(inv), bc, _ptr(d))
+    return d
+
+
+class System:
+    """One framework + force field + (optional) Ewald setup."""
+
+    def __init__(self, box, bc, x, y, z, charge, eps_k, sigma, mass=None, cutoff_vdw=12.0,
+                 shifted=True, charge_method=CHARGE_NONE, cutoff_charge=12.0,
+                 ewald_precision=1e-6, temperature=300.0):
+        L = lib()
+        self._keep = [_f64(box), _f64(x), _f64(y), _f64(z), _f64(charge), _f64(eps_k), _f64(sigma)]
+        self._keep.append(_f64(mass) if mass is not None else None)
+        b, xx, yy, zz, qq, ee, ss, mm = self._keep
+        self.n = len(xx)
+        self.temperature = temperature
+        self.h = L.orc_system_create(_ptr(b), bc, self.n, _ptr(xx), _ptr(yy), _ptr(zz), _ptr(qq),
+                                     _ptr(ee), _ptr(ss), _ptr(mm) if mm is not None else None,
+                                     cutoff_vdw, 1 if shifted else 0, charge_method,
+                                     cutoff_charge, ewald_precision, temperature)
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().orc_system_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def add_pseudo_atom(self, eps_k, sigma, charge=0.0):
+        return lib().orc_add_pseudo_atom(self.h, eps_k, sigma, charge)
+
+    def pair_params(self, t, atom):
+        out = np.zeros(3)
+        lib().orc_pair_params(self.h, t, atom, _ptr(out))
+        return out
+
+    def info(self):
+        out = np.zeros(8)
+        lib().orc_system_info(self.h, _ptr(out))
+        return dict(alpha=out[0], kx=int(out[1]), ky=int(out[2]), kz=int(out[3]), nk=int(out[4]),
+                    uion=out[5], beta=out[6], volume=out[7])
+
+    def vdw_at(self, t, pos, scaling=1.0):
+        p = _f64(pos)
+        return lib().orc_vdw_energy_at(self.h, t, _ptr(p), scaling)
+
+    def charge_at(self, t, pos, scaling=1.0):
+        p = _f64(pos)
+        return lib().orc_charge_energy_at(self.h, t, _ptr(p), scaling)
+
+    def energy_at_many(self, t, xyz):
+        xyz = _f64(xyz).reshape(-1, 3)
+        n = len(xyz)
+        uv, uc = np.zeros(n), np.zeros(n)
+        lib().orc_energy_at_many(self.h, t, n, _ptr(xyz), _ptr(uv), _ptr(uc))
+        return uv, uc
+
+    def ewald_tables(self):
+        nk = lib().orc_ewald_nk(self.h)
+        kv, kf, sr, si = np.zeros(3 * nk), np.zeros(nk), np.zeros(nk), np.zeros(nk)
+        ki = np.zeros(3 * nk, dtype=np.int32)
+        lib().orc_ewald_tables(self.h, _ptr(kv), _ptr(kf), _ptr(sr), _ptr(si), _ptr(ki))
+        return dict(kvec=kv.reshape(-1, 3), kfactor=kf, sf_re=sr, sf_im=si, kidx=ki.reshape(-1, 3))
+
+    def fourier_self_framework(self):
+        return lib().orc_ewald_fourier_self_energy_framework(self.h)
+
+    def real_energy_framework(self):
+        return lib().orc_ewald_real_energy_framework(self.h)
+
+    def self_energy_framework(self):
+        return lib().orc_ewald_self_energy_framework(self.h)
+
+    def fourier_adsorbate(self, types, xyz):
+        types = np.ascontiguousarray(types, dtype=np.int32)
+        xyz = _f64(xyz).reshape(-1)
+        out = np.zeros(5)
+        lib().orc_fourier_adsorbate(self.h, len(types), _ptr(types), _ptr(xyz), _ptr(out))
+        return out
+
+    def derivatives_vdw_at(self, t, pos):
+        p = _f64(pos)
+        out = np.zeros(14)
+        lib().orc_derivatives_vdw_at(self.h, t, _ptr(p), _ptr(out))
+        return out
+
+    def grid_node(self, t, geom, i, j, k):
+        g = _f64(geom)
+        out = np.zeros(8, dtype=np.float32)
+        lib().orc_grid_node(self.h, t, _ptr(g), i, j, k, _ptr(out))
+        return out
+
+
+def molecule(types, body, start=0, k_fb=10, k_or=10):
+    m = Molecule()
+    m.n_sites = len(types)
+    for i, t in enumerate(types):
+        m.type[i] = int(t)
+        for d in range(3):
+            m.body[i][d] = float(body[i][d])
+    m.start = start
+    m.k_fb = k_fb
+    m.k_or = k_or
+    return m
+
+
+def widom_run_stream(system, mol, seed, first, n):
+    rec = np.zeros((n, REC_N))
+    lib().orc_widom_run_stream(system.h, ctypes.byref(mol), seed, first, n, _ptr(rec))
+    return rec
+
+
+def stream_trials(system, mol, seed, insertion):
+    fb = np.zeros((mol.k_fb, 3))
+    cord = np.zeros((mol.k_or, mol.n_sites, 3))
+    usel = np.zeros(2)
+    lib().orc_stream_trials(system.h, ctypes.byref(mol), seed, insertion, _ptr(fb), _ptr(cord),
+                            _ptr(usel))
+    return fb, cord, usel
+
+
+def widom_insertion_trials(system, mol, fb, cord, usel):
+    t = Trials()
+    for i in range(len(fb)):
+        for d in range(3):
+            t.fb[i][d] = float(fb[i][d])
+    for iu in range(cord.shape[0]):
+        for j in range(cord.shape[1]):
+            for d in range(3):
+                t.cord[iu][j][d] = float(cord[iu][j][d])
+    t.u_sel_fb = float(usel[0])
+    t.u_sel_or = float(usel[1])
+    rec = np.zeros(REC_N)
+    lib().orc_widom_insertion_trials(system.h, ctypes.byref(mol), ctypes.byref(t), _ptr(rec))
+    return rec
+
+
+def widom_run_mt(system, mol, seed, n_cycles, steps_per_cycle=20):
+    blocks = np.zeros((5, 4))
+    n = lib().orc_widom_run_mt(system.h, ctypes.byref(mol), seed, n_cycles, steps_per_cycle,
+                               _ptr(blocks))
+    return n, blocks
+
+
+def block_of_cycles(n_cycles):
+    out = np.zeros(n_cycles, dtype=np.int32)
+    lib().orc_block_of_cycles(n_cycles, _ptr(out))
+    return out
+
+
+def widom_estimators(blocks, temperature, density, ideal_gas_weight=1.0):
+    b = _f64(blocks).reshape(-1)
+    out = np.zeros(16)
+    lib().orc_widom_estimators(_ptr(b), temperature, density, ideal_gas_weight, _ptr(out))
+    return dict(kh_blocks=out[0:5], kh=out[5], kh_err=out[6], u_blocks=out[7:12], u=out[12],
+                u_err=out[13], kh_overall=out[14], w_overall=out[15])
+
+
+def framework_density(mass_g_mol, volume):
+    return lib().orc_framework_density(mass_g_mol, volume)
+
+
+def grid_geometry(unit_box, spacing):
+    ub = _f64(unit_box)
+    out = np.zeros(12)
+    lib().orc_grid_geometry(_ptr(ub), spacing, _ptr(out))
+    return out
+
+
+def lm_coeff():
+    out = np.zeros(64 * 64, dtype=np.int32)
+    lib().orc_lm_coeff(_ptr(out))
+    return out.reshape(64, 64)
+
+
+def interpolate_grid(grid, geom, bc, unit_box, unit_inv, pos):
+    g = np.ascontiguousarray(grid, dtype=np.float32)
+    geom, ub, ui, p = _f64(geom), _f64(unit_box), _f64(unit_inv), _f64(pos)
+    return lib().orc_interpolate_grid(_ptr(g), _ptr(geom), bc, _ptr(ub), _ptr(ui), _ptr(p))

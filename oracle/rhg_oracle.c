@@ -1,0 +1,1427 @@
+/*
+ * rhg_oracle.c -- CPU FP64 restatement of RASPA2's host-guest energy path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Built with strict IEEE double
+ * arithmetic (-ffp-contract=off, no fast-math) and the reference's operation
+ * order.  Citations are to the RASPA2 sources (src/...).
+ */
+#include "rhg_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* strict IEEE: the Makefile passes -ffp-contract=off */
+
+/* ===================================================================== */
+/* constants (src/constants.h:39-62)                                      */
+/* ===================================================================== */
+#define ANGSTROM_ 1e-10
+#define PICO_SECOND_ 1e-12
+#define ATOMIC_MASS_UNIT_ 1.6605402e-27
+#define ELECTRONIC_CHARGE_UNIT_ 1.60217733e-19
+#define MOLAR_GAS_CONSTANT_ 8.314464919
+#define BOLTZMANN_CONSTANT_ 1.380650324e-23
+#define AVOGADRO_CONSTANT_ 6.0221419947e23
+#define ELECTRIC_CONSTANT_ 8.8541878176e-12
+#define ENERGY_OVERLAP_CRITERIA 1.0e7          /* src/input.c:367 */
+#define MINIMUM_ROSENBLUTH_FACTOR 1.0e-150     /* src/input.c:368 */
+#define NR_BLOCKS 5                            /* src/statistics.h:40 */
+
+#define SQR(x) ((x) * (x))
+#define CUBE(x) ((x) * (x) * (x))
+
+/* NINT of src/utils.h:60: round half away from zero through (int) */
+static inline int nint(double x) { return (int)((x) >= 0.0 ? ((x) + 0.5) : ((x)-0.5)); }
+
+/* src/constants.c:141-180 with LENGTH_UNIT=1e-10, TIME_UNIT=1e-12,
+ * MASS_UNIT=amu, CHARGE_UNIT=e, DielectricConstantOfTheMedium=1 */
+typedef struct { double E, C, E2K, K2E, KB; } units_t;
+static units_t make_units(void) {
+  volatile double length = ANGSTROM_, time = PICO_SECOND_, mass = ATOMIC_MASS_UNIT_,
+                  charge = ELECTRONIC_CHARGE_UNIT_, dielectric = 1.0;
+  units_t u;
+  u.E = ((mass)*SQR(length) / SQR(time));
+  u.KB = (BOLTZMANN_CONSTANT_ / u.E);
+  u.C = (((SQR(charge) / (4.0 * M_PI * ELECTRIC_CONSTANT_ * length * u.E))) / (dielectric));
+  u.E2K = ((u.E * AVOGADRO_CONSTANT_) / MOLAR_GAS_CONSTANT_);
+  u.K2E = (MOLAR_GAS_CONSTANT_ / (u.E * AVOGADRO_CONSTANT_));
+  return u;
+}
+
+void orc_units(double out[6]) {
+  units_t u = make_units();
+  out[0] = u.E;
+  out[1] = u.C;
+  out[2] = u.E2K;
+  out[3] = u.K2E;
+  out[4] = u.KB;
+  out[5] = u.C * u.E2K;
+}
+
+/* ===================================================================== */
+/* MT19937-64 (src/utils.c:308-392)                                      */
+/* ===================================================================== */
+#define MT_NN 312
+#define MT_MM 156
+#define MT_MATRIX_A 0xB5026F5AA96619E9ULL
+#define MT_UM 0xFFFFFFFF80000000ULL
+#define MT_LM 0x7FFFFFFFULL
+
+void orc_mt64_seed(orc_mt64 *g, uint64_t seed) {
+  g->mt[0] = seed;
+  for (g->mti = 1; g->mti < MT_NN; g->mti++)
+    g->mt[g->mti] = (6364136223846793005ULL * (g->mt[g->mti - 1] ^ (g->mt[g->mti - 1] >> 62)) +
+                     (uint64_t)g->mti);
+}
+
+uint64_t orc_mt64_next(orc_mt64 *g) {
+  static const uint64_t mag01[2] = {0ULL, MT_MATRIX_A};
+  int i;
+  uint64_t x;
+  if (g->mti >= MT_NN) {
+    if (g->mti == MT_NN + 1) orc_mt64_seed(g, 5489ULL);
+    for (i = 0; i < MT_NN - MT_MM; i++) {
+      x = (g->mt[i] & MT_UM) | (g->mt[i + 1] & MT_LM);
+      g->mt[i] = g->mt[i + MT_MM] ^ (x >> 1) ^ mag01[(int)(x & 1ULL)];
+    }
+    for (; i < MT_NN - 1; i++) {
+      x = (g->mt[i] & MT_UM) | (g->mt[i + 1] & MT_LM);
+      g->mt[i] = g->mt[i + (MT_MM - MT_NN)] ^ (x >> 1) ^ mag01[(int)(x & 1ULL)];
+    }
+    x = (g->mt[MT_NN - 1] & MT_UM) | (g->mt[0] & MT_LM);
+    g->mt[MT_NN - 1] = g->mt[MT_MM - 1] ^ (x >> 1) ^ mag01[(int)(x & 1ULL)];
+    g->mti = 0;
+  }
+  x = g->mt[g->mti++];
+  x ^= (x >> 29) & 0x5555555555555555ULL;
+  x ^= (x << 17) & 0x71D67FFFEDA60000ULL;
+  x ^= (x << 37) & 0xFFF7EEE000000000ULL;
+  x ^= (x >> 43);
+  return x;
+}
+
+/* RandomNumber(): closed interval [0,1] (src/utils.c:378-381) */
+double orc_mt64_uniform(orc_mt64 *g) {
+  return (double)(orc_mt64_next(g) >> 11) * (1.0 / 9007199254740991.0);
+}
+
+void orc_mt64_sequence(uint64_t seed, int64_t n, uint64_t *out) {
+  orc_mt64 g;
+  orc_mt64_seed(&g, seed);
+  for (int64_t i = 0; i < n; i++) out[i] = orc_mt64_next(&g);
+}
+
+void orc_mt64_uniform_sequence(uint64_t seed, int64_t n, double *out) {
+  orc_mt64 g;
+  orc_mt64_seed(&g, seed);
+  for (int64_t i = 0; i < n; i++) out[i] = orc_mt64_uniform(&g);
+}
+
+/* ===================================================================== */
+/* Counter-based per-insertion stream (SplitMix64 output function,        */
+/* Steele, Lea & Flood, OOPSLA 2014).  Not in the reference: it defines   */
+/* the seeded trial positions of the data-parallel Widom path.            */
+/* ===================================================================== */
+static inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+#define GOLDEN_GAMMA 0x9E3779B97F4A7C15ULL
+
+/* sin and cos of x in [0, 2*pi] evaluated identically on host and device:
+ * Cody-Waite reduction by pi/2 (two-part constant), Taylor series of sin
+ * through r^21 and cos through r^20 on |r| <= pi/4, Horner form.  Used for
+ * the Shoemake angles of counter-stream trials only. */
+static void sincos_portable(double x, double *s, double *c) {
+  const double two_over_pi = 0.63661977236758138243;
+  const double pio2_hi = 1.57079632673412561417e+00;
+  const double pio2_lo = 6.07710050650619224932e-11;
+  double kd = trunc(x * two_over_pi + 0.5);
+  int k = (int)kd;
+  double r = (x - kd * pio2_hi) - kd * pio2_lo;
+  double r2 = r * r;
+  const double s3 = -1.0 / 6.0, s5 = 1.0 / 120.0, s7 = -1.0 / 5040.0, s9 = 1.0 / 362880.0,
+               s11 = -1.0 / 39916800.0, s13 = 1.0 / 6227020800.0, s15 = -1.0 / 1307674368000.0,
+               s17 = 1.0 / 355687428096000.0, s19 = -1.0 / 121645100408832000.0,
+               s21 = 1.0 / 51090942171709440000.0;
+  const double c2 = -0.5, c4 = 1.0 / 24.0, c6 = -1.0 / 720.0, c8 = 1.0 / 40320.0,
+               c10 = -1.0 / 3628800.0, c12 = 1.0 / 479001600.0, c14 = -1.0 / 87178291200.0,
+               c16 = 1.0 / 20922789888000.0, c18 = -1.0 / 6402373705728000.0,
+               c20 = 1.0 / 2432902008176640000.0;
+  double ps = s21;
+  ps = ps * r2 + s19;
+  ps = ps * r2 + s17;
+  ps = ps * r2 + s15;
+  ps = ps * r2 + s13;
+  ps = ps * r2 + s11;
+  ps = ps * r2 + s9;
+  ps = ps * r2 + s7;
+  ps = ps * r2 + s5;
+  ps = ps * r2 + s3;
+  double sr = r + (r * r2) * ps;
+  double pc = c20;
+  pc = pc * r2 + c18;
+  pc = pc * r2 + c16;
+  pc = pc * r2 + c14;
+  pc = pc * r2 + c12;
+  pc = pc * r2 + c10;
+  pc = pc * r2 + c8;
+  pc = pc * r2 + c6;
+  pc = pc * r2 + c4;
+  pc = pc * r2 + c2;
+  double cr = 1.0 + r2 * pc;
+  switch (k & 3) {
+  case 0: *s = sr; *c = cr; break;
+  case 1: *s = cr; *c = -sr; break;
+  case 2: *s = -sr; *c = -cr; break;
+  default: *s = -cr; *c = sr; break;
+  }
+}
+
+void orc_sincos_portable(double x, double *s, double *c) { sincos_portable(x, s, c); }
+
+uint64_t orc_stream_key(uint64_t seed, uint64_t insertion) {
+  return mix64(seed ^ mix64(insertion + 0x632BE59BD9B4E019ULL));
+}
+
+double orc_stream_uniform(uint64_t key, uint64_t counter) {
+  uint64_t x = mix64(key + (counter + 1ULL) * GOLDEN_GAMMA);
+  return (double)(x >> 11) * (1.0 / 9007199254740991.0);
+}
+
+void orc_stream_uniforms(uint64_t seed, uint64_t insertion, int64_t n, double *out) {
+  uint64_t key = orc_stream_key(seed, insertion);
+  for (int64_t i = 0; i < n; i++) out[i] = orc_stream_uniform(key, (uint64_t)i);
+}
+
+/* ===================================================================== */
+/* cell (src/framework.c:3024-3066, src/matrix.c:902-927,                 */
+/*       src/framework.c:6338-6393)                                       */
+/* ===================================================================== */
+enum { AX = 0, AY, AZ, BX, BY, BZ, CX, CY, CZ };
+
+void orc_invert3x3(const double in[9], double out[9], double *determinant) {
+  double r, d;
+  out[AX] = in[BY] * in[CZ] - in[BZ] * in[CY];
+  out[AY] = in[AZ] * in[CY] - in[AY] * in[CZ];
+  out[AZ] = in[AY] * in[BZ] - in[AZ] * in[BY];
+  out[BX] = in[BZ] * in[CX] - in[BX] * in[CZ];
+  out[BY] = in[AX] * in[CZ] - in[AZ] * in[CX];
+  out[BZ] = in[AZ] * in[BX] - in[AX] * in[BZ];
+  out[CX] = in[BX] * in[CY] - in[BY] * in[CX];
+  out[CY] = in[AY] * in[CX] - in[AX] * in[CY];
+  out[CZ] = in[AX] * in[BY] - in[AY] * in[BX];
+  d = in[AX] * out[AX] + in[BX] * out[AY] + in[CX] * out[AZ];
+  r = 0.0;
+  if (fabs(d) > 0.0) r = 1.0 / d;
+  out[AX] *= r; out[BX] *= r; out[CX] *= r;
+  out[AY] *= r; out[BY] *= r; out[CY] *= r;
+  out[AZ] *= r; out[BZ] *= r; out[CZ] *= r;
+  *determinant = d;
+}
+
+void orc_cell_properties(const double in[9], double p[7]) {
+  double axb1, axb2, axb3, bxc1, bxc2, bxc3, cxa1, cxa2, cxa3, vol;
+  p[0] = sqrt(SQR(in[AX]) + SQR(in[AY]) + SQR(in[AZ]));
+  p[1] = sqrt(SQR(in[BX]) + SQR(in[BY]) + SQR(in[BZ]));
+  p[2] = sqrt(SQR(in[CX]) + SQR(in[CY]) + SQR(in[CZ]));
+  axb1 = in[AY] * in[BZ] - in[AZ] * in[BY];
+  axb2 = in[AZ] * in[BX] - in[AX] * in[BZ];
+  axb3 = in[AX] * in[BY] - in[AY] * in[BX];
+  bxc1 = in[BY] * in[CZ] - in[BZ] * in[CY];
+  bxc2 = in[BZ] * in[CX] - in[BX] * in[CZ];
+  bxc3 = in[BX] * in[CY] - in[BY] * in[CX];
+  cxa1 = in[CY] * in[AZ] - in[AY] * in[CZ];
+  cxa2 = in[AX] * in[CZ] - in[AZ] * in[CX];
+  cxa3 = in[AY] * in[CX] - in[AX] * in[CY];
+  vol = fabs(in[AX] * bxc1 + in[AY] * bxc2 + in[AZ] * bxc3);
+  p[3] = vol / sqrt(bxc1 * bxc1 + bxc2 * bxc2 + bxc3 * bxc3);
+  p[4] = vol / sqrt(cxa1 * cxa1 + cxa2 * cxa2 + cxa3 * cxa3);
+  p[5] = vol / sqrt(axb1 * axb1 + axb2 * axb2 + axb3 * axb3);
+  p[6] = vol;
+}
+
+void orc_cell_make(double A, double B, double C, double alpha_deg, double beta_deg,
+                   double gamma_deg, double box[9], double inv[9], double props[7]) {
+  double alpha = alpha_deg * (M_PI / 180.0);
+  double beta = beta_deg * (M_PI / 180.0);
+  double gamma = gamma_deg * (M_PI / 180.0);
+  double det;
+  double tempd = (cos(alpha) - cos(gamma) * cos(beta)) / sin(gamma);
+  box[AX] = A;
+  box[AY] = 0.0;
+  box[AZ] = 0.0;
+  box[BX] = B * cos(gamma);
+  box[BY] = B * sin(gamma);
+  box[BZ] = 0.0;
+  box[CX] = C * cos(beta);
+  box[CY] = C * tempd;
+  box[CZ] = C * sqrt(1.0 - SQR(cos(beta)) - SQR(tempd));
+  orc_invert3x3(box, inv, &det);
+  orc_cell_properties(box, props);
+}
+
+/* ApplyBoundaryCondition (src/potentials.h:232-271) */
+void orc_min_image(const double box[9], const double inv[9], int bc, double dr[3]) {
+  double s[3], t[3];
+  if (bc == ORC_BC_RECTANGULAR) {
+    dr[0] -= box[AX] * (double)nint(dr[0] * inv[AX]);
+    dr[1] -= box[BY] * (double)nint(dr[1] * inv[BY]);
+    dr[2] -= box[CZ] * (double)nint(dr[2] * inv[CZ]);
+  } else {
+    s[0] = inv[AX] * dr[0] + inv[BX] * dr[1] + inv[CX] * dr[2];
+    s[1] = inv[AY] * dr[0] + inv[BY] * dr[1] + inv[CY] * dr[2];
+    s[2] = inv[AZ] * dr[0] + inv[BZ] * dr[1] + inv[CZ] * dr[2];
+    t[0] = s[0] - (double)nint(s[0]);
+    t[1] = s[1] - (double)nint(s[1]);
+    t[2] = s[2] - (double)nint(s[2]);
+    dr[0] = box[AX] * t[0] + box[BX] * t[1] + box[CX] * t[2];
+    dr[1] = box[AY] * t[0] + box[BY] * t[1] + box[CY] * t[2];
+    dr[2] = box[AZ] * t[0] + box[BZ] * t[1] + box[CZ] * t[2];
+  }
+}
+
+/* ===================================================================== */
+/* system                                                                */
+/* ===================================================================== */
+typedef struct { double re, im; } cplx;
+
+struct orc_system {
+  double box[9], inv[9], props[7];
+  int bc;
+  int n;
+  double *x, *y, *z, *q, *eps, *sig;
+  double mass_total;
+  double rc_vdw2;
+  int shifted;
+  int charge_method;
+  double rc_charge, rc_charge2;
+  units_t u;
+  double temperature, beta;
+  int ntypes;
+  double t_eps[ORC_MAX_TYPES], t_sig[ORC_MAX_TYPES], t_q[ORC_MAX_TYPES];
+  double *pp[ORC_MAX_TYPES]; /* n x 3: eps_internal, sigma, shift */
+  /* Ewald */
+  double alpha;
+  int kx, ky, kz, nk;
+  double recip_cutoff;
+  double *kvec, *kfactor;
+  int *kidx;
+  cplx *sf;
+  double uion;
+  double net_charge_framework;
+};
+
+/* LENNARD_JONES branch of PotentialValue (src/potentials.c:5867-5877) */
+static inline double lj_value(double arg1, double sigma, double arg3, double rr) {
+  double arg2 = SQR(sigma);
+  double rri3 = CUBE(arg2 / rr);
+  return 4.0 * arg1 * (rri3 * (rri3 - 1.0)) - arg3;
+}
+
+int orc_add_pseudo_atom(orc_system *s, double eps_k, double sigma, double charge) {
+  if (s->ntypes >= ORC_MAX_TYPES) return -1;
+  int t = s->ntypes++;
+  s->t_eps[t] = eps_k;
+  s->t_sig[t] = sigma;
+  s->t_q[t] = charge;
+  s->pp[t] = (double *)malloc(sizeof(double) * 3 * (size_t)(s->n > 0 ? s->n : 1));
+  double ei = eps_k * s->u.K2E; /* src/potentials.c:268-274 */
+  for (int i = 0; i < s->n; i++) {
+    double ej = s->eps[i] * s->u.K2E;
+    double e = sqrt(ei * ej);                 /* src/potentials.c:3164-3171 */
+    double sg = 0.5 * (sigma + s->sig[i]);
+    double shift = 0.0;
+    if (s->shifted) shift = lj_value(e, sg, 0.0, s->rc_vdw2); /* src/potentials.c:5263-5285 */
+    s->pp[t][3 * i + 0] = e;
+    s->pp[t][3 * i + 1] = sg;
+    s->pp[t][3 * i + 2] = shift;
+  }
+  return t;
+}
+
+void orc_pair_params(const orc_system *s, int type, int atom, double out[3]) {
+  out[0] = s->pp[type][3 * atom + 0];
+  out[1] = s->pp[type][3 * atom + 1];
+  out[2] = s->pp[type][3 * atom + 2];
+}
+
+static int considered_charged(double q) { return fabs(q) > 1e-10; }
+
+/* InitializeEwald automatic branch (src/ewald.c:200-260) */
+static void ewald_init(orc_system *s, double precision) {
+  double eps = fmin(fabs(precision), 0.5);
+  double rc = s->rc_charge;
+  double tol = sqrt(fabs(log(eps * rc)));
+  s->alpha = sqrt(fabs(log(eps * rc * tol))) / rc;
+  double tol1 = sqrt(-log(eps * rc * SQR(2.0 * tol * s->alpha)));
+  s->kx = (int)nint((double)0.25 + s->props[0] * s->alpha * tol1 / M_PI);
+  s->ky = (int)nint((double)0.25 + s->props[1] * s->alpha * tol1 / M_PI);
+  s->kz = (int)nint((double)0.25 + s->props[2] * s->alpha * tol1 / M_PI);
+  int m = s->kx;
+  if (s->ky > m) m = s->ky;
+  if (s->kz > m) m = s->kz;
+  s->recip_cutoff = SQR(1.05 * m);
+  int nvec = 0;
+  for (int ii = 0; ii <= s->kx; ii++)
+    for (int jj = -s->ky; jj <= s->ky; jj++)
+      for (int kk = -s->kz; kk <= s->kz; kk++) {
+        double ksqr = SQR(ii) + SQR(jj) + SQR(kk);
+        if ((ksqr != 0) && (ksqr < s->recip_cutoff)) nvec++;
+      }
+  s->nk = nvec;
+}
+
+/* SetupKVectors (src/ewald.c:532-600) */
+static void ewald_setup_kvectors(orc_system *s) {
+  s->kvec = (double *)malloc(sizeof(double) * 3 * (size_t)s->nk);
+  s->kfactor = (double *)malloc(sizeof(double) * (size_t)s->nk);
+  s->kidx = (int *)malloc(sizeof(int) * 3 * (size_t)s->nk);
+  const double *inv = s->inv;
+  double volume = s->props[6];
+  double alpha = s->alpha;
+  int nvec = 0;
+  for (int ii = 0; ii <= s->kx; ii++) {
+    double rk2x = 2.0 * M_PI * ii * inv[AX];
+    double rk2y = 2.0 * M_PI * ii * inv[BX];
+    double rk2z = 2.0 * M_PI * ii * inv[CX];
+    for (int jj = -s->ky; jj <= s->ky; jj++) {
+      double rk1x = rk2x + 2.0 * M_PI * jj * inv[AY];
+      double rk1y = rk2y + 2.0 * M_PI * jj * inv[BY];
+      double rk1z = rk2z + 2.0 * M_PI * jj * inv[CY];
+      for (int kk = -s->kz; kk <= s->kz; kk++) {
+        double rkx = rk1x + 2.0 * M_PI * kk * inv[AZ];
+        double rky = rk1y + 2.0 * M_PI * kk * inv[BZ];
+        double rkz = rk1z + 2.0 * M_PI * kk * inv[CZ];
+        double ksqr = SQR(ii) + SQR(jj) + SQR(kk);
+        if ((ksqr != 0) && (ksqr < s->recip_cutoff)) {
+          s->kvec[3 * nvec + 0] = rkx;
+          s->kvec[3 * nvec + 1] = rky;
+          s->kvec[3 * nvec + 2] = rkz;
+          s->kidx[3 * nvec + 0] = ii;
+          s->kidx[3 * nvec + 1] = jj;
+          s->kidx[3 * nvec + 2] = kk;
+          double rksqr = SQR(rkx) + SQR(rky) + SQR(rkz);
+          if (ii == 0)
+            s->kfactor[nvec] = s->u.C * (2.0 * M_PI / volume) * exp((-0.25 / SQR(alpha)) * rksqr) / rksqr;
+          else
+            s->kfactor[nvec] = s->u.C * (4.0 * M_PI / volume) * exp((-0.25 / SQR(alpha)) * rksqr) / rksqr;
+          nvec++;
+        }
+      }
+    }
+  }
+}
+
+/* exp(i k.r) tables for a set of sites, as in src/ewald.c:7045-7110.
+ * eik{x,y,z}[j*ns + i] for j in [-kmax, kmax] stored with offset kmax. */
+typedef struct { int ns, kx, ky, kz; cplx *ex, *ey, *ez; } eik_t;
+
+static void eik_build(eik_t *e, int ns, const double (*fpos)[3], int kx, int ky, int kz) {
+  e->ns = ns;
+  e->kx = kx;
+  e->ky = ky;
+  e->kz = kz;
+  e->ex = (cplx *)calloc((size_t)(2 * kx + 1) * (size_t)ns, sizeof(cplx));
+  e->ey = (cplx *)calloc((size_t)(2 * ky + 1) * (size_t)ns, sizeof(cplx));
+  e->ez = (cplx *)calloc((size_t)(2 * kz + 1) * (size_t)ns, sizeof(cplx));
+#define EX(j, i) e->ex[((j) + kx) * ns + (i)]
+#define EY(j, i) e->ey[((j) + ky) * ns + (i)]
+#define EZ(j, i) e->ez[((j) + kz) * ns + (i)]
+  for (int i = 0; i < ns; i++) {
+    EX(0, i).re = 1.0; EX(0, i).im = 0.0;
+    EY(0, i).re = 1.0; EY(0, i).im = 0.0;
+    EZ(0, i).re = 1.0; EZ(0, i).im = 0.0;
+    double px = fpos[i][0], py = fpos[i][1], pz = fpos[i][2];
+    if (kx >= 1) { EX(1, i).re = cos(px); EX(1, i).im = sin(px); }
+    if (ky >= 1) { EY(1, i).re = cos(py); EY(1, i).im = sin(py); EY(-1, i).re = cos(py); EY(-1, i).im = -sin(py); }
+    if (kz >= 1) { EZ(1, i).re = cos(pz); EZ(1, i).im = sin(pz); EZ(-1, i).re = cos(pz); EZ(-1, i).im = -sin(pz); }
+  }
+  for (int j = 2; j <= kx; j++)
+    for (int i = 0; i < ns; i++) {
+      EX(j, i).re = EX(j - 1, i).re * EX(1, i).re - EX(j - 1, i).im * EX(1, i).im;
+      EX(j, i).im = EX(j - 1, i).im * EX(1, i).re + EX(j - 1, i).re * EX(1, i).im;
+    }
+  for (int j = 2; j <= ky; j++)
+    for (int i = 0; i < ns; i++) {
+      EY(j, i).re = EY(j - 1, i).re * EY(1, i).re - EY(j - 1, i).im * EY(1, i).im;
+      EY(j, i).im = EY(j - 1, i).im * EY(1, i).re + EY(j - 1, i).re * EY(1, i).im;
+      EY(-j, i).re = EY(j, i).re;
+      EY(-j, i).im = -EY(j, i).im;
+    }
+  for (int j = 2; j <= kz; j++)
+    for (int i = 0; i < ns; i++) {
+      EZ(j, i).re = EZ(j - 1, i).re * EZ(1, i).re - EZ(j - 1, i).im * EZ(1, i).im;
+      EZ(j, i).im = EZ(j - 1, i).im * EZ(1, i).re + EZ(j - 1, i).re * EZ(1, i).im;
+      EZ(-j, i).re = EZ(j, i).re;
+      EZ(-j, i).im = -EZ(j, i).im;
+    }
+}
+static void eik_free(eik_t *e) { free(e->ex); free(e->ey); free(e->ez); }
+
+/* ConvertFromXYZtoABC then *TWO_PI (src/ewald.c:6990-6992, src/potentials.c:5566) */
+static void frac_2pi(const orc_system *s, const double p[3], double out[3]) {
+  const double *inv = s->inv;
+  out[0] = inv[AX] * p[0] + inv[BX] * p[1] + inv[CX] * p[2];
+  out[1] = inv[AY] * p[0] + inv[BY] * p[1] + inv[CY] * p[2];
+  out[2] = inv[AZ] * p[0] + inv[BZ] * p[1] + inv[CZ] * p[2];
+  out[0] *= (2.0 * M_PI);
+  out[1] *= (2.0 * M_PI);
+  out[2] *= (2.0 * M_PI);
+}
+
+/* Structure factors sum_i q_i exp(i k.r_i) over the k-vector list, in the
+ * reference's loop order (src/ewald.c:1289-1731 framework part). */
+static void structure_factors(const orc_system *s, int ns, const double *qs, const double (*fpos)[3],
+                              cplx *out) {
+  eik_t E;
+  eik_t *e = &E;
+  eik_build(e, ns, fpos, s->kx, s->ky, s->kz);
+  cplx *xy = (cplx *)malloc(sizeof(cplx) * (size_t)(ns > 0 ? ns : 1));
+  int kx = s->kx, ky = s->ky, kz = s->kz;
+  int nvec = 0;
+  for (int ii = 0; ii <= kx; ii++)
+    for (int jj = -ky; jj <= ky; jj++) {
+      for (int i = 0; i < ns; i++) {
+        xy[i].re = EX(ii, i).re * EY(jj, i).re - EX(ii, i).im * EY(jj, i).im;
+        xy[i].im = EX(ii, i).im * EY(jj, i).re + EX(ii, i).re * EY(jj, i).im;
+      }
+      for (int kk = -kz; kk <= kz; kk++) {
+        double ksqr = SQR(ii) + SQR(jj) + SQR(kk);
+        if ((ksqr != 0) && (ksqr < s->recip_cutoff)) {
+          cplx sum = {0.0, 0.0};
+          for (int i = 0; i < ns; i++) {
+            double re = xy[i].re * EZ(kk, i).re - xy[i].im * EZ(kk, i).im;
+            double im = xy[i].im * EZ(kk, i).re + xy[i].re * EZ(kk, i).im;
+            double temp = qs[i];
+            sum.re += temp * re;
+            sum.im += temp * im;
+          }
+          out[nvec++] = sum;
+        }
+      }
+    }
+  free(xy);
+  eik_free(e);
+}
+#undef EX
+#undef EY
+#undef EZ
+
+/* EwaldEnergyIon (src/ewald.c:615-742): single +1 charge at the origin */
+static void ewald_energy_ion(orc_system *s) {
+  double energy = 0;
+  double f[1][3] = {{0.0, 0.0, 0.0}};
+  double one = 1.0;
+  cplx *sk = (cplx *)malloc(sizeof(cplx) * (size_t)s->nk);
+  structure_factors(s, 1, &one, (const double(*)[3])f, sk);
+  for (int k = 0; k < s->nk; k++) energy += s->kfactor[k] * ((SQR(sk[k].re) + SQR(sk[k].im)));
+  s->uion = -(energy - s->u.C * s->alpha / sqrt(M_PI));
+  free(sk);
+}
+
+orc_system *orc_system_create(const double box[9], int bc, int n_atoms, const double *x,
+                              const double *y, const double *z, const double *charge,
+                              const double *eps_k, const double *sigma, const double *mass,
+                              double cutoff_vdw, int shifted, int charge_method,
+                              double cutoff_charge, double ewald_precision, double temperature) {
+  orc_system *s = (orc_system *)calloc(1, sizeof(orc_system));
+  double det;
+  memcpy(s->box, box, sizeof(double) * 9);
+  orc_invert3x3(s->box, s->inv, &det);
+  orc_cell_properties(s->box, s->props);
+  s->bc = bc;
+  s->n = n_atoms;
+  size_t nb = sizeof(double) * (size_t)(n_atoms > 0 ? n_atoms : 1);
+  s->x = (double *)malloc(nb);
+  s->y = (double *)malloc(nb);
+  s->z = (double *)malloc(nb);
+  s->q = (double *)malloc(nb);
+  s->eps = (double *)malloc(nb);
+  s->sig = (double *)malloc(nb);
+  s->mass_total = 0.0;
+  for (int i = 0; i < n_atoms; i++) {
+    s->x[i] = x[i];
+    s->y[i] = y[i];
+    s->z[i] = z[i];
+    s->q[i] = charge ? charge[i] : 0.0;
+    s->eps[i] = eps_k[i];
+    s->sig[i] = sigma[i];
+    s->mass_total += mass ? mass[i] : 12.0;
+  }
+  s->u = make_units();
+  s->rc_vdw2 = SQR(cutoff_vdw);
+  s->shifted = shifted;
+  s->charge_method = charge_method;
+  s->rc_charge = cutoff_charge;
+  s->rc_charge2 = SQR(cutoff_charge);
+  s->temperature = temperature;
+  s->beta = 1.0 / (s->u.KB * temperature); /* src/input.c:2071 */
+  if (charge_method == ORC_CHARGE_EWALD) {
+    ewald_init(s, ewald_precision);
+    ewald_setup_kvectors(s);
+    ewald_energy_ion(s);
+    /* framework structure factor, charged atoms only (src/ewald.c:1330-1345) */
+    int nc = 0;
+    double(*fp)[3] = (double(*)[3])malloc(sizeof(double) * 3 * (size_t)(n_atoms > 0 ? n_atoms : 1));
+    double *qc = (double *)malloc(nb);
+    s->net_charge_framework = 0.0;
+    for (int i = 0; i < n_atoms; i++) {
+      if (considered_charged(s->q[i])) {
+        double p[3] = {s->x[i], s->y[i], s->z[i]};
+        qc[nc] = s->q[i];
+        s->net_charge_framework += s->q[i];
+        frac_2pi(s, p, fp[nc]);
+        nc++;
+      }
+    }
+    s->sf = (cplx *)malloc(sizeof(cplx) * (size_t)(s->nk > 0 ? s->nk : 1));
+    structure_factors(s, nc, qc, (const double(*)[3])fp, s->sf);
+    free(fp);
+    free(qc);
+  }
+  return s;
+}
+
+void orc_system_destroy(orc_system *s) {
+  if (!s) return;
+  free(s->x); free(s->y); free(s->z); free(s->q); free(s->eps); free(s->sig);
+  for (int t = 0; t < s->ntypes; t++) free(s->pp[t]);
+  free(s->kvec); free(s->kfactor); free(s->kidx); free(s->sf);
+  free(s);
+}
+
+void orc_system_info(const orc_system *s, double out[8]) {
+  out[0] = s->alpha;
+  out[1] = s->kx;
+  out[2] = s->ky;
+  out[3] = s->kz;
+  out[4] = s->nk;
+  out[5] = s->uion;
+  out[6] = s->beta;
+  out[7] = s->props[6];
+}
+
+/* ===================================================================== */
+/* energies at a position (src/framework_energy.c:60-311)                 */
+/* ===================================================================== */
+double orc_vdw_energy_at(const orc_system *s, int type, const double pos[3], double scaling) {
+  (void)scaling; /* the LENNARD_JONES branch does not use it */
+  double UVDW = 0.0;
+  const double *pp = s->pp[type];
+  for (int i = 0; i < s->n; i++) {
+    double dr[3] = {pos[0] - s->x[i], pos[1] - s->y[i], pos[2] - s->z[i]};
+    orc_min_image(s->box, s->inv, s->bc, dr);
+    double rr = SQR(dr[0]) + SQR(dr[1]) + SQR(dr[2]);
+    if (rr < s->rc_vdw2) UVDW += lj_value(pp[3 * i], pp[3 * i + 1], pp[3 * i + 2], rr);
+  }
+  return UVDW;
+}
+
+double orc_charge_energy_at(const orc_system *s, int type, const double pos[3], double scaling) {
+  double U = 0.0;
+  if (s->charge_method == ORC_CHARGE_NONE) return 0.0;
+  if (!considered_charged(s->t_q[type])) return 0.0;
+  double chargeA = scaling * s->t_q[type];
+  for (int i = 0; i < s->n; i++) {
+    double dr[3] = {pos[0] - s->x[i], pos[1] - s->y[i], pos[2] - s->z[i]};
+    orc_min_image(s->box, s->inv, s->bc, dr);
+    double rr = SQR(dr[0]) + SQR(dr[1]) + SQR(dr[2]);
+    if (rr < s->rc_charge2) {
+      double r = sqrt(rr);
+      double chargeB = s->q[i];
+      /* PotentialValueCoulombic EWALD (src/potentials.c:12471-12472) */
+      U += s->u.C * chargeA * chargeB * (erfc(s->alpha * r) / r);
+    }
+  }
+  return U;
+}
+
+void orc_energy_at_many(const orc_system *s, int type, int64_t n, const double *xyz,
+                        double *u_vdw, double *u_cc) {
+  for (int64_t i = 0; i < n; i++) {
+    if (u_vdw) u_vdw[i] = orc_vdw_energy_at(s, type, xyz + 3 * i, 1.0);
+    if (u_cc) u_cc[i] = orc_charge_energy_at(s, type, xyz + 3 * i, 1.0);
+  }
+}
+
+/* ===================================================================== */
+/* Ewald helpers                                                          */
+/* ===================================================================== */
+int orc_ewald_nk(const orc_system *s) { return s->nk; }
+
+void orc_ewald_tables(const orc_system *s, double *kvec3, double *kfactor, double *sf_re,
+                      double *sf_im, int *kidx3) {
+  for (int k = 0; k < s->nk; k++) {
+    if (kvec3) {
+      kvec3[3 * k] = s->kvec[3 * k];
+      kvec3[3 * k + 1] = s->kvec[3 * k + 1];
+      kvec3[3 * k + 2] = s->kvec[3 * k + 2];
+    }
+    if (kidx3) {
+      kidx3[3 * k] = s->kidx[3 * k];
+      kidx3[3 * k + 1] = s->kidx[3 * k + 1];
+      kidx3[3 * k + 2] = s->kidx[3 * k + 2];
+    }
+    if (kfactor) kfactor[k] = s->kfactor[k];
+    if (sf_re) sf_re[k] = s->sf[k].re;
+    if (sf_im) sf_im[k] = s->sf[k].im;
+  }
+}
+
+double orc_ewald_fourier_self_energy_framework(const orc_system *s) {
+  double e = 0.0;
+  for (int k = 0; k < s->nk; k++) e += s->kfactor[k] * (SQR(s->sf[k].re) + SQR(s->sf[k].im));
+  return e;
+}
+
+double orc_ewald_self_energy_framework(const orc_system *s) {
+  double e = 0.0;
+  for (int i = 0; i < s->n; i++)
+    if (considered_charged(s->q[i])) e += s->u.C * SQR(s->q[i]) * s->alpha / sqrt(M_PI);
+  return e;
+}
+
+double orc_ewald_real_energy_framework(const orc_system *s) {
+  double e = 0.0;
+  for (int i = 0; i < s->n; i++)
+    for (int j = i + 1; j < s->n; j++) {
+      double dr[3] = {s->x[i] - s->x[j], s->y[i] - s->y[j], s->z[i] - s->z[j]};
+      orc_min_image(s->box, s->inv, s->bc, dr);
+      double rr = SQR(dr[0]) + SQR(dr[1]) + SQR(dr[2]);
+      if (rr < s->rc_charge2) {
+        double r = sqrt(rr);
+        e += s->u.C * s->q[i] * s->q[j] * (erfc(s->alpha * r) / r);
+      }
+    }
+  return e;
+}
+
+/* CalculateEwaldFourierAdsorbate(TRUE,FALSE,mol,0) at zero loading
+ * (src/ewald.c:6859-7565); no bond dipoles, no cations. */
+void orc_fourier_adsorbate(const orc_system *s, int n_sites, const int *types,
+                           const double *xyz, double out[5]) {
+  double charge[ORC_MAX_SITES], fpos[ORC_MAX_SITES][3], cart[ORC_MAX_SITES][3];
+  int site_atom[ORC_MAX_SITES];
+  int nc = 0;
+  double energy_self_new = 0.0, net_charge_new = 0.0;
+  double alpha = s->alpha;
+  for (int i = 0; i < 5; i++) out[i] = 0.0;
+  if (s->charge_method != ORC_CHARGE_EWALD) return;
+  for (int i = 0; i < n_sites; i++) {
+    int type = types[i];
+    if (considered_charged(s->t_q[type])) {
+      double scaling = 1.0;
+      charge[nc] = scaling * s->t_q[type];
+      energy_self_new += s->u.C * SQR(charge[nc]) * alpha / sqrt(M_PI);
+      net_charge_new += charge[nc];
+      double p[3] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+      frac_2pi(s, p, fpos[nc]);
+      cart[nc][0] = p[0];
+      cart[nc][1] = p[1];
+      cart[nc][2] = p[2];
+      site_atom[nc] = i;
+      nc++;
+    }
+  }
+  (void)site_atom;
+  cplx *sn = (cplx *)malloc(sizeof(cplx) * (size_t)(s->nk > 0 ? s->nk : 1));
+  structure_factors(s, nc, charge, (const double(*)[3])fpos, sn);
+  double energy_charge_framework_adsorbates = 0.0, energy_charge_adsorbates = 0.0;
+  for (int k = 0; k < s->nk; k++) {
+    cplx sa;
+    sa.re = 0.0 + (sn[k].re - 0.0);
+    sa.im = 0.0 + (sn[k].im - 0.0);
+    double temp = s->kfactor[k];
+    energy_charge_framework_adsorbates += temp * (s->sf[k].re * (sa.re - 0.0) + s->sf[k].im * (sa.im - 0.0));
+    energy_charge_adsorbates += temp * (SQR(sa.re) - SQR(0.0) + SQR(sa.im) - SQR(0.0));
+  }
+  free(sn);
+  /* excluded intra pairs: all charged pairs i<j (src/molecule.c:2839-2851,
+   * src/ewald.c:7318-7335) */
+  double energy_excluded_new = 0.0;
+  for (int a = 0; a < nc; a++)
+    for (int b = a + 1; b < nc; b++) {
+      double dx = cart[a][0] - cart[b][0];
+      double dy = cart[a][1] - cart[b][1];
+      double dz = cart[a][2] - cart[b][2];
+      double r = sqrt(SQR(dx) + SQR(dy) + SQR(dz));
+      double Bt0 = -erf(alpha * r) / r;
+      energy_excluded_new -= s->u.C * charge[a] * charge[b] * Bt0;
+    }
+  double net_delta = net_charge_new - 0.0;
+  double ads = energy_charge_adsorbates - (energy_excluded_new - 0.0) - (energy_self_new - 0.0);
+  ads += s->uion * SQR(0.0 + net_delta) - s->uion * SQR(0.0);
+  double host = 2.0 * energy_charge_framework_adsorbates;
+  host += 2.0 * s->uion * s->net_charge_framework * (0.0 + net_delta) -
+          2.0 * s->uion * s->net_charge_framework * (0.0);
+  out[0] = host;
+  out[1] = ads;
+  out[2] = energy_self_new;
+  out[3] = energy_excluded_new;
+  out[4] = energy_charge_adsorbates;
+}
+
+/* ===================================================================== */
+/* Widom / CBMC (src/cbmc.c:521-795,1154-1578,2762-3185,                  */
+/*               src/mc_moves.c:7844-7946, src/utils.c:688-877)           */
+/* ===================================================================== */
+typedef struct {
+  orc_uniform_fn next;
+  void *state;
+  const orc_trials *given;
+  orc_trials *record;
+  int portable_trig; /* Shoemake trig from sincos_portable (counter-stream trials) */
+} draw_t;
+
+static double draw(draw_t *d) { return d->next(d->state); }
+
+/* SelectTrialPosition (src/cbmc.c:563-595) */
+static int select_trial(const double *bf, const int *overlap, int n, double u) {
+  double largest = -DBL_MAX, sum = 0.0, shifted[64];
+  for (int i = 0; i < n; i++)
+    if (!overlap[i])
+      if (bf[i] > largest) largest = bf[i];
+  for (int i = 0; i < n; i++) {
+    if (!overlap[i]) {
+      shifted[i] = exp(bf[i] - largest);
+      sum += shifted[i];
+    } else
+      shifted[i] = 0.0;
+  }
+  int selected = 0;
+  double cumw = shifted[0];
+  double ws = u * sum;
+  while (cumw < ws) cumw += shifted[++selected];
+  return selected;
+}
+
+/* RandomNumberOnUnitSphere (src/utils.c:688-708); the vector is unused for rigid groups */
+static void unit_sphere(draw_t *d) {
+  double ran1, ran2, ransq;
+  do {
+    ran1 = 2.0 * draw(d) - 1.0;
+    ran2 = 2.0 * draw(d) - 1.0;
+    ransq = SQR(ran1) + SQR(ran2);
+  } while (ransq >= 1.0);
+}
+
+/* RandomArrayRotationMatrix (src/utils.c:835-877) */
+static void random_rotation(draw_t *d, double (*cord)[3], int n) {
+  double R[3][3];
+  double X0 = draw(d);
+  double Y1 = 2.0 * M_PI * draw(d);
+  double Y2 = 2.0 * M_PI * draw(d);
+  double R1 = sqrt(1.0 - X0);
+  double R2 = sqrt(X0);
+  double sY1, cY1, sY2, cY2;
+  if (d->portable_trig) {
+    sincos_portable(Y1, &sY1, &cY1);
+    sincos_portable(Y2, &sY2, &cY2);
+  } else {
+    sY1 = sin(Y1);
+    cY1 = cos(Y1);
+    sY2 = sin(Y2);
+    cY2 = cos(Y2);
+  }
+  double U0 = cY2 * R2;
+  double U1 = sY1 * R1;
+  double U2 = cY1 * R1;
+  double U3 = sY2 * R2;
+  double COEFI = 2.0 * U0 * U0 - 1.0;
+  double COEFUU = 2.0;
+  double COEFE = 2.0 * U0;
+  R[0][0] = COEFI + COEFUU * U1 * U1;
+  R[1][1] = COEFI + COEFUU * U2 * U2;
+  R[2][2] = COEFI + COEFUU * U3 * U3;
+  R[1][2] = COEFUU * U2 * U3 - COEFE * U1;
+  R[2][0] = COEFUU * U3 * U1 - COEFE * U2;
+  R[0][1] = COEFUU * U1 * U2 - COEFE * U3;
+  R[2][1] = COEFUU * U3 * U2 + COEFE * U1;
+  R[0][2] = COEFUU * U1 * U3 + COEFE * U2;
+  R[1][0] = COEFUU * U2 * U1 + COEFE * U3;
+  for (int i = 0; i < n; i++) {
+    double px = cord[i][0] * R[0][0] + cord[i][1] * R[0][1] + cord[i][2] * R[0][2];
+    double py = cord[i][0] * R[1][0] + cord[i][1] * R[1][1] + cord[i][2] * R[1][2];
+    double pz = cord[i][0] * R[2][0] + cord[i][1] * R[2][1] + cord[i][2] * R[2][2];
+    cord[i][0] = px;
+    cord[i][1] = py;
+    cord[i][2] = pz;
+  }
+}
+
+static void widom_core(const orc_system *s, const orc_molecule *m, draw_t *d, double rec[ORC_REC_N]) {
+  const double crit = ENERGY_OVERLAP_CRITERIA;
+  const double beta = s->beta;
+  int k_fb = m->k_fb, k_or = m->k_or;
+  int start = m->start;
+  double bf[64], evdw[64], ecc[64];
+  int ov[64];
+  double fbpos[64][3];
+  for (int i = 0; i < ORC_REC_N; i++) rec[i] = 0.0;
+
+  /* HandleFirstBead(CBMC_INSERTION) (src/cbmc.c:623-795) */
+  int OVERLAP = 1;
+  uint64_t fbmask = 0;
+  int type0 = m->type[start];
+  for (int i = 0; i < k_fb; i++) {
+    double sv[3];
+    if (d->given) {
+      fbpos[i][0] = d->given->fb[i][0];
+      fbpos[i][1] = d->given->fb[i][1];
+      fbpos[i][2] = d->given->fb[i][2];
+    } else {
+      sv[2] = draw(d); /* fall-through z, y, x (src/cbmc.c:652-667) */
+      sv[1] = draw(d);
+      sv[0] = draw(d);
+      /* ConvertFromABCtoXYZ (src/potentials.c:5548-5556) */
+      fbpos[i][0] = s->box[AX] * sv[0] + s->box[BX] * sv[1] + s->box[CX] * sv[2];
+      fbpos[i][1] = s->box[AY] * sv[0] + s->box[BY] * sv[1] + s->box[CY] * sv[2];
+      fbpos[i][2] = s->box[AZ] * sv[0] + s->box[BZ] * sv[1] + s->box[CZ] * sv[2];
+    }
+    if (d->record) {
+      d->record->fb[i][0] = fbpos[i][0];
+      d->record->fb[i][1] = fbpos[i][1];
+      d->record->fb[i][2] = fbpos[i][2];
+    }
+    double Uvdw = orc_vdw_energy_at(s, type0, fbpos[i], 1.0);
+    double Ucc = orc_charge_energy_at(s, type0, fbpos[i], 1.0);
+    if ((Uvdw >= crit) || (Ucc >= crit)) {
+      ov[i] = 1;
+      fbmask |= (1ULL << i);
+    } else {
+      ov[i] = 0;
+      OVERLAP = 0;
+      evdw[i] = Uvdw;
+      ecc[i] = Ucc;
+      bf[i] = -beta * (Uvdw + 0.0 + 0.0);
+    }
+  }
+  rec[ORC_REC_FBMASK] = (double)fbmask;
+  if (OVERLAP) {
+    rec[ORC_REC_OVERLAP] = 1.0;
+    rec[ORC_REC_STAGE] = 0.0;
+    return;
+  }
+  double W1 = 0.0;
+  for (int i = 0; i < k_fb; i++)
+    if (!ov[i]) W1 += exp(bf[i]);
+  double usel = d->given ? d->given->u_sel_fb : draw(d);
+  if (d->record) d->record->u_sel_fb = usel;
+  int sel = select_trial(bf, ov, k_fb, usel);
+  W1 /= (double)k_fb;
+  rec[ORC_REC_SELFB] = sel;
+  rec[ORC_REC_W1] = W1;
+  if (W1 < MINIMUM_ROSENBLUTH_FACTOR) {
+    rec[ORC_REC_OVERLAP] = 1.0;
+    rec[ORC_REC_STAGE] = 1.0;
+    return;
+  }
+  double RosenbluthNew = W1;
+  double UHostVDWNew = evdw[sel];
+  double UHostChargeChargeNew = ecc[sel];
+  double newpos[ORC_MAX_SITES][3];
+  newpos[start][0] = fbpos[sel][0];
+  newpos[start][1] = fbpos[sel][1];
+  newpos[start][2] = fbpos[sel][2];
+
+  if (m->n_sites > 1) {
+    /* Rosen -> GenerateTrialOrientationsSimpleSphere -> ComputeExternalEnergies */
+    double tpos[64][ORC_MAX_SITES][3];
+    double uv[64], uc[64];
+    int ovo[64];
+    uint64_t ormask = 0;
+    int n = m->n_sites;
+    for (int iu = 0; iu < k_or; iu++) {
+      double cord[ORC_MAX_SITES][3];
+      if (d->given) {
+        for (int j = 0; j < n; j++) {
+          cord[j][0] = d->given->cord[iu][j][0];
+          cord[j][1] = d->given->cord[iu][j][1];
+          cord[j][2] = d->given->cord[iu][j][2];
+        }
+      } else {
+        unit_sphere(d);
+        for (int j = 0; j < n; j++) {
+          cord[j][0] = m->body[j][0] - m->body[start][0];
+          cord[j][1] = m->body[j][1] - m->body[start][1];
+          cord[j][2] = m->body[j][2] - m->body[start][2];
+        }
+        random_rotation(d, cord, n);
+      }
+      if (d->record)
+        for (int j = 0; j < n; j++) {
+          d->record->cord[iu][j][0] = cord[j][0];
+          d->record->cord[iu][j][1] = cord[j][1];
+          d->record->cord[iu][j][2] = cord[j][2];
+        }
+      for (int j = 0; j < n; j++) {
+        tpos[iu][j][0] = newpos[start][0] + cord[j][0];
+        tpos[iu][j][1] = newpos[start][1] + cord[j][1];
+        tpos[iu][j][2] = newpos[start][2] + cord[j][2];
+      }
+    }
+    int ALLOV = 1;
+    for (int iu = 0; iu < k_or; iu++) {
+      double EV = 0.0, EC = 0.0;
+      int trial_overlap = 0;
+      for (int j = 0; j < n; j++) {
+        if (j == start) continue;
+        double Uvdw = orc_vdw_energy_at(s, m->type[j], tpos[iu][j], 1.0);
+        if (Uvdw >= crit) {
+          trial_overlap = 1;
+          break;
+        }
+        double Ucc = orc_charge_energy_at(s, m->type[j], tpos[iu][j], 1.0);
+        EV += Uvdw;
+        EC += Ucc;
+      }
+      if (!trial_overlap) {
+        uv[iu] = EV;
+        uc[iu] = EC;
+        bf[iu] = -beta * (EV + 0.0 + 0.0 + 0.0);
+        ovo[iu] = 0;
+        ALLOV = 0;
+      } else {
+        bf[iu] = 0.0;
+        ovo[iu] = 1;
+        ormask |= (1ULL << iu);
+      }
+    }
+    rec[ORC_REC_ORMASK] = (double)ormask;
+    if (ALLOV) {
+      rec[ORC_REC_OVERLAP] = 1.0;
+      rec[ORC_REC_STAGE] = 2.0;
+      return;
+    }
+    double weight = 0.0;
+    for (int i = 0; i < k_or; i++)
+      if (!ovo[i]) weight += exp(bf[i]);
+    weight = weight / (double)k_or;
+    double uso = d->given ? d->given->u_sel_or : draw(d);
+    if (d->record) d->record->u_sel_or = uso;
+    int iwalk = select_trial(bf, ovo, k_or, uso);
+    rec[ORC_REC_SELOR] = iwalk;
+    rec[ORC_REC_W2] = weight;
+    RosenbluthNew *= weight * 1.0;
+    if (RosenbluthNew < MINIMUM_ROSENBLUTH_FACTOR) {
+      rec[ORC_REC_OVERLAP] = 1.0;
+      rec[ORC_REC_STAGE] = 3.0;
+      return;
+    }
+    UHostVDWNew += uv[iwalk];
+    UHostChargeChargeNew += uc[iwalk];
+    for (int j = 0; j < n; j++)
+      if (j != start) {
+        newpos[j][0] = tpos[iwalk][j][0];
+        newpos[j][1] = tpos[iwalk][j][1];
+        newpos[j][2] = tpos[iwalk][j][2];
+      }
+  }
+  /* GrowMolecule tail (src/cbmc.c:3133-3185): anisotropic, replica and
+   * bond cross-term corrections are exactly zero here; LJ_BIASING
+   * correction with the real-space Coulomb energy. */
+  RosenbluthNew *= exp(-beta * (0.0 + 0.0 + 0.0));
+  RosenbluthNew *= exp(-beta * (0.0 + 0.0));
+  RosenbluthNew *= exp(-beta * (0.0 + 0.0));
+  RosenbluthNew *= exp(-beta * (UHostChargeChargeNew + 0.0 + 0.0));
+  /* WidomAdsorbateMove (src/mc_moves.c:7878-7933): tail correction 0 */
+  RosenbluthNew *= exp(-beta * 0.0);
+  double fo[5] = {0, 0, 0, 0, 0};
+  if (s->charge_method == ORC_CHARGE_EWALD) {
+    double xyz[3 * ORC_MAX_SITES];
+    for (int j = 0; j < m->n_sites; j++) {
+      xyz[3 * j] = newpos[j][0];
+      xyz[3 * j + 1] = newpos[j][1];
+      xyz[3 * j + 2] = newpos[j][2];
+    }
+    orc_fourier_adsorbate(s, m->n_sites, m->type, xyz, fo);
+    RosenbluthNew *= exp(-beta * (fo[0] + fo[1] + 0.0));
+  }
+  double DeltaU = UHostVDWNew + UHostChargeChargeNew + fo[0] + fo[1];
+  rec[ORC_REC_W] = RosenbluthNew;
+  rec[ORC_REC_DU] = DeltaU;
+  rec[ORC_REC_UVDW] = UHostVDWNew;
+  rec[ORC_REC_UREAL] = UHostChargeChargeNew;
+  rec[ORC_REC_UFHOST] = fo[0];
+  rec[ORC_REC_UFADS] = fo[1];
+  rec[ORC_REC_OVERLAP] = 0.0;
+  rec[ORC_REC_STAGE] = 4.0;
+}
+
+static void insertion_impl(const orc_system *s, const orc_molecule *m, orc_uniform_fn next,
+                           void *state, double rec[ORC_REC_N], orc_trials *trials_out, int portable) {
+  draw_t d;
+  memset(&d, 0, sizeof(d));
+  d.next = next;
+  d.state = state;
+  d.record = trials_out;
+  d.portable_trig = portable;
+  widom_core(s, m, &d, rec);
+}
+
+void orc_widom_insertion(const orc_system *s, const orc_molecule *m, orc_uniform_fn next,
+                         void *state, double rec[ORC_REC_N], orc_trials *trials_out) {
+  insertion_impl(s, m, next, state, rec, trials_out, 0);
+}
+
+void orc_widom_insertion_trials(const orc_system *s, const orc_molecule *m,
+                                const orc_trials *t, double rec[ORC_REC_N]) {
+  draw_t d;
+  memset(&d, 0, sizeof(d));
+  d.given = t;
+  widom_core(s, m, &d, rec);
+}
+
+typedef struct { uint64_t key; uint64_t ctr; } stream_state;
+static double stream_next(void *p) {
+  stream_state *st = (stream_state *)p;
+  return orc_stream_uniform(st->key, st->ctr++);
+}
+
+/* Counter-stream layout of one insertion: 3*k_fb fb draws, one fb
+ * selection draw, then the orientation draws, then one selection draw.
+ * The fb selection draw index is fixed, so orientation draws start at
+ * 3*k_fb+1 whatever the first-bead outcome. */
+void orc_widom_run_stream(const orc_system *s, const orc_molecule *m, uint64_t seed, int64_t first,
+                          int64_t n, double *rec) {
+  for (int64_t i = 0; i < n; i++) {
+    stream_state st = {orc_stream_key(seed, (uint64_t)(first + i)), 0};
+    insertion_impl(s, m, stream_next, &st, rec + ORC_REC_N * i, NULL, 1);
+  }
+}
+
+void orc_stream_trials(const orc_system *s, const orc_molecule *m, uint64_t seed, int64_t insertion,
+                       double *fb, double *cord, double *u_sel) {
+  /* replay the draw layout without energies */
+  stream_state st = {orc_stream_key(seed, (uint64_t)insertion), 0};
+  draw_t d;
+  memset(&d, 0, sizeof(d));
+  d.next = stream_next;
+  d.state = &st;
+  d.portable_trig = 1;
+  for (int i = 0; i < m->k_fb; i++) {
+    double sv[3];
+    sv[2] = draw(&d);
+    sv[1] = draw(&d);
+    sv[0] = draw(&d);
+    fb[3 * i + 0] = s->box[AX] * sv[0] + s->box[BX] * sv[1] + s->box[CX] * sv[2];
+    fb[3 * i + 1] = s->box[AY] * sv[0] + s->box[BY] * sv[1] + s->box[CY] * sv[2];
+    fb[3 * i + 2] = s->box[AZ] * sv[0] + s->box[BZ] * sv[1] + s->box[CZ] * sv[2];
+  }
+  u_sel[0] = draw(&d);
+  u_sel[1] = 0.0;
+  if (m->n_sites > 1) {
+    int n = m->n_sites;
+    for (int iu = 0; iu < m->k_or; iu++) {
+      double c[ORC_MAX_SITES][3];
+      unit_sphere(&d);
+      for (int j = 0; j < n; j++) {
+        c[j][0] = m->body[j][0] - m->body[m->start][0];
+        c[j][1] = m->body[j][1] - m->body[m->start][1];
+        c[j][2] = m->body[j][2] - m->body[m->start][2];
+      }
+      random_rotation(&d, c, n);
+      for (int j = 0; j < n; j++) {
+        cord[(iu * n + j) * 3 + 0] = c[j][0];
+        cord[(iu * n + j) * 3 + 1] = c[j][1];
+        cord[(iu * n + j) * 3 + 2] = c[j][2];
+      }
+    }
+    u_sel[1] = draw(&d);
+  }
+}
+
+/* block index per cycle (src/statistics.c:898-910, 1126-1127) */
+void orc_block_of_cycles(int64_t n_cycles, int32_t *block) {
+  long long BlockCycle[NR_BLOCKS];
+  for (int i = 0; i < NR_BLOCKS; i++) BlockCycle[i] = (long long)((i + 1) * n_cycles / (double)NR_BLOCKS);
+  int Block = 0;
+  for (int64_t c = 0; c < n_cycles; c++) {
+    if (Block < NR_BLOCKS && c == BlockCycle[Block]) Block++;
+    block[c] = Block < NR_BLOCKS ? Block : NR_BLOCKS - 1;
+  }
+}
+
+typedef struct { orc_mt64 g; } mt_state;
+static double mt_next(void *p) { return orc_mt64_uniform(&((mt_state *)p)->g); }
+
+int64_t orc_widom_run_mt(const orc_system *s, const orc_molecule *m, uint64_t seed, int64_t n_cycles,
+                         int steps_per_cycle, double *blocks) {
+  mt_state st;
+  orc_mt64_seed(&st.g, seed);
+  int32_t *bmap = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n_cycles > 0 ? n_cycles : 1));
+  orc_block_of_cycles(n_cycles, bmap);
+  for (int i = 0; i < 4 * NR_BLOCKS; i++) blocks[i] = 0.0;
+  int64_t count = 0;
+  double volume = s->props[6];
+  for (int64_t c = 0; c < n_cycles; c++) {
+    int b = bmap[c];
+    (void)orc_mt64_uniform(&st.g); /* SelectedSystem (src/monte_carlo.c:626) */
+    for (int j = 0; j < steps_per_cycle; j++) {
+      (void)orc_mt64_uniform(&st.g);       /* CurrentComponent (src/monte_carlo.c:643) */
+      double ran = orc_mt64_uniform(&st.g); /* move selection (src/monte_carlo.c:649) */
+      if (ran < 1.0) {
+        double rec[ORC_REC_N];
+        blocks[4 * b + 0] += 1.0;
+        blocks[4 * b + 3] += 1.0 * (volume / ((double)0 + 1.0));
+        orc_widom_insertion(s, m, mt_next, &st, rec, NULL);
+        if (rec[ORC_REC_OVERLAP] == 0.0) {
+          blocks[4 * b + 1] += 1.0 * rec[ORC_REC_W];
+          blocks[4 * b + 2] += 1.0 * (0.0 + rec[ORC_REC_DU]) * rec[ORC_REC_W];
+        }
+        count++;
+      }
+    }
+  }
+  free(bmap);
+  return count;
+}
+
+double orc_framework_density(double mass_g_mol, double volume_A3) {
+  return 1e-3 * mass_g_mol / (volume_A3 * CUBE(ANGSTROM_) * AVOGADRO_CONSTANT_);
+}
+
+/* src/statistics.c:2539-2560, 2708-2722, 4747-4800 */
+void orc_widom_estimators(const double *blocks, double T, double rho, double igw, double out[16]) {
+  units_t u = make_units();
+  double sum = 0.0, sum2 = 0.0;
+  for (int i = 0; i < NR_BLOCKS; i++) {
+    double tmp = 0.0;
+    if (blocks[4 * i] > 0.0) {
+      tmp = (1.0 / (MOLAR_GAS_CONSTANT_ * T * rho)) * blocks[4 * i + 1] / (igw * blocks[4 * i]);
+      sum += tmp;
+      sum2 += SQR(tmp);
+    }
+    out[i] = tmp;
+  }
+  out[5] = sum / (double)NR_BLOCKS;
+  out[6] = 2.0 * sqrt(fabs((sum2 / (double)NR_BLOCKS) - SQR(sum) / (double)SQR(NR_BLOCKS)));
+  sum = sum2 = 0.0;
+  for (int i = 0; i < NR_BLOCKS; i++) {
+    double tmp = 0.0;
+    if (blocks[4 * i] > 0.0) {
+      tmp = (blocks[4 * i + 2] / blocks[4 * i + 1] - 0.0 / blocks[4 * i]) * u.E2K;
+      sum += tmp;
+      sum2 += SQR(tmp);
+    }
+    out[7 + i] = tmp;
+  }
+  out[12] = sum / (double)NR_BLOCKS;
+  out[13] = 2.0 * sqrt(fabs((sum2 / (double)NR_BLOCKS) - SQR(sum) / (double)SQR(NR_BLOCKS)));
+  double c = 0.0, sw = 0.0;
+  for (int i = 0; i < NR_BLOCKS; i++)
+    if (blocks[4 * i] > 0.0) {
+      sw += blocks[4 * i + 1] / igw;
+      c += blocks[4 * i];
+    }
+  out[14] = c > 0.0 ? (1.0 / (MOLAR_GAS_CONSTANT_ * T * rho)) * (sw / c) : 0.0;
+  out[15] = c > 0.0 ? sw * igw / c : 0.0;
+}
+
+/* ===================================================================== */
+/* grids (src/grids.c:431-742, 979-1077; src/framework_hessian.c:63-109;  */
+/*        src/potentials.c:10978-10994)                                   */
+/* ===================================================================== */
+void orc_grid_geometry(const double ub[9], double spacing, double out[12]) {
+  double size[3] = {0.0, 0.0, 0.0}, shift[3] = {0.0, 0.0, 0.0};
+  const int col[3][3] = {{AX, AY, AZ}, {BX, BY, BZ}, {CX, CY, CZ}};
+  for (int v = 0; v < 3; v++)
+    for (int d = 0; d < 3; d++) {
+      size[d] += fabs(ub[col[v][d]]);
+      if (ub[col[v][d]] < 0.0) shift[d] += ub[col[v][d]];
+    }
+  int N[3];
+  for (int d = 0; d < 3; d++) {
+    N[d] = (int)(size[d] / spacing);
+    if (N[d] % 2 == 0) N[d]++;
+  }
+  for (int d = 0; d < 3; d++) {
+    out[d] = (double)N[d];
+    out[3 + d] = size[d];
+    out[6 + d] = shift[d];
+    out[9 + d] = size[d] / (double)N[d];
+  }
+}
+
+void orc_derivatives_vdw_at(const orc_system *s, int type, const double pos[3], double out[14]) {
+  double value = 0, fx = 0, fy = 0, fz = 0;
+  double h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double third = 0.0;
+  const double *pp = s->pp[type];
+  for (int i = 0; i < s->n; i++) {
+    double dr[3] = {pos[0] - s->x[i], pos[1] - s->y[i], pos[2] - s->z[i]};
+    orc_min_image(s->box, s->inv, s->bc, dr);
+    double rr = SQR(dr[0]) + SQR(dr[1]) + SQR(dr[2]);
+    if (rr < s->rc_vdw2) {
+      double arg1 = pp[3 * i], arg2 = SQR(pp[3 * i + 1]), arg3 = pp[3 * i + 2];
+      double rri3 = CUBE(arg2 / rr);
+      double F = 4.0 * arg1 * (rri3 * (rri3 - 1.0)) - arg3;
+      double DF = 24.0 * arg1 * (rri3 * (1.0 - 2.0 * rri3)) / rr;
+      double DDF = 96.0 * arg1 * (rri3 * (7.0 * rri3 - 2.0)) / SQR(rr);
+      double DDDF = 384.0 * arg1 * (rri3 * (5.0 - 28.0 * rri3)) / SQR(SQR(rr));
+      double dx = dr[0], dy = dr[1], dz = dr[2];
+      value += F;
+      fx += dx * DF;
+      fy += dy * DF;
+      fz += dz * DF;
+      h[AX] += DDF * dx * dx + DF; h[BX] += DDF * dy * dx;      h[CX] += DDF * dz * dx;
+      h[AY] += DDF * dx * dy;      h[BY] += DDF * dy * dy + DF; h[CY] += DDF * dz * dy;
+      h[AZ] += DDF * dx * dz;      h[BZ] += DDF * dy * dz;      h[CZ] += DDF * dz * dz + DF;
+      third += DDDF * dx * dy * dz;
+    }
+  }
+  out[0] = value;
+  out[1] = fx;
+  out[2] = fy;
+  out[3] = fz;
+  for (int i = 0; i < 9; i++) out[4 + i] = h[i];
+  out[13] = third;
+}
+
+void orc_grid_node(const orc_system *s, int type, const double g[12], int i, int j, int k, float out[8]) {
+  double pos[3];
+  double d[14];
+  pos[0] = i * g[3] / (int)g[0] + g[6];
+  pos[1] = j * g[4] / (int)g[1] + g[7];
+  pos[2] = k * g[5] / (int)g[2] + g[8];
+  orc_derivatives_vdw_at(s, type, pos, d);
+  double value = d[0];
+  double f[3] = {d[1], d[2], d[3]};
+  double hay = d[4 + AY], haz = d[4 + AZ], hbz = d[4 + BZ];
+  double third = d[13];
+  const double crit = ENERGY_OVERLAP_CRITERIA;
+  if (value > crit) {
+    value = 2.0 * crit;
+    for (int a = 0; a < 3; a++) {
+      if (f[a] > crit) f[a] = crit;
+      if (f[a] < -crit) f[a] = -crit;
+    }
+    hay = haz = hbz = 0.0;
+    third = 0.0;
+  }
+  double dx = g[9], dy = g[10], dz = g[11];
+  out[0] = (float)value;
+  out[1] = (float)((double)(float)f[0] * dx);
+  out[2] = (float)((double)(float)f[1] * dy);
+  out[3] = (float)((double)(float)f[2] * dz);
+  out[4] = (float)((double)(float)hay * (dx * dy));
+  out[5] = (float)((double)(float)haz * (dx * dz));
+  out[6] = (float)((double)(float)hbz * (dy * dz));
+  out[7] = (float)((double)(float)third * (dx * dy * dz));
+}
+
+/* Lekien & Marsden (2005): the 64x64 matrix mapping corner values and
+ * derivatives (f, fx, fy, fz, fxy, fxz, fyz, fxyz at the 8 corners, row
+ * corner+8*derivative) to the coefficients a[i+4j+16k] of x^i y^j z^k.
+ * Generated here as the inverse of the constraint matrix. */
+void orc_lm_coeff(int out[64 * 64]) {
+  static const int dd[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
+                               {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+  double B[64][128];
+  for (int row = 0; row < 64; row++) {
+    int c = row % 8, d = row / 8;
+    int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
+    for (int col = 0; col < 64; col++) {
+      int p[3] = {col % 4, (col / 4) % 4, col / 16};
+      int cc[3] = {cx, cy, cz};
+      double v = 1.0;
+      for (int a = 0; a < 3; a++) {
+        int pw = p[a], der = dd[d][a], at = cc[a];
+        double t;
+        if (der == 0)
+          t = (pw == 0) ? 1.0 : (at ? 1.0 : 0.0);
+        else
+          t = (pw == 0) ? 0.0 : (pw == 1 ? 1.0 : (at ? (double)pw : 0.0));
+        v *= t;
+      }
+      B[row][col] = v;
+      B[row][64 + col] = (row == col) ? 1.0 : 0.0;
+    }
+  }
+  for (int col = 0; col < 64; col++) {
+    int piv = col;
+    for (int r = col + 1; r < 64; r++)
+      if (fabs(B[r][col]) > fabs(B[piv][col])) piv = r;
+    if (piv != col)
+      for (int c = 0; c < 128; c++) {
+        double t = B[col][c];
+        B[col][c] = B[piv][c];
+        B[piv][c] = t;
+      }
+    double p = B[col][col];
+    for (int c = 0; c < 128; c++) B[col][c] /= p;
+    for (int r = 0; r < 64; r++)
+      if (r != col && B[r][col] != 0.0) {
+        double f = B[r][col];
+        for (int c = 0; c < 128; c++) B[r][c] -= f * B[col][c];
+      }
+  }
+  for (int r = 0; r < 64; r++)
+    for (int c = 0; c < 64; c++) out[r * 64 + c] = (int)lround(B[r][64 + c]);
+}
+
+double orc_interpolate_grid(const float *grid, const double g[12], int bc, const double ub[9],
+                            const double uinv[9], const double pos_in[3]) {
+  static int Coeff[64 * 64];
+  static int have = 0;
+  if (!have) {
+    orc_lm_coeff(Coeff);
+    have = 1;
+  }
+  double pos[3] = {pos_in[0], pos_in[1], pos_in[2]};
+  double sv[3];
+  int N[3] = {(int)g[0], (int)g[1], (int)g[2]};
+  if (bc == ORC_BC_RECTANGULAR) {
+    /* UnitCellSize = lengths of the unit-cell vectors */
+    double L[3] = {ub[AX], ub[BY], ub[CZ]};
+    for (int d = 0; d < 3; d++) {
+      pos[d] -= L[d] * (double)(nint(pos[d] / L[d]));
+      if (pos[d] < 0.0) pos[d] += L[d];
+    }
+  } else {
+    sv[0] = uinv[AX] * pos[0] + uinv[BX] * pos[1] + uinv[CX] * pos[2];
+    sv[1] = uinv[AY] * pos[0] + uinv[BY] * pos[1] + uinv[CY] * pos[2];
+    sv[2] = uinv[AZ] * pos[0] + uinv[BZ] * pos[1] + uinv[CZ] * pos[2];
+    for (int d = 0; d < 3; d++) {
+      sv[d] -= (double)nint(sv[d]);
+      if (sv[d] < 0.0) sv[d] += 1.0;
+    }
+    pos[0] = ub[AX] * sv[0] + ub[BX] * sv[1] + ub[CX] * sv[2];
+    pos[1] = ub[AY] * sv[0] + ub[BY] * sv[1] + ub[CY] * sv[2];
+    pos[2] = ub[AZ] * sv[0] + ub[BZ] * sv[1] + ub[CZ] * sv[2];
+  }
+  double s3[3];
+  for (int d = 0; d < 3; d++) s3[d] = ((pos[d] - g[6 + d]) / g[3 + d]) * (double)(N[d]);
+  int x0 = (int)s3[0], y0 = (int)s3[1], z0 = (int)s3[2];
+  int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
+  double r[3] = {s3[0] - x0, s3[1] - y0, s3[2] - z0};
+  int ny = N[1] + 1, nz = N[2] + 1;
+#define NODE(a, b, c) (grid + 8 * (((size_t)(a) * ny + (b)) * nz + (c)))
+  double X[64], a[64];
+  for (int i = 0; i < 8; i++) {
+    X[0 + i * 8] = NODE(x0, y0, z0)[i];
+    X[1 + i * 8] = NODE(x1, y0, z0)[i];
+    X[2 + i * 8] = NODE(x0, y1, z0)[i];
+    X[3 + i * 8] = NODE(x1, y1, z0)[i];
+    X[4 + i * 8] = NODE(x0, y0, z1)[i];
+    X[5 + i * 8] = NODE(x1, y0, z1)[i];
+    X[6 + i * 8] = NODE(x0, y1, z1)[i];
+    X[7 + i * 8] = NODE(x1, y1, z1)[i];
+  }
+#undef NODE
+  for (int i = 0; i < 8; i++)
+    if (X[i] > 1e10) return 1e8;
+  for (int i = 0; i < 64; i++) {
+    a[i] = (double)(0.0);
+    for (int j = 0; j < 64; j++) a[i] += Coeff[i * 64 + j] * X[j];
+  }
+  double value = 0.0;
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++)
+      for (int k = 0; k < 4; k++) value += a[i + 4 * j + 16 * k] * pow(r[0], i) * pow(r[1], j) * pow(r[2], k);
+  return value;
+}

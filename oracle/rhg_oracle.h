@@ -1,0 +1,188 @@
+/*
+ * rhg_oracle.h -- CPU FP64 restatement of RASPA2's host-guest energy path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is linked or
+ * loaded by tests/, by __graft_entry__.smoke() and by bench.py's
+ * cpu_baseline leg, and by nothing else.  The product path (raspa2_amd/)
+ * never calls it.
+ *
+ * Every function restates the reference algorithm in plain C with strict
+ * IEEE double arithmetic (built with -ffp-contract=off, no fast-math) and the
+ * reference's operation order, so that values are bit-identical to what the
+ * reference computes on the same inputs wherever the reference's arithmetic
+ * is itself deterministic (libm calls aside).  Each function cites the
+ * reference file:line it follows.
+ */
+#ifndef RHG_ORACLE_H
+#define RHG_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_BC_RECTANGULAR 1
+#define ORC_BC_TRICLINIC   2
+
+#define ORC_CHARGE_NONE  0
+#define ORC_CHARGE_EWALD 1
+
+#define ORC_MAX_TYPES 64
+#define ORC_MAX_SITES 8
+
+/* ---------------------------------------------------------------- units */
+/* out[0] energy unit [J], out[1] Coulomb conversion [internal*A/e^2],
+ * out[2] energy->K, out[3] K->energy, out[4] k_B [internal/K],
+ * out[5] Coulomb potential unit [K]  (src/constants.c:141-180) */
+void orc_units(double out[6]);
+
+/* ------------------------------------------------------------------ RNG */
+/* MT19937-64 (src/utils.c:308-392) */
+typedef struct { uint64_t mt[312]; int mti; } orc_mt64;
+void     orc_mt64_seed(orc_mt64 *g, uint64_t seed);
+uint64_t orc_mt64_next(orc_mt64 *g);
+double   orc_mt64_uniform(orc_mt64 *g);             /* RandomNumber(): closed [0,1] */
+void     orc_mt64_sequence(uint64_t seed, int64_t n, uint64_t *out);
+void     orc_mt64_uniform_sequence(uint64_t seed, int64_t n, double *out);
+
+/* Counter-based per-insertion stream (SplitMix64 output function applied
+ * to key + (k+1)*golden), mapped to [0,1] with RandomNumber()'s formula. */
+uint64_t orc_stream_key(uint64_t seed, uint64_t insertion);
+double   orc_stream_uniform(uint64_t key, uint64_t counter);
+void     orc_stream_uniforms(uint64_t seed, uint64_t insertion, int64_t n, double *out);
+void     orc_sincos_portable(double x, double *s, double *c);
+
+/* ----------------------------------------------------------------- cell */
+/* box/inv use RASPA's REAL_MATRIX3x3 field order: ax ay az bx by bz cx cy cz
+ * (columns are the cell vectors a, b, c).  props: lengths[3], perpendicular
+ * widths[3], volume. */
+void orc_cell_make(double a, double b, double c, double alpha_deg, double beta_deg,
+                   double gamma_deg, double box[9], double inv[9], double props[7]);
+void orc_invert3x3(const double in[9], double out[9], double *det);
+void orc_cell_properties(const double box[9], double props[7]);
+void orc_min_image(const double box[9], const double inv[9], int bc, double dr[3]);
+
+/* --------------------------------------------------------------- system */
+typedef struct orc_system orc_system;
+
+orc_system *orc_system_create(const double box[9], int bc, int n_atoms,
+                              const double *x, const double *y, const double *z,
+                              const double *charge, const double *eps_k, const double *sigma,
+                              const double *mass,
+                              double cutoff_vdw, int shifted,
+                              int charge_method, double cutoff_charge, double ewald_precision,
+                              double temperature);
+void orc_system_destroy(orc_system *s);
+/* registers a probe pseudo-atom; returns its type id */
+int  orc_add_pseudo_atom(orc_system *s, double eps_k, double sigma, double charge);
+/* pair parameters of (type, atom): eps [internal], sigma [A], shift [internal] */
+void orc_pair_params(const orc_system *s, int type, int atom, double out[3]);
+void orc_system_info(const orc_system *s, double out[8]);  /* alpha,kx,ky,kz,nk,uion,beta,volume */
+
+/* CalculateFrameworkVDWEnergyAtPosition, plain all-pairs branch */
+double orc_vdw_energy_at(const orc_system *s, int type, const double pos[3], double scaling);
+/* CalculateFrameworkChargeEnergyAtPosition, plain branch (returns Ucc) */
+double orc_charge_energy_at(const orc_system *s, int type, const double pos[3], double scaling);
+void   orc_energy_at_many(const orc_system *s, int type, int64_t n, const double *xyz,
+                          double *u_vdw, double *u_cc);
+
+/* ---------------------------------------------------------------- Ewald */
+int    orc_ewald_nk(const orc_system *s);
+void   orc_ewald_tables(const orc_system *s, double *kvec3, double *kfactor,
+                        double *sf_re, double *sf_im, int *kidx3);
+double orc_ewald_fourier_self_energy_framework(const orc_system *s); /* sum kf |S_f|^2 */
+double orc_ewald_real_energy_framework(const orc_system *s);         /* sum_{i<j} erfc */
+double orc_ewald_self_energy_framework(const orc_system *s);
+/* CalculateEwaldFourierAdsorbate(TRUE,FALSE,...) at zero loading for one molecule:
+ * out[0] UHostAdsorbateChargeChargeFourierDelta, out[1] UAdsorbateAdsorbateChargeChargeFourierDelta,
+ * out[2] energy_self_new, out[3] energy_excluded_new, out[4] sum_k kf |S_new|^2 */
+void   orc_fourier_adsorbate(const orc_system *s, int n_sites, const int *types,
+                             const double *xyz, double out[5]);
+
+/* ---------------------------------------------------------------- Widom */
+typedef struct {
+  int n_sites;
+  int type[ORC_MAX_SITES];
+  double body[ORC_MAX_SITES][3];
+  int start;          /* StartingBead */
+  int k_fb, k_or;     /* NumberOfTrialPositionsForTheFirstBeadWidom / NumberOfTrialPositionsWidom */
+} orc_molecule;
+
+/* per-insertion result record (doubles), see ORC_REC_* */
+#define ORC_REC_W        0   /* RosenbluthNew incl. Fourier factor (0 on overlap) */
+#define ORC_REC_DU       1   /* DeltaU of WidomAdsorbateMove (0 on overlap) */
+#define ORC_REC_UVDW     2
+#define ORC_REC_UREAL    3
+#define ORC_REC_UFHOST   4
+#define ORC_REC_UFADS    5
+#define ORC_REC_OVERLAP  6   /* 1 if the insertion was rejected (OVERLAP) */
+#define ORC_REC_SELFB    7
+#define ORC_REC_SELOR    8
+#define ORC_REC_FBMASK   9   /* bitmask of overlapping first-bead trials */
+#define ORC_REC_ORMASK  10   /* bitmask of overlapping orientation trials */
+#define ORC_REC_W1      11
+#define ORC_REC_W2      12
+#define ORC_REC_STAGE   13   /* 0: fb all overlap, 1: fb W<min, 2: or all overlap, 3: or W<min, 4: ok, 5: blocked */
+#define ORC_REC_N       14
+
+/* trial record: what the insertion drew, usable as host-supplied trials */
+typedef struct {
+  double fb[64][3];
+  double cord[64][ORC_MAX_SITES][3];
+  double u_sel_fb, u_sel_or;
+} orc_trials;
+
+typedef double (*orc_uniform_fn)(void *state);
+
+/* One WidomAdsorbateMove (src/mc_moves.c:7844) using uniforms from `next`. */
+void orc_widom_insertion(const orc_system *s, const orc_molecule *m, orc_uniform_fn next,
+                         void *state, double rec[ORC_REC_N], orc_trials *trials_out);
+/* Same, with all draws taken from explicitly supplied trials. */
+void orc_widom_insertion_trials(const orc_system *s, const orc_molecule *m,
+                                const orc_trials *t, double rec[ORC_REC_N]);
+
+/* Counter-stream Widom over insertions [first, first+n): rec is n x ORC_REC_N */
+void orc_widom_run_stream(const orc_system *s, const orc_molecule *m, uint64_t seed,
+                          int64_t first, int64_t n, double *rec);
+/* Trials of one insertion of the counter stream (for host-supplied GPU runs) */
+void orc_stream_trials(const orc_system *s, const orc_molecule *m, uint64_t seed,
+                       int64_t insertion, double *fb /*k_fb*3*/, double *cord /*k_or*n_sites*3*/,
+                       double *u_sel /*2*/);
+
+/* RASPA's own MC driver for a Widom-only single-component run, with the
+ * MT19937-64 stream in RASPA's global draw order (src/monte_carlo.c:569-760).
+ * blocks: 5 x 4 (count, sumW, sumW*dU, idealgas) ; returns #insertions */
+int64_t orc_widom_run_mt(const orc_system *s, const orc_molecule *m, uint64_t seed,
+                         int64_t n_cycles, int steps_per_cycle, double *blocks);
+
+/* Block index of each cycle (src/statistics.c:904,1126) */
+void orc_block_of_cycles(int64_t n_cycles, int32_t *block);
+
+/* estimators from 5-block accumulators (count,sumW,sumWdU,idealgas):
+ * out[0..4] K_H per block, out[5] avg, out[6] 2sigma,
+ * out[7..11] <U_gh>_1-<U_h>_0 per block [K], out[12] avg, out[13] 2sigma,
+ * out[14] GetAverageHenryCoefficient, out[15] <W> overall */
+void orc_widom_estimators(const double *blocks, double temperature, double density_kg_m3,
+                          double ideal_gas_weight, double out[16]);
+double orc_framework_density(double mass_g_mol, double volume_A3);
+
+/* ----------------------------------------------------------------- grid */
+/* MakeGrid geometry from a unit-cell box: out: N[3] (as doubles), size[3], shift[3], delta[3] */
+void orc_grid_geometry(const double unit_box[9], double spacing, double out[12]);
+/* CalculateDerivativesAtPositionVDW: out[0] U, out[1..3] dU, out[4..12] H (ax,ay,az,bx,by,bz,cx,cy,cz), out[13] third */
+void orc_derivatives_vdw_at(const orc_system *s, int type, const double pos[3], double out[14]);
+/* one stored node of MakeGrid (capped, scaled, FP32) */
+void orc_grid_node(const orc_system *s, int type, const double geom[12], int i, int j, int k,
+                   float out[8]);
+/* Lekien-Marsden 64x64 matrix generated from its definition */
+void orc_lm_coeff(int out[64 * 64]);
+/* InterpolateVDWGrid with an explicit full grid in [i][j][k][8] float layout */
+double orc_interpolate_grid(const float *grid, const double geom[12], int bc,
+                            const double unit_box[9], const double unit_inv[9],
+                            const double pos[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

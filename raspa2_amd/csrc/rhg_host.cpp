@@ -1,0 +1,1268 @@
+// rhg_host.cpp -- host runtime and C ABI of librhg (see include/rhg.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/rhg.h"
+#include "rhg_kernels.h"
+
+using namespace rhg;
+
+namespace {
+
+// src/constants.h:48-60 and src/constants.c:141-180
+constexpr double kAngstrom = 1e-10, kPicoSecond = 1e-12, kAmu = 1.6605402e-27,
+                 kElectron = 1.60217733e-19, kMolarGas = 8.314464919,
+                 kBoltzmann = 1.380650324e-23, kAvogadro = 6.0221419947e23,
+                 kEpsilon0 = 8.8541878176e-12;
+
+struct Units {
+  double E, C, E2K, K2E, KB;
+};
+Units make_units() {
+  volatile double length = kAngstrom, time = kPicoSecond, mass = kAmu, charge = kElectron,
+                  dielectric = 1.0;
+  Units u;
+  u.E = ((mass) * (length * length) / (time * time));
+  u.KB = kBoltzmann / u.E;
+  u.C = (((charge * charge) / (4.0 * M_PI * kEpsilon0 * length * u.E))) / (dielectric);
+  u.E2K = ((u.E * kAvogadro) / kMolarGas);
+  u.K2E = (kMolarGas / (u.E * kAvogadro));
+  return u;
+}
+
+inline int nint(double x) { return (int)((x) >= 0.0 ? ((x) + 0.5) : ((x)-0.5)); }
+
+// Invert3x3Matrix (src/matrix.c:902-927)
+void invert3x3(const double *in, double *out) {
+  out[AX] = in[BY] * in[CZ] - in[BZ] * in[CY];
+  out[AY] = in[AZ] * in[CY] - in[AY] * in[CZ];
+  out[AZ] = in[AY] * in[BZ] - in[AZ] * in[BY];
+  out[BX] = in[BZ] * in[CX] - in[BX] * in[CZ];
+  out[BY] = in[AX] * in[CZ] - in[AZ] * in[CX];
+  out[BZ] = in[AZ] * in[BX] - in[AX] * in[BZ];
+  out[CX] = in[BX] * in[CY] - in[BY] * in[CX];
+  out[CY] = in[AY] * in[CX] - in[AX] * in[CY];
+  out[CZ] = in[AX] * in[BY] - in[AY] * in[BX];
+  double d = in[AX] * out[AX] + in[BX] * out[AY] + in[CX] * out[AZ];
+  double r = 0.0;
+  if (fabs(d) > 0.0) r = 1.0 / d;
+  for (int i = 0; i < 9; ++i) out[i] *= r;
+}
+
+// CellProperties (src/framework.c:6363-6391): lengths, perpendicular widths, volume
+void cell_properties(const double *in, double *p) {
+  p[0] = sqrt(in[AX] * in[AX] + in[AY] * in[AY] + in[AZ] * in[AZ]);
+  p[1] = sqrt(in[BX] * in[BX] + in[BY] * in[BY] + in[BZ] * in[BZ]);
+  p[2] = sqrt(in[CX] * in[CX] + in[CY] * in[CY] + in[CZ] * in[CZ]);
+  double axb1 = in[AY] * in[BZ] - in[AZ] * in[BY];
+  double axb2 = in[AZ] * in[BX] - in[AX] * in[BZ];
+  double axb3 = in[AX] * in[BY] - in[AY] * in[BX];
+  double bxc1 = in[BY] * in[CZ] - in[BZ] * in[CY];
+  double bxc2 = in[BZ] * in[CX] - in[BX] * in[CZ];
+  double bxc3 = in[BX] * in[CY] - in[BY] * in[CX];
+  double cxa1 = in[CY] * in[AZ] - in[AY] * in[CZ];
+  double cxa2 = in[AX] * in[CZ] - in[AZ] * in[CX];
+  double cxa3 = in[AY] * in[CX] - in[AX] * in[CY];
+  double vol = fabs(in[AX] * bxc1 + in[AY] * bxc2 + in[AZ] * bxc3);
+  p[3] = vol / sqrt(bxc1 * bxc1 + bxc2 * bxc2 + bxc3 * bxc3);
+  p[4] = vol / sqrt(cxa1 * cxa1 + cxa2 * cxa2 + cxa3 * cxa3);
+  p[5] = vol / sqrt(axb1 * axb1 + axb2 * axb2 + axb3 * axb3);
+  p[6] = vol;
+}
+
+Cell make_cell(const double *box, int bc) {
+  Cell c;
+  memcpy(c.box, box, sizeof(double) * 9);
+  invert3x3(c.box, c.inv);
+  c.bc = bc;
+  c.upper = (box[AY] == 0.0 && box[AZ] == 0.0 && box[BZ] == 0.0 && c.inv[AY] == 0.0 &&
+             c.inv[AZ] == 0.0 && c.inv[BZ] == 0.0)
+                ? 1
+                : 0;
+  return c;
+}
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+struct PseudoAtom {
+  double eps_k, sigma, q;
+  AtomRec64 *d64 = nullptr;
+  AtomRecMx *dmx = nullptr;
+  double *c24 = nullptr, *c96 = nullptr, *c384 = nullptr;
+};
+
+struct Pending {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+} // namespace
+
+struct rhg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  Cell cell;
+  double props[7];
+  int n = 0;
+  std::vector<double> x, y, z, q, eps, sig;
+  double mass_total = 0.0;
+  double rc_vdw = 12.0, rc2_vdw = 144.0;
+  int shifted = 1;
+  int charge_method = RHG_CHARGE_NONE;
+  double rc_cc = 12.0, rc2_cc = 144.0;
+  double temperature = 300.0;
+  Units u;
+  double beta = 0.0;
+  int precision = RHG_PREC_FP64;
+  std::vector<PseudoAtom> types;
+  // Ewald
+  double alpha = 0.0, uion = 0.0, net_q_fw = 0.0, recip_cutoff = 0.0;
+  int kx = 0, ky = 0, kz = 0, nk = 0;
+  std::vector<double> kfactor;
+  std::vector<int32_t> kidx, row_off, row_kz;
+  double *d_kfactor = nullptr;
+  double2 *d_sf = nullptr;
+  int32_t *d_row_off = nullptr, *d_row_kz = nullptr;
+  size_t footprint = 0;
+  std::map<std::string, DevBuf> bufs;
+  bool prof = false;
+  std::map<std::string, std::pair<int64_t, double>> stats;
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> event_pool;
+};
+
+struct rhg_grid {
+  rhg_ctx *ctx = nullptr;
+  int n[3];
+  double size[3], shift[3], delta[3];
+  double ubox[9], uinv[9], ulen[3];
+  int bc;
+  int fp64;
+  void *d = nullptr;
+  size_t bytes = 0;
+};
+
+namespace {
+
+int fail(rhg_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPC(ctx, call)                                                                   \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail((ctx), RHG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int dalloc(rhg_ctx *c, void **p, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return fail(c, RHG_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  c->footprint += bytes;
+  return RHG_OK;
+}
+
+// grow-only named work buffer
+void *work(rhg_ctx *c, const char *name, size_t bytes) {
+  DevBuf &b = c->bufs[name];
+  if (b.bytes >= bytes && b.p) return b.p;
+  if (b.p) hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+  size_t want = std::max<size_t>(bytes, 256);
+  if (hipMalloc(&b.p, want) != hipSuccess) {
+    b.p = nullptr;
+    return nullptr;
+  }
+  b.bytes = want;
+  return b.p;
+}
+
+hipEvent_t get_event(rhg_ctx *c) {
+  if (!c->event_pool.empty()) {
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  hipEventCreate(&e);
+  return e;
+}
+
+// brackets the launches of one named kernel with events when profiling
+struct Timed {
+  rhg_ctx *c;
+  const char *name;
+  hipEvent_t a = nullptr;
+  Timed(rhg_ctx *ctx, const char *nm) : c(ctx), name(nm) {
+    if (c->prof) {
+      a = get_event(c);
+      hipEventRecord(a, c->stream);
+    }
+  }
+  ~Timed() {
+    if (c->prof) {
+      hipEvent_t b = get_event(c);
+      hipEventRecord(b, c->stream);
+      c->pending.push_back({name, a, b});
+    }
+  }
+};
+
+void drain_profile(rhg_ctx *c) {
+  for (auto &p : c->pending) {
+    float ms = 0.0f;
+    hipEventSynchronize(p.b);
+    hipEventElapsedTime(&ms, p.a, p.b);
+    auto &s = c->stats[p.name];
+    s.first += 1;
+    s.second += ms;
+    c->event_pool.push_back(p.a);
+    c->event_pool.push_back(p.b);
+  }
+  c->pending.clear();
+}
+
+bool considered_charged(double q) { return fabs(q) > 1e-10; }
+
+// LENNARD_JONES PotentialValue at r^2 (src/potentials.c:5867-5877)
+double lj_ref(double arg1, double sigma, double arg3, double rr) {
+  double arg2 = sigma * sigma;
+  double t = arg2 / rr;
+  double rri3 = (t * t) * t;
+  return 4.0 * arg1 * (rri3 * (rri3 - 1.0)) - arg3;
+}
+
+// InitializeEwald (automatic), SetupKVectors, EwaldEnergyIon
+// (src/ewald.c:200-260, 532-600, 615-742)
+void ewald_setup(rhg_ctx *c, double precision) {
+  double eps = std::min(fabs(precision), 0.5);
+  double rc = c->rc_cc;
+  double tol = sqrt(fabs(log(eps * rc)));
+  c->alpha = sqrt(fabs(log(eps * rc * tol))) / rc;
+  double tol1 = sqrt(-log(eps * rc * ((2.0 * tol * c->alpha) * (2.0 * tol * c->alpha))));
+  c->kx = nint((double)0.25 + c->props[0] * c->alpha * tol1 / M_PI);
+  c->ky = nint((double)0.25 + c->props[1] * c->alpha * tol1 / M_PI);
+  c->kz = nint((double)0.25 + c->props[2] * c->alpha * tol1 / M_PI);
+  int m = std::max(c->kx, std::max(c->ky, c->kz));
+  c->recip_cutoff = (1.05 * m) * (1.05 * m);
+  const double *inv = c->cell.inv;
+  double volume = c->props[6];
+  double alpha = c->alpha;
+  c->kfactor.clear();
+  c->kidx.clear();
+  int ny = 2 * c->ky + 1;
+  c->row_off.assign((size_t)(c->kx + 1) * ny, 0);
+  c->row_kz.assign((size_t)(c->kx + 1) * ny, -1);
+  int nvec = 0;
+  for (int ii = 0; ii <= c->kx; ii++) {
+    double rk2x = 2.0 * M_PI * ii * inv[AX];
+    double rk2y = 2.0 * M_PI * ii * inv[BX];
+    double rk2z = 2.0 * M_PI * ii * inv[CX];
+    for (int jj = -c->ky; jj <= c->ky; jj++) {
+      double rk1x = rk2x + 2.0 * M_PI * jj * inv[AY];
+      double rk1y = rk2y + 2.0 * M_PI * jj * inv[BY];
+      double rk1z = rk2z + 2.0 * M_PI * jj * inv[CY];
+      int row = ii * ny + (jj + c->ky);
+      c->row_off[row] = nvec;
+      int kzm = -1;
+      for (int kk = -c->kz; kk <= c->kz; kk++) {
+        double rkx = rk1x + 2.0 * M_PI * kk * inv[AZ];
+        double rky = rk1y + 2.0 * M_PI * kk * inv[BZ];
+        double rkz = rk1z + 2.0 * M_PI * kk * inv[CZ];
+        double ksqr = ii * ii + jj * jj + kk * kk;
+        if ((ksqr != 0) && (ksqr < c->recip_cutoff)) {
+          double rksqr = rkx * rkx + rky * rky + rkz * rkz;
+          double f;
+          if (ii == 0)
+            f = c->u.C * (2.0 * M_PI / volume) * exp((-0.25 / (alpha * alpha)) * rksqr) / rksqr;
+          else
+            f = c->u.C * (4.0 * M_PI / volume) * exp((-0.25 / (alpha * alpha)) * rksqr) / rksqr;
+          c->kfactor.push_back(f);
+          c->kidx.push_back(ii);
+          c->kidx.push_back(jj);
+          c->kidx.push_back(kk);
+          kzm = std::max(kzm, std::abs(kk));
+          nvec++;
+        }
+      }
+      c->row_kz[row] = kzm;
+    }
+  }
+  c->nk = nvec;
+  double energy = 0;
+  for (int k = 0; k < nvec; k++) energy += c->kfactor[k] * ((1.0 * 1.0 + 0.0 * 0.0));
+  c->uion = -(energy - c->u.C * c->alpha / sqrt(M_PI));
+}
+
+int upload(rhg_ctx *c, void *dst, const void *src, size_t bytes) {
+  HIPC(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  return RHG_OK;
+}
+
+int setup_pseudo_atom(rhg_ctx *c, PseudoAtom &t) {
+  std::vector<AtomRec64> h64((size_t)std::max(c->n, 1));
+  std::vector<AtomRecMx> hmx((size_t)std::max(c->n, 1));
+  std::vector<double> h24((size_t)std::max(c->n, 1)), h96(h24.size()), h384(h24.size());
+  double ei = t.eps_k * c->u.K2E; // src/potentials.c:268-274
+  for (int i = 0; i < c->n; i++) {
+    double ej = c->eps[i] * c->u.K2E;
+    double e = sqrt(ei * ej);              // src/potentials.c:3164-3171
+    double sg = 0.5 * (t.sigma + c->sig[i]);
+    double shift = c->shifted ? lj_ref(e, sg, 0.0, c->rc2_vdw) : 0.0; // src/potentials.c:5263-5285
+    AtomRec64 &r = h64[i];
+    r.x = c->x[i];
+    r.y = c->y[i];
+    r.z = c->z[i];
+    r.c4 = 4.0 * e;
+    r.s2 = sg * sg;
+    r.shift = shift;
+    r.q = c->q[i];
+    r.pad = 0.0;
+    AtomRecMx &m = hmx[i];
+    m.x = r.x;
+    m.y = r.y;
+    m.z = r.z;
+    m.c4 = (float)r.c4;
+    m.s2 = (float)r.s2;
+    m.shift = (float)r.shift;
+    m.q = (float)r.q;
+    h24[i] = 24.0 * e;
+    h96[i] = 96.0 * e;
+    h384[i] = 384.0 * e;
+  }
+  size_t n = h64.size();
+  int rc;
+  if ((rc = dalloc(c, (void **)&t.d64, n * sizeof(AtomRec64)))) return rc;
+  if ((rc = dalloc(c, (void **)&t.dmx, n * sizeof(AtomRecMx)))) return rc;
+  if ((rc = dalloc(c, (void **)&t.c24, n * sizeof(double)))) return rc;
+  if ((rc = dalloc(c, (void **)&t.c96, n * sizeof(double)))) return rc;
+  if ((rc = dalloc(c, (void **)&t.c384, n * sizeof(double)))) return rc;
+  if ((rc = upload(c, t.d64, h64.data(), n * sizeof(AtomRec64)))) return rc;
+  if ((rc = upload(c, t.dmx, hmx.data(), n * sizeof(AtomRecMx)))) return rc;
+  if ((rc = upload(c, t.c24, h24.data(), n * sizeof(double)))) return rc;
+  if ((rc = upload(c, t.c96, h96.data(), n * sizeof(double)))) return rc;
+  if ((rc = upload(c, t.c384, h384.data(), n * sizeof(double)))) return rc;
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return RHG_OK;
+}
+
+EnergyArgs energy_args(rhg_ctx *c, int type, double scaling, int do_vdw, int do_cc) {
+  EnergyArgs a;
+  a.cell = c->cell;
+  a.rc2_vdw = c->rc2_vdw;
+  a.rc2_cc = c->rc2_cc;
+  a.alpha = c->alpha;
+  const PseudoAtom &t = c->types[type];
+  double chargeA = scaling * t.q;
+  a.cqa = c->u.C * chargeA;
+  a.n_atoms = c->n;
+  a.a64 = t.d64;
+  a.amx = t.dmx;
+  a.precision = c->precision;
+  a.do_vdw = do_vdw;
+  a.do_cc = (do_cc && c->charge_method == RHG_CHARGE_EWALD && considered_charged(t.q) && c->n > 0) ? 1 : 0;
+  return a;
+}
+
+__global__ void k_fill(double *p, int64_t n, double v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+void fill(double *p, int64_t n, double v, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+}
+
+EwaldDev ewald_dev(rhg_ctx *c) {
+  EwaldDev e;
+  e.kx = c->kx;
+  e.ky = c->ky;
+  e.kz = c->kz;
+  e.nk = c->nk;
+  e.alpha = c->alpha;
+  e.uion = c->uion;
+  e.net_q_fw = c->net_q_fw;
+  e.beta_unused = 0.0;
+  e.kfactor = c->d_kfactor;
+  e.sf = c->d_sf;
+  e.row_off = c->d_row_off;
+  e.row_kz = c->d_row_kz;
+  e.cell = c->cell;
+  e.C = c->u.C;
+  return e;
+}
+
+int check_type(rhg_ctx *c, int32_t type) {
+  if (type < 0 || type >= (int)c->types.size()) return fail(c, RHG_ERR_INVALID, "unknown pseudo-atom type");
+  return RHG_OK;
+}
+
+} // namespace
+
+// ======================================================================
+// C ABI
+// ======================================================================
+extern "C" {
+
+const char *rhg_version(void) { return "rhg 0.1 (gfx950)"; }
+
+const char *rhg_last_error(const rhg_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int rhg_create(const rhg_framework *fw, const rhg_cell *cellp, const rhg_forcefield *ff, int device,
+               rhg_ctx **out) {
+  if (!fw || !cellp || !ff || !out) return RHG_ERR_INVALID;
+  *out = nullptr;
+  if (fw->n_atoms < 0 || (fw->n_atoms > 0 && (!fw->x || !fw->y || !fw->z || !fw->eps_k || !fw->sigma)))
+    return RHG_ERR_INVALID;
+  if (cellp->boundary != RHG_BC_RECTANGULAR && cellp->boundary != RHG_BC_TRICLINIC) return RHG_ERR_INVALID;
+  if (ff->cutoff_vdw <= 0.0 || ff->temperature <= 0.0) return RHG_ERR_INVALID;
+  if (ff->charge_method != RHG_CHARGE_NONE && ff->charge_method != RHG_CHARGE_EWALD) return RHG_ERR_UNSUPPORTED;
+  std::unique_ptr<rhg_ctx> c(new rhg_ctx());
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess) return RHG_ERR_HIP;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return RHG_ERR_HIP;
+  c->u = make_units();
+  c->cell = make_cell(cellp->box, cellp->boundary);
+  cell_properties(c->cell.box, c->props);
+  c->n = fw->n_atoms;
+  c->x.assign(fw->x, fw->x + c->n);
+  c->y.assign(fw->y, fw->y + c->n);
+  c->z.assign(fw->z, fw->z + c->n);
+  c->eps.assign(fw->eps_k, fw->eps_k + c->n);
+  c->sig.assign(fw->sigma, fw->sigma + c->n);
+  c->q.assign((size_t)c->n, 0.0);
+  if (fw->charge)
+    for (int i = 0; i < c->n; ++i) c->q[i] = fw->charge[i];
+  c->mass_total = 0.0;
+  for (int i = 0; i < c->n; ++i) c->mass_total += fw->mass ? fw->mass[i] : 12.0;
+  c->rc_vdw = ff->cutoff_vdw;
+  c->rc2_vdw = ff->cutoff_vdw * ff->cutoff_vdw;
+  c->shifted = ff->shift_vdw ? 1 : 0;
+  c->charge_method = ff->charge_method;
+  c->rc_cc = ff->cutoff_charge > 0.0 ? ff->cutoff_charge : ff->cutoff_vdw;
+  c->rc2_cc = c->rc_cc * c->rc_cc;
+  c->temperature = ff->temperature;
+  c->beta = 1.0 / (c->u.KB * c->temperature); // src/input.c:2071
+  if (c->charge_method == RHG_CHARGE_EWALD) {
+    ewald_setup(c.get(), ff->ewald_precision > 0.0 ? ff->ewald_precision : 1e-6);
+    int rc;
+    size_t nk = (size_t)std::max(c->nk, 1);
+    if ((rc = dalloc(c.get(), (void **)&c->d_kfactor, nk * sizeof(double)))) return rc;
+    if ((rc = dalloc(c.get(), (void **)&c->d_sf, nk * sizeof(double2)))) return rc;
+    if ((rc = dalloc(c.get(), (void **)&c->d_row_off, c->row_off.size() * sizeof(int32_t)))) return rc;
+    if ((rc = dalloc(c.get(), (void **)&c->d_row_kz, c->row_kz.size() * sizeof(int32_t)))) return rc;
+    upload(c.get(), c->d_kfactor, c->kfactor.data(), c->kfactor.size() * sizeof(double));
+    upload(c.get(), c->d_row_off, c->row_off.data(), c->row_off.size() * sizeof(int32_t));
+    upload(c.get(), c->d_row_kz, c->row_kz.data(), c->row_kz.size() * sizeof(int32_t));
+    // framework structure factor of the charged atoms (src/ewald.c:1330-1345)
+    std::vector<double> fx, fy, fz, qq;
+    c->net_q_fw = 0.0;
+    const double *inv = c->cell.inv;
+    for (int i = 0; i < c->n; ++i) {
+      if (!considered_charged(c->q[i])) continue;
+      double p[3] = {c->x[i], c->y[i], c->z[i]};
+      double a = inv[AX] * p[0] + inv[BX] * p[1] + inv[CX] * p[2];
+      double b = inv[AY] * p[0] + inv[BY] * p[1] + inv[CY] * p[2];
+      double d = inv[AZ] * p[0] + inv[BZ] * p[1] + inv[CZ] * p[2];
+      fx.push_back(a * (2.0 * M_PI));
+      fy.push_back(b * (2.0 * M_PI));
+      fz.push_back(d * (2.0 * M_PI));
+      qq.push_back(c->q[i]);
+      c->net_q_fw += c->q[i];
+    }
+    int nc = (int)qq.size();
+    size_t bn = (size_t)std::max(nc, 1) * sizeof(double);
+    double *d_fx = (double *)work(c.get(), "sf_fx", bn), *d_fy = (double *)work(c.get(), "sf_fy", bn),
+           *d_fz = (double *)work(c.get(), "sf_fz", bn), *d_q = (double *)work(c.get(), "sf_q", bn);
+    int32_t *d_kidx = (int32_t *)work(c.get(), "sf_kidx", nk * 3 * sizeof(int32_t));
+    if (!d_fx || !d_fy || !d_fz || !d_q || !d_kidx) return RHG_ERR_OOM;
+    if (nc) {
+      upload(c.get(), d_fx, fx.data(), nc * sizeof(double));
+      upload(c.get(), d_fy, fy.data(), nc * sizeof(double));
+      upload(c.get(), d_fz, fz.data(), nc * sizeof(double));
+      upload(c.get(), d_q, qq.data(), nc * sizeof(double));
+    }
+    upload(c.get(), d_kidx, c->kidx.data(), c->kidx.size() * sizeof(int32_t));
+    {
+      Timed t(c.get(), "sf");
+      launch_structure_factor(ewald_dev(c.get()), d_kidx, c->nk, d_fx, d_fy, d_fz, d_q, nc, c->d_sf,
+                              c->stream);
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return RHG_ERR_HIP;
+  }
+  *out = c.release();
+  return RHG_OK;
+}
+
+void rhg_destroy(rhg_ctx *c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  for (auto &t : c->types) {
+    hipFree(t.d64);
+    hipFree(t.dmx);
+    hipFree(t.c24);
+    hipFree(t.c96);
+    hipFree(t.c384);
+  }
+  for (auto &b : c->bufs) hipFree(b.second.p);
+  hipFree(c->d_kfactor);
+  hipFree(c->d_sf);
+  hipFree(c->d_row_off);
+  hipFree(c->d_row_kz);
+  for (auto &p : c->pending) {
+    hipEventDestroy(p.a);
+    hipEventDestroy(p.b);
+  }
+  for (auto e : c->event_pool) hipEventDestroy(e);
+  hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int rhg_add_pseudo_atom(rhg_ctx *c, double eps_k, double sigma, double charge, int32_t *type_out) {
+  if (!c || !type_out) return RHG_ERR_INVALID;
+  if (!(eps_k >= 0.0) || !(sigma > 0.0)) return fail(c, RHG_ERR_INVALID, "bad LJ parameters");
+  hipSetDevice(c->device);
+  PseudoAtom t;
+  t.eps_k = eps_k;
+  t.sigma = sigma;
+  t.q = charge;
+  int rc = setup_pseudo_atom(c, t);
+  if (rc) return rc;
+  c->types.push_back(t);
+  *type_out = (int32_t)c->types.size() - 1;
+  return RHG_OK;
+}
+
+int rhg_set_precision(rhg_ctx *c, int32_t p) {
+  if (!c) return RHG_ERR_INVALID;
+  if (p != RHG_PREC_FP64 && p != RHG_PREC_MIXED) return fail(c, RHG_ERR_INVALID, "bad precision");
+  c->precision = p;
+  return RHG_OK;
+}
+
+int rhg_footprint(const rhg_ctx *c, size_t *bytes) {
+  if (!c || !bytes) return RHG_ERR_INVALID;
+  size_t b = c->footprint;
+  for (auto &kv : c->bufs) b += kv.second.bytes;
+  *bytes = b;
+  return RHG_OK;
+}
+
+int rhg_info(const rhg_ctx *c, double out[10]) {
+  if (!c || !out) return RHG_ERR_INVALID;
+  out[0] = c->alpha;
+  out[1] = c->kx;
+  out[2] = c->ky;
+  out[3] = c->kz;
+  out[4] = c->nk;
+  out[5] = c->uion;
+  out[6] = c->beta;
+  out[7] = c->props[6];
+  out[8] = c->mass_total;
+  // src/statistics.c:4751
+  out[9] = 1e-3 * c->mass_total / (c->props[6] * (kAngstrom * kAngstrom * kAngstrom) * kAvogadro);
+  return RHG_OK;
+}
+
+int rhg_framework_energy_at_positions(rhg_ctx *c, int64_t n, const double *xyz, int32_t type,
+                                      double *u_vdw, double *u_cc, uint8_t *overlap) {
+  if (!c || n < 0 || (n > 0 && !xyz)) return RHG_ERR_INVALID;
+  if (int rc = check_type(c, type)) return rc;
+  if (n == 0) return RHG_OK;
+  hipSetDevice(c->device);
+  std::vector<double> sx(n), sy(n), sz(n);
+  for (int64_t i = 0; i < n; ++i) {
+    sx[i] = xyz[3 * i];
+    sy[i] = xyz[3 * i + 1];
+    sz[i] = xyz[3 * i + 2];
+  }
+  size_t b = n * sizeof(double);
+  double *dx = (double *)work(c, "e_x", b), *dy = (double *)work(c, "e_y", b), *dz = (double *)work(c, "e_z", b);
+  double *dv = (double *)work(c, "e_v", b), *dc = (double *)work(c, "e_c", b);
+  uint8_t *dov = (uint8_t *)work(c, "e_o", n);
+  if (!dx || !dy || !dz || !dv || !dc || !dov) return fail(c, RHG_ERR_OOM, "work buffers");
+  upload(c, dx, sx.data(), b);
+  upload(c, dy, sy.data(), b);
+  upload(c, dz, sz.data(), b);
+  EnergyArgs a = energy_args(c, type, 1.0, 1, 1);
+  if (!a.do_cc) fill(dc, n, 0.0, c->stream);
+  SiteList s{dx, dy, dz, n, nullptr, 1};
+  EnergyOut o{dv, a.do_cc ? dc : nullptr, dov};
+  {
+    Timed t(c, "site_energy");
+    launch_site_energy(a, s, o, c->stream);
+  }
+  if (!a.do_cc) {
+    // overlap from the VDW term only; the charge term is zero
+  }
+  HIPC(c, hipGetLastError());
+  if (u_vdw) HIPC(c, hipMemcpyAsync(u_vdw, dv, b, hipMemcpyDeviceToHost, c->stream));
+  if (u_cc) HIPC(c, hipMemcpyAsync(u_cc, dc, b, hipMemcpyDeviceToHost, c->stream));
+  if (overlap) HIPC(c, hipMemcpyAsync(overlap, dov, n, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return RHG_OK;
+}
+
+int rhg_framework_vdw_energy_at_position(rhg_ctx *c, const double pos[3], int32_t type, double scaling,
+                                         double *u) {
+  (void)scaling; // not used by the LENNARD_JONES branch
+  if (!c || !pos || !u) return RHG_ERR_INVALID;
+  return rhg_framework_energy_at_positions(c, 1, pos, type, u, nullptr, nullptr);
+}
+
+int rhg_framework_charge_energy_at_position(rhg_ctx *c, const double pos[3], int32_t type,
+                                            double scaling, double *ucc, double *ucbd) {
+  if (!c || !pos || !ucc) return RHG_ERR_INVALID;
+  if (int rc = check_type(c, type)) return rc;
+  if (ucbd) *ucbd = 0.0;
+  *ucc = 0.0;
+  EnergyArgs a = energy_args(c, type, scaling, 0, 1);
+  if (!a.do_cc) return RHG_OK;
+  hipSetDevice(c->device);
+  double *d = (double *)work(c, "e_x", 4 * sizeof(double));
+  double *dv = (double *)work(c, "e_c", sizeof(double));
+  if (!d || !dv) return RHG_ERR_OOM;
+  upload(c, d, pos, 3 * sizeof(double));
+  SiteList s{d, d + 1, d + 2, 1, nullptr, 1};
+  EnergyOut o{nullptr, dv, nullptr};
+  launch_site_energy(a, s, o, c->stream);
+  HIPC(c, hipGetLastError());
+  HIPC(c, hipMemcpyAsync(ucc, dv, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return RHG_OK;
+}
+
+int rhg_ewald_tables(rhg_ctx *c, double *kfactor, double *sf_re, double *sf_im) {
+  if (!c) return RHG_ERR_INVALID;
+  if (c->charge_method != RHG_CHARGE_EWALD) return fail(c, RHG_ERR_UNSUPPORTED, "no Ewald");
+  hipSetDevice(c->device);
+  if (kfactor) memcpy(kfactor, c->kfactor.data(), c->nk * sizeof(double));
+  if (sf_re || sf_im) {
+    std::vector<double2> h(c->nk);
+    HIPC(c, hipMemcpy(h.data(), c->d_sf, c->nk * sizeof(double2), hipMemcpyDeviceToHost));
+    for (int k = 0; k < c->nk; ++k) {
+      if (sf_re) sf_re[k] = h[k].x;
+      if (sf_im) sf_im[k] = h[k].y;
+    }
+  }
+  return RHG_OK;
+}
+
+} // extern "C"
+
+namespace {
+
+int build_mol(rhg_ctx *c, const rhg_molecule *m, MolDev &d) {
+  if (!m || m->n_sites < 1 || m->n_sites > RHG_MAX_SITES) return fail(c, RHG_ERR_INVALID, "bad molecule");
+  if (m->start < 0 || m->start >= m->n_sites) return fail(c, RHG_ERR_INVALID, "bad starting bead");
+  if (m->k_fb < 1 || m->k_fb > kMaxTrials) return fail(c, RHG_ERR_INVALID, "k_fb out of range [1,64]");
+  if (m->n_sites > 1 && (m->k_or < 1 || m->k_or > kMaxTrials))
+    return fail(c, RHG_ERR_INVALID, "k_or out of range [1,64]");
+  memset(&d, 0, sizeof(d));
+  d.n_sites = m->n_sites;
+  d.start = m->start;
+  d.k_fb = m->k_fb;
+  d.k_or = m->n_sites > 1 ? m->k_or : 1;
+  d.n_place = 0;
+  for (int j = 0; j < m->n_sites; ++j) {
+    if (m->type[j] < 0 || m->type[j] >= (int)c->types.size()) return fail(c, RHG_ERR_INVALID, "unknown site type");
+    for (int q = 0; q < 3; ++q) d.body[j][q] = m->body[j][q];
+    d.q[j] = c->types[m->type[j]].q;
+    d.charged[j] = considered_charged(d.q[j]) ? 1 : 0;
+    if (j != m->start) d.place[d.n_place++] = j;
+  }
+  return RHG_OK;
+}
+
+struct WidomBufs {
+  WidomWork w;
+  int64_t *d_bounds;
+  double *d_part, *d_out10;
+};
+
+int widom_buffers(rhg_ctx *c, const MolDev &m, int64_t cap, WidomBufs &B) {
+  WidomWork &w = B.w;
+  memset(&w, 0, sizeof(w));
+  w.cap = cap;
+  size_t nfb = (size_t)cap * m.k_fb;
+  size_t nor = (size_t)cap * m.k_or * std::max(m.n_place, 1);
+  size_t ncf = (size_t)cap * m.n_sites;
+  size_t d8 = sizeof(double);
+#define WB(field, name, bytes)                           \
+  w.field = (decltype(w.field))work(c, name, (bytes)); \
+  if (!w.field) return fail(c, RHG_ERR_OOM, "widom work " name);
+  WB(fbx, "w_fbx", nfb * d8);
+  WB(fby, "w_fby", nfb * d8);
+  WB(fbz, "w_fbz", nfb * d8);
+  WB(fb_uv, "w_fbuv", nfb * d8);
+  WB(fb_uc, "w_fbuc", nfb * d8);
+  WB(stage, "w_stage", cap * 4);
+  WB(selfb, "w_selfb", cap * 4);
+  WB(selor, "w_selor", cap * 4);
+  WB(fbmask, "w_fbmask", cap * 8);
+  WB(ormask, "w_ormask", cap * 8);
+  WB(W1, "w_W1", cap * d8);
+  WB(W2, "w_W2", cap * d8);
+  WB(W, "w_W", cap * d8);
+  WB(dU, "w_dU", cap * d8);
+  WB(uvdw, "w_uvdw", cap * d8);
+  WB(ureal, "w_ureal", cap * d8);
+  WB(ufh, "w_ufh", cap * d8);
+  WB(ufa, "w_ufa", cap * d8);
+  WB(usel_or, "w_uselor", cap * d8);
+  WB(selx, "w_selx", cap * d8);
+  WB(sely, "w_sely", cap * d8);
+  WB(selz, "w_selz", cap * d8);
+  WB(active, "w_active", cap * 4);
+  WB(n_active, "w_nactive", 64);
+  WB(orx, "w_orx", nor * d8);
+  WB(ory, "w_ory", nor * d8);
+  WB(orz, "w_orz", nor * d8);
+  WB(or_uv, "w_oruv", nor * d8);
+  WB(active2, "w_active2", cap * 4);
+  WB(n_active2, "w_nactive2", 64);
+  WB(cfx, "w_cfx", ncf * d8);
+  WB(cfy, "w_cfy", ncf * d8);
+  WB(cfz, "w_cfz", ncf * d8);
+  WB(cc_place, "w_ccplace", (size_t)cap * std::max(m.n_place, 1) * d8);
+  WB(f_host, "w_fhost", cap * d8);
+  WB(f_ads, "w_fads", cap * d8);
+#undef WB
+  B.d_bounds = (int64_t *)work(c, "w_bounds", 8 * sizeof(int64_t));
+  const int seg = 4096;
+  B.d_part = (double *)work(c, "w_part", ((cap + seg - 1) / seg + 1) * 10 * d8);
+  B.d_out10 = (double *)work(c, "w_out10", 16 * d8);
+  if (!B.d_bounds || !B.d_part || !B.d_out10) return fail(c, RHG_ERR_OOM, "widom reduce buffers");
+  return RHG_OK;
+}
+
+// One chunk of the Widom pipeline (all launches on the context stream).
+int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs &B,
+                const WidomArgs &a) {
+  WidomWork &w = B.w;
+  const int64_t n = a.n;
+  hipStream_t st = c->stream;
+  launch_zero_counters(w.n_active, w.n_active2, st);
+  {
+    Timed t(c, "widom_gen");
+    launch_widom_fb_gen(a, w, st);
+  }
+  // HandleFirstBead energies: VDW and real-space Coulomb of every trial
+  // (src/cbmc.c:680-681)
+  const int type0 = mol->type[m.start];
+  {
+    EnergyArgs ea = energy_args(c, type0, 1.0, 1, 1);
+    if (!ea.do_cc) fill(w.fb_uc, n * m.k_fb, 0.0, st);
+    SiteList s{w.fbx, w.fby, w.fbz, n * m.k_fb, nullptr, 1};
+    EnergyOut o{w.fb_uv, ea.do_cc ? w.fb_uc : nullptr, nullptr};
+    Timed t(c, "site_energy");
+    launch_site_energy(ea, s, o, st);
+  }
+  {
+    Timed t(c, "widom_select");
+    launch_widom_fb_select(a, w, st);
+  }
+  if (m.n_sites > 1) {
+    // ComputeExternalEnergies VDW of every orientation trial (src/cbmc.c:1210)
+    const int64_t cap_or = w.cap * (int64_t)m.k_or;
+    for (int p = 0; p < m.n_place; ++p) {
+      EnergyArgs ea = energy_args(c, mol->type[m.place[p]], 1.0, 1, 0);
+      SiteList s{w.orx + p * cap_or, w.ory + p * cap_or, w.orz + p * cap_or, n * m.k_or,
+                 w.n_active, m.k_or};
+      EnergyOut o{w.or_uv + p * cap_or, nullptr, nullptr};
+      Timed t(c, "site_energy");
+      launch_site_energy(ea, s, o, st);
+    }
+    {
+      Timed t(c, "widom_select");
+      launch_widom_or_select(a, w, st);
+    }
+    // real-space Coulomb of the placed beads of the selected orientation
+    // (src/cbmc.c:1216; only the selected trial's value is used)
+    bool any_cc = false;
+    for (int p = 0; p < m.n_place; ++p) {
+      EnergyArgs ea = energy_args(c, mol->type[m.place[p]], 1.0, 0, 1);
+      double *dst = w.cc_place + p * w.cap;
+      if (!ea.do_cc) {
+        fill(dst, n, 0.0, st);
+        continue;
+      }
+      any_cc = true;
+      const int j = m.place[p];
+      SiteList s{w.cfx + j * w.cap, w.cfy + j * w.cap, w.cfz + j * w.cap, n, w.n_active2, 1};
+      EnergyOut o{nullptr, dst, nullptr};
+      Timed t(c, "site_energy");
+      launch_site_energy(ea, s, o, st);
+    }
+    (void)any_cc;
+  }
+  if (a.ewald) {
+    FourierArgs f;
+    memset(&f, 0, sizeof(f));
+    f.e = ewald_dev(c);
+    f.n_sites = m.n_sites;
+    f.nc = 0;
+    f.self_energy = 0.0;
+    for (int j = 0; j < m.n_sites; ++j)
+      if (m.charged[j]) {
+        f.cidx[f.nc] = j;
+        f.q[f.nc] = 1.0 * m.q[j];
+        // src/ewald.c:6973
+        f.self_energy += c->u.C * (f.q[f.nc] * f.q[f.nc]) * c->alpha / sqrt(M_PI);
+        f.nc++;
+      }
+    f.px = w.cfx;
+    f.py = w.cfy;
+    f.pz = w.cfz;
+    f.stride = w.cap;
+    f.n = n;
+    f.n_dev = w.n_active2;
+    f.d_host = w.f_host;
+    f.d_ads = w.f_ads;
+    if (f.nc > 0) {
+      Timed t(c, "fourier");
+      launch_fourier(f, st);
+    } else {
+      fill(w.f_host, n, 0.0, st);
+      fill(w.f_ads, n, 0.0, st);
+    }
+  }
+  {
+    Timed t(c, "widom_select");
+    launch_widom_finalize(a, w, st);
+  }
+  if (hipGetLastError() != hipSuccess) return fail(c, RHG_ERR_HIP, "widom launch failed");
+  return RHG_OK;
+}
+
+// block boundaries (insertion index) of a job (src/statistics.c:898-910,1126)
+void block_bounds(int64_t n_total, int spc, int64_t bounds[6]) {
+  int64_t n_cycles = (n_total + spc - 1) / spc;
+  long long BlockCycle[RHG_NR_BLOCKS];
+  for (int i = 0; i < RHG_NR_BLOCKS; i++)
+    BlockCycle[i] = (long long)((i + 1) * n_cycles / (double)RHG_NR_BLOCKS);
+  // first cycle of each block
+  int64_t first_cycle[RHG_NR_BLOCKS + 1];
+  for (int b = 0; b <= RHG_NR_BLOCKS; ++b) first_cycle[b] = n_cycles;
+  first_cycle[0] = 0;
+  int Block = 0;
+  for (int64_t cyc = 0; cyc < n_cycles; cyc++) {
+    if (Block < RHG_NR_BLOCKS && cyc == BlockCycle[Block]) {
+      Block++;
+      if (Block < RHG_NR_BLOCKS) first_cycle[Block] = cyc;
+    }
+  }
+  // blocks that are never entered start at the end
+  for (int b = 1; b < RHG_NR_BLOCKS; ++b)
+    if (first_cycle[b] < first_cycle[b - 1]) first_cycle[b] = first_cycle[b - 1];
+  for (int b = 0; b < RHG_NR_BLOCKS; ++b) bounds[b] = std::min(first_cycle[b] * spc, n_total);
+  bounds[RHG_NR_BLOCKS] = n_total;
+}
+
+} // namespace
+
+extern "C" {
+
+int rhg_widom_run(rhg_ctx *c, const rhg_molecule *mol, uint64_t seed, int64_t n_total, int64_t first,
+                  int64_t n, int32_t steps_per_cycle, rhg_widom_blocks *blocks, double *rec) {
+  if (!c || !mol) return RHG_ERR_INVALID;
+  if (n_total < 0 || first < 0 || n < 0 || first + n > n_total || steps_per_cycle < 1)
+    return fail(c, RHG_ERR_INVALID, "bad insertion range");
+  MolDev m;
+  if (int rc = build_mol(c, mol, m)) return rc;
+  if (n == 0) return RHG_OK;
+  hipSetDevice(c->device);
+  const int64_t chunk = std::min<int64_t>(n, (int64_t)1 << 20);
+  WidomBufs B;
+  if (int rc = widom_buffers(c, m, chunk, B)) return rc;
+  int64_t jb[6];
+  block_bounds(n_total, steps_per_cycle, jb);
+  WidomArgs a;
+  memset(&a, 0, sizeof(a));
+  a.mol = m;
+  a.cell = c->cell;
+  a.beta = c->beta;
+  a.seed = seed;
+  a.use_host_trials = 0;
+  a.ewald = (c->charge_method == RHG_CHARGE_EWALD) ? 1 : 0;
+  const int seg = 4096;
+  for (int64_t off = 0; off < n; off += chunk) {
+    a.first = first + off;
+    a.n = std::min(chunk, n - off);
+    if (int rc = widom_chunk(c, mol, m, B, a)) return rc;
+    if (rec) {
+      double *drec = (double *)work(c, "w_rec", (size_t)a.n * RHG_REC_N * sizeof(double));
+      if (!drec) return fail(c, RHG_ERR_OOM, "rec buffer");
+      launch_widom_records(a, B.w, drec, c->stream);
+      HIPC(c, hipMemcpyAsync(rec + off * RHG_REC_N, drec, (size_t)a.n * RHG_REC_N * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    }
+    if (blocks) {
+      int64_t lb[8];
+      for (int b = 0; b < 6; ++b) lb[b] = std::min(std::max(jb[b] - a.first, (int64_t)0), a.n);
+      HIPC(c, hipMemcpyAsync(B.d_bounds, lb, 6 * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+      {
+        Timed t(c, "block_reduce");
+        launch_block_reduce(B.w.W, B.w.dU, a.n, B.d_bounds, B.d_part, seg, B.d_out10, c->stream);
+      }
+      double o10[10];
+      HIPC(c, hipMemcpyAsync(o10, B.d_out10, 10 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIPC(c, hipStreamSynchronize(c->stream));
+      const double V = c->props[6];
+      for (int b = 0; b < RHG_NR_BLOCKS; ++b) {
+        double cnt = (double)(lb[b + 1] - lb[b]);
+        blocks->count[b] += cnt;
+        blocks->ideal_gas[b] += cnt * (V / ((double)0 + 1.0));
+        blocks->sum_w[b] += o10[2 * b];
+        blocks->sum_wdu[b] += o10[2 * b + 1];
+      }
+    }
+  }
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return RHG_OK;
+}
+
+int rhg_widom_run_trials(rhg_ctx *c, const rhg_molecule *mol, int64_t n, const double *fb,
+                         const double *cord, const double *u_sel, double *rec) {
+  if (!c || !mol || n < 0 || !rec || (n > 0 && (!fb || !u_sel))) return RHG_ERR_INVALID;
+  MolDev m;
+  if (int rc = build_mol(c, mol, m)) return rc;
+  if (m.n_sites > 1 && !cord) return fail(c, RHG_ERR_INVALID, "cord required");
+  if (n == 0) return RHG_OK;
+  hipSetDevice(c->device);
+  WidomBufs B;
+  if (int rc = widom_buffers(c, m, n, B)) return rc;
+  size_t bfb = (size_t)n * m.k_fb * 3 * sizeof(double);
+  size_t bcord = (size_t)n * m.k_or * m.n_sites * 3 * sizeof(double);
+  size_t bsel = (size_t)n * 2 * sizeof(double);
+  double *hfb = (double *)work(c, "h_fb", bfb), *hsel = (double *)work(c, "h_sel", bsel);
+  double *hcord = m.n_sites > 1 ? (double *)work(c, "h_cord", bcord) : nullptr;
+  if (!hfb || !hsel || (m.n_sites > 1 && !hcord)) return fail(c, RHG_ERR_OOM, "trial buffers");
+  upload(c, hfb, fb, bfb);
+  upload(c, hsel, u_sel, bsel);
+  if (hcord) upload(c, hcord, cord, bcord);
+  B.w.h_fb = hfb;
+  B.w.h_cord = hcord;
+  B.w.h_usel = hsel;
+  WidomArgs a;
+  memset(&a, 0, sizeof(a));
+  a.mol = m;
+  a.cell = c->cell;
+  a.beta = c->beta;
+  a.first = 0;
+  a.n = n;
+  a.use_host_trials = 1;
+  a.ewald = (c->charge_method == RHG_CHARGE_EWALD) ? 1 : 0;
+  if (int rc = widom_chunk(c, mol, m, B, a)) return rc;
+  double *drec = (double *)work(c, "w_rec", (size_t)n * RHG_REC_N * sizeof(double));
+  if (!drec) return fail(c, RHG_ERR_OOM, "rec buffer");
+  launch_widom_records(a, B.w, drec, c->stream);
+  HIPC(c, hipMemcpyAsync(rec, drec, (size_t)n * RHG_REC_N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return RHG_OK;
+}
+
+int rhg_widom_estimators(const rhg_widom_blocks *b, double T, double rho, double igw, double out[16]) {
+  if (!b || !out || T <= 0.0 || rho <= 0.0 || igw <= 0.0) return RHG_ERR_INVALID;
+  Units u = make_units();
+  const int NB = RHG_NR_BLOCKS;
+  double sum = 0.0, sum2 = 0.0;
+  for (int i = 0; i < NB; i++) {
+    double tmp = 0.0;
+    if (b->count[i] > 0.0) {
+      tmp = (1.0 / (kMolarGas * T * rho)) * b->sum_w[i] / (igw * b->count[i]);
+      sum += tmp;
+      sum2 += tmp * tmp;
+    }
+    out[i] = tmp;
+  }
+  out[5] = sum / (double)NB;
+  out[6] = 2.0 * sqrt(fabs((sum2 / (double)NB) - (sum * sum) / (double)(NB * NB)));
+  sum = sum2 = 0.0;
+  for (int i = 0; i < NB; i++) {
+    double tmp = 0.0;
+    if (b->count[i] > 0.0) {
+      tmp = (b->sum_wdu[i] / b->sum_w[i] - 0.0 / b->count[i]) * u.E2K;
+      sum += tmp;
+      sum2 += tmp * tmp;
+    }
+    out[7 + i] = tmp;
+  }
+  out[12] = sum / (double)NB;
+  out[13] = 2.0 * sqrt(fabs((sum2 / (double)NB) - (sum * sum) / (double)(NB * NB)));
+  double cnt = 0.0, sw = 0.0;
+  for (int i = 0; i < NB; i++)
+    if (b->count[i] > 0.0) {
+      sw += b->sum_w[i] / igw;
+      cnt += b->count[i];
+    }
+  out[14] = cnt > 0.0 ? (1.0 / (kMolarGas * T * rho)) * (sw / cnt) : 0.0;
+  out[15] = cnt > 0.0 ? sw * igw / cnt : 0.0;
+  return RHG_OK;
+}
+
+int rhg_ewald_fourier_adsorbate(rhg_ctx *c, int64_t n_mol, int32_t n_sites, const int32_t *types,
+                                const double *xyz, double *d_host, double *d_ads) {
+  if (!c || n_mol < 0 || n_sites < 1 || n_sites > RHG_MAX_SITES || !types || (n_mol > 0 && !xyz))
+    return RHG_ERR_INVALID;
+  if (c->charge_method != RHG_CHARGE_EWALD) {
+    for (int64_t i = 0; i < n_mol; ++i) {
+      if (d_host) d_host[i] = 0.0;
+      if (d_ads) d_ads[i] = 0.0;
+    }
+    return RHG_OK;
+  }
+  for (int j = 0; j < n_sites; ++j)
+    if (int rc = check_type(c, types[j])) return rc;
+  if (n_mol == 0) return RHG_OK;
+  hipSetDevice(c->device);
+  std::vector<double> px((size_t)n_sites * n_mol), py(px.size()), pz(px.size());
+  for (int64_t i = 0; i < n_mol; ++i)
+    for (int j = 0; j < n_sites; ++j) {
+      px[j * n_mol + i] = xyz[(i * n_sites + j) * 3 + 0];
+      py[j * n_mol + i] = xyz[(i * n_sites + j) * 3 + 1];
+      pz[j * n_mol + i] = xyz[(i * n_sites + j) * 3 + 2];
+    }
+  size_t b = px.size() * sizeof(double);
+  double *dx = (double *)work(c, "f_x", b), *dy = (double *)work(c, "f_y", b), *dz = (double *)work(c, "f_z", b);
+  double *dh = (double *)work(c, "f_h", n_mol * sizeof(double)), *da = (double *)work(c, "f_a", n_mol * sizeof(double));
+  if (!dx || !dy || !dz || !dh || !da) return fail(c, RHG_ERR_OOM, "fourier buffers");
+  upload(c, dx, px.data(), b);
+  upload(c, dy, py.data(), b);
+  upload(c, dz, pz.data(), b);
+  FourierArgs f;
+  memset(&f, 0, sizeof(f));
+  f.e = ewald_dev(c);
+  f.n_sites = n_sites;
+  for (int j = 0; j < n_sites; ++j) {
+    double q = c->types[types[j]].q;
+    if (considered_charged(q)) {
+      f.cidx[f.nc] = j;
+      f.q[f.nc] = 1.0 * q;
+      f.self_energy += c->u.C * (f.q[f.nc] * f.q[f.nc]) * c->alpha / sqrt(M_PI);
+      f.nc++;
+    }
+  }
+  f.px = dx;
+  f.py = dy;
+  f.pz = dz;
+  f.stride = n_mol;
+  f.n = n_mol;
+  f.n_dev = nullptr;
+  f.d_host = dh;
+  f.d_ads = da;
+  if (f.nc == 0) {
+    fill(dh, n_mol, 0.0, c->stream);
+    fill(da, n_mol, 0.0, c->stream);
+  } else {
+    Timed t(c, "fourier");
+    launch_fourier(f, c->stream);
+  }
+  HIPC(c, hipGetLastError());
+  if (d_host) HIPC(c, hipMemcpyAsync(d_host, dh, n_mol * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (d_ads) HIPC(c, hipMemcpyAsync(d_ads, da, n_mol * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return RHG_OK;
+}
+
+// ------------------------------------------------------------------ grids
+int rhg_make_grid(rhg_ctx *c, int32_t type, const double unit_box[9], double spacing, int32_t fp64,
+                  rhg_grid **out) {
+  if (!c || !unit_box || !out || !(spacing > 0.0)) return RHG_ERR_INVALID;
+  if (int rc = check_type(c, type)) return rc;
+  *out = nullptr;
+  // src/grids.c:441-449
+  if (std::min(c->props[3], std::min(c->props[4], c->props[5])) < c->rc_vdw)
+    return fail(c, RHG_ERR_INVALID, "cutoff larger than a perpendicular box width");
+  std::unique_ptr<rhg_grid> g(new rhg_grid());
+  g->ctx = c;
+  double size[3] = {0.0, 0.0, 0.0}, shift[3] = {0.0, 0.0, 0.0};
+  const int col[3][3] = {{AX, AY, AZ}, {BX, BY, BZ}, {CX, CY, CZ}};
+  for (int v = 0; v < 3; v++)
+    for (int d = 0; d < 3; d++) {
+      size[d] += fabs(unit_box[col[v][d]]);
+      if (unit_box[col[v][d]] < 0.0) shift[d] += unit_box[col[v][d]];
+    }
+  for (int d = 0; d < 3; d++) {
+    g->n[d] = (int)(size[d] / spacing); // src/grids.c:486-492
+    if (g->n[d] % 2 == 0) g->n[d]++;
+    g->size[d] = size[d];
+    g->shift[d] = shift[d];
+    g->delta[d] = size[d] / (double)g->n[d];
+  }
+  memcpy(g->ubox, unit_box, 9 * sizeof(double));
+  invert3x3(g->ubox, g->uinv);
+  double up[7];
+  cell_properties(g->ubox, up);
+  g->ulen[0] = up[0];
+  g->ulen[1] = up[1];
+  g->ulen[2] = up[2];
+  g->bc = c->cell.bc;
+  g->fp64 = fp64 ? 1 : 0;
+  int64_t nodes = (int64_t)(g->n[0] + 1) * (g->n[1] + 1) * (g->n[2] + 1);
+  g->bytes = (size_t)nodes * 8 * (g->fp64 ? sizeof(double) : sizeof(float));
+  hipSetDevice(c->device);
+  if (hipMalloc(&g->d, g->bytes) != hipSuccess) return fail(c, RHG_ERR_OOM, "grid allocation");
+  const PseudoAtom &t = c->types[type];
+  GridArgs a;
+  memset(&a, 0, sizeof(a));
+  a.cell = c->cell;
+  for (int d = 0; d < 3; ++d) {
+    a.n[d] = g->n[d];
+    a.size[d] = g->size[d];
+    a.shift[d] = g->shift[d];
+    a.delta[d] = g->delta[d];
+  }
+  a.rc2 = c->rc2_vdw;
+  a.n_atoms = c->n;
+  a.a64 = t.d64;
+  a.c24 = t.c24;
+  a.c96 = t.c96;
+  a.c384 = t.c384;
+  a.fp64 = g->fp64;
+  a.out = g->d;
+  const int64_t step = (int64_t)1 << 22;
+  for (int64_t n0 = 0; n0 < nodes; n0 += step) {
+    a.node0 = n0;
+    a.n_nodes = std::min(step, nodes - n0);
+    Timed tm(c, "grid_build");
+    launch_grid_build(a, c->stream);
+  }
+  HIPC(c, hipGetLastError());
+  HIPC(c, hipStreamSynchronize(c->stream));
+  *out = g.release();
+  return RHG_OK;
+}
+
+int rhg_grid_info(const rhg_grid *g, double out[13]) {
+  if (!g || !out) return RHG_ERR_INVALID;
+  for (int d = 0; d < 3; ++d) {
+    out[d] = g->n[d];
+    out[3 + d] = g->size[d];
+    out[6 + d] = g->shift[d];
+    out[9 + d] = g->delta[d];
+  }
+  out[12] = 8.0 * (g->fp64 ? 8.0 : 4.0);
+  return RHG_OK;
+}
+
+int rhg_grid_download(const rhg_grid *g, void *host, size_t bytes) {
+  if (!g || !host || bytes < g->bytes) return RHG_ERR_INVALID;
+  hipSetDevice(g->ctx->device);
+  if (hipMemcpy(host, g->d, g->bytes, hipMemcpyDeviceToHost) != hipSuccess) return RHG_ERR_HIP;
+  return RHG_OK;
+}
+
+int rhg_interpolate_vdw_grid(rhg_grid *g, int64_t n, const double *xyz, double *u) {
+  if (!g || n < 0 || (n > 0 && (!xyz || !u))) return RHG_ERR_INVALID;
+  if (n == 0) return RHG_OK;
+  rhg_ctx *c = g->ctx;
+  hipSetDevice(c->device);
+  double *dx = (double *)work(c, "i_xyz", (size_t)n * 3 * sizeof(double));
+  double *du = (double *)work(c, "i_u", (size_t)n * sizeof(double));
+  if (!dx || !du) return fail(c, RHG_ERR_OOM, "interp buffers");
+  upload(c, dx, xyz, (size_t)n * 3 * sizeof(double));
+  InterpArgs a;
+  memset(&a, 0, sizeof(a));
+  for (int d = 0; d < 3; ++d) {
+    a.n[d] = g->n[d];
+    a.size[d] = g->size[d];
+    a.shift[d] = g->shift[d];
+    a.ulen[d] = g->ulen[d];
+  }
+  memcpy(a.ubox, g->ubox, sizeof(a.ubox));
+  memcpy(a.uinv, g->uinv, sizeof(a.uinv));
+  a.bc = g->bc;
+  a.fp64 = g->fp64;
+  a.grid = g->d;
+  {
+    Timed t(c, "grid_interp");
+    launch_interp(a, dx, n, du, c->stream);
+  }
+  HIPC(c, hipGetLastError());
+  HIPC(c, hipMemcpyAsync(u, du, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return RHG_OK;
+}
+
+void rhg_grid_free(rhg_grid *g) {
+  if (!g) return;
+  hipSetDevice(g->ctx->device);
+  hipFree(g->d);
+  delete g;
+}
+
+int rhg_derivatives_at_positions_vdw(rhg_ctx *c, int32_t type, int64_t n, const double *xyz, double *out) {
+  if (!c || n < 0 || (n > 0 && (!xyz || !out))) return RHG_ERR_INVALID;
+  if (int rc = check_type(c, type)) return rc;
+  if (n == 0) return RHG_OK;
+  hipSetDevice(c->device);
+  double *dx = (double *)work(c, "d_xyz", (size_t)n * 3 * sizeof(double));
+  double *dout = (double *)work(c, "d_out", (size_t)n * 14 * sizeof(double));
+  if (!dx || !dout) return fail(c, RHG_ERR_OOM, "derivative buffers");
+  upload(c, dx, xyz, (size_t)n * 3 * sizeof(double));
+  const PseudoAtom &t = c->types[type];
+  GridArgs a;
+  memset(&a, 0, sizeof(a));
+  a.cell = c->cell;
+  a.rc2 = c->rc2_vdw;
+  a.n_atoms = c->n;
+  a.a64 = t.d64;
+  a.c24 = t.c24;
+  a.c96 = t.c96;
+  a.c384 = t.c384;
+  launch_derivatives(a, dx, n, dout, c->stream);
+  HIPC(c, hipGetLastError());
+  HIPC(c, hipMemcpyAsync(out, dout, (size_t)n * 14 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return RHG_OK;
+}
+
+// -------------------------------------------------------------- profiling
+int rhg_profile_enable(rhg_ctx *c, int32_t on) {
+  if (!c) return RHG_ERR_INVALID;
+  c->prof = on != 0;
+  return RHG_OK;
+}
+
+int rhg_profile_get(rhg_ctx *c, const char *kernel, int64_t *launches, double *total_ms) {
+  if (!c || !kernel) return RHG_ERR_INVALID;
+  hipSetDevice(c->device);
+  drain_profile(c);
+  auto it = c->stats.find(kernel);
+  if (launches) *launches = it == c->stats.end() ? 0 : it->second.first;
+  if (total_ms) *total_ms = it == c->stats.end() ? 0.0 : it->second.second;
+  return RHG_OK;
+}
+
+int rhg_profile_reset(rhg_ctx *c) {
+  if (!c) return RHG_ERR_INVALID;
+  drain_profile(c);
+  c->stats.clear();
+  return RHG_OK;
+}
+
+int rhg_synchronize(rhg_ctx *c) {
+  if (!c) return RHG_ERR_INVALID;
+  hipSetDevice(c->device);
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return RHG_OK;
+}
+
+} // extern "C"

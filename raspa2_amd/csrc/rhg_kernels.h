@@ -1,0 +1,151 @@
+// rhg_kernels.h -- launch interface between the host runtime and the gfx950
+// kernels of librhg.  All launches are asynchronous on the given stream.
+#pragma once
+
+#include "rhg_common.h"
+
+namespace rhg {
+
+// A list of sites (trial bead positions) of one pseudo-atom type.
+struct SiteList {
+  const double *x, *y, *z; // coordinates, SoA
+  int64_t n;               // number of sites (upper bound when n_dev is set)
+  const int32_t *n_dev;    // optional device-side count (<= n)
+  int32_t count_mult;      // sites = (*n_dev) * count_mult when n_dev is set
+};
+
+struct EnergyOut {
+  double *u_vdw; // nullable
+  double *u_cc;  // nullable
+  uint8_t *overlap; // nullable
+};
+
+struct EnergyArgs {
+  Cell cell;
+  double rc2_vdw, rc2_cc;
+  double alpha;  // Ewald alpha
+  double cqa;    // C * scaling * q_probe (0 disables the Coulomb term)
+  int n_atoms;
+  const AtomRec64 *a64; // FP64 records of this probe type
+  const AtomRecMx *amx; // mixed records of this probe type
+  int precision;        // 0 FP64, 1 mixed
+  int do_vdw, do_cc;
+};
+
+void launch_site_energy(const EnergyArgs &a, const SiteList &s, const EnergyOut &o,
+                        hipStream_t st);
+
+// ---------------------------------------------------------------- Ewald
+struct EwaldDev {
+  int kx, ky, kz, nk;
+  double alpha, uion, net_q_fw, beta_unused;
+  const double *kfactor;   // nk, RASPA2 order
+  const double2 *sf;       // nk framework structure factors
+  const int32_t *row_off;  // (kx+1)*(2ky+1): nvec index of the row's first valid kz
+  const int32_t *row_kz;   // (kx+1)*(2ky+1): largest valid |kz| of the row (-1 empty)
+  Cell cell;
+  double C;                // Coulomb conversion factor
+};
+
+// structure factors of the charged framework atoms (fractional coordinates)
+void launch_structure_factor(const EwaldDev &e, const int32_t *kidx, int nk, const double *fx,
+                             const double *fy, const double *fz, const double *q, int n,
+                             double2 *out, hipStream_t st);
+
+// Fourier Delta of n molecules; positions pos[j][m] (site-major SoA of
+// three arrays each n_sites*stride); list/n_dev select molecules.
+struct FourierArgs {
+  EwaldDev e;
+  int n_sites;
+  int nc;                 // charged sites
+  int cidx[kMaxSites];    // site index of each charged site, in site order
+  double q[kMaxSites];    // charge of each charged site
+  double self_energy; // sum C q^2 alpha/sqrt(pi) of the charged sites
+  const double *px, *py, *pz; // [site * stride + m]
+  int64_t stride;
+  int64_t n;                  // molecules (upper bound with n_dev)
+  const int32_t *n_dev;
+  double *d_host, *d_ads;     // per molecule
+};
+void launch_fourier(const FourierArgs &a, hipStream_t st);
+
+// ---------------------------------------------------------------- Widom
+struct WidomWork {
+  int64_t cap;       // insertions per chunk
+  // first-bead sites: site = i*k_fb + t
+  double *fbx, *fby, *fbz, *fb_uv, *fb_uc;
+  // per insertion state
+  int32_t *stage, *selfb, *selor;
+  uint64_t *fbmask, *ormask;
+  double *W1, *W2, *W, *dU, *uvdw, *ureal, *ufh, *ufa, *usel_or;
+  double *selx, *sely, *selz; // selected first-bead position
+  // compaction after the first-bead stage
+  int32_t *active, *n_active;
+  // orientation sites per placed bead p: [p][slot*k_or + iu]
+  double *orx, *ory, *orz, *or_uv;
+  // compaction after the orientation stage; selected configuration [j][slot2]
+  int32_t *active2, *n_active2;
+  double *cfx, *cfy, *cfz;
+  double *cc_place; // [p][slot2] real-space Coulomb of the placed beads
+  double *f_host, *f_ads; // [slot2]
+  // host-supplied trials (nullable)
+  const double *h_fb, *h_cord, *h_usel;
+};
+
+struct WidomArgs {
+  MolDev mol;
+  Cell cell;
+  double beta;
+  uint64_t seed;
+  int64_t first; // global index of the first insertion of the chunk
+  int64_t n;     // insertions in this chunk
+  int use_host_trials;
+  int ewald;
+};
+
+void launch_widom_fb_gen(const WidomArgs &a, const WidomWork &w, hipStream_t st);
+void launch_widom_fb_select(const WidomArgs &a, const WidomWork &w, hipStream_t st);
+void launch_widom_or_select(const WidomArgs &a, const WidomWork &w, hipStream_t st);
+void launch_widom_finalize(const WidomArgs &a, const WidomWork &w, hipStream_t st);
+void launch_widom_records(const WidomArgs &a, const WidomWork &w, double *rec, hipStream_t st);
+void launch_zero_counters(int32_t *a, int32_t *b, hipStream_t st);
+
+// deterministic block sums over insertions [0,n) of a chunk; bounds[6] are
+// the chunk-local insertion index where each block starts (clamped).
+struct BlockPartials {
+  double *part; // n_seg x 5 x 3
+  int n_seg;
+};
+void launch_block_reduce(const double *W, const double *dU, int64_t n, const int64_t bounds[6],
+                         double *part, int seg, double *out15, hipStream_t st);
+
+// ----------------------------------------------------------------- grids
+struct GridArgs {
+  Cell cell;
+  int n[3];          // NumberOfVDWGridPoints (nodes per axis = n+1)
+  double size[3], shift[3], delta[3];
+  double rc2;
+  int n_atoms;
+  const AtomRec64 *a64;
+  const double *c24, *c96, *c384; // per atom: 24*eps, 96*eps, 384*eps
+  int fp64;
+  void *out;         // [i][j][k][8] float or double
+  int64_t node0, n_nodes; // node range of this launch
+};
+void launch_grid_build(const GridArgs &g, hipStream_t st);
+void launch_derivatives(const GridArgs &g, const double *xyz, int64_t n, double *out,
+                        hipStream_t st);
+
+struct InterpArgs {
+  int n[3];
+  double size[3], shift[3];
+  double ubox[9], uinv[9];
+  double ulen[3];
+  int bc;
+  int fp64;
+  const void *grid;
+};
+void launch_interp(const InterpArgs &a, const double *xyz, int64_t n, double *out,
+                   hipStream_t st);
+
+} // namespace rhg

@@ -1,0 +1,1087 @@
+// rhg_kernels.hip -- gfx950 kernels of the host-guest energy engine.
+//
+// Site-energy kernels give every lane one trial site and stream the
+// framework through the scalar unit: the atom index of the inner loop is
+// wave-uniform, so each atom record is fetched once per wave with scalar
+// loads into SGPRs and broadcast to all 64 lanes as an instruction operand
+// (no LDS traffic, no bank conflicts, no VGPRs for the framework).
+#include <float.h>
+#include <hip/hip_runtime.h>
+
+#include "rhg_kernels.h"
+
+namespace rhg {
+
+// ====================================================================
+// minimum image variants
+// ====================================================================
+// BCK 1: rectangular, 2: upper-triangular triclinic, 3: general triclinic
+template <int BCK>
+__device__ __forceinline__ void mi_exact(const Cell &c, double &dx, double &dy, double &dz) {
+#pragma clang fp contract(off)
+  if (BCK == 1) {
+    dx -= c.box[AX] * nint_d(dx * c.inv[AX]);
+    dy -= c.box[BY] * nint_d(dy * c.inv[BY]);
+    dz -= c.box[CZ] * nint_d(dz * c.inv[CZ]);
+  } else if (BCK == 2) {
+    double sx = c.inv[AX] * dx + c.inv[BX] * dy + c.inv[CX] * dz;
+    double sy = c.inv[BY] * dy + c.inv[CY] * dz;
+    double sz = c.inv[CZ] * dz;
+    double tx = sx - nint_d(sx);
+    double ty = sy - nint_d(sy);
+    double tz = sz - nint_d(sz);
+    dx = c.box[AX] * tx + c.box[BX] * ty + c.box[CX] * tz;
+    dy = c.box[BY] * ty + c.box[CY] * tz;
+    dz = c.box[CZ] * tz;
+  } else {
+    double sx = c.inv[AX] * dx + c.inv[BX] * dy + c.inv[CX] * dz;
+    double sy = c.inv[AY] * dx + c.inv[BY] * dy + c.inv[CY] * dz;
+    double sz = c.inv[AZ] * dx + c.inv[BZ] * dy + c.inv[CZ] * dz;
+    double tx = sx - nint_d(sx);
+    double ty = sy - nint_d(sy);
+    double tz = sz - nint_d(sz);
+    dx = c.box[AX] * tx + c.box[BX] * ty + c.box[CX] * tz;
+    dy = c.box[AY] * tx + c.box[BY] * ty + c.box[CY] * tz;
+    dz = c.box[AZ] * tx + c.box[BZ] * ty + c.box[CZ] * tz;
+  }
+}
+
+// Fast minimum image for the mixed kernels: round-to-nearest-even and FMA.
+// Differs from NINT only when a coordinate sits exactly half a box away,
+// where both images are equidistant.
+template <int BCK>
+__device__ __forceinline__ void mi_fast(const Cell &c, double &dx, double &dy, double &dz) {
+  if (BCK == 1) {
+    dx = fma(-c.box[AX], rint(dx * c.inv[AX]), dx);
+    dy = fma(-c.box[BY], rint(dy * c.inv[BY]), dy);
+    dz = fma(-c.box[CZ], rint(dz * c.inv[CZ]), dz);
+  } else if (BCK == 2) {
+    double sx = fma(c.inv[AX], dx, fma(c.inv[BX], dy, c.inv[CX] * dz));
+    double sy = fma(c.inv[BY], dy, c.inv[CY] * dz);
+    double sz = c.inv[CZ] * dz;
+    double tx = sx - rint(sx), ty = sy - rint(sy), tz = sz - rint(sz);
+    dx = fma(c.box[AX], tx, fma(c.box[BX], ty, c.box[CX] * tz));
+    dy = fma(c.box[BY], ty, c.box[CY] * tz);
+    dz = c.box[CZ] * tz;
+  } else {
+    double sx = fma(c.inv[AX], dx, fma(c.inv[BX], dy, c.inv[CX] * dz));
+    double sy = fma(c.inv[AY], dx, fma(c.inv[BY], dy, c.inv[CY] * dz));
+    double sz = fma(c.inv[AZ], dx, fma(c.inv[BZ], dy, c.inv[CZ] * dz));
+    double tx = sx - rint(sx), ty = sy - rint(sy), tz = sz - rint(sz);
+    dx = fma(c.box[AX], tx, fma(c.box[BX], ty, c.box[CX] * tz));
+    dy = fma(c.box[AY], tx, fma(c.box[BY], ty, c.box[CY] * tz));
+    dz = fma(c.box[AZ], tx, fma(c.box[BZ], ty, c.box[CZ] * tz));
+  }
+}
+
+__device__ __forceinline__ int64_t site_count(const SiteList &s) {
+  return s.n_dev ? (int64_t)(*s.n_dev) * s.count_mult : s.n;
+}
+
+// Exact FP64 energies of one site in reference order (all atoms, index order).
+template <int BCK>
+__device__ __forceinline__ void exact_site(const Cell &cell, const AtomRec64 *__restrict__ a,
+                                           int n_atoms, double rc2v, double rc2c, double alpha,
+                                           double cqa, int do_vdw, int do_cc, double px, double py,
+                                           double pz, double &uv, double &uc) {
+#pragma clang fp contract(off)
+  uv = 0.0;
+  uc = 0.0;
+  for (int j = 0; j < n_atoms; ++j) {
+    const AtomRec64 r = a[j];
+    double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
+    mi_exact<BCK>(cell, dx, dy, dz);
+    double rr = dx * dx + dy * dy + dz * dz;
+    if (do_vdw && rr < rc2v) uv += lj_exact(r.c4, r.s2, r.shift, rr);
+    if (do_cc && rr < rc2c) {
+      double rd = sqrt(rr);
+      uc += cqa * r.q * (erfc(alpha * rd) / rd);
+    }
+  }
+}
+
+// ====================================================================
+// FP64 site energies: bit-identical to the sequential reference loop
+// (src/framework_energy.c:151-170 and 258-275) for the LJ term.
+// ====================================================================
+template <int BCK, int VDW, int CC>
+__global__ __launch_bounds__(256) void k_site_energy64(
+    const Cell cell, const AtomRec64 *__restrict__ atoms, const int n_atoms, const double rc2v,
+    const double rc2c, const double alpha, const double cqa, const SiteList s,
+    double *__restrict__ out_v, double *__restrict__ out_c, uint8_t *__restrict__ out_o) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = site_count(s);
+  if ((int64_t)blockIdx.x * blockDim.x >= n) return;
+  const bool live = i < n;
+  const int64_t k = live ? i : 0;
+  const double px = s.x[k], py = s.y[k], pz = s.z[k];
+  double uv = 0.0, uc = 0.0;
+#pragma unroll 2
+  for (int j = 0; j < n_atoms; ++j) {
+    const AtomRec64 r = atoms[j];
+    double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
+    mi_exact<BCK>(cell, dx, dy, dz);
+    double rr = dx * dx + dy * dy + dz * dz;
+    if (VDW) {
+      if (rr < rc2v) uv += lj_exact(r.c4, r.s2, r.shift, rr);
+    }
+    if (CC) {
+      if (rr < rc2c) {
+        double rd = sqrt(rr);
+        uc += cqa * r.q * (erfc(alpha * rd) / rd);
+      }
+    }
+  }
+  if (live) {
+    if (out_v) out_v[i] = uv;
+    if (out_c) out_c[i] = uc;
+    if (out_o) out_o[i] = (uint8_t)((VDW && uv >= kOverlap) || (CC && uc >= kOverlap));
+  }
+}
+
+// ====================================================================
+// Mixed site energies: FP64 differences and minimum image, FP32 pair
+// terms, FP64 accumulation.  Sites whose energy lands within 1e-4 of the
+// overlap threshold are recomputed with the exact FP64 loop so overlap
+// flags equal the FP64 reference bit for bit.
+// ====================================================================
+template <int BCK, int VDW, int CC>
+__global__ __launch_bounds__(256) void k_site_energy_mx(
+    const Cell cell, const AtomRecMx *__restrict__ atoms, const AtomRec64 *__restrict__ atoms64,
+    const int n_atoms, const double rc2v, const double rc2c, const double alpha, const double cqa,
+    const SiteList s, double *__restrict__ out_v, double *__restrict__ out_c,
+    uint8_t *__restrict__ out_o) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = site_count(s);
+  if ((int64_t)blockIdx.x * blockDim.x >= n) return;
+  const bool live = i < n;
+  const int64_t k = live ? i : 0;
+  const double px = s.x[k], py = s.y[k], pz = s.z[k];
+  const float alphaf = (float)alpha;
+  const float cqaf = (float)cqa;
+  double uv = 0.0, uc = 0.0;
+#pragma unroll 4
+  for (int j = 0; j < n_atoms; ++j) {
+    const AtomRecMx r = atoms[j];
+    double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
+    mi_fast<BCK>(cell, dx, dy, dz);
+    const double rr = fma(dx, dx, fma(dy, dy, dz * dz));
+    const float rrf = (float)rr;
+    if (VDW) {
+      const float ir = __builtin_amdgcn_rcpf(rrf);
+      const float t = r.s2 * ir;
+      const float r3 = t * t * t;
+      const float u = fmaf(r.c4, fmaf(r3, r3, -r3), -r.shift);
+      uv += (rr < rc2v) ? (double)u : 0.0;
+    }
+    if (CC) {
+      if (rr < rc2c) {
+        const float irs = __builtin_amdgcn_rsqf(rrf);
+        const float rd = rrf * irs;
+        const float e = erfcf(alphaf * rd);
+        uc += (double)((cqaf * r.q) * (e * irs));
+      }
+    }
+  }
+  const double band = 1.0e-4 * kOverlap;
+  const bool near = live && ((VDW && fabs(uv - kOverlap) <= band) || (CC && fabs(uc - kOverlap) <= band));
+  if (__any(near)) {
+    if (near) {
+      double ev, ec;
+      exact_site<BCK>(cell, atoms64, n_atoms, rc2v, rc2c, alpha, cqa, VDW, CC, px, py, pz, ev, ec);
+      if (VDW) uv = ev;
+      if (CC) uc = ec;
+    }
+  }
+  if (live) {
+    if (out_v) out_v[i] = uv;
+    if (out_c) out_c[i] = uc;
+    if (out_o) out_o[i] = (uint8_t)((VDW && uv >= kOverlap) || (CC && uc >= kOverlap));
+  }
+}
+
+template <int BCK, int VDW, int CC>
+static void site_energy_bck(const EnergyArgs &a, const SiteList &s, const EnergyOut &o,
+                            hipStream_t st) {
+  const int bs = 256;
+  int64_t nb = (s.n + bs - 1) / bs;
+  if (nb == 0) return;
+  if (a.precision == 0)
+    hipLaunchKernelGGL((k_site_energy64<BCK, VDW, CC>), dim3((unsigned)nb), dim3(bs), 0, st, a.cell,
+                       a.a64, a.n_atoms, a.rc2_vdw, a.rc2_cc, a.alpha, a.cqa, s, o.u_vdw, o.u_cc,
+                       o.overlap);
+  else
+    hipLaunchKernelGGL((k_site_energy_mx<BCK, VDW, CC>), dim3((unsigned)nb), dim3(bs), 0, st,
+                       a.cell, a.amx, a.a64, a.n_atoms, a.rc2_vdw, a.rc2_cc, a.alpha, a.cqa, s,
+                       o.u_vdw, o.u_cc, o.overlap);
+}
+
+template <int VDW, int CC>
+static void site_energy_mode(const EnergyArgs &a, const SiteList &s, const EnergyOut &o,
+                             hipStream_t st) {
+  if (a.cell.bc == 1)
+    site_energy_bck<1, VDW, CC>(a, s, o, st);
+  else if (a.cell.upper)
+    site_energy_bck<2, VDW, CC>(a, s, o, st);
+  else
+    site_energy_bck<3, VDW, CC>(a, s, o, st);
+}
+
+void launch_site_energy(const EnergyArgs &a, const SiteList &s, const EnergyOut &o,
+                        hipStream_t st) {
+  if (a.do_vdw && a.do_cc)
+    site_energy_mode<1, 1>(a, s, o, st);
+  else if (a.do_vdw)
+    site_energy_mode<1, 0>(a, s, o, st);
+  else if (a.do_cc)
+    site_energy_mode<0, 1>(a, s, o, st);
+}
+
+// ====================================================================
+// Ewald: framework structure factors (src/ewald.c:1289-1731)
+// one lane per k-vector, atoms streamed through the scalar unit
+// ====================================================================
+__global__ __launch_bounds__(256) void k_structure_factor(const int32_t *__restrict__ kidx, int nk,
+                                                          const double *__restrict__ fx,
+                                                          const double *__restrict__ fy,
+                                                          const double *__restrict__ fz,
+                                                          const double *__restrict__ q, int n,
+                                                          double2 *__restrict__ out) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x * blockDim.x >= nk) return;
+  int kk = k < nk ? k : 0;
+  double ii = kidx[3 * kk], jj = kidx[3 * kk + 1], ll = kidx[3 * kk + 2];
+  double sr = 0.0, si = 0.0;
+  for (int a = 0; a < n; ++a) {
+    // phase = 2pi (ii fx + jj fy + ll fz); fractional coordinates already *2pi
+    double ph = ii * fx[a] + jj * fy[a] + ll * fz[a];
+    double s, c;
+    sincos(ph, &s, &c);
+    sr = fma(q[a], c, sr);
+    si = fma(q[a], s, si);
+  }
+  if (k < nk) out[k] = make_double2(sr, si);
+}
+
+void launch_structure_factor(const EwaldDev &e, const int32_t *kidx, int nk, const double *fx,
+                             const double *fy, const double *fz, const double *q, int n,
+                             double2 *out, hipStream_t st) {
+  (void)e;
+  int bs = 256;
+  int nb = (nk + bs - 1) / bs;
+  if (nb) hipLaunchKernelGGL(k_structure_factor, dim3(nb), dim3(bs), 0, st, kidx, nk, fx, fy, fz, q, n, out);
+}
+
+// ====================================================================
+// Fourier Delta of inserted molecules (src/ewald.c:6859-7565), one lane
+// per molecule.  exp(ik.r) by the reference's recurrences; the loop visits
+// each (kx, |ky|, |kz|) once and forms the sign variants as conjugates,
+// reading KFactor and the framework structure factor at the reference's
+// k index (wave-uniform, scalar loads).
+// ====================================================================
+struct cx {
+  double re, im;
+};
+__device__ __forceinline__ cx cmul(cx a, cx b) {
+#pragma clang fp contract(off)
+  cx r;
+  r.re = a.re * b.re - a.im * b.im;
+  r.im = a.im * b.re + a.re * b.im;
+  return r;
+}
+__device__ __forceinline__ cx conjc(cx a) { return cx{a.re, -a.im}; }
+
+template <int NC>
+__global__ __launch_bounds__(128) void k_fourier(const FourierArgs a) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = a.n_dev ? (int64_t)(*a.n_dev) : a.n;
+  if ((int64_t)blockIdx.x * blockDim.x >= n) return;
+  const bool live = m < n;
+  const int64_t mm = live ? m : 0;
+  const EwaldDev &e = a.e;
+  constexpr int nc = NC;
+  cx e1x[NC], e1y[NC], e1z[NC], ex[NC];
+  double qs[NC];
+  double pxs[NC], pys[NC], pzs[NC];
+  double net = 0.0;
+#pragma unroll
+  for (int t = 0; t < NC; ++t) {
+    const int j = a.cidx[t];
+    double px = a.px[j * a.stride + mm], py = a.py[j * a.stride + mm], pz = a.pz[j * a.stride + mm];
+    pxs[t] = px;
+    pys[t] = py;
+    pzs[t] = pz;
+    qs[t] = a.q[t];
+    net += a.q[t];
+    double fx, fy, fz;
+    {
+#pragma clang fp contract(off)
+      fx = e.cell.inv[AX] * px + e.cell.inv[BX] * py + e.cell.inv[CX] * pz;
+      fy = e.cell.inv[AY] * px + e.cell.inv[BY] * py + e.cell.inv[CY] * pz;
+      fz = e.cell.inv[AZ] * px + e.cell.inv[BZ] * py + e.cell.inv[CZ] * pz;
+      fx *= (2.0 * M_PI);
+      fy *= (2.0 * M_PI);
+      fz *= (2.0 * M_PI);
+    }
+    double s, c;
+    sincos(fx, &s, &c);
+    e1x[t] = cx{c, s};
+    sincos(fy, &s, &c);
+    e1y[t] = cx{c, s};
+    sincos(fz, &s, &c);
+    e1z[t] = cx{c, s};
+    ex[t] = cx{1.0, 0.0};
+  }
+  double host = 0.0, ads = 0.0;
+  const int ny = 2 * e.ky + 1;
+  for (int ii = 0; ii <= e.kx; ++ii) {
+    if (ii > 0) {
+#pragma unroll
+      for (int t = 0; t < nc; ++t) ex[t] = cmul(ex[t], e1x[t]);
+    }
+    cx ey[NC];
+#pragma unroll
+    for (int t = 0; t < nc; ++t) ey[t] = cx{1.0, 0.0};
+    for (int jj = 0; jj <= e.ky; ++jj) {
+      if (jj > 0) {
+#pragma unroll
+        for (int t = 0; t < nc; ++t) ey[t] = cmul(ey[t], e1y[t]);
+      }
+      for (int sy = 0; sy < 2; ++sy) {
+        if (jj == 0 && sy == 1) break;
+        const int jv = sy ? -jj : jj;
+        const int row = ii * ny + (jv + e.ky);
+        const int kzm = e.row_kz[row];
+        if (kzm < 0) continue;
+        const int off = e.row_off[row];
+        cx xy[NC], ez[NC];
+#pragma unroll
+        for (int t = 0; t < nc; ++t) {
+          xy[t] = cmul(ex[t], sy ? conjc(ey[t]) : ey[t]);
+          ez[t] = cx{1.0, 0.0};
+        }
+        const bool origin_row = (ii == 0 && jv == 0);
+        for (int kk = 0; kk <= kzm; ++kk) {
+          if (kk > 0) {
+#pragma unroll
+            for (int t = 0; t < nc; ++t) ez[t] = cmul(ez[t], e1z[t]);
+          }
+          for (int sz = 0; sz < 2; ++sz) {
+            if (kk == 0 && sz == 1) break;
+            if (origin_row && kk == 0) continue;
+            const int kv = sz ? -kk : kk;
+            // index within the row: valid kz run from -kzm to kzm, minus kz=0 on the origin row
+            int pos = kv + kzm;
+            if (origin_row && kv > 0) pos -= 1;
+            const int idx = off + pos;
+            double sre = 0.0, sim = 0.0;
+#pragma unroll
+            for (int t = 0; t < nc; ++t) {
+              cx ek = cmul(xy[t], sz ? conjc(ez[t]) : ez[t]);
+              sre += qs[t] * ek.re;
+              sim += qs[t] * ek.im;
+            }
+            const double kf = e.kfactor[idx];
+            const double2 sf = e.sf[idx];
+            host += kf * (sf.x * sre + sf.y * sim);
+            ads += kf * (sre * sre + sim * sim);
+          }
+        }
+      }
+    }
+  }
+  // excluded intra pairs (all charged pairs, src/molecule.c:2839-2851;
+  // src/ewald.c:7318-7335) and net-charge terms (src/ewald.c:7536-7565)
+  double excl = 0.0;
+#pragma unroll
+  for (int p = 0; p < nc; ++p)
+#pragma unroll
+    for (int r = p + 1; r < nc; ++r) {
+#pragma clang fp contract(off)
+      double dx = pxs[p] - pxs[r], dy = pys[p] - pys[r], dz = pzs[p] - pzs[r];
+      double rd = sqrt(dx * dx + dy * dy + dz * dz);
+      double bt0 = -erf(e.alpha * rd) / rd;
+      excl -= e.C * qs[p] * qs[r] * bt0;
+    }
+  double dq = net - 0.0;
+  double dads = ads - (excl - 0.0) - (a.self_energy - 0.0);
+  dads += e.uion * ((0.0 + dq) * (0.0 + dq)) - e.uion * (0.0 * 0.0);
+  double dhost = 2.0 * host;
+  dhost += 2.0 * e.uion * e.net_q_fw * (0.0 + dq) - 2.0 * e.uion * e.net_q_fw * 0.0;
+  if (live) {
+    a.d_host[m] = dhost;
+    a.d_ads[m] = dads;
+  }
+}
+
+void launch_fourier(const FourierArgs &a, hipStream_t st) {
+  const int bs = 128;
+  const unsigned nb = (unsigned)((a.n + bs - 1) / bs);
+  if (!nb) return;
+  switch (a.nc) {
+  case 1: hipLaunchKernelGGL(k_fourier<1>, dim3(nb), dim3(bs), 0, st, a); break;
+  case 2: hipLaunchKernelGGL(k_fourier<2>, dim3(nb), dim3(bs), 0, st, a); break;
+  case 3: hipLaunchKernelGGL(k_fourier<3>, dim3(nb), dim3(bs), 0, st, a); break;
+  case 4: hipLaunchKernelGGL(k_fourier<4>, dim3(nb), dim3(bs), 0, st, a); break;
+  case 5: hipLaunchKernelGGL(k_fourier<5>, dim3(nb), dim3(bs), 0, st, a); break;
+  case 6: hipLaunchKernelGGL(k_fourier<6>, dim3(nb), dim3(bs), 0, st, a); break;
+  case 7: hipLaunchKernelGGL(k_fourier<7>, dim3(nb), dim3(bs), 0, st, a); break;
+  case 8: hipLaunchKernelGGL(k_fourier<8>, dim3(nb), dim3(bs), 0, st, a); break;
+  default: break;
+  }
+}
+
+// ====================================================================
+// Widom / CBMC
+// ====================================================================
+// SelectTrialPosition (src/cbmc.c:563-595) over k trials whose Boltzmann
+// factor and overlap come from the callback.
+template <class F>
+__device__ __forceinline__ int select_trial(F bf_of, int k, double u) {
+  double largest = -DBL_MAX;
+  for (int t = 0; t < k; ++t) {
+    double b;
+    if (bf_of(t, b))
+      if (b > largest) largest = b;
+  }
+  double sum = 0.0;
+  for (int t = 0; t < k; ++t) {
+    double b;
+    if (bf_of(t, b)) sum += exp(b - largest);
+  }
+  double ws = u * sum;
+  int sel = 0;
+  double b0;
+  double cumw = bf_of(0, b0) ? exp(b0 - largest) : 0.0;
+  while (cumw < ws && sel < k - 1) {
+    ++sel;
+    double b;
+    cumw += bf_of(sel, b) ? exp(b - largest) : 0.0;
+  }
+  return sel;
+}
+
+// ConvertFromABCtoXYZ (src/potentials.c:5548-5556)
+__device__ __forceinline__ void abc_to_xyz(const Cell &c, double sx, double sy, double sz,
+                                           double &x, double &y, double &z) {
+#pragma clang fp contract(off)
+  x = c.box[AX] * sx + c.box[BX] * sy + c.box[CX] * sz;
+  y = c.box[AY] * sx + c.box[BY] * sy + c.box[CY] * sz;
+  z = c.box[AZ] * sx + c.box[BZ] * sy + c.box[CZ] * sz;
+}
+
+// first-bead trial positions: counter draws 0..3k_fb-1 (z,y,x per trial,
+// src/cbmc.c:652-667)
+__global__ __launch_bounds__(256) void k_widom_fb_gen(const WidomArgs a, const WidomWork w) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int k = a.mol.k_fb;
+  if (a.use_host_trials) {
+    for (int t = 0; t < k; ++t) {
+      w.fbx[i * k + t] = w.h_fb[(i * k + t) * 3 + 0];
+      w.fby[i * k + t] = w.h_fb[(i * k + t) * 3 + 1];
+      w.fbz[i * k + t] = w.h_fb[(i * k + t) * 3 + 2];
+    }
+    return;
+  }
+  const uint64_t key = stream_key(a.seed, (uint64_t)(a.first + i));
+  uint64_t c = 0;
+  for (int t = 0; t < k; ++t) {
+    double sz = stream_uniform(key, c++);
+    double sy = stream_uniform(key, c++);
+    double sx = stream_uniform(key, c++);
+    double x, y, z;
+    abc_to_xyz(a.cell, sx, sy, sz, x, y, z);
+    w.fbx[i * k + t] = x;
+    w.fby[i * k + t] = y;
+    w.fbz[i * k + t] = z;
+  }
+}
+
+// HandleFirstBead selection (src/cbmc.c:695-795) and, for molecules with
+// more than one site, the orientation trials of GenerateTrialOrientations-
+// SimpleSphere (src/cbmc.c:1480-1578).
+__global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, const WidomWork w) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int k = a.mol.k_fb;
+  const double beta = a.beta;
+  uint64_t mask = 0;
+  bool any = false;
+  double W1 = 0.0;
+  for (int t = 0; t < k; ++t) {
+    double uv = w.fb_uv[i * k + t], uc = w.fb_uc[i * k + t];
+    if (uv >= kOverlap || uc >= kOverlap)
+      mask |= (1ULL << t);
+    else {
+      any = true;
+      W1 += exp(-beta * (uv + 0.0 + 0.0));
+    }
+  }
+  w.fbmask[i] = mask;
+  w.ormask[i] = 0;
+  w.selor[i] = 0;
+  w.W2[i] = 0.0;
+  w.selfb[i] = 0;
+  w.W1[i] = 0.0;
+  w.W[i] = 0.0;
+  w.dU[i] = 0.0;
+  w.uvdw[i] = 0.0;
+  w.ureal[i] = 0.0;
+  w.ufh[i] = 0.0;
+  w.ufa[i] = 0.0;
+  if (!any) {
+    w.stage[i] = 0;
+    return;
+  }
+  uint64_t key = 0;
+  double usel;
+  if (a.use_host_trials)
+    usel = w.h_usel[2 * i];
+  else {
+    key = stream_key(a.seed, (uint64_t)(a.first + i));
+    usel = stream_uniform(key, (uint64_t)(3 * k));
+  }
+  auto bf = [&](int t, double &b) -> bool {
+    double uv = w.fb_uv[i * k + t], uc = w.fb_uc[i * k + t];
+    if (uv >= kOverlap || uc >= kOverlap) return false;
+    b = -beta * (uv + 0.0 + 0.0);
+    return true;
+  };
+  int sel = select_trial(bf, k, usel);
+  W1 /= (double)k;
+  w.selfb[i] = sel;
+  w.W1[i] = W1;
+  if (W1 < kMinRosenbluth) {
+    w.stage[i] = 1;
+    return;
+  }
+  const double fx = w.fbx[i * k + sel], fy = w.fby[i * k + sel], fz = w.fbz[i * k + sel];
+  w.selx[i] = fx;
+  w.sely[i] = fy;
+  w.selz[i] = fz;
+  w.uvdw[i] = w.fb_uv[i * k + sel];
+  w.ureal[i] = w.fb_uc[i * k + sel];
+  w.W[i] = W1;
+  w.stage[i] = 4;
+  const MolDev &m = a.mol;
+  if (m.n_sites == 1) {
+    int s2 = atomicAdd(w.n_active2, 1);
+    w.active2[s2] = (int32_t)i;
+    w.cfx[s2] = fx;
+    w.cfy[s2] = fy;
+    w.cfz[s2] = fz;
+    return;
+  }
+  const int slot = atomicAdd(w.n_active, 1);
+  w.active[slot] = (int32_t)i;
+  const int ko = m.k_or;
+  const int64_t cap_or = w.cap * (int64_t)ko;
+  uint64_t c = (uint64_t)(3 * k + 1);
+  double body0[kMaxSites][3];
+  for (int j = 0; j < m.n_sites; ++j) {
+    body0[j][0] = m.body[j][0] - m.body[m.start][0];
+    body0[j][1] = m.body[j][1] - m.body[m.start][1];
+    body0[j][2] = m.body[j][2] - m.body[m.start][2];
+  }
+  for (int iu = 0; iu < ko; ++iu) {
+    double cord[kMaxSites][3];
+    if (a.use_host_trials) {
+      for (int j = 0; j < m.n_sites; ++j)
+        for (int d = 0; d < 3; ++d)
+          cord[j][d] = w.h_cord[((i * ko + iu) * m.n_sites + j) * 3 + d];
+    } else {
+      // RandomNumberOnUnitSphere (src/utils.c:688-708); unused for rigid groups
+      double r1, r2, rsq;
+      int guard = 0;
+      do {
+        r1 = 2.0 * stream_uniform(key, c++) - 1.0;
+        r2 = 2.0 * stream_uniform(key, c++) - 1.0;
+        rsq = r1 * r1 + r2 * r2;
+      } while (rsq >= 1.0 && ++guard < 4096);
+      double X0 = stream_uniform(key, c++);
+      double u1 = stream_uniform(key, c++);
+      double u2 = stream_uniform(key, c++);
+      Rot R = shoemake(X0, u1, u2);
+      for (int j = 0; j < m.n_sites; ++j) rotate(R, body0[j], cord[j]);
+    }
+    for (int p = 0; p < m.n_place; ++p) {
+      const int j = m.place[p];
+      double x, y, z;
+      {
+#pragma clang fp contract(off)
+        x = fx + cord[j][0];
+        y = fy + cord[j][1];
+        z = fz + cord[j][2];
+      }
+      const int64_t o = p * cap_or + (int64_t)slot * ko + iu;
+      w.orx[o] = x;
+      w.ory[o] = y;
+      w.orz[o] = z;
+    }
+  }
+  w.usel_or[i] = a.use_host_trials ? w.h_usel[2 * i + 1] : stream_uniform(key, c);
+}
+
+// Rosen's selection over orientations (src/cbmc.c:1154-1478, 2762-2849)
+__global__ __launch_bounds__(256) void k_widom_or_select(const WidomArgs a, const WidomWork w) {
+  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= *w.n_active) return;
+  const int64_t i = w.active[slot];
+  const MolDev &m = a.mol;
+  const int ko = m.k_or;
+  const int64_t cap_or = w.cap * (int64_t)ko;
+  const double beta = a.beta;
+  auto ev_of = [&](int iu, double &ev) -> bool {
+#pragma clang fp contract(off)
+    ev = 0.0;
+    for (int p = 0; p < m.n_place; ++p) {
+      double uv = w.or_uv[p * cap_or + slot * ko + iu];
+      if (uv >= kOverlap) return false;
+      ev += uv;
+    }
+    return true;
+  };
+  uint64_t mask = 0;
+  bool any = false;
+  double weight = 0.0;
+  for (int iu = 0; iu < ko; ++iu) {
+    double ev;
+    if (ev_of(iu, ev)) {
+      any = true;
+      weight += exp(-beta * (ev + 0.0 + 0.0 + 0.0));
+    } else
+      mask |= (1ULL << iu);
+  }
+  w.ormask[i] = mask;
+  if (!any) {
+    w.stage[i] = 2;
+    w.W[i] = 0.0;
+    w.uvdw[i] = 0.0;
+    w.ureal[i] = 0.0;
+    return;
+  }
+  weight = weight / (double)ko;
+  auto bf = [&](int iu, double &b) -> bool {
+    double ev;
+    if (!ev_of(iu, ev)) return false;
+    b = -beta * (ev + 0.0 + 0.0 + 0.0);
+    return true;
+  };
+  const int iwalk = select_trial(bf, ko, w.usel_or[i]);
+  double Wn = w.W1[i] * (weight * 1.0);
+  w.selor[i] = iwalk;
+  w.W2[i] = weight;
+  if (Wn < kMinRosenbluth) {
+    w.stage[i] = 3;
+    w.W[i] = 0.0;
+    w.uvdw[i] = 0.0;
+    w.ureal[i] = 0.0;
+    return;
+  }
+  double ev;
+  ev_of(iwalk, ev);
+  w.uvdw[i] = w.uvdw[i] + ev;
+  w.W[i] = Wn;
+  const int s2 = atomicAdd(w.n_active2, 1);
+  w.active2[s2] = (int32_t)i;
+  const int64_t cap = w.cap;
+  for (int j = 0; j < m.n_sites; ++j) {
+    double x, y, z;
+    if (j == m.start) {
+      x = w.selx[i];
+      y = w.sely[i];
+      z = w.selz[i];
+    } else {
+      int p = 0;
+      for (int q = 0; q < m.n_place; ++q)
+        if (m.place[q] == j) p = q;
+      const int64_t o = p * cap_or + slot * ko + iwalk;
+      x = w.orx[o];
+      y = w.ory[o];
+      z = w.orz[o];
+    }
+    w.cfx[j * cap + s2] = x;
+    w.cfy[j * cap + s2] = y;
+    w.cfz[j * cap + s2] = z;
+  }
+}
+
+// GrowMolecule tail and WidomAdsorbateMove bookkeeping
+// (src/cbmc.c:3133-3185, src/mc_moves.c:7878-7933)
+__global__ __launch_bounds__(256) void k_widom_finalize(const WidomArgs a, const WidomWork w) {
+#pragma clang fp contract(off)
+  const int64_t s2 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s2 >= *w.n_active2) return;
+  const int64_t i = w.active2[s2];
+  const MolDev &m = a.mol;
+  const double beta = a.beta;
+  double ucc = w.ureal[i];
+  if (m.n_sites > 1) {
+    double ec = 0.0;
+    for (int p = 0; p < m.n_place; ++p) ec += w.cc_place ? w.cc_place[p * w.cap + s2] : 0.0;
+    ucc = ucc + ec;
+  }
+  double W = w.W[i];
+  W *= exp(-beta * (ucc + 0.0 + 0.0));
+  double fh = 0.0, fa = 0.0;
+  if (a.ewald) {
+    fh = w.f_host[s2];
+    fa = w.f_ads[s2];
+    W *= exp(-beta * (fh + fa + 0.0));
+  }
+  const double uv = w.uvdw[i];
+  w.ureal[i] = ucc;
+  w.ufh[i] = fh;
+  w.ufa[i] = fa;
+  w.W[i] = W;
+  w.dU[i] = uv + ucc + fh + fa;
+}
+
+__global__ __launch_bounds__(256) void k_widom_records(const int64_t n, const WidomWork w,
+                                                       double *__restrict__ rec) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double *r = rec + i * 14;
+  const int stg = w.stage[i];
+  const bool acc = stg == 4;
+  r[0] = acc ? w.W[i] : 0.0;
+  r[1] = acc ? w.dU[i] : 0.0;
+  r[2] = acc ? w.uvdw[i] : 0.0;
+  r[3] = acc ? w.ureal[i] : 0.0;
+  r[4] = acc ? w.ufh[i] : 0.0;
+  r[5] = acc ? w.ufa[i] : 0.0;
+  r[6] = acc ? 0.0 : 1.0;
+  r[7] = (stg >= 1) ? (double)w.selfb[i] : 0.0;
+  r[8] = (stg >= 3) ? (double)w.selor[i] : 0.0;
+  r[9] = (double)w.fbmask[i];
+  r[10] = (double)w.ormask[i];
+  r[11] = (stg >= 1) ? w.W1[i] : 0.0;
+  r[12] = (stg >= 3) ? w.W2[i] : 0.0;
+  r[13] = (double)stg;
+}
+
+__global__ void k_zero2(int32_t *a, int32_t *b) {
+  if (threadIdx.x == 0) {
+    *a = 0;
+    *b = 0;
+  }
+}
+
+void launch_zero_counters(int32_t *a, int32_t *b, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero2, dim3(1), dim3(64), 0, st, a, b);
+}
+
+static unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+void launch_widom_fb_gen(const WidomArgs &a, const WidomWork &w, hipStream_t st) {
+  if (a.n > 0) hipLaunchKernelGGL(k_widom_fb_gen, dim3(nblk(a.n, 256)), dim3(256), 0, st, a, w);
+}
+void launch_widom_fb_select(const WidomArgs &a, const WidomWork &w, hipStream_t st) {
+  if (a.n > 0) hipLaunchKernelGGL(k_widom_fb_select, dim3(nblk(a.n, 256)), dim3(256), 0, st, a, w);
+}
+void launch_widom_or_select(const WidomArgs &a, const WidomWork &w, hipStream_t st) {
+  if (a.n > 0) hipLaunchKernelGGL(k_widom_or_select, dim3(nblk(a.n, 256)), dim3(256), 0, st, a, w);
+}
+void launch_widom_finalize(const WidomArgs &a, const WidomWork &w, hipStream_t st) {
+  if (a.n > 0) hipLaunchKernelGGL(k_widom_finalize, dim3(nblk(a.n, 256)), dim3(256), 0, st, a, w);
+}
+void launch_widom_records(const WidomArgs &a, const WidomWork &w, double *rec, hipStream_t st) {
+  if (a.n > 0) hipLaunchKernelGGL(k_widom_records, dim3(nblk(a.n, 256)), dim3(256), 0, st, a.n, w, rec);
+}
+
+// ====================================================================
+// deterministic block reduction: segments of `seg` insertions, each reduced
+// by one workgroup with a fixed tree; a single workgroup then reduces the
+// segment partials in order.
+// ====================================================================
+__global__ __launch_bounds__(256) void k_block_part(const double *__restrict__ W,
+                                                    const double *__restrict__ dU, int64_t n,
+                                                    const int64_t *__restrict__ bounds, int seg,
+                                                    double *__restrict__ part) {
+  __shared__ double sh[256][10];
+  const int64_t s0 = (int64_t)blockIdx.x * seg;
+  double acc[10];
+  for (int q = 0; q < 10; ++q) acc[q] = 0.0;
+  for (int64_t t = threadIdx.x; t < seg; t += blockDim.x) {
+    int64_t i = s0 + t;
+    if (i >= n) break;
+    int b = 0;
+    while (b < 4 && i >= bounds[b + 1]) ++b;
+    double w = W[i];
+    acc[2 * b] += w;
+    acc[2 * b + 1] += dU[i] * w;
+  }
+  for (int q = 0; q < 10; ++q) sh[threadIdx.x][q] = acc[q];
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int q = 0; q < 10; ++q) sh[threadIdx.x][q] += sh[threadIdx.x + s][q];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 10; ++q) part[(int64_t)blockIdx.x * 10 + q] = sh[0][q];
+}
+
+__global__ __launch_bounds__(256) void k_block_final(const double *__restrict__ part, int nseg,
+                                                     double *__restrict__ out) {
+  __shared__ double sh[256][10];
+  double acc[10];
+  for (int q = 0; q < 10; ++q) acc[q] = 0.0;
+  for (int s = threadIdx.x; s < nseg; s += blockDim.x)
+    for (int q = 0; q < 10; ++q) acc[q] += part[(int64_t)s * 10 + q];
+  for (int q = 0; q < 10; ++q) sh[threadIdx.x][q] = acc[q];
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int q = 0; q < 10; ++q) sh[threadIdx.x][q] += sh[threadIdx.x + s][q];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 10; ++q) out[q] = sh[0][q];
+}
+
+void launch_block_reduce(const double *W, const double *dU, int64_t n, const int64_t bounds_dev[6],
+                         double *part, int seg, double *out10, hipStream_t st) {
+  int nseg = (int)((n + seg - 1) / seg);
+  if (nseg <= 0) return;
+  hipLaunchKernelGGL(k_block_part, dim3(nseg), dim3(256), 0, st, W, dU, n, bounds_dev, seg, part);
+  hipLaunchKernelGGL(k_block_final, dim3(1), dim3(256), 0, st, part, nseg, out10);
+}
+
+// ====================================================================
+// grids: MakeGrid node values (src/grids.c:530-583) from
+// CalculateDerivativesAtPositionVDW (src/framework_hessian.c:63-109) and
+// PotentialThirdDerivative LJ (src/potentials.c:10978-10994), reference
+// operation order, FP64.
+// ====================================================================
+template <int BCK>
+__device__ __forceinline__ void derivs_at(const GridArgs &g, double px, double py, double pz,
+                                          double o[14]) {
+#pragma clang fp contract(off)
+  double value = 0, fx = 0, fy = 0, fz = 0, hax = 0, hbx = 0, hcx = 0, hay = 0, hby = 0, hcy = 0,
+         haz = 0, hbz = 0, hcz = 0, third = 0;
+  for (int j = 0; j < g.n_atoms; ++j) {
+    const AtomRec64 r = g.a64[j];
+    double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
+    mi_exact<BCK>(g.cell, dx, dy, dz);
+    double rr = dx * dx + dy * dy + dz * dz;
+    if (rr < g.rc2) {
+      double t = r.s2 / rr;
+      double rri3 = (t * t) * t;
+      double F = r.c4 * (rri3 * (rri3 - 1.0)) - r.shift;
+      double DF = g.c24[j] * (rri3 * (1.0 - 2.0 * rri3)) / rr;
+      double DDF = g.c96[j] * (rri3 * (7.0 * rri3 - 2.0)) / (rr * rr);
+      double DDDF = g.c384[j] * (rri3 * (5.0 - 28.0 * rri3)) / ((rr * rr) * (rr * rr));
+      value += F;
+      fx += dx * DF;
+      fy += dy * DF;
+      fz += dz * DF;
+      hax += DDF * dx * dx + DF;
+      hbx += DDF * dy * dx;
+      hcx += DDF * dz * dx;
+      hay += DDF * dx * dy;
+      hby += DDF * dy * dy + DF;
+      hcy += DDF * dz * dy;
+      haz += DDF * dx * dz;
+      hbz += DDF * dy * dz;
+      hcz += DDF * dz * dz + DF;
+      third += DDDF * dx * dy * dz;
+    }
+  }
+  o[0] = value;
+  o[1] = fx;
+  o[2] = fy;
+  o[3] = fz;
+  o[4] = hax;
+  o[5] = hay;
+  o[6] = haz;
+  o[7] = hbx;
+  o[8] = hby;
+  o[9] = hbz;
+  o[10] = hcx;
+  o[11] = hcy;
+  o[12] = hcz;
+  o[13] = third;
+}
+
+template <int BCK>
+__global__ __launch_bounds__(256) void k_grid_build(const GridArgs g) {
+#pragma clang fp contract(off)
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.n_nodes) return;
+  const int64_t node = g.node0 + t;
+  const int64_t nz = g.n[2] + 1, ny = g.n[1] + 1;
+  const int k = (int)(node % nz);
+  const int j = (int)((node / nz) % ny);
+  const int i = (int)(node / (nz * ny));
+  double px = i * g.size[0] / g.n[0] + g.shift[0];
+  double py = j * g.size[1] / g.n[1] + g.shift[1];
+  double pz = k * g.size[2] / g.n[2] + g.shift[2];
+  double d[14];
+  derivs_at<BCK>(g, px, py, pz, d);
+  double value = d[0], f0 = d[1], f1 = d[2], f2 = d[3], hay = d[5], haz = d[6], hbz = d[9],
+         third = d[13];
+  if (value > kOverlap) {
+    value = 2.0 * kOverlap;
+    f0 = fmin(fmax(f0, -kOverlap), kOverlap);
+    f1 = fmin(fmax(f1, -kOverlap), kOverlap);
+    f2 = fmin(fmax(f2, -kOverlap), kOverlap);
+    hay = haz = hbz = 0.0;
+    third = 0.0;
+  }
+  const double dx = g.delta[0], dy = g.delta[1], dz = g.delta[2];
+  float c[8];
+  c[0] = (float)value;
+  c[1] = (float)((double)(float)f0 * dx);
+  c[2] = (float)((double)(float)f1 * dy);
+  c[3] = (float)((double)(float)f2 * dz);
+  c[4] = (float)((double)(float)hay * (dx * dy));
+  c[5] = (float)((double)(float)haz * (dx * dz));
+  c[6] = (float)((double)(float)hbz * (dy * dz));
+  c[7] = (float)((double)(float)third * (dx * dy * dz));
+  if (g.fp64) {
+    double *o = (double *)g.out + node * 8;
+    o[0] = value;
+    o[1] = f0 * dx;
+    o[2] = f1 * dy;
+    o[3] = f2 * dz;
+    o[4] = hay * (dx * dy);
+    o[5] = haz * (dx * dz);
+    o[6] = hbz * (dy * dz);
+    o[7] = third * (dx * dy * dz);
+  } else {
+    float4 *o = (float4 *)((float *)g.out + node * 8);
+    o[0] = make_float4(c[0], c[1], c[2], c[3]);
+    o[1] = make_float4(c[4], c[5], c[6], c[7]);
+  }
+}
+
+template <int BCK>
+__global__ __launch_bounds__(256) void k_derivs(const GridArgs g, const double *__restrict__ xyz,
+                                                int64_t n, double *__restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if ((int64_t)blockIdx.x * blockDim.x >= n) return;
+  const int64_t m = t < n ? t : 0;
+  double d[14];
+  derivs_at<BCK>(g, xyz[3 * m], xyz[3 * m + 1], xyz[3 * m + 2], d);
+  if (t < n)
+    for (int q = 0; q < 14; ++q) out[t * 14 + q] = d[q];
+}
+
+void launch_grid_build(const GridArgs &g, hipStream_t st) {
+  unsigned nb = nblk(g.n_nodes, 256);
+  if (!nb) return;
+  if (g.cell.bc == 1)
+    hipLaunchKernelGGL(k_grid_build<1>, dim3(nb), dim3(256), 0, st, g);
+  else if (g.cell.upper)
+    hipLaunchKernelGGL(k_grid_build<2>, dim3(nb), dim3(256), 0, st, g);
+  else
+    hipLaunchKernelGGL(k_grid_build<3>, dim3(nb), dim3(256), 0, st, g);
+}
+
+void launch_derivatives(const GridArgs &g, const double *xyz, int64_t n, double *out,
+                        hipStream_t st) {
+  unsigned nb = nblk(n, 256);
+  if (!nb) return;
+  if (g.cell.bc == 1)
+    hipLaunchKernelGGL(k_derivs<1>, dim3(nb), dim3(256), 0, st, g, xyz, n, out);
+  else if (g.cell.upper)
+    hipLaunchKernelGGL(k_derivs<2>, dim3(nb), dim3(256), 0, st, g, xyz, n, out);
+  else
+    hipLaunchKernelGGL(k_derivs<3>, dim3(nb), dim3(256), 0, st, g, xyz, n, out);
+}
+
+// ====================================================================
+// InterpolateVDWGrid (src/grids.c:979-1077).  The Lekien-Marsden tricubic
+// with corner data (f, fx, fy, fz, fxy, fxz, fyz, fxyz) equals the tensor
+// product of 1D cubic Hermite bases, evaluated here directly (64 products)
+// instead of the 64x64 matrix-vector product.
+// ====================================================================
+__device__ __forceinline__ void hermite(double t, double h[2][2]) {
+  double t2 = t * t, t3 = t2 * t;
+  h[0][0] = 2.0 * t3 - 3.0 * t2 + 1.0; // value at corner 0
+  h[0][1] = t3 - 2.0 * t2 + t;         // derivative at corner 0
+  h[1][0] = -2.0 * t3 + 3.0 * t2;      // value at corner 1
+  h[1][1] = t3 - t2;                   // derivative at corner 1
+}
+
+__global__ __launch_bounds__(256) void k_interp(const InterpArgs a, const double *__restrict__ xyz,
+                                                int64_t n, double *__restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double pos[3] = {xyz[3 * t], xyz[3 * t + 1], xyz[3 * t + 2]};
+  {
+#pragma clang fp contract(off)
+    if (a.bc == 1) {
+      for (int d = 0; d < 3; ++d) {
+        pos[d] -= a.ulen[d] * nint_d(pos[d] / a.ulen[d]);
+        if (pos[d] < 0.0) pos[d] += a.ulen[d];
+      }
+    } else {
+      double s[3];
+      s[0] = a.uinv[AX] * pos[0] + a.uinv[BX] * pos[1] + a.uinv[CX] * pos[2];
+      s[1] = a.uinv[AY] * pos[0] + a.uinv[BY] * pos[1] + a.uinv[CY] * pos[2];
+      s[2] = a.uinv[AZ] * pos[0] + a.uinv[BZ] * pos[1] + a.uinv[CZ] * pos[2];
+      for (int d = 0; d < 3; ++d) {
+        s[d] -= nint_d(s[d]);
+        if (s[d] < 0.0) s[d] += 1.0;
+      }
+      pos[0] = a.ubox[AX] * s[0] + a.ubox[BX] * s[1] + a.ubox[CX] * s[2];
+      pos[1] = a.ubox[AY] * s[0] + a.ubox[BY] * s[1] + a.ubox[CY] * s[2];
+      pos[2] = a.ubox[AZ] * s[0] + a.ubox[BZ] * s[1] + a.ubox[CZ] * s[2];
+    }
+  }
+  double sv[3];
+  int c0[3];
+  double r[3];
+  for (int d = 0; d < 3; ++d) {
+    sv[d] = ((pos[d] - a.shift[d]) / a.size[d]) * (double)a.n[d];
+    c0[d] = (int)sv[d];
+    if (c0[d] < 0) c0[d] = 0;
+    if (c0[d] > a.n[d] - 1) c0[d] = a.n[d] - 1;
+    r[d] = sv[d] - c0[d];
+  }
+  const int64_t ny = a.n[1] + 1, nz = a.n[2] + 1;
+  double X[8][8]; // [corner][channel]
+  for (int c = 0; c < 8; ++c) {
+    const int64_t node = ((int64_t)(c0[0] + (c & 1)) * ny + (c0[1] + ((c >> 1) & 1))) * nz +
+                         (c0[2] + ((c >> 2) & 1));
+    if (a.fp64) {
+      const double *p = (const double *)a.grid + node * 8;
+      for (int q = 0; q < 8; ++q) X[c][q] = p[q];
+    } else {
+      const float4 *p = (const float4 *)((const float *)a.grid + node * 8);
+      float4 v0 = p[0], v1 = p[1];
+      X[c][0] = v0.x; X[c][1] = v0.y; X[c][2] = v0.z; X[c][3] = v0.w;
+      X[c][4] = v1.x; X[c][5] = v1.y; X[c][6] = v1.z; X[c][7] = v1.w;
+    }
+  }
+  for (int c = 0; c < 8; ++c)
+    if (X[c][0] > 1e10) {
+      out[t] = 1e8;
+      return;
+    }
+  double hx[2][2], hy[2][2], hz[2][2];
+  hermite(r[0], hx);
+  hermite(r[1], hy);
+  hermite(r[2], hz);
+  // channel -> derivative orders (dx, dy, dz)
+  const int dd[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
+                        {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+  double v = 0.0;
+  for (int c = 0; c < 8; ++c) {
+    const int cx0 = c & 1, cy0 = (c >> 1) & 1, cz0 = (c >> 2) & 1;
+    for (int q = 0; q < 8; ++q)
+      v += X[c][q] * (hx[cx0][dd[q][0]] * hy[cy0][dd[q][1]] * hz[cz0][dd[q][2]]);
+  }
+  out[t] = v;
+}
+
+void launch_interp(const InterpArgs &a, const double *xyz, int64_t n, double *out, hipStream_t st) {
+  unsigned nb = nblk(n, 256);
+  if (nb) hipLaunchKernelGGL(k_interp, dim3(nb), dim3(256), 0, st, a, xyz, n, out);
+}
+
+} // namespace rhg

@@ -1,0 +1,345 @@
+"""ctypes binding of librhg.so (include/rhg.h).
+
+The names mirror the RASPA2 routines each entry point replaces
+(src/framework_energy.h:43-46, src/ewald.h:86-94, src/grids.h:65-70,
+src/framework_hessian.h:43).  Every call raises RhgError with the library's
+message when the C ABI returns a non-zero status.  There is no CPU fallback:
+if librhg.so is missing the import of this module fails.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librhg.so")
+
+BC_RECTANGULAR = 1
+BC_TRICLINIC = 2
+CHARGE_NONE = 0
+CHARGE_EWALD = 1
+PREC_FP64 = 0
+PREC_MIXED = 1
+MAX_SITES = 8
+NR_BLOCKS = 5
+REC_N = 14
+REC = dict(W=0, DU=1, UVDW=2, UREAL=3, UFHOST=4, UFADS=5, OVERLAP=6, SELFB=7, SELOR=8,
+           FBMASK=9, ORMASK=10, W1=11, W2=12, STAGE=13)
+
+_D = ctypes.c_double
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_U64 = ctypes.c_uint64
+_P = ctypes.c_void_p
+_PD = ctypes.POINTER(ctypes.c_double)
+
+
+class RhgError(RuntimeError):
+    pass
+
+
+class Framework(ctypes.Structure):
+    _fields_ = [("n_atoms", _I32), ("x", _PD), ("y", _PD), ("z", _PD), ("charge", _PD),
+                ("eps_k", _PD), ("sigma", _PD), ("mass", _PD)]
+
+
+class CellT(ctypes.Structure):
+    _fields_ = [("box", _D * 9), ("boundary", _I32)]
+
+
+class ForceField(ctypes.Structure):
+    _fields_ = [("cutoff_vdw", _D), ("shift_vdw", _I32), ("charge_method", _I32),
+                ("cutoff_charge", _D), ("ewald_precision", _D), ("temperature", _D)]
+
+
+class MoleculeT(ctypes.Structure):
+    _fields_ = [("n_sites", _I32), ("type", _I32 * MAX_SITES), ("body", (_D * 3) * MAX_SITES),
+                ("start", _I32), ("k_fb", _I32), ("k_or", _I32)]
+
+
+class WidomBlocks(ctypes.Structure):
+    _fields_ = [("count", _D * NR_BLOCKS), ("sum_w", _D * NR_BLOCKS), ("sum_wdu", _D * NR_BLOCKS),
+                ("ideal_gas", _D * NR_BLOCKS)]
+
+    def as_array(self):
+        """5 x 4 array (count, sum_w, sum_wdu, ideal_gas) per block."""
+        return np.array([[self.count[b], self.sum_w[b], self.sum_wdu[b], self.ideal_gas[b]]
+                         for b in range(NR_BLOCKS)])
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"librhg.so not built at {LIB_PATH}; run python -m raspa2_amd.build")
+    L = ctypes.CDLL(LIB_PATH)
+
+    def f(name, *args, res=ctypes.c_int):
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = list(args)
+
+    f("rhg_version", res=ctypes.c_char_p)
+    f("rhg_last_error", _P, res=ctypes.c_char_p)
+    f("rhg_create", ctypes.POINTER(Framework), ctypes.POINTER(CellT), ctypes.POINTER(ForceField),
+      ctypes.c_int, ctypes.POINTER(_P))
+    f("rhg_destroy", _P, res=None)
+    f("rhg_add_pseudo_atom", _P, _D, _D, _D, ctypes.POINTER(_I32))
+    f("rhg_set_precision", _P, _I32)
+    f("rhg_footprint", _P, ctypes.POINTER(ctypes.c_size_t))
+    f("rhg_info", _P, _P)
+    f("rhg_framework_vdw_energy_at_position", _P, _P, _I32, _D, ctypes.POINTER(_D))
+    f("rhg_framework_charge_energy_at_position", _P, _P, _I32, _D, ctypes.POINTER(_D),
+      ctypes.POINTER(_D))
+    f("rhg_framework_energy_at_positions", _P, _I64, _P, _I32, _P, _P, _P)
+    f("rhg_ewald_tables", _P, _P, _P, _P)
+    f("rhg_ewald_fourier_adsorbate", _P, _I64, _I32, _P, _P, _P, _P)
+    f("rhg_widom_run", _P, ctypes.POINTER(MoleculeT), _U64, _I64, _I64, _I64, _I32,
+      ctypes.POINTER(WidomBlocks), _P)
+    f("rhg_widom_run_trials", _P, ctypes.POINTER(MoleculeT), _I64, _P, _P, _P, _P)
+    f("rhg_widom_estimators", ctypes.POINTER(WidomBlocks), _D, _D, _D, _P)
+    f("rhg_make_grid", _P, _I32, _P, _D, _I32, ctypes.POINTER(_P))
+    f("rhg_grid_info", _P, _P)
+    f("rhg_grid_download", _P, _P, ctypes.c_size_t)
+    f("rhg_interpolate_vdw_grid", _P, _I64, _P, _P)
+    f("rhg_grid_free", _P, res=None)
+    f("rhg_derivatives_at_positions_vdw", _P, _I32, _I64, _P, _P)
+    f("rhg_profile_enable", _P, _I32)
+    f("rhg_profile_get", _P, ctypes.c_char_p, ctypes.POINTER(_I64), ctypes.POINTER(_D))
+    f("rhg_profile_reset", _P)
+    f("rhg_synchronize", _P)
+    f("rhg_synthetic_positions", _U64, _P, _I32, _D, _I64, _P, ctypes.POINTER(_I32))
+    return L
+
+
+lib = _load()
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P) if a is not None else None
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_PD) if a is not None else None
+
+
+def version():
+    return lib.rhg_version().decode()
+
+
+def molecule(types, body, start=0, k_fb=10, k_or=10):
+    m = MoleculeT()
+    m.n_sites = len(types)
+    for i, t in enumerate(types):
+        m.type[i] = int(t)
+        for d in range(3):
+            m.body[i][d] = float(body[i][d])
+    m.start = int(start)
+    m.k_fb = int(k_fb)
+    m.k_or = int(k_or)
+    return m
+
+
+def widom_estimators(blocks, temperature, density, ideal_gas_weight=1.0):
+    out = np.zeros(16)
+    rc = lib.rhg_widom_estimators(ctypes.byref(blocks), temperature, density, ideal_gas_weight,
+                                  _ptr(out))
+    if rc:
+        raise RhgError(f"rhg_widom_estimators: {rc}")
+    return dict(kh_blocks=out[0:5], kh=out[5], kh_err=out[6], u_blocks=out[7:12], u=out[12],
+                u_err=out[13], kh_overall=out[14], w_overall=out[15])
+
+
+class Grid:
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.h = handle
+        info = np.zeros(13)
+        lib.rhg_grid_info(self.h, _ptr(info))
+        self.n = info[0:3].astype(int)
+        self.size, self.shift, self.delta = info[3:6], info[6:9], info[9:12]
+        self.node_bytes = int(info[12])
+        self.geom = info[:12].copy()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.rhg_grid_free(self.h)
+            self.h = None
+
+    @property
+    def shape(self):
+        return (self.n[0] + 1, self.n[1] + 1, self.n[2] + 1, 8)
+
+    def download(self):
+        dt = np.float64 if self.node_bytes == 64 else np.float32
+        out = np.zeros(self.shape, dtype=dt)
+        self.ctx._check(lib.rhg_grid_download(self.h, _ptr(out), out.nbytes), "grid_download")
+        return out
+
+    def interpolate(self, xyz):
+        """InterpolateVDWGrid (src/grids.c:979) at many positions."""
+        xyz = _f64(xyz).reshape(-1, 3)
+        u = np.zeros(len(xyz))
+        self.ctx._check(lib.rhg_interpolate_vdw_grid(self.h, len(xyz), _ptr(xyz), _ptr(u)),
+                        "interpolate_vdw_grid")
+        return u
+
+
+class Context:
+    """One framework resident on one GPU (rhg_ctx)."""
+
+    def __init__(self, box, bc, x, y, z, eps_k, sigma, charge=None, mass=None, cutoff_vdw=12.0,
+                 shifted=True, charge_method=CHARGE_NONE, cutoff_charge=12.0,
+                 ewald_precision=1e-6, temperature=300.0, device=0, precision=PREC_FP64):
+        self._arrs = [_f64(x), _f64(y), _f64(z), _f64(eps_k), _f64(sigma),
+                      _f64(charge) if charge is not None else None,
+                      _f64(mass) if mass is not None else None]
+        xx, yy, zz, ee, ss, qq, mm = self._arrs
+        fw = Framework(len(xx), _dptr(xx), _dptr(yy), _dptr(zz), _dptr(qq), _dptr(ee), _dptr(ss),
+                       _dptr(mm))
+        cell = CellT()
+        for i, v in enumerate(np.asarray(box, dtype=np.float64).reshape(9)):
+            cell.box[i] = v
+        cell.boundary = bc
+        ff = ForceField(cutoff_vdw, 1 if shifted else 0, charge_method, cutoff_charge,
+                        ewald_precision, temperature)
+        h = _P()
+        rc = lib.rhg_create(ctypes.byref(fw), ctypes.byref(cell), ctypes.byref(ff), device,
+                            ctypes.byref(h))
+        if rc:
+            raise RhgError(f"rhg_create failed: {rc}")
+        self.h = h
+        self.temperature = temperature
+        self.n_atoms = len(xx)
+        self.set_precision(precision)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.rhg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc:
+            msg = lib.rhg_last_error(self.h)
+            raise RhgError(f"{what}: status {rc}: {msg.decode() if msg else ''}")
+
+    def add_pseudo_atom(self, eps_k, sigma, charge=0.0):
+        t = _I32()
+        self._check(lib.rhg_add_pseudo_atom(self.h, eps_k, sigma, charge, ctypes.byref(t)),
+                    "add_pseudo_atom")
+        return t.value
+
+    def set_precision(self, p):
+        self._check(lib.rhg_set_precision(self.h, p), "set_precision")
+
+    def info(self):
+        out = np.zeros(10)
+        self._check(lib.rhg_info(self.h, _ptr(out)), "info")
+        return dict(alpha=out[0], kx=int(out[1]), ky=int(out[2]), kz=int(out[3]), nk=int(out[4]),
+                    uion=out[5], beta=out[6], volume=out[7], mass=out[8], density=out[9])
+
+    def footprint(self):
+        b = ctypes.c_size_t()
+        self._check(lib.rhg_footprint(self.h, ctypes.byref(b)), "footprint")
+        return b.value
+
+    # ---- src/framework_energy.c:60,176
+    def CalculateFrameworkVDWEnergyAtPosition(self, pos, type_a, scaling=1.0):
+        p = _f64(pos)
+        u = _D()
+        self._check(lib.rhg_framework_vdw_energy_at_position(self.h, _ptr(p), type_a, scaling,
+                                                               ctypes.byref(u)), "vdw_at")
+        return u.value
+
+    def CalculateFrameworkChargeEnergyAtPosition(self, pos, type_a, scaling=1.0):
+        p = _f64(pos)
+        ucc, ucbd = _D(), _D()
+        self._check(lib.rhg_framework_charge_energy_at_position(
+            self.h, _ptr(p), type_a, scaling, ctypes.byref(ucc), ctypes.byref(ucbd)), "charge_at")
+        return ucc.value, ucbd.value
+
+    def energy_at_positions(self, xyz, type_a):
+        xyz = _f64(xyz).reshape(-1, 3)
+        n = len(xyz)
+        uv, uc = np.zeros(n), np.zeros(n)
+        ov = np.zeros(n, dtype=np.uint8)
+        self._check(lib.rhg_framework_energy_at_positions(self.h, n, _ptr(xyz), type_a, _ptr(uv),
+                                                          _ptr(uc), _ptr(ov)), "energy_at_positions")
+        return uv, uc, ov.astype(bool)
+
+    # ---- src/ewald.c
+    def ewald_tables(self):
+        nk = self.info()["nk"]
+        kf, sr, si = np.zeros(nk), np.zeros(nk), np.zeros(nk)
+        self._check(lib.rhg_ewald_tables(self.h, _ptr(kf), _ptr(sr), _ptr(si)), "ewald_tables")
+        return dict(kfactor=kf, sf_re=sr, sf_im=si)
+
+    def CalculateEwaldFourierAdsorbate(self, types, xyz):
+        """xyz: n_mol x n_sites x 3; returns (d_host, d_ads) arrays."""
+        types = np.ascontiguousarray(types, dtype=np.int32)
+        xyz = _f64(xyz).reshape(-1, len(types), 3)
+        n = len(xyz)
+        dh, da = np.zeros(n), np.zeros(n)
+        self._check(lib.rhg_ewald_fourier_adsorbate(self.h, n, len(types), _ptr(types), _ptr(xyz),
+                                                    _ptr(dh), _ptr(da)), "fourier_adsorbate")
+        return dh, da
+
+    # ---- src/mc_moves.c:7844 WidomAdsorbateMove
+    def widom_run(self, mol, seed, n_total, first=0, n=None, steps_per_cycle=20, blocks=None,
+                  records=False):
+        n = n_total - first if n is None else n
+        blocks = WidomBlocks() if blocks is None else blocks
+        rec = np.zeros((n, REC_N)) if records else None
+        self._check(lib.rhg_widom_run(self.h, ctypes.byref(mol), seed, n_total, first, n,
+                                      steps_per_cycle, ctypes.byref(blocks), _ptr(rec)),
+                    "widom_run")
+        return blocks, rec
+
+    def widom_run_trials(self, mol, fb, cord, usel):
+        fb = _f64(fb)
+        n = fb.shape[0]
+        cord = _f64(cord) if cord is not None else None
+        usel = _f64(usel)
+        rec = np.zeros((n, REC_N))
+        self._check(lib.rhg_widom_run_trials(self.h, ctypes.byref(mol), n, _ptr(fb), _ptr(cord),
+                                             _ptr(usel), _ptr(rec)), "widom_run_trials")
+        return rec
+
+    # ---- src/grids.c
+    def MakeGrid(self, type_a, unit_box, spacing, fp64=False):
+        ub = _f64(unit_box).reshape(9)
+        h = _P()
+        self._check(lib.rhg_make_grid(self.h, type_a, _ptr(ub), spacing, 1 if fp64 else 0,
+                                      ctypes.byref(h)), "make_grid")
+        return Grid(self, h)
+
+    def CalculateDerivativesAtPositionVDW(self, xyz, type_a):
+        xyz = _f64(xyz).reshape(-1, 3)
+        out = np.zeros((len(xyz), 14))
+        self._check(lib.rhg_derivatives_at_positions_vdw(self.h, type_a, len(xyz), _ptr(xyz),
+                                                         _ptr(out)), "derivatives")
+        return out
+
+    # ---- profiling
+    def profile(self, on=True):
+        self._check(lib.rhg_profile_enable(self.h, 1 if on else 0), "profile_enable")
+
+    def profile_get(self, name):
+        n, ms = _I64(), _D()
+        self._check(lib.rhg_profile_get(self.h, name.encode(), ctypes.byref(n), ctypes.byref(ms)),
+                    "profile_get")
+        return n.value, ms.value
+
+    def profile_reset(self):
+        self._check(lib.rhg_profile_reset(self.h), "profile_reset")
+
+    def synchronize(self):
+        self._check(lib.rhg_synchronize(self.h), "synchronize")

@@ -1,0 +1,40 @@
+"""Builders shared by the tests: the same framework on the oracle and on the GPU."""
+import numpy as np
+
+import oracle as O
+from raspa2_amd import rhg as R
+from raspa2_amd import workloads as W
+
+
+def oracle_system(fw, charge_method=O.CHARGE_NONE, cutoff=12.0, precision=1e-6, T=300.0):
+    box, bc, x, y, z, q, eps, sig, mass = fw.oracle_args()
+    return O.System(box, bc, x, y, z, q, eps, sig, mass=mass, cutoff_vdw=cutoff, shifted=True,
+                    charge_method=charge_method, cutoff_charge=cutoff,
+                    ewald_precision=precision, temperature=T)
+
+
+def oracle_molecule(sys_o, probe, k_fb=None, k_or=None):
+    types = [sys_o.add_pseudo_atom(e, s, q) for e, s, q in probe.sites]
+    return types, O.molecule(types, probe.body, start=probe.start,
+                             k_fb=probe.k_fb if k_fb is None else k_fb,
+                             k_or=probe.k_or if k_or is None else k_or)
+
+
+def gpu_molecule(ctx, probe, k_fb=None, k_or=None):
+    types = [ctx.add_pseudo_atom(e, s, q) for e, s, q in probe.sites]
+    return types, R.molecule(types, probe.body, start=probe.start,
+                             k_fb=probe.k_fb if k_fb is None else k_fb,
+                             k_or=probe.k_or if k_or is None else k_or)
+
+
+def random_positions(box, n, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    s = rng.uniform(0.0, 1.0, (n, 3))
+    m = np.asarray(box).reshape(3, 3)  # rows a, b, c
+    return s @ m
+
+
+def same_bits(a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64))

@@ -1,0 +1,78 @@
+"""Energies at positions on the GPU against the oracle
+(src/framework_energy.c:60-311)."""
+import numpy as np
+import pytest
+
+from helpers import oracle_system, random_positions, same_bits
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(W, O, R, fw, charge_method, precision, probe_sites):
+    sys_o = oracle_system(fw, charge_method=charge_method)
+    ctx = fw.context(charge_method=charge_method, precision=precision)
+    to = [sys_o.add_pseudo_atom(*s) for s in probe_sites]
+    tg = [ctx.add_pseudo_atom(*s) for s in probe_sites]
+    return sys_o, ctx, to, tg
+
+
+@pytest.mark.parametrize("which", ["c0", "c2"])
+def test_vdw_fp64_bit_exact(W, O, R, which):
+    """Plain FP64 LJ energies equal the sequential reference loop bit for bit
+    (rectangular C0 cell and triclinic C2 cell)."""
+    fw = W.c0_framework() if which == "c0" else W.c2_framework()
+    sys_o, ctx, to, tg = _pair(W, O, R, fw, O.CHARGE_NONE, R.PREC_FP64, [(148.0, 3.73, 0.0)])
+    xyz = random_positions(fw.box, 3000, 7)
+    uv, uc, ov = ctx.energy_at_positions(xyz, tg[0])
+    uv_o, _ = sys_o.energy_at_many(to[0], xyz)
+    assert same_bits(uv, uv_o)
+    assert np.array_equal(ov, uv_o >= 1e7)
+    assert np.all(uc == 0.0)
+
+
+@pytest.mark.parametrize("which", ["c0", "c2"])
+def test_vdw_mixed_tolerance_and_flags(W, O, R, which):
+    """Mixed precision: overlap flags identical, relative error <= 1e-5 on
+    non-overlapping sites (|U| > 1 K), absolute <= 1e-3 K otherwise."""
+    fw = W.c0_framework() if which == "c0" else W.c2_framework()
+    sys_o, ctx, to, tg = _pair(W, O, R, fw, O.CHARGE_NONE, R.PREC_MIXED, [(148.0, 3.73, 0.0)])
+    xyz = random_positions(fw.box, 3000, 8)
+    uv, _, ov = ctx.energy_at_positions(xyz, tg[0])
+    uv_o, _ = sys_o.energy_at_many(to[0], xyz)
+    assert np.array_equal(ov, uv_o >= 1e7)
+    ok = uv_o < 1e7
+    e2k = O.units()["energy_to_kelvin"]
+    err = np.abs(uv[ok] - uv_o[ok]) * e2k
+    big = np.abs(uv_o[ok]) * e2k > 1.0
+    assert np.all(err[big] / (np.abs(uv_o[ok][big]) * e2k) <= 1e-5)
+    assert np.all(err[~big] <= 1e-3)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_charge_energy(W, O, R, precision):
+    """Real-space Ewald energy of a charged bead in the 2x2x2 C1 supercell."""
+    fw = W.c1_framework()
+    sys_o, ctx, to, tg = _pair(W, O, R, fw, O.CHARGE_EWALD, precision, [(27.0, 2.80, 0.70)])
+    xyz = random_positions(fw.box, 1500, 9)
+    uv, uc, ov = ctx.energy_at_positions(xyz, tg[0])
+    uv_o, uc_o = sys_o.energy_at_many(to[0], xyz)
+    ok = (uv_o < 1e7) & (uc_o < 1e7)
+    assert np.array_equal(ov, ~ok)
+    scale = np.maximum(np.abs(uc_o[ok]), 1.0)
+    tol = 1e-11 if precision == 0 else 2e-5
+    assert np.max(np.abs(uc[ok] - uc_o[ok]) / scale) <= tol
+    if precision == 0:
+        assert same_bits(uv, uv_o)
+
+
+def test_scalar_shims(W, O, R):
+    """The legacy call shapes (one position per call)."""
+    fw = W.c1_framework()
+    sys_o, ctx, to, tg = _pair(W, O, R, fw, O.CHARGE_EWALD, R.PREC_FP64, [(27.0, 2.80, 0.70)])
+    p = random_positions(fw.box, 3, 10)
+    for pos in p:
+        u = ctx.CalculateFrameworkVDWEnergyAtPosition(pos, tg[0])
+        assert u == sys_o.vdw_at(to[0], pos)
+        ucc, ucbd = ctx.CalculateFrameworkChargeEnergyAtPosition(pos, tg[0])
+        assert ucbd == 0.0
+        assert abs(ucc - sys_o.charge_at(to[0], pos)) <= 1e-11 * max(1.0, abs(ucc))

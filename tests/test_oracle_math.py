@@ -1,0 +1,320 @@
+"""Pins of the oracle by what the mathematics fixes: closed forms, invariants,
+brute force on small inputs, textbook constants.  Each test is chosen so that
+a plausible slip (dropped term, wrong sign or index, transposed operand)
+fails it."""
+import math
+
+import numpy as np
+import pytest
+from scipy.special import erf, erfc
+
+from helpers import oracle_system, random_positions
+
+
+def _one_atom_system(O, box, bc, eps=50.0, sig=3.0, q=0.0, cm=0, rc=12.0):
+    return O.System(box, bc, [0.0], [0.0], [0.0], [q], [eps], [sig], cutoff_vdw=rc,
+                    shifted=True, charge_method=cm, cutoff_charge=rc)
+
+
+def test_lj_closed_form(O):
+    """U(r) = 4 eps[(s/r)^12-(s/r)^6] - U(rc) with LB-mixed eps, sigma;
+    zero beyond the cutoff; minimum -eps - U(rc) at 2^(1/6) sigma."""
+    box, inv, props = O.cell_make(30.0, 30.0, 30.0)
+    s = _one_atom_system(O, box, O.BC_RECTANGULAR)
+    t = s.add_pseudo_atom(148.0, 3.73)
+    k2e = O.units()["kelvin_to_energy"]
+    eps = math.sqrt(50.0 * k2e * 148.0 * k2e)
+    sig = 0.5 * (3.0 + 3.73)
+
+    def lj(r):
+        return 4 * eps * ((sig / r) ** 12 - (sig / r) ** 6)
+
+    shift = lj(12.0)
+    for r in (2.5, 3.0, 3.7, 5.0, 8.0, 11.99):
+        u = s.vdw_at(t, [r, 0.0, 0.0])
+        assert abs(u - (lj(r) - shift)) <= 1e-12 * max(1.0, abs(lj(r)))
+    assert s.vdw_at(t, [12.0001, 0, 0]) == 0.0
+    rmin = 2 ** (1 / 6) * sig
+    assert abs(s.vdw_at(t, [0, rmin, 0]) - (-eps - shift)) < 1e-12
+    assert abs(s.vdw_at(t, [0, 0, 11.999999])) < 1e-8 * eps
+
+
+@pytest.mark.parametrize("bc", [1, 2])
+def test_lattice_translation_invariance(O, W, bc):
+    """Energies do not change when the probe moves by any lattice vector
+    (minimum image in rectangular and triclinic form)."""
+    fw = W.c2_framework() if bc == 2 else W.c0_framework()
+    s = oracle_system(fw)
+    t = s.add_pseudo_atom(148.0, 3.73)
+    m = np.asarray(fw.box).reshape(3, 3)
+    p = random_positions(fw.box, 20, 3)
+    for shift in ([1, 0, 0], [0, -1, 0], [0, 0, 2], [1, 1, -1]):
+        d = np.array(shift) @ m
+        for x in p:
+            u0, u1 = s.vdw_at(t, x), s.vdw_at(t, x + d)
+            assert abs(u0 - u1) <= 1e-9 * max(1.0, abs(u0))
+
+
+def test_triclinic_equals_rectangular_on_orthorhombic(O, W):
+    fw = W.c0_framework()
+    s1 = oracle_system(fw)
+    box, bc, x, y, z, q, eps, sig, mass = fw.oracle_args()
+    s2 = O.System(box, O.BC_TRICLINIC, x, y, z, q, eps, sig, mass=mass)
+    t1, t2 = s1.add_pseudo_atom(148.0, 3.73), s2.add_pseudo_atom(148.0, 3.73)
+    for p in random_positions(fw.box, 50, 4):
+        u1, u2 = s1.vdw_at(t1, p), s2.vdw_at(t2, p)
+        assert abs(u1 - u2) <= 1e-9 * max(1.0, abs(u1))
+
+
+def test_minimum_image_brute_force(O):
+    """All periodic images within the cutoff, enumerated explicitly, equal
+    the minimum-image sum when the cutoff is below half the perpendicular
+    width (triclinic cell)."""
+    box, inv, props = O.cell_make(25.0, 26.0, 27.0, 75.0, 100.0, 85.0)
+    rng = np.random.default_rng(2)
+    m = box.reshape(3, 3)
+    n = 40
+    xyz = rng.uniform(0, 1, (n, 3)) @ m
+    eps, sig = rng.uniform(20, 80, n), rng.uniform(2.5, 3.5, n)
+    rc = 11.0
+    assert props[3:6].min() > 2 * rc
+    s = O.System(box, O.BC_TRICLINIC, xyz[:, 0], xyz[:, 1], xyz[:, 2], np.zeros(n), eps, sig,
+                 cutoff_vdw=rc)
+    t = s.add_pseudo_atom(148.0, 3.73)
+    k2e = O.units()["kelvin_to_energy"]
+    shifts = np.array([[i, j, k] for i in (-1, 0, 1) for j in (-1, 0, 1) for k in (-1, 0, 1)]) @ m
+    for p in rng.uniform(0, 1, (10, 3)) @ m:
+        tot = 0.0
+        for a in range(n):
+            e = math.sqrt(eps[a] * k2e * 148.0 * k2e)
+            sg = 0.5 * (sig[a] + 3.73)
+            sh = 4 * e * ((sg / rc) ** 12 - (sg / rc) ** 6)
+            for d in shifts:
+                r = np.linalg.norm(p - (xyz[a] + d))
+                if r < rc:
+                    tot += 4 * e * ((sg / r) ** 12 - (sg / r) ** 6) - sh
+        u = s.vdw_at(t, p)
+        assert abs(u - tot) <= 1e-9 * max(1.0, abs(tot))
+
+
+def _nacl(O, reps=5, a=5.64, precision=1e-6, rc=12.0):
+    base = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0]])
+    na = base
+    cl = base + np.array([.5, 0, 0])
+    frac, q = [], []
+    for i in range(reps):
+        for j in range(reps):
+            for k in range(reps):
+                off = np.array([i, j, k])
+                frac += list(na + off) + list(cl + off)
+                q += [1.0] * 4 + [-1.0] * 4
+    L = reps * a
+    box, inv, props = O.cell_make(L, L, L)
+    xyz = np.array(frac) * a
+    n = len(q)
+    s = O.System(box, O.BC_RECTANGULAR, xyz[:, 0], xyz[:, 1], xyz[:, 2], q, np.zeros(n),
+                 np.full(n, 3.0), charge_method=O.CHARGE_EWALD, cutoff_charge=rc,
+                 cutoff_vdw=rc, ewald_precision=precision)
+    return s, n, a
+
+
+@pytest.mark.parametrize("precision,tol", [(1e-6, 2e-5), (1e-9, 5e-8)])
+def test_ewald_madelung_nacl(O, precision, tol):
+    """Real + Fourier - self Ewald energy of rock salt equals the Madelung
+    energy -M N C/(2 r0), M = 1.747564594633, to the requested precision."""
+    s, n, a = _nacl(O, precision=precision)
+    C = O.units()["coulomb"]
+    e = s.real_energy_framework() + s.fourier_self_framework() - s.self_energy_framework()
+    ref = -1.747564594633 * n * C / (2 * (a / 2))
+    assert abs(e / ref - 1.0) < tol
+
+
+def test_ewald_alpha_independence(O):
+    """The total Ewald energy does not depend on the splitting parameter."""
+    s1, n, a = _nacl(O, precision=1e-7)
+    s2, _, _ = _nacl(O, precision=1e-9)
+    assert s1.info()["alpha"] != s2.info()["alpha"]
+    e1 = s1.real_energy_framework() + s1.fourier_self_framework() - s1.self_energy_framework()
+    e2 = s2.real_energy_framework() + s2.fourier_self_framework() - s2.self_energy_framework()
+    assert abs(e1 / e2 - 1.0) < 2e-6
+
+
+def test_fourier_delta_is_energy_difference(O, W):
+    """CalculateEwaldFourierAdsorbate's host + adsorbate deltas plus the
+    excluded term equal the change of sum_k kf |S(k)|^2 when the molecule's
+    charges join the framework, minus the molecule's self energy."""
+    fw = W.c1_framework(n_rep=1)
+    s = oracle_system(fw, charge_method=O.CHARGE_EWALD)
+    types = [s.add_pseudo_atom(*site) for site in W.CO2.sites]
+    c = np.array([5.0, 6.0, 7.0])
+    v = np.array([0.3, -0.5, 0.81])
+    v /= np.linalg.norm(v)
+    mol = np.array([c + 1.16 * v, c, c - 1.16 * v])
+    out = s.fourier_adsorbate(types, mol)
+    qm = [site[2] for site in W.CO2.sites]
+    box, bc, x, y, z, q, eps, sig, mass = fw.oracle_args()
+    s2 = O.System(box, bc, np.r_[x, mol[:, 0]], np.r_[y, mol[:, 1]], np.r_[z, mol[:, 2]],
+                  np.r_[q, qm], np.r_[eps, [1, 1, 1]], np.r_[sig, [1, 1, 1]],
+                  charge_method=O.CHARGE_EWALD, ewald_precision=1e-6)
+    assert s2.info()["nk"] == s.info()["nk"]
+    d_recip = s2.fourier_self_framework() - s.fourier_self_framework()
+    C, alpha = O.units()["coulomb"], s.info()["alpha"]
+    self_mol = sum(C * qq * qq * alpha / math.sqrt(math.pi) for qq in qm)
+    assert abs(out[2] - self_mol) < 1e-9 * self_mol
+    excl = 0.0
+    for i in range(3):
+        for j in range(i + 1, 3):
+            r = np.linalg.norm(mol[i] - mol[j])
+            excl += C * qm[i] * qm[j] * erf(alpha * r) / r
+    assert abs(out[3] - excl) < 1e-9 * abs(excl)
+    assert abs((out[0] + out[1] + out[3]) - (d_recip - self_mol)) < 1e-8 * abs(self_mol)
+
+
+def test_charge_energy_closed_form(O):
+    """Real-space Ewald pair term C qa qb erfc(alpha r)/r (scipy erfc)."""
+    box, inv, props = O.cell_make(30.0, 30.0, 30.0)
+    s = _one_atom_system(O, box, O.BC_RECTANGULAR, q=-0.4, cm=O.CHARGE_EWALD)
+    t = s.add_pseudo_atom(27.0, 2.8, 0.7)
+    C, alpha = O.units()["coulomb"], s.info()["alpha"]
+    for r in (1.0, 2.5, 7.0, 11.9):
+        u = s.charge_at(t, [0, r, 0])
+        ref = C * 0.7 * -0.4 * erfc(alpha * r) / r
+        assert abs(u - ref) <= 1e-13 * abs(ref)
+    assert s.charge_at(t, [0, 12.01, 0]) == 0.0
+
+
+def test_lekien_marsden_matrix(O):
+    """The generated 64x64 matrix is integer, inverts the constraint matrix,
+    and interpolates any tricubic polynomial exactly."""
+    C = O.lm_coeff()
+    assert C.dtype == np.int32 and np.abs(C).max() <= 100
+    rng = np.random.default_rng(9)
+    coef = rng.normal(size=(4, 4, 4))
+
+    def f_and_derivs(x, y, z):
+        out = np.zeros(8)
+        p = [np.array([x ** i for i in range(4)]), np.array([y ** i for i in range(4)]),
+             np.array([z ** i for i in range(4)])]
+        dp = [np.array([i * x ** (i - 1) if i else 0.0 for i in range(4)]),
+              np.array([i * y ** (i - 1) if i else 0.0 for i in range(4)]),
+              np.array([i * z ** (i - 1) if i else 0.0 for i in range(4)])]
+        orders = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (1, 0, 1), (0, 1, 1),
+                  (1, 1, 1)]
+        for ch, (a, b, c) in enumerate(orders):
+            px = dp[0] if a else p[0]
+            py = dp[1] if b else p[1]
+            pz = dp[2] if c else p[2]
+            out[ch] = np.einsum("ijk,i,j,k->", coef, px, py, pz)
+        return out
+
+    # a 2x2x2-cell grid over [0,2]^3 with unit spacing
+    geom = np.array([2, 2, 2, 2.0, 2.0, 2.0, 0, 0, 0, 1.0, 1.0, 1.0])
+    grid = np.zeros((3, 3, 3, 8), dtype=np.float32)
+    # small coefficients keep float32 node values exact enough
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                grid[i, j, k] = f_and_derivs(i, j, k)
+    box = np.array([2.0, 0, 0, 0, 2.0, 0, 0, 0, 2.0])
+    inv = np.array([0.5, 0, 0, 0, 0.5, 0, 0, 0, 0.5])
+    g64 = grid.astype(np.float64)
+    for p in rng.uniform(0.05, 1.95, (50, 3)):
+        val = O.interpolate_grid(grid, geom, O.BC_RECTANGULAR, box, inv, p)
+        # exact tricubic of the float32-rounded node data: compare with the
+        # polynomial within the float32 rounding of the inputs
+        ref = f_and_derivs(*p)[0]
+        assert abs(val - ref) <= 1e-5 * max(1.0, np.abs(g64).max())
+
+
+def test_grid_derivatives_finite_differences(O, W):
+    """CalculateDerivativesAtPositionVDW against central differences of the
+    energy: gradient, the mixed second derivatives and the third derivative."""
+    fw = W.c2_framework()
+    s = oracle_system(fw)
+    t = s.add_pseudo_atom(148.0, 3.73)
+    h = 1e-3
+    for p in random_positions(fw.box, 6, 17):
+        d = s.derivatives_vdw_at(t, p)
+        if d[0] > 1e5:
+            continue
+        U = lambda q: s.derivatives_vdw_at(t, q)[0]
+        ex, ey, ez = np.eye(3) * h
+        g = [(U(p + e) - U(p - e)) / (2 * h) for e in (ex, ey, ez)]
+        scale = max(1.0, np.abs(d[1:4]).max())
+        assert np.allclose(d[1:4], g, atol=1e-5 * scale)
+        hxy = (U(p + ex + ey) - U(p + ex - ey) - U(p - ex + ey) + U(p - ex - ey)) / (4 * h * h)
+        assert abs(d[5] - hxy) <= 1e-3 * max(1.0, abs(hxy))
+        gxyz = lambda q: s.derivatives_vdw_at(t, q)[5]
+        t3 = (gxyz(p + ez) - gxyz(p - ez)) / (2 * h)
+        assert abs(d[13] - t3) <= 1e-3 * max(1.0, abs(t3))
+
+
+def test_widom_without_framework_interaction(O, W):
+    """With epsilon = 0 for every framework atom and no charges every trial
+    energy is 0: W = 1 for every insertion and <U> = 0 (CBMC weights are
+    normalised by the number of trials)."""
+    fw = W.c0_framework()
+    box, bc, x, y, z, q, eps, sig, mass = fw.oracle_args()
+    s = O.System(box, bc, x, y, z, q, np.zeros_like(eps), sig, mass=mass)
+    types = [s.add_pseudo_atom(e, sg, 0.0) for e, sg, _ in W.N2LIKE.sites]
+    mol = O.molecule(types, W.N2LIKE.body, start=0, k_fb=7, k_or=5)
+    rec = O.widom_run_stream(s, mol, 3, 0, 200)
+    assert np.all(rec[:, 0] == 1.0)
+    assert np.all(rec[:, 1] == 0.0)
+
+
+def test_stream_trial_geometry(O, W):
+    """Counter-stream trials: first beads inside the cell, rotated bodies
+    keep their bond lengths and shape (orthonormal Shoemake rotation)."""
+    fw = W.c1_framework(n_rep=1)
+    s = oracle_system(fw, charge_method=O.CHARGE_EWALD)
+    types = [s.add_pseudo_atom(*site) for site in W.CO2.sites]
+    mol = O.molecule(types, W.CO2.body, start=1, k_fb=3, k_or=10)
+    inv = np.linalg.inv(np.asarray(fw.box).reshape(3, 3))
+    for i in range(50):
+        fb, cord, usel = O.stream_trials(s, mol, 99, i)
+        f = fb @ inv
+        assert np.all((f >= 0) & (f <= 1))
+        assert np.all((usel >= 0) & (usel <= 1))
+        for o in range(10):
+            assert np.allclose(cord[o, 1], 0.0, atol=0)
+            assert abs(np.linalg.norm(cord[o, 0]) - 1.16) < 1e-14
+            assert np.allclose(cord[o, 0], -cord[o, 2], atol=1e-15)
+
+
+def test_stream_uniform_moments(O):
+    u = np.concatenate([O.stream_uniforms(5, i, 100) for i in range(2000)])
+    assert abs(u.mean() - 0.5) < 0.003
+    assert abs(u.var() - 1 / 12) < 0.002
+    assert len(np.unique(u)) == len(u)
+
+
+def test_block_of_cycles(O):
+    b = O.block_of_cycles(100)
+    assert np.array_equal(np.bincount(b), [20] * 5)
+    b = O.block_of_cycles(7)
+    assert b[0] == 0 and b[-1] == 4 and np.all(np.diff(b) >= 0)
+
+
+def test_mt_driver_matches_stream_statistically(O):
+    """RASPA's global MT19937-64 draw order and the per-insertion counter
+    stream sample the same distribution: He <W> in IRMOF-1 agrees."""
+    import json, os
+    G = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "irmof1.json")))
+    a, b, c, al, be, ga = G["cell"]
+    box, inv, props = O.cell_make(a, b, c, al, be, ga)
+    frac = np.array([at["frac"] for at in G["atoms"]])
+    xyz = frac @ box.reshape(3, 3)
+    n = len(frac)
+    s = O.System(box, O.BC_TRICLINIC, xyz[:, 0], xyz[:, 1], xyz[:, 2], np.zeros(n),
+                 [at["eps_k"] for at in G["atoms"]], [at["sigma"] for at in G["atoms"]],
+                 mass=[at["mass"] for at in G["atoms"]], temperature=298.0)
+    he = s.add_pseudo_atom(10.9, 2.64)
+    mol = O.molecule([he], [[0, 0, 0]], start=0, k_fb=10, k_or=10)
+    n_ins, blocks = O.widom_run_mt(s, mol, G["output"]["seed"], 500, 20)
+    assert n_ins == 10000
+    w_mt = blocks[:, 1].sum() / blocks[:, 0].sum()
+    rec = O.widom_run_stream(s, mol, 4, 0, 10000)
+    w_st = rec[:, 0].mean()
+    se = rec[:, 0].std() / np.sqrt(10000)
+    assert abs(w_mt - w_st) < 5 * np.sqrt(2) * se
