@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""Widom insertion throughput of rigid CO2 in a 4608-atom charged framework.
+
+Workload (one step): 10^6 Widom insertions (WidomAdsorbateMove,
+src/mc_moves.c:7844) of TraPPE-like rigid CO2 into the 2x2x2 supercell
+(51.6 A cubic, 4608 atoms, charges N(0,0.4 e) neutralised) with Lennard-Jones
+(12 A, shifted) and Ewald (relative precision 1e-6, alpha 0.2651/A,
+kmax 14, 6986 k-vectors); one first-bead trial and 10 orientation trials per
+insertion, LJ biasing, T = 300 K.  Inputs are resident in HBM before the timed
+region; trial positions come from the per-insertion counter stream on the
+device.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1]
+
+Prints one JSON line.  The CPU baseline is the oracle (the FP64 C
+restatement) on the host cores over a bounded sample of the same insertions.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Widom trial insertions/sec at stated rel. error; screening p50/p99 latency"
+PEAK = {"mixed": 157.3, "fp64": 78.6}  # MI355X vector FP32 / FP64, TFLOP/s
+FLOP_LJ_PAIR = 29      # minimum image + r^2 + cutoff test (21) + LJ (8)
+FLOP_CC_EXTRA = 9      # real-space erfc Coulomb term on top of a pair
+FLOP_CC_PAIR = 30      # Coulomb-only pair: 21 + 9
+SEED = 20241203
+
+
+def _cpu_worker(args):
+    fw_arrays, start, count = args
+    import numpy as np
+    import oracle as O
+    box, bc, x, y, z, q, eps, sig, mass, sites, body = fw_arrays
+    s = O.System(box, bc, x, y, z, q, eps, sig, mass=mass, charge_method=O.CHARGE_EWALD,
+                 ewald_precision=1e-6, temperature=300.0)
+    types = [s.add_pseudo_atom(*site) for site in sites]
+    mol = O.molecule(types, body, start=1, k_fb=1, k_or=10)
+    t0 = time.perf_counter()
+    rec = O.widom_run_stream(s, mol, SEED, start, count)
+    dt = time.perf_counter() - t0
+    return dt, rec
+
+
+def cpu_baseline(fw, probe, n_per_worker):
+    """Oracle (FP64 C restatement, RASPA2's all-pairs order and cost model)
+    on the host cores: one core alone, then one process per core."""
+    import multiprocessing as mp
+    import numpy as np
+    arrays = fw.oracle_args() + (probe.sites, probe.body)
+    dt1, rec1 = _cpu_worker((arrays, 0, n_per_worker))
+    single = n_per_worker / dt1
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        res = pool.map(_cpu_worker, [(arrays, k * n_per_worker, n_per_worker) for k in range(cores)])
+    wall = time.perf_counter() - t0
+    total = cores * n_per_worker
+    return dict(single=single, aggregate=total / wall, cores=cores, sample=total,
+                rec0=rec1)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--insertions", type=int, default=1_000_000, help="insertions per step")
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--cpu-sample", type=int, default=400, help="insertions per CPU process")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the Widom "
+              "insertions shard by index range with no exchange but a 5-block sum, see "
+              "DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from raspa2_amd import rhg as R
+    from raspa2_amd import workloads as W
+
+    fw = W.c1_framework()
+    probe = W.CO2
+    n = args.insertions
+
+    # CPU baseline first: forked workers, before this process touches the GPU
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(fw, probe, args.cpu_sample)
+
+    prec = R.PREC_MIXED if args.precision == "mixed" else R.PREC_FP64
+    ctx = fw.context(charge_method=R.CHARGE_EWALD, precision=prec)
+    _, mol = W.register_probe(ctx, probe)
+    info = ctx.info()
+    total = (args.warmup + args.steps) * n
+    for w in range(args.warmup):
+        ctx.widom_run(mol, SEED, total, first=w * n, n=n)
+    ctx.synchronize()
+    ctx.profile(True)
+    ctx.profile_reset()
+    ctx.widom_stats(reset=True)
+    blocks = R.WidomBlocks()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        ctx.widom_run(mol, SEED, total, first=(args.warmup + s) * n, n=n, blocks=blocks)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    value = args.steps * n / elapsed
+    st = ctx.widom_stats()
+    kernels = {}
+    for name in ("site_energy", "fourier", "widom_gen", "widom_select", "block_reduce"):
+        cnt, ms = ctx.profile_get(name)
+        kernels[name] = {"launches": cnt, "total_ms": round(ms, 4),
+                         "avg_ms": round(ms / cnt, 5) if cnt else None}
+    ctx.profile(False)
+    se = kernels["site_energy"]
+    # executed pair evaluations of the culled kernels (not the all-pairs count)
+    flops = (st["exec_lj_pairs"] * FLOP_LJ_PAIR
+             + st["exec_ljcc_pairs"] * (FLOP_LJ_PAIR + FLOP_CC_EXTRA)
+             + st["exec_cc_pairs"] * FLOP_CC_PAIR)
+    achieved = flops / (se["total_ms"] * 1e-3) / 1e12 if se["total_ms"] else None
+    peak = PEAK[args.precision]
+
+    # relative error of this precision against the plain FP64 GPU path on the
+    # first timed step, and the plain path's own throughput
+    ref_ctx = fw.context(charge_method=R.CHARGE_EWALD, precision=R.PREC_FP64)
+    _, mol64 = W.register_probe(ref_ctx, probe)
+    b_test = R.WidomBlocks()
+    b_ref = R.WidomBlocks()
+    ctx.widom_run(mol, SEED, total, first=args.warmup * n, n=n, blocks=b_test)
+    ref_ctx.synchronize()
+    t2 = time.perf_counter()
+    ref_ctx.widom_run(mol64, SEED, total, first=args.warmup * n, n=n, blocks=b_ref)
+    ref_ctx.synchronize()
+    plain_rate = n / (time.perf_counter() - t2)
+    rho = info["density"]
+    e_t = R.widom_estimators(b_test, 300.0, rho)
+    e_r = R.widom_estimators(b_ref, 300.0, rho)
+    rel_u = abs(e_t["u"] - e_r["u"]) / abs(e_r["u"]) if e_r["u"] else 0.0
+    rel_kh = abs(e_t["kh_overall"] - e_r["kh_overall"]) / e_r["kh_overall"]
+    est = R.widom_estimators(blocks, 300.0, rho)
+
+    cpu_obj = None
+    cpu_rel = None
+    if cpu is not None:
+        cpu_obj = {"value": round(cpu["aggregate"], 2), "unit": "insertions/s",
+                   "cores": cpu["cores"], "kind": "port",
+                   "single_core": round(cpu["single"], 2),
+                   "sample": f"{cpu['sample']} insertions of the same workload (insertions "
+                             f"0..{cpu['sample'] - 1} of the job's counter stream), oracle "
+                             f"FP64 C restatement, one process per core"}
+        # GPU FP64 on the same first insertions as the CPU sample
+        m0 = len(cpu["rec0"])
+        _, rec_g = ref_ctx.widom_run(mol64, SEED, m0, first=0, n=m0, records=True)
+        acc = cpu["rec0"][:, 13] == 4
+        flags_equal = bool(np.array_equal(cpu["rec0"][:, 6], rec_g[:, 6]))
+        wr = (np.max(np.abs(rec_g[acc, 0] - cpu["rec0"][acc, 0]) / cpu["rec0"][acc, 0])
+              if acc.any() else 0.0)
+        cpu_rel = {"flags_identical": flags_equal, "max_rel_err_W_fp64": float(wr),
+                   "insertions": m0}
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "insertions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 pair terms, fp64 accumulation/Ewald" if args.precision == "mixed" else "fp64",
+        "data": "synthetic",
+        "config": {"workload": "C1 CO2 Widom: rigid 3-site CO2, 4608-atom 51.6 A cubic "
+                               "supercell, LJ 12 A shifted + Ewald 1e-6 (6986 k), k_fb=1, "
+                               "k_or=10, 300 K",
+                   "insertions_per_step": n, "n_atoms": fw.n, "nk": info["nk"],
+                   "precision": args.precision, "parallelism": "single GPU"},
+        "roofline": {"bound": "valu", "kernel": "site_energy", "achieved": round(achieved, 3),
+                     "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+                     "traffic": None,
+                     "note": "algorithmic flops of the executed (cell-list culled) LJ (29/pair), "
+                             "LJ+Coulomb (38) and Coulomb (30) pair evaluations over the summed "
+                             "HIP-event durations of the site_energy launches; peak is the "
+                             "published vector rate (packed FP32; unpacked issue is half)"},
+        "cpu_baseline": cpu_obj,
+        "rel_err_vs_fp64_gpu": {"widom_energy": rel_u, "henry": rel_kh},
+        "fp64_plain_gpu": {"value": round(plain_rate, 1), "unit": "insertions/s"},
+        "cpu_parity": cpu_rel,
+        "estimators": {"henry_mol_kg_pa": est["kh_overall"], "u_gh_K": est["u"],
+                       "w": est["w_overall"]},
+        "work": {k: float(v) for k, v in st.items()},
+        "kernels": kernels,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

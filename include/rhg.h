@@ -216,6 +216,20 @@ int rhg_profile_enable(rhg_ctx *ctx, int32_t on);
 int rhg_profile_get(rhg_ctx *ctx, const char *kernel, int64_t *launches, double *total_ms);
 int rhg_profile_reset(rhg_ctx *ctx);
 int rhg_synchronize(rhg_ctx *ctx);
+/* Work done by rhg_widom_run since the last reset (no counterpart in the
+ * reference; feeds the roofline): out[0] insertions, [1] first-bead sites,
+ * [2] orientation sites, [3] Coulomb sites of selected orientations, [4]
+ * Fourier molecules, [5] LJ pair evaluations, [6] LJ+Coulomb pair
+ * evaluations, [7] Coulomb-only pair evaluations (all-pairs counts), [8]
+ * k-vector x charged site terms, [9..11] pair evaluations executed by the
+ * culled kernels (every site-energy call): LJ, LJ+Coulomb, Coulomb. */
+int rhg_widom_stats(rhg_ctx *ctx, double out[12], int32_t reset);
+/* Cell-list culling of the site-energy kernels (analogue of RASPA2's
+ * UseCellLists, src/framework_energy.c:78-126, which the reference leaves
+ * disabled): sites are binned into cells of about cell_size A and each
+ * wave walks the ascending list of atoms within reach of its cell.  The FP64
+ * result is bit-identical to the all-pairs loop.  Default: on, 2.5 A. */
+int rhg_set_culling(rhg_ctx *ctx, int32_t on, double cell_size);
 
 /* ------------------------------------------------------ workload utility */
 /* No counterpart in the reference (its frameworks come from CIF files):

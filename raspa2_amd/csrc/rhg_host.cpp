@@ -137,8 +137,20 @@ struct rhg_ctx {
   double2 *d_sf = nullptr;
   int32_t *d_row_off = nullptr, *d_row_kz = nullptr;
   size_t footprint = 0;
+  // cell lists
+  int cull = 1;
+  double cell_size = 2.5;
+  bool grid_ready = false;
+  CellGrid cg;
+  int32_t *d_nb_off = nullptr, *d_nb_idx = nullptr;
+  double *d_ax = nullptr, *d_ay = nullptr, *d_az = nullptr;
   std::map<std::string, DevBuf> bufs;
   bool prof = false;
+  // work counters of the Widom path since the last reset:
+  // insertions, first-bead sites, orientation sites, Coulomb sites of the
+  // selected orientations, Fourier molecules, LJ pairs, LJ+Coulomb pairs,
+  // Coulomb-only pairs, k-vector x charged-site terms
+  double wstats[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   std::map<std::string, std::pair<int64_t, double>> stats;
   std::vector<Pending> pending;
   std::vector<hipEvent_t> event_pool;
@@ -412,6 +424,110 @@ int check_type(rhg_ctx *c, int32_t type) {
   return RHG_OK;
 }
 
+// Cell grid and neighbour lists for culled site energies (built lazily).
+int ensure_cell_grid(rhg_ctx *c) {
+  if (c->grid_ready) return RHG_OK;
+  hipStream_t st = c->stream;
+  int nc[3];
+  for (int d = 0; d < 3; ++d) {
+    nc[d] = (int)floor(c->props[3 + d] / c->cell_size);
+    nc[d] = std::max(1, std::min(nc[d], 64));
+  }
+  int ncell = nc[0] * nc[1] * nc[2];
+  // half the longest diagonal of one cell
+  const double *B = c->cell.box;
+  double u[3] = {B[AX] / nc[0], B[AY] / nc[0], B[AZ] / nc[0]};
+  double v[3] = {B[BX] / nc[1], B[BY] / nc[1], B[BZ] / nc[1]};
+  double w[3] = {B[CX] / nc[2], B[CY] / nc[2], B[CZ] / nc[2]};
+  double R = 0.0;
+  for (int sv = -1; sv <= 1; sv += 2)
+    for (int sw = -1; sw <= 1; sw += 2) {
+      double d2 = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        double e = u[k] + sv * v[k] + sw * w[k];
+        d2 += e * e;
+      }
+      R = std::max(R, 0.5 * sqrt(d2));
+    }
+  double rc = std::max(c->rc_vdw, c->rc_cc);
+  double reach = rc + R + 1e-6;
+  int n = std::max(c->n, 1);
+  int rc_;
+  if (!c->d_ax) {
+    if ((rc_ = dalloc(c, (void **)&c->d_ax, n * sizeof(double)))) return rc_;
+    if ((rc_ = dalloc(c, (void **)&c->d_ay, n * sizeof(double)))) return rc_;
+    if ((rc_ = dalloc(c, (void **)&c->d_az, n * sizeof(double)))) return rc_;
+    if (c->n) {
+      upload(c, c->d_ax, c->x.data(), c->n * sizeof(double));
+      upload(c, c->d_ay, c->y.data(), c->n * sizeof(double));
+      upload(c, c->d_az, c->z.data(), c->n * sizeof(double));
+    }
+  }
+  int32_t *counts = (int32_t *)work(c, "nb_counts", (size_t)ncell * sizeof(int32_t));
+  hipFree(c->d_nb_off);
+  c->d_nb_off = nullptr;
+  if ((rc_ = dalloc(c, (void **)&c->d_nb_off, (size_t)(ncell + 1) * sizeof(int32_t)))) return rc_;
+  if (!counts) return fail(c, RHG_ERR_OOM, "cell counts");
+  launch_cell_neighbours(c->cell, nc[0], nc[1], nc[2], reach, c->d_ax, c->d_ay, c->d_az, c->n,
+                         counts, nullptr, nullptr, 0, st);
+  launch_scan_counts(counts, ncell, c->d_nb_off, st);
+  int32_t total = 0;
+  HIPC(c, hipMemcpyAsync(&total, c->d_nb_off + ncell, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIPC(c, hipStreamSynchronize(st));
+  hipFree(c->d_nb_idx);
+  c->d_nb_idx = nullptr;
+  if ((rc_ = dalloc(c, (void **)&c->d_nb_idx, (size_t)std::max(total, 1) * sizeof(int32_t)))) return rc_;
+  launch_cell_neighbours(c->cell, nc[0], nc[1], nc[2], reach, c->d_ax, c->d_ay, c->d_az, c->n,
+                         counts, c->d_nb_off, c->d_nb_idx, 1, st);
+  HIPC(c, hipGetLastError());
+  HIPC(c, hipStreamSynchronize(st));
+  c->cg.nc[0] = nc[0];
+  c->cg.nc[1] = nc[1];
+  c->cg.nc[2] = nc[2];
+  c->cg.ncell = ncell;
+  c->cg.nb_off = c->d_nb_off;
+  c->cg.nb_idx = c->d_nb_idx;
+  c->grid_ready = true;
+  return RHG_OK;
+}
+
+// Site energies of one site list, culled through the cell grid when enabled.
+// `tag` keeps the binning scratch of concurrent lists apart.
+int site_energy(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const EnergyOut &o,
+                const char *tag) {
+  if (s.n <= 0) return RHG_OK;
+  if (!c->cull || c->n == 0) {
+    Timed t(c, "site_energy");
+    launch_site_energy(a, s, o, c->stream);
+    return RHG_OK;
+  }
+  if (int rc = ensure_cell_grid(c)) return rc;
+  const int ncell = c->cg.ncell;
+  std::string t = tag;
+  BinWork b;
+  b.cell_of = (int32_t *)work(c, (t + "_cellof").c_str(), (size_t)s.n * sizeof(int32_t));
+  b.perm = (int32_t *)work(c, (t + "_perm").c_str(), (size_t)s.n * sizeof(int32_t));
+  b.count = (int32_t *)work(c, (t + "_count").c_str(), (size_t)ncell * sizeof(int32_t));
+  b.fill = (int32_t *)work(c, (t + "_fill").c_str(), (size_t)ncell * sizeof(int32_t));
+  b.start = (int32_t *)work(c, (t + "_start").c_str(), (size_t)(ncell + 1) * sizeof(int32_t));
+  b.tile_off = (int32_t *)work(c, (t + "_tiles").c_str(), (size_t)(ncell + 1) * sizeof(int32_t));
+  bool fresh = c->bufs["pair_counter"].p == nullptr;
+  b.pairs = (unsigned long long *)work(c, "pair_counter", 64);
+  if (fresh && b.pairs) {
+    unsigned long long z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    HIPC(c, hipMemcpyAsync(b.pairs, z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+  }
+  if (!b.cell_of || !b.perm || !b.count || !b.fill || !b.start || !b.tile_off || !b.pairs)
+    return fail(c, RHG_ERR_OOM, "binning scratch");
+  {
+    Timed tm(c, "bin");
+    launch_bin_sites(c->cell, c->cg, s, b, c->stream);
+  }
+  Timed tm(c, "site_energy");
+  launch_site_energy_culled(a, c->cg, s, b, o, c->stream);
+  return RHG_OK;
+}
+
 } // namespace
 
 // ======================================================================
@@ -523,6 +639,11 @@ void rhg_destroy(rhg_ctx *c) {
   for (auto &b : c->bufs) hipFree(b.second.p);
   hipFree(c->d_kfactor);
   hipFree(c->d_sf);
+  hipFree(c->d_nb_off);
+  hipFree(c->d_nb_idx);
+  hipFree(c->d_ax);
+  hipFree(c->d_ay);
+  hipFree(c->d_az);
   hipFree(c->d_row_off);
   hipFree(c->d_row_kz);
   for (auto &p : c->pending) {
@@ -604,10 +725,7 @@ int rhg_framework_energy_at_positions(rhg_ctx *c, int64_t n, const double *xyz, 
   if (!a.do_cc) fill(dc, n, 0.0, c->stream);
   SiteList s{dx, dy, dz, n, nullptr, 1};
   EnergyOut o{dv, a.do_cc ? dc : nullptr, dov};
-  {
-    Timed t(c, "site_energy");
-    launch_site_energy(a, s, o, c->stream);
-  }
+  if (int rc = site_energy(c, a, s, o, "e")) return rc;
   if (!a.do_cc) {
     // overlap from the VDW term only; the charge term is zero
   }
@@ -771,8 +889,7 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
     if (!ea.do_cc) fill(w.fb_uc, n * m.k_fb, 0.0, st);
     SiteList s{w.fbx, w.fby, w.fbz, n * m.k_fb, nullptr, 1};
     EnergyOut o{w.fb_uv, ea.do_cc ? w.fb_uc : nullptr, nullptr};
-    Timed t(c, "site_energy");
-    launch_site_energy(ea, s, o, st);
+    if (int rc = site_energy(c, ea, s, o, "bfb")) return rc;
   }
   {
     Timed t(c, "widom_select");
@@ -786,8 +903,7 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
       SiteList s{w.orx + p * cap_or, w.ory + p * cap_or, w.orz + p * cap_or, n * m.k_or,
                  w.n_active, m.k_or};
       EnergyOut o{w.or_uv + p * cap_or, nullptr, nullptr};
-      Timed t(c, "site_energy");
-      launch_site_energy(ea, s, o, st);
+      if (int rc = site_energy(c, ea, s, o, "bor")) return rc;
     }
     {
       Timed t(c, "widom_select");
@@ -807,8 +923,7 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
       const int j = m.place[p];
       SiteList s{w.cfx + j * w.cap, w.cfy + j * w.cap, w.cfz + j * w.cap, n, w.n_active2, 1};
       EnergyOut o{nullptr, dst, nullptr};
-      Timed t(c, "site_energy");
-      launch_site_energy(ea, s, o, st);
+      if (int rc = site_energy(c, ea, s, o, "bcc")) return rc;
     }
     (void)any_cc;
   }
@@ -848,6 +963,37 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
     launch_widom_finalize(a, w, st);
   }
   if (hipGetLastError() != hipSuccess) return fail(c, RHG_ERR_HIP, "widom launch failed");
+  return RHG_OK;
+}
+
+// Adds the chunk's work to the context counters (reads the two compaction
+// counts back; the caller synchronises once per chunk anyway).
+int widom_count(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs &B,
+                const WidomArgs &a) {
+  int32_t na[2] = {0, 0};
+  HIPC(c, hipMemcpyAsync(&na[0], B.w.n_active, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipMemcpyAsync(&na[1], B.w.n_active2, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  const double nf = (double)c->n;
+  const double fb_sites = (double)a.n * m.k_fb;
+  const double or_sites = m.n_sites > 1 ? (double)na[0] * m.k_or * m.n_place : 0.0;
+  const int type0 = mol->type[m.start];
+  const bool cc0 = c->charge_method == RHG_CHARGE_EWALD && considered_charged(c->types[type0].q);
+  double cc_sites = 0.0;
+  if (m.n_sites > 1 && c->charge_method == RHG_CHARGE_EWALD)
+    for (int p = 0; p < m.n_place; ++p)
+      if (m.charged[m.place[p]]) cc_sites += (double)na[1];
+  int nc = 0;
+  for (int j = 0; j < m.n_sites; ++j) nc += m.charged[j];
+  c->wstats[0] += (double)a.n;
+  c->wstats[1] += fb_sites;
+  c->wstats[2] += or_sites;
+  c->wstats[3] += cc_sites;
+  c->wstats[4] += (a.ewald && nc) ? (double)na[1] : 0.0;
+  c->wstats[5] += (cc0 ? 0.0 : fb_sites * nf) + or_sites * nf;
+  c->wstats[6] += cc0 ? fb_sites * nf : 0.0;
+  c->wstats[7] += cc_sites * nf;
+  c->wstats[8] += (a.ewald && nc) ? (double)na[1] * nc * (double)c->nk : 0.0;
   return RHG_OK;
 }
 
@@ -906,6 +1052,7 @@ int rhg_widom_run(rhg_ctx *c, const rhg_molecule *mol, uint64_t seed, int64_t n_
     a.first = first + off;
     a.n = std::min(chunk, n - off);
     if (int rc = widom_chunk(c, mol, m, B, a)) return rc;
+    if (int rc = widom_count(c, mol, m, B, a)) return rc;
     if (rec) {
       double *drec = (double *)work(c, "w_rec", (size_t)a.n * RHG_REC_N * sizeof(double));
       if (!drec) return fail(c, RHG_ERR_OOM, "rec buffer");
@@ -1255,6 +1402,42 @@ int rhg_profile_reset(rhg_ctx *c) {
   if (!c) return RHG_ERR_INVALID;
   drain_profile(c);
   c->stats.clear();
+  return RHG_OK;
+}
+
+int rhg_widom_stats(rhg_ctx *c, double out[12], int32_t reset) {
+  if (!c || !out) return RHG_ERR_INVALID;
+  hipSetDevice(c->device);
+  for (int i = 0; i < 9; ++i) out[i] = c->wstats[i];
+  unsigned long long pairs[3] = {0, 0, 0};
+  DevBuf &pb = c->bufs["pair_counter"];
+  if (pb.p) {
+    HIPC(c, hipMemcpyAsync(pairs, pb.p, sizeof(pairs), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+  }
+  out[9] = (double)pairs[0];
+  out[10] = (double)pairs[1];
+  out[11] = (double)pairs[2];
+  if (reset) {
+    for (int i = 0; i < 9; ++i) c->wstats[i] = 0.0;
+    if (pb.p) {
+      unsigned long long z[3] = {0, 0, 0};
+      HIPC(c, hipMemcpyAsync(pb.p, z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+      HIPC(c, hipStreamSynchronize(c->stream));
+    }
+  }
+  return RHG_OK;
+}
+
+int rhg_set_culling(rhg_ctx *c, int32_t on, double cell_size) {
+  if (!c) return RHG_ERR_INVALID;
+  if (on && !(cell_size > 0.0)) return fail(c, RHG_ERR_INVALID, "cell size must be > 0");
+  hipSetDevice(c->device);
+  c->cull = on ? 1 : 0;
+  if (on && cell_size != c->cell_size) {
+    c->cell_size = cell_size;
+    c->grid_ready = false;
+  }
   return RHG_OK;
 }
 

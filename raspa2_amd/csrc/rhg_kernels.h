@@ -35,6 +35,39 @@ struct EnergyArgs {
 void launch_site_energy(const EnergyArgs &a, const SiteList &s, const EnergyOut &o,
                         hipStream_t st);
 
+// ------------------------------------------------------------ cell lists
+// Cells are a regular nc[0] x nc[1] x nc[2] split of fractional space.  The
+// neighbour list of a cell holds, in ascending order, every atom whose
+// periodic (torus) distance to the cell centre is at most rc + R_cell, so it
+// contains every atom within rc of any point of the cell.
+struct CellGrid {
+  int nc[3];
+  int ncell;
+  const int32_t *nb_off; // ncell + 1
+  const int32_t *nb_idx; // concatenated ascending atom indices
+};
+
+// scratch for binning one site list into cells
+struct BinWork {
+  int32_t *cell_of;  // per site
+  int32_t *count;    // per cell
+  int32_t *fill;     // per cell
+  int32_t *start;    // per cell, exclusive scan of count
+  int32_t *tile_off; // per cell + 1, exclusive scan of ceil(count/64)
+  int32_t *perm;     // sorted position -> site index
+  unsigned long long *pairs; // executed pair evaluations [LJ, LJ+Coulomb, Coulomb]
+};
+
+void launch_cell_neighbours(const Cell &cell, int nc0, int nc1, int nc2, double reach,
+                            const double *ax, const double *ay, const double *az, int n_atoms,
+                            int32_t *counts, int32_t *nb_off, int32_t *nb_idx, int pass,
+                            hipStream_t st);
+void launch_scan_counts(const int32_t *counts, int n, int32_t *off, hipStream_t st);
+void launch_bin_sites(const Cell &cell, const CellGrid &g, const SiteList &s, const BinWork &b,
+                      hipStream_t st);
+void launch_site_energy_culled(const EnergyArgs &a, const CellGrid &g, const SiteList &s,
+                               const BinWork &b, const EnergyOut &o, hipStream_t st);
+
 // ---------------------------------------------------------------- Ewald
 struct EwaldDev {
   int kx, ky, kz, nk;

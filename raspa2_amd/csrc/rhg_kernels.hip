@@ -239,6 +239,305 @@ void launch_site_energy(const EnergyArgs &a, const SiteList &s, const EnergyOut 
 }
 
 // ====================================================================
+// cell lists: neighbour lists of cells and tile-binned site energies
+// ====================================================================
+// periodic (torus) distance^2 between two points: NINT image in fractional
+// coordinates, then the 27 neighbouring images (triclinic cells can put
+// the nearest image one step away from the NINT one).
+__device__ __forceinline__ double torus_dist2(const Cell &c, double dx, double dy, double dz) {
+  double sx = c.inv[AX] * dx + c.inv[BX] * dy + c.inv[CX] * dz;
+  double sy = c.inv[AY] * dx + c.inv[BY] * dy + c.inv[CY] * dz;
+  double sz = c.inv[AZ] * dx + c.inv[BZ] * dy + c.inv[CZ] * dz;
+  sx -= rint(sx);
+  sy -= rint(sy);
+  sz -= rint(sz);
+  double best = DBL_MAX;
+  const int r = (c.bc == 1) ? 0 : 1;
+  for (int i = -r; i <= r; ++i)
+    for (int j = -r; j <= r; ++j)
+      for (int k = -r; k <= r; ++k) {
+        double tx = sx + i, ty = sy + j, tz = sz + k;
+        double x = c.box[AX] * tx + c.box[BX] * ty + c.box[CX] * tz;
+        double y = c.box[AY] * tx + c.box[BY] * ty + c.box[CY] * tz;
+        double z = c.box[AZ] * tx + c.box[BZ] * ty + c.box[CZ] * tz;
+        best = fmin(best, x * x + y * y + z * z);
+      }
+  return best;
+}
+
+// one workgroup per cell; pass 0 counts, pass 1 writes in ascending order
+__global__ __launch_bounds__(256) void k_cell_neighbours(const Cell cell, int nc0, int nc1, int nc2,
+                                                         double reach2, const double *__restrict__ ax,
+                                                         const double *__restrict__ ay,
+                                                         const double *__restrict__ az, int n_atoms,
+                                                         int32_t *__restrict__ counts,
+                                                         const int32_t *__restrict__ nb_off,
+                                                         int32_t *__restrict__ nb_idx, int pass) {
+  __shared__ int wave_sum[4];
+  __shared__ int base;
+  const int c = blockIdx.x;
+  const int cz = c % nc2, cy = (c / nc2) % nc1, cx = c / (nc2 * nc1);
+  const double fx = (cx + 0.5) / nc0, fy = (cy + 0.5) / nc1, fz = (cz + 0.5) / nc2;
+  const double px = cell.box[AX] * fx + cell.box[BX] * fy + cell.box[CX] * fz;
+  const double py = cell.box[AY] * fx + cell.box[BY] * fy + cell.box[CY] * fz;
+  const double pz = cell.box[AZ] * fx + cell.box[BZ] * fy + cell.box[CZ] * fz;
+  if (threadIdx.x == 0) base = pass ? nb_off[c] : 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int j0 = 0; j0 < n_atoms; j0 += blockDim.x) {
+    const int j = j0 + threadIdx.x;
+    bool in = false;
+    if (j < n_atoms) in = torus_dist2(cell, ax[j] - px, ay[j] - py, az[j] - pz) <= reach2;
+    const unsigned long long m = __ballot(in);
+    const int before = __popcll(m & ((1ULL << lane) - 1ULL));
+    if (lane == 0) wave_sum[wave] = __popcll(m);
+    __syncthreads();
+    int off = base;
+    for (int w = 0; w < wave; ++w) off += wave_sum[w];
+    if (in && pass) nb_idx[off + before] = j;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int t = 0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += wave_sum[w];
+      base += t;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && !pass) counts[c] = base;
+}
+
+void launch_cell_neighbours(const Cell &cell, int nc0, int nc1, int nc2, double reach,
+                            const double *ax, const double *ay, const double *az, int n_atoms,
+                            int32_t *counts, int32_t *nb_off, int32_t *nb_idx, int pass,
+                            hipStream_t st) {
+  const int ncell = nc0 * nc1 * nc2;
+  hipLaunchKernelGGL(k_cell_neighbours, dim3(ncell), dim3(256), 0, st, cell, nc0, nc1, nc2,
+                     reach * reach, ax, ay, az, n_atoms, counts, nb_off, nb_idx, pass);
+}
+
+// single-workgroup exclusive scan of n ints into off[0..n]
+__global__ __launch_bounds__(1024) void k_scan(const int32_t *__restrict__ in, int n,
+                                               int32_t *__restrict__ off, int tiles) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int per = (n + blockDim.x - 1) / blockDim.x;
+  const int b = t * per, e = min(n, b + per);
+  int s = 0;
+  for (int i = b; i < e; ++i) s += tiles ? (in[i] + 63) / 64 : in[i];
+  part[t] = s;
+  __syncthreads();
+  for (int d = 1; d < (int)blockDim.x; d <<= 1) {
+    int v = (t >= d) ? part[t - d] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = (t == 0) ? 0 : part[t - 1];
+  for (int i = b; i < e; ++i) {
+    off[i] = run;
+    run += tiles ? (in[i] + 63) / 64 : in[i];
+  }
+  if (t == (int)blockDim.x - 1) off[n] = part[t];
+}
+
+void launch_scan_counts(const int32_t *counts, int n, int32_t *off, hipStream_t st) {
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, counts, n, off, 0);
+}
+
+__device__ __forceinline__ int cell_index(const Cell &c, const CellGrid &g, double x, double y,
+                                          double z) {
+  double f[3];
+  f[0] = c.inv[AX] * x + c.inv[BX] * y + c.inv[CX] * z;
+  f[1] = c.inv[AY] * x + c.inv[BY] * y + c.inv[CY] * z;
+  f[2] = c.inv[AZ] * x + c.inv[BZ] * y + c.inv[CZ] * z;
+  int ci[3];
+  for (int d = 0; d < 3; ++d) {
+    double u = f[d] - floor(f[d]);
+    int k = (int)(u * g.nc[d]);
+    ci[d] = k < 0 ? 0 : (k >= g.nc[d] ? g.nc[d] - 1 : k);
+  }
+  return (ci[0] * g.nc[1] + ci[1]) * g.nc[2] + ci[2];
+}
+
+__global__ __launch_bounds__(256) void k_zero_i32(int32_t *__restrict__ a, int32_t *__restrict__ b,
+                                                  int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    a[i] = 0;
+    b[i] = 0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bin_count(const Cell cell, const CellGrid g,
+                                                   const SiteList s, BinWork b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= site_count(s)) return;
+  const int c = cell_index(cell, g, s.x[i], s.y[i], s.z[i]);
+  b.cell_of[i] = c;
+  atomicAdd(&b.count[c], 1);
+}
+
+__global__ __launch_bounds__(256) void k_bin_scatter(const SiteList s, BinWork b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= site_count(s)) return;
+  const int c = b.cell_of[i];
+  const int p = b.start[c] + atomicAdd(&b.fill[c], 1);
+  b.perm[p] = (int32_t)i;
+}
+
+void launch_bin_sites(const Cell &cell, const CellGrid &g, const SiteList &s, const BinWork &b,
+                      hipStream_t st) {
+  const unsigned nbc = (unsigned)((g.ncell + 255) / 256);
+  hipLaunchKernelGGL(k_zero_i32, dim3(nbc), dim3(256), 0, st, b.count, b.fill, g.ncell);
+  const unsigned nb = (unsigned)((s.n + 255) / 256);
+  if (nb) hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(256), 0, st, cell, g, s, b);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, b.count, g.ncell, b.start, 0);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, b.count, g.ncell, b.tile_off, 1);
+  if (nb) hipLaunchKernelGGL(k_bin_scatter, dim3(nb), dim3(256), 0, st, s, b);
+}
+
+// One wave per tile of <= 64 sites of one cell; the wave walks the cell's
+// ascending neighbour list, so each lane adds its in-range pair terms in
+// atom-index order exactly as the all-pairs loop does.
+template <int PREC, int BCK, int VDW, int CC>
+__global__ __launch_bounds__(256) void k_site_energy_culled(
+    const Cell cell, const AtomRec64 *__restrict__ a64, const AtomRecMx *__restrict__ amx,
+    const double rc2v, const double rc2c, const double alpha, const double cqa, const CellGrid g,
+    const SiteList s, const BinWork b, double *__restrict__ out_v, double *__restrict__ out_c,
+    uint8_t *__restrict__ out_o) {
+  const int lane = threadIdx.x & 63;
+  const int tile = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  const int ntiles = b.tile_off[g.ncell];
+  if (tile >= ntiles) return;
+  // wave-uniform binary search for the cell owning this tile
+  int lo = 0, hi = g.ncell - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (b.tile_off[mid] <= tile) lo = mid; else hi = mid - 1;
+  }
+  const int c = __builtin_amdgcn_readfirstlane(lo);
+  const int k = (tile - b.tile_off[c]) * 64 + lane;
+  const bool live = k < b.count[c];
+  const int site = live ? b.perm[b.start[c] + k] : b.perm[b.start[c]];
+  const double px = s.x[site], py = s.y[site], pz = s.z[site];
+  const int j0 = g.nb_off[c], j1 = g.nb_off[c + 1];
+  double uv = 0.0, uc = 0.0;
+  if (PREC == 0) {
+#pragma clang fp contract(off)
+#pragma unroll 2
+    for (int q = j0; q < j1; ++q) {
+      const AtomRec64 r = a64[g.nb_idx[q]];
+      double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
+      mi_exact<BCK>(cell, dx, dy, dz);
+      double rr = dx * dx + dy * dy + dz * dz;
+      if (VDW) {
+        if (rr < rc2v) uv += lj_exact(r.c4, r.s2, r.shift, rr);
+      }
+      if (CC) {
+        if (rr < rc2c) {
+          double rd = sqrt(rr);
+          uc += cqa * r.q * (erfc(alpha * rd) / rd);
+        }
+      }
+    }
+  } else {
+    const float alphaf = (float)alpha, cqaf = (float)cqa;
+#pragma unroll 4
+    for (int q = j0; q < j1; ++q) {
+      const AtomRecMx r = amx[g.nb_idx[q]];
+      double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
+      mi_fast<BCK>(cell, dx, dy, dz);
+      const double rr = fma(dx, dx, fma(dy, dy, dz * dz));
+      const float rrf = (float)rr;
+      if (VDW) {
+        const float ir = __builtin_amdgcn_rcpf(rrf);
+        const float t = r.s2 * ir;
+        const float r3 = t * t * t;
+        const float u = fmaf(r.c4, fmaf(r3, r3, -r3), -r.shift);
+        uv += (rr < rc2v) ? (double)u : 0.0;
+      }
+      if (CC) {
+        if (rr < rc2c) {
+          const float irs = __builtin_amdgcn_rsqf(rrf);
+          const float rd = rrf * irs;
+          const float e = erfcf(alphaf * rd);
+          uc += (double)((cqaf * r.q) * (e * irs));
+        }
+      }
+    }
+    const double band = 1.0e-4 * kOverlap;
+    const bool near = live && ((VDW && fabs(uv - kOverlap) <= band) || (CC && fabs(uc - kOverlap) <= band));
+    if (__any(near)) {
+      if (near) {
+#pragma clang fp contract(off)
+        double ev = 0.0, ec = 0.0;
+        for (int q = j0; q < j1; ++q) {
+          const AtomRec64 r = a64[g.nb_idx[q]];
+          double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
+          mi_exact<BCK>(cell, dx, dy, dz);
+          double rr = dx * dx + dy * dy + dz * dz;
+          if (VDW && rr < rc2v) ev += lj_exact(r.c4, r.s2, r.shift, rr);
+          if (CC && rr < rc2c) {
+            double rd = sqrt(rr);
+            ec += cqa * r.q * (erfc(alpha * rd) / rd);
+          }
+        }
+        if (VDW) uv = ev;
+        if (CC) uc = ec;
+      }
+    }
+  }
+  if (lane == 0 && b.pairs) {
+    const int nl = min(64, b.count[c] - (tile - b.tile_off[c]) * 64);
+    // counters by pair kind: 0 LJ, 1 LJ+Coulomb, 2 Coulomb
+    atomicAdd(b.pairs + ((VDW && CC) ? 1 : (VDW ? 0 : 2)),
+              (unsigned long long)nl * (unsigned long long)(j1 - j0));
+  }
+  if (live) {
+    if (out_v) out_v[site] = uv;
+    if (out_c) out_c[site] = uc;
+    if (out_o) out_o[site] = (uint8_t)((VDW && uv >= kOverlap) || (CC && uc >= kOverlap));
+  }
+}
+
+template <int PREC, int BCK, int VDW, int CC>
+static void culled_launch(const EnergyArgs &a, const CellGrid &g, const SiteList &s,
+                          const BinWork &b, const EnergyOut &o, hipStream_t st) {
+  // upper bound of tiles: sites/64 + one partial tile per cell
+  const int64_t max_tiles = s.n / 64 + g.ncell + 1;
+  const unsigned nb = (unsigned)((max_tiles + 3) / 4);
+  hipLaunchKernelGGL((k_site_energy_culled<PREC, BCK, VDW, CC>), dim3(nb), dim3(256), 0, st, a.cell,
+                     a.a64, a.amx, a.rc2_vdw, a.rc2_cc, a.alpha, a.cqa, g, s, b, o.u_vdw, o.u_cc,
+                     o.overlap);
+}
+
+template <int VDW, int CC>
+static void culled_mode(const EnergyArgs &a, const CellGrid &g, const SiteList &s, const BinWork &b,
+                        const EnergyOut &o, hipStream_t st) {
+  const int bck = a.cell.bc == 1 ? 1 : (a.cell.upper ? 2 : 3);
+  if (a.precision == 0) {
+    if (bck == 1) culled_launch<0, 1, VDW, CC>(a, g, s, b, o, st);
+    else if (bck == 2) culled_launch<0, 2, VDW, CC>(a, g, s, b, o, st);
+    else culled_launch<0, 3, VDW, CC>(a, g, s, b, o, st);
+  } else {
+    if (bck == 1) culled_launch<1, 1, VDW, CC>(a, g, s, b, o, st);
+    else if (bck == 2) culled_launch<1, 2, VDW, CC>(a, g, s, b, o, st);
+    else culled_launch<1, 3, VDW, CC>(a, g, s, b, o, st);
+  }
+}
+
+void launch_site_energy_culled(const EnergyArgs &a, const CellGrid &g, const SiteList &s,
+                               const BinWork &b, const EnergyOut &o, hipStream_t st) {
+  if (s.n <= 0) return;
+  if (a.do_vdw && a.do_cc)
+    culled_mode<1, 1>(a, g, s, b, o, st);
+  else if (a.do_vdw)
+    culled_mode<1, 0>(a, g, s, b, o, st);
+  else if (a.do_cc)
+    culled_mode<0, 1>(a, g, s, b, o, st);
+}
+
+// ====================================================================
 // Ewald: framework structure factors (src/ewald.c:1289-1731)
 // one lane per k-vector, atoms streamed through the scalar unit
 // ====================================================================

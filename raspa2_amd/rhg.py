@@ -106,6 +106,8 @@ def _load():
     f("rhg_profile_get", _P, ctypes.c_char_p, ctypes.POINTER(_I64), ctypes.POINTER(_D))
     f("rhg_profile_reset", _P)
     f("rhg_synchronize", _P)
+    f("rhg_widom_stats", _P, _P, _I32)
+    f("rhg_set_culling", _P, _I32, _D)
     f("rhg_synthetic_positions", _U64, _P, _I32, _D, _I64, _P, ctypes.POINTER(_I32))
     return L
 
@@ -340,6 +342,17 @@ class Context:
 
     def profile_reset(self):
         self._check(lib.rhg_profile_reset(self.h), "profile_reset")
+
+    def set_culling(self, on=True, cell_size=2.5):
+        self._check(lib.rhg_set_culling(self.h, 1 if on else 0, cell_size), "set_culling")
+
+    def widom_stats(self, reset=False):
+        out = np.zeros(12)
+        self._check(lib.rhg_widom_stats(self.h, _ptr(out), 1 if reset else 0), "widom_stats")
+        keys = ("insertions", "fb_sites", "or_sites", "cc_sites", "fourier_mols", "lj_pairs",
+                "ljcc_pairs", "cc_pairs", "k_site_terms", "exec_lj_pairs", "exec_ljcc_pairs",
+                "exec_cc_pairs")
+        return dict(zip(keys, out))
 
     def synchronize(self):
         self._check(lib.rhg_synchronize(self.h), "synchronize")

@@ -8,20 +8,28 @@ from helpers import oracle_system, random_positions, same_bits
 pytestmark = pytest.mark.gpu
 
 
-def _pair(W, O, R, fw, charge_method, precision, probe_sites):
+def _pair(W, O, R, fw, charge_method, precision, probe_sites, cull=True, cell=2.5):
     sys_o = oracle_system(fw, charge_method=charge_method)
     ctx = fw.context(charge_method=charge_method, precision=precision)
+    ctx.set_culling(cull, cell)
     to = [sys_o.add_pseudo_atom(*s) for s in probe_sites]
     tg = [ctx.add_pseudo_atom(*s) for s in probe_sites]
     return sys_o, ctx, to, tg
 
 
-@pytest.mark.parametrize("which", ["c0", "c2"])
-def test_vdw_fp64_bit_exact(W, O, R, which):
+FRAMEWORKS = {"c0": lambda W: W.c0_framework(), "c1": lambda W: W.c1_framework(),
+              "c2": lambda W: W.c2_framework()}
+
+
+@pytest.mark.parametrize("cull,cell", [(False, 2.5), (True, 2.5), (True, 4.0)])
+@pytest.mark.parametrize("which", ["c0", "c1", "c2"])
+def test_vdw_fp64_bit_exact(W, O, R, which, cull, cell):
     """Plain FP64 LJ energies equal the sequential reference loop bit for bit
-    (rectangular C0 cell and triclinic C2 cell)."""
-    fw = W.c0_framework() if which == "c0" else W.c2_framework()
-    sys_o, ctx, to, tg = _pair(W, O, R, fw, O.CHARGE_NONE, R.PREC_FP64, [(148.0, 3.73, 0.0)])
+    (rectangular C0/C1 cells, triclinic C2 cell), with and without cell-list
+    culling."""
+    fw = FRAMEWORKS[which](W)
+    sys_o, ctx, to, tg = _pair(W, O, R, fw, O.CHARGE_NONE, R.PREC_FP64, [(148.0, 3.73, 0.0)],
+                               cull, cell)
     xyz = random_positions(fw.box, 3000, 7)
     uv, uc, ov = ctx.energy_at_positions(xyz, tg[0])
     uv_o, _ = sys_o.energy_at_many(to[0], xyz)
@@ -30,12 +38,14 @@ def test_vdw_fp64_bit_exact(W, O, R, which):
     assert np.all(uc == 0.0)
 
 
-@pytest.mark.parametrize("which", ["c0", "c2"])
-def test_vdw_mixed_tolerance_and_flags(W, O, R, which):
+@pytest.mark.parametrize("cull", [False, True])
+@pytest.mark.parametrize("which", ["c0", "c1", "c2"])
+def test_vdw_mixed_tolerance_and_flags(W, O, R, which, cull):
     """Mixed precision: overlap flags identical, relative error <= 1e-5 on
     non-overlapping sites (|U| > 1 K), absolute <= 1e-3 K otherwise."""
-    fw = W.c0_framework() if which == "c0" else W.c2_framework()
-    sys_o, ctx, to, tg = _pair(W, O, R, fw, O.CHARGE_NONE, R.PREC_MIXED, [(148.0, 3.73, 0.0)])
+    fw = FRAMEWORKS[which](W)
+    sys_o, ctx, to, tg = _pair(W, O, R, fw, O.CHARGE_NONE, R.PREC_MIXED, [(148.0, 3.73, 0.0)],
+                               cull)
     xyz = random_positions(fw.box, 3000, 8)
     uv, _, ov = ctx.energy_at_positions(xyz, tg[0])
     uv_o, _ = sys_o.energy_at_many(to[0], xyz)
@@ -48,11 +58,13 @@ def test_vdw_mixed_tolerance_and_flags(W, O, R, which):
     assert np.all(err[~big] <= 1e-3)
 
 
+@pytest.mark.parametrize("cull", [False, True])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_charge_energy(W, O, R, precision):
+def test_charge_energy(W, O, R, precision, cull):
     """Real-space Ewald energy of a charged bead in the 2x2x2 C1 supercell."""
     fw = W.c1_framework()
-    sys_o, ctx, to, tg = _pair(W, O, R, fw, O.CHARGE_EWALD, precision, [(27.0, 2.80, 0.70)])
+    sys_o, ctx, to, tg = _pair(W, O, R, fw, O.CHARGE_EWALD, precision, [(27.0, 2.80, 0.70)],
+                               cull)
     xyz = random_positions(fw.box, 1500, 9)
     uv, uc, ov = ctx.energy_at_positions(xyz, tg[0])
     uv_o, uc_o = sys_o.energy_at_many(to[0], xyz)

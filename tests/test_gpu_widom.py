@@ -11,10 +11,12 @@ FIELDS = dict(W=0, DU=1, UVDW=2, UREAL=3, UFHOST=4, UFADS=5, OVERLAP=6, SELFB=7,
               FBMASK=9, ORMASK=10, W1=11, W2=12, STAGE=13)
 
 
-def _run_both(W, O, R, fw, probe, charge_method, precision, n, seed, k_fb=None, k_or=None):
+def _run_both(W, O, R, fw, probe, charge_method, precision, n, seed, k_fb=None, k_or=None,
+              cull=True):
     sys_o = oracle_system(fw, charge_method=charge_method)
     _, mo = oracle_molecule(sys_o, probe, k_fb, k_or)
     ctx = fw.context(charge_method=charge_method, precision=precision)
+    ctx.set_culling(cull, 2.5)
     _, mg = gpu_molecule(ctx, probe, k_fb, k_or)
     rec_o = O.widom_run_stream(sys_o, mo, seed, 0, n)
     blocks, rec_g = ctx.widom_run(mg, seed, n, records=True)
@@ -41,11 +43,12 @@ def test_methane_c0_fp64(W, O, R):
     assert np.allclose(rg[:, 1], ro[:, 1], rtol=1e-13, atol=0)
 
 
-def test_co2_c1_fp64(W, O, R):
+@pytest.mark.parametrize("cull", [False, True])
+def test_co2_c1_fp64(W, O, R, cull):
     """Rigid CO2 with Ewald in the 4608-atom supercell: first bead + 10
     orientations, LJ biasing, Fourier Delta."""
     fw = W.c1_framework()
-    sys_o, ctx, ro, rg, _ = _run_both(W, O, R, fw, W.CO2, O.CHARGE_EWALD, 0, 300, 12)
+    sys_o, ctx, ro, rg, _ = _run_both(W, O, R, fw, W.CO2, O.CHARGE_EWALD, 0, 300, 12, cull=cull)
     _check_flags(ro, rg)
     assert same_bits(ro[:, FIELDS["UVDW"]], rg[:, FIELDS["UVDW"]])
     acc = ro[:, FIELDS["STAGE"]] == 4
