@@ -12,6 +12,18 @@
  * reference computes on the same inputs wherever the reference's arithmetic
  * is itself deterministic (libm calls aside).  Each function cites the
  * reference file:line it follows.
+ *
+ * Pins (tests/test_oracle_golden.py, tests/test_oracle_math.py): units,
+ * mixing and shifts, cell volume and density, and the Henry estimator against
+ * numbers printed in the reference's IRMOF-1 example output; MT19937-64
+ * against std::mt19937_64; LJ and minimum image against closed forms and a
+ * brute-force image sum; Ewald against the rock-salt Madelung constant;
+ * Fourier Delta against the change of the full Fourier energy; tricubic
+ * interpolation against exact tricubic polynomials; grid derivatives against
+ * finite differences; Widom against the reference's He/IRMOF-1 result.
+ * Parity unpinned: the libm-dependent last bits (erfc, erf, exp, sin, cos)
+ * and the global interleaving of a whole RASPA2 run's MT19937-64 stream
+ * (orc_widom_run_mt's absolute offset).
  */
 #ifndef RHG_ORACLE_H
 #define RHG_ORACLE_H

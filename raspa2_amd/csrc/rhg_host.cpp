@@ -951,8 +951,10 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
     f.d_host = w.f_host;
     f.d_ads = w.f_ads;
     if (f.nc > 0) {
+      double2 *part = (double2 *)work(c, "w_fpart", (size_t)(c->kx + 1) * w.cap * sizeof(double2));
+      if (!part) return fail(c, RHG_ERR_OOM, "fourier partials");
       Timed t(c, "fourier");
-      launch_fourier(f, st);
+      launch_fourier(f, part, st);
     } else {
       fill(w.f_host, n, 0.0, st);
       fill(w.f_ads, n, 0.0, st);
@@ -1218,8 +1220,10 @@ int rhg_ewald_fourier_adsorbate(rhg_ctx *c, int64_t n_mol, int32_t n_sites, cons
     fill(dh, n_mol, 0.0, c->stream);
     fill(da, n_mol, 0.0, c->stream);
   } else {
+    double2 *part = (double2 *)work(c, "f_part", (size_t)(c->kx + 1) * n_mol * sizeof(double2));
+    if (!part) return fail(c, RHG_ERR_OOM, "fourier partials");
     Timed t(c, "fourier");
-    launch_fourier(f, c->stream);
+    launch_fourier(f, part, c->stream);
   }
   HIPC(c, hipGetLastError());
   if (d_host) HIPC(c, hipMemcpyAsync(d_host, dh, n_mol * sizeof(double), hipMemcpyDeviceToHost, c->stream));

@@ -100,7 +100,8 @@ struct FourierArgs {
   const int32_t *n_dev;
   double *d_host, *d_ads;     // per molecule
 };
-void launch_fourier(const FourierArgs &a, hipStream_t st);
+// part: (kx+1) x stride double2 scratch for the per-kx partial sums
+void launch_fourier(const FourierArgs &a, double2 *part, hipStream_t st);
 
 // ---------------------------------------------------------------- Widom
 struct WidomWork {

@@ -591,28 +591,26 @@ __device__ __forceinline__ cx cmul(cx a, cx b) {
 }
 __device__ __forceinline__ cx conjc(cx a) { return cx{a.re, -a.im}; }
 
+// Partial k-sums of one kx slice (blockIdx.y = kx) for 64 molecules per
+// wave; part[kx][m] = (host, ads).  The slice's exp(i kx x) comes from the
+// same recurrence as the reference (kx repeated products of exp(i x)).
 template <int NC>
-__global__ __launch_bounds__(128) void k_fourier(const FourierArgs a) {
+__global__ __launch_bounds__(128) void k_fourier_part(const FourierArgs a, double2 *__restrict__ part) {
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t n = a.n_dev ? (int64_t)(*a.n_dev) : a.n;
   if ((int64_t)blockIdx.x * blockDim.x >= n) return;
   const bool live = m < n;
   const int64_t mm = live ? m : 0;
   const EwaldDev &e = a.e;
+  const int ii = blockIdx.y;
   constexpr int nc = NC;
   cx e1x[NC], e1y[NC], e1z[NC], ex[NC];
   double qs[NC];
-  double pxs[NC], pys[NC], pzs[NC];
-  double net = 0.0;
 #pragma unroll
   for (int t = 0; t < NC; ++t) {
     const int j = a.cidx[t];
     double px = a.px[j * a.stride + mm], py = a.py[j * a.stride + mm], pz = a.pz[j * a.stride + mm];
-    pxs[t] = px;
-    pys[t] = py;
-    pzs[t] = pz;
     qs[t] = a.q[t];
-    net += a.q[t];
     double fx, fy, fz;
     {
 #pragma clang fp contract(off)
@@ -631,72 +629,93 @@ __global__ __launch_bounds__(128) void k_fourier(const FourierArgs a) {
     sincos(fz, &s, &c);
     e1z[t] = cx{c, s};
     ex[t] = cx{1.0, 0.0};
+    for (int r = 1; r <= ii; ++r) ex[t] = (r == 1) ? e1x[t] : cmul(ex[t], e1x[t]);
   }
   double host = 0.0, ads = 0.0;
   const int ny = 2 * e.ky + 1;
-  for (int ii = 0; ii <= e.kx; ++ii) {
-    if (ii > 0) {
+  cx ey[NC];
 #pragma unroll
-      for (int t = 0; t < nc; ++t) ex[t] = cmul(ex[t], e1x[t]);
+  for (int t = 0; t < nc; ++t) ey[t] = cx{1.0, 0.0};
+  for (int jj = 0; jj <= e.ky; ++jj) {
+    if (jj > 0) {
+#pragma unroll
+      for (int t = 0; t < nc; ++t) ey[t] = (jj == 1) ? e1y[t] : cmul(ey[t], e1y[t]);
     }
-    cx ey[NC];
+    for (int sy = 0; sy < 2; ++sy) {
+      if (jj == 0 && sy == 1) break;
+      const int jv = sy ? -jj : jj;
+      const int row = ii * ny + (jv + e.ky);
+      const int kzm = e.row_kz[row];
+      if (kzm < 0) continue;
+      const int off = e.row_off[row];
+      cx xy[NC], ez[NC];
 #pragma unroll
-    for (int t = 0; t < nc; ++t) ey[t] = cx{1.0, 0.0};
-    for (int jj = 0; jj <= e.ky; ++jj) {
-      if (jj > 0) {
-#pragma unroll
-        for (int t = 0; t < nc; ++t) ey[t] = cmul(ey[t], e1y[t]);
+      for (int t = 0; t < nc; ++t) {
+        xy[t] = cmul(ex[t], sy ? conjc(ey[t]) : ey[t]);
+        ez[t] = cx{1.0, 0.0};
       }
-      for (int sy = 0; sy < 2; ++sy) {
-        if (jj == 0 && sy == 1) break;
-        const int jv = sy ? -jj : jj;
-        const int row = ii * ny + (jv + e.ky);
-        const int kzm = e.row_kz[row];
-        if (kzm < 0) continue;
-        const int off = e.row_off[row];
-        cx xy[NC], ez[NC];
+      const bool origin_row = (ii == 0 && jv == 0);
+      for (int kk = 0; kk <= kzm; ++kk) {
+        if (kk > 0) {
 #pragma unroll
-        for (int t = 0; t < nc; ++t) {
-          xy[t] = cmul(ex[t], sy ? conjc(ey[t]) : ey[t]);
-          ez[t] = cx{1.0, 0.0};
+          for (int t = 0; t < nc; ++t) ez[t] = (kk == 1) ? e1z[t] : cmul(ez[t], e1z[t]);
         }
-        const bool origin_row = (ii == 0 && jv == 0);
-        for (int kk = 0; kk <= kzm; ++kk) {
-          if (kk > 0) {
+        for (int sz = 0; sz < 2; ++sz) {
+          if (kk == 0 && sz == 1) break;
+          if (origin_row && kk == 0) continue;
+          const int kv = sz ? -kk : kk;
+          int pos = kv + kzm;
+          if (origin_row && kv > 0) pos -= 1;
+          const int idx = off + pos;
+          double sre = 0.0, sim = 0.0;
 #pragma unroll
-            for (int t = 0; t < nc; ++t) ez[t] = cmul(ez[t], e1z[t]);
+          for (int t = 0; t < nc; ++t) {
+            cx ek = cmul(xy[t], sz ? conjc(ez[t]) : ez[t]);
+            sre += qs[t] * ek.re;
+            sim += qs[t] * ek.im;
           }
-          for (int sz = 0; sz < 2; ++sz) {
-            if (kk == 0 && sz == 1) break;
-            if (origin_row && kk == 0) continue;
-            const int kv = sz ? -kk : kk;
-            // index within the row: valid kz run from -kzm to kzm, minus kz=0 on the origin row
-            int pos = kv + kzm;
-            if (origin_row && kv > 0) pos -= 1;
-            const int idx = off + pos;
-            double sre = 0.0, sim = 0.0;
-#pragma unroll
-            for (int t = 0; t < nc; ++t) {
-              cx ek = cmul(xy[t], sz ? conjc(ez[t]) : ez[t]);
-              sre += qs[t] * ek.re;
-              sim += qs[t] * ek.im;
-            }
-            const double kf = e.kfactor[idx];
-            const double2 sf = e.sf[idx];
-            host += kf * (sf.x * sre + sf.y * sim);
-            ads += kf * (sre * sre + sim * sim);
-          }
+          const double kf = e.kfactor[idx];
+          const double2 sf = e.sf[idx];
+          host += kf * (sf.x * sre + sf.y * sim);
+          ads += kf * (sre * sre + sim * sim);
         }
       }
     }
   }
-  // excluded intra pairs (all charged pairs, src/molecule.c:2839-2851;
-  // src/ewald.c:7318-7335) and net-charge terms (src/ewald.c:7536-7565)
+  if (live) part[(int64_t)ii * a.stride + m] = make_double2(host, ads);
+}
+
+// Sums the kx slices in order and adds the excluded-pair, self and
+// net-charge terms (src/ewald.c:7318-7335, 7536-7565).
+template <int NC>
+__global__ __launch_bounds__(256) void k_fourier_final(const FourierArgs a,
+                                                       const double2 *__restrict__ part) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = a.n_dev ? (int64_t)(*a.n_dev) : a.n;
+  if (m >= n) return;
+  const EwaldDev &e = a.e;
+  double host = 0.0, ads = 0.0;
+  for (int ii = 0; ii <= e.kx; ++ii) {
+    const double2 p = part[(int64_t)ii * a.stride + m];
+    host += p.x;
+    ads += p.y;
+  }
+  double pxs[NC], pys[NC], pzs[NC], qs[NC];
+  double net = 0.0;
+#pragma unroll
+  for (int t = 0; t < NC; ++t) {
+    const int j = a.cidx[t];
+    pxs[t] = a.px[j * a.stride + m];
+    pys[t] = a.py[j * a.stride + m];
+    pzs[t] = a.pz[j * a.stride + m];
+    qs[t] = a.q[t];
+    net += a.q[t];
+  }
   double excl = 0.0;
 #pragma unroll
-  for (int p = 0; p < nc; ++p)
+  for (int p = 0; p < NC; ++p)
 #pragma unroll
-    for (int r = p + 1; r < nc; ++r) {
+    for (int r = p + 1; r < NC; ++r) {
 #pragma clang fp contract(off)
       double dx = pxs[p] - pxs[r], dy = pys[p] - pys[r], dz = pzs[p] - pzs[r];
       double rd = sqrt(dx * dx + dy * dy + dz * dz);
@@ -708,25 +727,29 @@ __global__ __launch_bounds__(128) void k_fourier(const FourierArgs a) {
   dads += e.uion * ((0.0 + dq) * (0.0 + dq)) - e.uion * (0.0 * 0.0);
   double dhost = 2.0 * host;
   dhost += 2.0 * e.uion * e.net_q_fw * (0.0 + dq) - 2.0 * e.uion * e.net_q_fw * 0.0;
-  if (live) {
-    a.d_host[m] = dhost;
-    a.d_ads[m] = dads;
-  }
+  a.d_host[m] = dhost;
+  a.d_ads[m] = dads;
 }
 
-void launch_fourier(const FourierArgs &a, hipStream_t st) {
-  const int bs = 128;
-  const unsigned nb = (unsigned)((a.n + bs - 1) / bs);
-  if (!nb) return;
+template <int NC>
+static void fourier_nc(const FourierArgs &a, double2 *part, hipStream_t st) {
+  const unsigned nb = (unsigned)((a.n + 127) / 128);
+  hipLaunchKernelGGL(k_fourier_part<NC>, dim3(nb, a.e.kx + 1), dim3(128), 0, st, a, part);
+  hipLaunchKernelGGL(k_fourier_final<NC>, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, st, a,
+                     (const double2 *)part);
+}
+
+void launch_fourier(const FourierArgs &a, double2 *part, hipStream_t st) {
+  if (a.n <= 0) return;
   switch (a.nc) {
-  case 1: hipLaunchKernelGGL(k_fourier<1>, dim3(nb), dim3(bs), 0, st, a); break;
-  case 2: hipLaunchKernelGGL(k_fourier<2>, dim3(nb), dim3(bs), 0, st, a); break;
-  case 3: hipLaunchKernelGGL(k_fourier<3>, dim3(nb), dim3(bs), 0, st, a); break;
-  case 4: hipLaunchKernelGGL(k_fourier<4>, dim3(nb), dim3(bs), 0, st, a); break;
-  case 5: hipLaunchKernelGGL(k_fourier<5>, dim3(nb), dim3(bs), 0, st, a); break;
-  case 6: hipLaunchKernelGGL(k_fourier<6>, dim3(nb), dim3(bs), 0, st, a); break;
-  case 7: hipLaunchKernelGGL(k_fourier<7>, dim3(nb), dim3(bs), 0, st, a); break;
-  case 8: hipLaunchKernelGGL(k_fourier<8>, dim3(nb), dim3(bs), 0, st, a); break;
+  case 1: fourier_nc<1>(a, part, st); break;
+  case 2: fourier_nc<2>(a, part, st); break;
+  case 3: fourier_nc<3>(a, part, st); break;
+  case 4: fourier_nc<4>(a, part, st); break;
+  case 5: fourier_nc<5>(a, part, st); break;
+  case 6: fourier_nc<6>(a, part, st); break;
+  case 7: fourier_nc<7>(a, part, st); break;
+  case 8: fourier_nc<8>(a, part, st); break;
   default: break;
   }
 }
