@@ -183,6 +183,28 @@ int rhg_widom_run(rhg_ctx *ctx, const rhg_molecule *mol, uint64_t seed, int64_t 
  * uniforms.  No blocks; rec is required. */
 int rhg_widom_run_trials(rhg_ctx *ctx, const rhg_molecule *mol, int64_t n, const double *fb,
                          const double *cord, const double *u_sel, double *rec);
+/* Asynchronous form of rhg_widom_run, for callers that keep several systems
+ * in flight (RASPA2 runs its systems one after another, src/monte_carlo.c:
+ * 626-748; a screening loop over many frameworks wants them concurrent):
+ * rhg_widom_submit enqueues the insertions on the context's stream and
+ * returns; the block sums of every submit since the last collect accumulate
+ * on the device.  rhg_widom_query returns 1 when the stream is idle, 0 if
+ * work is pending.  rhg_widom_collect waits, ADDS the sums to *blocks
+ * (nullable) and clears them.  Results are bitwise those of rhg_widom_run
+ * over the same ranges. */
+int rhg_widom_submit(rhg_ctx *ctx, const rhg_molecule *mol, uint64_t seed, int64_t n_total,
+                     int64_t first, int64_t n, int32_t steps_per_cycle);
+int rhg_widom_query(rhg_ctx *ctx);
+int rhg_widom_collect(rhg_ctx *ctx, rhg_widom_blocks *blocks);
+/* Execution options of the Widom path (no counterpart in the reference):
+ * chunk = insertions per pipeline pass (1..2^24, default 2^20; the block
+ * sums are grouped by chunk, so equal chunks give bitwise equal sums);
+ * graphs = 1 replays chunks after the first as one captured HIP graph. */
+int rhg_set_widom_options(rhg_ctx *ctx, int64_t chunk, int32_t graphs);
+/* Host-side fault hook for failure-handling tests (no counterpart in the
+ * reference): the chunk after `after_chunks` more chunks returns
+ * RHG_ERR_FAULT_INJECTED before launching anything; < 0 disables it. */
+int rhg_inject_fault(rhg_ctx *ctx, int64_t after_chunks);
 /* GetAverageHenryCoefficient / block prints (src/statistics.c:2539,
  * 4747-4800): out[0..4] K_H per block [mol/kg/Pa], out[5] average, out[6]
  * 2 sigma, out[7..11] <U_gh>_1-<U_h>_0 per block [K], out[12] average,

@@ -150,7 +150,20 @@ struct rhg_ctx {
   // insertions, first-bead sites, orientation sites, Coulomb sites of the
   // selected orientations, Fourier molecules, LJ pairs, LJ+Coulomb pairs,
   // Coulomb-only pairs, k-vector x charged-site terms
-  double wstats[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // device accumulators of the Widom path: [0,15) block sums of the
+  // current collection (sum_w[5], sum_wdu[5], count[5]), [16,25) the work
+  // counters above since the last reset
+  double *d_acc = nullptr;
+  int64_t *d_ctl = nullptr;
+  int32_t *d_neff = nullptr;
+  int64_t chunk = (int64_t)1 << 20;
+  int graphs = 1;
+  // buffer generation: bumped whenever a device pointer that a captured
+  // chunk holds may have changed
+  uint64_t gen = 0;
+  std::map<std::string, hipGraphExec_t> graph_cache;
+  // host-side fault hook: fail the chunk after this many more chunks (<0 off)
+  int64_t fault_after = -1;
   std::map<std::string, std::pair<int64_t, double>> stats;
   std::vector<Pending> pending;
   std::vector<hipEvent_t> event_pool;
@@ -196,6 +209,7 @@ void *work(rhg_ctx *c, const char *name, size_t bytes) {
   if (b.p) hipFree(b.p);
   b.p = nullptr;
   b.bytes = 0;
+  c->gen++;
   size_t want = std::max<size_t>(bytes, 256);
   if (hipMalloc(&b.p, want) != hipSuccess) {
     b.p = nullptr;
@@ -488,6 +502,7 @@ int ensure_cell_grid(rhg_ctx *c) {
   c->cg.nb_off = c->d_nb_off;
   c->cg.nb_idx = c->d_nb_idx;
   c->grid_ready = true;
+  c->gen++;
   return RHG_OK;
 }
 
@@ -513,10 +528,7 @@ int site_energy(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const Energy
   b.tile_off = (int32_t *)work(c, (t + "_tiles").c_str(), (size_t)(ncell + 1) * sizeof(int32_t));
   bool fresh = c->bufs["pair_counter"].p == nullptr;
   b.pairs = (unsigned long long *)work(c, "pair_counter", 64);
-  if (fresh && b.pairs) {
-    unsigned long long z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    HIPC(c, hipMemcpyAsync(b.pairs, z, sizeof(z), hipMemcpyHostToDevice, c->stream));
-  }
+  if (fresh && b.pairs) fill((double *)b.pairs, 8, 0.0, c->stream); // all-zero bits
   if (!b.cell_of || !b.perm || !b.count || !b.fill || !b.start || !b.tile_off || !b.pairs)
     return fail(c, RHG_ERR_OOM, "binning scratch");
   {
@@ -646,6 +658,10 @@ void rhg_destroy(rhg_ctx *c) {
   hipFree(c->d_az);
   hipFree(c->d_row_off);
   hipFree(c->d_row_kz);
+  hipFree(c->d_acc);
+  hipFree(c->d_ctl);
+  hipFree(c->d_neff);
+  for (auto &kv : c->graph_cache) hipGraphExecDestroy(kv.second);
   for (auto &p : c->pending) {
     hipEventDestroy(p.a);
     hipEventDestroy(p.b);
@@ -810,9 +826,10 @@ int build_mol(rhg_ctx *c, const rhg_molecule *m, MolDev &d) {
 
 struct WidomBufs {
   WidomWork w;
-  int64_t *d_bounds;
-  double *d_part, *d_out10;
+  double *d_part;
 };
+
+constexpr int kSeg = 4096; // insertions per block-reduction segment
 
 int widom_buffers(rhg_ctx *c, const MolDev &m, int64_t cap, WidomBufs &B) {
   WidomWork &w = B.w;
@@ -862,17 +879,50 @@ int widom_buffers(rhg_ctx *c, const MolDev &m, int64_t cap, WidomBufs &B) {
   WB(f_host, "w_fhost", cap * d8);
   WB(f_ads, "w_fads", cap * d8);
 #undef WB
-  B.d_bounds = (int64_t *)work(c, "w_bounds", 8 * sizeof(int64_t));
-  const int seg = 4096;
-  B.d_part = (double *)work(c, "w_part", ((cap + seg - 1) / seg + 1) * 10 * d8);
-  B.d_out10 = (double *)work(c, "w_out10", 16 * d8);
-  if (!B.d_bounds || !B.d_part || !B.d_out10) return fail(c, RHG_ERR_OOM, "widom reduce buffers");
+  B.d_part = (double *)work(c, "w_part", ((cap + kSeg - 1) / kSeg + 1) * 10 * d8);
+  if (!B.d_part) return fail(c, RHG_ERR_OOM, "widom reduce buffers");
   return RHG_OK;
 }
 
-// One chunk of the Widom pipeline (all launches on the context stream).
+// accumulators and control block of the context (allocated once)
+int widom_state(rhg_ctx *c) {
+  if (c->d_acc) return RHG_OK;
+  int rc;
+  if ((rc = dalloc(c, (void **)&c->d_acc, 32 * sizeof(double)))) return rc;
+  if ((rc = dalloc(c, (void **)&c->d_ctl, CTL_N * sizeof(int64_t)))) return rc;
+  if ((rc = dalloc(c, (void **)&c->d_neff, 64))) return rc;
+  fill(c->d_acc, 32, 0.0, c->stream);
+  c->gen++;
+  return RHG_OK;
+}
+
+StatArgs stat_args(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, int ewald) {
+  StatArgs sa;
+  sa.n_atoms = (double)c->n;
+  sa.nk = (double)c->nk;
+  sa.k_fb = m.k_fb;
+  sa.k_or = m.k_or;
+  sa.n_place = m.n_place;
+  sa.n_charged = 0;
+  for (int j = 0; j < m.n_sites; ++j) sa.n_charged += m.charged[j];
+  sa.n_cc_place = 0;
+  if (m.n_sites > 1 && c->charge_method == RHG_CHARGE_EWALD)
+    for (int p = 0; p < m.n_place; ++p) sa.n_cc_place += m.charged[m.place[p]];
+  const int type0 = mol->type[m.start];
+  sa.fb_charged = (c->charge_method == RHG_CHARGE_EWALD && considered_charged(c->types[type0].q)) ? 1 : 0;
+  sa.ewald = ewald;
+  sa.multi = m.n_sites > 1 ? 1 : 0;
+  return sa;
+}
+
+// One chunk of the Widom pipeline, all launches on the context stream and
+// none of them waiting for the host: trial generation, first-bead energies
+// and selection, orientation energies and selection, Coulomb of the
+// selected beads, Fourier change, finalisation, then (accumulate) the block
+// sums and work counters into the device accumulators and the control block
+// moved on by the chunk.
 int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs &B,
-                const WidomArgs &a) {
+                const WidomArgs &a, bool accumulate) {
   WidomWork &w = B.w;
   const int64_t n = a.n;
   hipStream_t st = c->stream;
@@ -887,7 +937,7 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
   {
     EnergyArgs ea = energy_args(c, type0, 1.0, 1, 1);
     if (!ea.do_cc) fill(w.fb_uc, n * m.k_fb, 0.0, st);
-    SiteList s{w.fbx, w.fby, w.fbz, n * m.k_fb, nullptr, 1};
+    SiteList s{w.fbx, w.fby, w.fbz, n * m.k_fb, a.n_eff, m.k_fb};
     EnergyOut o{w.fb_uv, ea.do_cc ? w.fb_uc : nullptr, nullptr};
     if (int rc = site_energy(c, ea, s, o, "bfb")) return rc;
   }
@@ -911,7 +961,6 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
     }
     // real-space Coulomb of the placed beads of the selected orientation
     // (src/cbmc.c:1216; only the selected trial's value is used)
-    bool any_cc = false;
     for (int p = 0; p < m.n_place; ++p) {
       EnergyArgs ea = energy_args(c, mol->type[m.place[p]], 1.0, 0, 1);
       double *dst = w.cc_place + p * w.cap;
@@ -919,13 +968,11 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
         fill(dst, n, 0.0, st);
         continue;
       }
-      any_cc = true;
       const int j = m.place[p];
       SiteList s{w.cfx + j * w.cap, w.cfy + j * w.cap, w.cfz + j * w.cap, n, w.n_active2, 1};
       EnergyOut o{nullptr, dst, nullptr};
       if (int rc = site_energy(c, ea, s, o, "bcc")) return rc;
     }
-    (void)any_cc;
   }
   if (a.ewald) {
     FourierArgs f;
@@ -964,38 +1011,13 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
     Timed t(c, "widom_select");
     launch_widom_finalize(a, w, st);
   }
+  if (accumulate) {
+    Timed t(c, "block_reduce");
+    launch_block_reduce(a, w.W, w.dU, B.d_part, kSeg, c->d_acc, st);
+    launch_widom_stats(a, w, stat_args(c, mol, m, a.ewald), c->d_acc + 16, st);
+  }
+  launch_advance(c->d_ctl, n, st);
   if (hipGetLastError() != hipSuccess) return fail(c, RHG_ERR_HIP, "widom launch failed");
-  return RHG_OK;
-}
-
-// Adds the chunk's work to the context counters (reads the two compaction
-// counts back; the caller synchronises once per chunk anyway).
-int widom_count(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs &B,
-                const WidomArgs &a) {
-  int32_t na[2] = {0, 0};
-  HIPC(c, hipMemcpyAsync(&na[0], B.w.n_active, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipMemcpyAsync(&na[1], B.w.n_active2, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));
-  const double nf = (double)c->n;
-  const double fb_sites = (double)a.n * m.k_fb;
-  const double or_sites = m.n_sites > 1 ? (double)na[0] * m.k_or * m.n_place : 0.0;
-  const int type0 = mol->type[m.start];
-  const bool cc0 = c->charge_method == RHG_CHARGE_EWALD && considered_charged(c->types[type0].q);
-  double cc_sites = 0.0;
-  if (m.n_sites > 1 && c->charge_method == RHG_CHARGE_EWALD)
-    for (int p = 0; p < m.n_place; ++p)
-      if (m.charged[m.place[p]]) cc_sites += (double)na[1];
-  int nc = 0;
-  for (int j = 0; j < m.n_sites; ++j) nc += m.charged[j];
-  c->wstats[0] += (double)a.n;
-  c->wstats[1] += fb_sites;
-  c->wstats[2] += or_sites;
-  c->wstats[3] += cc_sites;
-  c->wstats[4] += (a.ewald && nc) ? (double)na[1] : 0.0;
-  c->wstats[5] += (cc0 ? 0.0 : fb_sites * nf) + or_sites * nf;
-  c->wstats[6] += cc0 ? fb_sites * nf : 0.0;
-  c->wstats[7] += cc_sites * nf;
-  c->wstats[8] += (a.ewald && nc) ? (double)na[1] * nc * (double)c->nk : 0.0;
   return RHG_OK;
 }
 
@@ -1023,67 +1045,188 @@ void block_bounds(int64_t n_total, int spc, int64_t bounds[6]) {
   bounds[RHG_NR_BLOCKS] = n_total;
 }
 
-} // namespace
-
-extern "C" {
-
-int rhg_widom_run(rhg_ctx *c, const rhg_molecule *mol, uint64_t seed, int64_t n_total, int64_t first,
-                  int64_t n, int32_t steps_per_cycle, rhg_widom_blocks *blocks, double *rec) {
-  if (!c || !mol) return RHG_ERR_INVALID;
-  if (n_total < 0 || first < 0 || n < 0 || first + n > n_total || steps_per_cycle < 1)
-    return fail(c, RHG_ERR_INVALID, "bad insertion range");
-  MolDev m;
-  if (int rc = build_mol(c, mol, m)) return rc;
-  if (n == 0) return RHG_OK;
-  hipSetDevice(c->device);
-  const int64_t chunk = std::min<int64_t>(n, (int64_t)1 << 20);
-  WidomBufs B;
-  if (int rc = widom_buffers(c, m, chunk, B)) return rc;
-  int64_t jb[6];
-  block_bounds(n_total, steps_per_cycle, jb);
+WidomArgs widom_args(rhg_ctx *c, const MolDev &m, uint64_t seed, int64_t cap) {
   WidomArgs a;
   memset(&a, 0, sizeof(a));
   a.mol = m;
   a.cell = c->cell;
   a.beta = c->beta;
   a.seed = seed;
+  a.n = cap;
+  a.ctl = c->d_ctl;
+  a.n_eff = c->d_neff;
   a.use_host_trials = 0;
   a.ewald = (c->charge_method == RHG_CHARGE_EWALD) ? 1 : 0;
-  const int seg = 4096;
-  for (int64_t off = 0; off < n; off += chunk) {
-    a.first = first + off;
-    a.n = std::min(chunk, n - off);
-    if (int rc = widom_chunk(c, mol, m, B, a)) return rc;
-    if (int rc = widom_count(c, mol, m, B, a)) return rc;
-    if (rec) {
-      double *drec = (double *)work(c, "w_rec", (size_t)a.n * RHG_REC_N * sizeof(double));
-      if (!drec) return fail(c, RHG_ERR_OOM, "rec buffer");
-      launch_widom_records(a, B.w, drec, c->stream);
-      HIPC(c, hipMemcpyAsync(rec + off * RHG_REC_N, drec, (size_t)a.n * RHG_REC_N * sizeof(double),
-                             hipMemcpyDeviceToHost, c->stream));
+  return a;
+}
+
+int check_fault(rhg_ctx *c) {
+  if (c->fault_after < 0) return RHG_OK;
+  if (c->fault_after == 0) {
+    c->fault_after = -1;
+    return fail(c, RHG_ERR_FAULT_INJECTED, "fault injected by the host-side hook");
+  }
+  c->fault_after--;
+  return RHG_OK;
+}
+
+// Enqueues insertions [first, first+n) of a job of n_total insertions; the
+// block sums and work counters go into the device accumulators.  With
+// graphs enabled and more than one chunk, the first chunk runs eagerly (it
+// sizes every buffer) and the others replay one captured chunk.
+int widom_enqueue(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, uint64_t seed, int64_t n_total,
+                  int64_t first, int64_t n, int32_t spc, double *rec) {
+  hipStream_t st = c->stream;
+  const int64_t cap = std::min<int64_t>(n, c->chunk);
+  WidomBufs B;
+  if (int rc = widom_state(c)) return rc;
+  if (int rc = widom_buffers(c, m, cap, B)) return rc;
+  int64_t jb[6];
+  block_bounds(n_total, spc, jb);
+  launch_set_ctl(c->d_ctl, first, first + n, jb, st);
+  const int64_t n_chunks = (n + cap - 1) / cap;
+  const bool use_graph = c->graphs && !c->prof && !rec && n_chunks > 1 && c->fault_after < 0;
+  for (int64_t k = 0; k < n_chunks; ++k) {
+    const int64_t off = k * cap;
+    const int64_t cnt = std::min(cap, n - off);
+    if (int rc = check_fault(c)) return rc;
+    if (use_graph && k > 0) {
+      // one captured chunk, keyed by everything its launches hold
+      std::string key((const char *)&m, sizeof(m));
+      key.append((const char *)mol->type, sizeof(mol->type));
+      key.append((const char *)&seed, sizeof(seed));
+      key.append((const char *)&cap, sizeof(cap));
+      key.append((const char *)&c->gen, sizeof(c->gen));
+      key.append((const char *)&c->precision, sizeof(c->precision));
+      key.append((const char *)&c->cull, sizeof(c->cull));
+      auto it = c->graph_cache.find(key);
+      if (it == c->graph_cache.end()) {
+        const uint64_t gen0 = c->gen;
+        WidomArgs a = widom_args(c, m, seed, cap);
+        HIPC(c, hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        int rc = widom_chunk(c, mol, m, B, a, true);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(st, &g);
+        if (rc) {
+          if (g) hipGraphDestroy(g);
+          return rc;
+        }
+        if (e != hipSuccess) return fail(c, RHG_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+        hipGraphExec_t ge = nullptr;
+        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(c, RHG_ERR_HIP, std::string("instantiate: ") + hipGetErrorString(e));
+        if (c->gen != gen0) {
+          // a buffer moved while capturing: this graph is stale
+          hipGraphExecDestroy(ge);
+          return fail(c, RHG_ERR_HIP, "buffers changed during capture");
+        }
+        if (c->graph_cache.size() >= 16) {
+          for (auto &kv : c->graph_cache) hipGraphExecDestroy(kv.second);
+          c->graph_cache.clear();
+        }
+        it = c->graph_cache.emplace(key, ge).first;
+      }
+      HIPC(c, hipGraphLaunch(it->second, st));
+      continue;
     }
-    if (blocks) {
-      int64_t lb[8];
-      for (int b = 0; b < 6; ++b) lb[b] = std::min(std::max(jb[b] - a.first, (int64_t)0), a.n);
-      HIPC(c, hipMemcpyAsync(B.d_bounds, lb, 6 * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
-      {
-        Timed t(c, "block_reduce");
-        launch_block_reduce(B.w.W, B.w.dU, a.n, B.d_bounds, B.d_part, seg, B.d_out10, c->stream);
-      }
-      double o10[10];
-      HIPC(c, hipMemcpyAsync(o10, B.d_out10, 10 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-      HIPC(c, hipStreamSynchronize(c->stream));
-      const double V = c->props[6];
-      for (int b = 0; b < RHG_NR_BLOCKS; ++b) {
-        double cnt = (double)(lb[b + 1] - lb[b]);
-        blocks->count[b] += cnt;
-        blocks->ideal_gas[b] += cnt * (V / ((double)0 + 1.0));
-        blocks->sum_w[b] += o10[2 * b];
-        blocks->sum_wdu[b] += o10[2 * b + 1];
-      }
+    WidomArgs a = widom_args(c, m, seed, use_graph ? cap : cnt);
+    if (int rc = widom_chunk(c, mol, m, B, a, true)) return rc;
+    if (rec) {
+      double *drec = (double *)work(c, "w_rec", (size_t)cnt * RHG_REC_N * sizeof(double));
+      if (!drec) return fail(c, RHG_ERR_OOM, "rec buffer");
+      launch_widom_records(a, B.w, drec, st);
+      HIPC(c, hipMemcpyAsync(rec + off * RHG_REC_N, drec, (size_t)cnt * RHG_REC_N * sizeof(double),
+                             hipMemcpyDeviceToHost, st));
     }
   }
+  return RHG_OK;
+}
+
+// Adds the device block sums to `blocks` and clears them (waits for the
+// stream).
+int widom_collect(rhg_ctx *c, rhg_widom_blocks *blocks) {
+  if (!c->d_acc) return RHG_OK;
+  double acc[16];
+  HIPC(c, hipMemcpyAsync(acc, c->d_acc, 15 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  fill(c->d_acc, 15, 0.0, c->stream);
   HIPC(c, hipStreamSynchronize(c->stream));
+  if (blocks) {
+    const double V = c->props[6];
+    for (int b = 0; b < RHG_NR_BLOCKS; ++b) {
+      blocks->count[b] += acc[10 + b];
+      // ideal-gas Rosenbluth weight of an insertion into N = 0 adsorbates:
+      // V / (N + 1) (src/mc_moves.c:7929)
+      blocks->ideal_gas[b] += acc[10 + b] * (V / ((double)0 + 1.0));
+      blocks->sum_w[b] += acc[b];
+      blocks->sum_wdu[b] += acc[5 + b];
+    }
+  }
+  return RHG_OK;
+}
+
+int check_range(rhg_ctx *c, const rhg_molecule *mol, int64_t n_total, int64_t first, int64_t n,
+                int32_t spc) {
+  if (!c || !mol) return RHG_ERR_INVALID;
+  if (n_total < 0 || first < 0 || n < 0 || first + n > n_total || spc < 1)
+    return fail(c, RHG_ERR_INVALID, "bad insertion range");
+  return RHG_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int rhg_widom_run(rhg_ctx *c, const rhg_molecule *mol, uint64_t seed, int64_t n_total, int64_t first,
+                  int64_t n, int32_t steps_per_cycle, rhg_widom_blocks *blocks, double *rec) {
+  if (int rc = check_range(c, mol, n_total, first, n, steps_per_cycle)) return rc;
+  MolDev m;
+  if (int rc = build_mol(c, mol, m)) return rc;
+  if (n == 0) return RHG_OK;
+  hipSetDevice(c->device);
+  // sums of earlier submits that were never collected stay out of this call
+  if (int rc = widom_state(c)) return rc;
+  fill(c->d_acc, 15, 0.0, c->stream);
+  if (int rc = widom_enqueue(c, mol, m, seed, n_total, first, n, steps_per_cycle, rec)) return rc;
+  return widom_collect(c, blocks);
+}
+
+int rhg_widom_submit(rhg_ctx *c, const rhg_molecule *mol, uint64_t seed, int64_t n_total, int64_t first,
+                     int64_t n, int32_t steps_per_cycle) {
+  if (int rc = check_range(c, mol, n_total, first, n, steps_per_cycle)) return rc;
+  MolDev m;
+  if (int rc = build_mol(c, mol, m)) return rc;
+  if (n == 0) return RHG_OK;
+  hipSetDevice(c->device);
+  return widom_enqueue(c, mol, m, seed, n_total, first, n, steps_per_cycle, nullptr);
+}
+
+int rhg_widom_query(rhg_ctx *c) {
+  if (!c) return RHG_ERR_INVALID;
+  hipSetDevice(c->device);
+  hipError_t e = hipStreamQuery(c->stream);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return fail(c, RHG_ERR_HIP, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+}
+
+int rhg_widom_collect(rhg_ctx *c, rhg_widom_blocks *blocks) {
+  if (!c) return RHG_ERR_INVALID;
+  hipSetDevice(c->device);
+  return widom_collect(c, blocks);
+}
+
+int rhg_set_widom_options(rhg_ctx *c, int64_t chunk, int32_t graphs) {
+  if (!c) return RHG_ERR_INVALID;
+  if (chunk < 1 || chunk > ((int64_t)1 << 24)) return fail(c, RHG_ERR_INVALID, "chunk out of range [1, 2^24]");
+  c->chunk = chunk;
+  c->graphs = graphs ? 1 : 0;
+  return RHG_OK;
+}
+
+int rhg_inject_fault(rhg_ctx *c, int64_t after_chunks) {
+  if (!c) return RHG_ERR_INVALID;
+  c->fault_after = after_chunks < 0 ? -1 : after_chunks;
   return RHG_OK;
 }
 
@@ -1095,6 +1238,7 @@ int rhg_widom_run_trials(rhg_ctx *c, const rhg_molecule *mol, int64_t n, const d
   if (m.n_sites > 1 && !cord) return fail(c, RHG_ERR_INVALID, "cord required");
   if (n == 0) return RHG_OK;
   hipSetDevice(c->device);
+  if (int rc = widom_state(c)) return rc;
   WidomBufs B;
   if (int rc = widom_buffers(c, m, n, B)) return rc;
   size_t bfb = (size_t)n * m.k_fb * 3 * sizeof(double);
@@ -1109,16 +1253,11 @@ int rhg_widom_run_trials(rhg_ctx *c, const rhg_molecule *mol, int64_t n, const d
   B.w.h_fb = hfb;
   B.w.h_cord = hcord;
   B.w.h_usel = hsel;
-  WidomArgs a;
-  memset(&a, 0, sizeof(a));
-  a.mol = m;
-  a.cell = c->cell;
-  a.beta = c->beta;
-  a.first = 0;
-  a.n = n;
+  int64_t jb[6] = {0, n, n, n, n, n};
+  launch_set_ctl(c->d_ctl, 0, n, jb, c->stream);
+  WidomArgs a = widom_args(c, m, 0, n);
   a.use_host_trials = 1;
-  a.ewald = (c->charge_method == RHG_CHARGE_EWALD) ? 1 : 0;
-  if (int rc = widom_chunk(c, mol, m, B, a)) return rc;
+  if (int rc = widom_chunk(c, mol, m, B, a, false)) return rc;
   double *drec = (double *)work(c, "w_rec", (size_t)n * RHG_REC_N * sizeof(double));
   if (!drec) return fail(c, RHG_ERR_OOM, "rec buffer");
   launch_widom_records(a, B.w, drec, c->stream);
@@ -1412,24 +1551,20 @@ int rhg_profile_reset(rhg_ctx *c) {
 int rhg_widom_stats(rhg_ctx *c, double out[12], int32_t reset) {
   if (!c || !out) return RHG_ERR_INVALID;
   hipSetDevice(c->device);
-  for (int i = 0; i < 9; ++i) out[i] = c->wstats[i];
+  double st[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   unsigned long long pairs[3] = {0, 0, 0};
   DevBuf &pb = c->bufs["pair_counter"];
-  if (pb.p) {
-    HIPC(c, hipMemcpyAsync(pairs, pb.p, sizeof(pairs), hipMemcpyDeviceToHost, c->stream));
-    HIPC(c, hipStreamSynchronize(c->stream));
+  if (c->d_acc) HIPC(c, hipMemcpyAsync(st, c->d_acc + 16, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+  if (pb.p) HIPC(c, hipMemcpyAsync(pairs, pb.p, sizeof(pairs), hipMemcpyDeviceToHost, c->stream));
+  if (reset) {
+    if (c->d_acc) fill(c->d_acc + 16, 9, 0.0, c->stream);
+    if (pb.p) fill((double *)pb.p, 3, 0.0, c->stream); // all-zero bits
   }
+  HIPC(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < 9; ++i) out[i] = st[i];
   out[9] = (double)pairs[0];
   out[10] = (double)pairs[1];
   out[11] = (double)pairs[2];
-  if (reset) {
-    for (int i = 0; i < 9; ++i) c->wstats[i] = 0.0;
-    if (pb.p) {
-      unsigned long long z[3] = {0, 0, 0};
-      HIPC(c, hipMemcpyAsync(pb.p, z, sizeof(z), hipMemcpyHostToDevice, c->stream));
-      HIPC(c, hipStreamSynchronize(c->stream));
-    }
-  }
   return RHG_OK;
 }
 

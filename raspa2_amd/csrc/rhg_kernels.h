@@ -126,17 +126,32 @@ struct WidomWork {
   const double *h_fb, *h_cord, *h_usel;
 };
 
+// Control block of a Widom run, device-resident so that a captured chunk
+// can be replayed: ctl[0] = job index of the chunk's first insertion,
+// ctl[1] = end of the submitted range, ctl[2..7] = job indices where the
+// five blocks start (and the job end).
+enum { CTL_FIRST = 0, CTL_END = 1, CTL_JB = 2, CTL_N = 8 };
+
 struct WidomArgs {
   MolDev mol;
   Cell cell;
   double beta;
   uint64_t seed;
-  int64_t first; // global index of the first insertion of the chunk
-  int64_t n;     // insertions in this chunk
+  int64_t n;           // capacity of a chunk (grid size); the chunk runs min(n, end - first)
+  const int64_t *ctl;  // control block (above)
+  int32_t *n_eff;      // device: insertions of this chunk, written by fb_gen
   int use_host_trials;
   int ewald;
 };
 
+__device__ __forceinline__ int64_t chunk_first(const WidomArgs &a) { return a.ctl[CTL_FIRST]; }
+__device__ __forceinline__ int64_t chunk_n(const WidomArgs &a) {
+  const int64_t left = a.ctl[CTL_END] - a.ctl[CTL_FIRST];
+  return left < a.n ? (left > 0 ? left : 0) : a.n;
+}
+
+// writes the control block (values by argument, no host memory involved)
+void launch_set_ctl(int64_t *ctl, int64_t first, int64_t end, const int64_t jb[6], hipStream_t st);
 void launch_widom_fb_gen(const WidomArgs &a, const WidomWork &w, hipStream_t st);
 void launch_widom_fb_select(const WidomArgs &a, const WidomWork &w, hipStream_t st);
 void launch_widom_or_select(const WidomArgs &a, const WidomWork &w, hipStream_t st);
@@ -150,8 +165,21 @@ struct BlockPartials {
   double *part; // n_seg x 5 x 3
   int n_seg;
 };
-void launch_block_reduce(const double *W, const double *dU, int64_t n, const int64_t bounds[6],
-                         double *part, int seg, double *out15, hipStream_t st);
+// Adds the chunk's per-block sums of W and W*dU and insertion counts to
+// acc[15] (sum_w[5], sum_wdu[5], count[5]) in a fixed order; jb[6] are the
+// job-global insertion indices where each block starts.
+void launch_block_reduce(const WidomArgs &a, const double *W, const double *dU, double *part, int seg,
+                         double *acc, hipStream_t st);
+
+// Adds the chunk's work counts to stats[9] (see rhg_widom_stats).
+struct StatArgs {
+  double n_atoms, nk;
+  int k_fb, k_or, n_place, n_charged, n_cc_place, fb_charged, ewald, multi;
+};
+void launch_widom_stats(const WidomArgs &a, const WidomWork &w, const StatArgs &sa, double *stats,
+                        hipStream_t st);
+// ctl[CTL_FIRST] += step (moves a captured chunk on to the next insertions)
+void launch_advance(int64_t *ctl, int64_t step, hipStream_t st);
 
 // ----------------------------------------------------------------- grids
 struct GridArgs {

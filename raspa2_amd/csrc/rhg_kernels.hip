@@ -797,8 +797,11 @@ __device__ __forceinline__ void abc_to_xyz(const Cell &c, double sx, double sy, 
 // src/cbmc.c:652-667)
 __global__ __launch_bounds__(256) void k_widom_fb_gen(const WidomArgs a, const WidomWork w) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
+  const int64_t n = chunk_n(a);
+  if (i == 0 && a.n_eff) *a.n_eff = (int32_t)n;
+  if (i >= n) return;
   const int k = a.mol.k_fb;
+  const int64_t first = chunk_first(a);
   if (a.use_host_trials) {
     for (int t = 0; t < k; ++t) {
       w.fbx[i * k + t] = w.h_fb[(i * k + t) * 3 + 0];
@@ -807,7 +810,7 @@ __global__ __launch_bounds__(256) void k_widom_fb_gen(const WidomArgs a, const W
     }
     return;
   }
-  const uint64_t key = stream_key(a.seed, (uint64_t)(a.first + i));
+  const uint64_t key = stream_key(a.seed, (uint64_t)(first + i));
   uint64_t c = 0;
   for (int t = 0; t < k; ++t) {
     double sz = stream_uniform(key, c++);
@@ -826,7 +829,7 @@ __global__ __launch_bounds__(256) void k_widom_fb_gen(const WidomArgs a, const W
 // SimpleSphere (src/cbmc.c:1480-1578).
 __global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, const WidomWork w) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
+  if (i >= chunk_n(a)) return;
   const int k = a.mol.k_fb;
   const double beta = a.beta;
   uint64_t mask = 0;
@@ -862,7 +865,7 @@ __global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, cons
   if (a.use_host_trials)
     usel = w.h_usel[2 * i];
   else {
-    key = stream_key(a.seed, (uint64_t)(a.first + i));
+    key = stream_key(a.seed, (uint64_t)(chunk_first(a) + i));
     usel = stream_uniform(key, (uint64_t)(3 * k));
   }
   auto bf = [&](int t, double &b) -> bool {
@@ -1113,24 +1116,80 @@ void launch_widom_records(const WidomArgs &a, const WidomWork &w, double *rec, h
   if (a.n > 0) hipLaunchKernelGGL(k_widom_records, dim3(nblk(a.n, 256)), dim3(256), 0, st, a.n, w, rec);
 }
 
+__global__ void k_widom_stats(const WidomArgs a, const WidomWork w, const StatArgs sa,
+                              double *__restrict__ st) {
+  if (threadIdx.x != 0) return;
+  const double n = (double)chunk_n(a);
+  const double na1 = sa.multi ? (double)*w.n_active : 0.0;
+  const double na2 = (double)*w.n_active2;
+  const double fb = n * sa.k_fb;
+  const double orr = na1 * sa.k_or * sa.n_place;
+  const double cc = sa.multi ? na2 * sa.n_cc_place : 0.0;
+  st[0] += n;
+  st[1] += fb;
+  st[2] += orr;
+  st[3] += cc;
+  st[4] += (sa.ewald && sa.n_charged) ? na2 : 0.0;
+  st[5] += (sa.fb_charged ? 0.0 : fb * sa.n_atoms) + orr * sa.n_atoms;
+  st[6] += sa.fb_charged ? fb * sa.n_atoms : 0.0;
+  st[7] += cc * sa.n_atoms;
+  st[8] += (sa.ewald && sa.n_charged) ? na2 * sa.n_charged * sa.nk : 0.0;
+}
+
+void launch_widom_stats(const WidomArgs &a, const WidomWork &w, const StatArgs &sa, double *stats,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_widom_stats, dim3(1), dim3(64), 0, st, a, w, sa, stats);
+}
+
+__global__ void k_advance(int64_t *ctl, int64_t step) {
+  if (threadIdx.x == 0) ctl[CTL_FIRST] += step;
+}
+
+void launch_advance(int64_t *ctl, int64_t step, hipStream_t st) {
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, ctl, step);
+}
+
+struct CtlVals {
+  int64_t v[CTL_N];
+};
+
+__global__ void k_set_ctl(int64_t *ctl, const CtlVals c) {
+  if (threadIdx.x < CTL_N) ctl[threadIdx.x] = c.v[threadIdx.x];
+}
+
+void launch_set_ctl(int64_t *ctl, int64_t first, int64_t end, const int64_t jb[6], hipStream_t st) {
+  CtlVals c;
+  c.v[CTL_FIRST] = first;
+  c.v[CTL_END] = end;
+  for (int b = 0; b < 6; ++b) c.v[CTL_JB + b] = jb[b];
+  hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(64), 0, st, ctl, c);
+}
+
 // ====================================================================
 // deterministic block reduction: segments of `seg` insertions, each reduced
-// by one workgroup with a fixed tree; a single workgroup then reduces the
-// segment partials in order.
+// by one workgroup with a fixed tree; one workgroup then reduces the
+// segment partials in order and adds them to the job's accumulators.
 // ====================================================================
-__global__ __launch_bounds__(256) void k_block_part(const double *__restrict__ W,
-                                                    const double *__restrict__ dU, int64_t n,
-                                                    const int64_t *__restrict__ bounds, int seg,
+__device__ __forceinline__ int block_of(int64_t gi, const int64_t *jb) {
+  int b = 0;
+  while (b < 4 && gi >= jb[b + 1]) ++b;
+  return b;
+}
+
+__global__ __launch_bounds__(256) void k_block_part(const WidomArgs a, const double *__restrict__ W,
+                                                    const double *__restrict__ dU, int seg,
                                                     double *__restrict__ part) {
+  const int64_t *jb = a.ctl + CTL_JB;
   __shared__ double sh[256][10];
+  const int64_t n = chunk_n(a);
+  const int64_t first = chunk_first(a);
   const int64_t s0 = (int64_t)blockIdx.x * seg;
   double acc[10];
   for (int q = 0; q < 10; ++q) acc[q] = 0.0;
   for (int64_t t = threadIdx.x; t < seg; t += blockDim.x) {
     int64_t i = s0 + t;
     if (i >= n) break;
-    int b = 0;
-    while (b < 4 && i >= bounds[b + 1]) ++b;
+    const int b = block_of(first + i, jb);
     double w = W[i];
     acc[2 * b] += w;
     acc[2 * b + 1] += dU[i] * w;
@@ -1146,9 +1205,12 @@ __global__ __launch_bounds__(256) void k_block_part(const double *__restrict__ W
     for (int q = 0; q < 10; ++q) part[(int64_t)blockIdx.x * 10 + q] = sh[0][q];
 }
 
-__global__ __launch_bounds__(256) void k_block_final(const double *__restrict__ part, int nseg,
-                                                     double *__restrict__ out) {
+__global__ __launch_bounds__(256) void k_block_final(const WidomArgs a, const double *__restrict__ part,
+                                                     int seg, double *__restrict__ acc15) {
+  const int64_t *jb = a.ctl + CTL_JB;
   __shared__ double sh[256][10];
+  const int64_t n = chunk_n(a);
+  const int nseg = (int)((n + seg - 1) / seg);
   double acc[10];
   for (int q = 0; q < 10; ++q) acc[q] = 0.0;
   for (int s = threadIdx.x; s < nseg; s += blockDim.x)
@@ -1160,16 +1222,24 @@ __global__ __launch_bounds__(256) void k_block_final(const double *__restrict__ 
       for (int q = 0; q < 10; ++q) sh[threadIdx.x][q] += sh[threadIdx.x + s][q];
     __syncthreads();
   }
-  if (threadIdx.x == 0)
-    for (int q = 0; q < 10; ++q) out[q] = sh[0][q];
+  if (threadIdx.x == 0) {
+    const int64_t first = chunk_first(a);
+    for (int b = 0; b < 5; ++b) {
+      acc15[b] += sh[0][2 * b];
+      acc15[5 + b] += sh[0][2 * b + 1];
+      int64_t lo = jb[b] > first ? jb[b] : first;
+      int64_t hi = jb[b + 1] < first + n ? jb[b + 1] : first + n;
+      acc15[10 + b] += hi > lo ? (double)(hi - lo) : 0.0;
+    }
+  }
 }
 
-void launch_block_reduce(const double *W, const double *dU, int64_t n, const int64_t bounds_dev[6],
-                         double *part, int seg, double *out10, hipStream_t st) {
-  int nseg = (int)((n + seg - 1) / seg);
+void launch_block_reduce(const WidomArgs &a, const double *W, const double *dU, double *part, int seg,
+                         double *acc, hipStream_t st) {
+  const int nseg = (int)((a.n + seg - 1) / seg);
   if (nseg <= 0) return;
-  hipLaunchKernelGGL(k_block_part, dim3(nseg), dim3(256), 0, st, W, dU, n, bounds_dev, seg, part);
-  hipLaunchKernelGGL(k_block_final, dim3(1), dim3(256), 0, st, part, nseg, out10);
+  hipLaunchKernelGGL(k_block_part, dim3(nseg), dim3(256), 0, st, a, W, dU, seg, part);
+  hipLaunchKernelGGL(k_block_final, dim3(1), dim3(256), 0, st, a, (const double *)part, seg, acc);
 }
 
 // ====================================================================

@@ -21,6 +21,8 @@ CHARGE_EWALD = 1
 PREC_FP64 = 0
 PREC_MIXED = 1
 MAX_SITES = 8
+ERR_INVALID, ERR_HIP, ERR_OOM, ERR_NOT_ADMITTED, ERR_UNSUPPORTED, ERR_FAULT_INJECTED = (
+    -1, -2, -3, -4, -5, -6)
 NR_BLOCKS = 5
 REC_N = 14
 REC = dict(W=0, DU=1, UVDW=2, UREAL=3, UFHOST=4, UFADS=5, OVERLAP=6, SELFB=7, SELOR=8,
@@ -35,7 +37,11 @@ _PD = ctypes.POINTER(ctypes.c_double)
 
 
 class RhgError(RuntimeError):
-    pass
+    """A failed librhg call; `code` is the RHG_ERR_* status."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 class Framework(ctypes.Structure):
@@ -95,6 +101,11 @@ def _load():
     f("rhg_widom_run", _P, ctypes.POINTER(MoleculeT), _U64, _I64, _I64, _I64, _I32,
       ctypes.POINTER(WidomBlocks), _P)
     f("rhg_widom_run_trials", _P, ctypes.POINTER(MoleculeT), _I64, _P, _P, _P, _P)
+    f("rhg_widom_submit", _P, ctypes.POINTER(MoleculeT), _U64, _I64, _I64, _I64, _I32)
+    f("rhg_widom_query", _P)
+    f("rhg_widom_collect", _P, ctypes.POINTER(WidomBlocks))
+    f("rhg_set_widom_options", _P, _I64, _I32)
+    f("rhg_inject_fault", _P, _I64)
     f("rhg_widom_estimators", ctypes.POINTER(WidomBlocks), _D, _D, _D, _P)
     f("rhg_make_grid", _P, _I32, _P, _D, _I32, ctypes.POINTER(_P))
     f("rhg_grid_info", _P, _P)
@@ -231,7 +242,7 @@ class Context:
     def _check(self, rc, what):
         if rc:
             msg = lib.rhg_last_error(self.h)
-            raise RhgError(f"{what}: status {rc}: {msg.decode() if msg else ''}")
+            raise RhgError(f"{what}: status {rc}: {msg.decode() if msg else ''}", rc)
 
     def add_pseudo_atom(self, eps_k, sigma, charge=0.0):
         t = _I32()
@@ -304,6 +315,32 @@ class Context:
                                       steps_per_cycle, ctypes.byref(blocks), _ptr(rec)),
                     "widom_run")
         return blocks, rec
+
+    def widom_submit(self, mol, seed, n_total, first=0, n=None, steps_per_cycle=20):
+        """Enqueue insertions and return at once (sums accumulate on the device)."""
+        n = n_total - first if n is None else n
+        self._check(lib.rhg_widom_submit(self.h, ctypes.byref(mol), seed, n_total, first, n,
+                                         steps_per_cycle), "widom_submit")
+
+    def widom_query(self):
+        """True when every submitted insertion has finished."""
+        rc = lib.rhg_widom_query(self.h)
+        if rc < 0:
+            self._check(rc, "widom_query")
+        return rc == 1
+
+    def widom_collect(self, blocks=None):
+        """Wait, add the accumulated block sums to `blocks` and clear them."""
+        blocks = WidomBlocks() if blocks is None else blocks
+        self._check(lib.rhg_widom_collect(self.h, ctypes.byref(blocks)), "widom_collect")
+        return blocks
+
+    def set_widom_options(self, chunk=1 << 20, graphs=True):
+        self._check(lib.rhg_set_widom_options(self.h, chunk, 1 if graphs else 0),
+                    "set_widom_options")
+
+    def inject_fault(self, after_chunks):
+        self._check(lib.rhg_inject_fault(self.h, after_chunks), "inject_fault")
 
     def widom_run_trials(self, mol, fb, cord, usel):
         fb = _f64(fb)
