@@ -10,10 +10,18 @@ insertion, LJ biasing, T = 300 K.  Inputs are resident in HBM before the timed
 region; trial positions come from the per-insertion counter stream on the
 device.
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 1]
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--workload c1|c3|c4]
 
 Prints one JSON line.  The CPU baseline is the oracle (the FP64 C
 restatement) on the host cores over a bounded sample of the same insertions.
+
+--workload c3: batched screening of 256 synthetic frameworks (100-8000
+atoms, methane or CO2, 10^4-10^6 insertions each) through the screening
+service; value = insertions/s over the whole screen, with p50/p99 of the
+per-framework completion time.  --workload c4: Poisson arrivals of
+requests (methane, CO2 or N2-like, any of the 256 frameworks) at 0.7x the
+measured screening capacity; p50/p99 request latency, and the same
+requests run one at a time for comparison.
 """
 import argparse
 import json
@@ -66,6 +74,106 @@ def cpu_baseline(fw, probe, n_per_worker):
                 rec0=rec1)
 
 
+def _cpu_job_worker(args):
+    job, count = args
+    import oracle as O
+    fw = job.framework
+    s = O.System(fw.box, fw.bc, fw.xyz[:, 0], fw.xyz[:, 1], fw.xyz[:, 2], fw.charge, fw.eps_k,
+                 fw.sigma, mass=fw.mass, charge_method=O.CHARGE_EWALD, ewald_precision=1e-6,
+                 temperature=300.0)
+    types = [s.add_pseudo_atom(*site) for site in job.probe.sites]
+    mol = O.molecule(types, job.probe.body, start=job.probe.start, k_fb=job.probe.k_fb,
+                     k_or=job.probe.k_or)
+    t0 = time.perf_counter()
+    O.widom_run_stream(s, mol, job.seed, 0, count)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline_screening(jobs, count):
+    """Oracle on the host cores: `count` insertions of each of `jobs`, one
+    process per core; rate = insertions / summed process time x cores."""
+    import multiprocessing as mp
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        times = pool.map(_cpu_job_worker, [(j, count) for j in jobs])
+    wall = time.perf_counter() - t0
+    total = count * len(jobs)
+    return dict(aggregate=total / wall, single=total / sum(times), cores=cores, sample=total)
+
+
+def screening_bench(args):
+    """C3 (batched screening) and C4 (Poisson-arriving requests) through the
+    screening service; prints one JSON line."""
+    from raspa2_amd import rhg as R
+    from raspa2_amd import screening as SC
+    jobs = SC.catalogue(args.frameworks)
+    cpu = None
+    if not args.no_cpu:
+        sample_jobs = jobs[:min(16, len(jobs))]
+        cpu = cpu_baseline_screening(sample_jobs, args.cpu_sample // 2)
+    tuning = SC.load_tuning()
+    reqs = SC.screening_requests(jobs)
+    # bring the device up (not timed): a short screen of four frameworks
+    warm = [SC.Request(r.framework, r.probe, 10_000, r.seed) for r in reqs[:4]]
+    SC.run(jobs, warm, slots=args.slots, tuning=tuning)
+    c3 = SC.run(jobs, reqs, slots=args.slots, tuning=tuning)
+    s3 = SC.summary(c3, reqs)
+    out = {"metric": METRIC, "unit": "insertions/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "fp32 pair terms, fp64 accumulation/Ewald", "data": "synthetic"}
+    cpu_obj = None
+    if cpu is not None:
+        cpu_obj = {"value": round(cpu["aggregate"], 2), "unit": "insertions/s",
+                   "cores": cpu["cores"], "kind": "port", "single_core": round(cpu["single"], 2),
+                   "sample": f"{cpu['sample']} insertions: the first {args.cpu_sample // 2} of "
+                             f"each of the first 16 screening requests, oracle FP64 C "
+                             f"restatement, one process per core"}
+    if args.workload == "c3":
+        serial = SC.run(jobs, reqs, slots=1, tuning=tuning)
+        s1 = SC.summary(serial, reqs)
+        out.update({
+            "value": round(s3["rate"], 1), "ms_per_step": round(s3["wall_s"] * 1e3, 3),
+            "config": {"workload": f"C3 batched screening: {len(jobs)} synthetic frameworks "
+                                   "(100-8000 atoms, charged, Ewald 1e-6), methane or CO2, "
+                                   "10^4-10^6 insertions each, mixed precision",
+                       "frameworks": len(jobs), "insertions": s3["insertions"],
+                       "slots": args.slots, "tuning": tuning, "parallelism": "single GPU"},
+            "latency": {"p50_s": s3["p50_s"], "p99_s": s3["p99_s"],
+                        "what": "completion time of each framework's request, all submitted "
+                                "at t=0"},
+            "unbatched": {"value": round(s1["rate"], 1), "p50_s": s1["p50_s"],
+                          "p99_s": s1["p99_s"], "what": "same requests one at a time"},
+            "roofline": None, "cpu_baseline": cpu_obj, "server": c3["stats"],
+            "failed": s3["failed"]})
+    else:
+        capacity = len(reqs) / s3["wall_s"]
+        rate = 0.7 * capacity
+        r4 = SC.poisson_requests(args.requests, len(jobs), rate)
+        c4 = SC.run(jobs, r4, slots=args.slots, tuning=tuning)
+        s4 = SC.summary(c4, r4)
+        plain_reqs = [SC.Request(r.framework, r.probe, r.n_trials, r.seed, 0.0) for r in r4]
+        plain = SC.run(jobs, plain_reqs, slots=1, tuning=tuning)
+        sp = SC.summary(plain, plain_reqs)
+        out.update({
+            "value": round(s4["rate"], 1), "ms_per_step": round(s4["wall_s"] * 1e3, 3),
+            "config": {"workload": f"C4 serving: {len(r4)} Poisson requests at "
+                                   f"{rate:.2f}/s (0.7 x measured screening capacity "
+                                   f"{capacity:.2f}/s) over {len(jobs)} frameworks, methane / "
+                                   "CO2 / N2-like, 10^4-10^6 insertions each, mixed precision",
+                       "requests": len(r4), "arrival_rate_per_s": rate,
+                       "insertions": s4["insertions"], "slots": args.slots, "tuning": tuning,
+                       "parallelism": "single GPU"},
+            "latency": {"p50_s": s4["p50_s"], "p99_s": s4["p99_s"],
+                        "what": "arrival to completion of each request"},
+            "unbatched": {"value": round(sp["rate"], 1), "what": "same requests one at a time, "
+                                                                 "all present at the start"},
+            "roofline": None, "cpu_baseline": cpu_obj, "server": c4["stats"],
+            "failed": s4["failed"]})
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,12 +183,20 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--cpu-sample", type=int, default=400, help="insertions per CPU process")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["c1", "c3", "c4"], default="c1")
+    ap.add_argument("--frameworks", type=int, default=256, help="c3/c4 catalogue size")
+    ap.add_argument("--requests", type=int, default=384, help="c4 requests")
+    ap.add_argument("--slots", type=int, default=8, help="c3/c4 requests in flight")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the Widom "
               "insertions shard by index range with no exchange but a 5-block sum, see "
               "DESIGN.md", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload != "c1":
+        screening_bench(args)
+        return
 
     import numpy as np
     from raspa2_amd import rhg as R

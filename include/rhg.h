@@ -253,6 +253,73 @@ int rhg_widom_stats(rhg_ctx *ctx, double out[12], int32_t reset);
  * result is bit-identical to the all-pairs loop.  Default: on, 2.5 A. */
 int rhg_set_culling(rhg_ctx *ctx, int32_t on, double cell_size);
 
+/* ---------------------------------------------------- screening service */
+/* No counterpart in the reference, which answers one request per process
+ * (python/raspa2.py:97-110; src/monte_carlo.c:626-748 runs the systems of
+ * a process one after another).  A server holds a catalogue of frameworks,
+ * keeps the ones in use resident on the device (atom records, k-tables,
+ * cell lists) within an HBM budget, evicting idle ones least-recently-used
+ * first, and runs up to `slots` Widom requests at once, each on its own
+ * stream, starting the next arrived request as soon as a slot frees
+ * (continuous batching).  A request advances in slices of `slice`
+ * insertions; each slice's block sums are collected on the host, so a
+ * request's result is bitwise the same alone or beside others, and a slice
+ * that fails is discarded and run again (at most 3 times). */
+typedef struct rhg_server rhg_server;
+
+typedef struct {
+  int32_t n_sites;
+  double eps_k[RHG_MAX_SITES], sigma[RHG_MAX_SITES], charge[RHG_MAX_SITES];
+  double body[RHG_MAX_SITES][3];
+  int32_t start, k_fb, k_or;
+} rhg_probe;
+
+typedef struct {
+  int64_t framework;     /* id given to rhg_server_add_framework */
+  rhg_probe probe;
+  uint64_t seed;
+  int64_t n_insertions;
+  int32_t steps_per_cycle;
+  double arrival_s;      /* arrival, seconds after rhg_server_run starts */
+} rhg_request;
+
+typedef struct {
+  int32_t status;        /* RHG_OK or the error that ended the request */
+  int32_t retries;       /* slices run again after a failure */
+  rhg_widom_blocks blocks;
+  double queued_s, started_s, finished_s, latency_s; /* latency = finished - arrival */
+} rhg_result;
+
+/* hbm_budget 0 = 80% of the free device memory; precision RHG_PREC_*;
+ * slice = insertions per checkpointed slice. */
+int rhg_server_create(int device, size_t hbm_budget, int32_t slots, int32_t precision, int64_t slice,
+                      rhg_server **out);
+void rhg_server_destroy(rhg_server *s);
+const char *rhg_server_last_error(const rhg_server *s);
+int rhg_server_add_framework(rhg_server *s, int64_t id, const rhg_framework *fw, const rhg_cell *cell,
+                             const rhg_forcefield *ff);
+/* Per size bucket (frameworks of up to max_atoms[i] atoms): cell-list cell
+ * size and chunk (insertions per pipeline pass), e.g. from an autotuner. */
+int rhg_server_set_tuning(rhg_server *s, int32_t n, const int32_t *max_atoms, const double *cell_size,
+                          const int64_t *chunk);
+int rhg_server_submit(rhg_server *s, const rhg_request *req, int64_t *id);
+/* Host-side fault hook of one request (as rhg_inject_fault, on its first slice). */
+int rhg_server_inject_fault(rhg_server *s, int64_t id, int64_t after_chunks);
+/* Runs every submitted request to completion (arrivals are honoured). */
+int rhg_server_run(rhg_server *s);
+int rhg_server_result(const rhg_server *s, int64_t id, rhg_result *out);
+/* out[0] contexts created, [1] evictions, [2] peak resident bytes, [3]
+ * resident bytes, [4] slices retried, [5] budget, [6] idle wait [s],
+ * [7] requests */
+int rhg_server_stats(const rhg_server *s, double out[8]);
+/* Device bytes a context of this framework holds after Widom insertions of
+ * a probe of n_sites sites (n_charged charged, n_types distinct
+ * pseudo-atoms) in chunks of `chunk` (host-only arithmetic; the admission
+ * estimate of the server). */
+size_t rhg_estimate_footprint(const rhg_framework *fw, const rhg_cell *cell, const rhg_forcefield *ff,
+                              int32_t n_types, int64_t chunk, int32_t n_sites, int32_t n_charged,
+                              int32_t k_fb, int32_t k_or, double cell_size);
+
 /* ------------------------------------------------------ workload utility */
 /* No counterpart in the reference (its frameworks come from CIF files):
  * synthetic framework positions for benchmark and test workloads, by random

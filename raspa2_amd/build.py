@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librhg.so")
 
-SOURCES = ["rhg_kernels.hip", "rhg_host.cpp", "rhg_synth.cpp"]
+SOURCES = ["rhg_kernels.hip", "rhg_host.cpp", "rhg_synth.cpp", "rhg_serve.cpp"]
 HIPFLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 instructions are kept out of every code object

@@ -1,0 +1,497 @@
+// Screening service: many Widom requests over a catalogue of frameworks on
+// one GPU.  The reference answers one request per process (python/raspa2.py:
+// 97-110 runs `simulate` once per call; src/monte_carlo.c:626-748 runs the
+// systems of a process one after another).  Here a single host thread keeps
+// a resident cache of framework contexts (records, k-tables, cell lists),
+// admits requests by device-memory footprint, and keeps up to `slots`
+// requests in flight, each on its own context stream, refilling a slot as
+// soon as its request finishes (continuous batching).  Requests advance in
+// fixed slices of insertions whose block sums are collected on the host, so a
+// request's result does not depend on what else runs beside it, and a slice
+// that fails is simply run again.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "../../include/rhg.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct FrameworkSpec {
+  std::vector<double> x, y, z, q, eps, sig, mass;
+  rhg_cell cell;
+  rhg_forcefield ff;
+  // resident state
+  rhg_ctx *ctx = nullptr;
+  size_t bytes = 0;          // device bytes held by the context
+  uint64_t last_used = 0;    // scheduler tick of the last use (LRU)
+  bool busy = false;         // a request is running on it
+  std::map<std::tuple<double, double, double>, int32_t> types;
+  double cell_size = 2.5;    // tuned options of this framework's size bucket
+  int64_t chunk = (int64_t)1 << 20;
+};
+
+struct Job {
+  rhg_request req;
+  rhg_result res;
+  int64_t done = 0;          // insertions whose sums are in res.blocks
+  int64_t slice_n = 0;       // insertions of the slice in flight
+  bool started = false, finished = false;
+  rhg_molecule mol;
+  int64_t fault_after = -1;  // host fault hook for this request
+};
+
+struct Tuning {
+  int32_t max_atoms;
+  double cell_size;
+  int64_t chunk;
+};
+
+} // namespace
+
+struct rhg_server {
+  int device = 0;
+  size_t budget = 0;
+  int slots = 4;
+  int precision = RHG_PREC_MIXED;
+  int64_t slice = (int64_t)1 << 18;
+  std::map<int64_t, FrameworkSpec> fw;
+  std::vector<Job> jobs;
+  std::vector<Tuning> tuning;
+  std::string err;
+  uint64_t tick = 0;
+  size_t resident = 0, peak_resident = 0;
+  double n_created = 0, n_evicted = 0, n_retries = 0, busy_wait_s = 0;
+};
+
+namespace {
+
+int sfail(rhg_server *s, int code, const std::string &m) {
+  s->err = m;
+  return code;
+}
+
+double secs(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration<double>(b - a).count();
+}
+
+// Device bytes a context of this framework holds after a Widom run of a
+// probe with n_sites sites (n_charged of them charged, n_types distinct
+// pseudo-atoms) in chunks of `chunk` insertions: the buffers the context
+// allocates, counted as it allocates them.
+size_t estimate_bytes(const rhg_framework *f, const rhg_cell *cell, const rhg_forcefield *ff,
+                      int32_t n_types, int64_t chunk, int32_t n_sites, int32_t n_charged, int32_t k_fb,
+                      int32_t k_or, double cell_size) {
+  const double *B = cell->box;
+  // volume and perpendicular widths of the cell
+  double a[3] = {B[0], B[1], B[2]}, b[3] = {B[3], B[4], B[5]}, c[3] = {B[6], B[7], B[8]};
+  auto cross = [](const double *u, const double *v, double *w) {
+    w[0] = u[1] * v[2] - u[2] * v[1];
+    w[1] = u[2] * v[0] - u[0] * v[2];
+    w[2] = u[0] * v[1] - u[1] * v[0];
+  };
+  double bc[3], ca[3], ab[3];
+  cross(b, c, bc);
+  cross(c, a, ca);
+  cross(a, b, ab);
+  double V = std::fabs(a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2]);
+  auto norm = [](const double *u) { return std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]); };
+  double w[3] = {V / norm(bc), V / norm(ca), V / norm(ab)};
+  const double n = std::max(f->n_atoms, 1);
+  const bool ewald = ff->charge_method == RHG_CHARGE_EWALD;
+  int n_q = 0;
+  for (int i = 0; i < f->n_atoms; ++i)
+    if (f->charge && std::fabs(f->charge[i]) > 1e-10) ++n_q;
+  // atom records per pseudo-atom (64 + 40 + 3 x 8 B) and positions (24 B)
+  double bytes = n * (128.0 * std::max(n_types, 1) + 24.0);
+  // Ewald: the automatic alpha / kmax rule and the spherical k cut-off
+  // (src/ewald.c:450-530), counted exactly
+  int kx = 0;
+  if (ewald) {
+    double eps = std::min(std::fabs(ff->ewald_precision > 0 ? ff->ewald_precision : 1e-6), 0.5);
+    double rc = ff->cutoff_charge > 0 ? ff->cutoff_charge : ff->cutoff_vdw;
+    double tol = std::sqrt(std::fabs(std::log(eps * rc)));
+    double alpha = std::sqrt(std::fabs(std::log(eps * rc * tol))) / rc;
+    double tol1 = std::sqrt(-std::log(eps * rc * ((2.0 * tol * alpha) * (2.0 * tol * alpha))));
+    int k[3];
+    for (int d = 0; d < 3; ++d) {
+      double v = 0.25 + w[d] * alpha * tol1 / M_PI;
+      k[d] = (int)(v >= 0 ? v + 0.5 : v - 0.5);
+    }
+    int m = std::max(k[0], std::max(k[1], k[2]));
+    double cut = (1.05 * m) * (1.05 * m);
+    double nk = 0;
+    for (int i = 0; i <= k[0]; ++i)
+      for (int j = -k[1]; j <= k[1]; ++j)
+        for (int l = -k[2]; l <= k[2]; ++l) {
+          double q2 = (double)(i * i + j * j + l * l);
+          if (q2 != 0 && q2 < cut) nk += 1;
+        }
+    kx = k[0];
+    // k factors, structure factors, k indices, row tables, charged-atom scratch
+    bytes += nk * (8 + 16 + 12) + (k[0] + 1) * (2 * k[1] + 1) * 8.0 + n_q * 32.0;
+  }
+  // cell lists: atoms within (rc + half cell diagonal) of each cell
+  double rc = std::max(ff->cutoff_vdw, ff->cutoff_charge);
+  double ncell = 1;
+  for (int d = 0; d < 3; ++d) ncell *= std::max(1.0, std::min(64.0, std::floor(w[d] / cell_size)));
+  double reach = rc + 0.5 * std::sqrt(3.0) * cell_size;
+  double per_cell = std::min(n, n / V * 4.0 / 3.0 * M_PI * reach * reach * reach);
+  bytes += ncell * (per_cell * 4.0 + 8.0);
+  // Widom buffers per insertion: first-bead trials (5 x 8 B each), per
+  // insertion scalars (132 B), orientation trials (4 x 8 B each), positions of
+  // the selected sites (24 B each), placed-bead Coulomb (8 B each), Fourier
+  // results (16 B) and, for a charged probe with Ewald, the partial sums of
+  // each kx slice (16 B each)
+  const int n_place = std::max(n_sites - 1, 1);
+  const int k_or_eff = n_sites > 1 ? k_or : 1;
+  double per_ins = k_fb * 40.0 + 132.0 + (double)k_or_eff * n_place * 32.0 + n_sites * 24.0 +
+                   n_place * 8.0 + 16.0;
+  if (ewald && n_charged > 0) per_ins += (kx + 1) * 16.0;
+  // binning scratch: 8 B per binned site and 16 B per cell for each list
+  per_ins += k_fb * 8.0;
+  double lists = 1;
+  if (n_sites > 1) {
+    per_ins += (double)k_or * 8.0;
+    lists += 1;
+    if (ewald && n_charged > 0) {
+      per_ins += 8.0;
+      lists += 1;
+    }
+  }
+  bytes += per_ins * (double)chunk + lists * ncell * 16.0;
+  // reduction partials and small control buffers (at least 256 B each)
+  bytes += ((double)(chunk + 4095) / 4096.0 + 1.0) * 80.0 + 64 * 256.0;
+  return (size_t)bytes;
+}
+
+void tuned_options(rhg_server *s, FrameworkSpec &f) {
+  f.cell_size = 2.5;
+  f.chunk = s->slice;
+  for (const Tuning &t : s->tuning)
+    if ((int)f.x.size() <= t.max_atoms) {
+      f.cell_size = t.cell_size;
+      f.chunk = t.chunk;
+      break;
+    }
+}
+
+void ctx_options(rhg_server *s, FrameworkSpec &f) {
+  rhg_set_culling(f.ctx, 1, f.cell_size);
+  rhg_set_widom_options(f.ctx, std::min(f.chunk, s->slice), 1);
+}
+
+void evict(rhg_server *s, FrameworkSpec &f) {
+  rhg_destroy(f.ctx);
+  f.ctx = nullptr;
+  s->resident -= f.bytes;
+  f.bytes = 0;
+  f.types.clear();
+  s->n_evicted += 1;
+}
+
+// Makes the framework resident, evicting idle contexts (least recently used
+// first) until its estimated footprint fits the budget.
+// *wait is set when the request fits the budget but not beside the busy
+// contexts (try again when one finishes).
+int make_resident(rhg_server *s, FrameworkSpec &f, const rhg_probe &p, bool *wait) {
+  *wait = false;
+  f.last_used = ++s->tick;
+  if (f.ctx) return RHG_OK;
+  tuned_options(s, f);
+  rhg_framework fr{(int32_t)f.x.size(), f.x.data(), f.y.data(), f.z.data(), f.q.data(),
+                   f.eps.data(), f.sig.data(), f.mass.data()};
+  int n_charged = 0;
+  for (int j = 0; j < p.n_sites; ++j) n_charged += std::fabs(p.charge[j]) > 1e-10 ? 1 : 0;
+  size_t need = estimate_bytes(&fr, &f.cell, &f.ff, p.n_sites, std::min(s->slice, f.chunk), p.n_sites,
+                               n_charged, p.k_fb, p.k_or, f.cell_size);
+  if (need > s->budget) return sfail(s, RHG_ERR_NOT_ADMITTED, "framework larger than the device budget");
+  while (s->resident + need > s->budget) {
+    FrameworkSpec *lru = nullptr;
+    for (auto &kv : s->fw)
+      if (kv.second.ctx && !kv.second.busy && (!lru || kv.second.last_used < lru->last_used))
+        lru = &kv.second;
+    if (!lru) {
+      *wait = true; // everything resident is busy
+      return RHG_ERR_NOT_ADMITTED;
+    }
+    evict(s, *lru);
+  }
+  int rc = rhg_create(&fr, &f.cell, &f.ff, s->device, &f.ctx);
+  if (rc) return sfail(s, rc, "rhg_create failed");
+  rhg_set_precision(f.ctx, s->precision);
+  ctx_options(s, f);
+  f.bytes = need;
+  s->resident += need;
+  s->peak_resident = std::max(s->peak_resident, s->resident);
+  s->n_created += 1;
+  return RHG_OK;
+}
+
+int probe_molecule(FrameworkSpec &f, const rhg_probe &p, rhg_molecule &m) {
+  memset(&m, 0, sizeof(m));
+  m.n_sites = p.n_sites;
+  for (int j = 0; j < p.n_sites; ++j) {
+    auto key = std::make_tuple(p.eps_k[j], p.sigma[j], p.charge[j]);
+    auto it = f.types.find(key);
+    int32_t t;
+    if (it == f.types.end()) {
+      if (int rc = rhg_add_pseudo_atom(f.ctx, p.eps_k[j], p.sigma[j], p.charge[j], &t)) return rc;
+      f.types[key] = t;
+    } else {
+      t = it->second;
+    }
+    m.type[j] = t;
+    for (int d = 0; d < 3; ++d) m.body[j][d] = p.body[j][d];
+  }
+  m.start = p.start;
+  m.k_fb = p.k_fb;
+  m.k_or = p.k_or;
+  return RHG_OK;
+}
+
+// Enqueues the next slice of a job; on a failed submit the slice's partial
+// sums are discarded and the slice is tried again (up to three times).
+int next_slice(rhg_server *s, Job &j, FrameworkSpec &f) {
+  for (;;) {
+    j.slice_n = std::min(s->slice, j.req.n_insertions - j.done);
+    if (j.fault_after >= 0) {
+      rhg_inject_fault(f.ctx, j.fault_after);
+      j.fault_after = -1;
+    }
+    int rc = rhg_widom_submit(f.ctx, &j.mol, j.req.seed, j.req.n_insertions, j.done, j.slice_n,
+                              j.req.steps_per_cycle);
+    if (rc == RHG_OK) return RHG_OK;
+    rhg_inject_fault(f.ctx, -1);
+    if (rhg_widom_collect(f.ctx, nullptr) != RHG_OK || ++j.res.retries > 3) {
+      j.res.status = rc;
+      return rc;
+    }
+    s->n_retries += 1;
+  }
+}
+
+} // namespace
+
+extern "C" {
+
+int rhg_server_create(int device, size_t hbm_budget, int32_t slots, int32_t precision, int64_t slice,
+                      rhg_server **out) {
+  if (!out || slots < 1 || slots > 64 || slice < 1) return RHG_ERR_INVALID;
+  if (precision != RHG_PREC_FP64 && precision != RHG_PREC_MIXED) return RHG_ERR_INVALID;
+  *out = nullptr;
+  if (hipSetDevice(device) != hipSuccess) return RHG_ERR_HIP;
+  rhg_server *s = new rhg_server();
+  s->device = device;
+  s->slots = slots;
+  s->precision = precision;
+  s->slice = slice;
+  if (hbm_budget == 0) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+      delete s;
+      return RHG_ERR_HIP;
+    }
+    hbm_budget = (size_t)(0.8 * (double)fr);
+  }
+  s->budget = hbm_budget;
+  *out = s;
+  return RHG_OK;
+}
+
+void rhg_server_destroy(rhg_server *s) {
+  if (!s) return;
+  for (auto &kv : s->fw)
+    if (kv.second.ctx) rhg_destroy(kv.second.ctx);
+  delete s;
+}
+
+const char *rhg_server_last_error(const rhg_server *s) { return s ? s->err.c_str() : "null server"; }
+
+int rhg_server_add_framework(rhg_server *s, int64_t id, const rhg_framework *f, const rhg_cell *cell,
+                             const rhg_forcefield *ff) {
+  if (!s || !f || !cell || !ff || f->n_atoms < 0) return RHG_ERR_INVALID;
+  if (s->fw.count(id)) return sfail(s, RHG_ERR_INVALID, "framework id already registered");
+  FrameworkSpec &sp = s->fw[id];
+  const int n = f->n_atoms;
+  sp.x.assign(f->x, f->x + n);
+  sp.y.assign(f->y, f->y + n);
+  sp.z.assign(f->z, f->z + n);
+  sp.eps.assign(f->eps_k, f->eps_k + n);
+  sp.sig.assign(f->sigma, f->sigma + n);
+  sp.q.assign((size_t)n, 0.0);
+  if (f->charge) sp.q.assign(f->charge, f->charge + n);
+  sp.mass.assign((size_t)n, 12.0);
+  if (f->mass) sp.mass.assign(f->mass, f->mass + n);
+  sp.cell = *cell;
+  sp.ff = *ff;
+  return RHG_OK;
+}
+
+int rhg_server_set_tuning(rhg_server *s, int32_t n, const int32_t *max_atoms, const double *cell_size,
+                          const int64_t *chunk) {
+  if (!s || n < 0 || (n > 0 && (!max_atoms || !cell_size || !chunk))) return RHG_ERR_INVALID;
+  s->tuning.clear();
+  for (int i = 0; i < n; ++i) {
+    if (!(cell_size[i] > 0.0) || chunk[i] < 1) return sfail(s, RHG_ERR_INVALID, "bad tuning row");
+    s->tuning.push_back({max_atoms[i], cell_size[i], chunk[i]});
+  }
+  std::sort(s->tuning.begin(), s->tuning.end(),
+            [](const Tuning &a, const Tuning &b) { return a.max_atoms < b.max_atoms; });
+  return RHG_OK;
+}
+
+int rhg_server_submit(rhg_server *s, const rhg_request *r, int64_t *id) {
+  if (!s || !r || !id) return RHG_ERR_INVALID;
+  if (!s->fw.count(r->framework)) return sfail(s, RHG_ERR_INVALID, "unknown framework id");
+  const rhg_probe &p = r->probe;
+  if (p.n_sites < 1 || p.n_sites > RHG_MAX_SITES || p.start < 0 || p.start >= p.n_sites)
+    return sfail(s, RHG_ERR_INVALID, "bad probe");
+  if (r->n_insertions < 1 || r->steps_per_cycle < 1 || r->arrival_s < 0.0)
+    return sfail(s, RHG_ERR_INVALID, "bad request");
+  Job j;
+  memset(&j.res, 0, sizeof(j.res));
+  j.req = *r;
+  memset(&j.mol, 0, sizeof(j.mol));
+  *id = (int64_t)s->jobs.size();
+  s->jobs.push_back(j);
+  return RHG_OK;
+}
+
+int rhg_server_inject_fault(rhg_server *s, int64_t id, int64_t after_chunks) {
+  if (!s || id < 0 || id >= (int64_t)s->jobs.size()) return RHG_ERR_INVALID;
+  s->jobs[id].fault_after = after_chunks;
+  return RHG_OK;
+}
+
+int rhg_server_run(rhg_server *s) {
+  if (!s) return RHG_ERR_INVALID;
+  hipSetDevice(s->device);
+  const Clock::time_point t0 = Clock::now();
+  // waiting requests in arrival order (stable for equal arrivals)
+  std::vector<int64_t> order;
+  for (int64_t i = 0; i < (int64_t)s->jobs.size(); ++i)
+    if (!s->jobs[i].finished) order.push_back(i);
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+    return s->jobs[a].req.arrival_s < s->jobs[b].req.arrival_s;
+  });
+  std::deque<int64_t> waiting(order.begin(), order.end());
+  std::vector<int64_t> active;
+  size_t remaining = waiting.size();
+  while (remaining > 0) {
+    bool progress = false;
+    const double now = secs(t0, Clock::now());
+    // harvest finished slices
+    for (size_t k = 0; k < active.size();) {
+      Job &j = s->jobs[active[k]];
+      FrameworkSpec &f = s->fw[j.req.framework];
+      int q = rhg_widom_query(f.ctx);
+      if (q == 0) {
+        ++k;
+        continue;
+      }
+      progress = true;
+      int rc = q < 0 ? q : rhg_widom_collect(f.ctx, &j.res.blocks);
+      if (rc == RHG_OK) {
+        j.done += j.slice_n;
+        if (j.done < j.req.n_insertions) rc = next_slice(s, j, f);
+        if (rc == RHG_OK && j.done < j.req.n_insertions) {
+          ++k;
+          continue;
+        }
+      }
+      if (rc != RHG_OK) j.res.status = rc;
+      j.finished = true;
+      j.res.finished_s = secs(t0, Clock::now());
+      j.res.latency_s = j.res.finished_s - j.req.arrival_s;
+      f.busy = false;
+      f.last_used = ++s->tick;
+      size_t b = 0;
+      if (f.ctx && rhg_footprint(f.ctx, &b) == RHG_OK) {
+        // from now on account for what the context really holds
+        s->resident = s->resident - f.bytes + b;
+        f.bytes = b;
+        s->peak_resident = std::max(s->peak_resident, s->resident);
+      }
+      active[k] = active.back();
+      active.pop_back();
+      --remaining;
+    }
+    // admit arrived requests into free slots (skipping ones whose framework
+    // is busy with another request)
+    for (auto it = waiting.begin(); it != waiting.end() && (int)active.size() < s->slots;) {
+      Job &j = s->jobs[*it];
+      if (j.req.arrival_s > now) break;
+      FrameworkSpec &f = s->fw[j.req.framework];
+      if (f.busy) {
+        ++it;
+        continue;
+      }
+      bool wait = false;
+      int rc = make_resident(s, f, j.req.probe, &wait);
+      if (wait && !active.empty()) break; // wait for memory
+      if (rc == RHG_OK) rc = probe_molecule(f, j.req.probe, j.mol);
+      j.started = true;
+      j.res.queued_s = j.req.arrival_s;
+      j.res.started_s = secs(t0, Clock::now());
+      if (rc == RHG_OK) rc = next_slice(s, j, f);
+      progress = true;
+      it = waiting.erase(it);
+      if (rc != RHG_OK) {
+        j.res.status = rc;
+        j.finished = true;
+        j.res.finished_s = secs(t0, Clock::now());
+        j.res.latency_s = j.res.finished_s - j.req.arrival_s;
+        --remaining;
+        continue;
+      }
+      f.busy = true;
+      active.push_back(&j - s->jobs.data());
+    }
+    if (!progress) {
+      const Clock::time_point w = Clock::now();
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+      s->busy_wait_s += secs(w, Clock::now());
+    }
+  }
+  return RHG_OK;
+}
+
+int rhg_server_result(const rhg_server *s, int64_t id, rhg_result *out) {
+  if (!s || !out || id < 0 || id >= (int64_t)s->jobs.size()) return RHG_ERR_INVALID;
+  *out = s->jobs[id].res;
+  return s->jobs[id].finished ? RHG_OK : RHG_ERR_INVALID;
+}
+
+int rhg_server_stats(const rhg_server *s, double out[8]) {
+  if (!s || !out) return RHG_ERR_INVALID;
+  out[0] = s->n_created;
+  out[1] = s->n_evicted;
+  out[2] = (double)s->peak_resident;
+  out[3] = (double)s->resident;
+  out[4] = s->n_retries;
+  out[5] = (double)s->budget;
+  out[6] = s->busy_wait_s;
+  out[7] = (double)s->jobs.size();
+  return RHG_OK;
+}
+
+size_t rhg_estimate_footprint(const rhg_framework *f, const rhg_cell *cell, const rhg_forcefield *ff,
+                              int32_t n_types, int64_t chunk, int32_t n_sites, int32_t n_charged,
+                              int32_t k_fb, int32_t k_or, double cell_size) {
+  if (!f || !cell || !ff || chunk < 1 || cell_size <= 0.0 || n_sites < 1) return 0;
+  return estimate_bytes(f, cell, ff, n_types, chunk, n_sites, n_charged, k_fb, k_or, cell_size);
+}
+
+} // extern "C"
