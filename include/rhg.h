@@ -49,6 +49,7 @@ extern "C" {
 #define RHG_NR_BLOCKS 5 /* src/statistics.h:40 */
 
 typedef struct rhg_ctx rhg_ctx;
+typedef struct rhg_workspace rhg_workspace;
 typedef struct rhg_grid rhg_grid;
 
 /* Framework atoms, one pseudo-atom per atom (Lorentz-Berthelot mixing on
@@ -252,6 +253,16 @@ int rhg_widom_stats(rhg_ctx *ctx, double out[12], int32_t reset);
  * wave walks the ascending list of atoms within reach of its cell.  The FP64
  * result is bit-identical to the all-pairs loop.  Default: on, 2.5 A. */
 int rhg_set_culling(rhg_ctx *ctx, int32_t on, double cell_size);
+/* Scratch buffers that contexts can borrow (no counterpart in the
+ * reference): while a workspace is attached, the context's per-call device
+ * buffers (Widom chunk buffers, binning scratch, Fourier partials) live in
+ * it, so that many resident contexts served one at a time per stream share
+ * one set of buffers.  Attaching a workspace owned by another context waits
+ * for that context's stream and detaches it there; ws = NULL detaches. */
+int rhg_workspace_create(int device, rhg_workspace **out);
+void rhg_workspace_destroy(rhg_workspace *ws);
+int rhg_workspace_footprint(const rhg_workspace *ws, size_t *bytes);
+int rhg_attach_workspace(rhg_ctx *ctx, rhg_workspace *ws);
 
 /* ---------------------------------------------------- screening service */
 /* No counterpart in the reference, which answers one request per process
@@ -310,8 +321,9 @@ int rhg_server_run(rhg_server *s);
 int rhg_server_result(const rhg_server *s, int64_t id, rhg_result *out);
 /* out[0] contexts created, [1] evictions, [2] peak resident bytes, [3]
  * resident bytes, [4] slices retried, [5] budget, [6] idle wait [s],
- * [7] requests */
-int rhg_server_stats(const rhg_server *s, double out[8]);
+ * [7] requests, host seconds in [8] context set-up, [9] submits, [10]
+ * collects, [11] reserved */
+int rhg_server_stats(const rhg_server *s, double out[12]);
 /* Device bytes a context of this framework holds after Widom insertions of
  * a probe of n_sites sites (n_charged charged, n_types distinct
  * pseudo-atoms) in chunks of `chunk` (host-only arithmetic; the admission

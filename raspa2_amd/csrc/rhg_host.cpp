@@ -110,8 +110,16 @@ struct Pending {
 
 } // namespace
 
+struct rhg_workspace {
+  int device = 0;
+  std::map<std::string, DevBuf> bufs;
+  uint64_t gen = 0;
+  rhg_ctx *owner = nullptr;
+};
+
 struct rhg_ctx {
   int device = 0;
+  rhg_workspace *ws = nullptr; // borrowed Widom/scratch buffers (nullable)
   hipStream_t stream = nullptr;
   std::string err;
   Cell cell;
@@ -202,15 +210,24 @@ int dalloc(rhg_ctx *c, void **p, size_t bytes) {
   return RHG_OK;
 }
 
-// grow-only named work buffer
+// grow-only named work buffer, in the attached workspace if there is one
+// (the context's own counters always stay with the context)
 void *work(rhg_ctx *c, const char *name, size_t bytes) {
-  DevBuf &b = c->bufs[name];
+  const bool shared = c->ws && strcmp(name, "pair_counter") != 0;
+  DevBuf &b = shared ? c->ws->bufs[name] : c->bufs[name];
   if (b.bytes >= bytes && b.p) return b.p;
-  if (b.p) hipFree(b.p);
+  size_t want = std::max<size_t>(bytes, 256);
+  if (b.p) {
+    // regrowth: leave headroom so that a slowly growing size settles
+    want = std::max(want, b.bytes + b.bytes / 2);
+    hipFree(b.p);
+  }
   b.p = nullptr;
   b.bytes = 0;
-  c->gen++;
-  size_t want = std::max<size_t>(bytes, 256);
+  if (shared)
+    c->ws->gen++;
+  else
+    c->gen++;
   if (hipMalloc(&b.p, want) != hipSuccess) {
     b.p = nullptr;
     return nullptr;
@@ -641,6 +658,7 @@ void rhg_destroy(rhg_ctx *c) {
   if (!c) return;
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
+  if (c->ws && c->ws->owner == c) c->ws->owner = nullptr;
   for (auto &t : c->types) {
     hipFree(t.d64);
     hipFree(t.dmx);
@@ -1097,11 +1115,14 @@ int widom_enqueue(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, uint64_t
       key.append((const char *)&seed, sizeof(seed));
       key.append((const char *)&cap, sizeof(cap));
       key.append((const char *)&c->gen, sizeof(c->gen));
+      const uint64_t wgen = c->ws ? c->ws->gen : 0;
+      key.append((const char *)&c->ws, sizeof(c->ws));
+      key.append((const char *)&wgen, sizeof(wgen));
       key.append((const char *)&c->precision, sizeof(c->precision));
       key.append((const char *)&c->cull, sizeof(c->cull));
       auto it = c->graph_cache.find(key);
       if (it == c->graph_cache.end()) {
-        const uint64_t gen0 = c->gen;
+        const uint64_t gen0 = c->gen, wgen0 = c->ws ? c->ws->gen : 0;
         WidomArgs a = widom_args(c, m, seed, cap);
         HIPC(c, hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
         int rc = widom_chunk(c, mol, m, B, a, true);
@@ -1116,7 +1137,7 @@ int widom_enqueue(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, uint64_t
         e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
         if (e != hipSuccess) return fail(c, RHG_ERR_HIP, std::string("instantiate: ") + hipGetErrorString(e));
-        if (c->gen != gen0) {
+        if (c->gen != gen0 || (c->ws ? c->ws->gen : 0) != wgen0) {
           // a buffer moved while capturing: this graph is stale
           hipGraphExecDestroy(ge);
           return fail(c, RHG_ERR_HIP, "buffers changed during capture");
@@ -1576,6 +1597,57 @@ int rhg_set_culling(rhg_ctx *c, int32_t on, double cell_size) {
   if (on && cell_size != c->cell_size) {
     c->cell_size = cell_size;
     c->grid_ready = false;
+  }
+  return RHG_OK;
+}
+
+int rhg_workspace_create(int device, rhg_workspace **out) {
+  if (!out) return RHG_ERR_INVALID;
+  *out = new rhg_workspace();
+  (*out)->device = device;
+  return RHG_OK;
+}
+
+void rhg_workspace_destroy(rhg_workspace *ws) {
+  if (!ws) return;
+  hipSetDevice(ws->device);
+  if (ws->owner) {
+    hipStreamSynchronize(ws->owner->stream);
+    ws->owner->ws = nullptr;
+  }
+  for (auto &b : ws->bufs) hipFree(b.second.p);
+  delete ws;
+}
+
+int rhg_workspace_footprint(const rhg_workspace *ws, size_t *bytes) {
+  if (!ws || !bytes) return RHG_ERR_INVALID;
+  size_t b = 0;
+  for (auto &kv : ws->bufs) b += kv.second.bytes;
+  *bytes = b;
+  return RHG_OK;
+}
+
+int rhg_attach_workspace(rhg_ctx *c, rhg_workspace *ws) {
+  if (!c) return RHG_ERR_INVALID;
+  if (ws && ws->device != c->device) return fail(c, RHG_ERR_INVALID, "workspace on another device");
+  hipSetDevice(c->device);
+  if (c->ws == ws) {
+    if (ws) ws->owner = c;
+    return RHG_OK;
+  }
+  if (c->ws) {
+    // the old workspace may still be in use by this context's stream
+    HIPC(c, hipStreamSynchronize(c->stream));
+    if (c->ws->owner == c) c->ws->owner = nullptr;
+    c->ws = nullptr;
+  }
+  if (ws) {
+    if (ws->owner && ws->owner != c) {
+      HIPC(c, hipStreamSynchronize(ws->owner->stream));
+      ws->owner->ws = nullptr;
+    }
+    ws->owner = c;
+    c->ws = ws;
   }
   return RHG_OK;
 }

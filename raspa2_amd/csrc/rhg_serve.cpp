@@ -15,8 +15,10 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <condition_variable>
 #include <deque>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <tuple>
@@ -43,6 +45,7 @@ struct FrameworkSpec {
 };
 
 struct Job {
+  int slot = -1;             // workspace of the running request
   rhg_request req;
   rhg_result res;
   int64_t done = 0;          // insertions whose sums are in res.blocks
@@ -73,6 +76,11 @@ struct rhg_server {
   uint64_t tick = 0;
   size_t resident = 0, peak_resident = 0;
   double n_created = 0, n_evicted = 0, n_retries = 0, busy_wait_s = 0;
+  double setup_s = 0, submit_s = 0, collect_s = 0; // host time in context set-up, submits, collects
+  // one workspace per slot: the scratch buffers of the request running there
+  std::vector<rhg_workspace *> ws;
+  std::vector<size_t> ws_bytes;
+  std::vector<char> ws_busy;
 };
 
 namespace {
@@ -90,9 +98,10 @@ double secs(Clock::time_point a, Clock::time_point b) {
 // probe with n_sites sites (n_charged of them charged, n_types distinct
 // pseudo-atoms) in chunks of `chunk` insertions: the buffers the context
 // allocates, counted as it allocates them.
+// *scratch receives the part that lives in a workspace when one is attached.
 size_t estimate_bytes(const rhg_framework *f, const rhg_cell *cell, const rhg_forcefield *ff,
                       int32_t n_types, int64_t chunk, int32_t n_sites, int32_t n_charged, int32_t k_fb,
-                      int32_t k_or, double cell_size) {
+                      int32_t k_or, double cell_size, size_t *scratch = nullptr) {
   const double *B = cell->box;
   // volume and perpendicular widths of the cell
   double a[3] = {B[0], B[1], B[2]}, b[3] = {B[3], B[4], B[5]}, c[3] = {B[6], B[7], B[8]};
@@ -170,10 +179,11 @@ size_t estimate_bytes(const rhg_framework *f, const rhg_cell *cell, const rhg_fo
       lists += 1;
     }
   }
-  bytes += per_ins * (double)chunk + lists * ncell * 16.0;
+  double ws = per_ins * (double)chunk + lists * ncell * 16.0;
   // reduction partials and small control buffers (at least 256 B each)
-  bytes += ((double)(chunk + 4095) / 4096.0 + 1.0) * 80.0 + 64 * 256.0;
-  return (size_t)bytes;
+  ws += ((double)(chunk + 4095) / 4096.0 + 1.0) * 80.0 + 64 * 256.0;
+  if (scratch) *scratch = (size_t)ws;
+  return (size_t)(bytes + ws);
 }
 
 void tuned_options(rhg_server *s, FrameworkSpec &f) {
@@ -201,41 +211,79 @@ void evict(rhg_server *s, FrameworkSpec &f) {
   s->n_evicted += 1;
 }
 
-// Makes the framework resident, evicting idle contexts (least recently used
-// first) until its estimated footprint fits the budget.
-// *wait is set when the request fits the budget but not beside the busy
-// contexts (try again when one finishes).
-int make_resident(rhg_server *s, FrameworkSpec &f, const rhg_probe &p, bool *wait) {
+size_t ws_total(const rhg_server *s) {
+  size_t t = 0;
+  for (size_t b : s->ws_bytes) t += b;
+  return t;
+}
+
+void note_peak(rhg_server *s) { s->peak_resident = std::max(s->peak_resident, s->resident + ws_total(s)); }
+
+// Reserves device memory for a request on workspace `slot` (caller holds
+// the server lock): the framework context's own bytes (if it is not
+// resident yet) plus the slot's scratch must fit the budget beside the other
+// contexts and workspaces; idle contexts are evicted least recently used
+// first, then idle workspaces are released.  *wait is set when the request
+// would fit once running requests finish.
+int reserve(rhg_server *s, FrameworkSpec &f, const rhg_probe &p, int slot, bool *wait) {
   *wait = false;
   f.last_used = ++s->tick;
-  if (f.ctx) return RHG_OK;
-  tuned_options(s, f);
+  if (!f.ctx) tuned_options(s, f);
   rhg_framework fr{(int32_t)f.x.size(), f.x.data(), f.y.data(), f.z.data(), f.q.data(),
                    f.eps.data(), f.sig.data(), f.mass.data()};
   int n_charged = 0;
   for (int j = 0; j < p.n_sites; ++j) n_charged += std::fabs(p.charge[j]) > 1e-10 ? 1 : 0;
-  size_t need = estimate_bytes(&fr, &f.cell, &f.ff, p.n_sites, std::min(s->slice, f.chunk), p.n_sites,
-                               n_charged, p.k_fb, p.k_or, f.cell_size);
-  if (need > s->budget) return sfail(s, RHG_ERR_NOT_ADMITTED, "framework larger than the device budget");
-  while (s->resident + need > s->budget) {
+  size_t scratch = 0;
+  const size_t total = estimate_bytes(&fr, &f.cell, &f.ff, p.n_sites, std::min(s->slice, f.chunk),
+                                      p.n_sites, n_charged, p.k_fb, p.k_or, f.cell_size, &scratch);
+  const size_t own = total - scratch;
+  if (total > s->budget) return sfail(s, RHG_ERR_NOT_ADMITTED, "request larger than the device budget");
+  auto need = [&]() {
+    size_t w = ws_total(s) - s->ws_bytes[slot] + std::max(s->ws_bytes[slot], scratch);
+    return s->resident + (f.ctx ? 0 : own) + w;
+  };
+  while (need() > s->budget) {
     FrameworkSpec *lru = nullptr;
     for (auto &kv : s->fw)
-      if (kv.second.ctx && !kv.second.busy && (!lru || kv.second.last_used < lru->last_used))
+      if (kv.second.ctx && !kv.second.busy && &kv.second != &f &&
+          (!lru || kv.second.last_used < lru->last_used))
         lru = &kv.second;
-    if (!lru) {
-      *wait = true; // everything resident is busy
-      return RHG_ERR_NOT_ADMITTED;
+    if (lru) {
+      evict(s, *lru);
+      continue;
     }
-    evict(s, *lru);
+    // release the buffers of idle workspaces
+    bool freed = false;
+    for (size_t w = 0; w < s->ws.size(); ++w)
+      if (!s->ws_busy[w] && s->ws_bytes[w] > 0) {
+        rhg_workspace_destroy(s->ws[w]);
+        rhg_workspace_create(s->device, &s->ws[w]);
+        s->ws_bytes[w] = 0;
+        freed = true;
+      }
+    if (freed) continue;
+    *wait = true;
+    return RHG_ERR_NOT_ADMITTED;
   }
+  if (!f.ctx) {
+    f.bytes = own; // the context is created by the worker, outside the lock
+    s->resident += own;
+  }
+  s->ws_bytes[slot] = std::max(s->ws_bytes[slot], scratch);
+  note_peak(s);
+  return RHG_OK;
+}
+
+int create_context(rhg_server *s, FrameworkSpec &f) {
+  rhg_framework fr{(int32_t)f.x.size(), f.x.data(), f.y.data(), f.z.data(), f.q.data(),
+                   f.eps.data(), f.sig.data(), f.mass.data()};
   int rc = rhg_create(&fr, &f.cell, &f.ff, s->device, &f.ctx);
-  if (rc) return sfail(s, rc, "rhg_create failed");
+  if (rc) {
+    f.ctx = nullptr;
+    return rc;
+  }
   rhg_set_precision(f.ctx, s->precision);
   ctx_options(s, f);
-  f.bytes = need;
-  s->resident += need;
-  s->peak_resident = std::max(s->peak_resident, s->resident);
-  s->n_created += 1;
   return RHG_OK;
 }
 
@@ -278,7 +326,6 @@ int next_slice(rhg_server *s, Job &j, FrameworkSpec &f) {
       j.res.status = rc;
       return rc;
     }
-    s->n_retries += 1;
   }
 }
 
@@ -306,6 +353,10 @@ int rhg_server_create(int device, size_t hbm_budget, int32_t slots, int32_t prec
     hbm_budget = (size_t)(0.8 * (double)fr);
   }
   s->budget = hbm_budget;
+  s->ws.assign((size_t)slots, nullptr);
+  s->ws_bytes.assign((size_t)slots, 0);
+  s->ws_busy.assign((size_t)slots, 0);
+  for (int w = 0; w < slots; ++w) rhg_workspace_create(device, &s->ws[w]);
   *out = s;
   return RHG_OK;
 }
@@ -314,6 +365,7 @@ void rhg_server_destroy(rhg_server *s) {
   if (!s) return;
   for (auto &kv : s->fw)
     if (kv.second.ctx) rhg_destroy(kv.second.ctx);
+  for (rhg_workspace *w : s->ws) rhg_workspace_destroy(w);
   delete s;
 }
 
@@ -375,11 +427,15 @@ int rhg_server_inject_fault(rhg_server *s, int64_t id, int64_t after_chunks) {
   return RHG_OK;
 }
 
+// One worker thread per slot: it takes the earliest arrived request whose
+// framework is idle, makes the framework resident, and runs the request's
+// slices on the framework's stream with its own workspace, waiting on its
+// own stream only.  Admission decisions and the footprint accounting are
+// made under one lock.
 int rhg_server_run(rhg_server *s) {
   if (!s) return RHG_ERR_INVALID;
   hipSetDevice(s->device);
   const Clock::time_point t0 = Clock::now();
-  // waiting requests in arrival order (stable for equal arrivals)
   std::vector<int64_t> order;
   for (int64_t i = 0; i < (int64_t)s->jobs.size(); ++i)
     if (!s->jobs[i].finished) order.push_back(i);
@@ -387,84 +443,129 @@ int rhg_server_run(rhg_server *s) {
     return s->jobs[a].req.arrival_s < s->jobs[b].req.arrival_s;
   });
   std::deque<int64_t> waiting(order.begin(), order.end());
-  std::vector<int64_t> active;
   size_t remaining = waiting.size();
-  while (remaining > 0) {
-    bool progress = false;
-    const double now = secs(t0, Clock::now());
-    // harvest finished slices
-    for (size_t k = 0; k < active.size();) {
-      Job &j = s->jobs[active[k]];
-      FrameworkSpec &f = s->fw[j.req.framework];
-      int q = rhg_widom_query(f.ctx);
-      if (q == 0) {
-        ++k;
-        continue;
-      }
-      progress = true;
-      int rc = q < 0 ? q : rhg_widom_collect(f.ctx, &j.res.blocks);
-      if (rc == RHG_OK) {
-        j.done += j.slice_n;
-        if (j.done < j.req.n_insertions) rc = next_slice(s, j, f);
-        if (rc == RHG_OK && j.done < j.req.n_insertions) {
-          ++k;
-          continue;
-        }
-      }
-      if (rc != RHG_OK) j.res.status = rc;
-      j.finished = true;
-      j.res.finished_s = secs(t0, Clock::now());
-      j.res.latency_s = j.res.finished_s - j.req.arrival_s;
-      f.busy = false;
-      f.last_used = ++s->tick;
+  int n_active = 0;
+  std::mutex mu;
+  std::condition_variable cv;
+
+  // caller holds the lock
+  auto finish = [&](Job &j, FrameworkSpec &f, int rc) {
+    if (rc != RHG_OK) j.res.status = rc;
+    j.finished = true;
+    j.res.finished_s = secs(t0, Clock::now());
+    j.res.latency_s = j.res.finished_s - j.req.arrival_s;
+    f.busy = false;
+    f.last_used = ++s->tick;
+    if (j.slot >= 0) {
       size_t b = 0;
-      if (f.ctx && rhg_footprint(f.ctx, &b) == RHG_OK) {
-        // from now on account for what the context really holds
+      if (rhg_workspace_footprint(s->ws[j.slot], &b) == RHG_OK) s->ws_bytes[j.slot] = b;
+      s->ws_busy[j.slot] = 0;
+      j.slot = -1;
+    }
+    if (f.ctx && rc == RHG_ERR_HIP) {
+      evict(s, f); // a device error may have left the context unusable
+    } else if (f.ctx) {
+      // from now on account for what the context really holds
+      size_t b = 0;
+      if (rhg_footprint(f.ctx, &b) == RHG_OK) {
         s->resident = s->resident - f.bytes + b;
         f.bytes = b;
-        s->peak_resident = std::max(s->peak_resident, s->resident);
       }
-      active[k] = active.back();
-      active.pop_back();
-      --remaining;
+    } else {
+      s->resident -= f.bytes; // reserved for a context that was never created
+      f.bytes = 0;
     }
-    // admit arrived requests into free slots (skipping ones whose framework
-    // is busy with another request)
-    for (auto it = waiting.begin(); it != waiting.end() && (int)active.size() < s->slots;) {
-      Job &j = s->jobs[*it];
-      if (j.req.arrival_s > now) break;
-      FrameworkSpec &f = s->fw[j.req.framework];
-      if (f.busy) {
-        ++it;
-        continue;
+    note_peak(s);
+    --remaining;
+    cv.notify_all();
+  };
+
+  auto worker = [&](int slot) {
+    hipSetDevice(s->device);
+    for (;;) {
+      Job *jp = nullptr;
+      FrameworkSpec *fp = nullptr;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+          if (remaining == 0 || waiting.empty()) return;
+          const double now = secs(t0, Clock::now());
+          double next_arrival = -1.0;
+          bool blocked = false;
+          for (auto it = waiting.begin(); it != waiting.end(); ++it) {
+            Job &j = s->jobs[*it];
+            if (j.req.arrival_s > now) {
+              next_arrival = j.req.arrival_s;
+              break;
+            }
+            FrameworkSpec &f = s->fw[j.req.framework];
+            if (f.busy) continue;
+            bool wait = false;
+            const Clock::time_point ts = Clock::now();
+            int rc = reserve(s, f, j.req.probe, slot, &wait);
+            s->setup_s += secs(ts, Clock::now());
+            if (wait && n_active > 0) {
+              blocked = true; // memory frees when a running request ends
+              break;
+            }
+            j.started = true;
+            j.res.queued_s = j.req.arrival_s;
+            j.res.started_s = now;
+            waiting.erase(it);
+            f.busy = true;
+            if (rc != RHG_OK) {
+              finish(j, f, rc);
+              break;
+            }
+            s->ws_busy[slot] = 1;
+            j.slot = slot;
+            ++n_active;
+            jp = &j;
+            fp = &f;
+            break;
+          }
+          if (jp) break;
+          if (remaining == 0) return;
+          if (!blocked && next_arrival > 0.0)
+            cv.wait_for(lk, std::chrono::duration<double>(std::max(next_arrival - now, 0.0) + 1e-5));
+          else
+            cv.wait_for(lk, std::chrono::milliseconds(1));
+        }
       }
-      bool wait = false;
-      int rc = make_resident(s, f, j.req.probe, &wait);
-      if (wait && !active.empty()) break; // wait for memory
+      Job &j = *jp;
+      FrameworkSpec &f = *fp;
+      int rc = RHG_OK;
+      const Clock::time_point ts = Clock::now();
+      const bool created = !f.ctx;
+      if (created) rc = create_context(s, f);
+      if (rc == RHG_OK) rc = rhg_attach_workspace(f.ctx, s->ws[slot]);
       if (rc == RHG_OK) rc = probe_molecule(f, j.req.probe, j.mol);
-      j.started = true;
-      j.res.queued_s = j.req.arrival_s;
-      j.res.started_s = secs(t0, Clock::now());
-      if (rc == RHG_OK) rc = next_slice(s, j, f);
-      progress = true;
-      it = waiting.erase(it);
-      if (rc != RHG_OK) {
-        j.res.status = rc;
-        j.finished = true;
-        j.res.finished_s = secs(t0, Clock::now());
-        j.res.latency_s = j.res.finished_s - j.req.arrival_s;
-        --remaining;
-        continue;
+      const double setup = secs(ts, Clock::now());
+      double t_sub = 0, t_col = 0;
+      while (rc == RHG_OK && j.done < j.req.n_insertions) {
+        const Clock::time_point tq = Clock::now();
+        rc = next_slice(s, j, f);
+        const Clock::time_point tc = Clock::now();
+        if (rc == RHG_OK) rc = rhg_widom_collect(f.ctx, &j.res.blocks);
+        t_sub += secs(tq, tc);
+        t_col += secs(tc, Clock::now());
+        if (rc == RHG_OK) j.done += j.slice_n;
       }
-      f.busy = true;
-      active.push_back(&j - s->jobs.data());
+      if (f.ctx) rhg_attach_workspace(f.ctx, nullptr);
+      std::lock_guard<std::mutex> lk(mu);
+      s->setup_s += setup;
+      s->submit_s += t_sub;
+      s->collect_s += t_col;
+      if (created && f.ctx) s->n_created += 1;
+      s->n_retries += j.res.retries;
+      --n_active;
+      finish(j, f, rc);
     }
-    if (!progress) {
-      const Clock::time_point w = Clock::now();
-      std::this_thread::sleep_for(std::chrono::microseconds(20));
-      s->busy_wait_s += secs(w, Clock::now());
-    }
-  }
+  };
+
+  std::vector<std::thread> th;
+  for (int w = 0; w < s->slots; ++w) th.emplace_back(worker, w);
+  for (auto &t : th) t.join();
   return RHG_OK;
 }
 
@@ -474,8 +575,12 @@ int rhg_server_result(const rhg_server *s, int64_t id, rhg_result *out) {
   return s->jobs[id].finished ? RHG_OK : RHG_ERR_INVALID;
 }
 
-int rhg_server_stats(const rhg_server *s, double out[8]) {
+int rhg_server_stats(const rhg_server *s, double out[12]) {
   if (!s || !out) return RHG_ERR_INVALID;
+  out[8] = s->setup_s;
+  out[9] = s->submit_s;
+  out[10] = s->collect_s;
+  out[11] = 0.0;
   out[0] = s->n_created;
   out[1] = s->n_evicted;
   out[2] = (double)s->peak_resident;

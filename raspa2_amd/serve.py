@@ -157,8 +157,8 @@ class Server:
                     started_s=r.started_s, finished_s=r.finished_s, latency_s=r.latency_s)
 
     def stats(self):
-        out = np.zeros(8)
+        out = np.zeros(12)
         self._check(R.lib.rhg_server_stats(self.h, out.ctypes.data_as(_P)), "stats")
         keys = ("created", "evicted", "peak_resident", "resident", "retries", "budget",
-                "idle_wait_s", "requests")
+                "idle_wait_s", "requests", "setup_s", "submit_s", "collect_s")
         return dict(zip(keys, out))
