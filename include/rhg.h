@@ -251,7 +251,10 @@ int rhg_widom_stats(rhg_ctx *ctx, double out[12], int32_t reset);
  * UseCellLists, src/framework_energy.c:78-126, which the reference leaves
  * disabled): sites are binned into cells of about cell_size A and each
  * wave walks the ascending list of atoms within reach of its cell.  The FP64
- * result is bit-identical to the all-pairs loop.  Default: on, 2.5 A. */
+ * result is bit-identical to the all-pairs loop.  on = 0: all pairs; 1: cell
+ * lists; 2 (default): cell lists, and in mixed precision per-cell lists of
+ * atom images in fixed point, so that no pair needs a minimum-image step
+ * (used when every perpendicular width is >= 2 cut-offs).  Default cell 2.5 A. */
 int rhg_set_culling(rhg_ctx *ctx, int32_t on, double cell_size);
 /* Scratch buffers that contexts can borrow (no counterpart in the
  * reference): while a workspace is attached, the context's per-call device
@@ -326,11 +329,11 @@ int rhg_server_result(const rhg_server *s, int64_t id, rhg_result *out);
 int rhg_server_stats(const rhg_server *s, double out[12]);
 /* Device bytes a context of this framework holds after Widom insertions of
  * a probe of n_sites sites (n_charged charged, n_types distinct
- * pseudo-atoms) in chunks of `chunk` (host-only arithmetic; the admission
- * estimate of the server). */
+ * pseudo-atoms) in chunks of `chunk` at the given precision (host-only
+ * arithmetic; the admission estimate of the server). */
 size_t rhg_estimate_footprint(const rhg_framework *fw, const rhg_cell *cell, const rhg_forcefield *ff,
                               int32_t n_types, int64_t chunk, int32_t n_sites, int32_t n_charged,
-                              int32_t k_fb, int32_t k_or, double cell_size);
+                              int32_t k_fb, int32_t k_or, double cell_size, int32_t precision);
 
 /* ------------------------------------------------------ workload utility */
 /* No counterpart in the reference (its frameworks come from CIF files):

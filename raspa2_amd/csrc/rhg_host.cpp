@@ -101,6 +101,7 @@ struct PseudoAtom {
   AtomRec64 *d64 = nullptr;
   AtomRecMx *dmx = nullptr;
   double *c24 = nullptr, *c96 = nullptr, *c384 = nullptr;
+  float4 *par = nullptr; // {4 eps, sigma^2 * kFix^2, shift, q} for the image-list kernels
 };
 
 struct Pending {
@@ -151,6 +152,11 @@ struct rhg_ctx {
   bool grid_ready = false;
   CellGrid cg;
   int32_t *d_nb_off = nullptr, *d_nb_idx = nullptr;
+  int32_t *d_img_off = nullptr;
+  int4 *d_img = nullptr;
+  bool img_ready = false;
+  int use_images = 1; // mixed precision walks the image lists when they exist
+  double cell_reach = 0.0, cell_halfdiag = 0.0;
   double *d_ax = nullptr, *d_ay = nullptr, *d_az = nullptr;
   std::map<std::string, DevBuf> bufs;
   bool prof = false;
@@ -362,6 +368,7 @@ int setup_pseudo_atom(rhg_ctx *c, PseudoAtom &t) {
   std::vector<AtomRec64> h64((size_t)std::max(c->n, 1));
   std::vector<AtomRecMx> hmx((size_t)std::max(c->n, 1));
   std::vector<double> h24((size_t)std::max(c->n, 1)), h96(h24.size()), h384(h24.size());
+  std::vector<float4> hpar(h24.size());
   double ei = t.eps_k * c->u.K2E; // src/potentials.c:268-274
   for (int i = 0; i < c->n; i++) {
     double ej = c->eps[i] * c->u.K2E;
@@ -388,6 +395,7 @@ int setup_pseudo_atom(rhg_ctx *c, PseudoAtom &t) {
     h24[i] = 24.0 * e;
     h96[i] = 96.0 * e;
     h384[i] = 384.0 * e;
+    hpar[i] = make_float4((float)r.c4, (float)(r.s2 * kFix * kFix), (float)r.shift, (float)r.q);
   }
   size_t n = h64.size();
   int rc;
@@ -396,6 +404,8 @@ int setup_pseudo_atom(rhg_ctx *c, PseudoAtom &t) {
   if ((rc = dalloc(c, (void **)&t.c24, n * sizeof(double)))) return rc;
   if ((rc = dalloc(c, (void **)&t.c96, n * sizeof(double)))) return rc;
   if ((rc = dalloc(c, (void **)&t.c384, n * sizeof(double)))) return rc;
+  if ((rc = dalloc(c, (void **)&t.par, n * sizeof(float4)))) return rc;
+  if ((rc = upload(c, t.par, hpar.data(), n * sizeof(float4)))) return rc;
   if ((rc = upload(c, t.d64, h64.data(), n * sizeof(AtomRec64)))) return rc;
   if ((rc = upload(c, t.dmx, hmx.data(), n * sizeof(AtomRecMx)))) return rc;
   if ((rc = upload(c, t.c24, h24.data(), n * sizeof(double)))) return rc;
@@ -417,6 +427,7 @@ EnergyArgs energy_args(rhg_ctx *c, int type, double scaling, int do_vdw, int do_
   a.n_atoms = c->n;
   a.a64 = t.d64;
   a.amx = t.dmx;
+  a.par = t.par;
   a.precision = c->precision;
   a.do_vdw = do_vdw;
   a.do_cc = (do_cc && c->charge_method == RHG_CHARGE_EWALD && considered_charged(t.q) && c->n > 0) ? 1 : 0;
@@ -518,7 +529,52 @@ int ensure_cell_grid(rhg_ctx *c) {
   c->cg.ncell = ncell;
   c->cg.nb_off = c->d_nb_off;
   c->cg.nb_idx = c->d_nb_idx;
+  c->cg.img_off = nullptr;
+  c->cg.img = nullptr;
+  c->img_ready = false;
+  c->cell_reach = reach;
+  c->cell_halfdiag = R;
   c->grid_ready = true;
+  c->gen++;
+  return RHG_OK;
+}
+
+// Image lists of the cell grid for the mixed-precision kernels.  They hold
+// every image within reach, which equals the minimum-image convention when
+// every perpendicular width is at least twice the cut-off; the fixed-point
+// offsets need |site - image| < 32 A, and the 27 lattice shifts must cover
+// the reach.  Otherwise the lists stay off and the mixed kernels take the
+// minimum image per pair.
+int ensure_image_list(rhg_ctx *c) {
+  if (c->img_ready) return RHG_OK;
+  c->img_ready = true;
+  const double wmin = std::min(c->props[3], std::min(c->props[4], c->props[5]));
+  const double rc = std::max(c->rc_vdw, c->rc_cc);
+  if (!(wmin >= 2.0 * rc && c->cell_reach + c->cell_halfdiag < 31.0 && c->cell_reach < 1.5 * wmin))
+    return RHG_OK;
+  hipStream_t st = c->stream;
+  const int ncell = c->cg.ncell;
+  int32_t *counts = (int32_t *)work(c, "img_counts", (size_t)ncell * sizeof(int32_t));
+  if (!counts) return fail(c, RHG_ERR_OOM, "image counts");
+  hipFree(c->d_img_off);
+  c->d_img_off = nullptr;
+  int rc_;
+  if ((rc_ = dalloc(c, (void **)&c->d_img_off, (size_t)(ncell + 1) * sizeof(int32_t)))) return rc_;
+  launch_image_list(c->cell, c->cg.nc[0], c->cg.nc[1], c->cg.nc[2], c->cell_reach, c->d_ax, c->d_ay,
+                    c->d_az, c->n, counts, nullptr, nullptr, 0, st);
+  launch_scan_counts(counts, ncell, c->d_img_off, st);
+  int32_t total = 0;
+  HIPC(c, hipMemcpyAsync(&total, c->d_img_off + ncell, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIPC(c, hipStreamSynchronize(st));
+  hipFree(c->d_img);
+  c->d_img = nullptr;
+  if ((rc_ = dalloc(c, (void **)&c->d_img, (size_t)std::max(total, 1) * sizeof(int4)))) return rc_;
+  launch_image_list(c->cell, c->cg.nc[0], c->cg.nc[1], c->cg.nc[2], c->cell_reach, c->d_ax, c->d_ay,
+                    c->d_az, c->n, counts, c->d_img_off, c->d_img, 1, st);
+  HIPC(c, hipGetLastError());
+  HIPC(c, hipStreamSynchronize(st));
+  c->cg.img_off = c->d_img_off;
+  c->cg.img = c->d_img;
   c->gen++;
   return RHG_OK;
 }
@@ -534,6 +590,8 @@ int site_energy(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const Energy
     return RHG_OK;
   }
   if (int rc = ensure_cell_grid(c)) return rc;
+  if (c->precision != RHG_PREC_FP64 && c->use_images)
+    if (int rc = ensure_image_list(c)) return rc;
   const int ncell = c->cg.ncell;
   std::string t = tag;
   BinWork b;
@@ -552,8 +610,13 @@ int site_energy(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const Energy
     Timed tm(c, "bin");
     launch_bin_sites(c->cell, c->cg, s, b, c->stream);
   }
+  CellGrid g = c->cg;
+  if (!c->use_images) {
+    g.img_off = nullptr;
+    g.img = nullptr;
+  }
   Timed tm(c, "site_energy");
-  launch_site_energy_culled(a, c->cg, s, b, o, c->stream);
+  launch_site_energy_culled(a, g, s, b, o, c->stream);
   return RHG_OK;
 }
 
@@ -665,12 +728,15 @@ void rhg_destroy(rhg_ctx *c) {
     hipFree(t.c24);
     hipFree(t.c96);
     hipFree(t.c384);
+    hipFree(t.par);
   }
   for (auto &b : c->bufs) hipFree(b.second.p);
   hipFree(c->d_kfactor);
   hipFree(c->d_sf);
   hipFree(c->d_nb_off);
   hipFree(c->d_nb_idx);
+  hipFree(c->d_img_off);
+  hipFree(c->d_img);
   hipFree(c->d_ax);
   hipFree(c->d_ay);
   hipFree(c->d_az);
@@ -1593,7 +1659,10 @@ int rhg_set_culling(rhg_ctx *c, int32_t on, double cell_size) {
   if (!c) return RHG_ERR_INVALID;
   if (on && !(cell_size > 0.0)) return fail(c, RHG_ERR_INVALID, "cell size must be > 0");
   hipSetDevice(c->device);
+  if (on < 0 || on > 2) return fail(c, RHG_ERR_INVALID, "culling mode must be 0, 1 or 2");
   c->cull = on ? 1 : 0;
+  c->use_images = on == 2 ? 1 : 0;
+  c->gen++;
   if (on && cell_size != c->cell_size) {
     c->cell_size = cell_size;
     c->grid_ready = false;

@@ -28,6 +28,7 @@ struct EnergyArgs {
   int n_atoms;
   const AtomRec64 *a64; // FP64 records of this probe type
   const AtomRecMx *amx; // mixed records of this probe type
+  const float4 *par;    // per atom {4 eps, sigma^2 * 2^52, shift, q} of this probe type
   int precision;        // 0 FP64, 1 mixed
   int do_vdw, do_cc;
 };
@@ -40,11 +41,20 @@ void launch_site_energy(const EnergyArgs &a, const SiteList &s, const EnergyOut 
 // neighbour list of a cell holds, in ascending order, every atom whose
 // periodic (torus) distance to the cell centre is at most rc + R_cell, so it
 // contains every atom within rc of any point of the cell.
+// Fixed-point scale of the image lists: offsets from a cell centre in units
+// of 2^-26 A (int32 covers +-32 A), so that a site-atom difference is exact
+// and rounds once when converted to FP32.
+constexpr double kFix = 67108864.0;
+
 struct CellGrid {
   int nc[3];
   int ncell;
   const int32_t *nb_off; // ncell + 1
   const int32_t *nb_idx; // concatenated ascending atom indices
+  // image lists (nullable): per cell every periodic image of every atom
+  // within reach of the cell centre, {dx, dy, dz, atom} in fixed point
+  const int32_t *img_off; // ncell + 1
+  const int4 *img;
 };
 
 // scratch for binning one site list into cells
@@ -63,6 +73,10 @@ void launch_cell_neighbours(const Cell &cell, int nc0, int nc1, int nc2, double 
                             int32_t *counts, int32_t *nb_off, int32_t *nb_idx, int pass,
                             hipStream_t st);
 void launch_scan_counts(const int32_t *counts, int n, int32_t *off, hipStream_t st);
+// pass 0 counts the images of each cell into counts, pass 1 writes them at off
+void launch_image_list(const Cell &cell, int nc0, int nc1, int nc2, double reach, const double *ax,
+                       const double *ay, const double *az, int n_atoms, int32_t *counts,
+                       const int32_t *off, int4 *img, int pass, hipStream_t st);
 void launch_bin_sites(const Cell &cell, const CellGrid &g, const SiteList &s, const BinWork &b,
                       hipStream_t st);
 void launch_site_energy_culled(const EnergyArgs &a, const CellGrid &g, const SiteList &s,

@@ -6,6 +6,7 @@
 // loads into SGPRs and broadcast to all 64 lanes as an instruction operand
 // (no LDS traffic, no bank conflicts, no VGPRs for the framework).
 #include <float.h>
+#include <cstring>
 #include <hip/hip_runtime.h>
 
 #include "rhg_kernels.h"
@@ -344,6 +345,85 @@ void launch_scan_counts(const int32_t *counts, int n, int32_t *off, hipStream_t 
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, counts, n, off, 0);
 }
 
+__device__ __forceinline__ void cell_centre(const Cell &cell, const CellGrid &g, int c, double &px,
+                                            double &py, double &pz) {
+  const int cz = c % g.nc[2], cy = (c / g.nc[2]) % g.nc[1], cx = c / (g.nc[2] * g.nc[1]);
+  const double fx = (cx + 0.5) / g.nc[0], fy = (cy + 0.5) / g.nc[1], fz = (cz + 0.5) / g.nc[2];
+  px = cell.box[AX] * fx + cell.box[BX] * fy + cell.box[CX] * fz;
+  py = cell.box[AY] * fx + cell.box[BY] * fy + cell.box[CY] * fz;
+  pz = cell.box[AZ] * fx + cell.box[BZ] * fy + cell.box[CZ] * fz;
+}
+
+// One workgroup per cell.  Images are visited image-major (27 lattice
+// shifts of the wrapped offset), atoms in ascending order within each, so
+// the list order is fixed.
+__global__ __launch_bounds__(256) void k_image_list(const Cell cell, const CellGrid g, double reach2,
+                                                    const double *__restrict__ ax,
+                                                    const double *__restrict__ ay,
+                                                    const double *__restrict__ az, int n_atoms,
+                                                    int32_t *__restrict__ counts,
+                                                    const int32_t *__restrict__ off,
+                                                    int4 *__restrict__ img, int pass) {
+  __shared__ int wave_sum[4];
+  __shared__ int base;
+  const int c = blockIdx.x;
+  double cx, cy, cz;
+  cell_centre(cell, g, c, cx, cy, cz);
+  if (threadIdx.x == 0) base = pass ? off[c] : 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int im = 0; im < 27; ++im) {
+    const double na = im / 9 - 1, nb = (im / 3) % 3 - 1, ncc = im % 3 - 1;
+    for (int j0 = 0; j0 < n_atoms; j0 += blockDim.x) {
+      const int j = j0 + threadIdx.x;
+      bool in = false;
+      double vx = 0.0, vy = 0.0, vz = 0.0;
+      if (j < n_atoms) {
+        const double dx = ax[j] - cx, dy = ay[j] - cy, dz = az[j] - cz;
+        double sx = cell.inv[AX] * dx + cell.inv[BX] * dy + cell.inv[CX] * dz;
+        double sy = cell.inv[AY] * dx + cell.inv[BY] * dy + cell.inv[CY] * dz;
+        double sz = cell.inv[AZ] * dx + cell.inv[BZ] * dy + cell.inv[CZ] * dz;
+        sx = sx - rint(sx) + na;
+        sy = sy - rint(sy) + nb;
+        sz = sz - rint(sz) + ncc;
+        vx = cell.box[AX] * sx + cell.box[BX] * sy + cell.box[CX] * sz;
+        vy = cell.box[AY] * sx + cell.box[BY] * sy + cell.box[CY] * sz;
+        vz = cell.box[AZ] * sx + cell.box[BZ] * sy + cell.box[CZ] * sz;
+        in = vx * vx + vy * vy + vz * vz <= reach2;
+      }
+      const unsigned long long m = __ballot(in);
+      const int before = __popcll(m & ((1ULL << lane) - 1ULL));
+      if (lane == 0) wave_sum[wave] = __popcll(m);
+      __syncthreads();
+      int o = base;
+      for (int w = 0; w < wave; ++w) o += wave_sum[w];
+      if (in && pass)
+        img[o + before] = make_int4((int)rint(vx * kFix), (int)rint(vy * kFix), (int)rint(vz * kFix), j);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += wave_sum[w];
+        base += t;
+      }
+      __syncthreads();
+    }
+  }
+  if (threadIdx.x == 0 && !pass) counts[c] = base;
+}
+
+void launch_image_list(const Cell &cell, int nc0, int nc1, int nc2, double reach, const double *ax,
+                       const double *ay, const double *az, int n_atoms, int32_t *counts,
+                       const int32_t *off, int4 *img, int pass, hipStream_t st) {
+  CellGrid g;
+  memset(&g, 0, sizeof(g));
+  g.nc[0] = nc0;
+  g.nc[1] = nc1;
+  g.nc[2] = nc2;
+  g.ncell = nc0 * nc1 * nc2;
+  hipLaunchKernelGGL(k_image_list, dim3(g.ncell), dim3(256), 0, st, cell, g, reach * reach, ax, ay, az,
+                     n_atoms, counts, off, img, pass);
+}
+
 __device__ __forceinline__ int cell_index(const Cell &c, const CellGrid &g, double x, double y,
                                           double z) {
   double f[3];
@@ -500,6 +580,123 @@ __global__ __launch_bounds__(256) void k_site_energy_culled(
   }
 }
 
+// Mixed-precision site energies over the image lists: one wave per tile of
+// <= 64 sites of one cell.  The site's offset from the cell centre and the
+// listed image offsets are fixed point, so their difference is exact; it is
+// converted to FP32 once, and the LJ / real-space Coulomb terms are FP32
+// with FP64 accumulation.  Sites whose sum lands within the guard band of
+// the overlap threshold are recomputed exactly (FP64, atom order) so the
+// overlap flags equal the FP64 path's.
+template <int BCK, int VDW, int CC>
+__global__ __launch_bounds__(256) void k_site_energy_img(
+    const Cell cell, const AtomRec64 *__restrict__ a64, const float4 *__restrict__ par,
+    const double rc2v, const double rc2c, const double alpha, const double cqa, const CellGrid g,
+    const SiteList s, const BinWork b, double *__restrict__ out_v, double *__restrict__ out_c,
+    uint8_t *__restrict__ out_o) {
+  const int lane = threadIdx.x & 63;
+  const int tile = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  const int ntiles = b.tile_off[g.ncell];
+  if (tile >= ntiles) return;
+  int lo = 0, hi = g.ncell - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (b.tile_off[mid] <= tile) lo = mid; else hi = mid - 1;
+  }
+  const int c = __builtin_amdgcn_readfirstlane(lo);
+  const int k = (tile - b.tile_off[c]) * 64 + lane;
+  const bool live = k < b.count[c];
+  const int site = live ? b.perm[b.start[c] + k] : b.perm[b.start[c]];
+  const double px = s.x[site], py = s.y[site], pz = s.z[site];
+  double cx, cy, cz;
+  cell_centre(cell, g, c, cx, cy, cz);
+  double ox = px - cx, oy = py - cy, oz = pz - cz;
+  mi_exact<BCK>(cell, ox, oy, oz);
+  const int ix = (int)rint(ox * kFix), iy = (int)rint(oy * kFix), iz = (int)rint(oz * kFix);
+  const float rc2vf = (float)(rc2v * kFix * kFix), rc2cf = (float)(rc2c * kFix * kFix);
+  const float alphaf = (float)(alpha / kFix), cqaf = (float)(cqa * kFix);
+  const int q0 = g.img_off[c], q1 = g.img_off[c + 1];
+  double uv = 0.0, uc = 0.0;
+  // one pair term in FP32 (zero beyond the cut-off)
+  auto term = [&](const int4 e, float &fv, float &fc) {
+    const float4 pr = par[e.w];
+    const float dx = (float)(ix - e.x), dy = (float)(iy - e.y), dz = (float)(iz - e.z);
+    const float rr = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    if (VDW) {
+      const float ir = __builtin_amdgcn_rcpf(rr);
+      const float t = pr.y * ir;
+      const float r3 = (t * t) * t;
+      const float u = fmaf(pr.x, fmaf(r3, r3, -r3), -pr.z);
+      fv = (rr < rc2vf) ? u : 0.0f;
+    }
+    if (CC) {
+      fc = 0.0f;
+      if (rr < rc2cf) {
+        const float irs = __builtin_amdgcn_rsqf(rr);
+        const float e2 = erfcf(alphaf * (rr * irs));
+        fc = (cqaf * pr.w) * (e2 * irs);
+      }
+    }
+  };
+  // four terms are summed in FP32, the groups in FP64
+  int q = q0;
+  for (; q + 4 <= q1; q += 4) {
+    float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
+    term(g.img[q], v0, c0);
+    term(g.img[q + 1], v1, c1);
+    term(g.img[q + 2], v2, c2);
+    term(g.img[q + 3], v3, c3);
+    if (VDW) uv += (double)((v0 + v1) + (v2 + v3));
+    if (CC) uc += (double)((c0 + c1) + (c2 + c3));
+  }
+  for (; q < q1; ++q) {
+    float v0 = 0.f, c0 = 0.f;
+    term(g.img[q], v0, c0);
+    if (VDW) uv += (double)v0;
+    if (CC) uc += (double)c0;
+  }
+  const double band = 1.0e-4 * kOverlap;
+  const bool near = live && ((VDW && fabs(uv - kOverlap) <= band) || (CC && fabs(uc - kOverlap) <= band));
+  if (__any(near)) {
+    if (near) {
+#pragma clang fp contract(off)
+      double ev = 0.0, ec = 0.0;
+      const int j0 = g.nb_off[c], j1 = g.nb_off[c + 1];
+      for (int q = j0; q < j1; ++q) {
+        const AtomRec64 r = a64[g.nb_idx[q]];
+        double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
+        mi_exact<BCK>(cell, dx, dy, dz);
+        double rr = dx * dx + dy * dy + dz * dz;
+        if (VDW && rr < rc2v) ev += lj_exact(r.c4, r.s2, r.shift, rr);
+        if (CC && rr < rc2c) {
+          double rd = sqrt(rr);
+          ec += cqa * r.q * (erfc(alpha * rd) / rd);
+        }
+      }
+      if (VDW) uv = ev;
+      if (CC) uc = ec;
+    }
+  }
+  if (lane == 0 && b.pairs) {
+    const int nl = min(64, b.count[c] - (tile - b.tile_off[c]) * 64);
+    atomicAdd(b.pairs + ((VDW && CC) ? 1 : (VDW ? 0 : 2)),
+              (unsigned long long)nl * (unsigned long long)(q1 - q0));
+  }
+  if (live) {
+    if (out_v) out_v[site] = uv;
+    if (out_c) out_c[site] = uc;
+    if (out_o) out_o[site] = (uint8_t)((VDW && uv >= kOverlap) || (CC && uc >= kOverlap));
+  }
+}
+
+template <int BCK, int VDW, int CC>
+static void img_launch(const EnergyArgs &a, const CellGrid &g, const SiteList &s, const BinWork &b,
+                       const EnergyOut &o, hipStream_t st) {
+  const int64_t max_tiles = s.n / 64 + g.ncell + 1;
+  const unsigned nb = (unsigned)((max_tiles + 3) / 4);
+  hipLaunchKernelGGL((k_site_energy_img<BCK, VDW, CC>), dim3(nb), dim3(256), 0, st, a.cell, a.a64,
+                     a.par, a.rc2_vdw, a.rc2_cc, a.alpha, a.cqa, g, s, b, o.u_vdw, o.u_cc, o.overlap);
+}
+
 template <int PREC, int BCK, int VDW, int CC>
 static void culled_launch(const EnergyArgs &a, const CellGrid &g, const SiteList &s,
                           const BinWork &b, const EnergyOut &o, hipStream_t st) {
@@ -515,6 +712,12 @@ template <int VDW, int CC>
 static void culled_mode(const EnergyArgs &a, const CellGrid &g, const SiteList &s, const BinWork &b,
                         const EnergyOut &o, hipStream_t st) {
   const int bck = a.cell.bc == 1 ? 1 : (a.cell.upper ? 2 : 3);
+  if (a.precision != 0 && g.img_off && a.par) {
+    if (bck == 1) img_launch<1, VDW, CC>(a, g, s, b, o, st);
+    else if (bck == 2) img_launch<2, VDW, CC>(a, g, s, b, o, st);
+    else img_launch<3, VDW, CC>(a, g, s, b, o, st);
+    return;
+  }
   if (a.precision == 0) {
     if (bck == 1) culled_launch<0, 1, VDW, CC>(a, g, s, b, o, st);
     else if (bck == 2) culled_launch<0, 2, VDW, CC>(a, g, s, b, o, st);

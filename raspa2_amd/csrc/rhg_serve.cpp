@@ -101,7 +101,7 @@ double secs(Clock::time_point a, Clock::time_point b) {
 // *scratch receives the part that lives in a workspace when one is attached.
 size_t estimate_bytes(const rhg_framework *f, const rhg_cell *cell, const rhg_forcefield *ff,
                       int32_t n_types, int64_t chunk, int32_t n_sites, int32_t n_charged, int32_t k_fb,
-                      int32_t k_or, double cell_size, size_t *scratch = nullptr) {
+                      int32_t k_or, double cell_size, int32_t precision, size_t *scratch = nullptr) {
   const double *B = cell->box;
   // volume and perpendicular widths of the cell
   double a[3] = {B[0], B[1], B[2]}, b[3] = {B[3], B[4], B[5]}, c[3] = {B[6], B[7], B[8]};
@@ -133,9 +133,11 @@ size_t estimate_bytes(const rhg_framework *f, const rhg_cell *cell, const rhg_fo
     double tol = std::sqrt(std::fabs(std::log(eps * rc)));
     double alpha = std::sqrt(std::fabs(std::log(eps * rc * tol))) / rc;
     double tol1 = std::sqrt(-std::log(eps * rc * ((2.0 * tol * alpha) * (2.0 * tol * alpha))));
+    // the rule takes the lengths of the cell vectors (src/ewald.c:219-221)
+    const double len[3] = {norm(a), norm(b), norm(c)};
     int k[3];
     for (int d = 0; d < 3; ++d) {
-      double v = 0.25 + w[d] * alpha * tol1 / M_PI;
+      double v = 0.25 + len[d] * alpha * tol1 / M_PI;
       k[d] = (int)(v >= 0 ? v + 0.5 : v - 0.5);
     }
     int m = std::max(k[0], std::max(k[1], k[2]));
@@ -158,6 +160,8 @@ size_t estimate_bytes(const rhg_framework *f, const rhg_cell *cell, const rhg_fo
   double reach = rc + 0.5 * std::sqrt(3.0) * cell_size;
   double per_cell = std::min(n, n / V * 4.0 / 3.0 * M_PI * reach * reach * reach);
   bytes += ncell * (per_cell * 4.0 + 8.0);
+  // mixed precision adds the fixed-point image lists (16 B per entry)
+  if (precision == RHG_PREC_MIXED) bytes += ncell * (per_cell * 16.0 + 8.0);
   // Widom buffers per insertion: first-bead trials (5 x 8 B each), per
   // insertion scalars (132 B), orientation trials (4 x 8 B each), positions of
   // the selected sites (24 B each), placed-bead Coulomb (8 B each), Fourier
@@ -198,7 +202,7 @@ void tuned_options(rhg_server *s, FrameworkSpec &f) {
 }
 
 void ctx_options(rhg_server *s, FrameworkSpec &f) {
-  rhg_set_culling(f.ctx, 1, f.cell_size);
+  rhg_set_culling(f.ctx, 2, f.cell_size);
   rhg_set_widom_options(f.ctx, std::min(f.chunk, s->slice), 1);
 }
 
@@ -235,7 +239,8 @@ int reserve(rhg_server *s, FrameworkSpec &f, const rhg_probe &p, int slot, bool 
   for (int j = 0; j < p.n_sites; ++j) n_charged += std::fabs(p.charge[j]) > 1e-10 ? 1 : 0;
   size_t scratch = 0;
   const size_t total = estimate_bytes(&fr, &f.cell, &f.ff, p.n_sites, std::min(s->slice, f.chunk),
-                                      p.n_sites, n_charged, p.k_fb, p.k_or, f.cell_size, &scratch);
+                                      p.n_sites, n_charged, p.k_fb, p.k_or, f.cell_size, s->precision,
+                                      &scratch);
   const size_t own = total - scratch;
   if (total > s->budget) return sfail(s, RHG_ERR_NOT_ADMITTED, "request larger than the device budget");
   auto need = [&]() {
@@ -594,9 +599,10 @@ int rhg_server_stats(const rhg_server *s, double out[12]) {
 
 size_t rhg_estimate_footprint(const rhg_framework *f, const rhg_cell *cell, const rhg_forcefield *ff,
                               int32_t n_types, int64_t chunk, int32_t n_sites, int32_t n_charged,
-                              int32_t k_fb, int32_t k_or, double cell_size) {
+                              int32_t k_fb, int32_t k_or, double cell_size, int32_t precision) {
   if (!f || !cell || !ff || chunk < 1 || cell_size <= 0.0 || n_sites < 1) return 0;
-  return estimate_bytes(f, cell, ff, n_types, chunk, n_sites, n_charged, k_fb, k_or, cell_size);
+  return estimate_bytes(f, cell, ff, n_types, chunk, n_sites, n_charged, k_fb, k_or, cell_size,
+                        precision);
 }
 
 } // extern "C"

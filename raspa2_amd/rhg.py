@@ -380,8 +380,11 @@ class Context:
     def profile_reset(self):
         self._check(lib.rhg_profile_reset(self.h), "profile_reset")
 
-    def set_culling(self, on=True, cell_size=2.5):
-        self._check(lib.rhg_set_culling(self.h, 1 if on else 0, cell_size), "set_culling")
+    def set_culling(self, on=True, cell_size=2.5, images=True):
+        """Cell-list culling; `images` also enables the fixed-point image
+        lists of the mixed-precision kernels."""
+        mode = 0 if not on else (2 if images else 1)
+        self._check(lib.rhg_set_culling(self.h, mode, cell_size), "set_culling")
 
     def widom_stats(self, reset=False):
         out = np.zeros(12)

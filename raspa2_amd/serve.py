@@ -52,7 +52,8 @@ def _decl():
     f("rhg_server_result", _P, _I64, ctypes.POINTER(ResultT))
     f("rhg_server_stats", _P, _P)
     f("rhg_estimate_footprint", ctypes.POINTER(R.Framework), ctypes.POINTER(R.CellT),
-      ctypes.POINTER(R.ForceField), _I32, _I64, _I32, _I32, _I32, _I32, _D, res=ctypes.c_size_t)
+      ctypes.POINTER(R.ForceField), _I32, _I64, _I32, _I32, _I32, _I32, _D, _I32,
+      res=ctypes.c_size_t)
 
 
 _decl()
@@ -85,14 +86,15 @@ def probe_struct(probe):
     return p
 
 
-def estimate_footprint(spec, probe, charge_method=R.CHARGE_EWALD, chunk=1 << 18, cell_size=2.5):
+def estimate_footprint(spec, probe, charge_method=R.CHARGE_EWALD, chunk=1 << 18, cell_size=2.5,
+                       precision=R.PREC_MIXED):
     """Device bytes of a context of `spec` for `probe` (host arithmetic only)."""
     fw, cell, ff, _keep = _structs(spec, charge_method)
     n_types = len(set(probe.sites))
     n_charged = sum(1 for _, _, q in probe.sites if abs(q) > 1e-10)
     return R.lib.rhg_estimate_footprint(ctypes.byref(fw), ctypes.byref(cell), ctypes.byref(ff),
                                         n_types, chunk, probe.n_sites, n_charged, probe.k_fb,
-                                        probe.k_or, cell_size)
+                                        probe.k_or, cell_size, precision)
 
 
 class Server:

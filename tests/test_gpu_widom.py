@@ -12,11 +12,11 @@ FIELDS = dict(W=0, DU=1, UVDW=2, UREAL=3, UFHOST=4, UFADS=5, OVERLAP=6, SELFB=7,
 
 
 def _run_both(W, O, R, fw, probe, charge_method, precision, n, seed, k_fb=None, k_or=None,
-              cull=True):
+              cull=True, images=True):
     sys_o = oracle_system(fw, charge_method=charge_method)
     _, mo = oracle_molecule(sys_o, probe, k_fb, k_or)
     ctx = fw.context(charge_method=charge_method, precision=precision)
-    ctx.set_culling(cull, 2.5)
+    ctx.set_culling(cull, 2.5, images)
     _, mg = gpu_molecule(ctx, probe, k_fb, k_or)
     rec_o = O.widom_run_stream(sys_o, mo, seed, 0, n)
     blocks, rec_g = ctx.widom_run(mg, seed, n, records=True)
@@ -81,11 +81,13 @@ def test_co2_host_trials_match_device_trials(W, O, R):
     _check_flags(rec_o, rec_h)
 
 
-def test_mixed_flags_and_tolerance(W, O, R):
+@pytest.mark.parametrize("images", [False, True])
+def test_mixed_flags_and_tolerance(W, O, R, images):
     """Mixed precision: every overlap flag equals the FP64 reference; W and
     <U> within the stated tolerances."""
     fw = W.c1_framework()
-    sys_o, ctx, ro, rg, _ = _run_both(W, O, R, fw, W.CO2, O.CHARGE_EWALD, 1, 300, 13)
+    sys_o, ctx, ro, rg, _ = _run_both(W, O, R, fw, W.CO2, O.CHARGE_EWALD, 1, 300, 13,
+                                      images=images)
     for f in ("OVERLAP", "FBMASK", "ORMASK"):
         assert np.array_equal(ro[:, FIELDS[f]], rg[:, FIELDS[f]]), f
     same_sel = (ro[:, FIELDS["SELOR"]] == rg[:, FIELDS["SELOR"]]) & (ro[:, 13] == 4) & (rg[:, 13] == 4)
