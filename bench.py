@@ -184,6 +184,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=400, help="insertions per CPU process")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["c1", "c3", "c4"], default="c1")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 512 MiB device copy kernel (k_stream_copy) three "
+                         "times, to scale profiler byte counters")
     ap.add_argument("--frameworks", type=int, default=256, help="c3/c4 catalogue size")
     ap.add_argument("--requests", type=int, default=384, help="c4 requests")
     ap.add_argument("--slots", type=int, default=8, help="c3/c4 requests in flight")
@@ -211,6 +214,8 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(fw, probe, args.cpu_sample)
 
+    if args.copy_calibration:
+        R.stream_copy(512 << 20, reps=3)
     prec = R.PREC_MIXED if args.precision == "mixed" else R.PREC_FP64
     ctx = fw.context(charge_method=R.CHARGE_EWALD, precision=prec)
     _, mol = W.register_probe(ctx, probe)
@@ -233,7 +238,7 @@ def main():
     value = args.steps * n / elapsed
     st = ctx.widom_stats()
     kernels = {}
-    for name in ("site_energy", "fourier", "widom_gen", "widom_select", "block_reduce"):
+    for name in ("site_energy", "bin", "fourier", "widom_gen", "widom_select", "block_reduce"):
         cnt, ms = ctx.profile_get(name)
         kernels[name] = {"launches": cnt, "total_ms": round(ms, 4),
                          "avg_ms": round(ms / cnt, 5) if cnt else None}

@@ -239,6 +239,11 @@ int rhg_profile_enable(rhg_ctx *ctx, int32_t on);
 int rhg_profile_get(rhg_ctx *ctx, const char *kernel, int64_t *launches, double *total_ms);
 int rhg_profile_reset(rhg_ctx *ctx);
 int rhg_synchronize(rhg_ctx *ctx);
+/* No counterpart in the reference: `reps` launches of a plain streaming
+ * copy kernel (k_stream_copy) over `bytes` (multiple of 16), which reads and
+ * writes exactly `bytes` each; a profiler's byte counters on it give the
+ * scale to turn counter readings into bytes. */
+int rhg_stream_copy(int device, int64_t bytes, int32_t reps);
 /* Work done by rhg_widom_run since the last reset (no counterpart in the
  * reference; feeds the roofline): out[0] insertions, [1] first-bead sites,
  * [2] orientation sites, [3] Coulomb sites of selected orientations, [4]

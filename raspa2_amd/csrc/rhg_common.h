@@ -209,6 +209,12 @@ struct MolDev {
   double body[kMaxSites][3];  // Components[c].Positions
   double q[kMaxSites];        // site charges (0 for uncharged)
   int charged[kMaxSites];     // HasCharges
+  // storage of the placed beads, grouped by pseudo-atom type so that each
+  // type is one site list: for placed bead p, the first storage slot of its
+  // group, the group size and p's index within the group
+  int g_start[kMaxSites], g_size[kMaxSites], g_b[kMaxSites];
+  // the same over the charged placed beads (h_start < 0: p is uncharged)
+  int h_start[kMaxSites], h_size[kMaxSites], h_b[kMaxSites];
 };
 
 } // namespace rhg

@@ -560,8 +560,8 @@ int ensure_image_list(rhg_ctx *c) {
   c->d_img_off = nullptr;
   int rc_;
   if ((rc_ = dalloc(c, (void **)&c->d_img_off, (size_t)(ncell + 1) * sizeof(int32_t)))) return rc_;
-  launch_image_list(c->cell, c->cg.nc[0], c->cg.nc[1], c->cg.nc[2], c->cell_reach, c->d_ax, c->d_ay,
-                    c->d_az, c->n, counts, nullptr, nullptr, 0, st);
+  launch_image_list(c->cell, c->cg, c->cell_reach, c->d_ax, c->d_ay, c->d_az, counts, nullptr, nullptr,
+                    0, st);
   launch_scan_counts(counts, ncell, c->d_img_off, st);
   int32_t total = 0;
   HIPC(c, hipMemcpyAsync(&total, c->d_img_off + ncell, sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -569,8 +569,8 @@ int ensure_image_list(rhg_ctx *c) {
   hipFree(c->d_img);
   c->d_img = nullptr;
   if ((rc_ = dalloc(c, (void **)&c->d_img, (size_t)std::max(total, 1) * sizeof(int4)))) return rc_;
-  launch_image_list(c->cell, c->cg.nc[0], c->cg.nc[1], c->cg.nc[2], c->cell_reach, c->d_ax, c->d_ay,
-                    c->d_az, c->n, counts, c->d_img_off, c->d_img, 1, st);
+  launch_image_list(c->cell, c->cg, c->cell_reach, c->d_ax, c->d_ay, c->d_az, counts, c->d_img_off,
+                    c->d_img, 1, st);
   HIPC(c, hipGetLastError());
   HIPC(c, hipStreamSynchronize(st));
   c->cg.img_off = c->d_img_off;
@@ -905,6 +905,40 @@ int build_mol(rhg_ctx *c, const rhg_molecule *m, MolDev &d) {
     d.charged[j] = considered_charged(d.q[j]) ? 1 : 0;
     if (j != m->start) d.place[d.n_place++] = j;
   }
+  // storage groups: placed beads whose pseudo-atoms have the same
+  // parameters are stored (and evaluated) together; groups in order of their
+  // first bead
+  auto same_params = [&](int ta, int tb) {
+    const PseudoAtom &A = c->types[ta], &B = c->types[tb];
+    return ta == tb || (A.eps_k == B.eps_k && A.sigma == B.sigma && A.q == B.q);
+  };
+  auto group = [&](bool charged_only, int *start, int *size, int *b) {
+    int next = 0;
+    bool done[kMaxSites] = {false};
+    for (int p = 0; p < d.n_place; ++p) {
+      start[p] = -1;
+      size[p] = 0;
+      b[p] = 0;
+    }
+    for (int p = 0; p < d.n_place; ++p) {
+      if (done[p] || (charged_only && !d.charged[d.place[p]])) continue;
+      int cnt = 0;
+      for (int q = p; q < d.n_place; ++q)
+        if (!done[q] && same_params(m->type[d.place[q]], m->type[d.place[p]]) &&
+            !(charged_only && !d.charged[d.place[q]])) {
+          start[q] = next;
+          b[q] = cnt++;
+          done[q] = true;
+        }
+      for (int q = p; q < d.n_place; ++q)
+        if (start[q] == next) size[q] = cnt;
+      next += cnt;
+    }
+  };
+  group(false, d.g_start, d.g_size, d.g_b);
+  group(true, d.h_start, d.h_size, d.h_b);
+  if (c->charge_method != RHG_CHARGE_EWALD)
+    for (int p = 0; p < d.n_place; ++p) d.h_start[p] = -1;
   return RHG_OK;
 }
 
@@ -960,6 +994,9 @@ int widom_buffers(rhg_ctx *c, const MolDev &m, int64_t cap, WidomBufs &B) {
   WB(cfy, "w_cfy", ncf * d8);
   WB(cfz, "w_cfz", ncf * d8);
   WB(cc_place, "w_ccplace", (size_t)cap * std::max(m.n_place, 1) * d8);
+  WB(ccx, "w_ccx", (size_t)cap * std::max(m.n_place, 1) * d8);
+  WB(ccy, "w_ccy", (size_t)cap * std::max(m.n_place, 1) * d8);
+  WB(ccz, "w_ccz", (size_t)cap * std::max(m.n_place, 1) * d8);
   WB(f_host, "w_fhost", cap * d8);
   WB(f_ads, "w_fads", cap * d8);
 #undef WB
@@ -1030,31 +1067,37 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
     launch_widom_fb_select(a, w, st);
   }
   if (m.n_sites > 1) {
-    // ComputeExternalEnergies VDW of every orientation trial (src/cbmc.c:1210)
+    // ComputeExternalEnergies VDW of every orientation trial (src/cbmc.c:1210),
+    // one site list per type group of placed beads; energies are written
+    // [storage slot][orientation][active slot] for coalesced selection
     const int64_t cap_or = w.cap * (int64_t)m.k_or;
     for (int p = 0; p < m.n_place; ++p) {
+      if (m.g_b[p] != 0) continue; // first bead of its group
       EnergyArgs ea = energy_args(c, mol->type[m.place[p]], 1.0, 1, 0);
-      SiteList s{w.orx + p * cap_or, w.ory + p * cap_or, w.orz + p * cap_or, n * m.k_or,
-                 w.n_active, m.k_or};
-      EnergyOut o{w.or_uv + p * cap_or, nullptr, nullptr};
+      const int64_t base = m.g_start[p] * cap_or;
+      const int rows = m.g_size[p] * m.k_or;
+      SiteList s{w.orx + base, w.ory + base, w.orz + base, n * rows, w.n_active, rows, rows, w.cap};
+      EnergyOut o{w.or_uv + base, nullptr, nullptr};
       if (int rc = site_energy(c, ea, s, o, "bor")) return rc;
     }
     {
       Timed t(c, "widom_select");
       launch_widom_or_select(a, w, st);
     }
-    // real-space Coulomb of the placed beads of the selected orientation
-    // (src/cbmc.c:1216; only the selected trial's value is used)
+    // real-space Coulomb of the charged placed beads of the selected
+    // orientation (src/cbmc.c:1216; only the selected trial's value is used),
+    // one site list per type group
     for (int p = 0; p < m.n_place; ++p) {
+      if (m.h_start[p] < 0 || m.h_b[p] != 0) continue;
       EnergyArgs ea = energy_args(c, mol->type[m.place[p]], 1.0, 0, 1);
-      double *dst = w.cc_place + p * w.cap;
+      const int64_t base = m.h_start[p] * w.cap;
+      const int rows = m.h_size[p];
       if (!ea.do_cc) {
-        fill(dst, n, 0.0, st);
+        fill(w.cc_place + base, (int64_t)rows * w.cap, 0.0, st);
         continue;
       }
-      const int j = m.place[p];
-      SiteList s{w.cfx + j * w.cap, w.cfy + j * w.cap, w.cfz + j * w.cap, n, w.n_active2, 1};
-      EnergyOut o{nullptr, dst, nullptr};
+      SiteList s{w.ccx + base, w.ccy + base, w.ccz + base, n * rows, w.n_active2, rows, rows, w.cap};
+      EnergyOut o{nullptr, w.cc_place + base, nullptr};
       if (int rc = site_energy(c, ea, s, o, "bcc")) return rc;
     }
   }
@@ -1719,6 +1762,26 @@ int rhg_attach_workspace(rhg_ctx *c, rhg_workspace *ws) {
     c->ws = ws;
   }
   return RHG_OK;
+}
+
+int rhg_stream_copy(int device, int64_t bytes, int32_t reps) {
+  if (bytes < 16 || bytes % 16 || reps < 1) return RHG_ERR_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return RHG_ERR_HIP;
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, (size_t)bytes) != hipSuccess) return RHG_ERR_OOM;
+  if (hipMalloc(&b, (size_t)bytes) != hipSuccess) {
+    hipFree(a);
+    return RHG_ERR_OOM;
+  }
+  int rc = RHG_OK;
+  hipStream_t st = nullptr;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) rc = RHG_ERR_HIP;
+  for (int r = 0; r < reps && rc == RHG_OK; ++r) launch_stream_copy(a, b, bytes, st);
+  if (rc == RHG_OK && hipStreamSynchronize(st) != hipSuccess) rc = RHG_ERR_HIP;
+  if (st) hipStreamDestroy(st);
+  hipFree(a);
+  hipFree(b);
+  return rc;
 }
 
 int rhg_synchronize(rhg_ctx *c) {

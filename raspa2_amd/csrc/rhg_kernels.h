@@ -12,7 +12,15 @@ struct SiteList {
   int64_t n;               // number of sites (upper bound when n_dev is set)
   const int32_t *n_dev;    // optional device-side count (<= n)
   int32_t count_mult;      // sites = (*n_dev) * count_mult when n_dev is set
+  // outputs of site s go to (s % out_rows) * out_stride + s / out_rows when
+  // out_rows > 0 (a transposed layout for coalesced reads), else to s
+  int32_t out_rows;
+  int64_t out_stride;
 };
+
+__device__ __forceinline__ int64_t out_index(const SiteList &s, int64_t site) {
+  return s.out_rows > 0 ? (site % s.out_rows) * s.out_stride + site / s.out_rows : site;
+}
 
 struct EnergyOut {
   double *u_vdw; // nullable
@@ -73,10 +81,11 @@ void launch_cell_neighbours(const Cell &cell, int nc0, int nc1, int nc2, double 
                             int32_t *counts, int32_t *nb_off, int32_t *nb_idx, int pass,
                             hipStream_t st);
 void launch_scan_counts(const int32_t *counts, int n, int32_t *off, hipStream_t st);
-// pass 0 counts the images of each cell into counts, pass 1 writes them at off
-void launch_image_list(const Cell &cell, int nc0, int nc1, int nc2, double reach, const double *ax,
-                       const double *ay, const double *az, int n_atoms, int32_t *counts,
-                       const int32_t *off, int4 *img, int pass, hipStream_t st);
+// pass 0 counts the images of each cell's neighbour atoms into counts, pass 1
+// writes them at off
+void launch_image_list(const Cell &cell, const CellGrid &g, double reach, const double *ax,
+                       const double *ay, const double *az, int32_t *counts, const int32_t *off,
+                       int4 *img, int pass, hipStream_t st);
 void launch_bin_sites(const Cell &cell, const CellGrid &g, const SiteList &s, const BinWork &b,
                       hipStream_t st);
 void launch_site_energy_culled(const EnergyArgs &a, const CellGrid &g, const SiteList &s,
@@ -131,10 +140,13 @@ struct WidomWork {
   int32_t *active, *n_active;
   // orientation sites per placed bead p: [p][slot*k_or + iu]
   double *orx, *ory, *orz, *or_uv;
+  // positions of the charged placed beads of the selected orientations,
+  // grouped by pseudo-atom type (one site list per group)
+  double *ccx, *ccy, *ccz;
   // compaction after the orientation stage; selected configuration [j][slot2]
   int32_t *active2, *n_active2;
   double *cfx, *cfy, *cfz;
-  double *cc_place; // [p][slot2] real-space Coulomb of the placed beads
+  double *cc_place; // [charged placed slot][slot2] real-space Coulomb of the placed beads
   double *f_host, *f_ads; // [slot2]
   // host-supplied trials (nullable)
   const double *h_fb, *h_cord, *h_usel;
@@ -194,6 +206,9 @@ void launch_widom_stats(const WidomArgs &a, const WidomWork &w, const StatArgs &
                         hipStream_t st);
 // ctl[CTL_FIRST] += step (moves a captured chunk on to the next insertions)
 void launch_advance(int64_t *ctl, int64_t step, hipStream_t st);
+
+// streaming copy of `bytes` (a multiple of 16) for byte-counter calibration
+void launch_stream_copy(const void *src, void *dst, int64_t bytes, hipStream_t st);
 
 // ----------------------------------------------------------------- grids
 struct GridArgs {

@@ -135,9 +135,10 @@ __global__ __launch_bounds__(256) void k_site_energy64(
     }
   }
   if (live) {
-    if (out_v) out_v[i] = uv;
-    if (out_c) out_c[i] = uc;
-    if (out_o) out_o[i] = (uint8_t)((VDW && uv >= kOverlap) || (CC && uc >= kOverlap));
+    const int64_t oi = out_index(s, i);
+    if (out_v) out_v[oi] = uv;
+    if (out_c) out_c[oi] = uc;
+    if (out_o) out_o[oi] = (uint8_t)((VDW && uv >= kOverlap) || (CC && uc >= kOverlap));
   }
 }
 
@@ -196,9 +197,10 @@ __global__ __launch_bounds__(256) void k_site_energy_mx(
     }
   }
   if (live) {
-    if (out_v) out_v[i] = uv;
-    if (out_c) out_c[i] = uc;
-    if (out_o) out_o[i] = (uint8_t)((VDW && uv >= kOverlap) || (CC && uc >= kOverlap));
+    const int64_t oi = out_index(s, i);
+    if (out_v) out_v[oi] = uv;
+    if (out_c) out_c[oi] = uc;
+    if (out_o) out_o[oi] = (uint8_t)((VDW && uv >= kOverlap) || (CC && uc >= kOverlap));
   }
 }
 
@@ -354,13 +356,14 @@ __device__ __forceinline__ void cell_centre(const Cell &cell, const CellGrid &g,
   pz = cell.box[AZ] * fx + cell.box[BZ] * fy + cell.box[CZ] * fz;
 }
 
-// One workgroup per cell.  Images are visited image-major (27 lattice
-// shifts of the wrapped offset), atoms in ascending order within each, so
-// the list order is fixed.
+// One workgroup per cell over the cell's neighbour atoms (every atom within
+// reach in torus distance): each of the 27 lattice shifts of the wrapped
+// offset that lies within reach of the centre is an entry.  Visited
+// image-major, then in neighbour-list order, so the list order is fixed.
 __global__ __launch_bounds__(256) void k_image_list(const Cell cell, const CellGrid g, double reach2,
                                                     const double *__restrict__ ax,
                                                     const double *__restrict__ ay,
-                                                    const double *__restrict__ az, int n_atoms,
+                                                    const double *__restrict__ az,
                                                     int32_t *__restrict__ counts,
                                                     const int32_t *__restrict__ off,
                                                     int4 *__restrict__ img, int pass) {
@@ -372,56 +375,53 @@ __global__ __launch_bounds__(256) void k_image_list(const Cell cell, const CellG
   if (threadIdx.x == 0) base = pass ? off[c] : 0;
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int im = 0; im < 27; ++im) {
-    const double na = im / 9 - 1, nb = (im / 3) % 3 - 1, ncc = im % 3 - 1;
-    for (int j0 = 0; j0 < n_atoms; j0 += blockDim.x) {
-      const int j = j0 + threadIdx.x;
-      bool in = false;
-      double vx = 0.0, vy = 0.0, vz = 0.0;
-      if (j < n_atoms) {
-        const double dx = ax[j] - cx, dy = ay[j] - cy, dz = az[j] - cz;
-        double sx = cell.inv[AX] * dx + cell.inv[BX] * dy + cell.inv[CX] * dz;
-        double sy = cell.inv[AY] * dx + cell.inv[BY] * dy + cell.inv[CY] * dz;
-        double sz = cell.inv[AZ] * dx + cell.inv[BZ] * dy + cell.inv[CZ] * dz;
-        sx = sx - rint(sx) + na;
-        sy = sy - rint(sy) + nb;
-        sz = sz - rint(sz) + ncc;
-        vx = cell.box[AX] * sx + cell.box[BX] * sy + cell.box[CX] * sz;
-        vy = cell.box[AY] * sx + cell.box[BY] * sy + cell.box[CY] * sz;
-        vz = cell.box[AZ] * sx + cell.box[BZ] * sy + cell.box[CZ] * sz;
-        in = vx * vx + vy * vy + vz * vz <= reach2;
-      }
-      const unsigned long long m = __ballot(in);
-      const int before = __popcll(m & ((1ULL << lane) - 1ULL));
-      if (lane == 0) wave_sum[wave] = __popcll(m);
-      __syncthreads();
-      int o = base;
-      for (int w = 0; w < wave; ++w) o += wave_sum[w];
-      if (in && pass)
-        img[o + before] = make_int4((int)rint(vx * kFix), (int)rint(vy * kFix), (int)rint(vz * kFix), j);
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        int t = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += wave_sum[w];
-        base += t;
-      }
-      __syncthreads();
+  const int q0 = g.nb_off[c], cnt = g.nb_off[c + 1] - q0;
+  const int total = 27 * cnt;
+  for (int i0 = 0; i0 < total; i0 += blockDim.x) {
+    const int i = i0 + threadIdx.x;
+    bool in = false;
+    double vx = 0.0, vy = 0.0, vz = 0.0;
+    int j = 0;
+    if (i < total) {
+      const int im = i / cnt;
+      j = g.nb_idx[q0 + i % cnt];
+      const double na = im / 9 - 1, nb = (im / 3) % 3 - 1, ncc = im % 3 - 1;
+      const double dx = ax[j] - cx, dy = ay[j] - cy, dz = az[j] - cz;
+      double sx = cell.inv[AX] * dx + cell.inv[BX] * dy + cell.inv[CX] * dz;
+      double sy = cell.inv[AY] * dx + cell.inv[BY] * dy + cell.inv[CY] * dz;
+      double sz = cell.inv[AZ] * dx + cell.inv[BZ] * dy + cell.inv[CZ] * dz;
+      sx = sx - rint(sx) + na;
+      sy = sy - rint(sy) + nb;
+      sz = sz - rint(sz) + ncc;
+      vx = cell.box[AX] * sx + cell.box[BX] * sy + cell.box[CX] * sz;
+      vy = cell.box[AY] * sx + cell.box[BY] * sy + cell.box[CY] * sz;
+      vz = cell.box[AZ] * sx + cell.box[BZ] * sy + cell.box[CZ] * sz;
+      in = vx * vx + vy * vy + vz * vz <= reach2;
     }
+    const unsigned long long m = __ballot(in);
+    const int before = __popcll(m & ((1ULL << lane) - 1ULL));
+    if (lane == 0) wave_sum[wave] = __popcll(m);
+    __syncthreads();
+    int o = base;
+    for (int w = 0; w < wave; ++w) o += wave_sum[w];
+    if (in && pass)
+      img[o + before] = make_int4((int)rint(vx * kFix), (int)rint(vy * kFix), (int)rint(vz * kFix), j);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int t = 0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += wave_sum[w];
+      base += t;
+    }
+    __syncthreads();
   }
   if (threadIdx.x == 0 && !pass) counts[c] = base;
 }
 
-void launch_image_list(const Cell &cell, int nc0, int nc1, int nc2, double reach, const double *ax,
-                       const double *ay, const double *az, int n_atoms, int32_t *counts,
-                       const int32_t *off, int4 *img, int pass, hipStream_t st) {
-  CellGrid g;
-  memset(&g, 0, sizeof(g));
-  g.nc[0] = nc0;
-  g.nc[1] = nc1;
-  g.nc[2] = nc2;
-  g.ncell = nc0 * nc1 * nc2;
+void launch_image_list(const Cell &cell, const CellGrid &g, double reach, const double *ax,
+                       const double *ay, const double *az, int32_t *counts, const int32_t *off,
+                       int4 *img, int pass, hipStream_t st) {
   hipLaunchKernelGGL(k_image_list, dim3(g.ncell), dim3(256), 0, st, cell, g, reach * reach, ax, ay, az,
-                     n_atoms, counts, off, img, pass);
+                     counts, off, img, pass);
 }
 
 __device__ __forceinline__ int cell_index(const Cell &c, const CellGrid &g, double x, double y,
@@ -448,21 +448,89 @@ __global__ __launch_bounds__(256) void k_zero_i32(int32_t *__restrict__ a, int32
   }
 }
 
+// Lanes of a wave that bin into the same cell as their left neighbour form a
+// run (sites of one insertion are consecutive and close together); the run's
+// first lane makes one atomic for the whole run and shares the result.
+struct RunInfo {
+  int head;   // lane that leads this lane's run
+  int length; // lanes in the run
+};
+
+__device__ __forceinline__ RunInfo wave_runs(int c, bool valid) {
+  const int lane = threadIdx.x & 63;
+  const int left = __shfl_up(c, 1);
+  const bool head = valid && (lane == 0 || left != c);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long live = __ballot(valid);
+  const unsigned long long upto = (lane == 63) ? ~0ULL : ((2ULL << lane) - 1ULL);
+  RunInfo r;
+  r.head = 63 - __builtin_clzll(heads & upto | 1ULL);
+  const unsigned long long after = heads & ~upto;
+  // runs end at the next head or at the first invalid lane
+  const unsigned long long stop = after | (~live & ~upto);
+  const int end = stop ? __builtin_ctzll(stop) : 64;
+  r.length = end - r.head;
+  return r;
+}
+
 __global__ __launch_bounds__(256) void k_bin_count(const Cell cell, const CellGrid g,
                                                    const SiteList s, BinWork b) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= site_count(s)) return;
-  const int c = cell_index(cell, g, s.x[i], s.y[i], s.z[i]);
+  const bool valid = i < site_count(s);
+  const int c = valid ? cell_index(cell, g, s.x[i], s.y[i], s.z[i]) : -1;
+  const RunInfo r = wave_runs(c, valid);
+  if (!valid) return;
   b.cell_of[i] = c;
-  atomicAdd(&b.count[c], 1);
+  if ((int)(threadIdx.x & 63) == r.head) atomicAdd(&b.count[c], r.length);
 }
 
 __global__ __launch_bounds__(256) void k_bin_scatter(const SiteList s, BinWork b) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= site_count(s)) return;
-  const int c = b.cell_of[i];
-  const int p = b.start[c] + atomicAdd(&b.fill[c], 1);
-  b.perm[p] = (int32_t)i;
+  const bool valid = i < site_count(s);
+  const int c = valid ? b.cell_of[i] : -1;
+  const RunInfo r = wave_runs(c, valid);
+  int base = 0;
+  const int lane = threadIdx.x & 63;
+  if (valid && lane == r.head) base = b.start[c] + atomicAdd(&b.fill[c], r.length);
+  base = __shfl(base, r.head);
+  if (valid) b.perm[base + (lane - r.head)] = (int32_t)i;
+}
+
+// exclusive scans of the per-cell site counts (start) and of the per-cell
+// tile counts ceil(count/64) (tile_off), one workgroup
+__global__ __launch_bounds__(1024) void k_scan_bins(const int32_t *__restrict__ in, int n,
+                                                    int32_t *__restrict__ start,
+                                                    int32_t *__restrict__ tiles) {
+  __shared__ int ps[1024], pt[1024];
+  const int t = threadIdx.x;
+  const int per = (n + blockDim.x - 1) / blockDim.x;
+  const int b = t * per, e = min(n, b + per);
+  int s = 0, u = 0;
+  for (int i = b; i < e; ++i) {
+    s += in[i];
+    u += (in[i] + 63) / 64;
+  }
+  ps[t] = s;
+  pt[t] = u;
+  __syncthreads();
+  for (int d = 1; d < (int)blockDim.x; d <<= 1) {
+    const int vs = (t >= d) ? ps[t - d] : 0, vt = (t >= d) ? pt[t - d] : 0;
+    __syncthreads();
+    ps[t] += vs;
+    pt[t] += vt;
+    __syncthreads();
+  }
+  int rs = (t == 0) ? 0 : ps[t - 1], rt = (t == 0) ? 0 : pt[t - 1];
+  for (int i = b; i < e; ++i) {
+    start[i] = rs;
+    tiles[i] = rt;
+    rs += in[i];
+    rt += (in[i] + 63) / 64;
+  }
+  if (t == (int)blockDim.x - 1) {
+    start[n] = ps[t];
+    tiles[n] = pt[t];
+  }
 }
 
 void launch_bin_sites(const Cell &cell, const CellGrid &g, const SiteList &s, const BinWork &b,
@@ -471,8 +539,7 @@ void launch_bin_sites(const Cell &cell, const CellGrid &g, const SiteList &s, co
   hipLaunchKernelGGL(k_zero_i32, dim3(nbc), dim3(256), 0, st, b.count, b.fill, g.ncell);
   const unsigned nb = (unsigned)((s.n + 255) / 256);
   if (nb) hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(256), 0, st, cell, g, s, b);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, b.count, g.ncell, b.start, 0);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, b.count, g.ncell, b.tile_off, 1);
+  hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(1024), 0, st, b.count, g.ncell, b.start, b.tile_off);
   if (nb) hipLaunchKernelGGL(k_bin_scatter, dim3(nb), dim3(256), 0, st, s, b);
 }
 
@@ -574,9 +641,10 @@ __global__ __launch_bounds__(256) void k_site_energy_culled(
               (unsigned long long)nl * (unsigned long long)(j1 - j0));
   }
   if (live) {
-    if (out_v) out_v[site] = uv;
-    if (out_c) out_c[site] = uc;
-    if (out_o) out_o[site] = (uint8_t)((VDW && uv >= kOverlap) || (CC && uc >= kOverlap));
+    const int64_t oi = out_index(s, site);
+    if (out_v) out_v[oi] = uv;
+    if (out_c) out_c[oi] = uc;
+    if (out_o) out_o[oi] = (uint8_t)((VDW && uv >= kOverlap) || (CC && uc >= kOverlap));
   }
 }
 
@@ -682,9 +750,10 @@ __global__ __launch_bounds__(256) void k_site_energy_img(
               (unsigned long long)nl * (unsigned long long)(q1 - q0));
   }
   if (live) {
-    if (out_v) out_v[site] = uv;
-    if (out_c) out_c[site] = uc;
-    if (out_o) out_o[site] = (uint8_t)((VDW && uv >= kOverlap) || (CC && uc >= kOverlap));
+    const int64_t oi = out_index(s, site);
+    if (out_v) out_v[oi] = uv;
+    if (out_c) out_c[oi] = uc;
+    if (out_o) out_o[oi] = (uint8_t)((VDW && uv >= kOverlap) || (CC && uc >= kOverlap));
   }
 }
 
@@ -1107,19 +1176,9 @@ __global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, cons
   const int ko = m.k_or;
   const int64_t cap_or = w.cap * (int64_t)ko;
   uint64_t c = (uint64_t)(3 * k + 1);
-  double body0[kMaxSites][3];
-  for (int j = 0; j < m.n_sites; ++j) {
-    body0[j][0] = m.body[j][0] - m.body[m.start][0];
-    body0[j][1] = m.body[j][1] - m.body[m.start][1];
-    body0[j][2] = m.body[j][2] - m.body[m.start][2];
-  }
   for (int iu = 0; iu < ko; ++iu) {
-    double cord[kMaxSites][3];
-    if (a.use_host_trials) {
-      for (int j = 0; j < m.n_sites; ++j)
-        for (int d = 0; d < 3; ++d)
-          cord[j][d] = w.h_cord[((i * ko + iu) * m.n_sites + j) * 3 + d];
-    } else {
+    Rot R;
+    if (!a.use_host_trials) {
       // RandomNumberOnUnitSphere (src/utils.c:688-708); unused for rigid groups
       double r1, r2, rsq;
       int guard = 0;
@@ -1131,19 +1190,29 @@ __global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, cons
       double X0 = stream_uniform(key, c++);
       double u1 = stream_uniform(key, c++);
       double u2 = stream_uniform(key, c++);
-      Rot R = shoemake(X0, u1, u2);
-      for (int j = 0; j < m.n_sites; ++j) rotate(R, body0[j], cord[j]);
+      R = shoemake(X0, u1, u2);
     }
+    // the placed beads: body offsets from the starting bead, rotated
     for (int p = 0; p < m.n_place; ++p) {
       const int j = m.place[p];
+      double cj[3];
+      if (a.use_host_trials) {
+        for (int d = 0; d < 3; ++d) cj[d] = w.h_cord[((i * ko + iu) * m.n_sites + j) * 3 + d];
+      } else {
+        double b0[3];
+        b0[0] = m.body[j][0] - m.body[m.start][0];
+        b0[1] = m.body[j][1] - m.body[m.start][1];
+        b0[2] = m.body[j][2] - m.body[m.start][2];
+        rotate(R, b0, cj);
+      }
       double x, y, z;
       {
 #pragma clang fp contract(off)
-        x = fx + cord[j][0];
-        y = fy + cord[j][1];
-        z = fz + cord[j][2];
+        x = fx + cj[0];
+        y = fy + cj[1];
+        z = fz + cj[2];
       }
-      const int64_t o = p * cap_or + (int64_t)slot * ko + iu;
+      const int64_t o = m.g_start[p] * cap_or + (int64_t)slot * (m.g_size[p] * ko) + m.g_b[p] * ko + iu;
       w.orx[o] = x;
       w.ory[o] = y;
       w.orz[o] = z;
@@ -1161,11 +1230,13 @@ __global__ __launch_bounds__(256) void k_widom_or_select(const WidomArgs a, cons
   const int ko = m.k_or;
   const int64_t cap_or = w.cap * (int64_t)ko;
   const double beta = a.beta;
+  const int64_t cap = w.cap;
   auto ev_of = [&](int iu, double &ev) -> bool {
 #pragma clang fp contract(off)
     ev = 0.0;
     for (int p = 0; p < m.n_place; ++p) {
-      double uv = w.or_uv[p * cap_or + slot * ko + iu];
+      // energies are stored [storage slot][orientation][active slot]
+      double uv = w.or_uv[((int64_t)(m.g_start[p] + m.g_b[p]) * ko + iu) * cap + slot];
       if (uv >= kOverlap) return false;
       ev += uv;
     }
@@ -1214,25 +1285,23 @@ __global__ __launch_bounds__(256) void k_widom_or_select(const WidomArgs a, cons
   w.W[i] = Wn;
   const int s2 = atomicAdd(w.n_active2, 1);
   w.active2[s2] = (int32_t)i;
-  const int64_t cap = w.cap;
-  for (int j = 0; j < m.n_sites; ++j) {
-    double x, y, z;
-    if (j == m.start) {
-      x = w.selx[i];
-      y = w.sely[i];
-      z = w.selz[i];
-    } else {
-      int p = 0;
-      for (int q = 0; q < m.n_place; ++q)
-        if (m.place[q] == j) p = q;
-      const int64_t o = p * cap_or + slot * ko + iwalk;
-      x = w.orx[o];
-      y = w.ory[o];
-      z = w.orz[o];
-    }
+  w.cfx[m.start * cap + s2] = w.selx[i];
+  w.cfy[m.start * cap + s2] = w.sely[i];
+  w.cfz[m.start * cap + s2] = w.selz[i];
+  for (int p = 0; p < m.n_place; ++p) {
+    const int j = m.place[p];
+    const int64_t o = m.g_start[p] * cap_or + slot * (m.g_size[p] * ko) + m.g_b[p] * ko + iwalk;
+    const double x = w.orx[o], y = w.ory[o], z = w.orz[o];
     w.cfx[j * cap + s2] = x;
     w.cfy[j * cap + s2] = y;
     w.cfz[j * cap + s2] = z;
+    if (m.h_start[p] >= 0) {
+      // charged placed beads of one type form one site list
+      const int64_t h = m.h_start[p] * cap + (int64_t)s2 * m.h_size[p] + m.h_b[p];
+      w.ccx[h] = x;
+      w.ccy[h] = y;
+      w.ccz[h] = z;
+    }
   }
 }
 
@@ -1248,7 +1317,8 @@ __global__ __launch_bounds__(256) void k_widom_finalize(const WidomArgs a, const
   double ucc = w.ureal[i];
   if (m.n_sites > 1) {
     double ec = 0.0;
-    for (int p = 0; p < m.n_place; ++p) ec += w.cc_place ? w.cc_place[p * w.cap + s2] : 0.0;
+    for (int p = 0; p < m.n_place; ++p)
+      if (m.h_start[p] >= 0) ec += w.cc_place[(int64_t)(m.h_start[p] + m.h_b[p]) * w.cap + s2];
     ucc = ucc + ec;
   }
   double W = w.W[i];
@@ -1679,4 +1749,20 @@ void launch_interp(const InterpArgs &a, const double *xyz, int64_t n, double *ou
   if (nb) hipLaunchKernelGGL(k_interp, dim3(nb), dim3(256), 0, st, a, xyz, n, out);
 }
 
+// ====================================================================
+// plain streaming copy: calibrates hardware byte counters against a
+// kernel whose traffic is known (bytes read = bytes written = size)
+// ====================================================================
+__global__ __launch_bounds__(256) void k_stream_copy(const float4 *__restrict__ src,
+                                                     float4 *__restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+void launch_stream_copy(const void *src, void *dst, int64_t bytes, hipStream_t st) {
+  const int64_t n = bytes / 16;
+  hipLaunchKernelGGL(k_stream_copy, dim3(4096), dim3(256), 0, st, (const float4 *)src, (float4 *)dst, n);
+}
 } // namespace rhg
+

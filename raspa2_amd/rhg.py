@@ -119,6 +119,7 @@ def _load():
     f("rhg_synchronize", _P)
     f("rhg_widom_stats", _P, _P, _I32)
     f("rhg_set_culling", _P, _I32, _D)
+    f("rhg_stream_copy", ctypes.c_int, _I64, _I32)
     f("rhg_synthetic_positions", _U64, _P, _I32, _D, _I64, _P, ctypes.POINTER(_I32))
     return L
 
@@ -136,6 +137,13 @@ def _ptr(a):
 
 def _dptr(a):
     return a.ctypes.data_as(_PD) if a is not None else None
+
+
+def stream_copy(nbytes, reps=3, device=0):
+    """Plain device copy kernel of known traffic (counter calibration)."""
+    rc = lib.rhg_stream_copy(device, nbytes, reps)
+    if rc:
+        raise RhgError(f"rhg_stream_copy: status {rc}", rc)
 
 
 def version():
