@@ -1057,8 +1057,9 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
   const int type0 = mol->type[m.start];
   {
     EnergyArgs ea = energy_args(c, type0, 1.0, 1, 1);
-    if (!ea.do_cc) fill(w.fb_uc, n * m.k_fb, 0.0, st);
-    SiteList s{w.fbx, w.fby, w.fbz, n * m.k_fb, a.n_eff, m.k_fb};
+    if (!ea.do_cc) fill(w.fb_uc, w.cap * m.k_fb, 0.0, st);
+    // trials stored [trial][insertion]
+    SiteList s{w.fbx, w.fby, w.fbz, n * m.k_fb, a.n_eff, m.k_fb, w.cap};
     EnergyOut o{w.fb_uv, ea.do_cc ? w.fb_uc : nullptr, nullptr};
     if (int rc = site_energy(c, ea, s, o, "bfb")) return rc;
   }
@@ -1068,15 +1069,15 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
   }
   if (m.n_sites > 1) {
     // ComputeExternalEnergies VDW of every orientation trial (src/cbmc.c:1210),
-    // one site list per type group of placed beads; energies are written
-    // [storage slot][orientation][active slot] for coalesced selection
+    // one site list per type group of placed beads; positions and energies
+    // are stored [bead][orientation][active slot]
     const int64_t cap_or = w.cap * (int64_t)m.k_or;
     for (int p = 0; p < m.n_place; ++p) {
       if (m.g_b[p] != 0) continue; // first bead of its group
       EnergyArgs ea = energy_args(c, mol->type[m.place[p]], 1.0, 1, 0);
       const int64_t base = m.g_start[p] * cap_or;
       const int rows = m.g_size[p] * m.k_or;
-      SiteList s{w.orx + base, w.ory + base, w.orz + base, n * rows, w.n_active, rows, rows, w.cap};
+      SiteList s{w.orx + base, w.ory + base, w.orz + base, n * rows, w.n_active, rows, w.cap};
       EnergyOut o{w.or_uv + base, nullptr, nullptr};
       if (int rc = site_energy(c, ea, s, o, "bor")) return rc;
     }
@@ -1096,7 +1097,7 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
         fill(w.cc_place + base, (int64_t)rows * w.cap, 0.0, st);
         continue;
       }
-      SiteList s{w.ccx + base, w.ccy + base, w.ccz + base, n * rows, w.n_active2, rows, rows, w.cap};
+      SiteList s{w.ccx + base, w.ccy + base, w.ccz + base, n * rows, w.n_active2, rows, w.cap};
       EnergyOut o{nullptr, w.cc_place + base, nullptr};
       if (int rc = site_energy(c, ea, s, o, "bcc")) return rc;
     }

@@ -12,15 +12,12 @@ struct SiteList {
   int64_t n;               // number of sites (upper bound when n_dev is set)
   const int32_t *n_dev;    // optional device-side count (<= n)
   int32_t count_mult;      // sites = (*n_dev) * count_mult when n_dev is set
-  // outputs of site s go to (s % out_rows) * out_stride + s / out_rows when
-  // out_rows > 0 (a transposed layout for coalesced reads), else to s
-  int32_t out_rows;
-  int64_t out_stride;
+  // row layout: with in_stride > 0 the sites are count_mult rows of *n_dev
+  // sites each, row r starting at r * in_stride (so that a per-insertion
+  // writer stores coalesced); site i of the list is element
+  // (i % count_mult) * in_stride + i / count_mult of x, y, z and of the outputs
+  int64_t in_stride;
 };
-
-__device__ __forceinline__ int64_t out_index(const SiteList &s, int64_t site) {
-  return s.out_rows > 0 ? (site % s.out_rows) * s.out_stride + site / s.out_rows : site;
-}
 
 struct EnergyOut {
   double *u_vdw; // nullable
@@ -129,7 +126,7 @@ void launch_fourier(const FourierArgs &a, double2 *part, hipStream_t st);
 // ---------------------------------------------------------------- Widom
 struct WidomWork {
   int64_t cap;       // insertions per chunk
-  // first-bead sites: site = i*k_fb + t
+  // first-bead trials, stored [trial][insertion] (row stride cap)
   double *fbx, *fby, *fbz, *fb_uv, *fb_uc;
   // per insertion state
   int32_t *stage, *selfb, *selor;
@@ -138,10 +135,11 @@ struct WidomWork {
   double *selx, *sely, *selz; // selected first-bead position
   // compaction after the first-bead stage
   int32_t *active, *n_active;
-  // orientation sites per placed bead p: [p][slot*k_or + iu]
+  // orientation trials of the placed beads, stored
+  // [(storage slot of the bead) * k_or + orientation][active slot]
   double *orx, *ory, *orz, *or_uv;
   // positions of the charged placed beads of the selected orientations,
-  // grouped by pseudo-atom type (one site list per group)
+  // [charged storage slot][slot2], grouped by pseudo-atom type
   double *ccx, *ccy, *ccz;
   // compaction after the orientation stage; selected configuration [j][slot2]
   int32_t *active2, *n_active2;

@@ -75,6 +75,15 @@ __device__ __forceinline__ void mi_fast(const Cell &c, double &dx, double &dy, d
   }
 }
 
+// Site i of a row-layout list: the rows of one insertion are visited
+// together (i = insertion * rows + row), so neighbouring lanes hold nearby
+// positions, which binning aggregates.
+__device__ __forceinline__ int64_t site_phys(const SiteList &s, int64_t i) {
+  if (s.in_stride <= 0) return i;
+  const int64_t rows = s.count_mult;
+  return (i % rows) * s.in_stride + i / rows;
+}
+
 __device__ __forceinline__ int64_t site_count(const SiteList &s) {
   return s.n_dev ? (int64_t)(*s.n_dev) * s.count_mult : s.n;
 }
@@ -115,7 +124,7 @@ __global__ __launch_bounds__(256) void k_site_energy64(
   const int64_t n = site_count(s);
   if ((int64_t)blockIdx.x * blockDim.x >= n) return;
   const bool live = i < n;
-  const int64_t k = live ? i : 0;
+  const int64_t k = site_phys(s, live ? i : 0);
   const double px = s.x[k], py = s.y[k], pz = s.z[k];
   double uv = 0.0, uc = 0.0;
 #pragma unroll 2
@@ -135,7 +144,7 @@ __global__ __launch_bounds__(256) void k_site_energy64(
     }
   }
   if (live) {
-    const int64_t oi = out_index(s, i);
+    const int64_t oi = k;
     if (out_v) out_v[oi] = uv;
     if (out_c) out_c[oi] = uc;
     if (out_o) out_o[oi] = (uint8_t)((VDW && uv >= kOverlap) || (CC && uc >= kOverlap));
@@ -158,7 +167,7 @@ __global__ __launch_bounds__(256) void k_site_energy_mx(
   const int64_t n = site_count(s);
   if ((int64_t)blockIdx.x * blockDim.x >= n) return;
   const bool live = i < n;
-  const int64_t k = live ? i : 0;
+  const int64_t k = site_phys(s, live ? i : 0);
   const double px = s.x[k], py = s.y[k], pz = s.z[k];
   const float alphaf = (float)alpha;
   const float cqaf = (float)cqa;
@@ -197,7 +206,7 @@ __global__ __launch_bounds__(256) void k_site_energy_mx(
     }
   }
   if (live) {
-    const int64_t oi = out_index(s, i);
+    const int64_t oi = k;
     if (out_v) out_v[oi] = uv;
     if (out_c) out_c[oi] = uc;
     if (out_o) out_o[oi] = (uint8_t)((VDW && uv >= kOverlap) || (CC && uc >= kOverlap));
@@ -477,7 +486,8 @@ __global__ __launch_bounds__(256) void k_bin_count(const Cell cell, const CellGr
                                                    const SiteList s, BinWork b) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < site_count(s);
-  const int c = valid ? cell_index(cell, g, s.x[i], s.y[i], s.z[i]) : -1;
+  const int64_t p = valid ? site_phys(s, i) : 0;
+  const int c = valid ? cell_index(cell, g, s.x[p], s.y[p], s.z[p]) : -1;
   const RunInfo r = wave_runs(c, valid);
   if (!valid) return;
   b.cell_of[i] = c;
@@ -493,7 +503,7 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const SiteList s, BinWork b
   const int lane = threadIdx.x & 63;
   if (valid && lane == r.head) base = b.start[c] + atomicAdd(&b.fill[c], r.length);
   base = __shfl(base, r.head);
-  if (valid) b.perm[base + (lane - r.head)] = (int32_t)i;
+  if (valid) b.perm[base + (lane - r.head)] = (int32_t)site_phys(s, i);
 }
 
 // exclusive scans of the per-cell site counts (start) and of the per-cell
@@ -641,7 +651,7 @@ __global__ __launch_bounds__(256) void k_site_energy_culled(
               (unsigned long long)nl * (unsigned long long)(j1 - j0));
   }
   if (live) {
-    const int64_t oi = out_index(s, site);
+    const int64_t oi = site;
     if (out_v) out_v[oi] = uv;
     if (out_c) out_c[oi] = uc;
     if (out_o) out_o[oi] = (uint8_t)((VDW && uv >= kOverlap) || (CC && uc >= kOverlap));
@@ -750,7 +760,7 @@ __global__ __launch_bounds__(256) void k_site_energy_img(
               (unsigned long long)nl * (unsigned long long)(q1 - q0));
   }
   if (live) {
-    const int64_t oi = out_index(s, site);
+    const int64_t oi = site;
     if (out_v) out_v[oi] = uv;
     if (out_c) out_c[oi] = uc;
     if (out_o) out_o[oi] = (uint8_t)((VDW && uv >= kOverlap) || (CC && uc >= kOverlap));
@@ -1076,9 +1086,9 @@ __global__ __launch_bounds__(256) void k_widom_fb_gen(const WidomArgs a, const W
   const int64_t first = chunk_first(a);
   if (a.use_host_trials) {
     for (int t = 0; t < k; ++t) {
-      w.fbx[i * k + t] = w.h_fb[(i * k + t) * 3 + 0];
-      w.fby[i * k + t] = w.h_fb[(i * k + t) * 3 + 1];
-      w.fbz[i * k + t] = w.h_fb[(i * k + t) * 3 + 2];
+      w.fbx[t * w.cap + i] = w.h_fb[(i * k + t) * 3 + 0];
+      w.fby[t * w.cap + i] = w.h_fb[(i * k + t) * 3 + 1];
+      w.fbz[t * w.cap + i] = w.h_fb[(i * k + t) * 3 + 2];
     }
     return;
   }
@@ -1090,9 +1100,9 @@ __global__ __launch_bounds__(256) void k_widom_fb_gen(const WidomArgs a, const W
     double sx = stream_uniform(key, c++);
     double x, y, z;
     abc_to_xyz(a.cell, sx, sy, sz, x, y, z);
-    w.fbx[i * k + t] = x;
-    w.fby[i * k + t] = y;
-    w.fbz[i * k + t] = z;
+    w.fbx[t * w.cap + i] = x;
+    w.fby[t * w.cap + i] = y;
+    w.fbz[t * w.cap + i] = z;
   }
 }
 
@@ -1108,7 +1118,7 @@ __global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, cons
   bool any = false;
   double W1 = 0.0;
   for (int t = 0; t < k; ++t) {
-    double uv = w.fb_uv[i * k + t], uc = w.fb_uc[i * k + t];
+    double uv = w.fb_uv[t * w.cap + i], uc = w.fb_uc[t * w.cap + i];
     if (uv >= kOverlap || uc >= kOverlap)
       mask |= (1ULL << t);
     else {
@@ -1141,7 +1151,7 @@ __global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, cons
     usel = stream_uniform(key, (uint64_t)(3 * k));
   }
   auto bf = [&](int t, double &b) -> bool {
-    double uv = w.fb_uv[i * k + t], uc = w.fb_uc[i * k + t];
+    double uv = w.fb_uv[t * w.cap + i], uc = w.fb_uc[t * w.cap + i];
     if (uv >= kOverlap || uc >= kOverlap) return false;
     b = -beta * (uv + 0.0 + 0.0);
     return true;
@@ -1154,12 +1164,12 @@ __global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, cons
     w.stage[i] = 1;
     return;
   }
-  const double fx = w.fbx[i * k + sel], fy = w.fby[i * k + sel], fz = w.fbz[i * k + sel];
+  const double fx = w.fbx[sel * w.cap + i], fy = w.fby[sel * w.cap + i], fz = w.fbz[sel * w.cap + i];
   w.selx[i] = fx;
   w.sely[i] = fy;
   w.selz[i] = fz;
-  w.uvdw[i] = w.fb_uv[i * k + sel];
-  w.ureal[i] = w.fb_uc[i * k + sel];
+  w.uvdw[i] = w.fb_uv[sel * w.cap + i];
+  w.ureal[i] = w.fb_uc[sel * w.cap + i];
   w.W[i] = W1;
   w.stage[i] = 4;
   const MolDev &m = a.mol;
@@ -1174,7 +1184,6 @@ __global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, cons
   const int slot = atomicAdd(w.n_active, 1);
   w.active[slot] = (int32_t)i;
   const int ko = m.k_or;
-  const int64_t cap_or = w.cap * (int64_t)ko;
   uint64_t c = (uint64_t)(3 * k + 1);
   for (int iu = 0; iu < ko; ++iu) {
     Rot R;
@@ -1212,7 +1221,7 @@ __global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, cons
         y = fy + cj[1];
         z = fz + cj[2];
       }
-      const int64_t o = m.g_start[p] * cap_or + (int64_t)slot * (m.g_size[p] * ko) + m.g_b[p] * ko + iu;
+      const int64_t o = ((int64_t)(m.g_start[p] + m.g_b[p]) * ko + iu) * w.cap + slot;
       w.orx[o] = x;
       w.ory[o] = y;
       w.orz[o] = z;
@@ -1228,7 +1237,6 @@ __global__ __launch_bounds__(256) void k_widom_or_select(const WidomArgs a, cons
   const int64_t i = w.active[slot];
   const MolDev &m = a.mol;
   const int ko = m.k_or;
-  const int64_t cap_or = w.cap * (int64_t)ko;
   const double beta = a.beta;
   const int64_t cap = w.cap;
   auto ev_of = [&](int iu, double &ev) -> bool {
@@ -1290,14 +1298,14 @@ __global__ __launch_bounds__(256) void k_widom_or_select(const WidomArgs a, cons
   w.cfz[m.start * cap + s2] = w.selz[i];
   for (int p = 0; p < m.n_place; ++p) {
     const int j = m.place[p];
-    const int64_t o = m.g_start[p] * cap_or + slot * (m.g_size[p] * ko) + m.g_b[p] * ko + iwalk;
+    const int64_t o = ((int64_t)(m.g_start[p] + m.g_b[p]) * ko + iwalk) * cap + slot;
     const double x = w.orx[o], y = w.ory[o], z = w.orz[o];
     w.cfx[j * cap + s2] = x;
     w.cfy[j * cap + s2] = y;
     w.cfz[j * cap + s2] = z;
     if (m.h_start[p] >= 0) {
-      // charged placed beads of one type form one site list
-      const int64_t h = m.h_start[p] * cap + (int64_t)s2 * m.h_size[p] + m.h_b[p];
+      // charged placed beads of one type form one site list, rows of beads
+      const int64_t h = (int64_t)(m.h_start[p] + m.h_b[p]) * cap + s2;
       w.ccx[h] = x;
       w.ccy[h] = y;
       w.ccz[h] = z;

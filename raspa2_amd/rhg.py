@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "librhg.so")
+LIB_PATH = os.environ.get("RHG_LIBRARY", os.path.join(_HERE, "librhg.so"))
 
 BC_RECTANGULAR = 1
 BC_TRICLINIC = 2
