@@ -930,10 +930,12 @@ __global__ __launch_bounds__(128) void k_fourier_part(const FourierArgs a, doubl
       const int kzm = e.row_kz[row];
       if (kzm < 0) continue;
       const int off = e.row_off[row];
+      // charge-weighted row phases q e^{i(kx x + ky y)} of each site
       cx xy[NC], ez[NC];
 #pragma unroll
       for (int t = 0; t < nc; ++t) {
-        xy[t] = cmul(ex[t], sy ? conjc(ey[t]) : ey[t]);
+        const cx r = cmul(ex[t], sy ? conjc(ey[t]) : ey[t]);
+        xy[t] = cx{qs[t] * r.re, qs[t] * r.im};
         ez[t] = cx{1.0, 0.0};
       }
       const bool origin_row = (ii == 0 && jv == 0);
@@ -942,6 +944,16 @@ __global__ __launch_bounds__(128) void k_fourier_part(const FourierArgs a, doubl
 #pragma unroll
           for (int t = 0; t < nc; ++t) ez[t] = (kk == 1) ? e1z[t] : cmul(ez[t], e1z[t]);
         }
+        // the +kz and -kz structure factors share the four products
+        // Re(xy) Re(ez), Im(xy) Im(ez), Re(xy) Im(ez), Im(xy) Re(ez)
+        double a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
+#pragma unroll
+        for (int t = 0; t < nc; ++t) {
+          a1 = fma(xy[t].re, ez[t].re, a1);
+          a2 = fma(xy[t].im, ez[t].im, a2);
+          a3 = fma(xy[t].re, ez[t].im, a3);
+          a4 = fma(xy[t].im, ez[t].re, a4);
+        }
         for (int sz = 0; sz < 2; ++sz) {
           if (kk == 0 && sz == 1) break;
           if (origin_row && kk == 0) continue;
@@ -949,13 +961,8 @@ __global__ __launch_bounds__(128) void k_fourier_part(const FourierArgs a, doubl
           int pos = kv + kzm;
           if (origin_row && kv > 0) pos -= 1;
           const int idx = off + pos;
-          double sre = 0.0, sim = 0.0;
-#pragma unroll
-          for (int t = 0; t < nc; ++t) {
-            cx ek = cmul(xy[t], sz ? conjc(ez[t]) : ez[t]);
-            sre += qs[t] * ek.re;
-            sim += qs[t] * ek.im;
-          }
+          const double sre = sz ? a1 + a2 : a1 - a2;
+          const double sim = sz ? a4 - a3 : a3 + a4;
           const double kf = e.kfactor[idx];
           const double2 sf = e.sf[idx];
           host += kf * (sf.x * sre + sf.y * sim);
