@@ -103,6 +103,96 @@ def cpu_baseline_screening(jobs, count):
     return dict(aggregate=total / wall, single=total / sum(times), cores=cores, sample=total)
 
 
+def _grid_worker(args):
+    arrays, geom, unit_box, inv, pts = args
+    import oracle as O
+    box, bc, x, y, z, q, eps, sig, mass = arrays
+    s = O.System(box, bc, x, y, z, q, eps, sig, mass=mass)
+    t = s.add_pseudo_atom(148.0, 3.73, 0.0)
+    t0 = time.perf_counter()
+    out = [s.interpolate_direct(t, geom, O.BC_TRICLINIC, unit_box, inv, p) for p in pts]
+    return out, time.perf_counter() - t0
+
+
+def grid_bench(args):
+    """C2: energy grid of a methane-like probe over the triclinic 26 A, 80 deg
+    cell (1200 atoms) at 0.1 A, plain FP64 and mixed builds, checked against
+    the oracle's FP64 grid interpolation at seeded off-grid points."""
+    import multiprocessing as mp
+    import numpy as np
+    import oracle as O
+    from raspa2_amd import workloads as W
+    fw = W.c2_framework()
+    spacing = 0.1
+    geom = O.grid_geometry(fw.unit_box, spacing)
+    inv, _ = O.invert3x3(fw.unit_box)
+    rng = np.random.Generator(np.random.PCG64(SEED))
+    frac = rng.uniform(0.0, 1.0, (args.grid_samples, 3))
+    pts = frac @ np.asarray(fw.unit_box).reshape(3, 3)
+    # the oracle's FP64 grid at the sample points (8 corner nodes each), on the
+    # host cores, before this process touches the GPU
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    n_cpu = len(pts) if not args.no_cpu else min(len(pts), 2000)
+    parts = np.array_split(np.arange(n_cpu), cores)
+    ctxp = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctxp.Pool(cores) as pool:
+        res = pool.map(_grid_worker, [(fw.oracle_args(), geom, fw.unit_box, inv, pts[ix])
+                                      for ix in parts])
+    cpu_wall = time.perf_counter() - t0
+    u_ref = np.concatenate([np.asarray(r[0]) for r in res])
+    cpu_nodes = 8 * n_cpu
+    from raspa2_amd import rhg as R
+    out = {}
+    data = {}
+    for name, prec in (("fp64", R.PREC_FP64), ("mixed", R.PREC_MIXED)):
+        ctx = fw.context(precision=prec)
+        t = ctx.add_pseudo_atom(148.0, 3.73, 0.0)
+        g = ctx.MakeGrid(t, fw.unit_box, 0.5)  # warm-up: cell and image lists, code objects
+        g.close()
+        ctx.synchronize()
+        ctx.profile(True)
+        ctx.profile_reset()
+        t0 = time.perf_counter()
+        g = ctx.MakeGrid(t, fw.unit_box, spacing)
+        ctx.synchronize()
+        wall = time.perf_counter() - t0
+        _, ms = ctx.profile_get("grid_build")
+        ctx.profile(False)
+        nodes = int(np.prod(np.asarray(g.geom[:3]) + 1))
+        u = g.interpolate(pts)
+        ok = np.abs(u_ref) < 1e7
+        e2k = O.units()["energy_to_kelvin"]
+        d = np.abs(u[:n_cpu][ok] - u_ref[ok]) * e2k
+        rel = d / np.maximum(np.abs(u_ref[ok]) * e2k, 1.0)
+        out[name] = {"nodes_per_s": nodes / wall, "wall_s": wall, "kernel_ms": ms,
+                     "max_abs_err_K": float(d.max()), "max_rel_err": float(rel.max()),
+                     "samples_checked": int(ok.sum())}
+        data[name] = (nodes,)
+        g.close()
+        ctx.close()
+    mixed = out["mixed"]
+    line = {
+        "metric": "energy-grid nodes/s (MakeGrid with derivatives)", "value": round(mixed["nodes_per_s"], 1),
+        "unit": "grid nodes/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(mixed["wall_s"] * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32 pair terms, fp64 accumulation", "data": "synthetic",
+        "config": {"workload": "C2 energy grid: methane-like LJ probe, triclinic 26 A 80 deg cell, "
+                               "1200 atoms, 0.1 A spacing, 8 channels (value, 3 first, 3 mixed "
+                               "second, 1 third derivative), FP32 storage",
+                   "nodes": data["mixed"][0], "spacing_A": spacing, "parallelism": "single GPU"},
+        "plain_fp64": out["fp64"], "mixed": mixed,
+        "check": f"{n_cpu} seeded off-grid points: device interpolation of each device grid vs "
+                 "the oracle's FP64 grid nodes and Lekien-Marsden interpolation",
+        "roofline": None,
+        "cpu_baseline": {"value": round(cpu_nodes / cpu_wall, 1), "unit": "grid nodes/s",
+                         "cores": cores, "kind": "port",
+                         "sample": f"{cpu_nodes} nodes (the 8 corners of {n_cpu} sample points), "
+                                   "oracle FP64 all-pairs, one process per core"},
+    }
+    print(json.dumps(line))
+
+
 def screening_bench(args):
     """C3 (batched screening) and C4 (Poisson-arriving requests) through the
     screening service; prints one JSON line."""
@@ -183,7 +273,8 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--cpu-sample", type=int, default=400, help="insertions per CPU process")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["c1", "c3", "c4"], default="c1")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4"], default="c1")
+    ap.add_argument("--grid-samples", type=int, default=100_000, help="c2 off-grid checks")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 512 MiB device copy kernel (k_stream_copy) three "
                          "times, to scale profiler byte counters")
@@ -197,6 +288,9 @@ def main():
               "DESIGN.md", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "c2":
+        grid_bench(args)
+        return
     if args.workload != "c1":
         screening_bench(args)
         return

@@ -99,6 +99,7 @@ def _declare(L):
     f("orc_grid_node", None, _P, _I, _P, _I, _I, _I, _P)
     f("orc_lm_coeff", None, _P)
     f("orc_interpolate_grid", _D, _P, _P, _I, _P, _P, _P)
+    f("orc_interpolate_direct", _D, _P, _I, _P, _I, _P, _P, _P)
 
 
 def _ptr(a):
@@ -237,6 +238,11 @@ class System:
         out = np.zeros(14)
         lib().orc_derivatives_vdw_at(self.h, t, _ptr(p), _ptr(out))
         return out
+
+    def interpolate_direct(self, t, geom, bc, unit_box, unit_inv, pos):
+        """InterpolateVDWGrid at pos with the corner nodes computed on the fly."""
+        geom, ub, ui, p = _f64(geom), _f64(unit_box), _f64(unit_inv), _f64(pos)
+        return lib().orc_interpolate_direct(self.h, t, _ptr(geom), bc, _ptr(ub), _ptr(ui), _ptr(p))
 
     def grid_node(self, t, geom, i, j, k):
         g = _f64(geom)

@@ -1364,14 +1364,9 @@ void orc_lm_coeff(int out[64 * 64]) {
     for (int c = 0; c < 64; c++) out[r * 64 + c] = (int)lround(B[r][64 + c]);
 }
 
-double orc_interpolate_grid(const float *grid, const double g[12], int bc, const double ub[9],
-                            const double uinv[9], const double pos_in[3]) {
-  static int Coeff[64 * 64];
-  static int have = 0;
-  if (!have) {
-    orc_lm_coeff(Coeff);
-    have = 1;
-  }
+/* Position -> grid cell and fractional offsets (src/grids.c:985-1012). */
+static void grid_locate(const double g[12], int bc, const double ub[9], const double uinv[9],
+                        const double pos_in[3], int x0[3], double r[3]) {
   double pos[3] = {pos_in[0], pos_in[1], pos_in[2]};
   double sv[3];
   int N[3] = {(int)g[0], (int)g[1], (int)g[2]};
@@ -1396,23 +1391,22 @@ double orc_interpolate_grid(const float *grid, const double g[12], int bc, const
   }
   double s3[3];
   for (int d = 0; d < 3; d++) s3[d] = ((pos[d] - g[6 + d]) / g[3 + d]) * (double)(N[d]);
-  int x0 = (int)s3[0], y0 = (int)s3[1], z0 = (int)s3[2];
-  int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
-  double r[3] = {s3[0] - x0, s3[1] - y0, s3[2] - z0};
-  int ny = N[1] + 1, nz = N[2] + 1;
-#define NODE(a, b, c) (grid + 8 * (((size_t)(a) * ny + (b)) * nz + (c)))
-  double X[64], a[64];
-  for (int i = 0; i < 8; i++) {
-    X[0 + i * 8] = NODE(x0, y0, z0)[i];
-    X[1 + i * 8] = NODE(x1, y0, z0)[i];
-    X[2 + i * 8] = NODE(x0, y1, z0)[i];
-    X[3 + i * 8] = NODE(x1, y1, z0)[i];
-    X[4 + i * 8] = NODE(x0, y0, z1)[i];
-    X[5 + i * 8] = NODE(x1, y0, z1)[i];
-    X[6 + i * 8] = NODE(x0, y1, z1)[i];
-    X[7 + i * 8] = NODE(x1, y1, z1)[i];
+  for (int d = 0; d < 3; d++) {
+    x0[d] = (int)s3[d];
+    r[d] = s3[d] - x0[d];
   }
-#undef NODE
+}
+
+/* Tricubic value from the 8 channels of the 8 corners, X[corner + 8 *
+ * channel], with the Lekien-Marsden matrix (src/grids.c:1014-1077). */
+static double grid_tricubic(const double X[64], const double r[3]) {
+  static int Coeff[64 * 64];
+  static int have = 0;
+  if (!have) {
+    orc_lm_coeff(Coeff);
+    have = 1;
+  }
+  double a[64];
   for (int i = 0; i < 8; i++)
     if (X[i] > 1e10) return 1e8;
   for (int i = 0; i < 64; i++) {
@@ -1424,4 +1418,33 @@ double orc_interpolate_grid(const float *grid, const double g[12], int bc, const
     for (int j = 0; j < 4; j++)
       for (int k = 0; k < 4; k++) value += a[i + 4 * j + 16 * k] * pow(r[0], i) * pow(r[1], j) * pow(r[2], k);
   return value;
+}
+
+double orc_interpolate_grid(const float *grid, const double g[12], int bc, const double ub[9],
+                            const double uinv[9], const double pos_in[3]) {
+  int x[3];
+  double r[3];
+  grid_locate(g, bc, ub, uinv, pos_in, x, r);
+  int ny = (int)g[1] + 1, nz = (int)g[2] + 1;
+  double X[64];
+  for (int c = 0; c < 8; c++) {
+    const float *node = grid + 8 * (((size_t)(x[0] + (c & 1)) * ny + (x[1] + ((c >> 1) & 1))) * nz +
+                                    (x[2] + ((c >> 2) & 1)));
+    for (int i = 0; i < 8; i++) X[c + i * 8] = node[i];
+  }
+  return grid_tricubic(X, r);
+}
+
+double orc_interpolate_direct(const orc_system *s, int type, const double g[12], int bc,
+                              const double ub[9], const double uinv[9], const double pos_in[3]) {
+  int x[3];
+  double r[3];
+  grid_locate(g, bc, ub, uinv, pos_in, x, r);
+  double X[64];
+  for (int c = 0; c < 8; c++) {
+    float node[8];
+    orc_grid_node(s, type, g, x[0] + (c & 1), x[1] + ((c >> 1) & 1), x[2] + ((c >> 2) & 1), node);
+    for (int i = 0; i < 8; i++) X[c + i * 8] = node[i];
+  }
+  return grid_tricubic(X, r);
 }

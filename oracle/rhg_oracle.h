@@ -193,6 +193,11 @@ void orc_lm_coeff(int out[64 * 64]);
 double orc_interpolate_grid(const float *grid, const double geom[12], int bc,
                             const double unit_box[9], const double unit_inv[9],
                             const double pos[3]);
+/* The same interpolation with the 8 corner nodes computed on the fly
+ * (orc_grid_node), for grids too large to hold on the host */
+double orc_interpolate_direct(const orc_system *s, int type, const double geom[12], int bc,
+                              const double unit_box[9], const double unit_inv[9],
+                              const double pos[3]);
 
 #ifdef __cplusplus
 }

@@ -1559,11 +1559,43 @@ int rhg_make_grid(rhg_ctx *c, int32_t type, const double unit_box[9], double spa
   a.fp64 = g->fp64;
   a.out = g->d;
   const int64_t step = (int64_t)1 << 22;
+  const bool culled = c->cull && c->n > 0;
+  if (culled) {
+    if (int rc = ensure_cell_grid(c)) return rc;
+    if (c->precision != RHG_PREC_FP64 && c->use_images)
+      if (int rc = ensure_image_list(c)) return rc;
+  }
+  const int prec = (c->precision != RHG_PREC_FP64 && c->use_images && c->cg.img_off) ? 1 : 0;
   for (int64_t n0 = 0; n0 < nodes; n0 += step) {
     a.node0 = n0;
     a.n_nodes = std::min(step, nodes - n0);
+    if (!culled) {
+      Timed tm(c, "grid_build");
+      launch_grid_build(a, c->stream);
+      continue;
+    }
+    // nodes are binned into the cell grid like trial sites, then each tile
+    // walks its cell's lists
+    const size_t nb = (size_t)step * sizeof(double);
+    double *gx = (double *)work(c, "g_x", nb), *gy = (double *)work(c, "g_y", nb),
+           *gz = (double *)work(c, "g_z", nb);
+    if (!gx || !gy || !gz) return fail(c, RHG_ERR_OOM, "grid node buffers");
+    launch_grid_nodes(a, gx, gy, gz, c->stream);
+    SiteList sl{gx, gy, gz, a.n_nodes, nullptr, 1, 0};
+    BinWork b;
+    const int ncell = c->cg.ncell;
+    b.cell_of = (int32_t *)work(c, "g_cellof", (size_t)step * sizeof(int32_t));
+    b.perm = (int32_t *)work(c, "g_perm", (size_t)step * sizeof(int32_t));
+    b.count = (int32_t *)work(c, "g_count", (size_t)ncell * sizeof(int32_t));
+    b.fill = (int32_t *)work(c, "g_fill", (size_t)ncell * sizeof(int32_t));
+    b.start = (int32_t *)work(c, "g_start", (size_t)(ncell + 1) * sizeof(int32_t));
+    b.tile_off = (int32_t *)work(c, "g_tiles", (size_t)(ncell + 1) * sizeof(int32_t));
+    b.pairs = nullptr;
+    if (!b.cell_of || !b.perm || !b.count || !b.fill || !b.start || !b.tile_off)
+      return fail(c, RHG_ERR_OOM, "grid binning scratch");
+    launch_bin_sites(c->cell, c->cg, sl, b, c->stream);
     Timed tm(c, "grid_build");
-    launch_grid_build(a, c->stream);
+    launch_grid_culled(a, c->cg, sl, b, t.par, prec, c->stream);
   }
   HIPC(c, hipGetLastError());
   HIPC(c, hipStreamSynchronize(c->stream));

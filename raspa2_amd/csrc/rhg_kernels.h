@@ -222,6 +222,13 @@ struct GridArgs {
   int64_t node0, n_nodes; // node range of this launch
 };
 void launch_grid_build(const GridArgs &g, hipStream_t st);
+// node positions of nodes [node0, node0 + n_nodes) into x, y, z
+void launch_grid_nodes(const GridArgs &g, double *x, double *y, double *z, hipStream_t st);
+// culled build of the same nodes, binned as site list s (site = node - node0):
+// precision 0 walks the neighbour lists in atom order (bit-identical to
+// launch_grid_build), 1 the fixed-point image lists with FP32 terms
+void launch_grid_culled(const GridArgs &g, const CellGrid &cg, const SiteList &s, const BinWork &b,
+                        const float4 *par, int precision, hipStream_t st);
 void launch_derivatives(const GridArgs &g, const double *xyz, int64_t n, double *out,
                         hipStream_t st);
 

@@ -1586,23 +1586,13 @@ __device__ __forceinline__ void derivs_at(const GridArgs &g, double px, double p
   o[13] = third;
 }
 
-template <int BCK>
-__global__ __launch_bounds__(256) void k_grid_build(const GridArgs g) {
+// MakeGrid's node record (src/grids.c:520-565): the cap above the overlap
+// energy, then the 8 channels scaled to the unit cube, FP32 (RASPA2's
+// layout) or FP64
+__device__ __forceinline__ void store_node(const GridArgs &g, int64_t node, double value, double f0,
+                                           double f1, double f2, double hay, double haz, double hbz,
+                                           double third) {
 #pragma clang fp contract(off)
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= g.n_nodes) return;
-  const int64_t node = g.node0 + t;
-  const int64_t nz = g.n[2] + 1, ny = g.n[1] + 1;
-  const int k = (int)(node % nz);
-  const int j = (int)((node / nz) % ny);
-  const int i = (int)(node / (nz * ny));
-  double px = i * g.size[0] / g.n[0] + g.shift[0];
-  double py = j * g.size[1] / g.n[1] + g.shift[1];
-  double pz = k * g.size[2] / g.n[2] + g.shift[2];
-  double d[14];
-  derivs_at<BCK>(g, px, py, pz, d);
-  double value = d[0], f0 = d[1], f1 = d[2], f2 = d[3], hay = d[5], haz = d[6], hbz = d[9],
-         third = d[13];
   if (value > kOverlap) {
     value = 2.0 * kOverlap;
     f0 = fmin(fmax(f0, -kOverlap), kOverlap);
@@ -1612,15 +1602,6 @@ __global__ __launch_bounds__(256) void k_grid_build(const GridArgs g) {
     third = 0.0;
   }
   const double dx = g.delta[0], dy = g.delta[1], dz = g.delta[2];
-  float c[8];
-  c[0] = (float)value;
-  c[1] = (float)((double)(float)f0 * dx);
-  c[2] = (float)((double)(float)f1 * dy);
-  c[3] = (float)((double)(float)f2 * dz);
-  c[4] = (float)((double)(float)hay * (dx * dy));
-  c[5] = (float)((double)(float)haz * (dx * dz));
-  c[6] = (float)((double)(float)hbz * (dy * dz));
-  c[7] = (float)((double)(float)third * (dx * dy * dz));
   if (g.fp64) {
     double *o = (double *)g.out + node * 8;
     o[0] = value;
@@ -1632,10 +1613,44 @@ __global__ __launch_bounds__(256) void k_grid_build(const GridArgs g) {
     o[6] = hbz * (dy * dz);
     o[7] = third * (dx * dy * dz);
   } else {
+    float c[8];
+    c[0] = (float)value;
+    c[1] = (float)((double)(float)f0 * dx);
+    c[2] = (float)((double)(float)f1 * dy);
+    c[3] = (float)((double)(float)f2 * dz);
+    c[4] = (float)((double)(float)hay * (dx * dy));
+    c[5] = (float)((double)(float)haz * (dx * dz));
+    c[6] = (float)((double)(float)hbz * (dy * dz));
+    c[7] = (float)((double)(float)third * (dx * dy * dz));
     float4 *o = (float4 *)((float *)g.out + node * 8);
     o[0] = make_float4(c[0], c[1], c[2], c[3]);
     o[1] = make_float4(c[4], c[5], c[6], c[7]);
   }
+}
+
+__device__ __forceinline__ void node_position(const GridArgs &g, int64_t node, double &px, double &py,
+                                              double &pz) {
+#pragma clang fp contract(off)
+  const int64_t nz = g.n[2] + 1, ny = g.n[1] + 1;
+  const int k = (int)(node % nz);
+  const int j = (int)((node / nz) % ny);
+  const int i = (int)(node / (nz * ny));
+  px = i * g.size[0] / g.n[0] + g.shift[0];
+  py = j * g.size[1] / g.n[1] + g.shift[1];
+  pz = k * g.size[2] / g.n[2] + g.shift[2];
+}
+
+template <int BCK>
+__global__ __launch_bounds__(256) void k_grid_build(const GridArgs g) {
+#pragma clang fp contract(off)
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.n_nodes) return;
+  const int64_t node = g.node0 + t;
+  double px, py, pz;
+  node_position(g, node, px, py, pz);
+  double d[14];
+  derivs_at<BCK>(g, px, py, pz, d);
+  store_node(g, node, d[0], d[1], d[2], d[3], d[5], d[6], d[9], d[13]);
 }
 
 template <int BCK>
@@ -1648,6 +1663,153 @@ __global__ __launch_bounds__(256) void k_derivs(const GridArgs g, const double *
   derivs_at<BCK>(g, xyz[3 * m], xyz[3 * m + 1], xyz[3 * m + 2], d);
   if (t < n)
     for (int q = 0; q < 14; ++q) out[t * 14 + q] = d[q];
+}
+
+__global__ __launch_bounds__(256) void k_grid_nodes(const GridArgs g, double *__restrict__ x,
+                                                    double *__restrict__ y, double *__restrict__ z) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.n_nodes) return;
+  double px, py, pz;
+  node_position(g, g.node0 + t, px, py, pz);
+  x[t] = px;
+  y[t] = py;
+  z[t] = pz;
+}
+
+void launch_grid_nodes(const GridArgs &g, double *x, double *y, double *z, hipStream_t st) {
+  const unsigned nb = nblk(g.n_nodes, 256);
+  if (nb) hipLaunchKernelGGL(k_grid_nodes, dim3(nb), dim3(256), 0, st, g, x, y, z);
+}
+
+// derivative channels of the grid (value, dU/dx, dU/dy, dU/dz, d2U/dxdy,
+// d2U/dxdz, d2U/dydz, d3U/dxdydz) over one cell's neighbour list, in atom
+// order, FP64 with the reference's operations (as derivs_at)
+template <int BCK>
+__device__ __forceinline__ void node_channels_exact(const GridArgs &g, const CellGrid &cg, int c,
+                                                    double px, double py, double pz, double o[8]) {
+#pragma clang fp contract(off)
+  double value = 0, fx = 0, fy = 0, fz = 0, hay = 0, haz = 0, hbz = 0, third = 0;
+  const int j0 = cg.nb_off[c], j1 = cg.nb_off[c + 1];
+  for (int q = j0; q < j1; ++q) {
+    const int j = cg.nb_idx[q];
+    const AtomRec64 r = g.a64[j];
+    double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
+    mi_exact<BCK>(g.cell, dx, dy, dz);
+    double rr = dx * dx + dy * dy + dz * dz;
+    if (rr < g.rc2) {
+      double t = r.s2 / rr;
+      double rri3 = (t * t) * t;
+      double F = r.c4 * (rri3 * (rri3 - 1.0)) - r.shift;
+      double DF = g.c24[j] * (rri3 * (1.0 - 2.0 * rri3)) / rr;
+      double DDF = g.c96[j] * (rri3 * (7.0 * rri3 - 2.0)) / (rr * rr);
+      double DDDF = g.c384[j] * (rri3 * (5.0 - 28.0 * rri3)) / ((rr * rr) * (rr * rr));
+      value += F;
+      fx += dx * DF;
+      fy += dy * DF;
+      fz += dz * DF;
+      hay += DDF * dx * dy;
+      haz += DDF * dx * dz;
+      hbz += DDF * dy * dz;
+      third += DDDF * dx * dy * dz;
+    }
+  }
+  o[0] = value;
+  o[1] = fx;
+  o[2] = fy;
+  o[3] = fz;
+  o[4] = hay;
+  o[5] = haz;
+  o[6] = hbz;
+  o[7] = third;
+}
+
+template <int PREC, int BCK>
+__global__ __launch_bounds__(256) void k_grid_culled(const GridArgs g, const CellGrid cg,
+                                                     const SiteList s, const BinWork b,
+                                                     const float4 *__restrict__ par) {
+  const int lane = threadIdx.x & 63;
+  const int tile = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  const int ntiles = b.tile_off[cg.ncell];
+  if (tile >= ntiles) return;
+  int lo = 0, hi = cg.ncell - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (b.tile_off[mid] <= tile) lo = mid; else hi = mid - 1;
+  }
+  const int c = __builtin_amdgcn_readfirstlane(lo);
+  const int k = (tile - b.tile_off[c]) * 64 + lane;
+  const bool live = k < b.count[c];
+  const int site = live ? b.perm[b.start[c] + k] : b.perm[b.start[c]];
+  const double px = s.x[site], py = s.y[site], pz = s.z[site];
+  double o[8];
+  if (PREC == 0) {
+    node_channels_exact<BCK>(g, cg, c, px, py, pz, o);
+  } else {
+    double cx, cy, cz;
+    cell_centre(g.cell, cg, c, cx, cy, cz);
+    double ox = px - cx, oy = py - cy, oz = pz - cz;
+    mi_exact<BCK>(g.cell, ox, oy, oz);
+    const int ix = (int)rint(ox * kFix), iy = (int)rint(oy * kFix), iz = (int)rint(oz * kFix);
+    const float rc2f = (float)g.rc2, unfix = (float)(1.0 / kFix), s2unfix = (float)(1.0 / (kFix * kFix));
+    for (int q = 0; q < 8; ++q) o[q] = 0.0;
+    const int q0 = cg.img_off[c], q1 = cg.img_off[c + 1];
+    for (int q = q0; q < q1; q += 4) {
+      float a[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (q + u < q1) {
+          const int4 e = cg.img[q + u];
+          const float4 pr = par[e.w];
+          const float dx = (float)(ix - e.x) * unfix, dy = (float)(iy - e.y) * unfix,
+                      dz = (float)(iz - e.z) * unfix;
+          const float rr = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+          if (rr < rc2f) {
+            const float ir = __builtin_amdgcn_rcpf(rr);
+            const float t = (pr.y * s2unfix) * ir;
+            const float r3 = (t * t) * t;
+            const float F = fmaf(pr.x, fmaf(r3, r3, -r3), -pr.z);
+            // 24 eps = 6 (4 eps), 96 eps = 24 (4 eps), 384 eps = 96 (4 eps)
+            const float DF = 6.0f * pr.x * (r3 * fmaf(-2.0f, r3, 1.0f)) * ir;
+            const float ir2 = ir * ir;
+            const float DDF = 24.0f * pr.x * (r3 * fmaf(7.0f, r3, -2.0f)) * ir2;
+            const float DDDF = 96.0f * pr.x * (r3 * fmaf(-28.0f, r3, 5.0f)) * (ir2 * ir2);
+            const float ddx = DDF * dx;
+            a[0] += F;
+            a[1] = fmaf(dx, DF, a[1]);
+            a[2] = fmaf(dy, DF, a[2]);
+            a[3] = fmaf(dz, DF, a[3]);
+            a[4] = fmaf(ddx, dy, a[4]);
+            a[5] = fmaf(ddx, dz, a[5]);
+            a[6] = fmaf(DDF * dy, dz, a[6]);
+            a[7] = fmaf(DDDF * dx * dy, dz, a[7]);
+          }
+        }
+      }
+#pragma unroll
+      for (int ch = 0; ch < 8; ++ch) o[ch] += (double)a[ch];
+    }
+    // nodes within the guard band of the cap are recomputed exactly
+    const bool near = live && fabs(o[0] - kOverlap) <= 1.0e-4 * kOverlap;
+    if (__any(near))
+      if (near) node_channels_exact<BCK>(g, cg, c, px, py, pz, o);
+  }
+  if (live) store_node(g, g.node0 + site, o[0], o[1], o[2], o[3], o[4], o[5], o[6], o[7]);
+}
+
+void launch_grid_culled(const GridArgs &g, const CellGrid &cg, const SiteList &s, const BinWork &b,
+                        const float4 *par, int precision, hipStream_t st) {
+  const int64_t max_tiles = s.n / 64 + cg.ncell + 1;
+  const unsigned nb = (unsigned)((max_tiles + 3) / 4);
+  const int bck = g.cell.bc == 1 ? 1 : (g.cell.upper ? 2 : 3);
+  if (precision == 0) {
+    if (bck == 1) hipLaunchKernelGGL((k_grid_culled<0, 1>), dim3(nb), dim3(256), 0, st, g, cg, s, b, par);
+    else if (bck == 2) hipLaunchKernelGGL((k_grid_culled<0, 2>), dim3(nb), dim3(256), 0, st, g, cg, s, b, par);
+    else hipLaunchKernelGGL((k_grid_culled<0, 3>), dim3(nb), dim3(256), 0, st, g, cg, s, b, par);
+  } else {
+    if (bck == 1) hipLaunchKernelGGL((k_grid_culled<1, 1>), dim3(nb), dim3(256), 0, st, g, cg, s, b, par);
+    else if (bck == 2) hipLaunchKernelGGL((k_grid_culled<1, 2>), dim3(nb), dim3(256), 0, st, g, cg, s, b, par);
+    else hipLaunchKernelGGL((k_grid_culled<1, 3>), dim3(nb), dim3(256), 0, st, g, cg, s, b, par);
+  }
 }
 
 void launch_grid_build(const GridArgs &g, hipStream_t st) {

@@ -184,10 +184,14 @@ class Grid:
         self.node_bytes = int(info[12])
         self.geom = info[:12].copy()
 
-    def __del__(self):
+    def close(self):
+        """Free the device grid now (also done when the object is collected)."""
         if getattr(self, "h", None):
             lib.rhg_grid_free(self.h)
             self.h = None
+
+    def __del__(self):
+        self.close()
 
     @property
     def shape(self):

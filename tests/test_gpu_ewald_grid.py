@@ -44,12 +44,15 @@ def test_fourier_adsorbate(W, O, R):
         assert abs(da[i] - out[1]) <= 1e-10 * max(1.0, abs(out[1]))
 
 
-def test_grid_nodes_bit_exact(W, O, R):
+@pytest.mark.parametrize("cull", [False, True])
+def test_grid_nodes_bit_exact(W, O, R, cull):
     """MakeGrid node values (FP32 channels) equal the oracle's bit for bit on a
-    coarse grid of the triclinic C2 framework."""
+    coarse grid of the triclinic C2 framework, built all-pairs or through the
+    cell lists."""
     fw = W.c2_framework()
     sys_o = oracle_system(fw)
     ctx = fw.context()
+    ctx.set_culling(cull, 2.5)
     to = sys_o.add_pseudo_atom(148.0, 3.73, 0.0)
     tg = ctx.add_pseudo_atom(148.0, 3.73, 0.0)
     g = ctx.MakeGrid(tg, fw.unit_box, 1.0)
@@ -92,3 +95,25 @@ def test_derivatives(W, O, R):
     for i in range(500):
         ref = sys_o.derivatives_vdw_at(to, pts[i])
         assert same_bits(d[i], ref), i
+
+
+def test_grid_mixed_vs_fp64(W, O, R):
+    """Mixed-precision grid build (FP32 pair terms over the fixed-point image
+    lists, FP64 sums): the same capped nodes as the FP64 build, and the
+    interpolated energy within 1e-5 relative (|U| > 1 K) or 1e-3 K."""
+    fw = W.c2_framework()
+    grids = []
+    for prec in (R.PREC_FP64, R.PREC_MIXED):
+        ctx = fw.context(precision=prec)
+        tg = ctx.add_pseudo_atom(148.0, 3.73, 0.0)
+        grids.append((ctx, ctx.MakeGrid(tg, fw.unit_box, 0.5)))
+    d64, dmx = grids[0][1].download(), grids[1][1].download()
+    assert np.array_equal(d64[..., 0] >= 2e7, dmx[..., 0] >= 2e7)
+    pts = random_positions(fw.box, 4000, 37)
+    u64, umx = grids[0][1].interpolate(pts), grids[1][1].interpolate(pts)
+    e2k = O.units()["energy_to_kelvin"]
+    ok = np.abs(u64) < 1e7
+    err = np.abs(umx[ok] - u64[ok]) * e2k
+    big = np.abs(u64[ok]) * e2k > 1.0
+    assert np.all(err[big] <= 1e-5 * np.abs(u64[ok][big]) * e2k)
+    assert np.all(err[~big] <= 1e-3)
