@@ -184,6 +184,18 @@ int rhg_widom_run(rhg_ctx *ctx, const rhg_molecule *mol, uint64_t seed, int64_t 
  * uniforms.  No blocks; rec is required. */
 int rhg_widom_run_trials(rhg_ctx *ctx, const rhg_molecule *mol, int64_t n, const double *fb,
                          const double *cord, const double *u_sel, double *rec);
+/* Blocked pockets of the Widom path (src/framework.c:6168-6300, checked at
+ * src/mc_moves.c:7882-7886): n centres (xyz, n x 3, already replicated over
+ * the unit cells of the system as src/framework.c:6209-6226 does) with radii
+ * [A], periodic with the unit cell unit_box (ApplyBoundaryConditionUnitCell,
+ * src/potentials.c:5715).  A grown molecule with any atom within a pocket
+ * (outside every pocket when invert = 1, InvertBlockPockets) contributes
+ * W = 0 with record stage 5.  Applies to later Widom calls on the context;
+ * n = 0 removes the pockets. */
+int rhg_set_block_pockets(rhg_ctx *ctx, int32_t n, const double *xyz, const double *radius,
+                          const double unit_box[9], int32_t invert);
+/* int BlockedPocket(VECTOR pos) (src/framework.c:6246) for n positions */
+int rhg_blocked_pocket(rhg_ctx *ctx, int64_t n, const double *xyz, uint8_t *blocked);
 /* Asynchronous form of rhg_widom_run, for callers that keep several systems
  * in flight (RASPA2 runs its systems one after another, src/monte_carlo.c:
  * 626-748; a screening loop over many frameworks wants them concurrent):

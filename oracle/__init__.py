@@ -100,6 +100,8 @@ def _declare(L):
     f("orc_lm_coeff", None, _P)
     f("orc_interpolate_grid", _D, _P, _P, _I, _P, _P, _P)
     f("orc_interpolate_direct", _D, _P, _I, _P, _I, _P, _P, _P)
+    f("orc_set_block_pockets", _I, _P, _I, _P, _P, _P, _I)
+    f("orc_blocked_pocket", _I, _P, _P)
 
 
 def _ptr(a):
@@ -238,6 +240,16 @@ class System:
         out = np.zeros(14)
         lib().orc_derivatives_vdw_at(self.h, t, _ptr(p), _ptr(out))
         return out
+
+    def set_block_pockets(self, centres, radii, unit_box, invert=False):
+        c = _f64(centres).reshape(-1, 3)
+        r = _f64(radii).reshape(-1)
+        self._pockets = (c, r)
+        lib().orc_set_block_pockets(self.h, len(r), _ptr(c), _ptr(r), _ptr(_f64(unit_box)),
+                                    1 if invert else 0)
+
+    def BlockedPocket(self, pos):
+        return bool(lib().orc_blocked_pocket(self.h, _ptr(_f64(pos))))
 
     def interpolate_direct(self, t, geom, bc, unit_box, unit_inv, pos):
         """InterpolateVDWGrid at pos with the corner nodes computed on the fly."""

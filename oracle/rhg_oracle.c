@@ -313,6 +313,12 @@ struct orc_system {
   cplx *sf;
   double uion;
   double net_charge_framework;
+  /* blocked pockets (src/framework.c:6168-6300): centres (xyz), radii, the
+   * unit cell they repeat with, and the inverted mode */
+  int n_pockets;
+  double *pk; /* n x 4: x, y, z, radius */
+  double pk_ubox[9], pk_uinv[9];
+  int pk_invert;
 };
 
 /* LENNARD_JONES branch of PotentialValue (src/potentials.c:5867-5877) */
@@ -320,6 +326,58 @@ static inline double lj_value(double arg1, double sigma, double arg3, double rr)
   double arg2 = SQR(sigma);
   double rri3 = CUBE(arg2 / rr);
   return 4.0 * arg1 * (rri3 * (rri3 - 1.0)) - arg3;
+}
+
+int orc_set_block_pockets(orc_system *s, int n, const double *xyz, const double *radius,
+                          const double unit_box[9], int invert) {
+  free(s->pk);
+  s->pk = NULL;
+  s->n_pockets = 0;
+  if (n <= 0) return 0;
+  s->pk = (double *)malloc(sizeof(double) * 4 * (size_t)n);
+  for (int i = 0; i < n; i++) {
+    s->pk[4 * i] = xyz[3 * i];
+    s->pk[4 * i + 1] = xyz[3 * i + 1];
+    s->pk[4 * i + 2] = xyz[3 * i + 2];
+    s->pk[4 * i + 3] = radius[i];
+  }
+  memcpy(s->pk_ubox, unit_box, 9 * sizeof(double));
+  double det;
+  orc_invert3x3(s->pk_ubox, s->pk_uinv, &det);
+  s->pk_invert = invert ? 1 : 0;
+  s->n_pockets = n;
+  return 0;
+}
+
+/* ApplyBoundaryConditionUnitCell (src/potentials.c:5715-5752) */
+static void min_image_unit_cell(const orc_system *s, double *dx, double *dy, double *dz) {
+  const double *ub = s->pk_ubox, *ui = s->pk_uinv;
+  if (s->bc == ORC_BC_RECTANGULAR) {
+    /* UnitCellSize: the unit-cell edge lengths of a rectangular cell */
+    *dx -= ub[AX] * (double)nint(*dx / ub[AX]);
+    *dy -= ub[BY] * (double)nint(*dy / ub[BY]);
+    *dz -= ub[CZ] * (double)nint(*dz / ub[CZ]);
+  } else {
+    double sx = ui[AX] * *dx + ui[BX] * *dy + ui[CX] * *dz;
+    double sy = ui[AY] * *dx + ui[BY] * *dy + ui[CY] * *dz;
+    double sz = ui[AZ] * *dx + ui[BZ] * *dy + ui[CZ] * *dz;
+    double tx = sx - (double)nint(sx), ty = sy - (double)nint(sy), tz = sz - (double)nint(sz);
+    *dx = ub[AX] * tx + ub[BX] * ty + ub[CX] * tz;
+    *dy = ub[AY] * tx + ub[BY] * ty + ub[CY] * tz;
+    *dz = ub[AZ] * tx + ub[BZ] * ty + ub[CZ] * tz;
+  }
+}
+
+/* BlockedPocket (src/framework.c:6246-6300) */
+int orc_blocked_pocket(const orc_system *s, const double pos[3]) {
+  if (s->n_pockets <= 0) return 0;
+  for (int i = 0; i < s->n_pockets; i++) {
+    double dx = s->pk[4 * i] - pos[0], dy = s->pk[4 * i + 1] - pos[1], dz = s->pk[4 * i + 2] - pos[2];
+    min_image_unit_cell(s, &dx, &dy, &dz);
+    double r = sqrt(SQR(dx) + SQR(dy) + SQR(dz));
+    if (r < s->pk[4 * i + 3]) return s->pk_invert ? 0 : 1;
+  }
+  return s->pk_invert ? 1 : 0;
 }
 
 int orc_add_pseudo_atom(orc_system *s, double eps_k, double sigma, double charge) {
@@ -587,6 +645,7 @@ orc_system *orc_system_create(const double box[9], int bc, int n_atoms, const do
 }
 
 void orc_system_destroy(orc_system *s) {
+  if (s) free(s->pk);
   if (!s) return;
   free(s->x); free(s->y); free(s->z); free(s->q); free(s->eps); free(s->sig);
   for (int t = 0; t < s->ntypes; t++) free(s->pp[t]);
@@ -1028,6 +1087,20 @@ static void widom_core(const orc_system *s, const orc_molecule *m, draw_t *d, do
   RosenbluthNew *= exp(-beta * (0.0 + 0.0));
   RosenbluthNew *= exp(-beta * (0.0 + 0.0));
   RosenbluthNew *= exp(-beta * (UHostChargeChargeNew + 0.0 + 0.0));
+  /* WidomAdsorbateMove (src/mc_moves.c:7882-7886): a grown molecule with an
+   * atom in a blocked pocket contributes nothing */
+  for (int j = 0; j < m->n_sites; j++)
+    if (orc_blocked_pocket(s, newpos[j])) {
+      rec[ORC_REC_W] = 0.0;
+      rec[ORC_REC_DU] = 0.0;
+      rec[ORC_REC_UVDW] = 0.0;
+      rec[ORC_REC_UREAL] = 0.0;
+      rec[ORC_REC_UFHOST] = 0.0;
+      rec[ORC_REC_UFADS] = 0.0;
+      rec[ORC_REC_OVERLAP] = 0.0;
+      rec[ORC_REC_STAGE] = 5.0;
+      return;
+    }
   /* WidomAdsorbateMove (src/mc_moves.c:7878-7933): tail correction 0 */
   RosenbluthNew *= exp(-beta * 0.0);
   double fo[5] = {0, 0, 0, 0, 0};

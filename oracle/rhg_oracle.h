@@ -88,6 +88,13 @@ orc_system *orc_system_create(const double box[9], int bc, int n_atoms,
 void orc_system_destroy(orc_system *s);
 /* registers a probe pseudo-atom; returns its type id */
 int  orc_add_pseudo_atom(orc_system *s, double eps_k, double sigma, double charge);
+/* blocked pockets: n centres (xyz, n x 3) with radii, periodic with the unit
+ * cell unit_box; invert = 1 blocks everything outside the pockets
+ * (src/framework.c:6168-6236) */
+int orc_set_block_pockets(orc_system *s, int n, const double *xyz, const double *radius,
+                          const double unit_box[9], int invert);
+/* int BlockedPocket(VECTOR) (src/framework.c:6246-6300) */
+int orc_blocked_pocket(const orc_system *s, const double pos[3]);
 /* pair parameters of (type, atom): eps [internal], sigma [A], shift [internal] */
 void orc_pair_params(const orc_system *s, int type, int atom, double out[3]);
 void orc_system_info(const orc_system *s, double out[8]);  /* alpha,kx,ky,kz,nk,uion,beta,volume */

@@ -178,6 +178,9 @@ struct rhg_ctx {
   std::map<std::string, hipGraphExec_t> graph_cache;
   // host-side fault hook: fail the chunk after this many more chunks (<0 off)
   int64_t fault_after = -1;
+  // blocked pockets of the Widom path (device copy of the centres)
+  PocketArgs pockets = {};
+  double *d_pockets = nullptr;
   std::map<std::string, std::pair<int64_t, double>> stats;
   std::vector<Pending> pending;
   std::vector<hipEvent_t> event_pool;
@@ -743,6 +746,7 @@ void rhg_destroy(rhg_ctx *c) {
   hipFree(c->d_row_off);
   hipFree(c->d_row_kz);
   hipFree(c->d_acc);
+  hipFree(c->d_pockets);
   hipFree(c->d_ctl);
   hipFree(c->d_neff);
   for (auto &kv : c->graph_cache) hipGraphExecDestroy(kv.second);
@@ -1176,6 +1180,7 @@ void block_bounds(int64_t n_total, int spc, int64_t bounds[6]) {
 WidomArgs widom_args(rhg_ctx *c, const MolDev &m, uint64_t seed, int64_t cap) {
   WidomArgs a;
   memset(&a, 0, sizeof(a));
+  a.pockets = c->pockets;
   a.mol = m;
   a.cell = c->cell;
   a.beta = c->beta;
@@ -1743,6 +1748,49 @@ int rhg_set_culling(rhg_ctx *c, int32_t on, double cell_size) {
     c->cell_size = cell_size;
     c->grid_ready = false;
   }
+  return RHG_OK;
+}
+
+int rhg_set_block_pockets(rhg_ctx *c, int32_t n, const double *xyz, const double *radius,
+                          const double unit_box[9], int32_t invert) {
+  if (!c || n < 0 || (n > 0 && (!xyz || !radius || !unit_box))) return RHG_ERR_INVALID;
+  hipSetDevice(c->device);
+  HIPC(c, hipStreamSynchronize(c->stream));
+  hipFree(c->d_pockets);
+  c->d_pockets = nullptr;
+  memset(&c->pockets, 0, sizeof(c->pockets));
+  c->gen++; // captured chunks hold the pocket arguments
+  if (n == 0) return RHG_OK;
+  std::vector<double> h((size_t)n * 4);
+  for (int i = 0; i < n; ++i) {
+    h[4 * i] = xyz[3 * i];
+    h[4 * i + 1] = xyz[3 * i + 1];
+    h[4 * i + 2] = xyz[3 * i + 2];
+    h[4 * i + 3] = radius[i];
+  }
+  if (int rc = dalloc(c, (void **)&c->d_pockets, h.size() * sizeof(double))) return rc;
+  HIPC(c, hipMemcpy(c->d_pockets, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  c->pockets.n = n;
+  c->pockets.invert = invert ? 1 : 0;
+  c->pockets.bc = c->cell.bc == RHG_BC_RECTANGULAR ? 1 : 2;
+  c->pockets.pk = c->d_pockets;
+  memcpy(c->pockets.ubox, unit_box, 9 * sizeof(double));
+  invert3x3(c->pockets.ubox, c->pockets.uinv);
+  return RHG_OK;
+}
+
+int rhg_blocked_pocket(rhg_ctx *c, int64_t n, const double *xyz, uint8_t *blocked) {
+  if (!c || n < 0 || (n > 0 && (!xyz || !blocked))) return RHG_ERR_INVALID;
+  if (n == 0) return RHG_OK;
+  hipSetDevice(c->device);
+  const size_t b = (size_t)n * 3 * sizeof(double);
+  double *dx = (double *)work(c, "p_xyz", b);
+  uint8_t *df = (uint8_t *)work(c, "p_flags", (size_t)n);
+  if (!dx || !df) return fail(c, RHG_ERR_OOM, "pocket buffers");
+  upload(c, dx, xyz, b);
+  launch_blocked_pockets(c->pockets, dx, n, df, c->stream);
+  HIPC(c, hipMemcpyAsync(blocked, df, (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
   return RHG_OK;
 }
 

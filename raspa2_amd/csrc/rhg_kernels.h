@@ -156,7 +156,19 @@ struct WidomWork {
 // five blocks start (and the job end).
 enum { CTL_FIRST = 0, CTL_END = 1, CTL_JB = 2, CTL_N = 8 };
 
+// Blocked pockets (src/framework.c:6168-6300): centres periodic with the
+// unit cell; a grown molecule with an atom inside one (outside all, when
+// inverted) is rejected (src/mc_moves.c:7882-7886)
+struct PocketArgs {
+  int n;            // pockets (0: none)
+  int invert;
+  int bc;           // 1 rectangular, otherwise triclinic
+  const double *pk; // n x 4: x, y, z, radius
+  double ubox[9], uinv[9];
+};
+
 struct WidomArgs {
+  PocketArgs pockets;
   MolDev mol;
   Cell cell;
   double beta;
@@ -173,6 +185,10 @@ __device__ __forceinline__ int64_t chunk_n(const WidomArgs &a) {
   const int64_t left = a.ctl[CTL_END] - a.ctl[CTL_FIRST];
   return left < a.n ? (left > 0 ? left : 0) : a.n;
 }
+
+// int BlockedPocket(VECTOR) for n positions (xyz interleaved) -> flags
+void launch_blocked_pockets(const PocketArgs &p, const double *xyz, int64_t n, uint8_t *flags,
+                            hipStream_t st);
 
 // writes the control block (values by argument, no host memory involved)
 void launch_set_ctl(int64_t *ctl, int64_t first, int64_t end, const int64_t jb[6], hipStream_t st);

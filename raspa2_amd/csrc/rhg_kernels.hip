@@ -1084,6 +1084,40 @@ __device__ __forceinline__ void abc_to_xyz(const Cell &c, double sx, double sy, 
 
 // first-bead trial positions: counter draws 0..3k_fb-1 (z,y,x per trial,
 // src/cbmc.c:652-667)
+// BlockedPocket (src/framework.c:6246-6300) with
+// ApplyBoundaryConditionUnitCell (src/potentials.c:5715-5752), FP64 in the
+// reference's operation order
+__device__ bool blocked_pocket(const PocketArgs &p, double x, double y, double z) {
+#pragma clang fp contract(off)
+  if (p.n <= 0) return false;
+  for (int i = 0; i < p.n; ++i) {
+    double dx = p.pk[4 * i] - x, dy = p.pk[4 * i + 1] - y, dz = p.pk[4 * i + 2] - z;
+    if (p.bc == 1) {
+      dx -= p.ubox[AX] * nint_d(dx / p.ubox[AX]);
+      dy -= p.ubox[BY] * nint_d(dy / p.ubox[BY]);
+      dz -= p.ubox[CZ] * nint_d(dz / p.ubox[CZ]);
+    } else {
+      const double sx = p.uinv[AX] * dx + p.uinv[BX] * dy + p.uinv[CX] * dz;
+      const double sy = p.uinv[AY] * dx + p.uinv[BY] * dy + p.uinv[CY] * dz;
+      const double sz = p.uinv[AZ] * dx + p.uinv[BZ] * dy + p.uinv[CZ] * dz;
+      const double tx = sx - nint_d(sx), ty = sy - nint_d(sy), tz = sz - nint_d(sz);
+      dx = p.ubox[AX] * tx + p.ubox[BX] * ty + p.ubox[CX] * tz;
+      dy = p.ubox[AY] * tx + p.ubox[BY] * ty + p.ubox[CY] * tz;
+      dz = p.ubox[AZ] * tx + p.ubox[BZ] * ty + p.ubox[CZ] * tz;
+    }
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    if (r < p.pk[4 * i + 3]) return !p.invert;
+  }
+  return p.invert != 0;
+}
+
+__global__ __launch_bounds__(256) void k_blocked_pockets(const PocketArgs p, const double *__restrict__ xyz,
+                                                         int64_t n, uint8_t *__restrict__ flags) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  flags[i] = blocked_pocket(p, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]) ? 1 : 0;
+}
+
 __global__ __launch_bounds__(256) void k_widom_fb_gen(const WidomArgs a, const WidomWork w) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t n = chunk_n(a);
@@ -1181,6 +1215,13 @@ __global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, cons
   w.stage[i] = 4;
   const MolDev &m = a.mol;
   if (m.n_sites == 1) {
+    if (blocked_pocket(a.pockets, fx, fy, fz)) {
+      w.stage[i] = 5;
+      w.W[i] = 0.0;
+      w.uvdw[i] = 0.0;
+      w.ureal[i] = 0.0;
+      return;
+    }
     int s2 = atomicAdd(w.n_active2, 1);
     w.active2[s2] = (int32_t)i;
     w.cfx[s2] = fx;
@@ -1294,6 +1335,28 @@ __global__ __launch_bounds__(256) void k_widom_or_select(const WidomArgs a, cons
     w.ureal[i] = 0.0;
     return;
   }
+  if (a.pockets.n > 0) {
+    // every atom of the grown molecule, in atom order
+    bool blocked = false;
+    for (int j = 0; j < m.n_sites && !blocked; ++j) {
+      double x = w.selx[i], y = w.sely[i], z = w.selz[i];
+      for (int p = 0; p < m.n_place; ++p)
+        if (m.place[p] == j) {
+          const int64_t o = ((int64_t)(m.g_start[p] + m.g_b[p]) * ko + iwalk) * cap + slot;
+          x = w.orx[o];
+          y = w.ory[o];
+          z = w.orz[o];
+        }
+      blocked = blocked_pocket(a.pockets, x, y, z);
+    }
+    if (blocked) {
+      w.stage[i] = 5;
+      w.W[i] = 0.0;
+      w.uvdw[i] = 0.0;
+      w.ureal[i] = 0.0;
+      return;
+    }
+  }
   double ev;
   ev_of(iwalk, ev);
   w.uvdw[i] = w.uvdw[i] + ev;
@@ -1365,7 +1428,7 @@ __global__ __launch_bounds__(256) void k_widom_records(const int64_t n, const Wi
   r[3] = acc ? w.ureal[i] : 0.0;
   r[4] = acc ? w.ufh[i] : 0.0;
   r[5] = acc ? w.ufa[i] : 0.0;
-  r[6] = acc ? 0.0 : 1.0;
+  r[6] = (acc || stg == 5) ? 0.0 : 1.0; // a blocked molecule did not overlap
   r[7] = (stg >= 1) ? (double)w.selfb[i] : 0.0;
   r[8] = (stg >= 3) ? (double)w.selor[i] : 0.0;
   r[9] = (double)w.fbmask[i];
@@ -1400,6 +1463,11 @@ void launch_widom_or_select(const WidomArgs &a, const WidomWork &w, hipStream_t 
 void launch_widom_finalize(const WidomArgs &a, const WidomWork &w, hipStream_t st) {
   if (a.n > 0) hipLaunchKernelGGL(k_widom_finalize, dim3(nblk(a.n, 256)), dim3(256), 0, st, a, w);
 }
+void launch_blocked_pockets(const PocketArgs &p, const double *xyz, int64_t n, uint8_t *flags,
+                            hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_blocked_pockets, dim3(nblk(n, 256)), dim3(256), 0, st, p, xyz, n, flags);
+}
+
 void launch_widom_records(const WidomArgs &a, const WidomWork &w, double *rec, hipStream_t st) {
   if (a.n > 0) hipLaunchKernelGGL(k_widom_records, dim3(nblk(a.n, 256)), dim3(256), 0, st, a.n, w, rec);
 }

@@ -101,6 +101,8 @@ def _load():
     f("rhg_widom_run", _P, ctypes.POINTER(MoleculeT), _U64, _I64, _I64, _I64, _I32,
       ctypes.POINTER(WidomBlocks), _P)
     f("rhg_widom_run_trials", _P, ctypes.POINTER(MoleculeT), _I64, _P, _P, _P, _P)
+    f("rhg_set_block_pockets", _P, _I32, _P, _P, _P, _I32)
+    f("rhg_blocked_pocket", _P, _I64, _P, _P)
     f("rhg_widom_submit", _P, ctypes.POINTER(MoleculeT), _U64, _I64, _I64, _I64, _I32)
     f("rhg_widom_query", _P)
     f("rhg_widom_collect", _P, ctypes.POINTER(WidomBlocks))
@@ -327,6 +329,21 @@ class Context:
                                       steps_per_cycle, ctypes.byref(blocks), _ptr(rec)),
                     "widom_run")
         return blocks, rec
+
+    # ---- src/framework.c:6168-6300 blocked pockets
+    def set_block_pockets(self, centres, radii, unit_box, invert=False):
+        c = _f64(centres).reshape(-1, 3)
+        r = _f64(radii).reshape(-1)
+        ub = _f64(unit_box).reshape(9)
+        self._check(lib.rhg_set_block_pockets(self.h, len(r), _ptr(c), _ptr(r), _ptr(ub),
+                                              1 if invert else 0), "set_block_pockets")
+
+    def BlockedPocket(self, xyz):
+        xyz = _f64(xyz).reshape(-1, 3)
+        out = np.zeros(len(xyz), dtype=np.uint8)
+        self._check(lib.rhg_blocked_pocket(self.h, len(xyz), _ptr(xyz), _ptr(out)),
+                    "blocked_pocket")
+        return out.astype(bool)
 
     def widom_submit(self, mol, seed, n_total, first=0, n=None, steps_per_cycle=20):
         """Enqueue insertions and return at once (sums accumulate on the device)."""

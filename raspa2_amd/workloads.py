@@ -207,3 +207,24 @@ def c3_job(i, min_width=24.0):
     fw = FrameworkSpec(f"c3_{i}", box, R.BC_RECTANGULAR, xyz, eps, sig, q, np.full(n, 12.0),
                        unit_box=box)
     return ScreeningJob(i, fw, probe, n_trials, 20_000 + i)
+
+
+def block_pocket_centres(fractional, radii, box, n_cells=(1, 1, 1), shift=(0.0, 0.0, 0.0)):
+    """Blocked-pocket centres of a system of n_cells unit cells, as
+    src/framework.c:6209-6226 builds them: for each pocket (fractional
+    coordinates of the unit cell plus the unit-cell shift) and each unit cell
+    (j, k, l), the point box @ ((f + (j, k, l)) / n_cells).  Returns the
+    centres (xyz) and their radii in that order."""
+    f = np.asarray(fractional, dtype=np.float64).reshape(-1, 3)
+    B = np.asarray(box, dtype=np.float64).reshape(3, 3)  # rows a, b, c
+    cen, rad = [], []
+    for i in range(len(f)):
+        t = f[i] + np.asarray(shift)
+        for j in range(n_cells[0]):
+            for k in range(n_cells[1]):
+                for l in range(n_cells[2]):
+                    v = np.array([(t[0] + j) / n_cells[0], (t[1] + k) / n_cells[1],
+                                  (t[2] + l) / n_cells[2]])
+                    cen.append(v @ B)
+                    rad.append(radii[i])
+    return np.array(cen), np.array(rad)
