@@ -364,6 +364,18 @@ def main():
     rel_kh = abs(e_t["kh_overall"] - e_r["kh_overall"]) / e_r["kh_overall"]
     est = R.widom_estimators(blocks, 300.0, rho)
 
+    # HBM bytes per site-energy launch, measured by two rocprofv3 --pmc passes
+    # (FETCH_SIZE, WRITE_SIZE) over this same command and scaled by the
+    # counters' readings for a copy kernel of known size
+    traffic, traffic_note = None, "no committed counter summary"
+    tpath = os.path.join(ROOT, "profiles", "c1_co2_widom_traffic.json")
+    if os.path.exists(tpath) and args.precision == "mixed":
+        with open(tpath) as f:
+            tj = json.load(f)
+        traffic = round(tj["bytes_per_launch"])
+        traffic_note = ("bytes per launch averaged over the three site-energy kernels of a step, "
+                        "from " + tj["source"])
+
     cpu_obj = None
     cpu_rel = None
     if cpu is not None:
@@ -403,7 +415,8 @@ def main():
                    "precision": args.precision, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "kernel": "site_energy", "achieved": round(achieved, 3),
                      "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
-                     "traffic": None,
+                     "traffic": traffic,
+                     "traffic_note": traffic_note,
                      "note": "algorithmic flops of the executed (cell-list culled) LJ (29/pair), "
                              "LJ+Coulomb (38) and Coulomb (30) pair evaluations over the summed "
                              "HIP-event durations of the site_energy launches; peak is the "

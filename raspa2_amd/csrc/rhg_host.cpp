@@ -604,6 +604,17 @@ int site_energy(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const Energy
   b.fill = (int32_t *)work(c, (t + "_fill").c_str(), (size_t)ncell * sizeof(int32_t));
   b.start = (int32_t *)work(c, (t + "_start").c_str(), (size_t)(ncell + 1) * sizeof(int32_t));
   b.tile_off = (int32_t *)work(c, (t + "_tiles").c_str(), (size_t)(ncell + 1) * sizeof(int32_t));
+  CellGrid g = c->cg;
+  if (!c->use_images) {
+    g.img_off = nullptr;
+    g.img = nullptr;
+  }
+  // the image-list kernels read the sites' sorted fixed-point offsets
+  b.fix = nullptr;
+  if (a.precision != RHG_PREC_FP64 && g.img_off && a.par) {
+    b.fix = (int4 *)work(c, (t + "_fix").c_str(), (size_t)s.n * sizeof(int4));
+    if (!b.fix) return fail(c, RHG_ERR_OOM, "binning scratch");
+  }
   bool fresh = c->bufs["pair_counter"].p == nullptr;
   b.pairs = (unsigned long long *)work(c, "pair_counter", 64);
   if (fresh && b.pairs) fill((double *)b.pairs, 8, 0.0, c->stream); // all-zero bits
@@ -612,11 +623,6 @@ int site_energy(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const Energy
   {
     Timed tm(c, "bin");
     launch_bin_sites(c->cell, c->cg, s, b, c->stream);
-  }
-  CellGrid g = c->cg;
-  if (!c->use_images) {
-    g.img_off = nullptr;
-    g.img = nullptr;
   }
   Timed tm(c, "site_energy");
   launch_site_energy_culled(a, g, s, b, o, c->stream);
@@ -1596,6 +1602,7 @@ int rhg_make_grid(rhg_ctx *c, int32_t type, const double unit_box[9], double spa
     b.start = (int32_t *)work(c, "g_start", (size_t)(ncell + 1) * sizeof(int32_t));
     b.tile_off = (int32_t *)work(c, "g_tiles", (size_t)(ncell + 1) * sizeof(int32_t));
     b.pairs = nullptr;
+    b.fix = nullptr;
     if (!b.cell_of || !b.perm || !b.count || !b.fill || !b.start || !b.tile_off)
       return fail(c, RHG_ERR_OOM, "grid binning scratch");
     launch_bin_sites(c->cell, c->cg, sl, b, c->stream);

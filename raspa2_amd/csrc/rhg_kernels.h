@@ -70,6 +70,9 @@ struct BinWork {
   int32_t *start;    // per cell, exclusive scan of count
   int32_t *tile_off; // per cell + 1, exclusive scan of ceil(count/64)
   int32_t *perm;     // sorted position -> site index
+  // optional (image-list kernels): per sorted position the site's offset from
+  // its cell centre in fixed point and the site index, {dx, dy, dz, site}
+  int4 *fix;
   unsigned long long *pairs; // executed pair evaluations [LJ, LJ+Coulomb, Coulomb]
 };
 

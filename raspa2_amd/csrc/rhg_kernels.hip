@@ -494,7 +494,9 @@ __global__ __launch_bounds__(256) void k_bin_count(const Cell cell, const CellGr
   if ((int)(threadIdx.x & 63) == r.head) atomicAdd(&b.count[c], r.length);
 }
 
-__global__ __launch_bounds__(256) void k_bin_scatter(const SiteList s, BinWork b) {
+template <int BCK>
+__global__ __launch_bounds__(256) void k_bin_scatter(const Cell cell, const CellGrid g, const SiteList s,
+                                                     BinWork b) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < site_count(s);
   const int c = valid ? b.cell_of[i] : -1;
@@ -503,7 +505,18 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const SiteList s, BinWork b
   const int lane = threadIdx.x & 63;
   if (valid && lane == r.head) base = b.start[c] + atomicAdd(&b.fill[c], r.length);
   base = __shfl(base, r.head);
-  if (valid) b.perm[base + (lane - r.head)] = (int32_t)site_phys(s, i);
+  if (!valid) return;
+  const int64_t site = site_phys(s, i);
+  const int pos = base + (lane - r.head);
+  b.perm[pos] = (int32_t)site;
+  if (b.fix) {
+    // the site's offset from its cell centre, sorted, for the image lists
+    double cx, cy, cz;
+    cell_centre(cell, g, c, cx, cy, cz);
+    double ox = s.x[site] - cx, oy = s.y[site] - cy, oz = s.z[site] - cz;
+    mi_exact<BCK>(cell, ox, oy, oz);
+    b.fix[pos] = make_int4((int)rint(ox * kFix), (int)rint(oy * kFix), (int)rint(oz * kFix), (int)site);
+  }
 }
 
 // exclusive scans of the per-cell site counts (start) and of the per-cell
@@ -550,7 +563,14 @@ void launch_bin_sites(const Cell &cell, const CellGrid &g, const SiteList &s, co
   const unsigned nb = (unsigned)((s.n + 255) / 256);
   if (nb) hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(256), 0, st, cell, g, s, b);
   hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(1024), 0, st, b.count, g.ncell, b.start, b.tile_off);
-  if (nb) hipLaunchKernelGGL(k_bin_scatter, dim3(nb), dim3(256), 0, st, s, b);
+  if (nb) {
+    if (cell.bc == 1)
+      hipLaunchKernelGGL(k_bin_scatter<1>, dim3(nb), dim3(256), 0, st, cell, g, s, b);
+    else if (cell.upper)
+      hipLaunchKernelGGL(k_bin_scatter<2>, dim3(nb), dim3(256), 0, st, cell, g, s, b);
+    else
+      hipLaunchKernelGGL(k_bin_scatter<3>, dim3(nb), dim3(256), 0, st, cell, g, s, b);
+  }
 }
 
 // One wave per tile of <= 64 sites of one cell; the wave walks the cell's
@@ -683,13 +703,10 @@ __global__ __launch_bounds__(256) void k_site_energy_img(
   const int c = __builtin_amdgcn_readfirstlane(lo);
   const int k = (tile - b.tile_off[c]) * 64 + lane;
   const bool live = k < b.count[c];
-  const int site = live ? b.perm[b.start[c] + k] : b.perm[b.start[c]];
-  const double px = s.x[site], py = s.y[site], pz = s.z[site];
-  double cx, cy, cz;
-  cell_centre(cell, g, c, cx, cy, cz);
-  double ox = px - cx, oy = py - cy, oz = pz - cz;
-  mi_exact<BCK>(cell, ox, oy, oz);
-  const int ix = (int)rint(ox * kFix), iy = (int)rint(oy * kFix), iz = (int)rint(oz * kFix);
+  // sorted fixed-point offsets from the cell centre (written by the binning)
+  const int4 fx = b.fix[b.start[c] + (live ? k : 0)];
+  const int site = fx.w;
+  const int ix = fx.x, iy = fx.y, iz = fx.z;
   const float rc2vf = (float)(rc2v * kFix * kFix), rc2cf = (float)(rc2c * kFix * kFix);
   const float alphaf = (float)(alpha / kFix), cqaf = (float)(cqa * kFix);
   const int q0 = g.img_off[c], q1 = g.img_off[c + 1];
@@ -737,6 +754,7 @@ __global__ __launch_bounds__(256) void k_site_energy_img(
   if (__any(near)) {
     if (near) {
 #pragma clang fp contract(off)
+      const double px = s.x[site], py = s.y[site], pz = s.z[site];
       double ev = 0.0, ec = 0.0;
       const int j0 = g.nb_off[c], j1 = g.nb_off[c + 1];
       for (int q = j0; q < j1; ++q) {
@@ -791,7 +809,7 @@ template <int VDW, int CC>
 static void culled_mode(const EnergyArgs &a, const CellGrid &g, const SiteList &s, const BinWork &b,
                         const EnergyOut &o, hipStream_t st) {
   const int bck = a.cell.bc == 1 ? 1 : (a.cell.upper ? 2 : 3);
-  if (a.precision != 0 && g.img_off && a.par) {
+  if (a.precision != 0 && g.img_off && a.par && b.fix) {
     if (bck == 1) img_launch<1, VDW, CC>(a, g, s, b, o, st);
     else if (bck == 2) img_launch<2, VDW, CC>(a, g, s, b, o, st);
     else img_launch<3, VDW, CC>(a, g, s, b, o, st);

@@ -172,14 +172,16 @@ size_t estimate_bytes(const rhg_framework *f, const rhg_cell *cell, const rhg_fo
   double per_ins = k_fb * 40.0 + 132.0 + (double)k_or_eff * n_place * 32.0 + n_sites * 24.0 +
                    n_place * 8.0 + 16.0;
   if (ewald && n_charged > 0) per_ins += (kx + 1) * 16.0;
-  // binning scratch: 8 B per binned site and 16 B per cell for each list
-  per_ins += k_fb * 8.0;
+  // binning scratch: 8 B per binned site (24 B with the sorted fixed-point
+  // offsets of mixed precision) and 16 B per cell for each list
+  const double per_site = precision == RHG_PREC_MIXED ? 24.0 : 8.0;
+  per_ins += k_fb * per_site;
   double lists = 1;
   if (n_sites > 1) {
-    per_ins += (double)k_or * 8.0;
+    per_ins += (double)k_or * (n_sites - 1) * per_site;
     lists += 1;
     if (ewald && n_charged > 0) {
-      per_ins += 8.0;
+      per_ins += (double)n_charged * per_site;
       lists += 1;
     }
   }
