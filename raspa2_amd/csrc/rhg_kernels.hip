@@ -13,6 +13,35 @@
 
 namespace rhg {
 
+// erfc(x) in FP32 for the mixed real-space Ewald terms:
+// exp(-x^2) P(t), t = 1 / (1 + x/2), with P of degree 8 fitted to
+// erfcx(x) = exp(x^2) erfc(x) on [0, 5] (raspa2_amd/csrc/fit_erfc.py; relative
+// error 3.5e-8 in exact arithmetic, below 5e-7 in FP32).  Larger arguments
+// (only for Ewald precisions beyond 1e-10) use the library erfcf.
+__device__ __forceinline__ float erfc_mixed(float x) {
+  if (x > 5.0f) return erfcf(x);
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.5f, x, 1.0f));
+  float p = -0.040972326f;
+  p = fmaf(p, t, 0.292736441f);
+  p = fmaf(p, t, -0.786886334f);
+  p = fmaf(p, t, 0.885848165f);
+  p = fmaf(p, t, -0.305710703f);
+  p = fmaf(p, t, 0.427213341f);
+  p = fmaf(p, t, 0.240593985f);
+  p = fmaf(p, t, 0.287479192f);
+  p = fmaf(p, t, -0.000301792286f);
+  // exp(-x^2) = 2^(-x^2 log2 e), with x^2 and the product carried as
+  // head + tail (FMA residuals) so that the exponent is accurate to FP32
+  // ulps of the result, not of x^2
+  const float x2 = x * x;
+  const float x2l = fmaf(x, x, -x2);
+  const float yh = -x2 * 1.44269502f;
+  float yl = fmaf(-x2, 1.44269502f, -yh);
+  yl = fmaf(-x2, 1.92596299e-08f, yl);
+  yl = fmaf(-x2l, 1.44269502f, yl);
+  return p * (__builtin_amdgcn_exp2f(yh) * fmaf(yl, 0.693147181f, 1.0f));
+}
+
 // ====================================================================
 // minimum image variants
 // ====================================================================
@@ -190,7 +219,7 @@ __global__ __launch_bounds__(256) void k_site_energy_mx(
       if (rr < rc2c) {
         const float irs = __builtin_amdgcn_rsqf(rrf);
         const float rd = rrf * irs;
-        const float e = erfcf(alphaf * rd);
+        const float e = erfc_mixed(alphaf * rd);
         uc += (double)((cqaf * r.q) * (e * irs));
       }
     }
@@ -637,7 +666,7 @@ __global__ __launch_bounds__(256) void k_site_energy_culled(
         if (rr < rc2c) {
           const float irs = __builtin_amdgcn_rsqf(rrf);
           const float rd = rrf * irs;
-          const float e = erfcf(alphaf * rd);
+          const float e = erfc_mixed(alphaf * rd);
           uc += (double)((cqaf * r.q) * (e * irs));
         }
       }
@@ -727,7 +756,7 @@ __global__ __launch_bounds__(256) void k_site_energy_img(
       fc = 0.0f;
       if (rr < rc2cf) {
         const float irs = __builtin_amdgcn_rsqf(rr);
-        const float e2 = erfcf(alphaf * (rr * irs));
+        const float e2 = erfc_mixed(alphaf * (rr * irs));
         fc = (cqaf * pr.w) * (e2 * irs);
       }
     }
