@@ -563,8 +563,8 @@ int ensure_image_list(rhg_ctx *c) {
   c->d_img_off = nullptr;
   int rc_;
   if ((rc_ = dalloc(c, (void **)&c->d_img_off, (size_t)(ncell + 1) * sizeof(int32_t)))) return rc_;
-  launch_image_list(c->cell, c->cg, c->cell_reach, c->d_ax, c->d_ay, c->d_az, counts, nullptr, nullptr,
-                    0, st);
+  launch_image_list(c->cell, c->cg, c->cell_reach, c->props + 3, c->d_ax, c->d_ay, c->d_az, counts,
+                    nullptr, nullptr, 0, st);
   launch_scan_counts(counts, ncell, c->d_img_off, st);
   int32_t total = 0;
   HIPC(c, hipMemcpyAsync(&total, c->d_img_off + ncell, sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -572,8 +572,8 @@ int ensure_image_list(rhg_ctx *c) {
   hipFree(c->d_img);
   c->d_img = nullptr;
   if ((rc_ = dalloc(c, (void **)&c->d_img, (size_t)std::max(total, 1) * sizeof(int4)))) return rc_;
-  launch_image_list(c->cell, c->cg, c->cell_reach, c->d_ax, c->d_ay, c->d_az, counts, c->d_img_off,
-                    c->d_img, 1, st);
+  launch_image_list(c->cell, c->cg, c->cell_reach, c->props + 3, c->d_ax, c->d_ay, c->d_az, counts,
+                    c->d_img_off, c->d_img, 1, st);
   HIPC(c, hipGetLastError());
   HIPC(c, hipStreamSynchronize(st));
   c->cg.img_off = c->d_img_off;
@@ -715,10 +715,13 @@ int rhg_create(const rhg_framework *fw, const rhg_cell *cellp, const rhg_forcefi
       upload(c.get(), d_q, qq.data(), nc * sizeof(double));
     }
     upload(c.get(), d_kidx, c->kidx.data(), c->kidx.size() * sizeof(int32_t));
+    const size_t tab_n = (size_t)std::max(nc, 1) * ((c->kx + 1) + (2 * c->ky + 1) + (2 * c->kz + 1));
+    double2 *d_tab = (double2 *)work(c.get(), "sf_tab", tab_n * sizeof(double2));
+    if (!d_tab) return RHG_ERR_OOM;
     {
       Timed t(c.get(), "sf");
-      launch_structure_factor(ewald_dev(c.get()), d_kidx, c->nk, d_fx, d_fy, d_fz, d_q, nc, c->d_sf,
-                              c->stream);
+      launch_structure_factor(ewald_dev(c.get()), d_kidx, c->nk, d_fx, d_fy, d_fz, d_q, nc, d_tab,
+                              c->d_sf, c->stream);
     }
     if (hipStreamSynchronize(c->stream) != hipSuccess) return RHG_ERR_HIP;
   }

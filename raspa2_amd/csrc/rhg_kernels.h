@@ -83,9 +83,10 @@ void launch_cell_neighbours(const Cell &cell, int nc0, int nc1, int nc2, double 
 void launch_scan_counts(const int32_t *counts, int n, int32_t *off, hipStream_t st);
 // pass 0 counts the images of each cell's neighbour atoms into counts, pass 1
 // writes them at off
-void launch_image_list(const Cell &cell, const CellGrid &g, double reach, const double *ax,
-                       const double *ay, const double *az, int32_t *counts, const int32_t *off,
-                       int4 *img, int pass, hipStream_t st);
+// widths: perpendicular widths of the cell
+void launch_image_list(const Cell &cell, const CellGrid &g, double reach, const double widths[3],
+                       const double *ax, const double *ay, const double *az, int32_t *counts,
+                       const int32_t *off, int4 *img, int pass, hipStream_t st);
 void launch_bin_sites(const Cell &cell, const CellGrid &g, const SiteList &s, const BinWork &b,
                       hipStream_t st);
 void launch_site_energy_culled(const EnergyArgs &a, const CellGrid &g, const SiteList &s,
@@ -104,9 +105,10 @@ struct EwaldDev {
 };
 
 // structure factors of the charged framework atoms (fractional coordinates)
+// tab: scratch of n x ((kx+1) + (2ky+1) + (2kz+1)) double2 (phase powers)
 void launch_structure_factor(const EwaldDev &e, const int32_t *kidx, int nk, const double *fx,
                              const double *fy, const double *fz, const double *q, int n,
-                             double2 *out, hipStream_t st);
+                             double2 *tab, double2 *out, hipStream_t st);
 
 // Fourier Delta of n molecules; positions pos[j][m] (site-major SoA of
 // three arrays each n_sites*stride); list/n_dev select molecules.

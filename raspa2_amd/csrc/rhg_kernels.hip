@@ -395,71 +395,90 @@ __device__ __forceinline__ void cell_centre(const Cell &cell, const CellGrid &g,
 }
 
 // One workgroup per cell over the cell's neighbour atoms (every atom within
-// reach in torus distance): each of the 27 lattice shifts of the wrapped
-// offset that lies within reach of the centre is an entry.  Visited
-// image-major, then in neighbour-list order, so the list order is fixed.
+// reach in torus distance).  The offset from the centre is wrapped to
+// fractional components s in [-1/2, 1/2]; along a cell vector the shifted
+// image s - sign(s) is at least (1 - |s|) W away (W: the perpendicular width)
+// and s + sign(s) at least W, so each atom has at most 2 candidates per
+// direction and 8 in all.  Candidates within reach are entries, atom-major in
+// neighbour-list order, shifts in a fixed order, so the list order is fixed.
 __global__ __launch_bounds__(256) void k_image_list(const Cell cell, const CellGrid g, double reach2,
+                                                    double w0, double w1, double w2,
                                                     const double *__restrict__ ax,
                                                     const double *__restrict__ ay,
                                                     const double *__restrict__ az,
                                                     int32_t *__restrict__ counts,
                                                     const int32_t *__restrict__ off,
                                                     int4 *__restrict__ img, int pass) {
-  __shared__ int wave_sum[4];
+  __shared__ int part[256];
   __shared__ int base;
   const int c = blockIdx.x;
   double cx, cy, cz;
   cell_centre(cell, g, c, cx, cy, cz);
   if (threadIdx.x == 0) base = pass ? off[c] : 0;
   __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const double reach = sqrt(reach2);
   const int q0 = g.nb_off[c], cnt = g.nb_off[c + 1] - q0;
-  const int total = 27 * cnt;
-  for (int i0 = 0; i0 < total; i0 += blockDim.x) {
+  for (int i0 = 0; i0 < cnt; i0 += blockDim.x) {
     const int i = i0 + threadIdx.x;
-    bool in = false;
-    double vx = 0.0, vy = 0.0, vz = 0.0;
-    int j = 0;
-    if (i < total) {
-      const int im = i / cnt;
-      j = g.nb_idx[q0 + i % cnt];
-      const double na = im / 9 - 1, nb = (im / 3) % 3 - 1, ncc = im % 3 - 1;
+    int j = 0, m = 0;
+    double s[3] = {0.0, 0.0, 0.0};
+    int cand[3][2] = {{0, 0}, {0, 0}, {0, 0}}, nc[3] = {1, 1, 1};
+    if (i < cnt) {
+      j = g.nb_idx[q0 + i];
       const double dx = ax[j] - cx, dy = ay[j] - cy, dz = az[j] - cz;
-      double sx = cell.inv[AX] * dx + cell.inv[BX] * dy + cell.inv[CX] * dz;
-      double sy = cell.inv[AY] * dx + cell.inv[BY] * dy + cell.inv[CY] * dz;
-      double sz = cell.inv[AZ] * dx + cell.inv[BZ] * dy + cell.inv[CZ] * dz;
-      sx = sx - rint(sx) + na;
-      sy = sy - rint(sy) + nb;
-      sz = sz - rint(sz) + ncc;
-      vx = cell.box[AX] * sx + cell.box[BX] * sy + cell.box[CX] * sz;
-      vy = cell.box[AY] * sx + cell.box[BY] * sy + cell.box[CY] * sz;
-      vz = cell.box[AZ] * sx + cell.box[BZ] * sy + cell.box[CZ] * sz;
-      in = vx * vx + vy * vy + vz * vz <= reach2;
+      s[0] = cell.inv[AX] * dx + cell.inv[BX] * dy + cell.inv[CX] * dz;
+      s[1] = cell.inv[AY] * dx + cell.inv[BY] * dy + cell.inv[CY] * dz;
+      s[2] = cell.inv[AZ] * dx + cell.inv[BZ] * dy + cell.inv[CZ] * dz;
+      const double w[3] = {w0, w1, w2};
+      for (int d = 0; d < 3; ++d) {
+        s[d] -= rint(s[d]);
+        if ((1.0 - fabs(s[d])) * w[d] <= reach) cand[d][nc[d]++] = s[d] > 0.0 ? -1 : 1;
+      }
     }
-    const unsigned long long m = __ballot(in);
-    const int before = __popcll(m & ((1ULL << lane) - 1ULL));
-    if (lane == 0) wave_sum[wave] = __popcll(m);
+    // count this thread's entries, then an exclusive scan over the block
+    auto image = [&](int a, int b, int e, double &vx, double &vy, double &vz) {
+      const double tx = s[0] + cand[0][a], ty = s[1] + cand[1][b], tz = s[2] + cand[2][e];
+      vx = cell.box[AX] * tx + cell.box[BX] * ty + cell.box[CX] * tz;
+      vy = cell.box[AY] * tx + cell.box[BY] * ty + cell.box[CY] * tz;
+      vz = cell.box[AZ] * tx + cell.box[BZ] * ty + cell.box[CZ] * tz;
+      return vx * vx + vy * vy + vz * vz <= reach2;
+    };
+    if (i < cnt)
+      for (int a = 0; a < nc[0]; ++a)
+        for (int b = 0; b < nc[1]; ++b)
+          for (int e = 0; e < nc[2]; ++e) {
+            double vx, vy, vz;
+            m += image(a, b, e, vx, vy, vz) ? 1 : 0;
+          }
+    part[threadIdx.x] = m;
     __syncthreads();
-    int o = base;
-    for (int w = 0; w < wave; ++w) o += wave_sum[w];
-    if (in && pass)
-      img[o + before] = make_int4((int)rint(vx * kFix), (int)rint(vy * kFix), (int)rint(vz * kFix), j);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int t = 0;
-      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += wave_sum[w];
-      base += t;
+    for (int d = 1; d < (int)blockDim.x; d <<= 1) {
+      const int v = (int)threadIdx.x >= d ? part[threadIdx.x - d] : 0;
+      __syncthreads();
+      part[threadIdx.x] += v;
+      __syncthreads();
     }
+    int o = base + part[threadIdx.x] - m;
+    if (pass && i < cnt)
+      for (int a = 0; a < nc[0]; ++a)
+        for (int b = 0; b < nc[1]; ++b)
+          for (int e = 0; e < nc[2]; ++e) {
+            double vx, vy, vz;
+            if (image(a, b, e, vx, vy, vz))
+              img[o++] = make_int4((int)rint(vx * kFix), (int)rint(vy * kFix), (int)rint(vz * kFix), j);
+          }
+    __syncthreads();
+    if (threadIdx.x == blockDim.x - 1) base += part[threadIdx.x];
     __syncthreads();
   }
   if (threadIdx.x == 0 && !pass) counts[c] = base;
 }
 
-void launch_image_list(const Cell &cell, const CellGrid &g, double reach, const double *ax,
-                       const double *ay, const double *az, int32_t *counts, const int32_t *off,
-                       int4 *img, int pass, hipStream_t st) {
-  hipLaunchKernelGGL(k_image_list, dim3(g.ncell), dim3(256), 0, st, cell, g, reach * reach, ax, ay, az,
-                     counts, off, img, pass);
+void launch_image_list(const Cell &cell, const CellGrid &g, double reach, const double widths[3],
+                       const double *ax, const double *ay, const double *az, int32_t *counts,
+                       const int32_t *off, int4 *img, int pass, hipStream_t st) {
+  hipLaunchKernelGGL(k_image_list, dim3(g.ncell), dim3(256), 0, st, cell, g, reach * reach, widths[0],
+                     widths[1], widths[2], ax, ay, az, counts, off, img, pass);
 }
 
 __device__ __forceinline__ int cell_index(const Cell &c, const CellGrid &g, double x, double y,
@@ -867,47 +886,11 @@ void launch_site_energy_culled(const EnergyArgs &a, const CellGrid &g, const Sit
 }
 
 // ====================================================================
-// Ewald: framework structure factors (src/ewald.c:1289-1731)
-// one lane per k-vector, atoms streamed through the scalar unit
+// Ewald: framework structure factors (src/ewald.c:1289-1731): phase power
+// tables per atom, then one lane per k-vector summing over the atoms
 // ====================================================================
-__global__ __launch_bounds__(256) void k_structure_factor(const int32_t *__restrict__ kidx, int nk,
-                                                          const double *__restrict__ fx,
-                                                          const double *__restrict__ fy,
-                                                          const double *__restrict__ fz,
-                                                          const double *__restrict__ q, int n,
-                                                          double2 *__restrict__ out) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (blockIdx.x * blockDim.x >= nk) return;
-  int kk = k < nk ? k : 0;
-  double ii = kidx[3 * kk], jj = kidx[3 * kk + 1], ll = kidx[3 * kk + 2];
-  double sr = 0.0, si = 0.0;
-  for (int a = 0; a < n; ++a) {
-    // phase = 2pi (ii fx + jj fy + ll fz); fractional coordinates already *2pi
-    double ph = ii * fx[a] + jj * fy[a] + ll * fz[a];
-    double s, c;
-    sincos(ph, &s, &c);
-    sr = fma(q[a], c, sr);
-    si = fma(q[a], s, si);
-  }
-  if (k < nk) out[k] = make_double2(sr, si);
-}
 
-void launch_structure_factor(const EwaldDev &e, const int32_t *kidx, int nk, const double *fx,
-                             const double *fy, const double *fz, const double *q, int n,
-                             double2 *out, hipStream_t st) {
-  (void)e;
-  int bs = 256;
-  int nb = (nk + bs - 1) / bs;
-  if (nb) hipLaunchKernelGGL(k_structure_factor, dim3(nb), dim3(bs), 0, st, kidx, nk, fx, fy, fz, q, n, out);
-}
-
-// ====================================================================
-// Fourier Delta of inserted molecules (src/ewald.c:6859-7565), one lane
-// per molecule.  exp(ik.r) by the reference's recurrences; the loop visits
-// each (kx, |ky|, |kz|) once and forms the sign variants as conjugates,
-// reading KFactor and the framework structure factor at the reference's
-// k index (wave-uniform, scalar loads).
-// ====================================================================
+// complex products in the reference's operation order (no contraction)
 struct cx {
   double re, im;
 };
@@ -919,6 +902,85 @@ __device__ __forceinline__ cx cmul(cx a, cx b) {
   return r;
 }
 __device__ __forceinline__ cx conjc(cx a) { return cx{a.re, -a.im}; }
+
+// Per charged atom the powers e^{i ii fx} (ii = 0..kx), e^{i jj fy}
+// (jj = -ky..ky) and e^{i kk fz} (kk = -kz..kz) by the recurrences of
+// src/ewald.c:1330-1420, stored [atom][ii | jj | kk].
+__global__ __launch_bounds__(256) void k_sf_tables(int kx, int ky, int kz, const double *__restrict__ fx,
+                                                   const double *__restrict__ fy,
+                                                   const double *__restrict__ fz, int n,
+                                                   double2 *__restrict__ tab) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const int T = (kx + 1) + (2 * ky + 1) + (2 * kz + 1);
+  double2 *t = tab + (int64_t)a * T;
+  double s, c;
+  sincos(fx[a], &s, &c);
+  cx e = cx{1.0, 0.0}, e1 = cx{c, s};
+  for (int i = 0; i <= kx; ++i) {
+    t[i] = make_double2(e.re, e.im);
+    e = cmul(e, e1);
+  }
+  double2 *ty = t + (kx + 1) + ky; // index -ky..ky
+  sincos(fy[a], &s, &c);
+  e = cx{1.0, 0.0};
+  e1 = cx{c, s};
+  for (int j = 0; j <= ky; ++j) {
+    ty[j] = make_double2(e.re, e.im);
+    ty[-j] = make_double2(e.re, -e.im);
+    e = cmul(e, e1);
+  }
+  double2 *tz = t + (kx + 1) + (2 * ky + 1) + kz;
+  sincos(fz[a], &s, &c);
+  e = cx{1.0, 0.0};
+  e1 = cx{c, s};
+  for (int k = 0; k <= kz; ++k) {
+    tz[k] = make_double2(e.re, e.im);
+    tz[-k] = make_double2(e.re, -e.im);
+    e = cmul(e, e1);
+  }
+}
+
+// One lane per k-vector: S_f(k) = sum over charged atoms (in order) of
+// q e^{i ii fx} e^{i jj fy} e^{i kk fz}, from the tables above.
+__global__ __launch_bounds__(256) void k_sf_sum(const int32_t *__restrict__ kidx, int nk, int kx, int ky,
+                                                int kz, const double2 *__restrict__ tab,
+                                                const double *__restrict__ q, int n,
+                                                double2 *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nk) return;
+  const int ii = kidx[3 * k], jj = kidx[3 * k + 1], kk = kidx[3 * k + 2];
+  const int T = (kx + 1) + (2 * ky + 1) + (2 * kz + 1);
+  const int oy = (kx + 1) + ky + jj, oz = (kx + 1) + (2 * ky + 1) + kz + kk;
+  double sr = 0.0, si = 0.0;
+  for (int a = 0; a < n; ++a) {
+    const double2 *t = tab + (int64_t)a * T;
+    const double2 x = t[ii], y = t[oy], z = t[oz];
+    const cx e = cmul(cmul(cx{x.x, x.y}, cx{y.x, y.y}), cx{z.x, z.y});
+    sr = fma(q[a], e.re, sr);
+    si = fma(q[a], e.im, si);
+  }
+  out[k] = make_double2(sr, si);
+}
+
+void launch_structure_factor(const EwaldDev &e, const int32_t *kidx, int nk, const double *fx,
+                             const double *fy, const double *fz, const double *q, int n,
+                             double2 *tab, double2 *out, hipStream_t st) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_sf_tables, dim3((n + 255) / 256), dim3(256), 0, st, e.kx, e.ky, e.kz, fx, fy, fz,
+                       n, tab);
+  if (nk > 0)
+    hipLaunchKernelGGL(k_sf_sum, dim3((nk + 255) / 256), dim3(256), 0, st, kidx, nk, e.kx, e.ky, e.kz,
+                       (const double2 *)tab, q, n, out);
+}
+
+// ====================================================================
+// Fourier Delta of inserted molecules (src/ewald.c:6859-7565), one lane
+// per molecule.  exp(ik.r) by the reference's recurrences; the loop visits
+// each (kx, |ky|, |kz|) once and forms the sign variants as conjugates,
+// reading KFactor and the framework structure factor at the reference's
+// k index (wave-uniform, scalar loads).
+// ====================================================================
 
 // Partial k-sums of one kx slice (blockIdx.y = kx) for 64 molecules per
 // wave; part[kx][m] = (host, ads).  The slice's exp(i kx x) comes from the
