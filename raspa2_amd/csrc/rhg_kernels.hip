@@ -1048,34 +1048,48 @@ __global__ __launch_bounds__(128) void k_fourier_part(const FourierArgs a, doubl
         ez[t] = cx{1.0, 0.0};
       }
       const bool origin_row = (ii == 0 && jv == 0);
-      for (int kk = 0; kk <= kzm; ++kk) {
-        if (kk > 0) {
+      // k_z = 0 (absent in the origin row, where k = 0 is excluded)
+      if (!origin_row) {
+        double sre = 0.0, sim = 0.0;
 #pragma unroll
-          for (int t = 0; t < nc; ++t) ez[t] = (kk == 1) ? e1z[t] : cmul(ez[t], e1z[t]);
+        for (int t = 0; t < nc; ++t) {
+          sre += xy[t].re;
+          sim += xy[t].im;
         }
-        // the +kz and -kz structure factors share the four products
-        // Re(xy) Re(ez), Im(xy) Im(ez), Re(xy) Im(ez), Im(xy) Re(ez)
+        const int idx = off + kzm;
+        const double kf = e.kfactor[idx];
+        const double2 sf = e.sf[idx];
+        host = fma(kf, fma(sf.y, sim, sf.x * sre), host);
+        ads = fma(kf, fma(sim, sim, sre * sre), ads);
+      }
+      // k_z = +kk and -kk share the four products Re(xy) Re(ez), Im(xy) Im(ez),
+      // Re(xy) Im(ez), Im(xy) Re(ez); in the origin row the +k_z entries sit
+      // one place lower (k = 0 is not stored)
+      const int ip = off + kzm - (origin_row ? 1 : 0), im = off + kzm;
+      for (int kk = 1; kk <= kzm; ++kk) {
         double a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
 #pragma unroll
         for (int t = 0; t < nc; ++t) {
+          ez[t] = cx{fma(ez[t].re, e1z[t].re, -ez[t].im * e1z[t].im),
+                     fma(ez[t].im, e1z[t].re, ez[t].re * e1z[t].im)};
           a1 = fma(xy[t].re, ez[t].re, a1);
           a2 = fma(xy[t].im, ez[t].im, a2);
           a3 = fma(xy[t].re, ez[t].im, a3);
           a4 = fma(xy[t].im, ez[t].re, a4);
         }
-        for (int sz = 0; sz < 2; ++sz) {
-          if (kk == 0 && sz == 1) break;
-          if (origin_row && kk == 0) continue;
-          const int kv = sz ? -kk : kk;
-          int pos = kv + kzm;
-          if (origin_row && kv > 0) pos -= 1;
-          const int idx = off + pos;
-          const double sre = sz ? a1 + a2 : a1 - a2;
-          const double sim = sz ? a4 - a3 : a3 + a4;
-          const double kf = e.kfactor[idx];
-          const double2 sf = e.sf[idx];
-          host += kf * (sf.x * sre + sf.y * sim);
-          ads += kf * (sre * sre + sim * sim);
+        {
+          const double sre = a1 - a2, sim = a3 + a4; // +k_z
+          const double kf = e.kfactor[ip + kk];
+          const double2 sf = e.sf[ip + kk];
+          host = fma(kf, fma(sf.y, sim, sf.x * sre), host);
+          ads = fma(kf, fma(sim, sim, sre * sre), ads);
+        }
+        {
+          const double sre = a1 + a2, sim = a4 - a3; // -k_z
+          const double kf = e.kfactor[im - kk];
+          const double2 sf = e.sf[im - kk];
+          host = fma(kf, fma(sf.y, sim, sf.x * sre), host);
+          ads = fma(kf, fma(sim, sim, sre * sre), ads);
         }
       }
     }
