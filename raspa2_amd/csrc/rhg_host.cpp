@@ -715,7 +715,8 @@ int rhg_create(const rhg_framework *fw, const rhg_cell *cellp, const rhg_forcefi
       upload(c.get(), d_q, qq.data(), nc * sizeof(double));
     }
     upload(c.get(), d_kidx, c->kidx.data(), c->kidx.size() * sizeof(int32_t));
-    const size_t tab_n = (size_t)std::max(nc, 1) * ((c->kx + 1) + (2 * c->ky + 1) + (2 * c->kz + 1));
+    const size_t tab_n = (size_t)std::max(nc, 1) * ((c->kx + 1) + (2 * c->ky + 1) + 1);
+    if (c->kz > 64) return RHG_ERR_UNSUPPORTED; // k_z range of the row kernel
     double2 *d_tab = (double2 *)work(c.get(), "sf_tab", tab_n * sizeof(double2));
     if (!d_tab) return RHG_ERR_OOM;
     {

@@ -904,74 +904,107 @@ __device__ __forceinline__ cx cmul(cx a, cx b) {
 __device__ __forceinline__ cx conjc(cx a) { return cx{a.re, -a.im}; }
 
 // Per charged atom the powers e^{i ii fx} (ii = 0..kx), e^{i jj fy}
-// (jj = -ky..ky) and e^{i kk fz} (kk = -kz..kz) by the recurrences of
-// src/ewald.c:1330-1420, stored [atom][ii | jj | kk].
-__global__ __launch_bounds__(256) void k_sf_tables(int kx, int ky, int kz, const double *__restrict__ fx,
+// (jj = -ky..ky) and e^{i fz}, by the recurrences of src/ewald.c:1330-1420,
+// stored [power][atom] (atom fastest).
+__global__ __launch_bounds__(256) void k_sf_tables(int kx, int ky, const double *__restrict__ fx,
                                                    const double *__restrict__ fy,
                                                    const double *__restrict__ fz, int n,
                                                    double2 *__restrict__ tab) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= n) return;
-  const int T = (kx + 1) + (2 * ky + 1) + (2 * kz + 1);
-  double2 *t = tab + (int64_t)a * T;
   double s, c;
   sincos(fx[a], &s, &c);
   cx e = cx{1.0, 0.0}, e1 = cx{c, s};
   for (int i = 0; i <= kx; ++i) {
-    t[i] = make_double2(e.re, e.im);
+    tab[(int64_t)i * n + a] = make_double2(e.re, e.im);
     e = cmul(e, e1);
   }
-  double2 *ty = t + (kx + 1) + ky; // index -ky..ky
+  const int y0 = kx + 1 + ky; // row of jj = 0
   sincos(fy[a], &s, &c);
   e = cx{1.0, 0.0};
   e1 = cx{c, s};
   for (int j = 0; j <= ky; ++j) {
-    ty[j] = make_double2(e.re, e.im);
-    ty[-j] = make_double2(e.re, -e.im);
+    tab[(int64_t)(y0 + j) * n + a] = make_double2(e.re, e.im);
+    tab[(int64_t)(y0 - j) * n + a] = make_double2(e.re, -e.im);
     e = cmul(e, e1);
   }
-  double2 *tz = t + (kx + 1) + (2 * ky + 1) + kz;
   sincos(fz[a], &s, &c);
-  e = cx{1.0, 0.0};
-  e1 = cx{c, s};
-  for (int k = 0; k <= kz; ++k) {
-    tz[k] = make_double2(e.re, e.im);
-    tz[-k] = make_double2(e.re, -e.im);
-    e = cmul(e, e1);
-  }
+  tab[(int64_t)(kx + 2 * ky + 2) * n + a] = make_double2(c, s);
 }
 
-// One lane per k-vector: S_f(k) = sum over charged atoms (in order) of
-// q e^{i ii fx} e^{i jj fy} e^{i kk fz}, from the tables above.
-__global__ __launch_bounds__(256) void k_sf_sum(const int32_t *__restrict__ kidx, int nk, int kx, int ky,
-                                                int kz, const double2 *__restrict__ tab,
-                                                const double *__restrict__ q, int n,
-                                                double2 *__restrict__ out) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nk) return;
-  const int ii = kidx[3 * k], jj = kidx[3 * k + 1], kk = kidx[3 * k + 2];
-  const int T = (kx + 1) + (2 * ky + 1) + (2 * kz + 1);
-  const int oy = (kx + 1) + ky + jj, oz = (kx + 1) + (2 * ky + 1) + kz + kk;
-  double sr = 0.0, si = 0.0;
-  for (int a = 0; a < n; ++a) {
-    const double2 *t = tab + (int64_t)a * T;
-    const double2 x = t[ii], y = t[oy], z = t[oz];
-    const cx e = cmul(cmul(cx{x.x, x.y}, cx{y.x, y.y}), cx{z.x, z.y});
-    sr = fma(q[a], e.re, sr);
-    si = fma(q[a], e.im, si);
+// One workgroup per (k_x, k_y) row of the k table: threads take atoms, form
+// q e^{i(k_x x + k_y y)} from the tables and run the k_z recurrence; the
+// +k_z and -k_z structure factors share four products per atom, which are
+// reduced over the workgroup in a fixed order (wave shuffles, then the four
+// waves in order, atom chunks in order).
+constexpr int kSfMaxKz = 64;
+
+__global__ __launch_bounds__(256) void k_sf_rows(const EwaldDev e, const double2 *__restrict__ tab,
+                                                 const double *__restrict__ q, int n,
+                                                 double2 *__restrict__ out) {
+  __shared__ double part[4][kSfMaxKz + 1][4];
+  __shared__ double acc[kSfMaxKz + 1][4];
+  const int ny = 2 * e.ky + 1;
+  const int row = blockIdx.x;
+  const int ii = row / ny, jv = row % ny - e.ky;
+  const int kzm = e.row_kz[row];
+  if (kzm < 0) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int t = threadIdx.x; t < (kzm + 1) * 4; t += blockDim.x) acc[t >> 2][t & 3] = 0.0;
+  __syncthreads();
+  for (int a0 = 0; a0 < n; a0 += blockDim.x) {
+    const int a = a0 + threadIdx.x;
+    cx xy = cx{0.0, 0.0}, e1z = cx{1.0, 0.0};
+    if (a < n) {
+      const double2 tx = tab[(int64_t)ii * n + a];
+      const double2 ty = tab[(int64_t)(e.kx + 1 + e.ky + jv) * n + a];
+      const double2 tz = tab[(int64_t)(e.kx + 2 * e.ky + 2) * n + a];
+      const cx r = cmul(cx{tx.x, tx.y}, cx{ty.x, ty.y});
+      xy = cx{q[a] * r.re, q[a] * r.im};
+      e1z = cx{tz.x, tz.y};
+    }
+    cx ez = cx{1.0, 0.0};
+    for (int kk = 0; kk <= kzm; ++kk) {
+      if (kk > 0) ez = cx{fma(ez.re, e1z.re, -ez.im * e1z.im), fma(ez.im, e1z.re, ez.re * e1z.im)};
+      double v[4] = {xy.re * ez.re, xy.im * ez.im, xy.re * ez.im, xy.im * ez.re};
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        for (int d = 32; d > 0; d >>= 1) v[c] += __shfl_down(v[c], d);
+      if (lane == 0)
+        for (int c = 0; c < 4; ++c) part[wave][kk][c] = v[c];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < (kzm + 1) * 4; t += blockDim.x) {
+      const int kk = t >> 2, c = t & 3;
+      acc[kk][c] += ((part[0][kk][c] + part[1][kk][c]) + part[2][kk][c]) + part[3][kk][c];
+    }
+    __syncthreads();
   }
-  out[k] = make_double2(sr, si);
+  // S(+k_z) = (a1 - a2, a3 + a4), S(-k_z) = (a1 + a2, a4 - a3); in the
+  // origin row k = 0 is not stored and the +k_z entries sit one place lower
+  const bool origin = (ii == 0 && jv == 0);
+  const int off = e.row_off[row];
+  for (int kk = threadIdx.x; kk <= kzm; kk += blockDim.x) {
+    const double a1 = acc[kk][0], a2 = acc[kk][1], a3 = acc[kk][2], a4 = acc[kk][3];
+    if (kk == 0) {
+      if (!origin) out[off + kzm] = make_double2(a1 - a2, a3 + a4);
+      continue;
+    }
+    out[off + kzm + kk - (origin ? 1 : 0)] = make_double2(a1 - a2, a3 + a4);
+    out[off + kzm - kk] = make_double2(a1 + a2, a4 - a3);
+  }
 }
 
 void launch_structure_factor(const EwaldDev &e, const int32_t *kidx, int nk, const double *fx,
                              const double *fy, const double *fz, const double *q, int n,
                              double2 *tab, double2 *out, hipStream_t st) {
+  (void)kidx;
+  (void)nk;
   if (n > 0)
-    hipLaunchKernelGGL(k_sf_tables, dim3((n + 255) / 256), dim3(256), 0, st, e.kx, e.ky, e.kz, fx, fy, fz,
-                       n, tab);
-  if (nk > 0)
-    hipLaunchKernelGGL(k_sf_sum, dim3((nk + 255) / 256), dim3(256), 0, st, kidx, nk, e.kx, e.ky, e.kz,
-                       (const double2 *)tab, q, n, out);
+    hipLaunchKernelGGL(k_sf_tables, dim3((n + 255) / 256), dim3(256), 0, st, e.kx, e.ky, fx, fy, fz, n,
+                       tab);
+  const int rows = (e.kx + 1) * (2 * e.ky + 1);
+  hipLaunchKernelGGL(k_sf_rows, dim3(rows), dim3(256), 0, st, e, (const double2 *)tab, q, n, out);
 }
 
 // ====================================================================
