@@ -94,6 +94,10 @@ Cell make_cell(const double *box, int bc) {
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
+  // buffers outgrown while work may still be queued on them; freed with the
+  // owner (hipFree waits for the whole device, which would stall the other
+  // streams of a server)
+  std::vector<void *> retired;
 };
 
 struct PseudoAtom {
@@ -229,7 +233,7 @@ void *work(rhg_ctx *c, const char *name, size_t bytes) {
   if (b.p) {
     // regrowth: leave headroom so that a slowly growing size settles
     want = std::max(want, b.bytes + b.bytes / 2);
-    hipFree(b.p);
+    b.retired.push_back(b.p);
   }
   b.p = nullptr;
   b.bytes = 0;
@@ -743,7 +747,10 @@ void rhg_destroy(rhg_ctx *c) {
     hipFree(t.c384);
     hipFree(t.par);
   }
-  for (auto &b : c->bufs) hipFree(b.second.p);
+  for (auto &b : c->bufs) {
+    hipFree(b.second.p);
+    for (void *r : b.second.retired) hipFree(r);
+  }
   hipFree(c->d_kfactor);
   hipFree(c->d_sf);
   hipFree(c->d_nb_off);
@@ -1819,7 +1826,10 @@ void rhg_workspace_destroy(rhg_workspace *ws) {
     hipStreamSynchronize(ws->owner->stream);
     ws->owner->ws = nullptr;
   }
-  for (auto &b : ws->bufs) hipFree(b.second.p);
+  for (auto &b : ws->bufs) {
+    hipFree(b.second.p);
+    for (void *r : b.second.retired) hipFree(r);
+  }
   delete ws;
 }
 
