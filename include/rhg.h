@@ -32,6 +32,7 @@ extern "C" {
 #define RHG_ERR_NOT_ADMITTED (-4)
 #define RHG_ERR_UNSUPPORTED (-5)
 #define RHG_ERR_FAULT_INJECTED (-6)
+#define RHG_ERR_IO (-7)
 
 /* BoundaryCondition values (src/potentials.h:232-271) */
 #define RHG_BC_RECTANGULAR 1
@@ -236,12 +237,42 @@ int rhg_grid_info(const rhg_grid *g, double out[13]);
 int rhg_grid_download(const rhg_grid *g, void *host, size_t bytes);
 /* REAL InterpolateVDWGrid(int,VECTOR) (src/grids.h:70, src/grids.c:979) */
 int rhg_interpolate_vdw_grid(rhg_grid *g, int64_t n, const double *xyz, double *u);
+/* Grids run on their context's stream: use them only while it lives.
+ * rhg_grid_free itself needs no live context. */
 void rhg_grid_free(rhg_grid *g);
 /* void CalculateDerivativesAtPositionVDW(VECTOR,int,REAL*,VECTOR*,REAL_MATRIX3x3*,REAL*)
  * (src/framework_hessian.h:43, src/framework_hessian.c:63): out n x 14
  * (U, dU[3], H[9] in ax..cz order, third). */
 int rhg_derivatives_at_positions_vdw(rhg_ctx *ctx, int32_t type, int64_t n, const double *xyz,
                                      double *out);
+/* MakeGrid's Ewald-Coulomb part (src/grids.c:587-689): the real-space Ewald
+ * potential of a unit charge (pseudo-atom "UNIT"), capped where it exceeds
+ * the overlap criterion or an atom is closer than 1 A.  Needs the Ewald
+ * charge method (other methods: RHG_ERR_UNSUPPORTED). */
+int rhg_make_coulomb_grid(rhg_ctx *ctx, const double unit_box[9], double spacing, int32_t fp64,
+                          rhg_grid **out);
+/* REAL InterpolateCoulombGrid(int,VECTOR) (src/grids.c:1200-1295): the
+ * tricubic value (no overlap short-cut) times the probe charge */
+int rhg_interpolate_coulomb_grid(rhg_grid *g, int64_t n, const double *xyz, double charge, double *u);
+/* REAL CalculateDerivativesAtPositionReal(VECTOR,int,REAL*,VECTOR*,REAL_MATRIX3x3*,REAL*)
+ * (src/framework_hessian.c:111-175) for a unit charge: out n x 14 as the
+ * VDW form, smallest_r[n] its return value (DBL_MAX when no atom is in range). */
+int rhg_derivatives_at_positions_real(rhg_ctx *ctx, int64_t n, const double *xyz, double *out,
+                                      double *smallest_r);
+/* WriteVDWGrid / WriteCoulombGrid (src/grids.c:692-742, 815-876): RASPA2's
+ * binary .grid file of an FP32 grid.  n_unit_cells null writes the number of
+ * unit cells of the context's box.  Errors: RHG_ERR_IO. */
+int rhg_grid_write(const rhg_grid *g, const char *path, const int32_t n_unit_cells[3]);
+/* ReadVDWGrid / ReadCoulombGrid (src/grids.c:745-813, 879-975): a .grid file
+ * onto the device of ctx.  The unit cell's vector lengths must match the
+ * file's within 1e-8 A and, for a Coulomb grid with n_unit_cells given, the
+ * number of unit cells exactly (RHG_ERR_INVALID).  A file whose length does
+ * not match its header is RHG_ERR_IO. */
+int rhg_grid_read(rhg_ctx *ctx, const char *path, int32_t coulomb, const double unit_box[9],
+                  const int32_t n_unit_cells[3], rhg_grid **out);
+/* spacing, unit-cell lengths[3], number of unit cells[3], coulomb flag,
+ * Ewald precision of a Coulomb grid (-1 otherwise) */
+int rhg_grid_meta(const rhg_grid *g, double out[9]);
 
 /* ------------------------------------------------------------- profiling */
 /* HIP-event timing of the kernels launched on the context's stream.

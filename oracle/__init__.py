@@ -99,8 +99,11 @@ def _declare(L):
     f("orc_grid_node", None, _P, _I, _P, _I, _I, _I, _P)
     f("orc_lm_coeff", None, _P)
     f("orc_interpolate_grid", _D, _P, _P, _I, _P, _P, _P)
+    f("orc_interpolate_coulomb_grid", _D, _P, _P, _I, _P, _P, _P, _D)
     f("orc_interpolate_direct", _D, _P, _I, _P, _I, _P, _P, _P)
     f("orc_set_block_pockets", _I, _P, _I, _P, _P, _P, _I)
+    f("orc_derivatives_real_at", _D, _P, _P, _P)
+    f("orc_coulomb_grid_node", None, _P, _P, _I, _I, _I, _P)
     f("orc_blocked_pocket", _I, _P, _P)
 
 
@@ -256,6 +259,17 @@ class System:
         geom, ub, ui, p = _f64(geom), _f64(unit_box), _f64(unit_inv), _f64(pos)
         return lib().orc_interpolate_direct(self.h, t, _ptr(geom), bc, _ptr(ub), _ptr(ui), _ptr(p))
 
+    def derivatives_real_at(self, pos):
+        """(out[14], smallest_r) of the real-space potential of a unit charge."""
+        out = np.zeros(14)
+        r = lib().orc_derivatives_real_at(self.h, _ptr(_f64(pos)), _ptr(out))
+        return out, r
+
+    def coulomb_grid_node(self, geom, i, j, k):
+        out = np.zeros(8, dtype=np.float32)
+        lib().orc_coulomb_grid_node(self.h, _ptr(_f64(geom)), i, j, k, _ptr(out))
+        return out
+
     def grid_node(self, t, geom, i, j, k):
         g = _f64(geom)
         out = np.zeros(8, dtype=np.float32)
@@ -349,3 +363,10 @@ def interpolate_grid(grid, geom, bc, unit_box, unit_inv, pos):
     g = np.ascontiguousarray(grid, dtype=np.float32)
     geom, ub, ui, p = _f64(geom), _f64(unit_box), _f64(unit_inv), _f64(pos)
     return lib().orc_interpolate_grid(_ptr(g), _ptr(geom), bc, _ptr(ub), _ptr(ui), _ptr(p))
+
+
+def interpolate_coulomb_grid(grid, geom, bc, unit_box, unit_inv, pos, charge):
+    g = np.ascontiguousarray(grid, dtype=np.float32)
+    geom, ub, ui, p = _f64(geom), _f64(unit_box), _f64(unit_inv), _f64(pos)
+    return lib().orc_interpolate_coulomb_grid(_ptr(g), _ptr(geom), bc, _ptr(ub), _ptr(ui), _ptr(p),
+                                              float(charge))

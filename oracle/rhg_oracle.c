@@ -1387,6 +1387,90 @@ void orc_grid_node(const orc_system *s, int type, const double g[12], int i, int
   out[7] = (float)((double)(float)third * (dx * dy * dz));
 }
 
+/* CalculateDerivativesAtPositionReal (src/framework_hessian.c:111-175): the
+ * real-space Ewald potential of a unit charge and its derivatives, in the
+ * same order as out[] of orc_derivatives_vdw_at.  Returns the smallest
+ * distance to an atom within the cut-off. */
+double orc_derivatives_real_at(const orc_system *s, const double pos[3], double out[14]) {
+  double value = 0, fx = 0, fy = 0, fz = 0;
+  double h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double third = 0.0;
+  const double C = s->u.C, alpha = s->alpha, ChargeA = 1.0;
+  double smallest_r = DBL_MAX;
+  for (int i = 0; i < s->n; i++) {
+    double ChargeB = s->q[i];
+    double dr[3] = {pos[0] - s->x[i], pos[1] - s->y[i], pos[2] - s->z[i]};
+    orc_min_image(s->box, s->inv, s->bc, dr);
+    double rr = SQR(dr[0]) + SQR(dr[1]) + SQR(dr[2]);
+    if (rr < s->rc_charge2) {
+      double r = sqrt(rr);
+      if (r < smallest_r) smallest_r = r;
+      double F = C * ChargeA * ChargeB * (erfc(alpha * r) / r);
+      double DF = -C * ChargeA * ChargeB *
+                  (2.0 * alpha * r * exp(-SQR(alpha) * rr) / sqrt(M_PI) + erfc(alpha * r)) / (r * rr);
+      double DDF = ChargeA * ChargeB * C *
+                   (2.0 * alpha * exp(-SQR(alpha) * rr) * r * (3.0 + 2.0 * SQR(alpha) * rr) / sqrt(M_PI) +
+                    3.0 * erfc(alpha * r)) /
+                   (rr * rr * r);
+      double DDDF = ChargeA * ChargeB * C *
+                    (-2.0 * alpha * exp(-SQR(alpha) * rr) * r *
+                         (15.0 + 10.0 * SQR(alpha) * rr + 4.0 * SQR(SQR(alpha) * rr)) / sqrt(M_PI) -
+                     15.0 * erfc(alpha * r)) /
+                    (rr * rr * rr * r);
+      double dx = dr[0], dy = dr[1], dz = dr[2];
+      value += F;
+      fx += dx * DF;
+      fy += dy * DF;
+      fz += dz * DF;
+      h[AX] += DDF * dx * dx + DF; h[BX] += DDF * dy * dx;      h[CX] += DDF * dz * dx;
+      h[AY] += DDF * dx * dy;      h[BY] += DDF * dy * dy + DF; h[CY] += DDF * dz * dy;
+      h[AZ] += DDF * dx * dz;      h[BZ] += DDF * dy * dz;      h[CZ] += DDF * dz * dz + DF;
+      third += DDDF * dx * dy * dz;
+    }
+  }
+  out[0] = value;
+  out[1] = fx;
+  out[2] = fy;
+  out[3] = fz;
+  for (int i = 0; i < 9; i++) out[4 + i] = h[i];
+  out[13] = third;
+  return smallest_r;
+}
+
+/* A node of the Ewald-Coulomb grid (src/grids.c:634-683): capped when the
+ * energy exceeds the overlap criterion or an atom is closer than 1 A. */
+void orc_coulomb_grid_node(const orc_system *s, const double g[12], int i, int j, int k, float out[8]) {
+  double pos[3];
+  double d[14];
+  pos[0] = i * g[3] / (int)g[0] + g[6];
+  pos[1] = j * g[4] / (int)g[1] + g[7];
+  pos[2] = k * g[5] / (int)g[2] + g[8];
+  double smallest_r = orc_derivatives_real_at(s, pos, d);
+  double value = d[0];
+  double f[3] = {d[1], d[2], d[3]};
+  double hay = d[4 + AY], haz = d[4 + AZ], hbz = d[4 + BZ];
+  double third = d[13];
+  const double crit = ENERGY_OVERLAP_CRITERIA;
+  if (value > crit || smallest_r < 1.0) {
+    value = 2.0 * crit;
+    for (int a = 0; a < 3; a++) {
+      if (f[a] > crit) f[a] = crit;
+      if (f[a] < -crit) f[a] = -crit;
+    }
+    hay = haz = hbz = 0.0;
+    third = 0.0;
+  }
+  double dx = g[9], dy = g[10], dz = g[11];
+  out[0] = (float)value;
+  out[1] = (float)((double)(float)f[0] * dx);
+  out[2] = (float)((double)(float)f[1] * dy);
+  out[3] = (float)((double)(float)f[2] * dz);
+  out[4] = (float)((double)(float)hay * (dx * dy));
+  out[5] = (float)((double)(float)haz * (dx * dz));
+  out[6] = (float)((double)(float)hbz * (dy * dz));
+  out[7] = (float)((double)(float)third * (dx * dy * dz));
+}
+
 /* Lekien & Marsden (2005): the 64x64 matrix mapping corner values and
  * derivatives (f, fx, fy, fz, fxy, fxz, fyz, fxyz at the 8 corners, row
  * corner+8*derivative) to the coefficients a[i+4j+16k] of x^i y^j z^k.
@@ -1471,8 +1555,10 @@ static void grid_locate(const double g[12], int bc, const double ub[9], const do
 }
 
 /* Tricubic value from the 8 channels of the 8 corners, X[corner + 8 *
- * channel], with the Lekien-Marsden matrix (src/grids.c:1014-1077). */
-static double grid_tricubic(const double X[64], const double r[3]) {
+ * channel], with the Lekien-Marsden matrix (src/grids.c:1014-1077); cap
+ * selects the overlap short-cut of the VDW grid (src/grids.c:1058-1060),
+ * which InterpolateCoulombGrid does not have. */
+static double grid_tricubic_cap(const double X[64], const double r[3], int cap) {
   static int Coeff[64 * 64];
   static int have = 0;
   if (!have) {
@@ -1480,8 +1566,9 @@ static double grid_tricubic(const double X[64], const double r[3]) {
     have = 1;
   }
   double a[64];
-  for (int i = 0; i < 8; i++)
-    if (X[i] > 1e10) return 1e8;
+  if (cap)
+    for (int i = 0; i < 8; i++)
+      if (X[i] > 1e10) return 1e8;
   for (int i = 0; i < 64; i++) {
     a[i] = (double)(0.0);
     for (int j = 0; j < 64; j++) a[i] += Coeff[i * 64 + j] * X[j];
@@ -1491,6 +1578,25 @@ static double grid_tricubic(const double X[64], const double r[3]) {
     for (int j = 0; j < 4; j++)
       for (int k = 0; k < 4; k++) value += a[i + 4 * j + 16 * k] * pow(r[0], i) * pow(r[1], j) * pow(r[2], k);
   return value;
+}
+
+static double grid_tricubic(const double X[64], const double r[3]) { return grid_tricubic_cap(X, r, 1); }
+
+/* InterpolateCoulombGrid (src/grids.c:1200-1295): the same wrap and
+ * tricubic without the overlap short-cut, times the probe charge. */
+double orc_interpolate_coulomb_grid(const float *grid, const double g[12], int bc, const double ub[9],
+                                    const double uinv[9], const double pos_in[3], double charge) {
+  int x[3];
+  double r[3];
+  grid_locate(g, bc, ub, uinv, pos_in, x, r);
+  int ny = (int)g[1] + 1, nz = (int)g[2] + 1;
+  double X[64];
+  for (int c = 0; c < 8; c++) {
+    const float *node = grid + 8 * (((size_t)(x[0] + (c & 1)) * ny + (x[1] + ((c >> 1) & 1))) * nz +
+                                    (x[2] + ((c >> 2) & 1)));
+    for (int i = 0; i < 8; i++) X[c + i * 8] = node[i];
+  }
+  return charge * grid_tricubic_cap(X, r, 0);
 }
 
 double orc_interpolate_grid(const float *grid, const double g[12], int bc, const double ub[9],

@@ -194,6 +194,14 @@ void orc_derivatives_vdw_at(const orc_system *s, int type, const double pos[3], 
 /* one stored node of MakeGrid (capped, scaled, FP32) */
 void orc_grid_node(const orc_system *s, int type, const double geom[12], int i, int j, int k,
                    float out[8]);
+/* CalculateDerivativesAtPositionReal (src/framework_hessian.c:111) for a
+ * unit charge; returns the smallest in-range distance */
+double orc_derivatives_real_at(const orc_system *s, const double pos[3], double out[14]);
+/* InterpolateCoulombGrid (src/grids.c:1200-1295) */
+double orc_interpolate_coulomb_grid(const float *grid, const double geom[12], int bc, const double ub[9],
+                                    const double uinv[9], const double pos[3], double charge);
+/* Ewald-Coulomb grid node (src/grids.c:634-683) */
+void orc_coulomb_grid_node(const orc_system *s, const double geom[12], int i, int j, int k, float out[8]);
 /* Lekien-Marsden 64x64 matrix generated from its definition */
 void orc_lm_coeff(int out[64 * 64]);
 /* InterpolateVDWGrid with an explicit full grid in [i][j][k][8] float layout */

@@ -142,7 +142,7 @@ struct rhg_ctx {
   int precision = RHG_PREC_FP64;
   std::vector<PseudoAtom> types;
   // Ewald
-  double alpha = 0.0, uion = 0.0, net_q_fw = 0.0, recip_cutoff = 0.0;
+  double alpha = 0.0, uion = 0.0, net_q_fw = 0.0, recip_cutoff = 0.0, ewald_precision = 0.0;
   int kx = 0, ky = 0, kz = 0, nk = 0;
   std::vector<double> kfactor;
   std::vector<int32_t> kidx, row_off, row_kz;
@@ -192,11 +192,16 @@ struct rhg_ctx {
 
 struct rhg_grid {
   rhg_ctx *ctx = nullptr;
+  int device = 0;                // freeing needs no live context
   int n[3];
   double size[3], shift[3], delta[3];
   double ubox[9], uinv[9], ulen[3];
   int bc;
   int fp64;
+  int coulomb = 0;               // Ewald-Coulomb grid of a unit charge
+  double spacing = 0.0;          // SpacingVDWGrid / SpacingCoulombGrid
+  double ewald_precision = -1.0; // of a Coulomb grid
+  int n_cells[3] = {1, 1, 1};    // NumberOfUnitCells of the system
   void *d = nullptr;
   size_t bytes = 0;
 };
@@ -307,6 +312,7 @@ double lj_ref(double arg1, double sigma, double arg3, double rr) {
 // InitializeEwald (automatic), SetupKVectors, EwaldEnergyIon
 // (src/ewald.c:200-260, 532-600, 615-742)
 void ewald_setup(rhg_ctx *c, double precision) {
+  c->ewald_precision = precision;
   double eps = std::min(fabs(precision), 0.5);
   double rc = c->rc_cc;
   double tol = sqrt(fabs(log(eps * rc)));
@@ -1525,16 +1531,11 @@ int rhg_ewald_fourier_adsorbate(rhg_ctx *c, int64_t n_mol, int32_t n_sites, cons
 }
 
 // ------------------------------------------------------------------ grids
-int rhg_make_grid(rhg_ctx *c, int32_t type, const double unit_box[9], double spacing, int32_t fp64,
-                  rhg_grid **out) {
-  if (!c || !unit_box || !out || !(spacing > 0.0)) return RHG_ERR_INVALID;
-  if (int rc = check_type(c, type)) return rc;
-  *out = nullptr;
-  // src/grids.c:441-449
-  if (std::min(c->props[3], std::min(c->props[4], c->props[5])) < c->rc_vdw)
-    return fail(c, RHG_ERR_INVALID, "cutoff larger than a perpendicular box width");
-  std::unique_ptr<rhg_grid> g(new rhg_grid());
-  g->ctx = c;
+namespace {
+
+// Grid geometry of MakeGrid (src/grids.c:460-498, 587-598): the box that
+// encloses the unit cell, N = (int)(size / spacing) made odd, Delta = size / N
+void grid_geometry(rhg_grid *g, const rhg_ctx *c, const double unit_box[9], double spacing) {
   double size[3] = {0.0, 0.0, 0.0}, shift[3] = {0.0, 0.0, 0.0};
   const int col[3][3] = {{AX, AY, AZ}, {BX, BY, BZ}, {CX, CY, CZ}};
   for (int v = 0; v < 3; v++)
@@ -1543,26 +1544,27 @@ int rhg_make_grid(rhg_ctx *c, int32_t type, const double unit_box[9], double spa
       if (unit_box[col[v][d]] < 0.0) shift[d] += unit_box[col[v][d]];
     }
   for (int d = 0; d < 3; d++) {
-    g->n[d] = (int)(size[d] / spacing); // src/grids.c:486-492
+    g->n[d] = (int)(size[d] / spacing);
     if (g->n[d] % 2 == 0) g->n[d]++;
     g->size[d] = size[d];
     g->shift[d] = shift[d];
     g->delta[d] = size[d] / (double)g->n[d];
   }
+  g->spacing = spacing;
   memcpy(g->ubox, unit_box, 9 * sizeof(double));
   invert3x3(g->ubox, g->uinv);
   double up[7];
   cell_properties(g->ubox, up);
-  g->ulen[0] = up[0];
-  g->ulen[1] = up[1];
-  g->ulen[2] = up[2];
+  for (int d = 0; d < 3; d++) {
+    g->ulen[d] = up[d];
+    g->n_cells[d] = std::max(1, (int)lround(c->props[d] / up[d]));
+  }
   g->bc = c->cell.bc;
-  g->fp64 = fp64 ? 1 : 0;
-  int64_t nodes = (int64_t)(g->n[0] + 1) * (g->n[1] + 1) * (g->n[2] + 1);
-  g->bytes = (size_t)nodes * 8 * (g->fp64 ? sizeof(double) : sizeof(float));
-  hipSetDevice(c->device);
-  if (hipMalloc(&g->d, g->bytes) != hipSuccess) return fail(c, RHG_ERR_OOM, "grid allocation");
-  const PseudoAtom &t = c->types[type];
+}
+
+int64_t grid_nodes(const rhg_grid *g) { return (int64_t)(g->n[0] + 1) * (g->n[1] + 1) * (g->n[2] + 1); }
+
+GridArgs grid_args(const rhg_ctx *c, const rhg_grid *g) {
   GridArgs a;
   memset(&a, 0, sizeof(a));
   a.cell = c->cell;
@@ -1572,22 +1574,36 @@ int rhg_make_grid(rhg_ctx *c, int32_t type, const double unit_box[9], double spa
     a.shift[d] = g->shift[d];
     a.delta[d] = g->delta[d];
   }
-  a.rc2 = c->rc2_vdw;
   a.n_atoms = c->n;
-  a.a64 = t.d64;
-  a.c24 = t.c24;
-  a.c96 = t.c96;
-  a.c384 = t.c384;
   a.fp64 = g->fp64;
   a.out = g->d;
+  return a;
+}
+
+// framework atoms as {x, y, z, q} for the real-space Coulomb kernels
+double4 *atoms_xyzq(rhg_ctx *c) {
+  double4 *d = (double4 *)work(c, "aq", (size_t)std::max(c->n, 1) * sizeof(double4));
+  if (!d) return nullptr;
+  std::vector<double4> h((size_t)std::max(c->n, 1));
+  for (int i = 0; i < c->n; ++i) h[i] = make_double4(c->x[i], c->y[i], c->z[i], c->q[i]);
+  upload(c, d, h.data(), h.size() * sizeof(double4));
+  return d;
+}
+
+// Every node of the grid, in launches of at most 2^22 nodes.  With cell
+// lists on, the nodes of a launch are binned into the cell grid like trial
+// sites and each tile walks its cell's lists; precision 1 (VDW grids only)
+// uses the image lists.
+int build_nodes(rhg_ctx *c, rhg_grid *g, GridArgs &a, const float4 *par) {
+  const int64_t nodes = grid_nodes(g);
   const int64_t step = (int64_t)1 << 22;
   const bool culled = c->cull && c->n > 0;
   if (culled) {
     if (int rc = ensure_cell_grid(c)) return rc;
-    if (c->precision != RHG_PREC_FP64 && c->use_images)
+    if (!a.coulomb && c->precision != RHG_PREC_FP64 && c->use_images)
       if (int rc = ensure_image_list(c)) return rc;
   }
-  const int prec = (c->precision != RHG_PREC_FP64 && c->use_images && c->cg.img_off) ? 1 : 0;
+  const int prec = (!a.coulomb && c->precision != RHG_PREC_FP64 && c->use_images && c->cg.img_off) ? 1 : 0;
   for (int64_t n0 = 0; n0 < nodes; n0 += step) {
     a.node0 = n0;
     a.n_nodes = std::min(step, nodes - n0);
@@ -1596,8 +1612,6 @@ int rhg_make_grid(rhg_ctx *c, int32_t type, const double unit_box[9], double spa
       launch_grid_build(a, c->stream);
       continue;
     }
-    // nodes are binned into the cell grid like trial sites, then each tile
-    // walks its cell's lists
     const size_t nb = (size_t)step * sizeof(double);
     double *gx = (double *)work(c, "g_x", nb), *gy = (double *)work(c, "g_y", nb),
            *gz = (double *)work(c, "g_z", nb);
@@ -1618,10 +1632,87 @@ int rhg_make_grid(rhg_ctx *c, int32_t type, const double unit_box[9], double spa
       return fail(c, RHG_ERR_OOM, "grid binning scratch");
     launch_bin_sites(c->cell, c->cg, sl, b, c->stream);
     Timed tm(c, "grid_build");
-    launch_grid_culled(a, c->cg, sl, b, t.par, prec, c->stream);
+    launch_grid_culled(a, c->cg, sl, b, par, prec, c->stream);
   }
   HIPC(c, hipGetLastError());
   HIPC(c, hipStreamSynchronize(c->stream));
+  return RHG_OK;
+}
+
+int grid_alloc(rhg_ctx *c, rhg_grid *g) {
+  g->bytes = (size_t)grid_nodes(g) * 8 * (g->fp64 ? sizeof(double) : sizeof(float));
+  hipSetDevice(c->device);
+  if (hipMalloc(&g->d, g->bytes) != hipSuccess) {
+    g->d = nullptr;
+    return fail(c, RHG_ERR_OOM, "grid allocation");
+  }
+  return RHG_OK;
+}
+
+} // namespace
+
+int rhg_make_grid(rhg_ctx *c, int32_t type, const double unit_box[9], double spacing, int32_t fp64,
+                  rhg_grid **out) {
+  if (!c || !unit_box || !out || !(spacing > 0.0)) return RHG_ERR_INVALID;
+  if (int rc = check_type(c, type)) return rc;
+  *out = nullptr;
+  // src/grids.c:441-449
+  if (std::min(c->props[3], std::min(c->props[4], c->props[5])) < c->rc_vdw)
+    return fail(c, RHG_ERR_INVALID, "cutoff larger than a perpendicular box width");
+  std::unique_ptr<rhg_grid> g(new rhg_grid());
+  g->ctx = c;
+  g->device = c->device;
+  grid_geometry(g.get(), c, unit_box, spacing);
+  g->fp64 = fp64 ? 1 : 0;
+  if (int rc = grid_alloc(c, g.get())) return rc;
+  const PseudoAtom &t = c->types[type];
+  GridArgs a = grid_args(c, g.get());
+  a.rc2 = c->rc2_vdw;
+  a.a64 = t.d64;
+  a.c24 = t.c24;
+  a.c96 = t.c96;
+  a.c384 = t.c384;
+  if (int rc = build_nodes(c, g.get(), a, t.par)) {
+    rhg_grid_free(g.release());
+    return rc;
+  }
+  *out = g.release();
+  return RHG_OK;
+}
+
+int rhg_make_coulomb_grid(rhg_ctx *c, const double unit_box[9], double spacing, int32_t fp64,
+                          rhg_grid **out) {
+  if (!c || !unit_box || !out || !(spacing > 0.0)) return RHG_ERR_INVALID;
+  *out = nullptr;
+  // src/grids.c:600-614: Ewald only here (Wolf's method is not a charge
+  // method of this library)
+  if (c->charge_method != RHG_CHARGE_EWALD)
+    return fail(c, RHG_ERR_UNSUPPORTED, "the Coulomb grid needs the Ewald charge method");
+  if (std::min(c->props[3], std::min(c->props[4], c->props[5])) < c->rc_cc)
+    return fail(c, RHG_ERR_INVALID, "cutoff larger than a perpendicular box width");
+  std::unique_ptr<rhg_grid> g(new rhg_grid());
+  g->ctx = c;
+  g->device = c->device;
+  grid_geometry(g.get(), c, unit_box, spacing);
+  g->fp64 = fp64 ? 1 : 0;
+  g->coulomb = 1;
+  g->ewald_precision = c->ewald_precision;
+  if (int rc = grid_alloc(c, g.get())) return rc;
+  GridArgs a = grid_args(c, g.get());
+  a.rc2 = c->rc2_cc;
+  a.coulomb = 1;
+  a.cc = c->u.C;
+  a.alpha = c->alpha;
+  a.sqrtpi = sqrt(M_PI);
+  a.aq = atoms_xyzq(c);
+  if (!a.aq) {
+    rhg_grid_free(g.release());
+    return fail(c, RHG_ERR_OOM, "atom buffer");
+  }
+  if (int rc = build_nodes(c, g.get(), a, nullptr)) {
+    rhg_grid_free(g.release());
+    return rc;
+  }
   *out = g.release();
   return RHG_OK;
 }
@@ -1645,9 +1736,8 @@ int rhg_grid_download(const rhg_grid *g, void *host, size_t bytes) {
   return RHG_OK;
 }
 
-int rhg_interpolate_vdw_grid(rhg_grid *g, int64_t n, const double *xyz, double *u) {
-  if (!g || n < 0 || (n > 0 && (!xyz || !u))) return RHG_ERR_INVALID;
-  if (n == 0) return RHG_OK;
+namespace {
+int interpolate(rhg_grid *g, int64_t n, const double *xyz, double charge, double *u) {
   rhg_ctx *c = g->ctx;
   hipSetDevice(c->device);
   double *dx = (double *)work(c, "i_xyz", (size_t)n * 3 * sizeof(double));
@@ -1666,6 +1756,8 @@ int rhg_interpolate_vdw_grid(rhg_grid *g, int64_t n, const double *xyz, double *
   memcpy(a.uinv, g->uinv, sizeof(a.uinv));
   a.bc = g->bc;
   a.fp64 = g->fp64;
+  a.coulomb = g->coulomb;
+  a.charge = charge;
   a.grid = g->d;
   {
     Timed t(c, "grid_interp");
@@ -1676,11 +1768,155 @@ int rhg_interpolate_vdw_grid(rhg_grid *g, int64_t n, const double *xyz, double *
   HIPC(c, hipStreamSynchronize(c->stream));
   return RHG_OK;
 }
+} // namespace
+
+int rhg_interpolate_vdw_grid(rhg_grid *g, int64_t n, const double *xyz, double *u) {
+  if (!g || n < 0 || (n > 0 && (!xyz || !u))) return RHG_ERR_INVALID;
+  if (g->coulomb) return fail(g->ctx, RHG_ERR_INVALID, "a Coulomb grid is interpolated with its charge");
+  if (n == 0) return RHG_OK;
+  return interpolate(g, n, xyz, 1.0, u);
+}
+
+int rhg_interpolate_coulomb_grid(rhg_grid *g, int64_t n, const double *xyz, double charge, double *u) {
+  if (!g || n < 0 || (n > 0 && (!xyz || !u))) return RHG_ERR_INVALID;
+  if (!g->coulomb) return fail(g->ctx, RHG_ERR_INVALID, "not a Coulomb grid");
+  if (n == 0) return RHG_OK;
+  return interpolate(g, n, xyz, charge, u);
+}
+
+// ------------------------------------------------------------ grid files
+// WriteVDWGrid / WriteCoulombGrid (src/grids.c:692-742, 815-876): the
+// header fields one after the other (spacing, N, SizeGrid, ShiftGrid,
+// Delta, UnitCellSize, NumberOfUnitCells, and for a Coulomb grid the Ewald
+// precision), then the float channels channel-major, m, i, j, k.
+namespace {
+constexpr size_t kVdwHeader = 8 + 12 + 4 * 24 + 12;
+constexpr size_t kCoulombHeader = kVdwHeader + 8;
+} // namespace
+
+int rhg_grid_write(const rhg_grid *g, const char *path, const int32_t n_unit_cells[3]) {
+  if (!g || !path) return RHG_ERR_INVALID;
+  rhg_ctx *c = g->ctx;
+  if (g->fp64) return fail(c, RHG_ERR_INVALID, "grid files hold FP32 channels; build the grid with fp64 = 0");
+  const int64_t nodes = grid_nodes(g);
+  hipSetDevice(c->device);
+  float *tmp = nullptr;
+  if (hipMalloc(&tmp, g->bytes) != hipSuccess) return fail(c, RHG_ERR_OOM, "grid transpose buffer");
+  std::vector<float> host((size_t)nodes * 8);
+  launch_grid_transpose((const float *)g->d, tmp, nodes, 1, c->stream);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(host.data(), tmp, g->bytes, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(tmp);
+  if (e != hipSuccess) return fail(c, RHG_ERR_HIP, std::string("grid transpose: ") + hipGetErrorString(e));
+  FILE *f = fopen(path, "wb");
+  if (!f) return fail(c, RHG_ERR_IO, std::string("cannot open ") + path + " for writing");
+  const int32_t nc[3] = {n_unit_cells ? n_unit_cells[0] : g->n_cells[0],
+                         n_unit_cells ? n_unit_cells[1] : g->n_cells[1],
+                         n_unit_cells ? n_unit_cells[2] : g->n_cells[2]};
+  const int32_t N[3] = {g->n[0], g->n[1], g->n[2]};
+  bool ok = fwrite(&g->spacing, sizeof(double), 1, f) == 1 && fwrite(N, sizeof(int32_t), 3, f) == 3 &&
+            fwrite(g->size, sizeof(double), 3, f) == 3 && fwrite(g->shift, sizeof(double), 3, f) == 3 &&
+            fwrite(g->delta, sizeof(double), 3, f) == 3 && fwrite(g->ulen, sizeof(double), 3, f) == 3 &&
+            fwrite(nc, sizeof(int32_t), 3, f) == 3;
+  if (ok && g->coulomb) ok = fwrite(&g->ewald_precision, sizeof(double), 1, f) == 1;
+  if (ok) ok = fwrite(host.data(), sizeof(float), host.size(), f) == host.size();
+  if (fclose(f) != 0) ok = false;
+  if (!ok) return fail(c, RHG_ERR_IO, std::string("short write to ") + path);
+  return RHG_OK;
+}
+
+int rhg_grid_read(rhg_ctx *c, const char *path, int32_t coulomb, const double unit_box[9],
+                  const int32_t n_unit_cells[3], rhg_grid **out) {
+  if (!c || !path || !unit_box || !out) return RHG_ERR_INVALID;
+  *out = nullptr;
+  FILE *f = fopen(path, "rb");
+  if (!f) return fail(c, RHG_ERR_IO, std::string("file ") + path + " does not exist");
+  std::unique_ptr<FILE, int (*)(FILE *)> guard(f, fclose);
+  std::unique_ptr<rhg_grid> g(new rhg_grid());
+  g->ctx = c;
+  g->device = c->device;
+  int32_t N[3], nc[3];
+  double ucs[3], ep = -1.0;
+  bool ok = fread(&g->spacing, sizeof(double), 1, f) == 1 && fread(N, sizeof(int32_t), 3, f) == 3 &&
+            fread(g->size, sizeof(double), 3, f) == 3 && fread(g->shift, sizeof(double), 3, f) == 3 &&
+            fread(g->delta, sizeof(double), 3, f) == 3 && fread(ucs, sizeof(double), 3, f) == 3 &&
+            fread(nc, sizeof(int32_t), 3, f) == 3;
+  if (ok && coulomb) ok = fread(&ep, sizeof(double), 1, f) == 1;
+  if (!ok) return fail(c, RHG_ERR_IO, std::string("short grid header in ") + path);
+  for (int d = 0; d < 3; ++d)
+    if (N[d] < 1 || N[d] > (1 << 20) || !(g->size[d] > 0.0))
+      return fail(c, RHG_ERR_IO, std::string("malformed grid header in ") + path);
+  for (int d = 0; d < 3; ++d) g->n[d] = N[d];
+  const int64_t nodes = grid_nodes(g.get());
+  const size_t header = coulomb ? kCoulombHeader : kVdwHeader;
+  if (fseek(f, 0, SEEK_END) != 0) return fail(c, RHG_ERR_IO, "seek");
+  const long len = ftell(f);
+  if (len < 0 || (size_t)len != header + (size_t)nodes * 8 * sizeof(float))
+    return fail(c, RHG_ERR_IO, std::string("grid file size does not match its header: ") + path);
+  fseek(f, (long)header, SEEK_SET);
+  // src/grids.c:787-793, 931-947
+  double up[7];
+  cell_properties(unit_box, up);
+  for (int d = 0; d < 3; ++d)
+    if (fabs(up[d] - ucs[d]) > 1e-8)
+      return fail(c, RHG_ERR_INVALID, "error in reading the grid: unit-cell size does not match");
+  if (coulomb && n_unit_cells)
+    for (int d = 0; d < 3; ++d)
+      if (nc[d] != n_unit_cells[d])
+        return fail(c, RHG_ERR_INVALID, "mismatch of the number of unit cells");
+  memcpy(g->ubox, unit_box, 9 * sizeof(double));
+  invert3x3(g->ubox, g->uinv);
+  for (int d = 0; d < 3; ++d) {
+    g->ulen[d] = up[d];
+    g->n_cells[d] = nc[d];
+  }
+  g->bc = c->cell.bc;
+  g->fp64 = 0;
+  g->coulomb = coulomb ? 1 : 0;
+  g->ewald_precision = ep;
+  std::vector<float> host((size_t)nodes * 8);
+  if (fread(host.data(), sizeof(float), host.size(), f) != host.size())
+    return fail(c, RHG_ERR_IO, std::string("short grid data in ") + path);
+  if (int rc = grid_alloc(c, g.get())) return rc;
+  float *tmp = nullptr;
+  if (hipMalloc(&tmp, g->bytes) != hipSuccess) {
+    rhg_grid_free(g.release());
+    return fail(c, RHG_ERR_OOM, "grid transpose buffer");
+  }
+  hipError_t e = hipMemcpyAsync(tmp, host.data(), g->bytes, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    launch_grid_transpose(tmp, (float *)g->d, nodes, 0, c->stream);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(tmp);
+  if (e != hipSuccess) {
+    rhg_grid_free(g.release());
+    return fail(c, RHG_ERR_HIP, std::string("grid upload: ") + hipGetErrorString(e));
+  }
+  *out = g.release();
+  return RHG_OK;
+}
+
+int rhg_grid_meta(const rhg_grid *g, double out[9]) {
+  if (!g || !out) return RHG_ERR_INVALID;
+  out[0] = g->spacing;
+  for (int d = 0; d < 3; ++d) {
+    out[1 + d] = g->ulen[d];
+    out[4 + d] = g->n_cells[d];
+  }
+  out[7] = g->coulomb;
+  out[8] = g->ewald_precision;
+  return RHG_OK;
+}
 
 void rhg_grid_free(rhg_grid *g) {
   if (!g) return;
-  hipSetDevice(g->ctx->device);
-  hipFree(g->d);
+  if (g->d) {
+    hipSetDevice(g->device);
+    hipFree(g->d);
+  }
   delete g;
 }
 
@@ -1706,6 +1942,37 @@ int rhg_derivatives_at_positions_vdw(rhg_ctx *c, int32_t type, int64_t n, const 
   launch_derivatives(a, dx, n, dout, c->stream);
   HIPC(c, hipGetLastError());
   HIPC(c, hipMemcpyAsync(out, dout, (size_t)n * 14 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return RHG_OK;
+}
+
+int rhg_derivatives_at_positions_real(rhg_ctx *c, int64_t n, const double *xyz, double *out,
+                                      double *smallest_r) {
+  if (!c || n < 0 || (n > 0 && (!xyz || !out || !smallest_r))) return RHG_ERR_INVALID;
+  if (c->charge_method != RHG_CHARGE_EWALD)
+    return fail(c, RHG_ERR_UNSUPPORTED, "real-space derivatives need the Ewald charge method");
+  if (n == 0) return RHG_OK;
+  hipSetDevice(c->device);
+  double *dx = (double *)work(c, "d_xyz", (size_t)n * 3 * sizeof(double));
+  double *dout = (double *)work(c, "d_out", (size_t)n * 14 * sizeof(double));
+  double *dr = (double *)work(c, "d_rmin", (size_t)n * sizeof(double));
+  if (!dx || !dout || !dr) return fail(c, RHG_ERR_OOM, "derivative buffers");
+  upload(c, dx, xyz, (size_t)n * 3 * sizeof(double));
+  GridArgs a;
+  memset(&a, 0, sizeof(a));
+  a.cell = c->cell;
+  a.rc2 = c->rc2_cc;
+  a.n_atoms = c->n;
+  a.coulomb = 1;
+  a.cc = c->u.C;
+  a.alpha = c->alpha;
+  a.sqrtpi = sqrt(M_PI);
+  a.aq = atoms_xyzq(c);
+  if (!a.aq) return fail(c, RHG_ERR_OOM, "atom buffer");
+  launch_derivatives_real(a, dx, n, dout, dr, c->stream);
+  HIPC(c, hipGetLastError());
+  HIPC(c, hipMemcpyAsync(out, dout, (size_t)n * 14 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipMemcpyAsync(smallest_r, dr, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
   return RHG_OK;
 }

@@ -241,6 +241,11 @@ struct GridArgs {
   int fp64;
   void *out;         // [i][j][k][8] float or double
   int64_t node0, n_nodes; // node range of this launch
+  // Ewald-Coulomb grid of a unit charge (coulomb = 1): atoms as {x, y, z, q},
+  // rc2 is the charge cut-off squared
+  int coulomb;
+  const double4 *aq;
+  double cc, alpha, sqrtpi; // conversion factor C, Ewald alpha, sqrt(pi)
 };
 void launch_grid_build(const GridArgs &g, hipStream_t st);
 // node positions of nodes [node0, node0 + n_nodes) into x, y, z
@@ -252,6 +257,13 @@ void launch_grid_culled(const GridArgs &g, const CellGrid &cg, const SiteList &s
                         const float4 *par, int precision, hipStream_t st);
 void launch_derivatives(const GridArgs &g, const double *xyz, int64_t n, double *out,
                         hipStream_t st);
+// CalculateDerivativesAtPositionReal at n positions: out[n][14] as
+// launch_derivatives, smallest[n] the smallest in-range distance
+void launch_derivatives_real(const GridArgs &g, const double *xyz, int64_t n, double *out,
+                             double *smallest, hipStream_t st);
+// node-major [node][8] <-> channel-major [8][node] float channels (the file
+// layout of grids); to_file 1 gathers node-major into channel-major
+void launch_grid_transpose(const float *src, float *dst, int64_t nodes, int to_file, hipStream_t st);
 
 struct InterpArgs {
   int n[3];
@@ -260,6 +272,8 @@ struct InterpArgs {
   double ulen[3];
   int bc;
   int fp64;
+  int coulomb;   // no overlap short-cut; the result is scaled by charge
+  double charge;
   const void *grid;
 };
 void launch_interp(const InterpArgs &a, const double *xyz, int64_t n, double *out,

@@ -1810,14 +1810,14 @@ __device__ __forceinline__ void derivs_at(const GridArgs &g, double px, double p
   o[13] = third;
 }
 
-// MakeGrid's node record (src/grids.c:520-565): the cap above the overlap
-// energy, then the 8 channels scaled to the unit cube, FP32 (RASPA2's
-// layout) or FP64
+// MakeGrid's node record (src/grids.c:520-565, 652-676): the cap (above
+// the overlap energy; for the Coulomb grid also an atom closer than 1 A),
+// then the 8 channels scaled to the unit cube, FP32 (RASPA2's layout) or FP64
 __device__ __forceinline__ void store_node(const GridArgs &g, int64_t node, double value, double f0,
                                            double f1, double f2, double hay, double haz, double hbz,
-                                           double third) {
+                                           double third, bool cap) {
 #pragma clang fp contract(off)
-  if (value > kOverlap) {
+  if (cap) {
     value = 2.0 * kOverlap;
     f0 = fmin(fmax(f0, -kOverlap), kOverlap);
     f1 = fmin(fmax(f1, -kOverlap), kOverlap);
@@ -1874,7 +1874,167 @@ __global__ __launch_bounds__(256) void k_grid_build(const GridArgs g) {
   node_position(g, node, px, py, pz);
   double d[14];
   derivs_at<BCK>(g, px, py, pz, d);
-  store_node(g, node, d[0], d[1], d[2], d[3], d[5], d[6], d[9], d[13]);
+  store_node(g, node, d[0], d[1], d[2], d[3], d[5], d[6], d[9], d[13], d[0] > kOverlap);
+}
+
+// CalculateDerivativesAtPositionReal (src/framework_hessian.c:111-175): the
+// real-space Ewald pair term of a unit charge and its first three radial
+// derivative factors, FP64 in the reference's operation order (the probe
+// charge 1.0 drops out of every product exactly)
+__device__ __forceinline__ void real_terms(const GridArgs &g, double qb, double rr, double &r,
+                                           double &F, double &DF, double &DDF, double &DDDF) {
+#pragma clang fp contract(off)
+  const double alpha = g.alpha, C = g.cc, sp = g.sqrtpi;
+  r = sqrt(rr);
+  const double ec = erfc(alpha * r);
+  const double a2 = alpha * alpha;
+  const double ex = exp(-a2 * rr);
+  F = C * qb * (ec / r);
+  DF = -C * qb * ((2.0 * alpha) * r * ex / sp + ec) / (r * rr);
+  DDF = qb * C * ((2.0 * alpha) * ex * r * (3.0 + (2.0 * a2) * rr) / sp + 3.0 * ec) / ((rr * rr) * r);
+  const double ar = a2 * rr;
+  DDDF = qb * C *
+         ((-2.0 * alpha) * ex * r * ((15.0 + (10.0 * a2) * rr) + 4.0 * (ar * ar)) / sp - 15.0 * ec) /
+         (((rr * rr) * rr) * r);
+}
+
+// the grid's 8 channels of the real-space potential over atoms idx[j0..j1)
+// (idx null: atoms j0..j1), in that order; returns the smallest in-range
+// distance (DBL_MAX when none)
+template <int BCK>
+__device__ __forceinline__ double coulomb_channels(const GridArgs &g, const int32_t *idx, int j0,
+                                                   int j1, double px, double py, double pz,
+                                                   double o[8]) {
+#pragma clang fp contract(off)
+  double value = 0, fx = 0, fy = 0, fz = 0, hay = 0, haz = 0, hbz = 0, third = 0;
+  double smallest = DBL_MAX;
+  for (int q = j0; q < j1; ++q) {
+    const int j = idx ? idx[q] : q;
+    const double4 a = g.aq[j];
+    double dx = px - a.x, dy = py - a.y, dz = pz - a.z;
+    mi_exact<BCK>(g.cell, dx, dy, dz);
+    const double rr = dx * dx + dy * dy + dz * dz;
+    if (rr < g.rc2) {
+      double r, F, DF, DDF, DDDF;
+      real_terms(g, a.w, rr, r, F, DF, DDF, DDDF);
+      if (r < smallest) smallest = r;
+      value += F;
+      fx += dx * DF;
+      fy += dy * DF;
+      fz += dz * DF;
+      hay += DDF * dx * dy;
+      haz += DDF * dx * dz;
+      hbz += DDF * dy * dz;
+      third += DDDF * dx * dy * dz;
+    }
+  }
+  o[0] = value;
+  o[1] = fx;
+  o[2] = fy;
+  o[3] = fz;
+  o[4] = hay;
+  o[5] = haz;
+  o[6] = hbz;
+  o[7] = third;
+  return smallest;
+}
+
+template <int BCK>
+__global__ __launch_bounds__(256) void k_coulomb_grid_build(const GridArgs g) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.n_nodes) return;
+  const int64_t node = g.node0 + t;
+  double px, py, pz, o[8];
+  node_position(g, node, px, py, pz);
+  const double rmin = coulomb_channels<BCK>(g, nullptr, 0, g.n_atoms, px, py, pz, o);
+  store_node(g, node, o[0], o[1], o[2], o[3], o[4], o[5], o[6], o[7], o[0] > kOverlap || rmin < 1.0);
+}
+
+template <int BCK>
+__global__ __launch_bounds__(256) void k_derivs_real(const GridArgs g, const double *__restrict__ xyz,
+                                                     int64_t n, double *__restrict__ out,
+                                                     double *__restrict__ smallest) {
+#pragma clang fp contract(off)
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double px = xyz[3 * t], py = xyz[3 * t + 1], pz = xyz[3 * t + 2];
+  double value = 0, fx = 0, fy = 0, fz = 0, h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, third = 0;
+  double rmin = DBL_MAX;
+  for (int j = 0; j < g.n_atoms; ++j) {
+    const double4 a = g.aq[j];
+    double dx = px - a.x, dy = py - a.y, dz = pz - a.z;
+    mi_exact<BCK>(g.cell, dx, dy, dz);
+    const double rr = dx * dx + dy * dy + dz * dz;
+    if (rr < g.rc2) {
+      double r, F, DF, DDF, DDDF;
+      real_terms(g, a.w, rr, r, F, DF, DDF, DDDF);
+      if (r < rmin) rmin = r;
+      value += F;
+      fx += dx * DF;
+      fy += dy * DF;
+      fz += dz * DF;
+      h[AX] += DDF * dx * dx + DF;
+      h[BX] += DDF * dy * dx;
+      h[CX] += DDF * dz * dx;
+      h[AY] += DDF * dx * dy;
+      h[BY] += DDF * dy * dy + DF;
+      h[CY] += DDF * dz * dy;
+      h[AZ] += DDF * dx * dz;
+      h[BZ] += DDF * dy * dz;
+      h[CZ] += DDF * dz * dz + DF;
+      third += DDDF * dx * dy * dz;
+    }
+  }
+  double *o = out + t * 14;
+  o[0] = value;
+  o[1] = fx;
+  o[2] = fy;
+  o[3] = fz;
+  for (int q = 0; q < 9; ++q) o[4 + q] = h[q];
+  o[13] = third;
+  smallest[t] = rmin;
+}
+
+void launch_derivatives_real(const GridArgs &g, const double *xyz, int64_t n, double *out,
+                             double *smallest, hipStream_t st) {
+  const unsigned nb = nblk(n, 256);
+  if (!nb) return;
+  if (g.cell.bc == 1)
+    hipLaunchKernelGGL(k_derivs_real<1>, dim3(nb), dim3(256), 0, st, g, xyz, n, out, smallest);
+  else if (g.cell.upper)
+    hipLaunchKernelGGL(k_derivs_real<2>, dim3(nb), dim3(256), 0, st, g, xyz, n, out, smallest);
+  else
+    hipLaunchKernelGGL(k_derivs_real<3>, dim3(nb), dim3(256), 0, st, g, xyz, n, out, smallest);
+}
+
+// one thread per node: 32 contiguous bytes on the node-major side, eight
+// coalesced 4-byte streams on the channel-major side
+__global__ __launch_bounds__(256) void k_grid_transpose(const float *__restrict__ src,
+                                                        float *__restrict__ dst, int64_t nodes,
+                                                        int to_file) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nodes) return;
+  if (to_file) {
+    const float4 *p = (const float4 *)(src + t * 8);
+    const float4 a = p[0], b = p[1];
+    dst[0 * nodes + t] = a.x;
+    dst[1 * nodes + t] = a.y;
+    dst[2 * nodes + t] = a.z;
+    dst[3 * nodes + t] = a.w;
+    dst[4 * nodes + t] = b.x;
+    dst[5 * nodes + t] = b.y;
+    dst[6 * nodes + t] = b.z;
+    dst[7 * nodes + t] = b.w;
+  } else {
+    float4 *p = (float4 *)(dst + t * 8);
+    p[0] = make_float4(src[0 * nodes + t], src[1 * nodes + t], src[2 * nodes + t], src[3 * nodes + t]);
+    p[1] = make_float4(src[4 * nodes + t], src[5 * nodes + t], src[6 * nodes + t], src[7 * nodes + t]);
+  }
+}
+
+void launch_grid_transpose(const float *src, float *dst, int64_t nodes, int to_file, hipStream_t st) {
+  const unsigned nb = nblk(nodes, 256);
+  if (nb) hipLaunchKernelGGL(k_grid_transpose, dim3(nb), dim3(256), 0, st, src, dst, nodes, to_file);
 }
 
 template <int BCK>
@@ -1966,8 +2126,13 @@ __global__ __launch_bounds__(256) void k_grid_culled(const GridArgs g, const Cel
   const int site = live ? b.perm[b.start[c] + k] : b.perm[b.start[c]];
   const double px = s.x[site], py = s.y[site], pz = s.z[site];
   double o[8];
-  if (PREC == 0) {
+  bool cap;
+  if (PREC == 2) {
+    const double rmin = coulomb_channels<BCK>(g, cg.nb_idx, cg.nb_off[c], cg.nb_off[c + 1], px, py, pz, o);
+    cap = o[0] > kOverlap || rmin < 1.0;
+  } else if (PREC == 0) {
     node_channels_exact<BCK>(g, cg, c, px, py, pz, o);
+    cap = o[0] > kOverlap;
   } else {
     double cx, cy, cz;
     cell_centre(g.cell, cg, c, cx, cy, cz);
@@ -2016,8 +2181,9 @@ __global__ __launch_bounds__(256) void k_grid_culled(const GridArgs g, const Cel
     const bool near = live && fabs(o[0] - kOverlap) <= 1.0e-4 * kOverlap;
     if (__any(near))
       if (near) node_channels_exact<BCK>(g, cg, c, px, py, pz, o);
+    cap = o[0] > kOverlap;
   }
-  if (live) store_node(g, g.node0 + site, o[0], o[1], o[2], o[3], o[4], o[5], o[6], o[7]);
+  if (live) store_node(g, g.node0 + site, o[0], o[1], o[2], o[3], o[4], o[5], o[6], o[7], cap);
 }
 
 void launch_grid_culled(const GridArgs &g, const CellGrid &cg, const SiteList &s, const BinWork &b,
@@ -2025,7 +2191,11 @@ void launch_grid_culled(const GridArgs &g, const CellGrid &cg, const SiteList &s
   const int64_t max_tiles = s.n / 64 + cg.ncell + 1;
   const unsigned nb = (unsigned)((max_tiles + 3) / 4);
   const int bck = g.cell.bc == 1 ? 1 : (g.cell.upper ? 2 : 3);
-  if (precision == 0) {
+  if (g.coulomb) {
+    if (bck == 1) hipLaunchKernelGGL((k_grid_culled<2, 1>), dim3(nb), dim3(256), 0, st, g, cg, s, b, par);
+    else if (bck == 2) hipLaunchKernelGGL((k_grid_culled<2, 2>), dim3(nb), dim3(256), 0, st, g, cg, s, b, par);
+    else hipLaunchKernelGGL((k_grid_culled<2, 3>), dim3(nb), dim3(256), 0, st, g, cg, s, b, par);
+  } else if (precision == 0) {
     if (bck == 1) hipLaunchKernelGGL((k_grid_culled<0, 1>), dim3(nb), dim3(256), 0, st, g, cg, s, b, par);
     else if (bck == 2) hipLaunchKernelGGL((k_grid_culled<0, 2>), dim3(nb), dim3(256), 0, st, g, cg, s, b, par);
     else hipLaunchKernelGGL((k_grid_culled<0, 3>), dim3(nb), dim3(256), 0, st, g, cg, s, b, par);
@@ -2039,6 +2209,15 @@ void launch_grid_culled(const GridArgs &g, const CellGrid &cg, const SiteList &s
 void launch_grid_build(const GridArgs &g, hipStream_t st) {
   unsigned nb = nblk(g.n_nodes, 256);
   if (!nb) return;
+  if (g.coulomb) {
+    if (g.cell.bc == 1)
+      hipLaunchKernelGGL(k_coulomb_grid_build<1>, dim3(nb), dim3(256), 0, st, g);
+    else if (g.cell.upper)
+      hipLaunchKernelGGL(k_coulomb_grid_build<2>, dim3(nb), dim3(256), 0, st, g);
+    else
+      hipLaunchKernelGGL(k_coulomb_grid_build<3>, dim3(nb), dim3(256), 0, st, g);
+    return;
+  }
   if (g.cell.bc == 1)
     hipLaunchKernelGGL(k_grid_build<1>, dim3(nb), dim3(256), 0, st, g);
   else if (g.cell.upper)
@@ -2124,11 +2303,12 @@ __global__ __launch_bounds__(256) void k_interp(const InterpArgs a, const double
       X[c][4] = v1.x; X[c][5] = v1.y; X[c][6] = v1.z; X[c][7] = v1.w;
     }
   }
-  for (int c = 0; c < 8; ++c)
-    if (X[c][0] > 1e10) {
-      out[t] = 1e8;
-      return;
-    }
+  if (!a.coulomb)
+    for (int c = 0; c < 8; ++c)
+      if (X[c][0] > 1e10) {
+        out[t] = 1e8;
+        return;
+      }
   double hx[2][2], hy[2][2], hz[2][2];
   hermite(r[0], hx);
   hermite(r[1], hy);
@@ -2142,7 +2322,7 @@ __global__ __launch_bounds__(256) void k_interp(const InterpArgs a, const double
     for (int q = 0; q < 8; ++q)
       v += X[c][q] * (hx[cx0][dd[q][0]] * hy[cy0][dd[q][1]] * hz[cz0][dd[q][2]]);
   }
-  out[t] = v;
+  out[t] = a.coulomb ? a.charge * v : v;
 }
 
 void launch_interp(const InterpArgs &a, const double *xyz, int64_t n, double *out, hipStream_t st) {

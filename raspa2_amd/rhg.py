@@ -8,6 +8,7 @@ if librhg.so is missing the import of this module fails.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -21,8 +22,8 @@ CHARGE_EWALD = 1
 PREC_FP64 = 0
 PREC_MIXED = 1
 MAX_SITES = 8
-ERR_INVALID, ERR_HIP, ERR_OOM, ERR_NOT_ADMITTED, ERR_UNSUPPORTED, ERR_FAULT_INJECTED = (
-    -1, -2, -3, -4, -5, -6)
+ERR_INVALID, ERR_HIP, ERR_OOM, ERR_NOT_ADMITTED, ERR_UNSUPPORTED, ERR_FAULT_INJECTED, ERR_IO = (
+    -1, -2, -3, -4, -5, -6, -7)
 NR_BLOCKS = 5
 REC_N = 14
 REC = dict(W=0, DU=1, UVDW=2, UREAL=3, UFHOST=4, UFADS=5, OVERLAP=6, SELFB=7, SELOR=8,
@@ -115,6 +116,12 @@ def _load():
     f("rhg_interpolate_vdw_grid", _P, _I64, _P, _P)
     f("rhg_grid_free", _P, res=None)
     f("rhg_derivatives_at_positions_vdw", _P, _I32, _I64, _P, _P)
+    f("rhg_make_coulomb_grid", _P, _P, _D, _I32, ctypes.POINTER(_P))
+    f("rhg_interpolate_coulomb_grid", _P, _I64, _P, _D, _P)
+    f("rhg_derivatives_at_positions_real", _P, _I64, _P, _P, _P)
+    f("rhg_grid_write", _P, ctypes.c_char_p, _P)
+    f("rhg_grid_read", _P, ctypes.c_char_p, _I32, _P, _P, ctypes.POINTER(_P))
+    f("rhg_grid_meta", _P, _P)
     f("rhg_profile_enable", _P, _I32)
     f("rhg_profile_get", _P, ctypes.c_char_p, ctypes.POINTER(_I64), ctypes.POINTER(_D))
     f("rhg_profile_reset", _P)
@@ -179,12 +186,20 @@ class Grid:
     def __init__(self, ctx, handle):
         self.ctx = ctx
         self.h = handle
+        ctx._grids.add(self)
         info = np.zeros(13)
         lib.rhg_grid_info(self.h, _ptr(info))
         self.n = info[0:3].astype(int)
         self.size, self.shift, self.delta = info[3:6], info[6:9], info[9:12]
         self.node_bytes = int(info[12])
         self.geom = info[:12].copy()
+        meta = np.zeros(9)
+        lib.rhg_grid_meta(self.h, _ptr(meta))
+        self.spacing = meta[0]
+        self.unit_cell_size = meta[1:4].copy()
+        self.n_unit_cells = meta[4:7].astype(int)
+        self.coulomb = bool(meta[7])
+        self.ewald_precision = meta[8]
 
     def close(self):
         """Free the device grid now (also done when the object is collected)."""
@@ -205,13 +220,40 @@ class Grid:
         self.ctx._check(lib.rhg_grid_download(self.h, _ptr(out), out.nbytes), "grid_download")
         return out
 
-    def interpolate(self, xyz):
-        """InterpolateVDWGrid (src/grids.c:979) at many positions."""
+    def interpolate(self, xyz, charge=None):
+        """InterpolateVDWGrid (src/grids.c:979) at many positions; for a
+        Coulomb grid InterpolateCoulombGrid (src/grids.c:1200) of a probe of
+        the given charge."""
         xyz = _f64(xyz).reshape(-1, 3)
         u = np.zeros(len(xyz))
-        self.ctx._check(lib.rhg_interpolate_vdw_grid(self.h, len(xyz), _ptr(xyz), _ptr(u)),
-                        "interpolate_vdw_grid")
+        if self.coulomb:
+            if charge is None:
+                raise RhgError("a Coulomb grid is interpolated for a probe charge", ERR_INVALID)
+            self.ctx._check(lib.rhg_interpolate_coulomb_grid(self.h, len(xyz), _ptr(xyz), float(charge),
+                                                             _ptr(u)), "interpolate_coulomb_grid")
+        else:
+            self.ctx._check(lib.rhg_interpolate_vdw_grid(self.h, len(xyz), _ptr(xyz), _ptr(u)),
+                            "interpolate_vdw_grid")
         return u
+
+    def write(self, path, n_unit_cells=None):
+        """WriteVDWGrid / WriteCoulombGrid (src/grids.c:692, 815) to `path`."""
+        nc = None if n_unit_cells is None else np.ascontiguousarray(n_unit_cells, dtype=np.int32)
+        self.ctx._check(lib.rhg_grid_write(self.h, os.fsencode(path), None if nc is None else _ptr(nc)),
+                        "grid_write")
+
+
+def grid_path(raspa_dir, forcefield, framework, spacing, pseudo_atom=None, shifted=True,
+              n_unit_cells=None, method="Ewald"):
+    """Where RASPA2 keeps a grid (src/grids.c:706-724, 838-856): the VDW grid
+    of `pseudo_atom`, or with pseudo_atom None the Coulomb grid of a system of
+    n_unit_cells unit cells."""
+    base = os.path.join(raspa_dir, "share", "raspa", "grids", forcefield, framework, "%f" % spacing)
+    if pseudo_atom is not None:
+        return os.path.join(base, "%s_%s_%s.grid" % (framework, pseudo_atom,
+                                                     "shifted" if shifted else "truncated"))
+    nc = "%dx%dx%d" % tuple(int(v) for v in n_unit_cells)
+    return os.path.join(base, nc, "%s_Electrostatics_%s.grid" % (framework, method))
 
 
 class Context:
@@ -233,6 +275,7 @@ class Context:
         ff = ForceField(cutoff_vdw, 1 if shifted else 0, charge_method, cutoff_charge,
                         ewald_precision, temperature)
         h = _P()
+        self._grids = weakref.WeakSet()
         rc = lib.rhg_create(ctypes.byref(fw), ctypes.byref(cell), ctypes.byref(ff), device,
                             ctypes.byref(h))
         if rc:
@@ -243,6 +286,10 @@ class Context:
         self.set_precision(precision)
 
     def close(self):
+        # grids run on this context's stream: they go first, also when the
+        # garbage collector finalises a cycle in arbitrary order
+        for g in list(getattr(self, "_grids", ())):
+            g.close()
         if getattr(self, "h", None):
             lib.rhg_destroy(self.h)
             self.h = None
@@ -388,6 +435,32 @@ class Context:
         self._check(lib.rhg_make_grid(self.h, type_a, _ptr(ub), spacing, 1 if fp64 else 0,
                                       ctypes.byref(h)), "make_grid")
         return Grid(self, h)
+
+    def MakeCoulombGrid(self, unit_box, spacing, fp64=False):
+        """MakeGrid's Ewald-Coulomb grid (src/grids.c:587-689)."""
+        ub = _f64(unit_box).reshape(9)
+        h = _P()
+        self._check(lib.rhg_make_coulomb_grid(self.h, _ptr(ub), spacing, 1 if fp64 else 0,
+                                              ctypes.byref(h)), "make_coulomb_grid")
+        return Grid(self, h)
+
+    def ReadGrid(self, path, unit_box, coulomb=False, n_unit_cells=None):
+        """ReadVDWGrid / ReadCoulombGrid (src/grids.c:745, 879) from `path`."""
+        ub = _f64(unit_box).reshape(9)
+        nc = None if n_unit_cells is None else np.ascontiguousarray(n_unit_cells, dtype=np.int32)
+        h = _P()
+        self._check(lib.rhg_grid_read(self.h, os.fsencode(path), 1 if coulomb else 0, _ptr(ub),
+                                      None if nc is None else _ptr(nc), ctypes.byref(h)), "grid_read")
+        return Grid(self, h)
+
+    def CalculateDerivativesAtPositionReal(self, xyz):
+        """(out n x 14, smallest_r n) of src/framework_hessian.c:111."""
+        xyz = _f64(xyz).reshape(-1, 3)
+        out = np.zeros((len(xyz), 14))
+        rmin = np.zeros(len(xyz))
+        self._check(lib.rhg_derivatives_at_positions_real(self.h, len(xyz), _ptr(xyz), _ptr(out),
+                                                          _ptr(rmin)), "derivatives_real")
+        return out, rmin
 
     def CalculateDerivativesAtPositionVDW(self, xyz, type_a):
         xyz = _f64(xyz).reshape(-1, 3)

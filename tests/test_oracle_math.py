@@ -183,12 +183,10 @@ def test_charge_energy_closed_form(O):
     assert s.charge_at(t, [0, 12.01, 0]) == 0.0
 
 
-def test_lekien_marsden_matrix(O):
-    """The generated 64x64 matrix is integer, inverts the constraint matrix,
-    and interpolates any tricubic polynomial exactly."""
-    C = O.lm_coeff()
-    assert C.dtype == np.int32 and np.abs(C).max() <= 100
-    rng = np.random.default_rng(9)
+def _tricubic_grid(seed):
+    """A 2x2x2-cell grid over [0,2]^3 (unit spacing) holding a random
+    tricubic polynomial and its channel derivatives, and the polynomial."""
+    rng = np.random.default_rng(seed)
     coef = rng.normal(size=(4, 4, 4))
 
     def f_and_derivs(x, y, z):
@@ -207,16 +205,24 @@ def test_lekien_marsden_matrix(O):
             out[ch] = np.einsum("ijk,i,j,k->", coef, px, py, pz)
         return out
 
-    # a 2x2x2-cell grid over [0,2]^3 with unit spacing
     geom = np.array([2, 2, 2, 2.0, 2.0, 2.0, 0, 0, 0, 1.0, 1.0, 1.0])
     grid = np.zeros((3, 3, 3, 8), dtype=np.float32)
-    # small coefficients keep float32 node values exact enough
     for i in range(3):
         for j in range(3):
             for k in range(3):
                 grid[i, j, k] = f_and_derivs(i, j, k)
     box = np.array([2.0, 0, 0, 0, 2.0, 0, 0, 0, 2.0])
     inv = np.array([0.5, 0, 0, 0, 0.5, 0, 0, 0, 0.5])
+    return grid, geom, box, inv, f_and_derivs
+
+
+def test_lekien_marsden_matrix(O):
+    """The generated 64x64 matrix is integer, inverts the constraint matrix,
+    and interpolates any tricubic polynomial exactly."""
+    C = O.lm_coeff()
+    assert C.dtype == np.int32 and np.abs(C).max() <= 100
+    grid, geom, box, inv, f_and_derivs = _tricubic_grid(9)
+    rng = np.random.default_rng(9)
     g64 = grid.astype(np.float64)
     for p in rng.uniform(0.05, 1.95, (50, 3)):
         val = O.interpolate_grid(grid, geom, O.BC_RECTANGULAR, box, inv, p)
@@ -224,6 +230,94 @@ def test_lekien_marsden_matrix(O):
         # polynomial within the float32 rounding of the inputs
         ref = f_and_derivs(*p)[0]
         assert abs(val - ref) <= 1e-5 * max(1.0, np.abs(g64).max())
+
+
+def test_coulomb_interpolation(O):
+    """InterpolateCoulombGrid: the same tricubic, times the charge, and no
+    overlap short-cut (a node above 1e10 still interpolates)."""
+    grid, geom, box, inv, f_and_derivs = _tricubic_grid(10)
+    rng = np.random.default_rng(10)
+    g64 = grid.astype(np.float64)
+    for p in rng.uniform(0.05, 1.95, (30, 3)):
+        ref = f_and_derivs(*p)[0]
+        v1 = O.interpolate_coulomb_grid(grid, geom, O.BC_RECTANGULAR, box, inv, p, 1.0)
+        assert abs(v1 - ref) <= 1e-5 * max(1.0, np.abs(g64).max())
+        v2 = O.interpolate_coulomb_grid(grid, geom, O.BC_RECTANGULAR, box, inv, p, -0.35)
+        assert v2 == -0.35 * v1
+    big = grid.copy()
+    big[1, 1, 1, 0] = 2e10
+    p = np.array([0.5, 0.5, 0.5])
+    assert O.interpolate_grid(big, geom, O.BC_RECTANGULAR, box, inv, p) == 1e8
+    v = O.interpolate_coulomb_grid(big, geom, O.BC_RECTANGULAR, box, inv, p, 1.0)
+    # the corner (1,1,1) enters with the weight h(0.5)^3 = 1/8 of its value
+    v0 = O.interpolate_coulomb_grid(grid, geom, O.BC_RECTANGULAR, box, inv, p, 1.0)
+    assert abs((v - v0) - (2e10 - float(grid[1, 1, 1, 0])) / 8.0) <= 1e-6 * 2e10 / 8.0
+
+
+def test_real_derivatives_finite_differences(O, W):
+    """CalculateDerivativesAtPositionReal for a unit charge: the value is the
+    real-space energy of a unit-charge probe, the gradient, the mixed second
+    derivatives and the third derivative match central differences, the
+    Hessian's diagonal too, and the return value is the nearest in-range
+    atom."""
+    fw = W.c1_framework(n_rep=1)
+    s = oracle_system(fw, charge_method=O.CHARGE_EWALD)
+    t = s.add_pseudo_atom(0.0, 1.0, 1.0)
+    h = 1e-3
+    xyz = np.stack([fw.xyz[:, 0], fw.xyz[:, 1], fw.xyz[:, 2]], axis=1)
+    L = fw.box[0]
+    checked = 0
+    for p in random_positions(fw.box, 12, 23):
+        d, rmin = s.derivatives_real_at(p)
+        dr = p - xyz
+        dr -= L * np.round(dr / L)
+        r = np.sqrt((dr ** 2).sum(axis=1))
+        assert rmin == pytest.approx(r[r < 12.0].min(), rel=1e-14)
+        if rmin < 1.2:
+            continue
+        checked += 1
+        u = s.charge_at(t, p)
+        terms = O.units()["coulomb"] * np.abs(fw.charge[r < 12.0]) / r[r < 12.0]
+        assert abs(d[0] - u) <= 1e-13 * terms.sum()
+        U = lambda q: s.derivatives_real_at(q)[0][0]
+        ex, ey, ez = np.eye(3) * h
+        g = [(U(p + e) - U(p - e)) / (2 * h) for e in (ex, ey, ez)]
+        scale = max(1.0, np.abs(d[1:4]).max())
+        assert np.allclose(d[1:4], g, atol=1e-5 * scale)
+        for (a, b, col) in ((ex, ey, 5), (ex, ez, 6), (ey, ez, 9)):
+            hab = (U(p + a + b) - U(p + a - b) - U(p - a + b) + U(p - a - b)) / (4 * h * h)
+            assert abs(d[col] - hab) <= 1e-3 * max(1.0, abs(hab))
+        hxx = (U(p + ex) - 2 * U(p) + U(p - ex)) / (h * h)
+        assert abs(d[4] - hxx) <= 1e-3 * max(1.0, abs(hxx))
+        gxy = lambda q: s.derivatives_real_at(q)[0][5]
+        t3 = (gxy(p + ez) - gxy(p - ez)) / (2 * h)
+        assert abs(d[13] - t3) <= 1e-3 * max(1.0, abs(t3))
+    assert checked >= 6
+
+
+def test_coulomb_grid_node_cap(O):
+    """The Coulomb grid's cap: a node closer than 1 A to an atom is capped
+    although its energy is small, a node whose energy exceeds 1e7 is capped
+    although no atom is that close, and an uncapped node stores the
+    derivatives scaled by the node spacing."""
+    box, inv, props = O.cell_make(30.0, 30.0, 30.0)
+    for q, r, capped in ((0.5, 0.8, True), (0.5, 1.5, False), (200.0, 1.1, True)):
+        s = O.System(box, O.BC_RECTANGULAR, [0.0], [0.0], [0.0], [q], [50.0], [3.0],
+                     cutoff_vdw=12.0, shifted=True, charge_method=O.CHARGE_EWALD, cutoff_charge=12.0)
+        # a 2-cell grid whose node (1, 0, 0) lies at distance r from the atom
+        geom = np.array([2, 2, 2, 2 * r, 2 * r, 2 * r, 0, 0, 0, r, r, r])
+        node = s.coulomb_grid_node(geom, 1, 0, 0)
+        d, rmin = s.derivatives_real_at([r, 0.0, 0.0])
+        assert rmin == r
+        if capped:
+            # exactly one of the two rules applies in each capped case
+            assert (d[0] > 1e7) != (rmin < 1.0)
+            assert node[0] == np.float32(2e7)
+            assert np.all(node[4:] == 0.0)
+        else:
+            assert d[0] < 1e7 and node[0] == np.float32(d[0])
+            assert node[1] == np.float32(np.float64(np.float32(d[1])) * r)
+            assert node[4] == np.float32(np.float64(np.float32(d[5])) * (r * r))
 
 
 def test_grid_derivatives_finite_differences(O, W):
