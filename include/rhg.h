@@ -273,6 +273,15 @@ int rhg_grid_read(rhg_ctx *ctx, const char *path, int32_t coulomb, const double 
 /* spacing, unit-cell lengths[3], number of unit cells[3], coulomb flag,
  * Ewald precision of a Coulomb grid (-1 otherwise) */
 int rhg_grid_meta(const rhg_grid *g, double out[9]);
+/* UseTabularGrid (src/input.c:5626-5630, 6190-6195): energies of pseudo-atom
+ * `type` at a position, in rhg_framework_energy_at_positions and in the
+ * Widom pipeline, interpolate its VDW grid (CalculateFrameworkVDWEnergyAtPosition's
+ * grid branch, src/framework_energy.c:73-76); with a Coulomb grid in use the
+ * real-space term of every charged pseudo-atom is the grid value times its
+ * charge (src/framework_energy.c:197-200).  The grid is borrowed: it must
+ * live, on the context's device, until it is replaced or detached (null). */
+int rhg_use_grid(rhg_ctx *ctx, int32_t type, rhg_grid *g);
+int rhg_use_coulomb_grid(rhg_ctx *ctx, rhg_grid *g);
 
 /* ------------------------------------------------------------- profiling */
 /* HIP-event timing of the kernels launched on the context's stream.

@@ -103,6 +103,8 @@ def _declare(L):
     f("orc_interpolate_direct", _D, _P, _I, _P, _I, _P, _P, _P)
     f("orc_set_block_pockets", _I, _P, _I, _P, _P, _P, _I)
     f("orc_derivatives_real_at", _D, _P, _P, _P)
+    f("orc_use_grid", _I, _P, _I, _P, _P, _P)
+    f("orc_use_coulomb_grid", _I, _P, _P, _P, _P)
     f("orc_coulomb_grid_node", None, _P, _P, _I, _I, _I, _P)
     f("orc_blocked_pocket", _I, _P, _P)
 
@@ -264,6 +266,30 @@ class System:
         out = np.zeros(14)
         r = lib().orc_derivatives_real_at(self.h, _ptr(_f64(pos)), _ptr(out))
         return out, r
+
+    def use_grid(self, t, grid, geom, unit_box):
+        """Energies of type t at a position come from the grid (None
+        detaches); the arrays are kept alive by this object."""
+        if not hasattr(self, "_grids"):
+            self._grids = {}
+        if grid is None:
+            self._grids.pop(t, None)
+            lib().orc_use_grid(self.h, t, None, None, None)
+            return
+        g = (np.ascontiguousarray(grid, dtype=np.float32), _f64(geom), _f64(unit_box))
+        self._grids[t] = g
+        assert lib().orc_use_grid(self.h, t, _ptr(g[0]), _ptr(g[1]), _ptr(g[2])) == 0
+
+    def use_coulomb_grid(self, grid, geom, unit_box):
+        if not hasattr(self, "_grids"):
+            self._grids = {}
+        if grid is None:
+            self._grids.pop("coulomb", None)
+            lib().orc_use_coulomb_grid(self.h, None, None, None)
+            return
+        g = (np.ascontiguousarray(grid, dtype=np.float32), _f64(geom), _f64(unit_box))
+        self._grids["coulomb"] = g
+        lib().orc_use_coulomb_grid(self.h, _ptr(g[0]), _ptr(g[1]), _ptr(g[2]))
 
     def coulomb_grid_node(self, geom, i, j, k):
         out = np.zeros(8, dtype=np.float32)

@@ -319,6 +319,14 @@ struct orc_system {
   double *pk; /* n x 4: x, y, z, radius */
   double pk_ubox[9], pk_uinv[9];
   int pk_invert;
+  /* tabulated grids (FrameworkModel GRID, src/input.c:5626-5630): borrowed
+   * node arrays [i][j][k][8] with their geometry, and the unit cell they
+   * wrap positions into */
+  const float *vgrid[ORC_MAX_TYPES];
+  double vgeom[ORC_MAX_TYPES][12];
+  const float *cgrid;
+  double cgeom[12];
+  double g_ubox[9], g_uinv[9];
 };
 
 /* LENNARD_JONES branch of PotentialValue (src/potentials.c:5867-5877) */
@@ -669,6 +677,8 @@ void orc_system_info(const orc_system *s, double out[8]) {
 /* ===================================================================== */
 double orc_vdw_energy_at(const orc_system *s, int type, const double pos[3], double scaling) {
   (void)scaling; /* the LENNARD_JONES branch does not use it */
+  /* grid branch (src/framework_energy.c:73-76) */
+  if (s->vgrid[type]) return orc_interpolate_grid(s->vgrid[type], s->vgeom[type], s->bc, s->g_ubox, s->g_uinv, pos);
   double UVDW = 0.0;
   const double *pp = s->pp[type];
   for (int i = 0; i < s->n; i++) {
@@ -684,6 +694,9 @@ double orc_charge_energy_at(const orc_system *s, int type, const double pos[3], 
   double U = 0.0;
   if (s->charge_method == ORC_CHARGE_NONE) return 0.0;
   if (!considered_charged(s->t_q[type])) return 0.0;
+  /* grid branch (src/framework_energy.c:197-200): the charge of the
+   * pseudo-atom, not scaled */
+  if (s->cgrid) return orc_interpolate_coulomb_grid(s->cgrid, s->cgeom, s->bc, s->g_ubox, s->g_uinv, pos, s->t_q[type]);
   double chargeA = scaling * s->t_q[type];
   for (int i = 0; i < s->n; i++) {
     double dr[3] = {pos[0] - s->x[i], pos[1] - s->y[i], pos[2] - s->z[i]};
@@ -705,6 +718,29 @@ void orc_energy_at_many(const orc_system *s, int type, int64_t n, const double *
     if (u_vdw) u_vdw[i] = orc_vdw_energy_at(s, type, xyz + 3 * i, 1.0);
     if (u_cc) u_cc[i] = orc_charge_energy_at(s, type, xyz + 3 * i, 1.0);
   }
+}
+
+int orc_use_grid(orc_system *s, int type, const float *grid, const double geom[12], const double unit_box[9]) {
+  if (type < 0 || type >= s->ntypes) return -1;
+  s->vgrid[type] = grid;
+  if (grid) {
+    memcpy(s->vgeom[type], geom, 12 * sizeof(double));
+    double det;
+    memcpy(s->g_ubox, unit_box, 9 * sizeof(double));
+    orc_invert3x3(s->g_ubox, s->g_uinv, &det);
+  }
+  return 0;
+}
+
+int orc_use_coulomb_grid(orc_system *s, const float *grid, const double geom[12], const double unit_box[9]) {
+  s->cgrid = grid;
+  if (grid) {
+    memcpy(s->cgeom, geom, 12 * sizeof(double));
+    double det;
+    memcpy(s->g_ubox, unit_box, 9 * sizeof(double));
+    orc_invert3x3(s->g_ubox, s->g_uinv, &det);
+  }
+  return 0;
 }
 
 /* ===================================================================== */

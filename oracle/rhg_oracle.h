@@ -197,6 +197,11 @@ void orc_grid_node(const orc_system *s, int type, const double geom[12], int i, 
 /* CalculateDerivativesAtPositionReal (src/framework_hessian.c:111) for a
  * unit charge; returns the smallest in-range distance */
 double orc_derivatives_real_at(const orc_system *s, const double pos[3], double out[14]);
+/* UseTabularGrid: energies at a position of `type` (VDW) or of every
+ * charged type (Coulomb) come from the grid (borrowed, [i][j][k][8] floats;
+ * null detaches), src/framework_energy.c:73-76, 197-200 */
+int orc_use_grid(orc_system *s, int type, const float *grid, const double geom[12], const double unit_box[9]);
+int orc_use_coulomb_grid(orc_system *s, const float *grid, const double geom[12], const double unit_box[9]);
 /* InterpolateCoulombGrid (src/grids.c:1200-1295) */
 double orc_interpolate_coulomb_grid(const float *grid, const double geom[12], int bc, const double ub[9],
                                     const double uinv[9], const double pos[3], double charge);

@@ -182,6 +182,10 @@ struct rhg_ctx {
   std::map<std::string, hipGraphExec_t> graph_cache;
   // host-side fault hook: fail the chunk after this many more chunks (<0 off)
   int64_t fault_after = -1;
+  // tabulated grids in use (FrameworkModel GRID): per pseudo-atom VDW grids
+  // and the Coulomb grid, borrowed from their owners
+  std::vector<rhg_grid *> vdw_grids;
+  rhg_grid *coulomb_grid = nullptr;
   // blocked pockets of the Widom path (device copy of the centres)
   PocketArgs pockets = {};
   double *d_pockets = nullptr;
@@ -207,6 +211,29 @@ struct rhg_grid {
 };
 
 namespace {
+
+InterpArgs interp_args(const rhg_grid *g, double charge) {
+  InterpArgs a;
+  memset(&a, 0, sizeof(a));
+  for (int d = 0; d < 3; ++d) {
+    a.n[d] = g->n[d];
+    a.size[d] = g->size[d];
+    a.shift[d] = g->shift[d];
+    a.ulen[d] = g->ulen[d];
+  }
+  memcpy(a.ubox, g->ubox, sizeof(a.ubox));
+  memcpy(a.uinv, g->uinv, sizeof(a.uinv));
+  a.bc = g->bc;
+  a.fp64 = g->fp64;
+  a.coulomb = g->coulomb;
+  a.charge = charge;
+  a.grid = g->d;
+  return a;
+}
+
+rhg_grid *vdw_grid_of(const rhg_ctx *c, int type) {
+  return type >= 0 && type < (int)c->vdw_grids.size() ? c->vdw_grids[type] : nullptr;
+}
 
 int fail(rhg_ctx *c, int code, const std::string &msg) {
   if (c) c->err = msg;
@@ -592,10 +619,37 @@ int ensure_image_list(rhg_ctx *c) {
   return RHG_OK;
 }
 
+int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const EnergyOut &o,
+                      const char *tag);
+
+// Site energies of one site list of pseudo-atom `type`: terms with a grid in
+// use are interpolated (src/framework_energy.c:73-76, 197-200: the Coulomb
+// grid value times the pseudo-atom's charge, unscaled), the others summed
+// over the framework atoms.
+int site_energy(rhg_ctx *c, int type, const EnergyArgs &a, const SiteList &s, const EnergyOut &o,
+                const char *tag) {
+  if (s.n <= 0) return RHG_OK;
+  rhg_grid *gv = a.do_vdw ? vdw_grid_of(c, type) : nullptr;
+  rhg_grid *gc = a.do_cc ? c->coulomb_grid : nullptr;
+  if (!gv && !gc) return site_energy_pairs(c, a, s, o, tag);
+  EnergyArgs rest = a;
+  rest.do_vdw = a.do_vdw && !gv;
+  rest.do_cc = a.do_cc && !gc;
+  const bool merge = rest.do_vdw || rest.do_cc;
+  if (merge)
+    if (int rc = site_energy_pairs(c, rest, s, o, tag)) return rc;
+  InterpArgs iv, ic;
+  if (gv) iv = interp_args(gv, 1.0);
+  if (gc) ic = interp_args(gc, c->types[type].q);
+  Timed tm(c, "grid_interp");
+  launch_site_grid(gv ? &iv : nullptr, gc ? &ic : nullptr, s, o, merge ? 1 : 0, c->stream);
+  return RHG_OK;
+}
+
 // Site energies of one site list, culled through the cell grid when enabled.
 // `tag` keeps the binning scratch of concurrent lists apart.
-int site_energy(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const EnergyOut &o,
-                const char *tag) {
+int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const EnergyOut &o,
+                      const char *tag) {
   if (s.n <= 0) return RHG_OK;
   if (!c->cull || c->n == 0) {
     Timed t(c, "site_energy");
@@ -852,7 +906,7 @@ int rhg_framework_energy_at_positions(rhg_ctx *c, int64_t n, const double *xyz, 
   if (!a.do_cc) fill(dc, n, 0.0, c->stream);
   SiteList s{dx, dy, dz, n, nullptr, 1};
   EnergyOut o{dv, a.do_cc ? dc : nullptr, dov};
-  if (int rc = site_energy(c, a, s, o, "e")) return rc;
+  if (int rc = site_energy(c, type, a, s, o, "e")) return rc;
   if (!a.do_cc) {
     // overlap from the VDW term only; the charge term is zero
   }
@@ -886,7 +940,11 @@ int rhg_framework_charge_energy_at_position(rhg_ctx *c, const double pos[3], int
   upload(c, d, pos, 3 * sizeof(double));
   SiteList s{d, d + 1, d + 2, 1, nullptr, 1};
   EnergyOut o{nullptr, dv, nullptr};
-  launch_site_energy(a, s, o, c->stream);
+  if (c->coulomb_grid) {
+    if (int rc = site_energy(c, type, a, s, o, "e")) return rc;
+  } else {
+    launch_site_energy(a, s, o, c->stream);
+  }
   HIPC(c, hipGetLastError());
   HIPC(c, hipMemcpyAsync(ucc, dv, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
@@ -937,7 +995,8 @@ int build_mol(rhg_ctx *c, const rhg_molecule *m, MolDev &d) {
   // first bead
   auto same_params = [&](int ta, int tb) {
     const PseudoAtom &A = c->types[ta], &B = c->types[tb];
-    return ta == tb || (A.eps_k == B.eps_k && A.sigma == B.sigma && A.q == B.q);
+    return ta == tb || (A.eps_k == B.eps_k && A.sigma == B.sigma && A.q == B.q &&
+                        vdw_grid_of(c, ta) == vdw_grid_of(c, tb));
   };
   auto group = [&](bool charged_only, int *start, int *size, int *b) {
     int next = 0;
@@ -1088,7 +1147,7 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
     // trials stored [trial][insertion]
     SiteList s{w.fbx, w.fby, w.fbz, n * m.k_fb, a.n_eff, m.k_fb, w.cap};
     EnergyOut o{w.fb_uv, ea.do_cc ? w.fb_uc : nullptr, nullptr};
-    if (int rc = site_energy(c, ea, s, o, "bfb")) return rc;
+    if (int rc = site_energy(c, type0, ea, s, o, "bfb")) return rc;
   }
   {
     Timed t(c, "widom_select");
@@ -1106,7 +1165,7 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
       const int rows = m.g_size[p] * m.k_or;
       SiteList s{w.orx + base, w.ory + base, w.orz + base, n * rows, w.n_active, rows, w.cap};
       EnergyOut o{w.or_uv + base, nullptr, nullptr};
-      if (int rc = site_energy(c, ea, s, o, "bor")) return rc;
+      if (int rc = site_energy(c, mol->type[m.place[p]], ea, s, o, "bor")) return rc;
     }
     {
       Timed t(c, "widom_select");
@@ -1126,7 +1185,7 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
       }
       SiteList s{w.ccx + base, w.ccy + base, w.ccz + base, n * rows, w.n_active2, rows, w.cap};
       EnergyOut o{nullptr, w.cc_place + base, nullptr};
-      if (int rc = site_energy(c, ea, s, o, "bcc")) return rc;
+      if (int rc = site_energy(c, mol->type[m.place[p]], ea, s, o, "bcc")) return rc;
     }
   }
   if (a.ewald) {
@@ -1744,21 +1803,7 @@ int interpolate(rhg_grid *g, int64_t n, const double *xyz, double charge, double
   double *du = (double *)work(c, "i_u", (size_t)n * sizeof(double));
   if (!dx || !du) return fail(c, RHG_ERR_OOM, "interp buffers");
   upload(c, dx, xyz, (size_t)n * 3 * sizeof(double));
-  InterpArgs a;
-  memset(&a, 0, sizeof(a));
-  for (int d = 0; d < 3; ++d) {
-    a.n[d] = g->n[d];
-    a.size[d] = g->size[d];
-    a.shift[d] = g->shift[d];
-    a.ulen[d] = g->ulen[d];
-  }
-  memcpy(a.ubox, g->ubox, sizeof(a.ubox));
-  memcpy(a.uinv, g->uinv, sizeof(a.uinv));
-  a.bc = g->bc;
-  a.fp64 = g->fp64;
-  a.coulomb = g->coulomb;
-  a.charge = charge;
-  a.grid = g->d;
+  const InterpArgs a = interp_args(g, charge);
   {
     Timed t(c, "grid_interp");
     launch_interp(a, dx, n, du, c->stream);
@@ -1896,6 +1941,28 @@ int rhg_grid_read(rhg_ctx *c, const char *path, int32_t coulomb, const double un
     return fail(c, RHG_ERR_HIP, std::string("grid upload: ") + hipGetErrorString(e));
   }
   *out = g.release();
+  return RHG_OK;
+}
+
+int rhg_use_grid(rhg_ctx *c, int32_t type, rhg_grid *g) {
+  if (!c) return RHG_ERR_INVALID;
+  if (int rc = check_type(c, type)) return rc;
+  if (g && (g->coulomb || g->device != c->device))
+    return fail(c, RHG_ERR_INVALID, "a VDW grid on the context's device is needed");
+  if ((int)c->vdw_grids.size() < (int)c->types.size()) c->vdw_grids.resize(c->types.size(), nullptr);
+  c->vdw_grids[type] = g;
+  c->gen++; // captured chunks hold the energy kernels
+  return RHG_OK;
+}
+
+int rhg_use_coulomb_grid(rhg_ctx *c, rhg_grid *g) {
+  if (!c) return RHG_ERR_INVALID;
+  if (g && (!g->coulomb || g->device != c->device))
+    return fail(c, RHG_ERR_INVALID, "a Coulomb grid on the context's device is needed");
+  if (g && c->charge_method != RHG_CHARGE_EWALD)
+    return fail(c, RHG_ERR_UNSUPPORTED, "the Coulomb grid needs the Ewald charge method");
+  c->coulomb_grid = g;
+  c->gen++;
   return RHG_OK;
 }
 

@@ -278,5 +278,9 @@ struct InterpArgs {
 };
 void launch_interp(const InterpArgs &a, const double *xyz, int64_t n, double *out,
                    hipStream_t st);
+// site energies from grids (gv and/or gc non-null); s.n bounds the grid of
+// threads, s.n_dev (when set) the live sites
+void launch_site_grid(const InterpArgs *gv, const InterpArgs *gc, const SiteList &s, const EnergyOut &o,
+                      int merge, hipStream_t st);
 
 } // namespace rhg

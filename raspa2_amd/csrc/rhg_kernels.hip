@@ -2252,11 +2252,8 @@ __device__ __forceinline__ void hermite(double t, double h[2][2]) {
   h[1][1] = t3 - t2;                   // derivative at corner 1
 }
 
-__global__ __launch_bounds__(256) void k_interp(const InterpArgs a, const double *__restrict__ xyz,
-                                                int64_t n, double *__restrict__ out) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  double pos[3] = {xyz[3 * t], xyz[3 * t + 1], xyz[3 * t + 2]};
+__device__ double interp_at(const InterpArgs &a, double px, double py, double pz) {
+  double pos[3] = {px, py, pz};
   {
 #pragma clang fp contract(off)
     if (a.bc == 1) {
@@ -2305,10 +2302,7 @@ __global__ __launch_bounds__(256) void k_interp(const InterpArgs a, const double
   }
   if (!a.coulomb)
     for (int c = 0; c < 8; ++c)
-      if (X[c][0] > 1e10) {
-        out[t] = 1e8;
-        return;
-      }
+      if (X[c][0] > 1e10) return 1e8;
   double hx[2][2], hy[2][2], hz[2][2];
   hermite(r[0], hx);
   hermite(r[1], hy);
@@ -2322,7 +2316,49 @@ __global__ __launch_bounds__(256) void k_interp(const InterpArgs a, const double
     for (int q = 0; q < 8; ++q)
       v += X[c][q] * (hx[cx0][dd[q][0]] * hy[cy0][dd[q][1]] * hz[cz0][dd[q][2]]);
   }
-  out[t] = a.coulomb ? a.charge * v : v;
+  return a.coulomb ? a.charge * v : v;
+}
+
+__global__ __launch_bounds__(256) void k_interp(const InterpArgs a, const double *__restrict__ xyz,
+                                                int64_t n, double *__restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  out[t] = interp_at(a, xyz[3 * t], xyz[3 * t + 1], xyz[3 * t + 2]);
+}
+
+// Site energies from tabulated grids (FrameworkModel GRID): the VDW grid of
+// the site's pseudo-atom and/or the Coulomb grid times its charge
+// (src/framework_energy.c:73-76, 197-200).  merge keeps the overlap flags a
+// pair kernel already wrote for the other term of the same sites.
+__global__ __launch_bounds__(256) void k_site_grid(const InterpArgs gv, const InterpArgs gc, int has_v,
+                                                   int has_c, const SiteList s, const EnergyOut o,
+                                                   int merge) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= site_count(s)) return;
+  const int64_t k = site_phys(s, i);
+  const double px = s.x[k], py = s.y[k], pz = s.z[k];
+  bool flag = false;
+  if (has_v) {
+    const double uv = interp_at(gv, px, py, pz);
+    if (o.u_vdw) o.u_vdw[k] = uv;
+    flag = flag || uv >= kOverlap;
+  }
+  if (has_c) {
+    const double uc = interp_at(gc, px, py, pz);
+    if (o.u_cc) o.u_cc[k] = uc;
+    flag = flag || uc >= kOverlap;
+  }
+  if (o.overlap) o.overlap[k] = (uint8_t)((merge && o.overlap[k]) || flag);
+}
+
+void launch_site_grid(const InterpArgs *gv, const InterpArgs *gc, const SiteList &s, const EnergyOut &o,
+                      int merge, hipStream_t st) {
+  const unsigned nb = nblk(s.n, 256);
+  if (!nb) return;
+  InterpArgs z;
+  memset(&z, 0, sizeof(z));
+  hipLaunchKernelGGL(k_site_grid, dim3(nb), dim3(256), 0, st, gv ? *gv : z, gc ? *gc : z, gv ? 1 : 0,
+                     gc ? 1 : 0, s, o, merge);
 }
 
 void launch_interp(const InterpArgs &a, const double *xyz, int64_t n, double *out, hipStream_t st) {

@@ -122,6 +122,8 @@ def _load():
     f("rhg_grid_write", _P, ctypes.c_char_p, _P)
     f("rhg_grid_read", _P, ctypes.c_char_p, _I32, _P, _P, ctypes.POINTER(_P))
     f("rhg_grid_meta", _P, _P)
+    f("rhg_use_grid", _P, _I32, _P)
+    f("rhg_use_coulomb_grid", _P, _P)
     f("rhg_profile_enable", _P, _I32)
     f("rhg_profile_get", _P, ctypes.c_char_p, ctypes.POINTER(_I64), ctypes.POINTER(_D))
     f("rhg_profile_reset", _P)
@@ -186,6 +188,7 @@ class Grid:
     def __init__(self, ctx, handle):
         self.ctx = ctx
         self.h = handle
+        self._users = weakref.WeakSet()  # contexts using this grid for energies
         ctx._grids.add(self)
         info = np.zeros(13)
         lib.rhg_grid_info(self.h, _ptr(info))
@@ -202,8 +205,11 @@ class Grid:
         self.ewald_precision = meta[8]
 
     def close(self):
-        """Free the device grid now (also done when the object is collected)."""
+        """Free the device grid now (also done when the object is collected);
+        contexts using it for energies stop doing so first."""
         if getattr(self, "h", None):
+            for ctx in list(getattr(self, "_users", ())):
+                ctx._detach(self)
             lib.rhg_grid_free(self.h)
             self.h = None
 
@@ -276,6 +282,7 @@ class Context:
                         ewald_precision, temperature)
         h = _P()
         self._grids = weakref.WeakSet()
+        self._in_use = {}
         rc = lib.rhg_create(ctypes.byref(fw), ctypes.byref(cell), ctypes.byref(ff), device,
                             ctypes.byref(h))
         if rc:
@@ -452,6 +459,32 @@ class Context:
         self._check(lib.rhg_grid_read(self.h, os.fsencode(path), 1 if coulomb else 0, _ptr(ub),
                                       None if nc is None else _ptr(nc), ctypes.byref(h)), "grid_read")
         return Grid(self, h)
+
+    def use_grid(self, type_a, grid):
+        """UseTabularGrid for pseudo-atom type_a (grid None detaches); the
+        context keeps the grid alive while it is in use."""
+        self._check(lib.rhg_use_grid(self.h, type_a, grid.h if grid is not None else None), "use_grid")
+        self._in_use[type_a] = grid
+        if grid is not None:
+            grid._users.add(self)
+
+    def use_coulomb_grid(self, grid):
+        self._check(lib.rhg_use_coulomb_grid(self.h, grid.h if grid is not None else None),
+                    "use_coulomb_grid")
+        self._in_use["coulomb"] = grid
+        if grid is not None:
+            grid._users.add(self)
+
+    def _detach(self, grid):
+        if not getattr(self, "h", None):
+            return
+        for key, g in list(self._in_use.items()):
+            if g is grid:
+                if key == "coulomb":
+                    lib.rhg_use_coulomb_grid(self.h, None)
+                else:
+                    lib.rhg_use_grid(self.h, key, None)
+                self._in_use[key] = None
 
     def CalculateDerivativesAtPositionReal(self, xyz):
         """(out n x 14, smallest_r n) of src/framework_hessian.c:111."""

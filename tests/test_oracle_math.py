@@ -412,3 +412,35 @@ def test_mt_driver_matches_stream_statistically(O):
     w_st = rec[:, 0].mean()
     se = rec[:, 0].std() / np.sqrt(10000)
     assert abs(w_mt - w_st) < 5 * np.sqrt(2) * se
+
+
+def test_tabular_grid_branches(O, W):
+    """UseTabularGrid in the oracle: the VDW energy of a type with a grid is
+    the interpolation (positions wrapped into the grid's unit cell), other
+    types stay direct; the Coulomb grid serves every charged type times its
+    charge, ignores the scaling, and never an uncharged type; detaching
+    restores the direct sums."""
+    grid, geom, ubox, uinv, f_and_derivs = _tricubic_grid(11)
+    fw = W.c1_framework(n_rep=1)
+    s = oracle_system(fw, charge_method=O.CHARGE_EWALD)
+    t_grid = s.add_pseudo_atom(148.0, 3.73, 0.0)
+    t_direct = s.add_pseudo_atom(148.0, 3.73, 0.0)
+    t_q = s.add_pseudo_atom(79.0, 3.05, -0.35)
+    rng = np.random.default_rng(11)
+    pts = rng.uniform(0.1, 1.9, (10, 3))
+    direct = [s.vdw_at(t_grid, p) for p in pts]
+    s.use_grid(t_grid, grid, geom, ubox)
+    s.use_coulomb_grid(grid, geom, ubox)
+    for p, u0 in zip(pts, direct):
+        poly = f_and_derivs(*p)[0]
+        tol = 1e-5 * max(1.0, np.abs(grid).max())
+        assert abs(s.vdw_at(t_grid, p) - poly) <= tol
+        # a lattice translation of the grid's unit cell gives the same value
+        assert abs(s.vdw_at(t_grid, p + np.array([2.0, -4.0, 6.0])) - s.vdw_at(t_grid, p)) <= 1e-9 * tol
+        assert s.vdw_at(t_direct, p) == u0
+        assert abs(s.charge_at(t_q, p) - (-0.35) * poly) <= 0.35 * tol
+        assert s.charge_at(t_q, p, scaling=0.5) == s.charge_at(t_q, p)
+        assert s.charge_at(t_direct, p) == 0.0
+    s.use_grid(t_grid, None, None, None)
+    s.use_coulomb_grid(None, None, None)
+    assert [s.vdw_at(t_grid, p) for p in pts] == direct
