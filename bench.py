@@ -172,6 +172,7 @@ def grid_bench(args):
         g.close()
         ctx.close()
     mixed = out["mixed"]
+    coulomb = coulomb_grid_bench(args)
     line = {
         "metric": "energy-grid nodes/s (MakeGrid with derivatives)", "value": round(mixed["nodes_per_s"], 1),
         "unit": "grid nodes/s", "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -181,7 +182,7 @@ def grid_bench(args):
                                "1200 atoms, 0.1 A spacing, 8 channels (value, 3 first, 3 mixed "
                                "second, 1 third derivative), FP32 storage",
                    "nodes": data["mixed"][0], "spacing_A": spacing, "parallelism": "single GPU"},
-        "plain_fp64": out["fp64"], "mixed": mixed,
+        "plain_fp64": out["fp64"], "mixed": mixed, "coulomb_grid": coulomb,
         "check": f"{n_cpu} seeded off-grid points: device interpolation of each device grid vs "
                  "the oracle's FP64 grid nodes and Lekien-Marsden interpolation",
         "roofline": None,
@@ -191,6 +192,71 @@ def grid_bench(args):
                                    "oracle FP64 all-pairs, one process per core"},
     }
     print(json.dumps(line))
+
+
+def coulomb_grid_bench(args):
+    """Ewald-Coulomb grid of the charged 25.8 A cubic cell (2x2x2 supercell,
+    4608 atoms) at 0.1 A: FP64 build, nodes checked against the oracle's at
+    seeded node indices, and the RASPA .grid file written and read back."""
+    import tempfile
+    import numpy as np
+    import oracle as O
+    from raspa2_amd import rhg as R
+    from raspa2_amd import workloads as W
+    fw = W.c1_framework()
+    spacing = 0.1
+    ctx = fw.context(charge_method=R.CHARGE_EWALD)
+    g = ctx.MakeCoulombGrid(fw.unit_box, 0.5)  # warm-up: cell lists, code objects
+    g.close()
+    ctx.synchronize()
+    ctx.profile(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    g = ctx.MakeCoulombGrid(fw.unit_box, spacing)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    _, ms = ctx.profile_get("grid_build")
+    ctx.profile(False)
+    nodes = int(np.prod(np.asarray(g.geom[:3]) + 1))
+    data = g.download()
+    geom = O.grid_geometry(fw.unit_box, spacing)
+    box, bc, x, y, z, q, eps, sig, mass = fw.oracle_args()
+    sys_o = O.System(box, bc, x, y, z, q, eps, sig, mass=mass, charge_method=O.CHARGE_EWALD)
+    rng = np.random.Generator(np.random.PCG64(SEED + 2))
+    idx = [tuple(int(rng.integers(0, s)) for s in data.shape[:3]) for _ in range(64)]
+    # CPU baseline leg: the oracle's FP64 nodes at the sampled indices, timed
+    # on one core; the same values check the device grid
+    t0 = time.perf_counter()
+    refs = [sys_o.coulomb_grid_node(geom, i, j, k) for (i, j, k) in idx]
+    cpu_s = time.perf_counter() - t0
+    worst = 0.0
+    capped_same = True
+    for (i, j, k), ref in zip(idx, refs):
+        got = data[i, j, k]
+        capped_same &= bool((ref[0] == np.float32(2e7)) == (got[0] == np.float32(2e7)))
+        ulp = np.spacing(np.abs(ref)).astype(np.float64)
+        worst = max(worst, float(np.max(np.abs(got.astype(np.float64) - ref) / np.maximum(ulp, 1e-30))))
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "c1_Electrostatics_Ewald.grid")
+        t0 = time.perf_counter()
+        g.write(path)
+        t_w = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        t0 = time.perf_counter()
+        h = ctx.ReadGrid(path, fw.unit_box, coulomb=True, n_unit_cells=(2, 2, 2))
+        t_r = time.perf_counter() - t0
+        same = bool(np.array_equal(h.download().view(np.uint32), data.view(np.uint32)))
+        h.close()
+    g.close()
+    ctx.close()
+    return {"workload": "Ewald-Coulomb grid (unit charge, real space, alpha from 1e-6), 25.8 A cubic "
+                        "cell of the 2x2x2 supercell, 4608 atoms, 0.1 A spacing, FP64 compute, FP32 storage",
+            "nodes": nodes, "nodes_per_s": nodes / wall, "wall_s": wall, "kernel_ms": ms,
+            "check": f"{len(idx)} seeded nodes vs the oracle: cap flags equal {capped_same}, "
+                     f"worst channel difference {worst:.1f} float32 ulps",
+            "file": {"bytes": size, "write_s": t_w, "read_s": t_r, "read_back_bit_identical": same},
+            "cpu_baseline": {"value": round(len(idx) / cpu_s, 1), "unit": "grid nodes/s", "cores": 1,
+                             "kind": "port", "sample": f"{len(idx)} seeded nodes, oracle FP64 all-pairs"}}
 
 
 def screening_bench(args):
@@ -428,6 +494,14 @@ def main():
         "estimators": {"henry_mol_kg_pa": est["kh_overall"], "u_gh_K": est["u"],
                        "w": est["w_overall"]},
         "work": {k: float(v) for k, v in st.items()},
+        # trial positions (first-bead trials, orientation-trial sites and the
+        # Coulomb sites of the selected orientations) and pair evaluations per
+        # second of the timed steps; "in range" counts the all-pairs algorithm's
+        # pairs, "executed" what the culled kernels evaluated
+        "positions_per_s": round((st["fb_sites"] + st["or_sites"] + st["cc_sites"]) / elapsed, 1),
+        "pair_evals_per_s": {
+            "all_pairs_equivalent": round((st["lj_pairs"] + st["ljcc_pairs"] + st["cc_pairs"]) / elapsed, 1),
+            "executed": round((st["exec_lj_pairs"] + st["exec_ljcc_pairs"] + st["exec_cc_pairs"]) / elapsed, 1)},
         "kernels": kernels,
     }
     print(json.dumps(out))
