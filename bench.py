@@ -173,6 +173,7 @@ def grid_bench(args):
         ctx.close()
     mixed = out["mixed"]
     coulomb = coulomb_grid_bench(args)
+    deck = grid_deck_bench(args)
     line = {
         "metric": "energy-grid nodes/s (MakeGrid with derivatives)", "value": round(mixed["nodes_per_s"], 1),
         "unit": "grid nodes/s", "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -182,7 +183,7 @@ def grid_bench(args):
                                "1200 atoms, 0.1 A spacing, 8 channels (value, 3 first, 3 mixed "
                                "second, 1 third derivative), FP32 storage",
                    "nodes": data["mixed"][0], "spacing_A": spacing, "parallelism": "single GPU"},
-        "plain_fp64": out["fp64"], "mixed": mixed, "coulomb_grid": coulomb,
+        "plain_fp64": out["fp64"], "mixed": mixed, "coulomb_grid": coulomb, "grid_decks": deck,
         "check": f"{n_cpu} seeded off-grid points: device interpolation of each device grid vs "
                  "the oracle's FP64 grid nodes and Lekien-Marsden interpolation",
         "roofline": None,
@@ -257,6 +258,87 @@ def coulomb_grid_bench(args):
             "file": {"bytes": size, "write_s": t_w, "read_s": t_r, "read_back_bit_identical": same},
             "cpu_baseline": {"value": round(len(idx) / cpu_s, 1), "unit": "grid nodes/s", "cores": 1,
                              "kind": "port", "sample": f"{len(idx)} seeded nodes, oracle FP64 all-pairs"}}
+
+
+def grid_deck_bench(args):
+    """The reference's MakingGrids/UsingGrids example decks: IRMOF-1 with the
+    Dubbeldam2007FlexibleIRMOF-1 force field, C_co2 and O_co2 grids and the
+    Ewald-Coulomb grid at 0.1 A, written under RASPA2's file names and read
+    back; then CO2 Widom insertion served from the grids against the direct
+    (mixed) sums over the same trials."""
+    import tempfile
+    import numpy as np
+    from raspa2_amd import rhg as R
+    from raspa2_amd import workloads as W
+    fw, probe, D = W.irmof1_dubbeldam(os.path.join(ROOT, "tests", "golden"))
+    sp = D["spacing_vdw"]
+
+    def context(prec):
+        c = fw.context(charge_method=R.CHARGE_EWALD, ewald_precision=D["ewald_precision"],
+                       temperature=D["temperature"], precision=prec)
+        types = [c.add_pseudo_atom(*s) for s in probe.sites]
+        return c, types, R.molecule(types, probe.body, start=probe.start, k_fb=probe.k_fb,
+                                    k_or=probe.k_or)
+
+    names = [n for n, _ in D["co2"]]
+    ctx, types, mol = context(R.PREC_FP64)
+    for g in (ctx.MakeGrid(types[0], fw.unit_box, 0.5), ctx.MakeCoulombGrid(fw.unit_box, 0.5)):
+        g.close()  # warm-up
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    grids = {n: ctx.MakeGrid(types[names.index(n)], fw.unit_box, sp) for n in D["grid_types"]}
+    coul = ctx.MakeCoulombGrid(fw.unit_box, D["spacing_coulomb"])
+    ctx.synchronize()
+    t_build = time.perf_counter() - t0
+    with tempfile.TemporaryDirectory() as root:
+        t0 = time.perf_counter()
+        paths = {}
+        for n, g in grids.items():
+            paths[n] = R.grid_path(root, D["forcefield"], D["framework"], sp, pseudo_atom=n,
+                                   shifted=D["shifted"])
+            os.makedirs(os.path.dirname(paths[n]), exist_ok=True)
+            g.write(paths[n])
+        pc = R.grid_path(root, D["forcefield"], D["framework"], D["spacing_coulomb"],
+                         n_unit_cells=D["unit_cells"])
+        os.makedirs(os.path.dirname(pc), exist_ok=True)
+        coul.write(pc)
+        t_write = time.perf_counter() - t0
+        for g in list(grids.values()) + [coul]:
+            g.close()
+        t0 = time.perf_counter()
+        read = {n: ctx.ReadGrid(p, fw.unit_box) for n, p in paths.items()}
+        rc = ctx.ReadGrid(pc, fw.unit_box, coulomb=True, n_unit_cells=D["unit_cells"])
+        t_read = time.perf_counter() - t0
+    for j, n in enumerate(names):
+        ctx.use_grid(types[j], read[n])
+    ctx.use_coulomb_grid(rc)
+    direct, dtypes, dmol = context(R.PREC_MIXED)
+    n = 1 << 20
+    rates = {}
+    for name, c, m in (("grids", ctx, mol), ("direct_mixed", direct, dmol)):
+        c.widom_run(m, SEED, 4 * n, first=0, n=n)
+        c.synchronize()
+        t0 = time.perf_counter()
+        for s in range(3):
+            c.widom_run(m, SEED, 4 * n, first=(s + 1) * n, n=n)
+        c.synchronize()
+        rates[name] = 3 * n / (time.perf_counter() - t0)
+    _, rec = ctx.widom_run(mol, SEED, 200000, records=True)
+    _, rec0 = direct.widom_run(dmol, SEED, 200000, records=True)
+    ratio = float(rec[:, 0].mean() / rec0[:, 0].mean())
+    nodes = int(np.prod(np.asarray(rc.geom[:3]) + 1))
+    for g in list(read.values()) + [rc]:
+        g.close()
+    ctx.close()
+    direct.close()
+    return {"workload": "examples/Auxiliary/MakingGrids + UsingGrids: IRMOF-1 (424 atoms, 1x1x1), "
+                        "Dubbeldam2007FlexibleIRMOF-1, grids C_co2, O_co2 and Ewald-Coulomb at 0.1 A; "
+                        "TraPPE CO2 Widom (k_fb=1, k_or=10) at 298 K",
+            "nodes_per_grid": nodes,
+            "build_3_grids_s": t_build, "write_s": t_write, "read_s": t_read,
+            "widom_insertions_per_s": {k: round(v, 1) for k, v in rates.items()},
+            "w_ratio_grids_over_direct": ratio,
+            "w_check": "mean Widom weight over the same 200000 insertions, grids vs direct mixed sums"}
 
 
 def screening_bench(args):

@@ -228,3 +228,29 @@ def block_pocket_centres(fractional, radii, box, n_cells=(1, 1, 1), shift=(0.0, 
                     cen.append(v @ B)
                     rad.append(radii[i])
     return np.array(cen), np.array(rad)
+
+
+def irmof1_dubbeldam(golden_dir):
+    """IRMOF-1 (the P1 expansion of the reference's CIF, one unit cell) with
+    the Dubbeldam2007FlexibleIRMOF-1 parameters and charges, the TraPPE CO2
+    probe (growth from bead 0, RASPA2's default) and the settings of the
+    reference's MakingGrids/UsingGrids example decks, read from the fixtures
+    irmof1.json and irmof1_dubbeldam.json in golden_dir.  Returns
+    (FrameworkSpec, Probe, deck settings)."""
+    import json
+    import os
+    G = json.load(open(os.path.join(golden_dir, "irmof1.json")))
+    D = json.load(open(os.path.join(golden_dir, "irmof1_dubbeldam.json")))
+    box = cell_box(*G["cell"])
+    frac = np.array([a["frac"] for a in G["atoms"]])
+    xyz = frac @ box.reshape(3, 3)
+    T = D["framework_types"]
+    lab = [a["label"] for a in G["atoms"]]
+    fw = FrameworkSpec("IRMOF-1", box, R.BC_RECTANGULAR, xyz,
+                       np.array([T[l]["eps_k"] for l in lab]), np.array([T[l]["sigma"] for l in lab]),
+                       np.array([T[l]["charge"] for l in lab]), np.array([T[l]["mass"] for l in lab]),
+                       unit_box=box)
+    P = D["probe_types"]
+    probe = Probe("CO2", [(P[n]["eps_k"], P[n]["sigma"], P[n]["charge"]) for n, _ in D["co2"]],
+                  np.array([p for _, p in D["co2"]]), start=0)
+    return fw, probe, D

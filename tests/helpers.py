@@ -38,3 +38,10 @@ def same_bits(a, b):
     a = np.ascontiguousarray(a, dtype=np.float64)
     b = np.ascontiguousarray(b, dtype=np.float64)
     return a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+def irmof1_dubbeldam():
+    """The reference's grid example decks (IRMOF-1, Dubbeldam2007FlexibleIRMOF-1,
+    TraPPE CO2) from the committed fixtures."""
+    import os
+    return W.irmof1_dubbeldam(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
