@@ -21,7 +21,7 @@ MAX_SITES = 8
 
 REC = dict(W=0, DU=1, UVDW=2, UREAL=3, UFHOST=4, UFADS=5, OVERLAP=6, SELFB=7, SELOR=8,
            FBMASK=9, ORMASK=10, W1=11, W2=12, STAGE=13)
-REC_N = 14
+REC_N = 16
 
 
 def build():
@@ -104,6 +104,10 @@ def _declare(L):
     f("orc_set_block_pockets", _I, _P, _I, _P, _P, _P, _I)
     f("orc_derivatives_real_at", _D, _P, _P, _P)
     f("orc_use_grid", _I, _P, _I, _P, _P, _P)
+    f("orc_set_adsorbates", _I, _P, _I, _P, _P, _P)
+    f("orc_inter_vdw_at", _D, _P, _I, _P, _I, _D)
+    f("orc_inter_charge_at", _D, _P, _I, _P, _I, _D)
+    f("orc_fourier_move", None, _P, _I, _P, _P, _I, _P)
     f("orc_use_coulomb_grid", _I, _P, _P, _P, _P)
     f("orc_coulomb_grid_node", None, _P, _P, _I, _I, _I, _P)
     f("orc_blocked_pocket", _I, _P, _P)
@@ -189,7 +193,14 @@ class System:
             pass
 
     def add_pseudo_atom(self, eps_k, sigma, charge=0.0):
-        return lib().orc_add_pseudo_atom(self.h, eps_k, sigma, charge)
+        t = lib().orc_add_pseudo_atom(self.h, eps_k, sigma, charge)
+        if not hasattr(self, "_tq"):
+            self._tq = {}
+        self._tq[t] = float(charge)
+        return t
+
+    def t_q(self, t):
+        return self._tq[t]
 
     def pair_params(self, t, atom):
         out = np.zeros(3)
@@ -266,6 +277,33 @@ class System:
         out = np.zeros(14)
         r = lib().orc_derivatives_real_at(self.h, _ptr(_f64(pos)), _ptr(out))
         return out, r
+
+    def set_adsorbates(self, start, types, xyz):
+        """Adsorbate molecules present: molecule m holds atoms
+        [start[m], start[m+1]) of the given pseudo-atom types at xyz."""
+        st = np.ascontiguousarray(start, dtype=np.int32)
+        ty = np.ascontiguousarray(types, dtype=np.int32)
+        xy = _f64(xyz).reshape(-1, 3)
+        self._ads = (st, ty, xy)
+        n_mol = len(st) - 1
+        assert lib().orc_set_adsorbates(self.h, n_mol, _ptr(st), _ptr(ty), _ptr(xy)) == 0
+
+    def inter_vdw_at(self, t, pos, exclude=-1):
+        return lib().orc_inter_vdw_at(self.h, t, _ptr(_f64(pos)), exclude, 1.0)
+
+    def inter_charge_at(self, t, pos, exclude=-1, charge=None):
+        q = self.t_q(t) if charge is None else charge
+        return lib().orc_inter_charge_at(self.h, t, _ptr(_f64(pos)), exclude, q)
+
+    def fourier_move(self, new_types=(), new_xyz=None, old_mol=-1):
+        """CalculateEwaldFourierAdsorbate(New, Old, mol): (host, ads, self,
+        excluded, sum) for a new molecule and/or removing adsorbate old_mol."""
+        out = np.zeros(5)
+        ty = np.ascontiguousarray(list(new_types), dtype=np.int32)
+        xy = _f64(new_xyz if new_xyz is not None else np.zeros((0, 3))).reshape(-1, 3)
+        lib().orc_fourier_move(self.h, len(ty), _ptr(ty) if len(ty) else None, _ptr(xy) if len(ty) else None,
+                               old_mol, _ptr(out))
+        return out
 
     def use_grid(self, t, grid, geom, unit_box):
         """Energies of type t at a position come from the grid (None

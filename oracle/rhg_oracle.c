@@ -327,6 +327,12 @@ struct orc_system {
   const float *cgrid;
   double cgeom[12];
   double g_ubox[9], g_uinv[9];
+  /* adsorbates present (finite loading) */
+  int ads_nmol, ads_n;
+  int *ads_start, *ads_type, *ads_mol;
+  double *ads_xyz;
+  cplx *sads;           /* StoreTotalChargeAdsorbates, k order */
+  double net_charge_ads; /* NetChargeAdsorbates */
 };
 
 /* LENNARD_JONES branch of PotentialValue (src/potentials.c:5867-5877) */
@@ -407,6 +413,15 @@ int orc_add_pseudo_atom(orc_system *s, double eps_k, double sigma, double charge
     s->pp[t][3 * i + 2] = shift;
   }
   return t;
+}
+
+static void pair_lj(const orc_system *s, int ta, int tb, double out[3]) {
+  double ei = s->t_eps[ta] * s->u.K2E, ej = s->t_eps[tb] * s->u.K2E;
+  double e = sqrt(ei * ej);
+  double sg = 0.5 * (s->t_sig[ta] + s->t_sig[tb]);
+  out[0] = e;
+  out[1] = sg;
+  out[2] = s->shifted ? lj_value(e, sg, 0.0, s->rc_vdw2) : 0.0;
 }
 
 void orc_pair_params(const orc_system *s, int type, int atom, double out[3]) {
@@ -653,6 +668,13 @@ orc_system *orc_system_create(const double box[9], int bc, int n_atoms, const do
 }
 
 void orc_system_destroy(orc_system *s) {
+  if (s) {
+    free(s->ads_start);
+    free(s->ads_type);
+    free(s->ads_mol);
+    free(s->ads_xyz);
+    free(s->sads);
+  }
   if (s) free(s->pk);
   if (!s) return;
   free(s->x); free(s->y); free(s->z); free(s->q); free(s->eps); free(s->sig);
@@ -797,67 +819,194 @@ double orc_ewald_real_energy_framework(const orc_system *s) {
 
 /* CalculateEwaldFourierAdsorbate(TRUE,FALSE,mol,0) at zero loading
  * (src/ewald.c:6859-7565); no bond dipoles, no cations. */
-void orc_fourier_adsorbate(const orc_system *s, int n_sites, const int *types,
-                           const double *xyz, double out[5]) {
-  double charge[ORC_MAX_SITES], fpos[ORC_MAX_SITES][3], cart[ORC_MAX_SITES][3];
-  int site_atom[ORC_MAX_SITES];
+void orc_fourier_move(const orc_system *s, int n_new, const int *new_types, const double *new_xyz,
+                      int old_mol, double out[5]) {
+  double charge[2 * ORC_MAX_SITES], fpos[2 * ORC_MAX_SITES][3], cart[2 * ORC_MAX_SITES][3];
   int nc = 0;
-  double energy_self_new = 0.0, net_charge_new = 0.0;
+  double energy_self_new = 0.0, energy_self_old = 0.0, net_charge_new = 0.0, net_charge_old = 0.0;
   double alpha = s->alpha;
   for (int i = 0; i < 5; i++) out[i] = 0.0;
   if (s->charge_method != ORC_CHARGE_EWALD) return;
-  for (int i = 0; i < n_sites; i++) {
-    int type = types[i];
+  /* old molecule's charged atoms first (src/ewald.c:6944-6962) */
+  if (old_mol >= 0) {
+    for (int a = s->ads_start[old_mol]; a < s->ads_start[old_mol + 1]; a++) {
+      double q = s->t_q[s->ads_type[a]];
+      if (considered_charged(q)) {
+        charge[nc] = q * 1.0;
+        energy_self_old += s->u.C * SQR(charge[nc]) * alpha / sqrt(M_PI);
+        net_charge_old += charge[nc];
+        const double *p = s->ads_xyz + 3 * a;
+        frac_2pi(s, p, fpos[nc]);
+        cart[nc][0] = p[0];
+        cart[nc][1] = p[1];
+        cart[nc][2] = p[2];
+        nc++;
+      }
+    }
+  }
+  const int nc_old = nc;
+  /* then the new molecule's (src/ewald.c:6965-6981) */
+  for (int i = 0; i < n_new; i++) {
+    int type = new_types[i];
     if (considered_charged(s->t_q[type])) {
       double scaling = 1.0;
       charge[nc] = scaling * s->t_q[type];
       energy_self_new += s->u.C * SQR(charge[nc]) * alpha / sqrt(M_PI);
       net_charge_new += charge[nc];
-      double p[3] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+      double p[3] = {new_xyz[3 * i], new_xyz[3 * i + 1], new_xyz[3 * i + 2]};
       frac_2pi(s, p, fpos[nc]);
       cart[nc][0] = p[0];
       cart[nc][1] = p[1];
       cart[nc][2] = p[2];
-      site_atom[nc] = i;
       nc++;
     }
   }
-  (void)site_atom;
-  cplx *sn = (cplx *)malloc(sizeof(cplx) * (size_t)(s->nk > 0 ? s->nk : 1));
-  structure_factors(s, nc, charge, (const double(*)[3])fpos, sn);
+  const int nc_new = nc - nc_old;
+  size_t nk = (size_t)(s->nk > 0 ? s->nk : 1);
+  cplx *so = (cplx *)malloc(sizeof(cplx) * nk), *sn = (cplx *)malloc(sizeof(cplx) * nk);
+  structure_factors(s, nc_old, charge, (const double(*)[3])fpos, so);
+  structure_factors(s, nc_new, charge + nc_old, (const double(*)[3])(fpos + nc_old), sn);
   double energy_charge_framework_adsorbates = 0.0, energy_charge_adsorbates = 0.0;
   for (int k = 0; k < s->nk; k++) {
+    /* src/ewald.c:7233-7259 */
+    cplx st = s->sads ? s->sads[k] : (cplx){0.0, 0.0};
     cplx sa;
-    sa.re = 0.0 + (sn[k].re - 0.0);
-    sa.im = 0.0 + (sn[k].im - 0.0);
+    sa.re = st.re + (sn[k].re - so[k].re);
+    sa.im = st.im + (sn[k].im - so[k].im);
     double temp = s->kfactor[k];
-    energy_charge_framework_adsorbates += temp * (s->sf[k].re * (sa.re - 0.0) + s->sf[k].im * (sa.im - 0.0));
-    energy_charge_adsorbates += temp * (SQR(sa.re) - SQR(0.0) + SQR(sa.im) - SQR(0.0));
+    energy_charge_framework_adsorbates += temp * (s->sf[k].re * (sa.re - st.re) + s->sf[k].im * (sa.im - st.im));
+    energy_charge_adsorbates += temp * (SQR(sa.re) - SQR(st.re) + SQR(sa.im) - SQR(st.im));
   }
+  free(so);
   free(sn);
-  /* excluded intra pairs: all charged pairs i<j (src/molecule.c:2839-2851,
-   * src/ewald.c:7318-7335) */
-  double energy_excluded_new = 0.0;
-  for (int a = 0; a < nc; a++)
-    for (int b = a + 1; b < nc; b++) {
-      double dx = cart[a][0] - cart[b][0];
-      double dy = cart[a][1] - cart[b][1];
-      double dz = cart[a][2] - cart[b][2];
-      double r = sqrt(SQR(dx) + SQR(dy) + SQR(dz));
-      double Bt0 = -erf(alpha * r) / r;
-      energy_excluded_new -= s->u.C * charge[a] * charge[b] * Bt0;
-    }
-  double net_delta = net_charge_new - 0.0;
-  double ads = energy_charge_adsorbates - (energy_excluded_new - 0.0) - (energy_self_new - 0.0);
-  ads += s->uion * SQR(0.0 + net_delta) - s->uion * SQR(0.0);
+  /* excluded intra pairs: all charged pairs i<j of each molecule
+   * (src/molecule.c:2839-2851, src/ewald.c:7318-7335, 7418-7440) */
+  double energy_excluded_new = 0.0, energy_excluded_old = 0.0;
+  for (int part = 0; part < 2; part++) {
+    int b0 = part ? nc_old : 0, b1 = part ? nc : nc_old;
+    double e = 0.0;
+    for (int a = b0; a < b1; a++)
+      for (int b = a + 1; b < b1; b++) {
+        double dx = cart[a][0] - cart[b][0];
+        double dy = cart[a][1] - cart[b][1];
+        double dz = cart[a][2] - cart[b][2];
+        double r = sqrt(SQR(dx) + SQR(dy) + SQR(dz));
+        double Bt0 = -erf(alpha * r) / r;
+        e -= s->u.C * charge[a] * charge[b] * Bt0;
+      }
+    if (part) energy_excluded_new = e;
+    else energy_excluded_old = e;
+  }
+  /* src/ewald.c:7536-7565 */
+  double net_delta = net_charge_new - net_charge_old;
+  double nca = s->net_charge_ads;
+  double ads = energy_charge_adsorbates - (energy_excluded_new - energy_excluded_old) -
+               (energy_self_new - energy_self_old);
+  ads += s->uion * SQR(nca + net_delta) - s->uion * SQR(nca);
   double host = 2.0 * energy_charge_framework_adsorbates;
-  host += 2.0 * s->uion * s->net_charge_framework * (0.0 + net_delta) -
-          2.0 * s->uion * s->net_charge_framework * (0.0);
+  host += 2.0 * s->uion * s->net_charge_framework * (nca + net_delta) -
+          2.0 * s->uion * s->net_charge_framework * (nca);
   out[0] = host;
   out[1] = ads;
-  out[2] = energy_self_new;
-  out[3] = energy_excluded_new;
+  out[2] = energy_self_new - energy_self_old;
+  out[3] = energy_excluded_new - energy_excluded_old;
   out[4] = energy_charge_adsorbates;
+}
+
+void orc_fourier_adsorbate(const orc_system *s, int n_sites, const int *types,
+                           const double *xyz, double out[5]) {
+  orc_fourier_move(s, n_sites, types, xyz, -1, out);
+}
+
+/* LB-mixed LJ parameters of the pair (typeA, typeB) as orc_add_pseudo_atom
+ * forms them for framework atoms: eps (internal units), sigma, shift */
+static void pair_lj(const orc_system *s, int ta, int tb, double out[3]);
+
+int orc_set_adsorbates(orc_system *s, int n_mol, const int *start, const int *types, const double *xyz) {
+  free(s->ads_start);
+  free(s->ads_type);
+  free(s->ads_mol);
+  free(s->ads_xyz);
+  free(s->sads);
+  s->ads_start = s->ads_type = s->ads_mol = NULL;
+  s->ads_xyz = NULL;
+  s->sads = NULL;
+  s->ads_nmol = s->ads_n = 0;
+  s->net_charge_ads = 0.0;
+  if (n_mol <= 0) return 0;
+  int n = start[n_mol];
+  for (int i = 0; i < n; i++)
+    if (types[i] < 0 || types[i] >= s->ntypes) return -1;
+  s->ads_nmol = n_mol;
+  s->ads_n = n;
+  s->ads_start = (int *)malloc(sizeof(int) * (size_t)(n_mol + 1));
+  s->ads_type = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  s->ads_mol = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  s->ads_xyz = (double *)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+  memcpy(s->ads_start, start, sizeof(int) * (size_t)(n_mol + 1));
+  memcpy(s->ads_type, types, sizeof(int) * (size_t)n);
+  memcpy(s->ads_xyz, xyz, sizeof(double) * 3 * (size_t)n);
+  for (int m = 0; m < n_mol; m++)
+    for (int a = start[m]; a < start[m + 1]; a++) s->ads_mol[a] = m;
+  /* StoreTotalChargeAdsorbates over every charged adsorbate atom in order */
+  double *q = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  double(*fp)[3] = (double(*)[3])malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+  int nc = 0;
+  for (int a = 0; a < n; a++) {
+    double qa = s->t_q[types[a]];
+    if (considered_charged(qa)) {
+      q[nc] = qa;
+      s->net_charge_ads += qa;
+      frac_2pi(s, xyz + 3 * a, fp[nc]);
+      nc++;
+    }
+  }
+  if (s->charge_method == ORC_CHARGE_EWALD) {
+    s->sads = (cplx *)malloc(sizeof(cplx) * (size_t)(s->nk > 0 ? s->nk : 1));
+    structure_factors(s, nc, q, (const double(*)[3])fp, s->sads);
+  }
+  free(q);
+  free(fp);
+  return 0;
+}
+
+double orc_inter_vdw_at(const orc_system *s, int type, const double pos[3], int exclude, double scaling) {
+  (void)scaling; /* the LENNARD_JONES branch does not use it */
+  double Uvdw = 0.0;
+  for (int j = 0; j < s->ads_nmol; j++) {
+    if (j == exclude) continue;
+    for (int a = s->ads_start[j]; a < s->ads_start[j + 1]; a++) {
+      double dr[3] = {pos[0] - s->ads_xyz[3 * a], pos[1] - s->ads_xyz[3 * a + 1], pos[2] - s->ads_xyz[3 * a + 2]};
+      orc_min_image(s->box, s->inv, s->bc, dr);
+      double r2 = SQR(dr[0]) + SQR(dr[1]) + SQR(dr[2]);
+      if (r2 < s->rc_vdw2) {
+        double pp[3];
+        pair_lj(s, type, s->ads_type[a], pp);
+        Uvdw += lj_value(pp[0], pp[1], pp[2], r2);
+      }
+    }
+  }
+  return Uvdw;
+}
+
+double orc_inter_charge_at(const orc_system *s, int type, const double pos[3], int exclude, double charge_a) {
+  (void)type;
+  double U = 0.0;
+  if (s->charge_method == ORC_CHARGE_NONE) return 0.0; /* PotentialValueCoulombic NONE */
+  for (int j = 0; j < s->ads_nmol; j++) {
+    if (j == exclude) continue;
+    for (int a = s->ads_start[j]; a < s->ads_start[j + 1]; a++) {
+      double chargeB = 1.0 * s->t_q[s->ads_type[a]];
+      double dr[3] = {pos[0] - s->ads_xyz[3 * a], pos[1] - s->ads_xyz[3 * a + 1], pos[2] - s->ads_xyz[3 * a + 2]};
+      orc_min_image(s->box, s->inv, s->bc, dr);
+      double rr = SQR(dr[0]) + SQR(dr[1]) + SQR(dr[2]);
+      if (rr < s->rc_charge2) {
+        double r = sqrt(rr);
+        U += s->u.C * charge_a * chargeB * (erfc(s->alpha * r) / r);
+      }
+    }
+  }
+  return U;
 }
 
 /* ===================================================================== */
@@ -953,8 +1102,11 @@ static void widom_core(const orc_system *s, const orc_molecule *m, draw_t *d, do
   const double beta = s->beta;
   int k_fb = m->k_fb, k_or = m->k_or;
   int start = m->start;
-  double bf[64], evdw[64], ecc[64];
+  double bf[64], evdw[64], ecc[64], eav[64], eac[64];
   int ov[64];
+  /* the inserted molecule is adsorbate number ads_nmol: nothing excluded
+   * (src/mc_moves.c:7856) */
+  const int excl = s->ads_nmol;
   double fbpos[64][3];
   for (int i = 0; i < ORC_REC_N; i++) rec[i] = 0.0;
 
@@ -984,7 +1136,10 @@ static void widom_core(const orc_system *s, const orc_molecule *m, draw_t *d, do
     }
     double Uvdw = orc_vdw_energy_at(s, type0, fbpos[i], 1.0);
     double Ucc = orc_charge_energy_at(s, type0, fbpos[i], 1.0);
-    if ((Uvdw >= crit) || (Ucc >= crit)) {
+    /* guest-guest terms (src/cbmc.c:683-689) */
+    double Uav = orc_inter_vdw_at(s, type0, fbpos[i], excl, 1.0);
+    double Uac = orc_inter_charge_at(s, type0, fbpos[i], excl, 1.0 * s->t_q[type0]);
+    if ((Uvdw >= crit) || (Ucc >= crit) || (Uav >= crit) || (Uac >= crit)) {
       ov[i] = 1;
       fbmask |= (1ULL << i);
     } else {
@@ -992,7 +1147,9 @@ static void widom_core(const orc_system *s, const orc_molecule *m, draw_t *d, do
       OVERLAP = 0;
       evdw[i] = Uvdw;
       ecc[i] = Ucc;
-      bf[i] = -beta * (Uvdw + 0.0 + 0.0);
+      eav[i] = Uav;
+      eac[i] = Uac;
+      bf[i] = -beta * (Uvdw + Uav + 0.0);
     }
   }
   rec[ORC_REC_FBMASK] = (double)fbmask;
@@ -1018,6 +1175,8 @@ static void widom_core(const orc_system *s, const orc_molecule *m, draw_t *d, do
   double RosenbluthNew = W1;
   double UHostVDWNew = evdw[sel];
   double UHostChargeChargeNew = ecc[sel];
+  double UAdsorbateVDWNew = eav[sel];
+  double UAdsorbateChargeChargeNew = eac[sel];
   double newpos[ORC_MAX_SITES][3];
   newpos[start][0] = fbpos[sel][0];
   newpos[start][1] = fbpos[sel][1];
@@ -1026,7 +1185,7 @@ static void widom_core(const orc_system *s, const orc_molecule *m, draw_t *d, do
   if (m->n_sites > 1) {
     /* Rosen -> GenerateTrialOrientationsSimpleSphere -> ComputeExternalEnergies */
     double tpos[64][ORC_MAX_SITES][3];
-    double uv[64], uc[64];
+    double uv[64], uc[64], ua[64], uac[64];
     int ovo[64];
     uint64_t ormask = 0;
     int n = m->n_sites;
@@ -1061,7 +1220,7 @@ static void widom_core(const orc_system *s, const orc_molecule *m, draw_t *d, do
     }
     int ALLOV = 1;
     for (int iu = 0; iu < k_or; iu++) {
-      double EV = 0.0, EC = 0.0;
+      double EV = 0.0, EC = 0.0, EA = 0.0, EAC = 0.0;
       int trial_overlap = 0;
       for (int j = 0; j < n; j++) {
         if (j == start) continue;
@@ -1073,11 +1232,18 @@ static void widom_core(const orc_system *s, const orc_molecule *m, draw_t *d, do
         double Ucc = orc_charge_energy_at(s, m->type[j], tpos[iu][j], 1.0);
         EV += Uvdw;
         EC += Ucc;
+        /* guest-guest terms: summed, not checked for overlap (src/cbmc.c:1222-1244) */
+        double Uav = orc_inter_vdw_at(s, m->type[j], tpos[iu][j], excl, 1.0);
+        double Uac = orc_inter_charge_at(s, m->type[j], tpos[iu][j], excl, 1.0 * s->t_q[m->type[j]]);
+        EA += Uav;
+        EAC += Uac;
       }
       if (!trial_overlap) {
         uv[iu] = EV;
         uc[iu] = EC;
-        bf[iu] = -beta * (EV + 0.0 + 0.0 + 0.0);
+        ua[iu] = EA;
+        uac[iu] = EAC;
+        bf[iu] = -beta * (EV + EA + 0.0 + 0.0);
         ovo[iu] = 0;
         ALLOV = 0;
       } else {
@@ -1107,8 +1273,11 @@ static void widom_core(const orc_system *s, const orc_molecule *m, draw_t *d, do
       rec[ORC_REC_STAGE] = 3.0;
       return;
     }
+    /* src/cbmc.c:2819-2823 */
+    UAdsorbateVDWNew += ua[iwalk];
     UHostVDWNew += uv[iwalk];
     UHostChargeChargeNew += uc[iwalk];
+    UAdsorbateChargeChargeNew += uac[iwalk];
     for (int j = 0; j < n; j++)
       if (j != start) {
         newpos[j][0] = tpos[iwalk][j][0];
@@ -1122,7 +1291,7 @@ static void widom_core(const orc_system *s, const orc_molecule *m, draw_t *d, do
   RosenbluthNew *= exp(-beta * (0.0 + 0.0 + 0.0));
   RosenbluthNew *= exp(-beta * (0.0 + 0.0));
   RosenbluthNew *= exp(-beta * (0.0 + 0.0));
-  RosenbluthNew *= exp(-beta * (UHostChargeChargeNew + 0.0 + 0.0));
+  RosenbluthNew *= exp(-beta * (UHostChargeChargeNew + UAdsorbateChargeChargeNew + 0.0));
   /* WidomAdsorbateMove (src/mc_moves.c:7882-7886): a grown molecule with an
    * atom in a blocked pocket contributes nothing */
   for (int j = 0; j < m->n_sites; j++)
@@ -1150,13 +1319,16 @@ static void widom_core(const orc_system *s, const orc_molecule *m, draw_t *d, do
     orc_fourier_adsorbate(s, m->n_sites, m->type, xyz, fo);
     RosenbluthNew *= exp(-beta * (fo[0] + fo[1] + 0.0));
   }
-  double DeltaU = UHostVDWNew + UHostChargeChargeNew + fo[0] + fo[1];
+  /* src/mc_moves.c:7903-7913, the zero terms left out */
+  double DeltaU = UAdsorbateVDWNew + UHostVDWNew + UAdsorbateChargeChargeNew + UHostChargeChargeNew + fo[0] + fo[1];
   rec[ORC_REC_W] = RosenbluthNew;
   rec[ORC_REC_DU] = DeltaU;
   rec[ORC_REC_UVDW] = UHostVDWNew;
   rec[ORC_REC_UREAL] = UHostChargeChargeNew;
   rec[ORC_REC_UFHOST] = fo[0];
   rec[ORC_REC_UFADS] = fo[1];
+  rec[ORC_REC_UADSVDW] = UAdsorbateVDWNew;
+  rec[ORC_REC_UADSREAL] = UAdsorbateChargeChargeNew;
   rec[ORC_REC_OVERLAP] = 0.0;
   rec[ORC_REC_STAGE] = 4.0;
 }

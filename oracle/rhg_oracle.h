@@ -143,7 +143,9 @@ typedef struct {
 #define ORC_REC_W1      11
 #define ORC_REC_W2      12
 #define ORC_REC_STAGE   13   /* 0: fb all overlap, 1: fb W<min, 2: or all overlap, 3: or W<min, 4: ok, 5: blocked */
-#define ORC_REC_N       14
+#define ORC_REC_UADSVDW 14   /* UAdsorbateVDWNew (guest-guest, finite loading) */
+#define ORC_REC_UADSREAL 15  /* UAdsorbateChargeChargeNew (real space) */
+#define ORC_REC_N       16
 
 /* trial record: what the insertion drew, usable as host-supplied trials */
 typedef struct {
@@ -202,6 +204,22 @@ double orc_derivatives_real_at(const orc_system *s, const double pos[3], double 
  * null detaches), src/framework_energy.c:73-76, 197-200 */
 int orc_use_grid(orc_system *s, int type, const float *grid, const double geom[12], const double unit_box[9]);
 int orc_use_coulomb_grid(orc_system *s, const float *grid, const double geom[12], const double unit_box[9]);
+/* Finite loading: the adsorbate molecules present, molecule m holding
+ * atoms [start[m], start[m+1]) of pseudo-atom types[] at xyz (n x 3).  The
+ * adsorbate structure factor (StoreTotalChargeAdsorbates) and net charge are
+ * computed from them (src/ewald.c:1289-1731).  n_mol = 0 clears. */
+int orc_set_adsorbates(orc_system *s, int n_mol, const int *start, const int *types, const double *xyz);
+/* CalculateInterVDWEnergyAdsorbateAtPosition (src/inter_energy.c:321) */
+double orc_inter_vdw_at(const orc_system *s, int type, const double pos[3], int exclude, double scaling);
+/* CalculateInterChargeEnergyAdsorbateAtPosition (src/inter_energy.c:390), chargeA = the
+ * caller's scaling x charge */
+double orc_inter_charge_at(const orc_system *s, int type, const double pos[3], int exclude, double charge_a);
+/* CalculateEwaldFourierAdsorbate(New, Old, mol) (src/ewald.c:6859-7565): the
+ * new molecule's n_new sites (0: none) and/or adsorbate molecule old_mol
+ * (-1: none).  out: host delta, adsorbate delta, self(new-old),
+ * excluded(new-old), sum_k kf(|S_ads'|^2-|S_ads|^2) */
+void orc_fourier_move(const orc_system *s, int n_new, const int *new_types, const double *new_xyz,
+                      int old_mol, double out[5]);
 /* InterpolateCoulombGrid (src/grids.c:1200-1295) */
 double orc_interpolate_coulomb_grid(const float *grid, const double geom[12], int bc, const double ub[9],
                                     const double uinv[9], const double pos[3], double charge);
