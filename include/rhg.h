@@ -116,7 +116,9 @@ typedef struct {
 #define RHG_REC_W1 11
 #define RHG_REC_W2 12
 #define RHG_REC_STAGE 13   /* 0 fb all overlap,1 W1<min,2 or all overlap,3 W<min,4 accepted,5 blocked */
-#define RHG_REC_N 14
+#define RHG_REC_UADSVDW 14 /* UAdsorbateVDWNew (guest-guest, finite loading) */
+#define RHG_REC_UADSREAL 15 /* UAdsorbateChargeChargeNew (real space) */
+#define RHG_REC_N 16
 
 /* ------------------------------------------------------------ context */
 /* Replaces ReadInput's framework/force-field/Ewald set-up for one system:
@@ -225,6 +227,32 @@ int rhg_inject_fault(rhg_ctx *ctx, int64_t after_chunks);
  * out[13] 2 sigma, out[14] overall K_H, out[15] overall <W>. */
 int rhg_widom_estimators(const rhg_widom_blocks *blocks, double temperature, double density_kg_m3,
                          double ideal_gas_weight, double out[16]);
+
+/* -------------------------------------------------------- finite loading */
+/* The adsorbate molecules present (Adsorbates[s], finite loading): molecule
+ * m holds atoms [start[m], start[m+1]) of the given pseudo-atom types at xyz
+ * (n x 3 doubles, n = start[n_mol]); n_mol = 0 clears.  The adsorbate
+ * structure factor (StoreTotalChargeAdsorbates) and net charge are computed
+ * on the device (src/ewald.c:1289-1731).  From then on Widom insertion adds
+ * the guest-guest terms and the Fourier change uses the stored sums. */
+int rhg_set_adsorbates(rhg_ctx *ctx, int32_t n_mol, const int32_t *start, const int32_t *types,
+                       const double *xyz);
+/* REAL CalculateInterVDWEnergyAdsorbateAtPosition(POINT,int,int,REAL) and
+ * int CalculateInterChargeEnergyAdsorbateAtPosition(POINT,int,REAL*,REAL*,int,REAL)
+ * (src/inter_energy.h, src/inter_energy.c:321-520) at n positions for
+ * pseudo-atom `type` (charge scaling 1), every molecule but `exclude`
+ * (-1: none).  u_vdw, u_cc nullable. */
+int rhg_inter_energy_at_positions(rhg_ctx *ctx, int64_t n, const double *xyz, int32_t type, int32_t exclude,
+                                  double *u_vdw, double *u_cc);
+/* int CalculateEwaldFourierAdsorbate(int New,int Old,int mol,int store)
+ * (src/ewald.h:86, src/ewald.c:6859-7565) for one move: the new molecule's
+ * n_new sites (0: none) and/or the removal of adsorbate molecule old_mol
+ * (-1: none), against the stored framework and adsorbate sums.  out[0]
+ * UHostAdsorbateChargeChargeFourierDelta, out[1]
+ * UAdsorbateAdsorbateChargeChargeFourierDelta.  At most 8 charged sites in
+ * total (RHG_ERR_UNSUPPORTED). */
+int rhg_ewald_fourier_move(rhg_ctx *ctx, int32_t n_new, const int32_t *new_types, const double *new_xyz,
+                           int32_t old_mol, double out[2]);
 
 /* ----------------------------------------------------------------- grids */
 /* MakeGrid (src/grids.c:431-583) for one probe type over a unit-cell box.

@@ -106,6 +106,8 @@ struct PseudoAtom {
   AtomRecMx *dmx = nullptr;
   double *c24 = nullptr, *c96 = nullptr, *c384 = nullptr;
   float4 *par = nullptr; // {4 eps, sigma^2 * kFix^2, shift, q} for the image-list kernels
+  AtomRec64 *dads = nullptr; // the adsorbate atoms as seen by this type (finite loading)
+  size_t dads_cap = 0;
 };
 
 struct Pending {
@@ -182,6 +184,16 @@ struct rhg_ctx {
   std::map<std::string, hipGraphExec_t> graph_cache;
   // host-side fault hook: fail the chunk after this many more chunks (<0 off)
   int64_t fault_after = -1;
+  // adsorbates present (finite loading): molecule m holds atoms
+  // [ads_start[m], ads_start[m+1]); the adsorbate structure factor
+  // (StoreTotalChargeAdsorbates) and net charge follow them
+  int ads_nmol = 0, ads_n = 0;
+  std::vector<int32_t> ads_start, ads_type;
+  std::vector<double> ads_xyz;
+  double net_q_ads = 0.0;
+  int32_t *d_ads_mol = nullptr;
+  size_t ads_mol_cap = 0;
+  double2 *d_sads = nullptr;
   // tabulated grids in use (FrameworkModel GRID): per pseudo-atom VDW grids
   // and the Coulomb grid, borrowed from their owners
   std::vector<rhg_grid *> vdw_grids;
@@ -474,6 +486,56 @@ EnergyArgs energy_args(rhg_ctx *c, int type, double scaling, int do_vdw, int do_
   return a;
 }
 
+// The adsorbate atoms as seen by pseudo-atom `type`: positions, LB-mixed LJ
+// of the type pair (as for framework atoms) and the atom's charge.
+int ads_records(rhg_ctx *c, PseudoAtom &t) {
+  if (c->ads_n == 0) return RHG_OK;
+  std::vector<AtomRec64> h((size_t)c->ads_n);
+  const double ei = t.eps_k * c->u.K2E;
+  for (int i = 0; i < c->ads_n; ++i) {
+    const PseudoAtom &b = c->types[c->ads_type[i]];
+    const double e = sqrt(ei * (b.eps_k * c->u.K2E));
+    const double sg = 0.5 * (t.sigma + b.sigma);
+    AtomRec64 &r = h[i];
+    r.x = c->ads_xyz[3 * i];
+    r.y = c->ads_xyz[3 * i + 1];
+    r.z = c->ads_xyz[3 * i + 2];
+    r.c4 = 4.0 * e;
+    r.s2 = sg * sg;
+    r.shift = c->shifted ? lj_ref(e, sg, 0.0, c->rc2_vdw) : 0.0;
+    r.q = 1.0 * b.q;
+    r.pad = 0.0;
+  }
+  if (t.dads_cap < h.size()) {
+    hipFree(t.dads);
+    t.dads = nullptr;
+    t.dads_cap = 0;
+    if (int rc = dalloc(c, (void **)&t.dads, h.size() * sizeof(AtomRec64))) return rc;
+    t.dads_cap = h.size();
+  }
+  if (int rc = upload(c, t.dads, h.data(), h.size() * sizeof(AtomRec64))) return rc;
+  HIPC(c, hipStreamSynchronize(c->stream)); // h is released on return
+  return RHG_OK;
+}
+
+AdsArgs ads_args(rhg_ctx *c, int type, int exclude, int do_vdw, int do_cc) {
+  AdsArgs a;
+  memset(&a, 0, sizeof(a));
+  const PseudoAtom &t = c->types[type];
+  a.cell = c->cell;
+  a.rec = t.dads;
+  a.mol = c->d_ads_mol;
+  a.n = c->ads_n;
+  a.exclude = exclude;
+  a.rc2v = c->rc2_vdw;
+  a.rc2c = c->rc2_cc;
+  a.alpha = c->alpha;
+  a.cqa = c->u.C * (1.0 * t.q); // chargeA = CFChargeScaling * Charge1 (src/cbmc.c:687)
+  a.do_vdw = do_vdw;
+  a.do_cc = (do_cc && c->charge_method == RHG_CHARGE_EWALD && considered_charged(t.q)) ? 1 : 0;
+  return a;
+}
+
 __global__ void k_fill(double *p, int64_t n, double v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -693,6 +755,43 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
   return RHG_OK;
 }
 
+// Structure factor sum_j q_j exp(i k.r_j) of the given charges (fractional
+// coordinates times 2 pi) in RASPA2's k order into out (src/ewald.c:1330-1345)
+int structure_factor(rhg_ctx *c, const std::vector<double> &fx, const std::vector<double> &fy,
+                     const std::vector<double> &fz, const std::vector<double> &qq, double2 *out) {
+  const int nc = (int)qq.size();
+  const size_t nk = (size_t)std::max(c->nk, 1);
+  size_t bn = (size_t)std::max(nc, 1) * sizeof(double);
+  double *d_fx = (double *)work(c, "sf_fx", bn), *d_fy = (double *)work(c, "sf_fy", bn),
+         *d_fz = (double *)work(c, "sf_fz", bn), *d_q = (double *)work(c, "sf_q", bn);
+  int32_t *d_kidx = (int32_t *)work(c, "sf_kidx", nk * 3 * sizeof(int32_t));
+  if (!d_fx || !d_fy || !d_fz || !d_q || !d_kidx) return fail(c, RHG_ERR_OOM, "structure factor buffers");
+  if (nc) {
+    upload(c, d_fx, fx.data(), nc * sizeof(double));
+    upload(c, d_fy, fy.data(), nc * sizeof(double));
+    upload(c, d_fz, fz.data(), nc * sizeof(double));
+    upload(c, d_q, qq.data(), nc * sizeof(double));
+  }
+  upload(c, d_kidx, c->kidx.data(), c->kidx.size() * sizeof(int32_t));
+  const size_t tab_n = (size_t)std::max(nc, 1) * ((c->kx + 1) + (2 * c->ky + 1) + 1);
+  double2 *d_tab = (double2 *)work(c, "sf_tab", tab_n * sizeof(double2));
+  if (!d_tab) return fail(c, RHG_ERR_OOM, "structure factor tables");
+  {
+    Timed t(c, "sf");
+    launch_structure_factor(ewald_dev(c), d_kidx, c->nk, d_fx, d_fy, d_fz, d_q, nc, d_tab, out, c->stream);
+  }
+  HIPC(c, hipGetLastError());
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return RHG_OK;
+}
+
+void frac_2pi(const rhg_ctx *c, const double p[3], double f[3]) {
+  const double *inv = c->cell.inv;
+  f[0] = (inv[AX] * p[0] + inv[BX] * p[1] + inv[CX] * p[2]) * (2.0 * M_PI);
+  f[1] = (inv[AY] * p[0] + inv[BY] * p[1] + inv[CY] * p[2]) * (2.0 * M_PI);
+  f[2] = (inv[AZ] * p[0] + inv[BZ] * p[1] + inv[CZ] * p[2]) * (2.0 * M_PI);
+}
+
 } // namespace
 
 // ======================================================================
@@ -766,29 +865,8 @@ int rhg_create(const rhg_framework *fw, const rhg_cell *cellp, const rhg_forcefi
       qq.push_back(c->q[i]);
       c->net_q_fw += c->q[i];
     }
-    int nc = (int)qq.size();
-    size_t bn = (size_t)std::max(nc, 1) * sizeof(double);
-    double *d_fx = (double *)work(c.get(), "sf_fx", bn), *d_fy = (double *)work(c.get(), "sf_fy", bn),
-           *d_fz = (double *)work(c.get(), "sf_fz", bn), *d_q = (double *)work(c.get(), "sf_q", bn);
-    int32_t *d_kidx = (int32_t *)work(c.get(), "sf_kidx", nk * 3 * sizeof(int32_t));
-    if (!d_fx || !d_fy || !d_fz || !d_q || !d_kidx) return RHG_ERR_OOM;
-    if (nc) {
-      upload(c.get(), d_fx, fx.data(), nc * sizeof(double));
-      upload(c.get(), d_fy, fy.data(), nc * sizeof(double));
-      upload(c.get(), d_fz, fz.data(), nc * sizeof(double));
-      upload(c.get(), d_q, qq.data(), nc * sizeof(double));
-    }
-    upload(c.get(), d_kidx, c->kidx.data(), c->kidx.size() * sizeof(int32_t));
-    const size_t tab_n = (size_t)std::max(nc, 1) * ((c->kx + 1) + (2 * c->ky + 1) + 1);
     if (c->kz > 64) return RHG_ERR_UNSUPPORTED; // k_z range of the row kernel
-    double2 *d_tab = (double2 *)work(c.get(), "sf_tab", tab_n * sizeof(double2));
-    if (!d_tab) return RHG_ERR_OOM;
-    {
-      Timed t(c.get(), "sf");
-      launch_structure_factor(ewald_dev(c.get()), d_kidx, c->nk, d_fx, d_fy, d_fz, d_q, nc, d_tab,
-                              c->d_sf, c->stream);
-    }
-    if (hipStreamSynchronize(c->stream) != hipSuccess) return RHG_ERR_HIP;
+    if ((rc = structure_factor(c.get(), fx, fy, fz, qq, c->d_sf))) return rc;
   }
   *out = c.release();
   return RHG_OK;
@@ -806,7 +884,10 @@ void rhg_destroy(rhg_ctx *c) {
     hipFree(t.c96);
     hipFree(t.c384);
     hipFree(t.par);
+    hipFree(t.dads);
   }
+  hipFree(c->d_ads_mol);
+  hipFree(c->d_sads);
   for (auto &b : c->bufs) {
     hipFree(b.second.p);
     for (void *r : b.second.retired) hipFree(r);
@@ -845,6 +926,7 @@ int rhg_add_pseudo_atom(rhg_ctx *c, double eps_k, double sigma, double charge, i
   t.sigma = sigma;
   t.q = charge;
   int rc = setup_pseudo_atom(c, t);
+  if (!rc) rc = ads_records(c, t);
   if (rc) return rc;
   c->types.push_back(t);
   *type_out = (int32_t)c->types.size() - 1;
@@ -1085,6 +1167,14 @@ int widom_buffers(rhg_ctx *c, const MolDev &m, int64_t cap, WidomBufs &B) {
   WB(ccz, "w_ccz", (size_t)cap * std::max(m.n_place, 1) * d8);
   WB(f_host, "w_fhost", cap * d8);
   WB(f_ads, "w_fads", cap * d8);
+  if (c->ads_n > 0) {
+    WB(fb_av, "w_fbav", nfb * d8);
+    WB(fb_ac, "w_fbac", nfb * d8);
+    WB(or_av, "w_orav", nor * d8);
+    WB(cc_ads, "w_ccads", (size_t)cap * std::max(m.n_place, 1) * d8);
+    WB(uadsv, "w_uadsv", cap * d8);
+    WB(uadsc, "w_uadsc", cap * d8);
+  }
 #undef WB
   B.d_part = (double *)work(c, "w_part", ((cap + kSeg - 1) / kSeg + 1) * 10 * d8);
   if (!B.d_part) return fail(c, RHG_ERR_OOM, "widom reduce buffers");
@@ -1148,6 +1238,13 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
     SiteList s{w.fbx, w.fby, w.fbz, n * m.k_fb, a.n_eff, m.k_fb, w.cap};
     EnergyOut o{w.fb_uv, ea.do_cc ? w.fb_uc : nullptr, nullptr};
     if (int rc = site_energy(c, type0, ea, s, o, "bfb")) return rc;
+    if (w.fb_av) {
+      // guest-guest terms; the inserted molecule is number ads_nmol, so none
+      // is excluded (src/mc_moves.c:7856, src/cbmc.c:683-689)
+      const AdsArgs aa = ads_args(c, type0, c->ads_nmol, 1, 1);
+      if (!aa.do_cc) fill(w.fb_ac, w.cap * m.k_fb, 0.0, st);
+      launch_site_ads(aa, s, w.fb_av, aa.do_cc ? w.fb_ac : nullptr, st);
+    }
   }
   {
     Timed t(c, "widom_select");
@@ -1166,6 +1263,8 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
       SiteList s{w.orx + base, w.ory + base, w.orz + base, n * rows, w.n_active, rows, w.cap};
       EnergyOut o{w.or_uv + base, nullptr, nullptr};
       if (int rc = site_energy(c, mol->type[m.place[p]], ea, s, o, "bor")) return rc;
+      if (w.or_av)
+        launch_site_ads(ads_args(c, mol->type[m.place[p]], c->ads_nmol, 1, 0), s, w.or_av + base, nullptr, st);
     }
     {
       Timed t(c, "widom_select");
@@ -1181,11 +1280,14 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
       const int rows = m.h_size[p];
       if (!ea.do_cc) {
         fill(w.cc_place + base, (int64_t)rows * w.cap, 0.0, st);
+        if (w.cc_ads) fill(w.cc_ads + base, (int64_t)rows * w.cap, 0.0, st);
         continue;
       }
       SiteList s{w.ccx + base, w.ccy + base, w.ccz + base, n * rows, w.n_active2, rows, w.cap};
       EnergyOut o{nullptr, w.cc_place + base, nullptr};
       if (int rc = site_energy(c, mol->type[m.place[p]], ea, s, o, "bcc")) return rc;
+      if (w.cc_ads)
+        launch_site_ads(ads_args(c, mol->type[m.place[p]], c->ads_nmol, 0, 1), s, nullptr, w.cc_ads + base, st);
     }
   }
   if (a.ewald) {
@@ -1211,6 +1313,8 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
     f.n_dev = w.n_active2;
     f.d_host = w.f_host;
     f.d_ads = w.f_ads;
+    f.sads = c->ads_n > 0 ? c->d_sads : nullptr;
+    f.net_q_ads = c->net_q_ads;
     if (f.nc > 0) {
       double2 *part = (double2 *)work(c, "w_fpart", (size_t)(c->kx + 1) * w.cap * sizeof(double2));
       if (!part) return fail(c, RHG_ERR_OOM, "fourier partials");
@@ -1573,6 +1677,8 @@ int rhg_ewald_fourier_adsorbate(rhg_ctx *c, int64_t n_mol, int32_t n_sites, cons
   f.n_dev = nullptr;
   f.d_host = dh;
   f.d_ads = da;
+  f.sads = c->ads_n > 0 ? c->d_sads : nullptr;
+  f.net_q_ads = c->net_q_ads;
   if (f.nc == 0) {
     fill(dh, n_mol, 0.0, c->stream);
     fill(da, n_mol, 0.0, c->stream);
@@ -1585,6 +1691,178 @@ int rhg_ewald_fourier_adsorbate(rhg_ctx *c, int64_t n_mol, int32_t n_sites, cons
   HIPC(c, hipGetLastError());
   if (d_host) HIPC(c, hipMemcpyAsync(d_host, dh, n_mol * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   if (d_ads) HIPC(c, hipMemcpyAsync(d_ads, da, n_mol * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return RHG_OK;
+}
+
+// --------------------------------------------------------- finite loading
+int rhg_set_adsorbates(rhg_ctx *c, int32_t n_mol, const int32_t *start, const int32_t *types,
+                       const double *xyz) {
+  if (!c || n_mol < 0 || (n_mol > 0 && (!start || !types || !xyz))) return RHG_ERR_INVALID;
+  int n = 0;
+  if (n_mol > 0) {
+    if (start[0] != 0) return fail(c, RHG_ERR_INVALID, "molecule 0 must start at atom 0");
+    for (int m = 0; m < n_mol; ++m)
+      if (start[m + 1] < start[m]) return fail(c, RHG_ERR_INVALID, "molecule starts must not decrease");
+    n = start[n_mol];
+    for (int i = 0; i < n; ++i)
+      if (int rc = check_type(c, types[i])) return rc;
+  }
+  hipSetDevice(c->device);
+  c->ads_nmol = n_mol;
+  c->ads_n = n;
+  c->ads_start.assign(start, start + (n_mol > 0 ? n_mol + 1 : 0));
+  c->ads_type.assign(types, types + n);
+  c->ads_xyz.assign(xyz, xyz + 3 * (size_t)n);
+  c->net_q_ads = 0.0;
+  if (n > 0) {
+    std::vector<int32_t> mol((size_t)n);
+    for (int m = 0; m < n_mol; ++m)
+      for (int a = start[m]; a < start[m + 1]; ++a) mol[a] = m;
+    if (c->ads_mol_cap < (size_t)n) {
+      hipFree(c->d_ads_mol);
+      c->d_ads_mol = nullptr;
+      c->ads_mol_cap = 0;
+      if (int rc = dalloc(c, (void **)&c->d_ads_mol, (size_t)n * sizeof(int32_t))) return rc;
+      c->ads_mol_cap = (size_t)n;
+    }
+    if (int rc = upload(c, c->d_ads_mol, mol.data(), (size_t)n * sizeof(int32_t))) return rc;
+    HIPC(c, hipStreamSynchronize(c->stream));
+    for (auto &t : c->types)
+      if (int rc = ads_records(c, t)) return rc;
+    // StoreTotalChargeAdsorbates, NetChargeAdsorbates
+    std::vector<double> fx, fy, fz, qq;
+    for (int a = 0; a < n; ++a) {
+      const double q = c->types[types[a]].q;
+      if (!considered_charged(q)) continue;
+      double f[3];
+      frac_2pi(c, xyz + 3 * a, f);
+      fx.push_back(f[0]);
+      fy.push_back(f[1]);
+      fz.push_back(f[2]);
+      qq.push_back(q);
+      c->net_q_ads += q;
+    }
+    if (c->charge_method == RHG_CHARGE_EWALD) {
+      if (!c->d_sads)
+        if (int rc = dalloc(c, (void **)&c->d_sads, (size_t)std::max(c->nk, 1) * sizeof(double2))) return rc;
+      if (int rc = structure_factor(c, fx, fy, fz, qq, c->d_sads)) return rc;
+    }
+  }
+  c->gen++; // captured Widom chunks hold the adsorbate arguments
+  return RHG_OK;
+}
+
+int rhg_inter_energy_at_positions(rhg_ctx *c, int64_t n, const double *xyz, int32_t type, int32_t exclude,
+                                  double *u_vdw, double *u_cc) {
+  if (!c || n < 0 || (n > 0 && !xyz)) return RHG_ERR_INVALID;
+  if (int rc = check_type(c, type)) return rc;
+  if (n == 0) return RHG_OK;
+  if (c->ads_n == 0) {
+    for (int64_t i = 0; i < n; ++i) {
+      if (u_vdw) u_vdw[i] = 0.0;
+      if (u_cc) u_cc[i] = 0.0;
+    }
+    return RHG_OK;
+  }
+  hipSetDevice(c->device);
+  std::vector<double> sx((size_t)n), sy((size_t)n), sz((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    sx[i] = xyz[3 * i];
+    sy[i] = xyz[3 * i + 1];
+    sz[i] = xyz[3 * i + 2];
+  }
+  const size_t b = (size_t)n * sizeof(double);
+  double *dx = (double *)work(c, "e_x", b), *dy = (double *)work(c, "e_y", b), *dz = (double *)work(c, "e_z", b);
+  double *dv = (double *)work(c, "e_v", b), *dc = (double *)work(c, "e_c", b);
+  if (!dx || !dy || !dz || !dv || !dc) return fail(c, RHG_ERR_OOM, "work buffers");
+  upload(c, dx, sx.data(), b);
+  upload(c, dy, sy.data(), b);
+  upload(c, dz, sz.data(), b);
+  const AdsArgs a = ads_args(c, type, exclude, 1, 1);
+  if (!a.do_cc) fill(dc, n, 0.0, c->stream);
+  SiteList sl{dx, dy, dz, n, nullptr, 1, 0};
+  launch_site_ads(a, sl, dv, a.do_cc ? dc : nullptr, c->stream);
+  HIPC(c, hipGetLastError());
+  if (u_vdw) HIPC(c, hipMemcpyAsync(u_vdw, dv, b, hipMemcpyDeviceToHost, c->stream));
+  if (u_cc) HIPC(c, hipMemcpyAsync(u_cc, dc, b, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return RHG_OK;
+}
+
+int rhg_ewald_fourier_move(rhg_ctx *c, int32_t n_new, const int32_t *new_types, const double *new_xyz,
+                           int32_t old_mol, double out[2]) {
+  if (!c || !out || n_new < 0 || (n_new > 0 && (!new_types || !new_xyz))) return RHG_ERR_INVALID;
+  if (old_mol >= c->ads_nmol) return fail(c, RHG_ERR_INVALID, "no such adsorbate molecule");
+  out[0] = out[1] = 0.0;
+  if (c->charge_method != RHG_CHARGE_EWALD) return RHG_OK;
+  for (int j = 0; j < n_new; ++j)
+    if (int rc = check_type(c, new_types[j])) return rc;
+  // charged sites: the new molecule's, then the removed molecule's with
+  // negated charges, so that one structure-factor sum is S_new - S_old
+  std::vector<double> px, py, pz, q;
+  std::vector<int> grp;
+  double self_new = 0.0, self_old = 0.0;
+  for (int j = 0; j < n_new; ++j) {
+    const double qj = c->types[new_types[j]].q;
+    if (!considered_charged(qj)) continue;
+    px.push_back(new_xyz[3 * j]);
+    py.push_back(new_xyz[3 * j + 1]);
+    pz.push_back(new_xyz[3 * j + 2]);
+    q.push_back(1.0 * qj);
+    grp.push_back(0);
+    self_new += c->u.C * (qj * qj) * c->alpha / sqrt(M_PI); // src/ewald.c:6973
+  }
+  if (old_mol >= 0)
+    for (int a = c->ads_start[old_mol]; a < c->ads_start[old_mol + 1]; ++a) {
+      const double qa = c->types[c->ads_type[a]].q;
+      if (!considered_charged(qa)) continue;
+      px.push_back(c->ads_xyz[3 * a]);
+      py.push_back(c->ads_xyz[3 * a + 1]);
+      pz.push_back(c->ads_xyz[3 * a + 2]);
+      q.push_back(-qa);
+      grp.push_back(1);
+      self_old += c->u.C * (qa * qa) * c->alpha / sqrt(M_PI); // src/ewald.c:6950
+    }
+  const int nc = (int)q.size();
+  if (nc > kMaxSites) return fail(c, RHG_ERR_UNSUPPORTED, "more than 8 charged sites in one move");
+  if (nc == 0) return RHG_OK;
+  hipSetDevice(c->device);
+  const size_t b = (size_t)nc * sizeof(double);
+  double *dx = (double *)work(c, "m_x", b), *dy = (double *)work(c, "m_y", b), *dz = (double *)work(c, "m_z", b);
+  double *dr = (double *)work(c, "m_out", 2 * sizeof(double));
+  double2 *part = (double2 *)work(c, "m_part", (size_t)(c->kx + 1) * sizeof(double2));
+  if (!dx || !dy || !dz || !dr || !part) return fail(c, RHG_ERR_OOM, "fourier move buffers");
+  upload(c, dx, px.data(), b);
+  upload(c, dy, py.data(), b);
+  upload(c, dz, pz.data(), b);
+  FourierArgs f;
+  memset(&f, 0, sizeof(f));
+  f.e = ewald_dev(c);
+  f.n_sites = nc;
+  f.nc = nc;
+  for (int t = 0; t < nc; ++t) {
+    f.cidx[t] = t;
+    f.q[t] = q[t];
+    f.grp[t] = grp[t];
+  }
+  f.self_energy = self_new - self_old;
+  f.px = dx;
+  f.py = dy;
+  f.pz = dz;
+  f.stride = 1;
+  f.n = 1;
+  f.n_dev = nullptr;
+  f.d_host = dr;
+  f.d_ads = dr + 1;
+  f.sads = c->ads_n > 0 ? c->d_sads : nullptr;
+  f.net_q_ads = c->net_q_ads;
+  {
+    Timed t(c, "fourier");
+    launch_fourier(f, part, c->stream);
+  }
+  HIPC(c, hipGetLastError());
+  HIPC(c, hipMemcpyAsync(out, dr, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
   return RHG_OK;
 }

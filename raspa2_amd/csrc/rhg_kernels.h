@@ -117,7 +117,10 @@ struct FourierArgs {
   int n_sites;
   int nc;                 // charged sites
   int cidx[kMaxSites];    // site index of each charged site, in site order
-  double q[kMaxSites];    // charge of each charged site
+  double q[kMaxSites];    // charge of each charged site (negated for a removed molecule's)
+  int grp[kMaxSites];     // 0: new molecule, 1: removed molecule (excluded pairs per group)
+  const double2 *sads;    // adsorbate structure factor (StoreTotalChargeAdsorbates), nullable
+  double net_q_ads;       // NetChargeAdsorbates
   double self_energy; // sum C q^2 alpha/sqrt(pi) of the charged sites
   const double *px, *py, *pz; // [site * stride + m]
   int64_t stride;
@@ -127,6 +130,20 @@ struct FourierArgs {
 };
 // part: (kx+1) x stride double2 scratch for the per-kx partial sums
 void launch_fourier(const FourierArgs &a, double2 *part, hipStream_t st);
+
+// ------------------------------------------------------- finite loading
+// Guest-guest terms (src/inter_energy.c:321-520): the adsorbate atoms as
+// seen by one probe pseudo-atom, molecule index per atom, FP64
+struct AdsArgs {
+  Cell cell;
+  const AtomRec64 *rec; // n records of this probe type
+  const int32_t *mol;   // molecule of each atom
+  int n;
+  int exclude;          // molecule left out (-1 or the number of molecules: none)
+  double rc2v, rc2c, alpha, cqa;
+  int do_vdw, do_cc;
+};
+void launch_site_ads(const AdsArgs &a, const SiteList &s, double *u_vdw, double *u_cc, hipStream_t st);
 
 // ---------------------------------------------------------------- Widom
 struct WidomWork {
@@ -151,6 +168,9 @@ struct WidomWork {
   double *cfx, *cfy, *cfz;
   double *cc_place; // [charged placed slot][slot2] real-space Coulomb of the placed beads
   double *f_host, *f_ads; // [slot2]
+  // guest-guest terms at finite loading (all null without adsorbates), laid
+  // out as fb_uv, or_uv and cc_place; per insertion the selected sums
+  double *fb_av, *fb_ac, *or_av, *cc_ads, *uadsv, *uadsc;
   // host-supplied trials (nullable)
   const double *h_fb, *h_cord, *h_usel;
 };

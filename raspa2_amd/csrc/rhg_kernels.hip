@@ -1015,10 +1015,20 @@ void launch_structure_factor(const EwaldDev &e, const int32_t *kidx, int nk, con
 // k index (wave-uniform, scalar loads).
 // ====================================================================
 
+template <int ADS>
+__device__ __forceinline__ double ads_term(const double2 *sads, int idx, double sre, double sim) {
+  const double d2 = fma(sim, sim, sre * sre);
+  if (!ADS) return d2;
+  const double2 st = sads[idx];
+  return fma(2.0 * st.x, sre, fma(2.0 * st.y, sim, d2));
+}
+
 // Partial k-sums of one kx slice (blockIdx.y = kx) for 64 molecules per
 // wave; part[kx][m] = (host, ads).  The slice's exp(i kx x) comes from the
 // same recurrence as the reference (kx repeated products of exp(i x)).
-template <int NC>
+// With adsorbates present (ADS) the adsorbate term of each k is
+// |S_ads + dS|^2 - |S_ads|^2 = 2 Re(S_ads* dS) + |dS|^2 (src/ewald.c:7233-7259).
+template <int NC, int ADS>
 __global__ __launch_bounds__(128) void k_fourier_part(const FourierArgs a, double2 *__restrict__ part) {
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t n = a.n_dev ? (int64_t)(*a.n_dev) : a.n;
@@ -1093,7 +1103,7 @@ __global__ __launch_bounds__(128) void k_fourier_part(const FourierArgs a, doubl
         const double kf = e.kfactor[idx];
         const double2 sf = e.sf[idx];
         host = fma(kf, fma(sf.y, sim, sf.x * sre), host);
-        ads = fma(kf, fma(sim, sim, sre * sre), ads);
+        ads = fma(kf, ads_term<ADS>(a.sads, idx, sre, sim), ads);
       }
       // k_z = +kk and -kk share the four products Re(xy) Re(ez), Im(xy) Im(ez),
       // Re(xy) Im(ez), Im(xy) Re(ez); in the origin row the +k_z entries sit
@@ -1115,14 +1125,14 @@ __global__ __launch_bounds__(128) void k_fourier_part(const FourierArgs a, doubl
           const double kf = e.kfactor[ip + kk];
           const double2 sf = e.sf[ip + kk];
           host = fma(kf, fma(sf.y, sim, sf.x * sre), host);
-          ads = fma(kf, fma(sim, sim, sre * sre), ads);
+          ads = fma(kf, ads_term<ADS>(a.sads, ip + kk, sre, sim), ads);
         }
         {
           const double sre = a1 + a2, sim = a4 - a3; // -k_z
           const double kf = e.kfactor[im - kk];
           const double2 sf = e.sf[im - kk];
           host = fma(kf, fma(sf.y, sim, sf.x * sre), host);
-          ads = fma(kf, fma(sim, sim, sre * sre), ads);
+          ads = fma(kf, ads_term<ADS>(a.sads, im - kk, sre, sim), ads);
         }
       }
     }
@@ -1156,22 +1166,28 @@ __global__ __launch_bounds__(256) void k_fourier_final(const FourierArgs a,
     qs[t] = a.q[t];
     net += a.q[t];
   }
+  // excluded intra pairs within each molecule: the new one's add, a removed
+  // one's (charges stored negated) subtract (src/ewald.c:7318-7335, 7418-7440)
   double excl = 0.0;
 #pragma unroll
   for (int p = 0; p < NC; ++p)
 #pragma unroll
     for (int r = p + 1; r < NC; ++r) {
 #pragma clang fp contract(off)
+      if (a.grp[p] != a.grp[r]) continue;
       double dx = pxs[p] - pxs[r], dy = pys[p] - pys[r], dz = pzs[p] - pzs[r];
       double rd = sqrt(dx * dx + dy * dy + dz * dz);
       double bt0 = -erf(e.alpha * rd) / rd;
-      excl -= e.C * qs[p] * qs[r] * bt0;
+      const double t = e.C * qs[p] * qs[r] * bt0;
+      excl = a.grp[p] ? excl + t : excl - t;
     }
+  // net-charge terms (src/ewald.c:7536-7565)
+  const double nca = a.net_q_ads;
   double dq = net - 0.0;
   double dads = ads - (excl - 0.0) - (a.self_energy - 0.0);
-  dads += e.uion * ((0.0 + dq) * (0.0 + dq)) - e.uion * (0.0 * 0.0);
+  dads += e.uion * ((nca + dq) * (nca + dq)) - e.uion * (nca * nca);
   double dhost = 2.0 * host;
-  dhost += 2.0 * e.uion * e.net_q_fw * (0.0 + dq) - 2.0 * e.uion * e.net_q_fw * 0.0;
+  dhost += 2.0 * e.uion * e.net_q_fw * (nca + dq) - 2.0 * e.uion * e.net_q_fw * nca;
   a.d_host[m] = dhost;
   a.d_ads[m] = dads;
 }
@@ -1179,7 +1195,10 @@ __global__ __launch_bounds__(256) void k_fourier_final(const FourierArgs a,
 template <int NC>
 static void fourier_nc(const FourierArgs &a, double2 *part, hipStream_t st) {
   const unsigned nb = (unsigned)((a.n + 127) / 128);
-  hipLaunchKernelGGL(k_fourier_part<NC>, dim3(nb, a.e.kx + 1), dim3(128), 0, st, a, part);
+  if (a.sads)
+    hipLaunchKernelGGL((k_fourier_part<NC, 1>), dim3(nb, a.e.kx + 1), dim3(128), 0, st, a, part);
+  else
+    hipLaunchKernelGGL((k_fourier_part<NC, 0>), dim3(nb, a.e.kx + 1), dim3(128), 0, st, a, part);
   hipLaunchKernelGGL(k_fourier_final<NC>, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, st, a,
                      (const double2 *)part);
 }
@@ -1314,13 +1333,16 @@ __global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, cons
   uint64_t mask = 0;
   bool any = false;
   double W1 = 0.0;
+  // guest-guest terms at finite loading (src/cbmc.c:683-689); zero without adsorbates
+  const bool ads = w.fb_av != nullptr;
   for (int t = 0; t < k; ++t) {
     double uv = w.fb_uv[t * w.cap + i], uc = w.fb_uc[t * w.cap + i];
-    if (uv >= kOverlap || uc >= kOverlap)
+    const double av = ads ? w.fb_av[t * w.cap + i] : 0.0, ac = ads ? w.fb_ac[t * w.cap + i] : 0.0;
+    if (uv >= kOverlap || uc >= kOverlap || av >= kOverlap || ac >= kOverlap)
       mask |= (1ULL << t);
     else {
       any = true;
-      W1 += exp(-beta * (uv + 0.0 + 0.0));
+      W1 += exp(-beta * (uv + av + 0.0));
     }
   }
   w.fbmask[i] = mask;
@@ -1349,8 +1371,9 @@ __global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, cons
   }
   auto bf = [&](int t, double &b) -> bool {
     double uv = w.fb_uv[t * w.cap + i], uc = w.fb_uc[t * w.cap + i];
-    if (uv >= kOverlap || uc >= kOverlap) return false;
-    b = -beta * (uv + 0.0 + 0.0);
+    const double av = ads ? w.fb_av[t * w.cap + i] : 0.0, ac = ads ? w.fb_ac[t * w.cap + i] : 0.0;
+    if (uv >= kOverlap || uc >= kOverlap || av >= kOverlap || ac >= kOverlap) return false;
+    b = -beta * (uv + av + 0.0);
     return true;
   };
   int sel = select_trial(bf, k, usel);
@@ -1367,6 +1390,10 @@ __global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, cons
   w.selz[i] = fz;
   w.uvdw[i] = w.fb_uv[sel * w.cap + i];
   w.ureal[i] = w.fb_uc[sel * w.cap + i];
+  if (ads) {
+    w.uadsv[i] = w.fb_av[sel * w.cap + i];
+    w.uadsc[i] = w.fb_ac[sel * w.cap + i];
+  }
   w.W[i] = W1;
   w.stage[i] = 4;
   const MolDev &m = a.mol;
@@ -1443,14 +1470,20 @@ __global__ __launch_bounds__(256) void k_widom_or_select(const WidomArgs a, cons
   const int ko = m.k_or;
   const double beta = a.beta;
   const int64_t cap = w.cap;
-  auto ev_of = [&](int iu, double &ev) -> bool {
+  // guest-guest VDW of the placed beads is summed but, as in the reference,
+  // not tested for overlap (src/cbmc.c:1208-1244)
+  const bool ads = w.or_av != nullptr;
+  auto ev_of = [&](int iu, double &ev, double &ea) -> bool {
 #pragma clang fp contract(off)
     ev = 0.0;
+    ea = 0.0;
     for (int p = 0; p < m.n_place; ++p) {
       // energies are stored [storage slot][orientation][active slot]
-      double uv = w.or_uv[((int64_t)(m.g_start[p] + m.g_b[p]) * ko + iu) * cap + slot];
+      const int64_t o = ((int64_t)(m.g_start[p] + m.g_b[p]) * ko + iu) * cap + slot;
+      double uv = w.or_uv[o];
       if (uv >= kOverlap) return false;
       ev += uv;
+      if (ads) ea += w.or_av[o];
     }
     return true;
   };
@@ -1458,10 +1491,10 @@ __global__ __launch_bounds__(256) void k_widom_or_select(const WidomArgs a, cons
   bool any = false;
   double weight = 0.0;
   for (int iu = 0; iu < ko; ++iu) {
-    double ev;
-    if (ev_of(iu, ev)) {
+    double ev, ea;
+    if (ev_of(iu, ev, ea)) {
       any = true;
-      weight += exp(-beta * (ev + 0.0 + 0.0 + 0.0));
+      weight += exp(-beta * (ev + ea + 0.0 + 0.0));
     } else
       mask |= (1ULL << iu);
   }
@@ -1475,9 +1508,9 @@ __global__ __launch_bounds__(256) void k_widom_or_select(const WidomArgs a, cons
   }
   weight = weight / (double)ko;
   auto bf = [&](int iu, double &b) -> bool {
-    double ev;
-    if (!ev_of(iu, ev)) return false;
-    b = -beta * (ev + 0.0 + 0.0 + 0.0);
+    double ev, ea;
+    if (!ev_of(iu, ev, ea)) return false;
+    b = -beta * (ev + ea + 0.0 + 0.0);
     return true;
   };
   const int iwalk = select_trial(bf, ko, w.usel_or[i]);
@@ -1513,8 +1546,10 @@ __global__ __launch_bounds__(256) void k_widom_or_select(const WidomArgs a, cons
       return;
     }
   }
-  double ev;
-  ev_of(iwalk, ev);
+  double ev, ea;
+  ev_of(iwalk, ev, ea);
+  // src/cbmc.c:2819-2820
+  if (ads) w.uadsv[i] = w.uadsv[i] + ea;
   w.uvdw[i] = w.uvdw[i] + ev;
   w.W[i] = Wn;
   const int s2 = atomicAdd(w.n_active2, 1);
@@ -1548,15 +1583,22 @@ __global__ __launch_bounds__(256) void k_widom_finalize(const WidomArgs a, const
   const int64_t i = w.active2[s2];
   const MolDev &m = a.mol;
   const double beta = a.beta;
+  const bool ads = w.uadsv != nullptr;
   double ucc = w.ureal[i];
+  double uac = ads ? w.uadsc[i] : 0.0;
   if (m.n_sites > 1) {
-    double ec = 0.0;
+    double ec = 0.0, eac = 0.0;
     for (int p = 0; p < m.n_place; ++p)
-      if (m.h_start[p] >= 0) ec += w.cc_place[(int64_t)(m.h_start[p] + m.h_b[p]) * w.cap + s2];
+      if (m.h_start[p] >= 0) {
+        const int64_t h = (int64_t)(m.h_start[p] + m.h_b[p]) * w.cap + s2;
+        ec += w.cc_place[h];
+        if (ads) eac += w.cc_ads[h];
+      }
     ucc = ucc + ec;
+    uac = uac + eac;
   }
   double W = w.W[i];
-  W *= exp(-beta * (ucc + 0.0 + 0.0));
+  W *= exp(-beta * (ucc + uac + 0.0));
   double fh = 0.0, fa = 0.0;
   if (a.ewald) {
     fh = w.f_host[s2];
@@ -1564,20 +1606,25 @@ __global__ __launch_bounds__(256) void k_widom_finalize(const WidomArgs a, const
     W *= exp(-beta * (fh + fa + 0.0));
   }
   const double uv = w.uvdw[i];
+  const double uav = ads ? w.uadsv[i] : 0.0;
   w.ureal[i] = ucc;
+  if (ads) w.uadsc[i] = uac;
   w.ufh[i] = fh;
   w.ufa[i] = fa;
   w.W[i] = W;
-  w.dU[i] = uv + ucc + fh + fa;
+  // src/mc_moves.c:7903-7913 with its zero terms left out
+  w.dU[i] = uav + uv + uac + ucc + fh + fa;
 }
 
 __global__ __launch_bounds__(256) void k_widom_records(const int64_t n, const WidomWork w,
                                                        double *__restrict__ rec) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double *r = rec + i * 14;
+  double *r = rec + i * 16;
   const int stg = w.stage[i];
   const bool acc = stg == 4;
+  r[14] = (acc && w.uadsv) ? w.uadsv[i] : 0.0;
+  r[15] = (acc && w.uadsc) ? w.uadsc[i] : 0.0;
   r[0] = acc ? w.W[i] : 0.0;
   r[1] = acc ? w.dU[i] : 0.0;
   r[2] = acc ? w.uvdw[i] : 0.0;
@@ -2349,6 +2396,51 @@ __global__ __launch_bounds__(256) void k_site_grid(const InterpArgs gv, const In
     flag = flag || uc >= kOverlap;
   }
   if (o.overlap) o.overlap[k] = (uint8_t)((merge && o.overlap[k]) || flag);
+}
+
+// Guest-guest energies at finite loading (CalculateInterVDWEnergyAdsorbate-
+// AtPosition, CalculateInterChargeEnergyAdsorbateAtPosition, src/inter_energy.c:
+// 321-520): every atom of every adsorbate molecule but the excluded one, in
+// molecule and atom order, FP64 in the reference's operation order.  The
+// atoms arrive wave-uniform, one scalar load each.
+template <int BCK>
+__global__ __launch_bounds__(256) void k_site_ads(const AdsArgs a, const SiteList s, double *__restrict__ out_v,
+                                                  double *__restrict__ out_c) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = site_count(s);
+  if ((int64_t)blockIdx.x * blockDim.x >= n) return;
+  const bool live = i < n;
+  const int64_t k = site_phys(s, live ? i : 0);
+  const double px = s.x[k], py = s.y[k], pz = s.z[k];
+  double uv = 0.0, uc = 0.0;
+  for (int j = 0; j < a.n; ++j) {
+    if (a.mol[j] == a.exclude) continue;
+    const AtomRec64 r = a.rec[j];
+    double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
+    mi_exact<BCK>(a.cell, dx, dy, dz);
+    const double rr = dx * dx + dy * dy + dz * dz;
+    if (a.do_vdw && rr < a.rc2v) uv += lj_exact(r.c4, r.s2, r.shift, rr);
+    if (a.do_cc && rr < a.rc2c) {
+      const double rd = sqrt(rr);
+      uc += a.cqa * r.q * (erfc(a.alpha * rd) / rd);
+    }
+  }
+  if (live) {
+    if (out_v) out_v[k] = uv;
+    if (out_c) out_c[k] = uc;
+  }
+}
+
+void launch_site_ads(const AdsArgs &a, const SiteList &s, double *u_vdw, double *u_cc, hipStream_t st) {
+  const unsigned nb = nblk(s.n, 256);
+  if (!nb) return;
+  if (a.cell.bc == 1)
+    hipLaunchKernelGGL(k_site_ads<1>, dim3(nb), dim3(256), 0, st, a, s, u_vdw, u_cc);
+  else if (a.cell.upper)
+    hipLaunchKernelGGL(k_site_ads<2>, dim3(nb), dim3(256), 0, st, a, s, u_vdw, u_cc);
+  else
+    hipLaunchKernelGGL(k_site_ads<3>, dim3(nb), dim3(256), 0, st, a, s, u_vdw, u_cc);
 }
 
 void launch_site_grid(const InterpArgs *gv, const InterpArgs *gc, const SiteList &s, const EnergyOut &o,

@@ -25,9 +25,9 @@ MAX_SITES = 8
 ERR_INVALID, ERR_HIP, ERR_OOM, ERR_NOT_ADMITTED, ERR_UNSUPPORTED, ERR_FAULT_INJECTED, ERR_IO = (
     -1, -2, -3, -4, -5, -6, -7)
 NR_BLOCKS = 5
-REC_N = 14
+REC_N = 16
 REC = dict(W=0, DU=1, UVDW=2, UREAL=3, UFHOST=4, UFADS=5, OVERLAP=6, SELFB=7, SELOR=8,
-           FBMASK=9, ORMASK=10, W1=11, W2=12, STAGE=13)
+           FBMASK=9, ORMASK=10, W1=11, W2=12, STAGE=13, UADSVDW=14, UADSREAL=15)
 
 _D = ctypes.c_double
 _I32 = ctypes.c_int32
@@ -123,6 +123,9 @@ def _load():
     f("rhg_grid_read", _P, ctypes.c_char_p, _I32, _P, _P, ctypes.POINTER(_P))
     f("rhg_grid_meta", _P, _P)
     f("rhg_use_grid", _P, _I32, _P)
+    f("rhg_set_adsorbates", _P, _I32, _P, _P, _P)
+    f("rhg_inter_energy_at_positions", _P, _I64, _P, _I32, _I32, _P, _P)
+    f("rhg_ewald_fourier_move", _P, _I32, _P, _P, _I32, _P)
     f("rhg_use_coulomb_grid", _P, _P)
     f("rhg_profile_enable", _P, _I32)
     f("rhg_profile_get", _P, ctypes.c_char_p, ctypes.POINTER(_I64), ctypes.POINTER(_D))
@@ -459,6 +462,38 @@ class Context:
         self._check(lib.rhg_grid_read(self.h, os.fsencode(path), 1 if coulomb else 0, _ptr(ub),
                                       None if nc is None else _ptr(nc), ctypes.byref(h)), "grid_read")
         return Grid(self, h)
+
+    # ---- finite loading (src/inter_energy.c, src/ewald.c:6859)
+    def set_adsorbates(self, start, types, xyz):
+        """The adsorbate molecules present: molecule m holds atoms
+        [start[m], start[m+1]) of the given pseudo-atom types at xyz."""
+        st = np.ascontiguousarray(start, dtype=np.int32)
+        ty = np.ascontiguousarray(types, dtype=np.int32)
+        xy = _f64(xyz).reshape(-1, 3)
+        n_mol = max(len(st) - 1, 0)
+        self._check(lib.rhg_set_adsorbates(self.h, n_mol, _ptr(st) if n_mol else None,
+                                           _ptr(ty) if n_mol else None, _ptr(xy) if n_mol else None),
+                    "set_adsorbates")
+
+    def CalculateInterEnergyAdsorbateAtPosition(self, xyz, type_a, exclude=-1):
+        """(u_vdw, u_cc) of pseudo-atom type_a at n positions against every
+        adsorbate molecule but `exclude`."""
+        xyz = _f64(xyz).reshape(-1, 3)
+        uv, uc = np.zeros(len(xyz)), np.zeros(len(xyz))
+        self._check(lib.rhg_inter_energy_at_positions(self.h, len(xyz), _ptr(xyz), type_a, exclude,
+                                                      _ptr(uv), _ptr(uc)), "inter_energy")
+        return uv, uc
+
+    def CalculateEwaldFourierMove(self, new_types=(), new_xyz=None, old_mol=-1):
+        """(host, adsorbate) Fourier changes of inserting a molecule and/or
+        removing adsorbate old_mol."""
+        ty = np.ascontiguousarray(list(new_types), dtype=np.int32)
+        xy = _f64(new_xyz if new_xyz is not None else np.zeros((0, 3))).reshape(-1, 3)
+        out = np.zeros(2)
+        self._check(lib.rhg_ewald_fourier_move(self.h, len(ty), _ptr(ty) if len(ty) else None,
+                                               _ptr(xy) if len(ty) else None, old_mol, _ptr(out)),
+                    "fourier_move")
+        return out
 
     def use_grid(self, type_a, grid):
         """UseTabularGrid for pseudo-atom type_a (grid None detaches); the
