@@ -161,6 +161,15 @@ int rhg_framework_energy_at_positions(rhg_ctx *ctx, int64_t n, const double *xyz
  * kz -kmax..kmax): KFactor and StoreTotalChargeFramework
  * (src/ewald.c:532-600, 1289-1731).  Arrays hold nk values. */
 int rhg_ewald_tables(rhg_ctx *ctx, double *kfactor, double *sf_re, double *sf_im);
+/* REAL RetraceMolecule(CBMC_DELETION) (src/cbmc.h, src/cbmc.c:2928-3054) for
+ * n existing molecules: old_xyz holds each one's n_sites x 3 positions; fb
+ * (n x k_fb x 3) and cord (n x k_or x n_sites x 3) the other trials, whose
+ * entries 0 are ignored (the molecule's own configuration takes that slot
+ * and is always chosen).  `exclude` is the molecule's adsorbate index (-1:
+ * none) so that it does not interact with itself.  rec as rhg_widom_run
+ * (W = RosenbluthOld; no Fourier term, which rhg_ewald_fourier_move gives). */
+int rhg_widom_retrace_trials(rhg_ctx *ctx, const rhg_molecule *mol, int32_t exclude, int64_t n,
+                             const double *old_xyz, const double *fb, const double *cord, double *rec);
 /* int CalculateEwaldFourierAdsorbate(int New,int Old,int mol,int store)
  * (src/ewald.h:86, src/ewald.c:6859) for n_mol independent new molecules
  * at zero loading: d_host = UHostAdsorbateChargeChargeFourierDelta,
@@ -244,6 +253,15 @@ int rhg_set_adsorbates(rhg_ctx *ctx, int32_t n_mol, const int32_t *start, const 
  * (-1: none).  u_vdw, u_cc nullable. */
 int rhg_inter_energy_at_positions(rhg_ctx *ctx, int64_t n, const double *xyz, int32_t type, int32_t exclude,
                                   double *u_vdw, double *u_cc);
+/* REAL RetraceMolecule(CBMC_DELETION) (src/cbmc.h, src/cbmc.c:2928-3054) for
+ * n existing molecules: old_xyz holds each one's n_sites x 3 positions; fb
+ * (n x k_fb x 3) and cord (n x k_or x n_sites x 3) the other trials, whose
+ * entries 0 are ignored (the molecule's own configuration takes that slot
+ * and is always chosen).  `exclude` is the molecule's adsorbate index (-1:
+ * none) so that it does not interact with itself.  rec as rhg_widom_run
+ * (W = RosenbluthOld; no Fourier term, which rhg_ewald_fourier_move gives). */
+int rhg_widom_retrace_trials(rhg_ctx *ctx, const rhg_molecule *mol, int32_t exclude, int64_t n,
+                             const double *old_xyz, const double *fb, const double *cord, double *rec);
 /* int CalculateEwaldFourierAdsorbate(int New,int Old,int mol,int store)
  * (src/ewald.h:86, src/ewald.c:6859-7565) for one move: the new molecule's
  * n_new sites (0: none) and/or the removal of adsorbate molecule old_mol

@@ -88,6 +88,7 @@ def _declare(L):
     f("orc_ewald_self_energy_framework", _D, _P)
     f("orc_fourier_adsorbate", None, _P, _I, _P, _P, _P)
     f("orc_widom_insertion_trials", None, _P, _P, _P, _P)
+    f("orc_widom_retrace_trials", None, _P, _P, _I, _P, _P, _P)
     f("orc_widom_run_stream", None, _P, _P, _U64, _I64, _I64, _P)
     f("orc_stream_trials", None, _P, _P, _U64, _I64, _P, _P, _P)
     f("orc_widom_run_mt", _I64, _P, _P, _U64, _I64, _I, _P)
@@ -367,6 +368,23 @@ def stream_trials(system, mol, seed, insertion):
     lib().orc_stream_trials(system.h, ctypes.byref(mol), seed, insertion, _ptr(fb), _ptr(cord),
                             _ptr(usel))
     return fb, cord, usel
+
+
+def widom_retrace_trials(system, mol, exclude, old, fb, cord):
+    """RetraceMolecule(CBMC_DELETION) of the molecule at `old` (n_sites x 3)
+    with the other trials fb, cord (entries 0 ignored)."""
+    t = Trials()
+    for i in range(len(fb)):
+        for d in range(3):
+            t.fb[i][d] = float(fb[i][d])
+    for iu in range(cord.shape[0]):
+        for j in range(cord.shape[1]):
+            for d in range(3):
+                t.cord[iu][j][d] = float(cord[iu][j][d])
+    rec = np.zeros(REC_N)
+    o = _f64(old).reshape(-1)
+    lib().orc_widom_retrace_trials(system.h, ctypes.byref(mol), exclude, _ptr(o), ctypes.byref(t), _ptr(rec))
+    return rec
 
 
 def widom_insertion_trials(system, mol, fb, cord, usel):

@@ -1097,7 +1097,27 @@ static void random_rotation(draw_t *d, double (*cord)[3], int n) {
   }
 }
 
+/* retrace of an existing molecule (RetraceMolecule with CBMC_DELETION,
+ * src/cbmc.c:2928-3054, 623-795, 2851-2924): trial 0 of both stages is the
+ * molecule's own configuration `old` (n_sites x 3) and is always "chosen",
+ * the molecule `exclude` is left out of the guest-guest sums, and there is no
+ * minimum-Rosenbluth, blocked-pocket or Fourier step */
+typedef struct {
+  int retrace;
+  int exclude;
+  const double *old;
+} core_opts;
+
+static void widom_core_opts(const orc_system *s, const orc_molecule *m, draw_t *d, double rec[ORC_REC_N],
+                            const core_opts *o);
+
 static void widom_core(const orc_system *s, const orc_molecule *m, draw_t *d, double rec[ORC_REC_N]) {
+  core_opts o = {0, s->ads_nmol, NULL};
+  widom_core_opts(s, m, d, rec, &o);
+}
+
+static void widom_core_opts(const orc_system *s, const orc_molecule *m, draw_t *d, double rec[ORC_REC_N],
+                            const core_opts *opt) {
   const double crit = ENERGY_OVERLAP_CRITERIA;
   const double beta = s->beta;
   int k_fb = m->k_fb, k_or = m->k_or;
@@ -1105,8 +1125,9 @@ static void widom_core(const orc_system *s, const orc_molecule *m, draw_t *d, do
   double bf[64], evdw[64], ecc[64], eav[64], eac[64];
   int ov[64];
   /* the inserted molecule is adsorbate number ads_nmol: nothing excluded
-   * (src/mc_moves.c:7856) */
-  const int excl = s->ads_nmol;
+   * (src/mc_moves.c:7856); a retraced one excludes itself */
+  const int excl = opt->exclude;
+  const int retrace = opt->retrace;
   double fbpos[64][3];
   for (int i = 0; i < ORC_REC_N; i++) rec[i] = 0.0;
 
@@ -1116,7 +1137,12 @@ static void widom_core(const orc_system *s, const orc_molecule *m, draw_t *d, do
   int type0 = m->type[start];
   for (int i = 0; i < k_fb; i++) {
     double sv[3];
-    if (d->given) {
+    if (retrace && i == 0) {
+      /* FirstBeadPosition = OldPosition[start] (src/cbmc.c:2990) */
+      fbpos[0][0] = opt->old[3 * start];
+      fbpos[0][1] = opt->old[3 * start + 1];
+      fbpos[0][2] = opt->old[3 * start + 2];
+    } else if (d->given) {
       fbpos[i][0] = d->given->fb[i][0];
       fbpos[i][1] = d->given->fb[i][1];
       fbpos[i][2] = d->given->fb[i][2];
@@ -1153,6 +1179,10 @@ static void widom_core(const orc_system *s, const orc_molecule *m, draw_t *d, do
     }
   }
   rec[ORC_REC_FBMASK] = (double)fbmask;
+  /* a retraced configuration that itself overlaps is reported as overlap
+   * (the reference would read stale trial energies; accepted moves never
+   * produce such a configuration) */
+  if (retrace && ov[0]) OVERLAP = 1;
   if (OVERLAP) {
     rec[ORC_REC_OVERLAP] = 1.0;
     rec[ORC_REC_STAGE] = 0.0;
@@ -1161,13 +1191,16 @@ static void widom_core(const orc_system *s, const orc_molecule *m, draw_t *d, do
   double W1 = 0.0;
   for (int i = 0; i < k_fb; i++)
     if (!ov[i]) W1 += exp(bf[i]);
-  double usel = d->given ? d->given->u_sel_fb : draw(d);
-  if (d->record) d->record->u_sel_fb = usel;
-  int sel = select_trial(bf, ov, k_fb, usel);
+  int sel = 0;
+  if (!retrace) {
+    double usel = d->given ? d->given->u_sel_fb : draw(d);
+    if (d->record) d->record->u_sel_fb = usel;
+    sel = select_trial(bf, ov, k_fb, usel);
+  }
   W1 /= (double)k_fb;
   rec[ORC_REC_SELFB] = sel;
   rec[ORC_REC_W1] = W1;
-  if (W1 < MINIMUM_ROSENBLUTH_FACTOR) {
+  if (!retrace && W1 < MINIMUM_ROSENBLUTH_FACTOR) {
     rec[ORC_REC_OVERLAP] = 1.0;
     rec[ORC_REC_STAGE] = 1.0;
     return;
@@ -1213,6 +1246,13 @@ static void widom_core(const orc_system *s, const orc_molecule *m, draw_t *d, do
           d->record->cord[iu][j][2] = cord[j][2];
         }
       for (int j = 0; j < n; j++) {
+        if (retrace && iu == 0) {
+          /* TrialPositions[0] = OldPosition (src/cbmc.c:2862-2864) */
+          tpos[0][j][0] = opt->old[3 * j];
+          tpos[0][j][1] = opt->old[3 * j + 1];
+          tpos[0][j][2] = opt->old[3 * j + 2];
+          continue;
+        }
         tpos[iu][j][0] = newpos[start][0] + cord[j][0];
         tpos[iu][j][1] = newpos[start][1] + cord[j][1];
         tpos[iu][j][2] = newpos[start][2] + cord[j][2];
@@ -1253,6 +1293,7 @@ static void widom_core(const orc_system *s, const orc_molecule *m, draw_t *d, do
       }
     }
     rec[ORC_REC_ORMASK] = (double)ormask;
+    if (retrace && ovo[0]) ALLOV = 1;
     if (ALLOV) {
       rec[ORC_REC_OVERLAP] = 1.0;
       rec[ORC_REC_STAGE] = 2.0;
@@ -1262,13 +1303,16 @@ static void widom_core(const orc_system *s, const orc_molecule *m, draw_t *d, do
     for (int i = 0; i < k_or; i++)
       if (!ovo[i]) weight += exp(bf[i]);
     weight = weight / (double)k_or;
-    double uso = d->given ? d->given->u_sel_or : draw(d);
-    if (d->record) d->record->u_sel_or = uso;
-    int iwalk = select_trial(bf, ovo, k_or, uso);
+    int iwalk = 0;
+    if (!retrace) {
+      double uso = d->given ? d->given->u_sel_or : draw(d);
+      if (d->record) d->record->u_sel_or = uso;
+      iwalk = select_trial(bf, ovo, k_or, uso);
+    }
     rec[ORC_REC_SELOR] = iwalk;
     rec[ORC_REC_W2] = weight;
     RosenbluthNew *= weight * 1.0;
-    if (RosenbluthNew < MINIMUM_ROSENBLUTH_FACTOR) {
+    if (!retrace && RosenbluthNew < MINIMUM_ROSENBLUTH_FACTOR) {
       rec[ORC_REC_OVERLAP] = 1.0;
       rec[ORC_REC_STAGE] = 3.0;
       return;
@@ -1294,7 +1338,7 @@ static void widom_core(const orc_system *s, const orc_molecule *m, draw_t *d, do
   RosenbluthNew *= exp(-beta * (UHostChargeChargeNew + UAdsorbateChargeChargeNew + 0.0));
   /* WidomAdsorbateMove (src/mc_moves.c:7882-7886): a grown molecule with an
    * atom in a blocked pocket contributes nothing */
-  for (int j = 0; j < m->n_sites; j++)
+  for (int j = 0; j < m->n_sites && !retrace; j++)
     if (orc_blocked_pocket(s, newpos[j])) {
       rec[ORC_REC_W] = 0.0;
       rec[ORC_REC_DU] = 0.0;
@@ -1309,7 +1353,7 @@ static void widom_core(const orc_system *s, const orc_molecule *m, draw_t *d, do
   /* WidomAdsorbateMove (src/mc_moves.c:7878-7933): tail correction 0 */
   RosenbluthNew *= exp(-beta * 0.0);
   double fo[5] = {0, 0, 0, 0, 0};
-  if (s->charge_method == ORC_CHARGE_EWALD) {
+  if (s->charge_method == ORC_CHARGE_EWALD && !retrace) {
     double xyz[3 * ORC_MAX_SITES];
     for (int j = 0; j < m->n_sites; j++) {
       xyz[3 * j] = newpos[j][0];
@@ -1347,6 +1391,15 @@ static void insertion_impl(const orc_system *s, const orc_molecule *m, orc_unifo
 void orc_widom_insertion(const orc_system *s, const orc_molecule *m, orc_uniform_fn next,
                          void *state, double rec[ORC_REC_N], orc_trials *trials_out) {
   insertion_impl(s, m, next, state, rec, trials_out, 0);
+}
+
+void orc_widom_retrace_trials(const orc_system *s, const orc_molecule *m, int exclude, const double *old,
+                              const orc_trials *t, double rec[ORC_REC_N]) {
+  draw_t d;
+  memset(&d, 0, sizeof(d));
+  d.given = t;
+  core_opts o = {1, exclude < 0 ? s->ads_nmol : exclude, old};
+  widom_core_opts(s, m, &d, rec, &o);
 }
 
 void orc_widom_insertion_trials(const orc_system *s, const orc_molecule *m,

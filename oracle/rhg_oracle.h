@@ -163,6 +163,11 @@ void orc_widom_insertion(const orc_system *s, const orc_molecule *m, orc_uniform
 void orc_widom_insertion_trials(const orc_system *s, const orc_molecule *m,
                                 const orc_trials *t, double rec[ORC_REC_N]);
 
+/* RetraceMolecule(CBMC_DELETION) of an existing molecule (src/cbmc.c:2928-3054):
+ * old = its n_sites x 3 positions, t the other trials (entries 0 ignored),
+ * exclude = its adsorbate index (-1: none) */
+void orc_widom_retrace_trials(const orc_system *s, const orc_molecule *m, int exclude, const double *old,
+                              const orc_trials *t, double rec[ORC_REC_N]);
 /* Counter-stream Widom over insertions [first, first+n): rec is n x ORC_REC_N */
 void orc_widom_run_stream(const orc_system *s, const orc_molecule *m, uint64_t seed,
                           int64_t first, int64_t n, double *rec);

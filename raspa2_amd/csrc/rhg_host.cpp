@@ -1241,7 +1241,7 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
     if (w.fb_av) {
       // guest-guest terms; the inserted molecule is number ads_nmol, so none
       // is excluded (src/mc_moves.c:7856, src/cbmc.c:683-689)
-      const AdsArgs aa = ads_args(c, type0, c->ads_nmol, 1, 1);
+      const AdsArgs aa = ads_args(c, type0, a.exclude, 1, 1);
       if (!aa.do_cc) fill(w.fb_ac, w.cap * m.k_fb, 0.0, st);
       launch_site_ads(aa, s, w.fb_av, aa.do_cc ? w.fb_ac : nullptr, st);
     }
@@ -1264,7 +1264,7 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
       EnergyOut o{w.or_uv + base, nullptr, nullptr};
       if (int rc = site_energy(c, mol->type[m.place[p]], ea, s, o, "bor")) return rc;
       if (w.or_av)
-        launch_site_ads(ads_args(c, mol->type[m.place[p]], c->ads_nmol, 1, 0), s, w.or_av + base, nullptr, st);
+        launch_site_ads(ads_args(c, mol->type[m.place[p]], a.exclude, 1, 0), s, w.or_av + base, nullptr, st);
     }
     {
       Timed t(c, "widom_select");
@@ -1287,7 +1287,7 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
       EnergyOut o{nullptr, w.cc_place + base, nullptr};
       if (int rc = site_energy(c, mol->type[m.place[p]], ea, s, o, "bcc")) return rc;
       if (w.cc_ads)
-        launch_site_ads(ads_args(c, mol->type[m.place[p]], c->ads_nmol, 0, 1), s, nullptr, w.cc_ads + base, st);
+        launch_site_ads(ads_args(c, mol->type[m.place[p]], a.exclude, 0, 1), s, nullptr, w.cc_ads + base, st);
     }
   }
   if (a.ewald) {
@@ -1376,6 +1376,8 @@ WidomArgs widom_args(rhg_ctx *c, const MolDev &m, uint64_t seed, int64_t cap) {
   a.n_eff = c->d_neff;
   a.use_host_trials = 0;
   a.ewald = (c->charge_method == RHG_CHARGE_EWALD) ? 1 : 0;
+  a.retrace = 0;
+  a.exclude = c->ads_nmol; // an inserted molecule is number ads_nmol: none left out
   return a;
 }
 
@@ -1585,6 +1587,51 @@ int rhg_widom_run_trials(rhg_ctx *c, const rhg_molecule *mol, int64_t n, const d
   launch_widom_records(a, B.w, drec, c->stream);
   HIPC(c, hipMemcpyAsync(rec, drec, (size_t)n * RHG_REC_N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
+  return RHG_OK;
+}
+
+int rhg_widom_retrace_trials(rhg_ctx *c, const rhg_molecule *mol, int32_t exclude, int64_t n,
+                             const double *old_xyz, const double *fb, const double *cord, double *rec) {
+  if (!c || !mol || n < 0 || !rec || (n > 0 && (!old_xyz || !fb))) return RHG_ERR_INVALID;
+  if (exclude >= c->ads_nmol) return fail(c, RHG_ERR_INVALID, "no such adsorbate molecule");
+  MolDev m;
+  if (int rc = build_mol(c, mol, m)) return rc;
+  if (m.n_sites > 1 && !cord) return fail(c, RHG_ERR_INVALID, "cord required");
+  if (n == 0) return RHG_OK;
+  hipSetDevice(c->device);
+  if (int rc = widom_state(c)) return rc;
+  WidomBufs B;
+  if (int rc = widom_buffers(c, m, n, B)) return rc;
+  // first-bead trial 0 is the molecule's starting bead (src/cbmc.c:2990)
+  std::vector<double> hfb_h(fb, fb + (size_t)n * m.k_fb * 3);
+  for (int64_t i = 0; i < n; ++i)
+    for (int d = 0; d < 3; ++d) hfb_h[(size_t)i * m.k_fb * 3 + d] = old_xyz[((size_t)i * m.n_sites + m.start) * 3 + d];
+  size_t bfb = (size_t)n * m.k_fb * 3 * sizeof(double);
+  size_t bcord = (size_t)n * m.k_or * m.n_sites * 3 * sizeof(double);
+  size_t bold = (size_t)n * m.n_sites * 3 * sizeof(double);
+  double *hfb = (double *)work(c, "h_fb", bfb), *hold = (double *)work(c, "h_old", bold);
+  double *hcord = m.n_sites > 1 ? (double *)work(c, "h_cord", bcord) : nullptr;
+  if (!hfb || !hold || (m.n_sites > 1 && !hcord)) return fail(c, RHG_ERR_OOM, "trial buffers");
+  upload(c, hfb, hfb_h.data(), bfb);
+  upload(c, hold, old_xyz, bold);
+  if (hcord) upload(c, hcord, cord, bcord);
+  B.w.h_fb = hfb;
+  B.w.h_cord = hcord;
+  B.w.h_usel = nullptr;
+  B.w.h_old = hold;
+  int64_t jb[6] = {0, n, n, n, n, n};
+  launch_set_ctl(c->d_ctl, 0, n, jb, c->stream);
+  WidomArgs a = widom_args(c, m, 0, n);
+  a.use_host_trials = 1;
+  a.retrace = 1;
+  a.ewald = 0; // RetraceMolecule has no Fourier step
+  a.exclude = exclude < 0 ? c->ads_nmol : exclude;
+  if (int rc = widom_chunk(c, mol, m, B, a, false)) return rc;
+  double *drec = (double *)work(c, "w_rec", (size_t)n * RHG_REC_N * sizeof(double));
+  if (!drec) return fail(c, RHG_ERR_OOM, "rec buffer");
+  launch_widom_records(a, B.w, drec, c->stream);
+  HIPC(c, hipMemcpyAsync(rec, drec, (size_t)n * RHG_REC_N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream)); // hfb_h is released on return
   return RHG_OK;
 }
 

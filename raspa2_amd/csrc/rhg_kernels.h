@@ -171,8 +171,9 @@ struct WidomWork {
   // guest-guest terms at finite loading (all null without adsorbates), laid
   // out as fb_uv, or_uv and cc_place; per insertion the selected sums
   double *fb_av, *fb_ac, *or_av, *cc_ads, *uadsv, *uadsc;
-  // host-supplied trials (nullable)
-  const double *h_fb, *h_cord, *h_usel;
+  // host-supplied trials (nullable); h_old: [insertion][site][3] positions
+  // of retraced molecules
+  const double *h_fb, *h_cord, *h_usel, *h_old;
 };
 
 // Control block of a Widom run, device-resident so that a captured chunk
@@ -203,6 +204,11 @@ struct WidomArgs {
   int32_t *n_eff;      // device: insertions of this chunk, written by fb_gen
   int use_host_trials;
   int ewald;
+  // retrace of existing molecules (RetraceMolecule, CBMC_DELETION): trial 0
+  // of both stages is the molecule's configuration (h_old) and is always
+  // chosen; no minimum-Rosenbluth, blocked-pocket or Fourier step
+  int retrace;
+  int exclude; // adsorbate molecule left out of the guest-guest terms
 };
 
 __device__ __forceinline__ int64_t chunk_first(const WidomArgs &a) { return a.ctl[CTL_FIRST]; }

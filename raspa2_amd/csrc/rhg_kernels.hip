@@ -1357,13 +1357,17 @@ __global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, cons
   w.ureal[i] = 0.0;
   w.ufh[i] = 0.0;
   w.ufa[i] = 0.0;
-  if (!any) {
+  // a retraced configuration that itself overlaps is reported as overlap
+  // (the reference would read stale trial energies there)
+  if (!any || (a.retrace && (mask & 1ULL))) {
     w.stage[i] = 0;
     return;
   }
   uint64_t key = 0;
-  double usel;
-  if (a.use_host_trials)
+  double usel = 0.0;
+  if (a.retrace) {
+    // the old position is trial 0 and is always chosen (src/cbmc.c:758-767)
+  } else if (a.use_host_trials)
     usel = w.h_usel[2 * i];
   else {
     key = stream_key(a.seed, (uint64_t)(chunk_first(a) + i));
@@ -1376,11 +1380,11 @@ __global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, cons
     b = -beta * (uv + av + 0.0);
     return true;
   };
-  int sel = select_trial(bf, k, usel);
+  int sel = a.retrace ? 0 : select_trial(bf, k, usel);
   W1 /= (double)k;
   w.selfb[i] = sel;
   w.W1[i] = W1;
-  if (W1 < kMinRosenbluth) {
+  if (!a.retrace && W1 < kMinRosenbluth) {
     w.stage[i] = 1;
     return;
   }
@@ -1398,7 +1402,7 @@ __global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, cons
   w.stage[i] = 4;
   const MolDev &m = a.mol;
   if (m.n_sites == 1) {
-    if (blocked_pocket(a.pockets, fx, fy, fz)) {
+    if (!a.retrace && blocked_pocket(a.pockets, fx, fy, fz)) {
       w.stage[i] = 5;
       w.W[i] = 0.0;
       w.uvdw[i] = 0.0;
@@ -1446,7 +1450,13 @@ __global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, cons
         rotate(R, b0, cj);
       }
       double x, y, z;
-      {
+      if (a.retrace && iu == 0) {
+        // TrialPositions[0] = OldPosition (src/cbmc.c:2862-2864)
+        const double *o = w.h_old + (i * m.n_sites + j) * 3;
+        x = o[0];
+        y = o[1];
+        z = o[2];
+      } else {
 #pragma clang fp contract(off)
         x = fx + cj[0];
         y = fy + cj[1];
@@ -1458,7 +1468,7 @@ __global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, cons
       w.orz[o] = z;
     }
   }
-  w.usel_or[i] = a.use_host_trials ? w.h_usel[2 * i + 1] : stream_uniform(key, c);
+  w.usel_or[i] = a.retrace ? 0.0 : (a.use_host_trials ? w.h_usel[2 * i + 1] : stream_uniform(key, c));
 }
 
 // Rosen's selection over orientations (src/cbmc.c:1154-1478, 2762-2849)
@@ -1499,7 +1509,7 @@ __global__ __launch_bounds__(256) void k_widom_or_select(const WidomArgs a, cons
       mask |= (1ULL << iu);
   }
   w.ormask[i] = mask;
-  if (!any) {
+  if (!any || (a.retrace && (mask & 1ULL))) {
     w.stage[i] = 2;
     w.W[i] = 0.0;
     w.uvdw[i] = 0.0;
@@ -1513,18 +1523,18 @@ __global__ __launch_bounds__(256) void k_widom_or_select(const WidomArgs a, cons
     b = -beta * (ev + ea + 0.0 + 0.0);
     return true;
   };
-  const int iwalk = select_trial(bf, ko, w.usel_or[i]);
+  const int iwalk = a.retrace ? 0 : select_trial(bf, ko, w.usel_or[i]);
   double Wn = w.W1[i] * (weight * 1.0);
   w.selor[i] = iwalk;
   w.W2[i] = weight;
-  if (Wn < kMinRosenbluth) {
+  if (!a.retrace && Wn < kMinRosenbluth) {
     w.stage[i] = 3;
     w.W[i] = 0.0;
     w.uvdw[i] = 0.0;
     w.ureal[i] = 0.0;
     return;
   }
-  if (a.pockets.n > 0) {
+  if (!a.retrace && a.pockets.n > 0) {
     // every atom of the grown molecule, in atom order
     bool blocked = false;
     for (int j = 0; j < m.n_sites && !blocked; ++j) {

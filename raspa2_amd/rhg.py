@@ -126,6 +126,7 @@ def _load():
     f("rhg_set_adsorbates", _P, _I32, _P, _P, _P)
     f("rhg_inter_energy_at_positions", _P, _I64, _P, _I32, _I32, _P, _P)
     f("rhg_ewald_fourier_move", _P, _I32, _P, _P, _I32, _P)
+    f("rhg_widom_retrace_trials", _P, _P, _I32, _I64, _P, _P, _P, _P)
     f("rhg_use_coulomb_grid", _P, _P)
     f("rhg_profile_enable", _P, _I32)
     f("rhg_profile_get", _P, ctypes.c_char_p, ctypes.POINTER(_I64), ctypes.POINTER(_D))
@@ -483,6 +484,19 @@ class Context:
         self._check(lib.rhg_inter_energy_at_positions(self.h, len(xyz), _ptr(xyz), type_a, exclude,
                                                       _ptr(uv), _ptr(uc)), "inter_energy")
         return uv, uc
+
+    def RetraceMolecule(self, mol, exclude, old_xyz, fb, cord):
+        """Rosenbluth weights of existing molecules (CBMC_DELETION): old_xyz
+        n x n_sites x 3, fb n x k_fb x 3, cord n x k_or x n_sites x 3 (entries
+        0 of the trials are taken by the molecules' own configuration)."""
+        old_xyz = _f64(old_xyz)
+        n = old_xyz.shape[0]
+        fb = _f64(fb)
+        cord = _f64(cord)
+        rec = np.zeros((n, REC_N))
+        self._check(lib.rhg_widom_retrace_trials(self.h, ctypes.byref(mol), exclude, n, _ptr(old_xyz),
+                                                 _ptr(fb), _ptr(cord), _ptr(rec)), "retrace")
+        return rec
 
     def CalculateEwaldFourierMove(self, new_types=(), new_xyz=None, old_mol=-1):
         """(host, adsorbate) Fourier changes of inserting a molecule and/or

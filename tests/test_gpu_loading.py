@@ -137,3 +137,37 @@ def test_widom_loaded_mixed_and_updates(W, O, R):
     _, rc = clean.widom_run(mc, 29, 20000, records=True)
     _, r3 = ctx.widom_run(mg, 29, 20000, records=True)
     assert same_bits(r3, rc)
+
+
+def test_retrace_existing_molecules(W, O, R):
+    """RetraceMolecule (CBMC_DELETION) of every adsorbed CO2 of a loaded
+    supercell, each excluding itself: trial masks, host and guest-guest VDW
+    bit for bit, W1, W2 and W within 1e-9 of the oracle."""
+    fw = W.c1_framework()
+    sys_o, ctx, to, tg, mo, mg, ion, (start, tys, xyz) = _both(W, O, R, fw, 40, 21)
+    n = len(start) - 2  # the CO2 molecules (the ion is last)
+    fbs, cords, olds = [], [], []
+    for i in range(n):
+        fb, cord, _ = O.stream_trials(sys_o, mo, 47, i)
+        fbs.append(fb)
+        cords.append(cord)
+        olds.append(xyz[start[i]:start[i + 1]])
+    rg = np.array([ctx.RetraceMolecule(mg, i, olds[i][None], fbs[i][None], cords[i][None])[0]
+                   for i in range(n)])
+    rb = ctx.RetraceMolecule(mg, -1, np.array(olds), np.array(fbs), np.array(cords))
+    ro = np.array([O.widom_retrace_trials(sys_o, mo, i, olds[i], fbs[i], cords[i]) for i in range(n)])
+    for f in ("FBMASK", "ORMASK", "STAGE", "SELFB", "SELOR"):
+        assert np.array_equal(ro[:, FIELDS[f]], rg[:, FIELDS[f]]), f
+    # randomly placed molecules can overlap the framework: their retrace
+    # stops (stage 0 or 2) on both sides; the others are compared
+    ok = ro[:, FIELDS["STAGE"]] == 4
+    assert ok.sum() >= 15
+    for f in ("UVDW", "UADSVDW"):
+        assert same_bits(ro[ok, FIELDS[f]], rg[ok, FIELDS[f]]), f
+    for f in ("W1", "W2", "W", "UREAL", "UADSREAL"):
+        a, b = ro[ok, FIELDS[f]], rg[ok, FIELDS[f]]
+        assert np.max(np.abs(a - b) / np.maximum(np.abs(a), 1e-300)) <= 1e-9, f
+    # a configuration that overlaps itself is reported as overlap (stage 0)
+    assert np.all(rb[:, FIELDS["STAGE"]] == 0)
+    # without excluding themselves the molecules sit on their own atoms
+    assert np.all(rb[:, FIELDS["FBMASK"]].astype(np.int64) & 1)

@@ -252,3 +252,44 @@ def test_widom_adsorbate_bookkeeping(O, W):
         du = rec[14] + rec[2] + rec[15] + rec[3] + rec[4] + rec[5]
         assert rec[1] == du
     assert checked >= 10
+
+
+def test_retrace_matches_insertion(O, W):
+    """CBMC detailed balance at finite loading: insert a molecule with given
+    trials, add it to the adsorbates, then retrace it with the same trial set
+    (its selected trials moved to slot 0): the retrace weight equals the
+    insertion weight without its Fourier factor, W_ins = W_ret
+    exp(-beta (UF_host + UF_ads)), with the same energies; without excluding
+    itself the retrace overlaps."""
+    fw, s, types, t_ion, start, tys, xyz = _loaded(O, W, n_mol=10, seed=17)
+    mol = O.molecule(types, W.CO2.body, start=W.CO2.start, k_fb=5, k_or=8)
+    beta = s.info()["beta"]
+    st = W.CO2.start
+    done = 0
+    for i in range(60):
+        fb, cord, usel = O.stream_trials(s, mol, 43, i)
+        rec = O.widom_insertion_trials(s, mol, fb, cord, usel)
+        if rec[13] != 4:
+            continue
+        sf, so = int(rec[7]), int(rec[8])
+        old = np.array([fb[sf] + cord[so][j] for j in range(3)])
+        old[st] = fb[sf]
+        fo = s.fourier_move(types, old, -1)
+        # the molecule joins the adsorbates as molecule number 10
+        s.set_adsorbates(start + [start[-1] + 3], tys + list(types), np.r_[xyz, old])
+        fb_r = np.r_[fb[sf:sf + 1], np.delete(fb, sf, axis=0)]
+        cord_r = np.r_[cord[so:so + 1], np.delete(cord, so, axis=0)]
+        ret = O.widom_retrace_trials(s, mol, 10, old, fb_r, cord_r)
+        assert ret[13] == 4 and ret[7] == 0 and ret[8] == 0
+        for f in (2, 3, 14, 15):  # host and guest-guest energies of the configuration
+            assert ret[f] == rec[f]
+        assert abs(ret[11] - rec[11]) <= 1e-14 * rec[11]
+        assert abs(ret[12] - rec[12]) <= 1e-14 * rec[12]
+        w_ins = ret[0] * math.exp(-beta * (fo[0] + fo[1] + 0.0))
+        assert abs(w_ins - rec[0]) <= 1e-12 * rec[0]
+        assert ret[4] == 0.0 and ret[5] == 0.0
+        bad = O.widom_retrace_trials(s, mol, -1, old, fb_r, cord_r)
+        assert int(bad[9]) & 1  # trial 0 sits on its own atoms
+        s.set_adsorbates(start, tys, xyz)
+        done += 1
+    assert done >= 10
