@@ -165,10 +165,12 @@ int rhg_ewald_tables(rhg_ctx *ctx, double *kfactor, double *sf_re, double *sf_im
  * n existing molecules: old_xyz holds each one's n_sites x 3 positions; fb
  * (n x k_fb x 3) and cord (n x k_or x n_sites x 3) the other trials, whose
  * entries 0 are ignored (the molecule's own configuration takes that slot
- * and is always chosen).  `exclude` is the molecule's adsorbate index (-1:
- * none) so that it does not interact with itself.  rec as rhg_widom_run
- * (W = RosenbluthOld; no Fourier term, which rhg_ewald_fourier_move gives). */
-int rhg_widom_retrace_trials(rhg_ctx *ctx, const rhg_molecule *mol, int32_t exclude, int64_t n,
+ * and is always chosen).  exclude[i] is molecule i's adsorbate index (-1:
+ * none; exclude null: none for all) so that it does not interact with
+ * itself.  rec as rhg_widom_run (W = RosenbluthOld; no Fourier term, which
+ * rhg_ewald_fourier_move gives).  A configuration that itself overlaps is
+ * reported as overlap (stage 0 or 2). */
+int rhg_widom_retrace_trials(rhg_ctx *ctx, const rhg_molecule *mol, const int32_t *exclude, int64_t n,
                              const double *old_xyz, const double *fb, const double *cord, double *rec);
 /* int CalculateEwaldFourierAdsorbate(int New,int Old,int mol,int store)
  * (src/ewald.h:86, src/ewald.c:6859) for n_mol independent new molecules
@@ -257,10 +259,12 @@ int rhg_inter_energy_at_positions(rhg_ctx *ctx, int64_t n, const double *xyz, in
  * n existing molecules: old_xyz holds each one's n_sites x 3 positions; fb
  * (n x k_fb x 3) and cord (n x k_or x n_sites x 3) the other trials, whose
  * entries 0 are ignored (the molecule's own configuration takes that slot
- * and is always chosen).  `exclude` is the molecule's adsorbate index (-1:
- * none) so that it does not interact with itself.  rec as rhg_widom_run
- * (W = RosenbluthOld; no Fourier term, which rhg_ewald_fourier_move gives). */
-int rhg_widom_retrace_trials(rhg_ctx *ctx, const rhg_molecule *mol, int32_t exclude, int64_t n,
+ * and is always chosen).  exclude[i] is molecule i's adsorbate index (-1:
+ * none; exclude null: none for all) so that it does not interact with
+ * itself.  rec as rhg_widom_run (W = RosenbluthOld; no Fourier term, which
+ * rhg_ewald_fourier_move gives).  A configuration that itself overlaps is
+ * reported as overlap (stage 0 or 2). */
+int rhg_widom_retrace_trials(rhg_ctx *ctx, const rhg_molecule *mol, const int32_t *exclude, int64_t n,
                              const double *old_xyz, const double *fb, const double *cord, double *rec);
 /* int CalculateEwaldFourierAdsorbate(int New,int Old,int mol,int store)
  * (src/ewald.h:86, src/ewald.c:6859-7565) for one move: the new molecule's

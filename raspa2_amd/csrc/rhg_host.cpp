@@ -1241,7 +1241,8 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
     if (w.fb_av) {
       // guest-guest terms; the inserted molecule is number ads_nmol, so none
       // is excluded (src/mc_moves.c:7856, src/cbmc.c:683-689)
-      const AdsArgs aa = ads_args(c, type0, a.exclude, 1, 1);
+      AdsArgs aa = ads_args(c, type0, a.exclude, 1, 1);
+      aa.excl = a.excl; // columns of the first-bead rows are insertions
       if (!aa.do_cc) fill(w.fb_ac, w.cap * m.k_fb, 0.0, st);
       launch_site_ads(aa, s, w.fb_av, aa.do_cc ? w.fb_ac : nullptr, st);
     }
@@ -1263,8 +1264,12 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
       SiteList s{w.orx + base, w.ory + base, w.orz + base, n * rows, w.n_active, rows, w.cap};
       EnergyOut o{w.or_uv + base, nullptr, nullptr};
       if (int rc = site_energy(c, mol->type[m.place[p]], ea, s, o, "bor")) return rc;
-      if (w.or_av)
-        launch_site_ads(ads_args(c, mol->type[m.place[p]], a.exclude, 1, 0), s, w.or_av + base, nullptr, st);
+      if (w.or_av) {
+        AdsArgs aa = ads_args(c, mol->type[m.place[p]], a.exclude, 1, 0);
+        aa.excl = a.excl;
+        aa.col_map = w.active; // columns are active slots
+        launch_site_ads(aa, s, w.or_av + base, nullptr, st);
+      }
     }
     {
       Timed t(c, "widom_select");
@@ -1286,8 +1291,12 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
       SiteList s{w.ccx + base, w.ccy + base, w.ccz + base, n * rows, w.n_active2, rows, w.cap};
       EnergyOut o{nullptr, w.cc_place + base, nullptr};
       if (int rc = site_energy(c, mol->type[m.place[p]], ea, s, o, "bcc")) return rc;
-      if (w.cc_ads)
-        launch_site_ads(ads_args(c, mol->type[m.place[p]], a.exclude, 0, 1), s, nullptr, w.cc_ads + base, st);
+      if (w.cc_ads) {
+        AdsArgs aa = ads_args(c, mol->type[m.place[p]], a.exclude, 0, 1);
+        aa.excl = a.excl;
+        aa.col_map = w.active2; // columns are accepted slots
+        launch_site_ads(aa, s, nullptr, w.cc_ads + base, st);
+      }
     }
   }
   if (a.ewald) {
@@ -1590,10 +1599,15 @@ int rhg_widom_run_trials(rhg_ctx *c, const rhg_molecule *mol, int64_t n, const d
   return RHG_OK;
 }
 
-int rhg_widom_retrace_trials(rhg_ctx *c, const rhg_molecule *mol, int32_t exclude, int64_t n,
+int rhg_widom_retrace_trials(rhg_ctx *c, const rhg_molecule *mol, const int32_t *exclude, int64_t n,
                              const double *old_xyz, const double *fb, const double *cord, double *rec) {
   if (!c || !mol || n < 0 || !rec || (n > 0 && (!old_xyz || !fb))) return RHG_ERR_INVALID;
-  if (exclude >= c->ads_nmol) return fail(c, RHG_ERR_INVALID, "no such adsorbate molecule");
+  std::vector<int32_t> ex((size_t)std::max<int64_t>(n, 1), c->ads_nmol);
+  if (exclude)
+    for (int64_t i = 0; i < n; ++i) {
+      if (exclude[i] >= c->ads_nmol) return fail(c, RHG_ERR_INVALID, "no such adsorbate molecule");
+      ex[i] = exclude[i] < 0 ? c->ads_nmol : exclude[i];
+    }
   MolDev m;
   if (int rc = build_mol(c, mol, m)) return rc;
   if (m.n_sites > 1 && !cord) return fail(c, RHG_ERR_INVALID, "cord required");
@@ -1611,9 +1625,11 @@ int rhg_widom_retrace_trials(rhg_ctx *c, const rhg_molecule *mol, int32_t exclud
   size_t bold = (size_t)n * m.n_sites * 3 * sizeof(double);
   double *hfb = (double *)work(c, "h_fb", bfb), *hold = (double *)work(c, "h_old", bold);
   double *hcord = m.n_sites > 1 ? (double *)work(c, "h_cord", bcord) : nullptr;
-  if (!hfb || !hold || (m.n_sites > 1 && !hcord)) return fail(c, RHG_ERR_OOM, "trial buffers");
+  int32_t *hex = (int32_t *)work(c, "h_excl", (size_t)n * sizeof(int32_t));
+  if (!hfb || !hold || !hex || (m.n_sites > 1 && !hcord)) return fail(c, RHG_ERR_OOM, "trial buffers");
   upload(c, hfb, hfb_h.data(), bfb);
   upload(c, hold, old_xyz, bold);
+  upload(c, hex, ex.data(), (size_t)n * sizeof(int32_t));
   if (hcord) upload(c, hcord, cord, bcord);
   B.w.h_fb = hfb;
   B.w.h_cord = hcord;
@@ -1625,13 +1641,13 @@ int rhg_widom_retrace_trials(rhg_ctx *c, const rhg_molecule *mol, int32_t exclud
   a.use_host_trials = 1;
   a.retrace = 1;
   a.ewald = 0; // RetraceMolecule has no Fourier step
-  a.exclude = exclude < 0 ? c->ads_nmol : exclude;
+  a.excl = hex;
   if (int rc = widom_chunk(c, mol, m, B, a, false)) return rc;
   double *drec = (double *)work(c, "w_rec", (size_t)n * RHG_REC_N * sizeof(double));
   if (!drec) return fail(c, RHG_ERR_OOM, "rec buffer");
   launch_widom_records(a, B.w, drec, c->stream);
   HIPC(c, hipMemcpyAsync(rec, drec, (size_t)n * RHG_REC_N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream)); // hfb_h is released on return
+  HIPC(c, hipStreamSynchronize(c->stream)); // hfb_h and ex are released on return
   return RHG_OK;
 }
 

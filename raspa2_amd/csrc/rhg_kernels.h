@@ -140,6 +140,11 @@ struct AdsArgs {
   const int32_t *mol;   // molecule of each atom
   int n;
   int exclude;          // molecule left out (-1 or the number of molecules: none)
+  // per-insertion exclusions (nullable): the site list's column (physical
+  // index modulo its row stride) is an insertion, or through col_map (nullable)
+  // the slot of one
+  const int32_t *excl;
+  const int32_t *col_map;
   double rc2v, rc2c, alpha, cqa;
   int do_vdw, do_cc;
 };
@@ -209,6 +214,7 @@ struct WidomArgs {
   // chosen; no minimum-Rosenbluth, blocked-pocket or Fourier step
   int retrace;
   int exclude; // adsorbate molecule left out of the guest-guest terms
+  const int32_t *excl; // per insertion instead of `exclude` (device, nullable)
 };
 
 __device__ __forceinline__ int64_t chunk_first(const WidomArgs &a) { return a.ctl[CTL_FIRST]; }

@@ -2423,9 +2423,14 @@ __global__ __launch_bounds__(256) void k_site_ads(const AdsArgs a, const SiteLis
   const bool live = i < n;
   const int64_t k = site_phys(s, live ? i : 0);
   const double px = s.x[k], py = s.y[k], pz = s.z[k];
+  int ex = a.exclude;
+  if (a.excl) {
+    const int64_t col = s.in_stride > 0 ? k % s.in_stride : k;
+    ex = a.excl[a.col_map ? a.col_map[col] : col];
+  }
   double uv = 0.0, uc = 0.0;
   for (int j = 0; j < a.n; ++j) {
-    if (a.mol[j] == a.exclude) continue;
+    if (a.mol[j] == ex) continue;
     const AtomRec64 r = a.rec[j];
     double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
     mi_exact<BCK>(a.cell, dx, dy, dz);

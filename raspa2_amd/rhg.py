@@ -126,7 +126,7 @@ def _load():
     f("rhg_set_adsorbates", _P, _I32, _P, _P, _P)
     f("rhg_inter_energy_at_positions", _P, _I64, _P, _I32, _I32, _P, _P)
     f("rhg_ewald_fourier_move", _P, _I32, _P, _P, _I32, _P)
-    f("rhg_widom_retrace_trials", _P, _P, _I32, _I64, _P, _P, _P, _P)
+    f("rhg_widom_retrace_trials", _P, _P, _P, _I64, _P, _P, _P, _P)
     f("rhg_use_coulomb_grid", _P, _P)
     f("rhg_profile_enable", _P, _I32)
     f("rhg_profile_get", _P, ctypes.c_char_p, ctypes.POINTER(_I64), ctypes.POINTER(_D))
@@ -493,8 +493,9 @@ class Context:
         n = old_xyz.shape[0]
         fb = _f64(fb)
         cord = _f64(cord)
+        ex = np.ascontiguousarray(np.broadcast_to(np.asarray(exclude, dtype=np.int32), (n,)))
         rec = np.zeros((n, REC_N))
-        self._check(lib.rhg_widom_retrace_trials(self.h, ctypes.byref(mol), exclude, n, _ptr(old_xyz),
+        self._check(lib.rhg_widom_retrace_trials(self.h, ctypes.byref(mol), _ptr(ex), n, _ptr(old_xyz),
                                                  _ptr(fb), _ptr(cord), _ptr(rec)), "retrace")
         return rec
 

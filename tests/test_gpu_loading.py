@@ -155,6 +155,9 @@ def test_retrace_existing_molecules(W, O, R):
     rg = np.array([ctx.RetraceMolecule(mg, i, olds[i][None], fbs[i][None], cords[i][None])[0]
                    for i in range(n)])
     rb = ctx.RetraceMolecule(mg, -1, np.array(olds), np.array(fbs), np.array(cords))
+    # one batch with a self-exclusion per molecule equals the single calls
+    rm = ctx.RetraceMolecule(mg, np.arange(n), np.array(olds), np.array(fbs), np.array(cords))
+    assert same_bits(rm, rg)
     ro = np.array([O.widom_retrace_trials(sys_o, mo, i, olds[i], fbs[i], cords[i]) for i in range(n)])
     for f in ("FBMASK", "ORMASK", "STAGE", "SELFB", "SELOR"):
         assert np.array_equal(ro[:, FIELDS[f]], rg[:, FIELDS[f]]), f
