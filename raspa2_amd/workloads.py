@@ -69,10 +69,10 @@ class FrameworkSpec:
         return len(self.xyz)
 
     def context(self, cutoff=12.0, charge_method=R.CHARGE_NONE, ewald_precision=1e-6,
-                temperature=300.0, precision=R.PREC_FP64, device=0):
+                temperature=300.0, precision=R.PREC_FP64, device=0, shifted=True):
         return R.Context(self.box, self.bc, self.xyz[:, 0], self.xyz[:, 1], self.xyz[:, 2],
                          self.eps_k, self.sigma, charge=self.charge, mass=self.mass,
-                         cutoff_vdw=cutoff, shifted=True, charge_method=charge_method,
+                         cutoff_vdw=cutoff, shifted=shifted, charge_method=charge_method,
                          cutoff_charge=cutoff, ewald_precision=ewald_precision,
                          temperature=temperature, device=device, precision=precision)
 

@@ -1,0 +1,79 @@
+"""The input front-end against the reference's own files and outputs: the
+IRMOF-1 CIF (structures/mofs/cif/IRMOF-1.cif) expands to the reference's 424
+atoms, the GenericMOFs force field resolves the reference's printed
+parameters, and the molecule files give the TraPPE probes
+(src/framework.c:334-2050, src/potentials.c:2990-3130)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+SHARE = os.path.join(GOLD, "share", "raspa")
+G = json.load(open(os.path.join(GOLD, "irmof1.json")))
+
+
+def _cif():
+    return open(os.path.join(SHARE, "structures", "cif", "IRMOF-1.cif")).read()
+
+
+def test_cif_p1_expansion(R):
+    from raspa2_amd import frontend as F
+    cif = F.parse_cif(_cif())
+    assert cif["cell"] == tuple(G["cell"])
+    assert len(cif["ops"]) == G["n_symmetry_ops"] == 192
+    atoms = F.p1_atoms(cif)
+    assert len(atoms) == len(G["atoms"]) == 424
+    for (lab, sym, f, q), ref in zip(atoms, G["atoms"]):
+        assert lab == ref["label"] and sym == ref["elem"]
+        assert np.allclose(f, ref["frac"], atol=1e-12, rtol=0)
+
+
+def test_forcefield_resolution(R):
+    """GenericMOFs: every framework label resolves through the '_' prefix
+    rules to the parameters behind the reference's printed mixing; helium is
+    an exact entry; masses give the printed framework mass."""
+    from raspa2_amd import frontend as F
+    ff = F.ForceField.load(os.path.join(SHARE, "forcefield", "GenericMOFs"))
+    assert ff.shifted
+    for ref in G["atoms"]:
+        assert ff.lj(ref["label"]) == (ref["eps_k"], ref["sigma"])
+    assert ff.lj("He") == (G["helium"]["eps_k"], G["helium"]["sigma"])
+    fw = F.framework_from_cif(_cif(), ff)
+    assert fw.n == 424 and fw.bc == R.BC_RECTANGULAR
+    assert abs(fw.mass.sum() - G["output"]["framework_mass"]) <= 1e-3 * G["output"]["framework_mass"]
+    fw2 = F.framework_from_cif(_cif(), ff, unit_cells=(2, 1, 1))
+    assert fw2.n == 848 and np.allclose(fw2.box[0], 2 * 25.832)
+    assert np.allclose(fw2.xyz[424:] - fw2.xyz[:424], np.tile([25.832, 0, 0], (424, 1)), atol=1e-12, rtol=0)
+
+
+def test_wildcards_and_overrides(tmp_path):
+    """Later lines override earlier ones, 'X_' matches every name starting
+    with X (case-insensitive), exact names match case-insensitively."""
+    from raspa2_amd import frontend as F
+    d = tmp_path / "ff"
+    d.mkdir()
+    (d / "force_field_mixing_rules.def").write_text(
+        "# general rule for shifted vs truncated\nshifted\n# tail\nno\n# number\n4\n# type\n"
+        "O_ lennard-jones 10.0 3.0\nO_co2 lennard-jones 79.0 3.05\nzn1 LENNARD_JONES 1.0 2.0\n"
+        "Zn_ lennard-jones 5.0 2.5 # a trailing comment\n# general mixing rule for Lennard-Jones\n"
+        "Lorentz-Berthelot\n")
+    ff = F.ForceField.load(str(d))
+    assert ff.lj("O1") == (10.0, 3.0) and ff.lj("Os") == (10.0, 3.0)
+    assert ff.lj("O_co2") == (79.0, 3.05)
+    assert ff.lj("ZN1") == (5.0, 2.5)  # the later wildcard overrides the exact line
+    with pytest.raises(KeyError):
+        ff.lj("C1")
+
+
+def test_molecules(R):
+    from raspa2_amd import frontend as F
+    ff = F.ForceField.load(os.path.join(SHARE, "forcefield", "GenericMOFs"))
+    co2 = F.read_molecule(os.path.join(SHARE, "molecules", "TraPPE", "CO2.def"), ff)
+    assert co2.n_sites == 3 and co2.start == 0
+    assert np.array_equal(co2.body, [[0, 0, 1.16], [0, 0, 0], [0, 0, -1.16]])
+    assert co2.sites[0] == (85.671, 3.017, -0.3256) and co2.sites[1] == (29.933, 2.745, 0.6512)
+    he = F.read_molecule(os.path.join(SHARE, "molecules", "TraPPE", "helium.def"), ff)
+    assert he.n_sites == 1 and he.sites[0] == (10.9, 2.64, 0.0)

@@ -1,0 +1,39 @@
+"""The reference's helium void-fraction example end to end on the GPU:
+examples/Auxiliary/HeliumVoidFraction (IRMOF-1 from its CIF, GenericMOFs,
+TraPPE helium, 298 K, 12 A cut-off) read through the front-end, against the
+reference's printed 1e7-insertion results (output lines cited in
+tests/golden/irmof1.json), and the wrapper call of python/raspa2.py:345."""
+import json
+import os
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+OUT = json.load(open(os.path.join(GOLD, "irmof1.json")))["output"]
+
+
+def _cif():
+    return open(os.path.join(GOLD, "share", "raspa", "structures", "cif", "IRMOF-1.cif")).read()
+
+
+def test_helium_void_fraction_example(R):
+    """<W>, K_H and <U_gh> of 4e6 helium insertions (10 first-bead trials
+    each) agree with the reference's printed averages within three times the
+    two printed 2-sigma spreads combined."""
+    from raspa2_amd import frontend as F
+    est = F.widom(_cif(), "helium", forcefield="GenericMOFs", cutoff=12.0, temperature=298.0,
+                  raspa_dir=GOLD, insertions=4_000_000, seed=11, precision=R.PREC_FP64)
+    assert abs(est["w"] - OUT["widom_w"]) < 3 * (OUT["widom_w_2sigma"] + est["w_err"])
+    assert abs(est["kh"] - OUT["henry"]) < 3 * (OUT["henry_2sigma"] + est["kh_err"]) + 1e-3 * OUT["henry"]
+    assert abs(est["u"] - OUT["u_gh_k"]) < 3 * (OUT["u_gh_2sigma_k"] + est["u_err"])
+
+
+def test_wrapper_call(R):
+    """get_helium_void_fraction with the wrapper's own settings (CrystalGenerator
+    force field, 12.8 A cut-off, 2000 cycles = 40000 insertions)."""
+    from raspa2_amd import frontend as F
+    w = F.get_helium_void_fraction(_cif(), cycles=2000, forcefield="CrystalGenerator", raspa_dir=GOLD)
+    assert 0.7 < w < 0.9
