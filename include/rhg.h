@@ -276,6 +276,18 @@ int rhg_widom_retrace_trials(rhg_ctx *ctx, const rhg_molecule *mol, const int32_
 int rhg_ewald_fourier_move(rhg_ctx *ctx, int32_t n_new, const int32_t *new_types, const double *new_xyz,
                            int32_t old_mol, double out[2]);
 
+/* ---------------------------------------------------------- surface area */
+/* SurfaceAreaMove / CheckSurfaceAreaOverlap (src/mc_moves.c:16434-16520,
+ * src/framework.c:11649-11720): `samples` points per framework atom on the
+ * sphere of radius factor x sigma(probe, atom) (factor 2^(1/6) for RASPA2's
+ * default "Minimum", 1 for "Sigma"), a point counting when it lies in no
+ * blocked pocket and within that distance of no other atom.  area = sum over
+ * atoms of 4 pi d^2 x the counted fraction [A^2]; counts (nullable) the
+ * accessible points per atom.  Points come from the counter stream of
+ * (seed, atom x samples + k). */
+int rhg_surface_area(rhg_ctx *ctx, int32_t probe_type, double distance_factor, int64_t samples, uint64_t seed,
+                     double *area, uint64_t *counts);
+
 /* ----------------------------------------------------------------- grids */
 /* MakeGrid (src/grids.c:431-583) for one probe type over a unit-cell box.
  * fp64 = 0 stores RASPA2's FP32 node layout [i][j][k][8]; fp64 = 1 keeps the

@@ -109,6 +109,7 @@ def _declare(L):
     f("orc_inter_vdw_at", _D, _P, _I, _P, _I, _D)
     f("orc_inter_charge_at", _D, _P, _I, _P, _I, _D)
     f("orc_fourier_move", None, _P, _I, _P, _P, _I, _P)
+    f("orc_surface_area", _D, _P, _I, _D, _I64, _U64, _P)
     f("orc_use_coulomb_grid", _I, _P, _P, _P, _P)
     f("orc_coulomb_grid_node", None, _P, _P, _I, _I, _I, _P)
     f("orc_blocked_pocket", _I, _P, _P)
@@ -305,6 +306,11 @@ class System:
         lib().orc_fourier_move(self.h, len(ty), _ptr(ty) if len(ty) else None, _ptr(xy) if len(ty) else None,
                                old_mol, _ptr(out))
         return out
+
+    def surface_area(self, t, samples, seed=1, factor=2.0 ** (1.0 / 6.0)):
+        counts = np.zeros(len(self._keep[1]), dtype=np.uint64)
+        area = lib().orc_surface_area(self.h, t, factor, samples, seed, _ptr(counts))
+        return area, counts
 
     def use_grid(self, t, grid, geom, unit_box):
         """Energies of type t at a position come from the grid (None

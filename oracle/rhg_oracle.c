@@ -918,6 +918,52 @@ void orc_fourier_adsorbate(const orc_system *s, int n_sites, const int *types,
   orc_fourier_move(s, n_sites, types, xyz, -1, out);
 }
 
+/* SurfaceAreaMove / CheckSurfaceAreaOverlap (src/mc_moves.c:16434-16520,
+ * src/framework.c:11649-11720), with the points of the counter stream of
+ * (seed, atom x samples + k) and RandomNumberOnUnitSphere's rejection
+ * (src/utils.c:688-708). */
+double orc_surface_area(const orc_system *s, int probe_type, double factor, int64_t samples, uint64_t seed,
+                        uint64_t *counts) {
+  double area = 0.0;
+  const double sp = s->t_sig[probe_type];
+  for (int i = 0; i < s->n; i++) {
+    double di = factor * (0.5 * (sp + s->sig[i]));
+    double total = 0.0, counted = 0.0;
+    uint64_t cnt = 0;
+    for (int64_t k = 0; k < samples; k++) {
+      total += 1.0;
+      uint64_t key = orc_stream_key(seed, (uint64_t)i * (uint64_t)samples + (uint64_t)k);
+      uint64_t c = 0;
+      double ran1, ran2, ransq;
+      do {
+        ran1 = 2.0 * orc_stream_uniform(key, c++) - 1.0;
+        ran2 = 2.0 * orc_stream_uniform(key, c++) - 1.0;
+        ransq = SQR(ran1) + SQR(ran2);
+      } while (ransq >= 1.0);
+      double ranh = 2.0 * sqrt(1.0 - ransq);
+      double v[3] = {ran1 * ranh, ran2 * ranh, 1.0 - 2.0 * ransq};
+      double p[3] = {s->x[i] + v[0] * di, s->y[i] + v[1] * di, s->z[i] + v[2] * di};
+      if (orc_blocked_pocket(s, p)) continue;
+      int overlap = 0;
+      for (int j = 0; j < s->n && !overlap; j++) {
+        if (j == i) continue;
+        double dj = factor * (0.5 * (sp + s->sig[j]));
+        double dr[3] = {p[0] - s->x[j], p[1] - s->y[j], p[2] - s->z[j]};
+        orc_min_image(s->box, s->inv, s->bc, dr);
+        double rr = SQR(dr[0]) + SQR(dr[1]) + SQR(dr[2]);
+        if (rr < SQR(dj)) overlap = 1;
+      }
+      if (!overlap) {
+        counted += 1.0;
+        cnt++;
+      }
+    }
+    if (counts) counts[i] = cnt;
+    area += (counted / total) * 4.0 * M_PI * SQR(di);
+  }
+  return area;
+}
+
 /* LB-mixed LJ parameters of the pair (typeA, typeB) as orc_add_pseudo_atom
  * forms them for framework atoms: eps (internal units), sigma, shift */
 static void pair_lj(const orc_system *s, int ta, int tb, double out[3]);

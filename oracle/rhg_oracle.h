@@ -225,6 +225,10 @@ double orc_inter_charge_at(const orc_system *s, int type, const double pos[3], i
  * excluded(new-old), sum_k kf(|S_ads'|^2-|S_ads|^2) */
 void orc_fourier_move(const orc_system *s, int n_new, const int *new_types, const double *new_xyz,
                       int old_mol, double out[5]);
+/* SurfaceAreaMove (src/mc_moves.c:16434-16520): area [A^2] and accessible
+ * points per atom (nullable) with counter-stream points */
+double orc_surface_area(const orc_system *s, int probe_type, double factor, int64_t samples, uint64_t seed,
+                        uint64_t *counts);
 /* InterpolateCoulombGrid (src/grids.c:1200-1295) */
 double orc_interpolate_coulomb_grid(const float *grid, const double geom[12], int bc, const double ub[9],
                                     const double uinv[9], const double pos[3], double charge);

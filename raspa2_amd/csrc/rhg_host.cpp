@@ -1930,6 +1930,55 @@ int rhg_ewald_fourier_move(rhg_ctx *c, int32_t n_new, const int32_t *new_types, 
   return RHG_OK;
 }
 
+// ------------------------------------------------------------ surface area
+int rhg_surface_area(rhg_ctx *c, int32_t probe_type, double distance_factor, int64_t samples, uint64_t seed,
+                     double *area, uint64_t *counts) {
+  if (!c || !area || samples < 1 || !(distance_factor > 0.0)) return RHG_ERR_INVALID;
+  if (int rc = check_type(c, probe_type)) return rc;
+  *area = 0.0;
+  if (c->n == 0) return RHG_OK;
+  if ((double)c->n * (double)samples > 9.0e18) return fail(c, RHG_ERR_INVALID, "too many sample points");
+  hipSetDevice(c->device);
+  const PseudoAtom &t = c->types[probe_type];
+  // equilibrium distance per atom: factor x the LB-mixed sigma of the pair
+  // (PotentialParms[typeA][typeB][1], src/mc_moves.c:16458)
+  std::vector<double4> h((size_t)c->n);
+  std::vector<double> d((size_t)c->n);
+  for (int i = 0; i < c->n; ++i) {
+    d[i] = distance_factor * (0.5 * (t.sigma + c->sig[i]));
+    h[i] = make_double4(c->x[i], c->y[i], c->z[i], d[i]);
+  }
+  double4 *da = (double4 *)work(c, "sa_atoms", (size_t)c->n * sizeof(double4));
+  unsigned long long *dc = (unsigned long long *)work(c, "sa_counts", (size_t)c->n * sizeof(unsigned long long));
+  if (!da || !dc) return fail(c, RHG_ERR_OOM, "surface area buffers");
+  upload(c, da, h.data(), h.size() * sizeof(double4));
+  fill((double *)dc, c->n, 0.0, c->stream); // all-zero bits
+  SurfaceArgs a;
+  memset(&a, 0, sizeof(a));
+  a.cell = c->cell;
+  a.pockets = c->pockets;
+  a.atoms = da;
+  a.n = c->n;
+  a.samples = samples;
+  a.seed = seed;
+  {
+    Timed tm(c, "surface_area");
+    launch_surface_area(a, dc, c->stream);
+  }
+  HIPC(c, hipGetLastError());
+  std::vector<unsigned long long> hc((size_t)c->n);
+  HIPC(c, hipMemcpyAsync(hc.data(), dc, hc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  // sum_i (counted/total) 4 pi d_i^2 in atom order (src/mc_moves.c:16478-16480)
+  double sum = 0.0;
+  for (int i = 0; i < c->n; ++i) {
+    sum += ((double)hc[i] / (double)samples) * 4.0 * M_PI * (d[i] * d[i]);
+    if (counts) counts[i] = hc[i];
+  }
+  *area = sum;
+  return RHG_OK;
+}
+
 // ------------------------------------------------------------------ grids
 namespace {
 

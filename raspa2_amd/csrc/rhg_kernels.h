@@ -198,6 +198,22 @@ struct PocketArgs {
   double ubox[9], uinv[9];
 };
 
+// ------------------------------------------------------- surface area
+// SurfaceAreaMove / CheckSurfaceAreaOverlap (src/mc_moves.c:16434-16520,
+// src/framework.c:11649-11720): per framework atom i, points on the sphere
+// of radius d[i] = factor * sigma(probe, i) around it; a point counts when it
+// is in no blocked pocket and within d[j] of no other atom j.
+struct SurfaceArgs {
+  Cell cell;
+  PocketArgs pockets;
+  const double4 *atoms; // x, y, z, d per atom
+  int n;
+  int64_t samples;      // points per atom
+  uint64_t seed;
+};
+// counts[i] += accessible points of atom i (counts zeroed by the caller)
+void launch_surface_area(const SurfaceArgs &a, unsigned long long *counts, hipStream_t st);
+
 struct WidomArgs {
   PocketArgs pockets;
   MolDev mol;

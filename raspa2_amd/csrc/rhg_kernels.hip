@@ -2474,6 +2474,62 @@ void launch_interp(const InterpArgs &a, const double *xyz, int64_t n, double *ou
 }
 
 // ====================================================================
+// Geometric surface area (src/mc_moves.c:16434-16520): one lane per
+// sampled point, the point on its atom's sphere from the counter stream of
+// (seed, point) with RandomNumberOnUnitSphere's rejection (src/utils.c:
+// 688-708); the overlap test of CheckSurfaceAreaOverlap (src/framework.c:
+// 11649-11690) in FP64 with the reference's minimum image, the atoms
+// wave-uniform, a wave leaving the loop once all its points overlap.
+// ====================================================================
+template <int BCK>
+__global__ __launch_bounds__(256) void k_surface_area(const SurfaceArgs a,
+                                                      unsigned long long *__restrict__ counts) {
+#pragma clang fp contract(off)
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t total = (int64_t)a.n * a.samples;
+  if ((int64_t)blockIdx.x * blockDim.x >= total) return;
+  const bool live = t < total;
+  const int64_t tt = live ? t : 0;
+  const int i = (int)(tt / a.samples);
+  const uint64_t key = stream_key(a.seed, (uint64_t)tt);
+  uint64_t c = 0;
+  double r1, r2, rsq;
+  int guard = 0;
+  do {
+    r1 = 2.0 * stream_uniform(key, c++) - 1.0;
+    r2 = 2.0 * stream_uniform(key, c++) - 1.0;
+    rsq = r1 * r1 + r2 * r2;
+  } while (rsq >= 1.0 && ++guard < 4096);
+  const double rh = 2.0 * sqrt(1.0 - rsq);
+  const double vx = r1 * rh, vy = r2 * rh, vz = 1.0 - 2.0 * rsq;
+  const double4 ai = a.atoms[i];
+  const double px = ai.x + vx * ai.w, py = ai.y + vy * ai.w, pz = ai.z + vz * ai.w;
+  bool hit = !live || blocked_pocket(a.pockets, px, py, pz);
+  for (int j = 0; j < a.n; ++j) {
+    if (__all(hit)) break;
+    if (hit || j == i) continue;
+    const double4 b = a.atoms[j];
+    double dx = px - b.x, dy = py - b.y, dz = pz - b.z;
+    mi_exact<BCK>(a.cell, dx, dy, dz);
+    const double rr = dx * dx + dy * dy + dz * dz;
+    if (rr < b.w * b.w) hit = true;
+  }
+  if (live && !hit) atomicAdd(counts + i, 1ULL);
+}
+
+void launch_surface_area(const SurfaceArgs &a, unsigned long long *counts, hipStream_t st) {
+  const int64_t total = (int64_t)a.n * a.samples;
+  const unsigned nb = nblk(total, 256);
+  if (!nb) return;
+  if (a.cell.bc == 1)
+    hipLaunchKernelGGL(k_surface_area<1>, dim3(nb), dim3(256), 0, st, a, counts);
+  else if (a.cell.upper)
+    hipLaunchKernelGGL(k_surface_area<2>, dim3(nb), dim3(256), 0, st, a, counts);
+  else
+    hipLaunchKernelGGL(k_surface_area<3>, dim3(nb), dim3(256), 0, st, a, counts);
+}
+
+// ====================================================================
 // plain streaming copy: calibrates hardware byte counters against a
 // kernel whose traffic is known (bytes read = bytes written = size)
 // ====================================================================

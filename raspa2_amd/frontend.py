@@ -376,3 +376,36 @@ def get_helium_void_fraction(structure, unit_cells=(1, 1, 1), cycles=2000, input
     est = widom(structure, "helium", unit_cells, cycles, forcefield, raspa_dir=raspa_dir,
                 insertions=insertions, seed=seed, precision=R.PREC_FP64, device=device)
     return est["w"]
+
+
+AVOGADRO = 6.02214076e23
+
+
+def get_geometric_surface_area(structure, unit_cells=(1, 1, 1), cycles=500, input_file_type="cif",
+                               units="m^2/g", forcefield="CrystalGenerator", raspa_dir=None, seed=1,
+                               samples=None, device=0):
+    """Geometric surface area with the wrapper's settings (python/raspa2.py:
+    297-342): N2's first bead as the probe, SurfaceAreaProbeDistance Sigma
+    (the contact distance is the mixed sigma).  A cycle of an empty system is
+    20 moves of 5 points per atom, so cycles -> 100 x cycles points per atom
+    unless `samples` is given.  units: "A^2", "m^2/g" or "m^2/cm^3"."""
+    if input_file_type != "cif":
+        raise NotImplementedError("only CIF structures")
+    share = _share(raspa_dir)
+    ff = ForceField.load(os.path.join(share, "forcefield", forcefield))
+    fw = framework_from_cif(structure, ff, unit_cells)
+    n2 = read_molecule(os.path.join(share, "molecules", "TraPPE", "N2.def"), ff)
+    ctx = fw.context(cutoff=12.8, temperature=298.0, device=device, shifted=ff.shifted)
+    t = ctx.add_pseudo_atom(*n2.sites[n2.start])
+    area, _ = ctx.SurfaceArea(t, samples if samples is not None else 100 * cycles, seed=seed,
+                              distance_factor=1.0)
+    info = ctx.info()
+    ctx.close()
+    if units == "A^2":
+        return area
+    if units == "m^2/g":
+        return area * 1.0e-20 * AVOGADRO / float(np.sum(fw.mass))
+    if units == "m^2/cm^3":
+        return area / info["volume"] * 1.0e4
+    raise ValueError(f"units {units!r}")
+

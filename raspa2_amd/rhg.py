@@ -127,6 +127,7 @@ def _load():
     f("rhg_inter_energy_at_positions", _P, _I64, _P, _I32, _I32, _P, _P)
     f("rhg_ewald_fourier_move", _P, _I32, _P, _P, _I32, _P)
     f("rhg_widom_retrace_trials", _P, _P, _P, _I64, _P, _P, _P, _P)
+    f("rhg_surface_area", _P, _I32, _D, _I64, _U64, _P, _P)
     f("rhg_use_coulomb_grid", _P, _P)
     f("rhg_profile_enable", _P, _I32)
     f("rhg_profile_get", _P, ctypes.c_char_p, ctypes.POINTER(_I64), ctypes.POINTER(_D))
@@ -498,6 +499,15 @@ class Context:
         self._check(lib.rhg_widom_retrace_trials(self.h, ctypes.byref(mol), _ptr(ex), n, _ptr(old_xyz),
                                                  _ptr(fb), _ptr(cord), _ptr(rec)), "retrace")
         return rec
+
+    def SurfaceArea(self, probe_type, samples, seed=1, distance_factor=2.0 ** (1.0 / 6.0)):
+        """(area [A^2], accessible points per atom) of SurfaceAreaMove with
+        `samples` points per framework atom."""
+        area = _D()
+        counts = np.zeros(self.n_atoms, dtype=np.uint64)
+        self._check(lib.rhg_surface_area(self.h, probe_type, distance_factor, samples, seed,
+                                         ctypes.byref(area), _ptr(counts)), "surface_area")
+        return area.value, counts
 
     def CalculateEwaldFourierMove(self, new_types=(), new_xyz=None, old_mol=-1):
         """(host, adsorbate) Fourier changes of inserting a molecule and/or

@@ -6,6 +6,7 @@ tests/golden/irmof1.json), and the wrapper call of python/raspa2.py:345."""
 import json
 import os
 
+import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
@@ -37,3 +38,36 @@ def test_wrapper_call(R):
     from raspa2_amd import frontend as F
     w = F.get_helium_void_fraction(_cif(), cycles=2000, forcefield="CrystalGenerator", raspa_dir=GOLD)
     assert 0.7 < w < 0.9
+
+
+def test_surface_area_device_equals_oracle(W, O, R):
+    """SurfaceAreaMove on the device: the accessible points of every atom
+    equal the oracle's on the same counter-stream points (IRMOF-1 with the
+    N2 probe atom, RASPA2's default distance 2^(1/6) sigma and 'Sigma')."""
+    from raspa2_amd import frontend as F
+    share = os.path.join(GOLD, "share", "raspa")
+    ff = F.ForceField.load(os.path.join(share, "forcefield", "GenericMOFs"))
+    fw = F.framework_from_cif(_cif(), ff)
+    n2 = F.read_molecule(os.path.join(share, "molecules", "TraPPE", "N2.def"), ff)
+    ctx = fw.context()
+    box, bc, x, y, z, q, eps, sig, mass = fw.oracle_args()
+    s = O.System(box, bc, x, y, z, q, eps, sig, mass=mass)
+    tg = ctx.add_pseudo_atom(*n2.sites[0])
+    to = s.add_pseudo_atom(*n2.sites[0])
+    for factor in (2.0 ** (1.0 / 6.0), 1.0):
+        ag, cg = ctx.SurfaceArea(tg, 300, seed=7, distance_factor=factor)
+        ao, co = s.surface_area(to, 300, seed=7, factor=factor)
+        assert np.array_equal(cg, co)
+        assert abs(ag - ao) <= 1e-12 * ao
+        assert 0 < cg.sum() < 300 * fw.n
+
+
+def test_surface_area_wrapper_call(R):
+    """get_geometric_surface_area with the wrapper's settings: IRMOF-1 in
+    m^2/g (the wrapper's default units) in the range reported for it."""
+    from raspa2_amd import frontend as F
+    sa = F.get_geometric_surface_area(_cif(), cycles=500, raspa_dir=GOLD)
+    assert 3000.0 < sa < 4200.0, sa
+    a2 = F.get_geometric_surface_area(_cif(), cycles=500, units="A^2", raspa_dir=GOLD)
+    v = F.get_geometric_surface_area(_cif(), cycles=500, units="m^2/cm^3", raspa_dir=GOLD)
+    assert abs(v - a2 / 25.832 ** 3 * 1e4) <= 1e-9 * v

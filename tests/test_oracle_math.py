@@ -444,3 +444,30 @@ def test_tabular_grid_branches(O, W):
     s.use_grid(t_grid, None, None, None)
     s.use_coulomb_grid(None, None, None)
     assert [s.vdw_at(t_grid, p) for p in pts] == direct
+
+
+def test_surface_area_geometry(O):
+    """SurfaceAreaMove's estimate: an isolated atom exposes its whole sphere
+    (4 pi d^2 exactly); a second atom at distance D hides the cap whose
+    points lie within its own radius, leaving the fraction (1 + c0)/2 with
+    c0 = (d^2 + D^2 - d_B^2) / (2 d D)."""
+    box, inv, props = O.cell_make(60.0, 60.0, 60.0)
+    s = O.System(box, O.BC_RECTANGULAR, [10.0], [10.0], [10.0], [0.0], [50.0], [3.0])
+    t = s.add_pseudo_atom(36.0, 3.31)
+    f = 2.0 ** (1.0 / 6.0)
+    d = f * 0.5 * (3.31 + 3.0)
+    area, counts = s.surface_area(t, 1000, seed=3)
+    assert counts[0] == 1000 and area == 4.0 * math.pi * d * d
+    D = 3.2
+    s2 = O.System(box, O.BC_RECTANGULAR, [10.0, 10.0 + D], [10.0, 10.0], [10.0, 10.0], [0.0, 0.0],
+                  [50.0, 40.0], [3.0, 2.6])
+    t2 = s2.add_pseudo_atom(36.0, 3.31)
+    n = 200000
+    area2, c2 = s2.surface_area(t2, n, seed=5)
+    dA, dB = d, f * 0.5 * (3.31 + 2.6)
+    for mine, other, got in ((dA, dB, c2[0]), (dB, dA, c2[1])):
+        c0 = (mine * mine + D * D - other * other) / (2.0 * mine * D)
+        p = (1.0 + c0) / 2.0
+        assert abs(got / n - p) < 5.0 * math.sqrt(p * (1.0 - p) / n)
+    expect = (c2[0] / n) * 4 * math.pi * dA * dA + (c2[1] / n) * 4 * math.pi * dB * dB
+    assert abs(area2 - expect) <= 1e-12 * expect
