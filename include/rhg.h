@@ -288,6 +288,75 @@ int rhg_ewald_fourier_move(rhg_ctx *ctx, int32_t n_new, const int32_t *new_types
 int rhg_surface_area(rhg_ctx *ctx, int32_t probe_type, double distance_factor, int64_t samples, uint64_t seed,
                      double *area, uint64_t *counts);
 
+/* ------------------------------------------------------- flexible chains */
+/* A flexible molecule made of one flexible group (the alkane family of
+ * molecules/TraPPE: ReadComponentDefinition, src/molecule.c:1430-3030).
+ * Bonded parameters are in the units of the molecule file; the library
+ * converts them as the reference does on reading (force constants divided
+ * by ENERGY_TO_KELVIN, angles to radians).  Intra-molecular LJ pairs use
+ * the pseudo-atoms' mixed parameters; pairs that close a bend-bend quartet
+ * get intra14_vdw_scaling (src/molecule.c:2997-3025). */
+#define RHG_CHAIN_MAX_ATOMS 64
+#define RHG_BOND_FIXED 0     /* FIXED_BOND: p0 = length [A] */
+#define RHG_BOND_HARMONIC 1  /* HARMONIC_BOND: 0.5 p0 (r - p1)^2, p0 [K/A^2], p1 [A] */
+#define RHG_BEND_HARMONIC 0  /* HARMONIC_BEND: 0.5 p0 (theta - p1)^2, p0 [K/rad^2], p1 [deg] */
+#define RHG_TORSION_TRAPPE 0 /* TRAPPE_DIHEDRAL: p0 + p1(1+cos) + p2(1-cos2) + p3(1+cos3) [K] */
+typedef struct {
+  int32_t n_atoms;              /* 2..RHG_CHAIN_MAX_ATOMS */
+  int32_t start;                /* StartingBead */
+  const int32_t *type;          /* pseudo-atom of each atom (uncharged) */
+  int32_t n_bonds;
+  const int32_t *bonds;         /* n_bonds x 3: atom a, atom b, RHG_BOND_* */
+  const double *bond_par;       /* n_bonds x 4 */
+  int32_t n_bends;
+  const int32_t *bends;         /* n_bends x 4: a, b (apex), c, RHG_BEND_* */
+  const double *bend_par;       /* n_bends x 4 */
+  int32_t n_torsions;
+  const int32_t *torsions;      /* n_torsions x 5: a, b, c, d, RHG_TORSION_* */
+  const double *torsion_par;    /* n_torsions x 4 */
+  int32_t n_intra_vdw;
+  const int32_t *intra_vdw;     /* n_intra_vdw x 2 atom pairs */
+  double intra14_vdw_scaling;   /* Intra14VDWScalingValue (1.0) */
+  int32_t k_fb;                 /* NumberOfTrialPositionsForTheFirstBeadWidom (1..64) */
+  int32_t k_trial;              /* NumberOfTrialPositionsWidom (1..64, <= k_torsion) */
+  int32_t k_torsion;            /* NumberOfTrialPositionsTorsion (100; 1..1024) */
+  int32_t moves_per_open_bead;  /* NumberOfTrialMovesPerOpenBead (150) */
+  const double *max_bond_change;   /* per atom MaximumCBMCChangeBondLength [A] (NULL: 0.3) */
+  const double *max_bend_change;   /* per atom MaximumCBMCChangeBendAngle [rad] (NULL: 0.3) */
+  const double *max_cone_rotation; /* per atom MaximumCBMCRotationOnCone [rad] (NULL: 0.3) */
+} rhg_chain;
+
+/* per-insertion chain record layout (n x RHG_CREC_N doubles) */
+#define RHG_CREC_W 0        /* RosenbluthNew; 0 when rejected or blocked */
+#define RHG_CREC_DU 1       /* DeltaU of WidomAdsorbateMove */
+#define RHG_CREC_UHOSTVDW 2 /* UHostVDWNew */
+#define RHG_CREC_UINTRA 3   /* UIntraVDWNew */
+#define RHG_CREC_UBOND 4    /* UBondNew */
+#define RHG_CREC_UBEND 5    /* UBendNew */
+#define RHG_CREC_UTORSION 6 /* UTorsionNew */
+#define RHG_CREC_OVERLAP 7  /* 1: rejected (OVERLAP) */
+#define RHG_CREC_STAGE 8    /* 0 fb all overlap, 1 fb W<min, 2 step all overlap, 3 step W<min, 4 accepted, 5 blocked */
+#define RHG_CREC_STEP 9     /* growth step (1-based) of stages 2 and 3 */
+#define RHG_CREC_W1 10      /* first-bead Rosenbluth weight */
+#define RHG_CREC_N 11
+
+/* WidomAdsorbateMove of a flexible chain (src/mc_moves.c:7844-7946;
+ * GrowMolecule src/cbmc.c:3056-3185 with HandleFirstBead :623-795, Rosen
+ * :2762-2849, GenerateTrialOrientationsSimpleSphere :1480-1578,
+ * GenerateTrialOrientationsMCScheme :1580-2482, ComputeExternalEnergies
+ * :1154-1478) for insertions [first, first+n) of a job of n_total, as
+ * rhg_widom_run does for rigid molecules: LJ biasing, framework VDW (grids
+ * in use are read), infinite dilution.  Every insertion grows its internal
+ * Metropolis walk from the 120 degree cone with 4 x moves_per_open_bead
+ * moves per branch (the reference's start without a stored configuration,
+ * src/cbmc.c:1662-1666, 1797-1800).  Draws come from the insertion's
+ * counter stream, one sub-stream per step and purpose.  Blocks are ADDED
+ * to *blocks; rec (nullable) receives n x RHG_CREC_N doubles, xyz
+ * (nullable) the grown molecules, n x n_atoms x 3 (zero when rejected). */
+int rhg_widom_chain_run(rhg_ctx *ctx, const rhg_chain *chain, uint64_t seed, int64_t n_total, int64_t first,
+                        int64_t n, int32_t steps_per_cycle, rhg_widom_blocks *blocks, double *rec,
+                        double *xyz);
+
 /* ----------------------------------------------------------------- grids */
 /* MakeGrid (src/grids.c:431-583) for one probe type over a unit-cell box.
  * fp64 = 0 stores RASPA2's FP32 node layout [i][j][k][8]; fp64 = 1 keeps the

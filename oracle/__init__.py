@@ -58,6 +58,29 @@ class Trials(ctypes.Structure):
                 ("u_sel_fb", _D), ("u_sel_or", _D)]
 
 
+CHAIN_MAX = 64
+CHAIN_MAX_INTRA = 2016
+CREC = dict(W=0, DU=1, UHOSTVDW=2, UINTRA=3, UBOND=4, UBEND=5, UTORSION=6, OVERLAP=7, STAGE=8,
+            STEP=9, W1=10)
+CREC_N = 11
+BOND_FIXED = 0
+BOND_HARMONIC = 1
+BEND_HARMONIC = 0
+TORSION_TRAPPE = 0
+
+
+class Chain(ctypes.Structure):
+    _fields_ = [("n_atoms", _I), ("start", _I), ("type", _I * CHAIN_MAX),
+                ("n_bonds", _I), ("bond", (_I * 3) * CHAIN_MAX), ("bond_par", (_D * 4) * CHAIN_MAX),
+                ("n_bends", _I), ("bend", (_I * 4) * (2 * CHAIN_MAX)),
+                ("bend_par", (_D * 4) * (2 * CHAIN_MAX)),
+                ("n_torsions", _I), ("torsion", (_I * 5) * (4 * CHAIN_MAX)),
+                ("torsion_par", (_D * 4) * (4 * CHAIN_MAX)),
+                ("n_intra", _I), ("intra", (_I * 2) * CHAIN_MAX_INTRA), ("intra14_scaling", _D),
+                ("k_fb", _I), ("k_trial", _I), ("k_torsion", _I), ("moves", _I),
+                ("max_bond", _D * CHAIN_MAX), ("max_bend", _D * CHAIN_MAX), ("max_cone", _D * CHAIN_MAX)]
+
+
 def _declare(L):
     def f(name, res, *args):
         fn = getattr(L, name)
@@ -113,6 +136,13 @@ def _declare(L):
     f("orc_use_coulomb_grid", _I, _P, _P, _P, _P)
     f("orc_coulomb_grid_node", None, _P, _P, _I, _I, _I, _P)
     f("orc_blocked_pocket", _I, _P, _P)
+    f("orc_chain_plan", _I, _P, _P, _P)
+    f("orc_chain_run_stream", None, _P, _P, _U64, _I64, _I64, _P, _P)
+    f("orc_chain_run_mt", None, _P, _P, _U64, _I64, _P, _P)
+    f("orc_chain_substream", _U64, _U64, _U64)
+    f("orc_chain_bond_energy", _D, _P, _I, _P, _D)
+    f("orc_chain_bend_energy", _D, _P, _I, _P, _P, _P, _P)
+    f("orc_chain_torsion_energy", _D, _P, _I, _P, _P, _P, _P, _P)
 
 
 def _ptr(a):
@@ -198,7 +228,9 @@ class System:
         t = lib().orc_add_pseudo_atom(self.h, eps_k, sigma, charge)
         if not hasattr(self, "_tq"):
             self._tq = {}
+            self._pseudo = {}
         self._tq[t] = float(charge)
+        self._pseudo[t] = (float(eps_k), float(sigma))
         return t
 
     def t_q(self, t):
@@ -458,3 +490,91 @@ def interpolate_coulomb_grid(grid, geom, bc, unit_box, unit_inv, pos, charge):
     geom, ub, ui, p = _f64(geom), _f64(unit_box), _f64(unit_inv), _f64(pos)
     return lib().orc_interpolate_coulomb_grid(_ptr(g), _ptr(geom), bc, _ptr(ub), _ptr(ui), _ptr(p),
                                               float(charge))
+
+
+# ------------------------------------------------------------ flexible chains
+def chain(types, bonds=(), bends=(), torsions=(), intra=(), start=0, k_fb=10, k_trial=10,
+          k_torsion=100, moves=150, intra14_scaling=1.0, max_bond=0.3, max_bend=0.3, max_cone=0.3):
+    """A flexible molecule of one group.  bonds: (a, b, kind, p0, p1...),
+    bends: (a, b, c, kind, p0, p1), torsions: (a, b, c, d, kind, p0..p3),
+    intra: (a, b) pairs; parameters in molecule-file units (K, A, degrees)."""
+    c = Chain()
+    c.n_atoms = len(types)
+    for i, t in enumerate(types):
+        c.type[i] = int(t)
+    c.start = start
+    c.n_bonds = len(bonds)
+    for j, b in enumerate(bonds):
+        c.bond[j][0], c.bond[j][1], c.bond[j][2] = int(b[0]), int(b[1]), int(b[2])
+        for q, v in enumerate(b[3:]):
+            c.bond_par[j][q] = float(v)
+    c.n_bends = len(bends)
+    for j, b in enumerate(bends):
+        for q in range(4):
+            c.bend[j][q] = int(b[q])
+        for q, v in enumerate(b[4:]):
+            c.bend_par[j][q] = float(v)
+    c.n_torsions = len(torsions)
+    for j, t in enumerate(torsions):
+        for q in range(5):
+            c.torsion[j][q] = int(t[q])
+        for q, v in enumerate(t[5:]):
+            c.torsion_par[j][q] = float(v)
+    c.n_intra = len(intra)
+    for j, p in enumerate(intra):
+        c.intra[j][0], c.intra[j][1] = int(p[0]), int(p[1])
+    c.intra14_scaling = intra14_scaling
+    c.k_fb, c.k_trial, c.k_torsion, c.moves = k_fb, k_trial, k_torsion, moves
+    for i in range(len(types)):
+        c.max_bond[i] = max_bond
+        c.max_bend[i] = max_bend
+        c.max_cone[i] = max_cone
+    return c
+
+
+def chain_plan(c):
+    info = np.zeros((CHAIN_MAX, 8), dtype=np.int32)
+    place = np.zeros((CHAIN_MAX, CHAIN_MAX), dtype=np.int32)
+    ns = lib().orc_chain_plan(ctypes.byref(c), _ptr(info), _ptr(place))
+    if ns < 0:
+        return None
+    return [dict(cur=int(info[i, 0]), prev=int(info[i, 1]), n_prev=int(info[i, 2]),
+                 place=[int(v) for v in place[i, :info[i, 3]]], n_bonds=int(info[i, 4]),
+                 n_bends=int(info[i, 5]), n_torsions=int(info[i, 6]), n_intra=int(info[i, 7]))
+            for i in range(ns)]
+
+
+def chain_run_stream(system, c, seed, first, n, positions=False):
+    rec = np.zeros((n, CREC_N))
+    xyz = np.zeros((n, c.n_atoms, 3)) if positions else None
+    lib().orc_chain_run_stream(system.h, ctypes.byref(c), seed, first, n, _ptr(rec),
+                               _ptr(xyz) if positions else None)
+    return (rec, xyz) if positions else rec
+
+
+def chain_run_mt(system, c, seed, n, positions=False):
+    rec = np.zeros((n, CREC_N))
+    xyz = np.zeros((n, c.n_atoms, 3)) if positions else None
+    lib().orc_chain_run_mt(system.h, ctypes.byref(c), seed, n, _ptr(rec), _ptr(xyz) if positions else None)
+    return (rec, xyz) if positions else rec
+
+
+def chain_substream(key, tag):
+    return lib().orc_chain_substream(key, tag)
+
+
+def chain_bond_energy(system, kind, par, r):
+    p = _f64(list(par) + [0.0] * (4 - len(par)))
+    return lib().orc_chain_bond_energy(system.h, kind, _ptr(p), r)
+
+
+def chain_bend_energy(system, par, a, b, c_):
+    p = _f64(list(par) + [0.0] * (4 - len(par)))
+    aa, bb, cc = _f64(a), _f64(b), _f64(c_)
+    return lib().orc_chain_bend_energy(system.h, 0, _ptr(p), _ptr(aa), _ptr(bb), _ptr(cc))
+
+
+def chain_torsion_energy(system, par, a, b, c_, d):
+    p = _f64(par)
+    aa, bb, cc, dd = _f64(a), _f64(b), _f64(c_), _f64(d)
+    return lib().orc_chain_torsion_energy(system.h, 0, _ptr(p), _ptr(aa), _ptr(bb), _ptr(cc), _ptr(dd))

@@ -1934,3 +1934,861 @@ double orc_interpolate_direct(const orc_system *s, int type, const double g[12],
   }
   return grid_tricubic(X, r);
 }
+
+/* ===================================================================== */
+/* Flexible chains: CBMC growth with the internal MC scheme               */
+/* (src/cbmc.c:623-795, 1154-2482, 2582-2849, 3056-3185,                  */
+/*  src/internal_energy.c:48-490, 1168-1340, 2177-2430,                   */
+/*  src/utils.c:640-990, src/mc_moves.c:7844-7946)                        */
+/* ===================================================================== */
+#define CH_FIXED_BOND 0
+#define CH_HARMONIC_BOND 1
+#define CH_MAX_TORSION_TRIALS 1024
+#define DEG2RAD_ (M_PI / 180.0) /* src/constants.h:108 */
+#define SIGN_(a, b) ((b) >= 0.0 ? fabs(a) : -fabs(a)) /* src/utils.h:65 */
+
+typedef struct {
+  int cur, prev, n_prev;
+  int n_place, place[ORC_CHAIN_MAX];
+  int n_branch, branch[ORC_CHAIN_MAX];
+  int n_bonds, bonds[ORC_CHAIN_MAX];
+  int n_bends, bends[2 * ORC_CHAIN_MAX];
+  int n_tors, tors[4 * ORC_CHAIN_MAX];
+  int n_intra, intra[ORC_CHAIN_MAX_INTRA];
+} chain_step;
+
+/* SetConnectivityMatrix, SetGrowingStatus and Interactions of a molecule
+ * made of one flexible group (src/cbmc.c:2582-2760, 901-1152); the first
+ * bead (StartingBead) is placed.  The growth path is fixed (the first
+ * possible bead, src/cbmc.c:2651), so the steps are the same for every
+ * insertion. */
+static int chain_steps(const orc_chain *c, chain_step *steps) {
+  const int n = c->n_atoms;
+  static char conn[ORC_CHAIN_MAX][ORC_CHAIN_MAX], todo[ORC_CHAIN_MAX][ORC_CHAIN_MAX];
+  memset(conn, 0, sizeof(conn));
+  for (int j = 0; j < c->n_bonds; j++) {
+    conn[c->bond[j][0]][c->bond[j][1]] = 1;
+    conn[c->bond[j][1]][c->bond[j][0]] = 1;
+  }
+  memcpy(todo, conn, sizeof(conn));
+  int placed[ORC_CHAIN_MAX], n_placed = 1;
+  placed[0] = c->start;
+  for (int j = 0; j < n; j++) todo[c->start][j] = 0;
+  int ns = 0;
+  while (n_placed != n) {
+    chain_step *st = &steps[ns];
+    memset(st, 0, sizeof(*st));
+    /* SetGrowingStatus: the first (placed bead, unplaced neighbour) pair */
+    int beadn = -1, curb = -1;
+    for (int i = 0; i < n_placed && beadn < 0; i++) {
+      int k = placed[i];
+      for (int j = 0; j < n; j++)
+        if (todo[j][k]) {
+          beadn = j;
+          curb = k;
+          break;
+        }
+    }
+    if (beadn < 0) return -1; /* not connected */
+    st->cur = curb;
+    st->prev = -1;
+    int attached = 0;
+    for (int i = 0; i < n; i++)
+      if (conn[i][curb]) {
+        if (todo[i][curb])
+          attached = 1;
+        else {
+          st->n_prev++;
+          st->prev = i;
+        }
+      }
+    /* the flexible group adds every unplaced atom bonded to the current
+     * bead, each as a branch of its own */
+    if (attached)
+      for (int j = 0; j < n; j++)
+        if (todo[j][curb]) {
+          st->place[st->n_place++] = j;
+          st->branch[st->n_branch++] = j;
+        }
+    if (st->n_prev == 0) {
+      st->n_place = 1;
+      st->place[0] = beadn;
+    }
+    if (st->n_prev > 1) return -1;
+    /* Interactions */
+    char done[ORC_CHAIN_MAX] = {0}, tbp[ORC_CHAIN_MAX] = {0};
+    for (int i = 0; i < n_placed; i++) done[placed[i]] = 1;
+    for (int i = 0; i < st->n_place; i++) {
+      tbp[st->place[i]] = 1;
+      done[st->place[i]] = 1;
+    }
+    for (int i = 0; i < st->n_place; i++) {
+      int A = st->place[i] < curb ? st->place[i] : curb, B = st->place[i] < curb ? curb : st->place[i];
+      for (int j = 0; j < c->n_bonds; j++)
+        if ((A == c->bond[j][0] && B == c->bond[j][1]) || (A == c->bond[j][1] && B == c->bond[j][0]))
+          st->bonds[st->n_bonds++] = j;
+    }
+    for (int j = 0; j < c->n_bends; j++) {
+      const int *b = c->bend[j];
+      if (done[b[0]] && done[b[1]] && done[b[2]] && (tbp[b[0]] || tbp[b[1]] || tbp[b[2]]))
+        st->bends[st->n_bends++] = j;
+    }
+    for (int j = 0; j < c->n_torsions; j++) {
+      const int *t = c->torsion[j];
+      if (done[t[0]] && done[t[1]] && done[t[2]] && done[t[3]] && (tbp[t[0]] || tbp[t[1]] || tbp[t[2]] || tbp[t[3]]))
+        st->tors[st->n_tors++] = j;
+    }
+    for (int j = 0; j < c->n_intra; j++) {
+      const int *p = c->intra[j];
+      if (done[p[0]] && done[p[1]] && (tbp[p[0]] || tbp[p[1]])) st->intra[st->n_intra++] = j;
+    }
+    if (st->n_prev == 0 && st->n_bonds == 0) return -1; /* nothing to grow the bead along */
+    for (int i = 0; i < st->n_place; i++) {
+      int j = st->place[i];
+      placed[n_placed++] = j;
+      for (int ip = 0; ip < n; ip++) todo[j][ip] = 0;
+    }
+    ns++;
+  }
+  return ns;
+}
+
+int orc_chain_plan(const orc_chain *c, int *info, int *place) {
+  chain_step *steps = (chain_step *)malloc(sizeof(chain_step) * ORC_CHAIN_MAX);
+  int ns = chain_steps(c, steps);
+  for (int i = 0; i < ns; i++) {
+    const chain_step *st = &steps[i];
+    int *o = info + 8 * i;
+    o[0] = st->cur;
+    o[1] = st->prev;
+    o[2] = st->n_prev;
+    o[3] = st->n_place;
+    o[4] = st->n_bonds;
+    o[5] = st->n_bends;
+    o[6] = st->n_tors;
+    o[7] = st->n_intra;
+    for (int j = 0; j < st->n_place; j++) place[ORC_CHAIN_MAX * i + j] = st->place[j];
+  }
+  free(steps);
+  return ns;
+}
+
+/* parameters as the reference holds them after reading the molecule file
+ * (src/molecule.c:1638-1644, 1790-1796, 2173-2184): force constants divided
+ * by ENERGY_TO_KELVIN, angles in radians; LJ of the intra pairs as
+ * PotentialValue mixes them, and the 1-4 scaling (src/molecule.c:2997-3025) */
+typedef struct {
+  double bond[ORC_CHAIN_MAX][4];
+  double bend[2 * ORC_CHAIN_MAX][4];
+  double tors[4 * ORC_CHAIN_MAX][4];
+  double scal[ORC_CHAIN_MAX_INTRA];
+  double lj[ORC_CHAIN_MAX_INTRA][3];
+} chain_par;
+
+static void chain_params(const orc_system *s, const orc_chain *c, chain_par *p) {
+  for (int j = 0; j < c->n_bonds; j++) {
+    for (int q = 0; q < 4; q++) p->bond[j][q] = c->bond_par[j][q];
+    if (c->bond[j][2] == CH_HARMONIC_BOND) p->bond[j][0] /= s->u.E2K;
+  }
+  for (int j = 0; j < c->n_bends; j++) {
+    for (int q = 0; q < 4; q++) p->bend[j][q] = c->bend_par[j][q];
+    p->bend[j][0] /= s->u.E2K;
+    p->bend[j][1] *= DEG2RAD_;
+  }
+  for (int j = 0; j < c->n_torsions; j++)
+    for (int q = 0; q < 4; q++) p->tors[j][q] = c->torsion_par[j][q] / s->u.E2K;
+  for (int i = 0; i < c->n_intra; i++) {
+    int A = c->intra[i][0], B = c->intra[i][1];
+    p->scal[i] = 1.0;
+    for (int j = 0; j < c->n_bends; j++) {
+      int A1 = c->bend[j][0], B1 = c->bend[j][1], C1 = c->bend[j][2];
+      for (int k = 0; k < c->n_bends; k++) {
+        int A2 = c->bend[k][0], B2 = c->bend[k][1], C2 = c->bend[k][2];
+        if ((B1 == A2) && (C1 == B2) && (((A1 == A) && (C2 == B)) || ((A1 == B) && (C2 == A)))) p->scal[i] = c->intra14_scaling;
+        if ((B1 == C2) && (C1 == B2) && (((A1 == A) && (A2 == B)) || ((A1 == B) && (A2 == A)))) p->scal[i] = c->intra14_scaling;
+        if ((A1 == B2) && (B1 == A2) && (((C1 == A) && (C2 == B)) || ((C1 == B) && (C2 == A)))) p->scal[i] = c->intra14_scaling;
+        if ((A1 == B2) && (B1 == C2) && (((C1 == A) && (A2 == B)) || ((C1 == B) && (A2 == A)))) p->scal[i] = c->intra14_scaling;
+      }
+    }
+    pair_lj(s, c->type[A], c->type[B], p->lj[i]);
+  }
+}
+
+/* CalculateBondEnergy (src/internal_energy.c:372-490), HARMONIC_BOND and FIXED_BOND */
+static double ch_bond_u(int kind, const double *par, const double a[3], const double b[3]) {
+  double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+  double rr = SQR(dx) + SQR(dy) + SQR(dz);
+  double r = sqrt(rr);
+  if (kind == CH_HARMONIC_BOND) return 0.5 * par[0] * SQR(r - par[1]);
+  return 0.0;
+}
+
+/* CalculateBendEnergy (src/internal_energy.c:1168-1340), HARMONIC_BEND */
+static double ch_bend_u(const double *par, const double a[3], const double b[3], const double c[3]) {
+  double Rab[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+  double rab = sqrt(SQR(Rab[0]) + SQR(Rab[1]) + SQR(Rab[2]));
+  Rab[0] /= rab;
+  Rab[1] /= rab;
+  Rab[2] /= rab;
+  double Rbc[3] = {c[0] - b[0], c[1] - b[1], c[2] - b[2]};
+  double rbc = sqrt(SQR(Rbc[0]) + SQR(Rbc[1]) + SQR(Rbc[2]));
+  Rbc[0] /= rbc;
+  Rbc[1] /= rbc;
+  Rbc[2] /= rbc;
+  double CosTheta = (Rab[0] * Rbc[0] + Rab[1] * Rbc[1] + Rab[2] * Rbc[2]);
+  const double lo = (-1.0 > CosTheta) ? -1.0 : CosTheta; /* MIN2(1.0,MAX2(-1.0,CosTheta)), src/utils.h:63-64 */
+  CosTheta = (1.0 <= lo) ? 1.0 : lo;
+  double Theta = acos(CosTheta);
+  return 0.5 * par[0] * SQR(Theta - par[1]);
+}
+
+/* CalculateTorsionEnergy (src/internal_energy.c:2177-2330), TRAPPE_DIHEDRAL */
+static double ch_torsion_u(const double *par, const double a[3], const double b[3], const double c[3],
+                           const double d[3]) {
+  double Dab[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+  double Dbc[3] = {c[0] - b[0], c[1] - b[1], c[2] - b[2]};
+  double rbc = sqrt(SQR(Dbc[0]) + SQR(Dbc[1]) + SQR(Dbc[2]));
+  Dbc[0] /= rbc;
+  Dbc[1] /= rbc;
+  Dbc[2] /= rbc;
+  double Dcd[3] = {d[0] - c[0], d[1] - c[1], d[2] - c[2]};
+  double dot_ab = Dab[0] * Dbc[0] + Dab[1] * Dbc[1] + Dab[2] * Dbc[2];
+  double dot_cd = Dcd[0] * Dbc[0] + Dcd[1] * Dbc[1] + Dcd[2] * Dbc[2];
+  double dr[3] = {Dab[0] - dot_ab * Dbc[0], Dab[1] - dot_ab * Dbc[1], Dab[2] - dot_ab * Dbc[2]};
+  double r = sqrt(SQR(dr[0]) + SQR(dr[1]) + SQR(dr[2]));
+  dr[0] /= r;
+  dr[1] /= r;
+  dr[2] /= r;
+  double ds[3] = {Dcd[0] - dot_cd * Dbc[0], Dcd[1] - dot_cd * Dbc[1], Dcd[2] - dot_cd * Dbc[2]};
+  double sv = sqrt(SQR(ds[0]) + SQR(ds[1]) + SQR(ds[2]));
+  ds[0] /= sv;
+  ds[1] /= sv;
+  ds[2] /= sv;
+  double CosPhi = dr[0] * ds[0] + dr[1] * ds[1] + dr[2] * ds[2];
+  double m = fabs(CosPhi) <= 1.0 ? fabs(CosPhi) : 1.0;
+  CosPhi = SIGN_(m, CosPhi);
+  return par[0] + (1.0 + CosPhi) * (par[1] + par[3] - 2.0 * (CosPhi - 1.0) * (par[2] - 2.0 * par[3] * CosPhi));
+}
+
+double orc_chain_bond_energy(const orc_system *s, int kind, const double par[4], double r) {
+  double p[4] = {par[0], par[1], par[2], par[3]};
+  if (kind == CH_HARMONIC_BOND) p[0] /= s->u.E2K;
+  double a[3] = {0.0, 0.0, 0.0}, b[3] = {r, 0.0, 0.0};
+  return ch_bond_u(kind, p, a, b);
+}
+
+double orc_chain_bend_energy(const orc_system *s, int kind, const double par[4], const double a[3],
+                             const double b[3], const double c[3]) {
+  (void)kind;
+  double p[4] = {par[0] / s->u.E2K, par[1] * DEG2RAD_, par[2], par[3]};
+  return ch_bend_u(p, a, b, c);
+}
+
+double orc_chain_torsion_energy(const orc_system *s, int kind, const double par[4], const double a[3],
+                                const double b[3], const double c[3], const double d[3]) {
+  (void)kind;
+  double p[4] = {par[0] / s->u.E2K, par[1] / s->u.E2K, par[2] / s->u.E2K, par[3] / s->u.E2K};
+  return ch_torsion_u(p, a, b, c, d);
+}
+
+/* draws: one MT19937-64 stream in the reference's order, or the counter
+ * stream with one sub-stream per purpose */
+typedef struct {
+  orc_mt64 *g;   /* non-null: MT mode */
+  uint64_t key;  /* insertion key */
+  uint64_t sub;  /* current sub-stream key */
+  uint64_t ctr;
+} cdraw_t;
+
+uint64_t orc_chain_substream(uint64_t insertion_key, uint64_t tag) {
+  return mix64(insertion_key ^ mix64(tag + 0x2545F4914F6CDD1DULL));
+}
+static void cd_sub(cdraw_t *d, uint64_t tag) {
+  if (d->g) return;
+  d->sub = orc_chain_substream(d->key, tag);
+  d->ctr = 0;
+}
+static double cd(cdraw_t *d) {
+  return d->g ? orc_mt64_uniform(d->g) : orc_stream_uniform(d->sub, d->ctr++);
+}
+static uint64_t ch_tag(int step, int j) { return ((uint64_t)step << 32) | (uint64_t)j; }
+
+/* RandomGaussianNumber (src/utils.c:640-651) */
+static double ch_gauss(cdraw_t *d) {
+  double ran1, ran2, r2;
+  do {
+    ran1 = 2.0 * cd(d) - 1.0;
+    ran2 = 2.0 * cd(d) - 1.0;
+    r2 = SQR(ran1) + SQR(ran2);
+  } while ((r2 > 1.0) || (r2 == 0.0));
+  return ran2 * sqrt(-2.0 * log(r2) / r2);
+}
+
+/* RandomNumberOnUnitSphere (src/utils.c:688-708) */
+static void ch_sphere(cdraw_t *d, double v[3]) {
+  double ran1, ran2, ransq;
+  do {
+    ran1 = 2.0 * cd(d) - 1.0;
+    ran2 = 2.0 * cd(d) - 1.0;
+    ransq = SQR(ran1) + SQR(ran2);
+  } while (ransq >= 1.0);
+  double ranh = 2.0 * sqrt(1.0 - ransq);
+  v[0] = ran1 * ranh;
+  v[1] = ran2 * ranh;
+  v[2] = 1.0 - 2.0 * ransq;
+}
+
+/* RandomNumberOnCone (src/utils.c:709-735) */
+static void ch_cone(cdraw_t *d, const double vin[3], double theta, double v[3]) {
+  double cosin = cos(theta);
+  double ran1 = 1.0 / sqrt(SQR(vin[0]) + SQR(vin[1]) + SQR(vin[2]));
+  double xprev = vin[0] * ran1, yprev = vin[1] * ran1, zprev = vin[2] * ran1;
+  ch_sphere(d, v);
+  ran1 = v[0] * xprev + v[1] * yprev + v[2] * zprev;
+  v[0] = v[0] - ran1 * xprev;
+  v[1] = v[1] - ran1 * yprev;
+  v[2] = v[2] - ran1 * zprev;
+  ran1 = sqrt((1.0 - cosin * cosin) / (SQR(v[0]) + SQR(v[1]) + SQR(v[2])));
+  v[0] = v[0] * ran1 + cosin * xprev;
+  v[1] = v[1] * ran1 + cosin * yprev;
+  v[2] = v[2] * ran1 + cosin * zprev;
+  ran1 = 1.0 / sqrt(SQR(v[0]) + SQR(v[1]) + SQR(v[2]));
+  v[0] = v[0] * ran1;
+  v[1] = v[1] * ran1;
+  v[2] = v[2] * ran1;
+}
+
+/* Perpendicular (src/utils.c:789-832) */
+static void ch_perpendicular(const double a[3], const double b[3], double v[3]) {
+  double fx = fabs(b[1] * a[2] - a[1] * b[2]);
+  double fy = fabs(b[0] * a[2] - a[0] * b[2]);
+  double fz = fabs(b[1] * a[0] - a[1] * b[0]);
+  int choise = 1;
+  if (fy > fx) {
+    choise = 2;
+    fx = fy;
+  }
+  if (fz > fx) choise = 3;
+  if (choise == 1) {
+    v[0] = 1.0;
+    v[1] = (a[0] * b[2] - b[0] * a[2]) / (b[1] * a[2] - a[1] * b[2]);
+    v[2] = (b[0] * a[1] - a[0] * b[1]) / (b[1] * a[2] - a[1] * b[2]);
+  } else if (choise == 2) {
+    v[0] = (a[1] * b[2] - b[1] * a[2]) / (b[0] * a[2] - a[0] * b[2]);
+    v[1] = 1.0;
+    v[2] = (b[1] * a[0] - a[1] * b[0]) / (b[0] * a[2] - a[0] * b[2]);
+  } else {
+    v[0] = (b[2] * a[1] - a[2] * b[1]) / (b[1] * a[0] - a[1] * b[0]);
+    v[1] = (a[2] * b[0] - b[2] * a[0]) / (b[1] * a[0] - a[1] * b[0]);
+    v[2] = 1.0;
+  }
+  double f = 1.0 / sqrt(SQR(v[0]) + SQR(v[1]) + SQR(v[2]));
+  v[0] *= f;
+  v[1] *= f;
+  v[2] *= f;
+}
+
+/* RotationAroundXYZAxis (src/utils.c:945-975) */
+static void ch_rotate(const double v[3], double (*cord)[3], int n, double theta) {
+  double c = cos(theta);
+  double w = 1.0 - c;
+  double s = sqrt(1.0 - c * c);
+  if (theta < 0.0) s = -s;
+  double r[3][3];
+  r[0][0] = (v[0]) * (v[0]) * w + c;
+  r[0][1] = (v[0]) * (v[1]) * w + (v[2]) * s;
+  r[0][2] = (v[0]) * (v[2]) * w - (v[1]) * s;
+  r[1][0] = (v[0]) * (v[1]) * w - (v[2]) * s;
+  r[1][1] = (v[1]) * (v[1]) * w + c;
+  r[1][2] = (v[1]) * (v[2]) * w + (v[0]) * s;
+  r[2][0] = (v[0]) * (v[2]) * w + (v[1]) * s;
+  r[2][1] = (v[1]) * (v[2]) * w - (v[0]) * s;
+  r[2][2] = (v[2]) * (v[2]) * w + c;
+  for (int i = 0; i < n; i++) {
+    double x = cord[i][0] * r[0][0] + cord[i][1] * r[0][1] + cord[i][2] * r[0][2];
+    double y = cord[i][0] * r[1][0] + cord[i][1] * r[1][1] + cord[i][2] * r[1][2];
+    double z = cord[i][0] * r[2][0] + cord[i][1] * r[2][1] + cord[i][2] * r[2][2];
+    cord[i][0] = x;
+    cord[i][1] = y;
+    cord[i][2] = z;
+  }
+}
+
+/* CalculateAngle (src/internal_energy.c:306-329) */
+static double ch_angle(const double a[3], const double b[3], const double c[3]) {
+  double Rab[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+  double rab = 1.0 / sqrt(SQR(Rab[0]) + SQR(Rab[1]) + SQR(Rab[2]));
+  Rab[0] *= rab;
+  Rab[1] *= rab;
+  Rab[2] *= rab;
+  double Rbc[3] = {c[0] - b[0], c[1] - b[1], c[2] - b[2]};
+  double rbc = 1.0 / sqrt(SQR(Rbc[0]) + SQR(Rbc[1]) + SQR(Rbc[2]));
+  Rbc[0] *= rbc;
+  Rbc[1] *= rbc;
+  Rbc[2] *= rbc;
+  return acos(Rab[0] * Rbc[0] + Rab[1] * Rbc[1] + Rab[2] * Rbc[2]);
+}
+
+/* GenerateBondlength (src/internal_energy.c:48-70, 185-187) */
+static double ch_bond_length(cdraw_t *d, double beta, int kind, const double *par) {
+  if (kind != CH_HARMONIC_BOND) return par[0];
+  double ran1 = 1.0 / sqrt(beta * par[0]);
+  double ran2 = 1.0 / SQR(par[1] + 3.0 * ran1);
+  double bl;
+  do
+    bl = par[1] + ran1 * ch_gauss(d);
+  while ((cd(d) > (SQR(bl) * ran2)) || (bl <= 0.0));
+  return bl;
+}
+
+/* GetReferenceBondLength (src/cbmc.c:809-895) */
+static double ch_ref_bond_length(const orc_chain *c, int A, int B) {
+  for (int i = 0; i < c->n_bonds; i++)
+    if ((c->bond[i][0] == A && c->bond[i][1] == B) || (c->bond[i][1] == A && c->bond[i][0] == B))
+      return c->bond[i][2] == CH_HARMONIC_BOND ? c->bond_par[i][1] : c->bond_par[i][0];
+  return 1.5;
+}
+
+/* SelectTrialPosition (src/cbmc.c:563-595) over up to CH_MAX_TORSION_TRIALS */
+static int ch_select(const double *bf, const int *overlap, int n, double u) {
+  static double shifted[CH_MAX_TORSION_TRIALS];
+  double largest = -DBL_MAX, sum = 0.0;
+  for (int i = 0; i < n; i++)
+    if (!overlap[i])
+      if (bf[i] > largest) largest = bf[i];
+  for (int i = 0; i < n; i++) {
+    if (!overlap[i]) {
+      shifted[i] = exp(bf[i] - largest);
+      sum += shifted[i];
+    } else
+      shifted[i] = 0.0;
+  }
+  int selected = 0;
+  double cumw = shifted[0];
+  double ws = u * sum;
+  while (cumw < ws) cumw += shifted[++selected];
+  return selected;
+}
+
+typedef struct {
+  chain_step steps[ORC_CHAIN_MAX];
+  int n_steps;
+  chain_par par;
+  double tpos[64][ORC_CHAIN_MAX][3]; /* TrialPositions[iu][atom] */
+} chain_work;
+
+static double ch_sum_bonds(const orc_chain *c, const chain_work *w, const chain_step *st, int iu) {
+  double u = 0.0;
+  for (int j = 0; j < st->n_bonds; j++) {
+    int b = st->bonds[j];
+    u += ch_bond_u(c->bond[b][2], w->par.bond[b], w->tpos[iu][c->bond[b][0]], w->tpos[iu][c->bond[b][1]]);
+  }
+  return u;
+}
+static double ch_sum_bends(const orc_chain *c, const chain_work *w, const chain_step *st, int iu) {
+  double u = 0.0;
+  for (int j = 0; j < st->n_bends; j++) {
+    int b = st->bends[j];
+    u += ch_bend_u(w->par.bend[b], w->tpos[iu][c->bend[b][0]], w->tpos[iu][c->bend[b][1]], w->tpos[iu][c->bend[b][2]]);
+  }
+  return u;
+}
+static double ch_sum_torsions(const orc_chain *c, const chain_work *w, const chain_step *st, int iu) {
+  double u = 0.0;
+  for (int j = 0; j < st->n_tors; j++) {
+    int t = st->tors[j];
+    u += ch_torsion_u(w->par.tors[t], w->tpos[iu][c->torsion[t][0]], w->tpos[iu][c->torsion[t][1]],
+                      w->tpos[iu][c->torsion[t][2]], w->tpos[iu][c->torsion[t][3]]);
+  }
+  return u;
+}
+
+/* GenerateTrialOrientationsMCScheme(FALSE) without a stored configuration
+ * (src/cbmc.c:1580-2482): bond lengths drawn (and unused), branches placed on
+ * the 120 degree cone at their reference bond lengths, then the internal
+ * Metropolis walk over bond lengths, bend angles and cone rotations of the
+ * open beads on trial 0, then per trial the torsion selection among
+ * k_torsion rotations about the previous-current axis.  RosenbluthTorsion
+ * averages the first NumberOfTrialPositions of the torsion trials, as the
+ * reference's ComputeNormalizedRosenbluthWeight reads the global count
+ * (src/cbmc.c:542-555). */
+static void ch_mc_scheme(const orc_system *s, const orc_chain *c, chain_work *w, const chain_step *st, int step,
+                         cdraw_t *d, double *RT) {
+  const double beta = s->beta;
+  const int nb = st->n_bonds, nbe = st->n_bends, k = c->k_trial, kt = c->k_torsion;
+  double (*t0)[3] = w->tpos[0];
+  const int cur = st->cur, prev = st->prev;
+  cd_sub(d, ch_tag(step, 0));
+  for (int j = 0; j < nb; j++) (void)ch_bond_length(d, beta, c->bond[st->bonds[j]][2], w->par.bond[st->bonds[j]]);
+  double pv[3] = {t0[prev][0] - t0[cur][0], t0[prev][1] - t0[cur][1], t0[prev][2] - t0[cur][2]};
+  double normalize = sqrt(SQR(pv[0]) + SQR(pv[1]) + SQR(pv[2]));
+  pv[0] /= normalize;
+  pv[1] /= normalize;
+  pv[2] /= normalize;
+  for (int jj = 0; jj < st->n_branch; jj++) {
+    double vec[3];
+    ch_cone(d, pv, 120 * DEG2RAD_, vec);
+    double bl = ch_ref_bond_length(c, cur, st->branch[jj]);
+    int a = st->branch[jj];
+    t0[a][0] = t0[cur][0] + bl * vec[0];
+    t0[a][1] = t0[cur][1] + bl * vec[1];
+    t0[a][2] = t0[cur][2] + bl * vec[2];
+  }
+  double eobond[ORC_CHAIN_MAX], enbond[ORC_CHAIN_MAX], eobend[2 * ORC_CHAIN_MAX], enbend[2 * ORC_CHAIN_MAX];
+#define CH_BOND_E(j) ch_bond_u(c->bond[st->bonds[j]][2], w->par.bond[st->bonds[j]], t0[c->bond[st->bonds[j]][0]], t0[c->bond[st->bonds[j]][1]])
+#define CH_BEND_E(j) ch_bend_u(w->par.bend[st->bends[j]], t0[c->bend[st->bends[j]][0]], t0[c->bend[st->bends[j]][1]], t0[c->bend[st->bends[j]][2]])
+  for (int j = 0; j < nb; j++) eobond[j] = enbond[j] = CH_BOND_E(j);
+  for (int j = 0; j < nbe; j++) eobend[j] = enbend[j] = CH_BEND_E(j);
+  const int trialmove = 4 * c->moves * st->n_branch;
+  /* no chiral centres: swap two branches with probability 1/2 */
+  if ((nb >= 2) && (cd(d) < 0.5)) {
+    int A = st->branch[0], C = st->branch[1];
+    double cA[3], cC[3];
+    for (int q = 0; q < 3; q++) {
+      cA[q] = t0[A][q] - t0[cur][q];
+      cC[q] = t0[C][q] - t0[cur][q];
+    }
+    double tmp = sqrt(SQR(cA[0]) + SQR(cA[1]) + SQR(cA[2])) / sqrt(SQR(cC[0]) + SQR(cC[1]) + SQR(cC[2]));
+    for (int q = 0; q < 3; q++) t0[A][q] = t0[cur][q] + tmp * cC[q];
+    tmp = 1.0 / tmp;
+    for (int q = 0; q < 3; q++) t0[C][q] = t0[cur][q] + tmp * cA[q];
+    for (int j = 0; j < nb; j++) eobond[j] = enbond[j] = CH_BOND_E(j);
+    for (int j = 0; j < nbe; j++) eobend[j] = enbend[j] = CH_BEND_E(j);
+  }
+  for (int jjj = 0; jjj < trialmove; jjj++) {
+    int choise;
+    if (nbe == 1)
+      choise = cd(d) < 0.3 ? 1 : 2;
+    else if (jjj < 20)
+      choise = 3;
+    else {
+      double tmp = cd(d);
+      choise = tmp < 0.2 ? 1 : (tmp < 0.6 ? 2 : 3);
+    }
+    for (int j = 0; j < nb; j++) enbond[j] = eobond[j];
+    for (int j = 0; j < nbe; j++) enbend[j] = eobend[j];
+    double newbond = 0.0, newbend = 0.0;
+    if (choise == 1) {
+      if (nb > 0) {
+        int jj = (int)(cd(d) * (double)nb);
+        if (jj >= nb) jj = nb - 1; /* u = 1 (closed interval) would index past the list */
+        if (c->bond[st->bonds[jj]][2] != CH_FIXED_BOND) {
+          int jo = st->branch[jj];
+          double store[3] = {t0[jo][0], t0[jo][1], t0[jo][2]};
+          double vec[3] = {t0[jo][0] - t0[cur][0], t0[jo][1] - t0[cur][1], t0[jo][2] - t0[cur][2]};
+          double blo = sqrt(SQR(vec[0]) + SQR(vec[1]) + SQR(vec[2]));
+          double bln = blo + (2.0 * cd(d) - 1.0) * c->max_bond[jo];
+          for (int q = 0; q < 3; q++) t0[jo][q] = t0[cur][q] + (bln / blo) * vec[q];
+          enbond[jj] = CH_BOND_E(jj);
+          newbond += enbond[jj] - eobond[jj];
+          if (cd(d) < SQR(bln / blo) * exp(-beta * (newbond + 0.0 + 0.0 + 0.0)))
+            eobond[jj] = enbond[jj];
+          else
+            for (int q = 0; q < 3; q++) t0[jo][q] = store[q];
+        }
+      }
+    } else if (choise == 2) {
+      if (st->n_branch > 0) {
+        int jj = (int)(cd(d) * (double)st->n_branch);
+        if (jj >= st->n_branch) jj = st->n_branch - 1;
+        int jo = st->branch[jj];
+        double store[3] = {t0[jo][0], t0[jo][1], t0[jo][2]};
+        double cord[1][3] = {{t0[jo][0] - t0[cur][0], t0[jo][1] - t0[cur][1], t0[jo][2] - t0[cur][2]}};
+        double SinusOld = sin(ch_angle(t0[prev], t0[cur], t0[jo]));
+        double dr[3] = {t0[cur][0] - t0[jo][0], t0[cur][1] - t0[jo][1], t0[cur][2] - t0[jo][2]};
+        double perpen[3];
+        ch_perpendicular(dr, pv, perpen);
+        ch_rotate(perpen, cord, 1, (2.0 * cd(d) - 1.0) * c->max_bend[cur]);
+        for (int q = 0; q < 3; q++) t0[jo][q] = t0[cur][q] + cord[0][q];
+        double SinusNew = sin(ch_angle(t0[prev], t0[cur], t0[jo]));
+        for (int j = 0; j < nbe; j++) {
+          enbend[j] = CH_BEND_E(j);
+          newbend += enbend[j] - eobend[j];
+        }
+        if (cd(d) < ((SinusNew / SinusOld) * exp(-beta * (newbend + 0.0 + 0.0 + 0.0)))) {
+          for (int j = 0; j < nbe; j++) eobend[j] = enbend[j];
+        } else
+          for (int q = 0; q < 3; q++) t0[jo][q] = store[q];
+      }
+    } else {
+      if (st->n_branch > 0) {
+        int jj = (int)(cd(d) * (double)st->n_branch);
+        if (jj >= st->n_branch) jj = st->n_branch - 1;
+        int jo = st->branch[jj];
+        double store[3] = {t0[jo][0], t0[jo][1], t0[jo][2]};
+        double cord[1][3] = {{t0[jo][0] - t0[cur][0], t0[jo][1] - t0[cur][1], t0[jo][2] - t0[cur][2]}};
+        ch_rotate(pv, cord, 1, (2.0 * cd(d) - 1.0) * c->max_cone[cur]);
+        for (int q = 0; q < 3; q++) t0[jo][q] = t0[cur][q] + cord[0][q];
+        for (int j = 0; j < nbe; j++) {
+          enbend[j] = CH_BEND_E(j);
+          newbend += enbend[j] - eobend[j];
+        }
+        if (cd(d) < exp(-beta * (newbend + 0.0 + 0.0 + 0.0))) {
+          for (int j = 0; j < nbe; j++) eobend[j] = enbend[j];
+        } else
+          for (int q = 0; q < 3; q++) t0[jo][q] = store[q];
+      }
+    }
+  }
+#undef CH_BOND_E
+#undef CH_BEND_E
+  /* torsion biasing: rotations about prev_vec leave bonds and bends alone */
+  double store[ORC_CHAIN_MAX][3];
+  for (int jj = 0; jj < st->n_place; jj++)
+    for (int q = 0; q < 3; q++) store[jj][q] = t0[st->place[jj]][q];
+  static double bf[CH_MAX_TORSION_TRIALS], angle[CH_MAX_TORSION_TRIALS];
+  static int ov[CH_MAX_TORSION_TRIALS];
+  for (int iu = 0; iu < k; iu++) {
+    cd_sub(d, ch_tag(step, 2 + iu));
+    double cord[ORC_CHAIN_MAX][3];
+    for (int kk = 0; kk < kt; kk++) {
+      for (int jj = 0; jj < st->n_place; jj++) {
+        int a = st->place[jj];
+        for (int q = 0; q < 3; q++) {
+          w->tpos[iu][a][q] = store[jj][q];
+          cord[jj][q] = w->tpos[iu][a][q] - w->tpos[iu][cur][q];
+        }
+      }
+      angle[kk] = (2.0 * cd(d) - 1.0) * M_PI;
+      ch_rotate(pv, cord, st->n_place, angle[kk]);
+      for (int jj = 0; jj < st->n_place; jj++)
+        for (int q = 0; q < 3; q++) w->tpos[iu][st->place[jj]][q] = cord[jj][q] + w->tpos[iu][cur][q];
+      double newtrs = ch_sum_torsions(c, w, st, iu);
+      bf[kk] = -beta * newtrs;
+      ov[kk] = 0;
+    }
+    double sum = 0.0;
+    for (int i = 0; i < k; i++)
+      if (!ov[i]) sum += exp(bf[i]);
+    RT[iu] = sum / (double)k;
+    int sel = st->n_tors > 0 ? ch_select(bf, ov, kt, cd(d)) : (int)(cd(d) * (double)kt);
+    if (sel >= kt) sel = kt - 1;
+    for (int jj = 0; jj < st->n_place; jj++) {
+      int a = st->place[jj];
+      for (int q = 0; q < 3; q++) {
+        w->tpos[iu][a][q] = store[jj][q];
+        cord[jj][q] = w->tpos[iu][a][q] - w->tpos[iu][cur][q];
+      }
+    }
+    ch_rotate(pv, cord, st->n_place, angle[sel]);
+    for (int jj = 0; jj < st->n_place; jj++)
+      for (int q = 0; q < 3; q++) w->tpos[iu][st->place[jj]][q] = cord[jj][q] + w->tpos[iu][cur][q];
+  }
+}
+
+static void chain_insertion(const orc_system *s, const orc_chain *c, chain_work *w, cdraw_t *d,
+                            double rec[ORC_CREC_N], double *xyz) {
+  const double crit = ENERGY_OVERLAP_CRITERIA, beta = s->beta;
+  const int k_fb = c->k_fb, k = c->k_trial, n = c->n_atoms;
+  for (int i = 0; i < ORC_CREC_N; i++) rec[i] = 0.0;
+  /* HandleFirstBead (src/cbmc.c:623-795) */
+  double fbpos[64][3], bf[64], ev[64], ei[64], eh[64];
+  int ov[64];
+  int OVERLAP = 1;
+  const int type0 = c->type[c->start];
+  for (int i = 0; i < k_fb; i++) {
+    double sv[3];
+    sv[2] = cd(d);
+    sv[1] = cd(d);
+    sv[0] = cd(d);
+    fbpos[i][0] = s->box[AX] * sv[0] + s->box[BX] * sv[1] + s->box[CX] * sv[2];
+    fbpos[i][1] = s->box[AY] * sv[0] + s->box[BY] * sv[1] + s->box[CY] * sv[2];
+    fbpos[i][2] = s->box[AZ] * sv[0] + s->box[BZ] * sv[1] + s->box[CZ] * sv[2];
+    double Uvdw = orc_vdw_energy_at(s, type0, fbpos[i], 1.0);
+    if (Uvdw >= crit) {
+      ov[i] = 1;
+    } else {
+      ov[i] = 0;
+      OVERLAP = 0;
+      ev[i] = Uvdw;
+      bf[i] = -beta * (Uvdw + 0.0 + 0.0);
+    }
+  }
+  if (OVERLAP) {
+    rec[ORC_CREC_OVERLAP] = 1.0;
+    rec[ORC_CREC_STAGE] = 0.0;
+    return;
+  }
+  double W1 = 0.0;
+  for (int i = 0; i < k_fb; i++)
+    if (!ov[i]) W1 += exp(bf[i]);
+  int sel = ch_select(bf, ov, k_fb, cd(d));
+  W1 /= (double)k_fb;
+  rec[ORC_CREC_W1] = W1;
+  if (W1 < MINIMUM_ROSENBLUTH_FACTOR) {
+    rec[ORC_CREC_OVERLAP] = 1.0;
+    rec[ORC_CREC_STAGE] = 1.0;
+    return;
+  }
+  double W = W1, UHost = ev[sel], UBond = 0.0, UBend = 0.0, UTors = 0.0, UIntra = 0.0;
+  double pos[ORC_CHAIN_MAX][3];
+  char placed[ORC_CHAIN_MAX] = {0};
+  for (int q = 0; q < 3; q++) pos[c->start][q] = fbpos[sel][q];
+  placed[c->start] = 1;
+  /* Rosen (src/cbmc.c:2762-2849) */
+  for (int si = 0; si < w->n_steps; si++) {
+    const chain_step *st = &w->steps[si];
+    const int step = si + 1;
+    for (int iu = 0; iu < k; iu++)
+      for (int a = 0; a < n; a++)
+        if (placed[a])
+          for (int q = 0; q < 3; q++) w->tpos[iu][a][q] = pos[a][q];
+    double RT[64], UBondT[64], UBendT[64], UTorsT[64];
+    for (int iu = 0; iu < k; iu++) {
+      RT[iu] = 1.0;
+      UBondT[iu] = UBendT[iu] = UTorsT[iu] = 0.0;
+    }
+    if (st->n_prev == 0) {
+      /* GenerateTrialOrientationsSimpleSphere(FALSE), flexible group */
+      cd_sub(d, ch_tag(step, 0));
+      double bl[ORC_CHAIN_MAX];
+      for (int j = 0; j < st->n_bonds; j++) bl[j] = ch_bond_length(d, beta, c->bond[st->bonds[j]][2], w->par.bond[st->bonds[j]]);
+      const int a = st->branch[0];
+      for (int iu = 0; iu < k; iu++) {
+        cd_sub(d, ch_tag(step, 2 + iu));
+        double vec[3];
+        ch_sphere(d, vec);
+        for (int q = 0; q < 3; q++) w->tpos[iu][a][q] = w->tpos[0][st->cur][q] + vec[q] * bl[0];
+      }
+      for (int iu = 0; iu < k; iu++) {
+        int b = st->bonds[0];
+        UBondT[iu] = ch_bond_u(c->bond[b][2], w->par.bond[b], w->tpos[iu][c->bond[b][0]], w->tpos[iu][c->bond[b][1]]);
+      }
+    } else {
+      ch_mc_scheme(s, c, w, st, step, d, RT);
+      for (int iu = 0; iu < k; iu++) {
+        UBondT[iu] = ch_sum_bonds(c, w, st, iu);
+        UBendT[iu] = ch_sum_bends(c, w, st, iu);
+        UTorsT[iu] = ch_sum_torsions(c, w, st, iu);
+      }
+    }
+    /* ComputeExternalEnergies (src/cbmc.c:1154-1478), LJ biasing */
+    int any = 0;
+    for (int iu = 0; iu < k; iu++) {
+      int trial_overlap = 0;
+      double ehost = 0.0;
+      for (int j = 0; j < st->n_place; j++) {
+        int a = st->place[j];
+        double u = orc_vdw_energy_at(s, c->type[a], w->tpos[iu][a], 1.0);
+        if (u >= crit) {
+          trial_overlap = 1;
+          break;
+        }
+        ehost += u;
+      }
+      if (trial_overlap) {
+        bf[iu] = 0.0;
+        ov[iu] = 1;
+        continue;
+      }
+      double intra = 0.0;
+      for (int i = 0; i < st->n_intra; i++) {
+        int p = st->intra[i];
+        int A = c->intra[p][0], B = c->intra[p][1];
+        double rr = SQR(w->tpos[iu][A][0] - w->tpos[iu][B][0]) + SQR(w->tpos[iu][A][1] - w->tpos[iu][B][1]) +
+                    SQR(w->tpos[iu][A][2] - w->tpos[iu][B][2]);
+        if (rr < s->rc_vdw2) intra += w->par.scal[p] * lj_value(w->par.lj[p][0], w->par.lj[p][1], w->par.lj[p][2], rr);
+      }
+      eh[iu] = ehost;
+      ei[iu] = intra;
+      bf[iu] = -beta * (ehost + 0.0 + 0.0 + intra);
+      ov[iu] = 0;
+      any = 1;
+    }
+    if (!any) {
+      rec[ORC_CREC_OVERLAP] = 1.0;
+      rec[ORC_CREC_STAGE] = 2.0;
+      rec[ORC_CREC_STEP] = step;
+      return;
+    }
+    double weight = 0.0;
+    for (int iu = 0; iu < k; iu++)
+      if (!ov[iu]) weight += exp(bf[iu]);
+    weight /= (double)k;
+    cd_sub(d, ch_tag(step, 1));
+    int iwalk = ch_select(bf, ov, k, cd(d));
+    W *= weight * RT[iwalk];
+    if (W < MINIMUM_ROSENBLUTH_FACTOR) {
+      rec[ORC_CREC_OVERLAP] = 1.0;
+      rec[ORC_CREC_STAGE] = 3.0;
+      rec[ORC_CREC_STEP] = step;
+      return;
+    }
+    UBond += UBondT[iwalk];
+    UBend += UBendT[iwalk];
+    UTors += UTorsT[iwalk];
+    UIntra += ei[iwalk];
+    UHost += eh[iwalk];
+    for (int j = 0; j < st->n_place; j++) {
+      int a = st->place[j];
+      for (int q = 0; q < 3; q++) pos[a][q] = w->tpos[iwalk][a][q];
+      placed[a] = 1;
+    }
+  }
+  if (xyz)
+    for (int a = 0; a < n; a++)
+      for (int q = 0; q < 3; q++) xyz[3 * a + q] = pos[a][q];
+  /* the GrowMolecule corrections (src/cbmc.c:3140-3177) and the tail term
+   * (src/mc_moves.c:7888-7891) are exp(-beta * 0) = 1 here */
+  for (int a = 0; a < n; a++)
+    if (orc_blocked_pocket(s, pos[a])) {
+      rec[ORC_CREC_STAGE] = 5.0;
+      rec[ORC_CREC_UHOSTVDW] = UHost;
+      return;
+    }
+  rec[ORC_CREC_W] = W;
+  rec[ORC_CREC_DU] = UBond + UBend + UTors + UIntra + UHost; /* src/mc_moves.c:7912-7925 */
+  rec[ORC_CREC_UHOSTVDW] = UHost;
+  rec[ORC_CREC_UINTRA] = UIntra;
+  rec[ORC_CREC_UBOND] = UBond;
+  rec[ORC_CREC_UBEND] = UBend;
+  rec[ORC_CREC_UTORSION] = UTors;
+  rec[ORC_CREC_STAGE] = 4.0;
+}
+
+static chain_work *chain_prepare(const orc_system *s, const orc_chain *c) {
+  chain_work *w = (chain_work *)malloc(sizeof(chain_work));
+  w->n_steps = chain_steps(c, w->steps);
+  chain_params(s, c, &w->par);
+  return w;
+}
+
+void orc_chain_insertion_stream(const orc_system *s, const orc_chain *c, uint64_t seed, int64_t insertion,
+                                double rec[ORC_CREC_N], double *xyz) {
+  chain_work *w = chain_prepare(s, c);
+  cdraw_t d = {NULL, orc_stream_key(seed, (uint64_t)insertion), 0, 0};
+  d.sub = d.key;
+  chain_insertion(s, c, w, &d, rec, xyz);
+  free(w);
+}
+
+void orc_chain_insertion_mt(const orc_system *s, const orc_chain *c, orc_mt64 *g, double rec[ORC_CREC_N],
+                            double *xyz) {
+  chain_work *w = chain_prepare(s, c);
+  cdraw_t d = {g, 0, 0, 0};
+  chain_insertion(s, c, w, &d, rec, xyz);
+  free(w);
+}
+
+void orc_chain_run_stream(const orc_system *s, const orc_chain *c, uint64_t seed, int64_t first, int64_t n,
+                          double *rec, double *xyz) {
+  chain_work *w = chain_prepare(s, c);
+  for (int64_t i = 0; i < n; i++) {
+    cdraw_t d = {NULL, orc_stream_key(seed, (uint64_t)(first + i)), 0, 0};
+    d.sub = d.key;
+    chain_insertion(s, c, w, &d, rec + ORC_CREC_N * i, xyz ? xyz + (size_t)3 * c->n_atoms * i : NULL);
+  }
+  free(w);
+}
+
+void orc_chain_run_mt(const orc_system *s, const orc_chain *c, uint64_t seed, int64_t n, double *rec,
+                      double *xyz) {
+  chain_work *w = chain_prepare(s, c);
+  orc_mt64 g;
+  orc_mt64_seed(&g, seed);
+  for (int64_t i = 0; i < n; i++) {
+    cdraw_t d = {&g, 0, 0, 0};
+    chain_insertion(s, c, w, &d, rec + ORC_CREC_N * i, xyz ? xyz + (size_t)3 * c->n_atoms * i : NULL);
+  }
+  free(w);
+}

@@ -246,6 +246,85 @@ double orc_interpolate_direct(const orc_system *s, int type, const double geom[1
                               const double unit_box[9], const double unit_inv[9],
                               const double pos[3]);
 
+/* ------------------------------------------------------ flexible chains */
+/* A flexible molecule of one flexible group (ReadComponentDefinition,
+ * src/molecule.c:1430-3030): bonded terms with their parameters in the
+ * units of the molecule file (K, A, degrees), converted as the reference
+ * does on reading.  Kinds: bonds 0 FIXED_BOND (p0 = length), 1
+ * HARMONIC_BOND; bends 0 HARMONIC_BEND; torsions 0 TRAPPE_DIHEDRAL. */
+#define ORC_CHAIN_MAX 64
+#define ORC_CHAIN_MAX_INTRA 2016
+typedef struct {
+  int n_atoms, start;
+  int type[ORC_CHAIN_MAX];
+  int n_bonds, bond[ORC_CHAIN_MAX][3];          /* a, b, kind */
+  double bond_par[ORC_CHAIN_MAX][4];
+  int n_bends, bend[2 * ORC_CHAIN_MAX][4];      /* a, b, c, kind */
+  double bend_par[2 * ORC_CHAIN_MAX][4];
+  int n_torsions, torsion[4 * ORC_CHAIN_MAX][5]; /* a, b, c, d, kind */
+  double torsion_par[4 * ORC_CHAIN_MAX][4];
+  int n_intra, intra[ORC_CHAIN_MAX_INTRA][2];
+  double intra14_scaling;                       /* Intra14VDWScalingValue */
+  int k_fb, k_trial, k_torsion, moves;          /* ...FirstBeadWidom, ...Widom, ...Torsion, ...PerOpenBead */
+  double max_bond[ORC_CHAIN_MAX], max_bend[ORC_CHAIN_MAX], max_cone[ORC_CHAIN_MAX];
+} orc_chain;
+
+/* growth plan (SetConnectivityMatrix/SetGrowingStatus/Interactions,
+ * src/cbmc.c:2582-2760, 901-1152): per step i (n_atoms - 1 of them)
+ * info[8*i..] = CurrentBead, PreviousBead (-1), NumberOfPreviousBeads,
+ * NumberOfBeadsToBePlaced, NumberOfBonds, NumberOfBends, NumberOfTorsions,
+ * NumberOfIntraVDW; place[ORC_CHAIN_MAX*i..] the beads to be placed.
+ * Returns the number of steps, or -1 for a molecule outside the supported
+ * growth (a step with more than one previous bead). */
+int orc_chain_plan(const orc_chain *c, int *info, int *place);
+
+/* per-insertion chain record */
+#define ORC_CREC_W        0  /* RosenbluthNew (0 on overlap or blocked) */
+#define ORC_CREC_DU       1  /* DeltaU of WidomAdsorbateMove */
+#define ORC_CREC_UHOSTVDW 2
+#define ORC_CREC_UINTRA   3  /* UIntraVDWNew */
+#define ORC_CREC_UBOND    4
+#define ORC_CREC_UBEND    5
+#define ORC_CREC_UTORSION 6
+#define ORC_CREC_OVERLAP  7
+#define ORC_CREC_STAGE    8  /* 0 fb overlap, 1 fb W<min, 2 step overlap, 3 step W<min, 4 ok, 5 blocked */
+#define ORC_CREC_STEP     9  /* growth step of stages 2 and 3 (1-based) */
+#define ORC_CREC_W1      10  /* first-bead weight */
+#define ORC_CREC_N       11
+
+/* WidomAdsorbateMove of a flexible chain (src/mc_moves.c:7844-7946,
+ * GrowMolecule src/cbmc.c:3056-3185, Rosen :2762-2849, HandleFirstBead
+ * :623-795, GenerateTrialOrientationsSimpleSphere :1480-1578,
+ * GenerateTrialOrientationsMCScheme :1580-2482, ComputeExternalEnergies
+ * :1154-1478) with the draws of insertion `insertion` of the counter stream
+ * of `seed` (one sub-stream per purpose, see orc_chain_substream); xyz
+ * (nullable) receives the grown molecule, n_atoms x 3. */
+void orc_chain_insertion_stream(const orc_system *s, const orc_chain *c, uint64_t seed, int64_t insertion,
+                                double rec[ORC_CREC_N], double *xyz);
+/* insertions [first, first+n) of the counter stream: rec n x ORC_CREC_N,
+ * xyz (nullable) n x n_atoms x 3 */
+void orc_chain_run_stream(const orc_system *s, const orc_chain *c, uint64_t seed, int64_t first, int64_t n,
+                          double *rec, double *xyz);
+/* n insertions drawing from one MT19937-64 stream of `seed` in the
+ * reference's order */
+void orc_chain_run_mt(const orc_system *s, const orc_chain *c, uint64_t seed, int64_t n, double *rec,
+                      double *xyz);
+/* the same move drawing from one MT19937-64 stream in the reference's order */
+void orc_chain_insertion_mt(const orc_system *s, const orc_chain *c, orc_mt64 *g, double rec[ORC_CREC_N],
+                            double *xyz);
+/* key of sub-stream `tag` of an insertion: tag = (step << 32) | j with j = 0
+ * the internal MC scheme of the step, 1 its trial selection, 2 + t trial t */
+uint64_t orc_chain_substream(uint64_t insertion_key, uint64_t tag);
+/* bonded energies as the reference evaluates them [internal units]:
+ * CalculateBondEnergy (src/internal_energy.c:372-490) of length r,
+ * CalculateBendEnergy (:1168-1340) of atoms a-b-c, CalculateTorsionEnergy
+ * (:2177-2430) of atoms a-b-c-d */
+double orc_chain_bond_energy(const orc_system *s, int kind, const double par[4], double r);
+double orc_chain_bend_energy(const orc_system *s, int kind, const double par[4], const double a[3],
+                             const double b[3], const double c[3]);
+double orc_chain_torsion_energy(const orc_system *s, int kind, const double par[4], const double a[3],
+                                const double b[3], const double c[3], const double d[3]);
+
 #ifdef __cplusplus
 }
 #endif

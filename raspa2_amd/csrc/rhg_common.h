@@ -106,6 +106,13 @@ RHG_HD double stream_uniform(uint64_t key, uint64_t counter) {
   uint64_t x = mix64(key + (counter + 1ULL) * 0x9E3779B97F4A7C15ULL);
   return (double)(x >> 11) * kInvMax53;
 }
+// Sub-stream `tag` of an insertion's key (flexible-chain growth draws one
+// sub-stream per purpose so that parallel kernels know where their draws
+// start): tag = (step << 32) | j.
+RHG_HD uint64_t chain_substream(uint64_t key, uint64_t tag) {
+  return mix64(key ^ mix64(tag + 0x2545F4914F6CDD1DULL));
+}
+RHG_HD uint64_t chain_tag(int step, int j) { return ((uint64_t)step << 32) | (uint64_t)j; }
 
 // ------------------------------------------------------ portable sincos
 // sin and cos of x in [0, 2*pi] (the Shoemake angles 2*pi*u), written so

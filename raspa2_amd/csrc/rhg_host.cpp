@@ -1506,6 +1506,333 @@ int check_range(rhg_ctx *c, const rhg_molecule *mol, int64_t n_total, int64_t fi
   return RHG_OK;
 }
 
+// ---------------------------------------------------------- flexible chains
+// Growth plan of a molecule of one flexible group: SetConnectivityMatrix,
+// SetGrowingStatus and Interactions (src/cbmc.c:2582-2760, 901-1152) run
+// once on the host, since the path (always the first possible bead,
+// src/cbmc.c:2651) does not depend on positions.  Each step becomes a
+// ChainStep table with atom references resolved into local codes.
+int chain_plan(rhg_ctx *c, const rhg_chain *ch, std::vector<ChainStep> &steps) {
+  const int n = ch->n_atoms;
+  if (n < 2 || n > kChainMax || ch->start < 0 || ch->start >= n || !ch->type)
+    return fail(c, RHG_ERR_INVALID, "chain: n_atoms out of [2, 64] or bad starting bead");
+  if (ch->k_fb < 1 || ch->k_fb > kMaxTrials || ch->k_trial < 1 || ch->k_trial > kMaxTrials ||
+      ch->k_torsion < ch->k_trial || ch->k_torsion > kChainTorsionTrials || ch->moves_per_open_bead < 0)
+    return fail(c, RHG_ERR_INVALID,
+                "chain: k_fb and k_trial must be in [1, 64], k_torsion in [k_trial, 1024], moves >= 0");
+  for (int a = 0; a < n; ++a) {
+    if (ch->type[a] < 0 || ch->type[a] >= (int)c->types.size()) return fail(c, RHG_ERR_INVALID, "chain: unknown pseudo-atom");
+    if (considered_charged(c->types[ch->type[a]].q))
+      return fail(c, RHG_ERR_UNSUPPORTED, "chain: charged beads are not supported on the flexible-chain path");
+  }
+  auto bad_atom = [&](int a) { return a < 0 || a >= n; };
+  if (ch->n_bonds < 0 || ch->n_bends < 0 || ch->n_torsions < 0 || ch->n_intra_vdw < 0 ||
+      (ch->n_bonds && (!ch->bonds || !ch->bond_par)) || (ch->n_bends && (!ch->bends || !ch->bend_par)) ||
+      (ch->n_torsions && (!ch->torsions || !ch->torsion_par)) || (ch->n_intra_vdw && !ch->intra_vdw))
+    return fail(c, RHG_ERR_INVALID, "chain: missing bonded-term arrays");
+  for (int j = 0; j < ch->n_bonds; ++j) {
+    const int32_t *b = ch->bonds + 3 * j;
+    if (bad_atom(b[0]) || bad_atom(b[1]) || b[0] == b[1]) return fail(c, RHG_ERR_INVALID, "chain: bad bond");
+    if (b[2] != RHG_BOND_FIXED && b[2] != RHG_BOND_HARMONIC) return fail(c, RHG_ERR_UNSUPPORTED, "chain: bond type");
+    if (b[2] == RHG_BOND_HARMONIC && !(ch->bond_par[4 * j] > 0.0)) return fail(c, RHG_ERR_INVALID, "chain: bond constant");
+  }
+  for (int j = 0; j < ch->n_bends; ++j) {
+    const int32_t *b = ch->bends + 4 * j;
+    if (bad_atom(b[0]) || bad_atom(b[1]) || bad_atom(b[2])) return fail(c, RHG_ERR_INVALID, "chain: bad bend");
+    if (b[3] != RHG_BEND_HARMONIC) return fail(c, RHG_ERR_UNSUPPORTED, "chain: bend type");
+  }
+  for (int j = 0; j < ch->n_torsions; ++j) {
+    const int32_t *t = ch->torsions + 5 * j;
+    if (bad_atom(t[0]) || bad_atom(t[1]) || bad_atom(t[2]) || bad_atom(t[3])) return fail(c, RHG_ERR_INVALID, "chain: bad torsion");
+    if (t[4] != RHG_TORSION_TRAPPE) return fail(c, RHG_ERR_UNSUPPORTED, "chain: torsion type");
+  }
+  for (int j = 0; j < ch->n_intra_vdw; ++j)
+    if (bad_atom(ch->intra_vdw[2 * j]) || bad_atom(ch->intra_vdw[2 * j + 1])) return fail(c, RHG_ERR_INVALID, "chain: bad intra pair");
+  const double E2K = c->u.E2K;
+  // intra-pair scaling (src/molecule.c:2997-3025)
+  std::vector<double> scal((size_t)ch->n_intra_vdw, 1.0);
+  for (int i = 0; i < ch->n_intra_vdw; ++i) {
+    const int A = ch->intra_vdw[2 * i], B = ch->intra_vdw[2 * i + 1];
+    for (int j = 0; j < ch->n_bends; ++j) {
+      const int A1 = ch->bends[4 * j], B1 = ch->bends[4 * j + 1], C1 = ch->bends[4 * j + 2];
+      for (int k = 0; k < ch->n_bends; ++k) {
+        const int A2 = ch->bends[4 * k], B2 = ch->bends[4 * k + 1], C2 = ch->bends[4 * k + 2];
+        if ((B1 == A2) && (C1 == B2) && (((A1 == A) && (C2 == B)) || ((A1 == B) && (C2 == A)))) scal[i] = ch->intra14_vdw_scaling;
+        if ((B1 == C2) && (C1 == B2) && (((A1 == A) && (A2 == B)) || ((A1 == B) && (A2 == A)))) scal[i] = ch->intra14_vdw_scaling;
+        if ((A1 == B2) && (B1 == A2) && (((C1 == A) && (C2 == B)) || ((C1 == B) && (C2 == A)))) scal[i] = ch->intra14_vdw_scaling;
+        if ((A1 == B2) && (B1 == C2) && (((C1 == A) && (A2 == B)) || ((C1 == B) && (A2 == A)))) scal[i] = ch->intra14_vdw_scaling;
+      }
+    }
+  }
+  auto ref_len = [&](int A, int B) {
+    for (int i = 0; i < ch->n_bonds; ++i) {
+      const int32_t *b = ch->bonds + 3 * i;
+      if ((b[0] == A && b[1] == B) || (b[1] == A && b[0] == B))
+        return b[2] == RHG_BOND_HARMONIC ? ch->bond_par[4 * i + 1] : ch->bond_par[4 * i];
+    }
+    return 1.5;
+  };
+  auto per_atom = [&](const double *v, int a) { return v ? v[a] : 0.3; };
+  std::vector<std::vector<char>> conn(n, std::vector<char>(n, 0));
+  for (int j = 0; j < ch->n_bonds; ++j) {
+    conn[ch->bonds[3 * j]][ch->bonds[3 * j + 1]] = 1;
+    conn[ch->bonds[3 * j + 1]][ch->bonds[3 * j]] = 1;
+  }
+  auto todo = conn;
+  std::vector<int> placed{ch->start};
+  for (int j = 0; j < n; ++j) todo[ch->start][j] = 0;
+  steps.clear();
+  while ((int)placed.size() != n) {
+    int beadn = -1, cur = -1;
+    for (size_t i = 0; i < placed.size() && beadn < 0; ++i)
+      for (int j = 0; j < n; ++j)
+        if (todo[j][placed[i]]) {
+          beadn = j;
+          cur = placed[i];
+          break;
+        }
+    if (beadn < 0) return fail(c, RHG_ERR_INVALID, "chain: the bonds do not connect every atom");
+    ChainStep S;
+    memset(&S, 0, sizeof(S));
+    S.cur = cur;
+    S.prev = -1;
+    bool attached = false;
+    for (int i = 0; i < n; ++i)
+      if (conn[i][cur]) {
+        if (todo[i][cur])
+          attached = true;
+        else {
+          S.n_prev++;
+          S.prev = i;
+        }
+      }
+    std::vector<int> place;
+    if (attached)
+      for (int j = 0; j < n; ++j)
+        if (todo[j][cur]) place.push_back(j);
+    if (S.n_prev == 0) place.assign(1, beadn);
+    if (S.n_prev > 1) return fail(c, RHG_ERR_UNSUPPORTED, "chain: a growth step with more than one previous bead");
+    if ((int)place.size() > kChainPlace) return fail(c, RHG_ERR_UNSUPPORTED, "chain: more than 4 branches at one bead");
+    S.n_place = (int)place.size();
+    std::vector<char> done(n, 0), tbp(n, 0);
+    for (int a : placed) done[a] = 1;
+    for (int a : place) done[a] = tbp[a] = 1;
+    auto code = [&](int a) {
+      for (int jj = 0; jj < S.n_place; ++jj)
+        if (place[jj] == a) return jj;
+      if (a == cur) return kLocCur;
+      if (a == S.prev) return kLocPrev;
+      return kLocGlobal + a;
+    };
+    for (int jj = 0; jj < S.n_place; ++jj) {
+      S.place[jj] = place[jj];
+      S.type[jj] = ch->type[place[jj]];
+      S.ref_len[jj] = ref_len(cur, place[jj]);
+      S.max_bond[jj] = per_atom(ch->max_bond_change, place[jj]);
+    }
+    S.max_bend = per_atom(ch->max_bend_change, cur);
+    S.max_cone = per_atom(ch->max_cone_rotation, cur);
+    for (int jj = 0; jj < S.n_place; ++jj) {
+      const int A = std::min(place[jj], cur), B = std::max(place[jj], cur);
+      for (int j = 0; j < ch->n_bonds; ++j) {
+        const int32_t *b = ch->bonds + 3 * j;
+        if (!((A == b[0] && B == b[1]) || (A == b[1] && B == b[0]))) continue;
+        if (S.n_bonds == kChainBonds) return fail(c, RHG_ERR_UNSUPPORTED, "chain: too many bonds in one step");
+        const int q = S.n_bonds++;
+        S.bond_a[q] = code(b[0]);
+        S.bond_b[q] = code(b[1]);
+        S.bond_kind[q] = b[2];
+        S.bond_branch[q] = jj;
+        S.bond_p0[q] = b[2] == RHG_BOND_HARMONIC ? ch->bond_par[4 * j] / E2K : ch->bond_par[4 * j];
+        S.bond_p1[q] = ch->bond_par[4 * j + 1];
+      }
+    }
+    if (S.n_prev == 0 && S.n_bonds == 0) return fail(c, RHG_ERR_INVALID, "chain: no bond to grow a bead along");
+    for (int j = 0; j < ch->n_bends; ++j) {
+      const int32_t *b = ch->bends + 4 * j;
+      if (!(done[b[0]] && done[b[1]] && done[b[2]] && (tbp[b[0]] || tbp[b[1]] || tbp[b[2]]))) continue;
+      if (S.n_bends == kChainBends) return fail(c, RHG_ERR_UNSUPPORTED, "chain: too many bends in one step");
+      const int q = S.n_bends++;
+      S.bend_a[q] = code(b[0]);
+      S.bend_b[q] = code(b[1]);
+      S.bend_c[q] = code(b[2]);
+      S.bend_p0[q] = ch->bend_par[4 * j] / E2K;
+      S.bend_p1[q] = ch->bend_par[4 * j + 1] * (M_PI / 180.0);
+    }
+    for (int j = 0; j < ch->n_torsions; ++j) {
+      const int32_t *t = ch->torsions + 5 * j;
+      if (!(done[t[0]] && done[t[1]] && done[t[2]] && done[t[3]] && (tbp[t[0]] || tbp[t[1]] || tbp[t[2]] || tbp[t[3]])))
+        continue;
+      if (S.n_tors == kChainTors) return fail(c, RHG_ERR_UNSUPPORTED, "chain: too many torsions in one step");
+      const int q = S.n_tors++;
+      S.tors_a[q] = code(t[0]);
+      S.tors_b[q] = code(t[1]);
+      S.tors_c[q] = code(t[2]);
+      S.tors_d[q] = code(t[3]);
+      for (int r = 0; r < 4; ++r) S.tors_p[q][r] = ch->torsion_par[4 * j + r] / E2K;
+    }
+    for (int j = 0; j < ch->n_intra_vdw; ++j) {
+      const int A = ch->intra_vdw[2 * j], B = ch->intra_vdw[2 * j + 1];
+      if (!(done[A] && done[B] && (tbp[A] || tbp[B]))) continue;
+      if (S.n_intra == kChainIntra) return fail(c, RHG_ERR_UNSUPPORTED, "chain: too many intra pairs in one step");
+      const int q = S.n_intra++;
+      S.intra_a[q] = code(A);
+      S.intra_b[q] = code(B);
+      const PseudoAtom &ta = c->types[ch->type[A]], &tb = c->types[ch->type[B]];
+      const double e = sqrt((ta.eps_k * c->u.K2E) * (tb.eps_k * c->u.K2E)); // src/potentials.c:3164-3171
+      const double sg = 0.5 * (ta.sigma + tb.sigma);
+      S.intra_scal[q] = scal[j];
+      S.intra_c4[q] = 4.0 * e;
+      S.intra_s2[q] = sg * sg;
+      S.intra_shift[q] = c->shifted ? lj_ref(e, sg, 0.0, c->rc2_vdw) : 0.0;
+    }
+    for (int a : place) {
+      placed.push_back(a);
+      for (int ip = 0; ip < n; ++ip) todo[a][ip] = 0;
+    }
+    steps.push_back(S);
+  }
+  return RHG_OK;
+}
+
+int chain_buffers(rhg_ctx *c, const rhg_chain *ch, int64_t cap, ChainWork &w) {
+  memset(&w, 0, sizeof(w));
+  w.cap = cap;
+  const size_t d8 = sizeof(double);
+  const size_t nfb = (size_t)cap * ch->k_fb, nat = (size_t)cap * ch->n_atoms;
+  const size_t ntr = (size_t)cap * ch->k_trial, ntp = ntr * kChainPlace;
+#define CB(field, name, bytes)                          \
+  w.field = (decltype(w.field))work(c, name, (bytes)); \
+  if (!w.field) return fail(c, RHG_ERR_OOM, "chain work " name);
+  CB(fbx, "c_fbx", nfb * d8);
+  CB(fby, "c_fby", nfb * d8);
+  CB(fbz, "c_fbz", nfb * d8);
+  CB(fb_uv, "c_fbuv", nfb * d8);
+  CB(W, "c_W", cap * d8);
+  CB(dU, "c_dU", cap * d8);
+  CB(active, "c_active", cap * 4);
+  CB(n_active, "c_nactive", 64);
+  CB(alive, "c_alive", cap * 4);
+  CB(sW, "c_sW", cap * d8);
+  CB(uhost, "c_uhost", cap * d8);
+  CB(uintra, "c_uintra", cap * d8);
+  CB(ubond, "c_ubond", cap * d8);
+  CB(ubend, "c_ubend", cap * d8);
+  CB(utors, "c_utors", cap * d8);
+  CB(W1, "c_W1", cap * d8);
+  CB(px, "c_px", nat * d8);
+  CB(py, "c_py", nat * d8);
+  CB(pz, "c_pz", nat * d8);
+  CB(bx, "c_bx", (size_t)cap * kChainPlace * d8);
+  CB(by, "c_by", (size_t)cap * kChainPlace * d8);
+  CB(bz, "c_bz", (size_t)cap * kChainPlace * d8);
+  CB(bl, "c_bl", cap * d8);
+  CB(tx, "c_tx", ntp * d8);
+  CB(ty, "c_ty", ntp * d8);
+  CB(tz, "c_tz", ntp * d8);
+  CB(tuv, "c_tuv", ntp * d8);
+  CB(tRT, "c_tRT", ntr * d8);
+  CB(tbond, "c_tbond", ntr * d8);
+  CB(tbend, "c_tbend", ntr * d8);
+  CB(ttors, "c_ttors", ntr * d8);
+  CB(tintra, "c_tintra", ntr * d8);
+#undef CB
+  return RHG_OK;
+}
+
+// One chunk of flexible-chain insertions [first, first + n): first-bead
+// trials and energies (shared with the rigid path), then per growth step
+// the internal walk, the trials, their framework energies and Rosen's
+// selection, then the finalisation and block sums.
+int chain_chunk(rhg_ctx *c, const rhg_chain *ch, const std::vector<ChainStep> &steps, const ChainStep *d_steps,
+                ChainWork &w, uint64_t seed, int64_t first, int64_t n, double *d_rec, double *d_xyz) {
+  hipStream_t st = c->stream;
+  w.rec = d_rec;
+  w.xyz = d_xyz;
+  launch_zero_counters(w.n_active, w.n_active, st);
+  // first-bead trial positions: the rigid path's generator
+  MolDev m;
+  memset(&m, 0, sizeof(m));
+  m.n_sites = 1;
+  m.k_fb = ch->k_fb;
+  m.k_or = 1;
+  WidomArgs wa = widom_args(c, m, seed, n);
+  WidomWork ww;
+  memset(&ww, 0, sizeof(ww));
+  ww.cap = w.cap;
+  ww.fbx = w.fbx;
+  ww.fby = w.fby;
+  ww.fbz = w.fbz;
+  {
+    Timed t(c, "widom_gen");
+    launch_widom_fb_gen(wa, ww, st);
+  }
+  const int type0 = ch->type[ch->start];
+  {
+    EnergyArgs ea = energy_args(c, type0, 1.0, 1, 0);
+    SiteList s{w.fbx, w.fby, w.fbz, n * ch->k_fb, wa.n_eff, ch->k_fb, w.cap};
+    EnergyOut o{w.fb_uv, nullptr, nullptr};
+    if (int rc = site_energy(c, type0, ea, s, o, "cfb")) return rc;
+  }
+  ChainArgs a;
+  memset(&a, 0, sizeof(a));
+  a.cell = c->cell;
+  a.pockets = c->pockets;
+  a.beta = c->beta;
+  a.seed = seed;
+  a.first = first;
+  a.n = n;
+  a.rc2 = c->rc2_vdw;
+  a.n_atoms = ch->n_atoms;
+  a.start = ch->start;
+  a.k_fb = ch->k_fb;
+  a.k = ch->k_trial;
+  a.kt = ch->k_torsion;
+  a.moves = ch->moves_per_open_bead;
+  {
+    Timed t(c, "chain_select");
+    launch_chain_fb_select(a, w, st);
+  }
+  for (size_t s = 0; s < steps.size(); ++s) {
+    const ChainStep &S = steps[s];
+    a.step = (int)s + 1;
+    a.st = d_steps + s;
+    {
+      Timed t(c, "chain_grow");
+      launch_chain_grow(a, w, st);
+    }
+    {
+      Timed t(c, "chain_trials");
+      launch_chain_trials(a, w, st);
+    }
+    // framework VDW of every trial, one site list per placed bead
+    for (int jj = 0; jj < S.n_place; ++jj) {
+      EnergyArgs ea = energy_args(c, S.type[jj], 1.0, 1, 0);
+      const int64_t base = (int64_t)jj * ch->k_trial * w.cap;
+      SiteList sl{w.tx + base, w.ty + base, w.tz + base, n * ch->k_trial, w.n_active, ch->k_trial, w.cap};
+      EnergyOut o{w.tuv + base, nullptr, nullptr};
+      if (int rc = site_energy(c, S.type[jj], ea, sl, o, "cgr")) return rc;
+    }
+    {
+      Timed t(c, "chain_select");
+      launch_chain_select(a, w, st);
+    }
+  }
+  {
+    Timed t(c, "chain_select");
+    launch_chain_finalize(a, w, st);
+  }
+  {
+    Timed t(c, "block_reduce");
+    double *part = (double *)work(c, "c_part", ((w.cap + kSeg - 1) / kSeg + 1) * 10 * sizeof(double));
+    if (!part) return fail(c, RHG_ERR_OOM, "chain reduce buffers");
+    launch_block_reduce(wa, w.W, w.dU, part, kSeg, c->d_acc, st);
+  }
+  launch_advance(c->d_ctl, n, st);
+  if (hipGetLastError() != hipSuccess) return fail(c, RHG_ERR_HIP, "chain launch failed");
+  return RHG_OK;
+}
+
 } // namespace
 
 extern "C" {
@@ -1521,6 +1848,51 @@ int rhg_widom_run(rhg_ctx *c, const rhg_molecule *mol, uint64_t seed, int64_t n_
   if (int rc = widom_state(c)) return rc;
   fill(c->d_acc, 15, 0.0, c->stream);
   if (int rc = widom_enqueue(c, mol, m, seed, n_total, first, n, steps_per_cycle, rec)) return rc;
+  return widom_collect(c, blocks);
+}
+
+int rhg_widom_chain_run(rhg_ctx *c, const rhg_chain *ch, uint64_t seed, int64_t n_total, int64_t first,
+                        int64_t n, int32_t steps_per_cycle, rhg_widom_blocks *blocks, double *rec, double *xyz) {
+  static_assert(kCrecN == RHG_CREC_N && kCrecStage == RHG_CREC_STAGE && kCrecW1 == RHG_CREC_W1,
+                "chain record layout");
+  if (!c || !ch) return RHG_ERR_INVALID;
+  if (n_total < 0 || first < 0 || n < 0 || first + n > n_total || steps_per_cycle < 1)
+    return fail(c, RHG_ERR_INVALID, "bad insertion range");
+  std::vector<ChainStep> steps;
+  if (int rc = chain_plan(c, ch, steps)) return rc;
+  if (c->ads_n > 0)
+    return fail(c, RHG_ERR_UNSUPPORTED, "chain: adsorbates are present (the flexible-chain path is at infinite dilution)");
+  if (n == 0) return RHG_OK;
+  hipSetDevice(c->device);
+  hipStream_t st = c->stream;
+  if (int rc = widom_state(c)) return rc;
+  fill(c->d_acc, 15, 0.0, st);
+  ChainStep *d_steps = (ChainStep *)work(c, "c_steps", steps.size() * sizeof(ChainStep));
+  if (!d_steps) return fail(c, RHG_ERR_OOM, "chain step tables");
+  HIPC(c, hipMemcpyAsync(d_steps, steps.data(), steps.size() * sizeof(ChainStep), hipMemcpyHostToDevice, st));
+  const int64_t cap = std::min<int64_t>(n, c->chunk);
+  ChainWork w;
+  if (int rc = chain_buffers(c, ch, cap, w)) return rc;
+  double *d_rec = nullptr, *d_xyz = nullptr;
+  if (rec && !(d_rec = (double *)work(c, "c_rec", (size_t)cap * kCrecN * sizeof(double))))
+    return fail(c, RHG_ERR_OOM, "chain records");
+  if (xyz && !(d_xyz = (double *)work(c, "c_xyz", (size_t)cap * ch->n_atoms * 3 * sizeof(double))))
+    return fail(c, RHG_ERR_OOM, "chain positions");
+  int64_t jb[6];
+  block_bounds(n_total, steps_per_cycle, jb);
+  launch_set_ctl(c->d_ctl, first, first + n, jb, st);
+  for (int64_t off = 0; off < n; off += cap) {
+    const int64_t cnt = std::min(cap, n - off);
+    if (int rc = check_fault(c)) return rc;
+    if (d_xyz) fill(d_xyz, cnt * ch->n_atoms * 3, 0.0, st);
+    if (int rc = chain_chunk(c, ch, steps, d_steps, w, seed, first + off, cnt, d_rec, d_xyz)) return rc;
+    if (rec)
+      HIPC(c, hipMemcpyAsync(rec + off * kCrecN, d_rec, (size_t)cnt * kCrecN * sizeof(double),
+                             hipMemcpyDeviceToHost, st));
+    if (xyz)
+      HIPC(c, hipMemcpyAsync(xyz + off * ch->n_atoms * 3, d_xyz, (size_t)cnt * ch->n_atoms * 3 * sizeof(double),
+                             hipMemcpyDeviceToHost, st));
+  }
   return widom_collect(c, blocks);
 }
 

@@ -331,4 +331,79 @@ void launch_interp(const InterpArgs &a, const double *xyz, int64_t n, double *ou
 void launch_site_grid(const InterpArgs *gv, const InterpArgs *gc, const SiteList &s, const EnergyOut &o,
                       int merge, hipStream_t st);
 
+// ------------------------------------------------------ flexible chains
+// CBMC growth of a flexible molecule of one group (src/cbmc.c:2762-2849,
+// 1480-2482, 1154-1478).  The growth path is the same for every insertion;
+// the host resolves it into ChainStep tables (device memory, read
+// wave-uniformly).  Atom references inside a step are local codes: 0..3 the
+// beads placed by the step, kLocCur the current bead, kLocPrev the previous
+// one, kLocGlobal + atom any other placed atom.
+constexpr int kChainMax = 64;   // atoms of a chain
+constexpr int kChainPlace = 4;  // beads placed by one step
+constexpr int kChainBonds = 4, kChainBends = 8, kChainTors = 16, kChainIntra = 64;
+constexpr int kChainTorsionTrials = 1024;
+constexpr int kLocCur = 4, kLocPrev = 5, kLocGlobal = 8;
+// per-insertion chain record fields (the C ABI's RHG_CREC_*)
+enum {
+  kCrecW = 0, kCrecDU, kCrecUHost, kCrecUIntra, kCrecUBond, kCrecUBend, kCrecUTors, kCrecOverlap,
+  kCrecStage, kCrecStep, kCrecW1, kCrecN
+};
+
+struct ChainStep {
+  int cur, prev, n_prev;
+  int n_place;                  // beads placed (= branches when n_prev = 1)
+  int place[kChainPlace];       // atom ids
+  int type[kChainPlace];        // pseudo-atom of each placed bead
+  int n_bonds, n_bends, n_tors, n_intra;
+  int bond_a[kChainBonds], bond_b[kChainBonds], bond_kind[kChainBonds]; // local codes; 0 fixed, 1 harmonic
+  int bond_branch[kChainBonds];  // branch slot whose bond this is (BranchAtoms[jj][0])
+  double bond_p0[kChainBonds], bond_p1[kChainBonds]; // p0 / ENERGY_TO_KELVIN, p1 [A] (fixed: p0 = length)
+  int bend_a[kChainBends], bend_b[kChainBends], bend_c[kChainBends];
+  double bend_p0[kChainBends], bend_p1[kChainBends]; // internal units, radians
+  int tors_a[kChainTors], tors_b[kChainTors], tors_c[kChainTors], tors_d[kChainTors];
+  double tors_p[kChainTors][4];
+  int intra_a[kChainIntra], intra_b[kChainIntra];
+  double intra_scal[kChainIntra], intra_c4[kChainIntra], intra_s2[kChainIntra], intra_shift[kChainIntra];
+  double ref_len[kChainPlace];   // GetReferenceBondLength(cur, branch)
+  double max_bond[kChainPlace];  // MaximumCBMCChangeBondLength of each branch atom
+  double max_bend, max_cone;     // ...ChangeBendAngle / ...RotationOnCone of the current bead
+};
+
+struct ChainWork {
+  int64_t cap;
+  // first-bead trials [trial][insertion] and per-insertion results
+  double *fbx, *fby, *fbz, *fb_uv;
+  double *W, *dU;             // [insertion]
+  double *rec;                // [insertion][RHG_CREC_N] (nullable)
+  int32_t *active, *n_active; // slot -> insertion
+  // per slot
+  int32_t *alive;             // 1 growing, 0 done (rejected)
+  double *sW, *uhost, *uintra, *ubond, *ubend, *utors, *W1;
+  double *px, *py, *pz;       // grown positions [atom][slot]
+  double *bx, *by, *bz;       // internal MC result [place][slot]
+  double *bl;                 // bond length of a sphere step [slot]
+  // trials of the step: positions [place * k + t][slot], energies [t][slot]
+  double *tx, *ty, *tz, *tuv;
+  double *tRT, *tbond, *tbend, *ttors, *tintra;
+  double *xyz;                // [insertion][atom][3] grown molecule (nullable)
+};
+
+struct ChainArgs {
+  Cell cell;
+  PocketArgs pockets;
+  double beta;
+  uint64_t seed;
+  int64_t first, n;     // insertions [first, first + n) of the chunk
+  double rc2;           // CutOffVDW^2 (intra-molecular pairs)
+  int n_atoms, start, k_fb, k, kt, moves;
+  int step;             // 1-based growth step
+  const ChainStep *st;  // device table of this step
+};
+
+void launch_chain_fb_select(const ChainArgs &a, const ChainWork &w, hipStream_t st);
+void launch_chain_grow(const ChainArgs &a, const ChainWork &w, hipStream_t st);
+void launch_chain_trials(const ChainArgs &a, const ChainWork &w, hipStream_t st);
+void launch_chain_select(const ChainArgs &a, const ChainWork &w, hipStream_t st);
+void launch_chain_finalize(const ChainArgs &a, const ChainWork &w, hipStream_t st);
+
 } // namespace rhg

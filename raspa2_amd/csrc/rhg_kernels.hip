@@ -2544,5 +2544,792 @@ void launch_stream_copy(const void *src, void *dst, int64_t bytes, hipStream_t s
   const int64_t n = bytes / 16;
   hipLaunchKernelGGL(k_stream_copy, dim3(4096), dim3(256), 0, st, (const float4 *)src, (float4 *)dst, n);
 }
-} // namespace rhg
 
+// ====================================================================
+// Flexible chains: CBMC growth with the internal MC scheme
+// (src/cbmc.c:1480-2482, 2762-2849; src/internal_energy.c:48-70,
+// 306-329, 372-490, 1168-1340, 2177-2330; src/utils.c:640-990).
+// FP64 throughout, no contraction, the reference's operation order; the
+// draws come from per-purpose sub-streams of the insertion's counter
+// stream.
+// ====================================================================
+namespace chain {
+
+struct Draw {
+  uint64_t key, ctr;
+  __device__ double next() { return stream_uniform(key, ctr++); }
+};
+
+// RandomGaussianNumber (src/utils.c:640-651)
+__device__ double gauss(Draw &d) {
+#pragma clang fp contract(off)
+  double ran1, ran2, r2;
+  do {
+    ran1 = 2.0 * d.next() - 1.0;
+    ran2 = 2.0 * d.next() - 1.0;
+    r2 = ran1 * ran1 + ran2 * ran2;
+  } while ((r2 > 1.0) || (r2 == 0.0));
+  return ran2 * sqrt(-2.0 * log(r2) / r2);
+}
+
+// RandomNumberOnUnitSphere (src/utils.c:688-708)
+__device__ void sphere(Draw &d, double v[3]) {
+#pragma clang fp contract(off)
+  double ran1, ran2, ransq;
+  do {
+    ran1 = 2.0 * d.next() - 1.0;
+    ran2 = 2.0 * d.next() - 1.0;
+    ransq = ran1 * ran1 + ran2 * ran2;
+  } while (ransq >= 1.0);
+  const double ranh = 2.0 * sqrt(1.0 - ransq);
+  v[0] = ran1 * ranh;
+  v[1] = ran2 * ranh;
+  v[2] = 1.0 - 2.0 * ransq;
+}
+
+// RandomNumberOnCone (src/utils.c:709-735)
+__device__ void cone(Draw &d, const double vin[3], double theta, double v[3]) {
+#pragma clang fp contract(off)
+  const double cosin = cos(theta);
+  double ran1 = 1.0 / sqrt(vin[0] * vin[0] + vin[1] * vin[1] + vin[2] * vin[2]);
+  const double xprev = vin[0] * ran1, yprev = vin[1] * ran1, zprev = vin[2] * ran1;
+  sphere(d, v);
+  ran1 = v[0] * xprev + v[1] * yprev + v[2] * zprev;
+  v[0] = v[0] - ran1 * xprev;
+  v[1] = v[1] - ran1 * yprev;
+  v[2] = v[2] - ran1 * zprev;
+  ran1 = sqrt((1.0 - cosin * cosin) / (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+  v[0] = v[0] * ran1 + cosin * xprev;
+  v[1] = v[1] * ran1 + cosin * yprev;
+  v[2] = v[2] * ran1 + cosin * zprev;
+  ran1 = 1.0 / sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  v[0] = v[0] * ran1;
+  v[1] = v[1] * ran1;
+  v[2] = v[2] * ran1;
+}
+
+// Perpendicular (src/utils.c:789-832)
+__device__ void perpendicular(const double a[3], const double b[3], double v[3]) {
+#pragma clang fp contract(off)
+  double fx = fabs(b[1] * a[2] - a[1] * b[2]);
+  const double fy = fabs(b[0] * a[2] - a[0] * b[2]);
+  const double fz = fabs(b[1] * a[0] - a[1] * b[0]);
+  int choise = 1;
+  if (fy > fx) {
+    choise = 2;
+    fx = fy;
+  }
+  if (fz > fx) choise = 3;
+  if (choise == 1) {
+    v[0] = 1.0;
+    v[1] = (a[0] * b[2] - b[0] * a[2]) / (b[1] * a[2] - a[1] * b[2]);
+    v[2] = (b[0] * a[1] - a[0] * b[1]) / (b[1] * a[2] - a[1] * b[2]);
+  } else if (choise == 2) {
+    v[0] = (a[1] * b[2] - b[1] * a[2]) / (b[0] * a[2] - a[0] * b[2]);
+    v[1] = 1.0;
+    v[2] = (b[1] * a[0] - a[1] * b[0]) / (b[0] * a[2] - a[0] * b[2]);
+  } else {
+    v[0] = (b[2] * a[1] - a[2] * b[1]) / (b[1] * a[0] - a[1] * b[0]);
+    v[1] = (a[2] * b[0] - b[2] * a[0]) / (b[1] * a[0] - a[1] * b[0]);
+    v[2] = 1.0;
+  }
+  const double f = 1.0 / sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  v[0] *= f;
+  v[1] *= f;
+  v[2] *= f;
+}
+
+// RotationAroundXYZAxis (src/utils.c:945-975): the matrix of a rotation by
+// theta about the unit axis v, applied as the reference applies it
+struct Rot {
+  double r[3][3];
+  __device__ Rot(const double v[3], double theta) {
+#pragma clang fp contract(off)
+    const double c = cos(theta);
+    const double w = 1.0 - c;
+    double s = sqrt(1.0 - c * c);
+    if (theta < 0.0) s = -s;
+    r[0][0] = (v[0]) * (v[0]) * w + c;
+    r[0][1] = (v[0]) * (v[1]) * w + (v[2]) * s;
+    r[0][2] = (v[0]) * (v[2]) * w - (v[1]) * s;
+    r[1][0] = (v[0]) * (v[1]) * w - (v[2]) * s;
+    r[1][1] = (v[1]) * (v[1]) * w + c;
+    r[1][2] = (v[1]) * (v[2]) * w + (v[0]) * s;
+    r[2][0] = (v[0]) * (v[2]) * w + (v[1]) * s;
+    r[2][1] = (v[1]) * (v[2]) * w - (v[0]) * s;
+    r[2][2] = (v[2]) * (v[2]) * w + c;
+  }
+  __device__ void apply(double p[3]) const {
+#pragma clang fp contract(off)
+    const double x = p[0] * r[0][0] + p[1] * r[0][1] + p[2] * r[0][2];
+    const double y = p[0] * r[1][0] + p[1] * r[1][1] + p[2] * r[1][2];
+    const double z = p[0] * r[2][0] + p[1] * r[2][1] + p[2] * r[2][2];
+    p[0] = x;
+    p[1] = y;
+    p[2] = z;
+  }
+};
+
+// CalculateAngle (src/internal_energy.c:306-329)
+__device__ double angle(const double a[3], const double b[3], const double c[3]) {
+#pragma clang fp contract(off)
+  double ab[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+  const double rab = 1.0 / sqrt(ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2]);
+  ab[0] *= rab;
+  ab[1] *= rab;
+  ab[2] *= rab;
+  double bc[3] = {c[0] - b[0], c[1] - b[1], c[2] - b[2]};
+  const double rbc = 1.0 / sqrt(bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2]);
+  bc[0] *= rbc;
+  bc[1] *= rbc;
+  bc[2] *= rbc;
+  return acos(ab[0] * bc[0] + ab[1] * bc[1] + ab[2] * bc[2]);
+}
+
+// CalculateBondEnergy, HARMONIC_BOND (src/internal_energy.c:372-395); FIXED_BOND is 0
+__device__ double bond_u(int kind, double p0, double p1, const double a[3], const double b[3]) {
+#pragma clang fp contract(off)
+  const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+  const double r = sqrt(dx * dx + dy * dy + dz * dz);
+  if (kind == 1) return 0.5 * p0 * ((r - p1) * (r - p1));
+  return 0.0;
+}
+
+// CalculateBendEnergy, HARMONIC_BEND (src/internal_energy.c:1225-1268)
+__device__ double bend_u(double p0, double p1, const double a[3], const double b[3], const double c[3]) {
+#pragma clang fp contract(off)
+  double ab[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+  const double rab = sqrt(ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2]);
+  ab[0] /= rab;
+  ab[1] /= rab;
+  ab[2] /= rab;
+  double bc[3] = {c[0] - b[0], c[1] - b[1], c[2] - b[2]};
+  const double rbc = sqrt(bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2]);
+  bc[0] /= rbc;
+  bc[1] /= rbc;
+  bc[2] /= rbc;
+  double ct = ab[0] * bc[0] + ab[1] * bc[1] + ab[2] * bc[2];
+  const double lo = (-1.0 > ct) ? -1.0 : ct;
+  ct = (1.0 <= lo) ? 1.0 : lo;
+  const double th = acos(ct);
+  return 0.5 * p0 * ((th - p1) * (th - p1));
+}
+
+// CalculateTorsionEnergy, TRAPPE_DIHEDRAL (src/internal_energy.c:2177-2230, 2324-2332)
+__device__ double torsion_u(const double *p, const double a[3], const double b[3], const double c[3],
+                            const double d[3]) {
+#pragma clang fp contract(off)
+  const double Dab[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+  double Dbc[3] = {c[0] - b[0], c[1] - b[1], c[2] - b[2]};
+  const double rbc = sqrt(Dbc[0] * Dbc[0] + Dbc[1] * Dbc[1] + Dbc[2] * Dbc[2]);
+  Dbc[0] /= rbc;
+  Dbc[1] /= rbc;
+  Dbc[2] /= rbc;
+  const double Dcd[3] = {d[0] - c[0], d[1] - c[1], d[2] - c[2]};
+  const double dot_ab = Dab[0] * Dbc[0] + Dab[1] * Dbc[1] + Dab[2] * Dbc[2];
+  const double dot_cd = Dcd[0] * Dbc[0] + Dcd[1] * Dbc[1] + Dcd[2] * Dbc[2];
+  double dr[3] = {Dab[0] - dot_ab * Dbc[0], Dab[1] - dot_ab * Dbc[1], Dab[2] - dot_ab * Dbc[2]};
+  const double r = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+  dr[0] /= r;
+  dr[1] /= r;
+  dr[2] /= r;
+  double ds[3] = {Dcd[0] - dot_cd * Dbc[0], Dcd[1] - dot_cd * Dbc[1], Dcd[2] - dot_cd * Dbc[2]};
+  const double s = sqrt(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2]);
+  ds[0] /= s;
+  ds[1] /= s;
+  ds[2] /= s;
+  double cp = dr[0] * ds[0] + dr[1] * ds[1] + dr[2] * ds[2];
+  const double m = fabs(cp) <= 1.0 ? fabs(cp) : 1.0;
+  cp = cp >= 0.0 ? m : -m;
+  return p[0] + (1.0 + cp) * (p[1] + p[3] - 2.0 * (cp - 1.0) * (p[2] - 2.0 * p[3] * cp));
+}
+
+// GenerateBondlength (src/internal_energy.c:48-70, 185-187)
+__device__ double bond_length(Draw &d, double beta, int kind, double p0, double p1) {
+#pragma clang fp contract(off)
+  if (kind != 1) return p0;
+  const double ran1 = 1.0 / sqrt(beta * p0);
+  const double ran2 = 1.0 / ((p1 + 3.0 * ran1) * (p1 + 3.0 * ran1));
+  double bl;
+  do
+    bl = p1 + ran1 * gauss(d);
+  while ((d.next() > (bl * bl * ran2)) || (bl <= 0.0));
+  return bl;
+}
+
+// The positions one lane works on: the beads placed by the step (0..3), the
+// current and previous beads, and any other placed atom read from the
+// grown molecule.
+struct Local {
+  double p[6][3];
+  const ChainWork *w;
+  int64_t slot;
+  __device__ void get(int code, double out[3]) const {
+    if (code < kLocGlobal) {
+      out[0] = p[code][0];
+      out[1] = p[code][1];
+      out[2] = p[code][2];
+    } else {
+      const int64_t o = (int64_t)(code - kLocGlobal) * w->cap + slot;
+      out[0] = w->px[o];
+      out[1] = w->py[o];
+      out[2] = w->pz[o];
+    }
+  }
+  __device__ void load(int atom, int code) {
+    const int64_t o = (int64_t)atom * w->cap + slot;
+    p[code][0] = w->px[o];
+    p[code][1] = w->py[o];
+    p[code][2] = w->pz[o];
+  }
+};
+
+__device__ double sum_bonds(const ChainStep &S, const Local &L) {
+#pragma clang fp contract(off)
+  double u = 0.0;
+  for (int j = 0; j < S.n_bonds; ++j) {
+    double a[3], b[3];
+    L.get(S.bond_a[j], a);
+    L.get(S.bond_b[j], b);
+    u += bond_u(S.bond_kind[j], S.bond_p0[j], S.bond_p1[j], a, b);
+  }
+  return u;
+}
+__device__ double bend_j(const ChainStep &S, const Local &L, int j) {
+  double a[3], b[3], c[3];
+  L.get(S.bend_a[j], a);
+  L.get(S.bend_b[j], b);
+  L.get(S.bend_c[j], c);
+  return bend_u(S.bend_p0[j], S.bend_p1[j], a, b, c);
+}
+__device__ double sum_bends(const ChainStep &S, const Local &L) {
+#pragma clang fp contract(off)
+  double u = 0.0;
+  for (int j = 0; j < S.n_bends; ++j) u += bend_j(S, L, j);
+  return u;
+}
+__device__ double sum_torsions(const ChainStep &S, const Local &L) {
+#pragma clang fp contract(off)
+  double u = 0.0;
+  for (int j = 0; j < S.n_tors; ++j) {
+    double a[3], b[3], c[3], d[3];
+    L.get(S.tors_a[j], a);
+    L.get(S.tors_b[j], b);
+    L.get(S.tors_c[j], c);
+    L.get(S.tors_d[j], d);
+    u += torsion_u(S.tors_p[j], a, b, c, d);
+  }
+  return u;
+}
+// intra-molecular LJ of the step's pairs (src/cbmc.c:1294-1311): plain
+// distance, cut-off CutOffVDW, scaling times PotentialValue
+__device__ double sum_intra(const ChainStep &S, const Local &L, double rc2) {
+#pragma clang fp contract(off)
+  double u = 0.0;
+  for (int j = 0; j < S.n_intra; ++j) {
+    double a[3], b[3];
+    L.get(S.intra_a[j], a);
+    L.get(S.intra_b[j], b);
+    const double rr = (a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]);
+    if (rr < rc2) u += S.intra_scal[j] * lj_exact(S.intra_c4[j], S.intra_s2[j], S.intra_shift[j], rr);
+  }
+  return u;
+}
+
+__device__ __forceinline__ uint64_t insertion_key(const ChainArgs &a, const ChainWork &w, int64_t slot) {
+  return stream_key(a.seed, (uint64_t)(a.first + w.active[slot]));
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+  for (int s = 32; s > 0; s >>= 1) v = fmax(v, __shfl_xor(v, s));
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int s = 32; s > 0; s >>= 1) v += __shfl_xor(v, s);
+  return v;
+}
+// inclusive prefix sum over the 64 lanes of a wave
+__device__ __forceinline__ double wave_scan(double v, int lane) {
+  for (int s = 1; s < 64; s <<= 1) {
+    const double u = __shfl_up(v, s);
+    if (lane >= s) v += u;
+  }
+  return v;
+}
+
+} // namespace chain
+
+// HandleFirstBead selection (src/cbmc.c:695-795) for chains: the accepted
+// insertions get a slot of the growing set.
+__global__ __launch_bounds__(256) void k_chain_fb_select(const ChainArgs a, const ChainWork w) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int k = a.k_fb;
+  const double beta = a.beta;
+  bool any = false;
+  double W1 = 0.0;
+  for (int t = 0; t < k; ++t) {
+    const double uv = w.fb_uv[t * w.cap + i];
+    if (uv < kOverlap) {
+      any = true;
+      W1 += exp(-beta * (uv + 0.0 + 0.0));
+    }
+  }
+  w.W[i] = 0.0;
+  w.dU[i] = 0.0;
+  double *rec = w.rec ? w.rec + i * kCrecN : nullptr;
+  if (rec)
+    for (int f = 0; f < kCrecN; ++f) rec[f] = 0.0;
+  if (!any) {
+    if (rec) rec[kCrecOverlap] = 1.0;
+    return;
+  }
+  const uint64_t key = stream_key(a.seed, (uint64_t)(a.first + i));
+  const double usel = stream_uniform(key, (uint64_t)(3 * k));
+  auto bf = [&](int t, double &b) -> bool {
+    const double uv = w.fb_uv[t * w.cap + i];
+    if (uv >= kOverlap) return false;
+    b = -beta * (uv + 0.0 + 0.0);
+    return true;
+  };
+  const int sel = select_trial(bf, k, usel);
+  W1 /= (double)k;
+  if (rec) rec[kCrecW1] = W1;
+  if (W1 < kMinRosenbluth) {
+    if (rec) {
+      rec[kCrecOverlap] = 1.0;
+      rec[kCrecStage] = 1.0;
+    }
+    return;
+  }
+  const int64_t slot = atomicAdd(w.n_active, 1);
+  w.active[slot] = (int32_t)i;
+  w.alive[slot] = 1;
+  w.sW[slot] = W1;
+  w.W1[slot] = W1;
+  w.uhost[slot] = w.fb_uv[sel * w.cap + i];
+  w.uintra[slot] = 0.0;
+  w.ubond[slot] = 0.0;
+  w.ubend[slot] = 0.0;
+  w.utors[slot] = 0.0;
+  const int64_t o = (int64_t)a.start * w.cap + slot;
+  w.px[o] = w.fbx[sel * w.cap + i];
+  w.py[o] = w.fby[sel * w.cap + i];
+  w.pz[o] = w.fbz[sel * w.cap + i];
+}
+
+// Bond lengths of a step (GenerateBondlength); for a step with a previous
+// bead the internal Metropolis walk of GenerateTrialOrientationsMCScheme
+// (src/cbmc.c:1580-2350) without a stored configuration, on trial 0.
+__global__ __launch_bounds__(64) void k_chain_grow(const ChainArgs a, const ChainWork w) {
+#pragma clang fp contract(off)
+  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= (int64_t)*w.n_active || !w.alive[slot]) return;
+  const ChainStep &S = *a.st;
+  const double beta = a.beta;
+  chain::Draw d{chain_substream(chain::insertion_key(a, w, slot), chain_tag(a.step, 0)), 0};
+  chain::Local L;
+  L.w = &w;
+  L.slot = slot;
+  L.load(S.cur, kLocCur);
+  if (S.n_prev == 0) {
+    double bl0 = 0.0;
+    for (int j = 0; j < S.n_bonds; ++j) {
+      const double b = chain::bond_length(d, beta, S.bond_kind[j], S.bond_p0[j], S.bond_p1[j]);
+      if (j == 0) bl0 = b;
+    }
+    w.bl[slot] = bl0;
+    return;
+  }
+  L.load(S.prev, kLocPrev);
+  for (int j = 0; j < S.n_bonds; ++j) (void)chain::bond_length(d, beta, S.bond_kind[j], S.bond_p0[j], S.bond_p1[j]);
+  const double *cur = L.p[kLocCur];
+  double pv[3] = {L.p[kLocPrev][0] - cur[0], L.p[kLocPrev][1] - cur[1], L.p[kLocPrev][2] - cur[2]};
+  const double nrm = sqrt(pv[0] * pv[0] + pv[1] * pv[1] + pv[2] * pv[2]);
+  pv[0] /= nrm;
+  pv[1] /= nrm;
+  pv[2] /= nrm;
+  const int nb = S.n_bonds, nbe = S.n_bends, np = S.n_place;
+  for (int jj = 0; jj < np; ++jj) {
+    double v[3];
+    chain::cone(d, pv, 120 * (M_PI / 180.0), v);
+    for (int q = 0; q < 3; ++q) L.p[jj][q] = cur[q] + S.ref_len[jj] * v[q];
+  }
+  double eobond[kChainBonds], enbond[kChainBonds], eobend[kChainBends], enbend[kChainBends];
+  auto bond_e = [&](int j) {
+    double pa[3], pb[3];
+    L.get(S.bond_a[j], pa);
+    L.get(S.bond_b[j], pb);
+    return chain::bond_u(S.bond_kind[j], S.bond_p0[j], S.bond_p1[j], pa, pb);
+  };
+  for (int j = 0; j < nb; ++j) eobond[j] = enbond[j] = bond_e(j);
+  for (int j = 0; j < nbe; ++j) eobend[j] = enbend[j] = chain::bend_j(S, L, j);
+  const int trialmove = 4 * a.moves * np;
+  if ((nb >= 2) && (d.next() < 0.5)) {
+    double cA[3], cC[3];
+    for (int q = 0; q < 3; ++q) {
+      cA[q] = L.p[0][q] - cur[q];
+      cC[q] = L.p[1][q] - cur[q];
+    }
+    double tmp = sqrt(cA[0] * cA[0] + cA[1] * cA[1] + cA[2] * cA[2]) / sqrt(cC[0] * cC[0] + cC[1] * cC[1] + cC[2] * cC[2]);
+    for (int q = 0; q < 3; ++q) L.p[0][q] = cur[q] + tmp * cC[q];
+    tmp = 1.0 / tmp;
+    for (int q = 0; q < 3; ++q) L.p[1][q] = cur[q] + tmp * cA[q];
+    for (int j = 0; j < nb; ++j) eobond[j] = enbond[j] = bond_e(j);
+    for (int j = 0; j < nbe; ++j) eobend[j] = enbend[j] = chain::bend_j(S, L, j);
+  }
+  for (int jjj = 0; jjj < trialmove; ++jjj) {
+    int choise;
+    if (nbe == 1)
+      choise = d.next() < 0.3 ? 1 : 2;
+    else if (jjj < 20)
+      choise = 3;
+    else {
+      const double tmp = d.next();
+      choise = tmp < 0.2 ? 1 : (tmp < 0.6 ? 2 : 3);
+    }
+    for (int j = 0; j < nb; ++j) enbond[j] = eobond[j];
+    for (int j = 0; j < nbe; ++j) enbend[j] = eobend[j];
+    double newbond = 0.0, newbend = 0.0;
+    if (choise == 1) {
+      if (nb > 0) {
+        int jj = (int)(d.next() * (double)nb);
+        if (jj >= nb) jj = nb - 1; // u = 1 (closed interval) would index past the table
+        if (S.bond_kind[jj] != 0) {
+          const int jo = S.bond_branch[jj];
+          const double store[3] = {L.p[jo][0], L.p[jo][1], L.p[jo][2]};
+          const double vec[3] = {L.p[jo][0] - cur[0], L.p[jo][1] - cur[1], L.p[jo][2] - cur[2]};
+          const double blo = sqrt(vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2]);
+          const double bln = blo + (2.0 * d.next() - 1.0) * S.max_bond[jo];
+          for (int q = 0; q < 3; ++q) L.p[jo][q] = cur[q] + (bln / blo) * vec[q];
+          enbond[jj] = bond_e(jj);
+          newbond += enbond[jj] - eobond[jj];
+          if (d.next() < ((bln / blo) * (bln / blo)) * exp(-beta * (newbond + 0.0 + 0.0 + 0.0)))
+            eobond[jj] = enbond[jj];
+          else
+            for (int q = 0; q < 3; ++q) L.p[jo][q] = store[q];
+        }
+      }
+    } else if (choise == 2) {
+      if (np > 0) {
+        int jo = (int)(d.next() * (double)np);
+        if (jo >= np) jo = np - 1;
+        const double store[3] = {L.p[jo][0], L.p[jo][1], L.p[jo][2]};
+        double cord[3] = {L.p[jo][0] - cur[0], L.p[jo][1] - cur[1], L.p[jo][2] - cur[2]};
+        const double sin_old = sin(chain::angle(L.p[kLocPrev], cur, L.p[jo]));
+        const double dr[3] = {cur[0] - L.p[jo][0], cur[1] - L.p[jo][1], cur[2] - L.p[jo][2]};
+        double perpen[3];
+        chain::perpendicular(dr, pv, perpen);
+        const chain::Rot R(perpen, (2.0 * d.next() - 1.0) * S.max_bend);
+        R.apply(cord);
+        for (int q = 0; q < 3; ++q) L.p[jo][q] = cur[q] + cord[q];
+        const double sin_new = sin(chain::angle(L.p[kLocPrev], cur, L.p[jo]));
+        for (int j = 0; j < nbe; ++j) {
+          enbend[j] = chain::bend_j(S, L, j);
+          newbend += enbend[j] - eobend[j];
+        }
+        if (d.next() < ((sin_new / sin_old) * exp(-beta * (newbend + 0.0 + 0.0 + 0.0)))) {
+          for (int j = 0; j < nbe; ++j) eobend[j] = enbend[j];
+        } else
+          for (int q = 0; q < 3; ++q) L.p[jo][q] = store[q];
+      }
+    } else {
+      if (np > 0) {
+        int jo = (int)(d.next() * (double)np);
+        if (jo >= np) jo = np - 1;
+        const double store[3] = {L.p[jo][0], L.p[jo][1], L.p[jo][2]};
+        double cord[3] = {L.p[jo][0] - cur[0], L.p[jo][1] - cur[1], L.p[jo][2] - cur[2]};
+        const chain::Rot R(pv, (2.0 * d.next() - 1.0) * S.max_cone);
+        R.apply(cord);
+        for (int q = 0; q < 3; ++q) L.p[jo][q] = cur[q] + cord[q];
+        for (int j = 0; j < nbe; ++j) {
+          enbend[j] = chain::bend_j(S, L, j);
+          newbend += enbend[j] - eobend[j];
+        }
+        if (d.next() < exp(-beta * (newbend + 0.0 + 0.0 + 0.0))) {
+          for (int j = 0; j < nbe; ++j) eobend[j] = enbend[j];
+        } else
+          for (int q = 0; q < 3; ++q) L.p[jo][q] = store[q];
+      }
+    }
+  }
+  for (int jj = 0; jj < np; ++jj) {
+    const int64_t o = (int64_t)jj * w.cap + slot;
+    w.bx[o] = L.p[jj][0];
+    w.by[o] = L.p[jj][1];
+    w.bz[o] = L.p[jj][2];
+  }
+}
+
+// Trial positions of a step, one wave per (slot, trial).  Sphere steps
+// (no previous bead): a random direction at the drawn bond length
+// (src/cbmc.c:1530-1577).  Otherwise the torsion selection of
+// GenerateTrialOrientationsMCScheme (src/cbmc.c:2350-2435): the lanes take
+// the k_torsion rotations of the walk's result about the previous-current
+// axis; a wave maximum, scan and ballot select one with probability
+// proportional to exp(-beta U_torsion), and RosenbluthTorsion averages the
+// first NumberOfTrialPositions of them (the reference's normalisation reads
+// the global trial count, src/cbmc.c:542-555).  Then the trial's bonded and
+// intra-molecular energies (src/cbmc.c:2437-2480, 1294-1311).
+__global__ __launch_bounds__(256) void k_chain_trials(const ChainArgs a, const ChainWork w) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
+  const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int k = a.k;
+  const int64_t slot = wid / k;
+  const int t = (int)(wid % k);
+  if (slot >= (int64_t)*w.n_active) return;
+  const ChainStep &S = *a.st;
+  const int64_t cap = w.cap;
+  chain::Local L;
+  L.w = &w;
+  L.slot = slot;
+  L.load(S.cur, kLocCur);
+  const double *cur = L.p[kLocCur];
+  const int np = S.n_place;
+  auto store_trial = [&](double rt, double ub, double ube, double ut, double ui) {
+    for (int jj = 0; jj < np; ++jj) {
+      const int64_t o = (int64_t)(jj * k + t) * cap + slot;
+      w.tx[o] = L.p[jj][0];
+      w.ty[o] = L.p[jj][1];
+      w.tz[o] = L.p[jj][2];
+    }
+    const int64_t o = (int64_t)t * cap + slot;
+    w.tRT[o] = rt;
+    w.tbond[o] = ub;
+    w.tbend[o] = ube;
+    w.ttors[o] = ut;
+    w.tintra[o] = ui;
+  };
+  if (!w.alive[slot]) {
+    // a rejected insertion keeps valid (unused) coordinates in the site lists
+    if (lane == 0) {
+      for (int jj = 0; jj < np; ++jj)
+        for (int q = 0; q < 3; ++q) L.p[jj][q] = cur[q];
+      store_trial(1.0, 0.0, 0.0, 0.0, 0.0);
+    }
+    return;
+  }
+  chain::Draw d{chain_substream(chain::insertion_key(a, w, slot), chain_tag(a.step, 2 + t)), 0};
+  if (S.n_prev == 0) {
+    if (lane == 0) {
+      double v[3];
+      chain::sphere(d, v);
+      const double bl = w.bl[slot];
+      for (int q = 0; q < 3; ++q) L.p[0][q] = cur[q] + v[q] * bl;
+      double pa[3], pb[3];
+      L.get(S.bond_a[0], pa);
+      L.get(S.bond_b[0], pb);
+      const double ub = chain::bond_u(S.bond_kind[0], S.bond_p0[0], S.bond_p1[0], pa, pb);
+      store_trial(1.0, ub, 0.0, 0.0, chain::sum_intra(S, L, a.rc2));
+    }
+    return;
+  }
+  L.load(S.prev, kLocPrev);
+  double pv[3] = {L.p[kLocPrev][0] - cur[0], L.p[kLocPrev][1] - cur[1], L.p[kLocPrev][2] - cur[2]};
+  const double nrm = sqrt(pv[0] * pv[0] + pv[1] * pv[1] + pv[2] * pv[2]);
+  pv[0] /= nrm;
+  pv[1] /= nrm;
+  pv[2] /= nrm;
+  double base[kChainPlace][3];
+  for (int jj = 0; jj < np; ++jj) {
+    const int64_t o = (int64_t)jj * cap + slot;
+    base[jj][0] = w.bx[o];
+    base[jj][1] = w.by[o];
+    base[jj][2] = w.bz[o];
+  }
+  const double beta = a.beta;
+  const int kt = a.kt;
+  auto place_at = [&](double ang) {
+    const chain::Rot R(pv, ang);
+    for (int jj = 0; jj < np; ++jj) {
+      double c[3] = {base[jj][0] - cur[0], base[jj][1] - cur[1], base[jj][2] - cur[2]};
+      R.apply(c);
+      for (int q = 0; q < 3; ++q) L.p[jj][q] = c[q] + cur[q];
+    }
+  };
+  constexpr int kPer = kChainTorsionTrials / 64;
+  double bf[kPer];
+  double largest = -DBL_MAX;
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {
+    const int kk = r * 64 + lane;
+    bf[r] = 0.0;
+    if (r * 64 < kt && kk < kt) {
+      place_at((2.0 * stream_uniform(d.key, (uint64_t)kk) - 1.0) * M_PI);
+      bf[r] = -beta * chain::sum_torsions(S, L);
+      largest = fmax(largest, bf[r]);
+    }
+  }
+  largest = chain::wave_max(largest);
+  // RosenbluthTorsion: the first k of the torsion trials (k <= 64)
+  const double rt = chain::wave_sum(lane < k ? exp(bf[0]) : 0.0) / (double)k;
+  const double u = stream_uniform(d.key, (uint64_t)kt);
+  int sel = 0;
+  if (S.n_tors > 0) {
+    double total = 0.0;
+#pragma unroll
+    for (int r = 0; r < kPer; ++r)
+      if (r * 64 < kt) total += chain::wave_sum((r * 64 + lane < kt) ? exp(bf[r] - largest) : 0.0);
+    const double ws = u * total;
+    double carry = 0.0;
+    sel = -1;
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      if (r * 64 >= kt || sel >= 0) continue;
+      const int kk = r * 64 + lane;
+      const double inc = chain::wave_scan(kk < kt ? exp(bf[r] - largest) : 0.0, lane);
+      const uint64_t hit = __ballot(kk < kt && carry + inc >= ws);
+      if (hit)
+        sel = r * 64 + (__ffsll((unsigned long long)hit) - 1);
+      else
+        carry += __shfl(inc, 63);
+    }
+    if (sel < 0) sel = kt - 1;
+  } else {
+    sel = (int)(u * (double)kt);
+    if (sel >= kt) sel = kt - 1;
+  }
+  if (lane != 0) return;
+  place_at((2.0 * stream_uniform(d.key, (uint64_t)sel) - 1.0) * M_PI);
+  store_trial(rt, chain::sum_bonds(S, L), chain::sum_bends(S, L), chain::sum_torsions(S, L),
+              chain::sum_intra(S, L, a.rc2));
+}
+
+// Rosen's selection among the step's trials (src/cbmc.c:2786-2840) with
+// ComputeExternalEnergies' overlap rule and LJ biasing (src/cbmc.c:1202-1215,
+// 1440-1446).
+__global__ __launch_bounds__(256) void k_chain_select(const ChainArgs a, const ChainWork w) {
+#pragma clang fp contract(off)
+  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= (int64_t)*w.n_active || !w.alive[slot]) return;
+  const ChainStep &S = *a.st;
+  const int k = a.k, np = S.n_place;
+  const int64_t cap = w.cap;
+  const double beta = a.beta;
+  auto host = [&](int t, double &e) -> bool {
+    e = 0.0;
+    for (int jj = 0; jj < np; ++jj) {
+      const double u = w.tuv[(int64_t)(jj * k + t) * cap + slot];
+      if (u >= kOverlap) return false;
+      e += u;
+    }
+    return true;
+  };
+  auto bf = [&](int t, double &b) -> bool {
+    double e;
+    if (!host(t, e)) return false;
+    b = -beta * (e + 0.0 + 0.0 + w.tintra[(int64_t)t * cap + slot]);
+    return true;
+  };
+  bool any = false;
+  double weight = 0.0;
+  for (int t = 0; t < k; ++t) {
+    double b;
+    if (bf(t, b)) {
+      any = true;
+      weight += exp(b);
+    }
+  }
+  const int64_t i = w.active[slot];
+  double *rec = w.rec ? w.rec + i * kCrecN : nullptr;
+  if (!any) {
+    w.alive[slot] = 0;
+    if (rec) {
+      rec[kCrecOverlap] = 1.0;
+      rec[kCrecStage] = 2.0;
+      rec[kCrecStep] = (double)a.step;
+    }
+    return;
+  }
+  weight /= (double)k;
+  const double u = stream_uniform(chain_substream(chain::insertion_key(a, w, slot), chain_tag(a.step, 1)), 0);
+  const int iwalk = select_trial(bf, k, u);
+  const double W = w.sW[slot] * (weight * w.tRT[(int64_t)iwalk * cap + slot]);
+  if (W < kMinRosenbluth) {
+    w.alive[slot] = 0;
+    if (rec) {
+      rec[kCrecOverlap] = 1.0;
+      rec[kCrecStage] = 3.0;
+      rec[kCrecStep] = (double)a.step;
+    }
+    return;
+  }
+  w.sW[slot] = W;
+  const int64_t o = (int64_t)iwalk * cap + slot;
+  double eh;
+  host(iwalk, eh);
+  w.ubond[slot] += w.tbond[o];
+  w.ubend[slot] += w.tbend[o];
+  w.utors[slot] += w.ttors[o];
+  w.uintra[slot] += w.tintra[o];
+  w.uhost[slot] += eh;
+  for (int jj = 0; jj < np; ++jj) {
+    const int64_t src = (int64_t)(jj * k + iwalk) * cap + slot;
+    const int64_t dst = (int64_t)S.place[jj] * cap + slot;
+    w.px[dst] = w.tx[src];
+    w.py[dst] = w.ty[src];
+    w.pz[dst] = w.tz[src];
+  }
+}
+
+// WidomAdsorbateMove after the growth (src/mc_moves.c:7880-7933): blocked
+// pockets, DeltaU in the reference's order, per-insertion results.
+__global__ __launch_bounds__(256) void k_chain_finalize(const ChainArgs a, const ChainWork w) {
+#pragma clang fp contract(off)
+  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= (int64_t)*w.n_active) return;
+  const int64_t i = w.active[slot];
+  double *rec = w.rec ? w.rec + i * kCrecN : nullptr;
+  if (rec) rec[kCrecW1] = w.W1[slot];
+  if (!w.alive[slot]) return;
+  const int64_t cap = w.cap;
+  bool blocked = false;
+  for (int j = 0; j < a.n_atoms && !blocked; ++j)
+    blocked = blocked_pocket(a.pockets, w.px[j * cap + slot], w.py[j * cap + slot], w.pz[j * cap + slot]);
+  if (w.xyz)
+    for (int j = 0; j < a.n_atoms; ++j) {
+      w.xyz[(i * a.n_atoms + j) * 3 + 0] = w.px[j * cap + slot];
+      w.xyz[(i * a.n_atoms + j) * 3 + 1] = w.py[j * cap + slot];
+      w.xyz[(i * a.n_atoms + j) * 3 + 2] = w.pz[j * cap + slot];
+    }
+  if (blocked) {
+    if (rec) {
+      rec[kCrecStage] = 5.0;
+      rec[kCrecUHost] = w.uhost[slot];
+    }
+    return;
+  }
+  const double du = w.ubond[slot] + w.ubend[slot] + w.utors[slot] + w.uintra[slot] + w.uhost[slot];
+  w.W[i] = w.sW[slot];
+  w.dU[i] = du;
+  if (rec) {
+    rec[kCrecW] = w.sW[slot];
+    rec[kCrecDU] = du;
+    rec[kCrecUHost] = w.uhost[slot];
+    rec[kCrecUIntra] = w.uintra[slot];
+    rec[kCrecUBond] = w.ubond[slot];
+    rec[kCrecUBend] = w.ubend[slot];
+    rec[kCrecUTors] = w.utors[slot];
+    rec[kCrecStage] = 4.0;
+  }
+}
+
+void launch_chain_fb_select(const ChainArgs &a, const ChainWork &w, hipStream_t st) {
+  hipLaunchKernelGGL(k_chain_fb_select, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, st, a, w);
+}
+void launch_chain_grow(const ChainArgs &a, const ChainWork &w, hipStream_t st) {
+  hipLaunchKernelGGL(k_chain_grow, dim3((unsigned)((a.n + 63) / 64)), dim3(64), 0, st, a, w);
+}
+void launch_chain_trials(const ChainArgs &a, const ChainWork &w, hipStream_t st) {
+  const int64_t waves = a.n * a.k;
+  hipLaunchKernelGGL(k_chain_trials, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, a, w);
+}
+void launch_chain_select(const ChainArgs &a, const ChainWork &w, hipStream_t st) {
+  hipLaunchKernelGGL(k_chain_select, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, st, a, w);
+}
+void launch_chain_finalize(const ChainArgs &a, const ChainWork &w, hipStream_t st) {
+  hipLaunchKernelGGL(k_chain_finalize, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, st, a, w);
+}
+} // namespace rhg

@@ -64,6 +64,72 @@ class MoleculeT(ctypes.Structure):
                 ("start", _I32), ("k_fb", _I32), ("k_or", _I32)]
 
 
+class ChainT(ctypes.Structure):
+    """rhg_chain: a flexible molecule of one flexible group."""
+    _fields_ = [("n_atoms", _I32), ("start", _I32), ("type", _P),
+                ("n_bonds", _I32), ("bonds", _P), ("bond_par", _P),
+                ("n_bends", _I32), ("bends", _P), ("bend_par", _P),
+                ("n_torsions", _I32), ("torsions", _P), ("torsion_par", _P),
+                ("n_intra_vdw", _I32), ("intra_vdw", _P), ("intra14_vdw_scaling", _D),
+                ("k_fb", _I32), ("k_trial", _I32), ("k_torsion", _I32), ("moves_per_open_bead", _I32),
+                ("max_bond_change", _P), ("max_bend_change", _P), ("max_cone_rotation", _P)]
+
+
+BOND_FIXED = 0
+BOND_HARMONIC = 1
+BEND_HARMONIC = 0
+TORSION_TRAPPE = 0
+CREC = dict(W=0, DU=1, UHOSTVDW=2, UINTRA=3, UBOND=4, UBEND=5, UTORSION=6, OVERLAP=7, STAGE=8,
+            STEP=9, W1=10)
+CREC_N = 11
+
+
+class Chain:
+    """A flexible molecule (ReadComponentDefinition, src/molecule.c:1430-3030)
+    with its CBMC settings.  bonds: (a, b, kind, p0, p1), bends: (a, b, c,
+    kind, p0, p1), torsions: (a, b, c, d, kind, p0, p1, p2, p3), intra: (a, b)
+    pairs; parameters in molecule-file units (K, A, degrees)."""
+
+    def __init__(self, types, bonds=(), bends=(), torsions=(), intra=(), start=0, k_fb=10,
+                 k_trial=10, k_torsion=100, moves_per_open_bead=150, intra14_scaling=1.0,
+                 max_bond_change=0.3, max_bend_change=0.3, max_cone_rotation=0.3):
+        n = len(types)
+        self.n_atoms = n
+        self.types = np.ascontiguousarray(types, dtype=np.int32)
+
+        def ints(rows, width):
+            a = np.zeros((max(len(rows), 1), width), dtype=np.int32)
+            for j, r in enumerate(rows):
+                a[j] = [int(v) for v in r[:width]]
+            return a
+
+        def pars(rows, skip):
+            a = np.zeros((max(len(rows), 1), 4))
+            for j, r in enumerate(rows):
+                v = [float(x) for x in r[skip:]]
+                a[j, :len(v)] = v
+            return a
+
+        self.bonds, self.bond_par = ints(bonds, 3), pars(bonds, 3)
+        self.bends, self.bend_par = ints(bends, 4), pars(bends, 4)
+        self.torsions, self.torsion_par = ints(torsions, 5), pars(torsions, 5)
+        self.intra = ints(intra, 2)
+        per = lambda v: _f64(np.broadcast_to(np.asarray(v, dtype=np.float64), (n,)).copy())
+        self.max_bond, self.max_bend, self.max_cone = per(max_bond_change), per(max_bend_change), \
+            per(max_cone_rotation)
+        t = ChainT()
+        t.n_atoms, t.start, t.type = n, start, _ptr(self.types)
+        t.n_bonds, t.bonds, t.bond_par = len(bonds), _ptr(self.bonds), _ptr(self.bond_par)
+        t.n_bends, t.bends, t.bend_par = len(bends), _ptr(self.bends), _ptr(self.bend_par)
+        t.n_torsions, t.torsions, t.torsion_par = len(torsions), _ptr(self.torsions), _ptr(self.torsion_par)
+        t.n_intra_vdw, t.intra_vdw, t.intra14_vdw_scaling = len(intra), _ptr(self.intra), intra14_scaling
+        t.k_fb, t.k_trial, t.k_torsion, t.moves_per_open_bead = k_fb, k_trial, k_torsion, moves_per_open_bead
+        t.max_bond_change, t.max_bend_change, t.max_cone_rotation = _ptr(self.max_bond), \
+            _ptr(self.max_bend), _ptr(self.max_cone)
+        self.struct = t
+        self.k_fb, self.k_trial, self.k_torsion = k_fb, k_trial, k_torsion
+
+
 class WidomBlocks(ctypes.Structure):
     _fields_ = [("count", _D * NR_BLOCKS), ("sum_w", _D * NR_BLOCKS), ("sum_wdu", _D * NR_BLOCKS),
                 ("ideal_gas", _D * NR_BLOCKS)]
@@ -99,6 +165,8 @@ def _load():
     f("rhg_framework_energy_at_positions", _P, _I64, _P, _I32, _P, _P, _P)
     f("rhg_ewald_tables", _P, _P, _P, _P)
     f("rhg_ewald_fourier_adsorbate", _P, _I64, _I32, _P, _P, _P, _P)
+    f("rhg_widom_chain_run", _P, ctypes.POINTER(ChainT), _U64, _I64, _I64, _I64, _I32,
+      ctypes.POINTER(WidomBlocks), _P, _P)
     f("rhg_widom_run", _P, ctypes.POINTER(MoleculeT), _U64, _I64, _I64, _I64, _I32,
       ctypes.POINTER(WidomBlocks), _P)
     f("rhg_widom_run_trials", _P, ctypes.POINTER(MoleculeT), _I64, _P, _P, _P, _P)
@@ -388,6 +456,20 @@ class Context:
                                       steps_per_cycle, ctypes.byref(blocks), _ptr(rec)),
                     "widom_run")
         return blocks, rec
+
+    def widom_chain_run(self, chain, seed, n_total, first=0, n=None, steps_per_cycle=20, blocks=None,
+                        records=False, positions=False):
+        """WidomAdsorbateMove of a flexible chain (GrowMolecule with the
+        internal MC scheme, src/cbmc.c:3056-3185, 1580-2482) for insertions
+        [first, first+n); returns (blocks, records or None, positions or None)."""
+        n = n_total - first if n is None else n
+        blocks = WidomBlocks() if blocks is None else blocks
+        rec = np.zeros((n, CREC_N)) if records else None
+        xyz = np.zeros((n, chain.n_atoms, 3)) if positions else None
+        self._check(lib.rhg_widom_chain_run(self.h, ctypes.byref(chain.struct), seed, n_total, first, n,
+                                            steps_per_cycle, ctypes.byref(blocks), _ptr(rec), _ptr(xyz)),
+                    "widom_chain_run")
+        return blocks, rec, xyz
 
     # ---- src/framework.c:6168-6300 blocked pockets
     def set_block_pockets(self, centres, radii, unit_box, invert=False):

@@ -109,6 +109,53 @@ N2LIKE = Probe("N2", [(36.0, 3.31, 0.0), (36.0, 3.31, 0.0)],
 HELIUM = Probe("helium", [(10.9, 2.64, 0.0)], np.zeros((1, 3)), k_fb=10)
 
 
+@dataclass
+class ChainProbe:
+    """A flexible adsorbate of one flexible group: per-atom sites (eps_k,
+    sigma, charge) and bonded terms in molecule-file units (K, A, degrees):
+    bonds (a, b, kind, p0, p1), bends (a, b, c, kind, p0, p1), torsions
+    (a, b, c, d, kind, p0..p3), intra-molecular LJ pairs (a, b)."""
+    name: str
+    sites: list
+    bonds: list
+    bends: list
+    torsions: list
+    intra: list
+    start: int = 0
+    k_fb: int = 10
+    k_trial: int = 10
+    k_torsion: int = 100
+    moves: int = 150
+
+    @property
+    def n_atoms(self):
+        return len(self.sites)
+
+
+def trappe_alkane(n, **kw):
+    """n-alkane in the TraPPE united-atom model (molecules/TraPPE/C<n>.def,
+    forcefield TraPPE CH3_sp3 98 K 3.75 A, CH2_sp3 46 K 3.95 A): harmonic
+    bonds 96500 K/A^2 at 1.54 A, harmonic bends 62500 K/rad^2 at 114 deg,
+    TRAPPE_DIHEDRAL 0, 355.03, -68.19, 791.32 K, LJ between beads four or
+    more bonds apart."""
+    ch3, ch2 = (98.0, 3.75, 0.0), (46.0, 3.95, 0.0)
+    sites = [ch3 if i in (0, n - 1) else ch2 for i in range(n)] if n > 1 else [(148.0, 3.73, 0.0)]
+    bonds = [(i, i + 1, R.BOND_HARMONIC, 96500.0, 1.54) for i in range(n - 1)]
+    bends = [(i, i + 1, i + 2, R.BEND_HARMONIC, 62500.0, 114.0) for i in range(n - 2)]
+    torsions = [(i, i + 1, i + 2, i + 3, R.TORSION_TRAPPE, 0.0, 355.03, -68.19, 791.32)
+                for i in range(n - 3)]
+    intra = [(i, j) for i in range(n) for j in range(i + 4, n)]
+    return ChainProbe(f"C{n}", sites, bonds, bends, torsions, intra, **kw)
+
+
+def register_chain(ctx, probe):
+    """Adds the chain's pseudo-atoms to a context and returns its rhg.Chain."""
+    types = [ctx.add_pseudo_atom(e, s, q) for e, s, q in probe.sites]
+    return types, R.Chain(types, probe.bonds, probe.bends, probe.torsions, probe.intra,
+                          start=probe.start, k_fb=probe.k_fb, k_trial=probe.k_trial,
+                          k_torsion=probe.k_torsion, moves_per_open_bead=probe.moves)
+
+
 def register_probe(ctx, probe):
     """Adds the probe's pseudo-atoms to a context and returns its rhg_molecule."""
     types = [ctx.add_pseudo_atom(e, s, q) for e, s, q in probe.sites]

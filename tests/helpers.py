@@ -27,6 +27,23 @@ def gpu_molecule(ctx, probe, k_fb=None, k_or=None):
                              k_or=probe.k_or if k_or is None else k_or)
 
 
+def oracle_chain(sys_o, probe, **kw):
+    """The chain probe on the oracle system (same pseudo-atom order as the GPU)."""
+    types = [sys_o.add_pseudo_atom(e, s, q) for e, s, q in probe.sites]
+    args = dict(start=probe.start, k_fb=probe.k_fb, k_trial=probe.k_trial, k_torsion=probe.k_torsion,
+                moves=probe.moves)
+    args.update(kw)
+    return types, O.chain(types, probe.bonds, probe.bends, probe.torsions, probe.intra, **args)
+
+
+def gpu_chain(ctx, probe, **kw):
+    types = [ctx.add_pseudo_atom(e, s, q) for e, s, q in probe.sites]
+    args = dict(start=probe.start, k_fb=probe.k_fb, k_trial=probe.k_trial, k_torsion=probe.k_torsion,
+                moves_per_open_bead=probe.moves)
+    args.update(kw)
+    return types, R.Chain(types, probe.bonds, probe.bends, probe.torsions, probe.intra, **args)
+
+
 def random_positions(box, n, seed):
     rng = np.random.Generator(np.random.PCG64(seed))
     s = rng.uniform(0.0, 1.0, (n, 3))

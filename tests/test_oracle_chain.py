@@ -1,0 +1,207 @@
+"""Pins of the oracle's flexible-chain CBMC (src/cbmc.c:623-795, 901-1152,
+1480-2482, 2582-2849, 3056-3185; src/internal_energy.c:48-70, 306-490,
+1168-1340, 2177-2330; src/mc_moves.c:7844-7946).
+
+The reference holds no fixture for this path and cannot be run here, so the
+restatement is pinned by what the mathematics fixes:
+* the growth plan of linear, mid-started and branched molecules (written out
+  by hand from SetGrowingStatus' rule);
+* bonded energies against their closed forms, with the dihedral built from
+  an explicit geometry (cis/trans convention, the compact TraPPE form);
+* in an empty box the Rosenbluth weight of butane is the mean of
+  exp(-beta U_torsion) over uniform dihedrals, and the weighted torsion
+  energy is its Boltzmann average (quadrature); the internal Metropolis walk
+  samples bend angles with density sin(theta) exp(-beta U) and bond lengths
+  with r^2 exp(-beta U) -- a dropped Jacobian moves the means by many
+  standard errors;
+* in a framework the record's energies equal the grown molecule's energies
+  recomputed from scratch.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import irmof1_dubbeldam, oracle_chain, oracle_system
+
+TRAPPE = (0.0, 355.03, -68.19, 791.32)
+
+
+def _empty(O, L=30.0, T=300.0):
+    box = np.diag([L, L, L]).T.reshape(-1)
+    return O.System(box, O.BC_RECTANGULAR, [], [], [], [], [], [], cutoff_vdw=12.0, temperature=T)
+
+
+def _e2k(O):
+    return O.units()["energy_to_kelvin"]
+
+
+def _trappe_u(phi, p=TRAPPE):
+    return p[0] + p[1] * (1 + np.cos(phi)) + p[2] * (1 - np.cos(2 * phi)) + p[3] * (1 + np.cos(3 * phi))
+
+
+def test_growth_plans(O, W):
+    s = _empty(O)
+    hexane = W.trappe_alkane(6)
+    _, c = oracle_chain(s, hexane)
+    plan = O.chain_plan(c)
+    assert [(p["cur"], p["prev"], p["place"]) for p in plan] == [
+        (0, -1, [1]), (1, 0, [2]), (2, 1, [3]), (3, 2, [4]), (4, 3, [5])]
+    assert [p["n_torsions"] for p in plan] == [0, 0, 1, 1, 1]
+    assert [p["n_bends"] for p in plan] == [0, 1, 1, 1, 1]
+    assert [p["n_intra"] for p in plan] == [0, 0, 0, 1, 2]
+    # started in the middle: the first placed bead's neighbours in atom order
+    _, c = oracle_chain(s, hexane, start=2)
+    plan = O.chain_plan(c)
+    assert [(p["cur"], p["prev"], p["place"]) for p in plan] == [
+        (2, -1, [1]), (2, 1, [3]), (1, 2, [0]), (3, 2, [4]), (4, 3, [5])]
+    # branched (isobutane): both methyls of the centre in one step
+    t = [s.add_pseudo_atom(98.0, 3.75)] * 4
+    iso = O.chain(t, [(0, 1, 1, 96500, 1.54), (1, 2, 1, 96500, 1.54), (1, 3, 1, 96500, 1.54)],
+                  [(0, 1, 2, 0, 62500, 112), (0, 1, 3, 0, 62500, 112), (2, 1, 3, 0, 62500, 112)])
+    plan = O.chain_plan(iso)
+    assert [(p["cur"], p["prev"], p["place"], p["n_bends"]) for p in plan] == [
+        (0, -1, [1], 0), (1, 0, [2, 3], 3)]
+    # a ring grows like a chain: the closing bond never enters a step's bond
+    # list (only bonds to the current bead do, src/cbmc.c:925-940)
+    ring = O.chain(t[:3], [(0, 1, 1, 96500, 1.54), (1, 2, 1, 96500, 1.54), (2, 0, 1, 96500, 1.54)])
+    plan = O.chain_plan(ring)
+    assert [(p["cur"], p["prev"], p["place"], p["n_bonds"]) for p in plan] == [(0, -1, [1], 1), (0, 1, [2], 1)]
+
+
+def test_bonded_energies_closed_forms(O):
+    s = _empty(O)
+    e2k = _e2k(O)
+    for r in (1.40, 1.54, 1.71):
+        u = O.chain_bond_energy(s, O.BOND_HARMONIC, (96500.0, 1.54), r)
+        assert u * e2k == pytest.approx(0.5 * 96500.0 * (r - 1.54) ** 2, rel=1e-12, abs=1e-12)
+        assert O.chain_bond_energy(s, O.BOND_FIXED, (1.54,), r) == 0.0
+    for th in (100.0, 114.0, 131.5):
+        t = math.radians(th)
+        a, b, c = (1.54, 0.0, 0.0), (0.0, 0.0, 0.0), (1.54 * math.cos(t), 1.54 * math.sin(t), 0.0)
+        u = O.chain_bend_energy(s, (62500.0, 114.0), a, b, c) * e2k
+        assert u == pytest.approx(0.5 * 62500.0 * (t - math.radians(114.0)) ** 2, rel=1e-9, abs=1e-9)
+    # dihedral phi between a (from b) and d (from c) about the b-c axis;
+    # phi = 0 is cis, pi is trans (protein convention)
+    rng = np.random.default_rng(5)
+    for phi in [0.0, math.pi, math.pi / 3, -math.pi / 3] + list(rng.uniform(-math.pi, math.pi, 8)):
+        b, c = np.zeros(3), np.array([0.3, -0.2, 1.5])
+        axis = (c - b) / np.linalg.norm(c - b)
+        e1 = np.cross(axis, [1.0, 0.0, 0.0])
+        e1 /= np.linalg.norm(e1)
+        e2 = np.cross(axis, e1)
+        a = b + 1.2 * e1 - 0.4 * axis
+        d = c + 1.1 * (math.cos(phi) * e1 + math.sin(phi) * e2) + 0.5 * axis
+        u = O.chain_torsion_energy(s, TRAPPE, a, b, c, d) * e2k
+        assert u == pytest.approx(_trappe_u(phi), rel=1e-9, abs=1e-8)
+    assert _trappe_u(math.pi) == pytest.approx(TRAPPE[0], abs=1e-12)
+
+
+def _quad_torsion(T=300.0):
+    phi = np.linspace(-math.pi, math.pi, 400001)
+    u = _trappe_u(phi)
+    w = np.exp(-u / T)
+    return w.mean(), (u * w).mean() / w.mean()
+
+
+def test_empty_box_butane_weights(O, W):
+    """<W> and the W-weighted torsion energy against quadrature.  With
+    k_trial = k_torsion the torsion weight averages every torsion trial, which
+    makes the weighted average exact."""
+    s = _empty(O)
+    _, c = oracle_chain(s, W.trappe_alkane(4), k_trial=10, k_torsion=10)
+    rec = O.chain_run_stream(s, c, 7, 0, 4000)
+    assert np.all(rec[:, O.CREC["STAGE"]] == 4)
+    w = rec[:, O.CREC["W"]]
+    wq, uq = _quad_torsion()
+    assert abs(w.mean() - wq) < 4.0 * w.std() / math.sqrt(len(w))
+    ut = rec[:, O.CREC["UTORSION"]] * _e2k(O)
+    est = (w * ut).sum() / w.sum()
+    # jackknife-free error bound from the per-insertion spread
+    err = np.std(w * (ut - est)) / w.mean() / math.sqrt(len(w))
+    assert abs(est - uq) < 4.0 * err
+    # the reference-order MT19937-64 draws sample the same weight
+    rec = O.chain_run_mt(s, c, 11, 4000)
+    w = rec[:, O.CREC["W"]]
+    assert abs(w.mean() - wq) < 4.0 * w.std() / math.sqrt(len(w))
+
+
+def _angle(a, b, c):
+    u, v = a - b, c - b
+    return np.arccos(np.sum(u * v, -1) / np.linalg.norm(u, axis=-1) / np.linalg.norm(v, axis=-1))
+
+
+def test_empty_box_bend_and_bond_sampling(O, W):
+    """Propane: the bend angle has density sin(theta) exp(-beta U_bend) and
+    both bonds r^2 exp(-beta U_bond) (the generated first bond and the walk's
+    bond moves); the same means without the Jacobians are excluded."""
+    s = _empty(O)
+    _, c = oracle_chain(s, W.trappe_alkane(3))
+    rec, xyz = O.chain_run_stream(s, c, 3, 0, 40000, positions=True)
+    assert np.all(rec[:, O.CREC["STAGE"]] == 4)
+    th = _angle(xyz[:, 0], xyz[:, 1], xyz[:, 2])
+    g = np.linspace(1.5, 2.5, 400001)
+    boltz = np.exp(-0.5 * 62500.0 * (g - math.radians(114.0)) ** 2 / 300.0)
+    with_j = (g * np.sin(g) * boltz).sum() / (np.sin(g) * boltz).sum()
+    without = (g * boltz).sum() / boltz.sum()
+    err = th.std() / math.sqrt(len(th))
+    assert abs(th.mean() - with_j) < 3.0 * err
+    assert abs(th.mean() - without) > 3.0 * err
+    r = np.linspace(1.2, 1.9, 400001)
+    boltz = np.exp(-0.5 * 96500.0 * (r - 1.54) ** 2 / 300.0)
+    with_j = (r ** 3 * boltz).sum() / (r ** 2 * boltz).sum()
+    without = (r * boltz).sum() / boltz.sum()
+    for a, b in ((0, 1), (1, 2)):
+        bl = np.linalg.norm(xyz[:, b] - xyz[:, a], axis=-1)
+        err = bl.std() / math.sqrt(len(bl))
+        assert abs(bl.mean() - with_j) < 3.0 * err
+        assert abs(bl.mean() - without) > 3.0 * err
+    # bonded energies of the record are those of the grown molecule
+    e2k = _e2k(O)
+    ub = sum(0.5 * 96500.0 * (np.linalg.norm(xyz[:, i + 1] - xyz[:, i], axis=-1) - 1.54) ** 2 for i in range(2))
+    assert np.allclose(rec[:, O.CREC["UBOND"]] * e2k, ub, rtol=1e-9, atol=1e-9)
+    ube = 0.5 * 62500.0 * (th - math.radians(114.0)) ** 2
+    assert np.allclose(rec[:, O.CREC["UBEND"]] * e2k, ube, rtol=1e-7, atol=1e-7)
+
+
+def test_framework_bookkeeping(O, W):
+    """Hexane in IRMOF-1: every accepted record's host, intra, bonded
+    energies and DeltaU equal those of its grown molecule recomputed from
+    scratch; rejected insertions carry W = 0."""
+    fw = irmof1_dubbeldam()[0]
+    s = oracle_system(fw)
+    types, c = oracle_chain(s, W.trappe_alkane(6))
+    rec, xyz = O.chain_run_stream(s, c, 5, 0, 120, positions=True)
+    ok = rec[:, O.CREC["STAGE"]] == 4
+    assert 10 < ok.sum() <= len(rec)
+    assert np.all(rec[~ok, O.CREC["W"]] == 0.0)
+    e2k = _e2k(O)
+    for i in np.flatnonzero(ok):
+        x = xyz[i]
+        uh = sum(s.vdw_at(types[a], x[a]) for a in range(6))
+        assert rec[i, O.CREC["UHOSTVDW"]] == pytest.approx(uh, rel=1e-12, abs=1e-9)
+        ui = 0.0
+        for a, b in W.trappe_alkane(6).intra:
+            eps, sg, shift = [v for v in _pair(O, s, types[a], types[b])]
+            rr = float(np.sum((x[a] - x[b]) ** 2))
+            if rr < 144.0:
+                t3 = (sg * sg / rr) ** 3
+                ui += 4.0 * eps * (t3 * (t3 - 1.0)) - shift
+        assert rec[i, O.CREC["UINTRA"]] == pytest.approx(ui, rel=1e-10, abs=1e-9)
+        ut = sum(O.chain_torsion_energy(s, TRAPPE, x[j], x[j + 1], x[j + 2], x[j + 3]) for j in range(3))
+        assert rec[i, O.CREC["UTORSION"]] == pytest.approx(ut, rel=1e-9, abs=1e-9)
+        du = (rec[i, O.CREC["UBOND"]] + rec[i, O.CREC["UBEND"]] + rec[i, O.CREC["UTORSION"]]
+              + rec[i, O.CREC["UINTRA"]] + rec[i, O.CREC["UHOSTVDW"]])
+        assert rec[i, O.CREC["DU"]] == du
+        assert rec[i, O.CREC["W"]] > 0.0 and rec[i, O.CREC["W1"]] > 0.0
+
+
+def _pair(O, s, ta, tb):
+    """LB-mixed LJ of two pseudo-atoms, shifted at 12 A (src/potentials.c:3164-3171, 5263-5285)."""
+    k2e = O.units()["kelvin_to_energy"]
+    ea, sa = s._pseudo[ta]
+    eb, sb = s._pseudo[tb]
+    e = math.sqrt((ea * k2e) * (eb * k2e))
+    sg = 0.5 * (sa + sb)
+    t3 = (sg * sg / 144.0) ** 3
+    return e, sg, 4.0 * e * (t3 * (t3 - 1.0))
