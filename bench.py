@@ -486,6 +486,93 @@ def loaded_bench(fw, probe, seed=SEED + 3, n_ads=100):
             "retrace_accepted_fraction": float(np.mean(rec[:, 13] == 4))}
 
 
+def chain_bench(no_cpu=False, n=1 << 18, steps=3, cpu_sample=1500):
+    """n-hexane (TraPPE united atom) Widom insertion by CBMC with the internal
+    MC scheme in IRMOF-1 (the reference's CIF in P1 with the
+    Dubbeldam2007FlexibleIRMOF-1 LJ parameters, 424 atoms, 25.83 A cubic,
+    from the committed fixtures), LJ 12 A shifted, 300 K, mixed precision,
+    k_fb = 10, k_trial = 10, k_torsion = 100, 150 moves per open bead.  The
+    ideal-gas Rosenbluth weight comes from the same growth in the empty cell.
+    The oracle times a bounded sample of the same insertions on one core, and
+    the FP64 GPU path is compared with it insertion by insertion."""
+    import numpy as np
+    import oracle as O
+    from raspa2_amd import rhg as R
+    from raspa2_amd import workloads as W
+    fw, _, _ = W.irmof1_dubbeldam(os.path.join(ROOT, "tests", "golden"))
+    probe = W.trappe_alkane(6)
+    ctx = fw.context(precision=R.PREC_MIXED)
+    _, ch = W.register_chain(ctx, probe)
+    total = (steps + 1) * n
+    ctx.widom_chain_run(ch, SEED, total, first=0, n=n)
+    ctx.synchronize()
+    ctx.profile(True)
+    ctx.profile_reset()
+    blocks = R.WidomBlocks()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        ctx.widom_chain_run(ch, SEED, total, first=(s + 1) * n, n=n, blocks=blocks)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    kernels = {}
+    for name in ("chain_grow", "chain_trials", "site_energy", "bin", "chain_select", "widom_gen",
+                 "block_reduce"):
+        cnt, ms = ctx.profile_get(name)
+        kernels[name] = {"launches": cnt, "total_ms": round(ms, 4),
+                         "avg_ms": round(ms / cnt, 5) if cnt else None}
+    ctx.profile(False)
+    e = np.zeros(0)
+    empty = R.Context(fw.box, fw.bc, e, e, e, e, e, charge=e, mass=e, precision=R.PREC_FP64)
+    _, che = W.register_chain(empty, probe)
+    bi, _, _ = empty.widom_chain_run(che, SEED + 1, 1 << 18)
+    b = bi.as_array()
+    w_ig = b[:, 1].sum() / b[:, 0].sum()
+    empty.close()
+    est = R.widom_estimators(blocks, 300.0, ctx.info()["density"], ideal_gas_weight=w_ig)
+    # plain FP64 GPU rate and per-insertion parity with the oracle on a sample
+    ref = fw.context(precision=R.PREC_FP64)
+    _, ch64 = W.register_chain(ref, probe)
+    ref.widom_chain_run(ch64, SEED, total, first=n, n=n)
+    ref.synchronize()
+    t2 = time.perf_counter()
+    ref.widom_chain_run(ch64, SEED, total, first=n, n=n)
+    ref.synchronize()
+    plain = n / (time.perf_counter() - t2)
+    out = {"workload": "n-hexane (TraPPE UA, 6 beads) CBMC Widom insertion in IRMOF-1 (424 atoms, "
+                       "25.83 A cubic), LJ 12 A shifted, 300 K, mixed precision, k_fb=10, k_trial=10, "
+                       "k_torsion=100, 150 internal moves per open bead",
+           "insertions_per_step": n, "steps": steps,
+           "insertions_per_s": round(steps * n / elapsed, 1),
+           "fp64_plain_gpu_insertions_per_s": round(plain, 1),
+           "kernels": kernels,
+           "ideal_gas_rosenbluth_weight": w_ig,
+           "estimators": {"henry_mol_kg_pa": est["kh_overall"], "u_gh_K": est["u"],
+                          "w": est["w_overall"]}}
+    if not no_cpu:
+        s_o = O.System(*fw.oracle_args()[:2], *fw.oracle_args()[2:8], mass=fw.mass, cutoff_vdw=12.0,
+                       shifted=True, temperature=300.0)
+        types = [s_o.add_pseudo_atom(ep, sg, q) for ep, sg, q in probe.sites]
+        co = O.chain(types, probe.bonds, probe.bends, probe.torsions, probe.intra, start=probe.start,
+                     k_fb=probe.k_fb, k_trial=probe.k_trial, k_torsion=probe.k_torsion, moves=probe.moves)
+        t3 = time.perf_counter()
+        rec_o = O.chain_run_stream(s_o, co, SEED, n, cpu_sample)
+        cpu_rate = cpu_sample / (time.perf_counter() - t3)
+        _, rec_g, _ = ref.widom_chain_run(ch64, SEED, total, first=n, n=cpu_sample, records=True)
+        st_o, st_g = rec_o[:, O.CREC["STAGE"]], rec_g[:, R.CREC["STAGE"]]
+        ok = (st_o == 4) & (st_g == 4)
+        werr = float(np.max(np.abs(rec_g[ok, 0] - rec_o[ok, 0]) / rec_o[ok, 0])) if ok.any() else None
+        out["cpu_baseline"] = {"value": round(cpu_rate, 2), "unit": "insertions/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{cpu_sample} insertions of the same workload (insertions "
+                                         f"{n}..{n + cpu_sample - 1} of the job's counter stream), oracle "
+                                         "FP64 C restatement, one core"}
+        out["cpu_parity_fp64"] = {"insertions": cpu_sample, "stages_identical": bool(np.array_equal(st_o, st_g)),
+                                  "max_rel_err_W": werr}
+    ref.close()
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -495,7 +582,7 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--cpu-sample", type=int, default=400, help="insertions per CPU process")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4"], default="c1")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "chain"], default="c1")
     ap.add_argument("--grid-samples", type=int, default=100_000, help="c2 off-grid checks")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 512 MiB device copy kernel (k_stream_copy) three "
@@ -512,6 +599,19 @@ def main():
 
     if args.workload == "c2":
         grid_bench(args)
+        return
+    if args.workload == "chain":
+        n = min(args.insertions, 1 << 20)
+        t0 = time.perf_counter()
+        r = chain_bench(no_cpu=args.no_cpu, n=n, steps=args.steps)
+        print(json.dumps({
+            "metric": METRIC, "value": r["insertions_per_s"], "unit": "insertions/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": 1, "ms_per_step": round(n / r["insertions_per_s"] * 1e3, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "fp32 pair terms, fp64 growth/accumulation", "data": "IRMOF-1 fixture, synthetic trials",
+            "config": {"workload": r["workload"], "insertions_per_step": n},
+            "cpu_baseline": r.get("cpu_baseline"), "detail": r,
+            "wall_s": round(time.perf_counter() - t0, 2)}))
         return
     if args.workload != "c1":
         screening_bench(args)
@@ -660,6 +760,7 @@ def main():
             "executed": round((st["exec_lj_pairs"] + st["exec_ljcc_pairs"] + st["exec_cc_pairs"]) / elapsed, 1)},
         "kernels": kernels,
         "finite_loading": loaded_bench(fw, probe),
+        "flexible_chain": chain_bench(no_cpu=args.no_cpu),
     }
     print(json.dumps(out))
 

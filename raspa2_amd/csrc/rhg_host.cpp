@@ -1799,11 +1799,11 @@ int chain_chunk(rhg_ctx *c, const rhg_chain *ch, const std::vector<ChainStep> &s
     a.st = d_steps + s;
     {
       Timed t(c, "chain_grow");
-      launch_chain_grow(a, w, st);
+      launch_chain_grow(a, w, S, st);
     }
     {
       Timed t(c, "chain_trials");
-      launch_chain_trials(a, w, st);
+      launch_chain_trials(a, w, S, st);
     }
     // framework VDW of every trial, one site list per placed bead
     for (int jj = 0; jj < S.n_place; ++jj) {

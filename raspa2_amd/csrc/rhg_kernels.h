@@ -401,8 +401,8 @@ struct ChainArgs {
 };
 
 void launch_chain_fb_select(const ChainArgs &a, const ChainWork &w, hipStream_t st);
-void launch_chain_grow(const ChainArgs &a, const ChainWork &w, hipStream_t st);
-void launch_chain_trials(const ChainArgs &a, const ChainWork &w, hipStream_t st);
+void launch_chain_grow(const ChainArgs &a, const ChainWork &w, const ChainStep &h, hipStream_t st);
+void launch_chain_trials(const ChainArgs &a, const ChainWork &w, const ChainStep &h, hipStream_t st);
 void launch_chain_select(const ChainArgs &a, const ChainWork &w, hipStream_t st);
 void launch_chain_finalize(const ChainArgs &a, const ChainWork &w, hipStream_t st);
 

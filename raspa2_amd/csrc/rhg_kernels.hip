@@ -2758,72 +2758,133 @@ __device__ double bond_length(Draw &d, double beta, int kind, double p0, double 
 }
 
 // The positions one lane works on: the beads placed by the step (0..3), the
-// current and previous beads, and any other placed atom read from the
-// grown molecule.
+// current and previous beads in registers, and any other placed atom read
+// from the grown molecule.  Codes come from the step table (wave-uniform),
+// so get() resolves them with scalar branches and constant register indices.
+template <int NP>
 struct Local {
-  double p[6][3];
+  double p[NP][3];
+  double c[3], v[3];
   const ChainWork *w;
   int64_t slot;
-  __device__ void get(int code, double out[3]) const {
-    if (code < kLocGlobal) {
-      out[0] = p[code][0];
-      out[1] = p[code][1];
-      out[2] = p[code][2];
-    } else {
+  __device__ __forceinline__ void get(int code, double out[3]) const {
+    if (code >= kLocGlobal) {
       const int64_t o = (int64_t)(code - kLocGlobal) * w->cap + slot;
       out[0] = w->px[o];
       out[1] = w->py[o];
       out[2] = w->pz[o];
+    } else if (code == kLocCur) {
+      out[0] = c[0];
+      out[1] = c[1];
+      out[2] = c[2];
+    } else if (code == kLocPrev) {
+      out[0] = v[0];
+      out[1] = v[1];
+      out[2] = v[2];
+    } else {
+#pragma unroll
+      for (int j = 0; j < NP; ++j)
+        if (code == j) {
+          out[0] = p[j][0];
+          out[1] = p[j][1];
+          out[2] = p[j][2];
+        }
     }
   }
-  __device__ void load(int atom, int code) {
+  __device__ __forceinline__ void load(int atom, double out[3]) const {
     const int64_t o = (int64_t)atom * w->cap + slot;
-    p[code][0] = w->px[o];
-    p[code][1] = w->py[o];
-    p[code][2] = w->pz[o];
+    out[0] = w->px[o];
+    out[1] = w->py[o];
+    out[2] = w->pz[o];
   }
 };
 
-__device__ double sum_bonds(const ChainStep &S, const Local &L) {
-#pragma clang fp contract(off)
-  double u = 0.0;
-  for (int j = 0; j < S.n_bonds; ++j) {
-    double a[3], b[3];
-    L.get(S.bond_a[j], a);
-    L.get(S.bond_b[j], b);
-    u += bond_u(S.bond_kind[j], S.bond_p0[j], S.bond_p1[j], a, b);
-  }
-  return u;
+// register arrays indexed by a per-lane value: compare-and-select over the
+// constant indices keeps them out of scratch memory
+template <int N>
+__device__ __forceinline__ void pick3(const double (&a)[N][3], int j, double out[3]) {
+#pragma unroll
+  for (int q = 0; q < N; ++q)
+    if (j == q) {
+      out[0] = a[q][0];
+      out[1] = a[q][1];
+      out[2] = a[q][2];
+    }
 }
-__device__ double bend_j(const ChainStep &S, const Local &L, int j) {
+template <int N>
+__device__ __forceinline__ void put3(double (&a)[N][3], int j, const double in[3]) {
+#pragma unroll
+  for (int q = 0; q < N; ++q)
+    if (j == q) {
+      a[q][0] = in[0];
+      a[q][1] = in[1];
+      a[q][2] = in[2];
+    }
+}
+template <int N>
+__device__ __forceinline__ double pick1(const double (&a)[N], int j) {
+  double r = 0.0;
+#pragma unroll
+  for (int q = 0; q < N; ++q)
+    if (j == q) r = a[q];
+  return r;
+}
+template <int N>
+__device__ __forceinline__ void put1(double (&a)[N], int j, double v) {
+#pragma unroll
+  for (int q = 0; q < N; ++q)
+    if (j == q) a[q] = v;
+}
+
+template <class LT>
+__device__ __forceinline__ double bond_j(const ChainStep &S, const LT &L, int j) {
+  double a[3], b[3];
+  L.get(S.bond_a[j], a);
+  L.get(S.bond_b[j], b);
+  return bond_u(S.bond_kind[j], S.bond_p0[j], S.bond_p1[j], a, b);
+}
+template <class LT>
+__device__ __forceinline__ double bend_j(const ChainStep &S, const LT &L, int j) {
   double a[3], b[3], c[3];
   L.get(S.bend_a[j], a);
   L.get(S.bend_b[j], b);
   L.get(S.bend_c[j], c);
   return bend_u(S.bend_p0[j], S.bend_p1[j], a, b, c);
 }
-__device__ double sum_bends(const ChainStep &S, const Local &L) {
+template <class LT>
+__device__ double sum_bonds(const ChainStep &S, const LT &L) {
+#pragma clang fp contract(off)
+  double u = 0.0;
+  for (int j = 0; j < S.n_bonds; ++j) u += bond_j(S, L, j);
+  return u;
+}
+template <class LT>
+__device__ double sum_bends(const ChainStep &S, const LT &L) {
 #pragma clang fp contract(off)
   double u = 0.0;
   for (int j = 0; j < S.n_bends; ++j) u += bend_j(S, L, j);
   return u;
 }
-__device__ double sum_torsions(const ChainStep &S, const Local &L) {
+template <class LT>
+__device__ double torsion_j(const ChainStep &S, const LT &L, int j) {
+  double a[3], b[3], c[3], d[3];
+  L.get(S.tors_a[j], a);
+  L.get(S.tors_b[j], b);
+  L.get(S.tors_c[j], c);
+  L.get(S.tors_d[j], d);
+  return torsion_u(S.tors_p[j], a, b, c, d);
+}
+template <class LT>
+__device__ double sum_torsions(const ChainStep &S, const LT &L) {
 #pragma clang fp contract(off)
   double u = 0.0;
-  for (int j = 0; j < S.n_tors; ++j) {
-    double a[3], b[3], c[3], d[3];
-    L.get(S.tors_a[j], a);
-    L.get(S.tors_b[j], b);
-    L.get(S.tors_c[j], c);
-    L.get(S.tors_d[j], d);
-    u += torsion_u(S.tors_p[j], a, b, c, d);
-  }
+  for (int j = 0; j < S.n_tors; ++j) u += torsion_j(S, L, j);
   return u;
 }
 // intra-molecular LJ of the step's pairs (src/cbmc.c:1294-1311): plain
 // distance, cut-off CutOffVDW, scaling times PotentialValue
-__device__ double sum_intra(const ChainStep &S, const Local &L, double rc2) {
+template <class LT>
+__device__ double sum_intra(const ChainStep &S, const LT &L, double rc2) {
 #pragma clang fp contract(off)
   double u = 0.0;
   for (int j = 0; j < S.n_intra; ++j) {
@@ -2834,6 +2895,88 @@ __device__ double sum_intra(const ChainStep &S, const Local &L, double rc2) {
     if (rr < rc2) u += S.intra_scal[j] * lj_exact(S.intra_c4[j], S.intra_s2[j], S.intra_shift[j], rr);
   }
   return u;
+}
+
+// A torsion whose only moving atom (one placed by the step) is an end atom:
+// the terms of CalculateTorsionEnergy that involve the three fixed atoms are
+// computed once per trial (the same operations, so the same values), and
+// each rotation evaluates the moving half only.
+struct TorsionHalf {
+  int mode;      // 0: generic, 1: a moves, 2: d moves
+  int mover;     // local slot of the moving atom
+  double dbc[3]; // normalised c - b
+  double fix[3]; // normalised perpendicular part of the fixed end
+  double q[3];   // the fixed atom adjacent to the moving one (b for a, c for d)
+};
+
+template <class LT>
+__device__ __forceinline__ void torsion_prepare(const ChainStep &S, const LT &L, int j, TorsionHalf &h) {
+#pragma clang fp contract(off)
+  const int ca = S.tors_a[j], cb = S.tors_b[j], cc = S.tors_c[j], cd = S.tors_d[j];
+  const bool ma = ca < kLocCur, mb = cb < kLocCur, mc = cc < kLocCur, md = cd < kLocCur;
+  h.mode = 0;
+  if (mb || mc || (ma && md) || (!ma && !md)) return;
+  double b[3], c[3];
+  L.get(cb, b);
+  L.get(cc, c);
+  h.dbc[0] = c[0] - b[0];
+  h.dbc[1] = c[1] - b[1];
+  h.dbc[2] = c[2] - b[2];
+  const double rbc = sqrt(h.dbc[0] * h.dbc[0] + h.dbc[1] * h.dbc[1] + h.dbc[2] * h.dbc[2]);
+  h.dbc[0] /= rbc;
+  h.dbc[1] /= rbc;
+  h.dbc[2] /= rbc;
+  double e[3], f[3];
+  if (md) {
+    // fixed a: Dab and dr
+    h.mode = 2;
+    h.mover = cd;
+    L.get(ca, e);
+    f[0] = e[0] - b[0];
+    f[1] = e[1] - b[1];
+    f[2] = e[2] - b[2];
+    h.q[0] = c[0];
+    h.q[1] = c[1];
+    h.q[2] = c[2];
+  } else {
+    // fixed d: Dcd and ds
+    h.mode = 1;
+    h.mover = ca;
+    L.get(cd, e);
+    f[0] = e[0] - c[0];
+    f[1] = e[1] - c[1];
+    f[2] = e[2] - c[2];
+    h.q[0] = b[0];
+    h.q[1] = b[1];
+    h.q[2] = b[2];
+  }
+  const double dot = f[0] * h.dbc[0] + f[1] * h.dbc[1] + f[2] * h.dbc[2];
+  h.fix[0] = f[0] - dot * h.dbc[0];
+  h.fix[1] = f[1] - dot * h.dbc[1];
+  h.fix[2] = f[2] - dot * h.dbc[2];
+  const double r = sqrt(h.fix[0] * h.fix[0] + h.fix[1] * h.fix[1] + h.fix[2] * h.fix[2]);
+  h.fix[0] /= r;
+  h.fix[1] /= r;
+  h.fix[2] /= r;
+}
+
+// CalculateTorsionEnergy from a prepared half (TRAPPE_DIHEDRAL)
+__device__ __forceinline__ double torsion_half(const double *par, const TorsionHalf &h, const double m[3]) {
+#pragma clang fp contract(off)
+  // the moving end: Dab = a - b (mode 1) or Dcd = d - c (mode 2)
+  const double g[3] = {m[0] - h.q[0], m[1] - h.q[1], m[2] - h.q[2]};
+  const double dot = g[0] * h.dbc[0] + g[1] * h.dbc[1] + g[2] * h.dbc[2];
+  double u[3] = {g[0] - dot * h.dbc[0], g[1] - dot * h.dbc[1], g[2] - dot * h.dbc[2]};
+  const double r = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  u[0] /= r;
+  u[1] /= r;
+  u[2] /= r;
+  // CosPhi = dr . ds with dr from the a end
+  double cp = (h.mode == 1) ? (u[0] * h.fix[0] + u[1] * h.fix[1] + u[2] * h.fix[2])
+                            : (h.fix[0] * u[0] + h.fix[1] * u[1] + h.fix[2] * u[2]);
+  const double mm = fabs(cp) <= 1.0 ? fabs(cp) : 1.0;
+  cp = cp >= 0.0 ? mm : -mm;
+  return par[0] + (1.0 + cp) * (par[1] + par[3] - 2.0 * (cp - 1.0) * (par[2] - 2.0 * par[3] * cp));
 }
 
 __device__ __forceinline__ uint64_t insertion_key(const ChainArgs &a, const ChainWork &w, int64_t slot) {
@@ -2922,17 +3065,19 @@ __global__ __launch_bounds__(256) void k_chain_fb_select(const ChainArgs a, cons
 // Bond lengths of a step (GenerateBondlength); for a step with a previous
 // bead the internal Metropolis walk of GenerateTrialOrientationsMCScheme
 // (src/cbmc.c:1580-2350) without a stored configuration, on trial 0.
+template <int NPM>
 __global__ __launch_bounds__(64) void k_chain_grow(const ChainArgs a, const ChainWork w) {
 #pragma clang fp contract(off)
+  constexpr int NBE = NPM == 1 ? 2 : kChainBends;
   const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= (int64_t)*w.n_active || !w.alive[slot]) return;
   const ChainStep &S = *a.st;
   const double beta = a.beta;
   chain::Draw d{chain_substream(chain::insertion_key(a, w, slot), chain_tag(a.step, 0)), 0};
-  chain::Local L;
+  chain::Local<NPM> L;
   L.w = &w;
   L.slot = slot;
-  L.load(S.cur, kLocCur);
+  L.load(S.cur, L.c);
   if (S.n_prev == 0) {
     double bl0 = 0.0;
     for (int j = 0; j < S.n_bonds; ++j) {
@@ -2942,42 +3087,50 @@ __global__ __launch_bounds__(64) void k_chain_grow(const ChainArgs a, const Chai
     w.bl[slot] = bl0;
     return;
   }
-  L.load(S.prev, kLocPrev);
+  L.load(S.prev, L.v);
   for (int j = 0; j < S.n_bonds; ++j) (void)chain::bond_length(d, beta, S.bond_kind[j], S.bond_p0[j], S.bond_p1[j]);
-  const double *cur = L.p[kLocCur];
-  double pv[3] = {L.p[kLocPrev][0] - cur[0], L.p[kLocPrev][1] - cur[1], L.p[kLocPrev][2] - cur[2]};
+  const double *cur = L.c;
+  double pv[3] = {L.v[0] - cur[0], L.v[1] - cur[1], L.v[2] - cur[2]};
   const double nrm = sqrt(pv[0] * pv[0] + pv[1] * pv[1] + pv[2] * pv[2]);
   pv[0] /= nrm;
   pv[1] /= nrm;
   pv[2] /= nrm;
   const int nb = S.n_bonds, nbe = S.n_bends, np = S.n_place;
-  for (int jj = 0; jj < np; ++jj) {
-    double v[3];
-    chain::cone(d, pv, 120 * (M_PI / 180.0), v);
-    for (int q = 0; q < 3; ++q) L.p[jj][q] = cur[q] + S.ref_len[jj] * v[q];
-  }
-  double eobond[kChainBonds], enbond[kChainBonds], eobend[kChainBends], enbend[kChainBends];
-  auto bond_e = [&](int j) {
-    double pa[3], pb[3];
-    L.get(S.bond_a[j], pa);
-    L.get(S.bond_b[j], pb);
-    return chain::bond_u(S.bond_kind[j], S.bond_p0[j], S.bond_p1[j], pa, pb);
-  };
-  for (int j = 0; j < nb; ++j) eobond[j] = enbond[j] = bond_e(j);
-  for (int j = 0; j < nbe; ++j) eobend[j] = enbend[j] = chain::bend_j(S, L, j);
-  const int trialmove = 4 * a.moves * np;
-  if ((nb >= 2) && (d.next() < 0.5)) {
-    double cA[3], cC[3];
-    for (int q = 0; q < 3; ++q) {
-      cA[q] = L.p[0][q] - cur[q];
-      cC[q] = L.p[1][q] - cur[q];
+#pragma unroll
+  for (int jj = 0; jj < NPM; ++jj)
+    if (jj < np) {
+      double v[3];
+      chain::cone(d, pv, 120 * (M_PI / 180.0), v);
+      for (int q = 0; q < 3; ++q) L.p[jj][q] = cur[q] + S.ref_len[jj] * v[q];
     }
-    double tmp = sqrt(cA[0] * cA[0] + cA[1] * cA[1] + cA[2] * cA[2]) / sqrt(cC[0] * cC[0] + cC[1] * cC[1] + cC[2] * cC[2]);
-    for (int q = 0; q < 3; ++q) L.p[0][q] = cur[q] + tmp * cC[q];
-    tmp = 1.0 / tmp;
-    for (int q = 0; q < 3; ++q) L.p[1][q] = cur[q] + tmp * cA[q];
-    for (int j = 0; j < nb; ++j) eobond[j] = enbond[j] = bond_e(j);
-    for (int j = 0; j < nbe; ++j) eobend[j] = enbend[j] = chain::bend_j(S, L, j);
+  double eobond[NPM], enbond[NPM], eobend[NBE], enbend[NBE];
+#pragma unroll
+  for (int j = 0; j < NPM; ++j)
+    if (j < nb) eobond[j] = enbond[j] = chain::bond_j(S, L, j);
+#pragma unroll
+  for (int j = 0; j < NBE; ++j)
+    if (j < nbe) eobend[j] = enbend[j] = chain::bend_j(S, L, j);
+  const int trialmove = 4 * a.moves * np;
+  // no chiral centres: two branches swap places with probability 1/2
+  // (src/cbmc.c:1814-1838); steps placing one bead have a single bond
+  if constexpr (NPM >= 2) {
+    if ((nb >= 2) && (d.next() < 0.5)) {
+      double cA[3], cC[3];
+      for (int q = 0; q < 3; ++q) {
+        cA[q] = L.p[0][q] - cur[q];
+        cC[q] = L.p[1][q] - cur[q];
+      }
+      double tmp = sqrt(cA[0] * cA[0] + cA[1] * cA[1] + cA[2] * cA[2]) / sqrt(cC[0] * cC[0] + cC[1] * cC[1] + cC[2] * cC[2]);
+      for (int q = 0; q < 3; ++q) L.p[0][q] = cur[q] + tmp * cC[q];
+      tmp = 1.0 / tmp;
+      for (int q = 0; q < 3; ++q) L.p[1][q] = cur[q] + tmp * cA[q];
+  #pragma unroll
+      for (int j = 0; j < NPM; ++j)
+        if (j < nb) eobond[j] = enbond[j] = chain::bond_j(S, L, j);
+  #pragma unroll
+      for (int j = 0; j < NBE; ++j)
+        if (j < nbe) eobend[j] = enbend[j] = chain::bend_j(S, L, j);
+    }
   }
   for (int jjj = 0; jjj < trialmove; ++jjj) {
     int choise;
@@ -2989,8 +3142,12 @@ __global__ __launch_bounds__(64) void k_chain_grow(const ChainArgs a, const Chai
       const double tmp = d.next();
       choise = tmp < 0.2 ? 1 : (tmp < 0.6 ? 2 : 3);
     }
-    for (int j = 0; j < nb; ++j) enbond[j] = eobond[j];
-    for (int j = 0; j < nbe; ++j) enbend[j] = eobend[j];
+#pragma unroll
+    for (int j = 0; j < NPM; ++j)
+      if (j < nb) enbond[j] = eobond[j];
+#pragma unroll
+    for (int j = 0; j < NBE; ++j)
+      if (j < nbe) enbend[j] = eobend[j];
     double newbond = 0.0, newbend = 0.0;
     if (choise == 1) {
       if (nb > 0) {
@@ -2998,68 +3155,88 @@ __global__ __launch_bounds__(64) void k_chain_grow(const ChainArgs a, const Chai
         if (jj >= nb) jj = nb - 1; // u = 1 (closed interval) would index past the table
         if (S.bond_kind[jj] != 0) {
           const int jo = S.bond_branch[jj];
-          const double store[3] = {L.p[jo][0], L.p[jo][1], L.p[jo][2]};
-          const double vec[3] = {L.p[jo][0] - cur[0], L.p[jo][1] - cur[1], L.p[jo][2] - cur[2]};
+          double store[3];
+          chain::pick3(L.p, jo, store);
+          const double vec[3] = {store[0] - cur[0], store[1] - cur[1], store[2] - cur[2]};
           const double blo = sqrt(vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2]);
           const double bln = blo + (2.0 * d.next() - 1.0) * S.max_bond[jo];
-          for (int q = 0; q < 3; ++q) L.p[jo][q] = cur[q] + (bln / blo) * vec[q];
-          enbond[jj] = bond_e(jj);
-          newbond += enbond[jj] - eobond[jj];
+          double np3[3];
+          for (int q = 0; q < 3; ++q) np3[q] = cur[q] + (bln / blo) * vec[q];
+          chain::put3(L.p, jo, np3);
+          const double e = chain::bond_j(S, L, jj);
+          chain::put1(enbond, jj, e);
+          newbond += e - chain::pick1(eobond, jj);
           if (d.next() < ((bln / blo) * (bln / blo)) * exp(-beta * (newbond + 0.0 + 0.0 + 0.0)))
-            eobond[jj] = enbond[jj];
+            chain::put1(eobond, jj, e);
           else
-            for (int q = 0; q < 3; ++q) L.p[jo][q] = store[q];
+            chain::put3(L.p, jo, store);
         }
       }
     } else if (choise == 2) {
       if (np > 0) {
         int jo = (int)(d.next() * (double)np);
         if (jo >= np) jo = np - 1;
-        const double store[3] = {L.p[jo][0], L.p[jo][1], L.p[jo][2]};
-        double cord[3] = {L.p[jo][0] - cur[0], L.p[jo][1] - cur[1], L.p[jo][2] - cur[2]};
-        const double sin_old = sin(chain::angle(L.p[kLocPrev], cur, L.p[jo]));
-        const double dr[3] = {cur[0] - L.p[jo][0], cur[1] - L.p[jo][1], cur[2] - L.p[jo][2]};
+        double store[3];
+        chain::pick3(L.p, jo, store);
+        double cord[3] = {store[0] - cur[0], store[1] - cur[1], store[2] - cur[2]};
+        const double sin_old = sin(chain::angle(L.v, cur, store));
+        const double dr[3] = {cur[0] - store[0], cur[1] - store[1], cur[2] - store[2]};
         double perpen[3];
         chain::perpendicular(dr, pv, perpen);
         const chain::Rot R(perpen, (2.0 * d.next() - 1.0) * S.max_bend);
         R.apply(cord);
-        for (int q = 0; q < 3; ++q) L.p[jo][q] = cur[q] + cord[q];
-        const double sin_new = sin(chain::angle(L.p[kLocPrev], cur, L.p[jo]));
-        for (int j = 0; j < nbe; ++j) {
-          enbend[j] = chain::bend_j(S, L, j);
-          newbend += enbend[j] - eobend[j];
-        }
+        double np3[3];
+        for (int q = 0; q < 3; ++q) np3[q] = cur[q] + cord[q];
+        chain::put3(L.p, jo, np3);
+        const double sin_new = sin(chain::angle(L.v, cur, np3));
+#pragma unroll
+        for (int j = 0; j < NBE; ++j)
+          if (j < nbe) {
+            enbend[j] = chain::bend_j(S, L, j);
+            newbend += enbend[j] - eobend[j];
+          }
         if (d.next() < ((sin_new / sin_old) * exp(-beta * (newbend + 0.0 + 0.0 + 0.0)))) {
-          for (int j = 0; j < nbe; ++j) eobend[j] = enbend[j];
+#pragma unroll
+          for (int j = 0; j < NBE; ++j)
+            if (j < nbe) eobend[j] = enbend[j];
         } else
-          for (int q = 0; q < 3; ++q) L.p[jo][q] = store[q];
+          chain::put3(L.p, jo, store);
       }
     } else {
       if (np > 0) {
         int jo = (int)(d.next() * (double)np);
         if (jo >= np) jo = np - 1;
-        const double store[3] = {L.p[jo][0], L.p[jo][1], L.p[jo][2]};
-        double cord[3] = {L.p[jo][0] - cur[0], L.p[jo][1] - cur[1], L.p[jo][2] - cur[2]};
+        double store[3];
+        chain::pick3(L.p, jo, store);
+        double cord[3] = {store[0] - cur[0], store[1] - cur[1], store[2] - cur[2]};
         const chain::Rot R(pv, (2.0 * d.next() - 1.0) * S.max_cone);
         R.apply(cord);
-        for (int q = 0; q < 3; ++q) L.p[jo][q] = cur[q] + cord[q];
-        for (int j = 0; j < nbe; ++j) {
-          enbend[j] = chain::bend_j(S, L, j);
-          newbend += enbend[j] - eobend[j];
-        }
+        double np3[3];
+        for (int q = 0; q < 3; ++q) np3[q] = cur[q] + cord[q];
+        chain::put3(L.p, jo, np3);
+#pragma unroll
+        for (int j = 0; j < NBE; ++j)
+          if (j < nbe) {
+            enbend[j] = chain::bend_j(S, L, j);
+            newbend += enbend[j] - eobend[j];
+          }
         if (d.next() < exp(-beta * (newbend + 0.0 + 0.0 + 0.0))) {
-          for (int j = 0; j < nbe; ++j) eobend[j] = enbend[j];
+#pragma unroll
+          for (int j = 0; j < NBE; ++j)
+            if (j < nbe) eobend[j] = enbend[j];
         } else
-          for (int q = 0; q < 3; ++q) L.p[jo][q] = store[q];
+          chain::put3(L.p, jo, store);
       }
     }
   }
-  for (int jj = 0; jj < np; ++jj) {
-    const int64_t o = (int64_t)jj * w.cap + slot;
-    w.bx[o] = L.p[jj][0];
-    w.by[o] = L.p[jj][1];
-    w.bz[o] = L.p[jj][2];
-  }
+#pragma unroll
+  for (int jj = 0; jj < NPM; ++jj)
+    if (jj < np) {
+      const int64_t o = (int64_t)jj * w.cap + slot;
+      w.bx[o] = L.p[jj][0];
+      w.by[o] = L.p[jj][1];
+      w.bz[o] = L.p[jj][2];
+    }
 }
 
 // Trial positions of a step, one wave per (slot, trial).  Sphere steps
@@ -3067,11 +3244,13 @@ __global__ __launch_bounds__(64) void k_chain_grow(const ChainArgs a, const Chai
 // (src/cbmc.c:1530-1577).  Otherwise the torsion selection of
 // GenerateTrialOrientationsMCScheme (src/cbmc.c:2350-2435): the lanes take
 // the k_torsion rotations of the walk's result about the previous-current
-// axis; a wave maximum, scan and ballot select one with probability
-// proportional to exp(-beta U_torsion), and RosenbluthTorsion averages the
-// first NumberOfTrialPositions of them (the reference's normalisation reads
-// the global trial count, src/cbmc.c:542-555).  Then the trial's bonded and
-// intra-molecular energies (src/cbmc.c:2437-2480, 1294-1311).
+// axis (KPER per lane); a wave maximum, scan and ballot select one with
+// probability proportional to exp(-beta U_torsion), and RosenbluthTorsion
+// averages the first NumberOfTrialPositions of them (the reference's
+// normalisation reads the global trial count, src/cbmc.c:542-555).  Then
+// the trial's bonded and intra-molecular energies (src/cbmc.c:2437-2480,
+// 1294-1311).
+template <int KPER, int NPM>
 __global__ __launch_bounds__(256) void k_chain_trials(const ChainArgs a, const ChainWork w) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & 63;
@@ -3082,19 +3261,21 @@ __global__ __launch_bounds__(256) void k_chain_trials(const ChainArgs a, const C
   if (slot >= (int64_t)*w.n_active) return;
   const ChainStep &S = *a.st;
   const int64_t cap = w.cap;
-  chain::Local L;
+  chain::Local<NPM> L;
   L.w = &w;
   L.slot = slot;
-  L.load(S.cur, kLocCur);
-  const double *cur = L.p[kLocCur];
+  L.load(S.cur, L.c);
+  const double *cur = L.c;
   const int np = S.n_place;
   auto store_trial = [&](double rt, double ub, double ube, double ut, double ui) {
-    for (int jj = 0; jj < np; ++jj) {
-      const int64_t o = (int64_t)(jj * k + t) * cap + slot;
-      w.tx[o] = L.p[jj][0];
-      w.ty[o] = L.p[jj][1];
-      w.tz[o] = L.p[jj][2];
-    }
+#pragma unroll
+    for (int jj = 0; jj < NPM; ++jj)
+      if (jj < np) {
+        const int64_t o = (int64_t)(jj * k + t) * cap + slot;
+        w.tx[o] = L.p[jj][0];
+        w.ty[o] = L.p[jj][1];
+        w.tz[o] = L.p[jj][2];
+      }
     const int64_t o = (int64_t)t * cap + slot;
     w.tRT[o] = rt;
     w.tbond[o] = ub;
@@ -3105,7 +3286,8 @@ __global__ __launch_bounds__(256) void k_chain_trials(const ChainArgs a, const C
   if (!w.alive[slot]) {
     // a rejected insertion keeps valid (unused) coordinates in the site lists
     if (lane == 0) {
-      for (int jj = 0; jj < np; ++jj)
+#pragma unroll
+      for (int jj = 0; jj < NPM; ++jj)
         for (int q = 0; q < 3; ++q) L.p[jj][q] = cur[q];
       store_trial(1.0, 0.0, 0.0, 0.0, 0.0);
     }
@@ -3118,47 +3300,71 @@ __global__ __launch_bounds__(256) void k_chain_trials(const ChainArgs a, const C
       chain::sphere(d, v);
       const double bl = w.bl[slot];
       for (int q = 0; q < 3; ++q) L.p[0][q] = cur[q] + v[q] * bl;
-      double pa[3], pb[3];
-      L.get(S.bond_a[0], pa);
-      L.get(S.bond_b[0], pb);
-      const double ub = chain::bond_u(S.bond_kind[0], S.bond_p0[0], S.bond_p1[0], pa, pb);
-      store_trial(1.0, ub, 0.0, 0.0, chain::sum_intra(S, L, a.rc2));
+      store_trial(1.0, chain::bond_j(S, L, 0), 0.0, 0.0, chain::sum_intra(S, L, a.rc2));
     }
     return;
   }
-  L.load(S.prev, kLocPrev);
-  double pv[3] = {L.p[kLocPrev][0] - cur[0], L.p[kLocPrev][1] - cur[1], L.p[kLocPrev][2] - cur[2]};
+  L.load(S.prev, L.v);
+  double pv[3] = {L.v[0] - cur[0], L.v[1] - cur[1], L.v[2] - cur[2]};
   const double nrm = sqrt(pv[0] * pv[0] + pv[1] * pv[1] + pv[2] * pv[2]);
   pv[0] /= nrm;
   pv[1] /= nrm;
   pv[2] /= nrm;
-  double base[kChainPlace][3];
-  for (int jj = 0; jj < np; ++jj) {
-    const int64_t o = (int64_t)jj * cap + slot;
+  double base[NPM][3];
+#pragma unroll
+  for (int jj = 0; jj < NPM; ++jj) {
+    const int64_t o = (int64_t)(jj < np ? jj : 0) * cap + slot;
     base[jj][0] = w.bx[o];
     base[jj][1] = w.by[o];
     base[jj][2] = w.bz[o];
+    L.p[jj][0] = base[jj][0];
+    L.p[jj][1] = base[jj][1];
+    L.p[jj][2] = base[jj][2];
   }
   const double beta = a.beta;
   const int kt = a.kt;
+  constexpr int kTC = NPM == 1 ? 2 : 4; // torsions whose fixed halves stay in registers
+  chain::TorsionHalf th[kTC];
+  const bool cached = S.n_tors <= kTC;
+#pragma unroll
+  for (int j = 0; j < kTC; ++j) {
+    th[j].mode = 0;
+    if (cached && j < S.n_tors) chain::torsion_prepare(S, L, j, th[j]);
+  }
   auto place_at = [&](double ang) {
     const chain::Rot R(pv, ang);
-    for (int jj = 0; jj < np; ++jj) {
-      double c[3] = {base[jj][0] - cur[0], base[jj][1] - cur[1], base[jj][2] - cur[2]};
-      R.apply(c);
-      for (int q = 0; q < 3; ++q) L.p[jj][q] = c[q] + cur[q];
-    }
+#pragma unroll
+    for (int jj = 0; jj < NPM; ++jj)
+      if (jj < np) {
+        double c[3] = {base[jj][0] - cur[0], base[jj][1] - cur[1], base[jj][2] - cur[2]};
+        R.apply(c);
+        for (int q = 0; q < 3; ++q) L.p[jj][q] = c[q] + cur[q];
+      }
   };
-  constexpr int kPer = kChainTorsionTrials / 64;
-  double bf[kPer];
+  auto torsions = [&]() -> double {
+    if (!cached) return chain::sum_torsions(S, L);
+    double u = 0.0;
+#pragma unroll
+    for (int j = 0; j < kTC; ++j)
+      if (j < S.n_tors) {
+        if (th[j].mode) {
+          double m[3];
+          chain::pick3(L.p, th[j].mover, m);
+          u += chain::torsion_half(S.tors_p[j], th[j], m);
+        } else
+          u += chain::torsion_j(S, L, j);
+      }
+    return u;
+  };
+  double bf[KPER], e[KPER];
   double largest = -DBL_MAX;
 #pragma unroll
-  for (int r = 0; r < kPer; ++r) {
+  for (int r = 0; r < KPER; ++r) {
     const int kk = r * 64 + lane;
     bf[r] = 0.0;
     if (r * 64 < kt && kk < kt) {
       place_at((2.0 * stream_uniform(d.key, (uint64_t)kk) - 1.0) * M_PI);
-      bf[r] = -beta * chain::sum_torsions(S, L);
+      bf[r] = -beta * torsions();
       largest = fmax(largest, bf[r]);
     }
   }
@@ -3170,16 +3376,18 @@ __global__ __launch_bounds__(256) void k_chain_trials(const ChainArgs a, const C
   if (S.n_tors > 0) {
     double total = 0.0;
 #pragma unroll
-    for (int r = 0; r < kPer; ++r)
-      if (r * 64 < kt) total += chain::wave_sum((r * 64 + lane < kt) ? exp(bf[r] - largest) : 0.0);
+    for (int r = 0; r < KPER; ++r) {
+      e[r] = (r * 64 + lane < kt) ? exp(bf[r] - largest) : 0.0;
+      if (r * 64 < kt) total += chain::wave_sum(e[r]);
+    }
     const double ws = u * total;
     double carry = 0.0;
     sel = -1;
 #pragma unroll
-    for (int r = 0; r < kPer; ++r) {
+    for (int r = 0; r < KPER; ++r) {
       if (r * 64 >= kt || sel >= 0) continue;
       const int kk = r * 64 + lane;
-      const double inc = chain::wave_scan(kk < kt ? exp(bf[r] - largest) : 0.0, lane);
+      const double inc = chain::wave_scan(e[r], lane);
       const uint64_t hit = __ballot(kk < kt && carry + inc >= ws);
       if (hit)
         sel = r * 64 + (__ffsll((unsigned long long)hit) - 1);
@@ -3193,8 +3401,7 @@ __global__ __launch_bounds__(256) void k_chain_trials(const ChainArgs a, const C
   }
   if (lane != 0) return;
   place_at((2.0 * stream_uniform(d.key, (uint64_t)sel) - 1.0) * M_PI);
-  store_trial(rt, chain::sum_bonds(S, L), chain::sum_bends(S, L), chain::sum_torsions(S, L),
-              chain::sum_intra(S, L, a.rc2));
+  store_trial(rt, chain::sum_bonds(S, L), chain::sum_bends(S, L), torsions(), chain::sum_intra(S, L, a.rc2));
 }
 
 // Rosen's selection among the step's trials (src/cbmc.c:2786-2840) with
@@ -3319,12 +3526,32 @@ __global__ __launch_bounds__(256) void k_chain_finalize(const ChainArgs a, const
 void launch_chain_fb_select(const ChainArgs &a, const ChainWork &w, hipStream_t st) {
   hipLaunchKernelGGL(k_chain_fb_select, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, st, a, w);
 }
-void launch_chain_grow(const ChainArgs &a, const ChainWork &w, hipStream_t st) {
-  hipLaunchKernelGGL(k_chain_grow, dim3((unsigned)((a.n + 63) / 64)), dim3(64), 0, st, a, w);
+// steps that place one bead with at most one bond and two bends (every step
+// of an unbranched chain) run the one-bead instances
+static bool one_bead(const ChainStep &h) { return h.n_place == 1 && h.n_bonds <= 1 && h.n_bends <= 2; }
+
+void launch_chain_grow(const ChainArgs &a, const ChainWork &w, const ChainStep &h, hipStream_t st) {
+  const dim3 grid((unsigned)((a.n + 63) / 64));
+  if (one_bead(h))
+    hipLaunchKernelGGL(k_chain_grow<1>, grid, dim3(64), 0, st, a, w);
+  else
+    hipLaunchKernelGGL(k_chain_grow<kChainPlace>, grid, dim3(64), 0, st, a, w);
 }
-void launch_chain_trials(const ChainArgs &a, const ChainWork &w, hipStream_t st) {
+void launch_chain_trials(const ChainArgs &a, const ChainWork &w, const ChainStep &h, hipStream_t st) {
   const int64_t waves = a.n * a.k;
-  hipLaunchKernelGGL(k_chain_trials, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, a, w);
+  const dim3 grid((unsigned)((waves + 3) / 4));
+  const bool one = one_bead(h);
+  if (a.kt <= 128) {
+    if (one)
+      hipLaunchKernelGGL((k_chain_trials<2, 1>), grid, dim3(256), 0, st, a, w);
+    else
+      hipLaunchKernelGGL((k_chain_trials<2, kChainPlace>), grid, dim3(256), 0, st, a, w);
+  } else {
+    if (one)
+      hipLaunchKernelGGL((k_chain_trials<kChainTorsionTrials / 64, 1>), grid, dim3(256), 0, st, a, w);
+    else
+      hipLaunchKernelGGL((k_chain_trials<kChainTorsionTrials / 64, kChainPlace>), grid, dim3(256), 0, st, a, w);
+  }
 }
 void launch_chain_select(const ChainArgs &a, const ChainWork &w, hipStream_t st) {
   hipLaunchKernelGGL(k_chain_select, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, st, a, w);
