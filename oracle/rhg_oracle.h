@@ -20,7 +20,10 @@
  * brute-force image sum; Ewald against the rock-salt Madelung constant;
  * Fourier Delta against the change of the full Fourier energy; tricubic
  * interpolation against exact tricubic polynomials; grid derivatives against
- * finite differences; Widom against the reference's He/IRMOF-1 result.
+ * finite differences; Widom against the reference's He/IRMOF-1 result;
+ * flexible-chain growth against hand-written growth plans, closed-form
+ * bonded energies and empty-box quadrature of the torsion weight, bend-angle
+ * and bond-length distributions (tests/test_oracle_chain.py).
  * Parity unpinned: the libm-dependent last bits (erfc, erf, exp, sin, cos)
  * and the global interleaving of a whole RASPA2 run's MT19937-64 stream
  * (orc_widom_run_mt's absolute offset).

@@ -300,29 +300,75 @@ def framework_from_cif(text, ff, unit_cells=(1, 1, 1), name="framework", use_cif
     return spec
 
 
-def read_molecule(path, ff, k_fb=10, k_or=10, start=0):
-    """A molecule definition as a rigid probe: pseudo-atom parameters from the
-    force field, body coordinates from the file (single-atom molecules need
-    none)."""
+_BONDS = {"FIXED_BOND": R.BOND_FIXED, "HARMONIC_BOND": R.BOND_HARMONIC}
+_BENDS = {"HARMONIC_BEND": R.BEND_HARMONIC}
+_TORSIONS = {"TRAPPE_DIHEDRAL": R.TORSION_TRAPPE}
+
+
+def read_molecule(path, ff, k_fb=10, k_or=10, start=0, k_torsion=100, moves_per_open_bead=150):
+    """A molecule definition (ReadComponentDefinition, src/molecule.c:1430-3030).
+    A rigid molecule (or a single atom) becomes a rigid probe with its body
+    coordinates; a molecule made of one flexible group becomes a chain probe
+    with its bonds, bends, torsions and intra-molecular LJ pairs, grown by
+    CBMC with the internal MC scheme (k_or trial positions per bead).
+    Pseudo-atom parameters come from the force field."""
     rows = []
     with open(path) as fh:
         rows = [ln.split() for ln in fh]
     data = [r for r in rows if r and not r[0].startswith("#")]
     n_atoms = int(data[3][0])
+    n_groups = int(data[4][0])
     kind = data[5][0].lower()
-    atoms = [r for r in data if len(r) >= 2 and r[0].isdigit() and not r[1][0].isdigit()][:n_atoms]
-    if len(atoms) != n_atoms:
+    first = 7
+    atoms = data[first:first + n_atoms]
+    if len(atoms) != n_atoms or any(not (r[0].isdigit() and not r[1][0].isdigit()) for r in atoms):
         raise ValueError(f"{path}: expected {n_atoms} atoms")
-    if n_atoms > 1 and kind != "rigid":
-        raise NotImplementedError(f"{path}: flexible molecules of more than one atom")
-    body = np.array([[float(v) for v in r[2:5]] if len(r) >= 5 else [0.0, 0.0, 0.0] for r in atoms])
     sites = []
     for r in atoms:
         eps, sig = ff.lj(r[1])
         _, q = ff.mass_charge(r[1], r[1])
         sites.append((eps, sig, q))
     name = os.path.splitext(os.path.basename(path))[0]
-    return W.Probe(name, sites, body, start=start, k_fb=k_fb, k_or=k_or)
+    if n_atoms == 1 or kind == "rigid":
+        if n_groups != 1:
+            raise NotImplementedError(f"{path}: molecules of more than one group")
+        body = np.array([[float(v) for v in r[2:5]] if len(r) >= 5 else [0.0, 0.0, 0.0] for r in atoms])
+        return W.Probe(name, sites, body, start=start, k_fb=k_fb, k_or=k_or)
+    if n_groups != 1 or kind != "flexible":
+        raise NotImplementedError(f"{path}: only molecules of one flexible or rigid group")
+    # counts: chiral centres, bonds, bond dipoles, bends, Urey-Bradley,
+    # inversion bends, torsions, improper torsions, bond/bond,
+    # stretch/bend, bend/bend, stretch/torsion, bend/torsion, intra VDW,
+    # intra Coulomb (src/molecule.c:1445-1466)
+    i = first + n_atoms
+    counts = [int(v) for v in data[i][:15]]
+    i += 1
+    (n_chiral, n_bond, n_dip, n_bend, n_ub, n_inv, n_tors, n_imp, n_bb, n_sb, n_bb2, n_st, n_bt,
+     n_intra, n_intra_cc) = counts
+    if n_chiral or n_dip or n_ub or n_inv or n_imp or n_bb or n_sb or n_bb2 or n_st or n_bt:
+        raise NotImplementedError(f"{path}: chiral centres, bond dipoles or cross terms")
+
+    def section(n):
+        nonlocal i
+        out = data[i:i + n]
+        i += n
+        return out
+
+    def kind_of(table, word):
+        if word not in table:
+            raise NotImplementedError(f"{path}: {word}")
+        return table[word]
+
+    bonds = [(int(r[0]), int(r[1]), kind_of(_BONDS, r[2]), *[float(v) for v in r[3:]]) for r in section(n_bond)]
+    bends = [(int(r[0]), int(r[1]), int(r[2]), kind_of(_BENDS, r[3]), *[float(v) for v in r[4:]])
+             for r in section(n_bend)]
+    torsions = [(int(r[0]), int(r[1]), int(r[2]), int(r[3]), kind_of(_TORSIONS, r[4]),
+                 *[float(v) for v in r[5:9]]) for r in section(n_tors)]
+    intra = [(int(r[0]), int(r[1])) for r in section(n_intra)]
+    if n_intra_cc and any(abs(q) > 1e-10 for _, _, q in sites):
+        raise NotImplementedError(f"{path}: charged flexible molecules")
+    return W.ChainProbe(name, sites, bonds, bends, torsions, intra, start=start, k_fb=k_fb,
+                        k_trial=k_or, k_torsion=k_torsion, moves=moves_per_open_bead)
 
 
 # --------------------------------------- the reference wrapper's calls
@@ -335,25 +381,34 @@ def _share(raspa_dir):
 
 def widom(structure, molecule, unit_cells=(1, 1, 1), cycles=2000, forcefield="CrystalGenerator",
           temperature=298.0, cutoff=12.8, raspa_dir=None, molecule_set="TraPPE", insertions=None,
-          seed=1, charge_method=None, precision=R.PREC_MIXED, device=0):
+          seed=1, charge_method=None, precision=R.PREC_MIXED, device=0, ideal_gas_weight=1.0):
     """Widom insertion of `molecule` in a CIF structure given as text, as the
     reference's scripts set it up (python/raspa2.py:345-387): CutOff 12.8 A,
     298 K, first-bead trials 10; a cycle of an empty system is 20 moves, so
-    cycles -> 20 x cycles insertions unless `insertions` is given.  Returns
-    the estimators of src/statistics.c (average Widom Rosenbluth weight,
-    Henry coefficient [mol/kg/Pa] with their 2-sigma errors, <U_gh> [K])."""
+    cycles -> 20 x cycles insertions unless `insertions` is given.  Flexible
+    molecules grow by CBMC with the internal MC scheme.  Returns the
+    estimators of src/statistics.c (average Widom Rosenbluth weight, Henry
+    coefficient [mol/kg/Pa] with their 2-sigma errors, <U_gh> [K]); the Henry
+    coefficient divides by ideal_gas_weight (IdealGasRosenbluthWeight, 1 unless
+    set, src/input.c:902, 2906; see ideal_gas_rosenbluth_weight)."""
     share = _share(raspa_dir)
     ff = ForceField.load(os.path.join(share, "forcefield", forcefield))
     fw = framework_from_cif(structure, ff, unit_cells)
     probe = read_molecule(os.path.join(share, "molecules", molecule_set, molecule + ".def"), ff)
-    charged = any(abs(q) > 1e-10 for q in fw.charge) and probe.charged
+    flexible = isinstance(probe, W.ChainProbe)
+    charged = (not flexible) and any(abs(q) > 1e-10 for q in fw.charge) and probe.charged
     cm = charge_method if charge_method is not None else (R.CHARGE_EWALD if charged else R.CHARGE_NONE)
     ctx = fw.context(cutoff=cutoff, charge_method=cm, temperature=temperature, precision=precision,
                      device=device, shifted=ff.shifted)
-    _, mol = W.register_probe(ctx, probe)
     n = insertions if insertions is not None else 20 * cycles
-    blocks, _ = ctx.widom_run(mol, seed, n)
-    est = R.widom_estimators(blocks, temperature, ctx.info()["density"])
+    if flexible:
+        _, chain = W.register_chain(ctx, probe)
+        blocks, _, _ = ctx.widom_chain_run(chain, seed, n)
+    else:
+        _, mol = W.register_probe(ctx, probe)
+        blocks, _ = ctx.widom_run(mol, seed, n)
+    est = R.widom_estimators(blocks, temperature, ctx.info()["density"],
+                             ideal_gas_weight=ideal_gas_weight)
     # "Average Widom Rosenbluth-weight" per block and its 2-sigma spread over
     # the five blocks (src/statistics.c:4747-4800)
     b = blocks.as_array()
@@ -364,6 +419,32 @@ def widom(structure, molecule, unit_cells=(1, 1, 1), cycles=2000, forcefield="Cr
     est["insertions"] = n
     ctx.close()
     return est
+
+
+def ideal_gas_rosenbluth_weight(molecule, forcefield="CrystalGenerator", temperature=298.0, cutoff=12.8,
+                                raspa_dir=None, molecule_set="TraPPE", insertions=1 << 18, seed=1,
+                                device=0):
+    """IdealGasRosenbluthWeight of a molecule: the average Rosenbluth weight
+    of its growth with no framework and no other molecules (the value a RASPA2
+    input supplies for the Henry coefficient of a flexible molecule,
+    src/input.c:2906, src/mc_moves.c:7013), measured by the same growth on
+    the device in an empty box."""
+    share = _share(raspa_dir)
+    ff = ForceField.load(os.path.join(share, "forcefield", forcefield))
+    probe = read_molecule(os.path.join(share, "molecules", molecule_set, molecule + ".def"), ff)
+    if not isinstance(probe, W.ChainProbe):
+        return 1.0
+    e = np.zeros(0)
+    box = np.diag([4.0 * cutoff] * 3).T.reshape(-1)
+    ctx = R.Context(box, R.BC_RECTANGULAR, e, e, e, e, e, charge=e, mass=e, cutoff_vdw=cutoff,
+                    shifted=ff.shifted, temperature=temperature, device=device, precision=R.PREC_FP64)
+    try:
+        _, chain = W.register_chain(ctx, probe)
+        blocks, _, _ = ctx.widom_chain_run(chain, seed, insertions)
+        b = blocks.as_array()
+        return float(b[:, 1].sum() / b[:, 0].sum())
+    finally:
+        ctx.close()
 
 
 def get_helium_void_fraction(structure, unit_cells=(1, 1, 1), cycles=2000, input_file_type="cif",

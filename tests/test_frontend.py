@@ -77,3 +77,25 @@ def test_molecules(R):
     assert co2.sites[0] == (85.671, 3.017, -0.3256) and co2.sites[1] == (29.933, 2.745, 0.6512)
     he = F.read_molecule(os.path.join(SHARE, "molecules", "TraPPE", "helium.def"), ff)
     assert he.n_sites == 1 and he.sites[0] == (10.9, 2.64, 0.0)
+
+
+def test_flexible_molecules(R):
+    """molecules/TraPPE/C4.def and C6.def as chains: the bonded terms and
+    intra pairs of the files, GenericMOFs' CH3_sp3/CH2_sp3 parameters, and the
+    growth plan of the chain-growth oracle."""
+    import oracle as O
+    from raspa2_amd import frontend as F
+    ff = F.ForceField.load(os.path.join(SHARE, "forcefield", "GenericMOFs"))
+    c4 = F.read_molecule(os.path.join(SHARE, "molecules", "TraPPE", "C4.def"), ff)
+    assert c4.n_atoms == 4 and c4.intra == []
+    assert c4.sites[0] == (108.0, 3.76, 0.0) and c4.sites[1] == (56.0, 3.96, 0.0)
+    assert c4.bonds == [(i, i + 1, R.BOND_HARMONIC, 96500.0, 1.54) for i in range(3)]
+    assert c4.bends == [(i, i + 1, i + 2, R.BEND_HARMONIC, 62500.0, 114.0) for i in range(2)]
+    assert c4.torsions == [(0, 1, 2, 3, R.TORSION_TRAPPE, 0.0, 355.03, -68.19, 791.32)]
+    c6 = F.read_molecule(os.path.join(SHARE, "molecules", "TraPPE", "C6.def"), ff, k_or=8)
+    assert c6.intra == [(0, 4), (0, 5), (1, 5)] and len(c6.torsions) == 3 and c6.k_trial == 8
+    box = np.diag([30.0] * 3).T.reshape(-1)
+    s = O.System(box, O.BC_RECTANGULAR, [], [], [], [], [], [], cutoff_vdw=12.0)
+    types = [s.add_pseudo_atom(e, sg, q) for e, sg, q in c6.sites]
+    plan = O.chain_plan(O.chain(types, c6.bonds, c6.bends, c6.torsions, c6.intra))
+    assert [p["n_intra"] for p in plan] == [0, 0, 0, 1, 2]

@@ -71,3 +71,26 @@ def test_surface_area_wrapper_call(R):
     a2 = F.get_geometric_surface_area(_cif(), cycles=500, units="A^2", raspa_dir=GOLD)
     v = F.get_geometric_surface_area(_cif(), cycles=500, units="m^2/cm^3", raspa_dir=GOLD)
     assert abs(v - a2 / 25.832 ** 3 * 1e4) <= 1e-9 * v
+
+
+def test_flexible_chain_widom_wrapper(R, W):
+    """n-butane (molecules/TraPPE/C4.def) in IRMOF-1 through the wrapper
+    call: the blocks equal a direct chain run with the same seed, and the
+    Henry coefficient divides by the ideal-gas Rosenbluth weight, which is the
+    empty-box mean of exp(-beta U_torsion) for butane within its error."""
+    import math
+    from raspa2_amd import frontend as F
+    n = 200_000
+    w_ig = F.ideal_gas_rosenbluth_weight("C4", forcefield="GenericMOFs", temperature=298.0, cutoff=12.0,
+                                         raspa_dir=GOLD, insertions=400_000)
+    phi = np.linspace(-math.pi, math.pi, 400001)
+    u = 355.03 * (1 + np.cos(phi)) - 68.19 * (1 - np.cos(2 * phi)) + 791.32 * (1 + np.cos(3 * phi))
+    # RosenbluthTorsion averages 10 of the 100 torsion trials: same mean
+    assert w_ig == pytest.approx(np.exp(-u / 298.0).mean(), rel=0.01)
+    est = F.widom(_cif(), "C4", forcefield="GenericMOFs", cutoff=12.0, temperature=298.0, raspa_dir=GOLD,
+                  insertions=n, seed=3, precision=R.PREC_FP64, ideal_gas_weight=w_ig)
+    est1 = F.widom(_cif(), "C4", forcefield="GenericMOFs", cutoff=12.0, temperature=298.0, raspa_dir=GOLD,
+                   insertions=n, seed=3, precision=R.PREC_FP64)
+    assert est["kh_overall"] == pytest.approx(est1["kh_overall"] / w_ig, rel=1e-12)
+    assert est["w"] > w_ig  # butane is attracted into the framework
+    assert est["u"] < 0.0
