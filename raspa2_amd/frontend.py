@@ -216,6 +216,25 @@ class ForceField:
     rule: str = "Lorentz-Berthelot"
     pseudo: dict = field(default_factory=dict)
 
+    overrides: set = field(default_factory=set)
+
+    def check_pairs(self, probe_labels, other_labels):
+        """Refuses a calculation that forms a pair with an explicit
+        force_field.def interaction (the kernels use Lorentz-Berthelot mixing
+        of the self parameters)."""
+        def hit(pat, name):
+            if pat.endswith("_"):
+                return name.lower().startswith(pat[:-1].lower())
+            return name.lower() == pat.lower()
+
+        others = set(other_labels) | set(probe_labels)
+        for p1, p2 in self.overrides:
+            for a in set(probe_labels):
+                for b in others:
+                    if (hit(p1, a) and hit(p2, b)) or (hit(p2, a) and hit(p1, b)):
+                        raise NotImplementedError(f"force_field.def defines the pair {a}-{b}: explicit "
+                                                  "pair interactions are not supported")
+
     @classmethod
     def load(cls, path):
         ff = cls(path)
@@ -224,11 +243,18 @@ class ForceField:
         if os.path.exists(pa):
             ff.pseudo = read_pseudo_atoms(pa)
         over = os.path.join(path, "force_field.def")
+        ff.overrides = set()
         if os.path.exists(over):
+            # rules to overwrite, then explicitly defined pair interactions
+            # (src/potentials.c:3000-3130); the kernels mix self parameters,
+            # so a calculation that forms one of these pairs is refused
             rows = _data_lines(over)
-            if any(int(r[0]) != 0 for r in rows[:3] if r and r[0].lstrip("-").isdigit()):
-                raise NotImplementedError("force_field.def pair overrides are not supported: "
-                                          "the kernels use Lorentz-Berthelot mixing of self parameters")
+            n_rules = int(rows[0][0])
+            if n_rules:
+                raise NotImplementedError("force_field.def mixing-rule overwrites are not supported")
+            n_pairs = int(rows[1 + n_rules][0]) if len(rows) > 1 + n_rules else 0
+            for r in rows[2 + n_rules:2 + n_rules + n_pairs]:
+                ff.overrides.add(tuple(sorted((r[0], r[1]))))
         if ff.rule.lower() != "lorentz-berthelot":
             raise NotImplementedError(f"mixing rule {ff.rule}")
         if ff.tail:
@@ -333,7 +359,9 @@ def read_molecule(path, ff, k_fb=10, k_or=10, start=0, k_torsion=100, moves_per_
         if n_groups != 1:
             raise NotImplementedError(f"{path}: molecules of more than one group")
         body = np.array([[float(v) for v in r[2:5]] if len(r) >= 5 else [0.0, 0.0, 0.0] for r in atoms])
-        return W.Probe(name, sites, body, start=start, k_fb=k_fb, k_or=k_or)
+        probe = W.Probe(name, sites, body, start=start, k_fb=k_fb, k_or=k_or)
+        probe.labels = [r[1] for r in atoms]
+        return probe
     if n_groups != 1 or kind != "flexible":
         raise NotImplementedError(f"{path}: only molecules of one flexible or rigid group")
     # counts: chiral centres, bonds, bond dipoles, bends, Urey-Bradley,
@@ -367,8 +395,10 @@ def read_molecule(path, ff, k_fb=10, k_or=10, start=0, k_torsion=100, moves_per_
     intra = [(int(r[0]), int(r[1])) for r in section(n_intra)]
     if n_intra_cc and any(abs(q) > 1e-10 for _, _, q in sites):
         raise NotImplementedError(f"{path}: charged flexible molecules")
-    return W.ChainProbe(name, sites, bonds, bends, torsions, intra, start=start, k_fb=k_fb,
-                        k_trial=k_or, k_torsion=k_torsion, moves=moves_per_open_bead)
+    probe = W.ChainProbe(name, sites, bonds, bends, torsions, intra, start=start, k_fb=k_fb,
+                         k_trial=k_or, k_torsion=k_torsion, moves=moves_per_open_bead)
+    probe.labels = [r[1] for r in atoms]
+    return probe
 
 
 # --------------------------------------- the reference wrapper's calls
@@ -395,6 +425,7 @@ def widom(structure, molecule, unit_cells=(1, 1, 1), cycles=2000, forcefield="Cr
     ff = ForceField.load(os.path.join(share, "forcefield", forcefield))
     fw = framework_from_cif(structure, ff, unit_cells)
     probe = read_molecule(os.path.join(share, "molecules", molecule_set, molecule + ".def"), ff)
+    ff.check_pairs(probe.labels, fw.labels)
     flexible = isinstance(probe, W.ChainProbe)
     charged = (not flexible) and any(abs(q) > 1e-10 for q in fw.charge) and probe.charged
     cm = charge_method if charge_method is not None else (R.CHARGE_EWALD if charged else R.CHARGE_NONE)
@@ -434,6 +465,7 @@ def ideal_gas_rosenbluth_weight(molecule, forcefield="CrystalGenerator", tempera
     probe = read_molecule(os.path.join(share, "molecules", molecule_set, molecule + ".def"), ff)
     if not isinstance(probe, W.ChainProbe):
         return 1.0
+    ff.check_pairs(probe.labels, ())
     e = np.zeros(0)
     box = np.diag([4.0 * cutoff] * 3).T.reshape(-1)
     ctx = R.Context(box, R.BC_RECTANGULAR, e, e, e, e, e, charge=e, mass=e, cutoff_vdw=cutoff,
@@ -476,6 +508,7 @@ def get_geometric_surface_area(structure, unit_cells=(1, 1, 1), cycles=500, inpu
     ff = ForceField.load(os.path.join(share, "forcefield", forcefield))
     fw = framework_from_cif(structure, ff, unit_cells)
     n2 = read_molecule(os.path.join(share, "molecules", "TraPPE", "N2.def"), ff)
+    ff.check_pairs(n2.labels[:1], fw.labels)
     ctx = fw.context(cutoff=12.8, temperature=298.0, device=device, shifted=ff.shifted)
     t = ctx.add_pseudo_atom(*n2.sites[n2.start])
     area, _ = ctx.SurfaceArea(t, samples if samples is not None else 100 * cycles, seed=seed,

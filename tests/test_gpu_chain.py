@@ -150,3 +150,34 @@ def test_chain_errors(W, R):
     _, cg = gpu_chain(ctx, probe)
     blocks, rec, _ = ctx.widom_chain_run(cg, 1, 50, records=True)
     assert blocks.as_array()[:, 0].sum() == 50
+
+
+def test_ideal_rosenbluth_weights_printed(O, R):
+    """The reference's IdealRosenbluthWeights-C5-C9-573K example on the
+    device: 10^6 growths of each n-alkane in an empty box at 573 K give the
+    printed average Rosenbluth weights within three combined standard errors
+    (fixture tests/golden/ideal_rosenbluth_c5_c9.json; the run's printed
+    torsion parameters)."""
+    import json
+    import os
+    from raspa2_amd import frontend as F
+    here = os.path.dirname(os.path.abspath(__file__))
+    G = json.load(open(os.path.join(here, "golden", "ideal_rosenbluth_c5_c9.json")))
+    share = os.path.join(here, "golden", "share", "raspa")
+    ff = F.ForceField.load(os.path.join(share, "forcefield", "GarciaPerez2006"))
+    box = np.diag(G["box"]).T.reshape(-1)
+    e = np.zeros(0)
+    ctx = R.Context(box, R.BC_RECTANGULAR, e, e, e, e, e, charge=e, mass=e, cutoff_vdw=12.0,
+                    shifted=ff.shifted, temperature=G["temperature"], precision=R.PREC_FP64)
+    n = 1_000_000
+    for name in ("C5", "C6", "C7", "C8", "C9"):
+        p = F.read_molecule(os.path.join(share, "molecules", "TraPPE", name + ".def"), ff)
+        tp = G["printed_torsions"][name]["p"]
+        types = [ctx.add_pseudo_atom(eps, sg, q) for eps, sg, q in p.sites]
+        ch = R.Chain(types, p.bonds, p.bends, [t[:5] + tuple(tp) for t in p.torsions], p.intra)
+        _, rec, _ = ctx.widom_chain_run(ch, 7, n, records=True)
+        w = rec[:, R.CREC["W"]]
+        ref = G["average_widom"][name]
+        sig = math.hypot(w.std() / math.sqrt(n), ref["plus_minus"] / 2.0)
+        print(f"{name}: {w.mean():.6g} vs printed {ref['average']:.6g} +/- {ref['plus_minus']:.2g}", flush=True)
+        assert abs(w.mean() - ref["average"]) < 3.0 * sig, (name, w.mean(), ref)

@@ -205,3 +205,40 @@ def _pair(O, s, ta, tb):
     sg = 0.5 * (sa + sb)
     t3 = (sg * sg / 144.0) ** 3
     return e, sg, 4.0 * e * (t3 * (t3 - 1.0))
+
+
+def _ideal_chain(O, name, G, SHARE):
+    """An n-alkane of the reference's ideal-weight example: topology from
+    molecules/TraPPE, GarciaPerez2006 LJ, the torsion parameters the run
+    printed (its molecule files had p_1 = 335.03 K)."""
+    import os
+    from raspa2_amd import frontend as F
+    ff = F.ForceField.load(os.path.join(SHARE, "forcefield", "GarciaPerez2006"))
+    p = F.read_molecule(os.path.join(SHARE, "molecules", "TraPPE", name + ".def"), ff)
+    tp = G["printed_torsions"][name]["p"]
+    p.torsions = [t[:5] + tuple(tp) for t in p.torsions]
+    return p, ff
+
+
+def test_ideal_rosenbluth_weights_printed(O):
+    """examples/Auxiliary/IdealRosenbluthWeights-C5-C9-573K: the average
+    Rosenbluth weight of n-pentane, n-hexane and n-heptane grown in an empty
+    box at 573 K against the reference's printed averages (fixture
+    tests/golden/ideal_rosenbluth_c5_c9.json), within three combined standard
+    errors (the printed +/- taken as two)."""
+    import json
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    G = json.load(open(os.path.join(here, "golden", "ideal_rosenbluth_c5_c9.json")))
+    share = os.path.join(here, "golden", "share", "raspa")
+    box = np.diag(G["box"]).T.reshape(-1)
+    for name, n in (("C5", 20000), ("C6", 10000), ("C7", 6000)):
+        p, ff = _ideal_chain(O, name, G, share)
+        s = O.System(box, O.BC_RECTANGULAR, [], [], [], [], [], [], cutoff_vdw=12.0, shifted=ff.shifted,
+                     temperature=G["temperature"])
+        types = [s.add_pseudo_atom(e, sg, q) for e, sg, q in p.sites]
+        c = O.chain(types, p.bonds, p.bends, p.torsions, p.intra)
+        w = O.chain_run_stream(s, c, 1, 0, n)[:, O.CREC["W"]]
+        ref = G["average_widom"][name]
+        sig = math.hypot(w.std() / math.sqrt(n), ref["plus_minus"] / 2.0)
+        assert abs(w.mean() - ref["average"]) < 3.0 * sig, (name, w.mean(), ref)
