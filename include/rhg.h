@@ -288,6 +288,22 @@ int rhg_ewald_fourier_move(rhg_ctx *ctx, int32_t n_new, const int32_t *new_types
 int rhg_surface_area(rhg_ctx *ctx, int32_t probe_type, double distance_factor, int64_t samples, uint64_t seed,
                      double *area, uint64_t *counts);
 
+/* ------------------------------------------------ pore-size distribution */
+/* SimulationType PSD (ComputePoreSizeDistribution, src/monte_carlo.c:1515-
+ * 1540; SamplePoreSizeDistribution, src/sample.c:3444-3560): `cycles`
+ * points A and n_b sphere centres per point (the reference draws
+ * 20 x NumberOfCycles), PSDProbeDistance probe_factor (1 for Sigma,
+ * 2^(1/6) for Minimum: an atom excludes 0.5 factor sigma_jj), blocked
+ * pockets of the context honoured.  hist[bins] (PSDHistogramSize over
+ * PSDRange) receives, ADDED, the reference's PoreSizeDistributionHistogram:
+ * the number of cycles whose largest sphere holding A reaches bin k or
+ * beyond (a cycle beyond the range counts nowhere).  largest (nullable,
+ * cycles doubles) receives each cycle's largest radius (-10: none).  Points
+ * come from the counter stream of (seed, cycle): A from counters 0..2,
+ * centre j from 3+3j..5+3j, each in x, y, z order. */
+int rhg_pore_size_distribution(rhg_ctx *ctx, double probe_factor, int64_t cycles, int64_t n_b, uint64_t seed,
+                               int32_t bins, double range, uint64_t *hist, double *largest);
+
 /* ------------------------------------------------------- flexible chains */
 /* A flexible molecule made of one flexible group (the alkane family of
  * molecules/TraPPE: ReadComponentDefinition, src/molecule.c:1430-3030).

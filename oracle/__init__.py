@@ -136,6 +136,8 @@ def _declare(L):
     f("orc_use_coulomb_grid", _I, _P, _P, _P, _P)
     f("orc_coulomb_grid_node", None, _P, _P, _I, _I, _I, _P)
     f("orc_blocked_pocket", _I, _P, _P)
+    f("orc_psd_largest", _D, _P, _D, _I64, _U64, _I64)
+    f("orc_psd_histogram", None, _P, _D, _I64, _U64, _I64, _I64, _I, _D, _P)
     f("orc_chain_plan", _I, _P, _P, _P)
     f("orc_chain_run_stream", None, _P, _P, _U64, _I64, _I64, _P, _P)
     f("orc_chain_run_mt", None, _P, _P, _U64, _I64, _P, _P)
@@ -578,3 +580,14 @@ def chain_torsion_energy(system, par, a, b, c_, d):
     p = _f64(par)
     aa, bb, cc, dd = _f64(a), _f64(b), _f64(c_), _f64(d)
     return lib().orc_chain_torsion_energy(system.h, 0, _ptr(p), _ptr(aa), _ptr(bb), _ptr(cc), _ptr(dd))
+
+
+# ------------------------------------------------------ pore-size distribution
+def psd_largest(system, factor, n_b, seed, i):
+    return lib().orc_psd_largest(system.h, factor, n_b, seed, i)
+
+
+def psd_histogram(system, factor, n_b, seed, first, n, bins, range_):
+    hist = np.zeros(bins)
+    lib().orc_psd_histogram(system.h, factor, n_b, seed, first, n, bins, range_, _ptr(hist))
+    return hist

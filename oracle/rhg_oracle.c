@@ -2792,3 +2792,72 @@ void orc_chain_run_mt(const orc_system *s, const orc_chain *c, uint64_t seed, in
   }
   free(w);
 }
+
+/* ===================================================================== */
+/* Pore-size distribution (SamplePoreSizeDistribution, src/sample.c:3444-  */
+/* 3560; CheckSphereOverlap :3376-3440; ComputePoreSizeDistribution        */
+/* src/monte_carlo.c:1515-1540)                                            */
+/* ===================================================================== */
+
+/* CheckSphereOverlap: 1 when pos is blocked or closer than
+ * 0.5 factor sigma_jj to a framework atom j (the first such atom ends the
+ * search); otherwise *radius = min_j |pos - r_j| - 0.5 factor sigma_jj */
+static int psd_overlap(const orc_system *s, double factor, const double pos[3], double *radius) {
+  *radius = DBL_MAX;
+  if (orc_blocked_pocket(s, pos)) return 1;
+  for (int j = 0; j < s->n; j++) {
+    double dr[3] = {pos[0] - s->x[j], pos[1] - s->y[j], pos[2] - s->z[j]};
+    orc_min_image(s->box, s->inv, s->bc, dr);
+    double rr = SQR(dr[0]) + SQR(dr[1]) + SQR(dr[2]);
+    if (rr < SQR(0.5 * factor * s->sig[j])) return 1;
+    double r = sqrt(rr) - 0.5 * factor * s->sig[j];
+    if (r < *radius) *radius = r;
+  }
+  return 0;
+}
+
+static void psd_point(const orc_system *s, double u0, double u1, double u2, double p[3]) {
+  /* ConvertFromABCtoXYZ (src/potentials.c:5548-5556) */
+  p[0] = s->box[AX] * u0 + s->box[BX] * u1 + s->box[CX] * u2;
+  p[1] = s->box[AY] * u0 + s->box[BY] * u1 + s->box[CY] * u2;
+  p[2] = s->box[AZ] * u0 + s->box[BZ] * u1 + s->box[CZ] * u2;
+}
+
+/* LargestRadius of cycle `i` (-10 when A overlaps or no sphere holds A):
+ * point A from counters 0..2 of the cycle's stream, centre B_j from counters
+ * 3+3j..5+3j, each in x, y, z order */
+double orc_psd_largest(const orc_system *s, double factor, int64_t n_b, uint64_t seed, int64_t i) {
+  uint64_t key = orc_stream_key(seed, (uint64_t)i);
+  double a[3], radius;
+  psd_point(s, orc_stream_uniform(key, 0), orc_stream_uniform(key, 1), orc_stream_uniform(key, 2), a);
+  if (psd_overlap(s, factor, a, &radius)) return -10.0;
+  double largest = -10.0;
+  for (int64_t j = 0; j < n_b; j++) {
+    double b[3];
+    psd_point(s, orc_stream_uniform(key, (uint64_t)(3 + 3 * j)), orc_stream_uniform(key, (uint64_t)(4 + 3 * j)),
+              orc_stream_uniform(key, (uint64_t)(5 + 3 * j)), b);
+    if (psd_overlap(s, factor, b, &radius)) continue;
+    double dr[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+    orc_min_image(s->box, s->inv, s->bc, dr);
+    double rr = SQR(dr[0]) + SQR(dr[1]) + SQR(dr[2]);
+    double r = sqrt(rr);
+    if (r > radius) continue;
+    if (radius > largest) largest = radius;
+  }
+  return largest;
+}
+
+/* PoreSizeDistributionHistogram after cycles [first, first+n): hist[k] (k <
+ * bins) counts the cycles whose largest radius falls in bin k or above, as
+ * the reference adds 1 to every bin up to the cycle's (a radius beyond the
+ * range adds nothing) */
+void orc_psd_histogram(const orc_system *s, double factor, int64_t n_b, uint64_t seed, int64_t first,
+                       int64_t n, int bins, double range, double *hist) {
+  double deltaR = range / bins;
+  for (int64_t i = first; i < first + n; i++) {
+    double largest = orc_psd_largest(s, factor, n_b, seed, i);
+    int index = (int)(largest / deltaR);
+    if ((index >= 0) && (index < bins))
+      for (int k = 0; k <= index; k++) hist[k] += 1.0;
+  }
+}

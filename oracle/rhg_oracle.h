@@ -328,6 +328,16 @@ double orc_chain_bend_energy(const orc_system *s, int kind, const double par[4],
 double orc_chain_torsion_energy(const orc_system *s, int kind, const double par[4], const double a[3],
                                 const double b[3], const double c[3], const double d[3]);
 
+/* ------------------------------------------------- pore-size distribution */
+/* SamplePoreSizeDistribution (src/sample.c:3444-3560) with CheckSphereOverlap
+ * (:3376-3440): factor = PSDProbeDistance (1 Sigma, 2^(1/6) Minimum), n_b
+ * sphere centres per cycle (20 x NumberOfCycles), counter-stream points.
+ * orc_psd_largest: LargestRadius of one cycle (-10: none); orc_psd_histogram
+ * ADDS the cycles' cumulative counts to hist[bins] */
+double orc_psd_largest(const orc_system *s, double factor, int64_t n_b, uint64_t seed, int64_t i);
+void orc_psd_histogram(const orc_system *s, double factor, int64_t n_b, uint64_t seed, int64_t first,
+                       int64_t n, int bins, double range, double *hist);
+
 #ifdef __cplusplus
 }
 #endif

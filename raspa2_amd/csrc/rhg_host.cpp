@@ -1851,6 +1851,64 @@ int rhg_widom_run(rhg_ctx *c, const rhg_molecule *mol, uint64_t seed, int64_t n_
   return widom_collect(c, blocks);
 }
 
+int rhg_pore_size_distribution(rhg_ctx *c, double probe_factor, int64_t cycles, int64_t n_b, uint64_t seed,
+                               int32_t bins, double range, uint64_t *hist, double *largest) {
+  if (!c || !hist) return RHG_ERR_INVALID;
+  if (cycles < 0 || n_b < 0 || bins < 1 || !(range > 0.0) || !(probe_factor > 0.0))
+    return fail(c, RHG_ERR_INVALID, "pore-size distribution: bad cycles, centres, bins, range or factor");
+  if (cycles == 0) return RHG_OK;
+  hipSetDevice(c->device);
+  hipStream_t st = c->stream;
+  const int n = c->n;
+  std::vector<double4> at((size_t)std::max(n, 1));
+  for (int j = 0; j < n; ++j) at[j] = make_double4(c->x[j], c->y[j], c->z[j], 0.5 * probe_factor * c->sig[j]);
+  double4 *d_at = (double4 *)work(c, "psd_atoms", at.size() * sizeof(double4));
+  const int64_t chunk = std::min<int64_t>(cycles, 32768);
+  constexpr int kPerThread = 8;
+  const int b_per_block = 256 * kPerThread;
+  const int blocks_per_cycle = (int)std::max<int64_t>(1, (n_b + b_per_block - 1) / b_per_block);
+  double *ax = (double *)work(c, "psd_a", (size_t)chunk * 4 * sizeof(double));
+  double *part = (double *)work(c, "psd_part", (size_t)chunk * blocks_per_cycle * sizeof(double));
+  double *lg = (double *)work(c, "psd_largest", (size_t)chunk * sizeof(double));
+  unsigned long long *cnt = (unsigned long long *)work(c, "psd_counts", (size_t)bins * sizeof(unsigned long long));
+  if (!d_at || !ax || !part || !lg || !cnt) return fail(c, RHG_ERR_OOM, "pore-size distribution buffers");
+  HIPC(c, hipMemcpyAsync(d_at, at.data(), at.size() * sizeof(double4), hipMemcpyHostToDevice, st));
+  fill((double *)cnt, bins, 0.0, st); // all-zero bits
+  PsdArgs a;
+  memset(&a, 0, sizeof(a));
+  a.cell = c->cell;
+  a.pockets = c->pockets;
+  a.atoms = d_at;
+  a.n = n;
+  a.seed = seed;
+  a.n_b = n_b;
+  a.b_per_block = b_per_block;
+  a.blocks_per_cycle = blocks_per_cycle;
+  a.bins = bins;
+  a.delta = range / bins;
+  for (int64_t f = 0; f < cycles; f += chunk) {
+    a.first = f;
+    a.n_cycles = std::min(chunk, cycles - f);
+    {
+      Timed t(c, "psd");
+      launch_psd(a, ax, part, lg, cnt, st);
+    }
+    if (largest)
+      HIPC(c, hipMemcpyAsync(largest + f, lg, (size_t)a.n_cycles * sizeof(double), hipMemcpyDeviceToHost, st));
+  }
+  HIPC(c, hipGetLastError());
+  std::vector<unsigned long long> h((size_t)bins);
+  HIPC(c, hipMemcpyAsync(h.data(), cnt, (size_t)bins * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+  HIPC(c, hipStreamSynchronize(st));
+  // the reference adds 1 to every bin up to a cycle's index (src/sample.c:3514-3516)
+  uint64_t acc = 0;
+  for (int k = bins - 1; k >= 0; --k) {
+    acc += h[k];
+    hist[k] += acc;
+  }
+  return RHG_OK;
+}
+
 int rhg_widom_chain_run(rhg_ctx *c, const rhg_chain *ch, uint64_t seed, int64_t n_total, int64_t first,
                         int64_t n, int32_t steps_per_cycle, rhg_widom_blocks *blocks, double *rec, double *xyz) {
   static_assert(kCrecN == RHG_CREC_N && kCrecStage == RHG_CREC_STAGE && kCrecW1 == RHG_CREC_W1,

@@ -331,6 +331,28 @@ void launch_interp(const InterpArgs &a, const double *xyz, int64_t n, double *ou
 void launch_site_grid(const InterpArgs *gv, const InterpArgs *gc, const SiteList &s, const EnergyOut &o,
                       int merge, hipStream_t st);
 
+// ------------------------------------------------ pore-size distribution
+// SamplePoreSizeDistribution / CheckSphereOverlap (src/sample.c:3376-3560):
+// per cycle a point A and n_b sphere centres B from the cycle's counter
+// stream; the largest free sphere (radius = distance to the nearest atom
+// minus half its scaled sigma) whose centre holds A.
+struct PsdArgs {
+  Cell cell;
+  PocketArgs pockets;
+  const double4 *atoms; // x, y, z, 0.5 factor sigma
+  int n;
+  uint64_t seed;
+  int64_t first, n_cycles, n_b;
+  int b_per_block;      // centres per block (blockDim.x x per-thread count)
+  int blocks_per_cycle;
+  int bins;
+  double delta;         // PSDRange / PSDHistogramSize
+};
+// largest[i] per cycle (-10 when none) via partial[i][blocks_per_cycle],
+// then counts[index] += 1 for the cycles whose index is in range
+void launch_psd(const PsdArgs &a, double *ax, double *partial, double *largest,
+                unsigned long long *counts, hipStream_t st);
+
 // ------------------------------------------------------ flexible chains
 // CBMC growth of a flexible molecule of one group (src/cbmc.c:2762-2849,
 // 1480-2482, 1154-1478).  The growth path is the same for every insertion;

@@ -3559,4 +3559,116 @@ void launch_chain_select(const ChainArgs &a, const ChainWork &w, hipStream_t st)
 void launch_chain_finalize(const ChainArgs &a, const ChainWork &w, hipStream_t st) {
   hipLaunchKernelGGL(k_chain_finalize, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, st, a, w);
 }
+
+// ====================================================================
+// Pore-size distribution (src/sample.c:3376-3560), FP64 in the
+// reference's operation order so that the counts equal the oracle's.
+// ====================================================================
+namespace psd {
+
+// ConvertFromABCtoXYZ of three uniforms
+__device__ __forceinline__ void point(const Cell &c, uint64_t key, uint64_t ctr, double p[3]) {
+  abc_to_xyz(c, stream_uniform(key, ctr), stream_uniform(key, ctr + 1), stream_uniform(key, ctr + 2), p[0], p[1],
+             p[2]);
+}
+
+// CheckSphereOverlap: true when blocked or within 0.5 factor sigma of an
+// atom; otherwise radius = min_j (|pos - r_j| - 0.5 factor sigma_j).  The
+// atom records are wave-uniform (scalar loads).
+__device__ bool overlap(const PsdArgs &a, const double p[3], double &radius) {
+#pragma clang fp contract(off)
+  radius = DBL_MAX;
+  if (blocked_pocket(a.pockets, p[0], p[1], p[2])) return true;
+  bool ov = false;
+  for (int j = 0; j < a.n; ++j) {
+    const double4 at = a.atoms[j];
+    double dx = p[0] - at.x, dy = p[1] - at.y, dz = p[2] - at.z;
+    min_image_exact(a.cell, dx, dy, dz);
+    const double rr = dx * dx + dy * dy + dz * dz;
+    if (rr < at.w * at.w) ov = true;
+    const double r = sqrt(rr) - at.w;
+    if (r < radius) radius = r;
+    if (__all(ov)) break;
+  }
+  return ov;
+}
+
+} // namespace psd
+
+__global__ __launch_bounds__(256) void k_psd_a(const PsdArgs a, double *__restrict__ ax) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n_cycles) return;
+  const uint64_t key = stream_key(a.seed, (uint64_t)(a.first + i));
+  double p[3], radius;
+  psd::point(a.cell, key, 0, p);
+  const bool ov = psd::overlap(a, p, radius);
+  ax[4 * i + 0] = p[0];
+  ax[4 * i + 1] = p[1];
+  ax[4 * i + 2] = p[2];
+  ax[4 * i + 3] = ov ? 1.0 : 0.0;
+}
+
+// blockIdx.y = cycle, blockIdx.x = block of centres; partial[cycle][block]
+// = the largest radius of the block's centres that hold A (-10 if none)
+__global__ __launch_bounds__(256) void k_psd_b(const PsdArgs a, const double *__restrict__ ax,
+                                               double *__restrict__ partial) {
+#pragma clang fp contract(off)
+  const int64_t i = blockIdx.y;
+  const int64_t out = i * a.blocks_per_cycle + blockIdx.x;
+  if (ax[4 * i + 3] != 0.0) {
+    if (threadIdx.x == 0) partial[out] = -10.0;
+    return;
+  }
+  const double A[3] = {ax[4 * i + 0], ax[4 * i + 1], ax[4 * i + 2]};
+  const uint64_t key = stream_key(a.seed, (uint64_t)(a.first + i));
+  double largest = -10.0;
+  const int64_t j0 = (int64_t)blockIdx.x * a.b_per_block;
+  for (int64_t j = j0 + threadIdx.x; j < j0 + a.b_per_block; j += blockDim.x) {
+    // every lane runs the atom loop (wave-uniform trip count), out-of-range
+    // centres are discarded afterwards
+    const bool live = j < a.n_b;
+    double B[3], radius;
+    psd::point(a.cell, key, (uint64_t)(3 + 3 * (live ? j : 0)), B);
+    const bool ov = psd::overlap(a, B, radius);
+    if (!live || ov) continue;
+    double dx = A[0] - B[0], dy = A[1] - B[1], dz = A[2] - B[2];
+    min_image_exact(a.cell, dx, dy, dz);
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    if (r > radius) continue;
+    if (radius > largest) largest = radius;
+  }
+  for (int s = 32; s > 0; s >>= 1) largest = fmax(largest, __shfl_xor(largest, s));
+  __shared__ double wmax[4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) wmax[wv] = largest;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = wmax[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, wmax[w]);
+    partial[out] = m;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_psd_bin(const PsdArgs a, const double *__restrict__ partial,
+                                                 double *__restrict__ largest, unsigned long long *__restrict__ counts) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n_cycles) return;
+  double m = -10.0;
+  for (int b = 0; b < a.blocks_per_cycle; ++b) m = fmax(m, partial[i * a.blocks_per_cycle + b]);
+  largest[i] = m;
+  const int index = (int)(m / a.delta);
+  if ((index >= 0) && (index < a.bins)) atomicAdd(&counts[index], 1ULL);
+}
+
+void launch_psd(const PsdArgs &a, double *ax, double *partial, double *largest, unsigned long long *counts,
+                hipStream_t st) {
+  if (a.n_cycles <= 0) return;
+  hipLaunchKernelGGL(k_psd_a, dim3((unsigned)((a.n_cycles + 255) / 256)), dim3(256), 0, st, a, ax);
+  hipLaunchKernelGGL(k_psd_b, dim3((unsigned)a.blocks_per_cycle, (unsigned)a.n_cycles), dim3(256), 0, st, a,
+                     (const double *)ax, partial);
+  hipLaunchKernelGGL(k_psd_bin, dim3((unsigned)((a.n_cycles + 255) / 256)), dim3(256), 0, st, a,
+                     (const double *)partial, largest, counts);
+}
 } // namespace rhg

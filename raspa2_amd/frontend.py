@@ -479,6 +479,53 @@ def ideal_gas_rosenbluth_weight(molecule, forcefield="CrystalGenerator", tempera
         ctx.close()
 
 
+def psd_columns(hist, cycles, range_):
+    """The three columns of the reference's PSD histogram file
+    (src/sample.c:3552-3575): diameter 2 delta (k + 1/2), the cumulative
+    fraction hist[k] / cycles, and minus its finite-difference derivative
+    (one-sided at the ends, central inside)."""
+    hist = np.asarray(hist, dtype=np.float64)
+    bins = len(hist)
+    delta = range_ / bins
+    d, mids, psd = [], [], []
+    for k in range(bins):
+        mid = hist[k] / cycles
+        if k == 0:
+            deriv = (hist[k + 1] / cycles - mid) / delta
+        elif k == bins - 1:
+            deriv = (mid - hist[k - 1] / cycles) / delta
+        else:
+            deriv = (hist[k + 1] / cycles - hist[k - 1] / cycles) / (2.0 * delta)
+        d.append(2.0 * delta * (k + 0.5))
+        mids.append(mid)
+        psd.append(-deriv)
+    return np.array(d), np.array(mids), np.array(psd)
+
+
+def get_pore_size_distribution(structure, unit_cells=(1, 1, 1), cycles=500, input_file_type="cif",
+                               forcefield="CrystalGenerator", bins=50, raspa_dir=None, seed=1,
+                               probe_distance="Sigma", psd_range=12.5, device=0):
+    """The wrapper's pore-size distribution (python/raspa2.py:390-441:
+    SimulationType PSD, PSDProbeDistance Sigma, PSDRange 12.5,
+    PSDHistogramSize `bins`, 20 x cycles sphere centres per cycle) on the
+    device.  Returns [[d1..dn], [psd1..psdn]]: pore diameter [A] and the PSD
+    column of the reference's histogram file."""
+    if input_file_type.lower() != "cif":
+        raise NotImplementedError("only CIF structures")
+    share = _share(raspa_dir)
+    ff = ForceField.load(os.path.join(share, "forcefield", forcefield))
+    fw = framework_from_cif(structure, ff, unit_cells)
+    factor = 2.0 ** (1.0 / 6.0) if probe_distance.lower() == "minimum" else 1.0
+    ctx = fw.context(device=device, precision=R.PREC_FP64, shifted=ff.shifted)
+    try:
+        hist = ctx.PoreSizeDistribution(cycles, 20 * cycles, seed=seed, bins=bins, range_=psd_range,
+                                        probe_factor=factor)
+    finally:
+        ctx.close()
+    d, _, psd = psd_columns(hist, cycles, psd_range)
+    return [list(d), list(psd)]
+
+
 def get_helium_void_fraction(structure, unit_cells=(1, 1, 1), cycles=2000, input_file_type="cif",
                              forcefield="CrystalGenerator", raspa_dir=None, insertions=None, seed=1,
                              device=0):

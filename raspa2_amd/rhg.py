@@ -165,6 +165,7 @@ def _load():
     f("rhg_framework_energy_at_positions", _P, _I64, _P, _I32, _P, _P, _P)
     f("rhg_ewald_tables", _P, _P, _P, _P)
     f("rhg_ewald_fourier_adsorbate", _P, _I64, _I32, _P, _P, _P, _P)
+    f("rhg_pore_size_distribution", _P, _D, _I64, _I64, _U64, _I32, _D, _P, _P)
     f("rhg_widom_chain_run", _P, ctypes.POINTER(ChainT), _U64, _I64, _I64, _I64, _I32,
       ctypes.POINTER(WidomBlocks), _P, _P)
     f("rhg_widom_run", _P, ctypes.POINTER(MoleculeT), _U64, _I64, _I64, _I64, _I32,
@@ -470,6 +471,19 @@ class Context:
                                             steps_per_cycle, ctypes.byref(blocks), _ptr(rec), _ptr(xyz)),
                     "widom_chain_run")
         return blocks, rec, xyz
+
+    def PoreSizeDistribution(self, cycles, n_b=None, seed=1, bins=50, range_=12.5, probe_factor=1.0,
+                             largest=False):
+        """SamplePoreSizeDistribution over `cycles` points with n_b sphere
+        centres each (default 20 x cycles, the reference's count): the
+        cumulative histogram PoreSizeDistributionHistogram[bins] and, when
+        asked, each cycle's largest radius."""
+        n_b = 20 * cycles if n_b is None else n_b
+        hist = np.zeros(bins, dtype=np.uint64)
+        lg = np.zeros(cycles) if largest else None
+        self._check(lib.rhg_pore_size_distribution(self.h, probe_factor, cycles, n_b, seed, bins, range_,
+                                                   _ptr(hist), _ptr(lg)), "pore_size_distribution")
+        return (hist, lg) if largest else hist
 
     # ---- src/framework.c:6168-6300 blocked pockets
     def set_block_pockets(self, centres, radii, unit_box, invert=False):
