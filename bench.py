@@ -573,6 +573,63 @@ def chain_bench(no_cpu=False, n=1 << 18, steps=3, cpu_sample=1500):
     return out
 
 
+FLOP_PSD_PAIR = 31  # per (sphere centre, atom): 3 sub, rectangular minimum image 18, r^2 5, cut test 2, sqrt, sub, min
+
+
+def psd_bench(no_cpu=False):
+    """The reference's pore-size distribution example (examples/Auxiliary/PSD/
+    IRMOF-1): IRMOF-1 with GenericMOFs sigmas, PSDProbeDistance Sigma, 10^4
+    cycles of 2 x 10^5 sphere centres each, 1000 bins over 10 A, FP64.  Rate
+    in sphere centres per second (each tested against all 424 atoms); the
+    oracle runs three cycles of the same job on one core."""
+    import json as _json
+    import numpy as np
+    import oracle as O
+    from raspa2_amd import frontend as F
+    from raspa2_amd import rhg as R
+    gold = os.path.join(ROOT, "tests", "golden")
+    G = _json.load(open(os.path.join(gold, "psd_irmof1.json")))
+    ff = F.ForceField.load(os.path.join(gold, "share", "raspa", "forcefield", "GenericMOFs"))
+    cif = open(os.path.join(gold, "share", "raspa", "structures", "cif", "IRMOF-1.cif")).read()
+    fw = F.framework_from_cif(cif, ff)
+    ctx = fw.context(precision=R.PREC_FP64)
+    n = G["cycles"]
+    ctx.PoreSizeDistribution(200, 20 * n, seed=2, bins=G["bins"], range_=G["range"])  # warm-up
+    ctx.profile(True)
+    ctx.profile_reset()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    hist = ctx.PoreSizeDistribution(n, 20 * n, seed=12, bins=G["bins"], range_=G["range"])
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    cnt, ms = ctx.profile_get("psd")
+    ctx.profile(False)
+    centres = float(n) * 20 * n
+    pairs = centres * fw.n
+    ours, ref = hist / n, np.array(G["cumulative"])
+    out = {"workload": "IRMOF-1 pore-size distribution (examples/Auxiliary/PSD/IRMOF-1): 424 atoms, "
+                       "GenericMOFs sigma, probe distance Sigma, 10^4 cycles x 2x10^5 sphere centres, "
+                       "1000 bins over 10 A, FP64",
+           "seconds": round(el, 3), "centres_per_s": round(centres / el, 1),
+           "centre_atom_pairs_per_s": round(pairs / el, 1),
+           "roofline": {"bound": "valu", "kernel": "k_psd_b", "achieved": round(pairs * FLOP_PSD_PAIR / (ms * 1e-3) / 1e12, 3),
+                        "peak": 78.6, "unit": "TFLOP/s (FP64)",
+                        "frac": round(pairs * FLOP_PSD_PAIR / (ms * 1e-3) / 1e12 / 78.6, 4),
+                        "note": f"{FLOP_PSD_PAIR} flops per centre-atom pair over the HIP-event time of the PSD launches"},
+           "void_fraction": float(ours[0]), "void_fraction_printed": float(ref[0]),
+           "max_abs_diff_cumulative": float(np.max(np.abs(ours - ref)))}
+    if not no_cpu:
+        s = O.System(fw.box, fw.bc, fw.xyz[:, 0], fw.xyz[:, 1], fw.xyz[:, 2], fw.charge, fw.eps_k, fw.sigma,
+                     mass=fw.mass, cutoff_vdw=12.0)
+        t1 = time.perf_counter()
+        O.psd_histogram(s, 1.0, 20 * n, 12, 0, 3, G["bins"], G["range"])
+        cpu = 3 * 20 * n / (time.perf_counter() - t1)
+        out["cpu_baseline"] = {"value": round(cpu, 1), "unit": "sphere centres/s", "cores": 1, "kind": "port",
+                               "sample": "cycles 0..2 of the same job (6x10^5 centres), oracle FP64 C restatement"}
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -582,7 +639,7 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--cpu-sample", type=int, default=400, help="insertions per CPU process")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "chain"], default="c1")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "chain", "psd"], default="c1")
     ap.add_argument("--grid-samples", type=int, default=100_000, help="c2 off-grid checks")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 512 MiB device copy kernel (k_stream_copy) three "
@@ -599,6 +656,9 @@ def main():
 
     if args.workload == "c2":
         grid_bench(args)
+        return
+    if args.workload == "psd":
+        print(json.dumps(psd_bench(no_cpu=args.no_cpu)))
         return
     if args.workload == "chain":
         n = min(args.insertions, 1 << 20)
@@ -761,6 +821,7 @@ def main():
         "kernels": kernels,
         "finite_loading": loaded_bench(fw, probe),
         "flexible_chain": chain_bench(no_cpu=args.no_cpu),
+        "pore_size_distribution": psd_bench(no_cpu=args.no_cpu),
     }
     print(json.dumps(out))
 

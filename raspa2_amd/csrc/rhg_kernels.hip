@@ -3572,22 +3572,55 @@ __device__ __forceinline__ void point(const Cell &c, uint64_t key, uint64_t ctr,
              p[2]);
 }
 
+// ApplyBoundaryCondition specialised by boundary: the rectangular form keeps
+// six cell constants live through the atom loop (the same operations as
+// min_image_exact)
+template <int BC>
+struct Image {
+  Cell c;
+  double bx, by, bz, ix, iy, iz;
+  __device__ explicit Image(const Cell &cell) : c(cell) {
+    bx = cell.box[AX];
+    by = cell.box[BY];
+    bz = cell.box[CZ];
+    ix = cell.inv[AX];
+    iy = cell.inv[BY];
+    iz = cell.inv[CZ];
+  }
+  __device__ __forceinline__ void apply(double &dx, double &dy, double &dz) const {
+#pragma clang fp contract(off)
+    if (BC == 1) {
+      dx -= bx * nint_d(dx * ix);
+      dy -= by * nint_d(dy * iy);
+      dz -= bz * nint_d(dz * iz);
+    } else
+      min_image_exact(c, dx, dy, dz);
+  }
+};
+
 // CheckSphereOverlap: true when blocked or within 0.5 factor sigma of an
 // atom; otherwise radius = min_j (|pos - r_j| - 0.5 factor sigma_j).  The
 // atom records are wave-uniform (scalar loads).
-__device__ bool overlap(const PsdArgs &a, const double p[3], double &radius) {
+template <int BC, int POCK>
+__device__ bool overlap(const PsdArgs &a, const Image<BC> &im, const double p[3], double &radius) {
 #pragma clang fp contract(off)
   radius = DBL_MAX;
-  if (blocked_pocket(a.pockets, p[0], p[1], p[2])) return true;
+  if (POCK && blocked_pocket(a.pockets, p[0], p[1], p[2])) return true;
   bool ov = false;
   for (int j = 0; j < a.n; ++j) {
     const double4 at = a.atoms[j];
     double dx = p[0] - at.x, dy = p[1] - at.y, dz = p[2] - at.z;
-    min_image_exact(a.cell, dx, dy, dz);
+    im.apply(dx, dy, dz);
     const double rr = dx * dx + dy * dy + dz * dz;
     if (rr < at.w * at.w) ov = true;
-    const double r = sqrt(rr) - at.w;
-    if (r < radius) radius = r;
+    // sqrt(rr) - w can undercut the running minimum only when rr < (radius +
+    // w)^2; the relative margin keeps every atom within rounding of that
+    // bound, so the minimum (and the result) is the one of all atoms
+    const double lim = radius + at.w;
+    if (rr < lim * lim * (1.0 + 1e-9)) {
+      const double r = sqrt(rr) - at.w;
+      if (r < radius) radius = r;
+    }
     if (__all(ov)) break;
   }
   return ov;
@@ -3595,14 +3628,16 @@ __device__ bool overlap(const PsdArgs &a, const double p[3], double &radius) {
 
 } // namespace psd
 
+template <int BC, int POCK>
 __global__ __launch_bounds__(256) void k_psd_a(const PsdArgs a, double *__restrict__ ax) {
 #pragma clang fp contract(off)
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n_cycles) return;
   const uint64_t key = stream_key(a.seed, (uint64_t)(a.first + i));
+  const psd::Image<BC> im(a.cell);
   double p[3], radius;
   psd::point(a.cell, key, 0, p);
-  const bool ov = psd::overlap(a, p, radius);
+  const bool ov = psd::overlap<BC, POCK>(a, im, p, radius);
   ax[4 * i + 0] = p[0];
   ax[4 * i + 1] = p[1];
   ax[4 * i + 2] = p[2];
@@ -3611,6 +3646,7 @@ __global__ __launch_bounds__(256) void k_psd_a(const PsdArgs a, double *__restri
 
 // blockIdx.y = cycle, blockIdx.x = block of centres; partial[cycle][block]
 // = the largest radius of the block's centres that hold A (-10 if none)
+template <int BC, int POCK>
 __global__ __launch_bounds__(256) void k_psd_b(const PsdArgs a, const double *__restrict__ ax,
                                                double *__restrict__ partial) {
 #pragma clang fp contract(off)
@@ -3622,6 +3658,7 @@ __global__ __launch_bounds__(256) void k_psd_b(const PsdArgs a, const double *__
   }
   const double A[3] = {ax[4 * i + 0], ax[4 * i + 1], ax[4 * i + 2]};
   const uint64_t key = stream_key(a.seed, (uint64_t)(a.first + i));
+  const psd::Image<BC> im(a.cell);
   double largest = -10.0;
   const int64_t j0 = (int64_t)blockIdx.x * a.b_per_block;
   for (int64_t j = j0 + threadIdx.x; j < j0 + a.b_per_block; j += blockDim.x) {
@@ -3630,10 +3667,10 @@ __global__ __launch_bounds__(256) void k_psd_b(const PsdArgs a, const double *__
     const bool live = j < a.n_b;
     double B[3], radius;
     psd::point(a.cell, key, (uint64_t)(3 + 3 * (live ? j : 0)), B);
-    const bool ov = psd::overlap(a, B, radius);
+    const bool ov = psd::overlap<BC, POCK>(a, im, B, radius);
     if (!live || ov) continue;
     double dx = A[0] - B[0], dy = A[1] - B[1], dz = A[2] - B[2];
-    min_image_exact(a.cell, dx, dy, dz);
+    im.apply(dx, dy, dz);
     const double r = sqrt(dx * dx + dy * dy + dz * dz);
     if (r > radius) continue;
     if (radius > largest) largest = radius;
@@ -3662,12 +3699,25 @@ __global__ __launch_bounds__(256) void k_psd_bin(const PsdArgs a, const double *
   if ((index >= 0) && (index < a.bins)) atomicAdd(&counts[index], 1ULL);
 }
 
+template <int BC, int POCK>
+static void psd_launch(const PsdArgs &a, double *ax, double *partial, hipStream_t st) {
+  hipLaunchKernelGGL((k_psd_a<BC, POCK>), dim3((unsigned)((a.n_cycles + 255) / 256)), dim3(256), 0, st, a, ax);
+  hipLaunchKernelGGL((k_psd_b<BC, POCK>), dim3((unsigned)a.blocks_per_cycle, (unsigned)a.n_cycles), dim3(256), 0,
+                     st, a, (const double *)ax, partial);
+}
+
 void launch_psd(const PsdArgs &a, double *ax, double *partial, double *largest, unsigned long long *counts,
                 hipStream_t st) {
   if (a.n_cycles <= 0) return;
-  hipLaunchKernelGGL(k_psd_a, dim3((unsigned)((a.n_cycles + 255) / 256)), dim3(256), 0, st, a, ax);
-  hipLaunchKernelGGL(k_psd_b, dim3((unsigned)a.blocks_per_cycle, (unsigned)a.n_cycles), dim3(256), 0, st, a,
-                     (const double *)ax, partial);
+  const bool rect = a.cell.bc == 1, pock = a.pockets.n > 0;
+  if (rect && !pock)
+    psd_launch<1, 0>(a, ax, partial, st);
+  else if (rect)
+    psd_launch<1, 1>(a, ax, partial, st);
+  else if (!pock)
+    psd_launch<2, 0>(a, ax, partial, st);
+  else
+    psd_launch<2, 1>(a, ax, partial, st);
   hipLaunchKernelGGL(k_psd_bin, dim3((unsigned)((a.n_cycles + 255) / 256)), dim3(256), 0, st, a,
                      (const double *)partial, largest, counts);
 }
