@@ -105,7 +105,7 @@ struct PseudoAtom {
   AtomRec64 *d64 = nullptr;
   AtomRecMx *dmx = nullptr;
   double *c24 = nullptr, *c96 = nullptr, *c384 = nullptr;
-  float4 *par = nullptr; // {4 eps, sigma^2 * kFix^2, shift, q} for the image-list kernels
+  float4 *par = nullptr; // {4 eps sigma^12, 4 eps sigma^6, shift, q} for the image-list kernels
   AtomRec64 *dads = nullptr; // the adsorbate atoms as seen by this type (finite loading)
   size_t dads_cap = 0;
 };
@@ -159,7 +159,7 @@ struct rhg_ctx {
   CellGrid cg;
   int32_t *d_nb_off = nullptr, *d_nb_idx = nullptr;
   int32_t *d_img_off = nullptr;
-  int4 *d_img = nullptr;
+  float4 *d_img = nullptr;
   bool img_ready = false;
   int use_images = 1; // mixed precision walks the image lists when they exist
   double cell_reach = 0.0, cell_halfdiag = 0.0;
@@ -447,7 +447,8 @@ int setup_pseudo_atom(rhg_ctx *c, PseudoAtom &t) {
     h24[i] = 24.0 * e;
     h96[i] = 96.0 * e;
     h384[i] = 384.0 * e;
-    hpar[i] = make_float4((float)r.c4, (float)(r.s2 * kFix * kFix), (float)r.shift, (float)r.q);
+    const double s6 = r.s2 * r.s2 * r.s2;
+    hpar[i] = make_float4((float)(r.c4 * s6 * s6), (float)(r.c4 * s6), (float)r.shift, (float)r.q);
   }
   size_t n = h64.size();
   int rc;
@@ -643,9 +644,9 @@ int ensure_cell_grid(rhg_ctx *c) {
 
 // Image lists of the cell grid for the mixed-precision kernels.  They hold
 // every image within reach, which equals the minimum-image convention when
-// every perpendicular width is at least twice the cut-off; the fixed-point
-// offsets need |site - image| < 32 A, and the 27 lattice shifts must cover
-// the reach.  Otherwise the lists stay off and the mixed kernels take the
+// every perpendicular width is at least twice the cut-off; the FP32 offsets
+// stay below 31 A (rounding below 1e-6 A), and the 27 lattice shifts must
+// cover the reach.  Otherwise the lists stay off and the mixed kernels take the
 // minimum image per pair.
 int ensure_image_list(rhg_ctx *c) {
   if (c->img_ready) return RHG_OK;
@@ -670,7 +671,7 @@ int ensure_image_list(rhg_ctx *c) {
   HIPC(c, hipStreamSynchronize(st));
   hipFree(c->d_img);
   c->d_img = nullptr;
-  if ((rc_ = dalloc(c, (void **)&c->d_img, (size_t)std::max(total, 1) * sizeof(int4)))) return rc_;
+  if ((rc_ = dalloc(c, (void **)&c->d_img, (size_t)std::max(total, 1) * sizeof(float4)))) return rc_;
   launch_image_list(c->cell, c->cg, c->cell_reach, c->props + 3, c->d_ax, c->d_ay, c->d_az, counts,
                     c->d_img_off, c->d_img, 1, st);
   HIPC(c, hipGetLastError());
@@ -735,10 +736,10 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
     g.img_off = nullptr;
     g.img = nullptr;
   }
-  // the image-list kernels read the sites' sorted fixed-point offsets
+  // the image-list kernels read the sites' sorted FP32 offsets
   b.fix = nullptr;
   if (a.precision != RHG_PREC_FP64 && g.img_off && a.par) {
-    b.fix = (int4 *)work(c, (t + "_fix").c_str(), (size_t)s.n * sizeof(int4));
+    b.fix = (float4 *)work(c, (t + "_fix").c_str(), (size_t)s.n * sizeof(float4));
     if (!b.fix) return fail(c, RHG_ERR_OOM, "binning scratch");
   }
   bool fresh = c->bufs["pair_counter"].p == nullptr;

@@ -46,20 +46,16 @@ void launch_site_energy(const EnergyArgs &a, const SiteList &s, const EnergyOut 
 // neighbour list of a cell holds, in ascending order, every atom whose
 // periodic (torus) distance to the cell centre is at most rc + R_cell, so it
 // contains every atom within rc of any point of the cell.
-// Fixed-point scale of the image lists: offsets from a cell centre in units
-// of 2^-26 A (int32 covers +-32 A), so that a site-atom difference is exact
-// and rounds once when converted to FP32.
-constexpr double kFix = 67108864.0;
-
 struct CellGrid {
   int nc[3];
   int ncell;
   const int32_t *nb_off; // ncell + 1
   const int32_t *nb_idx; // concatenated ascending atom indices
   // image lists (nullable): per cell every periodic image of every atom
-  // within reach of the cell centre, {dx, dy, dz, atom} in fixed point
+  // within reach of the cell centre, {dx, dy, dz} in A as FP32 and the atom
+  // index bit-cast into w
   const int32_t *img_off; // ncell + 1
-  const int4 *img;
+  const float4 *img;
 };
 
 // scratch for binning one site list into cells
@@ -71,8 +67,8 @@ struct BinWork {
   int32_t *tile_off; // per cell + 1, exclusive scan of ceil(count/64)
   int32_t *perm;     // sorted position -> site index
   // optional (image-list kernels): per sorted position the site's offset from
-  // its cell centre in fixed point and the site index, {dx, dy, dz, site}
-  int4 *fix;
+  // its cell centre in A as FP32 and the site index bit-cast into w
+  float4 *fix;
   unsigned long long *pairs; // executed pair evaluations [LJ, LJ+Coulomb, Coulomb]
 };
 
@@ -86,7 +82,7 @@ void launch_scan_counts(const int32_t *counts, int n, int32_t *off, hipStream_t 
 // widths: perpendicular widths of the cell
 void launch_image_list(const Cell &cell, const CellGrid &g, double reach, const double widths[3],
                        const double *ax, const double *ay, const double *az, int32_t *counts,
-                       const int32_t *off, int4 *img, int pass, hipStream_t st);
+                       const int32_t *off, float4 *img, int pass, hipStream_t st);
 void launch_bin_sites(const Cell &cell, const CellGrid &g, const SiteList &s, const BinWork &b,
                       hipStream_t st);
 void launch_site_energy_culled(const EnergyArgs &a, const CellGrid &g, const SiteList &s,
@@ -300,7 +296,7 @@ void launch_grid_build(const GridArgs &g, hipStream_t st);
 void launch_grid_nodes(const GridArgs &g, double *x, double *y, double *z, hipStream_t st);
 // culled build of the same nodes, binned as site list s (site = node - node0):
 // precision 0 walks the neighbour lists in atom order (bit-identical to
-// launch_grid_build), 1 the fixed-point image lists with FP32 terms
+// launch_grid_build), 1 the image lists with FP32 terms
 void launch_grid_culled(const GridArgs &g, const CellGrid &cg, const SiteList &s, const BinWork &b,
                         const float4 *par, int precision, hipStream_t st);
 void launch_derivatives(const GridArgs &g, const double *xyz, int64_t n, double *out,

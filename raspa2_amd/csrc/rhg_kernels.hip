@@ -408,7 +408,7 @@ __global__ __launch_bounds__(256) void k_image_list(const Cell cell, const CellG
                                                     const double *__restrict__ az,
                                                     int32_t *__restrict__ counts,
                                                     const int32_t *__restrict__ off,
-                                                    int4 *__restrict__ img, int pass) {
+                                                    float4 *__restrict__ img, int pass) {
   __shared__ int part[256];
   __shared__ int base;
   const int c = blockIdx.x;
@@ -465,7 +465,7 @@ __global__ __launch_bounds__(256) void k_image_list(const Cell cell, const CellG
           for (int e = 0; e < nc[2]; ++e) {
             double vx, vy, vz;
             if (image(a, b, e, vx, vy, vz))
-              img[o++] = make_int4((int)rint(vx * kFix), (int)rint(vy * kFix), (int)rint(vz * kFix), j);
+              img[o++] = make_float4((float)vx, (float)vy, (float)vz, __int_as_float(j));
           }
     __syncthreads();
     if (threadIdx.x == blockDim.x - 1) base += part[threadIdx.x];
@@ -476,7 +476,7 @@ __global__ __launch_bounds__(256) void k_image_list(const Cell cell, const CellG
 
 void launch_image_list(const Cell &cell, const CellGrid &g, double reach, const double widths[3],
                        const double *ax, const double *ay, const double *az, int32_t *counts,
-                       const int32_t *off, int4 *img, int pass, hipStream_t st) {
+                       const int32_t *off, float4 *img, int pass, hipStream_t st) {
   hipLaunchKernelGGL(k_image_list, dim3(g.ncell), dim3(256), 0, st, cell, g, reach * reach, widths[0],
                      widths[1], widths[2], ax, ay, az, counts, off, img, pass);
 }
@@ -563,7 +563,7 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const Cell cell, const Cell
     cell_centre(cell, g, c, cx, cy, cz);
     double ox = s.x[site] - cx, oy = s.y[site] - cy, oz = s.z[site] - cz;
     mi_exact<BCK>(cell, ox, oy, oz);
-    b.fix[pos] = make_int4((int)rint(ox * kFix), (int)rint(oy * kFix), (int)rint(oz * kFix), (int)site);
+    b.fix[pos] = make_float4((float)ox, (float)oy, (float)oz, __int_as_float((int)site));
   }
 }
 
@@ -728,11 +728,14 @@ __global__ __launch_bounds__(256) void k_site_energy_culled(
 
 // Mixed-precision site energies over the image lists: one wave per tile of
 // <= 64 sites of one cell.  The site's offset from the cell centre and the
-// listed image offsets are fixed point, so their difference is exact; it is
-// converted to FP32 once, and the LJ / real-space Coulomb terms are FP32
-// with FP64 accumulation.  Sites whose sum lands within the guard band of
-// the overlap threshold are recomputed exactly (FP64, atom order) so the
-// overlap flags equal the FP64 path's.
+// listed image offsets are FP32 (both rounded once from FP64, |offset| below
+// the cut-off plus the cell's half diagonal), so a pair's separation costs
+// one subtraction per axis; the LJ term is c12 ir^6 - c6 ir^3 - shift and the
+// real-space Coulomb term q_j erfc(alpha r)/r in FP32, four terms summed in
+// FP32 and the groups in FP64 (the Coulomb prefactor applied there).  Sites
+// whose sum lands within the guard band of the overlap threshold are
+// recomputed exactly (FP64, atom order) so the overlap flags equal the FP64
+// path's.
 template <int BCK, int VDW, int CC>
 __global__ __launch_bounds__(256) void k_site_energy_img(
     const Cell cell, const AtomRec64 *__restrict__ a64, const float4 *__restrict__ par,
@@ -751,24 +754,28 @@ __global__ __launch_bounds__(256) void k_site_energy_img(
   const int c = __builtin_amdgcn_readfirstlane(lo);
   const int k = (tile - b.tile_off[c]) * 64 + lane;
   const bool live = k < b.count[c];
-  // sorted fixed-point offsets from the cell centre (written by the binning)
-  const int4 fx = b.fix[b.start[c] + (live ? k : 0)];
-  const int site = fx.w;
-  const int ix = fx.x, iy = fx.y, iz = fx.z;
-  const float rc2vf = (float)(rc2v * kFix * kFix), rc2cf = (float)(rc2c * kFix * kFix);
-  const float alphaf = (float)(alpha / kFix), cqaf = (float)(cqa * kFix);
+  // sorted offsets from the cell centre (written by the binning)
+  const float4 so = b.fix[b.start[c] + (live ? k : 0)];
+  const int site = __float_as_int(so.w);
+  const float rc2vf = (float)rc2v, rc2cf = (float)rc2c, alphaf = (float)alpha;
   const int q0 = g.img_off[c], q1 = g.img_off[c + 1];
   double uv = 0.0, uc = 0.0;
-  // one pair term in FP32 (zero beyond the cut-off)
-  auto term = [&](const int4 e, float &fv, float &fc) {
-    const float4 pr = par[e.w];
-    const float dx = (float)(ix - e.x), dy = (float)(iy - e.y), dz = (float)(iz - e.z);
+  // one pair term in FP32 (zero beyond the cut-off); the image record and
+  // its parameters are wave-uniform (SGPR operands), and the LJ form keeps
+  // one SGPR per instruction
+  auto term = [&](const float4 e, float &fv, float &fc) {
+    const float4 pr = par[__float_as_int(e.w)];
+    const float dx = so.x - e.x, dy = so.y - e.y, dz = so.z - e.z;
     const float rr = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     if (VDW) {
       const float ir = __builtin_amdgcn_rcpf(rr);
-      const float t = pr.y * ir;
-      const float r3 = (t * t) * t;
-      const float u = fmaf(pr.x, fmaf(r3, r3, -r3), -pr.z);
+      const float ir3 = (ir * ir) * ir;
+      float y = pr.x * ir3;
+      {
+#pragma clang fp contract(off)
+        y = y - pr.y;
+      }
+      const float u = fmaf(ir3, y, -pr.z);
       fv = (rr < rc2vf) ? u : 0.0f;
     }
     if (CC) {
@@ -776,7 +783,7 @@ __global__ __launch_bounds__(256) void k_site_energy_img(
       if (rr < rc2cf) {
         const float irs = __builtin_amdgcn_rsqf(rr);
         const float e2 = erfc_mixed(alphaf * (rr * irs));
-        fc = (cqaf * pr.w) * (e2 * irs);
+        fc = pr.w * (e2 * irs);
       }
     }
   };
@@ -789,13 +796,13 @@ __global__ __launch_bounds__(256) void k_site_energy_img(
     term(g.img[q + 2], v2, c2);
     term(g.img[q + 3], v3, c3);
     if (VDW) uv += (double)((v0 + v1) + (v2 + v3));
-    if (CC) uc += (double)((c0 + c1) + (c2 + c3));
+    if (CC) uc = fma(cqa, (double)((c0 + c1) + (c2 + c3)), uc);
   }
   for (; q < q1; ++q) {
     float v0 = 0.f, c0 = 0.f;
     term(g.img[q], v0, c0);
     if (VDW) uv += (double)v0;
-    if (CC) uc += (double)c0;
+    if (CC) uc = fma(cqa, (double)c0, uc);
   }
   const double band = 1.0e-4 * kOverlap;
   const bool near = live && ((VDW && fabs(uv - kOverlap) <= band) || (CC && fabs(uc - kOverlap) <= band));
@@ -2195,8 +2202,8 @@ __global__ __launch_bounds__(256) void k_grid_culled(const GridArgs g, const Cel
     cell_centre(g.cell, cg, c, cx, cy, cz);
     double ox = px - cx, oy = py - cy, oz = pz - cz;
     mi_exact<BCK>(g.cell, ox, oy, oz);
-    const int ix = (int)rint(ox * kFix), iy = (int)rint(oy * kFix), iz = (int)rint(oz * kFix);
-    const float rc2f = (float)g.rc2, unfix = (float)(1.0 / kFix), s2unfix = (float)(1.0 / (kFix * kFix));
+    const float fx = (float)ox, fy = (float)oy, fz = (float)oz;
+    const float rc2f = (float)g.rc2;
     for (int q = 0; q < 8; ++q) o[q] = 0.0;
     const int q0 = cg.img_off[c], q1 = cg.img_off[c + 1];
     for (int q = q0; q < q1; q += 4) {
@@ -2204,21 +2211,23 @@ __global__ __launch_bounds__(256) void k_grid_culled(const GridArgs g, const Cel
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         if (q + u < q1) {
-          const int4 e = cg.img[q + u];
-          const float4 pr = par[e.w];
-          const float dx = (float)(ix - e.x) * unfix, dy = (float)(iy - e.y) * unfix,
-                      dz = (float)(iz - e.z) * unfix;
+          const float4 e = cg.img[q + u];
+          const float4 pr = par[__float_as_int(e.w)];
+          const float dx = fx - e.x, dy = fy - e.y, dz = fz - e.z;
           const float rr = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
           if (rr < rc2f) {
             const float ir = __builtin_amdgcn_rcpf(rr);
-            const float t = (pr.y * s2unfix) * ir;
-            const float r3 = (t * t) * t;
-            const float F = fmaf(pr.x, fmaf(r3, r3, -r3), -pr.z);
-            // 24 eps = 6 (4 eps), 96 eps = 24 (4 eps), 384 eps = 96 (4 eps)
-            const float DF = 6.0f * pr.x * (r3 * fmaf(-2.0f, r3, 1.0f)) * ir;
             const float ir2 = ir * ir;
-            const float DDF = 24.0f * pr.x * (r3 * fmaf(7.0f, r3, -2.0f)) * ir2;
-            const float DDDF = 96.0f * pr.x * (r3 * fmaf(-28.0f, r3, 5.0f)) * (ir2 * ir2);
+            const float ir3 = ir2 * ir;
+            // with s = (sigma/r)^6: A = 4 eps s^2 = c12/r^12, B = 4 eps s = c6/r^6,
+            // U = A - B - shift and the derivative factors
+            // 24 eps s (1 - 2 s)/r^2, 96 eps s (7 s - 2)/r^4, 384 eps s (5 - 28 s)/r^6
+            const float B = pr.y * ir3;
+            const float A = (pr.x * ir3) * ir3;
+            const float F = (A - B) - pr.z;
+            const float DF = fmaf(-12.0f, A, 6.0f * B) * ir;
+            const float DDF = fmaf(168.0f, A, -48.0f * B) * ir2;
+            const float DDDF = fmaf(-2688.0f, A, 480.0f * B) * (ir2 * ir2);
             const float ddx = DDF * dx;
             a[0] += F;
             a[1] = fmaf(dx, DF, a[1]);

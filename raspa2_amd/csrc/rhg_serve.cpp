@@ -160,7 +160,7 @@ size_t estimate_bytes(const rhg_framework *f, const rhg_cell *cell, const rhg_fo
   double reach = rc + 0.5 * std::sqrt(3.0) * cell_size;
   double per_cell = std::min(n, n / V * 4.0 / 3.0 * M_PI * reach * reach * reach);
   bytes += ncell * (per_cell * 4.0 + 8.0);
-  // mixed precision adds the fixed-point image lists (16 B per entry)
+  // mixed precision adds the image lists (16 B per entry)
   if (precision == RHG_PREC_MIXED) bytes += ncell * (per_cell * 16.0 + 8.0);
   // Widom buffers per insertion: first-bead trials (5 x 8 B each), per
   // insertion scalars (132 B), orientation trials (4 x 8 B each), positions of
@@ -172,7 +172,7 @@ size_t estimate_bytes(const rhg_framework *f, const rhg_cell *cell, const rhg_fo
   double per_ins = k_fb * 40.0 + 132.0 + (double)k_or_eff * n_place * 32.0 + n_sites * 24.0 +
                    n_place * 8.0 + 16.0;
   if (ewald && n_charged > 0) per_ins += (kx + 1) * 16.0;
-  // binning scratch: 8 B per binned site (24 B with the sorted fixed-point
+  // binning scratch: 8 B per binned site (24 B with the sorted FP32
   // offsets of mixed precision) and 16 B per cell for each list
   const double per_site = precision == RHG_PREC_MIXED ? 24.0 : 8.0;
   per_ins += k_fb * per_site;

@@ -672,8 +672,8 @@ class Context:
         self._check(lib.rhg_profile_reset(self.h), "profile_reset")
 
     def set_culling(self, on=True, cell_size=2.5, images=True):
-        """Cell-list culling; `images` also enables the fixed-point image
-        lists of the mixed-precision kernels."""
+        """Cell-list culling; `images` also enables the image lists of
+        the mixed-precision kernels."""
         mode = 0 if not on else (2 if images else 1)
         self._check(lib.rhg_set_culling(self.h, mode, cell_size), "set_culling")
 

@@ -43,7 +43,7 @@ def test_vdw_fp64_bit_exact(W, O, R, which, cull, cell):
 def test_vdw_mixed_tolerance_and_flags(W, O, R, which, cull, images):
     """Mixed precision: overlap flags identical, relative error <= 1e-5 on
     non-overlapping sites (|U| > 1 K), absolute <= 1e-3 K otherwise; per-pair
-    minimum image (all pairs or cell lists) and fixed-point image lists."""
+    minimum image (all pairs or cell lists) and image lists."""
     fw = FRAMEWORKS[which](W)
     sys_o, ctx, to, tg = _pair(W, O, R, fw, O.CHARGE_NONE, R.PREC_MIXED, [(148.0, 3.73, 0.0)],
                                cull, images=images)

@@ -98,7 +98,7 @@ def test_derivatives(W, O, R):
 
 
 def test_grid_mixed_vs_fp64(W, O, R):
-    """Mixed-precision grid build (FP32 pair terms over the fixed-point image
+    """Mixed-precision grid build (FP32 pair terms over the image
     lists, FP64 sums): the same capped nodes as the FP64 build, and the
     interpolated energy within 1e-5 relative (|U| > 1 K) or 1e-3 K."""
     fw = W.c2_framework()
