@@ -64,7 +64,8 @@ struct BinWork {
   int32_t *count;    // per cell
   int32_t *fill;     // per cell
   int32_t *start;    // per cell, exclusive scan of count
-  int32_t *tile_off; // per cell + 1, exclusive scan of ceil(count/64)
+  int32_t *tile_off; // per cell + 1, exclusive scan of ceil(count/ts): ts = 128
+                     // sites when fix is set (image-list kernels), else 64
   int32_t *perm;     // sorted position -> site index
   // optional (image-list kernels): per sorted position the site's offset from
   // its cell centre in A as FP32 and the site index bit-cast into w

@@ -568,10 +568,10 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const Cell cell, const Cell
 }
 
 // exclusive scans of the per-cell site counts (start) and of the per-cell
-// tile counts ceil(count/64) (tile_off), one workgroup
+// tile counts ceil(count/ts) (tile_off), one workgroup
 __global__ __launch_bounds__(1024) void k_scan_bins(const int32_t *__restrict__ in, int n,
                                                     int32_t *__restrict__ start,
-                                                    int32_t *__restrict__ tiles) {
+                                                    int32_t *__restrict__ tiles, int ts) {
   __shared__ int ps[1024], pt[1024];
   const int t = threadIdx.x;
   const int per = (n + blockDim.x - 1) / blockDim.x;
@@ -579,7 +579,7 @@ __global__ __launch_bounds__(1024) void k_scan_bins(const int32_t *__restrict__ 
   int s = 0, u = 0;
   for (int i = b; i < e; ++i) {
     s += in[i];
-    u += (in[i] + 63) / 64;
+    u += (in[i] + ts - 1) / ts;
   }
   ps[t] = s;
   pt[t] = u;
@@ -596,7 +596,7 @@ __global__ __launch_bounds__(1024) void k_scan_bins(const int32_t *__restrict__ 
     start[i] = rs;
     tiles[i] = rt;
     rs += in[i];
-    rt += (in[i] + 63) / 64;
+    rt += (in[i] + ts - 1) / ts;
   }
   if (t == (int)blockDim.x - 1) {
     start[n] = ps[t];
@@ -610,7 +610,8 @@ void launch_bin_sites(const Cell &cell, const CellGrid &g, const SiteList &s, co
   hipLaunchKernelGGL(k_zero_i32, dim3(nbc), dim3(256), 0, st, b.count, b.fill, g.ncell);
   const unsigned nb = (unsigned)((s.n + 255) / 256);
   if (nb) hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(256), 0, st, cell, g, s, b);
-  hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(1024), 0, st, b.count, g.ncell, b.start, b.tile_off);
+  hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(1024), 0, st, b.count, g.ncell, b.start, b.tile_off,
+                     b.fix ? 128 : 64);
   if (nb) {
     if (cell.bc == 1)
       hipLaunchKernelGGL(k_bin_scatter<1>, dim3(nb), dim3(256), 0, st, cell, g, s, b);
@@ -726,46 +727,20 @@ __global__ __launch_bounds__(256) void k_site_energy_culled(
   }
 }
 
-// Mixed-precision site energies over the image lists: one wave per tile of
-// <= 64 sites of one cell.  The site's offset from the cell centre and the
-// listed image offsets are FP32 (both rounded once from FP64, |offset| below
-// the cut-off plus the cell's half diagonal), so a pair's separation costs
-// one subtraction per axis; the LJ term is c12 ir^6 - c6 ir^3 - shift and the
-// real-space Coulomb term q_j erfc(alpha r)/r in FP32, four terms summed in
-// FP32 and the groups in FP64 (the Coulomb prefactor applied there).  Sites
-// whose sum lands within the guard band of the overlap threshold are
-// recomputed exactly (FP64, atom order) so the overlap flags equal the FP64
-// path's.
-template <int BCK, int VDW, int CC>
-__global__ __launch_bounds__(256) void k_site_energy_img(
-    const Cell cell, const AtomRec64 *__restrict__ a64, const float4 *__restrict__ par,
-    const double rc2v, const double rc2c, const double alpha, const double cqa, const CellGrid g,
-    const SiteList s, const BinWork b, double *__restrict__ out_v, double *__restrict__ out_c,
-    uint8_t *__restrict__ out_o) {
-  const int lane = threadIdx.x & 63;
-  const int tile = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-  const int ntiles = b.tile_off[g.ncell];
-  if (tile >= ntiles) return;
-  int lo = 0, hi = g.ncell - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (b.tile_off[mid] <= tile) lo = mid; else hi = mid - 1;
-  }
-  const int c = __builtin_amdgcn_readfirstlane(lo);
-  const int k = (tile - b.tile_off[c]) * 64 + lane;
-  const bool live = k < b.count[c];
-  // sorted offsets from the cell centre (written by the binning)
-  const float4 so = b.fix[b.start[c] + (live ? k : 0)];
-  const int site = __float_as_int(so.w);
-  const float rc2vf = (float)rc2v, rc2cf = (float)rc2c, alphaf = (float)alpha;
-  const int q0 = g.img_off[c], q1 = g.img_off[c + 1];
-  double uv = 0.0, uc = 0.0;
-  // one pair term in FP32 (zero beyond the cut-off); the image record and
-  // its parameters are wave-uniform (SGPR operands), and the LJ form keeps
-  // one SGPR per instruction
-  auto term = [&](const float4 e, float &fv, float &fc) {
-    const float4 pr = par[__float_as_int(e.w)];
-    const float dx = so.x - e.x, dy = so.y - e.y, dz = so.z - e.z;
+// Pair sums of NS sites per lane over a cell's image list (FP32 terms, four
+// summed in FP32 and the groups in FP64).  Each wave-uniform image record and
+// its parameters are loaded once (scalar loads into SGPR operands) and used
+// for all NS sites of the lane: the scalar loads of the list, not the
+// arithmetic, set the pace of one site per lane.
+template <int NS, int VDW, int CC>
+__device__ __forceinline__ void img_sums(const float4 *__restrict__ img, const float4 *__restrict__ par,
+                                         int q0, int q1, const float4 (&so)[2], float rc2vf,
+                                         float rc2cf, float alphaf, double cqa, double (&uv)[2],
+                                         double (&uc)[2]) {
+  // one pair term in FP32 (zero beyond the cut-off); the LJ form keeps one
+  // SGPR operand per instruction
+  auto term = [&](const float4 e, const float4 pr, const float4 o, float &fv, float &fc) {
+    const float dx = o.x - e.x, dy = o.y - e.y, dz = o.z - e.z;
     const float rr = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     if (VDW) {
       const float ir = __builtin_amdgcn_rcpf(rr);
@@ -787,63 +762,127 @@ __global__ __launch_bounds__(256) void k_site_energy_img(
       }
     }
   };
-  // four terms are summed in FP32, the groups in FP64
   int q = q0;
   for (; q + 4 <= q1; q += 4) {
-    float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
-    term(g.img[q], v0, c0);
-    term(g.img[q + 1], v1, c1);
-    term(g.img[q + 2], v2, c2);
-    term(g.img[q + 3], v3, c3);
-    if (VDW) uv += (double)((v0 + v1) + (v2 + v3));
-    if (CC) uc = fma(cqa, (double)((c0 + c1) + (c2 + c3)), uc);
-  }
-  for (; q < q1; ++q) {
-    float v0 = 0.f, c0 = 0.f;
-    term(g.img[q], v0, c0);
-    if (VDW) uv += (double)v0;
-    if (CC) uc = fma(cqa, (double)c0, uc);
-  }
-  const double band = 1.0e-4 * kOverlap;
-  const bool near = live && ((VDW && fabs(uv - kOverlap) <= band) || (CC && fabs(uc - kOverlap) <= band));
-  if (__any(near)) {
-    if (near) {
-#pragma clang fp contract(off)
-      const double px = s.x[site], py = s.y[site], pz = s.z[site];
-      double ev = 0.0, ec = 0.0;
-      const int j0 = g.nb_off[c], j1 = g.nb_off[c + 1];
-      for (int q = j0; q < j1; ++q) {
-        const AtomRec64 r = a64[g.nb_idx[q]];
-        double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
-        mi_exact<BCK>(cell, dx, dy, dz);
-        double rr = dx * dx + dy * dy + dz * dz;
-        if (VDW && rr < rc2v) ev += lj_exact(r.c4, r.s2, r.shift, rr);
-        if (CC && rr < rc2c) {
-          double rd = sqrt(rr);
-          ec += cqa * r.q * (erfc(alpha * rd) / rd);
-        }
-      }
-      if (VDW) uv = ev;
-      if (CC) uc = ec;
+    float4 e[4], pr[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      e[u] = img[q + u];
+      pr[u] = par[__float_as_int(e[u].w)];
+    }
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      float v[4] = {0.f, 0.f, 0.f, 0.f}, c[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int u = 0; u < 4; ++u) term(e[u], pr[u], so[k], v[u], c[u]);
+      if (VDW) uv[k] += (double)((v[0] + v[1]) + (v[2] + v[3]));
+      if (CC) uc[k] = fma(cqa, (double)((c[0] + c[1]) + (c[2] + c[3])), uc[k]);
     }
   }
-  if (lane == 0 && b.pairs) {
-    const int nl = min(64, b.count[c] - (tile - b.tile_off[c]) * 64);
+  for (; q < q1; ++q) {
+    const float4 e = img[q];
+    const float4 pr = par[__float_as_int(e.w)];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      float v = 0.f, c = 0.f;
+      term(e, pr, so[k], v, c);
+      if (VDW) uv[k] += (double)v;
+      if (CC) uc[k] = fma(cqa, (double)c, uc[k]);
+    }
+  }
+}
+
+// Mixed-precision site energies over the image lists: one wave per tile of
+// <= 128 sites of one cell (tile_off counts tiles of 128 when the sorted
+// offsets are written), lane l taking sites l and l + 64.  The site's offset
+// from the cell centre and the listed image offsets are FP32 (both rounded
+// once from FP64, |offset| below the cut-off plus the cell's half diagonal),
+// so a pair's separation costs one subtraction per axis; the LJ term is
+// c12 ir^6 - c6 ir^3 - shift and the real-space Coulomb term
+// q_j erfc(alpha r)/r in FP32, the Coulomb prefactor applied to the FP64
+// group sums.  Sites whose sum lands within the guard band of the overlap
+// threshold are recomputed exactly (FP64, atom order) so the overlap flags
+// equal the FP64 path's.
+template <int BCK, int VDW, int CC>
+__global__ __launch_bounds__(256) void k_site_energy_img(
+    const Cell cell, const AtomRec64 *__restrict__ a64, const float4 *__restrict__ par,
+    const double rc2v, const double rc2c, const double alpha, const double cqa, const CellGrid g,
+    const SiteList s, const BinWork b, double *__restrict__ out_v, double *__restrict__ out_c,
+    uint8_t *__restrict__ out_o) {
+  const int lane = threadIdx.x & 63;
+  const int tile = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  const int ntiles = b.tile_off[g.ncell];
+  if (tile >= ntiles) return;
+  int lo = 0, hi = g.ncell - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (b.tile_off[mid] <= tile) lo = mid; else hi = mid - 1;
+  }
+  const int c = __builtin_amdgcn_readfirstlane(lo);
+  const int first = (tile - b.tile_off[c]) * 128;
+  const int nl = min(128, b.count[c] - first); // live sites of the tile, 1..128
+  // sorted offsets from the cell centre (written by the binning)
+  float4 so[2];
+  bool live[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int kk = lane + 64 * k;
+    live[k] = kk < nl;
+    so[k] = b.fix[b.start[c] + first + (live[k] ? kk : 0)];
+  }
+  const int q0 = g.img_off[c], q1 = g.img_off[c + 1];
+  double uv[2] = {0.0, 0.0}, uc[2] = {0.0, 0.0};
+  if (nl > 64)
+    img_sums<2, VDW, CC>(g.img, par, q0, q1, so, (float)rc2v, (float)rc2c, (float)alpha, cqa, uv, uc);
+  else
+    img_sums<1, VDW, CC>(g.img, par, q0, q1, so, (float)rc2v, (float)rc2c, (float)alpha, cqa, uv, uc);
+  const double band = 1.0e-4 * kOverlap;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const bool near = live[k] && ((VDW && fabs(uv[k] - kOverlap) <= band) ||
+                                  (CC && fabs(uc[k] - kOverlap) <= band));
+    if (__any(near)) {
+      if (near) {
+#pragma clang fp contract(off)
+        const int site = __float_as_int(so[k].w);
+        const double px = s.x[site], py = s.y[site], pz = s.z[site];
+        double ev = 0.0, ec = 0.0;
+        const int j0 = g.nb_off[c], j1 = g.nb_off[c + 1];
+        for (int q = j0; q < j1; ++q) {
+          const AtomRec64 r = a64[g.nb_idx[q]];
+          double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
+          mi_exact<BCK>(cell, dx, dy, dz);
+          double rr = dx * dx + dy * dy + dz * dz;
+          if (VDW && rr < rc2v) ev += lj_exact(r.c4, r.s2, r.shift, rr);
+          if (CC && rr < rc2c) {
+            double rd = sqrt(rr);
+            ec += cqa * r.q * (erfc(alpha * rd) / rd);
+          }
+        }
+        if (VDW) uv[k] = ev;
+        if (CC) uc[k] = ec;
+      }
+    }
+  }
+  if (lane == 0 && b.pairs)
     atomicAdd(b.pairs + ((VDW && CC) ? 1 : (VDW ? 0 : 2)),
               (unsigned long long)nl * (unsigned long long)(q1 - q0));
-  }
-  if (live) {
-    const int64_t oi = site;
-    if (out_v) out_v[oi] = uv;
-    if (out_c) out_c[oi] = uc;
-    if (out_o) out_o[oi] = (uint8_t)((VDW && uv >= kOverlap) || (CC && uc >= kOverlap));
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    if (live[k]) {
+      const int64_t oi = __float_as_int(so[k].w);
+      if (out_v) out_v[oi] = uv[k];
+      if (out_c) out_c[oi] = uc[k];
+      if (out_o) out_o[oi] = (uint8_t)((VDW && uv[k] >= kOverlap) || (CC && uc[k] >= kOverlap));
+    }
   }
 }
 
 template <int BCK, int VDW, int CC>
 static void img_launch(const EnergyArgs &a, const CellGrid &g, const SiteList &s, const BinWork &b,
                        const EnergyOut &o, hipStream_t st) {
-  const int64_t max_tiles = s.n / 64 + g.ncell + 1;
+  // upper bound of tiles: sites/128 + one partial tile per cell
+  const int64_t max_tiles = s.n / 128 + g.ncell + 1;
   const unsigned nb = (unsigned)((max_tiles + 3) / 4);
   hipLaunchKernelGGL((k_site_energy_img<BCK, VDW, CC>), dim3(nb), dim3(256), 0, st, a.cell, a.a64,
                      a.par, a.rc2_vdw, a.rc2_cc, a.alpha, a.cqa, g, s, b, o.u_vdw, o.u_cc, o.overlap);
