@@ -58,13 +58,17 @@ struct CellGrid {
   const float4 *img;
 };
 
+// sites per lane of the image-list kernels: their tiles hold 64 kImgLane
+// sites of one cell
+constexpr int kImgLane = 4;
+
 // scratch for binning one site list into cells
 struct BinWork {
   int32_t *cell_of;  // per site
   int32_t *count;    // per cell
   int32_t *fill;     // per cell
   int32_t *start;    // per cell, exclusive scan of count
-  int32_t *tile_off; // per cell + 1, exclusive scan of ceil(count/ts): ts = 128
+  int32_t *tile_off; // per cell + 1, exclusive scan of ceil(count/ts): ts = 256
                      // sites when fix is set (image-list kernels), else 64
   int32_t *perm;     // sorted position -> site index
   // optional (image-list kernels): per sorted position the site's offset from

@@ -611,7 +611,7 @@ void launch_bin_sites(const Cell &cell, const CellGrid &g, const SiteList &s, co
   const unsigned nb = (unsigned)((s.n + 255) / 256);
   if (nb) hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(256), 0, st, cell, g, s, b);
   hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(1024), 0, st, b.count, g.ncell, b.start, b.tile_off,
-                     b.fix ? 128 : 64);
+                     b.fix ? 64 * kImgLane : 64);
   if (nb) {
     if (cell.bc == 1)
       hipLaunchKernelGGL(k_bin_scatter<1>, dim3(nb), dim3(256), 0, st, cell, g, s, b);
@@ -734,9 +734,9 @@ __global__ __launch_bounds__(256) void k_site_energy_culled(
 // arithmetic, set the pace of one site per lane.
 template <int NS, int VDW, int CC>
 __device__ __forceinline__ void img_sums(const float4 *__restrict__ img, const float4 *__restrict__ par,
-                                         int q0, int q1, const float4 (&so)[2], float rc2vf,
-                                         float rc2cf, float alphaf, double cqa, double (&uv)[2],
-                                         double (&uc)[2]) {
+                                         int q0, int q1, const float4 (&so)[kImgLane], float rc2vf,
+                                         float rc2cf, float alphaf, double cqa,
+                                         double (&uv)[kImgLane], double (&uc)[kImgLane]) {
   // one pair term in FP32 (zero beyond the cut-off); the LJ form keeps one
   // SGPR operand per instruction
   auto term = [&](const float4 e, const float4 pr, const float4 o, float &fv, float &fc) {
@@ -793,8 +793,8 @@ __device__ __forceinline__ void img_sums(const float4 *__restrict__ img, const f
 }
 
 // Mixed-precision site energies over the image lists: one wave per tile of
-// <= 128 sites of one cell (tile_off counts tiles of 128 when the sorted
-// offsets are written), lane l taking sites l and l + 64.  The site's offset
+// <= 64 kImgLane sites of one cell (tile_off counts tiles of that size when
+// the sorted offsets are written), lane l taking sites l, l + 64, ...  The site's offset
 // from the cell centre and the listed image offsets are FP32 (both rounded
 // once from FP64, |offset| below the cut-off plus the cell's half diagonal),
 // so a pair's separation costs one subtraction per axis; the LJ term is
@@ -819,26 +819,35 @@ __global__ __launch_bounds__(256) void k_site_energy_img(
     if (b.tile_off[mid] <= tile) lo = mid; else hi = mid - 1;
   }
   const int c = __builtin_amdgcn_readfirstlane(lo);
-  const int first = (tile - b.tile_off[c]) * 128;
-  const int nl = min(128, b.count[c] - first); // live sites of the tile, 1..128
+  const int first = (tile - b.tile_off[c]) * (64 * kImgLane);
+  const int nl = min(64 * kImgLane, b.count[c] - first); // live sites of the tile
   // sorted offsets from the cell centre (written by the binning)
-  float4 so[2];
-  bool live[2];
+  float4 so[kImgLane];
+  bool live[kImgLane];
 #pragma unroll
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < kImgLane; ++k) {
     const int kk = lane + 64 * k;
     live[k] = kk < nl;
     so[k] = b.fix[b.start[c] + first + (live[k] ? kk : 0)];
   }
   const int q0 = g.img_off[c], q1 = g.img_off[c + 1];
-  double uv[2] = {0.0, 0.0}, uc[2] = {0.0, 0.0};
-  if (nl > 64)
-    img_sums<2, VDW, CC>(g.img, par, q0, q1, so, (float)rc2v, (float)rc2c, (float)alpha, cqa, uv, uc);
+  double uv[kImgLane], uc[kImgLane];
+#pragma unroll
+  for (int k = 0; k < kImgLane; ++k) uv[k] = uc[k] = 0.0;
+  const float rc2vf = (float)rc2v, rc2cf = (float)rc2c, alphaf = (float)alpha;
+  // only as many sites per lane as the tile holds (wave-uniform)
+  static_assert(kImgLane == 4, "the dispatch below covers 1..4 sites per lane");
+  if (nl > 192)
+    img_sums<4, VDW, CC>(g.img, par, q0, q1, so, rc2vf, rc2cf, alphaf, cqa, uv, uc);
+  else if (nl > 128)
+    img_sums<3, VDW, CC>(g.img, par, q0, q1, so, rc2vf, rc2cf, alphaf, cqa, uv, uc);
+  else if (nl > 64)
+    img_sums<2, VDW, CC>(g.img, par, q0, q1, so, rc2vf, rc2cf, alphaf, cqa, uv, uc);
   else
-    img_sums<1, VDW, CC>(g.img, par, q0, q1, so, (float)rc2v, (float)rc2c, (float)alpha, cqa, uv, uc);
+    img_sums<1, VDW, CC>(g.img, par, q0, q1, so, rc2vf, rc2cf, alphaf, cqa, uv, uc);
   const double band = 1.0e-4 * kOverlap;
 #pragma unroll
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < kImgLane; ++k) {
     const bool near = live[k] && ((VDW && fabs(uv[k] - kOverlap) <= band) ||
                                   (CC && fabs(uc[k] - kOverlap) <= band));
     if (__any(near)) {
@@ -868,7 +877,7 @@ __global__ __launch_bounds__(256) void k_site_energy_img(
     atomicAdd(b.pairs + ((VDW && CC) ? 1 : (VDW ? 0 : 2)),
               (unsigned long long)nl * (unsigned long long)(q1 - q0));
 #pragma unroll
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < kImgLane; ++k) {
     if (live[k]) {
       const int64_t oi = __float_as_int(so[k].w);
       if (out_v) out_v[oi] = uv[k];
@@ -881,8 +890,8 @@ __global__ __launch_bounds__(256) void k_site_energy_img(
 template <int BCK, int VDW, int CC>
 static void img_launch(const EnergyArgs &a, const CellGrid &g, const SiteList &s, const BinWork &b,
                        const EnergyOut &o, hipStream_t st) {
-  // upper bound of tiles: sites/128 + one partial tile per cell
-  const int64_t max_tiles = s.n / 128 + g.ncell + 1;
+  // upper bound of tiles: sites/(64 kImgLane) + one partial tile per cell
+  const int64_t max_tiles = s.n / (64 * kImgLane) + g.ncell + 1;
   const unsigned nb = (unsigned)((max_tiles + 3) / 4);
   hipLaunchKernelGGL((k_site_energy_img<BCK, VDW, CC>), dim3(nb), dim3(256), 0, st, a.cell, a.a64,
                      a.par, a.rc2_vdw, a.rc2_cc, a.alpha, a.cqa, g, s, b, o.u_vdw, o.u_cc, o.overlap);
