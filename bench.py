@@ -644,6 +644,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 512 MiB device copy kernel (k_stream_copy) three "
                          "times, to scale profiler byte counters")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="c1: leave out the finite-loading, flexible-chain and pore-size legs "
+                         "(profiles of the headline steps)")
     ap.add_argument("--frameworks", type=int, default=256, help="c3/c4 catalogue size")
     ap.add_argument("--requests", type=int, default=384, help="c4 requests")
     ap.add_argument("--slots", type=int, default=8, help="c3/c4 requests in flight")
@@ -819,10 +822,11 @@ def main():
             "all_pairs_equivalent": round((st["lj_pairs"] + st["ljcc_pairs"] + st["cc_pairs"]) / elapsed, 1),
             "executed": round((st["exec_lj_pairs"] + st["exec_ljcc_pairs"] + st["exec_cc_pairs"]) / elapsed, 1)},
         "kernels": kernels,
-        "finite_loading": loaded_bench(fw, probe),
-        "flexible_chain": chain_bench(no_cpu=args.no_cpu),
-        "pore_size_distribution": psd_bench(no_cpu=args.no_cpu),
     }
+    if not args.headline_only:
+        out["finite_loading"] = loaded_bench(fw, probe)
+        out["flexible_chain"] = chain_bench(no_cpu=args.no_cpu)
+        out["pore_size_distribution"] = psd_bench(no_cpu=args.no_cpu)
     print(json.dumps(out))
 
 
