@@ -18,8 +18,7 @@ namespace rhg {
 // erfcx(x) = exp(x^2) erfc(x) on [0, 5] (raspa2_amd/csrc/fit_erfc.py; relative
 // error 3.5e-8 in exact arithmetic, below 5e-7 in FP32).  Larger arguments
 // (only for Ewald precisions beyond 1e-10) use the library erfcf.
-__device__ __forceinline__ float erfc_mixed(float x) {
-  if (x > 5.0f) return erfcf(x);
+__device__ __forceinline__ float erfc_fit(float x) {
   const float t = __builtin_amdgcn_rcpf(fmaf(0.5f, x, 1.0f));
   float p = -0.040972326f;
   p = fmaf(p, t, 0.292736441f);
@@ -40,6 +39,10 @@ __device__ __forceinline__ float erfc_mixed(float x) {
   yl = fmaf(-x2, 1.92596299e-08f, yl);
   yl = fmaf(-x2l, 1.44269502f, yl);
   return p * (__builtin_amdgcn_exp2f(yh) * fmaf(yl, 0.693147181f, 1.0f));
+}
+__device__ __forceinline__ float erfc_mixed(float x) {
+  if (x > 5.0f) return erfcf(x);
+  return erfc_fit(x);
 }
 
 // ====================================================================
@@ -753,7 +756,13 @@ __device__ __forceinline__ void img_sums(const float4 *__restrict__ img, const f
       const float u = fmaf(ir3, y, -pr.z);
       fv = (rr < rc2vf) ? u : 0.0f;
     }
-    if (CC) {
+    if (CC == 1) {
+      // alpha rc <= 5: the fit covers every in-range pair; computed for all
+      // pairs and selected, like the LJ term (out-of-range values are finite)
+      const float irs = __builtin_amdgcn_rsqf(rr);
+      const float e2 = erfc_fit(alphaf * (rr * irs));
+      fc = (rr < rc2cf) ? pr.w * (e2 * irs) : 0.0f;
+    } else if (CC == 2) {
       fc = 0.0f;
       if (rr < rc2cf) {
         const float irs = __builtin_amdgcn_rsqf(rr);
@@ -802,7 +811,8 @@ __device__ __forceinline__ void img_sums(const float4 *__restrict__ img, const f
 // q_j erfc(alpha r)/r in FP32, the Coulomb prefactor applied to the FP64
 // group sums.  Sites whose sum lands within the guard band of the overlap
 // threshold are recomputed exactly (FP64, atom order) so the overlap flags
-// equal the FP64 path's.
+// equal the FP64 path's.  CC 1: Coulomb terms from the FP32 erfc fit
+// (alpha rc <= 5, branch-free); CC 2: the library erfcf beyond the fit.
 template <int BCK, int VDW, int CC>
 __global__ __launch_bounds__(256) void k_site_energy_img(
     const Cell cell, const AtomRec64 *__restrict__ a64, const float4 *__restrict__ par,
@@ -913,9 +923,16 @@ static void culled_mode(const EnergyArgs &a, const CellGrid &g, const SiteList &
                         const EnergyOut &o, hipStream_t st) {
   const int bck = a.cell.bc == 1 ? 1 : (a.cell.upper ? 2 : 3);
   if (a.precision != 0 && g.img_off && a.par && b.fix) {
-    if (bck == 1) img_launch<1, VDW, CC>(a, g, s, b, o, st);
-    else if (bck == 2) img_launch<2, VDW, CC>(a, g, s, b, o, st);
-    else img_launch<3, VDW, CC>(a, g, s, b, o, st);
+    // CC 2: erfc arguments beyond the fit's range (alpha rc > 5) possible
+    if (CC && a.alpha * sqrt(a.rc2_cc) > 5.0) {
+      if (bck == 1) img_launch<1, VDW, 2>(a, g, s, b, o, st);
+      else if (bck == 2) img_launch<2, VDW, 2>(a, g, s, b, o, st);
+      else img_launch<3, VDW, 2>(a, g, s, b, o, st);
+    } else {
+      if (bck == 1) img_launch<1, VDW, CC>(a, g, s, b, o, st);
+      else if (bck == 2) img_launch<2, VDW, CC>(a, g, s, b, o, st);
+      else img_launch<3, VDW, CC>(a, g, s, b, o, st);
+    }
     return;
   }
   if (a.precision == 0) {
