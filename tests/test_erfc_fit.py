@@ -1,5 +1,5 @@
-"""The FP32 erfc of the mixed real-space Ewald kernels (erfc_mixed in
-raspa2_amd/csrc/rhg_kernels.hip): its coefficients, evaluated as the kernel
+"""The FP32 erfc of the mixed real-space Ewald kernels (erfc_fit, called by
+erfc_mixed, in raspa2_amd/csrc/rhg_kernels.hip): its coefficients, evaluated as the kernel
 does in FP32 (FMA Horner), stay within 5e-7 relative of scipy's erfc on
 [0, 5], the range the Ewald cut-offs use."""
 import os
@@ -13,11 +13,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _coefficients():
     src = open(os.path.join(ROOT, "raspa2_amd", "csrc", "rhg_kernels.hip")).read()
-    body = src[src.index("float erfc_mixed(float x)"):]
+    body = src[src.index("float erfc_fit(float x)"):]
     body = body[:body.index("// exp(-x^2)")]
     vals = [float(v) for v in re.findall(r"(-?\d+\.\d+(?:e-?\d+)?)f", body)]
-    # 5.0f (range), 0.5f and 1.0f (t), then the nine coefficients high to low
-    return vals[3:12]
+    # 0.5f and 1.0f (t), then the nine coefficients high to low
+    return vals[2:11]
 
 
 def _fma32(a, b, c):
