@@ -457,8 +457,9 @@ int rhg_widom_stats(rhg_ctx *ctx, double out[12], int32_t reset);
  * wave walks the ascending list of atoms within reach of its cell.  The FP64
  * result is bit-identical to the all-pairs loop.  on = 0: all pairs; 1: cell
  * lists; 2 (default): cell lists, and in mixed precision per-cell lists of
- * atom images in fixed point, so that no pair needs a minimum-image step
- * (used when every perpendicular width is >= 2 cut-offs).  Default cell 2.5 A. */
+ * atom images (FP32 offsets from the cell centre), so that no pair needs a
+ * minimum-image step (used when every perpendicular width is >= 2 cut-offs).
+ * Default cell 2.5 A. */
 int rhg_set_culling(rhg_ctx *ctx, int32_t on, double cell_size);
 /* Scratch buffers that contexts can borrow (no counterpart in the
  * reference): while a workspace is attached, the context's per-call device
