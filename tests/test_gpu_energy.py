@@ -109,3 +109,44 @@ def test_scalar_shims(W, O, R):
         ucc, ucbd = ctx.CalculateFrameworkChargeEnergyAtPosition(pos, tg[0])
         assert ucbd == 0.0
         assert abs(ucc - sys_o.charge_at(to[0], pos)) <= 1e-11 * max(1.0, abs(ucc))
+
+
+def _clustered_positions(box, n, seed):
+    """Half of the points in a 1.5 A cube, a quarter in a 6 A cube, the rest
+    uniform: cells holding one to several hundred sites, so the mixed
+    image-list kernels run tiles of every size (one to four sites per lane,
+    several tiles per cell)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    m = np.asarray(box).reshape(3, 3)
+    c1, c2 = np.array([0.31, 0.42, 0.53]) @ m, np.array([0.71, 0.22, 0.64]) @ m
+    a = c1 + rng.uniform(-0.75, 0.75, (n // 2, 3))
+    b = c2 + rng.uniform(-3.0, 3.0, (n // 4, 3))
+    u = rng.uniform(0.0, 1.0, (n - n // 2 - n // 4, 3)) @ m
+    return np.concatenate([a, b, u])
+
+
+@pytest.mark.parametrize("charged", [False, True])
+def test_mixed_dense_cells(W, O, R, charged):
+    """Mixed precision with hundreds of sites per cell (C1 supercell, LJ and
+    LJ + real-space Ewald): overlap flags identical to the oracle's, LJ within
+    1e-5 relative (|U| > 1 K) or 1e-3 K, Coulomb within 1e-6 of the summed
+    term magnitudes."""
+    fw = W.c1_framework()
+    site = (27.0, 2.80, 0.70) if charged else (148.0, 3.73, 0.0)
+    cm = O.CHARGE_EWALD if charged else O.CHARGE_NONE
+    sys_o, ctx, to, tg = _pair(W, O, R, fw, cm, R.PREC_MIXED, [site], True, 2.5, True)
+    xyz = _clustered_positions(fw.box, 6000, 21)
+    uv, uc, ov = ctx.energy_at_positions(xyz, tg[0])
+    uv_o, uc_o = sys_o.energy_at_many(to[0], xyz)
+    bad = (uv_o >= 1e7) | ((uc_o >= 1e7) if charged else False)
+    assert np.array_equal(ov, bad)
+    ok = ~bad
+    assert ok.sum() > 500
+    e2k = O.units()["energy_to_kelvin"]
+    err = np.abs(uv[ok] - uv_o[ok]) * e2k
+    big = np.abs(uv_o[ok]) * e2k > 1.0
+    assert np.all(err[big] / (np.abs(uv_o[ok][big]) * e2k) <= 1e-5)
+    assert np.all(err[~big] <= 1e-3)
+    if charged:
+        scale = _coulomb_abs_sum(fw, xyz[ok], 0.70, ctx.info()["alpha"], O.units()["coulomb"])
+        assert np.max(np.abs(uc[ok] - uc_o[ok]) / scale) <= 1e-6
