@@ -653,8 +653,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the Widom "
-              "insertions shard by index range with no exchange but a 5-block sum, see "
-              "DESIGN.md", file=sys.stderr)
+              "insertions shard by index range with no exchange but a 5-block sum",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "c2":
