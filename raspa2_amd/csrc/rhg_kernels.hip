@@ -3700,6 +3700,40 @@ __device__ bool overlap(const PsdArgs &a, const Image<BC> &im, const double p[3]
   return ov;
 }
 
+// CheckSphereOverlap of two points against the same wave-uniform atom
+// records (one scalar load of each record serves both).  The loop ends early
+// only when both points of every lane overlap, so a point that does not
+// overlap sees every atom: ov and radius equal those of overlap().
+template <int BC, int POCK>
+__device__ void overlap2(const PsdArgs &a, const Image<BC> &im, const double p[2][3], bool ov[2],
+                         double radius[2]) {
+#pragma clang fp contract(off)
+  bool done[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    radius[u] = DBL_MAX;
+    ov[u] = POCK && blocked_pocket(a.pockets, p[u][0], p[u][1], p[u][2]);
+    done[u] = ov[u];
+  }
+  for (int j = 0; j < a.n; ++j) {
+    const double4 at = a.atoms[j];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      if (done[u]) continue;
+      double dx = p[u][0] - at.x, dy = p[u][1] - at.y, dz = p[u][2] - at.z;
+      im.apply(dx, dy, dz);
+      const double rr = dx * dx + dy * dy + dz * dz;
+      if (rr < at.w * at.w) ov[u] = true;
+      const double lim = radius[u] + at.w;
+      if (rr < lim * lim * (1.0 + 1e-9)) {
+        const double r = sqrt(rr) - at.w;
+        if (r < radius[u]) radius[u] = r;
+      }
+    }
+    if (__all(ov[0] && ov[1])) break;
+  }
+}
+
 } // namespace psd
 
 template <int BC, int POCK>
@@ -3734,20 +3768,30 @@ __global__ __launch_bounds__(256) void k_psd_b(const PsdArgs a, const double *__
   const uint64_t key = stream_key(a.seed, (uint64_t)(a.first + i));
   const psd::Image<BC> im(a.cell);
   double largest = -10.0;
-  const int64_t j0 = (int64_t)blockIdx.x * a.b_per_block;
-  for (int64_t j = j0 + threadIdx.x; j < j0 + a.b_per_block; j += blockDim.x) {
-    // every lane runs the atom loop (wave-uniform trip count), out-of-range
-    // centres are discarded afterwards
-    const bool live = j < a.n_b;
-    double B[3], radius;
-    psd::point(a.cell, key, (uint64_t)(3 + 3 * (live ? j : 0)), B);
-    const bool ov = psd::overlap<BC, POCK>(a, im, B, radius);
-    if (!live || ov) continue;
-    double dx = A[0] - B[0], dy = A[1] - B[1], dz = A[2] - B[2];
-    im.apply(dx, dy, dz);
-    const double r = sqrt(dx * dx + dy * dy + dz * dz);
-    if (r > radius) continue;
-    if (radius > largest) largest = radius;
+  const int64_t j0 = (int64_t)blockIdx.x * a.b_per_block, j1 = j0 + a.b_per_block;
+  // two centres per lane (j and j + blockDim.x); every lane runs the atom
+  // loop (wave-uniform trip count), out-of-range centres are discarded
+  for (int64_t j = j0 + threadIdx.x; j < j1; j += 2 * (int64_t)blockDim.x) {
+    const int64_t jj[2] = {j, j + blockDim.x};
+    bool live[2];
+    double B[2][3];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      live[u] = jj[u] < j1 && jj[u] < a.n_b;
+      psd::point(a.cell, key, (uint64_t)(3 + 3 * (live[u] ? jj[u] : 0)), B[u]);
+    }
+    bool ov[2];
+    double radius[2];
+    psd::overlap2<BC, POCK>(a, im, B, ov, radius);
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      if (!live[u] || ov[u]) continue;
+      double dx = A[0] - B[u][0], dy = A[1] - B[u][1], dz = A[2] - B[u][2];
+      im.apply(dx, dy, dz);
+      const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      if (r > radius[u]) continue;
+      if (radius[u] > largest) largest = radius[u];
+    }
   }
   for (int s = 32; s > 0; s >>= 1) largest = fmax(largest, __shfl_xor(largest, s));
   __shared__ double wmax[4];
