@@ -161,17 +161,6 @@ int rhg_framework_energy_at_positions(rhg_ctx *ctx, int64_t n, const double *xyz
  * kz -kmax..kmax): KFactor and StoreTotalChargeFramework
  * (src/ewald.c:532-600, 1289-1731).  Arrays hold nk values. */
 int rhg_ewald_tables(rhg_ctx *ctx, double *kfactor, double *sf_re, double *sf_im);
-/* REAL RetraceMolecule(CBMC_DELETION) (src/cbmc.h, src/cbmc.c:2928-3054) for
- * n existing molecules: old_xyz holds each one's n_sites x 3 positions; fb
- * (n x k_fb x 3) and cord (n x k_or x n_sites x 3) the other trials, whose
- * entries 0 are ignored (the molecule's own configuration takes that slot
- * and is always chosen).  exclude[i] is molecule i's adsorbate index (-1:
- * none; exclude null: none for all) so that it does not interact with
- * itself.  rec as rhg_widom_run (W = RosenbluthOld; no Fourier term, which
- * rhg_ewald_fourier_move gives).  A configuration that itself overlaps is
- * reported as overlap (stage 0 or 2). */
-int rhg_widom_retrace_trials(rhg_ctx *ctx, const rhg_molecule *mol, const int32_t *exclude, int64_t n,
-                             const double *old_xyz, const double *fb, const double *cord, double *rec);
 /* int CalculateEwaldFourierAdsorbate(int New,int Old,int mol,int store)
  * (src/ewald.h:86, src/ewald.c:6859) for n_mol independent new molecules
  * at zero loading: d_host = UHostAdsorbateChargeChargeFourierDelta,
@@ -283,10 +272,14 @@ int rhg_ewald_fourier_move(rhg_ctx *ctx, int32_t n_new, const int32_t *new_types
  * default "Minimum", 1 for "Sigma"), a point counting when it lies in no
  * blocked pocket and within that distance of no other atom.  area = sum over
  * atoms of 4 pi d^2 x the counted fraction [A^2]; counts (nullable) the
- * accessible points per atom.  Points come from the counter stream of
+ * accessible points per atom.  interacts (nullable: every atom) flags the
+ * atoms whose pseudo-atom interacts (PseudoAtoms[].Interaction,
+ * src/potentials.c:12170-12185): an atom of a NONE pseudo-atom has no sphere
+ * (src/mc_moves.c:16456; count 0) and excludes no point (its mixed sigma is
+ * 0, src/framework.c:11676).  Points come from the counter stream of
  * (seed, atom x samples + k). */
 int rhg_surface_area(rhg_ctx *ctx, int32_t probe_type, double distance_factor, int64_t samples, uint64_t seed,
-                     double *area, uint64_t *counts);
+                     const uint8_t *interacts, double *area, uint64_t *counts);
 
 /* ------------------------------------------------ pore-size distribution */
 /* SimulationType PSD (ComputePoreSizeDistribution, src/monte_carlo.c:1515-

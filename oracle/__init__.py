@@ -115,6 +115,7 @@ def _declare(L):
     f("orc_widom_run_stream", None, _P, _P, _U64, _I64, _I64, _P)
     f("orc_stream_trials", None, _P, _P, _U64, _I64, _P, _P, _P)
     f("orc_widom_run_mt", _I64, _P, _P, _U64, _I64, _I, _P)
+    f("orc_widom_run_mt_trials", _I64, _P, _P, _U64, _I64, _I, _P, _I64, _P, _P, _P, _P)
     f("orc_block_of_cycles", None, _I64, _P)
     f("orc_widom_estimators", None, _P, _D, _D, _D, _P)
     f("orc_framework_density", _D, _D, _D)
@@ -132,7 +133,7 @@ def _declare(L):
     f("orc_inter_vdw_at", _D, _P, _I, _P, _I, _D)
     f("orc_inter_charge_at", _D, _P, _I, _P, _I, _D)
     f("orc_fourier_move", None, _P, _I, _P, _P, _I, _P)
-    f("orc_surface_area", _D, _P, _I, _D, _I64, _U64, _P)
+    f("orc_surface_area", _D, _P, _I, _D, _I64, _U64, _P, _P)
     f("orc_use_coulomb_grid", _I, _P, _P, _P, _P)
     f("orc_coulomb_grid_node", None, _P, _P, _I, _I, _I, _P)
     f("orc_blocked_pocket", _I, _P, _P)
@@ -341,9 +342,11 @@ class System:
                                old_mol, _ptr(out))
         return out
 
-    def surface_area(self, t, samples, seed=1, factor=2.0 ** (1.0 / 6.0)):
+    def surface_area(self, t, samples, seed=1, factor=2.0 ** (1.0 / 6.0), interacts=None):
         counts = np.zeros(len(self._keep[1]), dtype=np.uint64)
-        area = lib().orc_surface_area(self.h, t, factor, samples, seed, _ptr(counts))
+        mask = None if interacts is None else np.ascontiguousarray(interacts, dtype=np.uint8)
+        area = lib().orc_surface_area(self.h, t, factor, samples, seed,
+                                      None if mask is None else _ptr(mask), _ptr(counts))
         return area, counts
 
     def use_grid(self, t, grid, geom, unit_box):
@@ -448,6 +451,25 @@ def widom_run_mt(system, mol, seed, n_cycles, steps_per_cycle=20):
     n = lib().orc_widom_run_mt(system.h, ctypes.byref(mol), seed, n_cycles, steps_per_cycle,
                                _ptr(blocks))
     return n, blocks
+
+
+def widom_run_mt_trials(system, mol, seed, n_cycles, steps_per_cycle=20, max_ins=None):
+    """RASPA2's MT19937-64 driver (orc_widom_run_mt) that also returns, for its
+    first max_ins insertions, the records and the trials each drew, laid out as
+    host-supplied trials: (n, blocks, rec, fb, cord, u_sel)."""
+    if max_ins is None:
+        max_ins = n_cycles * steps_per_cycle
+    nsites = mol.n_sites
+    blocks = np.zeros((5, 4))
+    rec = np.zeros((max_ins, REC_N))
+    fb = np.zeros((max_ins, mol.k_fb, 3))
+    cord = np.zeros((max_ins, mol.k_or, nsites, 3)) if nsites > 1 else np.zeros((max_ins, 1, 1, 3))
+    usel = np.zeros((max_ins, 2))
+    n = lib().orc_widom_run_mt_trials(system.h, ctypes.byref(mol), seed, n_cycles, steps_per_cycle,
+                                      _ptr(blocks), max_ins, _ptr(rec), _ptr(fb),
+                                      _ptr(cord) if nsites > 1 else None, _ptr(usel))
+    k = min(n, max_ins)
+    return n, blocks, rec[:k], fb[:k], cord[:k], usel[:k]
 
 
 def block_of_cycles(n_cycles):

@@ -923,10 +923,15 @@ void orc_fourier_adsorbate(const orc_system *s, int n_sites, const int *types,
  * (seed, atom x samples + k) and RandomNumberOnUnitSphere's rejection
  * (src/utils.c:688-708). */
 double orc_surface_area(const orc_system *s, int probe_type, double factor, int64_t samples, uint64_t seed,
-                        uint64_t *counts) {
+                        const uint8_t *interacts, uint64_t *counts) {
   double area = 0.0;
   const double sp = s->t_sig[probe_type];
   for (int i = 0; i < s->n; i++) {
+    /* only atoms whose pseudo-atom interacts carry a sphere (src/mc_moves.c:16456) */
+    if (interacts && !interacts[i]) {
+      if (counts) counts[i] = 0;
+      continue;
+    }
     double di = factor * (0.5 * (sp + s->sig[i]));
     double total = 0.0, counted = 0.0;
     uint64_t cnt = 0;
@@ -947,7 +952,9 @@ double orc_surface_area(const orc_system *s, int probe_type, double factor, int6
       int overlap = 0;
       for (int j = 0; j < s->n && !overlap; j++) {
         if (j == i) continue;
-        double dj = factor * (0.5 * (sp + s->sig[j]));
+        /* a NONE pair has sigma 0 in PotentialParms: rr < 0 never holds
+           (src/framework.c:11676) */
+        double dj = (interacts && !interacts[j]) ? 0.0 : factor * (0.5 * (sp + s->sig[j]));
         double dr[3] = {p[0] - s->x[j], p[1] - s->y[j], p[2] - s->z[j]};
         orc_min_image(s->box, s->inv, s->bc, dr);
         double rr = SQR(dr[0]) + SQR(dr[1]) + SQR(dr[2]);
@@ -1529,8 +1536,9 @@ void orc_block_of_cycles(int64_t n_cycles, int32_t *block) {
 typedef struct { orc_mt64 g; } mt_state;
 static double mt_next(void *p) { return orc_mt64_uniform(&((mt_state *)p)->g); }
 
-int64_t orc_widom_run_mt(const orc_system *s, const orc_molecule *m, uint64_t seed, int64_t n_cycles,
-                         int steps_per_cycle, double *blocks) {
+int64_t orc_widom_run_mt_trials(const orc_system *s, const orc_molecule *m, uint64_t seed, int64_t n_cycles,
+                                int steps_per_cycle, double *blocks, int64_t max_ins, double *recs,
+                                double *fb, double *cord, double *u_sel) {
   mt_state st;
   orc_mt64_seed(&st.g, seed);
   int32_t *bmap = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n_cycles > 0 ? n_cycles : 1));
@@ -1538,6 +1546,7 @@ int64_t orc_widom_run_mt(const orc_system *s, const orc_molecule *m, uint64_t se
   for (int i = 0; i < 4 * NR_BLOCKS; i++) blocks[i] = 0.0;
   int64_t count = 0;
   double volume = s->props[6];
+  const int nc = m->n_sites > 1 ? m->k_or * m->n_sites * 3 : 0;
   for (int64_t c = 0; c < n_cycles; c++) {
     int b = bmap[c];
     (void)orc_mt64_uniform(&st.g); /* SelectedSystem (src/monte_carlo.c:626) */
@@ -1546,12 +1555,32 @@ int64_t orc_widom_run_mt(const orc_system *s, const orc_molecule *m, uint64_t se
       double ran = orc_mt64_uniform(&st.g); /* move selection (src/monte_carlo.c:649) */
       if (ran < 1.0) {
         double rec[ORC_REC_N];
+        orc_trials tr;
+        memset(&tr, 0, sizeof(tr));
+        /* WidomIdealGasAccumulated: weight * Volume / (NumberOfMolecules + 1)
+           with the integer molecules present (src/mc_moves.c:7861-7863) */
         blocks[4 * b + 0] += 1.0;
-        blocks[4 * b + 3] += 1.0 * (volume / ((double)0 + 1.0));
-        orc_widom_insertion(s, m, mt_next, &st, rec, NULL);
+        blocks[4 * b + 3] += 1.0 * (volume / ((double)s->ads_nmol + 1.0));
+        orc_widom_insertion(s, m, mt_next, &st, rec, &tr);
         if (rec[ORC_REC_OVERLAP] == 0.0) {
           blocks[4 * b + 1] += 1.0 * rec[ORC_REC_W];
           blocks[4 * b + 2] += 1.0 * (0.0 + rec[ORC_REC_DU]) * rec[ORC_REC_W];
+        }
+        if (count < max_ins) {
+          /* what the insertion drew, in the layout of host-supplied trials */
+          if (recs) memcpy(recs + (size_t)count * ORC_REC_N, rec, sizeof(rec));
+          if (fb)
+            for (int t = 0; t < m->k_fb; t++)
+              for (int d = 0; d < 3; d++) fb[((size_t)count * m->k_fb + t) * 3 + d] = tr.fb[t][d];
+          if (cord && nc)
+            for (int t = 0; t < m->k_or; t++)
+              for (int a = 0; a < m->n_sites; a++)
+                for (int d = 0; d < 3; d++)
+                  cord[(size_t)count * nc + (t * m->n_sites + a) * 3 + d] = tr.cord[t][a][d];
+          if (u_sel) {
+            u_sel[2 * count] = tr.u_sel_fb;
+            u_sel[2 * count + 1] = tr.u_sel_or;
+          }
         }
         count++;
       }
@@ -1559,6 +1588,11 @@ int64_t orc_widom_run_mt(const orc_system *s, const orc_molecule *m, uint64_t se
   }
   free(bmap);
   return count;
+}
+
+int64_t orc_widom_run_mt(const orc_system *s, const orc_molecule *m, uint64_t seed, int64_t n_cycles,
+                         int steps_per_cycle, double *blocks) {
+  return orc_widom_run_mt_trials(s, m, seed, n_cycles, steps_per_cycle, blocks, 0, NULL, NULL, NULL, NULL);
 }
 
 double orc_framework_density(double mass_g_mol, double volume_A3) {

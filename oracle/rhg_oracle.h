@@ -184,6 +184,14 @@ void orc_stream_trials(const orc_system *s, const orc_molecule *m, uint64_t seed
  * blocks: 5 x 4 (count, sumW, sumW*dU, idealgas) ; returns #insertions */
 int64_t orc_widom_run_mt(const orc_system *s, const orc_molecule *m, uint64_t seed,
                          int64_t n_cycles, int steps_per_cycle, double *blocks);
+/* The same run, also returning for its first max_ins insertions the record
+ * and the trials each drew (first-bead positions k_fb x 3, rotated offsets
+ * k_or x n_sites x 3 from the starting bead, the two selection uniforms; the
+ * layout of host-supplied trials; zero where the move drew nothing). Any
+ * output may be NULL. */
+int64_t orc_widom_run_mt_trials(const orc_system *s, const orc_molecule *m, uint64_t seed, int64_t n_cycles,
+                                int steps_per_cycle, double *blocks, int64_t max_ins, double *recs,
+                                double *fb, double *cord, double *u_sel);
 
 /* Block index of each cycle (src/statistics.c:904,1126) */
 void orc_block_of_cycles(int64_t n_cycles, int32_t *block);
@@ -231,7 +239,7 @@ void orc_fourier_move(const orc_system *s, int n_new, const int *new_types, cons
 /* SurfaceAreaMove (src/mc_moves.c:16434-16520): area [A^2] and accessible
  * points per atom (nullable) with counter-stream points */
 double orc_surface_area(const orc_system *s, int probe_type, double factor, int64_t samples, uint64_t seed,
-                        uint64_t *counts);
+                        const uint8_t *interacts /* nullable: all */, uint64_t *counts);
 /* InterpolateCoulombGrid (src/grids.c:1200-1295) */
 double orc_interpolate_coulomb_grid(const float *grid, const double geom[12], int bc, const double ub[9],
                                     const double uinv[9], const double pos[3], double charge);

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -122,6 +123,9 @@ struct rhg_workspace {
   std::map<std::string, DevBuf> bufs;
   uint64_t gen = 0;
   rhg_ctx *owner = nullptr;
+  // process-wide unique id: captured graphs are keyed by it, never by the
+  // address, which a new workspace may reuse after this one is destroyed
+  uint64_t uid = 0;
 };
 
 struct rhg_ctx {
@@ -184,6 +188,10 @@ struct rhg_ctx {
   std::map<std::string, hipGraphExec_t> graph_cache;
   // host-side fault hook: fail the chunk after this many more chunks (<0 off)
   int64_t fault_after = -1;
+  // ideal-gas Rosenbluth sums of the insertions enqueued since the last
+  // collect, per block: count x V / (N + 1) with N the molecules present
+  // when they were enqueued (src/mc_moves.c:7861-7863)
+  double ig_pending[RHG_NR_BLOCKS] = {0.0, 0.0, 0.0, 0.0, 0.0};
   // adsorbates present (finite loading): molecule m holds atoms
   // [ads_start[m], ads_start[m+1]); the adsorbate structure factor
   // (StoreTotalChargeAdsorbates) and net charge follow them
@@ -683,24 +691,24 @@ int ensure_image_list(rhg_ctx *c) {
 }
 
 int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const EnergyOut &o,
-                      const char *tag);
+                      const char *tag, const BinWork *given = nullptr);
 
 // Site energies of one site list of pseudo-atom `type`: terms with a grid in
 // use are interpolated (src/framework_energy.c:73-76, 197-200: the Coulomb
 // grid value times the pseudo-atom's charge, unscaled), the others summed
 // over the framework atoms.
 int site_energy(rhg_ctx *c, int type, const EnergyArgs &a, const SiteList &s, const EnergyOut &o,
-                const char *tag) {
+                const char *tag, const BinWork *given = nullptr) {
   if (s.n <= 0) return RHG_OK;
   rhg_grid *gv = a.do_vdw ? vdw_grid_of(c, type) : nullptr;
   rhg_grid *gc = a.do_cc ? c->coulomb_grid : nullptr;
-  if (!gv && !gc) return site_energy_pairs(c, a, s, o, tag);
+  if (!gv && !gc) return site_energy_pairs(c, a, s, o, tag, given);
   EnergyArgs rest = a;
   rest.do_vdw = a.do_vdw && !gv;
   rest.do_cc = a.do_cc && !gc;
   const bool merge = rest.do_vdw || rest.do_cc;
   if (merge)
-    if (int rc = site_energy_pairs(c, rest, s, o, tag)) return rc;
+    if (int rc = site_energy_pairs(c, rest, s, o, tag, given)) return rc;
   InterpArgs iv, ic;
   if (gv) iv = interp_args(gv, 1.0);
   if (gc) ic = interp_args(gc, c->types[type].q);
@@ -710,9 +718,11 @@ int site_energy(rhg_ctx *c, int type, const EnergyArgs &a, const SiteList &s, co
 }
 
 // Site energies of one site list, culled through the cell grid when enabled.
-// `tag` keeps the binning scratch of concurrent lists apart.
+// `tag` keeps the binning scratch of concurrent lists apart; `given`
+// (nullable) is binning scratch the caller provides, for lists whose cell
+// grid (and image lists) already exist.
 int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const EnergyOut &o,
-                      const char *tag) {
+                      const char *tag, const BinWork *given) {
   if (s.n <= 0) return RHG_OK;
   if (!c->cull || c->n == 0) {
     Timed t(c, "site_energy");
@@ -722,6 +732,22 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
   if (int rc = ensure_cell_grid(c)) return rc;
   if (c->precision != RHG_PREC_FP64 && c->use_images)
     if (int rc = ensure_image_list(c)) return rc;
+  CellGrid g = c->cg;
+  if (!c->use_images) {
+    g.img_off = nullptr;
+    g.img = nullptr;
+  }
+  if (given) {
+    BinWork b = *given;
+    if (!(a.precision != RHG_PREC_FP64 && g.img_off && a.par)) b.fix = nullptr;
+    {
+      Timed tm(c, "bin");
+      launch_bin_sites(c->cell, c->cg, s, b, c->stream);
+    }
+    Timed tm(c, "site_energy");
+    launch_site_energy_culled(a, g, s, b, o, c->stream);
+    return RHG_OK;
+  }
   const int ncell = c->cg.ncell;
   std::string t = tag;
   BinWork b;
@@ -731,11 +757,6 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
   b.fill = (int32_t *)work(c, (t + "_fill").c_str(), (size_t)ncell * sizeof(int32_t));
   b.start = (int32_t *)work(c, (t + "_start").c_str(), (size_t)(ncell + 1) * sizeof(int32_t));
   b.tile_off = (int32_t *)work(c, (t + "_tiles").c_str(), (size_t)(ncell + 1) * sizeof(int32_t));
-  CellGrid g = c->cg;
-  if (!c->use_images) {
-    g.img_off = nullptr;
-    g.img = nullptr;
-  }
   // the image-list kernels read the sites' sorted FP32 offsets
   b.fix = nullptr;
   if (a.precision != RHG_PREC_FP64 && g.img_off && a.par) {
@@ -1373,6 +1394,18 @@ void block_bounds(int64_t n_total, int spc, int64_t bounds[6]) {
   bounds[RHG_NR_BLOCKS] = n_total;
 }
 
+// WidomIdealGasAccumulated of insertions [first, first+n) of a job whose
+// blocks start at jb: weight x Volume / (NumberOfMolecules + 1) per insertion
+// (src/mc_moves.c:7861-7863; CFBiasingWeight is 1 on this path), with the
+// integer molecules present now (the adsorbates; no cations on this path).
+void note_ideal_gas(rhg_ctx *c, const int64_t jb[6], int64_t first, int64_t n) {
+  const double per = c->props[6] / ((double)c->ads_nmol + 1.0);
+  for (int b = 0; b < RHG_NR_BLOCKS; ++b) {
+    const int64_t lo = std::max(first, jb[b]), hi = std::min(first + n, jb[b + 1]);
+    if (hi > lo) c->ig_pending[b] += (double)(hi - lo) * per;
+  }
+}
+
 WidomArgs widom_args(rhg_ctx *c, const MolDev &m, uint64_t seed, int64_t cap) {
   WidomArgs a;
   memset(&a, 0, sizeof(a));
@@ -1415,6 +1448,7 @@ int widom_enqueue(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, uint64_t
   int64_t jb[6];
   block_bounds(n_total, spc, jb);
   launch_set_ctl(c->d_ctl, first, first + n, jb, st);
+  note_ideal_gas(c, jb, first, n);
   const int64_t n_chunks = (n + cap - 1) / cap;
   const bool use_graph = c->graphs && !c->prof && !rec && n_chunks > 1 && c->fault_after < 0;
   for (int64_t k = 0; k < n_chunks; ++k) {
@@ -1428,8 +1462,8 @@ int widom_enqueue(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, uint64_t
       key.append((const char *)&seed, sizeof(seed));
       key.append((const char *)&cap, sizeof(cap));
       key.append((const char *)&c->gen, sizeof(c->gen));
-      const uint64_t wgen = c->ws ? c->ws->gen : 0;
-      key.append((const char *)&c->ws, sizeof(c->ws));
+      const uint64_t wgen = c->ws ? c->ws->gen : 0, wuid = c->ws ? c->ws->uid : 0;
+      key.append((const char *)&wuid, sizeof(wuid));
       key.append((const char *)&wgen, sizeof(wgen));
       key.append((const char *)&c->precision, sizeof(c->precision));
       key.append((const char *)&c->cull, sizeof(c->cull));
@@ -1484,14 +1518,16 @@ int widom_collect(rhg_ctx *c, rhg_widom_blocks *blocks) {
   double acc[16];
   HIPC(c, hipMemcpyAsync(acc, c->d_acc, 15 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   fill(c->d_acc, 15, 0.0, c->stream);
+  double ig[RHG_NR_BLOCKS];
+  for (int b = 0; b < RHG_NR_BLOCKS; ++b) {
+    ig[b] = c->ig_pending[b];
+    c->ig_pending[b] = 0.0;
+  }
   HIPC(c, hipStreamSynchronize(c->stream));
   if (blocks) {
-    const double V = c->props[6];
     for (int b = 0; b < RHG_NR_BLOCKS; ++b) {
       blocks->count[b] += acc[10 + b];
-      // ideal-gas Rosenbluth weight of an insertion into N = 0 adsorbates:
-      // V / (N + 1) (src/mc_moves.c:7929)
-      blocks->ideal_gas[b] += acc[10 + b] * (V / ((double)0 + 1.0));
+      blocks->ideal_gas[b] += ig[b];
       blocks->sum_w[b] += acc[b];
       blocks->sum_wdu[b] += acc[5 + b];
     }
@@ -1848,6 +1884,7 @@ int rhg_widom_run(rhg_ctx *c, const rhg_molecule *mol, uint64_t seed, int64_t n_
   // sums of earlier submits that were never collected stay out of this call
   if (int rc = widom_state(c)) return rc;
   fill(c->d_acc, 15, 0.0, c->stream);
+  for (double &v : c->ig_pending) v = 0.0;
   if (int rc = widom_enqueue(c, mol, m, seed, n_total, first, n, steps_per_cycle, rec)) return rc;
   return widom_collect(c, blocks);
 }
@@ -1926,6 +1963,7 @@ int rhg_widom_chain_run(rhg_ctx *c, const rhg_chain *ch, uint64_t seed, int64_t 
   hipStream_t st = c->stream;
   if (int rc = widom_state(c)) return rc;
   fill(c->d_acc, 15, 0.0, st);
+  for (double &v : c->ig_pending) v = 0.0;
   ChainStep *d_steps = (ChainStep *)work(c, "c_steps", steps.size() * sizeof(ChainStep));
   if (!d_steps) return fail(c, RHG_ERR_OOM, "chain step tables");
   HIPC(c, hipMemcpyAsync(d_steps, steps.data(), steps.size() * sizeof(ChainStep), hipMemcpyHostToDevice, st));
@@ -1940,6 +1978,7 @@ int rhg_widom_chain_run(rhg_ctx *c, const rhg_chain *ch, uint64_t seed, int64_t 
   int64_t jb[6];
   block_bounds(n_total, steps_per_cycle, jb);
   launch_set_ctl(c->d_ctl, first, first + n, jb, st);
+  note_ideal_gas(c, jb, first, n);
   for (int64_t off = 0; off < n; off += cap) {
     const int64_t cnt = std::min(cap, n - off);
     if (int rc = check_fault(c)) return rc;
@@ -2363,7 +2402,7 @@ int rhg_ewald_fourier_move(rhg_ctx *c, int32_t n_new, const int32_t *new_types, 
 
 // ------------------------------------------------------------ surface area
 int rhg_surface_area(rhg_ctx *c, int32_t probe_type, double distance_factor, int64_t samples, uint64_t seed,
-                     double *area, uint64_t *counts) {
+                     const uint8_t *interacts, double *area, uint64_t *counts) {
   if (!c || !area || samples < 1 || !(distance_factor > 0.0)) return RHG_ERR_INVALID;
   if (int rc = check_type(c, probe_type)) return rc;
   *area = 0.0;
@@ -2375,8 +2414,10 @@ int rhg_surface_area(rhg_ctx *c, int32_t probe_type, double distance_factor, int
   // (PotentialParms[typeA][typeB][1], src/mc_moves.c:16458)
   std::vector<double4> h((size_t)c->n);
   std::vector<double> d((size_t)c->n);
+  // an atom whose pseudo-atom does not interact gets distance 0: no sphere
+  // of its own, and rr < 0 never excludes a point
   for (int i = 0; i < c->n; ++i) {
-    d[i] = distance_factor * (0.5 * (t.sigma + c->sig[i]));
+    d[i] = (!interacts || interacts[i]) ? distance_factor * (0.5 * (t.sigma + c->sig[i])) : 0.0;
     h[i] = make_double4(c->x[i], c->y[i], c->z[i], d[i]);
   }
   double4 *da = (double4 *)work(c, "sa_atoms", (size_t)c->n * sizeof(double4));
@@ -2403,8 +2444,9 @@ int rhg_surface_area(rhg_ctx *c, int32_t probe_type, double distance_factor, int
   // sum_i (counted/total) 4 pi d_i^2 in atom order (src/mc_moves.c:16478-16480)
   double sum = 0.0;
   for (int i = 0; i < c->n; ++i) {
-    sum += ((double)hc[i] / (double)samples) * 4.0 * M_PI * (d[i] * d[i]);
-    if (counts) counts[i] = hc[i];
+    const bool on = !interacts || interacts[i];
+    if (on) sum += ((double)hc[i] / (double)samples) * 4.0 * M_PI * (d[i] * d[i]);
+    if (counts) counts[i] = on ? hc[i] : 0;
   }
   *area = sum;
   return RHG_OK;
@@ -2969,8 +3011,10 @@ int rhg_blocked_pocket(rhg_ctx *c, int64_t n, const double *xyz, uint8_t *blocke
 
 int rhg_workspace_create(int device, rhg_workspace **out) {
   if (!out) return RHG_ERR_INVALID;
+  static std::atomic<uint64_t> next_uid{1};
   *out = new rhg_workspace();
   (*out)->device = device;
+  (*out)->uid = next_uid.fetch_add(1);
   return RHG_OK;
 }
 

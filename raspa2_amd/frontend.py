@@ -281,6 +281,16 @@ class ForceField:
             raise KeyError(f"no force-field parameters for pseudo-atom {name!r}")
         return got
 
+    def interacts(self, name):
+        """False when the pseudo-atom's potential is NONE (PseudoAtoms[].Interaction,
+        src/potentials.c:12170-12185), the last matching line deciding."""
+        got = True
+        for pat, pot, _ in self.entries:
+            hit = name.lower().startswith(pat[:-1].lower()) if pat.endswith("_") else name.lower() == pat.lower()
+            if hit:
+                got = pot != "NONE"
+        return got
+
     def mass_charge(self, name, element, cif_charge=None, use_cif_charges=False):
         if name in self.pseudo:
             _, m, q = self.pseudo[name]
@@ -558,8 +568,9 @@ def get_geometric_surface_area(structure, unit_cells=(1, 1, 1), cycles=500, inpu
     ff.check_pairs(n2.labels[:1], fw.labels)
     ctx = fw.context(cutoff=12.8, temperature=298.0, device=device, shifted=ff.shifted)
     t = ctx.add_pseudo_atom(*n2.sites[n2.start])
+    mask = np.array([ff.interacts(lab) for lab in fw.labels], dtype=np.uint8)
     area, _ = ctx.SurfaceArea(t, samples if samples is not None else 100 * cycles, seed=seed,
-                              distance_factor=1.0)
+                              distance_factor=1.0, interacts=mask)
     info = ctx.info()
     ctx.close()
     if units == "A^2":

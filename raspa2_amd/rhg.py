@@ -196,7 +196,7 @@ def _load():
     f("rhg_inter_energy_at_positions", _P, _I64, _P, _I32, _I32, _P, _P)
     f("rhg_ewald_fourier_move", _P, _I32, _P, _P, _I32, _P)
     f("rhg_widom_retrace_trials", _P, _P, _P, _I64, _P, _P, _P, _P)
-    f("rhg_surface_area", _P, _I32, _D, _I64, _U64, _P, _P)
+    f("rhg_surface_area", _P, _I32, _D, _I64, _U64, _P, _P, _P)
     f("rhg_use_coulomb_grid", _P, _P)
     f("rhg_profile_enable", _P, _I32)
     f("rhg_profile_get", _P, ctypes.c_char_p, ctypes.POINTER(_I64), ctypes.POINTER(_D))
@@ -596,12 +596,15 @@ class Context:
                                                  _ptr(fb), _ptr(cord), _ptr(rec)), "retrace")
         return rec
 
-    def SurfaceArea(self, probe_type, samples, seed=1, distance_factor=2.0 ** (1.0 / 6.0)):
+    def SurfaceArea(self, probe_type, samples, seed=1, distance_factor=2.0 ** (1.0 / 6.0), interacts=None):
         """(area [A^2], accessible points per atom) of SurfaceAreaMove with
-        `samples` points per framework atom."""
+        `samples` points per framework atom; interacts (optional, per atom)
+        False for atoms of NONE pseudo-atoms."""
         area = _D()
         counts = np.zeros(self.n_atoms, dtype=np.uint64)
+        mask = None if interacts is None else np.ascontiguousarray(interacts, dtype=np.uint8)
         self._check(lib.rhg_surface_area(self.h, probe_type, distance_factor, samples, seed,
+                                         None if mask is None else _ptr(mask),
                                          ctypes.byref(area), _ptr(counts)), "surface_area")
         return area.value, counts
 

@@ -312,3 +312,44 @@ def test_grid_file_from_oracle(W, O, R, tmp_path):
     for p, ug in zip(pts, u):
         ref = O.interpolate_grid(nodes, geom, O.BC_TRICLINIC, fw.unit_box, inv, p)
         assert abs(ug - ref) <= 1e-9 * max(1.0, abs(ref))
+
+
+def test_c2_full_grid_against_oracle(W, O, R):
+    """The C2 energy grid at its full size: methane probe over the triclinic
+    26 A / 80 deg cell (1200 atoms) at 0.1 A spacing, 352 x 294 x 254 nodes
+    (MakeGrid's node rule, src/grids.c:431-583).  At 10^4 seeded off-grid
+    points the device interpolation of the FP64 build (FP32 node storage, as
+    RASPA2 stores grids) equals the oracle's InterpolateVDWGrid over corner
+    nodes computed on the fly within 1e-9 relative (max(1, |U|)); the mixed
+    build keeps every capped node and agrees within 1e-5 relative where
+    |U| > 1 K, 1e-3 K below."""
+    fw = W.c2_framework()
+    sys_o = oracle_system(fw)
+    to = sys_o.add_pseudo_atom(148.0, 3.73, 0.0)
+    geom = O.grid_geometry(fw.unit_box, 0.1)
+    assert tuple(int(v) + 1 for v in geom[:3]) == (352, 294, 254)
+    inv, _ = O.invert3x3(fw.unit_box)
+    rng = np.random.Generator(np.random.PCG64(1201))
+    pts = rng.uniform(0.0, 1.0, (10000, 3)) @ np.asarray(fw.unit_box).reshape(3, 3)
+    ref = np.array([sys_o.interpolate_direct(to, geom, O.BC_TRICLINIC, fw.unit_box, inv, p) for p in pts])
+    e2k = O.units()["energy_to_kelvin"]
+    for prec in (R.PREC_FP64, R.PREC_MIXED):
+        print(f"C2 0.1 A grid, precision {prec}", flush=True)
+        ctx = fw.context(precision=prec)
+        tg = ctx.add_pseudo_atom(148.0, 3.73, 0.0)
+        g = ctx.MakeGrid(tg, fw.unit_box, 0.1)
+        assert np.array_equal(g.geom, geom)
+        u = g.interpolate(pts)
+        capped = ref >= 1e7
+        assert np.array_equal(u >= 1e7, capped)
+        ok = ~capped
+        assert ok.sum() > 5000
+        if prec == R.PREC_FP64:
+            assert np.max(np.abs(u[ok] - ref[ok]) / np.maximum(np.abs(ref[ok]), 1.0)) <= 1e-9
+        else:
+            err = np.abs(u[ok] - ref[ok]) * e2k
+            big = np.abs(ref[ok]) * e2k > 1.0
+            assert np.all(err[big] <= 1e-5 * np.abs(ref[ok][big]) * e2k)
+            assert np.all(err[~big] <= 1e-3)
+        g.close()
+        ctx.close()

@@ -60,6 +60,14 @@ def test_surface_area_device_equals_oracle(W, O, R):
         assert np.array_equal(cg, co)
         assert abs(ag - ao) <= 1e-12 * ao
         assert 0 < cg.sum() < 300 * fw.n
+    # atoms of NONE pseudo-atoms (here every hydrogen) carry no sphere and
+    # exclude no point
+    mask = np.array([not lab.startswith("H") for lab in fw.labels], dtype=np.uint8)
+    assert 0 < mask.sum() < fw.n
+    ag, cg = ctx.SurfaceArea(tg, 300, seed=7, distance_factor=1.0, interacts=mask)
+    ao, co = s.surface_area(to, 300, seed=7, factor=1.0, interacts=mask)
+    assert np.array_equal(cg, co) and not cg[mask == 0].any()
+    assert abs(ag - ao) <= 1e-12 * ao
 
 
 def test_surface_area_wrapper_call(R):

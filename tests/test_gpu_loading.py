@@ -174,3 +174,29 @@ def test_retrace_existing_molecules(W, O, R):
     assert np.all(rb[:, FIELDS["STAGE"]] == 0)
     # without excluding themselves the molecules sit on their own atoms
     assert np.all(rb[:, FIELDS["FBMASK"]].astype(np.int64) & 1)
+
+
+def test_ideal_gas_accumulator_counts_loaded_molecules(W, O, R):
+    """WidomIdealGasAccumulated adds Volume / (N + 1) per insertion with the
+    N molecules present when the insertions run (src/mc_moves.c:7861-7863):
+    100 adsorbed CO2 give V/101 per insertion, cleared adsorbates V/1, and a
+    run split into submits gives the same sums as one call."""
+    fw = W.c1_framework()
+    ctx = fw.context(charge_method=R.CHARGE_EWALD, precision=R.PREC_MIXED)
+    tg, mg = gpu_molecule(ctx, W.CO2)
+    start, tys, xyz = _load(W, fw, 100, 41, tg)
+    ctx.set_adsorbates(start, tys, xyz)
+    V = ctx.info()["volume"]
+    b, _ = ctx.widom_run(mg, 5, 5000, steps_per_cycle=20)
+    a = b.as_array()
+    assert a[:, 0].sum() == 5000
+    assert np.array_equal(a[:, 3], a[:, 0] * (V / 101.0))
+    ctx.widom_submit(mg, 5, 5000, first=0, n=1700)
+    ctx.widom_submit(mg, 5, 5000, first=1700, n=3300)
+    a2 = ctx.widom_collect().as_array()
+    assert np.array_equal(a2[:, 0], a[:, 0])
+    assert np.allclose(a2[:, 3], a[:, 3], rtol=1e-15, atol=0)
+    ctx.set_adsorbates([0], [], np.zeros((0, 3)))
+    a3 = ctx.widom_run(mg, 5, 5000, steps_per_cycle=20)[0].as_array()
+    assert np.array_equal(a3[:, 3], a3[:, 0] * V)
+    ctx.close()

@@ -130,3 +130,53 @@ def test_chunked_equals_single(W, O, R):
     for first in range(0, 3000, 1000):
         ctx.widom_run(mg, 5, 3000, first=first, n=1000, blocks=b2)
     assert np.allclose(b1.as_array(), b2.as_array(), rtol=1e-14, atol=0)
+
+
+def _irmof1_helium(O, W):
+    import os
+
+    from raspa2_amd import frontend as F
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    share = os.path.join(gold, "share", "raspa")
+    ff = F.ForceField.load(os.path.join(share, "forcefield", "GenericMOFs"))
+    text = open(os.path.join(share, "structures", "cif", "IRMOF-1.cif")).read()
+    fw = F.framework_from_cif(text, ff)
+    he = F.read_molecule(os.path.join(share, "molecules", "TraPPE", "helium.def"), ff, k_fb=10)
+    assert ff.shifted
+    return fw, he, O.CHARGE_NONE
+
+
+@pytest.mark.parametrize("case", ["helium_irmof1", "co2_c1"])
+def test_raspa_mt_order_through_host_trials(W, O, R, case):
+    """The drop-in path of a RASPA2 caller that keeps its own random stream:
+    the oracle runs RASPA2's MT19937-64 driver (seed 1417341545, the global
+    draw order of src/monte_carlo.c:626-649 with src/cbmc.c:652-667,753,1535,
+    src/utils.c:688-708,835-877) and records what each insertion drew; the
+    same trials through rhg_widom_run_trials give the oracle's records
+    insertion by insertion (FP64: flags, masks, selections and stages equal,
+    host VDW bit for bit, the other terms and W within 1e-9), and the
+    Rosenbluth sum of the records equals the driver's block sums."""
+    if case == "helium_irmof1":
+        fw, probe, cm = _irmof1_helium(O, W)
+        n_cycles, spc = 150, 20
+    else:
+        fw, probe, cm = W.c1_framework(), W.CO2, O.CHARGE_EWALD
+        n_cycles, spc = 30, 20
+    print(f"MT-order trials: {case}, {n_cycles * spc} insertions", flush=True)
+    sys_o = oracle_system(fw, charge_method=cm)
+    _, mo = oracle_molecule(sys_o, probe)
+    n, blocks, ro, fb, cord, usel = O.widom_run_mt_trials(sys_o, mo, 1417341545, n_cycles, spc)
+    assert n == n_cycles * spc == len(ro)
+    ctx = fw.context(charge_method=cm, precision=R.PREC_FP64)
+    _, mg = gpu_molecule(ctx, probe)
+    rg = ctx.widom_run_trials(mg, fb, cord if probe.n_sites > 1 else None, usel)
+    _check_flags(ro, rg)
+    assert same_bits(ro[:, FIELDS["UVDW"]], rg[:, FIELDS["UVDW"]])
+    acc = ro[:, FIELDS["STAGE"]] == 4
+    assert acc.sum() > 50
+    for f in ("UREAL", "UFHOST", "UFADS", "DU"):
+        a, b = ro[acc, FIELDS[f]], rg[acc, FIELDS[f]]
+        assert np.max(np.abs(a - b) / np.maximum(np.abs(a), 1.0)) <= 1e-9, f
+    assert np.max(np.abs(ro[acc, 0] - rg[acc, 0]) / ro[acc, 0]) <= 1e-9
+    assert abs(rg[:, 0].sum() - blocks[:, 1].sum()) <= 1e-9 * blocks[:, 1].sum()
+    ctx.close()

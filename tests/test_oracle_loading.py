@@ -213,6 +213,16 @@ def test_widom_ghost_adsorbates_change_nothing(O, W):
     r0 = O.widom_run_stream(s0, mols[0], 9, 0, 60)
     r1 = O.widom_run_stream(s1, mols[1], 9, 0, 60)
     assert np.array_equal(r0, r1)
+    # RASPA2's driver: the same Rosenbluth sums, and an ideal-gas sum of
+    # Volume / (N + 1) per insertion with the N = 12 (ghost) molecules present
+    # (src/mc_moves.c:7861-7863) against Volume / 1 for the empty framework
+    n0, b0 = O.widom_run_mt(s0, mols[0], 17, 6, 10)
+    n1, b1 = O.widom_run_mt(s1, mols[1], 17, 6, 10)
+    V = s0.info()["volume"]
+    assert n0 == n1 == 60 and np.array_equal(b0[:, :3], b1[:, :3])
+    # (sums of equal terms: equal to the product up to rounding)
+    assert np.allclose(b0[:, 3], b0[:, 0] * V, rtol=1e-13, atol=0)
+    assert np.allclose(b1[:, 3], b1[:, 0] * (V / 13.0), rtol=1e-13, atol=0)
 
 
 def test_widom_adsorbate_bookkeeping(O, W):

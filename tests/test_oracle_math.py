@@ -414,6 +414,66 @@ def test_mt_driver_matches_stream_statistically(O):
     assert abs(w_mt - w_st) < 5 * np.sqrt(2) * se
 
 
+def test_mt_trials_record_replays(O, W):
+    """orc_widom_run_mt_trials records everything an insertion of RASPA2's
+    MT19937-64 driver drew: replaying the recorded first-bead positions,
+    rotated offsets and selection uniforms through the host-trial path gives
+    every record again, bit for bit, and the driver's block sums are the
+    records' sums (helium in IRMOF-1 with 10 first-bead trials; CO2 with
+    Ewald in the 25.8 A charged cell)."""
+    import json, os
+    G = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "irmof1.json")))
+    a, b, c, al, be, ga = G["cell"]
+    box, inv, props = O.cell_make(a, b, c, al, be, ga)
+    frac = np.array([at["frac"] for at in G["atoms"]])
+    xyz = frac @ box.reshape(3, 3)
+    n = len(frac)
+    s = O.System(box, O.BC_TRICLINIC, xyz[:, 0], xyz[:, 1], xyz[:, 2], np.zeros(n),
+                 [at["eps_k"] for at in G["atoms"]], [at["sigma"] for at in G["atoms"]],
+                 mass=[at["mass"] for at in G["atoms"]], temperature=298.0)
+    he = s.add_pseudo_atom(10.9, 2.64)
+    cases = [(s, O.molecule([he], [[0, 0, 0]], start=0, k_fb=10, k_or=10), 40)]
+    fw = W.c1_framework(n_rep=1)
+    s2 = O.System(*fw.oracle_args()[:8], mass=fw.mass, charge_method=O.CHARGE_EWALD)
+    ty = [s2.add_pseudo_atom(*site) for site in W.CO2.sites]
+    cases.append((s2, O.molecule(ty, W.CO2.body, start=W.CO2.start, k_fb=3, k_or=10), 20))
+    for sys_o, mol, n_cycles in cases:
+        n_ins, blocks, rec, fb, cord, usel = O.widom_run_mt_trials(sys_o, mol, 1417341545, n_cycles, 20)
+        assert n_ins == 20 * n_cycles == len(rec)
+        again = np.array([O.widom_insertion_trials(sys_o, mol, fb[i], cord[i] if mol.n_sites > 1 else
+                                                   np.zeros((mol.k_or, 1, 3)), usel[i]) for i in range(n_ins)])
+        assert np.array_equal(again, rec)
+        assert (rec[:, 13] == 4).sum() > 10
+        ok = rec[:, 6] == 0
+        assert abs(rec[ok, 0].sum() - blocks[:, 1].sum()) <= 1e-12 * blocks[:, 1].sum()
+        assert blocks[:, 0].sum() == n_ins
+
+
+def test_sincos_portable_against_libm(O):
+    """The portable sin/cos of the Shoemake angles (written identically for
+    the device) against glibc's sin and cos on [0, 2 pi]: absolute error at
+    most 2^-53 (half an ulp of 1), and at most one ulp of the value wherever
+    |value| > 1e-3; the reduction is exact at multiples of pi/2."""
+    import ctypes
+    f = O.lib().orc_sincos_portable
+    f.restype = None
+    f.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    rng = np.random.default_rng(7)
+    x = np.concatenate([rng.uniform(0.0, 2.0 * np.pi, 50000), np.arange(5) * (np.pi / 2),
+                        np.nextafter(np.arange(1, 9) * (np.pi / 4), 0.0), [1e-300, 2.0 * np.pi]])
+    sv, cv = ctypes.c_double(), ctypes.c_double()
+    S, C = np.empty_like(x), np.empty_like(x)
+    for i, v in enumerate(x):
+        f(float(v), ctypes.byref(sv), ctypes.byref(cv))
+        S[i], C[i] = sv.value, cv.value
+    for got, ref in ((S, np.sin(x)), (C, np.cos(x))):
+        err = np.abs(got - ref)
+        assert err.max() <= 2.0 ** -53
+        big = np.abs(ref) > 1e-3
+        assert np.all(err[big] <= np.spacing(np.abs(ref[big])))
+    assert S[50000] == 0.0 and C[50000] == 1.0
+
+
 def test_tabular_grid_branches(O, W):
     """UseTabularGrid in the oracle: the VDW energy of a type with a grid is
     the interpolation (positions wrapped into the grid's unit cell), other
@@ -471,3 +531,29 @@ def test_surface_area_geometry(O):
         assert abs(got / n - p) < 5.0 * math.sqrt(p * (1.0 - p) / n)
     expect = (c2[0] / n) * 4 * math.pi * dA * dA + (c2[1] / n) * 4 * math.pi * dB * dB
     assert abs(area2 - expect) <= 1e-12 * expect
+
+
+def test_surface_area_none_atom(O):
+    """An atom of a NONE pseudo-atom carries no sphere and excludes no point
+    (src/mc_moves.c:16456, src/framework.c:11676): placed last (so the other
+    atoms keep their counter-stream points) and masked, it leaves area and
+    counts exactly those of the system without it; interacting, the same
+    atom hides part of its neighbour's sphere and adds its own."""
+    box, inv, props = O.cell_make(40.0, 40.0, 40.0)
+    base = ([10.0, 20.0], [10.0, 10.0], [10.0, 10.0], [0.0, 0.0], [50.0, 40.0], [3.0, 2.6])
+    s0 = O.System(box, O.BC_RECTANGULAR, *base)
+    t0 = s0.add_pseudo_atom(36.0, 3.31)
+    a0, c0 = s0.surface_area(t0, 20000, seed=11)
+    x, y, z, q, e, sg = ([v for v in col] for col in base)
+    x.append(11.5)  # 1.5 A from atom 0: inside its contact sphere
+    y.append(10.0)
+    z.append(10.0)
+    q.append(0.0)
+    e.append(0.0)
+    sg.append(1.0)
+    s1 = O.System(box, O.BC_RECTANGULAR, x, y, z, q, e, sg)
+    t1 = s1.add_pseudo_atom(36.0, 3.31)
+    a1, c1 = s1.surface_area(t1, 20000, seed=11, interacts=[1, 1, 0])
+    assert a1 == a0 and list(c1[:2]) == list(c0) and c1[2] == 0
+    a2, c2 = s1.surface_area(t1, 20000, seed=11)
+    assert c2[0] < c0[0] and c2[2] > 0 and a2 != a0
