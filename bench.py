@@ -13,15 +13,23 @@ device.
     python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--workload c1|c3|c4]
 
 Prints one JSON line.  The CPU baseline is the oracle (the FP64 C
-restatement) on the host cores over a bounded sample of the same insertions.
+restatement) on every host core granted to the process over a bounded
+sample of the same insertions.  Beside the C1 headline the line carries
+`screening` (C3 and C4 below: served insertions/s, p50/p99 latency, the
+unbatched plain GPU rate, an end-to-end roofline and the CPU baseline) and
+`grid` (C2: the 0.1 A energy grid, nodes/s, roofline from the executed
+pair evaluations, CPU baseline), then finite loading and flexible chains;
+--headline-only leaves those out.
 
---workload c3: batched screening of 256 synthetic frameworks (100-8000
+--workload c3: batched screening of 256 synthetic frameworks (637-3954
 atoms, methane or CO2, 10^4-10^6 insertions each) through the screening
 service; value = insertions/s over the whole screen, with p50/p99 of the
 per-framework completion time.  --workload c4: Poisson arrivals of
 requests (methane, CO2 or N2-like, any of the 256 frameworks) at 0.7x the
-measured screening capacity; p50/p99 request latency, and the same
-requests run one at a time for comparison.
+measured screening capacity, submitted from a second thread while the
+request loop runs; p50/p99 request latency, and the same requests run one
+at a time for comparison.  --workload c2: the grid summary with the
+Ewald-Coulomb grid and the reference's grid decks.
 """
 import argparse
 import json
@@ -38,6 +46,12 @@ FLOP_LJ_PAIR = 29      # minimum image + r^2 + cutoff test (21) + LJ (8)
 FLOP_CC_EXTRA = 9      # real-space erfc Coulomb term on top of a pair
 FLOP_CC_PAIR = 30      # Coulomb-only pair: 21 + 9
 SEED = 20241203
+FLOP_GRID_PAIR = 69    # grid node per (node, atom) pair: pair 29 + the derivative channels 40
+
+
+def host_cores():
+    """Host cores this process may run on (all of them serve the CPU baseline)."""
+    return max(1, len(os.sched_getaffinity(0)))
 
 
 def _cpu_worker(args):
@@ -63,7 +77,7 @@ def cpu_baseline(fw, probe, n_per_worker):
     arrays = fw.oracle_args() + (probe.sites, probe.body)
     dt1, rec1 = _cpu_worker((arrays, 0, n_per_worker))
     single = n_per_worker / dt1
-    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    cores = host_cores()
     ctx = mp.get_context("fork")
     t0 = time.perf_counter()
     with ctx.Pool(cores) as pool:
@@ -93,7 +107,7 @@ def cpu_baseline_screening(jobs, count):
     """Oracle on the host cores: `count` insertions of each of `jobs`, one
     process per core; rate = insertions / summed process time x cores."""
     import multiprocessing as mp
-    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    cores = host_cores()
     ctx = mp.get_context("fork")
     t0 = time.perf_counter()
     with ctx.Pool(cores) as pool:
@@ -114,84 +128,101 @@ def _grid_worker(args):
     return out, time.perf_counter() - t0
 
 
-def grid_bench(args):
-    """C2: energy grid of a methane-like probe over the triclinic 26 A, 80 deg
-    cell (1200 atoms) at 0.1 A, plain FP64 and mixed builds, checked against
-    the oracle's FP64 grid interpolation at seeded off-grid points."""
+def grid_cpu_reference(n_points, cores=None):
+    """The oracle's FP64 InterpolateVDWGrid over corner nodes computed on the
+    fly at n_points seeded off-grid points of the C2 cell (0.1 A grid), one
+    process per host core; run before this process touches the GPU (forked
+    workers).  Returns the points, values and the node rate."""
     import multiprocessing as mp
     import numpy as np
     import oracle as O
     from raspa2_amd import workloads as W
     fw = W.c2_framework()
-    spacing = 0.1
-    geom = O.grid_geometry(fw.unit_box, spacing)
+    geom = O.grid_geometry(fw.unit_box, 0.1)
     inv, _ = O.invert3x3(fw.unit_box)
     rng = np.random.Generator(np.random.PCG64(SEED))
-    frac = rng.uniform(0.0, 1.0, (args.grid_samples, 3))
-    pts = frac @ np.asarray(fw.unit_box).reshape(3, 3)
-    # the oracle's FP64 grid at the sample points (8 corner nodes each), on the
-    # host cores, before this process touches the GPU
-    cores = max(1, min(16, len(os.sched_getaffinity(0))))
-    n_cpu = len(pts) if not args.no_cpu else min(len(pts), 2000)
-    parts = np.array_split(np.arange(n_cpu), cores)
-    ctxp = mp.get_context("fork")
+    pts = rng.uniform(0.0, 1.0, (n_points, 3)) @ np.asarray(fw.unit_box).reshape(3, 3)
+    cores = cores or host_cores()
+    parts = np.array_split(np.arange(n_points), cores)
     t0 = time.perf_counter()
-    with ctxp.Pool(cores) as pool:
-        res = pool.map(_grid_worker, [(fw.oracle_args(), geom, fw.unit_box, inv, pts[ix])
-                                      for ix in parts])
-    cpu_wall = time.perf_counter() - t0
-    u_ref = np.concatenate([np.asarray(r[0]) for r in res])
-    cpu_nodes = 8 * n_cpu
+    with mp.get_context("fork").Pool(cores) as pool:
+        res = pool.map(_grid_worker, [(fw.oracle_args(), geom, fw.unit_box, inv, pts[ix]) for ix in parts])
+    wall = time.perf_counter() - t0
+    return {"points": pts, "u": np.concatenate([np.asarray(r[0]) for r in res]),
+            "cpu_baseline": {"value": round(8 * n_points / wall, 1), "unit": "grid nodes/s", "cores": cores,
+                             "kind": "port",
+                             "sample": f"{8 * n_points} nodes (the 8 corners of {n_points} seeded off-grid "
+                                       "points), oracle FP64 all-pairs, one process per core"}}
+
+
+def grid_measure(ref):
+    """C2 on the device: the 0.1 A energy grid of a methane-like probe over
+    the triclinic 26 A / 80 deg cell (1200 atoms), plain FP64 and mixed
+    builds (FP32 node storage), each checked against the oracle's values at
+    the reference points; the mixed build's roofline from the (node, atom or
+    image) evaluations it executed."""
+    import numpy as np
+    import oracle as O
     from raspa2_amd import rhg as R
+    from raspa2_amd import workloads as W
+    fw = W.c2_framework()
+    pts, u_ref = ref["points"], ref["u"]
+    e2k = O.units()["energy_to_kelvin"]
     out = {}
-    data = {}
     for name, prec in (("fp64", R.PREC_FP64), ("mixed", R.PREC_MIXED)):
         ctx = fw.context(precision=prec)
         t = ctx.add_pseudo_atom(148.0, 3.73, 0.0)
-        g = ctx.MakeGrid(t, fw.unit_box, 0.5)  # warm-up: cell and image lists, code objects
-        g.close()
+        ctx.MakeGrid(t, fw.unit_box, 0.5).close()  # warm-up: cell and image lists, code objects
         ctx.synchronize()
         ctx.profile(True)
         ctx.profile_reset()
         t0 = time.perf_counter()
-        g = ctx.MakeGrid(t, fw.unit_box, spacing)
+        g = ctx.MakeGrid(t, fw.unit_box, 0.1)
         ctx.synchronize()
         wall = time.perf_counter() - t0
         _, ms = ctx.profile_get("grid_build")
         ctx.profile(False)
         nodes = int(np.prod(np.asarray(g.geom[:3]) + 1))
         u = g.interpolate(pts)
-        ok = np.abs(u_ref) < 1e7
-        e2k = O.units()["energy_to_kelvin"]
-        d = np.abs(u[:n_cpu][ok] - u_ref[ok]) * e2k
+        capped_same = bool(np.array_equal(u >= 1e7, u_ref >= 1e7))
+        ok = u_ref < 1e7
+        d = np.abs(u[ok] - u_ref[ok]) * e2k
         rel = d / np.maximum(np.abs(u_ref[ok]) * e2k, 1.0)
-        out[name] = {"nodes_per_s": nodes / wall, "wall_s": wall, "kernel_ms": ms,
-                     "max_abs_err_K": float(d.max()), "max_rel_err": float(rel.max()),
-                     "samples_checked": int(ok.sum())}
-        data[name] = (nodes,)
+        peak = PEAK["mixed" if prec == R.PREC_MIXED else "fp64"]
+        ach = g.exec_pairs * FLOP_GRID_PAIR / (ms * 1e-3) / 1e12 if ms else 0.0
+        out[name] = {"nodes": nodes, "nodes_per_s": round(nodes / wall, 1), "wall_s": round(wall, 5),
+                     "kernel_ms": round(ms, 4), "executed_pairs": g.exec_pairs,
+                     "max_rel_err": float(rel.max()), "max_abs_err_K": float(d.max()),
+                     "capped_flags_identical": capped_same, "samples_checked": int(ok.sum()),
+                     "roofline": {"bound": "valu", "kernel": "grid_build", "achieved": round(ach, 3),
+                                  "peak": peak, "unit": "TFLOP/s", "frac": round(ach / peak, 4),
+                                  "note": f"{FLOP_GRID_PAIR} flops per executed (node, atom) evaluation "
+                                          "over the HIP-event time of the build launches"}}
         g.close()
         ctx.close()
-    mixed = out["mixed"]
-    coulomb = coulomb_grid_bench(args)
-    deck = grid_deck_bench(args)
-    line = {
-        "metric": "energy-grid nodes/s (MakeGrid with derivatives)", "value": round(mixed["nodes_per_s"], 1),
-        "unit": "grid nodes/s", "n_gpus": 1, "steps": 1, "warmup": 1,
-        "ms_per_step": round(mixed["wall_s"] * 1e3, 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32 pair terms, fp64 accumulation", "data": "synthetic",
-        "config": {"workload": "C2 energy grid: methane-like LJ probe, triclinic 26 A 80 deg cell, "
-                               "1200 atoms, 0.1 A spacing, 8 channels (value, 3 first, 3 mixed "
-                               "second, 1 third derivative), FP32 storage",
-                   "nodes": data["mixed"][0], "spacing_A": spacing, "parallelism": "single GPU"},
-        "plain_fp64": out["fp64"], "mixed": mixed, "coulomb_grid": coulomb, "grid_decks": deck,
-        "check": f"{n_cpu} seeded off-grid points: device interpolation of each device grid vs "
-                 "the oracle's FP64 grid nodes and Lekien-Marsden interpolation",
-        "roofline": None,
-        "cpu_baseline": {"value": round(cpu_nodes / cpu_wall, 1), "unit": "grid nodes/s",
-                         "cores": cores, "kind": "port",
-                         "sample": f"{cpu_nodes} nodes (the 8 corners of {n_cpu} sample points), "
-                                   "oracle FP64 all-pairs, one process per core"},
-    }
+    return {"workload": "C2 energy grid: methane-like LJ probe, triclinic 26 A 80 deg cell, 1200 atoms, "
+                        "0.1 A spacing (352 x 294 x 254 nodes), 8 channels, FP32 storage",
+            "value": out["mixed"]["nodes_per_s"], "unit": "grid nodes/s",
+            "plain_fp64": out["fp64"], "mixed": out["mixed"], "roofline": out["mixed"]["roofline"],
+            "check": f"{len(pts)} seeded off-grid points: device interpolation vs the oracle's "
+                     "InterpolateVDWGrid over corner nodes computed on the fly (FP64)",
+            "cpu_baseline": ref["cpu_baseline"]}
+
+
+def grid_bench(args):
+    """--workload c2: the grid summary plus the Ewald-Coulomb grid and the
+    reference's grid decks."""
+    ref = grid_cpu_reference(args.grid_samples if not args.no_cpu else 2000)
+    g = grid_measure(ref)
+    line = {"metric": "energy-grid nodes/s (MakeGrid with derivatives)", "value": g["value"],
+            "unit": "grid nodes/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+            "ms_per_step": round(g["mixed"]["wall_s"] * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp32 pair terms, fp64 accumulation", "data": "synthetic",
+            "config": {"workload": g["workload"], "nodes": g["mixed"]["nodes"], "spacing_A": 0.1,
+                       "parallelism": "single GPU"},
+            "plain_fp64": g["plain_fp64"], "mixed": g["mixed"], "roofline": g["roofline"],
+            "coulomb_grid": coulomb_grid_bench(args), "grid_decks": grid_deck_bench(args),
+            "check": g["check"], "cpu_baseline": g["cpu_baseline"]}
     print(json.dumps(line))
 
 
@@ -341,16 +372,40 @@ def grid_deck_bench(args):
             "w_check": "mean Widom weight over the same 200000 insertions, grids vs direct mixed sums"}
 
 
-def screening_bench(args):
-    """C3 (batched screening) and C4 (Poisson-arriving requests) through the
-    screening service; prints one JSON line."""
-    from raspa2_amd import rhg as R
+def screening_cpu(args, jobs):
+    """CPU baseline of the screening workloads (before the GPU is touched):
+    the oracle on the first screen_cpu_sample insertions of each of the
+    first 16 screening requests, one process per host core."""
+    if args.no_cpu:
+        return None
+    count = args.screen_cpu_sample
+    cpu = cpu_baseline_screening(jobs[:min(16, len(jobs))], count)
+    return {"value": round(cpu["aggregate"], 2), "unit": "insertions/s", "cores": cpu["cores"],
+            "kind": "port", "single_core": round(cpu["single"], 2),
+            "sample": f"{cpu['sample']} insertions: the first {count} of each of the first 16 screening "
+                      "requests (their own frameworks and probes), oracle FP64 C restatement, one process "
+                      "per core"}
+
+
+def _served_roofline(stats, wall):
+    """Executed-pair flops of the culled site kernels over the served wall
+    time (every kernel, set-up and host work included)."""
+    flops = (stats["exec_lj_pairs"] * FLOP_LJ_PAIR + stats["exec_ljcc_pairs"] * (FLOP_LJ_PAIR + FLOP_CC_EXTRA)
+             + stats["exec_cc_pairs"] * FLOP_CC_PAIR)
+    ach = flops / wall / 1e12
+    return {"bound": "valu", "kernel": "site_energy (end to end)", "achieved": round(ach, 3), "peak": PEAK["mixed"],
+            "unit": "TFLOP/s", "frac": round(ach / PEAK["mixed"], 4),
+            "note": "algorithmic flops of the pair evaluations the culled site kernels executed for the "
+                    "served requests, over the wall time of the whole served run (set-up, every kernel "
+                    "and host work included), against the packed FP32 vector peak"}
+
+
+def screening_measure(args, jobs, cpu_obj, which=("c3", "c4")):
+    """C3 (one request per framework, all present at t=0) and C4 (Poisson
+    arrivals at 0.7x the measured C3 request capacity, methane / CO2 /
+    N2-like) through the screening service, each against the same requests
+    run one at a time (unbatched plain GPU path)."""
     from raspa2_amd import screening as SC
-    jobs = SC.catalogue(args.frameworks)
-    cpu = None
-    if not args.no_cpu:
-        sample_jobs = jobs[:min(16, len(jobs))]
-        cpu = cpu_baseline_screening(sample_jobs, args.cpu_sample // 2)
     tuning = SC.load_tuning()
     reqs = SC.screening_requests(jobs)
     # bring the device up (not timed): a short screen of four frameworks
@@ -358,57 +413,57 @@ def screening_bench(args):
     SC.run(jobs, warm, slots=args.slots, tuning=tuning)
     c3 = SC.run(jobs, reqs, slots=args.slots, tuning=tuning)
     s3 = SC.summary(c3, reqs)
-    out = {"metric": METRIC, "unit": "insertions/s", "n_gpus": 1, "steps": 1, "warmup": 1,
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "fp32 pair terms, fp64 accumulation/Ewald", "data": "synthetic"}
-    cpu_obj = None
-    if cpu is not None:
-        cpu_obj = {"value": round(cpu["aggregate"], 2), "unit": "insertions/s",
-                   "cores": cpu["cores"], "kind": "port", "single_core": round(cpu["single"], 2),
-                   "sample": f"{cpu['sample']} insertions: the first {args.cpu_sample // 2} of "
-                             f"each of the first 16 screening requests, oracle FP64 C "
-                             f"restatement, one process per core"}
-    if args.workload == "c3":
+    out = {}
+    if "c3" in which:
         serial = SC.run(jobs, reqs, slots=1, tuning=tuning)
         s1 = SC.summary(serial, reqs)
-        out.update({
-            "value": round(s3["rate"], 1), "ms_per_step": round(s3["wall_s"] * 1e3, 3),
-            "config": {"workload": f"C3 batched screening: {len(jobs)} synthetic frameworks "
-                                   "(100-8000 atoms, charged, Ewald 1e-6), methane or CO2, "
-                                   "10^4-10^6 insertions each, mixed precision",
-                       "frameworks": len(jobs), "insertions": s3["insertions"],
-                       "slots": args.slots, "tuning": tuning, "parallelism": "single GPU"},
+        out["c3"] = {
+            "workload": f"C3 batched screening: {len(jobs)} synthetic frameworks (637-3954 atoms, "
+                        "charged, Ewald 1e-6), methane or CO2, 10^4-10^6 insertions each, mixed precision",
+            "value": round(s3["rate"], 1), "unit": "insertions/s", "wall_s": round(s3["wall_s"], 4),
+            "frameworks": len(jobs), "insertions": s3["insertions"], "slots": args.slots,
             "latency": {"p50_s": s3["p50_s"], "p99_s": s3["p99_s"],
-                        "what": "completion time of each framework's request, all submitted "
-                                "at t=0"},
-            "unbatched": {"value": round(s1["rate"], 1), "p50_s": s1["p50_s"],
-                          "p99_s": s1["p99_s"], "what": "same requests one at a time"},
-            "roofline": None, "cpu_baseline": cpu_obj, "server": c3["stats"],
-            "failed": s3["failed"]})
-    else:
+                        "what": "completion time of each framework's request, all submitted at t=0"},
+            "unbatched_plain_gpu": {"value": round(s1["rate"], 1), "p50_s": s1["p50_s"], "p99_s": s1["p99_s"],
+                                    "what": "same requests one at a time"},
+            "roofline": _served_roofline(c3["stats"], s3["wall_s"]),
+            "cpu_baseline": cpu_obj, "server": c3["stats"], "failed": s3["failed"], "tuning": tuning}
+    if "c4" in which:
         capacity = len(reqs) / s3["wall_s"]
         rate = 0.7 * capacity
         r4 = SC.poisson_requests(args.requests, len(jobs), rate)
-        c4 = SC.run(jobs, r4, slots=args.slots, tuning=tuning)
+        c4 = SC.run(jobs, r4, slots=args.slots, tuning=tuning, live=True)
         s4 = SC.summary(c4, r4)
         plain_reqs = [SC.Request(r.framework, r.probe, r.n_trials, r.seed, 0.0) for r in r4]
         plain = SC.run(jobs, plain_reqs, slots=1, tuning=tuning)
         sp = SC.summary(plain, plain_reqs)
-        out.update({
-            "value": round(s4["rate"], 1), "ms_per_step": round(s4["wall_s"] * 1e3, 3),
-            "config": {"workload": f"C4 serving: {len(r4)} Poisson requests at "
-                                   f"{rate:.2f}/s (0.7 x measured screening capacity "
-                                   f"{capacity:.2f}/s) over {len(jobs)} frameworks, methane / "
-                                   "CO2 / N2-like, 10^4-10^6 insertions each, mixed precision",
-                       "requests": len(r4), "arrival_rate_per_s": rate,
-                       "insertions": s4["insertions"], "slots": args.slots, "tuning": tuning,
-                       "parallelism": "single GPU"},
-            "latency": {"p50_s": s4["p50_s"], "p99_s": s4["p99_s"],
-                        "what": "arrival to completion of each request"},
-            "unbatched": {"value": round(sp["rate"], 1), "what": "same requests one at a time, "
-                                                                 "all present at the start"},
-            "roofline": None, "cpu_baseline": cpu_obj, "server": c4["stats"],
-            "failed": s4["failed"]})
+        out["c4"] = {
+            "workload": f"C4 serving: {len(r4)} Poisson requests at {rate:.2f}/s (0.7 x measured screening "
+                        f"capacity {capacity:.2f}/s) over {len(jobs)} frameworks, methane / CO2 / N2-like, "
+                        "10^4-10^6 insertions each, mixed precision, submitted live from a second thread",
+            "value": round(s4["rate"], 1), "unit": "insertions/s", "wall_s": round(s4["wall_s"], 4),
+            "requests": len(r4), "arrival_rate_per_s": rate, "insertions": s4["insertions"],
+            "slots": args.slots,
+            "latency": {"p50_s": s4["p50_s"], "p99_s": s4["p99_s"], "what": "arrival to completion of each request"},
+            "unbatched_plain_gpu": {"value": round(sp["rate"], 1),
+                                    "what": "same requests one at a time, all present at the start"},
+            "roofline": _served_roofline(c4["stats"], s4["wall_s"]),
+            "cpu_baseline": cpu_obj, "server": c4["stats"], "failed": s4["failed"]}
+    return out
+
+
+def screening_bench(args):
+    """--workload c3 / c4: one JSON line for the screening workload."""
+    from raspa2_amd import screening as SC
+    jobs = SC.catalogue(args.frameworks)
+    cpu_obj = screening_cpu(args, jobs)
+    r = screening_measure(args, jobs, cpu_obj, which=(args.workload,))[args.workload]
+    out = {"metric": METRIC, "value": r["value"], "unit": "insertions/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+           "ms_per_step": round(r["wall_s"] * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "fp32 pair terms, fp64 accumulation/Ewald", "data": "synthetic",
+           "config": {"workload": r["workload"], "parallelism": "single GPU"},
+           "latency": r["latency"], "unbatched": r["unbatched_plain_gpu"], "roofline": r["roofline"],
+           "cpu_baseline": r["cpu_baseline"], "detail": r}
     print(json.dumps(out))
 
 
@@ -637,16 +692,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--insertions", type=int, default=1_000_000, help="insertions per step")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
-    ap.add_argument("--cpu-sample", type=int, default=400, help="insertions per CPU process")
+    ap.add_argument("--cpu-sample", type=int, default=2000, help="C1 insertions per CPU process")
+    ap.add_argument("--screen-cpu-sample", type=int, default=2000,
+                    help="insertions of each of 16 screening requests in their CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "chain", "psd"], default="c1")
     ap.add_argument("--grid-samples", type=int, default=100_000, help="c2 off-grid checks")
+    ap.add_argument("--grid-check", type=int, default=10_000,
+                    help="off-grid checks of the grid summary in the default line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 512 MiB device copy kernel (k_stream_copy) three "
                          "times, to scale profiler byte counters")
     ap.add_argument("--headline-only", action="store_true",
-                    help="c1: leave out the finite-loading, flexible-chain and pore-size legs "
-                         "(profiles of the headline steps)")
+                    help="c1: leave out the screening, grid, finite-loading and flexible-chain "
+                         "legs (profiles of the headline steps)")
     ap.add_argument("--frameworks", type=int, default=256, help="c3/c4 catalogue size")
     ap.add_argument("--requests", type=int, default=384, help="c4 requests")
     ap.add_argument("--slots", type=int, default=8, help="c3/c4 requests in flight")
@@ -688,10 +747,16 @@ def main():
     probe = W.CO2
     n = args.insertions
 
-    # CPU baseline first: forked workers, before this process touches the GPU
+    # CPU baselines first: forked workers, before this process touches the GPU
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(fw, probe, args.cpu_sample)
+    extras = not args.headline_only
+    if extras:
+        from raspa2_amd import screening as SC
+        jobs = SC.catalogue(args.frameworks)
+        scr_cpu = screening_cpu(args, jobs)
+        grid_ref = grid_cpu_reference(args.grid_check)
 
     if args.copy_calibration:
         R.stream_copy(512 << 20, reps=3)
@@ -731,17 +796,22 @@ def main():
     peak = PEAK[args.precision]
 
     # relative error of this precision against the plain FP64 GPU path on the
-    # first timed step, and the plain path's own throughput
+    # first timed step, and the plain path's own throughput, timed like the
+    # headline (the same warm-up and steps)
     ref_ctx = fw.context(charge_method=R.CHARGE_EWALD, precision=R.PREC_FP64)
     _, mol64 = W.register_probe(ref_ctx, probe)
     b_test = R.WidomBlocks()
     b_ref = R.WidomBlocks()
     ctx.widom_run(mol, SEED, total, first=args.warmup * n, n=n, blocks=b_test)
+    for w in range(args.warmup):
+        ref_ctx.widom_run(mol64, SEED, total, first=w * n, n=n)
     ref_ctx.synchronize()
     t2 = time.perf_counter()
-    ref_ctx.widom_run(mol64, SEED, total, first=args.warmup * n, n=n, blocks=b_ref)
+    for s in range(args.steps):
+        ref_ctx.widom_run(mol64, SEED, total, first=(args.warmup + s) * n, n=n,
+                          blocks=b_ref if s == 0 else None)
     ref_ctx.synchronize()
-    plain_rate = n / (time.perf_counter() - t2)
+    plain_rate = args.steps * n / (time.perf_counter() - t2)
     rho = info["density"]
     e_t = R.widom_estimators(b_test, 300.0, rho)
     e_r = R.widom_estimators(b_ref, 300.0, rho)
@@ -823,10 +893,15 @@ def main():
             "executed": round((st["exec_lj_pairs"] + st["exec_ljcc_pairs"] + st["exec_cc_pairs"]) / elapsed, 1)},
         "kernels": kernels,
     }
-    if not args.headline_only:
+    if extras:
+        ctx.close()
+        ref_ctx.close()
+        # the other half of the metric: batched screening (C3) and serving
+        # (C4) with their p50/p99 latency, and the C2 energy grid
+        out["screening"] = screening_measure(args, jobs, scr_cpu)
+        out["grid"] = grid_measure(grid_ref)
         out["finite_loading"] = loaded_bench(fw, probe)
         out["flexible_chain"] = chain_bench(no_cpu=args.no_cpu)
-        out["pore_size_distribution"] = psd_bench(no_cpu=args.no_cpu)
     print(json.dumps(out))
 
 

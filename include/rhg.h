@@ -411,8 +411,10 @@ int rhg_grid_write(const rhg_grid *g, const char *path, const int32_t n_unit_cel
 int rhg_grid_read(rhg_ctx *ctx, const char *path, int32_t coulomb, const double unit_box[9],
                   const int32_t n_unit_cells[3], rhg_grid **out);
 /* spacing, unit-cell lengths[3], number of unit cells[3], coulomb flag,
- * Ewald precision of a Coulomb grid (-1 otherwise) */
-int rhg_grid_meta(const rhg_grid *g, double out[9]);
+ * Ewald precision of a Coulomb grid (-1 otherwise), (node, atom) pair
+ * evaluations the build executed (through the cell lists; all pairs when
+ * built without them; 0 for a grid read from a file) */
+int rhg_grid_meta(const rhg_grid *g, double out[10]);
 /* UseTabularGrid (src/input.c:5626-5630, 6190-6195): energies of pseudo-atom
  * `type` at a position, in rhg_framework_energy_at_positions and in the
  * Widom pipeline, interpolate its VDW grid (CalculateFrameworkVDWEnergyAtPosition's
@@ -454,6 +456,11 @@ int rhg_widom_stats(rhg_ctx *ctx, double out[12], int32_t reset);
  * minimum-image step (used when every perpendicular width is >= 2 cut-offs).
  * Default cell 2.5 A. */
 int rhg_set_culling(rhg_ctx *ctx, int32_t on, double cell_size);
+/* Launch shape of the culled site kernels (no counterpart in the reference;
+ * a tuning knob): sites per lane of the image-list kernels (1..4: tiles of
+ * 64 x that many sites of one cell; default 4) and waves per workgroup
+ * (1, 2 or 4; default 4).  Results do not depend on either. */
+int rhg_set_tiles(rhg_ctx *ctx, int32_t sites_per_lane, int32_t waves_per_block);
 /* Scratch buffers that contexts can borrow (no counterpart in the
  * reference): while a workspace is attached, the context's per-call device
  * buffers (Widom chunk buffers, binning scratch, Fourier partials) live in
@@ -470,13 +477,16 @@ int rhg_attach_workspace(rhg_ctx *ctx, rhg_workspace *ws);
  * (python/raspa2.py:97-110; src/monte_carlo.c:626-748 runs the systems of
  * a process one after another).  A server holds a catalogue of frameworks,
  * keeps the ones in use resident on the device (atom records, k-tables,
- * cell lists) within an HBM budget, evicting idle ones least-recently-used
- * first, and runs up to `slots` Widom requests at once, each on its own
- * stream, starting the next arrived request as soon as a slot frees
- * (continuous batching).  A request advances in slices of `slice`
- * insertions; each slice's block sums are collected on the host, so a
- * request's result is bitwise the same alone or beside others, and a slice
- * that fails is discarded and run again (at most 3 times). */
+ * cell lists and, for grid-served frameworks, energy grids) within an HBM
+ * budget, evicting idle ones least-recently-used first, and runs up to
+ * `slots` Widom requests at once, each on its slot's stream, starting the
+ * next arrived request as soon as a slot and the memory free up (continuous
+ * batching).  Requests can be submitted before the request loop starts or
+ * while it runs.  A request is enqueued whole, in chunks of at most `slice`
+ * insertions whose block sums accumulate on the device (chunks after the
+ * first replay one captured HIP graph), and waited for once; its result is
+ * bitwise the same alone or beside others, and a request that fails is
+ * discarded and run again (at most 3 times). */
 typedef struct rhg_server rhg_server;
 
 typedef struct {
@@ -492,39 +502,64 @@ typedef struct {
   uint64_t seed;
   int64_t n_insertions;
   int32_t steps_per_cycle;
-  double arrival_s;      /* arrival, seconds after rhg_server_run starts */
+  double arrival_s;      /* arrival, seconds after the request loop starts; a
+                            request submitted while it runs arrives no earlier
+                            than its submission */
 } rhg_request;
 
 typedef struct {
   int32_t status;        /* RHG_OK or the error that ended the request */
-  int32_t retries;       /* slices run again after a failure */
+  int32_t retries;       /* times the request was run again after a failure */
   rhg_widom_blocks blocks;
   double queued_s, started_s, finished_s, latency_s; /* latency = finished - arrival */
 } rhg_result;
 
 /* hbm_budget 0 = 80% of the free device memory; precision RHG_PREC_*;
- * slice = insertions per checkpointed slice. */
+ * slice = largest chunk (insertions per pipeline pass, <= 2^24). */
 int rhg_server_create(int device, size_t hbm_budget, int32_t slots, int32_t precision, int64_t slice,
                       rhg_server **out);
+/* Stops the request loop (finishing every submitted request) and frees all. */
 void rhg_server_destroy(rhg_server *s);
 const char *rhg_server_last_error(const rhg_server *s);
 int rhg_server_add_framework(rhg_server *s, int64_t id, const rhg_framework *fw, const rhg_cell *cell,
                              const rhg_forcefield *ff);
+/* Serves the requests of framework id from tabulated energy grids
+ * (UseTabularGrid, src/input.c:5626-5630; src/framework_energy.c:73-76,
+ * 197-200): a VDW grid per probe pseudo-atom and, for charged probes with
+ * Ewald, the Ewald-Coulomb grid, built on first use over the framework's box
+ * at `spacing` [A] (FP32 nodes, MakeGrid's layout), kept resident with the
+ * framework, counted in admission and evicted with it.  spacing = 0 returns
+ * to the direct sums. */
+int rhg_server_use_grids(rhg_server *s, int64_t id, double spacing);
 /* Per size bucket (frameworks of up to max_atoms[i] atoms): cell-list cell
- * size and chunk (insertions per pipeline pass), e.g. from an autotuner. */
+ * size, chunk, and the site kernels' sites per lane (1..4) and waves per
+ * workgroup (1, 2, 4) (nullable: 4 and 4), e.g. from an autotuner. */
 int rhg_server_set_tuning(rhg_server *s, int32_t n, const int32_t *max_atoms, const double *cell_size,
-                          const int64_t *chunk);
+                          const int64_t *chunk, const int32_t *sites_per_lane, const int32_t *waves_per_block);
+/* Thread-safe; may be called while the request loop runs. */
 int rhg_server_submit(rhg_server *s, const rhg_request *req, int64_t *id);
-/* Host-side fault hook of one request (as rhg_inject_fault, on its first slice). */
+/* Host-side fault hook of a request not yet started (as rhg_inject_fault). */
 int rhg_server_inject_fault(rhg_server *s, int64_t id, int64_t after_chunks);
-/* Runs every submitted request to completion (arrivals are honoured). */
+/* Starts the request loop and returns; requests are served as they arrive. */
+int rhg_server_start(rhg_server *s);
+/* Waits until request id has finished: 1 finished, 0 not within timeout_s
+ * (< 0: no limit), negative on error. */
+int rhg_server_wait(rhg_server *s, int64_t id, double timeout_s);
+/* Finishes every submitted request and stops the request loop. */
+int rhg_server_stop(rhg_server *s);
+/* rhg_server_start then rhg_server_stop: serves a trace of requests
+ * submitted beforehand (arrivals are honoured) and those submitted meanwhile. */
 int rhg_server_run(rhg_server *s);
-int rhg_server_result(const rhg_server *s, int64_t id, rhg_result *out);
+/* RHG_OK once the request has finished (RHG_ERR_INVALID before). */
+int rhg_server_result(rhg_server *s, int64_t id, rhg_result *out);
 /* out[0] contexts created, [1] evictions, [2] peak resident bytes, [3]
- * resident bytes, [4] slices retried, [5] budget, [6] idle wait [s],
- * [7] requests, host seconds in [8] context set-up, [9] submits, [10]
- * collects, [11] reserved */
-int rhg_server_stats(const rhg_server *s, double out[12]);
+ * resident bytes, [4] retries, [5] budget, [6] idle wait of the workers [s],
+ * [7] requests, host seconds in [8] set-up, [9] submits, [10] collects,
+ * [11] grids built, [12] grids evicted, [13] grid bytes resident, [14]
+ * requests finished, [15] insertions run, [16..18] pair evaluations the
+ * culled site kernels executed for them (LJ, LJ + Coulomb, Coulomb; as
+ * rhg_widom_stats), [19] reserved */
+int rhg_server_stats(rhg_server *s, double out[20]);
 /* Device bytes a context of this framework holds after Widom insertions of
  * a probe of n_sites sites (n_charged charged, n_types distinct
  * pseudo-atoms) in chunks of `chunk` at the given precision (host-only

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/rhg.h"
+#include "rhg_internal.h"
 #include "rhg_kernels.h"
 
 using namespace rhg;
@@ -116,6 +117,17 @@ struct Pending {
   hipEvent_t a, b;
 };
 
+// Device memory of a context's fixed tables: small buffers are carved out of
+// 2 MiB blocks, large ones get a block each, so that setting up a framework
+// costs a handful of hipMalloc calls (they contend across the threads of a
+// screening server) instead of one per table.  Carved buffers are released
+// with the context; large ones can be released early.
+struct Arena {
+  std::vector<std::pair<char *, size_t>> small; // carved blocks
+  std::vector<std::pair<char *, size_t>> large; // one buffer each
+  size_t used = 0;                              // bytes carved from the last small block
+};
+
 } // namespace
 
 struct rhg_workspace {
@@ -132,6 +144,8 @@ struct rhg_ctx {
   int device = 0;
   rhg_workspace *ws = nullptr; // borrowed Widom/scratch buffers (nullable)
   hipStream_t stream = nullptr;
+  bool own_stream = true;      // false: the stream is borrowed (screening server)
+  Arena arena;
   std::string err;
   Cell cell;
   double props[7];
@@ -159,6 +173,8 @@ struct rhg_ctx {
   // cell lists
   int cull = 1;
   double cell_size = 2.5;
+  // launch shape of the site kernels (rhg_set_tiles; 0: defaults)
+  int tile_sites = 0, waves = 0;
   bool grid_ready = false;
   CellGrid cg;
   int32_t *d_nb_off = nullptr, *d_nb_idx = nullptr;
@@ -226,6 +242,7 @@ struct rhg_grid {
   double spacing = 0.0;          // SpacingVDWGrid / SpacingCoulombGrid
   double ewald_precision = -1.0; // of a Coulomb grid
   int n_cells[3] = {1, 1, 1};    // NumberOfUnitCells of the system
+  double exec_pairs = 0.0;       // (node, atom or image) evaluations of the build
   void *d = nullptr;
   size_t bytes = 0;
 };
@@ -267,12 +284,51 @@ int fail(rhg_ctx *c, int code, const std::string &msg) {
       return fail((ctx), RHG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+constexpr size_t kArenaBlock = (size_t)2 << 20;
+
+// a table of the context, in its arena (256-byte aligned)
 int dalloc(rhg_ctx *c, void **p, size_t bytes) {
-  if (bytes == 0) bytes = 8;
-  hipError_t e = hipMalloc(p, bytes);
-  if (e != hipSuccess) return fail(c, RHG_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
-  c->footprint += bytes;
+  *p = nullptr;
+  bytes = (std::max<size_t>(bytes, 8) + 255) & ~(size_t)255;
+  Arena &A = c->arena;
+  if (bytes >= kArenaBlock / 2) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(c, RHG_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    A.large.push_back({(char *)*p, bytes});
+    c->footprint += bytes;
+    return RHG_OK;
+  }
+  if (A.small.empty() || A.used + bytes > A.small.back().second) {
+    void *blk = nullptr;
+    hipError_t e = hipMalloc(&blk, kArenaBlock);
+    if (e != hipSuccess) return fail(c, RHG_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    A.small.push_back({(char *)blk, kArenaBlock});
+    A.used = 0;
+    c->footprint += kArenaBlock;
+  }
+  *p = A.small.back().first + A.used;
+  A.used += bytes;
   return RHG_OK;
+}
+
+// releases a table of the context: large buffers at once, carved ones with
+// the context
+void dfree(rhg_ctx *c, void *p) {
+  if (!p) return;
+  Arena &A = c->arena;
+  for (size_t i = 0; i < A.large.size(); ++i)
+    if (A.large[i].first == (char *)p) {
+      hipFree(p);
+      c->footprint -= A.large[i].second;
+      A.large.erase(A.large.begin() + (long)i);
+      return;
+    }
+}
+
+void arena_release(rhg_ctx *c) {
+  for (auto &b : c->arena.small) hipFree(b.first);
+  for (auto &b : c->arena.large) hipFree(b.first);
+  c->arena = Arena();
 }
 
 // grow-only named work buffer, in the attached workspace if there is one
@@ -458,21 +514,29 @@ int setup_pseudo_atom(rhg_ctx *c, PseudoAtom &t) {
     const double s6 = r.s2 * r.s2 * r.s2;
     hpar[i] = make_float4((float)(r.c4 * s6 * s6), (float)(r.c4 * s6), (float)r.shift, (float)r.q);
   }
-  size_t n = h64.size();
-  int rc;
-  if ((rc = dalloc(c, (void **)&t.d64, n * sizeof(AtomRec64)))) return rc;
-  if ((rc = dalloc(c, (void **)&t.dmx, n * sizeof(AtomRecMx)))) return rc;
-  if ((rc = dalloc(c, (void **)&t.c24, n * sizeof(double)))) return rc;
-  if ((rc = dalloc(c, (void **)&t.c96, n * sizeof(double)))) return rc;
-  if ((rc = dalloc(c, (void **)&t.c384, n * sizeof(double)))) return rc;
-  if ((rc = dalloc(c, (void **)&t.par, n * sizeof(float4)))) return rc;
-  if ((rc = upload(c, t.par, hpar.data(), n * sizeof(float4)))) return rc;
-  if ((rc = upload(c, t.d64, h64.data(), n * sizeof(AtomRec64)))) return rc;
-  if ((rc = upload(c, t.dmx, hmx.data(), n * sizeof(AtomRecMx)))) return rc;
-  if ((rc = upload(c, t.c24, h24.data(), n * sizeof(double)))) return rc;
-  if ((rc = upload(c, t.c96, h96.data(), n * sizeof(double)))) return rc;
-  if ((rc = upload(c, t.c384, h384.data(), n * sizeof(double)))) return rc;
-  HIPC(c, hipStreamSynchronize(c->stream));
+  // the six record arrays of this type in one table and one copy
+  const size_t n = h64.size();
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t o64 = 0, omx = o64 + up(n * sizeof(AtomRec64)), o24 = omx + up(n * sizeof(AtomRecMx)),
+               o96 = o24 + up(n * sizeof(double)), o384 = o96 + up(n * sizeof(double)),
+               opar = o384 + up(n * sizeof(double)), total = opar + up(n * sizeof(float4));
+  std::vector<char> host(total, 0);
+  memcpy(host.data() + o64, h64.data(), n * sizeof(AtomRec64));
+  memcpy(host.data() + omx, hmx.data(), n * sizeof(AtomRecMx));
+  memcpy(host.data() + o24, h24.data(), n * sizeof(double));
+  memcpy(host.data() + o96, h96.data(), n * sizeof(double));
+  memcpy(host.data() + o384, h384.data(), n * sizeof(double));
+  memcpy(host.data() + opar, hpar.data(), n * sizeof(float4));
+  char *d = nullptr;
+  if (int rc = dalloc(c, (void **)&d, total)) return rc;
+  t.d64 = (AtomRec64 *)(d + o64);
+  t.dmx = (AtomRecMx *)(d + omx);
+  t.c24 = (double *)(d + o24);
+  t.c96 = (double *)(d + o96);
+  t.c384 = (double *)(d + o384);
+  t.par = (float4 *)(d + opar);
+  if (int rc = upload(c, d, host.data(), total)) return rc;
+  HIPC(c, hipStreamSynchronize(c->stream)); // host is released on return
   return RHG_OK;
 }
 
@@ -516,7 +580,7 @@ int ads_records(rhg_ctx *c, PseudoAtom &t) {
     r.pad = 0.0;
   }
   if (t.dads_cap < h.size()) {
-    hipFree(t.dads);
+    dfree(c, t.dads);
     t.dads = nullptr;
     t.dads_cap = 0;
     if (int rc = dalloc(c, (void **)&t.dads, h.size() * sizeof(AtomRec64))) return rc;
@@ -617,7 +681,7 @@ int ensure_cell_grid(rhg_ctx *c) {
     }
   }
   int32_t *counts = (int32_t *)work(c, "nb_counts", (size_t)ncell * sizeof(int32_t));
-  hipFree(c->d_nb_off);
+  dfree(c, c->d_nb_off);
   c->d_nb_off = nullptr;
   if ((rc_ = dalloc(c, (void **)&c->d_nb_off, (size_t)(ncell + 1) * sizeof(int32_t)))) return rc_;
   if (!counts) return fail(c, RHG_ERR_OOM, "cell counts");
@@ -627,7 +691,7 @@ int ensure_cell_grid(rhg_ctx *c) {
   int32_t total = 0;
   HIPC(c, hipMemcpyAsync(&total, c->d_nb_off + ncell, sizeof(int32_t), hipMemcpyDeviceToHost, st));
   HIPC(c, hipStreamSynchronize(st));
-  hipFree(c->d_nb_idx);
+  dfree(c, c->d_nb_idx);
   c->d_nb_idx = nullptr;
   if ((rc_ = dalloc(c, (void **)&c->d_nb_idx, (size_t)std::max(total, 1) * sizeof(int32_t)))) return rc_;
   launch_cell_neighbours(c->cell, nc[0], nc[1], nc[2], reach, c->d_ax, c->d_ay, c->d_az, c->n,
@@ -667,7 +731,7 @@ int ensure_image_list(rhg_ctx *c) {
   const int ncell = c->cg.ncell;
   int32_t *counts = (int32_t *)work(c, "img_counts", (size_t)ncell * sizeof(int32_t));
   if (!counts) return fail(c, RHG_ERR_OOM, "image counts");
-  hipFree(c->d_img_off);
+  dfree(c, c->d_img_off);
   c->d_img_off = nullptr;
   int rc_;
   if ((rc_ = dalloc(c, (void **)&c->d_img_off, (size_t)(ncell + 1) * sizeof(int32_t)))) return rc_;
@@ -677,7 +741,7 @@ int ensure_image_list(rhg_ctx *c) {
   int32_t total = 0;
   HIPC(c, hipMemcpyAsync(&total, c->d_img_off + ncell, sizeof(int32_t), hipMemcpyDeviceToHost, st));
   HIPC(c, hipStreamSynchronize(st));
-  hipFree(c->d_img);
+  dfree(c, c->d_img);
   c->d_img = nullptr;
   if ((rc_ = dalloc(c, (void **)&c->d_img, (size_t)std::max(total, 1) * sizeof(float4)))) return rc_;
   launch_image_list(c->cell, c->cg, c->cell_reach, c->props + 3, c->d_ax, c->d_ay, c->d_az, counts,
@@ -739,6 +803,8 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
   }
   if (given) {
     BinWork b = *given;
+    b.tile_sites = c->tile_sites;
+    b.waves = c->waves;
     if (!(a.precision != RHG_PREC_FP64 && g.img_off && a.par)) b.fix = nullptr;
     {
       Timed tm(c, "bin");
@@ -751,6 +817,8 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
   const int ncell = c->cg.ncell;
   std::string t = tag;
   BinWork b;
+  b.tile_sites = c->tile_sites;
+  b.waves = c->waves;
   b.cell_of = (int32_t *)work(c, (t + "_cellof").c_str(), (size_t)s.n * sizeof(int32_t));
   b.perm = (int32_t *)work(c, (t + "_perm").c_str(), (size_t)s.n * sizeof(int32_t));
   b.count = (int32_t *)work(c, (t + "_count").c_str(), (size_t)ncell * sizeof(int32_t));
@@ -825,8 +893,17 @@ const char *rhg_version(void) { return "rhg 0.1 (gfx950)"; }
 
 const char *rhg_last_error(const rhg_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
-int rhg_create(const rhg_framework *fw, const rhg_cell *cellp, const rhg_forcefield *ff, int device,
-               rhg_ctx **out) {
+} // extern "C"
+
+namespace {
+struct CtxDeleter {
+  void operator()(rhg_ctx *c) const { rhg_destroy(c); }
+};
+
+// rhg_create; with `stream` given the context borrows it instead of creating
+// its own, and with `ws` given its set-up scratch lives in that workspace
+int create_impl(const rhg_framework *fw, const rhg_cell *cellp, const rhg_forcefield *ff, int device,
+                hipStream_t stream, rhg_workspace *ws, rhg_ctx **out) {
   if (!fw || !cellp || !ff || !out) return RHG_ERR_INVALID;
   *out = nullptr;
   if (fw->n_atoms < 0 || (fw->n_atoms > 0 && (!fw->x || !fw->y || !fw->z || !fw->eps_k || !fw->sigma)))
@@ -834,10 +911,23 @@ int rhg_create(const rhg_framework *fw, const rhg_cell *cellp, const rhg_forcefi
   if (cellp->boundary != RHG_BC_RECTANGULAR && cellp->boundary != RHG_BC_TRICLINIC) return RHG_ERR_INVALID;
   if (ff->cutoff_vdw <= 0.0 || ff->temperature <= 0.0) return RHG_ERR_INVALID;
   if (ff->charge_method != RHG_CHARGE_NONE && ff->charge_method != RHG_CHARGE_EWALD) return RHG_ERR_UNSUPPORTED;
-  std::unique_ptr<rhg_ctx> c(new rhg_ctx());
-  c->device = device;
+  if (ws && ws->device != device) return RHG_ERR_INVALID;
   if (hipSetDevice(device) != hipSuccess) return RHG_ERR_HIP;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return RHG_ERR_HIP;
+  hipStream_t st = stream;
+  if (!st && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return RHG_ERR_HIP;
+  std::unique_ptr<rhg_ctx, CtxDeleter> c(new rhg_ctx());
+  c->device = device;
+  c->stream = st;
+  c->own_stream = stream == nullptr;
+  if (ws) {
+    // the workspace's previous user finishes first (set-up runs on this stream)
+    if (ws->owner && ws->owner != c.get()) {
+      if (hipStreamSynchronize(ws->owner->stream) != hipSuccess) return RHG_ERR_HIP;
+      ws->owner->ws = nullptr;
+    }
+    ws->owner = c.get();
+    c->ws = ws;
+  }
   c->u = make_units();
   c->cell = make_cell(cellp->box, cellp->boundary);
   cell_properties(c->cell.box, c->props);
@@ -893,49 +983,48 @@ int rhg_create(const rhg_framework *fw, const rhg_cell *cellp, const rhg_forcefi
   *out = c.release();
   return RHG_OK;
 }
+} // namespace
+
+int rhg_create_on_stream(const rhg_framework *fw, const rhg_cell *cell, const rhg_forcefield *ff, int device,
+                         hipStream_t stream, rhg_workspace *ws, rhg_ctx **out) {
+  return create_impl(fw, cell, ff, device, stream, ws, out);
+}
+
+int rhg_use_stream(rhg_ctx *c, hipStream_t stream) {
+  if (!c) return RHG_ERR_INVALID;
+  if (c->stream == stream) return RHG_OK;
+  hipSetDevice(c->device);
+  if (c->stream) HIPC(c, hipStreamSynchronize(c->stream));
+  if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
+  c->stream = stream;
+  c->own_stream = false;
+  return RHG_OK;
+}
+
+extern "C" {
+
+int rhg_create(const rhg_framework *fw, const rhg_cell *cellp, const rhg_forcefield *ff, int device,
+               rhg_ctx **out) {
+  return create_impl(fw, cellp, ff, device, nullptr, nullptr, out);
+}
 
 void rhg_destroy(rhg_ctx *c) {
   if (!c) return;
   hipSetDevice(c->device);
-  hipStreamSynchronize(c->stream);
+  if (c->stream) hipStreamSynchronize(c->stream);
   if (c->ws && c->ws->owner == c) c->ws->owner = nullptr;
-  for (auto &t : c->types) {
-    hipFree(t.d64);
-    hipFree(t.dmx);
-    hipFree(t.c24);
-    hipFree(t.c96);
-    hipFree(t.c384);
-    hipFree(t.par);
-    hipFree(t.dads);
-  }
-  hipFree(c->d_ads_mol);
-  hipFree(c->d_sads);
   for (auto &b : c->bufs) {
     hipFree(b.second.p);
     for (void *r : b.second.retired) hipFree(r);
   }
-  hipFree(c->d_kfactor);
-  hipFree(c->d_sf);
-  hipFree(c->d_nb_off);
-  hipFree(c->d_nb_idx);
-  hipFree(c->d_img_off);
-  hipFree(c->d_img);
-  hipFree(c->d_ax);
-  hipFree(c->d_ay);
-  hipFree(c->d_az);
-  hipFree(c->d_row_off);
-  hipFree(c->d_row_kz);
-  hipFree(c->d_acc);
-  hipFree(c->d_pockets);
-  hipFree(c->d_ctl);
-  hipFree(c->d_neff);
+  arena_release(c); // every table allocated with dalloc
   for (auto &kv : c->graph_cache) hipGraphExecDestroy(kv.second);
   for (auto &p : c->pending) {
     hipEventDestroy(p.a);
     hipEventDestroy(p.b);
   }
   for (auto e : c->event_pool) hipEventDestroy(e);
-  hipStreamDestroy(c->stream);
+  if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
 
@@ -1448,13 +1537,13 @@ int widom_enqueue(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, uint64_t
   int64_t jb[6];
   block_bounds(n_total, spc, jb);
   launch_set_ctl(c->d_ctl, first, first + n, jb, st);
-  note_ideal_gas(c, jb, first, n);
   const int64_t n_chunks = (n + cap - 1) / cap;
   const bool use_graph = c->graphs && !c->prof && !rec && n_chunks > 1 && c->fault_after < 0;
   for (int64_t k = 0; k < n_chunks; ++k) {
     const int64_t off = k * cap;
     const int64_t cnt = std::min(cap, n - off);
     if (int rc = check_fault(c)) return rc;
+    note_ideal_gas(c, jb, first + off, cnt);
     if (use_graph && k > 0) {
       // one captured chunk, keyed by everything its launches hold
       std::string key((const char *)&m, sizeof(m));
@@ -1978,10 +2067,10 @@ int rhg_widom_chain_run(rhg_ctx *c, const rhg_chain *ch, uint64_t seed, int64_t 
   int64_t jb[6];
   block_bounds(n_total, steps_per_cycle, jb);
   launch_set_ctl(c->d_ctl, first, first + n, jb, st);
-  note_ideal_gas(c, jb, first, n);
   for (int64_t off = 0; off < n; off += cap) {
     const int64_t cnt = std::min(cap, n - off);
     if (int rc = check_fault(c)) return rc;
+    note_ideal_gas(c, jb, first + off, cnt);
     if (d_xyz) fill(d_xyz, cnt * ch->n_atoms * 3, 0.0, st);
     if (int rc = chain_chunk(c, ch, steps, d_steps, w, seed, first + off, cnt, d_rec, d_xyz)) return rc;
     if (rec)
@@ -2253,7 +2342,7 @@ int rhg_set_adsorbates(rhg_ctx *c, int32_t n_mol, const int32_t *start, const in
     for (int m = 0; m < n_mol; ++m)
       for (int a = start[m]; a < start[m + 1]; ++a) mol[a] = m;
     if (c->ads_mol_cap < (size_t)n) {
-      hipFree(c->d_ads_mol);
+      dfree(c, c->d_ads_mol);
       c->d_ads_mol = nullptr;
       c->ads_mol_cap = 0;
       if (int rc = dalloc(c, (void **)&c->d_ads_mol, (size_t)n * sizeof(int32_t))) return rc;
@@ -2526,6 +2615,9 @@ int build_nodes(rhg_ctx *c, rhg_grid *g, GridArgs &a, const float4 *par) {
       if (int rc = ensure_image_list(c)) return rc;
   }
   const int prec = (!a.coulomb && c->precision != RHG_PREC_FP64 && c->use_images && c->cg.img_off) ? 1 : 0;
+  unsigned long long *pairs = culled ? (unsigned long long *)work(c, "grid_pairs", 64) : nullptr;
+  if (culled && !pairs) return fail(c, RHG_ERR_OOM, "grid pair counter");
+  if (pairs) fill((double *)pairs, 2, 0.0, c->stream); // all-zero bits
   for (int64_t n0 = 0; n0 < nodes; n0 += step) {
     a.node0 = n0;
     a.n_nodes = std::min(step, nodes - n0);
@@ -2548,7 +2640,7 @@ int build_nodes(rhg_ctx *c, rhg_grid *g, GridArgs &a, const float4 *par) {
     b.fill = (int32_t *)work(c, "g_fill", (size_t)ncell * sizeof(int32_t));
     b.start = (int32_t *)work(c, "g_start", (size_t)(ncell + 1) * sizeof(int32_t));
     b.tile_off = (int32_t *)work(c, "g_tiles", (size_t)(ncell + 1) * sizeof(int32_t));
-    b.pairs = nullptr;
+    b.pairs = pairs;
     b.fix = nullptr;
     if (!b.cell_of || !b.perm || !b.count || !b.fill || !b.start || !b.tile_off)
       return fail(c, RHG_ERR_OOM, "grid binning scratch");
@@ -2557,7 +2649,10 @@ int build_nodes(rhg_ctx *c, rhg_grid *g, GridArgs &a, const float4 *par) {
     launch_grid_culled(a, c->cg, sl, b, par, prec, c->stream);
   }
   HIPC(c, hipGetLastError());
+  unsigned long long cnt[2] = {0, 0};
+  if (pairs) HIPC(c, hipMemcpyAsync(cnt, pairs, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
+  g->exec_pairs = pairs ? (double)(cnt[0] + cnt[1]) : (double)nodes * (double)c->n;
   return RHG_OK;
 }
 
@@ -2829,7 +2924,7 @@ int rhg_use_coulomb_grid(rhg_ctx *c, rhg_grid *g) {
   return RHG_OK;
 }
 
-int rhg_grid_meta(const rhg_grid *g, double out[9]) {
+int rhg_grid_meta(const rhg_grid *g, double out[10]) {
   if (!g || !out) return RHG_ERR_INVALID;
   out[0] = g->spacing;
   for (int d = 0; d < 3; ++d) {
@@ -2838,6 +2933,7 @@ int rhg_grid_meta(const rhg_grid *g, double out[9]) {
   }
   out[7] = g->coulomb;
   out[8] = g->ewald_precision;
+  out[9] = g->exec_pairs;
   return RHG_OK;
 }
 
@@ -2951,6 +3047,18 @@ int rhg_widom_stats(rhg_ctx *c, double out[12], int32_t reset) {
   return RHG_OK;
 }
 
+int rhg_set_tiles(rhg_ctx *c, int32_t sites_per_lane, int32_t waves_per_block) {
+  if (!c) return RHG_ERR_INVALID;
+  if (sites_per_lane < 1 || sites_per_lane > kImgLane)
+    return fail(c, RHG_ERR_INVALID, "sites per lane out of range [1, 4]");
+  if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4)
+    return fail(c, RHG_ERR_INVALID, "waves per block must be 1, 2 or 4");
+  c->tile_sites = 64 * sites_per_lane;
+  c->waves = waves_per_block;
+  c->gen++; // captured chunks hold the launch shapes
+  return RHG_OK;
+}
+
 int rhg_set_culling(rhg_ctx *c, int32_t on, double cell_size) {
   if (!c) return RHG_ERR_INVALID;
   if (on && !(cell_size > 0.0)) return fail(c, RHG_ERR_INVALID, "cell size must be > 0");
@@ -2971,7 +3079,7 @@ int rhg_set_block_pockets(rhg_ctx *c, int32_t n, const double *xyz, const double
   if (!c || n < 0 || (n > 0 && (!xyz || !radius || !unit_box))) return RHG_ERR_INVALID;
   hipSetDevice(c->device);
   HIPC(c, hipStreamSynchronize(c->stream));
-  hipFree(c->d_pockets);
+  dfree(c, c->d_pockets);
   c->d_pockets = nullptr;
   memset(&c->pockets, 0, sizeof(c->pockets));
   c->gen++; // captured chunks hold the pocket arguments
@@ -3022,7 +3130,7 @@ void rhg_workspace_destroy(rhg_workspace *ws) {
   if (!ws) return;
   hipSetDevice(ws->device);
   if (ws->owner) {
-    hipStreamSynchronize(ws->owner->stream);
+    if (ws->owner->stream) hipStreamSynchronize(ws->owner->stream);
     ws->owner->ws = nullptr;
   }
   for (auto &b : ws->bufs) {
@@ -3050,13 +3158,13 @@ int rhg_attach_workspace(rhg_ctx *c, rhg_workspace *ws) {
   }
   if (c->ws) {
     // the old workspace may still be in use by this context's stream
-    HIPC(c, hipStreamSynchronize(c->stream));
+    if (c->stream) HIPC(c, hipStreamSynchronize(c->stream));
     if (c->ws->owner == c) c->ws->owner = nullptr;
     c->ws = nullptr;
   }
   if (ws) {
     if (ws->owner && ws->owner != c) {
-      HIPC(c, hipStreamSynchronize(ws->owner->stream));
+      if (ws->owner->stream) HIPC(c, hipStreamSynchronize(ws->owner->stream));
       ws->owner->ws = nullptr;
     }
     ws->owner = c;

@@ -75,7 +75,15 @@ struct BinWork {
   // its cell centre in A as FP32 and the site index bit-cast into w
   float4 *fix;
   unsigned long long *pairs; // executed pair evaluations [LJ, LJ+Coulomb, Coulomb]
+  // launch shape of the site kernels: sites per tile of the image-list
+  // kernels (a multiple of 64 up to 64 kImgLane; 0: 64 kImgLane) and waves
+  // per workgroup (1, 2 or 4; 0: 4)
+  int32_t tile_sites = 0;
+  int32_t waves = 0;
 };
+
+__host__ __device__ inline int img_tile_sites(const BinWork &b) { return b.tile_sites ? b.tile_sites : 64 * kImgLane; }
+inline int site_waves(const BinWork &b) { return b.waves ? b.waves : 4; }
 
 void launch_cell_neighbours(const Cell &cell, int nc0, int nc1, int nc2, double reach,
                             const double *ax, const double *ay, const double *az, int n_atoms,

@@ -614,7 +614,7 @@ void launch_bin_sites(const Cell &cell, const CellGrid &g, const SiteList &s, co
   const unsigned nb = (unsigned)((s.n + 255) / 256);
   if (nb) hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(256), 0, st, cell, g, s, b);
   hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(1024), 0, st, b.count, g.ncell, b.start, b.tile_off,
-                     b.fix ? 64 * kImgLane : 64);
+                     b.fix ? img_tile_sites(b) : 64);
   if (nb) {
     if (cell.bc == 1)
       hipLaunchKernelGGL(k_bin_scatter<1>, dim3(nb), dim3(256), 0, st, cell, g, s, b);
@@ -829,8 +829,9 @@ __global__ __launch_bounds__(256) void k_site_energy_img(
     if (b.tile_off[mid] <= tile) lo = mid; else hi = mid - 1;
   }
   const int c = __builtin_amdgcn_readfirstlane(lo);
-  const int first = (tile - b.tile_off[c]) * (64 * kImgLane);
-  const int nl = min(64 * kImgLane, b.count[c] - first); // live sites of the tile
+  const int ts = img_tile_sites(b);
+  const int first = (tile - b.tile_off[c]) * ts;
+  const int nl = min(ts, b.count[c] - first); // live sites of the tile
   // sorted offsets from the cell centre (written by the binning)
   float4 so[kImgLane];
   bool live[kImgLane];
@@ -900,10 +901,11 @@ __global__ __launch_bounds__(256) void k_site_energy_img(
 template <int BCK, int VDW, int CC>
 static void img_launch(const EnergyArgs &a, const CellGrid &g, const SiteList &s, const BinWork &b,
                        const EnergyOut &o, hipStream_t st) {
-  // upper bound of tiles: sites/(64 kImgLane) + one partial tile per cell
-  const int64_t max_tiles = s.n / (64 * kImgLane) + g.ncell + 1;
-  const unsigned nb = (unsigned)((max_tiles + 3) / 4);
-  hipLaunchKernelGGL((k_site_energy_img<BCK, VDW, CC>), dim3(nb), dim3(256), 0, st, a.cell, a.a64,
+  // upper bound of tiles: sites/(tile size) + one partial tile per cell
+  const int64_t max_tiles = s.n / img_tile_sites(b) + g.ncell + 1;
+  const int w = site_waves(b);
+  const unsigned nb = (unsigned)((max_tiles + w - 1) / w);
+  hipLaunchKernelGGL((k_site_energy_img<BCK, VDW, CC>), dim3(nb), dim3(64 * w), 0, st, a.cell, a.a64,
                      a.par, a.rc2_vdw, a.rc2_cc, a.alpha, a.cqa, g, s, b, o.u_vdw, o.u_cc, o.overlap);
 }
 
@@ -912,8 +914,9 @@ static void culled_launch(const EnergyArgs &a, const CellGrid &g, const SiteList
                           const BinWork &b, const EnergyOut &o, hipStream_t st) {
   // upper bound of tiles: sites/64 + one partial tile per cell
   const int64_t max_tiles = s.n / 64 + g.ncell + 1;
-  const unsigned nb = (unsigned)((max_tiles + 3) / 4);
-  hipLaunchKernelGGL((k_site_energy_culled<PREC, BCK, VDW, CC>), dim3(nb), dim3(256), 0, st, a.cell,
+  const int w = site_waves(b);
+  const unsigned nb = (unsigned)((max_tiles + w - 1) / w);
+  hipLaunchKernelGGL((k_site_energy_culled<PREC, BCK, VDW, CC>), dim3(nb), dim3(64 * w), 0, st, a.cell,
                      a.a64, a.amx, a.rc2_vdw, a.rc2_cc, a.alpha, a.cqa, g, s, b, o.u_vdw, o.u_cc,
                      o.overlap);
 }
@@ -2313,6 +2316,13 @@ __global__ __launch_bounds__(256) void k_grid_culled(const GridArgs g, const Cel
     if (__any(near))
       if (near) node_channels_exact<BCK>(g, cg, c, px, py, pz, o);
     cap = o[0] > kOverlap;
+  }
+  if (lane == 0 && b.pairs) {
+    // executed (node, list entry) evaluations: image entries in the mixed
+    // form, neighbour atoms otherwise
+    const int nl = min(64, b.count[c] - (tile - b.tile_off[c]) * 64);
+    const int len = PREC == 1 ? cg.img_off[c + 1] - cg.img_off[c] : cg.nb_off[c + 1] - cg.nb_off[c];
+    atomicAdd(b.pairs + (PREC == 2 ? 1 : 0), (unsigned long long)nl * (unsigned long long)len);
   }
   if (live) store_node(g, g.node0 + site, o[0], o[1], o[2], o[3], o[4], o[5], o[6], o[7], cap);
 }

@@ -163,6 +163,7 @@ def _load():
     f("rhg_framework_charge_energy_at_position", _P, _P, _I32, _D, ctypes.POINTER(_D),
       ctypes.POINTER(_D))
     f("rhg_framework_energy_at_positions", _P, _I64, _P, _I32, _P, _P, _P)
+    f("rhg_set_tiles", _P, _I32, _I32)
     f("rhg_ewald_tables", _P, _P, _P, _P)
     f("rhg_ewald_fourier_adsorbate", _P, _I64, _I32, _P, _P, _P, _P)
     f("rhg_pore_size_distribution", _P, _D, _I64, _I64, _U64, _I32, _D, _P, _P)
@@ -270,13 +271,14 @@ class Grid:
         self.size, self.shift, self.delta = info[3:6], info[6:9], info[9:12]
         self.node_bytes = int(info[12])
         self.geom = info[:12].copy()
-        meta = np.zeros(9)
+        meta = np.zeros(10)
         lib.rhg_grid_meta(self.h, _ptr(meta))
         self.spacing = meta[0]
         self.unit_cell_size = meta[1:4].copy()
         self.n_unit_cells = meta[4:7].astype(int)
         self.coulomb = bool(meta[7])
         self.ewald_precision = meta[8]
+        self.exec_pairs = meta[9]
 
     def close(self):
         """Free the device grid now (also done when the object is collected);
@@ -679,6 +681,10 @@ class Context:
         the mixed-precision kernels."""
         mode = 0 if not on else (2 if images else 1)
         self._check(lib.rhg_set_culling(self.h, mode, cell_size), "set_culling")
+
+    def set_tiles(self, sites_per_lane=4, waves_per_block=4):
+        """Launch shape of the culled site kernels (results are unchanged)."""
+        self._check(lib.rhg_set_tiles(self.h, sites_per_lane, waves_per_block), "set_tiles")
 
     def widom_stats(self, reset=False):
         out = np.zeros(12)

@@ -18,12 +18,14 @@ TUNING_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.j
 
 
 def load_tuning(path=TUNING_PATH):
-    """[(max_atoms, cell_size, chunk)] rows of the tuning table, or []."""
+    """[(max_atoms, cell_size, chunk, sites_per_lane, waves_per_block)] rows
+    of the tuning table, or []."""
     if not os.path.exists(path):
         return []
     with open(path) as f:
         d = json.load(f)
-    return [(int(r["max_atoms"]), float(r["cell_size"]), int(r["chunk"])) for r in d["buckets"]]
+    return [(int(r["max_atoms"]), float(r["cell_size"]), int(r["chunk"]), int(r.get("sites_per_lane", 4)),
+             int(r.get("waves_per_block", 4))) for r in d["buckets"]]
 
 
 def catalogue(n):
@@ -62,20 +64,44 @@ def poisson_requests(n_req, n_fw, rate, seed=4242):
 
 
 def run(jobs, requests, slots=8, tuning=None, precision=R.PREC_MIXED, slice_=1 << 18,
-        hbm_budget=0):
+        hbm_budget=0, live=False):
     """Serves `requests` over the frameworks of `jobs`; returns wall time,
-    per-request results and the server counters."""
+    per-request results and the server counters.  live: the request loop
+    starts first and a second thread submits each request at its arrival
+    time (otherwise the requests are submitted beforehand with their arrival
+    times)."""
     srv = S.Server(slots=slots, precision=precision, slice_=slice_, hbm_budget=hbm_budget)
     if tuning:
         srv.set_tuning(tuning)
     used = sorted({r.framework for r in requests})
     for k in used:
         srv.add_framework(k, jobs[k].framework)
-    ids = [srv.submit(r.framework, r.probe, r.seed, r.n_trials, arrival_s=r.arrival_s)
-           for r in requests]
-    t0 = time.perf_counter()
-    srv.run()
-    wall = time.perf_counter() - t0
+    if live:
+        import threading
+        order = sorted(range(len(requests)), key=lambda k: requests[k].arrival_s)
+        ids = [None] * len(requests)
+        srv.start()
+        t0 = time.perf_counter()
+
+        def feed():
+            for k in order:
+                r = requests[k]
+                delay = r.arrival_s - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+                ids[k] = srv.submit(r.framework, r.probe, r.seed, r.n_trials)
+
+        th = threading.Thread(target=feed)
+        th.start()
+        th.join()
+        srv.stop()
+        wall = time.perf_counter() - t0
+    else:
+        ids = [srv.submit(r.framework, r.probe, r.seed, r.n_trials, arrival_s=r.arrival_s)
+               for r in requests]
+        t0 = time.perf_counter()
+        srv.run()
+        wall = time.perf_counter() - t0
     res = [srv.result(i) for i in ids]
     st = srv.stats()
     srv.close()

@@ -45,10 +45,14 @@ def _decl():
     f("rhg_server_last_error", _P, res=ctypes.c_char_p)
     f("rhg_server_add_framework", _P, _I64, ctypes.POINTER(R.Framework), ctypes.POINTER(R.CellT),
       ctypes.POINTER(R.ForceField))
-    f("rhg_server_set_tuning", _P, _I32, _P, _P, _P)
+    f("rhg_server_set_tuning", _P, _I32, _P, _P, _P, _P, _P)
+    f("rhg_server_use_grids", _P, _I64, _D)
     f("rhg_server_submit", _P, ctypes.POINTER(RequestT), ctypes.POINTER(_I64))
     f("rhg_server_inject_fault", _P, _I64, _I64)
     f("rhg_server_run", _P)
+    f("rhg_server_start", _P)
+    f("rhg_server_stop", _P)
+    f("rhg_server_wait", _P, _I64, _D)
     f("rhg_server_result", _P, _I64, ctypes.POINTER(ResultT))
     f("rhg_server_stats", _P, _P)
     f("rhg_estimate_footprint", ctypes.POINTER(R.Framework), ctypes.POINTER(R.CellT),
@@ -132,13 +136,20 @@ class Server:
         self.temperature[fid] = temperature
 
     def set_tuning(self, rows):
-        """rows: [(max_atoms, cell_size, chunk)]"""
+        """rows: [(max_atoms, cell_size, chunk[, sites_per_lane, waves_per_block])]"""
         ma = np.array([r[0] for r in rows], dtype=np.int32)
         cs = np.array([r[1] for r in rows], dtype=np.float64)
         ch = np.array([r[2] for r in rows], dtype=np.int64)
+        la = np.array([r[3] if len(r) > 3 else 4 for r in rows], dtype=np.int32)
+        wa = np.array([r[4] if len(r) > 4 else 4 for r in rows], dtype=np.int32)
         self._check(R.lib.rhg_server_set_tuning(self.h, len(rows), ma.ctypes.data_as(_P),
-                                                cs.ctypes.data_as(_P), ch.ctypes.data_as(_P)),
+                                                cs.ctypes.data_as(_P), ch.ctypes.data_as(_P),
+                                                la.ctypes.data_as(_P), wa.ctypes.data_as(_P)),
                     "set_tuning")
+
+    def use_grids(self, fid, spacing):
+        """Serve framework fid's requests from energy grids at `spacing` [A] (0: direct sums)."""
+        self._check(R.lib.rhg_server_use_grids(self.h, fid, spacing), "use_grids")
 
     def submit(self, fid, probe, seed, n_insertions, steps_per_cycle=20, arrival_s=0.0):
         r = RequestT(fid, probe_struct(probe), seed, n_insertions, steps_per_cycle, arrival_s)
@@ -150,7 +161,23 @@ class Server:
         self._check(R.lib.rhg_server_inject_fault(self.h, rid, after_chunks), "inject_fault")
 
     def run(self):
+        """Serves every submitted request (and those submitted meanwhile), then returns."""
         self._check(R.lib.rhg_server_run(self.h), "run")
+
+    def start(self):
+        """Starts the request loop; submit() may be called from any thread meanwhile."""
+        self._check(R.lib.rhg_server_start(self.h), "start")
+
+    def stop(self):
+        """Finishes every submitted request and stops the loop."""
+        self._check(R.lib.rhg_server_stop(self.h), "stop")
+
+    def wait(self, rid, timeout_s=-1.0):
+        """True once request rid has finished (False when timeout_s passed first)."""
+        rc = R.lib.rhg_server_wait(self.h, rid, timeout_s)
+        if rc < 0:
+            self._check(rc, "wait")
+        return rc == 1
 
     def result(self, rid):
         r = ResultT()
@@ -159,8 +186,10 @@ class Server:
                     started_s=r.started_s, finished_s=r.finished_s, latency_s=r.latency_s)
 
     def stats(self):
-        out = np.zeros(12)
+        out = np.zeros(20)
         self._check(R.lib.rhg_server_stats(self.h, out.ctypes.data_as(_P)), "stats")
         keys = ("created", "evicted", "peak_resident", "resident", "retries", "budget",
-                "idle_wait_s", "requests", "setup_s", "submit_s", "collect_s")
+                "idle_wait_s", "requests", "setup_s", "submit_s", "collect_s", "grids_built",
+                "grids_evicted", "grid_bytes", "finished", "insertions", "exec_lj_pairs",
+                "exec_ljcc_pairs", "exec_cc_pairs")
         return dict(zip(keys, out))

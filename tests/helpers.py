@@ -62,3 +62,40 @@ def irmof1_dubbeldam():
     TraPPE CO2) from the committed fixtures."""
     import os
     return W.irmof1_dubbeldam(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+
+
+class DeviceArray:
+    """A device allocation through the HIP runtime librhg links against (the
+    test's own memory; used to hand device pointers to the C ABI)."""
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        if cls._hip is None:
+            import ctypes
+            h = ctypes.CDLL("libamdhip64.so.7")
+            h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+            h.hipFree.argtypes = [ctypes.c_void_p]
+            h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+            h.hipDeviceSynchronize.argtypes = []
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, host):
+        import ctypes
+        self.host = np.ascontiguousarray(host)
+        self.ptr = ctypes.c_void_p()
+        assert self.hip().hipMalloc(ctypes.byref(self.ptr), max(self.host.nbytes, 8)) == 0
+        if self.host.nbytes:
+            assert self.hip().hipMemcpy(self.ptr, self.host.ctypes.data, self.host.nbytes, 1) == 0
+
+    def get(self):
+        out = np.empty_like(self.host)
+        if out.nbytes:
+            assert self.hip().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2) == 0
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.hip().hipFree(self.ptr)
+            self.ptr = None
