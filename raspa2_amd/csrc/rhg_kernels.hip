@@ -1368,6 +1368,19 @@ __global__ __launch_bounds__(256) void k_blocked_pockets(const PocketArgs p, con
   flags[i] = blocked_pocket(p, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]) ? 1 : 0;
 }
 
+// A slot of a compacted list for each lane that calls this: one atomic per
+// wave and consecutive slots for the calling lanes in lane order, so that
+// the rows a wave writes by slot are contiguous (coalesced stores).
+__device__ __forceinline__ int wave_slot(int32_t *counter) {
+  const unsigned long long m = __ballot(1);
+  const int lane = threadIdx.x & 63;
+  const int leader = __builtin_ctzll(m);
+  int base = 0;
+  if (lane == leader) base = atomicAdd(counter, (int)__popcll(m));
+  base = __shfl(base, leader);
+  return base + (int)__popcll(m & ((1ULL << lane) - 1ULL));
+}
+
 __global__ __launch_bounds__(256) void k_widom_fb_gen(const WidomArgs a, const WidomWork w) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t n = chunk_n(a);
@@ -1484,14 +1497,14 @@ __global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, cons
       w.ureal[i] = 0.0;
       return;
     }
-    int s2 = atomicAdd(w.n_active2, 1);
+    int s2 = wave_slot(w.n_active2);
     w.active2[s2] = (int32_t)i;
     w.cfx[s2] = fx;
     w.cfy[s2] = fy;
     w.cfz[s2] = fz;
     return;
   }
-  const int slot = atomicAdd(w.n_active, 1);
+  const int slot = wave_slot(w.n_active);
   w.active[slot] = (int32_t)i;
   const int ko = m.k_or;
   uint64_t c = (uint64_t)(3 * k + 1);
@@ -1637,7 +1650,7 @@ __global__ __launch_bounds__(256) void k_widom_or_select(const WidomArgs a, cons
   if (ads) w.uadsv[i] = w.uadsv[i] + ea;
   w.uvdw[i] = w.uvdw[i] + ev;
   w.W[i] = Wn;
-  const int s2 = atomicAdd(w.n_active2, 1);
+  const int s2 = wave_slot(w.n_active2);
   w.active2[s2] = (int32_t)i;
   w.cfx[m.start * cap + s2] = w.selx[i];
   w.cfy[m.start * cap + s2] = w.sely[i];
@@ -3130,7 +3143,7 @@ __global__ __launch_bounds__(256) void k_chain_fb_select(const ChainArgs a, cons
     }
     return;
   }
-  const int64_t slot = atomicAdd(w.n_active, 1);
+  const int64_t slot = wave_slot(w.n_active);
   w.active[slot] = (int32_t)i;
   w.alive[slot] = 1;
   w.sW[slot] = W1;

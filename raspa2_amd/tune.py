@@ -1,28 +1,33 @@
 """Autotuner of the screening service: for each framework-size bucket, the
-cell-list cell size and the chunk (insertions per pipeline pass) that give
-the highest Widom insertion rate on this GPU, written to a JSON table that
-the service loads (screening.load_tuning).
+cell-list cell size, the chunk (insertions per pipeline pass) and the launch
+shape of the culled site kernels (sites per lane of the image-list tiles,
+waves per workgroup) that give the highest Widom insertion rate on this GPU,
+written to a JSON table that the service loads (screening.load_tuning).
 
     python -m raspa2_amd.tune [--out raspa2_amd/tuning.json]
+
+The search is a coordinate descent per bucket: cell size first (at the
+default tiles and chunk), then tiles, then the chunk, each step keeping the
+best so far; every candidate is timed on the bucket's representative
+frameworks (one request each, methane or CO2 as the screening draws them).
 """
 import argparse
 import json
-import os
 import time
-
-import numpy as np
 
 from . import rhg as R
 from . import workloads as W
 from .screening import TUNING_PATH
 
 BUCKETS = (1000, 2000, 4000, 8000)
-CELLS = (2.0, 2.5, 3.0, 3.5, 4.0)
-CHUNKS = (1 << 16, 1 << 18)
+CELLS = (2.0, 2.5, 3.0, 3.5)
+TILES = ((4, 4), (4, 2), (2, 4), (2, 2), (1, 4))  # (sites per lane, waves per workgroup)
+CHUNKS = (1 << 16, 1 << 17, 1 << 18)
 
 
-def representatives(per_bucket=3, scan=256):
-    """Up to `per_bucket` screening frameworks in each size bucket."""
+def representatives(per_bucket=3, scan=4096):
+    """Up to `per_bucket` screening frameworks in each size bucket (the
+    first 256 hold none above 4000 atoms; the draw reaches about 6600)."""
     out = {b: [] for b in BUCKETS}
     for i in range(scan):
         j = W.c3_job(i)
@@ -34,10 +39,11 @@ def representatives(per_bucket=3, scan=256):
     return out
 
 
-def rate(job, cell, chunk, n):
+def rate(job, cell, chunk, tiles, n):
     ctx = job.framework.context(charge_method=R.CHARGE_EWALD, precision=R.PREC_MIXED)
     ctx.set_culling(True, cell)
     ctx.set_widom_options(chunk=chunk, graphs=True)
+    ctx.set_tiles(*tiles)
     types = [ctx.add_pseudo_atom(*s) for s in job.probe.sites]
     mol = R.molecule(types, job.probe.body, start=job.probe.start, k_fb=job.probe.k_fb,
                      k_or=job.probe.k_or)
@@ -57,17 +63,28 @@ def tune(n=1 << 18, per_bucket=3, log=print):
         jobs = reps[b]
         if not jobs:
             continue
-        best = None
-        for cell in CELLS:
-            for chunk in CHUNKS:
-                # time for the bucket's frameworks, one request each
-                t = sum(n / rate(j, cell, chunk, n) for j in jobs)
-                log(f"bucket <= {b} atoms: cell {cell} chunk {chunk}: {len(jobs) * n / t:.4g} ins/s")
-                if best is None or t < best[0]:
-                    best = (t, cell, chunk)
-        rows.append({"max_atoms": b, "cell_size": best[1], "chunk": best[2],
-                     "rate": len(jobs) * n / best[0],
-                     "frameworks": [j.index for j in jobs]})
+
+        def score(cell, chunk, tiles):
+            # time for the bucket's frameworks, one request each
+            t = sum(n / rate(j, cell, chunk, tiles, n) for j in jobs)
+            r = len(jobs) * n / t
+            log(f"bucket <= {b} atoms: cell {cell} chunk {chunk} tiles {tiles}: {r:.4g} ins/s")
+            return r
+
+        best = {"cell": 2.5, "chunk": 1 << 18, "tiles": (4, 4)}
+        best_rate = score(best["cell"], best["chunk"], best["tiles"])
+        for key, values in (("cell", CELLS), ("tiles", TILES), ("chunk", CHUNKS)):
+            for v in values:
+                if v == best[key]:
+                    continue
+                trial = dict(best, **{key: v})
+                r = score(trial["cell"], trial["chunk"], trial["tiles"])
+                if r > best_rate:
+                    best, best_rate = trial, r
+        rows.append({"max_atoms": b, "cell_size": best["cell"], "chunk": best["chunk"],
+                     "sites_per_lane": best["tiles"][0], "waves_per_block": best["tiles"][1],
+                     "rate": best_rate, "frameworks": [j.index for j in jobs],
+                     "atoms": [j.framework.n for j in jobs]})
     return rows
 
 
@@ -77,8 +94,8 @@ def main():
     ap.add_argument("--insertions", type=int, default=1 << 18)
     args = ap.parse_args()
     rows = tune(args.insertions)
-    doc = {"what": "cell-list cell size [A] and chunk per framework-size bucket, mixed "
-                   "precision, Ewald 1e-6, measured Widom insertion rate",
+    doc = {"what": "per framework-size bucket: cell-list cell size [A], chunk, sites per lane and waves per "
+                   "workgroup of the site kernels; mixed precision, Ewald 1e-6, measured Widom insertion rate",
            "command": "python -m raspa2_amd.tune", "insertions_per_measurement": args.insertions,
            "buckets": rows}
     with open(args.out, "w") as f:
