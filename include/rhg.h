@@ -155,6 +155,21 @@ int rhg_framework_charge_energy_at_position(rhg_ctx *ctx, const double pos[3], i
  * of src/cbmc.c:699-712.  Any output may be NULL. */
 int rhg_framework_energy_at_positions(rhg_ctx *ctx, int64_t n, const double *xyz, int32_t type,
                                       double *u_vdw, double *u_cc, uint8_t *overlap);
+/* No counterpart in the reference.  rhg_framework_energy_at_positions for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  It calls no
+ * hipMalloc/hipFree and no allocating library routine.  Positions and
+ * outputs are DEVICE pointers (d_xyz: n x 3 interleaved; outputs nullable),
+ * the results equal the allocating form's, and the work is only enqueued on
+ * the context's stream (rhg_synchronize waits for it).  Binning scratch is a
+ * fixed device array of the library (16 MiB, one per GPU, held until the
+ * process ends): n <= 262144 (RHG_ERR_INVALID beyond).  The context's cell
+ * lists must exist (RHG_ERR_INVALID otherwise): one call of the allocating
+ * form at set-up builds them.  Every call on one GPU shares that GPU's array,
+ * so calls on different contexts (streams) of one GPU that overlap in time
+ * race; calls on one context are ordered by its stream. */
+int rhg_framework_energy_at_positions_noalloc(rhg_ctx *ctx, int64_t n, const double *d_xyz, int32_t type,
+                                              double *d_u_vdw, double *d_u_cc, uint8_t *d_overlap);
 
 /* ---------------------------------------------------------------- Ewald */
 /* Ewald tables in RASPA2's k order (kx 0..kmax, ky -kmax..kmax,

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -885,6 +886,48 @@ void frac_2pi(const rhg_ctx *c, const double p[3], double f[3]) {
 } // namespace
 
 // ======================================================================
+// Binning scratch of rhg_framework_energy_at_positions_noalloc: a fixed
+// device array of the library, so that the entry point allocates nothing.
+// Layout (each part 256-byte aligned): site x, y, z (FP64), cell of each
+// site, sorted order, sorted FP32 offsets, then per cell the site counts,
+// fill counters, starts and tile offsets.  Each GPU holds its own copy; its
+// address comes from hipGetSymbolAddress once per device, at context
+// creation with that device current.
+// ======================================================================
+namespace {
+constexpr int64_t kNaSites = (int64_t)1 << 18;   // positions per call
+constexpr int64_t kNaCells = 64 * 64 * 64;       // the cell grid's largest split
+constexpr size_t na_round(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kNaXyz = na_round(kNaSites * sizeof(double));
+constexpr size_t kNaI32 = na_round(kNaSites * sizeof(int32_t));
+constexpr size_t kNaFix = na_round(kNaSites * sizeof(float4));
+constexpr size_t kNaCell = na_round((kNaCells + 1) * sizeof(int32_t));
+constexpr size_t kNaBytes = 3 * kNaXyz + 2 * kNaI32 + kNaFix + 4 * kNaCell;
+constexpr int kNaDevices = 64;
+} // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+namespace {
+std::mutex na_mu;
+unsigned char *na_ptr[kNaDevices] = {};   // device address of `scratch`, per device
+hipError_t na_err[kNaDevices] = {};
+
+// called with `device` current
+void noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kNaDevices) return;
+  std::lock_guard<std::mutex> lk(na_mu);
+  if (na_ptr[device] || na_err[device] != hipSuccess) return;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e == hipSuccess && p)
+    na_ptr[device] = (unsigned char *)p;
+  else
+    na_err[device] = e != hipSuccess ? e : hipErrorInvalidSymbol;
+}
+} // namespace
+
+// ======================================================================
 // C ABI
 // ======================================================================
 extern "C" {
@@ -913,6 +956,7 @@ int create_impl(const rhg_framework *fw, const rhg_cell *cellp, const rhg_forcef
   if (ff->charge_method != RHG_CHARGE_NONE && ff->charge_method != RHG_CHARGE_EWALD) return RHG_ERR_UNSUPPORTED;
   if (ws && ws->device != device) return RHG_ERR_INVALID;
   if (hipSetDevice(device) != hipSuccess) return RHG_ERR_HIP;
+  noalloc_scratch_init(device);
   hipStream_t st = stream;
   if (!st && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return RHG_ERR_HIP;
   std::unique_ptr<rhg_ctx, CtxDeleter> c(new rhg_ctx());
@@ -1108,6 +1152,63 @@ int rhg_framework_energy_at_positions(rhg_ctx *c, int64_t n, const double *xyz, 
   if (u_cc) HIPC(c, hipMemcpyAsync(u_cc, dc, b, hipMemcpyDeviceToHost, c->stream));
   if (overlap) HIPC(c, hipMemcpyAsync(overlap, dov, n, hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
+  return RHG_OK;
+}
+
+int rhg_framework_energy_at_positions_noalloc(rhg_ctx *c, int64_t n, const double *d_xyz, int32_t type,
+                                              double *d_u_vdw, double *d_u_cc, uint8_t *d_overlap) {
+  if (!c || n < 0 || (n > 0 && !d_xyz)) return RHG_ERR_INVALID;
+  if (int rc = check_type(c, type)) return rc;
+  if (n > kNaSites)
+    return fail(c, RHG_ERR_INVALID, "noalloc: at most 262144 positions per call");
+  if (n == 0) return RHG_OK;
+  if (c->device < 0 || c->device >= kNaDevices) return fail(c, RHG_ERR_INVALID, "noalloc: device ordinal >= 64");
+  unsigned char *base = na_ptr[c->device];
+  if (!base)
+    return fail(c, RHG_ERR_HIP, std::string("noalloc: hipGetSymbolAddress(scratch): ") +
+                                    hipGetErrorString(na_err[c->device]));
+  const bool culled = c->cull && c->n > 0;
+  if (culled && (!c->grid_ready || (c->precision != RHG_PREC_FP64 && c->use_images && !c->img_ready)))
+    return fail(c, RHG_ERR_INVALID,
+                "noalloc: the context's cell lists are not built yet (one call of "
+                "rhg_framework_energy_at_positions at set-up builds them)");
+  if (culled && c->cg.ncell > kNaCells) return fail(c, RHG_ERR_INVALID, "noalloc: cell grid larger than 64^3");
+  hipSetDevice(c->device);
+  // carve the array; the per-cell counters are cleared by the binning itself
+  unsigned char *p = base;
+  double *sx = (double *)p;
+  p += kNaXyz;
+  double *sy = (double *)p;
+  p += kNaXyz;
+  double *sz = (double *)p;
+  p += kNaXyz;
+  BinWork b;
+  b.cell_of = (int32_t *)p;
+  p += kNaI32;
+  b.perm = (int32_t *)p;
+  p += kNaI32;
+  b.fix = (float4 *)p;
+  p += kNaFix;
+  b.count = (int32_t *)p;
+  p += kNaCell;
+  b.fill = (int32_t *)p;
+  p += kNaCell;
+  b.start = (int32_t *)p;
+  p += kNaCell;
+  b.tile_off = (int32_t *)p;
+  b.pairs = nullptr; // no work counters on this path
+  // event timing creates events: off for the duration of the call
+  const bool prof = c->prof;
+  c->prof = false;
+  launch_deinterleave(d_xyz, n, sx, sy, sz, c->stream);
+  EnergyArgs a = energy_args(c, type, 1.0, 1, 1);
+  if (!a.do_cc && d_u_cc) fill(d_u_cc, n, 0.0, c->stream);
+  SiteList s{sx, sy, sz, n, nullptr, 1};
+  EnergyOut o{d_u_vdw, a.do_cc ? d_u_cc : nullptr, d_overlap};
+  int rc = site_energy(c, type, a, s, o, "e", culled ? &b : nullptr);
+  c->prof = prof;
+  if (rc) return rc;
+  HIPC(c, hipGetLastError());
   return RHG_OK;
 }
 

@@ -100,6 +100,8 @@ void launch_bin_sites(const Cell &cell, const CellGrid &g, const SiteList &s, co
                       hipStream_t st);
 void launch_site_energy_culled(const EnergyArgs &a, const CellGrid &g, const SiteList &s,
                                const BinWork &b, const EnergyOut &o, hipStream_t st);
+// n interleaved positions (x, y, z per site) into three SoA arrays
+void launch_deinterleave(const double *xyz, int64_t n, double *x, double *y, double *z, hipStream_t st);
 
 // ---------------------------------------------------------------- Ewald
 struct EwaldDev {

@@ -949,6 +949,21 @@ static void culled_mode(const EnergyArgs &a, const CellGrid &g, const SiteList &
   }
 }
 
+__global__ __launch_bounds__(256) void k_deinterleave(const double *__restrict__ xyz, int64_t n,
+                                                      double *__restrict__ x, double *__restrict__ y,
+                                                      double *__restrict__ z) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[i] = xyz[3 * i];
+  y[i] = xyz[3 * i + 1];
+  z[i] = xyz[3 * i + 2];
+}
+
+void launch_deinterleave(const double *xyz, int64_t n, double *x, double *y, double *z, hipStream_t st) {
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  if (nb) hipLaunchKernelGGL(k_deinterleave, dim3(nb), dim3(256), 0, st, xyz, n, x, y, z);
+}
+
 void launch_site_energy_culled(const EnergyArgs &a, const CellGrid &g, const SiteList &s,
                                const BinWork &b, const EnergyOut &o, hipStream_t st) {
   if (s.n <= 0) return;

@@ -164,6 +164,7 @@ def _load():
       ctypes.POINTER(_D))
     f("rhg_framework_energy_at_positions", _P, _I64, _P, _I32, _P, _P, _P)
     f("rhg_set_tiles", _P, _I32, _I32)
+    f("rhg_framework_energy_at_positions_noalloc", _P, _I64, _P, _I32, _P, _P, _P)
     f("rhg_ewald_tables", _P, _P, _P, _P)
     f("rhg_ewald_fourier_adsorbate", _P, _I64, _I32, _P, _P, _P, _P)
     f("rhg_pore_size_distribution", _P, _D, _I64, _I64, _U64, _I32, _D, _P, _P)
