@@ -281,6 +281,42 @@ void free_grids(rhg_server *s, FrameworkSpec &f) {
   f.grid_bytes = 0;
 }
 
+// Frees the grids of an idle framework that probe `p` does not use (VDW grids
+// of other pseudo-atoms, the Coulomb grid when no site is charged), detaching
+// them from the context first; returns the bytes freed.  Caller holds the lock.
+size_t drop_unused_grids(rhg_server *s, FrameworkSpec &f, const rhg_probe &p) {
+  const size_t gb = grid_bytes_of(f.cell, f.grid_spacing);
+  size_t freed = 0;
+  bool charged = false;
+  for (int j = 0; j < p.n_sites; ++j) charged = charged || std::fabs(p.charge[j]) > 1e-10;
+  for (auto it = f.vdw_grids.begin(); it != f.vdw_grids.end();) {
+    bool used = false;
+    for (int j = 0; j < p.n_sites; ++j) used = used || it->first == std::make_pair(p.eps_k[j], p.sigma[j]);
+    if (used) {
+      ++it;
+      continue;
+    }
+    if (f.ctx)
+      for (const auto &t : f.types)
+        if (std::get<0>(t.first) == it->first.first && std::get<1>(t.first) == it->first.second)
+          rhg_use_grid(f.ctx, t.second, nullptr);
+    rhg_grid_free(it->second);
+    it = f.vdw_grids.erase(it);
+    freed += gb;
+  }
+  if (f.coulomb_grid && !charged) {
+    if (f.ctx) rhg_use_coulomb_grid(f.ctx, nullptr);
+    rhg_grid_free(f.coulomb_grid);
+    f.coulomb_grid = nullptr;
+    freed += gb;
+  }
+  s->grids_evicted += (double)(freed / std::max(gb, (size_t)1));
+  f.grid_bytes -= std::min(f.grid_bytes, freed);
+  f.bytes -= std::min(f.bytes, freed);
+  s->resident -= std::min(s->resident, freed);
+  return freed;
+}
+
 // caller holds the lock; the framework is idle (its context has no stream)
 void evict(rhg_server *s, FrameworkSpec &f) {
   rhg_destroy(f.ctx); // grids are borrowed by the context: it goes first
@@ -304,7 +340,8 @@ void note_peak(rhg_server *s) { s->peak_resident = std::max(s->peak_resident, s-
 // lock): the framework context's own bytes (if it is not resident yet), the
 // grids it still lacks, plus the slot's scratch must fit the budget beside
 // the other contexts and workspaces; idle frameworks (contexts and grids) are
-// evicted least recently used first, then idle workspaces are released.
+// evicted least recently used first, then the framework's own grids that this
+// probe does not use, then idle workspaces are released.
 // *wait is set when the request would fit once running requests finish.
 int reserve(rhg_server *s, FrameworkSpec &f, const rhg_probe &p, int slot, bool *wait) {
   *wait = false;
@@ -325,8 +362,10 @@ int reserve(rhg_server *s, FrameworkSpec &f, const rhg_probe &p, int slot, bool 
     // a grid build bins up to 2^22 nodes at a time (32 B each) in the workspace
     scratch = std::max(scratch, ((size_t)1 << 22) * 32 + ((size_t)64 * 64 * 64) * 16);
   }
-  const size_t all = f.bytes + own + scratch; // f.bytes: what the framework holds now
-  if (all > s->budget) return sfail(s, RHG_ERR_NOT_ADMITTED, "request larger than the device budget");
+  // f.bytes: what the framework holds now; grids of other probes go first
+  if (f.bytes + own + scratch > s->budget) drop_unused_grids(s, f, p);
+  if (f.bytes + own + scratch > s->budget)
+    return sfail(s, RHG_ERR_NOT_ADMITTED, "request larger than the device budget");
   auto need = [&]() {
     size_t w = ws_total(s) - s->ws_bytes[slot] + std::max(s->ws_bytes[slot], scratch);
     return s->resident + own + w;
@@ -341,6 +380,7 @@ int reserve(rhg_server *s, FrameworkSpec &f, const rhg_probe &p, int slot, bool 
       evict(s, *lru);
       continue;
     }
+    if (drop_unused_grids(s, f, p) > 0) continue;
     // release the buffers of idle workspaces
     bool freed = false;
     for (size_t w = 0; w < s->ws.size(); ++w)

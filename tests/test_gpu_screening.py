@@ -211,7 +211,7 @@ def test_grid_served_framework_evicted_and_rebuilt(W, S, SC, R, catalogue):
     res = [srv.result(i) for i in ids]
     st = srv.stats()
     srv.close()
-    assert all(r["status"] == 0 for r in res)
+    assert all(r["status"] == 0 for r in res), [r["status"] for r in res]
     assert st["grids_built"] >= 4 and st["grids_evicted"] >= 3 and st["evicted"] >= 2
     for i, (k, p) in enumerate(zip(order, probes)):
         alone = _alone(S, SC, R, jobs, k, p, 500 + i, 20_000, slice_=1 << 14,
