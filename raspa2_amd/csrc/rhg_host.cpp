@@ -111,6 +111,11 @@ struct PseudoAtom {
   float4 *par = nullptr; // {4 eps sigma^12, 4 eps sigma^6, shift, q} for the image-list kernels
   AtomRec64 *dads = nullptr; // the adsorbate atoms as seen by this type (finite loading)
   size_t dads_cap = 0;
+  // per cell of the image lists: the sum of this type's LJ shifts over the
+  // cell's list, for image lists of version cell_shift_version
+  double *cell_shift = nullptr;
+  int cell_shift_cap = 0;
+  uint64_t cell_shift_version = 0;
 };
 
 struct Pending {
@@ -182,6 +187,7 @@ struct rhg_ctx {
   int32_t *d_img_off = nullptr;
   float4 *d_img = nullptr;
   bool img_ready = false;
+  uint64_t img_version = 0; // counts image-list builds
   int use_images = 1; // mixed precision walks the image lists when they exist
   double cell_reach = 0.0, cell_halfdiag = 0.0;
   double *d_ax = nullptr, *d_ay = nullptr, *d_az = nullptr;
@@ -715,6 +721,8 @@ int ensure_cell_grid(rhg_ctx *c) {
   return RHG_OK;
 }
 
+int cell_shift_room(rhg_ctx *c, PseudoAtom &t);
+
 // Image lists of the cell grid for the mixed-precision kernels.  They hold
 // every image within reach, which equals the minimum-image convention when
 // every perpendicular width is at least twice the cut-off; the FP32 offsets
@@ -751,7 +759,46 @@ int ensure_image_list(rhg_ctx *c) {
   HIPC(c, hipStreamSynchronize(st));
   c->cg.img_off = c->d_img_off;
   c->cg.img = c->d_img;
+  c->img_version++;
   c->gen++;
+  for (PseudoAtom &t : c->types)
+    if (int rc = cell_shift_room(c, t)) return rc;
+  return RHG_OK;
+}
+
+// Room for the per-cell shift sums of pseudo-atom t over the current image
+// lists (allocated with the lists and with each new pseudo-atom, so that
+// the sums can be formed later without allocating).
+int cell_shift_room(rhg_ctx *c, PseudoAtom &t) {
+  if (!c->cg.img || t.cell_shift_cap >= c->cg.ncell) return RHG_OK;
+  dfree(c, t.cell_shift);
+  t.cell_shift = nullptr;
+  t.cell_shift_cap = 0;
+  if (int rc = dalloc(c, (void **)&t.cell_shift, (size_t)c->cg.ncell * sizeof(double))) return rc;
+  t.cell_shift_cap = c->cg.ncell;
+  t.cell_shift_version = 0;
+  return RHG_OK;
+}
+
+// The per-cell shift sums over the current image lists of the pseudo-atom
+// whose FP64 records are a64 (*out = null when a64 is no pseudo-atom's),
+// formed on first use and after the lists are rebuilt; may_alloc = false
+// never allocates (the room exists unless an allocation failed).
+int cell_shifts(rhg_ctx *c, const AtomRec64 *a64, bool may_alloc, const double **out) {
+  *out = nullptr;
+  for (PseudoAtom &t : c->types) {
+    if (t.d64 != a64) continue;
+    if (may_alloc)
+      if (int rc = cell_shift_room(c, t)) return rc;
+    if (t.cell_shift_cap < c->cg.ncell) return RHG_OK;
+    if (t.cell_shift_version != c->img_version) {
+      launch_img_shift_sums(c->cg, t.d64, t.cell_shift, c->stream);
+      HIPC(c, hipGetLastError());
+      t.cell_shift_version = c->img_version;
+    }
+    *out = t.cell_shift;
+    return RHG_OK;
+  }
   return RHG_OK;
 }
 
@@ -802,6 +849,11 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
     g.img_off = nullptr;
     g.img = nullptr;
   }
+  // shifted LJ on the image lists: the clamped form with the cells' shift sums
+  EnergyArgs ae = a;
+  ae.cell_shift = nullptr;
+  if (a.do_vdw && c->shifted && a.precision != RHG_PREC_FP64 && g.img_off && a.par)
+    if (int rc = cell_shifts(c, a.a64, given == nullptr, &ae.cell_shift)) return rc;
   if (given) {
     BinWork b = *given;
     b.tile_sites = c->tile_sites;
@@ -812,7 +864,7 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
       launch_bin_sites(c->cell, c->cg, s, b, c->stream);
     }
     Timed tm(c, "site_energy");
-    launch_site_energy_culled(a, g, s, b, o, c->stream);
+    launch_site_energy_culled(ae, g, s, b, o, c->stream);
     return RHG_OK;
   }
   const int ncell = c->cg.ncell;
@@ -842,7 +894,7 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
     launch_bin_sites(c->cell, c->cg, s, b, c->stream);
   }
   Timed tm(c, "site_energy");
-  launch_site_energy_culled(a, g, s, b, o, c->stream);
+  launch_site_energy_culled(ae, g, s, b, o, c->stream);
   return RHG_OK;
 }
 
@@ -1082,6 +1134,7 @@ int rhg_add_pseudo_atom(rhg_ctx *c, double eps_k, double sigma, double charge, i
   t.q = charge;
   int rc = setup_pseudo_atom(c, t);
   if (!rc) rc = ads_records(c, t);
+  if (!rc) rc = cell_shift_room(c, t);
   if (rc) return rc;
   c->types.push_back(t);
   *type_out = (int32_t)c->types.size() - 1;

@@ -33,9 +33,12 @@ struct EnergyArgs {
   int n_atoms;
   const AtomRec64 *a64; // FP64 records of this probe type
   const AtomRecMx *amx; // mixed records of this probe type
-  const float4 *par;    // per atom {4 eps, sigma^2 * 2^52, shift, q} of this probe type
+  const float4 *par;    // per atom {4 eps sigma^12, 4 eps sigma^6, shift, q} of this probe type (FP32)
   int precision;        // 0 FP64, 1 mixed
   int do_vdw, do_cc;
+  // shifted LJ on the image lists: per cell, the FP64 sum of this type's
+  // shifts over the cell's list (launch_img_shift_sums); null: not used
+  const double *cell_shift = nullptr;
 };
 
 void launch_site_energy(const EnergyArgs &a, const SiteList &s, const EnergyOut &o,
@@ -93,6 +96,8 @@ void launch_scan_counts(const int32_t *counts, int n, int32_t *off, hipStream_t 
 // pass 0 counts the images of each cell's neighbour atoms into counts, pass 1
 // writes them at off
 // widths: perpendicular widths of the cell
+// out[c] = sum of a64[j].shift over the image list of cell c, c < g.ncell
+void launch_img_shift_sums(const CellGrid &g, const AtomRec64 *a64, double *out, hipStream_t st);
 void launch_image_list(const Cell &cell, const CellGrid &g, double reach, const double widths[3],
                        const double *ax, const double *ay, const double *az, int32_t *counts,
                        const int32_t *off, float4 *img, int pass, hipStream_t st);

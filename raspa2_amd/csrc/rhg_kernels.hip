@@ -484,6 +484,30 @@ void launch_image_list(const Cell &cell, const CellGrid &g, double reach, const 
                      widths[1], widths[2], ax, ay, az, counts, off, img, pass);
 }
 
+// Per cell, the FP64 sum of the LJ shifts of one pseudo-atom type over the
+// cell's image list: one workgroup per cell, each lane summing a strided part
+// of the list in order, then a fixed pairwise tree (the sum does not depend
+// on timing).
+__global__ __launch_bounds__(256) void k_img_shift_sums(const CellGrid g, const AtomRec64 *__restrict__ a64,
+                                                        double *__restrict__ out) {
+  __shared__ double part[256];
+  const int c = blockIdx.x;
+  const int q0 = g.img_off[c], q1 = g.img_off[c + 1];
+  double sum = 0.0;
+  for (int q = q0 + (int)threadIdx.x; q < q1; q += blockDim.x) sum += a64[__float_as_int(g.img[q].w)].shift;
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = part[0];
+}
+
+void launch_img_shift_sums(const CellGrid &g, const AtomRec64 *a64, double *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_img_shift_sums, dim3(g.ncell), dim3(256), 0, st, g, a64, out);
+}
+
 __device__ __forceinline__ int cell_index(const Cell &c, const CellGrid &g, double x, double y,
                                           double z) {
   double f[3];
@@ -735,17 +759,34 @@ __global__ __launch_bounds__(256) void k_site_energy_culled(
 // its parameters are loaded once (scalar loads into SGPR operands) and used
 // for all NS sites of the lane: the scalar loads of the list, not the
 // arithmetic, set the pace of one site per lane.
+//
+// VDW 1: the LJ term c12/r^12 - c6/r^6 - shift, zero beyond the cut-off.
+// VDW 2 (shifted LJ only): r^2 is clamped to the cut-off instead, so that an
+// entry beyond it contributes c12/rc^12 - c6/rc^6, which is its shift; the
+// caller subtracts the cell's sum of shifts over the whole list once per
+// site.  In-range entries then give c12/r^12 - c6/r^6 - shift as well, with
+// no select and the shift out of the pair loop (the product with r^-6 folds
+// into the FP32 group accumulation).
 template <int NS, int VDW, int CC>
 __device__ __forceinline__ void img_sums(const float4 *__restrict__ img, const float4 *__restrict__ par,
                                          int q0, int q1, const float4 (&so)[kImgLane], float rc2vf,
                                          float rc2cf, float alphaf, double cqa,
                                          double (&uv)[kImgLane], double (&uc)[kImgLane]) {
-  // one pair term in FP32 (zero beyond the cut-off); the LJ form keeps one
-  // SGPR operand per instruction
+  // one pair term in FP32 (VDW 1, CC: zero beyond the cut-off; VDW 2: added
+  // to fv); the LJ form keeps one SGPR operand per instruction
   auto term = [&](const float4 e, const float4 pr, const float4 o, float &fv, float &fc) {
     const float dx = o.x - e.x, dy = o.y - e.y, dz = o.z - e.z;
     const float rr = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-    if (VDW) {
+    if (VDW == 2) {
+      const float ir = __builtin_amdgcn_rcpf(fminf(rr, rc2vf));
+      const float ir3 = (ir * ir) * ir;
+      float y = pr.x * ir3;
+      {
+#pragma clang fp contract(off)
+        y = y - pr.y;
+      }
+      fv = fmaf(ir3, y, fv);
+    } else if (VDW) {
       const float ir = __builtin_amdgcn_rcpf(rr);
       const float ir3 = (ir * ir) * ir;
       float y = pr.x * ir3;
@@ -782,9 +823,16 @@ __device__ __forceinline__ void img_sums(const float4 *__restrict__ img, const f
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
       float v[4] = {0.f, 0.f, 0.f, 0.f}, c[4] = {0.f, 0.f, 0.f, 0.f};
+      if (VDW == 2) {
+        // one FP32 accumulator through the group, in list order
 #pragma unroll
-      for (int u = 0; u < 4; ++u) term(e[u], pr[u], so[k], v[u], c[u]);
-      if (VDW) uv[k] += (double)((v[0] + v[1]) + (v[2] + v[3]));
+        for (int u = 0; u < 4; ++u) term(e[u], pr[u], so[k], v[0], c[u]);
+        uv[k] += (double)v[0];
+      } else {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) term(e[u], pr[u], so[k], v[u], c[u]);
+        if (VDW) uv[k] += (double)((v[0] + v[1]) + (v[2] + v[3]));
+      }
       if (CC) uc[k] = fma(cqa, (double)((c[0] + c[1]) + (c[2] + c[3])), uc[k]);
     }
   }
@@ -813,12 +861,14 @@ __device__ __forceinline__ void img_sums(const float4 *__restrict__ img, const f
 // threshold are recomputed exactly (FP64, atom order) so the overlap flags
 // equal the FP64 path's.  CC 1: Coulomb terms from the FP32 erfc fit
 // (alpha rc <= 5, branch-free); CC 2: the library erfcf beyond the fit.
+// VDW 2: shifted LJ with r^2 clamped to the cut-off, cell_shift[c] the FP64
+// sum of the shifts over cell c's image list (k_img_shift_sums).
 template <int BCK, int VDW, int CC>
 __global__ __launch_bounds__(256) void k_site_energy_img(
     const Cell cell, const AtomRec64 *__restrict__ a64, const float4 *__restrict__ par,
     const double rc2v, const double rc2c, const double alpha, const double cqa, const CellGrid g,
-    const SiteList s, const BinWork b, double *__restrict__ out_v, double *__restrict__ out_c,
-    uint8_t *__restrict__ out_o) {
+    const SiteList s, const BinWork b, const double *__restrict__ cell_shift,
+    double *__restrict__ out_v, double *__restrict__ out_c, uint8_t *__restrict__ out_o) {
   const int lane = threadIdx.x & 63;
   const int tile = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
   const int ntiles = b.tile_off[g.ncell];
@@ -856,6 +906,11 @@ __global__ __launch_bounds__(256) void k_site_energy_img(
     img_sums<2, VDW, CC>(g.img, par, q0, q1, so, rc2vf, rc2cf, alphaf, cqa, uv, uc);
   else
     img_sums<1, VDW, CC>(g.img, par, q0, q1, so, rc2vf, rc2cf, alphaf, cqa, uv, uc);
+  if (VDW == 2) {
+    const double sh = cell_shift[c];
+#pragma unroll
+    for (int k = 0; k < kImgLane; ++k) uv[k] -= sh;
+  }
   const double band = 1.0e-4 * kOverlap;
 #pragma unroll
   for (int k = 0; k < kImgLane; ++k) {
@@ -906,7 +961,8 @@ static void img_launch(const EnergyArgs &a, const CellGrid &g, const SiteList &s
   const int w = site_waves(b);
   const unsigned nb = (unsigned)((max_tiles + w - 1) / w);
   hipLaunchKernelGGL((k_site_energy_img<BCK, VDW, CC>), dim3(nb), dim3(64 * w), 0, st, a.cell, a.a64,
-                     a.par, a.rc2_vdw, a.rc2_cc, a.alpha, a.cqa, g, s, b, o.u_vdw, o.u_cc, o.overlap);
+                     a.par, a.rc2_vdw, a.rc2_cc, a.alpha, a.cqa, g, s, b, a.cell_shift, o.u_vdw,
+                     o.u_cc, o.overlap);
 }
 
 template <int PREC, int BCK, int VDW, int CC>
@@ -922,20 +978,29 @@ static void culled_launch(const EnergyArgs &a, const CellGrid &g, const SiteList
 }
 
 template <int VDW, int CC>
+static void img_mode(const EnergyArgs &a, const CellGrid &g, const SiteList &s, const BinWork &b,
+                     const EnergyOut &o, hipStream_t st) {
+  const int bck = a.cell.bc == 1 ? 1 : (a.cell.upper ? 2 : 3);
+  // CC 2: erfc arguments beyond the fit's range (alpha rc > 5) possible
+  if (CC && a.alpha * sqrt(a.rc2_cc) > 5.0) {
+    if (bck == 1) img_launch<1, VDW, 2>(a, g, s, b, o, st);
+    else if (bck == 2) img_launch<2, VDW, 2>(a, g, s, b, o, st);
+    else img_launch<3, VDW, 2>(a, g, s, b, o, st);
+  } else {
+    if (bck == 1) img_launch<1, VDW, CC>(a, g, s, b, o, st);
+    else if (bck == 2) img_launch<2, VDW, CC>(a, g, s, b, o, st);
+    else img_launch<3, VDW, CC>(a, g, s, b, o, st);
+  }
+}
+
+template <int VDW, int CC>
 static void culled_mode(const EnergyArgs &a, const CellGrid &g, const SiteList &s, const BinWork &b,
                         const EnergyOut &o, hipStream_t st) {
   const int bck = a.cell.bc == 1 ? 1 : (a.cell.upper ? 2 : 3);
   if (a.precision != 0 && g.img_off && a.par && b.fix) {
-    // CC 2: erfc arguments beyond the fit's range (alpha rc > 5) possible
-    if (CC && a.alpha * sqrt(a.rc2_cc) > 5.0) {
-      if (bck == 1) img_launch<1, VDW, 2>(a, g, s, b, o, st);
-      else if (bck == 2) img_launch<2, VDW, 2>(a, g, s, b, o, st);
-      else img_launch<3, VDW, 2>(a, g, s, b, o, st);
-    } else {
-      if (bck == 1) img_launch<1, VDW, CC>(a, g, s, b, o, st);
-      else if (bck == 2) img_launch<2, VDW, CC>(a, g, s, b, o, st);
-      else img_launch<3, VDW, CC>(a, g, s, b, o, st);
-    }
+    // shifted LJ with the cells' shift sums: the clamped form
+    if (VDW && a.cell_shift) img_mode<2, CC>(a, g, s, b, o, st);
+    else img_mode<VDW, CC>(a, g, s, b, o, st);
     return;
   }
   if (a.precision == 0) {
