@@ -174,6 +174,7 @@ struct rhg_ctx {
   std::vector<int32_t> kidx, row_off, row_kz;
   double *d_kfactor = nullptr;
   double2 *d_sf = nullptr;
+  double2 *d_ks = nullptr; // kfactor times d_sf
   int32_t *d_row_off = nullptr, *d_row_kz = nullptr;
   size_t footprint = 0;
   // cell lists
@@ -636,6 +637,7 @@ EwaldDev ewald_dev(rhg_ctx *c) {
   e.beta_unused = 0.0;
   e.kfactor = c->d_kfactor;
   e.sf = c->d_sf;
+  e.ks = c->d_ks;
   e.row_off = c->d_row_off;
   e.row_kz = c->d_row_kz;
   e.cell = c->cell;
@@ -1052,6 +1054,7 @@ int create_impl(const rhg_framework *fw, const rhg_cell *cellp, const rhg_forcef
     size_t nk = (size_t)std::max(c->nk, 1);
     if ((rc = dalloc(c.get(), (void **)&c->d_kfactor, nk * sizeof(double)))) return rc;
     if ((rc = dalloc(c.get(), (void **)&c->d_sf, nk * sizeof(double2)))) return rc;
+    if ((rc = dalloc(c.get(), (void **)&c->d_ks, nk * sizeof(double2)))) return rc;
     if ((rc = dalloc(c.get(), (void **)&c->d_row_off, c->row_off.size() * sizeof(int32_t)))) return rc;
     if ((rc = dalloc(c.get(), (void **)&c->d_row_kz, c->row_kz.size() * sizeof(int32_t)))) return rc;
     upload(c.get(), c->d_kfactor, c->kfactor.data(), c->kfactor.size() * sizeof(double));
@@ -1075,6 +1078,8 @@ int create_impl(const rhg_framework *fw, const rhg_cell *cellp, const rhg_forcef
     }
     if (c->kz > 64) return RHG_ERR_UNSUPPORTED; // k_z range of the row kernel
     if ((rc = structure_factor(c.get(), fx, fy, fz, qq, c->d_sf))) return rc;
+    launch_ks_table(c->d_kfactor, c->d_sf, c->nk, c->d_ks, c->stream);
+    HIPC(c.get(), hipGetLastError());
   }
   *out = c.release();
   return RHG_OK;

@@ -114,6 +114,7 @@ struct EwaldDev {
   double alpha, uion, net_q_fw, beta_unused;
   const double *kfactor;   // nk, RASPA2 order
   const double2 *sf;       // nk framework structure factors
+  const double2 *ks;       // nk: kfactor times sf (the host term's weights)
   const int32_t *row_off;  // (kx+1)*(2ky+1): nvec index of the row's first valid kz
   const int32_t *row_kz;   // (kx+1)*(2ky+1): largest valid |kz| of the row (-1 empty)
   Cell cell;
@@ -125,6 +126,9 @@ struct EwaldDev {
 void launch_structure_factor(const EwaldDev &e, const int32_t *kidx, int nk, const double *fx,
                              const double *fy, const double *fz, const double *q, int n,
                              double2 *tab, double2 *out, hipStream_t st);
+
+// ks[i] = kf[i] sf[i], i < nk
+void launch_ks_table(const double *kf, const double2 *sf, int nk, double2 *ks, hipStream_t st);
 
 // Fourier Delta of n molecules; positions pos[j][m] (site-major SoA of
 // three arrays each n_sites*stride); list/n_dev select molecules.

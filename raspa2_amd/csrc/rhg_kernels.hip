@@ -1162,6 +1162,16 @@ void launch_structure_factor(const EwaldDev &e, const int32_t *kidx, int nk, con
   hipLaunchKernelGGL(k_sf_rows, dim3(rows), dim3(256), 0, st, e, (const double2 *)tab, q, n, out);
 }
 
+__global__ __launch_bounds__(256) void k_ks_table(const double *__restrict__ kf, const double2 *__restrict__ sf,
+                                                  int nk, double2 *__restrict__ ks) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nk) ks[i] = make_double2(kf[i] * sf[i].x, kf[i] * sf[i].y);
+}
+
+void launch_ks_table(const double *kf, const double2 *sf, int nk, double2 *ks, hipStream_t st) {
+  if (nk > 0) hipLaunchKernelGGL(k_ks_table, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, st, kf, sf, nk, ks);
+}
+
 // ====================================================================
 // Fourier Delta of inserted molecules (src/ewald.c:6859-7565), one lane
 // per molecule.  exp(ik.r) by the reference's recurrences; the loop visits
@@ -1255,15 +1265,16 @@ __global__ __launch_bounds__(128) void k_fourier_part(const FourierArgs a, doubl
           sim += xy[t].im;
         }
         const int idx = off + kzm;
-        const double kf = e.kfactor[idx];
-        const double2 sf = e.sf[idx];
-        host = fma(kf, fma(sf.y, sim, sf.x * sre), host);
-        ads = fma(kf, ads_term<ADS>(a.sads, idx, sre, sim), ads);
+        const double2 ks = e.ks[idx];
+        host = fma(ks.y, sim, fma(ks.x, sre, host));
+        ads = fma(e.kfactor[idx], ads_term<ADS>(a.sads, idx, sre, sim), ads);
       }
       // k_z = +kk and -kk share the four products Re(xy) Re(ez), Im(xy) Im(ez),
       // Re(xy) Im(ez), Im(xy) Re(ez); in the origin row the +k_z entries sit
-      // one place lower (k = 0 is not stored)
+      // one place lower (k = 0 is not stored).  Two k_z steps per iteration
+      // keep the phase recurrence in place (no register copies).
       const int ip = off + kzm - (origin_row ? 1 : 0), im = off + kzm;
+#pragma unroll 2
       for (int kk = 1; kk <= kzm; ++kk) {
         double a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
 #pragma unroll
@@ -1277,17 +1288,15 @@ __global__ __launch_bounds__(128) void k_fourier_part(const FourierArgs a, doubl
         }
         {
           const double sre = a1 - a2, sim = a3 + a4; // +k_z
-          const double kf = e.kfactor[ip + kk];
-          const double2 sf = e.sf[ip + kk];
-          host = fma(kf, fma(sf.y, sim, sf.x * sre), host);
-          ads = fma(kf, ads_term<ADS>(a.sads, ip + kk, sre, sim), ads);
+          const double2 ks = e.ks[ip + kk];
+          host = fma(ks.y, sim, fma(ks.x, sre, host));
+          ads = fma(e.kfactor[ip + kk], ads_term<ADS>(a.sads, ip + kk, sre, sim), ads);
         }
         {
           const double sre = a1 + a2, sim = a4 - a3; // -k_z
-          const double kf = e.kfactor[im - kk];
-          const double2 sf = e.sf[im - kk];
-          host = fma(kf, fma(sf.y, sim, sf.x * sre), host);
-          ads = fma(kf, ads_term<ADS>(a.sads, im - kk, sre, sim), ads);
+          const double2 ks = e.ks[im - kk];
+          host = fma(ks.y, sim, fma(ks.x, sre, host));
+          ads = fma(e.kfactor[im - kk], ads_term<ADS>(a.sads, im - kk, sre, sim), ads);
         }
       }
     }
