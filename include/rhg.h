@@ -162,7 +162,7 @@ int rhg_framework_energy_at_positions(rhg_ctx *ctx, int64_t n, const double *xyz
  * outputs are DEVICE pointers (d_xyz: n x 3 interleaved; outputs nullable),
  * the results equal the allocating form's, and the work is only enqueued on
  * the context's stream (rhg_synchronize waits for it).  Binning scratch is a
- * fixed device array of the library (16 MiB, one per GPU, held until the
+ * fixed device array of the library (18 MiB, one per GPU, held until the
  * process ends): n <= 262144 (RHG_ERR_INVALID beyond).  The context's cell
  * lists must exist (RHG_ERR_INVALID otherwise): one call of the allocating
  * form at set-up builds them.  Every call on one GPU shares that GPU's array,
@@ -472,9 +472,10 @@ int rhg_widom_stats(rhg_ctx *ctx, double out[12], int32_t reset);
  * Default cell 2.5 A. */
 int rhg_set_culling(rhg_ctx *ctx, int32_t on, double cell_size);
 /* Launch shape of the culled site kernels (no counterpart in the reference;
- * a tuning knob): sites per lane of the image-list kernels (1..4: tiles of
- * 64 x that many sites of one cell; default 4) and waves per workgroup
- * (1, 2 or 4; default 4).  Results do not depend on either. */
+ * a tuning knob): sites per lane of the LJ-only image-list kernels (1..4:
+ * tiles of 64 x that many sites of one cell; default 4; lists with Coulomb
+ * terms always take one) and waves per workgroup (1, 2 or 4; default 2).
+ * Results do not depend on either. */
 int rhg_set_tiles(rhg_ctx *ctx, int32_t sites_per_lane, int32_t waves_per_block);
 /* Scratch buffers that contexts can borrow (no counterpart in the
  * reference): while a workspace is attached, the context's per-call device

@@ -804,6 +804,15 @@ int cell_shifts(rhg_ctx *c, const AtomRec64 *a64, bool may_alloc, const double *
   return RHG_OK;
 }
 
+// Sites per tile of a list's image-list kernel: the launch-shape knob for
+// LJ-only lists; one site per lane where Coulomb terms are summed (their
+// pair loop holds more registers per site, and 64-site tiles give sparse
+// lists, such as first beads and charged beads, more waves).
+int list_tile_sites(const rhg_ctx *c, const EnergyArgs &a) {
+  if (a.do_cc) return 64;
+  return c->tile_sites;
+}
+
 int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const EnergyOut &o,
                       const char *tag, const BinWork *given = nullptr);
 
@@ -858,7 +867,7 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
     if (int rc = cell_shifts(c, a.a64, given == nullptr, &ae.cell_shift)) return rc;
   if (given) {
     BinWork b = *given;
-    b.tile_sites = c->tile_sites;
+    b.tile_sites = list_tile_sites(c, a);
     b.waves = c->waves;
     if (!(a.precision != RHG_PREC_FP64 && g.img_off && a.par)) b.fix = nullptr;
     {
@@ -872,7 +881,7 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
   const int ncell = c->cg.ncell;
   std::string t = tag;
   BinWork b;
-  b.tile_sites = c->tile_sites;
+  b.tile_sites = list_tile_sites(c, a);
   b.waves = c->waves;
   b.cell_of = (int32_t *)work(c, (t + "_cellof").c_str(), (size_t)s.n * sizeof(int32_t));
   b.perm = (int32_t *)work(c, (t + "_perm").c_str(), (size_t)s.n * sizeof(int32_t));
@@ -884,7 +893,9 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
   b.fix = nullptr;
   if (a.precision != RHG_PREC_FP64 && g.img_off && a.par) {
     b.fix = (float4 *)work(c, (t + "_fix").c_str(), (size_t)s.n * sizeof(float4));
-    if (!b.fix) return fail(c, RHG_ERR_OOM, "binning scratch");
+    b.redo = (int2 *)work(c, (t + "_redo").c_str(), (size_t)s.n * sizeof(int2));
+    b.n_redo = (int32_t *)work(c, (t + "_nredo").c_str(), 256);
+    if (!b.fix || !b.redo || !b.n_redo) return fail(c, RHG_ERR_OOM, "binning scratch");
   }
   bool fresh = c->bufs["pair_counter"].p == nullptr;
   b.pairs = (unsigned long long *)work(c, "pair_counter", 64);
@@ -944,7 +955,8 @@ void frac_2pi(const rhg_ctx *c, const double p[3], double f[3]) {
 // device array of the library, so that the entry point allocates nothing.
 // Layout (each part 256-byte aligned): site x, y, z (FP64), cell of each
 // site, sorted order, sorted FP32 offsets, then per cell the site counts,
-// fill counters, starts and tile offsets.  Each GPU holds its own copy; its
+// fill counters, starts and tile offsets, then the list of sites to
+// recompute exactly and its count.  Each GPU holds its own copy; its
 // address comes from hipGetSymbolAddress once per device, at context
 // creation with that device current.
 // ======================================================================
@@ -956,7 +968,7 @@ constexpr size_t kNaXyz = na_round(kNaSites * sizeof(double));
 constexpr size_t kNaI32 = na_round(kNaSites * sizeof(int32_t));
 constexpr size_t kNaFix = na_round(kNaSites * sizeof(float4));
 constexpr size_t kNaCell = na_round((kNaCells + 1) * sizeof(int32_t));
-constexpr size_t kNaBytes = 3 * kNaXyz + 2 * kNaI32 + kNaFix + 4 * kNaCell;
+constexpr size_t kNaBytes = 3 * kNaXyz + 4 * kNaI32 + kNaFix + 4 * kNaCell + 256;
 constexpr int kNaDevices = 64;
 } // namespace
 
@@ -1254,6 +1266,10 @@ int rhg_framework_energy_at_positions_noalloc(rhg_ctx *c, int64_t n, const doubl
   b.start = (int32_t *)p;
   p += kNaCell;
   b.tile_off = (int32_t *)p;
+  p += kNaCell;
+  b.redo = (int2 *)p;
+  p += 2 * kNaI32;
+  b.n_redo = (int32_t *)p;
   b.pairs = nullptr; // no work counters on this path
   // event timing creates events: off for the duration of the call
   const bool prof = c->prof;
@@ -1394,6 +1410,9 @@ int widom_buffers(rhg_ctx *c, const MolDev &m, int64_t cap, WidomBufs &B) {
   size_t nfb = (size_t)cap * m.k_fb;
   size_t nor = (size_t)cap * m.k_or * std::max(m.n_place, 1);
   size_t ncf = (size_t)cap * m.n_sites;
+  // site lists are indexed with 32-bit positions
+  if (std::max(nfb, std::max(nor, ncf)) > (size_t)INT32_MAX)
+    return fail(c, RHG_ERR_INVALID, "chunk x trials x sites above 2^31 - 1: use a smaller chunk");
   size_t d8 = sizeof(double);
 #define WB(field, name, bytes)                           \
   w.field = (decltype(w.field))work(c, name, (bytes)); \

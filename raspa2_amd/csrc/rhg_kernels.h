@@ -73,20 +73,24 @@ struct BinWork {
   int32_t *start;    // per cell, exclusive scan of count
   int32_t *tile_off; // per cell + 1, exclusive scan of ceil(count/ts): ts = 256
                      // sites when fix is set (image-list kernels), else 64
-  int32_t *perm;     // sorted position -> site index
+  int32_t *perm;     // sorted position -> site index (written only when fix is not)
   // optional (image-list kernels): per sorted position the site's offset from
   // its cell centre in A as FP32 and the site index bit-cast into w
   float4 *fix;
+  // image-list kernels: (sorted position, cell) of the sites to recompute
+  // exactly (room for every site) and their count (cleared by the binning)
+  int2 *redo = nullptr;
+  int32_t *n_redo = nullptr;
   unsigned long long *pairs; // executed pair evaluations [LJ, LJ+Coulomb, Coulomb]
   // launch shape of the site kernels: sites per tile of the image-list
-  // kernels (a multiple of 64 up to 64 kImgLane; 0: 64 kImgLane) and waves
-  // per workgroup (1, 2 or 4; 0: 4)
+  // kernels (64, 128, 192 or 256; 0: 256; the kernel's registers are sized
+  // for it) and waves per workgroup (1, 2 or 4; 0: 2)
   int32_t tile_sites = 0;
   int32_t waves = 0;
 };
 
 __host__ __device__ inline int img_tile_sites(const BinWork &b) { return b.tile_sites ? b.tile_sites : 64 * kImgLane; }
-inline int site_waves(const BinWork &b) { return b.waves ? b.waves : 4; }
+inline int site_waves(const BinWork &b) { return b.waves ? b.waves : 2; }
 
 void launch_cell_neighbours(const Cell &cell, int nc0, int nc1, int nc2, double reach,
                             const double *ax, const double *ay, const double *az, int n_atoms,

@@ -112,8 +112,9 @@ __device__ __forceinline__ void mi_fast(const Cell &c, double &dx, double &dy, d
 // positions, which binning aggregates.
 __device__ __forceinline__ int64_t site_phys(const SiteList &s, int64_t i) {
   if (s.in_stride <= 0) return i;
-  const int64_t rows = s.count_mult;
-  return (i % rows) * s.in_stride + i / rows;
+  // list indices stay below 2^31: 32-bit division
+  const uint32_t rows = (uint32_t)s.count_mult, u = (uint32_t)i;
+  return (int64_t)(u % rows) * s.in_stride + (int64_t)(u / rows);
 }
 
 __device__ __forceinline__ int64_t site_count(const SiteList &s) {
@@ -523,13 +524,27 @@ __device__ __forceinline__ int cell_index(const Cell &c, const CellGrid &g, doub
   return (ci[0] * g.nc[1] + ci[1]) * g.nc[2] + ci[2];
 }
 
+// A slot of a compacted list for each lane that calls this: one atomic per
+// wave and consecutive slots for the calling lanes in lane order, so that
+// the rows a wave writes by slot are contiguous (coalesced stores).
+__device__ __forceinline__ int wave_slot(int32_t *counter) {
+  const unsigned long long m = __ballot(1);
+  const int lane = threadIdx.x & 63;
+  const int leader = __builtin_ctzll(m);
+  int base = 0;
+  if (lane == leader) base = atomicAdd(counter, (int)__popcll(m));
+  base = __shfl(base, leader);
+  return base + (int)__popcll(m & ((1ULL << lane) - 1ULL));
+}
+
 __global__ __launch_bounds__(256) void k_zero_i32(int32_t *__restrict__ a, int32_t *__restrict__ b,
-                                                  int n) {
+                                                  int n, int32_t *__restrict__ one) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     a[i] = 0;
     b[i] = 0;
   }
+  if (i == 0 && one) *one = 0;
 }
 
 // Lanes of a wave that bin into the same cell as their left neighbour form a
@@ -575,23 +590,31 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const Cell cell, const Cell
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < site_count(s);
   const int c = valid ? b.cell_of[i] : -1;
+  const int64_t site = site_phys(s, valid ? i : 0);
+  // the site's offset from its cell centre (for the image-list kernels, which
+  // read no permutation), formed before the slot is known so that the
+  // position loads overlap the atomic's round trip
+  float4 f = make_float4(0.f, 0.f, 0.f, __int_as_float((int)site));
+  if (b.fix && valid) {
+    double cx, cy, cz;
+    cell_centre(cell, g, c, cx, cy, cz);
+    double ox = s.x[site] - cx, oy = s.y[site] - cy, oz = s.z[site] - cz;
+    mi_exact<BCK>(cell, ox, oy, oz);
+    f.x = (float)ox;
+    f.y = (float)oy;
+    f.z = (float)oz;
+  }
   const RunInfo r = wave_runs(c, valid);
   int base = 0;
   const int lane = threadIdx.x & 63;
   if (valid && lane == r.head) base = b.start[c] + atomicAdd(&b.fill[c], r.length);
   base = __shfl(base, r.head);
   if (!valid) return;
-  const int64_t site = site_phys(s, i);
   const int pos = base + (lane - r.head);
-  b.perm[pos] = (int32_t)site;
-  if (b.fix) {
-    // the site's offset from its cell centre, sorted, for the image lists
-    double cx, cy, cz;
-    cell_centre(cell, g, c, cx, cy, cz);
-    double ox = s.x[site] - cx, oy = s.y[site] - cy, oz = s.z[site] - cz;
-    mi_exact<BCK>(cell, ox, oy, oz);
-    b.fix[pos] = make_float4((float)ox, (float)oy, (float)oz, __int_as_float((int)site));
-  }
+  if (b.fix)
+    b.fix[pos] = f;
+  else
+    b.perm[pos] = (int32_t)site;
 }
 
 // exclusive scans of the per-cell site counts (start) and of the per-cell
@@ -634,7 +657,7 @@ __global__ __launch_bounds__(1024) void k_scan_bins(const int32_t *__restrict__ 
 void launch_bin_sites(const Cell &cell, const CellGrid &g, const SiteList &s, const BinWork &b,
                       hipStream_t st) {
   const unsigned nbc = (unsigned)((g.ncell + 255) / 256);
-  hipLaunchKernelGGL(k_zero_i32, dim3(nbc), dim3(256), 0, st, b.count, b.fill, g.ncell);
+  hipLaunchKernelGGL(k_zero_i32, dim3(nbc), dim3(256), 0, st, b.count, b.fill, g.ncell, b.n_redo);
   const unsigned nb = (unsigned)((s.n + 255) / 256);
   if (nb) hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(256), 0, st, cell, g, s, b);
   hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(1024), 0, st, b.count, g.ncell, b.start, b.tile_off,
@@ -767,11 +790,11 @@ __global__ __launch_bounds__(256) void k_site_energy_culled(
 // site.  In-range entries then give c12/r^12 - c6/r^6 - shift as well, with
 // no select and the shift out of the pair loop (the product with r^-6 folds
 // into the FP32 group accumulation).
-template <int NS, int VDW, int CC>
+template <int NS, int VDW, int CC, int NSM>
 __device__ __forceinline__ void img_sums(const float4 *__restrict__ img, const float4 *__restrict__ par,
-                                         int q0, int q1, const float4 (&so)[kImgLane], float rc2vf,
+                                         int q0, int q1, const float4 (&so)[NSM], float rc2vf,
                                          float rc2cf, float alphaf, double cqa,
-                                         double (&uv)[kImgLane], double (&uc)[kImgLane]) {
+                                         double (&uv)[NSM], double (&uc)[NSM]) {
   // one pair term in FP32 (VDW 1, CC: zero beyond the cut-off; VDW 2: added
   // to fv); the LJ form keeps one SGPR operand per instruction
   auto term = [&](const float4 e, const float4 pr, const float4 o, float &fv, float &fc) {
@@ -858,12 +881,12 @@ __device__ __forceinline__ void img_sums(const float4 *__restrict__ img, const f
 // c12 ir^6 - c6 ir^3 - shift and the real-space Coulomb term
 // q_j erfc(alpha r)/r in FP32, the Coulomb prefactor applied to the FP64
 // group sums.  Sites whose sum lands within the guard band of the overlap
-// threshold are recomputed exactly (FP64, atom order) so the overlap flags
-// equal the FP64 path's.  CC 1: Coulomb terms from the FP32 erfc fit
+// threshold are listed and recomputed exactly (FP64, atom order) by
+// k_site_redo, so the overlap flags equal the FP64 path's.  CC 1: Coulomb terms from the FP32 erfc fit
 // (alpha rc <= 5, branch-free); CC 2: the library erfcf beyond the fit.
 // VDW 2: shifted LJ with r^2 clamped to the cut-off, cell_shift[c] the FP64
 // sum of the shifts over cell c's image list (k_img_shift_sums).
-template <int BCK, int VDW, int CC>
+template <int BCK, int VDW, int CC, int NSM>
 __global__ __launch_bounds__(256) void k_site_energy_img(
     const Cell cell, const AtomRec64 *__restrict__ a64, const float4 *__restrict__ par,
     const double rc2v, const double rc2c, const double alpha, const double cqa, const CellGrid g,
@@ -883,67 +906,47 @@ __global__ __launch_bounds__(256) void k_site_energy_img(
   const int first = (tile - b.tile_off[c]) * ts;
   const int nl = min(ts, b.count[c] - first); // live sites of the tile
   // sorted offsets from the cell centre (written by the binning)
-  float4 so[kImgLane];
-  bool live[kImgLane];
+  float4 so[NSM];
+  bool live[NSM];
 #pragma unroll
-  for (int k = 0; k < kImgLane; ++k) {
+  for (int k = 0; k < NSM; ++k) {
     const int kk = lane + 64 * k;
     live[k] = kk < nl;
     so[k] = b.fix[b.start[c] + first + (live[k] ? kk : 0)];
   }
   const int q0 = g.img_off[c], q1 = g.img_off[c + 1];
-  double uv[kImgLane], uc[kImgLane];
+  double uv[NSM], uc[NSM];
 #pragma unroll
-  for (int k = 0; k < kImgLane; ++k) uv[k] = uc[k] = 0.0;
+  for (int k = 0; k < NSM; ++k) uv[k] = uc[k] = 0.0;
   const float rc2vf = (float)rc2v, rc2cf = (float)rc2c, alphaf = (float)alpha;
   // only as many sites per lane as the tile holds (wave-uniform)
-  static_assert(kImgLane == 4, "the dispatch below covers 1..4 sites per lane");
-  if (nl > 192)
-    img_sums<4, VDW, CC>(g.img, par, q0, q1, so, rc2vf, rc2cf, alphaf, cqa, uv, uc);
-  else if (nl > 128)
-    img_sums<3, VDW, CC>(g.img, par, q0, q1, so, rc2vf, rc2cf, alphaf, cqa, uv, uc);
-  else if (nl > 64)
-    img_sums<2, VDW, CC>(g.img, par, q0, q1, so, rc2vf, rc2cf, alphaf, cqa, uv, uc);
+  static_assert(NSM >= 1 && NSM <= kImgLane, "1..4 sites per lane");
+  if (NSM >= 4 && nl > 192)
+    img_sums<(NSM >= 4 ? 4 : 1), VDW, CC, NSM>(g.img, par, q0, q1, so, rc2vf, rc2cf, alphaf, cqa, uv, uc);
+  else if (NSM >= 3 && nl > 128)
+    img_sums<(NSM >= 3 ? 3 : 1), VDW, CC, NSM>(g.img, par, q0, q1, so, rc2vf, rc2cf, alphaf, cqa, uv, uc);
+  else if (NSM >= 2 && nl > 64)
+    img_sums<(NSM >= 2 ? 2 : 1), VDW, CC, NSM>(g.img, par, q0, q1, so, rc2vf, rc2cf, alphaf, cqa, uv, uc);
   else
-    img_sums<1, VDW, CC>(g.img, par, q0, q1, so, rc2vf, rc2cf, alphaf, cqa, uv, uc);
+    img_sums<1, VDW, CC, NSM>(g.img, par, q0, q1, so, rc2vf, rc2cf, alphaf, cqa, uv, uc);
   if (VDW == 2) {
     const double sh = cell_shift[c];
 #pragma unroll
-    for (int k = 0; k < kImgLane; ++k) uv[k] -= sh;
+    for (int k = 0; k < NSM; ++k) uv[k] -= sh;
   }
   const double band = 1.0e-4 * kOverlap;
 #pragma unroll
-  for (int k = 0; k < kImgLane; ++k) {
+  for (int k = 0; k < NSM; ++k) {
     const bool near = live[k] && ((VDW && fabs(uv[k] - kOverlap) <= band) ||
                                   (CC && fabs(uc[k] - kOverlap) <= band));
-    if (__any(near)) {
-      if (near) {
-#pragma clang fp contract(off)
-        const int site = __float_as_int(so[k].w);
-        const double px = s.x[site], py = s.y[site], pz = s.z[site];
-        double ev = 0.0, ec = 0.0;
-        const int j0 = g.nb_off[c], j1 = g.nb_off[c + 1];
-        for (int q = j0; q < j1; ++q) {
-          const AtomRec64 r = a64[g.nb_idx[q]];
-          double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
-          mi_exact<BCK>(cell, dx, dy, dz);
-          double rr = dx * dx + dy * dy + dz * dz;
-          if (VDW && rr < rc2v) ev += lj_exact(r.c4, r.s2, r.shift, rr);
-          if (CC && rr < rc2c) {
-            double rd = sqrt(rr);
-            ec += cqa * r.q * (erfc(alpha * rd) / rd);
-          }
-        }
-        if (VDW) uv[k] = ev;
-        if (CC) uc[k] = ec;
-      }
-    }
+    // listed for k_site_redo, which recomputes them exactly after this kernel
+    if (near) b.redo[wave_slot(b.n_redo)] = make_int2(b.start[c] + first + lane + 64 * k, c);
   }
   if (lane == 0 && b.pairs)
     atomicAdd(b.pairs + ((VDW && CC) ? 1 : (VDW ? 0 : 2)),
               (unsigned long long)nl * (unsigned long long)(q1 - q0));
 #pragma unroll
-  for (int k = 0; k < kImgLane; ++k) {
+  for (int k = 0; k < NSM; ++k) {
     if (live[k]) {
       const int64_t oi = __float_as_int(so[k].w);
       if (out_v) out_v[oi] = uv[k];
@@ -953,16 +956,104 @@ __global__ __launch_bounds__(256) void k_site_energy_img(
   }
 }
 
+// Exact FP64 energies (atom order, as the culled FP64 kernel sums them) of
+// the sites the image-list kernel listed as within the guard band of the
+// overlap threshold: b.redo holds their sorted positions and cells.  One wave per
+// listed site: the lanes evaluate 256 pair terms of the cell's ascending
+// neighbour list at a time into LDS, then lane 0 adds them in list order,
+// so each sum is the sequential FP64 sum bit for bit.  Keeping this rare
+// path out of the image-list kernel keeps that kernel's registers (and so
+// its occupancy) those of the FP32 pair loop.
+template <int BCK, int VDW, int CC>
+__global__ __launch_bounds__(64) void k_site_redo(const Cell cell, const AtomRec64 *__restrict__ a64,
+                                                  const double rc2v, const double rc2c, const double alpha,
+                                                  const double cqa, const CellGrid g, const SiteList s,
+                                                  const BinWork b, double *__restrict__ out_v,
+                                                  double *__restrict__ out_c, uint8_t *__restrict__ out_o) {
+  constexpr int kPass = 256;
+  __shared__ double tv[kPass], tc[kPass];
+  const int lane = threadIdx.x;
+  const int n = *b.n_redo;
+  for (int e = blockIdx.x; e < n; e += gridDim.x) {
+#pragma clang fp contract(off)
+    const int2 rd = b.redo[e];
+    const int c = rd.y;
+    const int site = __float_as_int(b.fix[rd.x].w);
+    const double px = s.x[site], py = s.y[site], pz = s.z[site];
+    double ev = 0.0, ec = 0.0; // lane 0's
+    const int j0 = g.nb_off[c], j1 = g.nb_off[c + 1];
+    for (int q0 = j0; q0 < j1; q0 += kPass) {
+      // a term out of range (or past the list) is stored as +0.0: adding it
+      // leaves a sum that started at +0.0 unchanged, bit for bit
+      for (int u = lane; u < kPass; u += 64) {
+        const int q = q0 + u;
+        double v = 0.0, w = 0.0;
+        if (q < j1) {
+          const AtomRec64 r = a64[g.nb_idx[q]];
+          double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
+          mi_exact<BCK>(cell, dx, dy, dz);
+          double rr = dx * dx + dy * dy + dz * dz;
+          if (VDW && rr < rc2v) v = lj_exact(r.c4, r.s2, r.shift, rr);
+          if (CC && rr < rc2c) {
+            double rd = sqrt(rr);
+            w = cqa * r.q * (erfc(alpha * rd) / rd);
+          }
+        }
+        tv[u] = v;
+        tc[u] = w;
+      }
+      __syncthreads();
+      if (lane == 0) {
+        const int m = min(kPass, j1 - q0);
+        for (int u = 0; u < m; u += 8) {
+          double v8[8], c8[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            v8[k] = tv[u + k];
+            c8[k] = tc[u + k];
+          }
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            if (VDW) ev += v8[k];
+            if (CC) ec += c8[k];
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (lane == 0) {
+      if (out_v && VDW) out_v[site] = ev;
+      if (out_c && CC) out_c[site] = ec;
+      if (out_o) out_o[site] = (uint8_t)((VDW && ev >= kOverlap) || (CC && ec >= kOverlap));
+    }
+  }
+}
+
 template <int BCK, int VDW, int CC>
 static void img_launch(const EnergyArgs &a, const CellGrid &g, const SiteList &s, const BinWork &b,
                        const EnergyOut &o, hipStream_t st) {
   // upper bound of tiles: sites/(tile size) + one partial tile per cell
-  const int64_t max_tiles = s.n / img_tile_sites(b) + g.ncell + 1;
+  const int ts = img_tile_sites(b);
+  const int64_t max_tiles = s.n / ts + g.ncell + 1;
   const int w = site_waves(b);
   const unsigned nb = (unsigned)((max_tiles + w - 1) / w);
-  hipLaunchKernelGGL((k_site_energy_img<BCK, VDW, CC>), dim3(nb), dim3(64 * w), 0, st, a.cell, a.a64,
-                     a.par, a.rc2_vdw, a.rc2_cc, a.alpha, a.cqa, g, s, b, a.cell_shift, o.u_vdw,
-                     o.u_cc, o.overlap);
+  // the kernel's registers are sized for the tile's sites per lane
+  if (ts <= 64)
+    hipLaunchKernelGGL((k_site_energy_img<BCK, VDW, CC, 1>), dim3(nb), dim3(64 * w), 0, st, a.cell, a.a64,
+                       a.par, a.rc2_vdw, a.rc2_cc, a.alpha, a.cqa, g, s, b, a.cell_shift, o.u_vdw,
+                       o.u_cc, o.overlap);
+  else if (ts <= 128)
+    hipLaunchKernelGGL((k_site_energy_img<BCK, VDW, CC, 2>), dim3(nb), dim3(64 * w), 0, st, a.cell, a.a64,
+                       a.par, a.rc2_vdw, a.rc2_cc, a.alpha, a.cqa, g, s, b, a.cell_shift, o.u_vdw,
+                       o.u_cc, o.overlap);
+  else
+    hipLaunchKernelGGL((k_site_energy_img<BCK, VDW, CC, 4>), dim3(nb), dim3(64 * w), 0, st, a.cell, a.a64,
+                       a.par, a.rc2_vdw, a.rc2_cc, a.alpha, a.cqa, g, s, b, a.cell_shift, o.u_vdw,
+                       o.u_cc, o.overlap);
+  // the listed near-threshold sites, exactly (the list is short: a grid of
+  // 256 waves strides over it, one wave per site)
+  hipLaunchKernelGGL((k_site_redo<BCK, VDW, CC>), dim3(256), dim3(64), 0, st, a.cell, a.a64, a.rc2_vdw,
+                     a.rc2_cc, a.alpha, a.cqa, g, s, b, o.u_vdw, o.u_cc, o.overlap);
 }
 
 template <int PREC, int BCK, int VDW, int CC>
@@ -1457,18 +1548,6 @@ __global__ __launch_bounds__(256) void k_blocked_pockets(const PocketArgs p, con
   flags[i] = blocked_pocket(p, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]) ? 1 : 0;
 }
 
-// A slot of a compacted list for each lane that calls this: one atomic per
-// wave and consecutive slots for the calling lanes in lane order, so that
-// the rows a wave writes by slot are contiguous (coalesced stores).
-__device__ __forceinline__ int wave_slot(int32_t *counter) {
-  const unsigned long long m = __ballot(1);
-  const int lane = threadIdx.x & 63;
-  const int leader = __builtin_ctzll(m);
-  int base = 0;
-  if (lane == leader) base = atomicAdd(counter, (int)__popcll(m));
-  base = __shfl(base, leader);
-  return base + (int)__popcll(m & ((1ULL << lane) - 1ULL));
-}
 
 __global__ __launch_bounds__(256) void k_widom_fb_gen(const WidomArgs a, const WidomWork w) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

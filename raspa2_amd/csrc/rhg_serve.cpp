@@ -46,7 +46,7 @@ struct FrameworkSpec {
   // tuned options of this framework's size bucket
   double cell_size = 2.5;
   int64_t chunk = (int64_t)1 << 20;
-  int32_t lanes = 4, waves = 4;
+  int32_t lanes = 4, waves = 2;
   // tabulated energy grids (rhg_server_use_grids): spacing > 0 serves every
   // request on this framework from grids built on first use
   double grid_spacing = 0.0;
@@ -256,7 +256,7 @@ void tuned_options(rhg_server *s, FrameworkSpec &f) {
   f.cell_size = 2.5;
   f.chunk = s->slice;
   f.lanes = 4;
-  f.waves = 4;
+  f.waves = 2;
   for (const Tuning &t : s->tuning)
     if ((int)f.x.size() <= t.max_atoms) {
       f.cell_size = t.cell_size;

@@ -150,3 +150,58 @@ def test_mixed_dense_cells(W, O, R, charged):
     if charged:
         scale = _coulomb_abs_sum(fw, xyz[ok], 0.70, ctx.info()["alpha"], O.units()["coulomb"])
         assert np.max(np.abs(uc[ok] - uc_o[ok]) / scale) <= 1e-6
+
+
+def _threshold_positions(sys_o, t, fw, n_dir, seed):
+    """Positions whose FP64 VDW energy lies within a few 1e-6 (relative) of
+    the overlap threshold 1e7, on both sides: bisection on the oracle along
+    rays from framework atoms (the energy falls steeply with the distance
+    there)."""
+    rng = np.random.default_rng(seed)
+    atoms = fw.xyz[rng.choice(fw.n, n_dir, replace=False)]
+    d = rng.normal(size=(n_dir, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+
+    def excess(r):
+        uv, _ = sys_o.energy_at_many(t, atoms + r[:, None] * d)
+        return uv - 1e7
+
+    lo, hi = np.full(n_dir, 0.3), np.full(n_dir, 3.0)
+    keep = (excess(lo) > 0) & (excess(hi) < 0)
+    lo, hi, atoms, d = lo[keep], hi[keep], atoms[keep], d[keep]
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        above = excess(mid) > 0
+        lo, hi = np.where(above, mid, lo), np.where(above, hi, mid)
+    r0 = 0.5 * (lo + hi)
+    steps = np.array([-3e-7, -1e-7, -3e-8, 0.0, 3e-8, 1e-7, 3e-7])
+    r = (r0[:, None] * (1.0 + steps[None, :])).ravel()
+    return np.repeat(atoms, len(steps), axis=0) + r[:, None] * np.repeat(d, len(steps), axis=0)
+
+
+@pytest.mark.parametrize("charged", [False, True])
+def test_guard_band_sites_recomputed_exactly(W, O, R, charged):
+    """Mixed precision at positions whose energy straddles the overlap
+    threshold within 1e-5: the image-list kernel lists them and the exact
+    FP64 recompute gives flags equal to the oracle's and energies bitwise
+    equal to the FP64 path's (C1 supercell, image lists)."""
+    fw = W.c1_framework()
+    cm = O.CHARGE_EWALD if charged else O.CHARGE_NONE
+    probe = [(27.0, 2.80, 0.70)] if charged else [(148.0, 3.73, 0.0)]
+    sys_o, ctx, to, tg = _pair(W, O, R, fw, cm, R.PREC_MIXED, probe, True, 2.5, True)
+    _, ref, _, rg = _pair(W, O, R, fw, cm, R.PREC_FP64, probe, True, 2.5, True)
+    xyz = _threshold_positions(sys_o, to[0], fw, 96, 21)
+    xyz = np.concatenate([xyz, random_positions(fw.box, 2000, 22)])
+    print(f"guard band: {len(xyz)} positions, charged={charged}", flush=True)
+    uv, uc, ov = ctx.energy_at_positions(xyz, tg[0])
+    uv64, uc64, ov64 = ref.energy_at_positions(xyz, rg[0])
+    uv_o, uc_o = sys_o.energy_at_many(to[0], xyz)
+    flag_o = (uv_o >= 1e7) | ((uc_o >= 1e7) if charged else False)
+    assert np.array_equal(ov, flag_o) and np.array_equal(ov64, flag_o)
+    band = np.abs(uv_o / 1e7 - 1.0) < 1e-5
+    assert band.sum() >= 100 and 0 < flag_o[band].sum() < band.sum()
+    assert same_bits(uv[band], uv64[band])
+    if charged:
+        assert same_bits(uc[band], uc64[band])
+    ctx.close()
+    ref.close()
