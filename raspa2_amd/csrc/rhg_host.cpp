@@ -898,8 +898,8 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
     if (!b.fix || !b.redo || !b.n_redo) return fail(c, RHG_ERR_OOM, "binning scratch");
   }
   bool fresh = c->bufs["pair_counter"].p == nullptr;
-  b.pairs = (unsigned long long *)work(c, "pair_counter", 64);
-  if (fresh && b.pairs) fill((double *)b.pairs, 8, 0.0, c->stream); // all-zero bits
+  b.pairs = (unsigned long long *)work(c, "pair_counter", 3 * kPairLanes * sizeof(unsigned long long));
+  if (fresh && b.pairs) fill((double *)b.pairs, 3 * kPairLanes, 0.0, c->stream); // all-zero bits
   if (!b.cell_of || !b.perm || !b.count || !b.fill || !b.start || !b.tile_off || !b.pairs)
     return fail(c, RHG_ERR_OOM, "binning scratch");
   {
@@ -2793,9 +2793,10 @@ int build_nodes(rhg_ctx *c, rhg_grid *g, GridArgs &a, const float4 *par) {
       if (int rc = ensure_image_list(c)) return rc;
   }
   const int prec = (!a.coulomb && c->precision != RHG_PREC_FP64 && c->use_images && c->cg.img_off) ? 1 : 0;
-  unsigned long long *pairs = culled ? (unsigned long long *)work(c, "grid_pairs", 64) : nullptr;
+  unsigned long long *pairs =
+      culled ? (unsigned long long *)work(c, "grid_pairs", 2 * kPairLanes * sizeof(unsigned long long)) : nullptr;
   if (culled && !pairs) return fail(c, RHG_ERR_OOM, "grid pair counter");
-  if (pairs) fill((double *)pairs, 2, 0.0, c->stream); // all-zero bits
+  if (pairs) fill((double *)pairs, 2 * kPairLanes, 0.0, c->stream); // all-zero bits
   for (int64_t n0 = 0; n0 < nodes; n0 += step) {
     a.node0 = n0;
     a.n_nodes = std::min(step, nodes - n0);
@@ -2827,10 +2828,14 @@ int build_nodes(rhg_ctx *c, rhg_grid *g, GridArgs &a, const float4 *par) {
     launch_grid_culled(a, c->cg, sl, b, par, prec, c->stream);
   }
   HIPC(c, hipGetLastError());
-  unsigned long long cnt[2] = {0, 0};
-  if (pairs) HIPC(c, hipMemcpyAsync(cnt, pairs, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+  std::vector<unsigned long long> cnt(2 * kPairLanes, 0);
+  if (pairs)
+    HIPC(c, hipMemcpyAsync(cnt.data(), pairs, cnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
-  g->exec_pairs = pairs ? (double)(cnt[0] + cnt[1]) : (double)nodes * (double)c->n;
+  unsigned long long total = 0;
+  for (unsigned long long v : cnt) total += v;
+  g->exec_pairs = pairs ? (double)total : (double)nodes * (double)c->n;
   return RHG_OK;
 }
 
@@ -3209,19 +3214,23 @@ int rhg_widom_stats(rhg_ctx *c, double out[12], int32_t reset) {
   if (!c || !out) return RHG_ERR_INVALID;
   hipSetDevice(c->device);
   double st[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  unsigned long long pairs[3] = {0, 0, 0};
+  std::vector<unsigned long long> lanes(3 * kPairLanes, 0);
   DevBuf &pb = c->bufs["pair_counter"];
   if (c->d_acc) HIPC(c, hipMemcpyAsync(st, c->d_acc + 16, sizeof(st), hipMemcpyDeviceToHost, c->stream));
-  if (pb.p) HIPC(c, hipMemcpyAsync(pairs, pb.p, sizeof(pairs), hipMemcpyDeviceToHost, c->stream));
+  if (pb.p)
+    HIPC(c, hipMemcpyAsync(lanes.data(), pb.p, lanes.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           c->stream));
   if (reset) {
     if (c->d_acc) fill(c->d_acc + 16, 9, 0.0, c->stream);
-    if (pb.p) fill((double *)pb.p, 3, 0.0, c->stream); // all-zero bits
+    if (pb.p) fill((double *)pb.p, 3 * kPairLanes, 0.0, c->stream); // all-zero bits
   }
   HIPC(c, hipStreamSynchronize(c->stream));
   for (int i = 0; i < 9; ++i) out[i] = st[i];
-  out[9] = (double)pairs[0];
-  out[10] = (double)pairs[1];
-  out[11] = (double)pairs[2];
+  for (int k = 0; k < 3; ++k) {
+    unsigned long long t = 0;
+    for (int l = 0; l < kPairLanes; ++l) t += lanes[k * kPairLanes + l];
+    out[9 + k] = (double)t;
+  }
   return RHG_OK;
 }
 

@@ -65,6 +65,8 @@ struct CellGrid {
 // sites of one cell
 constexpr int kImgLane = 4;
 
+constexpr int kPairLanes = 256; // counters per kind of the executed-pair tallies
+
 // scratch for binning one site list into cells
 struct BinWork {
   int32_t *cell_of;  // per site
@@ -81,7 +83,10 @@ struct BinWork {
   // exactly (room for every site) and their count (cleared by the binning)
   int2 *redo = nullptr;
   int32_t *n_redo = nullptr;
-  unsigned long long *pairs; // executed pair evaluations [LJ, LJ+Coulomb, Coulomb]
+  // executed pair evaluations by kind (LJ, LJ+Coulomb, Coulomb), each kind
+  // spread over kPairLanes counters (kind * kPairLanes + block % kPairLanes)
+  // so that the per-wave atomics do not serialise on one address
+  unsigned long long *pairs;
   // launch shape of the site kernels: sites per tile of the image-list
   // kernels (64, 128, 192 or 256; 0: 256; the kernel's registers are sized
   // for it) and waves per workgroup (1, 2 or 4; 0: 2)

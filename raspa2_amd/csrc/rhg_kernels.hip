@@ -537,6 +537,30 @@ __device__ __forceinline__ int wave_slot(int32_t *counter) {
   return base + (int)__popcll(m & ((1ULL << lane) - 1ULL));
 }
 
+// Slots for the lanes of a workgroup that want one, with one atomic on
+// counter per workgroup (a single counter shared by a whole grid serialises
+// its atomics); every thread of the workgroup must call it (two barriers),
+// with at most 16 waves per workgroup.  Returns -1 where want is false.
+__device__ __forceinline__ int block_slot(int32_t *counter, bool want) {
+  __shared__ int s_wave[16];
+  __shared__ int s_base;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  const unsigned long long m = __ballot(want);
+  if (lane == 0) s_wave[wid] = (int)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int v = 0; v < nw; ++v) {
+      const int n = s_wave[v];
+      s_wave[v] = t;
+      t += n;
+    }
+    s_base = t ? atomicAdd(counter, t) : 0;
+  }
+  __syncthreads();
+  return want ? s_base + s_wave[wid] + (int)__popcll(m & ((1ULL << lane) - 1ULL)) : -1;
+}
+
 __global__ __launch_bounds__(256) void k_zero_i32(int32_t *__restrict__ a, int32_t *__restrict__ b,
                                                   int n, int32_t *__restrict__ one) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -766,7 +790,7 @@ __global__ __launch_bounds__(256) void k_site_energy_culled(
   if (lane == 0 && b.pairs) {
     const int nl = min(64, b.count[c] - (tile - b.tile_off[c]) * 64);
     // counters by pair kind: 0 LJ, 1 LJ+Coulomb, 2 Coulomb
-    atomicAdd(b.pairs + ((VDW && CC) ? 1 : (VDW ? 0 : 2)),
+    atomicAdd(b.pairs + ((VDW && CC) ? 1 : (VDW ? 0 : 2)) * kPairLanes + (blockIdx.x & (kPairLanes - 1)),
               (unsigned long long)nl * (unsigned long long)(j1 - j0));
   }
   if (live) {
@@ -943,7 +967,7 @@ __global__ __launch_bounds__(256) void k_site_energy_img(
     if (near) b.redo[wave_slot(b.n_redo)] = make_int2(b.start[c] + first + lane + 64 * k, c);
   }
   if (lane == 0 && b.pairs)
-    atomicAdd(b.pairs + ((VDW && CC) ? 1 : (VDW ? 0 : 2)),
+    atomicAdd(b.pairs + ((VDW && CC) ? 1 : (VDW ? 0 : 2)) * kPairLanes + (blockIdx.x & (kPairLanes - 1)),
               (unsigned long long)nl * (unsigned long long)(q1 - q0));
 #pragma unroll
   for (int k = 0; k < NSM; ++k) {
@@ -1581,9 +1605,11 @@ __global__ __launch_bounds__(256) void k_widom_fb_gen(const WidomArgs a, const W
 // HandleFirstBead selection (src/cbmc.c:695-795) and, for molecules with
 // more than one site, the orientation trials of GenerateTrialOrientations-
 // SimpleSphere (src/cbmc.c:1480-1578).
-__global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, const WidomWork w) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= chunk_n(a)) return;
+// Selection of insertion i's first bead; true if it goes on (to the
+// orientation trials, or, for one-site molecules, to the Fourier and
+// finalize stages at position (fx, fy, fz)).
+__device__ __forceinline__ bool fb_select_one(const WidomArgs &a, const WidomWork &w, int64_t i, double &fx,
+                                              double &fy, double &fz) {
   const int k = a.mol.k_fb;
   const double beta = a.beta;
   uint64_t mask = 0;
@@ -1617,7 +1643,7 @@ __global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, cons
   // (the reference would read stale trial energies there)
   if (!any || (a.retrace && (mask & 1ULL))) {
     w.stage[i] = 0;
-    return;
+    return false;
   }
   uint64_t key = 0;
   double usel = 0.0;
@@ -1642,9 +1668,11 @@ __global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, cons
   w.W1[i] = W1;
   if (!a.retrace && W1 < kMinRosenbluth) {
     w.stage[i] = 1;
-    return;
+    return false;
   }
-  const double fx = w.fbx[sel * w.cap + i], fy = w.fby[sel * w.cap + i], fz = w.fbz[sel * w.cap + i];
+  fx = w.fbx[sel * w.cap + i];
+  fy = w.fby[sel * w.cap + i];
+  fz = w.fbz[sel * w.cap + i];
   w.selx[i] = fx;
   w.sely[i] = fy;
   w.selz[i] = fz;
@@ -1663,17 +1691,44 @@ __global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, cons
       w.W[i] = 0.0;
       w.uvdw[i] = 0.0;
       w.ureal[i] = 0.0;
-      return;
+      return false;
     }
-    int s2 = wave_slot(w.n_active2);
-    w.active2[s2] = (int32_t)i;
-    w.cfx[s2] = fx;
-    w.cfy[s2] = fy;
-    w.cfz[s2] = fz;
-    return;
   }
-  const int slot = wave_slot(w.n_active);
-  w.active[slot] = (int32_t)i;
+  return true;
+}
+
+__global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, const WidomWork w) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  const bool on = i < chunk_n(a) && fb_select_one(a, w, i, fx, fy, fz);
+  // the survivors' slots, one atomic per workgroup; orientation trials are
+  // generated by k_widom_or_gen over the compacted survivors (full waves for
+  // the rotations)
+  const bool one = a.mol.n_sites == 1;
+  const int slot = block_slot(one ? w.n_active2 : w.n_active, on);
+  if (!on) return;
+  if (one) {
+    w.active2[slot] = (int32_t)i;
+    w.cfx[slot] = fx;
+    w.cfy[slot] = fy;
+    w.cfz[slot] = fz;
+  } else {
+    w.active[slot] = (int32_t)i;
+  }
+}
+
+// Orientation trials of the insertions whose first bead was selected
+// (src/cbmc.c:1480-1578): one lane per active slot, the same counter
+// sub-stream and draw order as if generated right after the selection.
+__global__ __launch_bounds__(256) void k_widom_or_gen(const WidomArgs a, const WidomWork w) {
+  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= *w.n_active) return;
+  const int64_t i = w.active[slot];
+  const MolDev &m = a.mol;
+  const int k = m.k_fb;
+  const uint64_t key =
+      (a.retrace || a.use_host_trials) ? 0 : stream_key(a.seed, (uint64_t)(chunk_first(a) + i));
+  const double fx = w.selx[i], fy = w.sely[i], fz = w.selz[i];
   const int ko = m.k_or;
   uint64_t c = (uint64_t)(3 * k + 1);
   for (int iu = 0; iu < ko; ++iu) {
@@ -1728,9 +1783,10 @@ __global__ __launch_bounds__(256) void k_widom_fb_select(const WidomArgs a, cons
 }
 
 // Rosen's selection over orientations (src/cbmc.c:1154-1478, 2762-2849)
-__global__ __launch_bounds__(256) void k_widom_or_select(const WidomArgs a, const WidomWork w) {
-  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= *w.n_active) return;
+// Rosen's selection of active slot `slot`; true if the insertion goes on
+// (to the charged beads and the Fourier stage) with orientation *iwalk_out.
+__device__ __forceinline__ bool or_select_one(const WidomArgs &a, const WidomWork &w, int64_t slot,
+                                              int &iwalk_out) {
   const int64_t i = w.active[slot];
   const MolDev &m = a.mol;
   const int ko = m.k_or;
@@ -1770,7 +1826,7 @@ __global__ __launch_bounds__(256) void k_widom_or_select(const WidomArgs a, cons
     w.W[i] = 0.0;
     w.uvdw[i] = 0.0;
     w.ureal[i] = 0.0;
-    return;
+    return false;
   }
   weight = weight / (double)ko;
   auto bf = [&](int iu, double &b) -> bool {
@@ -1788,7 +1844,7 @@ __global__ __launch_bounds__(256) void k_widom_or_select(const WidomArgs a, cons
     w.W[i] = 0.0;
     w.uvdw[i] = 0.0;
     w.ureal[i] = 0.0;
-    return;
+    return false;
   }
   if (!a.retrace && a.pockets.n > 0) {
     // every atom of the grown molecule, in atom order
@@ -1809,7 +1865,7 @@ __global__ __launch_bounds__(256) void k_widom_or_select(const WidomArgs a, cons
       w.W[i] = 0.0;
       w.uvdw[i] = 0.0;
       w.ureal[i] = 0.0;
-      return;
+      return false;
     }
   }
   double ev, ea;
@@ -1818,7 +1874,21 @@ __global__ __launch_bounds__(256) void k_widom_or_select(const WidomArgs a, cons
   if (ads) w.uadsv[i] = w.uadsv[i] + ea;
   w.uvdw[i] = w.uvdw[i] + ev;
   w.W[i] = Wn;
-  const int s2 = wave_slot(w.n_active2);
+  iwalk_out = iwalk;
+  return true;
+}
+
+__global__ __launch_bounds__(256) void k_widom_or_select(const WidomArgs a, const WidomWork w) {
+  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int iwalk = 0;
+  const bool on = slot < *w.n_active && or_select_one(a, w, slot, iwalk);
+  // slots of the charged-bead list, one atomic per workgroup
+  const int s2 = block_slot(w.n_active2, on);
+  if (!on) return;
+  const int64_t i = w.active[slot];
+  const MolDev &m = a.mol;
+  const int ko = m.k_or;
+  const int64_t cap = w.cap;
   w.active2[s2] = (int32_t)i;
   w.cfx[m.start * cap + s2] = w.selx[i];
   w.cfy[m.start * cap + s2] = w.sely[i];
@@ -1925,6 +1995,8 @@ void launch_widom_fb_gen(const WidomArgs &a, const WidomWork &w, hipStream_t st)
 }
 void launch_widom_fb_select(const WidomArgs &a, const WidomWork &w, hipStream_t st) {
   if (a.n > 0) hipLaunchKernelGGL(k_widom_fb_select, dim3(nblk(a.n, 256)), dim3(256), 0, st, a, w);
+  if (a.n > 0 && a.mol.n_sites > 1)
+    hipLaunchKernelGGL(k_widom_or_gen, dim3(nblk(a.n, 256)), dim3(256), 0, st, a, w);
 }
 void launch_widom_or_select(const WidomArgs &a, const WidomWork &w, hipStream_t st) {
   if (a.n > 0) hipLaunchKernelGGL(k_widom_or_select, dim3(nblk(a.n, 256)), dim3(256), 0, st, a, w);
@@ -2503,7 +2575,8 @@ __global__ __launch_bounds__(256) void k_grid_culled(const GridArgs g, const Cel
     // form, neighbour atoms otherwise
     const int nl = min(64, b.count[c] - (tile - b.tile_off[c]) * 64);
     const int len = PREC == 1 ? cg.img_off[c + 1] - cg.img_off[c] : cg.nb_off[c + 1] - cg.nb_off[c];
-    atomicAdd(b.pairs + (PREC == 2 ? 1 : 0), (unsigned long long)nl * (unsigned long long)len);
+    atomicAdd(b.pairs + (PREC == 2 ? 1 : 0) * kPairLanes + (blockIdx.x & (kPairLanes - 1)),
+              (unsigned long long)nl * (unsigned long long)len);
   }
   if (live) store_node(g, g.node0 + site, o[0], o[1], o[2], o[3], o[4], o[5], o[6], o[7], cap);
 }
