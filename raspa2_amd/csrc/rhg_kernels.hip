@@ -571,28 +571,31 @@ __global__ __launch_bounds__(256) void k_zero_i32(int32_t *__restrict__ a, int32
   if (i == 0 && one) *one = 0;
 }
 
-// Lanes of a wave that bin into the same cell as their left neighbour form a
-// run (sites of one insertion are consecutive and close together); the run's
-// first lane makes one atomic for the whole run and shares the result.
-struct RunInfo {
-  int head;   // lane that leads this lane's run
-  int length; // lanes in the run
+// Lanes of a wave binned into the same cell form a group (in any lane
+// order): the group's lowest lane leads it and makes one atomic for all of
+// its lanes (atomics on the per-cell counters, not the arithmetic, set the
+// pace of the binning).  The loop takes one distinct cell per pass, at most
+// 16 passes; lanes left after that (waves of scattered sites) are groups of
+// their own.
+struct CellGroup {
+  unsigned long long mask; // lanes of this lane's group
+  int lead;                // the group's lowest lane
 };
 
-__device__ __forceinline__ RunInfo wave_runs(int c, bool valid) {
+__device__ __forceinline__ CellGroup cell_group(int c, bool valid) {
   const int lane = threadIdx.x & 63;
-  const int left = __shfl_up(c, 1);
-  const bool head = valid && (lane == 0 || left != c);
-  const unsigned long long heads = __ballot(head);
-  const unsigned long long live = __ballot(valid);
-  const unsigned long long upto = (lane == 63) ? ~0ULL : ((2ULL << lane) - 1ULL);
-  RunInfo r;
-  r.head = 63 - __builtin_clzll(heads & upto | 1ULL);
-  const unsigned long long after = heads & ~upto;
-  // runs end at the next head or at the first invalid lane
-  const unsigned long long stop = after | (~live & ~upto);
-  const int end = stop ? __builtin_ctzll(stop) : 64;
-  r.length = end - r.head;
+  CellGroup r{1ULL << lane, lane};
+  unsigned long long todo = __ballot(valid);
+  for (int pass = 0; todo && pass < 16; ++pass) {
+    const int l = __builtin_ctzll(todo);
+    const int cl = __builtin_amdgcn_readlane(c, l);
+    const unsigned long long m = __ballot(valid && c == cl);
+    if (valid && c == cl) {
+      r.mask = m;
+      r.lead = l;
+    }
+    todo &= ~m;
+  }
   return r;
 }
 
@@ -602,10 +605,10 @@ __global__ __launch_bounds__(256) void k_bin_count(const Cell cell, const CellGr
   const bool valid = i < site_count(s);
   const int64_t p = valid ? site_phys(s, i) : 0;
   const int c = valid ? cell_index(cell, g, s.x[p], s.y[p], s.z[p]) : -1;
-  const RunInfo r = wave_runs(c, valid);
+  const CellGroup r = cell_group(c, valid);
   if (!valid) return;
   b.cell_of[i] = c;
-  if ((int)(threadIdx.x & 63) == r.head) atomicAdd(&b.count[c], r.length);
+  if ((int)(threadIdx.x & 63) == r.lead) atomicAdd(&b.count[c], (int)__popcll(r.mask));
 }
 
 template <int BCK>
@@ -628,13 +631,13 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const Cell cell, const Cell
     f.y = (float)oy;
     f.z = (float)oz;
   }
-  const RunInfo r = wave_runs(c, valid);
+  const CellGroup r = cell_group(c, valid);
   int base = 0;
   const int lane = threadIdx.x & 63;
-  if (valid && lane == r.head) base = b.start[c] + atomicAdd(&b.fill[c], r.length);
-  base = __shfl(base, r.head);
+  if (valid && lane == r.lead) base = b.start[c] + atomicAdd(&b.fill[c], (int)__popcll(r.mask));
+  base = __shfl(base, r.lead);
   if (!valid) return;
-  const int pos = base + (lane - r.head);
+  const int pos = base + (int)__popcll(r.mask & ((1ULL << lane) - 1ULL));
   if (b.fix)
     b.fix[pos] = f;
   else
