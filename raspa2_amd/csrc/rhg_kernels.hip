@@ -804,6 +804,15 @@ __global__ __launch_bounds__(256) void k_site_energy_culled(
   }
 }
 
+// A wave-uniform value in a VGPR (a v_mov from its SGPR): an instruction
+// may read only one SGPR operand, so a value used by many instructions next
+// to another SGPR operand is copied once.
+__device__ __forceinline__ float vgpr_copy(float x) {
+  float v;
+  asm("v_mov_b32 %0, %1" : "=v"(v) : "s"(x));
+  return v;
+}
+
 // Pair sums of NS sites per lane over a cell's image list (FP32 terms, four
 // summed in FP32 and the groups in FP64).  Each wave-uniform image record and
 // its parameters are loaded once (scalar loads into SGPR operands) and used
@@ -824,18 +833,15 @@ __device__ __forceinline__ void img_sums(const float4 *__restrict__ img, const f
                                          double (&uv)[NSM], double (&uc)[NSM]) {
   // one pair term in FP32 (VDW 1, CC: zero beyond the cut-off; VDW 2: added
   // to fv); the LJ form keeps one SGPR operand per instruction
-  auto term = [&](const float4 e, const float4 pr, const float4 o, float &fv, float &fc) {
+  auto term = [&](const float4 e, const float4 pr, const float c6v, const float4 o, float &fv, float &fc) {
     const float dx = o.x - e.x, dy = o.y - e.y, dz = o.z - e.z;
     const float rr = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     if (VDW == 2) {
       const float ir = __builtin_amdgcn_rcpf(fminf(rr, rc2vf));
       const float ir3 = (ir * ir) * ir;
-      float y = pr.x * ir3;
-      {
-#pragma clang fp contract(off)
-        y = y - pr.y;
-      }
-      fv = fmaf(ir3, y, fv);
+      // c12 ir3 - c6 as one FMA: c6 is moved into a VGPR once per entry (the
+      // caller's c6v), so that the FMA reads one SGPR operand
+      fv = fmaf(ir3, fmaf(pr.x, ir3, -c6v), fv);
     } else if (VDW) {
       const float ir = __builtin_amdgcn_rcpf(rr);
       const float ir3 = (ir * ir) * ir;
@@ -865,10 +871,12 @@ __device__ __forceinline__ void img_sums(const float4 *__restrict__ img, const f
   int q = q0;
   for (; q + 4 <= q1; q += 4) {
     float4 e[4], pr[4];
+    float c6v[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       e[u] = img[q + u];
       pr[u] = par[__float_as_int(e[u].w)];
+      c6v[u] = vgpr_copy(pr[u].y);
     }
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
@@ -876,11 +884,11 @@ __device__ __forceinline__ void img_sums(const float4 *__restrict__ img, const f
       if (VDW == 2) {
         // one FP32 accumulator through the group, in list order
 #pragma unroll
-        for (int u = 0; u < 4; ++u) term(e[u], pr[u], so[k], v[0], c[u]);
+        for (int u = 0; u < 4; ++u) term(e[u], pr[u], c6v[u], so[k], v[0], c[u]);
         uv[k] += (double)v[0];
       } else {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) term(e[u], pr[u], so[k], v[u], c[u]);
+        for (int u = 0; u < 4; ++u) term(e[u], pr[u], c6v[u], so[k], v[u], c[u]);
         if (VDW) uv[k] += (double)((v[0] + v[1]) + (v[2] + v[3]));
       }
       if (CC) uc[k] = fma(cqa, (double)((c[0] + c[1]) + (c[2] + c[3])), uc[k]);
@@ -889,10 +897,11 @@ __device__ __forceinline__ void img_sums(const float4 *__restrict__ img, const f
   for (; q < q1; ++q) {
     const float4 e = img[q];
     const float4 pr = par[__float_as_int(e.w)];
+    const float c6v = vgpr_copy(pr.y);
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
       float v = 0.f, c = 0.f;
-      term(e, pr, so[k], v, c);
+      term(e, pr, c6v, so[k], v, c);
       if (VDW) uv[k] += (double)v;
       if (CC) uc[k] = fma(cqa, (double)c, uc[k]);
     }
