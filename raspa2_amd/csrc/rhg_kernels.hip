@@ -1362,27 +1362,32 @@ __global__ __launch_bounds__(128) void k_fourier_part(const FourierArgs a, doubl
   cx ey[NC];
 #pragma unroll
   for (int t = 0; t < nc; ++t) ey[t] = cx{1.0, 0.0};
+  // one k-vector's contribution: host += K Re(S_fw* dS), ads += K (|dS|^2 or
+  // the adsorbate form)
+  auto add_k = [&](int idx, double sre, double sim) {
+    const double2 ks = e.ks[idx];
+    host = fma(ks.y, sim, fma(ks.x, sre, host));
+    ads = fma(e.kfactor[idx], ads_term<ADS>(a.sads, idx, sre, sim), ads);
+  };
   for (int jj = 0; jj <= e.ky; ++jj) {
     if (jj > 0) {
 #pragma unroll
       for (int t = 0; t < nc; ++t) ey[t] = (jj == 1) ? e1y[t] : cmul(ey[t], e1y[t]);
     }
-    for (int sy = 0; sy < 2; ++sy) {
-      if (jj == 0 && sy == 1) break;
-      const int jv = sy ? -jj : jj;
-      const int row = ii * ny + (jv + e.ky);
+    if (jj == 0) {
+      const int row = ii * ny + e.ky;
       const int kzm = e.row_kz[row];
       if (kzm < 0) continue;
       const int off = e.row_off[row];
-      // charge-weighted row phases q e^{i(kx x + ky y)} of each site
+      // charge-weighted row phases q e^{i kx x} of each site (ky = 0)
       cx xy[NC], ez[NC];
 #pragma unroll
       for (int t = 0; t < nc; ++t) {
-        const cx r = cmul(ex[t], sy ? conjc(ey[t]) : ey[t]);
+        const cx r = cmul(ex[t], ey[t]);
         xy[t] = cx{qs[t] * r.re, qs[t] * r.im};
         ez[t] = cx{1.0, 0.0};
       }
-      const bool origin_row = (ii == 0 && jv == 0);
+      const bool origin_row = (ii == 0);
       // k_z = 0 (absent in the origin row, where k = 0 is excluded)
       if (!origin_row) {
         double sre = 0.0, sim = 0.0;
@@ -1391,15 +1396,11 @@ __global__ __launch_bounds__(128) void k_fourier_part(const FourierArgs a, doubl
           sre += xy[t].re;
           sim += xy[t].im;
         }
-        const int idx = off + kzm;
-        const double2 ks = e.ks[idx];
-        host = fma(ks.y, sim, fma(ks.x, sre, host));
-        ads = fma(e.kfactor[idx], ads_term<ADS>(a.sads, idx, sre, sim), ads);
+        add_k(off + kzm, sre, sim);
       }
       // k_z = +kk and -kk share the four products Re(xy) Re(ez), Im(xy) Im(ez),
       // Re(xy) Im(ez), Im(xy) Re(ez); in the origin row the +k_z entries sit
-      // one place lower (k = 0 is not stored).  Two k_z steps per iteration
-      // keep the phase recurrence in place (no register copies).
+      // one place lower (k = 0 is not stored)
       const int ip = off + kzm - (origin_row ? 1 : 0), im = off + kzm;
 #pragma unroll 2
       for (int kk = 1; kk <= kzm; ++kk) {
@@ -1413,19 +1414,56 @@ __global__ __launch_bounds__(128) void k_fourier_part(const FourierArgs a, doubl
           a3 = fma(xy[t].re, ez[t].im, a3);
           a4 = fma(xy[t].im, ez[t].re, a4);
         }
-        {
-          const double sre = a1 - a2, sim = a3 + a4; // +k_z
-          const double2 ks = e.ks[ip + kk];
-          host = fma(ks.y, sim, fma(ks.x, sre, host));
-          ads = fma(e.kfactor[ip + kk], ads_term<ADS>(a.sads, ip + kk, sre, sim), ads);
-        }
-        {
-          const double sre = a1 + a2, sim = a4 - a3; // -k_z
-          const double2 ks = e.ks[im - kk];
-          host = fma(ks.y, sim, fma(ks.x, sre, host));
-          ads = fma(e.kfactor[im - kk], ads_term<ADS>(a.sads, im - kk, sre, sim), ads);
-        }
+        add_k(ip + kk, a1 - a2, a3 + a4); // +k_z
+        add_k(im - kk, a1 + a2, a4 - a3); // -k_z
       }
+      continue;
+    }
+    // the rows +jj and -jj have the same k_z range (the cut-off depends on
+    // jj^2) and share the k_z phases: four k-vectors per k_z step
+    const int rp = ii * ny + (jj + e.ky), rm = ii * ny + (e.ky - jj);
+    const int kzm = e.row_kz[rp];
+    if (kzm < 0) continue;
+    const int offp = e.row_off[rp] + kzm, offm = e.row_off[rm] + kzm;
+    cx xp[NC], xm[NC], ez[NC];
+#pragma unroll
+    for (int t = 0; t < nc; ++t) {
+      const cx r = cmul(ex[t], ey[t]), rc = cmul(ex[t], conjc(ey[t]));
+      xp[t] = cx{qs[t] * r.re, qs[t] * r.im};
+      xm[t] = cx{qs[t] * rc.re, qs[t] * rc.im};
+      ez[t] = cx{1.0, 0.0};
+    }
+    {
+      double spr = 0.0, spi = 0.0, smr = 0.0, smi = 0.0;
+#pragma unroll
+      for (int t = 0; t < nc; ++t) {
+        spr += xp[t].re;
+        spi += xp[t].im;
+        smr += xm[t].re;
+        smi += xm[t].im;
+      }
+      add_k(offp, spr, spi);
+      add_k(offm, smr, smi);
+    }
+    for (int kk = 1; kk <= kzm; ++kk) {
+      double a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0, b4 = 0.0;
+#pragma unroll
+      for (int t = 0; t < nc; ++t) {
+        ez[t] = cx{fma(ez[t].re, e1z[t].re, -ez[t].im * e1z[t].im),
+                   fma(ez[t].im, e1z[t].re, ez[t].re * e1z[t].im)};
+        a1 = fma(xp[t].re, ez[t].re, a1);
+        a2 = fma(xp[t].im, ez[t].im, a2);
+        a3 = fma(xp[t].re, ez[t].im, a3);
+        a4 = fma(xp[t].im, ez[t].re, a4);
+        b1 = fma(xm[t].re, ez[t].re, b1);
+        b2 = fma(xm[t].im, ez[t].im, b2);
+        b3 = fma(xm[t].re, ez[t].im, b3);
+        b4 = fma(xm[t].im, ez[t].re, b4);
+      }
+      add_k(offp + kk, a1 - a2, a3 + a4); // (+jj, +k_z)
+      add_k(offp - kk, a1 + a2, a4 - a3); // (+jj, -k_z)
+      add_k(offm + kk, b1 - b2, b3 + b4); // (-jj, +k_z)
+      add_k(offm - kk, b1 + b2, b4 - b3); // (-jj, -k_z)
     }
   }
   if (live) part[(int64_t)ii * a.stride + m] = make_double2(host, ads);
