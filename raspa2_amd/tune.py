@@ -1,7 +1,7 @@
 """Autotuner of the screening service: for each framework-size bucket, the
 cell-list cell size, the chunk (insertions per pipeline pass) and the launch
-shape of the culled site kernels (sites per lane of the image-list tiles,
-waves per workgroup) that give the highest Widom insertion rate on this GPU,
+shape of the culled site kernels (sites per lane of the LJ-only image-list
+tiles, waves per workgroup) that give the highest Widom insertion rate on this GPU,
 written to a JSON table that the service loads (screening.load_tuning).
 
     python -m raspa2_amd.tune [--out raspa2_amd/tuning.json]
@@ -21,7 +21,7 @@ from .screening import TUNING_PATH
 
 BUCKETS = (1000, 2000, 4000, 8000)
 CELLS = (2.0, 2.5, 3.0, 3.5)
-TILES = ((4, 4), (4, 2), (2, 4), (2, 2), (1, 4))  # (sites per lane, waves per workgroup)
+TILES = ((4, 2), (4, 4), (2, 2), (4, 1), (2, 4))  # (sites per lane, waves per workgroup)
 CHUNKS = (1 << 16, 1 << 17, 1 << 18)
 
 
@@ -71,7 +71,7 @@ def tune(n=1 << 18, per_bucket=3, log=print):
             log(f"bucket <= {b} atoms: cell {cell} chunk {chunk} tiles {tiles}: {r:.4g} ins/s")
             return r
 
-        best = {"cell": 2.5, "chunk": 1 << 18, "tiles": (4, 4)}
+        best = {"cell": 2.5, "chunk": 1 << 18, "tiles": (4, 2)}
         best_rate = score(best["cell"], best["chunk"], best["tiles"])
         for key, values in (("cell", CELLS), ("tiles", TILES), ("chunk", CHUNKS)):
             for v in values:
@@ -94,7 +94,7 @@ def main():
     ap.add_argument("--insertions", type=int, default=1 << 18)
     args = ap.parse_args()
     rows = tune(args.insertions)
-    doc = {"what": "per framework-size bucket: cell-list cell size [A], chunk, sites per lane and waves per "
+    doc = {"what": "per framework-size bucket: cell-list cell size [A], chunk, sites per lane (LJ-only lists) and waves per "
                    "workgroup of the site kernels; mixed precision, Ewald 1e-6, measured Widom insertion rate",
            "command": "python -m raspa2_amd.tune", "insertions_per_measurement": args.insertions,
            "buckets": rows}
