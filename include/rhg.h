@@ -544,12 +544,14 @@ int rhg_server_add_framework(rhg_server *s, int64_t id, const rhg_framework *fw,
  * 197-200): a VDW grid per probe pseudo-atom and, for charged probes with
  * Ewald, the Ewald-Coulomb grid, built on first use over the framework's box
  * at `spacing` [A] (FP32 nodes, MakeGrid's layout), kept resident with the
- * framework, counted in admission and evicted with it.  spacing = 0 returns
- * to the direct sums. */
+ * framework, counted in admission and evicted with it; grids a request's
+ * probe does not use are freed first when its footprint does not fit.
+ * spacing = 0 returns to the direct sums. */
 int rhg_server_use_grids(rhg_server *s, int64_t id, double spacing);
 /* Per size bucket (frameworks of up to max_atoms[i] atoms): cell-list cell
- * size, chunk, and the site kernels' sites per lane (1..4) and waves per
- * workgroup (1, 2, 4) (nullable: 4 and 4), e.g. from an autotuner. */
+ * size, chunk, and the site kernels' launch shape as rhg_set_tiles takes it:
+ * sites per lane of the LJ-only lists (1..4) and waves per workgroup (1, 2,
+ * 4) (nullable: 4 and 2), e.g. from an autotuner. */
 int rhg_server_set_tuning(rhg_server *s, int32_t n, const int32_t *max_atoms, const double *cell_size,
                           const int64_t *chunk, const int32_t *sites_per_lane, const int32_t *waves_per_block);
 /* Thread-safe; may be called while the request loop runs. */

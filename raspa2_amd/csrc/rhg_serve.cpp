@@ -732,7 +732,7 @@ int rhg_server_set_tuning(rhg_server *s, int32_t n, const int32_t *max_atoms, co
   std::lock_guard<std::mutex> lk(s->mu);
   std::vector<Tuning> rows;
   for (int i = 0; i < n; ++i) {
-    const int32_t lanes = sites_per_lane ? sites_per_lane[i] : 4, waves = waves_per_block ? waves_per_block[i] : 4;
+    const int32_t lanes = sites_per_lane ? sites_per_lane[i] : 4, waves = waves_per_block ? waves_per_block[i] : 2;
     if (!(cell_size[i] > 0.0) || chunk[i] < 1 || chunk[i] > ((int64_t)1 << 24) || lanes < 1 || lanes > 4 ||
         (waves != 1 && waves != 2 && waves != 4))
       return sfail(s, RHG_ERR_INVALID, "bad tuning row");

@@ -1,4 +1,5 @@
-/* The RASPA2-side binding of INTEGRATION.md sections 1-3, compiled against
+/* The RASPA2-side binding a maintainer adds to RASPA2 (context creation,
+ * energies at a position, Widom moves), compiled against
  * include/rhg.h and linked with librhg.so.  RASPA2's globals are replaced by
  * a small host model with the same roles (framework atoms with pseudo-atom
  * types, the PotentialType / PotentialParms pair tables, the probe's
@@ -74,7 +75,7 @@ static int RhgCheckMixing(void)
   return bad;
 }
 
-/* INTEGRATION.md section 1 */
+/* context creation, once per system after ReadInput (src/input.c:8482-8488) */
 static int RhgCreate(int device)
 {
   int n = NumberOfAtoms, i, t;
@@ -103,8 +104,8 @@ static int RhgCreate(int device)
   return rhg_set_precision(RhgContext, RHG_PREC_FP64);
 }
 
-/* INTEGRATION.md section 2: CalculateFrameworkVDWEnergyAtPosition and
- * CalculateFrameworkChargeEnergyAtPosition */
+/* CalculateFrameworkVDWEnergyAtPosition and
+ * CalculateFrameworkChargeEnergyAtPosition (src/framework_energy.c:60, 176) */
 static double CalculateFrameworkVDWEnergyAtPosition(POINT posA, int typeA, double scaling)
 {
   double p[3] = {posA.x, posA.y, posA.z}, u = 0.0;
@@ -122,7 +123,7 @@ static void CalculateFrameworkChargeEnergyAtPosition(POINT posA, int typeA, doub
                                             UChargeBondDipole);
 }
 
-/* a Widom-only component's insertions */
+/* a Widom-only component's insertions (src/mc_moves.c:7844) */
 static int RhgWidom(uint64_t seed, int64_t n_total, int steps_per_cycle)
 {
   int i, k;

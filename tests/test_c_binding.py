@@ -1,5 +1,6 @@
-"""The RASPA2-side C binding of INTEGRATION.md (sections 1-3) as a compiled C
-program against include/rhg.h, linked with librhg.so: it compiles with
+"""The RASPA2-side C binding (context creation, energies at a position,
+Widom moves) as a compiled C program against include/rhg.h, linked with
+librhg.so: it compiles with
 -Wall -Werror as C99, refuses force fields whose pair table is not a
 Lorentz-Berthelot mix, and (on the GPU) gives the energies and Widom block
 sums of the Python binding of the same library."""

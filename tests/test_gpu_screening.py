@@ -1,5 +1,5 @@
-"""The screening configurations at full size on the GPU (BASELINE configs 3
-and 4): every one of the 256 screening frameworks with its real trial count
+"""The screening configurations at full size on the GPU (batched screening
+of the 256-framework catalogue, and live serving): every one of the 256 screening frameworks with its real trial count
 through the service, requests submitted from a second thread while the
 request loop runs (C4, with the N2-like probe), and a grid-served framework
 under a budget that forces evictions.  Each sampled request's estimators are
