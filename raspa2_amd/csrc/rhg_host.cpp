@@ -100,7 +100,7 @@ struct DevBuf {
   // buffers outgrown while work may still be queued on them; freed with the
   // owner (hipFree waits for the whole device, which would stall the other
   // streams of a server)
-  std::vector<void *> retired;
+  std::vector<std::pair<void *, size_t>> retired;
 };
 
 struct PseudoAtom {
@@ -132,7 +132,87 @@ struct Arena {
   std::vector<std::pair<char *, size_t>> small; // carved blocks
   std::vector<std::pair<char *, size_t>> large; // one buffer each
   size_t used = 0;                              // bytes carved from the last small block
+  rhg_pool *pool = nullptr;                     // where blocks come from (null: hipMalloc)
 };
+
+} // namespace
+
+// Device memory shared by the contexts of a screening server: slabs taken
+// with hipMalloc (one call per slab instead of one per table: hipMalloc and
+// hipFree serialise across the server's threads), carved at the tail or
+// reused from blocks that evicted contexts gave back (smallest block that
+// fits; a remainder of 2 MiB or more goes back to the free blocks).
+struct rhg_pool {
+  int device = 0;
+  size_t slab = 0;
+  std::mutex mu;
+  std::vector<std::pair<char *, size_t>> slabs;
+  size_t tail = 0; // bytes carved from the last slab
+  std::multimap<size_t, char *> free_blocks;
+};
+
+rhg_pool *rhg_pool_create(int device, size_t slab_bytes) {
+  rhg_pool *p = new rhg_pool();
+  p->device = device;
+  p->slab = std::max<size_t>(slab_bytes, (size_t)64 << 20);
+  return p;
+}
+
+void rhg_pool_destroy(rhg_pool *p) {
+  if (!p) return;
+  hipSetDevice(p->device);
+  for (auto &sl : p->slabs) hipFree(sl.first);
+  delete p;
+}
+
+namespace {
+
+constexpr size_t kPoolSplit = (size_t)2 << 20;
+
+hipError_t pool_alloc(rhg_pool *P, void **out, size_t bytes) {
+  bytes = (bytes + 255) & ~(size_t)255;
+  std::lock_guard<std::mutex> lk(P->mu);
+  auto it = P->free_blocks.lower_bound(bytes);
+  if (it != P->free_blocks.end()) {
+    const size_t have = it->first;
+    char *b = it->second;
+    P->free_blocks.erase(it);
+    if (have - bytes >= kPoolSplit) P->free_blocks.emplace(have - bytes, b + bytes);
+    *out = b;
+    return hipSuccess;
+  }
+  if (P->slabs.empty() || P->tail + bytes > P->slabs.back().second) {
+    if (!P->slabs.empty() && P->slabs.back().second - P->tail >= kPoolSplit)
+      P->free_blocks.emplace(P->slabs.back().second - P->tail, P->slabs.back().first + P->tail);
+    void *sl = nullptr;
+    const size_t n = std::max(P->slab, bytes);
+    hipError_t e = hipMalloc(&sl, n);
+    if (e != hipSuccess) return e;
+    P->slabs.push_back({(char *)sl, n});
+    P->tail = 0;
+  }
+  *out = P->slabs.back().first + P->tail;
+  P->tail += bytes;
+  return hipSuccess;
+}
+
+void pool_free(rhg_pool *P, void *p, size_t bytes) {
+  if (!p) return;
+  bytes = (bytes + 255) & ~(size_t)255;
+  std::lock_guard<std::mutex> lk(P->mu);
+  P->free_blocks.emplace(bytes, (char *)p);
+}
+
+hipError_t dev_malloc(rhg_pool *P, void **out, size_t bytes) {
+  return P ? pool_alloc(P, out, bytes) : hipMalloc(out, bytes);
+}
+
+void dev_free(rhg_pool *P, void *p, size_t bytes) {
+  if (P)
+    pool_free(P, p, bytes);
+  else if (p)
+    hipFree(p);
+}
 
 } // namespace
 
@@ -300,7 +380,7 @@ int dalloc(rhg_ctx *c, void **p, size_t bytes) {
   bytes = (std::max<size_t>(bytes, 8) + 255) & ~(size_t)255;
   Arena &A = c->arena;
   if (bytes >= kArenaBlock / 2) {
-    hipError_t e = hipMalloc(p, bytes);
+    hipError_t e = dev_malloc(A.pool, p, bytes);
     if (e != hipSuccess) return fail(c, RHG_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
     A.large.push_back({(char *)*p, bytes});
     c->footprint += bytes;
@@ -308,7 +388,7 @@ int dalloc(rhg_ctx *c, void **p, size_t bytes) {
   }
   if (A.small.empty() || A.used + bytes > A.small.back().second) {
     void *blk = nullptr;
-    hipError_t e = hipMalloc(&blk, kArenaBlock);
+    hipError_t e = dev_malloc(A.pool, &blk, kArenaBlock);
     if (e != hipSuccess) return fail(c, RHG_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
     A.small.push_back({(char *)blk, kArenaBlock});
     A.used = 0;
@@ -326,7 +406,7 @@ void dfree(rhg_ctx *c, void *p) {
   Arena &A = c->arena;
   for (size_t i = 0; i < A.large.size(); ++i)
     if (A.large[i].first == (char *)p) {
-      hipFree(p);
+      dev_free(A.pool, p, A.large[i].second);
       c->footprint -= A.large[i].second;
       A.large.erase(A.large.begin() + (long)i);
       return;
@@ -334,9 +414,11 @@ void dfree(rhg_ctx *c, void *p) {
 }
 
 void arena_release(rhg_ctx *c) {
-  for (auto &b : c->arena.small) hipFree(b.first);
-  for (auto &b : c->arena.large) hipFree(b.first);
+  rhg_pool *P = c->arena.pool;
+  for (auto &b : c->arena.small) dev_free(P, b.first, b.second);
+  for (auto &b : c->arena.large) dev_free(P, b.first, b.second);
   c->arena = Arena();
+  c->arena.pool = P;
 }
 
 // grow-only named work buffer, in the attached workspace if there is one
@@ -349,7 +431,7 @@ void *work(rhg_ctx *c, const char *name, size_t bytes) {
   if (b.p) {
     // regrowth: leave headroom so that a slowly growing size settles
     want = std::max(want, b.bytes + b.bytes / 2);
-    b.retired.push_back(b.p);
+    b.retired.push_back({b.p, b.bytes});
   }
   b.p = nullptr;
   b.bytes = 0;
@@ -357,7 +439,8 @@ void *work(rhg_ctx *c, const char *name, size_t bytes) {
     c->ws->gen++;
   else
     c->gen++;
-  if (hipMalloc(&b.p, want) != hipSuccess) {
+  // the context's own buffers come from its pool, a workspace's from hipMalloc
+  if (dev_malloc(shared ? nullptr : c->arena.pool, &b.p, want) != hipSuccess) {
     b.p = nullptr;
     return nullptr;
   }
@@ -1012,7 +1095,7 @@ struct CtxDeleter {
 // rhg_create; with `stream` given the context borrows it instead of creating
 // its own, and with `ws` given its set-up scratch lives in that workspace
 int create_impl(const rhg_framework *fw, const rhg_cell *cellp, const rhg_forcefield *ff, int device,
-                hipStream_t stream, rhg_workspace *ws, rhg_ctx **out) {
+                hipStream_t stream, rhg_workspace *ws, rhg_pool *pool, rhg_ctx **out) {
   if (!fw || !cellp || !ff || !out) return RHG_ERR_INVALID;
   *out = nullptr;
   if (fw->n_atoms < 0 || (fw->n_atoms > 0 && (!fw->x || !fw->y || !fw->z || !fw->eps_k || !fw->sigma)))
@@ -1020,7 +1103,7 @@ int create_impl(const rhg_framework *fw, const rhg_cell *cellp, const rhg_forcef
   if (cellp->boundary != RHG_BC_RECTANGULAR && cellp->boundary != RHG_BC_TRICLINIC) return RHG_ERR_INVALID;
   if (ff->cutoff_vdw <= 0.0 || ff->temperature <= 0.0) return RHG_ERR_INVALID;
   if (ff->charge_method != RHG_CHARGE_NONE && ff->charge_method != RHG_CHARGE_EWALD) return RHG_ERR_UNSUPPORTED;
-  if (ws && ws->device != device) return RHG_ERR_INVALID;
+  if ((ws && ws->device != device) || (pool && pool->device != device)) return RHG_ERR_INVALID;
   if (hipSetDevice(device) != hipSuccess) return RHG_ERR_HIP;
   noalloc_scratch_init(device);
   hipStream_t st = stream;
@@ -1029,6 +1112,7 @@ int create_impl(const rhg_framework *fw, const rhg_cell *cellp, const rhg_forcef
   c->device = device;
   c->stream = st;
   c->own_stream = stream == nullptr;
+  c->arena.pool = pool;
   if (ws) {
     // the workspace's previous user finishes first (set-up runs on this stream)
     if (ws->owner && ws->owner != c.get()) {
@@ -1099,8 +1183,8 @@ int create_impl(const rhg_framework *fw, const rhg_cell *cellp, const rhg_forcef
 } // namespace
 
 int rhg_create_on_stream(const rhg_framework *fw, const rhg_cell *cell, const rhg_forcefield *ff, int device,
-                         hipStream_t stream, rhg_workspace *ws, rhg_ctx **out) {
-  return create_impl(fw, cell, ff, device, stream, ws, out);
+                         hipStream_t stream, rhg_workspace *ws, rhg_pool *pool, rhg_ctx **out) {
+  return create_impl(fw, cell, ff, device, stream, ws, pool, out);
 }
 
 int rhg_use_stream(rhg_ctx *c, hipStream_t stream) {
@@ -1118,7 +1202,7 @@ extern "C" {
 
 int rhg_create(const rhg_framework *fw, const rhg_cell *cellp, const rhg_forcefield *ff, int device,
                rhg_ctx **out) {
-  return create_impl(fw, cellp, ff, device, nullptr, nullptr, out);
+  return create_impl(fw, cellp, ff, device, nullptr, nullptr, nullptr, out);
 }
 
 void rhg_destroy(rhg_ctx *c) {
@@ -1127,8 +1211,8 @@ void rhg_destroy(rhg_ctx *c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->ws && c->ws->owner == c) c->ws->owner = nullptr;
   for (auto &b : c->bufs) {
-    hipFree(b.second.p);
-    for (void *r : b.second.retired) hipFree(r);
+    dev_free(c->arena.pool, b.second.p, b.second.bytes);
+    for (auto &r : b.second.retired) dev_free(c->arena.pool, r.first, r.second);
   }
   arena_release(c); // every table allocated with dalloc
   for (auto &kv : c->graph_cache) hipGraphExecDestroy(kv.second);
@@ -3322,7 +3406,7 @@ void rhg_workspace_destroy(rhg_workspace *ws) {
   }
   for (auto &b : ws->bufs) {
     hipFree(b.second.p);
-    for (void *r : b.second.retired) hipFree(r);
+    for (auto &r : b.second.retired) hipFree(r.first);
   }
   delete ws;
 }

@@ -311,6 +311,9 @@ __device__ __forceinline__ double torus_dist2(const Cell &c, double dx, double d
 }
 
 // one workgroup per cell; pass 0 counts, pass 1 writes in ascending order
+// One wave per cell: the lanes test 64 atoms at a time against the cell
+// centre and write the atoms within reach in ascending order (ballot
+// compaction; no barriers).  pass 0 counts, pass 1 writes at nb_off[c].
 __global__ __launch_bounds__(256) void k_cell_neighbours(const Cell cell, int nc0, int nc1, int nc2,
                                                          double reach2, const double *__restrict__ ax,
                                                          const double *__restrict__ ay,
@@ -318,37 +321,24 @@ __global__ __launch_bounds__(256) void k_cell_neighbours(const Cell cell, int nc
                                                          int32_t *__restrict__ counts,
                                                          const int32_t *__restrict__ nb_off,
                                                          int32_t *__restrict__ nb_idx, int pass) {
-  __shared__ int wave_sum[4];
-  __shared__ int base;
-  const int c = blockIdx.x;
+  const int c = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  if (c >= nc0 * nc1 * nc2) return;
+  const int lane = threadIdx.x & 63;
   const int cz = c % nc2, cy = (c / nc2) % nc1, cx = c / (nc2 * nc1);
   const double fx = (cx + 0.5) / nc0, fy = (cy + 0.5) / nc1, fz = (cz + 0.5) / nc2;
   const double px = cell.box[AX] * fx + cell.box[BX] * fy + cell.box[CX] * fz;
   const double py = cell.box[AY] * fx + cell.box[BY] * fy + cell.box[CY] * fz;
   const double pz = cell.box[AZ] * fx + cell.box[BZ] * fy + cell.box[CZ] * fz;
-  if (threadIdx.x == 0) base = pass ? nb_off[c] : 0;
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int j0 = 0; j0 < n_atoms; j0 += blockDim.x) {
-    const int j = j0 + threadIdx.x;
-    bool in = false;
-    if (j < n_atoms) in = torus_dist2(cell, ax[j] - px, ay[j] - py, az[j] - pz) <= reach2;
+  int base = pass ? nb_off[c] : 0;
+  const unsigned long long lower = (1ULL << lane) - 1ULL;
+  for (int j0 = 0; j0 < n_atoms; j0 += 64) {
+    const int j = j0 + lane;
+    const bool in = j < n_atoms && torus_dist2(cell, ax[j] - px, ay[j] - py, az[j] - pz) <= reach2;
     const unsigned long long m = __ballot(in);
-    const int before = __popcll(m & ((1ULL << lane) - 1ULL));
-    if (lane == 0) wave_sum[wave] = __popcll(m);
-    __syncthreads();
-    int off = base;
-    for (int w = 0; w < wave; ++w) off += wave_sum[w];
-    if (in && pass) nb_idx[off + before] = j;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int t = 0;
-      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += wave_sum[w];
-      base += t;
-    }
-    __syncthreads();
+    if (in && pass) nb_idx[base + __popcll(m & lower)] = j;
+    base += __popcll(m);
   }
-  if (threadIdx.x == 0 && !pass) counts[c] = base;
+  if (lane == 0 && !pass) counts[c] = base;
 }
 
 void launch_cell_neighbours(const Cell &cell, int nc0, int nc1, int nc2, double reach,
@@ -356,7 +346,7 @@ void launch_cell_neighbours(const Cell &cell, int nc0, int nc1, int nc2, double 
                             int32_t *counts, int32_t *nb_off, int32_t *nb_idx, int pass,
                             hipStream_t st) {
   const int ncell = nc0 * nc1 * nc2;
-  hipLaunchKernelGGL(k_cell_neighbours, dim3(ncell), dim3(256), 0, st, cell, nc0, nc1, nc2,
+  hipLaunchKernelGGL(k_cell_neighbours, dim3((ncell + 3) / 4), dim3(256), 0, st, cell, nc0, nc1, nc2,
                      reach * reach, ax, ay, az, n_atoms, counts, nb_off, nb_idx, pass);
 }
 
@@ -405,6 +395,11 @@ __device__ __forceinline__ void cell_centre(const Cell &cell, const CellGrid &g,
 // and s + sign(s) at least W, so each atom has at most 2 candidates per
 // direction and 8 in all.  Candidates within reach are entries, atom-major in
 // neighbour-list order, shifts in a fixed order, so the list order is fixed.
+// One wave per cell: each lane takes one atom of the cell's neighbour list,
+// finds its images within reach (at most two candidate shifts per axis) and
+// the wave writes them in list order (a prefix sum of the per-lane counts,
+// at most 8, from four ballots; no barriers).  pass 0 counts, pass 1 writes
+// at off[c].
 __global__ __launch_bounds__(256) void k_image_list(const Cell cell, const CellGrid g, double reach2,
                                                     double w0, double w1, double w2,
                                                     const double *__restrict__ ax,
@@ -413,17 +408,17 @@ __global__ __launch_bounds__(256) void k_image_list(const Cell cell, const CellG
                                                     int32_t *__restrict__ counts,
                                                     const int32_t *__restrict__ off,
                                                     float4 *__restrict__ img, int pass) {
-  __shared__ int part[256];
-  __shared__ int base;
-  const int c = blockIdx.x;
+  const int c = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  if (c >= g.ncell) return;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lower = (1ULL << lane) - 1ULL;
   double cx, cy, cz;
   cell_centre(cell, g, c, cx, cy, cz);
-  if (threadIdx.x == 0) base = pass ? off[c] : 0;
-  __syncthreads();
+  int base = pass ? off[c] : 0;
   const double reach = sqrt(reach2);
   const int q0 = g.nb_off[c], cnt = g.nb_off[c + 1] - q0;
-  for (int i0 = 0; i0 < cnt; i0 += blockDim.x) {
-    const int i = i0 + threadIdx.x;
+  for (int i0 = 0; i0 < cnt; i0 += 64) {
+    const int i = i0 + lane;
     int j = 0, m = 0;
     double s[3] = {0.0, 0.0, 0.0};
     int cand[3][2] = {{0, 0}, {0, 0}, {0, 0}}, nc[3] = {1, 1, 1};
@@ -439,7 +434,6 @@ __global__ __launch_bounds__(256) void k_image_list(const Cell cell, const CellG
         if ((1.0 - fabs(s[d])) * w[d] <= reach) cand[d][nc[d]++] = s[d] > 0.0 ? -1 : 1;
       }
     }
-    // count this thread's entries, then an exclusive scan over the block
     auto image = [&](int a, int b, int e, double &vx, double &vy, double &vz) {
       const double tx = s[0] + cand[0][a], ty = s[1] + cand[1][b], tz = s[2] + cand[2][e];
       vx = cell.box[AX] * tx + cell.box[BX] * ty + cell.box[CX] * tz;
@@ -454,15 +448,15 @@ __global__ __launch_bounds__(256) void k_image_list(const Cell cell, const CellG
             double vx, vy, vz;
             m += image(a, b, e, vx, vy, vz) ? 1 : 0;
           }
-    part[threadIdx.x] = m;
-    __syncthreads();
-    for (int d = 1; d < (int)blockDim.x; d <<= 1) {
-      const int v = (int)threadIdx.x >= d ? part[threadIdx.x - d] : 0;
-      __syncthreads();
-      part[threadIdx.x] += v;
-      __syncthreads();
+    // exclusive prefix of m over the lanes and the wave's total (m <= 8)
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int bit = 0; bit < 4; ++bit) {
+      const unsigned long long v = __ballot((m >> bit) & 1);
+      pre += __popcll(v & lower) << bit;
+      tot += __popcll(v) << bit;
     }
-    int o = base + part[threadIdx.x] - m;
+    int o = base + pre;
     if (pass && i < cnt)
       for (int a = 0; a < nc[0]; ++a)
         for (int b = 0; b < nc[1]; ++b)
@@ -471,18 +465,16 @@ __global__ __launch_bounds__(256) void k_image_list(const Cell cell, const CellG
             if (image(a, b, e, vx, vy, vz))
               img[o++] = make_float4((float)vx, (float)vy, (float)vz, __int_as_float(j));
           }
-    __syncthreads();
-    if (threadIdx.x == blockDim.x - 1) base += part[threadIdx.x];
-    __syncthreads();
+    base += tot;
   }
-  if (threadIdx.x == 0 && !pass) counts[c] = base;
+  if (lane == 0 && !pass) counts[c] = base;
 }
 
 void launch_image_list(const Cell &cell, const CellGrid &g, double reach, const double widths[3],
                        const double *ax, const double *ay, const double *az, int32_t *counts,
                        const int32_t *off, float4 *img, int pass, hipStream_t st) {
-  hipLaunchKernelGGL(k_image_list, dim3(g.ncell), dim3(256), 0, st, cell, g, reach * reach, widths[0],
-                     widths[1], widths[2], ax, ay, az, counts, off, img, pass);
+  hipLaunchKernelGGL(k_image_list, dim3((g.ncell + 3) / 4), dim3(256), 0, st, cell, g, reach * reach,
+                     widths[0], widths[1], widths[2], ax, ay, az, counts, off, img, pass);
 }
 
 // Per cell, the FP64 sum of the LJ shifts of one pseudo-atom type over the

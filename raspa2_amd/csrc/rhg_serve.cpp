@@ -92,6 +92,8 @@ struct rhg_server {
   // work of the finished requests: insertions and executed pair evaluations
   // of the culled site kernels by kind (LJ, LJ + Coulomb, Coulomb)
   double work_ins = 0, work_pairs[3] = {0, 0, 0};
+  // device memory of the contexts' tables (one hipMalloc per slab)
+  rhg_pool *pool = nullptr;
   // per slot: the scratch buffers and the stream of the request running there
   std::vector<rhg_workspace *> ws;
   std::vector<size_t> ws_bytes;
@@ -406,7 +408,7 @@ int reserve(rhg_server *s, FrameworkSpec &f, const rhg_probe &p, int slot, bool 
 int create_context(rhg_server *s, FrameworkSpec &f, int slot) {
   rhg_framework fr{(int32_t)f.x.size(), f.x.data(), f.y.data(), f.z.data(), f.q.data(),
                    f.eps.data(), f.sig.data(), f.mass.data()};
-  int rc = rhg_create_on_stream(&fr, &f.cell, &f.ff, s->device, s->streams[slot], s->ws[slot], &f.ctx);
+  int rc = rhg_create_on_stream(&fr, &f.cell, &f.ff, s->device, s->streams[slot], s->ws[slot], s->pool, &f.ctx);
   if (rc) {
     f.ctx = nullptr;
     return rc;
@@ -661,6 +663,7 @@ int rhg_server_create(int device, size_t hbm_budget, int32_t slots, int32_t prec
     hbm_budget = (size_t)(0.8 * (double)fr);
   }
   s->budget = hbm_budget;
+  s->pool = rhg_pool_create(device, (size_t)256 << 20);
   s->ws.assign((size_t)slots, nullptr);
   s->ws_bytes.assign((size_t)slots, 0);
   s->ws_busy.assign((size_t)slots, 0);
@@ -688,6 +691,7 @@ void rhg_server_destroy(rhg_server *s) {
   for (rhg_workspace *w : s->ws) rhg_workspace_destroy(w);
   for (hipStream_t st : s->streams)
     if (st) hipStreamDestroy(st);
+  rhg_pool_destroy(s->pool); // after every context that carved from it
   delete s;
 }
 
