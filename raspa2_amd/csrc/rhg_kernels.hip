@@ -16,8 +16,13 @@ namespace rhg {
 // erfc(x) in FP32 for the mixed real-space Ewald terms:
 // exp(-x^2) P(t), t = 1 / (1 + x/2), with P of degree 8 fitted to
 // erfcx(x) = exp(x^2) erfc(x) on [0, 5] (raspa2_amd/csrc/fit_erfc.py; relative
-// error 3.5e-8 in exact arithmetic, below 5e-7 in FP32).  Larger arguments
-// (only for Ewald precisions beyond 1e-10) use the library erfcf.
+// error 3.5e-8 in exact arithmetic).  exp(-x^2) is one exp2 of the rounded
+// -x^2 log2(e): the rounding of that exponent grows with x^2, so the relative
+// error of the result reaches 1.1e-6 at x = 3.2 (2.6e-6 at 5), while the
+// absolute error stays below 4e-7 over [0, 5]; a pair term enters a site's
+// sum with its absolute error, which is what the Coulomb tolerances bound.
+// Larger arguments (only for Ewald precisions beyond 1e-10) use the library
+// erfcf.
 __device__ __forceinline__ float erfc_fit(float x) {
   const float t = __builtin_amdgcn_rcpf(fmaf(0.5f, x, 1.0f));
   float p = -0.040972326f;
@@ -29,16 +34,8 @@ __device__ __forceinline__ float erfc_fit(float x) {
   p = fmaf(p, t, 0.240593985f);
   p = fmaf(p, t, 0.287479192f);
   p = fmaf(p, t, -0.000301792286f);
-  // exp(-x^2) = 2^(-x^2 log2 e), with x^2 and the product carried as
-  // head + tail (FMA residuals) so that the exponent is accurate to FP32
-  // ulps of the result, not of x^2
-  const float x2 = x * x;
-  const float x2l = fmaf(x, x, -x2);
-  const float yh = -x2 * 1.44269502f;
-  float yl = fmaf(-x2, 1.44269502f, -yh);
-  yl = fmaf(-x2, 1.92596299e-08f, yl);
-  yl = fmaf(-x2l, 1.44269502f, yl);
-  return p * (__builtin_amdgcn_exp2f(yh) * fmaf(yl, 0.693147181f, 1.0f));
+  // exp(-x^2) = 2^(-x^2 log2 e)
+  return p * __builtin_amdgcn_exp2f((x * x) * -1.44269502f);
 }
 __device__ __forceinline__ float erfc_mixed(float x) {
   if (x > 5.0f) return erfcf(x);
