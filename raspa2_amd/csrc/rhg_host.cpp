@@ -256,6 +256,10 @@ struct rhg_ctx {
   double2 *d_sf = nullptr;
   double2 *d_ks = nullptr; // kfactor times d_sf
   int32_t *d_row_off = nullptr, *d_row_kz = nullptr;
+  // k-vector records in the Fourier kernel's visiting order and each kx
+  // slice's first record
+  double4 *d_kst = nullptr;
+  std::vector<int32_t> kst_off;
   size_t footprint = 0;
   // cell lists
   int cull = 1;
@@ -708,6 +712,48 @@ void fill(double *p, int64_t n, double v, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
 }
 
+// The k-vectors (indices into RASPA2's k order) in the order k_fourier_part
+// visits them: per kx slice the k_y = 0 row (k_z = 0 unless kx = 0, then
+// +k_z and -k_z per step), then per k_y > 0 the rows +k_y and -k_y together
+// (k_z = 0 of each, then (+k_y, +k_z), (+k_y, -k_z), (-k_y, +k_z),
+// (-k_y, -k_z) per step); off[kx] is the first of slice kx, off[kx + 1] the
+// count.
+void fourier_order(const rhg_ctx *c, std::vector<int32_t> &order, std::vector<int32_t> &off) {
+  const int ny = 2 * c->ky + 1;
+  order.clear();
+  off.assign((size_t)c->kx + 2, 0);
+  for (int ii = 0; ii <= c->kx; ++ii) {
+    off[ii] = (int32_t)order.size();
+    for (int jj = 0; jj <= c->ky; ++jj) {
+      if (jj == 0) {
+        const int row = ii * ny + c->ky, kzm = c->row_kz[row];
+        if (kzm < 0) continue;
+        const int o = c->row_off[row];
+        if (ii != 0) order.push_back(o + kzm);
+        const int ip = o + kzm - (ii == 0 ? 1 : 0), im = o + kzm;
+        for (int kk = 1; kk <= kzm; ++kk) {
+          order.push_back(ip + kk);
+          order.push_back(im - kk);
+        }
+        continue;
+      }
+      const int rp = ii * ny + (jj + c->ky), rm = ii * ny + (c->ky - jj);
+      const int kzm = c->row_kz[rp];
+      if (kzm < 0) continue;
+      const int op = c->row_off[rp] + kzm, om = c->row_off[rm] + kzm;
+      order.push_back(op);
+      order.push_back(om);
+      for (int kk = 1; kk <= kzm; ++kk) {
+        order.push_back(op + kk);
+        order.push_back(op - kk);
+        order.push_back(om + kk);
+        order.push_back(om - kk);
+      }
+    }
+  }
+  off[(size_t)c->kx + 1] = (int32_t)order.size();
+}
+
 EwaldDev ewald_dev(rhg_ctx *c) {
   EwaldDev e;
   e.kx = c->kx;
@@ -723,6 +769,9 @@ EwaldDev ewald_dev(rhg_ctx *c) {
   e.ks = c->d_ks;
   e.row_off = c->d_row_off;
   e.row_kz = c->d_row_kz;
+  e.kst = c->d_kst;
+  for (int i = 0; i < kMaxKx + 2; ++i)
+    e.kst_off[i] = i < (int)c->kst_off.size() ? c->kst_off[i] : 0;
   e.cell = c->cell;
   e.C = c->u.C;
   return e;
@@ -829,8 +878,11 @@ int ensure_image_list(rhg_ctx *c) {
   c->d_img_off = nullptr;
   int rc_;
   if ((rc_ = dalloc(c, (void **)&c->d_img_off, (size_t)(ncell + 1) * sizeof(int32_t)))) return rc_;
-  launch_image_list(c->cell, c->cg, c->cell_reach, c->props + 3, c->d_ax, c->d_ay, c->d_az, counts,
-                    nullptr, nullptr, 0, st);
+  // rectangular cells: the images within the cut-off of the cell's box (a
+  // 1e-3 A margin covers the FP32 rounding of the offsets, below 1e-5 A)
+  const double box_reach = rc + 1e-3;
+  launch_image_list(c->cell, c->cg, c->cell_reach, c->props + 3, box_reach, c->d_ax, c->d_ay, c->d_az,
+                    counts, nullptr, nullptr, 0, st);
   launch_scan_counts(counts, ncell, c->d_img_off, st);
   int32_t total = 0;
   HIPC(c, hipMemcpyAsync(&total, c->d_img_off + ncell, sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -838,8 +890,8 @@ int ensure_image_list(rhg_ctx *c) {
   dfree(c, c->d_img);
   c->d_img = nullptr;
   if ((rc_ = dalloc(c, (void **)&c->d_img, (size_t)std::max(total, 1) * sizeof(float4)))) return rc_;
-  launch_image_list(c->cell, c->cg, c->cell_reach, c->props + 3, c->d_ax, c->d_ay, c->d_az, counts,
-                    c->d_img_off, c->d_img, 1, st);
+  launch_image_list(c->cell, c->cg, c->cell_reach, c->props + 3, box_reach, c->d_ax, c->d_ay, c->d_az,
+                    counts, c->d_img_off, c->d_img, 1, st);
   HIPC(c, hipGetLastError());
   HIPC(c, hipStreamSynchronize(st));
   c->cg.img_off = c->d_img_off;
@@ -1172,9 +1224,20 @@ int create_impl(const rhg_framework *fw, const rhg_cell *cellp, const rhg_forcef
       qq.push_back(c->q[i]);
       c->net_q_fw += c->q[i];
     }
-    if (c->kz > 64) return RHG_ERR_UNSUPPORTED; // k_z range of the row kernel
+    if (c->kz > 64 || c->kx > kMaxKx) return RHG_ERR_UNSUPPORTED; // k ranges of the row kernel
     if ((rc = structure_factor(c.get(), fx, fy, fz, qq, c->d_sf))) return rc;
     launch_ks_table(c->d_kfactor, c->d_sf, c->nk, c->d_ks, c->stream);
+    HIPC(c.get(), hipGetLastError());
+    // the k-vector records in the Fourier kernel's visiting order
+    std::vector<int32_t> order;
+    fourier_order(c.get(), order, c->kst_off);
+    if ((int)order.size() != c->nk) return fail(c.get(), RHG_ERR_INVALID, "k-vector visiting order");
+    order.resize(order.size() + 8, 0); // read-ahead padding
+    int32_t *d_order = nullptr;
+    if ((rc = dalloc(c.get(), (void **)&d_order, order.size() * sizeof(int32_t)))) return rc;
+    if ((rc = dalloc(c.get(), (void **)&c->d_kst, order.size() * sizeof(double4)))) return rc;
+    upload(c.get(), d_order, order.data(), order.size() * sizeof(int32_t));
+    launch_kstream(c->d_kfactor, c->d_ks, d_order, (int)order.size(), c->d_kst, c->stream);
     HIPC(c.get(), hipGetLastError());
   }
   *out = c.release();

@@ -107,9 +107,12 @@ void launch_scan_counts(const int32_t *counts, int n, int32_t *off, hipStream_t 
 // widths: perpendicular widths of the cell
 // out[c] = sum of a64[j].shift over the image list of cell c, c < g.ncell
 void launch_img_shift_sums(const CellGrid &g, const AtomRec64 *a64, double *out, hipStream_t st);
+// box_reach > 0: on rectangular cells keep the images within box_reach of the
+// cell's box (the cut-off plus a rounding margin) rather than within reach
+// of its centre
 void launch_image_list(const Cell &cell, const CellGrid &g, double reach, const double widths[3],
-                       const double *ax, const double *ay, const double *az, int32_t *counts,
-                       const int32_t *off, float4 *img, int pass, hipStream_t st);
+                       double box_reach, const double *ax, const double *ay, const double *az,
+                       int32_t *counts, const int32_t *off, float4 *img, int pass, hipStream_t st);
 void launch_bin_sites(const Cell &cell, const CellGrid &g, const SiteList &s, const BinWork &b,
                       hipStream_t st);
 void launch_site_energy_culled(const EnergyArgs &a, const CellGrid &g, const SiteList &s,
@@ -118,6 +121,9 @@ void launch_site_energy_culled(const EnergyArgs &a, const CellGrid &g, const Sit
 void launch_deinterleave(const double *xyz, int64_t n, double *x, double *y, double *z, hipStream_t st);
 
 // ---------------------------------------------------------------- Ewald
+// largest k_x of the Fourier kernel (k_x + 1 slices)
+constexpr int kMaxKx = 64;
+
 struct EwaldDev {
   int kx, ky, kz, nk;
   double alpha, uion, net_q_fw, beta_unused;
@@ -126,6 +132,11 @@ struct EwaldDev {
   const double2 *ks;       // nk: kfactor times sf (the host term's weights)
   const int32_t *row_off;  // (kx+1)*(2ky+1): nvec index of the row's first valid kz
   const int32_t *row_kz;   // (kx+1)*(2ky+1): largest valid |kz| of the row (-1 empty)
+  // the k-vectors in the Fourier kernel's visiting order, {K Re S_fw,
+  // K Im S_fw, K, index} (nk records and 8 of padding), and the first
+  // record of each kx slice (kst_off[kx + 1] = nk)
+  const double4 *kst;
+  int kst_off[kMaxKx + 2];
   Cell cell;
   double C;                // Coulomb conversion factor
 };
@@ -138,6 +149,9 @@ void launch_structure_factor(const EwaldDev &e, const int32_t *kidx, int nk, con
 
 // ks[i] = kf[i] sf[i], i < nk
 void launch_ks_table(const double *kf, const double2 *sf, int nk, double2 *ks, hipStream_t st);
+// kst[i] = {ks[idx[i]].x, ks[idx[i]].y, kf[idx[i]], idx[i]}, i < n
+void launch_kstream(const double *kf, const double2 *ks, const int32_t *idx, int n, double4 *kst,
+                    hipStream_t st);
 
 // Fourier Delta of n molecules; positions pos[j][m] (site-major SoA of
 // three arrays each n_sites*stride); list/n_dev select molecules.

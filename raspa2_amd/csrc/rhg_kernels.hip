@@ -23,8 +23,7 @@ namespace rhg {
 // sum with its absolute error, which is what the Coulomb tolerances bound.
 // Larger arguments (only for Ewald precisions beyond 1e-10) use the library
 // erfcf.
-__device__ __forceinline__ float erfc_fit(float x) {
-  const float t = __builtin_amdgcn_rcpf(fmaf(0.5f, x, 1.0f));
+__device__ __forceinline__ float erfcx_poly(float t) {
   float p = -0.040972326f;
   p = fmaf(p, t, 0.292736441f);
   p = fmaf(p, t, -0.786886334f);
@@ -34,8 +33,30 @@ __device__ __forceinline__ float erfc_fit(float x) {
   p = fmaf(p, t, 0.240593985f);
   p = fmaf(p, t, 0.287479192f);
   p = fmaf(p, t, -0.000301792286f);
+  return p;
+}
+__device__ __forceinline__ float erfc_fit(float x) {
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.5f, x, 1.0f));
+  const float p = erfcx_poly(t);
   // exp(-x^2) = 2^(-x^2 log2 e)
   return p * __builtin_amdgcn_exp2f((x * x) * -1.44269502f);
+}
+
+// The same fit as erfc(alpha r)/r from r^2 and 1/r (the pair loops have
+// both): t = 1 / (1 + (alpha/2 r^2) (1/r)) and exp(-alpha^2 r^2) as one exp2
+// of r^2 ke, with the constants of coulomb_consts (two instructions fewer
+// than forming r and alpha r first).
+struct CoulConsts {
+  float bh; // alpha / 2
+  float ke; // -alpha^2 log2(e)
+};
+__device__ __forceinline__ CoulConsts coulomb_consts(double alpha) {
+  const float a = (float)alpha;
+  return CoulConsts{0.5f * a, -(a * a) * 1.44269502f};
+}
+__device__ __forceinline__ float erfc_over_r(float r2, float irs, CoulConsts k) {
+  const float t = __builtin_amdgcn_rcpf(fmaf(k.bh * r2, irs, 1.0f));
+  return (erfcx_poly(t) * __builtin_amdgcn_exp2f(r2 * k.ke)) * irs;
 }
 __device__ __forceinline__ float erfc_mixed(float x) {
   if (x > 5.0f) return erfcf(x);
@@ -396,9 +417,14 @@ __device__ __forceinline__ void cell_centre(const Cell &cell, const CellGrid &g,
 // finds its images within reach (at most two candidate shifts per axis) and
 // the wave writes them in list order (a prefix sum of the per-lane counts,
 // at most 8, from four ballots; no barriers).  pass 0 counts, pass 1 writes
-// at off[c].
+// at off[c].  With box2 > 0 (rectangular cells) an image is kept only when
+// its distance to the cell's box (half edges hx, hy, hz about the centre) is
+// at most sqrt(box2): the images within the cut-off of some point of the
+// cell, instead of all within the cut-off plus the half diagonal of its
+// centre (the list holds about 7% fewer entries for a cube).
 __global__ __launch_bounds__(256) void k_image_list(const Cell cell, const CellGrid g, double reach2,
-                                                    double w0, double w1, double w2,
+                                                    double w0, double w1, double w2, double box2,
+                                                    double hx, double hy, double hz,
                                                     const double *__restrict__ ax,
                                                     const double *__restrict__ ay,
                                                     const double *__restrict__ az,
@@ -436,6 +462,11 @@ __global__ __launch_bounds__(256) void k_image_list(const Cell cell, const CellG
       vx = cell.box[AX] * tx + cell.box[BX] * ty + cell.box[CX] * tz;
       vy = cell.box[AY] * tx + cell.box[BY] * ty + cell.box[CY] * tz;
       vz = cell.box[AZ] * tx + cell.box[BZ] * ty + cell.box[CZ] * tz;
+      if (box2 > 0.0) {
+        const double ex = fmax(fabs(vx) - hx, 0.0), ey = fmax(fabs(vy) - hy, 0.0),
+                     ez = fmax(fabs(vz) - hz, 0.0);
+        return ex * ex + ey * ey + ez * ez <= box2;
+      }
       return vx * vx + vy * vy + vz * vz <= reach2;
     };
     if (i < cnt)
@@ -468,10 +499,16 @@ __global__ __launch_bounds__(256) void k_image_list(const Cell cell, const CellG
 }
 
 void launch_image_list(const Cell &cell, const CellGrid &g, double reach, const double widths[3],
-                       const double *ax, const double *ay, const double *az, int32_t *counts,
-                       const int32_t *off, float4 *img, int pass, hipStream_t st) {
+                       double box_reach, const double *ax, const double *ay, const double *az,
+                       int32_t *counts, const int32_t *off, float4 *img, int pass, hipStream_t st) {
+  // the box test for rectangular cells (cell vectors along the axes)
+  const bool rect = cell.bc == 1 && box_reach > 0.0;
+  const double box2 = rect ? box_reach * box_reach : 0.0;
+  const double hx = 0.5 * cell.box[AX] / g.nc[0], hy = 0.5 * cell.box[BY] / g.nc[1],
+               hz = 0.5 * cell.box[CZ] / g.nc[2];
   hipLaunchKernelGGL(k_image_list, dim3((g.ncell + 3) / 4), dim3(256), 0, st, cell, g, reach * reach,
-                     widths[0], widths[1], widths[2], ax, ay, az, counts, off, img, pass);
+                     widths[0], widths[1], widths[2], box2, hx, hy, hz, ax, ay, az, counts, off, img,
+                     pass);
 }
 
 // Per cell, the FP64 sum of the LJ shifts of one pseudo-atom type over the
@@ -818,7 +855,7 @@ __device__ __forceinline__ float vgpr_copy(float x) {
 template <int NS, int VDW, int CC, int NSM>
 __device__ __forceinline__ void img_sums(const float4 *__restrict__ img, const float4 *__restrict__ par,
                                          int q0, int q1, const float4 (&so)[NSM], float rc2vf,
-                                         float rc2cf, float alphaf, double cqa,
+                                         float rc2cf, float alphaf, CoulConsts ck, double cqa,
                                          double (&uv)[NSM], double (&uc)[NSM]) {
   // one pair term in FP32 (VDW 1, CC: zero beyond the cut-off; VDW 2: added
   // to fv); the LJ form keeps one SGPR operand per instruction
@@ -845,9 +882,8 @@ __device__ __forceinline__ void img_sums(const float4 *__restrict__ img, const f
     if (CC == 1) {
       // alpha rc <= 5: the fit covers every in-range pair; computed for all
       // pairs and selected, like the LJ term (out-of-range values are finite)
-      const float irs = __builtin_amdgcn_rsqf(rr);
-      const float e2 = erfc_fit(alphaf * (rr * irs));
-      fc = (rr < rc2cf) ? pr.w * (e2 * irs) : 0.0f;
+      const float g = erfc_over_r(rr, __builtin_amdgcn_rsqf(rr), ck);
+      fc = (rr < rc2cf) ? pr.w * g : 0.0f;
     } else if (CC == 2) {
       fc = 0.0f;
       if (rr < rc2cf) {
@@ -944,16 +980,17 @@ __global__ __launch_bounds__(256) void k_site_energy_img(
 #pragma unroll
   for (int k = 0; k < NSM; ++k) uv[k] = uc[k] = 0.0;
   const float rc2vf = (float)rc2v, rc2cf = (float)rc2c, alphaf = (float)alpha;
+  const CoulConsts ck = coulomb_consts(alpha);
   // only as many sites per lane as the tile holds (wave-uniform)
   static_assert(NSM >= 1 && NSM <= kImgLane, "1..4 sites per lane");
   if (NSM >= 4 && nl > 192)
-    img_sums<(NSM >= 4 ? 4 : 1), VDW, CC, NSM>(g.img, par, q0, q1, so, rc2vf, rc2cf, alphaf, cqa, uv, uc);
+    img_sums<(NSM >= 4 ? 4 : 1), VDW, CC, NSM>(g.img, par, q0, q1, so, rc2vf, rc2cf, alphaf, ck, cqa, uv, uc);
   else if (NSM >= 3 && nl > 128)
-    img_sums<(NSM >= 3 ? 3 : 1), VDW, CC, NSM>(g.img, par, q0, q1, so, rc2vf, rc2cf, alphaf, cqa, uv, uc);
+    img_sums<(NSM >= 3 ? 3 : 1), VDW, CC, NSM>(g.img, par, q0, q1, so, rc2vf, rc2cf, alphaf, ck, cqa, uv, uc);
   else if (NSM >= 2 && nl > 64)
-    img_sums<(NSM >= 2 ? 2 : 1), VDW, CC, NSM>(g.img, par, q0, q1, so, rc2vf, rc2cf, alphaf, cqa, uv, uc);
+    img_sums<(NSM >= 2 ? 2 : 1), VDW, CC, NSM>(g.img, par, q0, q1, so, rc2vf, rc2cf, alphaf, ck, cqa, uv, uc);
   else
-    img_sums<1, VDW, CC, NSM>(g.img, par, q0, q1, so, rc2vf, rc2cf, alphaf, cqa, uv, uc);
+    img_sums<1, VDW, CC, NSM>(g.img, par, q0, q1, so, rc2vf, rc2cf, alphaf, ck, cqa, uv, uc);
   if (VDW == 2) {
     const double sh = cell_shift[c];
 #pragma unroll
@@ -1297,16 +1334,22 @@ void launch_ks_table(const double *kf, const double2 *sf, int nk, double2 *ks, h
 // ====================================================================
 
 template <int ADS>
-__device__ __forceinline__ double ads_term(const double2 *sads, int idx, double sre, double sim) {
+__device__ __forceinline__ double ads_term(const double2 *sads, const double4 &r, double sre, double sim) {
   const double d2 = fma(sim, sim, sre * sre);
   if (!ADS) return d2;
-  const double2 st = sads[idx];
+  const double2 st = sads[(int)r.w];
   return fma(2.0 * st.x, sre, fma(2.0 * st.y, sim, d2));
 }
 
 // Partial k-sums of one kx slice (blockIdx.y = kx) for 64 molecules per
 // wave; part[kx][m] = (host, ads).  The slice's exp(i kx x) comes from the
-// same recurrence as the reference (kx repeated products of exp(i x)).
+// same recurrence as the reference (kx repeated products of exp(i x)).  The
+// k-vectors are visited row by row (k_y, and for k_y > 0 the rows +k_y and
+// -k_y together, which share the k_z phases), and their records {K Re S_fw,
+// K Im S_fw, K, index} are read from e.kst, laid out in exactly that order
+// (kst_off[kx]: the slice's first record): each k_z step reads one
+// contiguous block of records through the scalar unit with one wait,
+// instead of a dependent load per k-vector.
 // With adsorbates present (ADS) the adsorbate term of each k is
 // |S_ads + dS|^2 - |S_ads|^2 = 2 Re(S_ads* dS) + |dS|^2 (src/ewald.c:7233-7259).
 template <int NC, int ADS>
@@ -1346,116 +1389,146 @@ __global__ __launch_bounds__(128) void k_fourier_part(const FourierArgs a, doubl
     ex[t] = cx{1.0, 0.0};
     for (int r = 1; r <= ii; ++r) ex[t] = (r == 1) ? e1x[t] : cmul(ex[t], e1x[t]);
   }
-  double host = 0.0, ads = 0.0;
   const int ny = 2 * e.ky + 1;
-  cx ey[NC];
+  {
+    double host = 0.0, ads = 0.0;
+    // one k-vector's contribution: host += K Re(S_fw* dS), ads += K (|dS|^2
+    // or the adsorbate form)
+    auto add_k = [&](const double4 &r, double sre, double sim) {
+      host = fma(r.y, sim, fma(r.x, sre, host));
+      ads = fma(r.z, ads_term<ADS>(a.sads, r, sre, sim), ads);
+    };
+    const double4 *__restrict__ p = e.kst + e.kst_off[ii];
+    cx ey[NC];
 #pragma unroll
-  for (int t = 0; t < nc; ++t) ey[t] = cx{1.0, 0.0};
-  // one k-vector's contribution: host += K Re(S_fw* dS), ads += K (|dS|^2 or
-  // the adsorbate form)
-  auto add_k = [&](int idx, double sre, double sim) {
-    const double2 ks = e.ks[idx];
-    host = fma(ks.y, sim, fma(ks.x, sre, host));
-    ads = fma(e.kfactor[idx], ads_term<ADS>(a.sads, idx, sre, sim), ads);
-  };
-  for (int jj = 0; jj <= e.ky; ++jj) {
-    if (jj > 0) {
+    for (int t = 0; t < nc; ++t) ey[t] = cx{1.0, 0.0};
+    for (int jj = 0; jj <= e.ky; ++jj) {
+      if (jj > 0) {
 #pragma unroll
-      for (int t = 0; t < nc; ++t) ey[t] = (jj == 1) ? e1y[t] : cmul(ey[t], e1y[t]);
-    }
-    if (jj == 0) {
-      const int row = ii * ny + e.ky;
-      const int kzm = e.row_kz[row];
-      if (kzm < 0) continue;
-      const int off = e.row_off[row];
-      // charge-weighted row phases q e^{i kx x} of each site (ky = 0)
-      cx xy[NC], ez[NC];
-#pragma unroll
-      for (int t = 0; t < nc; ++t) {
-        const cx r = cmul(ex[t], ey[t]);
-        xy[t] = cx{qs[t] * r.re, qs[t] * r.im};
-        ez[t] = cx{1.0, 0.0};
+        for (int t = 0; t < nc; ++t) ey[t] = (jj == 1) ? e1y[t] : cmul(ey[t], e1y[t]);
       }
-      const bool origin_row = (ii == 0);
-      // k_z = 0 (absent in the origin row, where k = 0 is excluded)
-      if (!origin_row) {
-        double sre = 0.0, sim = 0.0;
+      if (jj == 0) {
+        const int row = ii * ny + e.ky;
+        const int kzm = e.row_kz[row];
+        if (kzm < 0) continue;
+        // charge-weighted row phases q e^{i kx x} of each site (ky = 0)
+        cx xy[NC], ez[NC];
 #pragma unroll
         for (int t = 0; t < nc; ++t) {
-          sre += xy[t].re;
-          sim += xy[t].im;
+          const cx r = cmul(ex[t], ey[t]);
+          xy[t] = cx{qs[t] * r.re, qs[t] * r.im};
+          ez[t] = cx{1.0, 0.0};
         }
-        add_k(off + kzm, sre, sim);
+        // k_z = 0 (absent in the origin row, where k = 0 is excluded)
+        if (ii != 0) {
+          double sre = 0.0, sim = 0.0;
+#pragma unroll
+          for (int t = 0; t < nc; ++t) {
+            sre += xy[t].re;
+            sim += xy[t].im;
+          }
+          add_k(p[0], sre, sim);
+          ++p;
+        }
+        // k_z = +kk and -kk share the four products Re(xy) Re(ez), Im(xy)
+        // Im(ez), Re(xy) Im(ez), Im(xy) Re(ez); records (+kk, -kk) per step
+        double4 n0 = p[0], n1 = p[1];
+        for (int kk = 1; kk <= kzm; ++kk) {
+          const double4 r0 = n0, r1 = n1;
+          p += 2;
+          n0 = p[0];
+          n1 = p[1];
+          double a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
+#pragma unroll
+          for (int t = 0; t < nc; ++t) {
+            ez[t] = cx{fma(ez[t].re, e1z[t].re, -ez[t].im * e1z[t].im),
+                       fma(ez[t].im, e1z[t].re, ez[t].re * e1z[t].im)};
+            a1 = fma(xy[t].re, ez[t].re, a1);
+            a2 = fma(xy[t].im, ez[t].im, a2);
+            a3 = fma(xy[t].re, ez[t].im, a3);
+            a4 = fma(xy[t].im, ez[t].re, a4);
+          }
+          add_k(r0, a1 - a2, a3 + a4); // +k_z
+          add_k(r1, a1 + a2, a4 - a3); // -k_z
+        }
+        continue;
       }
-      // k_z = +kk and -kk share the four products Re(xy) Re(ez), Im(xy) Im(ez),
-      // Re(xy) Im(ez), Im(xy) Re(ez); in the origin row the +k_z entries sit
-      // one place lower (k = 0 is not stored)
-      const int ip = off + kzm - (origin_row ? 1 : 0), im = off + kzm;
-#pragma unroll 2
+      // the rows +jj and -jj have the same k_z range (the cut-off depends on
+      // jj^2) and share the k_z phases: four k-vectors per k_z step
+      const int rp = ii * ny + (jj + e.ky);
+      const int kzm = e.row_kz[rp];
+      if (kzm < 0) continue;
+      cx xp[NC], xm[NC], ez[NC];
+#pragma unroll
+      for (int t = 0; t < nc; ++t) {
+        const cx r = cmul(ex[t], ey[t]), rc = cmul(ex[t], conjc(ey[t]));
+        xp[t] = cx{qs[t] * r.re, qs[t] * r.im};
+        xm[t] = cx{qs[t] * rc.re, qs[t] * rc.im};
+        ez[t] = cx{1.0, 0.0};
+      }
+      {
+        double spr = 0.0, spi = 0.0, smr = 0.0, smi = 0.0;
+#pragma unroll
+        for (int t = 0; t < nc; ++t) {
+          spr += xp[t].re;
+          spi += xp[t].im;
+          smr += xm[t].re;
+          smi += xm[t].im;
+        }
+        add_k(p[0], spr, spi);
+        add_k(p[1], smr, smi);
+        p += 2;
+      }
+      // records (+jj, +kk), (+jj, -kk), (-jj, +kk), (-jj, -kk) per step
+      double4 n0 = p[0], n1 = p[1], n2 = p[2], n3 = p[3];
       for (int kk = 1; kk <= kzm; ++kk) {
-        double a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
+        const double4 r0 = n0, r1 = n1, r2 = n2, r3 = n3;
+        p += 4;
+        n0 = p[0];
+        n1 = p[1];
+        n2 = p[2];
+        n3 = p[3];
+        double a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0, b4 = 0.0;
 #pragma unroll
         for (int t = 0; t < nc; ++t) {
           ez[t] = cx{fma(ez[t].re, e1z[t].re, -ez[t].im * e1z[t].im),
                      fma(ez[t].im, e1z[t].re, ez[t].re * e1z[t].im)};
-          a1 = fma(xy[t].re, ez[t].re, a1);
-          a2 = fma(xy[t].im, ez[t].im, a2);
-          a3 = fma(xy[t].re, ez[t].im, a3);
-          a4 = fma(xy[t].im, ez[t].re, a4);
+          a1 = fma(xp[t].re, ez[t].re, a1);
+          a2 = fma(xp[t].im, ez[t].im, a2);
+          a3 = fma(xp[t].re, ez[t].im, a3);
+          a4 = fma(xp[t].im, ez[t].re, a4);
+          b1 = fma(xm[t].re, ez[t].re, b1);
+          b2 = fma(xm[t].im, ez[t].im, b2);
+          b3 = fma(xm[t].re, ez[t].im, b3);
+          b4 = fma(xm[t].im, ez[t].re, b4);
         }
-        add_k(ip + kk, a1 - a2, a3 + a4); // +k_z
-        add_k(im - kk, a1 + a2, a4 - a3); // -k_z
+        add_k(r0, a1 - a2, a3 + a4); // (+jj, +k_z)
+        add_k(r1, a1 + a2, a4 - a3); // (+jj, -k_z)
+        add_k(r2, b1 - b2, b3 + b4); // (-jj, +k_z)
+        add_k(r3, b1 + b2, b4 - b3); // (-jj, -k_z)
       }
-      continue;
     }
-    // the rows +jj and -jj have the same k_z range (the cut-off depends on
-    // jj^2) and share the k_z phases: four k-vectors per k_z step
-    const int rp = ii * ny + (jj + e.ky), rm = ii * ny + (e.ky - jj);
-    const int kzm = e.row_kz[rp];
-    if (kzm < 0) continue;
-    const int offp = e.row_off[rp] + kzm, offm = e.row_off[rm] + kzm;
-    cx xp[NC], xm[NC], ez[NC];
-#pragma unroll
-    for (int t = 0; t < nc; ++t) {
-      const cx r = cmul(ex[t], ey[t]), rc = cmul(ex[t], conjc(ey[t]));
-      xp[t] = cx{qs[t] * r.re, qs[t] * r.im};
-      xm[t] = cx{qs[t] * rc.re, qs[t] * rc.im};
-      ez[t] = cx{1.0, 0.0};
-    }
-    {
-      double spr = 0.0, spi = 0.0, smr = 0.0, smi = 0.0;
-#pragma unroll
-      for (int t = 0; t < nc; ++t) {
-        spr += xp[t].re;
-        spi += xp[t].im;
-        smr += xm[t].re;
-        smi += xm[t].im;
-      }
-      add_k(offp, spr, spi);
-      add_k(offm, smr, smi);
-    }
-    for (int kk = 1; kk <= kzm; ++kk) {
-      double a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0, b4 = 0.0;
-#pragma unroll
-      for (int t = 0; t < nc; ++t) {
-        ez[t] = cx{fma(ez[t].re, e1z[t].re, -ez[t].im * e1z[t].im),
-                   fma(ez[t].im, e1z[t].re, ez[t].re * e1z[t].im)};
-        a1 = fma(xp[t].re, ez[t].re, a1);
-        a2 = fma(xp[t].im, ez[t].im, a2);
-        a3 = fma(xp[t].re, ez[t].im, a3);
-        a4 = fma(xp[t].im, ez[t].re, a4);
-        b1 = fma(xm[t].re, ez[t].re, b1);
-        b2 = fma(xm[t].im, ez[t].im, b2);
-        b3 = fma(xm[t].re, ez[t].im, b3);
-        b4 = fma(xm[t].im, ez[t].re, b4);
-      }
-      add_k(offp + kk, a1 - a2, a3 + a4); // (+jj, +k_z)
-      add_k(offp - kk, a1 + a2, a4 - a3); // (+jj, -k_z)
-      add_k(offm + kk, b1 - b2, b3 + b4); // (-jj, +k_z)
-      add_k(offm - kk, b1 + b2, b4 - b3); // (-jj, -k_z)
-    }
+    // the only global store, after every record load (an earlier store would
+    // keep the compiler from reading the records through the scalar unit)
+    if (live) part[(int64_t)ii * a.stride + m] = make_double2(host, ads);
   }
-  if (live) part[(int64_t)ii * a.stride + m] = make_double2(host, ads);
+}
+
+// Gathers the k-vector records of the Fourier kernel in its visiting order:
+// kst[i] = {ks[idx[i]].x, ks[idx[i]].y, kf[idx[i]], idx[i]}, i < n.
+__global__ __launch_bounds__(256) void k_kstream(const double *__restrict__ kf, const double2 *__restrict__ ks,
+                                                 const int32_t *__restrict__ idx, int n,
+                                                 double4 *__restrict__ kst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = idx[i];
+  const double2 v = ks[k];
+  kst[i] = make_double4(v.x, v.y, kf[k], (double)k);
+}
+
+void launch_kstream(const double *kf, const double2 *ks, const int32_t *idx, int n, double4 *kst,
+                    hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_kstream, dim3((n + 255) / 256), dim3(256), 0, st, kf, ks, idx, n, kst);
 }
 
 // Sums the kx slices in order and adds the excluded-pair, self and

@@ -177,8 +177,9 @@ size_t estimate_bytes(const rhg_framework *f, const rhg_cell *cell, const rhg_fo
           if (q2 != 0 && q2 < cut) nk += 1;
         }
     kx = k[0];
-    // k factors, structure factors, k indices, row tables, charged-atom scratch
-    bytes += nk * (8 + 16 + 12) + (k[0] + 1) * (2 * k[1] + 1) * 8.0 + n_q * 32.0;
+    // k factors, structure factors, k indices, row tables, charged-atom
+    // scratch, the Fourier kernel's k records and their order
+    bytes += nk * (8 + 16 + 12) + (k[0] + 1) * (2 * k[1] + 1) * 8.0 + n_q * 32.0 + (nk + 8) * (32 + 4);
   }
   // cell lists: atoms within (rc + half cell diagonal) of each cell
   double rc = std::max(ff->cutoff_vdw, ff->cutoff_charge);
