@@ -442,8 +442,13 @@ int rhg_use_coulomb_grid(rhg_ctx *ctx, rhg_grid *g);
 
 /* ------------------------------------------------------------- profiling */
 /* HIP-event timing of the kernels launched on the context's stream.
- * kernel names: "site_energy", "fourier", "widom_fb", "widom_or", "grid_build",
- * "grid_interp", "sf".  Returns launch count and summed milliseconds. */
+ * kernel families: "site_energy", "bin", "fourier", "widom_gen",
+ * "widom_select", "block_reduce", "grid_build", "grid_interp", "sf", ...
+ * Returns launch count and summed milliseconds.  on: 0 off, 1 every family,
+ * RHG_PROFILE_SITE_ENERGY only the site-energy kernels (each timed pair of
+ * events adds a gap of some microseconds between launches, so timing only
+ * the family of interest leaves the others back to back). */
+#define RHG_PROFILE_SITE_ENERGY 2
 int rhg_profile_enable(rhg_ctx *ctx, int32_t on);
 int rhg_profile_get(rhg_ctx *ctx, const char *kernel, int64_t *launches, double *total_ms);
 int rhg_profile_reset(rhg_ctx *ctx);

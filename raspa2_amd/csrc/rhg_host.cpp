@@ -277,7 +277,9 @@ struct rhg_ctx {
   double cell_reach = 0.0, cell_halfdiag = 0.0;
   double *d_ax = nullptr, *d_ay = nullptr, *d_az = nullptr;
   std::map<std::string, DevBuf> bufs;
-  bool prof = false;
+  // kernel timing with HIP events: 0 off, 1 every kernel family, 2 only the
+  // site-energy kernels (RHG_PROFILE_SITE_ENERGY)
+  int prof = 0;
   // work counters of the Widom path since the last reset:
   // insertions, first-bead sites, orientation sites, Coulomb sites of the
   // selected orientations, Fourier molecules, LJ pairs, LJ+Coulomb pairs,
@@ -468,14 +470,16 @@ struct Timed {
   rhg_ctx *c;
   const char *name;
   hipEvent_t a = nullptr;
-  Timed(rhg_ctx *ctx, const char *nm) : c(ctx), name(nm) {
-    if (c->prof) {
+  bool on;
+  Timed(rhg_ctx *ctx, const char *nm)
+      : c(ctx), name(nm), on(ctx->prof == 1 || (ctx->prof == 2 && std::strcmp(nm, "site_energy") == 0)) {
+    if (on) {
       a = get_event(c);
       hipEventRecord(a, c->stream);
     }
   }
   ~Timed() {
-    if (c->prof) {
+    if (on) {
       hipEvent_t b = get_event(c);
       hipEventRecord(b, c->stream);
       c->pending.push_back({name, a, b});
@@ -1021,7 +1025,7 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
   b.cell_of = (int32_t *)work(c, (t + "_cellof").c_str(), (size_t)s.n * sizeof(int32_t));
   b.perm = (int32_t *)work(c, (t + "_perm").c_str(), (size_t)s.n * sizeof(int32_t));
   b.count = (int32_t *)work(c, (t + "_count").c_str(), (size_t)ncell * sizeof(int32_t));
-  b.fill = (int32_t *)work(c, (t + "_fill").c_str(), (size_t)ncell * sizeof(int32_t));
+  b.rank = (int32_t *)work(c, (t + "_rank").c_str(), (size_t)s.n * sizeof(int32_t));
   b.start = (int32_t *)work(c, (t + "_start").c_str(), (size_t)(ncell + 1) * sizeof(int32_t));
   b.tile_off = (int32_t *)work(c, (t + "_tiles").c_str(), (size_t)(ncell + 1) * sizeof(int32_t));
   // the image-list kernels read the sites' sorted FP32 offsets
@@ -1035,7 +1039,7 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
   bool fresh = c->bufs["pair_counter"].p == nullptr;
   b.pairs = (unsigned long long *)work(c, "pair_counter", 3 * kPairLanes * sizeof(unsigned long long));
   if (fresh && b.pairs) fill((double *)b.pairs, 3 * kPairLanes, 0.0, c->stream); // all-zero bits
-  if (!b.cell_of || !b.perm || !b.count || !b.fill || !b.start || !b.tile_off || !b.pairs)
+  if (!b.cell_of || !b.perm || !b.count || !b.rank || !b.start || !b.tile_off || !b.pairs)
     return fail(c, RHG_ERR_OOM, "binning scratch");
   {
     Timed tm(c, "bin");
@@ -1104,6 +1108,7 @@ constexpr size_t kNaI32 = na_round(kNaSites * sizeof(int32_t));
 constexpr size_t kNaFix = na_round(kNaSites * sizeof(float4));
 constexpr size_t kNaCell = na_round((kNaCells + 1) * sizeof(int32_t));
 constexpr size_t kNaBytes = 3 * kNaXyz + 4 * kNaI32 + kNaFix + 4 * kNaCell + 256;
+static_assert(kNaCell >= kNaI32, "the per-site ranks use a per-cell slot");
 constexpr int kNaDevices = 64;
 } // namespace
 
@@ -1408,7 +1413,7 @@ int rhg_framework_energy_at_positions_noalloc(rhg_ctx *c, int64_t n, const doubl
   p += kNaFix;
   b.count = (int32_t *)p;
   p += kNaCell;
-  b.fill = (int32_t *)p;
+  b.rank = (int32_t *)p; // kNaSites ranks in a per-cell slot (kNaCell >= kNaI32)
   p += kNaCell;
   b.start = (int32_t *)p;
   p += kNaCell;
@@ -1419,8 +1424,8 @@ int rhg_framework_energy_at_positions_noalloc(rhg_ctx *c, int64_t n, const doubl
   b.n_redo = (int32_t *)p;
   b.pairs = nullptr; // no work counters on this path
   // event timing creates events: off for the duration of the call
-  const bool prof = c->prof;
-  c->prof = false;
+  const int prof = c->prof;
+  c->prof = 0;
   launch_deinterleave(d_xyz, n, sx, sy, sz, c->stream);
   EnergyArgs a = energy_args(c, type, 1.0, 1, 1);
   if (!a.do_cc && d_u_cc) fill(d_u_cc, n, 0.0, c->stream);
@@ -2963,12 +2968,12 @@ int build_nodes(rhg_ctx *c, rhg_grid *g, GridArgs &a, const float4 *par) {
     b.cell_of = (int32_t *)work(c, "g_cellof", (size_t)step * sizeof(int32_t));
     b.perm = (int32_t *)work(c, "g_perm", (size_t)step * sizeof(int32_t));
     b.count = (int32_t *)work(c, "g_count", (size_t)ncell * sizeof(int32_t));
-    b.fill = (int32_t *)work(c, "g_fill", (size_t)ncell * sizeof(int32_t));
+    b.rank = (int32_t *)work(c, "g_rank", (size_t)step * sizeof(int32_t));
     b.start = (int32_t *)work(c, "g_start", (size_t)(ncell + 1) * sizeof(int32_t));
     b.tile_off = (int32_t *)work(c, "g_tiles", (size_t)(ncell + 1) * sizeof(int32_t));
     b.pairs = pairs;
     b.fix = nullptr;
-    if (!b.cell_of || !b.perm || !b.count || !b.fill || !b.start || !b.tile_off)
+    if (!b.cell_of || !b.perm || !b.count || !b.rank || !b.start || !b.tile_off)
       return fail(c, RHG_ERR_OOM, "grid binning scratch");
     launch_bin_sites(c->cell, c->cg, sl, b, c->stream);
     Timed tm(c, "grid_build");
@@ -3336,7 +3341,8 @@ int rhg_derivatives_at_positions_real(rhg_ctx *c, int64_t n, const double *xyz, 
 // -------------------------------------------------------------- profiling
 int rhg_profile_enable(rhg_ctx *c, int32_t on) {
   if (!c) return RHG_ERR_INVALID;
-  c->prof = on != 0;
+  if (on < 0 || on > 2) return fail(c, RHG_ERR_INVALID, "profiling mode must be 0, 1 or 2");
+  c->prof = on;
   return RHG_OK;
 }
 

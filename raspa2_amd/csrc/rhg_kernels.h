@@ -71,7 +71,7 @@ constexpr int kPairLanes = 256; // counters per kind of the executed-pair tallie
 struct BinWork {
   int32_t *cell_of;  // per site
   int32_t *count;    // per cell
-  int32_t *fill;     // per cell
+  int32_t *rank;     // per site: its place among its cell's sites
   int32_t *start;    // per cell, exclusive scan of count
   int32_t *tile_off; // per cell + 1, exclusive scan of ceil(count/ts): ts = 256
                      // sites when fix is set (image-list kernels), else 64

@@ -587,13 +587,9 @@ __device__ __forceinline__ int block_slot(int32_t *counter, bool want) {
   return want ? s_base + s_wave[wid] + (int)__popcll(m & ((1ULL << lane) - 1ULL)) : -1;
 }
 
-__global__ __launch_bounds__(256) void k_zero_i32(int32_t *__restrict__ a, int32_t *__restrict__ b,
-                                                  int n, int32_t *__restrict__ one) {
+__global__ __launch_bounds__(256) void k_zero_i32(int32_t *__restrict__ a, int n, int32_t *__restrict__ one) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    a[i] = 0;
-    b[i] = 0;
-  }
+  if (i < n) a[i] = 0;
   if (i == 0 && one) *one = 0;
 }
 
@@ -625,6 +621,9 @@ __device__ __forceinline__ CellGroup cell_group(int c, bool valid) {
   return r;
 }
 
+// Counts the sites of each cell and gives each site its rank among its
+// cell's sites (the value of the cell's counter its group's atomic returned
+// plus its place in the group), so that the scatter needs no atomics.
 __global__ __launch_bounds__(256) void k_bin_count(const Cell cell, const CellGrid g,
                                                    const SiteList s, BinWork b) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -632,42 +631,34 @@ __global__ __launch_bounds__(256) void k_bin_count(const Cell cell, const CellGr
   const int64_t p = valid ? site_phys(s, i) : 0;
   const int c = valid ? cell_index(cell, g, s.x[p], s.y[p], s.z[p]) : -1;
   const CellGroup r = cell_group(c, valid);
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (valid && lane == r.lead) base = atomicAdd(&b.count[c], (int)__popcll(r.mask));
+  base = __shfl(base, r.lead);
   if (!valid) return;
   b.cell_of[i] = c;
-  if ((int)(threadIdx.x & 63) == r.lead) atomicAdd(&b.count[c], (int)__popcll(r.mask));
+  b.rank[i] = base + (int)__popcll(r.mask & ((1ULL << lane) - 1ULL));
 }
 
+// Writes each site at start[cell] + rank: the sorted FP32 offset from its
+// cell centre (image-list kernels) or its index (perm).
 template <int BCK>
 __global__ __launch_bounds__(256) void k_bin_scatter(const Cell cell, const CellGrid g, const SiteList s,
                                                      BinWork b) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = i < site_count(s);
-  const int c = valid ? b.cell_of[i] : -1;
-  const int64_t site = site_phys(s, valid ? i : 0);
-  // the site's offset from its cell centre (for the image-list kernels, which
-  // read no permutation), formed before the slot is known so that the
-  // position loads overlap the atomic's round trip
-  float4 f = make_float4(0.f, 0.f, 0.f, __int_as_float((int)site));
-  if (b.fix && valid) {
+  if (i >= site_count(s)) return;
+  const int c = b.cell_of[i];
+  const int pos = b.start[c] + b.rank[i];
+  const int64_t site = site_phys(s, i);
+  if (b.fix) {
     double cx, cy, cz;
     cell_centre(cell, g, c, cx, cy, cz);
     double ox = s.x[site] - cx, oy = s.y[site] - cy, oz = s.z[site] - cz;
     mi_exact<BCK>(cell, ox, oy, oz);
-    f.x = (float)ox;
-    f.y = (float)oy;
-    f.z = (float)oz;
-  }
-  const CellGroup r = cell_group(c, valid);
-  int base = 0;
-  const int lane = threadIdx.x & 63;
-  if (valid && lane == r.lead) base = b.start[c] + atomicAdd(&b.fill[c], (int)__popcll(r.mask));
-  base = __shfl(base, r.lead);
-  if (!valid) return;
-  const int pos = base + (int)__popcll(r.mask & ((1ULL << lane) - 1ULL));
-  if (b.fix)
-    b.fix[pos] = f;
-  else
+    b.fix[pos] = make_float4((float)ox, (float)oy, (float)oz, __int_as_float((int)site));
+  } else {
     b.perm[pos] = (int32_t)site;
+  }
 }
 
 // exclusive scans of the per-cell site counts (start) and of the per-cell
@@ -710,7 +701,7 @@ __global__ __launch_bounds__(1024) void k_scan_bins(const int32_t *__restrict__ 
 void launch_bin_sites(const Cell &cell, const CellGrid &g, const SiteList &s, const BinWork &b,
                       hipStream_t st) {
   const unsigned nbc = (unsigned)((g.ncell + 255) / 256);
-  hipLaunchKernelGGL(k_zero_i32, dim3(nbc), dim3(256), 0, st, b.count, b.fill, g.ncell, b.n_redo);
+  hipLaunchKernelGGL(k_zero_i32, dim3(nbc), dim3(256), 0, st, b.count, g.ncell, b.n_redo);
   const unsigned nb = (unsigned)((s.n + 255) / 256);
   if (nb) hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(256), 0, st, cell, g, s, b);
   hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(1024), 0, st, b.count, g.ncell, b.start, b.tile_off,

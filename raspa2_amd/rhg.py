@@ -665,8 +665,11 @@ class Context:
         return out
 
     # ---- profiling
-    def profile(self, on=True):
-        self._check(lib.rhg_profile_enable(self.h, 1 if on else 0), "profile_enable")
+    def profile(self, on=True, site_energy_only=False):
+        """HIP-event timing of the kernel families; site_energy_only times the
+        site-energy kernels alone (RHG_PROFILE_SITE_ENERGY)."""
+        mode = (2 if site_energy_only else 1) if on else 0
+        self._check(lib.rhg_profile_enable(self.h, mode), "profile_enable")
 
     def profile_get(self, name):
         n, ms = _I64(), _D()
