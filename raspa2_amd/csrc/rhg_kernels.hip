@@ -1344,7 +1344,7 @@ __device__ __forceinline__ double ads_term(const double2 *sads, const double4 &r
 // With adsorbates present (ADS) the adsorbate term of each k is
 // |S_ads + dS|^2 - |S_ads|^2 = 2 Re(S_ads* dS) + |dS|^2 (src/ewald.c:7233-7259).
 template <int NC, int ADS>
-__global__ __launch_bounds__(128) void k_fourier_part(const FourierArgs a, double2 *__restrict__ part) {
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k_fourier_part(const FourierArgs a, double2 *__restrict__ part) {
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t n = a.n_dev ? (int64_t)(*a.n_dev) : a.n;
   if ((int64_t)blockIdx.x * blockDim.x >= n) return;
@@ -1379,6 +1379,8 @@ __global__ __launch_bounds__(128) void k_fourier_part(const FourierArgs a, doubl
     e1z[t] = cx{c, s};
     ex[t] = cx{1.0, 0.0};
     for (int r = 1; r <= ii; ++r) ex[t] = (r == 1) ? e1x[t] : cmul(ex[t], e1x[t]);
+    // the charge-weighted slice phase q e^{i kx x}, once per slice
+    ex[t] = cx{qs[t] * ex[t].re, qs[t] * ex[t].im};
   }
   const int ny = 2 * e.ky + 1;
   {
@@ -1406,8 +1408,7 @@ __global__ __launch_bounds__(128) void k_fourier_part(const FourierArgs a, doubl
         cx xy[NC], ez[NC];
 #pragma unroll
         for (int t = 0; t < nc; ++t) {
-          const cx r = cmul(ex[t], ey[t]);
-          xy[t] = cx{qs[t] * r.re, qs[t] * r.im};
+          xy[t] = ex[t];
           ez[t] = cx{1.0, 0.0};
         }
         // k_z = 0 (absent in the origin row, where k = 0 is excluded)
@@ -1452,9 +1453,8 @@ __global__ __launch_bounds__(128) void k_fourier_part(const FourierArgs a, doubl
       cx xp[NC], xm[NC], ez[NC];
 #pragma unroll
       for (int t = 0; t < nc; ++t) {
-        const cx r = cmul(ex[t], ey[t]), rc = cmul(ex[t], conjc(ey[t]));
-        xp[t] = cx{qs[t] * r.re, qs[t] * r.im};
-        xm[t] = cx{qs[t] * rc.re, qs[t] * rc.im};
+        xp[t] = cmul(ex[t], ey[t]);
+        xm[t] = cmul(ex[t], conjc(ey[t]));
         ez[t] = cx{1.0, 0.0};
       }
       {
@@ -1472,6 +1472,7 @@ __global__ __launch_bounds__(128) void k_fourier_part(const FourierArgs a, doubl
       }
       // records (+jj, +kk), (+jj, -kk), (-jj, +kk), (-jj, -kk) per step
       double4 n0 = p[0], n1 = p[1], n2 = p[2], n3 = p[3];
+#pragma unroll 2
       for (int kk = 1; kk <= kzm; ++kk) {
         const double4 r0 = n0, r1 = n1, r2 = n2, r3 = n3;
         p += 4;
