@@ -254,12 +254,14 @@ struct rhg_ctx {
   std::vector<int32_t> kidx, row_off, row_kz;
   double *d_kfactor = nullptr;
   double2 *d_sf = nullptr;
-  double2 *d_ks = nullptr; // kfactor times d_sf
   int32_t *d_row_off = nullptr, *d_row_kz = nullptr;
   // k-vector records in the Fourier kernel's visiting order and each kx
   // slice's first record
   double4 *d_kst = nullptr;
   std::vector<int32_t> kst_off;
+  // host copies of uploads that may still be in flight, released after the
+  // next synchronisation of the stream (so that set-up need not wait for them)
+  std::vector<std::vector<char>> kept;
   size_t footprint = 0;
   // cell lists
   int cull = 1;
@@ -579,6 +581,33 @@ int upload(rhg_ctx *c, void *dst, const void *src, size_t bytes) {
   return RHG_OK;
 }
 
+// One copy of a staged host buffer, which the context keeps until its stream
+// has next been synchronised.
+int upload_kept(rhg_ctx *c, void *dst, std::vector<char> &&host) {
+  if (host.empty()) return RHG_OK;
+  HIPC(c, hipMemcpyAsync(dst, host.data(), host.size(), hipMemcpyHostToDevice, c->stream));
+  c->kept.push_back(std::move(host));
+  return RHG_OK;
+}
+
+int sync_stream(rhg_ctx *c) {
+  HIPC(c, hipStreamSynchronize(c->stream));
+  c->kept.clear();
+  return RHG_OK;
+}
+
+// Packs host arrays into one staging buffer, each part 256-byte aligned:
+// add() returns the part's offset.
+struct Staging {
+  std::vector<char> h;
+  size_t add(const void *src, size_t bytes) {
+    const size_t o = h.size();
+    h.resize(o + ((bytes + 255) & ~(size_t)255), 0);
+    if (bytes) memcpy(h.data() + o, src, bytes);
+    return o;
+  }
+};
+
 int setup_pseudo_atom(rhg_ctx *c, PseudoAtom &t) {
   std::vector<AtomRec64> h64((size_t)std::max(c->n, 1));
   std::vector<AtomRecMx> hmx((size_t)std::max(c->n, 1));
@@ -634,9 +663,7 @@ int setup_pseudo_atom(rhg_ctx *c, PseudoAtom &t) {
   t.c96 = (double *)(d + o96);
   t.c384 = (double *)(d + o384);
   t.par = (float4 *)(d + opar);
-  if (int rc = upload(c, d, host.data(), total)) return rc;
-  HIPC(c, hipStreamSynchronize(c->stream)); // host is released on return
-  return RHG_OK;
+  return upload_kept(c, d, std::move(host));
 }
 
 EnergyArgs energy_args(rhg_ctx *c, int type, double scaling, int do_vdw, int do_cc) {
@@ -686,7 +713,7 @@ int ads_records(rhg_ctx *c, PseudoAtom &t) {
     t.dads_cap = h.size();
   }
   if (int rc = upload(c, t.dads, h.data(), h.size() * sizeof(AtomRec64))) return rc;
-  HIPC(c, hipStreamSynchronize(c->stream)); // h is released on return
+  if (int rs_ = sync_stream(c)) return rs_; // h is released on return
   return RHG_OK;
 }
 
@@ -770,7 +797,6 @@ EwaldDev ewald_dev(rhg_ctx *c) {
   e.beta_unused = 0.0;
   e.kfactor = c->d_kfactor;
   e.sf = c->d_sf;
-  e.ks = c->d_ks;
   e.row_off = c->d_row_off;
   e.row_kz = c->d_row_kz;
   e.kst = c->d_kst;
@@ -787,6 +813,11 @@ int check_type(rhg_ctx *c, int32_t type) {
 }
 
 // Cell grid and neighbour lists for culled site energies (built lazily).
+bool image_lists_apply(const rhg_ctx *c);
+double image_box_reach(const rhg_ctx *c);
+int image_lists_ready(rhg_ctx *c);
+int cell_shift_room(rhg_ctx *c, PseudoAtom &t);
+
 int ensure_cell_grid(rhg_ctx *c) {
   if (c->grid_ready) return RHG_OK;
   hipStream_t st = c->stream;
@@ -816,50 +847,79 @@ int ensure_cell_grid(rhg_ctx *c) {
   int n = std::max(c->n, 1);
   int rc_;
   if (!c->d_ax) {
-    if ((rc_ = dalloc(c, (void **)&c->d_ax, n * sizeof(double)))) return rc_;
-    if ((rc_ = dalloc(c, (void **)&c->d_ay, n * sizeof(double)))) return rc_;
-    if ((rc_ = dalloc(c, (void **)&c->d_az, n * sizeof(double)))) return rc_;
-    if (c->n) {
-      upload(c, c->d_ax, c->x.data(), c->n * sizeof(double));
-      upload(c, c->d_ay, c->y.data(), c->n * sizeof(double));
-      upload(c, c->d_az, c->z.data(), c->n * sizeof(double));
-    }
+    // the atom positions in one block and one copy
+    Staging xyz;
+    const size_t ox = xyz.add(c->x.data(), c->n * sizeof(double)), oy = xyz.add(c->y.data(), c->n * sizeof(double)),
+                 oz = xyz.add(c->z.data(), c->n * sizeof(double));
+    char *d = nullptr;
+    if ((rc_ = dalloc(c, (void **)&d, std::max<size_t>(xyz.h.size(), 3 * 256)))) return rc_;
+    c->d_ax = (double *)(d + ox);
+    c->d_ay = (double *)(d + (c->n ? oy : 256));
+    c->d_az = (double *)(d + (c->n ? oz : 512));
+    if ((rc_ = upload_kept(c, d, std::move(xyz.h)))) return rc_;
   }
+  c->cg.nc[0] = nc[0];
+  c->cg.nc[1] = nc[1];
+  c->cg.nc[2] = nc[2];
+  c->cg.ncell = ncell;
+  c->cg.img_off = nullptr;
+  c->cg.img = nullptr;
+  c->img_ready = false;
+  c->cell_reach = reach;
+  c->cell_halfdiag = R;
   int32_t *counts = (int32_t *)work(c, "nb_counts", (size_t)ncell * sizeof(int32_t));
   dfree(c, c->d_nb_off);
   c->d_nb_off = nullptr;
   if ((rc_ = dalloc(c, (void **)&c->d_nb_off, (size_t)(ncell + 1) * sizeof(int32_t)))) return rc_;
   if (!counts) return fail(c, RHG_ERR_OOM, "cell counts");
+  // mixed precision with image lists: both lists in one pass pair
+  if (c->precision != RHG_PREC_FP64 && c->use_images && image_lists_apply(c)) {
+    int32_t *icounts = (int32_t *)work(c, "img_counts", (size_t)ncell * sizeof(int32_t));
+    int32_t *tot = (int32_t *)work(c, "list_totals", 2 * sizeof(int32_t));
+    if (!icounts || !tot) return fail(c, RHG_ERR_OOM, "cell counts");
+    dfree(c, c->d_img_off);
+    c->d_img_off = nullptr;
+    if ((rc_ = dalloc(c, (void **)&c->d_img_off, (size_t)(ncell + 1) * sizeof(int32_t)))) return rc_;
+    const double box_reach = image_box_reach(c);
+    launch_cell_lists(c->cell, c->cg, reach, c->props + 3, box_reach, c->d_ax, c->d_ay, c->d_az, c->n, counts,
+                      icounts, nullptr, nullptr, nullptr, nullptr, 0, st);
+    launch_scan2(counts, c->d_nb_off, icounts, c->d_img_off, ncell, tot, st);
+    int32_t totals[2] = {0, 0};
+    HIPC(c, hipMemcpyAsync(totals, tot, sizeof(totals), hipMemcpyDeviceToHost, st));
+    if (int rs = sync_stream(c)) return rs;
+    dfree(c, c->d_nb_idx);
+    c->d_nb_idx = nullptr;
+    dfree(c, c->d_img);
+    c->d_img = nullptr;
+    if ((rc_ = dalloc(c, (void **)&c->d_nb_idx, (size_t)std::max(totals[0], 1) * sizeof(int32_t)))) return rc_;
+    if ((rc_ = dalloc(c, (void **)&c->d_img, (size_t)std::max(totals[1], 1) * sizeof(float4)))) return rc_;
+    launch_cell_lists(c->cell, c->cg, reach, c->props + 3, box_reach, c->d_ax, c->d_ay, c->d_az, c->n, counts,
+                      icounts, c->d_nb_off, c->d_nb_idx, c->d_img_off, c->d_img, 1, st);
+    HIPC(c, hipGetLastError());
+    c->cg.nb_off = c->d_nb_off;
+    c->cg.nb_idx = c->d_nb_idx;
+    c->grid_ready = true;
+    c->gen++;
+    return image_lists_ready(c);
+  }
   launch_cell_neighbours(c->cell, nc[0], nc[1], nc[2], reach, c->d_ax, c->d_ay, c->d_az, c->n,
                          counts, nullptr, nullptr, 0, st);
   launch_scan_counts(counts, ncell, c->d_nb_off, st);
   int32_t total = 0;
   HIPC(c, hipMemcpyAsync(&total, c->d_nb_off + ncell, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-  HIPC(c, hipStreamSynchronize(st));
+  if (int rs = sync_stream(c)) return rs;
   dfree(c, c->d_nb_idx);
   c->d_nb_idx = nullptr;
   if ((rc_ = dalloc(c, (void **)&c->d_nb_idx, (size_t)std::max(total, 1) * sizeof(int32_t)))) return rc_;
   launch_cell_neighbours(c->cell, nc[0], nc[1], nc[2], reach, c->d_ax, c->d_ay, c->d_az, c->n,
                          counts, c->d_nb_off, c->d_nb_idx, 1, st);
   HIPC(c, hipGetLastError());
-  HIPC(c, hipStreamSynchronize(st));
-  c->cg.nc[0] = nc[0];
-  c->cg.nc[1] = nc[1];
-  c->cg.nc[2] = nc[2];
-  c->cg.ncell = ncell;
   c->cg.nb_off = c->d_nb_off;
   c->cg.nb_idx = c->d_nb_idx;
-  c->cg.img_off = nullptr;
-  c->cg.img = nullptr;
-  c->img_ready = false;
-  c->cell_reach = reach;
-  c->cell_halfdiag = R;
   c->grid_ready = true;
   c->gen++;
   return RHG_OK;
 }
-
-int cell_shift_room(rhg_ctx *c, PseudoAtom &t);
 
 // Image lists of the cell grid for the mixed-precision kernels.  They hold
 // every image within reach, which equals the minimum-image convention when
@@ -867,13 +927,33 @@ int cell_shift_room(rhg_ctx *c, PseudoAtom &t);
 // stay below 31 A (rounding below 1e-6 A), and the 27 lattice shifts must
 // cover the reach.  Otherwise the lists stay off and the mixed kernels take the
 // minimum image per pair.
+bool image_lists_apply(const rhg_ctx *c) {
+  const double wmin = std::min(c->props[3], std::min(c->props[4], c->props[5]));
+  const double rc = std::max(c->rc_vdw, c->rc_cc);
+  return wmin >= 2.0 * rc && c->cell_reach + c->cell_halfdiag < 31.0 && c->cell_reach < 1.5 * wmin;
+}
+
+// rectangular cells: the images within the cut-off of the cell's box (a
+// 1e-3 A margin covers the FP32 rounding of the offsets, below 1e-5 A)
+double image_box_reach(const rhg_ctx *c) { return std::max(c->rc_vdw, c->rc_cc) + 1e-3; }
+
+// The image lists of d_img_off / d_img are complete: publish them and make
+// room for the per-cell sums that depend on them.
+int image_lists_ready(rhg_ctx *c) {
+  c->img_ready = true;
+  c->cg.img_off = c->d_img_off;
+  c->cg.img = c->d_img;
+  c->img_version++;
+  c->gen++;
+  for (PseudoAtom &t : c->types)
+    if (int rc = cell_shift_room(c, t)) return rc;
+  return RHG_OK;
+}
+
 int ensure_image_list(rhg_ctx *c) {
   if (c->img_ready) return RHG_OK;
   c->img_ready = true;
-  const double wmin = std::min(c->props[3], std::min(c->props[4], c->props[5]));
-  const double rc = std::max(c->rc_vdw, c->rc_cc);
-  if (!(wmin >= 2.0 * rc && c->cell_reach + c->cell_halfdiag < 31.0 && c->cell_reach < 1.5 * wmin))
-    return RHG_OK;
+  if (!image_lists_apply(c)) return RHG_OK;
   hipStream_t st = c->stream;
   const int ncell = c->cg.ncell;
   int32_t *counts = (int32_t *)work(c, "img_counts", (size_t)ncell * sizeof(int32_t));
@@ -882,29 +962,20 @@ int ensure_image_list(rhg_ctx *c) {
   c->d_img_off = nullptr;
   int rc_;
   if ((rc_ = dalloc(c, (void **)&c->d_img_off, (size_t)(ncell + 1) * sizeof(int32_t)))) return rc_;
-  // rectangular cells: the images within the cut-off of the cell's box (a
-  // 1e-3 A margin covers the FP32 rounding of the offsets, below 1e-5 A)
-  const double box_reach = rc + 1e-3;
+  const double box_reach = image_box_reach(c);
   launch_image_list(c->cell, c->cg, c->cell_reach, c->props + 3, box_reach, c->d_ax, c->d_ay, c->d_az,
                     counts, nullptr, nullptr, 0, st);
   launch_scan_counts(counts, ncell, c->d_img_off, st);
   int32_t total = 0;
   HIPC(c, hipMemcpyAsync(&total, c->d_img_off + ncell, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-  HIPC(c, hipStreamSynchronize(st));
+  if (int rs = sync_stream(c)) return rs;
   dfree(c, c->d_img);
   c->d_img = nullptr;
   if ((rc_ = dalloc(c, (void **)&c->d_img, (size_t)std::max(total, 1) * sizeof(float4)))) return rc_;
   launch_image_list(c->cell, c->cg, c->cell_reach, c->props + 3, box_reach, c->d_ax, c->d_ay, c->d_az,
                     counts, c->d_img_off, c->d_img, 1, st);
   HIPC(c, hipGetLastError());
-  HIPC(c, hipStreamSynchronize(st));
-  c->cg.img_off = c->d_img_off;
-  c->cg.img = c->d_img;
-  c->img_version++;
-  c->gen++;
-  for (PseudoAtom &t : c->types)
-    if (int rc = cell_shift_room(c, t)) return rc;
-  return RHG_OK;
+  return image_lists_ready(c);
 }
 
 // Room for the per-cell shift sums of pseudo-atom t over the current image
@@ -1055,28 +1126,22 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
 int structure_factor(rhg_ctx *c, const std::vector<double> &fx, const std::vector<double> &fy,
                      const std::vector<double> &fz, const std::vector<double> &qq, double2 *out) {
   const int nc = (int)qq.size();
-  const size_t nk = (size_t)std::max(c->nk, 1);
-  size_t bn = (size_t)std::max(nc, 1) * sizeof(double);
-  double *d_fx = (double *)work(c, "sf_fx", bn), *d_fy = (double *)work(c, "sf_fy", bn),
-         *d_fz = (double *)work(c, "sf_fz", bn), *d_q = (double *)work(c, "sf_q", bn);
-  int32_t *d_kidx = (int32_t *)work(c, "sf_kidx", nk * 3 * sizeof(int32_t));
-  if (!d_fx || !d_fy || !d_fz || !d_q || !d_kidx) return fail(c, RHG_ERR_OOM, "structure factor buffers");
-  if (nc) {
-    upload(c, d_fx, fx.data(), nc * sizeof(double));
-    upload(c, d_fy, fy.data(), nc * sizeof(double));
-    upload(c, d_fz, fz.data(), nc * sizeof(double));
-    upload(c, d_q, qq.data(), nc * sizeof(double));
-  }
-  upload(c, d_kidx, c->kidx.data(), c->kidx.size() * sizeof(int32_t));
+  const size_t bn = (size_t)nc * sizeof(double);
+  Staging in;
+  const size_t ox = in.add(fx.data(), bn), oy = in.add(fy.data(), bn), oz = in.add(fz.data(), bn),
+               oq = in.add(qq.data(), bn);
+  char *d_in = (char *)work(c, "sf_in", std::max<size_t>(in.h.size(), 256));
   const size_t tab_n = (size_t)std::max(nc, 1) * ((c->kx + 1) + (2 * c->ky + 1) + 1);
   double2 *d_tab = (double2 *)work(c, "sf_tab", tab_n * sizeof(double2));
-  if (!d_tab) return fail(c, RHG_ERR_OOM, "structure factor tables");
+  if (!d_in || !d_tab) return fail(c, RHG_ERR_OOM, "structure factor buffers");
+  if (int rc = upload_kept(c, d_in, std::move(in.h))) return rc;
   {
     Timed t(c, "sf");
-    launch_structure_factor(ewald_dev(c), d_kidx, c->nk, d_fx, d_fy, d_fz, d_q, nc, d_tab, out, c->stream);
+    launch_structure_factor(ewald_dev(c), (const double *)(d_in + ox), (const double *)(d_in + oy),
+                            (const double *)(d_in + oz), (const double *)(d_in + oq), nc, d_tab, out,
+                            c->stream);
   }
   HIPC(c, hipGetLastError());
-  HIPC(c, hipStreamSynchronize(c->stream));
   return RHG_OK;
 }
 
@@ -1094,8 +1159,8 @@ void frac_2pi(const rhg_ctx *c, const double p[3], double f[3]) {
 // device array of the library, so that the entry point allocates nothing.
 // Layout (each part 256-byte aligned): site x, y, z (FP64), cell of each
 // site, sorted order, sorted FP32 offsets, then per cell the site counts,
-// fill counters, starts and tile offsets, then the list of sites to
-// recompute exactly and its count.  Each GPU holds its own copy; its
+// the sites' ranks (in a slot of per-cell size), starts and tile offsets,
+// then the list of sites to recompute exactly and its count.  Each GPU holds its own copy; its
 // address comes from hipGetSymbolAddress once per device, at context
 // creation with that device current.
 // ======================================================================
@@ -1205,14 +1270,27 @@ int create_impl(const rhg_framework *fw, const rhg_cell *cellp, const rhg_forcef
     ewald_setup(c.get(), ff->ewald_precision > 0.0 ? ff->ewald_precision : 1e-6);
     int rc;
     size_t nk = (size_t)std::max(c->nk, 1);
-    if ((rc = dalloc(c.get(), (void **)&c->d_kfactor, nk * sizeof(double)))) return rc;
+    if (c->kz > 64 || c->kx > kMaxKx) return RHG_ERR_UNSUPPORTED; // k ranges of the row kernel
+    // the k-vector records in the Fourier kernel's visiting order
+    std::vector<int32_t> order;
+    fourier_order(c.get(), order, c->kst_off);
+    if ((int)order.size() != c->nk) return fail(c.get(), RHG_ERR_INVALID, "k-vector visiting order");
+    order.resize(order.size() + 8, 0); // read-ahead padding
+    // the host-built k tables in one block and one copy
+    Staging kt;
+    const size_t okf = kt.add(c->kfactor.data(), c->kfactor.size() * sizeof(double)),
+                 ooff = kt.add(c->row_off.data(), c->row_off.size() * sizeof(int32_t)),
+                 okz = kt.add(c->row_kz.data(), c->row_kz.size() * sizeof(int32_t)),
+                 oord = kt.add(order.data(), order.size() * sizeof(int32_t));
+    char *d_kt = nullptr;
+    if ((rc = dalloc(c.get(), (void **)&d_kt, kt.h.size()))) return rc;
+    c->d_kfactor = (double *)(d_kt + okf);
+    c->d_row_off = (int32_t *)(d_kt + ooff);
+    c->d_row_kz = (int32_t *)(d_kt + okz);
+    const int32_t *d_order = (const int32_t *)(d_kt + oord);
     if ((rc = dalloc(c.get(), (void **)&c->d_sf, nk * sizeof(double2)))) return rc;
-    if ((rc = dalloc(c.get(), (void **)&c->d_ks, nk * sizeof(double2)))) return rc;
-    if ((rc = dalloc(c.get(), (void **)&c->d_row_off, c->row_off.size() * sizeof(int32_t)))) return rc;
-    if ((rc = dalloc(c.get(), (void **)&c->d_row_kz, c->row_kz.size() * sizeof(int32_t)))) return rc;
-    upload(c.get(), c->d_kfactor, c->kfactor.data(), c->kfactor.size() * sizeof(double));
-    upload(c.get(), c->d_row_off, c->row_off.data(), c->row_off.size() * sizeof(int32_t));
-    upload(c.get(), c->d_row_kz, c->row_kz.data(), c->row_kz.size() * sizeof(int32_t));
+    if ((rc = dalloc(c.get(), (void **)&c->d_kst, order.size() * sizeof(double4)))) return rc;
+    if ((rc = upload_kept(c.get(), d_kt, std::move(kt.h)))) return rc;
     // framework structure factor of the charged atoms (src/ewald.c:1330-1345)
     std::vector<double> fx, fy, fz, qq;
     c->net_q_fw = 0.0;
@@ -1229,20 +1307,8 @@ int create_impl(const rhg_framework *fw, const rhg_cell *cellp, const rhg_forcef
       qq.push_back(c->q[i]);
       c->net_q_fw += c->q[i];
     }
-    if (c->kz > 64 || c->kx > kMaxKx) return RHG_ERR_UNSUPPORTED; // k ranges of the row kernel
     if ((rc = structure_factor(c.get(), fx, fy, fz, qq, c->d_sf))) return rc;
-    launch_ks_table(c->d_kfactor, c->d_sf, c->nk, c->d_ks, c->stream);
-    HIPC(c.get(), hipGetLastError());
-    // the k-vector records in the Fourier kernel's visiting order
-    std::vector<int32_t> order;
-    fourier_order(c.get(), order, c->kst_off);
-    if ((int)order.size() != c->nk) return fail(c.get(), RHG_ERR_INVALID, "k-vector visiting order");
-    order.resize(order.size() + 8, 0); // read-ahead padding
-    int32_t *d_order = nullptr;
-    if ((rc = dalloc(c.get(), (void **)&d_order, order.size() * sizeof(int32_t)))) return rc;
-    if ((rc = dalloc(c.get(), (void **)&c->d_kst, order.size() * sizeof(double4)))) return rc;
-    upload(c.get(), d_order, order.data(), order.size() * sizeof(int32_t));
-    launch_kstream(c->d_kfactor, c->d_ks, d_order, (int)order.size(), c->d_kst, c->stream);
+    launch_kstream(c->d_kfactor, c->d_sf, d_order, (int)order.size(), c->d_kst, c->stream);
     HIPC(c.get(), hipGetLastError());
   }
   *out = c.release();
@@ -1259,7 +1325,9 @@ int rhg_use_stream(rhg_ctx *c, hipStream_t stream) {
   if (!c) return RHG_ERR_INVALID;
   if (c->stream == stream) return RHG_OK;
   hipSetDevice(c->device);
-  if (c->stream) HIPC(c, hipStreamSynchronize(c->stream));
+  if (c->stream) {
+    if (int rs_ = sync_stream(c)) return rs_;
+  }
   if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
   c->stream = stream;
   c->own_stream = false;
@@ -1373,7 +1441,7 @@ int rhg_framework_energy_at_positions(rhg_ctx *c, int64_t n, const double *xyz, 
   if (u_vdw) HIPC(c, hipMemcpyAsync(u_vdw, dv, b, hipMemcpyDeviceToHost, c->stream));
   if (u_cc) HIPC(c, hipMemcpyAsync(u_cc, dc, b, hipMemcpyDeviceToHost, c->stream));
   if (overlap) HIPC(c, hipMemcpyAsync(overlap, dov, n, hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));
+  if (int rs_ = sync_stream(c)) return rs_;
   return RHG_OK;
 }
 
@@ -1467,7 +1535,7 @@ int rhg_framework_charge_energy_at_position(rhg_ctx *c, const double pos[3], int
   }
   HIPC(c, hipGetLastError());
   HIPC(c, hipMemcpyAsync(ucc, dv, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));
+  if (int rs_ = sync_stream(c)) return rs_;
   return RHG_OK;
 }
 
@@ -1942,7 +2010,7 @@ int widom_collect(rhg_ctx *c, rhg_widom_blocks *blocks) {
     ig[b] = c->ig_pending[b];
     c->ig_pending[b] = 0.0;
   }
-  HIPC(c, hipStreamSynchronize(c->stream));
+  if (int rs_ = sync_stream(c)) return rs_;
   if (blocks) {
     for (int b = 0; b < RHG_NR_BLOCKS; ++b) {
       blocks->count[b] += acc[10 + b];
@@ -2484,7 +2552,7 @@ int rhg_widom_run_trials(rhg_ctx *c, const rhg_molecule *mol, int64_t n, const d
   if (!drec) return fail(c, RHG_ERR_OOM, "rec buffer");
   launch_widom_records(a, B.w, drec, c->stream);
   HIPC(c, hipMemcpyAsync(rec, drec, (size_t)n * RHG_REC_N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));
+  if (int rs_ = sync_stream(c)) return rs_;
   return RHG_OK;
 }
 
@@ -2536,7 +2604,7 @@ int rhg_widom_retrace_trials(rhg_ctx *c, const rhg_molecule *mol, const int32_t 
   if (!drec) return fail(c, RHG_ERR_OOM, "rec buffer");
   launch_widom_records(a, B.w, drec, c->stream);
   HIPC(c, hipMemcpyAsync(rec, drec, (size_t)n * RHG_REC_N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream)); // hfb_h and ex are released on return
+  if (int rs_ = sync_stream(c)) return rs_; // hfb_h and ex are released on return
   return RHG_OK;
 }
 
@@ -2643,7 +2711,7 @@ int rhg_ewald_fourier_adsorbate(rhg_ctx *c, int64_t n_mol, int32_t n_sites, cons
   HIPC(c, hipGetLastError());
   if (d_host) HIPC(c, hipMemcpyAsync(d_host, dh, n_mol * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   if (d_ads) HIPC(c, hipMemcpyAsync(d_ads, da, n_mol * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));
+  if (int rs_ = sync_stream(c)) return rs_;
   return RHG_OK;
 }
 
@@ -2679,7 +2747,7 @@ int rhg_set_adsorbates(rhg_ctx *c, int32_t n_mol, const int32_t *start, const in
       c->ads_mol_cap = (size_t)n;
     }
     if (int rc = upload(c, c->d_ads_mol, mol.data(), (size_t)n * sizeof(int32_t))) return rc;
-    HIPC(c, hipStreamSynchronize(c->stream));
+    if (int rs_ = sync_stream(c)) return rs_;
     for (auto &t : c->types)
       if (int rc = ads_records(c, t)) return rc;
     // StoreTotalChargeAdsorbates, NetChargeAdsorbates
@@ -2738,7 +2806,7 @@ int rhg_inter_energy_at_positions(rhg_ctx *c, int64_t n, const double *xyz, int3
   HIPC(c, hipGetLastError());
   if (u_vdw) HIPC(c, hipMemcpyAsync(u_vdw, dv, b, hipMemcpyDeviceToHost, c->stream));
   if (u_cc) HIPC(c, hipMemcpyAsync(u_cc, dc, b, hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));
+  if (int rs_ = sync_stream(c)) return rs_;
   return RHG_OK;
 }
 
@@ -2815,7 +2883,7 @@ int rhg_ewald_fourier_move(rhg_ctx *c, int32_t n_new, const int32_t *new_types, 
   }
   HIPC(c, hipGetLastError());
   HIPC(c, hipMemcpyAsync(out, dr, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));
+  if (int rs_ = sync_stream(c)) return rs_;
   return RHG_OK;
 }
 
@@ -2859,7 +2927,7 @@ int rhg_surface_area(rhg_ctx *c, int32_t probe_type, double distance_factor, int
   HIPC(c, hipGetLastError());
   std::vector<unsigned long long> hc((size_t)c->n);
   HIPC(c, hipMemcpyAsync(hc.data(), dc, hc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));
+  if (int rs_ = sync_stream(c)) return rs_;
   // sum_i (counted/total) 4 pi d_i^2 in atom order (src/mc_moves.c:16478-16480)
   double sum = 0.0;
   for (int i = 0; i < c->n; ++i) {
@@ -2984,7 +3052,7 @@ int build_nodes(rhg_ctx *c, rhg_grid *g, GridArgs &a, const float4 *par) {
   if (pairs)
     HIPC(c, hipMemcpyAsync(cnt.data(), pairs, cnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                            c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));
+  if (int rs_ = sync_stream(c)) return rs_;
   unsigned long long total = 0;
   for (unsigned long long v : cnt) total += v;
   g->exec_pairs = pairs ? (double)total : (double)nodes * (double)c->n;
@@ -3103,7 +3171,7 @@ int interpolate(rhg_grid *g, int64_t n, const double *xyz, double charge, double
   }
   HIPC(c, hipGetLastError());
   HIPC(c, hipMemcpyAsync(u, du, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));
+  if (int rs_ = sync_stream(c)) return rs_;
   return RHG_OK;
 }
 } // namespace
@@ -3303,7 +3371,7 @@ int rhg_derivatives_at_positions_vdw(rhg_ctx *c, int32_t type, int64_t n, const 
   launch_derivatives(a, dx, n, dout, c->stream);
   HIPC(c, hipGetLastError());
   HIPC(c, hipMemcpyAsync(out, dout, (size_t)n * 14 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));
+  if (int rs_ = sync_stream(c)) return rs_;
   return RHG_OK;
 }
 
@@ -3334,7 +3402,7 @@ int rhg_derivatives_at_positions_real(rhg_ctx *c, int64_t n, const double *xyz, 
   HIPC(c, hipGetLastError());
   HIPC(c, hipMemcpyAsync(out, dout, (size_t)n * 14 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipMemcpyAsync(smallest_r, dr, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));
+  if (int rs_ = sync_stream(c)) return rs_;
   return RHG_OK;
 }
 
@@ -3377,7 +3445,7 @@ int rhg_widom_stats(rhg_ctx *c, double out[12], int32_t reset) {
     if (c->d_acc) fill(c->d_acc + 16, 9, 0.0, c->stream);
     if (pb.p) fill((double *)pb.p, 3 * kPairLanes, 0.0, c->stream); // all-zero bits
   }
-  HIPC(c, hipStreamSynchronize(c->stream));
+  if (int rs_ = sync_stream(c)) return rs_;
   for (int i = 0; i < 9; ++i) out[i] = st[i];
   for (int k = 0; k < 3; ++k) {
     unsigned long long t = 0;
@@ -3418,7 +3486,7 @@ int rhg_set_block_pockets(rhg_ctx *c, int32_t n, const double *xyz, const double
                           const double unit_box[9], int32_t invert) {
   if (!c || n < 0 || (n > 0 && (!xyz || !radius || !unit_box))) return RHG_ERR_INVALID;
   hipSetDevice(c->device);
-  HIPC(c, hipStreamSynchronize(c->stream));
+  if (int rs_ = sync_stream(c)) return rs_;
   dfree(c, c->d_pockets);
   c->d_pockets = nullptr;
   memset(&c->pockets, 0, sizeof(c->pockets));
@@ -3453,7 +3521,7 @@ int rhg_blocked_pocket(rhg_ctx *c, int64_t n, const double *xyz, uint8_t *blocke
   upload(c, dx, xyz, b);
   launch_blocked_pockets(c->pockets, dx, n, df, c->stream);
   HIPC(c, hipMemcpyAsync(blocked, df, (size_t)n, hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));
+  if (int rs_ = sync_stream(c)) return rs_;
   return RHG_OK;
 }
 
@@ -3498,7 +3566,9 @@ int rhg_attach_workspace(rhg_ctx *c, rhg_workspace *ws) {
   }
   if (c->ws) {
     // the old workspace may still be in use by this context's stream
-    if (c->stream) HIPC(c, hipStreamSynchronize(c->stream));
+    if (c->stream) {
+      if (int rs_ = sync_stream(c)) return rs_;
+    }
     if (c->ws->owner == c) c->ws->owner = nullptr;
     c->ws = nullptr;
   }
@@ -3536,7 +3606,7 @@ int rhg_stream_copy(int device, int64_t bytes, int32_t reps) {
 int rhg_synchronize(rhg_ctx *c) {
   if (!c) return RHG_ERR_INVALID;
   hipSetDevice(c->device);
-  HIPC(c, hipStreamSynchronize(c->stream));
+  if (int rs_ = sync_stream(c)) return rs_;
   return RHG_OK;
 }
 

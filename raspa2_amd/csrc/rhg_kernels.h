@@ -104,6 +104,18 @@ void launch_cell_neighbours(const Cell &cell, int nc0, int nc1, int nc2, double 
 void launch_scan_counts(const int32_t *counts, int n, int32_t *off, hipStream_t st);
 // pass 0 counts the images of each cell's neighbour atoms into counts, pass 1
 // writes them at off
+// Neighbour lists and image lists of every cell in one walk over the atoms,
+// entry for entry those of launch_cell_neighbours followed by
+// launch_image_list: pass 0 counts both into nb_counts and img_counts,
+// pass 1 writes them at nb_off and img_off
+void launch_cell_lists(const Cell &cell, const CellGrid &g, double reach, const double widths[3],
+                       double box_reach, const double *ax, const double *ay, const double *az, int n_atoms,
+                       int32_t *nb_counts, int32_t *img_counts, const int32_t *nb_off, int32_t *nb_idx,
+                       const int32_t *img_off, float4 *img, int pass, hipStream_t st);
+// exclusive scans of in0 and in1 (n counts each) into off0 and off1 (n + 1),
+// tot[0] and tot[1] the totals
+void launch_scan2(const int32_t *in0, int32_t *off0, const int32_t *in1, int32_t *off1, int n, int32_t *tot,
+                  hipStream_t st);
 // widths: perpendicular widths of the cell
 // out[c] = sum of a64[j].shift over the image list of cell c, c < g.ncell
 void launch_img_shift_sums(const CellGrid &g, const AtomRec64 *a64, double *out, hipStream_t st);
@@ -129,7 +141,6 @@ struct EwaldDev {
   double alpha, uion, net_q_fw, beta_unused;
   const double *kfactor;   // nk, RASPA2 order
   const double2 *sf;       // nk framework structure factors
-  const double2 *ks;       // nk: kfactor times sf (the host term's weights)
   const int32_t *row_off;  // (kx+1)*(2ky+1): nvec index of the row's first valid kz
   const int32_t *row_kz;   // (kx+1)*(2ky+1): largest valid |kz| of the row (-1 empty)
   // the k-vectors in the Fourier kernel's visiting order, {K Re S_fw,
@@ -143,14 +154,11 @@ struct EwaldDev {
 
 // structure factors of the charged framework atoms (fractional coordinates)
 // tab: scratch of n x ((kx+1) + (2ky+1) + (2kz+1)) double2 (phase powers)
-void launch_structure_factor(const EwaldDev &e, const int32_t *kidx, int nk, const double *fx,
-                             const double *fy, const double *fz, const double *q, int n,
-                             double2 *tab, double2 *out, hipStream_t st);
+void launch_structure_factor(const EwaldDev &e, const double *fx, const double *fy, const double *fz,
+                             const double *q, int n, double2 *tab, double2 *out, hipStream_t st);
 
-// ks[i] = kf[i] sf[i], i < nk
-void launch_ks_table(const double *kf, const double2 *sf, int nk, double2 *ks, hipStream_t st);
-// kst[i] = {ks[idx[i]].x, ks[idx[i]].y, kf[idx[i]], idx[i]}, i < n
-void launch_kstream(const double *kf, const double2 *ks, const int32_t *idx, int n, double4 *kst,
+// kst[i] = {kf[k] sf[k].x, kf[k] sf[k].y, kf[k], k} with k = idx[i], i < n
+void launch_kstream(const double *kf, const double2 *sf, const int32_t *idx, int n, double4 *kst,
                     hipStream_t st);
 
 // Fourier Delta of n molecules; positions pos[j][m] (site-major SoA of

@@ -511,6 +511,150 @@ void launch_image_list(const Cell &cell, const CellGrid &g, double reach, const 
                      pass);
 }
 
+// Neighbour lists and image lists of every cell in one walk over the atoms:
+// one wave per cell, lane = atom, so each cell's neighbour list comes out in
+// ascending atom order and its image list atom-major in that order, the
+// images of an atom in the fixed shift order, exactly as k_cell_neighbours
+// and then k_image_list over that neighbour list give them.  An atom is a
+// neighbour when its torus distance to the cell centre is at most reach; its
+// images are those k_image_list keeps (box2 > 0: within sqrt(box2) of the
+// cell's box, else within reach of the centre).  pass 0 counts both lists,
+// pass 1 writes them at nb_off[c] and img_off[c].
+__global__ __launch_bounds__(256) void k_cell_lists(const Cell cell, const CellGrid g, double reach2,
+                                                    double w0, double w1, double w2, double box2,
+                                                    double hx, double hy, double hz,
+                                                    const double *__restrict__ ax,
+                                                    const double *__restrict__ ay,
+                                                    const double *__restrict__ az, int n_atoms,
+                                                    int32_t *__restrict__ nb_counts,
+                                                    int32_t *__restrict__ img_counts,
+                                                    const int32_t *__restrict__ nb_off,
+                                                    int32_t *__restrict__ nb_idx,
+                                                    const int32_t *__restrict__ img_off,
+                                                    float4 *__restrict__ img, int pass) {
+  const int c = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  if (c >= g.ncell) return;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lower = (1ULL << lane) - 1ULL;
+  double cx, cy, cz;
+  cell_centre(cell, g, c, cx, cy, cz);
+  const double reach = sqrt(reach2);
+  int nb_base = pass ? nb_off[c] : 0, img_base = pass ? img_off[c] : 0;
+  for (int j0 = 0; j0 < n_atoms; j0 += 64) {
+    const int j = j0 + lane;
+    const double dx = j < n_atoms ? ax[j] - cx : 0.0, dy = j < n_atoms ? ay[j] - cy : 0.0,
+                 dz = j < n_atoms ? az[j] - cz : 0.0;
+    const bool in = j < n_atoms && torus_dist2(cell, dx, dy, dz) <= reach2;
+    int m = 0;
+    double s[3] = {0.0, 0.0, 0.0};
+    int cand[3][2] = {{0, 0}, {0, 0}, {0, 0}}, nc[3] = {1, 1, 1};
+    if (in) {
+      s[0] = cell.inv[AX] * dx + cell.inv[BX] * dy + cell.inv[CX] * dz;
+      s[1] = cell.inv[AY] * dx + cell.inv[BY] * dy + cell.inv[CY] * dz;
+      s[2] = cell.inv[AZ] * dx + cell.inv[BZ] * dy + cell.inv[CZ] * dz;
+      const double w[3] = {w0, w1, w2};
+      for (int d = 0; d < 3; ++d) {
+        s[d] -= rint(s[d]);
+        if ((1.0 - fabs(s[d])) * w[d] <= reach) cand[d][nc[d]++] = s[d] > 0.0 ? -1 : 1;
+      }
+    }
+    auto image = [&](int a, int b, int e, double &vx, double &vy, double &vz) {
+      const double tx = s[0] + cand[0][a], ty = s[1] + cand[1][b], tz = s[2] + cand[2][e];
+      vx = cell.box[AX] * tx + cell.box[BX] * ty + cell.box[CX] * tz;
+      vy = cell.box[AY] * tx + cell.box[BY] * ty + cell.box[CY] * tz;
+      vz = cell.box[AZ] * tx + cell.box[BZ] * ty + cell.box[CZ] * tz;
+      if (box2 > 0.0) {
+        const double ex = fmax(fabs(vx) - hx, 0.0), ey = fmax(fabs(vy) - hy, 0.0),
+                     ez = fmax(fabs(vz) - hz, 0.0);
+        return ex * ex + ey * ey + ez * ez <= box2;
+      }
+      return vx * vx + vy * vy + vz * vz <= reach2;
+    };
+    if (in)
+      for (int a = 0; a < nc[0]; ++a)
+        for (int b = 0; b < nc[1]; ++b)
+          for (int e = 0; e < nc[2]; ++e) {
+            double vx, vy, vz;
+            m += image(a, b, e, vx, vy, vz) ? 1 : 0;
+          }
+    const unsigned long long bal = __ballot(in);
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int bit = 0; bit < 4; ++bit) {
+      const unsigned long long v = __ballot((m >> bit) & 1);
+      pre += __popcll(v & lower) << bit;
+      tot += __popcll(v) << bit;
+    }
+    if (pass && in) {
+      nb_idx[nb_base + __popcll(bal & lower)] = j;
+      int o = img_base + pre;
+      for (int a = 0; a < nc[0]; ++a)
+        for (int b = 0; b < nc[1]; ++b)
+          for (int e = 0; e < nc[2]; ++e) {
+            double vx, vy, vz;
+            if (image(a, b, e, vx, vy, vz))
+              img[o++] = make_float4((float)vx, (float)vy, (float)vz, __int_as_float(j));
+          }
+    }
+    nb_base += __popcll(bal);
+    img_base += tot;
+  }
+  if (lane == 0 && !pass) {
+    nb_counts[c] = nb_base;
+    img_counts[c] = img_base;
+  }
+}
+
+void launch_cell_lists(const Cell &cell, const CellGrid &g, double reach, const double widths[3],
+                       double box_reach, const double *ax, const double *ay, const double *az, int n_atoms,
+                       int32_t *nb_counts, int32_t *img_counts, const int32_t *nb_off, int32_t *nb_idx,
+                       const int32_t *img_off, float4 *img, int pass, hipStream_t st) {
+  const bool rect = cell.bc == 1 && box_reach > 0.0;
+  const double box2 = rect ? box_reach * box_reach : 0.0;
+  const double hx = 0.5 * cell.box[AX] / g.nc[0], hy = 0.5 * cell.box[BY] / g.nc[1],
+               hz = 0.5 * cell.box[CZ] / g.nc[2];
+  hipLaunchKernelGGL(k_cell_lists, dim3((g.ncell + 3) / 4), dim3(256), 0, st, cell, g, reach * reach,
+                     widths[0], widths[1], widths[2], box2, hx, hy, hz, ax, ay, az, n_atoms, nb_counts,
+                     img_counts, nb_off, nb_idx, img_off, img, pass);
+}
+
+// Exclusive scans of two count arrays at once (blockIdx.x picks the pair),
+// each by one workgroup as k_scan; tot[b] receives array b's total.
+__global__ __launch_bounds__(1024) void k_scan2(const int32_t *__restrict__ in0, int32_t *__restrict__ off0,
+                                                const int32_t *__restrict__ in1, int32_t *__restrict__ off1,
+                                                int n, int32_t *__restrict__ tot) {
+  __shared__ int part[1024];
+  const int32_t *in = blockIdx.x ? in1 : in0;
+  int32_t *off = blockIdx.x ? off1 : off0;
+  const int t = threadIdx.x;
+  const int per = (n + blockDim.x - 1) / blockDim.x;
+  const int b = t * per, e = min(n, b + per);
+  int s = 0;
+  for (int i = b; i < e; ++i) s += in[i];
+  part[t] = s;
+  __syncthreads();
+  for (int d = 1; d < (int)blockDim.x; d <<= 1) {
+    int v = (t >= d) ? part[t - d] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = (t == 0) ? 0 : part[t - 1];
+  for (int i = b; i < e; ++i) {
+    off[i] = run;
+    run += in[i];
+  }
+  if (t == (int)blockDim.x - 1) {
+    off[n] = part[t];
+    tot[blockIdx.x] = part[t];
+  }
+}
+
+void launch_scan2(const int32_t *in0, int32_t *off0, const int32_t *in1, int32_t *off1, int n, int32_t *tot,
+                  hipStream_t st) {
+  hipLaunchKernelGGL(k_scan2, dim3(2), dim3(1024), 0, st, in0, off0, in1, off1, n, tot);
+}
+
 // Per cell, the FP64 sum of the LJ shifts of one pseudo-atom type over the
 // cell's image list: one workgroup per cell, each lane summing a strided part
 // of the list in order, then a fixed pairwise tree (the sum does not depend
@@ -1294,11 +1438,8 @@ __global__ __launch_bounds__(256) void k_sf_rows(const EwaldDev e, const double2
   }
 }
 
-void launch_structure_factor(const EwaldDev &e, const int32_t *kidx, int nk, const double *fx,
-                             const double *fy, const double *fz, const double *q, int n,
-                             double2 *tab, double2 *out, hipStream_t st) {
-  (void)kidx;
-  (void)nk;
+void launch_structure_factor(const EwaldDev &e, const double *fx, const double *fy, const double *fz,
+                             const double *q, int n, double2 *tab, double2 *out, hipStream_t st) {
   if (n > 0)
     hipLaunchKernelGGL(k_sf_tables, dim3((n + 255) / 256), dim3(256), 0, st, e.kx, e.ky, fx, fy, fz, n,
                        tab);
@@ -1306,15 +1447,7 @@ void launch_structure_factor(const EwaldDev &e, const int32_t *kidx, int nk, con
   hipLaunchKernelGGL(k_sf_rows, dim3(rows), dim3(256), 0, st, e, (const double2 *)tab, q, n, out);
 }
 
-__global__ __launch_bounds__(256) void k_ks_table(const double *__restrict__ kf, const double2 *__restrict__ sf,
-                                                  int nk, double2 *__restrict__ ks) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nk) ks[i] = make_double2(kf[i] * sf[i].x, kf[i] * sf[i].y);
-}
 
-void launch_ks_table(const double *kf, const double2 *sf, int nk, double2 *ks, hipStream_t st) {
-  if (nk > 0) hipLaunchKernelGGL(k_ks_table, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, st, kf, sf, nk, ks);
-}
 
 // ====================================================================
 // Fourier Delta of inserted molecules (src/ewald.c:6859-7565), one lane
@@ -1507,20 +1640,21 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k
 }
 
 // Gathers the k-vector records of the Fourier kernel in its visiting order:
-// kst[i] = {ks[idx[i]].x, ks[idx[i]].y, kf[idx[i]], idx[i]}, i < n.
-__global__ __launch_bounds__(256) void k_kstream(const double *__restrict__ kf, const double2 *__restrict__ ks,
+// kst[i] = {K S_fw.re, K S_fw.im, K, index} of k-vector idx[i], i < n.
+__global__ __launch_bounds__(256) void k_kstream(const double *__restrict__ kf, const double2 *__restrict__ sf,
                                                  const int32_t *__restrict__ idx, int n,
                                                  double4 *__restrict__ kst) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int k = idx[i];
-  const double2 v = ks[k];
-  kst[i] = make_double4(v.x, v.y, kf[k], (double)k);
+  const double2 v = sf[k];
+  const double f = kf[k];
+  kst[i] = make_double4(f * v.x, f * v.y, f, (double)k);
 }
 
-void launch_kstream(const double *kf, const double2 *ks, const int32_t *idx, int n, double4 *kst,
+void launch_kstream(const double *kf, const double2 *sf, const int32_t *idx, int n, double4 *kst,
                     hipStream_t st) {
-  if (n > 0) hipLaunchKernelGGL(k_kstream, dim3((n + 255) / 256), dim3(256), 0, st, kf, ks, idx, n, kst);
+  if (n > 0) hipLaunchKernelGGL(k_kstream, dim3((n + 255) / 256), dim3(256), 0, st, kf, sf, idx, n, kst);
 }
 
 // Sums the kx slices in order and adds the excluded-pair, self and
