@@ -259,6 +259,11 @@ struct rhg_ctx {
   // slice's first record
   double4 *d_kst = nullptr;
   std::vector<int32_t> kst_off;
+  // the framework structure factor and the Fourier records are formed on the
+  // first Fourier use (a request with uncharged probes never needs them)
+  const int32_t *d_korder = nullptr;
+  int korder_n = 0;
+  bool sf_ready = false;
   // host copies of uploads that may still be in flight, released after the
   // next synchronisation of the stream (so that set-up need not wait for them)
   std::vector<std::vector<char>> kept;
@@ -430,11 +435,14 @@ void arena_release(rhg_ctx *c) {
 }
 
 // grow-only named work buffer, in the attached workspace if there is one
-// (the context's own counters always stay with the context)
-void *work(rhg_ctx *c, const char *name, size_t bytes) {
+// (the context's own counters always stay with the context); *fresh
+// (nullable) tells whether the buffer was (re)allocated by this call
+void *work(rhg_ctx *c, const char *name, size_t bytes, bool *fresh = nullptr) {
   const bool shared = c->ws && strcmp(name, "pair_counter") != 0;
   DevBuf &b = shared ? c->ws->bufs[name] : c->bufs[name];
+  if (fresh) *fresh = false;
   if (b.bytes >= bytes && b.p) return b.p;
+  if (fresh) *fresh = true;
   size_t want = std::max<size_t>(bytes, 256);
   if (b.p) {
     // regrowth: leave headroom so that a slowly growing size settles
@@ -1095,7 +1103,10 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
   b.waves = c->waves;
   b.cell_of = (int32_t *)work(c, (t + "_cellof").c_str(), (size_t)s.n * sizeof(int32_t));
   b.perm = (int32_t *)work(c, (t + "_perm").c_str(), (size_t)s.n * sizeof(int32_t));
-  b.count = (int32_t *)work(c, (t + "_count").c_str(), (size_t)ncell * sizeof(int32_t));
+  bool fresh_count = false;
+  b.count = (int32_t *)work(c, (t + "_count").c_str(), (size_t)ncell * sizeof(int32_t), &fresh_count);
+  // a reused count buffer was cleared by the scan of its last binning
+  b.counts_zero = fresh_count ? 0 : 1;
   b.rank = (int32_t *)work(c, (t + "_rank").c_str(), (size_t)s.n * sizeof(int32_t));
   b.start = (int32_t *)work(c, (t + "_start").c_str(), (size_t)(ncell + 1) * sizeof(int32_t));
   b.tile_off = (int32_t *)work(c, (t + "_tiles").c_str(), (size_t)(ncell + 1) * sizeof(int32_t));
@@ -1142,6 +1153,30 @@ int structure_factor(rhg_ctx *c, const std::vector<double> &fx, const std::vecto
                             c->stream);
   }
   HIPC(c, hipGetLastError());
+  return RHG_OK;
+}
+
+// The framework structure factor of the charged atoms (src/ewald.c:1330-1345)
+// and the Fourier kernel's k records, formed once, on the first Fourier use.
+int ensure_fourier(rhg_ctx *c) {
+  if (c->sf_ready || c->charge_method != RHG_CHARGE_EWALD) return RHG_OK;
+  std::vector<double> fx, fy, fz, qq;
+  const double *inv = c->cell.inv;
+  for (int i = 0; i < c->n; ++i) {
+    if (!considered_charged(c->q[i])) continue;
+    double p[3] = {c->x[i], c->y[i], c->z[i]};
+    double a = inv[AX] * p[0] + inv[BX] * p[1] + inv[CX] * p[2];
+    double b = inv[AY] * p[0] + inv[BY] * p[1] + inv[CY] * p[2];
+    double d = inv[AZ] * p[0] + inv[BZ] * p[1] + inv[CZ] * p[2];
+    fx.push_back(a * (2.0 * M_PI));
+    fy.push_back(b * (2.0 * M_PI));
+    fz.push_back(d * (2.0 * M_PI));
+    qq.push_back(c->q[i]);
+  }
+  if (int rc = structure_factor(c, fx, fy, fz, qq, c->d_sf)) return rc;
+  launch_kstream(c->d_kfactor, c->d_sf, c->d_korder, c->korder_n, c->d_kst, c->stream);
+  HIPC(c, hipGetLastError());
+  c->sf_ready = true;
   return RHG_OK;
 }
 
@@ -1291,25 +1326,11 @@ int create_impl(const rhg_framework *fw, const rhg_cell *cellp, const rhg_forcef
     if ((rc = dalloc(c.get(), (void **)&c->d_sf, nk * sizeof(double2)))) return rc;
     if ((rc = dalloc(c.get(), (void **)&c->d_kst, order.size() * sizeof(double4)))) return rc;
     if ((rc = upload_kept(c.get(), d_kt, std::move(kt.h)))) return rc;
-    // framework structure factor of the charged atoms (src/ewald.c:1330-1345)
-    std::vector<double> fx, fy, fz, qq;
+    c->d_korder = d_order;
+    c->korder_n = (int)order.size();
     c->net_q_fw = 0.0;
-    const double *inv = c->cell.inv;
-    for (int i = 0; i < c->n; ++i) {
-      if (!considered_charged(c->q[i])) continue;
-      double p[3] = {c->x[i], c->y[i], c->z[i]};
-      double a = inv[AX] * p[0] + inv[BX] * p[1] + inv[CX] * p[2];
-      double b = inv[AY] * p[0] + inv[BY] * p[1] + inv[CY] * p[2];
-      double d = inv[AZ] * p[0] + inv[BZ] * p[1] + inv[CZ] * p[2];
-      fx.push_back(a * (2.0 * M_PI));
-      fy.push_back(b * (2.0 * M_PI));
-      fz.push_back(d * (2.0 * M_PI));
-      qq.push_back(c->q[i]);
-      c->net_q_fw += c->q[i];
-    }
-    if ((rc = structure_factor(c.get(), fx, fy, fz, qq, c->d_sf))) return rc;
-    launch_kstream(c->d_kfactor, c->d_sf, d_order, (int)order.size(), c->d_kst, c->stream);
-    HIPC(c.get(), hipGetLastError());
+    for (int i = 0; i < c->n; ++i)
+      if (considered_charged(c->q[i])) c->net_q_fw += c->q[i];
   }
   *out = c.release();
   return RHG_OK;
@@ -1545,8 +1566,10 @@ int rhg_ewald_tables(rhg_ctx *c, double *kfactor, double *sf_re, double *sf_im) 
   hipSetDevice(c->device);
   if (kfactor) memcpy(kfactor, c->kfactor.data(), c->nk * sizeof(double));
   if (sf_re || sf_im) {
+    if (int rc = ensure_fourier(c)) return rc;
     std::vector<double2> h(c->nk);
-    HIPC(c, hipMemcpy(h.data(), c->d_sf, c->nk * sizeof(double2), hipMemcpyDeviceToHost));
+    HIPC(c, hipMemcpyAsync(h.data(), c->d_sf, c->nk * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+    if (int rs_ = sync_stream(c)) return rs_;
     for (int k = 0; k < c->nk; ++k) {
       if (sf_re) sf_re[k] = h[k].x;
       if (sf_im) sf_im[k] = h[k].y;
@@ -1732,8 +1755,8 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
   WidomWork &w = B.w;
   const int64_t n = a.n;
   hipStream_t st = c->stream;
-  launch_zero_counters(w.n_active, w.n_active2, st);
   {
+    // the generator also clears the selection kernels' slot counters
     Timed t(c, "widom_gen");
     launch_widom_fb_gen(a, w, st);
   }
@@ -1834,6 +1857,7 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
     f.sads = c->ads_n > 0 ? c->d_sads : nullptr;
     f.net_q_ads = c->net_q_ads;
     if (f.nc > 0) {
+      if (int rc = ensure_fourier(c)) return rc;
       double2 *part = (double2 *)work(c, "w_fpart", (size_t)(c->kx + 1) * w.cap * sizeof(double2));
       if (!part) return fail(c, RHG_ERR_OOM, "fourier partials");
       Timed t(c, "fourier");
@@ -1849,10 +1873,11 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
   }
   if (accumulate) {
     Timed t(c, "block_reduce");
-    launch_block_reduce(a, w.W, w.dU, B.d_part, kSeg, c->d_acc, st);
-    launch_widom_stats(a, w, stat_args(c, mol, m, a.ewald), c->d_acc + 16, st);
+    launch_chunk_tail(a, w, stat_args(c, mol, m, a.ewald), w.W, w.dU, B.d_part, kSeg, c->d_acc,
+                      c->d_acc + 16, c->d_ctl, n, st);
+  } else {
+    launch_advance(c->d_ctl, n, st);
   }
-  launch_advance(c->d_ctl, n, st);
   if (hipGetLastError() != hipSuccess) return fail(c, RHG_ERR_HIP, "widom launch failed");
   return RHG_OK;
 }
@@ -2676,6 +2701,7 @@ int rhg_ewald_fourier_adsorbate(rhg_ctx *c, int64_t n_mol, int32_t n_sites, cons
   upload(c, dx, px.data(), b);
   upload(c, dy, py.data(), b);
   upload(c, dz, pz.data(), b);
+  if (int rc = ensure_fourier(c)) return rc;
   FourierArgs f;
   memset(&f, 0, sizeof(f));
   f.e = ewald_dev(c);
@@ -2856,6 +2882,7 @@ int rhg_ewald_fourier_move(rhg_ctx *c, int32_t n_new, const int32_t *new_types, 
   upload(c, dx, px.data(), b);
   upload(c, dy, py.data(), b);
   upload(c, dz, pz.data(), b);
+  if (int rc = ensure_fourier(c)) return rc;
   FourierArgs f;
   memset(&f, 0, sizeof(f));
   f.e = ewald_dev(c);
@@ -3035,7 +3062,9 @@ int build_nodes(rhg_ctx *c, rhg_grid *g, GridArgs &a, const float4 *par) {
     const int ncell = c->cg.ncell;
     b.cell_of = (int32_t *)work(c, "g_cellof", (size_t)step * sizeof(int32_t));
     b.perm = (int32_t *)work(c, "g_perm", (size_t)step * sizeof(int32_t));
-    b.count = (int32_t *)work(c, "g_count", (size_t)ncell * sizeof(int32_t));
+    bool fresh_count = false;
+    b.count = (int32_t *)work(c, "g_count", (size_t)ncell * sizeof(int32_t), &fresh_count);
+    b.counts_zero = fresh_count ? 0 : 1;
     b.rank = (int32_t *)work(c, "g_rank", (size_t)step * sizeof(int32_t));
     b.start = (int32_t *)work(c, "g_start", (size_t)(ncell + 1) * sizeof(int32_t));
     b.tile_off = (int32_t *)work(c, "g_tiles", (size_t)(ncell + 1) * sizeof(int32_t));

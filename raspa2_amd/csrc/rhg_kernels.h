@@ -70,7 +70,7 @@ constexpr int kPairLanes = 256; // counters per kind of the executed-pair tallie
 // scratch for binning one site list into cells
 struct BinWork {
   int32_t *cell_of;  // per site
-  int32_t *count;    // per cell
+  int32_t *count;    // per cell (cleared by the scan once read)
   int32_t *rank;     // per site: its place among its cell's sites
   int32_t *start;    // per cell, exclusive scan of count
   int32_t *tile_off; // per cell + 1, exclusive scan of ceil(count/ts): ts = 256
@@ -92,6 +92,9 @@ struct BinWork {
   // for it) and waves per workgroup (1, 2 or 4; 0: 2)
   int32_t tile_sites = 0;
   int32_t waves = 0;
+  // the counts are known to be zero (fresh scratch cleared at allocation, or
+  // left cleared by the previous binning's scan): no clearing launch
+  int32_t counts_zero = 0;
 };
 
 __host__ __device__ inline int img_tile_sites(const BinWork &b) { return b.tile_sites ? b.tile_sites : 64 * kImgLane; }
@@ -324,6 +327,11 @@ void launch_widom_stats(const WidomArgs &a, const WidomWork &w, const StatArgs &
                         hipStream_t st);
 // ctl[CTL_FIRST] += step (moves a captured chunk on to the next insertions)
 void launch_advance(int64_t *ctl, int64_t step, hipStream_t st);
+// the block reduction, the work counters and the advance of a chunk in two
+// launches (k_block_part, then k_chunk_tail in one workgroup)
+void launch_chunk_tail(const WidomArgs &a, const WidomWork &w, const StatArgs &sa, const double *W,
+                       const double *dU, double *part, int seg, double *acc, double *stats, int64_t *ctl,
+                       int64_t step, hipStream_t st);
 
 // streaming copy of `bytes` (a multiple of 16) for byte-counter calibration
 void launch_stream_copy(const void *src, void *dst, int64_t bytes, hipStream_t st);

@@ -694,6 +694,9 @@ __device__ __forceinline__ int cell_index(const Cell &c, const CellGrid &g, doub
   return (ci[0] * g.nc[1] + ci[1]) * g.nc[2] + ci[2];
 }
 
+// Sites binned into cell c (the counts themselves are cleared by the scan).
+__device__ __forceinline__ int cell_sites(const BinWork &b, int c) { return b.start[c + 1] - b.start[c]; }
+
 // A slot of a compacted list for each lane that calls this: one atomic per
 // wave and consecutive slots for the calling lanes in lane order, so that
 // the rows a wave writes by slot are contiguous (coalesced stores).
@@ -807,9 +810,13 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const Cell cell, const Cell
 
 // exclusive scans of the per-cell site counts (start) and of the per-cell
 // tile counts ceil(count/ts) (tile_off), one workgroup
-__global__ __launch_bounds__(1024) void k_scan_bins(const int32_t *__restrict__ in, int n,
+// The counts are cleared once read (so that the next binning with this
+// scratch starts from zero without a clearing launch), and so is the count
+// of sites to recompute exactly (n_redo, nullable).
+__global__ __launch_bounds__(1024) void k_scan_bins(int32_t *__restrict__ in, int n,
                                                     int32_t *__restrict__ start,
-                                                    int32_t *__restrict__ tiles, int ts) {
+                                                    int32_t *__restrict__ tiles, int ts,
+                                                    int32_t *__restrict__ n_redo) {
   __shared__ int ps[1024], pt[1024];
   const int t = threadIdx.x;
   const int per = (n + blockDim.x - 1) / blockDim.x;
@@ -831,25 +838,31 @@ __global__ __launch_bounds__(1024) void k_scan_bins(const int32_t *__restrict__ 
   }
   int rs = (t == 0) ? 0 : ps[t - 1], rt = (t == 0) ? 0 : pt[t - 1];
   for (int i = b; i < e; ++i) {
+    const int v = in[i];
     start[i] = rs;
     tiles[i] = rt;
-    rs += in[i];
-    rt += (in[i] + ts - 1) / ts;
+    rs += v;
+    rt += (v + ts - 1) / ts;
+    in[i] = 0;
   }
   if (t == (int)blockDim.x - 1) {
     start[n] = ps[t];
     tiles[n] = pt[t];
   }
+  if (t == 0 && n_redo) *n_redo = 0;
 }
 
 void launch_bin_sites(const Cell &cell, const CellGrid &g, const SiteList &s, const BinWork &b,
                       hipStream_t st) {
-  const unsigned nbc = (unsigned)((g.ncell + 255) / 256);
-  hipLaunchKernelGGL(k_zero_i32, dim3(nbc), dim3(256), 0, st, b.count, g.ncell, b.n_redo);
+  // counts left by an earlier binning are zero (the scan clears them)
+  if (!b.counts_zero) {
+    const unsigned nbc = (unsigned)((g.ncell + 255) / 256);
+    hipLaunchKernelGGL(k_zero_i32, dim3(nbc), dim3(256), 0, st, b.count, g.ncell, b.n_redo);
+  }
   const unsigned nb = (unsigned)((s.n + 255) / 256);
   if (nb) hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(256), 0, st, cell, g, s, b);
   hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(1024), 0, st, b.count, g.ncell, b.start, b.tile_off,
-                     b.fix ? img_tile_sites(b) : 64);
+                     b.fix ? img_tile_sites(b) : 64, b.n_redo);
   if (nb) {
     if (cell.bc == 1)
       hipLaunchKernelGGL(k_bin_scatter<1>, dim3(nb), dim3(256), 0, st, cell, g, s, b);
@@ -881,7 +894,7 @@ __global__ __launch_bounds__(256) void k_site_energy_culled(
   }
   const int c = __builtin_amdgcn_readfirstlane(lo);
   const int k = (tile - b.tile_off[c]) * 64 + lane;
-  const bool live = k < b.count[c];
+  const bool live = k < cell_sites(b, c);
   const int site = live ? b.perm[b.start[c] + k] : b.perm[b.start[c]];
   const double px = s.x[site], py = s.y[site], pz = s.z[site];
   const int j0 = g.nb_off[c], j1 = g.nb_off[c + 1];
@@ -952,7 +965,7 @@ __global__ __launch_bounds__(256) void k_site_energy_culled(
     }
   }
   if (lane == 0 && b.pairs) {
-    const int nl = min(64, b.count[c] - (tile - b.tile_off[c]) * 64);
+    const int nl = min(64, cell_sites(b, c) - (tile - b.tile_off[c]) * 64);
     // counters by pair kind: 0 LJ, 1 LJ+Coulomb, 2 Coulomb
     atomicAdd(b.pairs + ((VDW && CC) ? 1 : (VDW ? 0 : 2)) * kPairLanes + (blockIdx.x & (kPairLanes - 1)),
               (unsigned long long)nl * (unsigned long long)(j1 - j0));
@@ -1100,7 +1113,7 @@ __global__ __launch_bounds__(256) void k_site_energy_img(
   const int c = __builtin_amdgcn_readfirstlane(lo);
   const int ts = img_tile_sites(b);
   const int first = (tile - b.tile_off[c]) * ts;
-  const int nl = min(ts, b.count[c] - first); // live sites of the tile
+  const int nl = min(ts, cell_sites(b, c) - first); // live sites of the tile
   // sorted offsets from the cell centre (written by the binning)
   float4 so[NSM];
   bool live[NSM];
@@ -1814,7 +1827,12 @@ __global__ __launch_bounds__(256) void k_blocked_pockets(const PocketArgs p, con
 __global__ __launch_bounds__(256) void k_widom_fb_gen(const WidomArgs a, const WidomWork w) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t n = chunk_n(a);
-  if (i == 0 && a.n_eff) *a.n_eff = (int32_t)n;
+  if (i == 0) {
+    if (a.n_eff) *a.n_eff = (int32_t)n;
+    // the compaction counters of the selection kernels that follow
+    if (w.n_active) *w.n_active = 0;
+    if (w.n_active2) *w.n_active2 = 0;
+  }
   if (i >= n) return;
   const int k = a.mol.k_fb;
   const int64_t first = chunk_first(a);
@@ -2369,6 +2387,65 @@ __global__ __launch_bounds__(256) void k_block_final(const WidomArgs a, const do
   }
 }
 
+// The end of a Widom chunk in one workgroup: the block sums of the segment
+// partials (k_block_final), the chunk's work counters (k_widom_stats), then
+// the control block's advance to the next chunk (k_advance), after every
+// thread has read it.
+__global__ __launch_bounds__(256) void k_chunk_tail(const WidomArgs a, const WidomWork w, const StatArgs sa,
+                                                    const double *__restrict__ part, int seg,
+                                                    double *__restrict__ acc15, double *__restrict__ stats,
+                                                    int64_t *ctl, int64_t step) {
+  const int64_t *jb = a.ctl + CTL_JB;
+  __shared__ double sh[256][10];
+  const int64_t n = chunk_n(a);
+  const int64_t first = chunk_first(a);
+  const int nseg = (int)((n + seg - 1) / seg);
+  double acc[10];
+  for (int q = 0; q < 10; ++q) acc[q] = 0.0;
+  for (int s = threadIdx.x; s < nseg; s += blockDim.x)
+    for (int q = 0; q < 10; ++q) acc[q] += part[(int64_t)s * 10 + q];
+  for (int q = 0; q < 10; ++q) sh[threadIdx.x][q] = acc[q];
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int q = 0; q < 10; ++q) sh[threadIdx.x][q] += sh[threadIdx.x + s][q];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  for (int b = 0; b < 5; ++b) {
+    acc15[b] += sh[0][2 * b];
+    acc15[5 + b] += sh[0][2 * b + 1];
+    int64_t lo = jb[b] > first ? jb[b] : first;
+    int64_t hi = jb[b + 1] < first + n ? jb[b + 1] : first + n;
+    acc15[10 + b] += hi > lo ? (double)(hi - lo) : 0.0;
+  }
+  const double nd = (double)n;
+  const double na1 = sa.multi ? (double)*w.n_active : 0.0;
+  const double na2 = (double)*w.n_active2;
+  const double fb = nd * sa.k_fb;
+  const double orr = na1 * sa.k_or * sa.n_place;
+  const double cc = sa.multi ? na2 * sa.n_cc_place : 0.0;
+  stats[0] += nd;
+  stats[1] += fb;
+  stats[2] += orr;
+  stats[3] += cc;
+  stats[4] += (sa.ewald && sa.n_charged) ? na2 : 0.0;
+  stats[5] += (sa.fb_charged ? 0.0 : fb * sa.n_atoms) + orr * sa.n_atoms;
+  stats[6] += sa.fb_charged ? fb * sa.n_atoms : 0.0;
+  stats[7] += cc * sa.n_atoms;
+  stats[8] += (sa.ewald && sa.n_charged) ? na2 * sa.n_charged * sa.nk : 0.0;
+  ctl[CTL_FIRST] += step;
+}
+
+void launch_chunk_tail(const WidomArgs &a, const WidomWork &w, const StatArgs &sa, const double *W,
+                       const double *dU, double *part, int seg, double *acc, double *stats, int64_t *ctl,
+                       int64_t step, hipStream_t st) {
+  const int nseg = (int)((a.n + seg - 1) / seg);
+  if (nseg > 0) hipLaunchKernelGGL(k_block_part, dim3(nseg), dim3(256), 0, st, a, W, dU, seg, part);
+  hipLaunchKernelGGL(k_chunk_tail, dim3(1), dim3(256), 0, st, a, w, sa, (const double *)part, seg, acc, stats,
+                     ctl, step);
+}
+
 void launch_block_reduce(const WidomArgs &a, const double *W, const double *dU, double *part, int seg,
                          double *acc, hipStream_t st) {
   const int nseg = (int)((a.n + seg - 1) / seg);
@@ -2745,7 +2822,7 @@ __global__ __launch_bounds__(256) void k_grid_culled(const GridArgs g, const Cel
   }
   const int c = __builtin_amdgcn_readfirstlane(lo);
   const int k = (tile - b.tile_off[c]) * 64 + lane;
-  const bool live = k < b.count[c];
+  const bool live = k < cell_sites(b, c);
   const int site = live ? b.perm[b.start[c] + k] : b.perm[b.start[c]];
   const double px = s.x[site], py = s.y[site], pz = s.z[site];
   double o[8];
@@ -2811,7 +2888,7 @@ __global__ __launch_bounds__(256) void k_grid_culled(const GridArgs g, const Cel
   if (lane == 0 && b.pairs) {
     // executed (node, list entry) evaluations: image entries in the mixed
     // form, neighbour atoms otherwise
-    const int nl = min(64, b.count[c] - (tile - b.tile_off[c]) * 64);
+    const int nl = min(64, cell_sites(b, c) - (tile - b.tile_off[c]) * 64);
     const int len = PREC == 1 ? cg.img_off[c + 1] - cg.img_off[c] : cg.nb_off[c + 1] - cg.nb_off[c];
     atomicAdd(b.pairs + (PREC == 2 ? 1 : 0) * kPairLanes + (blockIdx.x & (kPairLanes - 1)),
               (unsigned long long)nl * (unsigned long long)len);
