@@ -519,7 +519,9 @@ void launch_image_list(const Cell &cell, const CellGrid &g, double reach, const 
 // neighbour when its torus distance to the cell centre is at most reach; its
 // images are those k_image_list keeps (box2 > 0: within sqrt(box2) of the
 // cell's box, else within reach of the centre).  pass 0 counts both lists,
-// pass 1 writes them at nb_off[c] and img_off[c].
+// pass 1 writes them at nb_off[c] and img_off[c].  RECT (rectangular cells):
+// the same distances per axis, without the zero off-diagonal products.
+template <int RECT>
 __global__ __launch_bounds__(256) void k_cell_lists(const Cell cell, const CellGrid g, double reach2,
                                                     double w0, double w1, double w2, double box2,
                                                     double hx, double hy, double hz,
@@ -544,25 +546,43 @@ __global__ __launch_bounds__(256) void k_cell_lists(const Cell cell, const CellG
     const int j = j0 + lane;
     const double dx = j < n_atoms ? ax[j] - cx : 0.0, dy = j < n_atoms ? ay[j] - cy : 0.0,
                  dz = j < n_atoms ? az[j] - cz : 0.0;
-    const bool in = j < n_atoms && torus_dist2(cell, dx, dy, dz) <= reach2;
-    int m = 0;
     double s[3] = {0.0, 0.0, 0.0};
+    bool in;
+    if (RECT) {
+      // the nearest image per axis is the torus distance's image
+      s[0] = cell.inv[AX] * dx;
+      s[1] = cell.inv[BY] * dy;
+      s[2] = cell.inv[CZ] * dz;
+      for (int d = 0; d < 3; ++d) s[d] -= rint(s[d]);
+      const double x = cell.box[AX] * s[0], y = cell.box[BY] * s[1], z = cell.box[CZ] * s[2];
+      in = j < n_atoms && x * x + y * y + z * z <= reach2;
+    } else {
+      in = j < n_atoms && torus_dist2(cell, dx, dy, dz) <= reach2;
+    }
+    int m = 0;
     int cand[3][2] = {{0, 0}, {0, 0}, {0, 0}}, nc[3] = {1, 1, 1};
     if (in) {
-      s[0] = cell.inv[AX] * dx + cell.inv[BX] * dy + cell.inv[CX] * dz;
-      s[1] = cell.inv[AY] * dx + cell.inv[BY] * dy + cell.inv[CY] * dz;
-      s[2] = cell.inv[AZ] * dx + cell.inv[BZ] * dy + cell.inv[CZ] * dz;
-      const double w[3] = {w0, w1, w2};
-      for (int d = 0; d < 3; ++d) {
-        s[d] -= rint(s[d]);
-        if ((1.0 - fabs(s[d])) * w[d] <= reach) cand[d][nc[d]++] = s[d] > 0.0 ? -1 : 1;
+      if (!RECT) {
+        s[0] = cell.inv[AX] * dx + cell.inv[BX] * dy + cell.inv[CX] * dz;
+        s[1] = cell.inv[AY] * dx + cell.inv[BY] * dy + cell.inv[CY] * dz;
+        s[2] = cell.inv[AZ] * dx + cell.inv[BZ] * dy + cell.inv[CZ] * dz;
+        for (int d = 0; d < 3; ++d) s[d] -= rint(s[d]);
       }
+      const double w[3] = {w0, w1, w2};
+      for (int d = 0; d < 3; ++d)
+        if ((1.0 - fabs(s[d])) * w[d] <= reach) cand[d][nc[d]++] = s[d] > 0.0 ? -1 : 1;
     }
     auto image = [&](int a, int b, int e, double &vx, double &vy, double &vz) {
       const double tx = s[0] + cand[0][a], ty = s[1] + cand[1][b], tz = s[2] + cand[2][e];
-      vx = cell.box[AX] * tx + cell.box[BX] * ty + cell.box[CX] * tz;
-      vy = cell.box[AY] * tx + cell.box[BY] * ty + cell.box[CY] * tz;
-      vz = cell.box[AZ] * tx + cell.box[BZ] * ty + cell.box[CZ] * tz;
+      if (RECT) {
+        vx = cell.box[AX] * tx;
+        vy = cell.box[BY] * ty;
+        vz = cell.box[CZ] * tz;
+      } else {
+        vx = cell.box[AX] * tx + cell.box[BX] * ty + cell.box[CX] * tz;
+        vy = cell.box[AY] * tx + cell.box[BY] * ty + cell.box[CY] * tz;
+        vz = cell.box[AZ] * tx + cell.box[BZ] * ty + cell.box[CZ] * tz;
+      }
       if (box2 > 0.0) {
         const double ex = fmax(fabs(vx) - hx, 0.0), ey = fmax(fabs(vy) - hy, 0.0),
                      ez = fmax(fabs(vz) - hz, 0.0);
@@ -613,9 +633,14 @@ void launch_cell_lists(const Cell &cell, const CellGrid &g, double reach, const 
   const double box2 = rect ? box_reach * box_reach : 0.0;
   const double hx = 0.5 * cell.box[AX] / g.nc[0], hy = 0.5 * cell.box[BY] / g.nc[1],
                hz = 0.5 * cell.box[CZ] / g.nc[2];
-  hipLaunchKernelGGL(k_cell_lists, dim3((g.ncell + 3) / 4), dim3(256), 0, st, cell, g, reach * reach,
-                     widths[0], widths[1], widths[2], box2, hx, hy, hz, ax, ay, az, n_atoms, nb_counts,
-                     img_counts, nb_off, nb_idx, img_off, img, pass);
+  if (cell.bc == 1)
+    hipLaunchKernelGGL(k_cell_lists<1>, dim3((g.ncell + 3) / 4), dim3(256), 0, st, cell, g, reach * reach,
+                       widths[0], widths[1], widths[2], box2, hx, hy, hz, ax, ay, az, n_atoms, nb_counts,
+                       img_counts, nb_off, nb_idx, img_off, img, pass);
+  else
+    hipLaunchKernelGGL(k_cell_lists<0>, dim3((g.ncell + 3) / 4), dim3(256), 0, st, cell, g, reach * reach,
+                       widths[0], widths[1], widths[2], box2, hx, hy, hz, ax, ay, az, n_atoms, nb_counts,
+                       img_counts, nb_off, nb_idx, img_off, img, pass);
 }
 
 // Exclusive scans of two count arrays at once (blockIdx.x picks the pair),
