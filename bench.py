@@ -708,7 +708,7 @@ def main():
                          "legs (profiles of the headline steps)")
     ap.add_argument("--frameworks", type=int, default=256, help="c3/c4 catalogue size")
     ap.add_argument("--requests", type=int, default=384, help="c4 requests")
-    ap.add_argument("--slots", type=int, default=8, help="c3/c4 requests in flight")
+    ap.add_argument("--slots", type=int, default=4, help="c3/c4 requests in flight")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the Widom "

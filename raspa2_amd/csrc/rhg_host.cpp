@@ -434,21 +434,24 @@ void arena_release(rhg_ctx *c) {
   c->arena.pool = P;
 }
 
+void fill(double *p, int64_t n, double v, hipStream_t st);
+
 // grow-only named work buffer, in the attached workspace if there is one
-// (the context's own counters always stay with the context); *fresh
-// (nullable) tells whether the buffer was (re)allocated by this call
-void *work(rhg_ctx *c, const char *name, size_t bytes, bool *fresh = nullptr) {
+// (the context's own counters always stay with the context).  zeroed: a
+// (re)allocated buffer is cleared in full, headroom included, on the
+// context's stream (buffers whose users keep them cleared, such as the
+// per-cell counts of the binning, rely on every byte starting at zero)
+void *work(rhg_ctx *c, const char *name, size_t bytes, bool zeroed = false) {
   const bool shared = c->ws && strcmp(name, "pair_counter") != 0;
   DevBuf &b = shared ? c->ws->bufs[name] : c->bufs[name];
-  if (fresh) *fresh = false;
   if (b.bytes >= bytes && b.p) return b.p;
-  if (fresh) *fresh = true;
   size_t want = std::max<size_t>(bytes, 256);
   if (b.p) {
     // regrowth: leave headroom so that a slowly growing size settles
     want = std::max(want, b.bytes + b.bytes / 2);
     b.retired.push_back({b.p, b.bytes});
   }
+  want = (want + 255) & ~(size_t)255;
   b.p = nullptr;
   b.bytes = 0;
   if (shared)
@@ -461,6 +464,7 @@ void *work(rhg_ctx *c, const char *name, size_t bytes, bool *fresh = nullptr) {
     return nullptr;
   }
   b.bytes = want;
+  if (zeroed) fill((double *)b.p, (int64_t)(want / sizeof(double)), 0.0, c->stream); // all-zero bits
   return b.p;
 }
 
@@ -1103,10 +1107,9 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
   b.waves = c->waves;
   b.cell_of = (int32_t *)work(c, (t + "_cellof").c_str(), (size_t)s.n * sizeof(int32_t));
   b.perm = (int32_t *)work(c, (t + "_perm").c_str(), (size_t)s.n * sizeof(int32_t));
-  bool fresh_count = false;
-  b.count = (int32_t *)work(c, (t + "_count").c_str(), (size_t)ncell * sizeof(int32_t), &fresh_count);
-  // a reused count buffer was cleared by the scan of its last binning
-  b.counts_zero = fresh_count ? 0 : 1;
+  // cleared in full when allocated; each binning's scan clears what it used
+  b.count = (int32_t *)work(c, (t + "_count").c_str(), (size_t)ncell * sizeof(int32_t), true);
+  b.counts_zero = 1;
   b.rank = (int32_t *)work(c, (t + "_rank").c_str(), (size_t)s.n * sizeof(int32_t));
   b.start = (int32_t *)work(c, (t + "_start").c_str(), (size_t)(ncell + 1) * sizeof(int32_t));
   b.tile_off = (int32_t *)work(c, (t + "_tiles").c_str(), (size_t)(ncell + 1) * sizeof(int32_t));
@@ -3062,9 +3065,9 @@ int build_nodes(rhg_ctx *c, rhg_grid *g, GridArgs &a, const float4 *par) {
     const int ncell = c->cg.ncell;
     b.cell_of = (int32_t *)work(c, "g_cellof", (size_t)step * sizeof(int32_t));
     b.perm = (int32_t *)work(c, "g_perm", (size_t)step * sizeof(int32_t));
-    bool fresh_count = false;
-    b.count = (int32_t *)work(c, "g_count", (size_t)ncell * sizeof(int32_t), &fresh_count);
-    b.counts_zero = fresh_count ? 0 : 1;
+    // cleared in full when allocated; each binning's scan clears what it used
+    b.count = (int32_t *)work(c, "g_count", (size_t)ncell * sizeof(int32_t), true);
+    b.counts_zero = 1;
     b.rank = (int32_t *)work(c, "g_rank", (size_t)step * sizeof(int32_t));
     b.start = (int32_t *)work(c, "g_start", (size_t)(ncell + 1) * sizeof(int32_t));
     b.tile_off = (int32_t *)work(c, "g_tiles", (size_t)(ncell + 1) * sizeof(int32_t));

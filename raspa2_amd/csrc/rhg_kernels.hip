@@ -818,9 +818,12 @@ template <int BCK>
 __global__ __launch_bounds__(256) void k_bin_scatter(const Cell cell, const CellGrid g, const SiteList s,
                                                      BinWork b) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= site_count(s)) return;
+  const int64_t n = site_count(s);
+  if (i >= n) return;
   const int c = b.cell_of[i];
   const int pos = b.start[c] + b.rank[i];
+  // in range whenever the counts started at zero; never written outside
+  if (pos < 0 || pos >= n) return;
   const int64_t site = site_phys(s, i);
   if (b.fix) {
     double cx, cy, cz;

@@ -63,7 +63,7 @@ def poisson_requests(n_req, n_fw, rate, seed=4242):
     return out
 
 
-def run(jobs, requests, slots=8, tuning=None, precision=R.PREC_MIXED, slice_=1 << 18,
+def run(jobs, requests, slots=4, tuning=None, precision=R.PREC_MIXED, slice_=1 << 18,
         hbm_budget=0, live=False):
     """Serves `requests` over the frameworks of `jobs`; returns wall time,
     per-request results and the server counters.  live: the request loop

@@ -113,3 +113,26 @@ def test_estimate_against_actual(W, R, S, jobs):
         actual = ctx.footprint()
         est = S.estimate_footprint(j.framework, j.probe, chunk=SLICE)
         assert 0.6 <= est / actual <= 1.6, (k, est, actual)
+
+
+def test_scratch_reused_by_growing_cell_grids(W, R, S):
+    """One slot serves frameworks whose cell grids grow from request to
+    request, so each reuses the slot's binning scratch sized (with headroom)
+    by an earlier, smaller one; every request's block sums equal the same
+    request served alone on a fresh service (a regression test: the per-cell
+    counts must be cleared beyond the cells of the request that allocated
+    them)."""
+    cand = []
+    for i in range(40):
+        j = W.c3_job(i)
+        j.n_trials = 4096
+        cand.append(j)
+    # frameworks by increasing cell volume (the cell grid grows with it)
+    vol = [float(np.prod(np.diag(np.asarray(j.framework.box).reshape(3, 3)))) for j in cand]
+    order = list(np.argsort(vol))[::3][:10]
+    jobs = {int(k): cand[int(k)] for k in order}
+    seq, _ = _serve(S, R, jobs, list(jobs), slots=1)
+    for k in jobs:
+        alone, _ = _serve(S, R, jobs, [k], slots=1)
+        assert seq[k]["status"] == 0 and alone[k]["status"] == 0
+        assert np.array_equal(seq[k]["blocks"].as_array(), alone[k]["blocks"].as_array()), k
