@@ -795,7 +795,8 @@ def main():
         cnt, ms = ctx.profile_get(name)
         kernels[name] = {"launches": cnt, "total_ms": round(ms, 4),
                          "avg_ms": round(ms / cnt, 5) if cnt else None,
-                         "from": "one untimed step after the timed ones"}
+                         "from": ("one untimed step after the timed ones, every family timed with HIP "
+                                  "events: no overlap between families, event gaps included")}
     ctx.profile(False)
     ctx.widom_stats(reset=True)
     # executed pair evaluations of the culled kernels (not the all-pairs count)

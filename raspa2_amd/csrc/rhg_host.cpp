@@ -231,6 +231,11 @@ struct rhg_ctx {
   rhg_workspace *ws = nullptr; // borrowed Widom/scratch buffers (nullable)
   hipStream_t stream = nullptr;
   bool own_stream = true;      // false: the stream is borrowed (screening server)
+  // a second stream of a context that owns its stream (created on first
+  // use): a Widom chunk's Fourier stage runs there beside the placed-bead
+  // Coulomb stage, joined back before finalisation (default priority)
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   Arena arena;
   std::string err;
   Cell cell;
@@ -1381,6 +1386,13 @@ void rhg_destroy(rhg_ctx *c) {
     hipEventDestroy(p.b);
   }
   for (auto e : c->event_pool) hipEventDestroy(e);
+  // the main stream waited for the second one's last work (its join event)
+  if (c->aux) {
+    hipStreamSynchronize(c->aux);
+    hipStreamDestroy(c->aux);
+  }
+  if (c->ev_fork) hipEventDestroy(c->ev_fork);
+  if (c->ev_join) hipEventDestroy(c->ev_join);
   if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1753,6 +1765,28 @@ StatArgs stat_args(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, int ewa
 // selected beads, Fourier change, finalisation, then (accumulate) the block
 // sums and work counters into the device accumulators and the control block
 // moved on by the chunk.
+// Starts work on the context's second stream after everything enqueued on
+// its main stream so far; *forked is false (nothing done) for a borrowed
+// stream (a screening slot, whose streams already fill the hardware queues),
+// with every kernel family timed (the events are on the main stream) and
+// while the main stream is being captured into a graph.
+int fork_aux(rhg_ctx *c, bool *forked) {
+  *forked = false;
+  if (!c->own_stream || c->prof == 1) return RHG_OK;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  HIPC(c, hipStreamIsCapturing(c->stream, &cs));
+  if (cs != hipStreamCaptureStatusNone) return RHG_OK;
+  if (!c->aux) {
+    HIPC(c, hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+    HIPC(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    HIPC(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+  }
+  HIPC(c, hipEventRecord(c->ev_fork, c->stream));
+  HIPC(c, hipStreamWaitEvent(c->aux, c->ev_fork, 0));
+  *forked = true;
+  return RHG_OK;
+}
+
 int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs &B,
                 const WidomArgs &a, bool accumulate) {
   WidomWork &w = B.w;
@@ -1786,55 +1820,11 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
     Timed t(c, "widom_select");
     launch_widom_fb_select(a, w, st);
   }
-  if (m.n_sites > 1) {
-    // ComputeExternalEnergies VDW of every orientation trial (src/cbmc.c:1210),
-    // one site list per type group of placed beads; positions and energies
-    // are stored [bead][orientation][active slot]
-    const int64_t cap_or = w.cap * (int64_t)m.k_or;
-    for (int p = 0; p < m.n_place; ++p) {
-      if (m.g_b[p] != 0) continue; // first bead of its group
-      EnergyArgs ea = energy_args(c, mol->type[m.place[p]], 1.0, 1, 0);
-      const int64_t base = m.g_start[p] * cap_or;
-      const int rows = m.g_size[p] * m.k_or;
-      SiteList s{w.orx + base, w.ory + base, w.orz + base, n * rows, w.n_active, rows, w.cap};
-      EnergyOut o{w.or_uv + base, nullptr, nullptr};
-      if (int rc = site_energy(c, mol->type[m.place[p]], ea, s, o, "bor")) return rc;
-      if (w.or_av) {
-        AdsArgs aa = ads_args(c, mol->type[m.place[p]], a.exclude, 1, 0);
-        aa.excl = a.excl;
-        aa.col_map = w.active; // columns are active slots
-        launch_site_ads(aa, s, w.or_av + base, nullptr, st);
-      }
-    }
-    {
-      Timed t(c, "widom_select");
-      launch_widom_or_select(a, w, st);
-    }
-    // real-space Coulomb of the charged placed beads of the selected
-    // orientation (src/cbmc.c:1216; only the selected trial's value is used),
-    // one site list per type group
-    for (int p = 0; p < m.n_place; ++p) {
-      if (m.h_start[p] < 0 || m.h_b[p] != 0) continue;
-      EnergyArgs ea = energy_args(c, mol->type[m.place[p]], 1.0, 0, 1);
-      const int64_t base = m.h_start[p] * w.cap;
-      const int rows = m.h_size[p];
-      if (!ea.do_cc) {
-        fill(w.cc_place + base, (int64_t)rows * w.cap, 0.0, st);
-        if (w.cc_ads) fill(w.cc_ads + base, (int64_t)rows * w.cap, 0.0, st);
-        continue;
-      }
-      SiteList s{w.ccx + base, w.ccy + base, w.ccz + base, n * rows, w.n_active2, rows, w.cap};
-      EnergyOut o{nullptr, w.cc_place + base, nullptr};
-      if (int rc = site_energy(c, mol->type[m.place[p]], ea, s, o, "bcc")) return rc;
-      if (w.cc_ads) {
-        AdsArgs aa = ads_args(c, mol->type[m.place[p]], a.exclude, 0, 1);
-        aa.excl = a.excl;
-        aa.col_map = w.active2; // columns are accepted slots
-        launch_site_ads(aa, s, nullptr, w.cc_ads + base, st);
-      }
-    }
-  }
-  if (a.ewald) {
+  // Fourier Delta of the accepted molecules (CalculateEwaldFourierAdsorbate)
+  // on stream fs; it needs only the selected configuration, so for
+  // multi-site molecules it can run beside the placed-bead Coulomb stage
+  auto fourier_stage = [&](hipStream_t fs) -> int {
+    if (!a.ewald) return RHG_OK;
     FourierArgs f;
     memset(&f, 0, sizeof(f));
     f.e = ewald_dev(c);
@@ -1860,16 +1850,85 @@ int widom_chunk(rhg_ctx *c, const rhg_molecule *mol, const MolDev &m, WidomBufs 
     f.sads = c->ads_n > 0 ? c->d_sads : nullptr;
     f.net_q_ads = c->net_q_ads;
     if (f.nc > 0) {
-      if (int rc = ensure_fourier(c)) return rc;
       double2 *part = (double2 *)work(c, "w_fpart", (size_t)(c->kx + 1) * w.cap * sizeof(double2));
       if (!part) return fail(c, RHG_ERR_OOM, "fourier partials");
       Timed t(c, "fourier");
-      launch_fourier(f, part, st);
+      launch_fourier(f, part, fs);
     } else {
-      fill(w.f_host, n, 0.0, st);
-      fill(w.f_ads, n, 0.0, st);
+      fill(w.f_host, n, 0.0, fs);
+      fill(w.f_ads, n, 0.0, fs);
+    }
+    return RHG_OK;
+  };
+  bool forked = false;
+  if (m.n_sites > 1) {
+    // ComputeExternalEnergies VDW of every orientation trial (src/cbmc.c:1210),
+    // one site list per type group of placed beads; positions and energies
+    // are stored [bead][orientation][active slot]
+    const int64_t cap_or = w.cap * (int64_t)m.k_or;
+    for (int p = 0; p < m.n_place; ++p) {
+      if (m.g_b[p] != 0) continue; // first bead of its group
+      EnergyArgs ea = energy_args(c, mol->type[m.place[p]], 1.0, 1, 0);
+      const int64_t base = m.g_start[p] * cap_or;
+      const int rows = m.g_size[p] * m.k_or;
+      SiteList s{w.orx + base, w.ory + base, w.orz + base, n * rows, w.n_active, rows, w.cap};
+      EnergyOut o{w.or_uv + base, nullptr, nullptr};
+      if (int rc = site_energy(c, mol->type[m.place[p]], ea, s, o, "bor")) return rc;
+      if (w.or_av) {
+        AdsArgs aa = ads_args(c, mol->type[m.place[p]], a.exclude, 1, 0);
+        aa.excl = a.excl;
+        aa.col_map = w.active; // columns are active slots
+        launch_site_ads(aa, s, w.or_av + base, nullptr, st);
+      }
+    }
+    {
+      Timed t(c, "widom_select");
+      launch_widom_or_select(a, w, st);
+    }
+    if (a.ewald) {
+      // the structure factor (first use) on the main stream, before the fork
+      for (int j = 0; j < m.n_sites; ++j)
+        if (m.charged[j]) {
+          if (int rc = ensure_fourier(c)) return rc;
+          break;
+        }
+      hipStream_t fs = st;
+      if (int rc = fork_aux(c, &forked)) return rc;
+      if (forked) fs = c->aux;
+      if (int rc = fourier_stage(fs)) return rc;
+      if (forked) HIPC(c, hipEventRecord(c->ev_join, c->aux));
+    }
+    // real-space Coulomb of the charged placed beads of the selected
+    // orientation (src/cbmc.c:1216; only the selected trial's value is used),
+    // one site list per type group
+    for (int p = 0; p < m.n_place; ++p) {
+      if (m.h_start[p] < 0 || m.h_b[p] != 0) continue;
+      EnergyArgs ea = energy_args(c, mol->type[m.place[p]], 1.0, 0, 1);
+      const int64_t base = m.h_start[p] * w.cap;
+      const int rows = m.h_size[p];
+      if (!ea.do_cc) {
+        fill(w.cc_place + base, (int64_t)rows * w.cap, 0.0, st);
+        if (w.cc_ads) fill(w.cc_ads + base, (int64_t)rows * w.cap, 0.0, st);
+        continue;
+      }
+      SiteList s{w.ccx + base, w.ccy + base, w.ccz + base, n * rows, w.n_active2, rows, w.cap};
+      EnergyOut o{nullptr, w.cc_place + base, nullptr};
+      if (int rc = site_energy(c, mol->type[m.place[p]], ea, s, o, "bcc")) return rc;
+      if (w.cc_ads) {
+        AdsArgs aa = ads_args(c, mol->type[m.place[p]], a.exclude, 0, 1);
+        aa.excl = a.excl;
+        aa.col_map = w.active2; // columns are accepted slots
+        launch_site_ads(aa, s, nullptr, w.cc_ads + base, st);
+      }
     }
   }
+  if (m.n_sites <= 1) {
+    for (int j = 0; j < m.n_sites; ++j)
+      if (a.ewald && m.charged[j])
+        if (int rc = ensure_fourier(c)) return rc;
+    if (int rc = fourier_stage(st)) return rc;
+  }
+  if (forked) HIPC(c, hipStreamWaitEvent(st, c->ev_join, 0));
   {
     Timed t(c, "widom_select");
     launch_widom_finalize(a, w, st);

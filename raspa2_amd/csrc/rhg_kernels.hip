@@ -2096,13 +2096,35 @@ __device__ __forceinline__ bool or_select_one(const WidomArgs &a, const WidomWor
   uint64_t mask = 0;
   bool any = false;
   double weight = 0.0;
-  for (int iu = 0; iu < ko; ++iu) {
-    double ev, ea;
-    if (ev_of(iu, ev, ea)) {
-      any = true;
-      weight += exp(-beta * (ev + ea + 0.0 + 0.0));
-    } else
-      mask |= (1ULL << iu);
+  // up to kOrCache orientations: each orientation's energies read once into
+  // registers (fully unrolled, so the arrays stay in VGPRs) and the shifted
+  // exponentials of the selection walk formed once; the same values in the
+  // same order as the general path below
+  constexpr int kOrCache = 16;
+  double bs[kOrCache]; // -beta (U_vdw + U_ads) of each orientation
+  if (ko <= kOrCache) {
+#pragma unroll
+    for (int iu = 0; iu < kOrCache; ++iu) {
+      if (iu >= ko) break;
+      double ev, ea;
+      if (ev_of(iu, ev, ea)) {
+        any = true;
+        bs[iu] = -beta * (ev + ea + 0.0 + 0.0);
+        weight += exp(bs[iu]);
+      } else {
+        bs[iu] = 0.0;
+        mask |= (1ULL << iu);
+      }
+    }
+  } else {
+    for (int iu = 0; iu < ko; ++iu) {
+      double ev, ea;
+      if (ev_of(iu, ev, ea)) {
+        any = true;
+        weight += exp(-beta * (ev + ea + 0.0 + 0.0));
+      } else
+        mask |= (1ULL << iu);
+    }
   }
   w.ormask[i] = mask;
   if (!any || (a.retrace && (mask & 1ULL))) {
@@ -2119,7 +2141,36 @@ __device__ __forceinline__ bool or_select_one(const WidomArgs &a, const WidomWor
     b = -beta * (ev + ea + 0.0 + 0.0);
     return true;
   };
-  const int iwalk = a.retrace ? 0 : select_trial(bf, ko, w.usel_or[i]);
+  int iwalk = 0;
+  if (!a.retrace) {
+    if (ko <= kOrCache) {
+      // SelectTrialPosition over the cached weights (select_trial's steps);
+      // bs[t] becomes exp(bs[t] - largest), 0 for an overlapping trial
+      double largest = -DBL_MAX;
+#pragma unroll
+      for (int t = 0; t < kOrCache; ++t) {
+        if (t >= ko) break;
+        if (!((mask >> t) & 1ULL) && bs[t] > largest) largest = bs[t];
+      }
+      double sum = 0.0;
+#pragma unroll
+      for (int t = 0; t < kOrCache; ++t) {
+        if (t >= ko) break;
+        bs[t] = ((mask >> t) & 1ULL) ? 0.0 : exp(bs[t] - largest);
+        if (!((mask >> t) & 1ULL)) sum += bs[t];
+      }
+      const double ws = w.usel_or[i] * sum;
+      double cumw = bs[0];
+#pragma unroll
+      for (int t = 1; t < kOrCache; ++t) {
+        if (t >= ko || !(cumw < ws)) break;
+        iwalk = t;
+        cumw += bs[t];
+      }
+    } else {
+      iwalk = select_trial(bf, ko, w.usel_or[i]);
+    }
+  }
   double Wn = w.W1[i] * (weight * 1.0);
   w.selor[i] = iwalk;
   w.W2[i] = weight;
