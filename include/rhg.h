@@ -444,6 +444,12 @@ int rhg_use_coulomb_grid(rhg_ctx *ctx, rhg_grid *g);
 /* HIP-event timing of the kernels launched on the context's stream.
  * kernel families: "site_energy", "bin", "fourier", "widom_gen",
  * "widom_select", "block_reduce", "grid_build", "grid_interp", "sf", ...
+ * The site-energy launches also count per site list, under
+ * "site_energy:bfb" (Widom first-bead trials), "site_energy:bor"
+ * (orientation trials of the other beads), "site_energy:bcc" (charged beads
+ * of the chosen orientations), "site_energy:e" (energies at given
+ * positions), "site_energy:cfb" and "site_energy:cgr" (chain first beads
+ * and grown beads).
  * Returns launch count and summed milliseconds.  on: 0 off, 1 every family,
  * RHG_PROFILE_SITE_ENERGY only the site-energy kernels (each timed pair of
  * events adds a gap of some microseconds between launches, so timing only

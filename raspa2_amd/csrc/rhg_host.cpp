@@ -120,6 +120,7 @@ struct PseudoAtom {
 
 struct Pending {
   std::string name;
+  std::string stage; // also recorded under this family when not empty
   hipEvent_t a, b;
 };
 
@@ -484,14 +485,17 @@ hipEvent_t get_event(rhg_ctx *c) {
   return e;
 }
 
-// brackets the launches of one named kernel with events when profiling
+// brackets the launches of one named kernel with events when profiling;
+// with a list tag the time also counts under "<name>:<tag>"
 struct Timed {
   rhg_ctx *c;
   const char *name;
+  const char *tag;
   hipEvent_t a = nullptr;
   bool on;
-  Timed(rhg_ctx *ctx, const char *nm)
-      : c(ctx), name(nm), on(ctx->prof == 1 || (ctx->prof == 2 && std::strcmp(nm, "site_energy") == 0)) {
+  Timed(rhg_ctx *ctx, const char *nm, const char *tg = nullptr)
+      : c(ctx), name(nm), tag(tg),
+        on(ctx->prof == 1 || (ctx->prof == 2 && std::strcmp(nm, "site_energy") == 0)) {
     if (on) {
       a = get_event(c);
       hipEventRecord(a, c->stream);
@@ -501,7 +505,7 @@ struct Timed {
     if (on) {
       hipEvent_t b = get_event(c);
       hipEventRecord(b, c->stream);
-      c->pending.push_back({name, a, b});
+      c->pending.push_back({name, tag ? std::string(name) + ":" + tag : std::string(), a, b});
     }
   }
 };
@@ -514,6 +518,11 @@ void drain_profile(rhg_ctx *c) {
     auto &s = c->stats[p.name];
     s.first += 1;
     s.second += ms;
+    if (!p.stage.empty()) {
+      auto &t = c->stats[p.stage];
+      t.first += 1;
+      t.second += ms;
+    }
     c->event_pool.push_back(p.a);
     c->event_pool.push_back(p.b);
   }
@@ -1075,7 +1084,7 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
                       const char *tag, const BinWork *given) {
   if (s.n <= 0) return RHG_OK;
   if (!c->cull || c->n == 0) {
-    Timed t(c, "site_energy");
+    Timed t(c, "site_energy", tag);
     launch_site_energy(a, s, o, c->stream);
     return RHG_OK;
   }
@@ -1101,7 +1110,7 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
       Timed tm(c, "bin");
       launch_bin_sites(c->cell, c->cg, s, b, c->stream);
     }
-    Timed tm(c, "site_energy");
+    Timed tm(c, "site_energy", tag);
     launch_site_energy_culled(ae, g, s, b, o, c->stream);
     return RHG_OK;
   }
@@ -1135,7 +1144,7 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
     Timed tm(c, "bin");
     launch_bin_sites(c->cell, c->cg, s, b, c->stream);
   }
-  Timed tm(c, "site_energy");
+  Timed tm(c, "site_energy", tag);
   launch_site_energy_culled(ae, g, s, b, o, c->stream);
   return RHG_OK;
 }

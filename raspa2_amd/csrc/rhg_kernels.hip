@@ -534,92 +534,118 @@ __global__ __launch_bounds__(256) void k_cell_lists(const Cell cell, const CellG
                                                     int32_t *__restrict__ nb_idx,
                                                     const int32_t *__restrict__ img_off,
                                                     float4 *__restrict__ img, int pass) {
+  // the workgroup's four waves (four cells) walk the atoms in tiles of 256
+  // staged in LDS; each thread loads its atom of the next tile while the
+  // waves work through the current one
+  __shared__ double tx_[256], ty_[256], tz_[256];
   const int c = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-  if (c >= g.ncell) return;
+  const bool live = c < g.ncell; // every wave joins the workgroup's barriers
   const int lane = threadIdx.x & 63;
   const unsigned long long lower = (1ULL << lane) - 1ULL;
-  double cx, cy, cz;
-  cell_centre(cell, g, c, cx, cy, cz);
+  double cx = 0.0, cy = 0.0, cz = 0.0;
+  if (live) cell_centre(cell, g, c, cx, cy, cz);
   const double reach = sqrt(reach2);
-  int nb_base = pass ? nb_off[c] : 0, img_base = pass ? img_off[c] : 0;
-  for (int j0 = 0; j0 < n_atoms; j0 += 64) {
-    const int j = j0 + lane;
-    const double dx = j < n_atoms ? ax[j] - cx : 0.0, dy = j < n_atoms ? ay[j] - cy : 0.0,
-                 dz = j < n_atoms ? az[j] - cz : 0.0;
-    double s[3] = {0.0, 0.0, 0.0};
-    bool in;
-    if (RECT) {
-      // the nearest image per axis is the torus distance's image
-      s[0] = cell.inv[AX] * dx;
-      s[1] = cell.inv[BY] * dy;
-      s[2] = cell.inv[CZ] * dz;
-      for (int d = 0; d < 3; ++d) s[d] -= rint(s[d]);
-      const double x = cell.box[AX] * s[0], y = cell.box[BY] * s[1], z = cell.box[CZ] * s[2];
-      in = j < n_atoms && x * x + y * y + z * z <= reach2;
-    } else {
-      in = j < n_atoms && torus_dist2(cell, dx, dy, dz) <= reach2;
-    }
-    int m = 0;
-    int cand[3][2] = {{0, 0}, {0, 0}, {0, 0}}, nc[3] = {1, 1, 1};
-    if (in) {
-      if (!RECT) {
-        s[0] = cell.inv[AX] * dx + cell.inv[BX] * dy + cell.inv[CX] * dz;
-        s[1] = cell.inv[AY] * dx + cell.inv[BY] * dy + cell.inv[CY] * dz;
-        s[2] = cell.inv[AZ] * dx + cell.inv[BZ] * dy + cell.inv[CZ] * dz;
-        for (int d = 0; d < 3; ++d) s[d] -= rint(s[d]);
-      }
-      const double w[3] = {w0, w1, w2};
-      for (int d = 0; d < 3; ++d)
-        if ((1.0 - fabs(s[d])) * w[d] <= reach) cand[d][nc[d]++] = s[d] > 0.0 ? -1 : 1;
-    }
-    auto image = [&](int a, int b, int e, double &vx, double &vy, double &vz) {
-      const double tx = s[0] + cand[0][a], ty = s[1] + cand[1][b], tz = s[2] + cand[2][e];
-      if (RECT) {
-        vx = cell.box[AX] * tx;
-        vy = cell.box[BY] * ty;
-        vz = cell.box[CZ] * tz;
-      } else {
-        vx = cell.box[AX] * tx + cell.box[BX] * ty + cell.box[CX] * tz;
-        vy = cell.box[AY] * tx + cell.box[BY] * ty + cell.box[CY] * tz;
-        vz = cell.box[AZ] * tx + cell.box[BZ] * ty + cell.box[CZ] * tz;
-      }
-      if (box2 > 0.0) {
-        const double ex = fmax(fabs(vx) - hx, 0.0), ey = fmax(fabs(vy) - hy, 0.0),
-                     ez = fmax(fabs(vz) - hz, 0.0);
-        return ex * ex + ey * ey + ez * ez <= box2;
-      }
-      return vx * vx + vy * vy + vz * vz <= reach2;
-    };
-    if (in)
-      for (int a = 0; a < nc[0]; ++a)
-        for (int b = 0; b < nc[1]; ++b)
-          for (int e = 0; e < nc[2]; ++e) {
-            double vx, vy, vz;
-            m += image(a, b, e, vx, vy, vz) ? 1 : 0;
-          }
-    const unsigned long long bal = __ballot(in);
-    int pre = 0, tot = 0;
-#pragma unroll
-    for (int bit = 0; bit < 4; ++bit) {
-      const unsigned long long v = __ballot((m >> bit) & 1);
-      pre += __popcll(v & lower) << bit;
-      tot += __popcll(v) << bit;
-    }
-    if (pass && in) {
-      nb_idx[nb_base + __popcll(bal & lower)] = j;
-      int o = img_base + pre;
-      for (int a = 0; a < nc[0]; ++a)
-        for (int b = 0; b < nc[1]; ++b)
-          for (int e = 0; e < nc[2]; ++e) {
-            double vx, vy, vz;
-            if (image(a, b, e, vx, vy, vz))
-              img[o++] = make_float4((float)vx, (float)vy, (float)vz, __int_as_float(j));
-          }
-    }
-    nb_base += __popcll(bal);
-    img_base += tot;
+  int nb_base = (pass && live) ? nb_off[c] : 0, img_base = (pass && live) ? img_off[c] : 0;
+  const int tid = threadIdx.x;
+  double rx = 0.0, ry = 0.0, rz = 0.0;
+  if (tid < n_atoms) {
+    rx = ax[tid];
+    ry = ay[tid];
+    rz = az[tid];
   }
-  if (lane == 0 && !pass) {
+  for (int t0 = 0; t0 < n_atoms; t0 += 256) {
+    __syncthreads();
+    tx_[tid] = rx;
+    ty_[tid] = ry;
+    tz_[tid] = rz;
+    __syncthreads();
+    const int tn = t0 + 256 + tid;
+    if (tn < n_atoms) {
+      rx = ax[tn];
+      ry = ay[tn];
+      rz = az[tn];
+    }
+    if (!live) continue;
+    for (int j0 = t0; j0 < min(t0 + 256, n_atoms); j0 += 64) {
+      const int j = j0 + lane;
+      const int u = j - t0;
+      const double dx = j < n_atoms ? tx_[u] - cx : 0.0, dy = j < n_atoms ? ty_[u] - cy : 0.0,
+                   dz = j < n_atoms ? tz_[u] - cz : 0.0;
+      double s[3] = {0.0, 0.0, 0.0};
+      bool in;
+      if (RECT) {
+        // the nearest image per axis is the torus distance's image
+        s[0] = cell.inv[AX] * dx;
+        s[1] = cell.inv[BY] * dy;
+        s[2] = cell.inv[CZ] * dz;
+        for (int d = 0; d < 3; ++d) s[d] -= rint(s[d]);
+        const double x = cell.box[AX] * s[0], y = cell.box[BY] * s[1], z = cell.box[CZ] * s[2];
+        in = j < n_atoms && x * x + y * y + z * z <= reach2;
+      } else {
+        in = j < n_atoms && torus_dist2(cell, dx, dy, dz) <= reach2;
+      }
+      int m = 0;
+      int cand[3][2] = {{0, 0}, {0, 0}, {0, 0}}, nc[3] = {1, 1, 1};
+      if (in) {
+        if (!RECT) {
+          s[0] = cell.inv[AX] * dx + cell.inv[BX] * dy + cell.inv[CX] * dz;
+          s[1] = cell.inv[AY] * dx + cell.inv[BY] * dy + cell.inv[CY] * dz;
+          s[2] = cell.inv[AZ] * dx + cell.inv[BZ] * dy + cell.inv[CZ] * dz;
+          for (int d = 0; d < 3; ++d) s[d] -= rint(s[d]);
+        }
+        const double w[3] = {w0, w1, w2};
+        for (int d = 0; d < 3; ++d)
+          if ((1.0 - fabs(s[d])) * w[d] <= reach) cand[d][nc[d]++] = s[d] > 0.0 ? -1 : 1;
+      }
+      auto image = [&](int a, int b, int e, double &vx, double &vy, double &vz) {
+        const double tx = s[0] + cand[0][a], ty = s[1] + cand[1][b], tz = s[2] + cand[2][e];
+        if (RECT) {
+          vx = cell.box[AX] * tx;
+          vy = cell.box[BY] * ty;
+          vz = cell.box[CZ] * tz;
+        } else {
+          vx = cell.box[AX] * tx + cell.box[BX] * ty + cell.box[CX] * tz;
+          vy = cell.box[AY] * tx + cell.box[BY] * ty + cell.box[CY] * tz;
+          vz = cell.box[AZ] * tx + cell.box[BZ] * ty + cell.box[CZ] * tz;
+        }
+        if (box2 > 0.0) {
+          const double ex = fmax(fabs(vx) - hx, 0.0), ey = fmax(fabs(vy) - hy, 0.0),
+                       ez = fmax(fabs(vz) - hz, 0.0);
+          return ex * ex + ey * ey + ez * ez <= box2;
+        }
+        return vx * vx + vy * vy + vz * vz <= reach2;
+      };
+      if (in)
+        for (int a = 0; a < nc[0]; ++a)
+          for (int b = 0; b < nc[1]; ++b)
+            for (int e = 0; e < nc[2]; ++e) {
+              double vx, vy, vz;
+              m += image(a, b, e, vx, vy, vz) ? 1 : 0;
+            }
+      const unsigned long long bal = __ballot(in);
+      int pre = 0, tot = 0;
+  #pragma unroll
+      for (int bit = 0; bit < 4; ++bit) {
+        const unsigned long long v = __ballot((m >> bit) & 1);
+        pre += __popcll(v & lower) << bit;
+        tot += __popcll(v) << bit;
+      }
+      if (pass && in) {
+        nb_idx[nb_base + __popcll(bal & lower)] = j;
+        int o = img_base + pre;
+        for (int a = 0; a < nc[0]; ++a)
+          for (int b = 0; b < nc[1]; ++b)
+            for (int e = 0; e < nc[2]; ++e) {
+              double vx, vy, vz;
+              if (image(a, b, e, vx, vy, vz))
+                img[o++] = make_float4((float)vx, (float)vy, (float)vz, __int_as_float(j));
+            }
+      }
+      nb_base += __popcll(bal);
+      img_base += tot;
+    }
+  }
+  if (live && lane == 0 && !pass) {
     nb_counts[c] = nb_base;
     img_counts[c] = img_base;
   }
