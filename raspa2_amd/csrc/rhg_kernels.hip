@@ -512,10 +512,13 @@ void launch_image_list(const Cell &cell, const CellGrid &g, double reach, const 
 }
 
 // Neighbour lists and image lists of every cell in one walk over the atoms:
-// one wave per cell, lane = atom, so each cell's neighbour list comes out in
-// ascending atom order and its image list atom-major in that order, the
-// images of an atom in the fixed shift order, exactly as k_cell_neighbours
-// and then k_image_list over that neighbour list give them.  An atom is a
+// one workgroup of four waves per cell, lane = atom, the waves taking the
+// four 64-atom parts of each 256-atom tile (wave w the atoms t0 + 64 w +
+// lane) and placing their entries after those of the lower waves, so each
+// cell's neighbour list comes out in ascending atom order and its image list
+// atom-major in that order, the images of an atom in the fixed shift order,
+// exactly as k_cell_neighbours and then k_image_list over that neighbour
+// list give them.  An atom is a
 // neighbour when its torus distance to the cell centre is at most reach; its
 // images are those k_image_list keeps (box2 > 0: within sqrt(box2) of the
 // cell's box, else within reach of the centre).  pass 0 counts both lists,
@@ -534,120 +537,133 @@ __global__ __launch_bounds__(256) void k_cell_lists(const Cell cell, const CellG
                                                     int32_t *__restrict__ nb_idx,
                                                     const int32_t *__restrict__ img_off,
                                                     float4 *__restrict__ img, int pass) {
-  // the workgroup's four waves (four cells) walk the atoms in tiles of 256
-  // staged in LDS; each thread loads its atom of the next tile while the
-  // waves work through the current one
-  __shared__ double tx_[256], ty_[256], tz_[256];
-  const int c = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-  const bool live = c < g.ncell; // every wave joins the workgroup's barriers
-  const int lane = threadIdx.x & 63;
+  __shared__ int s_nb[4], s_img[4];
+  const int c = blockIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const unsigned long long lower = (1ULL << lane) - 1ULL;
-  double cx = 0.0, cy = 0.0, cz = 0.0;
-  if (live) cell_centre(cell, g, c, cx, cy, cz);
+  double cx, cy, cz;
+  cell_centre(cell, g, c, cx, cy, cz);
   const double reach = sqrt(reach2);
-  int nb_base = (pass && live) ? nb_off[c] : 0, img_base = (pass && live) ? img_off[c] : 0;
-  const int tid = threadIdx.x;
+  int nb_base = pass ? nb_off[c] : 0, img_base = pass ? img_off[c] : 0;
+  // this thread's atom of the next tile, loaded while the current one is worked on
   double rx = 0.0, ry = 0.0, rz = 0.0;
-  if (tid < n_atoms) {
-    rx = ax[tid];
-    ry = ay[tid];
-    rz = az[tid];
+  if ((int)threadIdx.x < n_atoms) {
+    rx = ax[threadIdx.x];
+    ry = ay[threadIdx.x];
+    rz = az[threadIdx.x];
   }
   for (int t0 = 0; t0 < n_atoms; t0 += 256) {
-    __syncthreads();
-    tx_[tid] = rx;
-    ty_[tid] = ry;
-    tz_[tid] = rz;
-    __syncthreads();
-    const int tn = t0 + 256 + tid;
-    if (tn < n_atoms) {
-      rx = ax[tn];
-      ry = ay[tn];
-      rz = az[tn];
+    const int j = t0 + 64 * wv + lane;
+    const double dx = j < n_atoms ? rx - cx : 0.0, dy = j < n_atoms ? ry - cy : 0.0,
+                 dz = j < n_atoms ? rz - cz : 0.0;
+    const int jn = j + 256;
+    if (jn < n_atoms) {
+      rx = ax[jn];
+      ry = ay[jn];
+      rz = az[jn];
     }
-    if (!live) continue;
-    for (int j0 = t0; j0 < min(t0 + 256, n_atoms); j0 += 64) {
-      const int j = j0 + lane;
-      const int u = j - t0;
-      const double dx = j < n_atoms ? tx_[u] - cx : 0.0, dy = j < n_atoms ? ty_[u] - cy : 0.0,
-                   dz = j < n_atoms ? tz_[u] - cz : 0.0;
-      double s[3] = {0.0, 0.0, 0.0};
-      bool in;
-      if (RECT) {
-        // the nearest image per axis is the torus distance's image
-        s[0] = cell.inv[AX] * dx;
-        s[1] = cell.inv[BY] * dy;
-        s[2] = cell.inv[CZ] * dz;
+    double s[3] = {0.0, 0.0, 0.0};
+    bool in;
+    if (RECT) {
+      // the nearest image per axis is the torus distance's image
+      s[0] = cell.inv[AX] * dx;
+      s[1] = cell.inv[BY] * dy;
+      s[2] = cell.inv[CZ] * dz;
+      for (int d = 0; d < 3; ++d) s[d] -= rint(s[d]);
+      const double x = cell.box[AX] * s[0], y = cell.box[BY] * s[1], z = cell.box[CZ] * s[2];
+      in = j < n_atoms && x * x + y * y + z * z <= reach2;
+    } else {
+      in = j < n_atoms && torus_dist2(cell, dx, dy, dz) <= reach2;
+    }
+    int m = 0;
+    int cand[3][2] = {{0, 0}, {0, 0}, {0, 0}}, nc[3] = {1, 1, 1};
+    if (in) {
+      if (!RECT) {
+        s[0] = cell.inv[AX] * dx + cell.inv[BX] * dy + cell.inv[CX] * dz;
+        s[1] = cell.inv[AY] * dx + cell.inv[BY] * dy + cell.inv[CY] * dz;
+        s[2] = cell.inv[AZ] * dx + cell.inv[BZ] * dy + cell.inv[CZ] * dz;
         for (int d = 0; d < 3; ++d) s[d] -= rint(s[d]);
-        const double x = cell.box[AX] * s[0], y = cell.box[BY] * s[1], z = cell.box[CZ] * s[2];
-        in = j < n_atoms && x * x + y * y + z * z <= reach2;
-      } else {
-        in = j < n_atoms && torus_dist2(cell, dx, dy, dz) <= reach2;
       }
-      int m = 0;
-      int cand[3][2] = {{0, 0}, {0, 0}, {0, 0}}, nc[3] = {1, 1, 1};
-      if (in) {
-        if (!RECT) {
-          s[0] = cell.inv[AX] * dx + cell.inv[BX] * dy + cell.inv[CX] * dz;
-          s[1] = cell.inv[AY] * dx + cell.inv[BY] * dy + cell.inv[CY] * dz;
-          s[2] = cell.inv[AZ] * dx + cell.inv[BZ] * dy + cell.inv[CZ] * dz;
-          for (int d = 0; d < 3; ++d) s[d] -= rint(s[d]);
-        }
-        const double w[3] = {w0, w1, w2};
-        for (int d = 0; d < 3; ++d)
-          if ((1.0 - fabs(s[d])) * w[d] <= reach) cand[d][nc[d]++] = s[d] > 0.0 ? -1 : 1;
-      }
-      auto image = [&](int a, int b, int e, double &vx, double &vy, double &vz) {
-        const double tx = s[0] + cand[0][a], ty = s[1] + cand[1][b], tz = s[2] + cand[2][e];
-        if (RECT) {
-          vx = cell.box[AX] * tx;
-          vy = cell.box[BY] * ty;
-          vz = cell.box[CZ] * tz;
-        } else {
-          vx = cell.box[AX] * tx + cell.box[BX] * ty + cell.box[CX] * tz;
-          vy = cell.box[AY] * tx + cell.box[BY] * ty + cell.box[CY] * tz;
-          vz = cell.box[AZ] * tx + cell.box[BZ] * ty + cell.box[CZ] * tz;
-        }
-        if (box2 > 0.0) {
-          const double ex = fmax(fabs(vx) - hx, 0.0), ey = fmax(fabs(vy) - hy, 0.0),
-                       ez = fmax(fabs(vz) - hz, 0.0);
-          return ex * ex + ey * ey + ez * ez <= box2;
-        }
-        return vx * vx + vy * vy + vz * vz <= reach2;
-      };
-      if (in)
-        for (int a = 0; a < nc[0]; ++a)
-          for (int b = 0; b < nc[1]; ++b)
-            for (int e = 0; e < nc[2]; ++e) {
-              double vx, vy, vz;
-              m += image(a, b, e, vx, vy, vz) ? 1 : 0;
-            }
-      const unsigned long long bal = __ballot(in);
-      int pre = 0, tot = 0;
-  #pragma unroll
-      for (int bit = 0; bit < 4; ++bit) {
-        const unsigned long long v = __ballot((m >> bit) & 1);
-        pre += __popcll(v & lower) << bit;
-        tot += __popcll(v) << bit;
-      }
-      if (pass && in) {
-        nb_idx[nb_base + __popcll(bal & lower)] = j;
-        int o = img_base + pre;
-        for (int a = 0; a < nc[0]; ++a)
-          for (int b = 0; b < nc[1]; ++b)
-            for (int e = 0; e < nc[2]; ++e) {
-              double vx, vy, vz;
-              if (image(a, b, e, vx, vy, vz))
-                img[o++] = make_float4((float)vx, (float)vy, (float)vz, __int_as_float(j));
-            }
-      }
-      nb_base += __popcll(bal);
-      img_base += tot;
+      const double w[3] = {w0, w1, w2};
+      for (int d = 0; d < 3; ++d)
+        if ((1.0 - fabs(s[d])) * w[d] <= reach) cand[d][nc[d]++] = s[d] > 0.0 ? -1 : 1;
     }
+    auto image = [&](int a, int b, int e, double &vx, double &vy, double &vz) {
+      const double tx = s[0] + cand[0][a], ty = s[1] + cand[1][b], tz = s[2] + cand[2][e];
+      if (RECT) {
+        vx = cell.box[AX] * tx;
+        vy = cell.box[BY] * ty;
+        vz = cell.box[CZ] * tz;
+      } else {
+        vx = cell.box[AX] * tx + cell.box[BX] * ty + cell.box[CX] * tz;
+        vy = cell.box[AY] * tx + cell.box[BY] * ty + cell.box[CY] * tz;
+        vz = cell.box[AZ] * tx + cell.box[BZ] * ty + cell.box[CZ] * tz;
+      }
+      if (box2 > 0.0) {
+        const double ex = fmax(fabs(vx) - hx, 0.0), ey = fmax(fabs(vy) - hy, 0.0),
+                     ez = fmax(fabs(vz) - hz, 0.0);
+        return ex * ex + ey * ey + ez * ez <= box2;
+      }
+      return vx * vx + vy * vy + vz * vz <= reach2;
+    };
+    if (in)
+      for (int a = 0; a < nc[0]; ++a)
+        for (int b = 0; b < nc[1]; ++b)
+          for (int e = 0; e < nc[2]; ++e) {
+            double vx, vy, vz;
+            m += image(a, b, e, vx, vy, vz) ? 1 : 0;
+          }
+    const unsigned long long bal = __ballot(in);
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int bit = 0; bit < 4; ++bit) {
+      const unsigned long long v = __ballot((m >> bit) & 1);
+      pre += __popcll(v & lower) << bit;
+      tot += __popcll(v) << bit;
+    }
+    // entries of the lower waves' atoms of this tile come first
+    int nb_pre = 0, img_pre = 0, nb_all = __popcll(bal), img_all = tot;
+    if (pass) {
+      if (lane == 0) {
+        s_nb[wv] = __popcll(bal);
+        s_img[wv] = tot;
+      }
+      __syncthreads();
+      nb_all = img_all = 0;
+      for (int v = 0; v < 4; ++v) {
+        if (v < wv) {
+          nb_pre += s_nb[v];
+          img_pre += s_img[v];
+        }
+        nb_all += s_nb[v];
+        img_all += s_img[v];
+      }
+      __syncthreads(); // the counts are read before the next tile's are written
+    }
+    if (pass && in) {
+      nb_idx[nb_base + nb_pre + __popcll(bal & lower)] = j;
+      int o = img_base + img_pre + pre;
+      for (int a = 0; a < nc[0]; ++a)
+        for (int b = 0; b < nc[1]; ++b)
+          for (int e = 0; e < nc[2]; ++e) {
+            double vx, vy, vz;
+            if (image(a, b, e, vx, vy, vz))
+              img[o++] = make_float4((float)vx, (float)vy, (float)vz, __int_as_float(j));
+          }
+    }
+    nb_base += nb_all;
+    img_base += img_all;
   }
-  if (live && lane == 0 && !pass) {
-    nb_counts[c] = nb_base;
-    img_counts[c] = img_base;
+  if (!pass) {
+    if (lane == 0) {
+      s_nb[wv] = nb_base;
+      s_img[wv] = img_base;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      nb_counts[c] = s_nb[0] + s_nb[1] + s_nb[2] + s_nb[3];
+      img_counts[c] = s_img[0] + s_img[1] + s_img[2] + s_img[3];
+    }
   }
 }
 
@@ -659,12 +675,13 @@ void launch_cell_lists(const Cell &cell, const CellGrid &g, double reach, const 
   const double box2 = rect ? box_reach * box_reach : 0.0;
   const double hx = 0.5 * cell.box[AX] / g.nc[0], hy = 0.5 * cell.box[BY] / g.nc[1],
                hz = 0.5 * cell.box[CZ] / g.nc[2];
+  // one workgroup of four waves per cell
   if (cell.bc == 1)
-    hipLaunchKernelGGL(k_cell_lists<1>, dim3((g.ncell + 3) / 4), dim3(256), 0, st, cell, g, reach * reach,
+    hipLaunchKernelGGL(k_cell_lists<1>, dim3(g.ncell), dim3(256), 0, st, cell, g, reach * reach,
                        widths[0], widths[1], widths[2], box2, hx, hy, hz, ax, ay, az, n_atoms, nb_counts,
                        img_counts, nb_off, nb_idx, img_off, img, pass);
   else
-    hipLaunchKernelGGL(k_cell_lists<0>, dim3((g.ncell + 3) / 4), dim3(256), 0, st, cell, g, reach * reach,
+    hipLaunchKernelGGL(k_cell_lists<0>, dim3(g.ncell), dim3(256), 0, st, cell, g, reach * reach,
                        widths[0], widths[1], widths[2], box2, hx, hy, hz, ax, ay, az, n_atoms, nb_counts,
                        img_counts, nb_off, nb_idx, img_off, img, pass);
 }
