@@ -839,45 +839,58 @@ __device__ __forceinline__ CellGroup cell_group(int c, bool valid) {
 // Counts the sites of each cell and gives each site its rank among its
 // cell's sites (the value of the cell's counter its group's atomic returned
 // plus its place in the group), so that the scatter needs no atomics.
+// Grid-stride over the list's sites (the device-side count when the list
+// has one), so that a list far below its upper bound launches no idle
+// workgroups beyond the grid.
 __global__ __launch_bounds__(256) void k_bin_count(const Cell cell, const CellGrid g,
                                                    const SiteList s, BinWork b) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = i < site_count(s);
-  const int64_t p = valid ? site_phys(s, i) : 0;
-  const int c = valid ? cell_index(cell, g, s.x[p], s.y[p], s.z[p]) : -1;
-  const CellGroup r = cell_group(c, valid);
+  const int64_t n = site_count(s);
   const int lane = threadIdx.x & 63;
-  int base = 0;
-  if (valid && lane == r.lead) base = atomicAdd(&b.count[c], (int)__popcll(r.mask));
-  base = __shfl(base, r.lead);
-  if (!valid) return;
-  b.cell_of[i] = c;
-  b.rank[i] = base + (int)__popcll(r.mask & ((1ULL << lane) - 1ULL));
+  for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = i0 + threadIdx.x;
+    const bool valid = i < n;
+    const int64_t p = valid ? site_phys(s, i) : 0;
+    const int c = valid ? cell_index(cell, g, s.x[p], s.y[p], s.z[p]) : -1;
+    const CellGroup r = cell_group(c, valid);
+    int base = 0;
+    if (valid && lane == r.lead) base = atomicAdd(&b.count[c], (int)__popcll(r.mask));
+    base = __shfl(base, r.lead);
+    if (valid) {
+      b.cell_of[i] = c;
+      b.rank[i] = base + (int)__popcll(r.mask & ((1ULL << lane) - 1ULL));
+    }
+  }
 }
 
 // Writes each site at start[cell] + rank: the sorted FP32 offset from its
-// cell centre (image-list kernels) or its index (perm).
+// cell centre (image-list kernels) or its index (perm).  Grid-stride as
+// k_bin_count.
 template <int BCK>
 __global__ __launch_bounds__(256) void k_bin_scatter(const Cell cell, const CellGrid g, const SiteList s,
                                                      BinWork b) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t n = site_count(s);
-  if (i >= n) return;
-  const int c = b.cell_of[i];
-  const int pos = b.start[c] + b.rank[i];
-  // in range whenever the counts started at zero; never written outside
-  if (pos < 0 || pos >= n) return;
-  const int64_t site = site_phys(s, i);
-  if (b.fix) {
-    double cx, cy, cz;
-    cell_centre(cell, g, c, cx, cy, cz);
-    double ox = s.x[site] - cx, oy = s.y[site] - cy, oz = s.z[site] - cz;
-    mi_exact<BCK>(cell, ox, oy, oz);
-    b.fix[pos] = make_float4((float)ox, (float)oy, (float)oz, __int_as_float((int)site));
-  } else {
-    b.perm[pos] = (int32_t)site;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = b.cell_of[i];
+    const int pos = b.start[c] + b.rank[i];
+    // in range whenever the counts started at zero; never written outside
+    if (pos < 0 || pos >= n) continue;
+    const int64_t site = site_phys(s, i);
+    if (b.fix) {
+      double cx, cy, cz;
+      cell_centre(cell, g, c, cx, cy, cz);
+      double ox = s.x[site] - cx, oy = s.y[site] - cy, oz = s.z[site] - cz;
+      mi_exact<BCK>(cell, ox, oy, oz);
+      b.fix[pos] = make_float4((float)ox, (float)oy, (float)oz, __int_as_float((int)site));
+    } else {
+      b.perm[pos] = (int32_t)site;
+    }
   }
 }
+
+// workgroups of 256 threads for the grid-stride binning kernels: eight per
+// CU of a 256-CU device at most
+constexpr unsigned kBinGrid = 2048;
 
 // exclusive scans of the per-cell site counts (start) and of the per-cell
 // tile counts ceil(count/ts) (tile_off), one workgroup
@@ -930,7 +943,7 @@ void launch_bin_sites(const Cell &cell, const CellGrid &g, const SiteList &s, co
     const unsigned nbc = (unsigned)((g.ncell + 255) / 256);
     hipLaunchKernelGGL(k_zero_i32, dim3(nbc), dim3(256), 0, st, b.count, g.ncell, b.n_redo);
   }
-  const unsigned nb = (unsigned)((s.n + 255) / 256);
+  const unsigned nb = (unsigned)std::min<int64_t>((s.n + 255) / 256, kBinGrid);
   if (nb) hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(256), 0, st, cell, g, s, b);
   hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(1024), 0, st, b.count, g.ncell, b.start, b.tile_off,
                      b.fix ? img_tile_sites(b) : 64, b.n_redo);
