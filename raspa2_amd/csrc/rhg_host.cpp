@@ -485,6 +485,10 @@ hipEvent_t get_event(rhg_ctx *c) {
   return e;
 }
 
+bool timing_on(const rhg_ctx *c, const char *nm) {
+  return c->prof == 1 || (c->prof == 2 && std::strcmp(nm, "site_energy") == 0);
+}
+
 // brackets the launches of one named kernel with events when profiling;
 // with a list tag the time also counts under "<name>:<tag>"
 struct Timed {
@@ -494,8 +498,7 @@ struct Timed {
   hipEvent_t a = nullptr;
   bool on;
   Timed(rhg_ctx *ctx, const char *nm, const char *tg = nullptr)
-      : c(ctx), name(nm), tag(tg),
-        on(ctx->prof == 1 || (ctx->prof == 2 && std::strcmp(nm, "site_energy") == 0)) {
+      : c(ctx), name(nm), tag(tg), on(timing_on(ctx, nm)) {
     if (on) {
       a = get_event(c);
       hipEventRecord(a, c->stream);
@@ -1076,6 +1079,21 @@ int site_energy(rhg_ctx *c, int type, const EnergyArgs &a, const SiteList &s, co
   return RHG_OK;
 }
 
+// The culled site-energy kernel of one list, timed (when profiling) by
+// events its own dispatch records: the kernel alone, under "site_energy"
+// and "site_energy:<tag>".
+void site_energy_launch(rhg_ctx *c, const EnergyArgs &a, const CellGrid &g, const SiteList &s,
+                        const BinWork &b, const EnergyOut &o, const char *tag) {
+  LaunchEvents ev;
+  const bool timed = timing_on(c, "site_energy");
+  if (timed) {
+    ev.start = get_event(c);
+    ev.stop = get_event(c);
+  }
+  launch_site_energy_culled(a, g, s, b, o, c->stream, ev);
+  if (timed) c->pending.push_back({"site_energy", std::string("site_energy:") + tag, ev.start, ev.stop});
+}
+
 // Site energies of one site list, culled through the cell grid when enabled.
 // `tag` keeps the binning scratch of concurrent lists apart; `given`
 // (nullable) is binning scratch the caller provides, for lists whose cell
@@ -1110,8 +1128,7 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
       Timed tm(c, "bin");
       launch_bin_sites(c->cell, c->cg, s, b, c->stream);
     }
-    Timed tm(c, "site_energy", tag);
-    launch_site_energy_culled(ae, g, s, b, o, c->stream);
+    site_energy_launch(c, ae, g, s, b, o, tag);
     return RHG_OK;
   }
   const int ncell = c->cg.ncell;
@@ -1144,8 +1161,7 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
     Timed tm(c, "bin");
     launch_bin_sites(c->cell, c->cg, s, b, c->stream);
   }
-  Timed tm(c, "site_energy", tag);
-  launch_site_energy_culled(ae, g, s, b, o, c->stream);
+  site_energy_launch(c, ae, g, s, b, o, tag);
   return RHG_OK;
 }
 

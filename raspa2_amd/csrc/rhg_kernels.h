@@ -130,8 +130,15 @@ void launch_image_list(const Cell &cell, const CellGrid &g, double reach, const 
                        int32_t *counts, const int32_t *off, float4 *img, int pass, hipStream_t st);
 void launch_bin_sites(const Cell &cell, const CellGrid &g, const SiteList &s, const BinWork &b,
                       hipStream_t st);
+// Start and stop events that the dispatch of a site-energy kernel records
+// itself (hipExtLaunchKernelGGL): the kernel's own time, with no marker
+// packets between launches; both null: a plain launch.
+struct LaunchEvents {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
 void launch_site_energy_culled(const EnergyArgs &a, const CellGrid &g, const SiteList &s,
-                               const BinWork &b, const EnergyOut &o, hipStream_t st);
+                               const BinWork &b, const EnergyOut &o, hipStream_t st,
+                               LaunchEvents ev = LaunchEvents());
 // n interleaved positions (x, y, z per site) into three SoA arrays
 void launch_deinterleave(const double *xyz, int64_t n, double *x, double *y, double *z, hipStream_t st);
 

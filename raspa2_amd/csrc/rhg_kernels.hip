@@ -8,6 +8,7 @@
 #include <float.h>
 #include <cstring>
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include "rhg_kernels.h"
 
@@ -1323,9 +1324,20 @@ __global__ __launch_bounds__(64) void k_site_redo(const Cell cell, const AtomRec
   }
 }
 
+// A kernel launch that records ev's events around the kernel itself when
+// they are given.
+template <typename K, typename... Args>
+static void site_dispatch(K kernel, dim3 grid, dim3 block, uint32_t lds, hipStream_t st, LaunchEvents ev,
+                          Args... args) {
+  if (ev.start)
+    hipExtLaunchKernelGGL(kernel, grid, block, lds, st, ev.start, ev.stop, 0, args...);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, lds, st, args...);
+}
+
 template <int BCK, int VDW, int CC>
 static void img_launch(const EnergyArgs &a, const CellGrid &g, const SiteList &s, const BinWork &b,
-                       const EnergyOut &o, hipStream_t st) {
+                       const EnergyOut &o, hipStream_t st, LaunchEvents ev) {
   // upper bound of tiles: sites/(tile size) + one partial tile per cell
   const int ts = img_tile_sites(b);
   const int64_t max_tiles = s.n / ts + g.ncell + 1;
@@ -1333,17 +1345,14 @@ static void img_launch(const EnergyArgs &a, const CellGrid &g, const SiteList &s
   const unsigned nb = (unsigned)((max_tiles + w - 1) / w);
   // the kernel's registers are sized for the tile's sites per lane
   if (ts <= 64)
-    hipLaunchKernelGGL((k_site_energy_img<BCK, VDW, CC, 1>), dim3(nb), dim3(64 * w), 0, st, a.cell, a.a64,
-                       a.par, a.rc2_vdw, a.rc2_cc, a.alpha, a.cqa, g, s, b, a.cell_shift, o.u_vdw,
-                       o.u_cc, o.overlap);
+    site_dispatch(k_site_energy_img<BCK, VDW, CC, 1>, dim3(nb), dim3(64 * w), 0, st, ev, a.cell, a.a64, a.par,
+                  a.rc2_vdw, a.rc2_cc, a.alpha, a.cqa, g, s, b, a.cell_shift, o.u_vdw, o.u_cc, o.overlap);
   else if (ts <= 128)
-    hipLaunchKernelGGL((k_site_energy_img<BCK, VDW, CC, 2>), dim3(nb), dim3(64 * w), 0, st, a.cell, a.a64,
-                       a.par, a.rc2_vdw, a.rc2_cc, a.alpha, a.cqa, g, s, b, a.cell_shift, o.u_vdw,
-                       o.u_cc, o.overlap);
+    site_dispatch(k_site_energy_img<BCK, VDW, CC, 2>, dim3(nb), dim3(64 * w), 0, st, ev, a.cell, a.a64, a.par,
+                  a.rc2_vdw, a.rc2_cc, a.alpha, a.cqa, g, s, b, a.cell_shift, o.u_vdw, o.u_cc, o.overlap);
   else
-    hipLaunchKernelGGL((k_site_energy_img<BCK, VDW, CC, 4>), dim3(nb), dim3(64 * w), 0, st, a.cell, a.a64,
-                       a.par, a.rc2_vdw, a.rc2_cc, a.alpha, a.cqa, g, s, b, a.cell_shift, o.u_vdw,
-                       o.u_cc, o.overlap);
+    site_dispatch(k_site_energy_img<BCK, VDW, CC, 4>, dim3(nb), dim3(64 * w), 0, st, ev, a.cell, a.a64, a.par,
+                  a.rc2_vdw, a.rc2_cc, a.alpha, a.cqa, g, s, b, a.cell_shift, o.u_vdw, o.u_cc, o.overlap);
   // the listed near-threshold sites, exactly (the list is short: a grid of
   // 256 waves strides over it, one wave per site)
   hipLaunchKernelGGL((k_site_redo<BCK, VDW, CC>), dim3(256), dim3(64), 0, st, a.cell, a.a64, a.rc2_vdw,
@@ -1352,50 +1361,49 @@ static void img_launch(const EnergyArgs &a, const CellGrid &g, const SiteList &s
 
 template <int PREC, int BCK, int VDW, int CC>
 static void culled_launch(const EnergyArgs &a, const CellGrid &g, const SiteList &s,
-                          const BinWork &b, const EnergyOut &o, hipStream_t st) {
+                          const BinWork &b, const EnergyOut &o, hipStream_t st, LaunchEvents ev) {
   // upper bound of tiles: sites/64 + one partial tile per cell
   const int64_t max_tiles = s.n / 64 + g.ncell + 1;
   const int w = site_waves(b);
   const unsigned nb = (unsigned)((max_tiles + w - 1) / w);
-  hipLaunchKernelGGL((k_site_energy_culled<PREC, BCK, VDW, CC>), dim3(nb), dim3(64 * w), 0, st, a.cell,
-                     a.a64, a.amx, a.rc2_vdw, a.rc2_cc, a.alpha, a.cqa, g, s, b, o.u_vdw, o.u_cc,
-                     o.overlap);
+  site_dispatch(k_site_energy_culled<PREC, BCK, VDW, CC>, dim3(nb), dim3(64 * w), 0, st, ev, a.cell, a.a64,
+                a.amx, a.rc2_vdw, a.rc2_cc, a.alpha, a.cqa, g, s, b, o.u_vdw, o.u_cc, o.overlap);
 }
 
 template <int VDW, int CC>
 static void img_mode(const EnergyArgs &a, const CellGrid &g, const SiteList &s, const BinWork &b,
-                     const EnergyOut &o, hipStream_t st) {
+                     const EnergyOut &o, hipStream_t st, LaunchEvents ev) {
   const int bck = a.cell.bc == 1 ? 1 : (a.cell.upper ? 2 : 3);
   // CC 2: erfc arguments beyond the fit's range (alpha rc > 5) possible
   if (CC && a.alpha * sqrt(a.rc2_cc) > 5.0) {
-    if (bck == 1) img_launch<1, VDW, 2>(a, g, s, b, o, st);
-    else if (bck == 2) img_launch<2, VDW, 2>(a, g, s, b, o, st);
-    else img_launch<3, VDW, 2>(a, g, s, b, o, st);
+    if (bck == 1) img_launch<1, VDW, 2>(a, g, s, b, o, st, ev);
+    else if (bck == 2) img_launch<2, VDW, 2>(a, g, s, b, o, st, ev);
+    else img_launch<3, VDW, 2>(a, g, s, b, o, st, ev);
   } else {
-    if (bck == 1) img_launch<1, VDW, CC>(a, g, s, b, o, st);
-    else if (bck == 2) img_launch<2, VDW, CC>(a, g, s, b, o, st);
-    else img_launch<3, VDW, CC>(a, g, s, b, o, st);
+    if (bck == 1) img_launch<1, VDW, CC>(a, g, s, b, o, st, ev);
+    else if (bck == 2) img_launch<2, VDW, CC>(a, g, s, b, o, st, ev);
+    else img_launch<3, VDW, CC>(a, g, s, b, o, st, ev);
   }
 }
 
 template <int VDW, int CC>
 static void culled_mode(const EnergyArgs &a, const CellGrid &g, const SiteList &s, const BinWork &b,
-                        const EnergyOut &o, hipStream_t st) {
+                        const EnergyOut &o, hipStream_t st, LaunchEvents ev) {
   const int bck = a.cell.bc == 1 ? 1 : (a.cell.upper ? 2 : 3);
   if (a.precision != 0 && g.img_off && a.par && b.fix) {
     // shifted LJ with the cells' shift sums: the clamped form
-    if (VDW && a.cell_shift) img_mode<2, CC>(a, g, s, b, o, st);
-    else img_mode<VDW, CC>(a, g, s, b, o, st);
+    if (VDW && a.cell_shift) img_mode<2, CC>(a, g, s, b, o, st, ev);
+    else img_mode<VDW, CC>(a, g, s, b, o, st, ev);
     return;
   }
   if (a.precision == 0) {
-    if (bck == 1) culled_launch<0, 1, VDW, CC>(a, g, s, b, o, st);
-    else if (bck == 2) culled_launch<0, 2, VDW, CC>(a, g, s, b, o, st);
-    else culled_launch<0, 3, VDW, CC>(a, g, s, b, o, st);
+    if (bck == 1) culled_launch<0, 1, VDW, CC>(a, g, s, b, o, st, ev);
+    else if (bck == 2) culled_launch<0, 2, VDW, CC>(a, g, s, b, o, st, ev);
+    else culled_launch<0, 3, VDW, CC>(a, g, s, b, o, st, ev);
   } else {
-    if (bck == 1) culled_launch<1, 1, VDW, CC>(a, g, s, b, o, st);
-    else if (bck == 2) culled_launch<1, 2, VDW, CC>(a, g, s, b, o, st);
-    else culled_launch<1, 3, VDW, CC>(a, g, s, b, o, st);
+    if (bck == 1) culled_launch<1, 1, VDW, CC>(a, g, s, b, o, st, ev);
+    else if (bck == 2) culled_launch<1, 2, VDW, CC>(a, g, s, b, o, st, ev);
+    else culled_launch<1, 3, VDW, CC>(a, g, s, b, o, st, ev);
   }
 }
 
@@ -1415,14 +1423,21 @@ void launch_deinterleave(const double *xyz, int64_t n, double *x, double *y, dou
 }
 
 void launch_site_energy_culled(const EnergyArgs &a, const CellGrid &g, const SiteList &s,
-                               const BinWork &b, const EnergyOut &o, hipStream_t st) {
-  if (s.n <= 0) return;
+                               const BinWork &b, const EnergyOut &o, hipStream_t st, LaunchEvents ev) {
+  if (s.n <= 0 || !(a.do_vdw || a.do_cc)) {
+    // nothing to launch: the events still mark this point of the stream
+    if (ev.start) {
+      hipEventRecord(ev.start, st);
+      hipEventRecord(ev.stop, st);
+    }
+    return;
+  }
   if (a.do_vdw && a.do_cc)
-    culled_mode<1, 1>(a, g, s, b, o, st);
+    culled_mode<1, 1>(a, g, s, b, o, st, ev);
   else if (a.do_vdw)
-    culled_mode<1, 0>(a, g, s, b, o, st);
+    culled_mode<1, 0>(a, g, s, b, o, st, ev);
   else if (a.do_cc)
-    culled_mode<0, 1>(a, g, s, b, o, st);
+    culled_mode<0, 1>(a, g, s, b, o, st, ev);
 }
 
 // ====================================================================
