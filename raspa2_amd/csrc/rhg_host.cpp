@@ -1148,6 +1148,8 @@ int site_energy_pairs(rhg_ctx *c, const EnergyArgs &a, const SiteList &s, const 
   b.fix = nullptr;
   if (a.precision != RHG_PREC_FP64 && g.img_off && a.par) {
     b.fix = (float4 *)work(c, (t + "_fix").c_str(), (size_t)s.n * sizeof(float4));
+    b.fixu = (float4 *)work(c, (t + "_fixu").c_str(), (size_t)s.n * sizeof(float4));
+    if (!b.fixu) return fail(c, RHG_ERR_OOM, "binning scratch");
     b.redo = (int2 *)work(c, (t + "_redo").c_str(), (size_t)s.n * sizeof(int2));
     b.n_redo = (int32_t *)work(c, (t + "_nredo").c_str(), 256);
     if (!b.fix || !b.redo || !b.n_redo) return fail(c, RHG_ERR_OOM, "binning scratch");

@@ -79,6 +79,11 @@ struct BinWork {
   // optional (image-list kernels): per sorted position the site's offset from
   // its cell centre in A as FP32 and the site index bit-cast into w
   float4 *fix;
+  // optional, with fix: per site in list order its FP32 offset from its cell
+  // centre and the cell index bit-cast into w, written by the counting pass
+  // (in place of cell_of), so that the scatter moves 16 B per site and reads
+  // no positions
+  float4 *fixu = nullptr;
   // image-list kernels: (sorted position, cell) of the sites to recompute
   // exactly (room for every site) and their count (cleared by the binning)
   int2 *redo = nullptr;

@@ -843,6 +843,7 @@ __device__ __forceinline__ CellGroup cell_group(int c, bool valid) {
 // Grid-stride over the list's sites (the device-side count when the list
 // has one), so that a list far below its upper bound launches no idle
 // workgroups beyond the grid.
+template <int BCK>
 __global__ __launch_bounds__(256) void k_bin_count(const Cell cell, const CellGrid g,
                                                    const SiteList s, BinWork b) {
   const int64_t n = site_count(s);
@@ -851,13 +852,28 @@ __global__ __launch_bounds__(256) void k_bin_count(const Cell cell, const CellGr
     const int64_t i = i0 + threadIdx.x;
     const bool valid = i < n;
     const int64_t p = valid ? site_phys(s, i) : 0;
-    const int c = valid ? cell_index(cell, g, s.x[p], s.y[p], s.z[p]) : -1;
+    double x = 0.0, y = 0.0, z = 0.0;
+    if (valid) {
+      x = s.x[p];
+      y = s.y[p];
+      z = s.z[p];
+    }
+    const int c = valid ? cell_index(cell, g, x, y, z) : -1;
     const CellGroup r = cell_group(c, valid);
     int base = 0;
     if (valid && lane == r.lead) base = atomicAdd(&b.count[c], (int)__popcll(r.mask));
     base = __shfl(base, r.lead);
     if (valid) {
-      b.cell_of[i] = c;
+      if (b.fixu) {
+        // the offset k_bin_scatter would form, formed here from the loaded position
+        double cx, cy, cz;
+        cell_centre(cell, g, c, cx, cy, cz);
+        double ox = x - cx, oy = y - cy, oz = z - cz;
+        mi_exact<BCK>(cell, ox, oy, oz);
+        b.fixu[i] = make_float4((float)ox, (float)oy, (float)oz, __int_as_float(c));
+      } else {
+        b.cell_of[i] = c;
+      }
       b.rank[i] = base + (int)__popcll(r.mask & ((1ULL << lane) - 1ULL));
     }
   }
@@ -872,6 +888,14 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const Cell cell, const Cell
   const int64_t n = site_count(s);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
+    if (b.fixu) {
+      const float4 u = b.fixu[i];
+      const int pos = b.start[__float_as_int(u.w)] + b.rank[i];
+      // in range whenever the counts started at zero; never written outside
+      if (pos < 0 || pos >= n) continue;
+      b.fix[pos] = make_float4(u.x, u.y, u.z, __int_as_float((int)site_phys(s, i)));
+      continue;
+    }
     const int c = b.cell_of[i];
     const int pos = b.start[c] + b.rank[i];
     // in range whenever the counts started at zero; never written outside
@@ -945,7 +969,14 @@ void launch_bin_sites(const Cell &cell, const CellGrid &g, const SiteList &s, co
     hipLaunchKernelGGL(k_zero_i32, dim3(nbc), dim3(256), 0, st, b.count, g.ncell, b.n_redo);
   }
   const unsigned nb = (unsigned)std::min<int64_t>((s.n + 255) / 256, kBinGrid);
-  if (nb) hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(256), 0, st, cell, g, s, b);
+  if (nb) {
+    if (cell.bc == 1)
+      hipLaunchKernelGGL(k_bin_count<1>, dim3(nb), dim3(256), 0, st, cell, g, s, b);
+    else if (cell.upper)
+      hipLaunchKernelGGL(k_bin_count<2>, dim3(nb), dim3(256), 0, st, cell, g, s, b);
+    else
+      hipLaunchKernelGGL(k_bin_count<3>, dim3(nb), dim3(256), 0, st, cell, g, s, b);
+  }
   hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(1024), 0, st, b.count, g.ncell, b.start, b.tile_off,
                      b.fix ? img_tile_sites(b) : 64, b.n_redo);
   if (nb) {

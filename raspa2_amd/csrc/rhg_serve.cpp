@@ -200,9 +200,9 @@ size_t estimate_bytes(const rhg_framework *f, const rhg_cell *cell, const rhg_fo
   double per_ins = k_fb * 40.0 + 132.0 + (double)k_or_eff * n_place * 32.0 + n_sites * 24.0 +
                    n_place * 8.0 + 16.0;
   if (ewald && n_charged > 0) per_ins += (kx + 1) * 16.0;
-  // binning scratch: 8 B per binned site (24 B with the sorted FP32
-  // offsets of mixed precision) and 16 B per cell for each list
-  const double per_site = precision == RHG_PREC_MIXED ? 24.0 : 8.0;
+  // binning scratch: 8 B per binned site (40 B with the unsorted and sorted
+  // FP32 offsets of mixed precision) and 16 B per cell for each list
+  const double per_site = precision == RHG_PREC_MIXED ? 40.0 : 8.0;
   per_ins += k_fb * per_site;
   double lists = 1;
   if (n_sites > 1) {
