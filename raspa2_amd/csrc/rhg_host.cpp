@@ -1048,6 +1048,7 @@ int cell_shifts(rhg_ctx *c, const AtomRec64 *a64, bool may_alloc, const double *
 // pair loop holds more registers per site, and 64-site tiles give sparse
 // lists, such as first beads and charged beads, more waves).
 int list_tile_sites(const rhg_ctx *c, const EnergyArgs &a) {
+  if (a.do_cc && !a.do_vdw) return 32; // Coulomb-only lists: two 32-site tiles per wave
   if (a.do_cc) return 64;
   return c->tile_sites;
 }

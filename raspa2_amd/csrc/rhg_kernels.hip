@@ -1116,6 +1116,26 @@ __device__ __forceinline__ float vgpr_copy(float x) {
 // site.  In-range entries then give c12/r^12 - c6/r^6 - shift as well, with
 // no select and the shift out of the pair loop (the product with r^-6 folds
 // into the FP32 group accumulation).
+// The real-space Coulomb pair term of the image-list kernels at squared
+// separation rr: CC 1 from the FP32 erfc fit (alpha rc <= 5: the fit covers
+// every in-range pair; computed for all pairs and selected, like the LJ term,
+// out-of-range values being finite), CC 2 from the library erfcf beyond the
+// fit; q is the listed atom's charge.
+template <int CC>
+__device__ __forceinline__ float img_cc_term(float rr, float q, float rc2cf, float alphaf, CoulConsts ck) {
+  if (CC == 1) {
+    const float g = erfc_over_r(rr, __builtin_amdgcn_rsqf(rr), ck);
+    return (rr < rc2cf) ? q * g : 0.0f;
+  }
+  float fc = 0.0f;
+  if (rr < rc2cf) {
+    const float irs = __builtin_amdgcn_rsqf(rr);
+    const float e2 = erfc_mixed(alphaf * (rr * irs));
+    fc = q * (e2 * irs);
+  }
+  return fc;
+}
+
 template <int NS, int VDW, int CC, int NSM>
 __device__ __forceinline__ void img_sums(const float4 *__restrict__ img, const float4 *__restrict__ par,
                                          int q0, int q1, const float4 (&so)[NSM], float rc2vf,
@@ -1143,19 +1163,7 @@ __device__ __forceinline__ void img_sums(const float4 *__restrict__ img, const f
       const float u = fmaf(ir3, y, -pr.z);
       fv = (rr < rc2vf) ? u : 0.0f;
     }
-    if (CC == 1) {
-      // alpha rc <= 5: the fit covers every in-range pair; computed for all
-      // pairs and selected, like the LJ term (out-of-range values are finite)
-      const float g = erfc_over_r(rr, __builtin_amdgcn_rsqf(rr), ck);
-      fc = (rr < rc2cf) ? pr.w * g : 0.0f;
-    } else if (CC == 2) {
-      fc = 0.0f;
-      if (rr < rc2cf) {
-        const float irs = __builtin_amdgcn_rsqf(rr);
-        const float e2 = erfc_mixed(alphaf * (rr * irs));
-        fc = pr.w * (e2 * irs);
-      }
-    }
+    if (CC) fc = img_cc_term<CC>(rr, pr.w, rc2cf, alphaf, ck);
   };
   int q = q0;
   for (; q + 4 <= q1; q += 4) {
@@ -1282,6 +1290,82 @@ __global__ __launch_bounds__(256) void k_site_energy_img(
   }
 }
 
+// Coulomb-only image-list kernel for sparse lists (the charged beads of the
+// chosen orientations, some tens of sites per cell): tiles of 32 sites, two
+// per wave, each half-wave walking its own cell's list, so that a cell with
+// 20-40 sites fills most of 64 lanes.  The entries are vector loads (one
+// address per half-wave) instead of scalar ones; each lane sums its list
+// exactly as k_site_energy_img<BCK, 0, CC, 1> does (groups of four FP32 terms
+// from the list start, then single terms, into the FP64 sum), so the energies
+// are the same bit for bit.
+template <int BCK, int CC>
+__global__ __launch_bounds__(256) void k_site_energy_img_pair(
+    const float4 *__restrict__ par, const double rc2c, const double alpha, const double cqa,
+    const CellGrid g, const BinWork b, double *__restrict__ out_c, uint8_t *__restrict__ out_o) {
+  const int lane = threadIdx.x & 63, hl = lane & 31;
+  const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  const int ntiles = b.tile_off[g.ncell];
+  if (2 * wave >= ntiles) return;
+  const int tile = 2 * wave + (lane >> 5); // this half-wave's tile of 32 sites
+  const bool tile_ok = tile < ntiles;
+  int lo = 0, hi = g.ncell - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (b.tile_off[mid] <= min(tile, ntiles - 1)) lo = mid; else hi = mid - 1;
+  }
+  const int c = lo;
+  const int first = (min(tile, ntiles - 1) - b.tile_off[c]) * 32;
+  const int nl = tile_ok ? min(32, cell_sites(b, c) - first) : 0; // live sites of the half-tile
+  const bool live = hl < nl;
+  const float4 so = b.fix[b.start[c] + first + (live ? hl : 0)];
+  const int q0 = g.img_off[c];
+  const int len = tile_ok ? g.img_off[c + 1] - q0 : 0;
+  // the longer of the two halves' lists bounds the wave's walk
+  const int len_max = max(__builtin_amdgcn_readlane(len, 0), __builtin_amdgcn_readlane(len, 32));
+  const float rc2cf = (float)rc2c, alphaf = (float)alpha;
+  const CoulConsts ck = coulomb_consts(alpha);
+  double uc = 0.0;
+  for (int j = 0; j < len_max; j += 4) {
+    const int m = len - j; // entries of this lane's list from j on
+    if (m >= 4) {
+      float4 e[4], pr[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        e[u] = g.img[q0 + j + u];
+        pr[u] = par[__float_as_int(e[u].w)];
+      }
+      float t[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float dx = so.x - e[u].x, dy = so.y - e[u].y, dz = so.z - e[u].z;
+        const float rr = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        t[u] = img_cc_term<CC>(rr, pr[u].w, rc2cf, alphaf, ck);
+      }
+      uc = fma(cqa, (double)((t[0] + t[1]) + (t[2] + t[3])), uc);
+    } else if (m > 0) {
+      for (int u = 0; u < m; ++u) {
+        const float4 e = g.img[q0 + j + u];
+        const float4 pr = par[__float_as_int(e.w)];
+        const float dx = so.x - e.x, dy = so.y - e.y, dz = so.z - e.z;
+        const float rr = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        uc = fma(cqa, (double)img_cc_term<CC>(rr, pr.w, rc2cf, alphaf, ck), uc);
+      }
+    }
+  }
+  const double band = 1.0e-4 * kOverlap;
+  const bool near = live && fabs(uc - kOverlap) <= band;
+  // listed for k_site_redo, which recomputes them exactly after this kernel
+  if (near) b.redo[wave_slot(b.n_redo)] = make_int2(b.start[c] + first + hl, c);
+  if (hl == 0 && nl > 0 && b.pairs)
+    atomicAdd(b.pairs + 2 * kPairLanes + (blockIdx.x & (kPairLanes - 1)),
+              (unsigned long long)nl * (unsigned long long)len);
+  if (live) {
+    const int64_t oi = __float_as_int(so.w);
+    if (out_c) out_c[oi] = uc;
+    if (out_o) out_o[oi] = (uint8_t)(uc >= kOverlap);
+  }
+}
+
 // Exact FP64 energies (atom order, as the culled FP64 kernel sums them) of
 // the sites the image-list kernel listed as within the guard band of the
 // overlap threshold: b.redo holds their sorted positions and cells.  One wave per
@@ -1373,6 +1457,17 @@ static void img_launch(const EnergyArgs &a, const CellGrid &g, const SiteList &s
   const int ts = img_tile_sites(b);
   const int64_t max_tiles = s.n / ts + g.ncell + 1;
   const int w = site_waves(b);
+  if constexpr (VDW == 0) {
+    if (ts == 32) {
+      // sparse Coulomb-only lists: two 32-site tiles per wave
+      const unsigned nbp = (unsigned)(((max_tiles + 1) / 2 + w - 1) / w);
+      site_dispatch(k_site_energy_img_pair<BCK, CC>, dim3(nbp), dim3(64 * w), 0, st, ev, a.par, a.rc2_cc,
+                    a.alpha, a.cqa, g, b, o.u_cc, o.overlap);
+      hipLaunchKernelGGL((k_site_redo<BCK, VDW, CC>), dim3(256), dim3(64), 0, st, a.cell, a.a64, a.rc2_vdw,
+                         a.rc2_cc, a.alpha, a.cqa, g, s, b, o.u_vdw, o.u_cc, o.overlap);
+      return;
+    }
+  }
   const unsigned nb = (unsigned)((max_tiles + w - 1) / w);
   // the kernel's registers are sized for the tile's sites per lane
   if (ts <= 64)
