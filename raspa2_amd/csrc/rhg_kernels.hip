@@ -926,26 +926,47 @@ __global__ __launch_bounds__(1024) void k_scan_bins(int32_t *__restrict__ in, in
                                                     int32_t *__restrict__ start,
                                                     int32_t *__restrict__ tiles, int ts,
                                                     int32_t *__restrict__ n_redo) {
-  __shared__ int ps[1024], pt[1024];
-  const int t = threadIdx.x;
+  // each thread a run of consecutive cells; the runs' sums scanned within
+  // each wave by shuffles and across the 16 waves through LDS (two barriers)
+  __shared__ int w_s[16], w_t[16];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
   const int per = (n + blockDim.x - 1) / blockDim.x;
   const int b = t * per, e = min(n, b + per);
   int s = 0, u = 0;
   for (int i = b; i < e; ++i) {
-    s += in[i];
-    u += (in[i] + ts - 1) / ts;
+    const int v = in[i];
+    s += v;
+    u += (v + ts - 1) / ts;
   }
-  ps[t] = s;
-  pt[t] = u;
+  int is = s, iu = u; // inclusive within the wave
+  for (int d = 1; d < 64; d <<= 1) {
+    const int ps = __shfl_up(is, d), pu = __shfl_up(iu, d);
+    if (lane >= d) {
+      is += ps;
+      iu += pu;
+    }
+  }
+  if (lane == 63) {
+    w_s[wid] = is;
+    w_t[wid] = iu;
+  }
   __syncthreads();
-  for (int d = 1; d < (int)blockDim.x; d <<= 1) {
-    const int vs = (t >= d) ? ps[t - d] : 0, vt = (t >= d) ? pt[t - d] : 0;
-    __syncthreads();
-    ps[t] += vs;
-    pt[t] += vt;
-    __syncthreads();
+  if (wid == 0) {
+    int vs = lane < 16 ? w_s[lane] : 0, vt = lane < 16 ? w_t[lane] : 0;
+    for (int d = 1; d < 16; d <<= 1) {
+      const int ps = __shfl_up(vs, d), pt = __shfl_up(vt, d);
+      if (lane >= d) {
+        vs += ps;
+        vt += pt;
+      }
+    }
+    if (lane < 16) {
+      w_s[lane] = vs; // inclusive over the waves
+      w_t[lane] = vt;
+    }
   }
-  int rs = (t == 0) ? 0 : ps[t - 1], rt = (t == 0) ? 0 : pt[t - 1];
+  __syncthreads();
+  int rs = (wid ? w_s[wid - 1] : 0) + is - s, rt = (wid ? w_t[wid - 1] : 0) + iu - u;
   for (int i = b; i < e; ++i) {
     const int v = in[i];
     start[i] = rs;
@@ -955,8 +976,8 @@ __global__ __launch_bounds__(1024) void k_scan_bins(int32_t *__restrict__ in, in
     in[i] = 0;
   }
   if (t == (int)blockDim.x - 1) {
-    start[n] = ps[t];
-    tiles[n] = pt[t];
+    start[n] = w_s[15];
+    tiles[n] = w_t[15];
   }
   if (t == 0 && n_redo) *n_redo = 0;
 }
