@@ -205,3 +205,24 @@ def test_guard_band_sites_recomputed_exactly(W, O, R, charged):
         assert same_bits(uc[band], uc64[band])
     ctx.close()
     ref.close()
+
+
+def test_coulomb_only_lists_match_lj_coulomb_lists(W, O, R):
+    """Coulomb-only image lists (two 32-site tiles per wave, one cell per
+    half-wave) sum each site's real-space Ewald terms exactly as the LJ +
+    Coulomb kernel does: with the LJ energy served from a grid, the positions'
+    Coulomb energies equal those of the direct LJ + Coulomb pass bit for bit
+    (scattered and crowded cells; sites near the overlap threshold, which the
+    direct pass recomputes in FP64 for their LJ sum, left out)."""
+    fw = W.c1_framework()
+    site = (27.0, 2.80, 0.70)
+    _, ctx_a, _, ta = _pair(W, O, R, fw, O.CHARGE_EWALD, R.PREC_MIXED, [site], True, 2.5, True)
+    _, ctx_b, _, tb = _pair(W, O, R, fw, O.CHARGE_EWALD, R.PREC_MIXED, [site], True, 2.5, True)
+    g = ctx_b.MakeGrid(tb[0], fw.unit_box, 0.5)
+    ctx_b.use_grid(tb[0], g)
+    xyz = np.concatenate([random_positions(fw.box, 3000, 31), _clustered_positions(fw.box, 6000, 32)])
+    uv_a, uc_a, _ = ctx_a.energy_at_positions(xyz, ta[0])
+    _, uc_b, _ = ctx_b.energy_at_positions(xyz, tb[0])
+    keep = (uv_a < 1e6) & (uc_a < 1e6)
+    assert keep.sum() > 2000
+    assert same_bits(uc_a[keep], uc_b[keep])
