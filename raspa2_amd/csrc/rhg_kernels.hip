@@ -982,6 +982,78 @@ __global__ __launch_bounds__(1024) void k_scan_bins(int32_t *__restrict__ in, in
   if (t == 0 && n_redo) *n_redo = 0;
 }
 
+// k_scan_bins for at most kScanLds bins: the counts are read and cleared,
+// and the starts and tile offsets written, by coalesced passes through LDS;
+// each thread's run of consecutive bins is summed from LDS.
+constexpr int kScanLds = 8160; // two arrays and the wave sums within 64 KiB of LDS
+__global__ __launch_bounds__(1024) void k_scan_bins_lds(int32_t *__restrict__ in, int n,
+                                                        int32_t *__restrict__ start,
+                                                        int32_t *__restrict__ tiles, int ts,
+                                                        int32_t *__restrict__ n_redo) {
+  __shared__ int l_s[kScanLds], l_t[kScanLds];
+  __shared__ int w_s[16], w_t[16];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  for (int i = t; i < n; i += blockDim.x) {
+    l_s[i] = in[i];
+    in[i] = 0;
+  }
+  __syncthreads();
+  const int per = (n + blockDim.x - 1) / blockDim.x;
+  const int b = t * per, e = min(n, b + per);
+  int s = 0, u = 0;
+  for (int i = b; i < e; ++i) {
+    const int v = l_s[i];
+    s += v;
+    u += (v + ts - 1) / ts;
+  }
+  int is = s, iu = u; // inclusive within the wave
+  for (int d = 1; d < 64; d <<= 1) {
+    const int ps = __shfl_up(is, d), pu = __shfl_up(iu, d);
+    if (lane >= d) {
+      is += ps;
+      iu += pu;
+    }
+  }
+  if (lane == 63) {
+    w_s[wid] = is;
+    w_t[wid] = iu;
+  }
+  __syncthreads();
+  if (wid == 0) {
+    int vs = lane < 16 ? w_s[lane] : 0, vt = lane < 16 ? w_t[lane] : 0;
+    for (int d = 1; d < 16; d <<= 1) {
+      const int ps = __shfl_up(vs, d), pt = __shfl_up(vt, d);
+      if (lane >= d) {
+        vs += ps;
+        vt += pt;
+      }
+    }
+    if (lane < 16) {
+      w_s[lane] = vs;
+      w_t[lane] = vt;
+    }
+  }
+  __syncthreads();
+  int rs = (wid ? w_s[wid - 1] : 0) + is - s, rt = (wid ? w_t[wid - 1] : 0) + iu - u;
+  for (int i = b; i < e; ++i) {
+    const int v = l_s[i];
+    l_s[i] = rs;
+    l_t[i] = rt;
+    rs += v;
+    rt += (v + ts - 1) / ts;
+  }
+  __syncthreads();
+  for (int i = t; i < n; i += blockDim.x) {
+    start[i] = l_s[i];
+    tiles[i] = l_t[i];
+  }
+  if (t == 0) {
+    start[n] = w_s[15];
+    tiles[n] = w_t[15];
+    if (n_redo) *n_redo = 0;
+  }
+}
+
 void launch_bin_sites(const Cell &cell, const CellGrid &g, const SiteList &s, const BinWork &b,
                       hipStream_t st) {
   // counts left by an earlier binning are zero (the scan clears them)
@@ -998,8 +1070,12 @@ void launch_bin_sites(const Cell &cell, const CellGrid &g, const SiteList &s, co
     else
       hipLaunchKernelGGL(k_bin_count<3>, dim3(nb), dim3(256), 0, st, cell, g, s, b);
   }
-  hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(1024), 0, st, b.count, g.ncell, b.start, b.tile_off,
-                     b.fix ? img_tile_sites(b) : 64, b.n_redo);
+  if (g.ncell <= kScanLds)
+    hipLaunchKernelGGL(k_scan_bins_lds, dim3(1), dim3(1024), 0, st, b.count, g.ncell, b.start, b.tile_off,
+                       b.fix ? img_tile_sites(b) : 64, b.n_redo);
+  else
+    hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(1024), 0, st, b.count, g.ncell, b.start, b.tile_off,
+                       b.fix ? img_tile_sites(b) : 64, b.n_redo);
   if (nb) {
     if (cell.bc == 1)
       hipLaunchKernelGGL(k_bin_scatter<1>, dim3(nb), dim3(256), 0, st, cell, g, s, b);
